@@ -1,0 +1,125 @@
+"""ctypes binding of the C ABI declared in ``include/dtmmx.h``.
+
+The library is ``dtmm_amd/libdtmmx.so`` built by ``dtmm_amd.build``.  There
+is no fallback: if the library is missing every entry point raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdtmmx.so")
+
+DTMMX_OK = 0
+DTMMX_E_ARG = -1
+DTMMX_E_UNSUPPORTED = -2
+DTMMX_E_HIP = -3
+DTMMX_E_NOMEM = -4
+DTMMX_C64 = 0
+DTMMX_C128 = 1
+DTMMX_METHOD_2X2 = 2
+DTMMX_METHOD_4X4 = 4
+MAX_WAVELENGTHS = 64
+MAX_BEAMS = 64
+
+# Every symbol include/dtmmx.h declares.
+EXPORTS = (
+    "dtmmx_abi_version",
+    "dtmmx_last_error",
+    "dtmmx_plan_create",
+    "dtmmx_plan_destroy",
+    "dtmmx_plan_workspace_size",
+    "dtmmx_transfer",
+    "dtmmx_fft2",
+    "dtmmx_host_alphaf",
+)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("height", ctypes.c_int),
+        ("width", ctypes.c_int),
+        ("n_wavelengths", ctypes.c_int),
+        ("n_rays", ctypes.c_int),
+        ("rays_per_beam", ctypes.c_int),
+        ("method", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("betamax", ctypes.c_double),
+        ("k0", _dp),
+        ("beta", _dp),
+        ("phi", _dp),
+    ]
+
+
+class Stack(ctypes.Structure):
+    _fields_ = [
+        ("n_layers", ctypes.c_int),
+        ("thickness", _dp),
+        ("eps_eff_re", _dp),
+        ("eps_eff_im", _dp),
+        ("eps_re", ctypes.c_void_p),
+        ("eps_im", ctypes.c_void_p),
+        ("angles", ctypes.c_void_p),
+        ("n_in", ctypes.c_double),
+        ("n_out", ctypes.c_double),
+    ]
+
+
+class DtmmxError(RuntimeError):
+    """Error code and message returned by the engine library."""
+
+    def __init__(self, code, msg):
+        super().__init__("dtmmx error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load (once) and return the engine library; raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "dtmm_amd engine library %s is missing: build it with "
+            "`python -m dtmm_amd.build` (there is no CPU fallback)" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.dtmmx_abi_version.restype = ctypes.c_int
+    lib.dtmmx_last_error.restype = ctypes.c_char_p
+    lib.dtmmx_plan_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(PlanDesc)]
+    lib.dtmmx_plan_create.restype = ctypes.c_int
+    lib.dtmmx_plan_destroy.argtypes = [ctypes.c_void_p]
+    lib.dtmmx_plan_destroy.restype = ctypes.c_int
+    lib.dtmmx_plan_workspace_size.argtypes = [ctypes.c_void_p]
+    lib.dtmmx_plan_workspace_size.restype = ctypes.c_size_t
+    lib.dtmmx_transfer.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stack), ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+    lib.dtmmx_transfer.restype = ctypes.c_int
+    lib.dtmmx_fft2.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    lib.dtmmx_fft2.restype = ctypes.c_int
+    lib.dtmmx_host_alphaf.argtypes = [ctypes.c_int] + [_dp] * 7
+    lib.dtmmx_host_alphaf.restype = ctypes.c_int
+    if lib.dtmmx_abi_version() != 1:
+        raise RuntimeError("dtmmx ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def check(code):
+    """Raise DtmmxError for a non-zero return code."""
+    if code != DTMMX_OK:
+        msg = load().dtmmx_last_error()
+        raise DtmmxError(code, msg.decode() if msg else "")
+    return code
+
+
+def dptr(a):
+    """ctypes double* of a C-contiguous float64 numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)

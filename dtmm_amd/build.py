@@ -1,0 +1,44 @@
+"""Build the gfx950 engine library ``dtmm_amd/libdtmmx.so`` with hipcc.
+
+Usage: ``python -m dtmm_amd.build`` (or ``dtmm_amd.build.build()``).
+"""
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc", "dtmmx.hip")
+OUT = os.path.join(HERE, "libdtmmx.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-shared",
+    "-fPIC",
+    # never emit packed fp32 arithmetic (v_pk_*_f32)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"),
+    "-I", os.path.join(HERE, "csrc"),
+]
+
+
+def build(verbose=True):
+    """Compile the library (always recompiles)."""
+    cmd = [HIPCC] + FLAGS + [SRC, "-o", OUT + ".tmp"]
+    t = time.time()
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(OUT + ".tmp", OUT)
+    if verbose:
+        print("built %s in %.1f s" % (OUT, time.time() - t), flush=True)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

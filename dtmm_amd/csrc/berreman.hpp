@@ -1,0 +1,266 @@
+// Per-pixel Berreman 4x4 eigen-solution in double precision.
+//
+// Field vector order is (Ex, Hy, Ey, Hx); mode columns are interleaved
+// (+, -, +, -).  The isotropic and uniaxial closed forms restate
+// dtmm/tmm.py:123-288 (reference); the beam-azimuth rotation of the rows
+// restates dtmm/linalg.py:674-713 (_dotr2m).
+#pragma once
+#include "cpx.hpp"
+
+namespace dtmmx {
+
+using cd = cpx<double>;
+
+struct Modes {
+  cd a[4];     // eigenvalues alpha
+  cd F[4][4];  // field matrix, columns = modes
+};
+
+enum MaterialKind { KIND_ISO = 0, KIND_UNIAXIAL = 1, KIND_BIAXIAL = 2 };
+
+// Exact-equality branch selection of dtmm/tmm.py:384-389.
+__host__ __device__ inline int material_kind(cd e0, cd e1, cd e2) {
+  if (eq(e0, e1) && eq(e1, e2)) return KIND_ISO;
+  if (eq(e0, e1)) return KIND_UNIAXIAL;
+  return KIND_BIAXIAL;
+}
+
+// Isotropic medium, columns not normalised; all zero where sqrt(eps - beta^2)
+// vanishes (dtmm/tmm.py:123-154).
+__host__ __device__ inline void iso_modes(double beta, cd eps, Modes& m) {
+  cd aout = csqrt(eps - mk<double>(beta * beta));
+  for (int i = 0; i < 4; ++i) {
+    m.a[i] = mk<double>(0.);
+    for (int j = 0; j < 4; ++j) m.F[i][j] = mk<double>(0.);
+  }
+  if (is_zero(aout)) return;
+  cd gp = cdiv(eps, aout);
+  cd gs = -aout;
+  m.a[0] = aout;
+  m.a[1] = -aout;
+  m.a[2] = aout;
+  m.a[3] = -aout;
+  m.F[0][0] = mk<double>(0.5);
+  m.F[0][1] = mk<double>(0.5);
+  m.F[1][0] = gp * 0.5;
+  m.F[1][1] = gp * -0.5;
+  m.F[2][2] = mk<double>(0.5);
+  m.F[2][3] = mk<double>(0.5);
+  m.F[3][2] = gs * 0.5;
+  m.F[3][3] = gs * -0.5;
+}
+
+// Uniaxial medium (ordinary eps_o, extraordinary eps_e, optic axis at polar
+// angle theta and azimuth phi relative to the plane of incidence), columns
+// normalised to unit 2-norm (dtmm/tmm.py:156-288 with the rotation matrix of
+// dtmm/rotation.py:176-193).
+__host__ __device__ inline void uniaxial_modes(double beta, cd eps11, cd eps33, double theta,
+                                               double phi, Modes& m) {
+  const double ct = cos(theta), st = sin(theta), cf = cos(phi), sf = sin(phi);
+  const double sfst = st * sf, cfst = cf * st;
+  cd delta = eps33 - eps11;
+  cd evpp, evpm, evs, ev02;
+  if (beta == 0.) {
+    ev02 = eps11;
+    evs = csqrt(ev02);
+    cd u = eps11 + delta * (ct * ct);
+    cd w = eps11 * (ev02 + delta);
+    cd sq = cdiv(csqrt(u * w), u);
+    evpp = sq;
+    evpm = -sq;
+  } else {
+    ev02 = eps11 - mk<double>(beta * beta);
+    evs = csqrt(ev02);
+    cd u = eps11 + delta * (ct * ct);
+    double gama = beta * cf;
+    cd v = delta * (gama * 2. * st * ct);
+    cd w = delta * (gama * gama * st * st) - eps11 * (ev02 + delta);
+    cd sq = cdiv(csqrt(v * v - u * w * 4.), u) * 0.5;
+    v = cdiv(v, u) * 0.5;
+    evpp = -v + sq;
+    evpm = -v - sq;
+  }
+  m.a[0] = evpp;
+  m.a[1] = evpm;
+  m.a[2] = evs;
+  m.a[3] = -evs;
+  if (beta == 0.) {
+    m.F[0][2] = evs * (-sf);
+    m.F[1][2] = eps11 * (-sf);
+    m.F[2][2] = evs * cf;
+    m.F[3][2] = eps11 * (-cf);
+    m.F[0][3] = evs * (-sf);
+    m.F[1][3] = eps11 * sf;
+    m.F[2][3] = evs * cf;
+    m.F[3][3] = eps11 * cf;
+    m.F[0][0] = mk<double>(cf);
+    m.F[1][0] = evpp * cf;
+    m.F[2][0] = mk<double>(sf);
+    m.F[3][0] = evpp * (-sf);
+    m.F[0][1] = mk<double>(cf);
+    m.F[1][1] = evpm * cf;
+    m.F[2][1] = mk<double>(sf);
+    m.F[3][1] = evpm * (-sf);
+  } else {
+    const double ctbeta = ct * beta;
+    cd ctbetaeps11 = cdiv(mk<double>(ctbeta), eps11);
+    cd e11sfst = eps11 * sfst;
+    cd evssfst = evs * sfst;
+    cd evscfst = evs * cfst;
+    cd evsctbeta = evs * ctbeta;
+    cd ev02cfst = ev02 * cfst;
+    cd ev02cfsteps11 = cdiv(ev02cfst, eps11);
+    m.F[0][2] = evssfst;
+    m.F[1][2] = e11sfst;
+    m.F[2][2] = mk<double>(ctbeta) - evscfst;
+    m.F[3][2] = ev02cfst - evsctbeta;
+    m.F[0][3] = -evssfst;
+    m.F[1][3] = e11sfst;
+    m.F[2][3] = evscfst + mk<double>(ctbeta);
+    m.F[3][3] = ev02cfst + evsctbeta;
+    m.F[0][0] = evpp * ctbetaeps11 - ev02cfsteps11;
+    m.F[1][0] = mk<double>(ctbeta) - evpp * cfst;
+    m.F[2][0] = mk<double>(-sfst);
+    m.F[3][0] = evpp * sfst;
+    m.F[0][1] = -(evpm * ctbetaeps11) + ev02cfsteps11;
+    m.F[1][1] = evpm * cfst - mk<double>(ctbeta);
+    m.F[2][1] = mk<double>(sfst);
+    m.F[3][1] = -(evpm * sfst);
+  }
+  for (int j = 0; j < 4; ++j) {
+    double t = 0.;
+    for (int i = 0; i < 4; ++i) t += norm2(m.F[i][j]);
+    t = sqrt(t);
+    for (int i = 0; i < 4; ++i) m.F[i][j] = mk<double>(m.F[i][j].x / t, m.F[i][j].y / t);
+  }
+}
+
+// Rotate the (Ex, Hy, Ey, Hx) rows by the beam azimuth (c, s) = (cos, sin).
+__host__ __device__ inline void rotate_rows(Modes& m, double c, double s) {
+  for (int j = 0; j < 4; ++j) {
+    cd f0 = m.F[0][j], f1 = m.F[1][j], f2 = m.F[2][j], f3 = m.F[3][j];
+    m.F[0][j] = f0 * c - f2 * s;
+    m.F[1][j] = f1 * c + f3 * s;
+    m.F[2][j] = f0 * s + f2 * c;
+    m.F[3][j] = f3 * c - f1 * s;
+  }
+}
+
+// 2x2 inverse; zeros (and false) where the determinant is exactly zero
+// (dtmm/linalg.py:428-457).
+__host__ __device__ inline bool inv2(const cd a[2][2], cd o[2][2]) {
+  cd det = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+  if (is_zero(det)) {
+    o[0][0] = o[0][1] = o[1][0] = o[1][1] = mk<double>(0.);
+    return false;
+  }
+  cd id = cdiv(mk<double>(1.), det);
+  o[0][0] = a[1][1] * id;
+  o[0][1] = -(a[0][1] * id);
+  o[1][0] = -(a[1][0] * id);
+  o[1][1] = a[0][0] * id;
+  return true;
+}
+
+// 4x4 inverse by cofactors; zeros where the determinant is exactly zero
+// (dtmm/linalg.py:460-518).
+__host__ __device__ inline bool inv4(const cd m[4][4], cd o[4][4]) {
+  // 2x2 sub-determinants of rows (0,1) and rows (2,3)
+  cd s0 = m[0][0] * m[1][1] - m[1][0] * m[0][1];
+  cd s1 = m[0][0] * m[1][2] - m[1][0] * m[0][2];
+  cd s2 = m[0][0] * m[1][3] - m[1][0] * m[0][3];
+  cd s3 = m[0][1] * m[1][2] - m[1][1] * m[0][2];
+  cd s4 = m[0][1] * m[1][3] - m[1][1] * m[0][3];
+  cd s5 = m[0][2] * m[1][3] - m[1][2] * m[0][3];
+  cd c5 = m[2][2] * m[3][3] - m[3][2] * m[2][3];
+  cd c4 = m[2][1] * m[3][3] - m[3][1] * m[2][3];
+  cd c3 = m[2][1] * m[3][2] - m[3][1] * m[2][2];
+  cd c2 = m[2][0] * m[3][3] - m[3][0] * m[2][3];
+  cd c1 = m[2][0] * m[3][2] - m[3][0] * m[2][2];
+  cd c0 = m[2][0] * m[3][1] - m[3][0] * m[2][1];
+  cd det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+  if (is_zero(det)) {
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) o[i][j] = mk<double>(0.);
+    return false;
+  }
+  cd id = cdiv(mk<double>(1.), det);
+  o[0][0] = (m[1][1] * c5 - m[1][2] * c4 + m[1][3] * c3) * id;
+  o[0][1] = (-(m[0][1] * c5) + m[0][2] * c4 - m[0][3] * c3) * id;
+  o[0][2] = (m[3][1] * s5 - m[3][2] * s4 + m[3][3] * s3) * id;
+  o[0][3] = (-(m[2][1] * s5) + m[2][2] * s4 - m[2][3] * s3) * id;
+  o[1][0] = (-(m[1][0] * c5) + m[1][2] * c2 - m[1][3] * c1) * id;
+  o[1][1] = (m[0][0] * c5 - m[0][2] * c2 + m[0][3] * c1) * id;
+  o[1][2] = (-(m[3][0] * s5) + m[3][2] * s2 - m[3][3] * s1) * id;
+  o[1][3] = (m[2][0] * s5 - m[2][2] * s2 + m[2][3] * s1) * id;
+  o[2][0] = (m[1][0] * c4 - m[1][1] * c2 + m[1][3] * c0) * id;
+  o[2][1] = (-(m[0][0] * c4) + m[0][1] * c2 - m[0][3] * c0) * id;
+  o[2][2] = (m[3][0] * s4 - m[3][1] * s2 + m[3][3] * s0) * id;
+  o[2][3] = (-(m[2][0] * s4) + m[2][1] * s2 - m[2][3] * s0) * id;
+  o[3][0] = (-(m[1][0] * c3) + m[1][1] * c1 - m[1][2] * c0) * id;
+  o[3][1] = (m[0][0] * c3 - m[0][1] * c1 + m[0][2] * c0) * id;
+  o[3][2] = (-(m[3][0] * s3) + m[3][1] * s1 - m[3][2] * s0) * id;
+  o[3][3] = (m[2][0] * s3 - m[2][1] * s1 + m[2][2] * s0) * id;
+  return true;
+}
+
+__host__ __device__ inline void matmul4(const cd a[4][4], const cd b[4][4], cd o[4][4]) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      cd s = mk<double>(0.);
+      for (int k = 0; k < 4; ++k) s = s + a[i][k] * b[k][j];
+      o[i][j] = s;
+    }
+}
+
+// Eigen data of one pixel for the real-space layer operator
+// g <- E diag(exp(i kd alpha)) E^-1 g written in the Epn basis
+// (E+x, E-x, E+y, E-y).  With E+ = F[(0,2),(0,2)] and E- = F[(0,2),(1,3)],
+//   E+ diag(p0, p2) E+^-1 = p2 I + (p0 - p2) P0,  P0 = E+[:,0] (x) E+^-1[0,:]
+//   E- diag(p1, p3) E-^-1 = p3 I + (p1 - p3) P1,  P1 = E-[:,0] (x) E-^-1[0,:]
+// ok4: E = blockdiag(E+, E-) is invertible (4x4 method, dtmm/tmm.py:974-997
+// with linalg.inv); ok2: E+ is invertible (2x2 method, mode = +1).
+struct RopData {
+  cd a[4];
+  cd P0[2][2];
+  cd P1[2][2];
+  bool ok4, ok2;
+  int kind;
+};
+
+__host__ __device__ inline void layer_modes(double beta, double c0, double s0, double phi0, cd e0,
+                                            cd e1, cd e2, double theta, double phi, Modes& m,
+                                            int& kind) {
+  kind = material_kind(e0, e1, e2);
+  if (kind == KIND_ISO) {
+    iso_modes(beta, e0, m);
+  } else if (kind == KIND_UNIAXIAL) {
+    uniaxial_modes(beta, e0, e2, theta, phi - phi0, m);
+  } else {
+    const double nan = __builtin_nan("");
+    for (int i = 0; i < 4; ++i) {
+      m.a[i] = mk<double>(nan, nan);
+      for (int j = 0; j < 4; ++j) m.F[i][j] = mk<double>(nan, nan);
+    }
+  }
+  rotate_rows(m, c0, s0);
+}
+
+__host__ __device__ inline void rop_data(const Modes& m, int kind, RopData& r) {
+  r.kind = kind;
+  for (int i = 0; i < 4; ++i) r.a[i] = m.a[i];
+  cd ep[2][2] = {{m.F[0][0], m.F[0][2]}, {m.F[2][0], m.F[2][2]}};
+  cd en[2][2] = {{m.F[0][1], m.F[0][3]}, {m.F[2][1], m.F[2][3]}};
+  cd ip[2][2], in[2][2];
+  bool okp = inv2(ep, ip);
+  bool okn = inv2(en, in);
+  r.ok2 = okp;
+  r.ok4 = okp && okn;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      r.P0[i][j] = ep[i][0] * ip[0][j];
+      r.P1[i][j] = en[i][0] * in[0][j];
+    }
+}
+
+}  // namespace dtmmx

@@ -1,0 +1,217 @@
+"""Field-data model: illumination construction and Poynting intensity.
+
+Host-side NumPy input builders with the reference's names and argument
+meanings (dtmm/field.py, dtmm/wave.py, dtmm/window.py).
+"""
+import numpy as np
+
+from .data import refind2eps
+
+FDTYPE = np.float64
+CDTYPE = np.complex128
+BETAMAX = 0.8
+N_COVER = 1.5
+
+
+def k0(wavelength, pixelsize=1.):
+    """Vacuum wave number in pixel units (dtmm/wave.py:280-296)."""
+    return np.asarray(2 * np.pi / np.asarray(wavelength) * pixelsize, dtype=FDTYPE)
+
+
+def betaphi(shape, k0):
+    """beta, phi of the FFT eigen-waves (dtmm/wave.py:54-81)."""
+    k0 = np.abs(np.asarray(k0, FDTYPE)[..., None, None])
+    ay, ax = [np.asarray(np.fft.fftfreq(x), dtype=FDTYPE) for x in shape[-2:]]
+    xx, yy = np.meshgrid(ax, ay, copy=False, indexing="xy")
+    beta = (2 * np.pi / k0) * np.sqrt(xx ** 2 + yy ** 2)
+    phi = np.arctan2(yy, xx)
+    return beta, phi
+
+
+def blackman(shape):
+    """2D radial Blackman window (dtmm/window.py:33-54)."""
+    ny, nx = [l / 2 for l in shape]
+    ay, ax = [np.arange(-l / 2. + .5, l / 2. + .5) for l in shape]
+    xx, yy = np.meshgrid(ax, ay, indexing="xy")
+    r = ((xx / nx) ** 2 + (yy / ny) ** 2) ** 0.5
+    out = 0.42 + 0.5 * np.cos(np.pi * r) + 0.08 * np.cos(2 * np.pi * r)
+    out[r >= 1.] = 0.
+    return np.asarray(out, FDTYPE)
+
+
+def planewave(shape, k0, beta, phi):
+    """Plane wave normalised to 1 at pixel (0, 0) (dtmm/wave.py:376-409)."""
+    k0 = np.asarray(k0)[..., None, None]
+    beta = np.asarray(beta)[..., None, None]
+    phi = np.asarray(phi)[..., None, None]
+    ay, ax = [np.arange(-l // 2 + 1., l // 2 + 1.) for l in shape[-2:]]
+    xx, yy = np.meshgrid(ax, ay, indexing="xy", copy=False)
+    kx = np.asarray(k0 * beta * np.cos(phi), dtype=FDTYPE)
+    ky = np.asarray(k0 * beta * np.sin(phi), dtype=FDTYPE)
+    out = np.exp(1j * (kx * xx + ky * yy))
+    return out / out[..., 0, 0][..., None, None]
+
+
+def wave2eigenwave(wave):
+    """Snap each 2D wave to its strongest FFT eigen-wave with the same power
+    (dtmm/wave.py:432-476, eigenwave :140)."""
+    wave = np.asarray(wave)
+    shape = wave.shape
+    flat = wave.reshape((-1,) + shape[-2:])
+    out = np.empty(flat.shape, CDTYPE)
+    s = np.abs(np.fft.fft2(flat)) ** 2
+    for i, a in enumerate(s):
+        avg = a.sum()
+        ii, jj = np.unravel_index(np.argmax(a), a.shape)
+        f = np.zeros(a.shape, CDTYPE)
+        f[ii, jj] = avg ** 0.5
+        out[i] = np.fft.ifft2(f)
+    return out.reshape(shape)
+
+
+def _iso_fmat(beta, phi, n):
+    """Field matrix of an isotropic medium, rotated by phi
+    (dtmm/tmm.py:123-154 and dtmm/linalg.py:674-713)."""
+    beta = np.asarray(beta, FDTYPE)
+    phi = np.asarray(phi, FDTYPE)
+    eps = complex(refind2eps(n))
+    a = np.sqrt(eps - beta ** 2 + 0j)
+    F = np.zeros(np.broadcast_shapes(beta.shape, phi.shape) + (4, 4), CDTYPE)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        gp = eps / a
+        F[..., 1, 0] = 0.5 * gp
+        F[..., 1, 1] = -0.5 * gp
+    F[..., 0, 0] = 0.5
+    F[..., 0, 1] = 0.5
+    F[..., 2, 2] = 0.5
+    F[..., 2, 3] = 0.5
+    F[..., 3, 2] = -0.5 * a
+    F[..., 3, 3] = 0.5 * a
+    c, s = np.cos(phi)[..., None], np.sin(phi)[..., None]
+    r0 = F[..., 0, :] * c - F[..., 2, :] * s
+    r1 = F[..., 1, :] * c + F[..., 3, :] * s
+    r2 = F[..., 0, :] * s + F[..., 2, :] * c
+    r3 = -F[..., 1, :] * s + F[..., 3, :] * c
+    return np.stack([r0, r1, r2, r3], -2)
+
+
+def _normalize_f(fmat):
+    """Unit |Poynting| columns (dtmm/tmm.py:699-724)."""
+    p = (fmat[..., 0, :] * np.conj(fmat[..., 1, :])).real - \
+        (fmat[..., 2, :] * np.conj(fmat[..., 3, :])).real
+    n = np.abs(p) ** 0.5
+    s = np.where(n == 0., 0., 1. / np.where(n == 0., 1., n))
+    return fmat * s[..., None, :]
+
+
+def _jonesvec(pol, phi):
+    """Normalised Jones vector in the frame rotated by phi (dtmm/jones.py:55-98)."""
+    pol = np.asarray(pol, CDTYPE)
+    norm = (pol[..., 0] * pol[..., 0].conj() + pol[..., 1] * pol[..., 1].conj()) ** 0.5
+    pol = pol / norm[..., None]
+    c, s = np.cos(-phi), np.sin(-phi)
+    return np.stack([c * pol[..., 0] - s * pol[..., 1], s * pol[..., 0] + c * pol[..., 1]], -1)
+
+
+def _fvec(fmat, jones, amplitude, mode=+1):
+    """Field vectors of the given polarisation and amplitude
+    (dtmm/tmm.py:1750-1843, :1877-1904)."""
+    c, s = jones[..., 0], jones[..., 1]
+    shape = np.broadcast(c, amplitude).shape + (4,)
+    a = np.zeros(shape, CDTYPE)
+    i0, i1 = (0, 2) if mode == +1 else (1, 3)
+    a[..., i0] = c
+    a[..., i1] = s
+    a = a * amplitude[..., None]
+    return np.matmul(_normalize_f(fmat), a[..., None])[..., 0]
+
+
+def _projection_fft(field, ks, n, mode, betamax):
+    """Apply F diag(1 on forward (mode=+1) modes) F^-1 per FFT mode in a
+    medium of index n, zero for beta >= betamax (dtmm/diffract.py:107-119,
+    :254-299)."""
+    shape = field.shape[-2:]
+    beta, phi = betaphi(shape, ks)
+    F = _iso_fmat(beta, phi, n)
+    mask = beta >= betamax
+    ok = ~mask
+    Fi = np.zeros_like(F)
+    Fi[ok] = np.linalg.inv(F[ok])
+    F[mask] = 0.
+    d = np.zeros(4)
+    if mode == +1:
+        d[0::2] = 1.
+    else:
+        d[1::2] = 1.
+    P = np.matmul(F * d, Fi)
+    ff = np.fft.fft2(field)
+    v = np.moveaxis(ff, -3, -1)[..., None]
+    v = np.matmul(P, v)[..., 0]
+    return np.fft.ifft2(np.moveaxis(v, -1, -3))
+
+
+def field2intensity(field):
+    """Poynting z per pixel of a (..., 4, H, W) field (dtmm/field.py:771-799).
+    Accepts numpy arrays or torch tensors."""
+    if hasattr(field, "is_cuda"):
+        return (field[..., 0, :, :] * field[..., 1, :, :].conj()).real - \
+            (field[..., 2, :, :] * field[..., 3, :, :].conj()).real
+    f = np.asarray(field)
+    return (f[..., 0, :, :] * np.conj(f[..., 1, :, :])).real - \
+        (f[..., 2, :, :] * np.conj(f[..., 3, :, :])).real
+
+
+def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensity=1.,
+                      n=None, focus=0., window=None, backdir=False, jones=None,
+                      diffraction=True, eigenmodes=None, betamax=BETAMAX):
+    """Forward (or backward) propagating input field data
+    (dtmm/field.py:586-713).  Returns (field, wavelengths, pixelsize) with
+    field of shape (..., [2 pol], n_wavelengths, 4, H, W)."""
+    n = N_COVER if n is None else n
+    eigenmodes = (window is None) if eigenmodes is None else bool(eigenmodes)
+    wavelengths = np.asarray(wavelengths)
+    wavenumbers = 2 * np.pi / wavelengths * pixelsize
+    if wavenumbers.ndim != 1:
+        raise ValueError("Wavelengths should be 1D array")
+    if jones is None:
+        intensity = intensity / 2.
+    mode = -1 if backdir else +1
+    _beta = np.asarray(beta, FDTYPE)
+    _phi = np.asarray(phi, FDTYPE)
+    _intensity = np.asarray(intensity, FDTYPE)
+    nrays = len(_beta) if _beta.ndim > 0 else 1
+    b = _beta[:, None] if _beta.ndim == 1 else _beta
+    p = _phi[:, None] if _phi.ndim == 1 else _phi
+    b = b if b.ndim >= 1 else b[..., None]
+    p = p if p.ndim >= 1 else p[..., None]
+    waves = planewave(shape, wavenumbers, b, p)
+    if window is not None:
+        waves = waves * window
+    if eigenmodes:
+        waves = wave2eigenwave(waves)
+    b = _beta[:, None, None, None] if _beta.ndim == 1 else _beta[..., None, None]
+    p = _phi[:, None, None, None] if _phi.ndim == 1 else _phi[..., None, None]
+    inten = _intensity[..., None, None, None, None]
+    fmat = _iso_fmat(b, p, n)
+    if jones is None:
+        f1 = np.moveaxis(_fvec(fmat, _jonesvec((1, 0), p), waves, mode), -1, -3)
+        f2 = np.moveaxis(_fvec(fmat, _jonesvec((0, 1), p), waves, mode), -1, -3)
+        shp = list(f1.shape)
+        shp.insert(-4, 2)
+        field = np.empty(shp, CDTYPE)
+        field[..., 0, :, :, :, :] = f1
+        field[..., 1, :, :, :, :] = f2
+    else:
+        field = np.moveaxis(_fvec(fmat, _jonesvec(jones, p), waves, mode), -1, -3).copy()
+    i1 = field2intensity(field)
+    norm = np.ones_like(i1)
+    norm[:, ...] = (inten / nrays) ** 0.5
+    if diffraction:
+        if focus != 0:
+            raise NotImplementedError("focus != 0 is not supported by this builder")
+        field = _projection_fft(field, wavenumbers, n, mode, betamax)
+        i2 = field2intensity(field)
+        ratio = (i1.sum((-2, -1)) / i2.sum((-2, -1))) ** 0.5
+        norm = norm * ratio[..., None, None]
+    field = norm[..., None, :, :] * field
+    return field, wavelengths, pixelsize

@@ -1,0 +1,329 @@
+"""Drop-in ``transfer_field`` for the diffractive transfer-matrix path.
+
+Mirrors dtmm/transfer.py:322 ``transfer_field`` (same arguments, defaults,
+return value and errors) for the modes built on the GPU:
+
+* method "4x4": reflection mode 1, diffraction 1, npass 1
+  (dtmm/transfer.py:618-884, dtmm/propagate_4x4.py:376-389, :80-101);
+* method "2x2": reflection mode 0, diffraction 1, npass 1
+  (dtmm/transfer.py:1009-1183, dtmm/propagate_2x2.py:193-211, :19-87).
+
+Everything else the reference accepts raises ``NotImplementedError``.  The
+propagation runs in ``libdtmmx.so`` (HIP, gfx950); torch provides device
+memory and the stream.  There is no CPU fallback.
+"""
+import ctypes
+import warnings
+
+import numpy as np
+
+from . import _lib
+from .data import refind2eps, uniaxial_order, validate_optical_data
+from .field import blackman, k0 as _k0
+
+DTMM_NORM_FFT = 1 << 0
+DTMM_NORM_REF = 1 << 1
+SMOOTH = 0.1
+BETAMAX = 0.8
+N_COVER = 1.5
+# dtmm/dtmm.ini:23 holds `method = "2x2"`; configparser returns the quotes,
+# so the configured default is this string (it selects the 2x2 method with
+# reflection mode 1, dtmm/transfer.py:459).
+_DEFAULT_METHOD = '"2x2"'
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _require_gpu():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError("dtmm_amd.transfer_field needs a ROCm GPU (no CPU fallback)")
+    _lib.load()
+    return torch
+
+
+def _validate_betaphi(beta, phi):
+    """(dtmm/transfer.py:43-65) -- returns 1-D arrays of beams."""
+    if beta is None or phi is None:
+        raise ValueError("Both beta and phi must be defined!")
+    beta = np.asarray(beta, np.float64)
+    phi = np.asarray(phi, np.float64)
+    if beta.ndim != phi.ndim:
+        raise ValueError("Beta nad phi should have same dimensions!")
+    if beta.ndim == 1:
+        if len(beta) != len(phi):
+            raise ValueError("Beta nad phi should have same length!")
+        return beta, phi, True
+    if beta.ndim != 0:
+        raise ValueError("Only length 1 or scalar values are supported for beta and phi")
+    return beta[None], phi[None], False
+
+
+def mean_betaphi(field, k0):
+    """Intensity-weighted mean beta, phi of a field (dtmm/field.py:849-872)."""
+    field = np.asarray(field)
+    shape = field.shape[-2:]
+    f = np.fft.fft2(field * blackman(shape))
+    k0 = np.asarray(k0, np.float64)[..., None, None]
+    ay, ax = [np.fft.fftfreq(x) for x in shape]
+    xx, yy = np.meshgrid(ax, ay, indexing="xy")
+    bx = (2 * np.pi / k0) * xx
+    by = (2 * np.pi / k0) * yy
+    s = (np.abs(f) ** 2)
+    sx = (s * bx[..., None, :, :]).sum((-3, -2, -1))
+    sy = (s * by[..., None, :, :]).sum((-3, -2, -1))
+    ss = s.sum((-3, -2, -1))
+    return (sx ** 2 + sy ** 2) ** 0.5 / ss, np.arctan2(sy, sx)
+
+
+def field2betaphi(field_in, ks, multiray=False):
+    """Mean beam of the input field (dtmm/field.py:492-500)."""
+    beta, phi = mean_betaphi(field_in, ks)
+    if field_in.ndim > 4 and multiray:
+        axes = tuple(range(1, field_in.ndim - 3))
+        return beta.mean(axis=axes), phi.mean(axis=axes)
+    return beta.mean(), phi.mean()
+
+
+class Stack:
+    """Layer stack resident on the device: material as float64
+    structure-of-arrays [3][L][H][W] plus per-layer thickness and isotropic
+    effective eps.  Built from reference optical data with input/output
+    media and nstep sub-layers (dtmm/transfer.py:926-1005, eff_data=0)."""
+
+    def __init__(self, optical_data, shape, nstep=1, nin=N_COVER, nout=N_COVER,
+                 split_layers=False, device="cuda"):
+        torch = _torch()
+        blocks = validate_optical_data(optical_data, shape=shape)
+        H, W = shape
+        thick, eff, eps_l, ang_l = [], [], [], []
+        for d, material, angles in blocks:
+            n = len(d)
+            substeps = np.broadcast_to(np.asarray(nstep), (n,))
+            mat = np.broadcast_to(material, (n, H, W, 3))
+            ang = np.broadcast_to(angles, (n, H, W, 3))
+            if split_layers:
+                effs = [uniaxial_order(0., mat[i:i + 1]).mean(axis=(0, 1, 2))[0] for i in range(n)]
+            else:
+                e = uniaxial_order(0., material).mean(axis=(0, 1, 2))
+                effs = [e[0]] * n
+            for i in range(n):
+                ns = int(substeps[i])
+                if ns < 1:
+                    raise ValueError("nstep must be >= 1")
+                for _ in range(ns):
+                    thick.append(float(d[i]) / ns)
+                    eff.append(complex(effs[i]))
+                    eps_l.append((len(eps_l), mat[i], ang[i]))
+        self.n_layers = len(thick)
+        self.thickness = np.asarray(thick, np.float64)
+        self.eps_eff_re = np.asarray([e.real for e in eff], np.float64)
+        self.eps_eff_im = np.asarray([e.imag for e in eff], np.float64)
+        self.nin = float(nin)
+        self.nout = float(nout)
+        L = self.n_layers
+        is_complex = any(np.iscomplexobj(b[1]) for b in blocks)
+        eps_re = np.empty((3, L, H, W), np.float64)
+        eps_im = np.empty((3, L, H, W), np.float64) if is_complex else None
+        angs = np.empty((3, L, H, W), np.float64)
+        for l, (_, m, a) in enumerate(eps_l):
+            mm = np.moveaxis(m, -1, 0)
+            eps_re[:, l] = mm.real
+            if eps_im is not None:
+                eps_im[:, l] = mm.imag
+            angs[:, l] = np.moveaxis(a, -1, 0)
+        self.eps_re = torch.from_numpy(eps_re).to(device)
+        self.eps_im = torch.from_numpy(eps_im).to(device) if eps_im is not None else None
+        self.angles = torch.from_numpy(angs).to(device)
+        self.has_imag_eff = bool(np.any(self.eps_eff_im != 0))
+
+    def c_struct(self):
+        s = _lib.Stack()
+        s.n_layers = self.n_layers
+        s.thickness = _lib.dptr(self.thickness)
+        s.eps_eff_re = _lib.dptr(self.eps_eff_re)
+        s.eps_eff_im = _lib.dptr(self.eps_eff_im)
+        s.eps_re = self.eps_re.data_ptr()
+        s.eps_im = self.eps_im.data_ptr() if self.eps_im is not None else None
+        s.angles = self.angles.data_ptr()
+        s.n_in = self.nin
+        s.n_out = self.nout
+        return s
+
+
+class Plan:
+    """Owner of a ``dtmmx_plan``: grid, wavelengths, beams, method, precision."""
+
+    def __init__(self, shape, ks, beta, phi, n_rays, rays_per_beam, method, precision="c64",
+                 betamax=BETAMAX):
+        lib = _lib.load()
+        self.shape = tuple(shape)
+        self.ks = np.ascontiguousarray(ks, np.float64)
+        self.beta = np.ascontiguousarray(beta, np.float64)
+        self.phi = np.ascontiguousarray(phi, np.float64)
+        self.n_rays = int(n_rays)
+        self.rays_per_beam = int(rays_per_beam)
+        self.method = method
+        self.precision = precision
+        d = _lib.PlanDesc()
+        d.height, d.width = self.shape
+        d.n_wavelengths = len(self.ks)
+        d.n_rays = self.n_rays
+        d.rays_per_beam = self.rays_per_beam
+        d.method = _lib.DTMMX_METHOD_4X4 if method == "4x4" else _lib.DTMMX_METHOD_2X2
+        d.precision = _lib.DTMMX_C64 if precision == "c64" else _lib.DTMMX_C128
+        d.betamax = float(betamax)
+        d.k0 = _lib.dptr(self.ks)
+        d.beta = _lib.dptr(self.beta)
+        d.phi = _lib.dptr(self.phi)
+        h = ctypes.c_void_p()
+        _lib.check(lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)))
+        self._h = h
+        self._lib = lib
+        self.workspace_bytes = int(lib.dtmmx_plan_workspace_size(h))
+
+    @property
+    def dtype(self):
+        torch = _torch()
+        return torch.complex64 if self.precision == "c64" else torch.complex128
+
+    def execute(self, stack, field_in, field_out, workspace=None, status=None, stream=None):
+        """Run the transfer on device tensors of shape (n_rays, n_wavelengths,
+        4, H, W); asynchronous on `stream` (default: torch's current)."""
+        torch = _torch()
+        if stream is None:
+            stream = torch.cuda.current_stream(field_in.device)
+        for t in (field_in, field_out):
+            if t.dtype != self.dtype or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError("fields must be contiguous %s device tensors" % self.dtype)
+            if tuple(t.shape) != (self.n_rays, len(self.ks), 4) + self.shape:
+                raise ValueError("field shape %s does not match the plan" % (tuple(t.shape),))
+        if self.workspace_bytes and (workspace is None or
+                                     workspace.numel() * workspace.element_size() < self.workspace_bytes):
+            workspace = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=field_in.device)
+        s = stack.c_struct()
+        _lib.check(self._lib.dtmmx_transfer(
+            self._h, ctypes.byref(s), field_in.data_ptr(), field_out.data_ptr(),
+            workspace.data_ptr() if workspace is not None else None,
+            self.workspace_bytes, status.data_ptr() if status is not None else None,
+            stream.cuda_stream))
+        return field_out
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None and self._h.value:
+                self._lib.dtmmx_plan_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk):
+    """Defaults and reflection auto-selection (dtmm/transfer.py:433-465) and
+    the envelope of modes built on the GPU."""
+    if npass == -1 or npass == np.inf:
+        raise NotImplementedError("npass=-1 (full 3D TMM, dtmm/tmm3d.py) is not supported")
+    if method == "4x4" and npass > 1 and diffraction is False:
+        warnings.warn("The 4x4 method with diffraction disabled is not yet supported"
+                      " for input fields with beta >0. Use 2x2 method insted.")
+    if reflection is None:
+        reflection = 0 if method == "2x2" else 1
+        if method == "4x4" and diffraction == 0:
+            reflection = 2
+        if npass > 1:
+            reflection = 1
+            if diffraction > 1 and method == "2x2":
+                reflection = 2
+    kind = "4x4" if method == "4x4" else "2x2"
+    if kind == "4x4" and reflection not in (1, 2, 3, 4):
+        raise ValueError("Invalid reflection. The 4x4 method is either reflection mode 1 or 2.")
+    if kind == "2x2" and reflection not in (0, 1, 2):
+        raise ValueError("Invalid reflection. The 2x2 method supports reflection mode 0,1 or 2.")
+    if npass != 1:
+        raise NotImplementedError("npass > 1 (reflection iterations) is not supported yet")
+    if ret_bulk:
+        raise NotImplementedError("ret_bulk is not supported yet")
+    if eff_data not in (None, 0, "isotropic"):
+        raise NotImplementedError("only isotropic effective data (eff_data=0) is supported")
+    if kind == "4x4":
+        if reflection != 1:
+            raise NotImplementedError("4x4 method: only reflection mode 1 is supported")
+        if diffraction != 1:
+            if diffraction == 0 or diffraction < 0 or diffraction == np.inf:
+                raise NotImplementedError("4x4 method: only diffraction=1 is supported")
+            raise NotImplementedError("4x4 method: diffraction > 1 (mode windows) not supported yet")
+    else:
+        if reflection != 0:
+            raise NotImplementedError("2x2 method: only reflection mode 0 is supported "
+                                      "(note: the configured default method '\"2x2\"' selects "
+                                      "reflection mode 1; pass method='2x2')")
+        if diffraction != 1:
+            raise NotImplementedError("2x2 method: only diffraction=1 is supported")
+    return kind, reflection
+
+
+def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
+                   npass=None, nstep=1, diffraction=None, reflection=None, method=None,
+                   multiray=False, norm=DTMM_NORM_FFT, betamax=None, smooth=SMOOTH,
+                   split_rays=False, split_diffraction=False, split_wavelengths=False,
+                   split_layers=False, eff_data=None, ret_bulk=False, out=None,
+                   precision=None):
+    """Transfers input field data through optical data (dtmm/transfer.py:322).
+
+    Returns ``(field_out, wavelengths, pixelsize)``.  ``precision`` ("c64" or
+    "c128") selects the GPU arithmetic; by default it follows the input
+    field dtype (complex64 -> "c64", otherwise "c128").  The split_* options
+    only bound memory in the reference and do not change the result; they
+    are accepted and ignored here (the GPU batches all planes).
+    """
+    torch = _require_gpu()
+    nin = N_COVER if nin is None else nin
+    nout = N_COVER if nout is None else nout
+    method = _DEFAULT_METHOD if method is None else method
+    npass = 1 if npass is None else npass
+    diffraction = 1 if diffraction is None else diffraction
+    betamax = BETAMAX if betamax is None else betamax
+    kind, reflection = _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk)
+
+    field_in, wavelengths, pixelsize = field_data
+    if isinstance(field_in, tuple):
+        raise NotImplementedError("tuple (per-wavelength) field data is not supported")
+    field_np = np.asarray(field_in)
+    shape = field_np.shape[-2:]
+    wavelengths = np.asarray(wavelengths)
+    ks = _k0(wavelengths, pixelsize)
+    if field_np.ndim < 4 or field_np.shape[-3] != 4 or field_np.shape[-4] != len(ks):
+        raise ValueError("field must have shape (..., n_wavelengths, 4, H, W)")
+    if precision is None:
+        precision = "c64" if field_np.dtype == np.complex64 else "c128"
+    if beta is None and phi is None:
+        beta, phi = field2betaphi(field_np, ks, multiray)
+    b, p, per_ray = _validate_betaphi(beta, phi)
+    lead = field_np.shape[:-4]
+    n_rays = int(np.prod(lead)) if lead else 1
+    if per_ray:
+        if not lead or lead[0] != len(b):
+            raise ValueError("beta/phi of length %d need a leading ray axis of that length" % len(b))
+    if n_rays % len(b):
+        raise ValueError("rays do not divide into beams")
+    rays_per_beam = n_rays // len(b)
+
+    plan = Plan(shape, ks, b, p, n_rays, rays_per_beam, kind, precision, betamax)
+    stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout, split_layers=split_layers)
+    dt = plan.dtype
+    fin = torch.from_numpy(np.ascontiguousarray(field_np.reshape((n_rays, len(ks), 4) + shape)))
+    fin = fin.to(device="cuda", dtype=dt)
+    fout = torch.empty_like(fin)
+    status = torch.zeros(4, dtype=torch.int32, device="cuda")
+    plan.execute(stack, fin, fout, status=status)
+    torch.cuda.synchronize()
+    if int(status[0].item()) & 1:
+        raise NotImplementedError("biaxial voxels (eps0 != eps1) are not supported yet")
+    res = fout.cpu().numpy().reshape(field_np.shape)
+    if out is not None:
+        out[...] = res
+        res = out
+    return res, wavelengths, pixelsize

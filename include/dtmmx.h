@@ -1,0 +1,129 @@
+/*
+ * dtmmx -- C ABI of the MI355X diffractive transfer-matrix engine.
+ *
+ * The reference (IJSComplexMatter/dtmm) exposes this path only as Python:
+ * dtmm/transfer.py:322 transfer_field -> :618 transfer_4x4 / :1009
+ * transfer_2x2, whose layer loops (transfer.py:752-804, :1117-1174) call
+ * dtmm/propagate_4x4.py:376 propagate_4x4_effective_1 and
+ * dtmm/propagate_2x2.py:193 propagate_2x2_effective_1.  This header is the
+ * boundary those Python functions bind through ctypes.  Plain C types only:
+ * device buffers are raw pointers (e.g. torch.Tensor.data_ptr()), streams
+ * are hipStream_t passed as void*.
+ *
+ * Conventions
+ *   - Field buffers are complex (interleaved re, im), complex64 when
+ *     precision == DTMMX_C64 and complex128 when DTMMX_C128, laid out as
+ *     [n_rays][n_wavelengths][4][height][width] (the reference's
+ *     (..., wavelengths, 4, H, W) field_data layout, components
+ *     Ex, Hy, Ey, Hx; tmm.py:13).
+ *   - Material buffers are float64 structure-of-arrays
+ *     [3][n_layers][height][width]: eps_re / eps_im are the eigenvalues
+ *     of the dielectric tensor (eps_im may be NULL), angles are
+ *     (psi, theta, phi) in radians (data.py:534-618 block layout,
+ *     transposed component-first).
+ *   - Every call returns 0 on success or a negative DTMMX_E* code; the
+ *     text of the last error of the calling thread is available from
+ *     dtmmx_last_error().  Calls are asynchronous on the given stream.
+ *   - A plan is immutable after creation and may be used from several
+ *     threads on different streams, provided each call gets its own
+ *     workspace.
+ */
+#ifndef DTMMX_H
+#define DTMMX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DTMMX_ABI_VERSION 1
+
+enum {
+  DTMMX_OK = 0,
+  DTMMX_E_ARG = -1,         /* invalid argument                          */
+  DTMMX_E_UNSUPPORTED = -2, /* valid in the reference, not built here    */
+  DTMMX_E_HIP = -3,         /* HIP runtime error                         */
+  DTMMX_E_NOMEM = -4        /* device allocation failed                  */
+};
+
+enum { DTMMX_C64 = 0, DTMMX_C128 = 1 };
+enum { DTMMX_METHOD_2X2 = 2, DTMMX_METHOD_4X4 = 4 };
+
+#define DTMMX_MAX_WAVELENGTHS 64
+#define DTMMX_MAX_BEAMS 64
+
+/* Static description of a transfer problem: grid, wavelengths and beams.
+ * Rays: the field has n_rays outer planes (polarisations x beams);
+ * consecutive groups of rays_per_beam rays share one beam (beta, phi). */
+typedef struct {
+  int height, width;
+  int n_wavelengths;
+  int n_rays;
+  int rays_per_beam;
+  int method;     /* DTMMX_METHOD_2X2 or DTMMX_METHOD_4X4              */
+  int precision;  /* DTMMX_C64 or DTMMX_C128                            */
+  double betamax; /* transfer.py:440, dtmm.ini:6                        */
+  const double* k0;   /* [n_wavelengths] host: 2 pi pixelsize / lambda  */
+  const double* beta; /* [n_rays / rays_per_beam] host                  */
+  const double* phi;  /* [n_rays / rays_per_beam] host                  */
+} dtmmx_plan_desc;
+
+/* A layer stack (already split into nstep sub-layers, transfer.py:934). */
+typedef struct {
+  int n_layers;
+  const double* thickness;   /* [n_layers] host, pixel units            */
+  const double* eps_eff_re;  /* [n_layers] host: isotropic effective eps */
+  const double* eps_eff_im;  /* [n_layers] host or NULL                  */
+  const double* eps_re;      /* device [3][n_layers][H][W]               */
+  const double* eps_im;      /* device [3][n_layers][H][W] or NULL       */
+  const double* angles;      /* device [3][n_layers][H][W]               */
+  double n_in, n_out;        /* input/output media (transfer.py:936-939) */
+} dtmmx_stack;
+
+typedef struct dtmmx_plan dtmmx_plan;
+
+int dtmmx_abi_version(void);
+const char* dtmmx_last_error(void);
+
+/* Plan: validates the description, builds twiddle and betamax-mask
+ * tables on the current device.  Replaces the cached k-space matrix
+ * builders of dtmm/matrix.py:141-174 and dtmm/diffract.py:47-93. */
+int dtmmx_plan_create(dtmmx_plan** plan, const dtmmx_plan_desc* desc);
+int dtmmx_plan_destroy(dtmmx_plan* plan);
+
+/* Device scratch bytes one transfer call on this plan needs. */
+size_t dtmmx_plan_workspace_size(const dtmmx_plan* plan);
+
+/* Full transfer through the stack, npass = 1.
+ *   4x4 (reflection mode 1, diffraction 1): replaces transfer.py:694-875
+ *       with propagate_4x4.py:376-389 and :80-101 per layer.
+ *   2x2 (reflection mode 0, diffraction 1): replaces transfer.py:1071-1176
+ *       with propagate_2x2.py:193-211 and :19-87 per interface.
+ * field_in and field_out are device buffers of n_rays*n_wavelengths*4*H*W
+ * complex values; they may not alias.  field_in is not modified.
+ * status (device int[4], may be NULL) receives flags: status[0] bit 0 =
+ * a biaxial voxel was met (not built: its output is NaN). */
+int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, const void* field_in,
+                   void* field_out, void* workspace, size_t workspace_bytes, int* status,
+                   void* stream);
+
+/* In-place unnormalised 2D FFT of `batch` planes of H x W complex values
+ * (forward exp(-i..) when inverse == 0), the transform of dtmm/fft.py:277
+ * fft2 / :306 ifft2 without the 1/(HW) factor. */
+int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int precision,
+               void* stream);
+
+/* Host evaluation of the per-pixel eigen-solution used on the device
+ * (dtmm/tmm.py:435 alphaf for isotropic and uniaxial voxels), for checks
+ * without a GPU.  Inputs are n voxels; outputs alpha[n][4] and
+ * fmat[n][4][4] as interleaved complex128.  Returns DTMMX_E_UNSUPPORTED
+ * if any voxel is biaxial. */
+int dtmmx_host_alphaf(int n, const double* beta, const double* phi, const double* eps_re,
+                      const double* eps_im, const double* angles, double* alpha, double* fmat);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DTMMX_H */

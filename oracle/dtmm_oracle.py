@@ -1,0 +1,1038 @@
+"""Complex128 NumPy restatement of the reference diffractive transfer-matrix path.
+
+TEST INFRASTRUCTURE ONLY -- the checker for ``dtmm_amd``; never shipped or
+called by the product.  All citations are to the reference package
+(IJSComplexMatter/dtmm v0.7.0.dev), paths relative to its repository root.
+
+Scope (what the GPU engine implements and this module checks):
+
+* wave-vector grids ``betaphi``/``betaxy``/``k0``          (dtmm/wave.py)
+* per-pixel Berreman eigen-solution ``alphaf``             (dtmm/tmm.py, rotation.py)
+* isotropic-medium k-space diffraction/correction matrices (dtmm/diffract.py, matrix.py)
+* ``transfer_4x4`` with reflection=1, diffraction=1, npass=1 (dtmm/transfer.py,
+  dtmm/propagate_4x4.py)
+* ``transfer_2x2`` with reflection=0, diffraction in {0, 1}, npass=1
+  (dtmm/transfer.py, dtmm/propagate_2x2.py)
+* ``illumination_data`` and ``field2intensity``            (dtmm/field.py)
+
+Pinning of each function is done in ``tests/test_oracle_*.py`` (golden
+vectors of dtmm/test/test_wave.py, closed forms, invariants, brute force).
+The biaxial eigen branch (``_alphaf_biaxial``) is pinned only by the Berreman
+eigen-equation and the Poynting-sort invariants: the reference's own tests
+hold no vector for it.
+"""
+import numpy as np
+
+__all__ = [
+    "BETAMAX", "N_COVER", "k0", "betaphi", "betaxy", "alphaf", "alphaffi",
+    "inv", "dotmm", "dotmf", "dotmdmf", "dotmv", "phase_mat", "E_mat", "E2H_mat",
+    "normalize_f", "field2intensity", "fft2", "ifft2", "refind2eps",
+    "uniaxial_order", "validate_optical_data", "create_layers",
+    "projection_matrix", "field_diffraction_matrix", "diffract",
+    "transmitted_field", "first_corrected_Epn_diffraction_matrix",
+    "second_corrected_Epn_diffraction_matrix", "first_E_diffraction_matrix",
+    "second_E_diffraction_matrix", "propagate_4x4_effective_1",
+    "propagate_2x2_effective_1", "transfer_4x4", "transfer_2x2",
+    "transfer_field", "illumination_data", "blackman", "planewave",
+    "jonesvec", "avec", "fvec", "auxiliary_matrix", "poynting",
+]
+
+# dtmm/dtmm.ini:6 (betamax = 0.8) and dtmm/conf.py:148, :503-510 (nin/nout
+# default to n_cover = 1.5, dtmm.ini "n_cover = 1.5").
+BETAMAX = 0.8
+N_COVER = 1.5
+SMOOTH = 0.1
+
+CDTYPE = np.complex128
+FDTYPE = np.float64
+
+
+# ---------------------------------------------------------------------------
+# wave.py
+# ---------------------------------------------------------------------------
+
+def k0(wavelength, pixelsize=1.):
+    """Vacuum wave number in pixel units (dtmm/wave.py:280-296)."""
+    return np.asarray(2 * np.pi / np.asarray(wavelength) * pixelsize, dtype=FDTYPE)
+
+
+def betaphi(shape, k0):
+    """beta, phi of the FFT eigen-waves (dtmm/wave.py:54-81).
+
+    beta = (2 pi / |k0|) * sqrt(fx^2 + fy^2) with fftfreq grids, phi = atan2(fy, fx).
+    """
+    k0 = np.abs(np.asarray(k0, FDTYPE)[..., None, None])
+    ay, ax = [np.asarray(np.fft.fftfreq(x), dtype=FDTYPE) for x in shape[-2:]]
+    xx, yy = np.meshgrid(ax, ay, copy=False, indexing="xy")
+    beta = (2 * np.pi / k0) * np.sqrt(xx ** 2 + yy ** 2)
+    phi = np.arctan2(yy, xx)
+    return beta, phi
+
+
+def betaxy(shape, k0):
+    """betax, betay of the FFT eigen-waves (dtmm/wave.py:84-110)."""
+    k0 = np.asarray(k0, dtype=FDTYPE)[..., None, None]
+    ay, ax = [np.asarray(np.fft.fftfreq(x), dtype=FDTYPE) for x in shape[-2:]]
+    xx, yy = np.meshgrid(ax, ay, copy=False, indexing="xy")
+    l = 2 * np.pi / k0
+    return l * xx, l * yy
+
+
+def planewave(shape, k0, beta, phi):
+    """Plane wave normalised to 1 at pixel (0,0) (dtmm/wave.py:376-409)."""
+    k0 = np.asarray(k0)[..., None, None]
+    beta = np.asarray(beta)[..., None, None]
+    phi = np.asarray(phi)[..., None, None]
+    ay, ax = [np.arange(-l // 2 + 1., l // 2 + 1.) for l in shape[-2:]]
+    xx, yy = np.meshgrid(ax, ay, indexing="xy", copy=False)
+    kx = np.asarray(k0 * beta * np.cos(phi), dtype=FDTYPE)
+    ky = np.asarray(k0 * beta * np.sin(phi), dtype=FDTYPE)
+    out = np.exp(1j * (kx * xx + ky * yy))
+    return out / out[..., 0, 0][..., None, None]
+
+
+# ---------------------------------------------------------------------------
+# window.py
+# ---------------------------------------------------------------------------
+
+def blackman(shape):
+    """2D radial Blackman window (dtmm/window.py:17-54)."""
+    ny, nx = [l / 2 for l in shape]
+    ay, ax = [np.arange(-l / 2. + .5, l / 2. + .5) for l in shape]
+    xx, yy = np.meshgrid(ax, ay, indexing="xy")
+    r = ((xx / nx) ** 2 + (yy / ny) ** 2) ** 0.5
+    out = 0.42 + 0.5 * np.cos(np.pi * r) + 0.08 * np.cos(2 * np.pi * r)
+    out[r >= 1.] = 0.
+    return np.asarray(out, FDTYPE)
+
+
+# ---------------------------------------------------------------------------
+# linalg.py (as mathematics, not loops)
+# ---------------------------------------------------------------------------
+
+def inv(mat):
+    """Batched 4x4/2x2 inverse that returns zeros where det == 0
+    (dtmm/linalg.py:428-457 _inv2x2, :460-518 _inv4x4, :521-550 inv)."""
+    mat = np.asarray(mat, CDTYPE)
+    out = np.zeros_like(mat)
+    flat = mat.reshape((-1,) + mat.shape[-2:])
+    o = out.reshape(flat.shape)
+    det = np.linalg.det(flat)
+    ok = det != 0
+    if np.any(ok):
+        o[ok] = np.linalg.inv(flat[ok])
+    return out
+
+
+def dotmm(a, b):
+    """Matrix-matrix product (dtmm/linalg.py:1046)."""
+    return np.matmul(a, b)
+
+
+def dotmv(a, v):
+    """Matrix-vector product over the last axis (dtmm/linalg.py dotmv)."""
+    return np.matmul(a, np.asarray(v)[..., None])[..., 0]
+
+
+def dotmdm(a, d, b):
+    """a . diag(d) . b (dtmm/linalg.py:1233)."""
+    return np.matmul(a * np.asarray(d)[..., None, :], b)
+
+
+def _f2v(f):
+    return np.moveaxis(f, -3, -1)[..., None]
+
+
+def _v2f(v):
+    return np.moveaxis(v[..., 0], -1, -3)
+
+
+def dotmf(a, f):
+    """Per-pixel matrix times field (..., C, H, W) (dtmm/linalg.py:833-862, :979-991)."""
+    return _v2f(np.matmul(a, _f2v(f)))
+
+
+def dotmdmf(a, d, b, f):
+    """a . diag(d) . b times field (dtmm/linalg.py:905-942, :1002-1018)."""
+    v = np.matmul(b, _f2v(f))
+    v = v * np.asarray(d)[..., None]
+    return _v2f(np.matmul(a, v))
+
+
+def fft2(f):
+    """2D FFT over the last two axes (dtmm/fft.py:277)."""
+    return np.fft.fft2(f)
+
+
+def ifft2(f):
+    """2D inverse FFT over the last two axes (dtmm/fft.py:306)."""
+    return np.fft.ifft2(f)
+
+
+# ---------------------------------------------------------------------------
+# rotation.py
+# ---------------------------------------------------------------------------
+
+def _rotation_matrix_uniaxial(theta, phi):
+    """R = R_phi . R_theta (dtmm/rotation.py:176-193)."""
+    ct, st, cf, sf = np.cos(theta), np.sin(theta), np.cos(phi), np.sin(phi)
+    z = np.zeros_like(ct)
+    return np.stack([np.stack([ct * cf, -sf, cf * st], -1),
+                     np.stack([ct * sf, cf, st * sf], -1),
+                     np.stack([-st, z, ct], -1)], -2)
+
+
+def _rotation_matrix(psi, theta, phi):
+    """R = R_phi . R_theta . R_psi (dtmm/rotation.py:125-150)."""
+    cp, sp = np.cos(psi), np.sin(psi)
+    ct, st = np.cos(theta), np.sin(theta)
+    cf, sf = np.cos(phi), np.sin(phi)
+    return np.stack([
+        np.stack([ct * cf * cp - sf * sp, -ct * cf * sp - sf * cp, cf * st], -1),
+        np.stack([ct * sf * cp + cf * sp, cf * cp - ct * sf * sp, st * sf], -1),
+        np.stack([-cp * st, st * sp, ct], -1)], -2)
+
+
+def _rotate_diagonal_tensor(R, d):
+    """(eps_xx, eps_yy, eps_zz, eps_xy, eps_xz, eps_yz) of R.diag(d).R^T
+    (dtmm/rotation.py:285-292)."""
+    out = [sum(d[..., k] * R[..., i, k] * R[..., i, k] for k in range(3)) for i in range(3)]
+    out.append(sum(d[..., k] * R[..., 0, k] * R[..., 1, k] for k in range(3)))
+    out.append(sum(d[..., k] * R[..., 0, k] * R[..., 2, k] for k in range(3)))
+    out.append(sum(d[..., k] * R[..., 1, k] * R[..., 2, k] for k in range(3)))
+    return np.stack(out, -1)
+
+
+def _dotr2m(c, s, F):
+    """Rotate the (Ex, Hy, Ey, Hx) rows of F by the beam azimuth
+    (dtmm/linalg.py:674-713)."""
+    c = c[..., None]
+    s = s[..., None]
+    r0 = F[..., 0, :] * c - F[..., 2, :] * s
+    r1 = F[..., 1, :] * c + F[..., 3, :] * s
+    r2 = F[..., 0, :] * s + F[..., 2, :] * c
+    r3 = -F[..., 1, :] * s + F[..., 3, :] * c
+    return np.stack([r0, r1, r2, r3], -2)
+
+
+# ---------------------------------------------------------------------------
+# tmm.py -- Berreman eigen-solution
+# ---------------------------------------------------------------------------
+
+def auxiliary_matrix(beta, eps):
+    """Berreman Delta matrix for the (Ex, Hy, Ey, Hx) vector
+    (dtmm/tmm.py:99-121).  eps = (xx, yy, zz, xy, xz, yz)."""
+    beta = np.asarray(beta, FDTYPE)
+    e2m = 1. / eps[..., 2]
+    e4 = eps[..., 4] * e2m
+    e5 = eps[..., 5] * e2m
+    z = np.zeros(np.broadcast_shapes(beta.shape, e2m.shape), CDTYPE)
+    L00 = -beta * e4 + z
+    L01 = 1. - beta * beta * e2m + z
+    L02 = -beta * e5 + z
+    L10 = eps[..., 0] - eps[..., 4] * e4 + z
+    L12 = eps[..., 3] - eps[..., 5] * e4 + z
+    L32 = beta * beta + eps[..., 5] * e5 - eps[..., 1] + z
+    return np.stack([np.stack([L00, L01, L02, z], -1),
+                     np.stack([L10, L00, L12, z], -1),
+                     np.stack([z, z, z, z - 1.], -1),
+                     np.stack([-L12, -L02, L32, z], -1)], -2)
+
+
+def poynting(fvec):
+    """z Poynting component Re(Ex Hy*) - Re(Ey Hx*) (dtmm/tmm.py:292-297)."""
+    f = np.asarray(fvec)
+    return (f[..., 0] * np.conj(f[..., 1])).real - (f[..., 2] * np.conj(f[..., 3])).real
+
+
+def _alphaf_iso(beta, eps):
+    """Isotropic modes, F not normalised (dtmm/tmm.py:123-154)."""
+    n = beta.shape[0]
+    alpha = np.zeros((n, 4), CDTYPE)
+    F = np.zeros((n, 4, 4), CDTYPE)
+    aout = np.sqrt(eps[:, 0] - beta ** 2)
+    ok = aout != 0.
+    a = aout[ok]
+    gp = eps[ok, 0] / a
+    gs = -a
+    alpha[ok] = np.stack([a, -a, a, -a], -1)
+    F[ok, 0, 0] = 0.5
+    F[ok, 0, 1] = 0.5
+    F[ok, 1, 0] = 0.5 * gp
+    F[ok, 1, 1] = -0.5 * gp
+    F[ok, 2, 2] = 0.5
+    F[ok, 2, 3] = 0.5
+    F[ok, 3, 2] = 0.5 * gs
+    F[ok, 3, 3] = -0.5 * gs
+    return alpha, F
+
+
+def _alphaf_uniaxial(beta, eps, theta, phi):
+    """Uniaxial modes with unit-norm columns (dtmm/tmm.py:156-288), with the
+    rotation of dtmm/rotation.py:390-395 (phi already relative to the beam)."""
+    n = beta.shape[0]
+    R = _rotation_matrix_uniaxial(theta, phi)
+    ct = R[:, 2, 2]
+    st = -R[:, 2, 0]
+    sf = -R[:, 0, 1]
+    cf = R[:, 1, 1]
+    eps11 = eps[:, 0]
+    delta = eps[:, 2] - eps11
+    alpha = np.zeros((n, 4), CDTYPE)
+    F = np.zeros((n, 4, 4), CDTYPE)
+    z = beta == 0.
+    nz = ~z
+    # beta == 0 branch (tmm.py:172-179, :203-233)
+    if np.any(z):
+        e11 = eps11[z]
+        ev02 = e11
+        evs = np.sqrt(ev02)
+        u = e11 + delta[z] * ct[z] ** 2
+        w = e11 * (ev02 + delta[z])
+        sq = np.sqrt(u * w) / u
+        evpp, evpm = sq, -sq
+        alpha[z] = np.stack([evpp, evpm, evs, -evs], -1)
+        s, c = sf[z], cf[z]
+        F[z, :, 2] = np.stack([-evs * s, -e11 * s, evs * c, -e11 * c], -1)
+        F[z, :, 3] = np.stack([-evs * s, e11 * s, evs * c, e11 * c], -1)
+        F[z, :, 0] = np.stack([c + 0j, evpp * c, s + 0j, -evpp * s], -1)
+        F[z, :, 1] = np.stack([c + 0j, evpm * c, s + 0j, -evpm * s], -1)
+    # general branch (tmm.py:181-194, :235-276)
+    if np.any(nz):
+        b = beta[nz]
+        e11 = eps11[nz]
+        d = delta[nz]
+        ct_, st_, cf_ = ct[nz], st[nz], cf[nz]
+        ev02 = e11 - b * b
+        evs = np.sqrt(ev02)
+        u = e11 + d * ct_ ** 2
+        gama = b * cf_
+        v = gama * d * 2 * st_ * ct_
+        w = gama * gama * (d * st_ ** 2) - e11 * (ev02 + d)
+        sq = np.sqrt(v * v - 4 * u * w) / 2 / u
+        v = v / 2 / u
+        evpp = -v + sq
+        evpm = -v - sq
+        alpha[nz] = np.stack([evpp, evpm, evs, -evs], -1)
+        sfst = R[nz, 1, 2]
+        cfst = R[nz, 0, 2]
+        ctbeta = ct_ * b
+        ctbetaeps11 = ctbeta / e11
+        e11sfst = e11 * sfst
+        evssfst = evs * sfst
+        evscfst = evs * cfst
+        evsctbeta = evs * ctbeta
+        ev02cfst = ev02 * cfst
+        ev02cfsteps11 = ev02cfst / e11
+        F[nz, :, 2] = np.stack([evssfst, e11sfst, ctbeta - evscfst, ev02cfst - evsctbeta], -1)
+        F[nz, :, 3] = np.stack([-evssfst, e11sfst, evscfst + ctbeta, ev02cfst + evsctbeta], -1)
+        F[nz, :, 0] = np.stack([evpp * ctbetaeps11 - ev02cfsteps11, ctbeta - evpp * cfst,
+                                -sfst + 0j, evpp * sfst], -1)
+        F[nz, :, 1] = np.stack([-evpm * ctbetaeps11 + ev02cfsteps11, evpm * cfst - ctbeta,
+                                sfst + 0j, -evpm * sfst], -1)
+    norm = np.sqrt((F.real ** 2 + F.imag ** 2).sum(-2))
+    F = F / norm[:, None, :]
+    return alpha, F
+
+
+def _alphaf_biaxial(beta, eps, psi, theta, phi):
+    """General (biaxial) modes by numeric eigen-decomposition of the Berreman
+    matrix, sorted by Poynting sign (+ modes to columns 0, 2)
+    (dtmm/tmm.py:395-403, :299-369)."""
+    n = beta.shape[0]
+    R = _rotation_matrix(psi, theta, phi)
+    e6 = _rotate_diagonal_tensor(R, eps)
+    L = auxiliary_matrix(beta, e6)
+    a0, F0 = np.linalg.eig(L)
+    alpha = np.zeros((n, 4), CDTYPE)
+    F = np.zeros((n, 4, 4), CDTYPE)
+    p = poynting(np.swapaxes(F0, -1, -2))  # (n, 4): one value per column
+    for m in range(n):
+        i, j, ok = 0, 1, True
+        for k in range(4):
+            if p[m, k] >= 0.:
+                if i >= 4:
+                    ok = False
+                if ok:
+                    alpha[m, i] = a0[m, k]
+                    F[m, :, i] = F0[m, :, k]
+                i += 2
+            else:
+                if j >= 4:
+                    ok = False
+                if ok:
+                    alpha[m, j] = a0[m, k]
+                    F[m, :, j] = F0[m, :, k]
+                j += 2
+        if not ok:
+            alpha[m] = np.nan
+            F[m] = 0.
+    return alpha, F
+
+
+def alphaf(beta=0., phi=0., epsv=(1., 1., 1.), epsa=(0., 0., 0.)):
+    """Eigenvalues alpha (..., 4) and field matrix F (..., 4, 4)
+    (dtmm/tmm.py:372-403 _alphaf_vec, :435-464 alphaf).
+
+    Branch selection by exact equality: isotropic if eps0 == eps1 == eps2,
+    uniaxial if eps0 == eps1, biaxial otherwise (tmm.py:384-389).
+    """
+    beta = np.asarray(0. if beta is None else beta, FDTYPE)
+    phi = np.asarray(0. if phi is None else phi, FDTYPE)
+    epsv = np.asarray(epsv, CDTYPE)
+    epsa = np.asarray(epsa, FDTYPE)
+    shape = np.broadcast_shapes(beta.shape, phi.shape, epsv.shape[:-1], epsa.shape[:-1])
+    b = np.broadcast_to(beta, shape).ravel()
+    p = np.broadcast_to(phi, shape).ravel()
+    ev = np.broadcast_to(epsv, shape + (3,)).reshape(-1, 3)
+    ea = np.broadcast_to(epsa, shape + (3,)).reshape(-1, 3)
+    n = b.shape[0]
+    alpha = np.zeros((n, 4), CDTYPE)
+    F = np.zeros((n, 4, 4), CDTYPE)
+    iso = (ev[:, 0] == ev[:, 1]) & (ev[:, 1] == ev[:, 2])
+    uni = (~iso) & (ev[:, 0] == ev[:, 1])
+    bia = ~(iso | uni)
+    if np.any(iso):
+        alpha[iso], F[iso] = _alphaf_iso(b[iso], ev[iso])
+    if np.any(uni):
+        alpha[uni], F[uni] = _alphaf_uniaxial(b[uni], ev[uni], ea[uni, 1], ea[uni, 2] - p[uni])
+    if np.any(bia):
+        alpha[bia], F[bia] = _alphaf_biaxial(b[bia], ev[bia], ea[bia, 0], ea[bia, 1],
+                                             ea[bia, 2] - p[bia])
+    F = _dotr2m(np.cos(p), np.sin(p), F)
+    return alpha.reshape(shape + (4,)), F.reshape(shape + (4, 4))
+
+
+def alphaffi(beta=0., phi=0., epsv=(1., 1., 1.), epsa=(0., 0., 0.)):
+    """alphaf plus inverse of F (dtmm/tmm.py:466-505)."""
+    a, f = alphaf(beta, phi, epsv, epsa)
+    return a, f, inv(f)
+
+
+def phase_mat(alpha, kd, mode=None):
+    """exp(i kd alpha); mode +1/-1 keeps the even/odd modes
+    (dtmm/tmm.py:582-646)."""
+    alpha = np.asarray(alpha)
+    kd = np.asarray(kd, FDTYPE)[..., None]
+    if mode == +1:
+        alpha = alpha[..., ::2]
+    elif mode == -1:
+        alpha = alpha[..., 1::2]
+    return np.exp(1j * kd * alpha)
+
+
+def E_mat(fmat, mode=None):
+    """E-field part of F: +1 -> F[::2, ::2], -1 -> F[::2, 1::2], None -> the
+    interleaved 4x4 block matrix (dtmm/tmm.py:974-997)."""
+    if mode == +1:
+        return fmat[..., ::2, ::2].copy()
+    if mode == -1:
+        return fmat[..., ::2, 1::2].copy()
+    out = np.zeros_like(fmat)
+    out[..., ::2, ::2] = fmat[..., ::2, ::2]
+    out[..., 1::2, 1::2] = fmat[..., ::2, 1::2]
+    return out
+
+
+def E2H_mat(fmat, mode=+1):
+    """H = B.A^-1 E for the forward (+1) or backward (-1) modes
+    (dtmm/tmm.py:1051-1072)."""
+    if mode == +1:
+        A, B = fmat[..., ::2, ::2], fmat[..., 1::2, ::2]
+    else:
+        A, B = fmat[..., ::2, 1::2], fmat[..., 1::2, 1::2]
+    return np.matmul(B, inv(A))
+
+
+def normalize_f(fmat):
+    """Scale columns to unit |Poynting| (dtmm/tmm.py:699-724)."""
+    fmat = np.asarray(fmat, CDTYPE)
+    p = poynting(np.swapaxes(fmat, -1, -2))
+    n = np.abs(p) ** 0.5
+    with np.errstate(divide="ignore"):
+        s = np.where(n == 0., 0., 1. / np.where(n == 0., 1., n))
+    return fmat * s[..., None, :]
+
+
+def jonesvec(pol, phi=0.):
+    """Normalised Jones vector seen in a frame rotated by phi
+    (dtmm/jones.py:55-98)."""
+    pol = np.asarray(pol, CDTYPE)
+    phi = np.asarray(phi)
+    norm = (pol[..., 0] * pol[..., 0].conj() + pol[..., 1] * pol[..., 1].conj()) ** 0.5
+    pol = pol / norm[..., None]
+    c, s = np.cos(-phi), np.sin(-phi)
+    r = np.empty(c.shape + (2, 2), FDTYPE)
+    r[..., 0, 0] = c
+    r[..., 1, 1] = c
+    r[..., 0, 1] = -s
+    r[..., 1, 0] = s
+    return dotmv(r, pol)
+
+
+def avec(jones=(1, 0), amplitude=1., mode=+1):
+    """Mode amplitude vector (dtmm/tmm.py:1750-1806)."""
+    jones = np.asarray(jones)
+    amplitude = np.asarray(amplitude)
+    c, s = jones[..., 0], jones[..., 1]
+    shape = np.broadcast(c, amplitude).shape + (4,)
+    out = np.zeros(shape, CDTYPE)
+    if mode == +1:
+        out[..., 0] = c
+        out[..., 2] = s
+    else:
+        out[..., 1] = c
+        out[..., 3] = s
+    return out * amplitude[..., None]
+
+
+def fvec(fmat, jones=(1, 0), amplitude=1., mode=+1):
+    """Field vector F_norm . avec (dtmm/tmm.py:1808-1843, :1877-1904)."""
+    a = avec(jones, amplitude, mode)
+    return dotmv(normalize_f(fmat), a)
+
+
+# ---------------------------------------------------------------------------
+# data.py
+# ---------------------------------------------------------------------------
+
+def refind2eps(refind):
+    """eps = n^2 (dtmm/data.py:1059-1062)."""
+    return np.asarray(refind) ** 2
+
+
+def uniaxial_order(order, eps):
+    """Scale a diagonal tensor to the given uniaxial order
+    (dtmm/data.py:1215-1233)."""
+    eps = np.asarray(eps)
+    order = np.asarray(order, FDTYPE)
+    m = (eps[..., 0] + eps[..., 1] + eps[..., 2]) / 3.
+    delta = eps[..., 2] - (eps[..., 0] + eps[..., 1]) / 2.
+    o = np.broadcast_to(order, m.shape)
+    e1 = np.where(o == 0., m, m - 1. / 3. * o * delta)
+    e3 = np.where(o == 0., m, m + 2. / 3. * o * delta)
+    e1 = np.where(o >= 0, e1, eps[..., 0])
+    e2 = np.where(o >= 0, e1, eps[..., 1])
+    e3 = np.where(o >= 0, e3, eps[..., 2])
+    return np.stack([e1, e2, e3], -1)
+
+
+def validate_optical_block(block):
+    """Cast and add singleton axes so that a block broadcasts to
+    (L, H, W, 3) (dtmm/data.py:534-618; dispersive blocks not covered)."""
+    d, material, angles = block
+    d = np.asarray(d, FDTYPE)
+    material = np.asarray(material)
+    material = np.asarray(material, CDTYPE if np.iscomplexobj(material) else FDTYPE)
+    angles = np.asarray(angles, FDTYPE)
+    if d.ndim == 0:
+        d, material, angles = d[None], material[None], angles[None]
+    if material.ndim == 2:
+        material = material[:, None, None, :]
+    elif material.ndim == 3:
+        material = material[:, None, :, :]
+    return d, material, angles
+
+
+def validate_optical_data(data):
+    """A list of validated blocks (dtmm/data.py:620-643)."""
+    if isinstance(data, list):
+        return [validate_optical_block(b) for b in data]
+    return [validate_optical_block(data)]
+
+
+def _isotropic_effective_data(block):
+    """Per-layer isotropic effective medium: the volume mean of the
+    isotropic part of eps (dtmm/transfer.py:34-41)."""
+    d, material, angles = block
+    n = len(d)
+    epseff = uniaxial_order(0., material).mean(axis=(0, 1, 2))
+    epseff = np.broadcast_to(epseff, (n, 3)).copy()
+    aeff = np.zeros((n, 3), FDTYPE)
+    return d, epseff, aeff
+
+
+def create_layers(optical_data, nin, nout, nstep):
+    """Layer and effective-layer lists with the input/output media as
+    zero-thickness layers (dtmm/transfer.py:926-1005; eff_data=0 only)."""
+    blocks = validate_optical_data(optical_data)
+    layers, eff_layers = [], []
+    nb = len(blocks)
+    for ib, (d, epsv, epsa) in enumerate(blocks):
+        substeps = np.broadcast_to(np.asarray(nstep), (len(d),))
+        ev_b = np.broadcast_to(epsv, (len(d),) + epsv.shape[1:])
+        ea_b = np.broadcast_to(epsa, (len(d),) + epsa.shape[1:])
+        ls = [(int(n), (t / n, ev, ea)) for n, t, ev, ea in zip(substeps, d, ev_b, ea_b)]
+        de, ee, ae = _isotropic_effective_data((d, epsv, epsa))
+        es = [(int(n), (t / n, ev, ea)) for n, t, ev, ea in zip(substeps, de, ee, ae)]
+        if ib == 0:
+            ls.insert(0, (1, (0., np.broadcast_to(refind2eps([nin] * 3), ev_b[0].shape),
+                              np.zeros(3, FDTYPE))))
+            es.insert(0, (1, (0., refind2eps([nin] * 3), np.zeros(3, FDTYPE))))
+        if ib == nb - 1:
+            ls.append((1, (0., np.broadcast_to(refind2eps([nout] * 3), ev_b[0].shape),
+                           np.zeros(3, FDTYPE))))
+            es.append((1, (0., refind2eps([nout] * 3), np.zeros(3, FDTYPE))))
+        layers += ls
+        eff_layers += es
+    return layers, eff_layers
+
+
+# ---------------------------------------------------------------------------
+# diffract.py / matrix.py (isotropic or general effective media)
+# ---------------------------------------------------------------------------
+
+def _mask(beta, betamax):
+    return beta >= betamax
+
+
+def diffraction_alphaffi(shape, ks, epsv, epsa, betamax):
+    """Eigen data on the FFT grid, zeroed where beta >= betamax
+    (dtmm/diffract.py:47-76)."""
+    ks = np.abs(np.asarray(ks))
+    beta, phi = betaphi(shape, ks)
+    alpha, f = alphaf(beta, phi, epsv, epsa)
+    fi = inv(f)
+    m = _mask(beta, betamax)
+    fi[m] = 0.
+    f[m] = 0.
+    alpha[m] = 0.
+    return alpha, f, fi
+
+
+def diffraction_alphaf(shape, ks, epsv, epsa, betamax):
+    """(dtmm/diffract.py:22-42)."""
+    ks = np.abs(np.asarray(ks))
+    beta, phi = betaphi(shape, ks)
+    alpha, f = alphaf(beta, phi, epsv, epsa)
+    m = _mask(beta, betamax)
+    f[m] = 0.
+    alpha[m] = 0.
+    return alpha, f
+
+
+def _phase_matrix(alpha, kd, mode=None):
+    """(dtmm/diffract.py:96-105)."""
+    kd = np.asarray(kd, FDTYPE)
+    out = phase_mat(alpha, kd[..., None, None])
+    if mode in ("t", +1):
+        out[..., 1::2] = 0.
+    elif mode in ("r", -1):
+        out[..., ::2] = 0.
+    return out
+
+
+def field_diffraction_matrix(shape, ks, d=1., epsv=(1, 1, 1), epsa=(0, 0, 0.), mode="b",
+                             betamax=BETAMAX):
+    """F . P(kd) . F^-1 on the FFT grid (dtmm/diffract.py:107-119)."""
+    ks = np.asarray(ks, FDTYPE)
+    alpha, f, fi = diffraction_alphaffi(shape, ks, epsv, epsa, betamax)
+    pmat = _phase_matrix(alpha, ks * d, mode=mode)
+    return dotmdm(f, pmat, fi)
+
+
+def projection_matrix(shape, ks, epsv=(1, 1, 1), epsa=(0, 0, 0.), mode=+1, betamax=BETAMAX):
+    """F . diag(1 on +/- modes) . F^-1 (dtmm/diffract.py:242-251)."""
+    ks = np.asarray(ks, FDTYPE)
+    alpha, f, fi = diffraction_alphaffi(shape, ks, epsv, epsa, betamax)
+    pmat = _phase_matrix(alpha, np.zeros_like(ks), mode=mode)
+    return dotmdm(f, pmat, fi)
+
+
+def diffract(field, dmat):
+    """FFT -> per-k matrix -> IFFT (dtmm/diffract.py:254-299)."""
+    return ifft2(dotmf(dmat, fft2(field)))
+
+
+def transmitted_field(field, ks, n=1., betamax=BETAMAX):
+    """Forward-propagating part of a field in a medium of index n
+    (dtmm/transfer.py:238-242, :270-293)."""
+    pmat = projection_matrix(field.shape[-2:], ks, epsv=refind2eps([n] * 3),
+                             epsa=(0., 0., 0.), mode=+1, betamax=betamax)
+    return diffract(field, pmat)
+
+
+def _epn_correction_matrix(beta, phi, ks, d, epsv, epsa):
+    """E . P(-kd) . E^-1 for the mean beam (dtmm/matrix.py:132-139)."""
+    alpha, f = alphaf(beta, phi, epsv, epsa)
+    kd = -np.asarray(ks) * d
+    pmat = phase_mat(alpha, kd[..., None, None])
+    e = E_mat(f)
+    return dotmdm(e, pmat, inv(e))
+
+
+def _first_epn_diffraction_matrix(shape, ks, d, epsv, epsa, betamax):
+    """E . P(kd) . F^-1 on the FFT grid (dtmm/matrix.py:109-118)."""
+    ks = np.asarray(ks, FDTYPE)
+    alpha, f, fi = diffraction_alphaffi(shape, ks, epsv, epsa, betamax)
+    kd = ks * d
+    pmat = phase_mat(alpha, kd[..., None, None])
+    return dotmdm(E_mat(f), pmat, fi)
+
+
+def _second_epn_diffraction_matrix(shape, ks, d, epsv, epsa, betamax):
+    """F . P(kd) . E^-1 on the FFT grid (dtmm/matrix.py:120-130)."""
+    ks = np.asarray(ks, FDTYPE)
+    alpha, f = diffraction_alphaf(shape, ks, epsv, epsa, betamax)
+    kd = ks * d
+    ei = inv(E_mat(f))
+    pmat = phase_mat(alpha, kd[..., None, None])
+    return dotmdm(f, pmat, ei)
+
+
+def first_corrected_Epn_diffraction_matrix(shape, ks, beta, phi, d, epsv, epsa, betamax=BETAMAX):
+    """C . D1 (dtmm/matrix.py:141-157, window=None)."""
+    dmat = _first_epn_diffraction_matrix(shape, ks, d, epsv, epsa, betamax)
+    cmat = _epn_correction_matrix(beta, phi, ks, d, epsv, epsa)
+    return dotmm(cmat, dmat)
+
+
+def second_corrected_Epn_diffraction_matrix(shape, ks, beta, phi, d, epsv, epsa, betamax=BETAMAX):
+    """D2 . C (dtmm/matrix.py:159-174, window=None)."""
+    dmat = _second_epn_diffraction_matrix(shape, ks, d, epsv, epsa, betamax)
+    cmat = _epn_correction_matrix(beta, phi, ks, d, epsv, epsa)
+    return dotmm(dmat, cmat)
+
+
+def _alphaEEi(beta, phi, epsv, epsa, mode):
+    """(dtmm/tmm.py:509-579)."""
+    alpha, f = alphaf(beta, phi, epsv, epsa)
+    e = E_mat(f, mode=mode)
+    alpha = alpha[..., ::2] if mode == +1 else alpha[..., 1::2]
+    return alpha, e, inv(e)
+
+
+def _E_diffraction_matrix(shape, ks, d, epsv, epsa, mode, betamax):
+    """J . P(kd) . J^-1 on the FFT grid (dtmm/diffract.py:78-93, :172-180)."""
+    ks = np.asarray(ks, FDTYPE)
+    kabs = np.abs(ks)
+    beta, phi = betaphi(shape, kabs)
+    alpha, j, ji = _alphaEEi(beta, phi, epsv, epsa, mode)
+    m = _mask(beta, betamax)
+    ji[m] = 0.
+    j[m] = 0.
+    alpha[m] = 0.
+    pmat = _phase_matrix(alpha, ks * d)
+    return dotmdm(j, pmat, ji)
+
+
+def _E_correction_matrix(beta, phi, ks, d, epsv, epsa, mode):
+    """(dtmm/matrix.py:31-36)."""
+    alpha, j, ji = _alphaEEi(beta, phi, epsv, epsa, mode)
+    kd = -np.asarray(ks) * d
+    pmat = _phase_matrix(alpha, kd)
+    return dotmdm(j, pmat, ji)
+
+
+def first_E_diffraction_matrix(shape, ks, beta, phi, d, epsv, epsa, mode=+1, betamax=BETAMAX):
+    """C . D (dtmm/matrix.py:53-58)."""
+    dmat = _E_diffraction_matrix(shape, ks, d, epsv, epsa, mode, betamax)
+    cmat = _E_correction_matrix(beta, phi, ks, d, epsv, epsa, mode)
+    return dotmm(cmat, dmat)
+
+
+def second_E_diffraction_matrix(shape, ks, beta, phi, d, epsv, epsa, mode=+1, betamax=BETAMAX):
+    """D . C (dtmm/matrix.py:46-51)."""
+    dmat = _E_diffraction_matrix(shape, ks, d, epsv, epsa, mode, betamax)
+    cmat = _E_correction_matrix(beta, phi, ks, d, epsv, epsa, mode)
+    return dotmm(dmat, cmat)
+
+
+# ---------------------------------------------------------------------------
+# propagate_4x4.py / propagate_2x2.py
+# ---------------------------------------------------------------------------
+
+class _Cache:
+    """Memo for the per-layer k-space matrices; the reference caches the same
+    pure functions (dtmm/conf.py:224-330), so this changes no number."""
+
+    def __init__(self):
+        self.d = {}
+
+    def get(self, key, fn):
+        if key not in self.d:
+            if len(self.d) > 8:
+                self.d.clear()
+            self.d[key] = fn()
+        return self.d[key]
+
+
+def _key(*args):
+    out = []
+    for a in args:
+        a = np.asarray(a)
+        out.append((a.shape, a.dtype.str, a.tobytes()))
+    return tuple(out)
+
+
+def _transfer_ray_4x4_1(field, ks, layer, dmat1, dmat2, beta, phi, nsteps):
+    """D1 -> IFFT -> e.P.e^-1 -> FFT -> D2 per substep
+    (dtmm/propagate_4x4.py:80-101)."""
+    d, epsv, epsa = layer
+    kd = ks * d
+    alpha, f = alphaf(beta, phi, epsv, epsa)
+    p = phase_mat(alpha, kd[..., None, None])
+    e = E_mat(f)
+    ei = inv(e)
+    for _ in range(nsteps):
+        field = dotmf(dmat1, field)
+        field = ifft2(field)
+        field = dotmdmf(e, p, ei, field)
+        field = fft2(field)
+        field = dotmf(dmat2, field)
+    return field
+
+
+def propagate_4x4_effective_1(field, ks, layer, effective_layer, beta=0, phi=0, nsteps=1,
+                              diffraction=1, betamax=BETAMAX, cache=None):
+    """Layer step of the 4x4 method, reflection mode 1
+    (dtmm/propagate_4x4.py:376-444; diffraction == 1 only)."""
+    if diffraction != 1:
+        raise NotImplementedError("oracle covers diffraction == 1 for the 4x4 method")
+    d_eff, epsv_eff, epsa_eff = effective_layer
+    shape = field.shape[-2:]
+    cache = cache or _Cache()
+    key = _key(shape, ks, beta, phi, d_eff, epsv_eff, epsa_eff, betamax)
+    dmat1 = cache.get(("d1",) + key, lambda: first_corrected_Epn_diffraction_matrix(
+        shape, ks, beta, phi, d_eff / 2, epsv_eff, epsa_eff, betamax))
+    dmat2 = cache.get(("d2",) + key, lambda: second_corrected_Epn_diffraction_matrix(
+        shape, ks, beta, phi, d_eff / 2, epsv_eff, epsa_eff, betamax))
+    return _transfer_ray_4x4_1(field, ks, layer, dmat1, dmat2, beta, phi, nsteps)
+
+
+def _transfer_ray_2x2_1(fft_field, ks, layer, dmat1, dmat2, beta, phi, nsteps, mode, bulk):
+    """Forward 2x2 layer step without reflections
+    (dtmm/propagate_2x2.py:19-87 with reflection=0, refl=None)."""
+    d, epsv, epsa = layer
+    alpha, fmat = alphaf(beta, phi, epsv, epsa)
+    e = E_mat(fmat, mode=mode)
+    ei = inv(e)
+    kd = ks * d
+    p = phase_mat(alpha, kd[..., None, None], mode=mode)
+    for _ in range(nsteps):
+        if dmat1 is not None:
+            fft_field = dotmf(dmat1, fft_field)
+        field = ifft2(fft_field)
+        field = dotmdmf(e, p, ei, field)
+        fft_field = fft2(field)
+        if dmat2 is not None:
+            fft_field = dotmf(dmat2, fft_field)
+    if mode == +1 and bulk is not None:
+        field = ifft2(fft_field)
+        e2h = E2H_mat(fmat, mode=mode)
+        bulk[..., 1::2, :, :] += dotmf(e2h, field)
+        bulk[..., ::2, :, :] += field
+    return fft_field
+
+
+def propagate_2x2_effective_1(field, ks, layer_out, effective_layer_out, beta=0, phi=0,
+                              nsteps=1, diffraction=1, betamax=BETAMAX, mode=+1, bulk=None,
+                              cache=None):
+    """Layer step of the 2x2 method, reflection mode 0
+    (dtmm/propagate_2x2.py:193-211; diffraction in {0, 1})."""
+    d_out, epsv_out, epsa_out = effective_layer_out
+    shape = field.shape[-2:]
+    if diffraction == 1:
+        cache = cache or _Cache()
+        key = _key(shape, ks, beta, phi, d_out, epsv_out, epsa_out, betamax, mode)
+        dmat1 = cache.get(("e1",) + key, lambda: first_E_diffraction_matrix(
+            shape, ks, beta, phi, d_out / 2, epsv_out, epsa_out, mode, betamax))
+        dmat2 = cache.get(("e2",) + key, lambda: second_E_diffraction_matrix(
+            shape, ks, beta, phi, d_out / 2, epsv_out, epsa_out, mode, betamax))
+    elif diffraction == 0:
+        dmat1 = dmat2 = None
+    else:
+        raise NotImplementedError("oracle covers diffraction in {0, 1} for the 2x2 method")
+    return _transfer_ray_2x2_1(field, ks, layer_out, dmat1, dmat2, beta, phi, nsteps, mode, bulk)
+
+
+# ---------------------------------------------------------------------------
+# transfer.py
+# ---------------------------------------------------------------------------
+
+def _validate_betaphi(beta, phi, extendeddim=0):
+    """(dtmm/transfer.py:43-65)."""
+    if beta is None or phi is None:
+        raise ValueError("Both beta and phi must be defined!")
+    beta = np.asarray(beta, FDTYPE)
+    phi = np.asarray(phi, FDTYPE)
+    if beta.ndim != phi.ndim:
+        raise ValueError("Beta nad phi should have same dimensions!")
+    if beta.ndim == 1:
+        if len(beta) != len(phi):
+            raise ValueError("Beta nad phi should have same length!")
+        for _ in range(extendeddim):
+            beta = beta[..., None]
+            phi = phi[..., None]
+    elif beta.ndim != 0:
+        raise ValueError("Only length 1 or scalar values are supported for beta and phi")
+    else:
+        for _ in range(extendeddim + 1):
+            beta = beta[..., None]
+            phi = phi[..., None]
+    return beta, phi
+
+
+def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1,
+                 nstep=1, diffraction=1, reflection=1, betamax=BETAMAX, layer_callback=None):
+    """4x4 transfer, npass=1, reflection mode 1 (dtmm/transfer.py:618-884).
+
+    The input is not projected onto forward modes and the input/output media
+    are not propagated (transfer.py:694, :700).
+    """
+    if npass != 1 or reflection != 1:
+        raise NotImplementedError("oracle covers npass=1, reflection=1 for the 4x4 method")
+    field_in, wavelengths, pixelsize = field_data
+    field_in = np.asarray(field_in, CDTYPE)
+    layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
+    ks = k0(wavelengths, pixelsize)
+    n = len(layers)
+    beta, phi = _validate_betaphi(beta, phi, extendeddim=field_in.ndim - 2)
+    cache = _Cache()
+    field = fft2(field_in)
+    for j in range(1, n - 1):
+        nsub, layer = layers[j]
+        _, eff = eff_layers[j]
+        field = propagate_4x4_effective_1(field, ks, layer, eff, beta, phi, nsub,
+                                          diffraction, betamax, cache)
+        if layer_callback is not None:
+            layer_callback(j)
+    return ifft2(field), wavelengths, pixelsize
+
+
+def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1, nstep=1,
+                 diffraction=1, reflection=0, betamax=BETAMAX, layer_callback=None):
+    """2x2 (E-field) transfer, npass=1, reflection mode 0
+    (dtmm/transfer.py:1009-1183)."""
+    if npass != 1 or reflection != 0:
+        raise NotImplementedError("oracle covers npass=1, reflection=0 for the 2x2 method")
+    field_in, wavelengths, pixelsize = field_data
+    field_in = np.asarray(field_in, CDTYPE)
+    layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
+    ks = k0(wavelengths, pixelsize)
+    n = len(layers) - 1
+    beta, phi = _validate_betaphi(beta, phi, extendeddim=field_in.ndim - 2)
+    field_out = np.zeros_like(field_in)
+    if diffraction > 0:
+        field0 = transmitted_field(field_in, ks, n=nin, betamax=betamax)
+    else:
+        field0 = _transmitted_field_direct(field_in, beta, phi, n=nin)
+    field = field0[..., ::2, :, :].copy()
+    if diffraction == 0:
+        raise NotImplementedError("oracle covers diffraction == 1 for the 2x2 method transfer")
+    field = fft2(field)
+    cache = _Cache()
+    for j in range(n):
+        nsub, layer_out = layers[j + 1]
+        _, eff_out = eff_layers[j + 1]
+        bulk = field_out if j + 1 == n else None
+        field = propagate_2x2_effective_1(field, ks, layer_out, eff_out, beta, phi, nsub,
+                                          diffraction, betamax, +1, bulk, cache)
+        if layer_callback is not None:
+            layer_callback(j)
+    return field_out, wavelengths, pixelsize
+
+
+def _transmitted_field_direct(field, beta, phi, n=1.):
+    """(dtmm/transfer.py:94-101)."""
+    ev = refind2eps([n] * 3)
+    alpha, fmat = alphaf(beta, phi, ev, np.zeros_like(ev))
+    diag = np.zeros(fmat.shape[:-1], CDTYPE)
+    diag[..., 0::2] = 1.
+    pmat = dotmdm(fmat, diag, inv(fmat))
+    return _v2f(np.matmul(pmat[..., None, None, :, :], _f2v(field)))
+
+
+def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
+                   npass=None, nstep=1, diffraction=None, reflection=None, method=None,
+                   betamax=None, layer_callback=None):
+    """Defaults and dispatch of dtmm.transfer_field (dtmm/transfer.py:322-615)
+    for the covered modes.  The configured default method is the quoted string
+    '"2x2"' (dtmm/dtmm.ini:23 read by configparser), which makes the automatic
+    reflection mode 1 (transfer.py:459)."""
+    nin = N_COVER if nin is None else nin
+    nout = N_COVER if nout is None else nout
+    method = '"2x2"' if method is None else method
+    npass = 1 if npass is None else npass
+    diffraction = 1 if diffraction is None else diffraction
+    betamax = BETAMAX if betamax is None else betamax
+    if reflection is None:
+        reflection = 0 if method == "2x2" else 1
+        if method == "4x4" and diffraction == 0:
+            reflection = 2
+        if npass > 1:
+            reflection = 1
+            if diffraction > 1 and method == "2x2":
+                reflection = 2
+    if method == "4x4":
+        return transfer_4x4(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
+                            diffraction, reflection, betamax, layer_callback)
+    return transfer_2x2(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
+                        diffraction, reflection, betamax, layer_callback)
+
+
+# ---------------------------------------------------------------------------
+# field.py
+# ---------------------------------------------------------------------------
+
+def field2intensity(field):
+    """Poynting z per pixel of a (..., 4, H, W) field (dtmm/field.py:771-799)."""
+    f = np.asarray(field)
+    return (f[..., 0, :, :] * np.conj(f[..., 1, :, :])).real - \
+        (f[..., 2, :, :] * np.conj(f[..., 3, :, :])).real
+
+
+def _diffracted_field(field, ks, d=0., n=1., mode="t", betamax=BETAMAX):
+    """(dtmm/diffract.py:301-304)."""
+    pmat = field_diffraction_matrix(field.shape[-2:], ks, d=d, epsv=refind2eps([n] * 3),
+                                    epsa=(0., 0., 0.), mode=mode, betamax=betamax)
+    return diffract(field, pmat)
+
+
+def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensity=1.,
+                      n=None, focus=0., window=None, jones=None, diffraction=True,
+                      betamax=BETAMAX):
+    """Forward-propagating input field data (dtmm/field.py:586-713).
+    Only the windowed (non-eigenmode) construction is restated; parity
+    unpinned for window=None (eigen-snapping, field.py:686-687)."""
+    if window is None:
+        raise NotImplementedError("oracle restates the windowed illumination only")
+    n = N_COVER if n is None else n
+    wavelengths = np.asarray(wavelengths)
+    wavenumbers = 2 * np.pi / wavelengths * pixelsize
+    if jones is None:
+        intensity = intensity / 2.
+    mode = +1
+    _beta = np.asarray(beta, FDTYPE)
+    _phi = np.asarray(phi, FDTYPE)
+    _intensity = np.asarray(intensity, FDTYPE)
+    nrays = len(_beta) if _beta.ndim > 0 else 1
+    b = _beta[:, None] if _beta.ndim == 1 else _beta
+    p = _phi[:, None] if _phi.ndim == 1 else _phi
+    bb = b if b.ndim >= 1 else b[..., None]
+    pp = p if p.ndim >= 1 else p[..., None]
+    waves = planewave(shape, wavenumbers, bb, pp) * window
+    b = _beta[:, None, None, None] if _beta.ndim == 1 else _beta[..., None, None]
+    p = _phi[:, None, None, None] if _phi.ndim == 1 else _phi[..., None, None]
+    inten = _intensity[..., None, None, None, None]
+    alpha, fmat = alphaf(b, p, refind2eps([n] * 3), np.zeros(3))
+    if jones is None:
+        f1 = np.moveaxis(fvec(fmat, jonesvec((1, 0), p), waves, mode), -1, -3)
+        f2 = np.moveaxis(fvec(fmat, jonesvec((0, 1), p), waves, mode), -1, -3)
+        shp = list(f1.shape)
+        shp.insert(-4, 2)
+        field = np.empty(shp, CDTYPE)
+        field[..., 0, :, :, :, :] = f1
+        field[..., 1, :, :, :, :] = f2
+    else:
+        field = np.moveaxis(fvec(fmat, jonesvec(jones, p), waves, mode), -1, -3).copy()
+    i1 = field2intensity(field)
+    norm = np.ones_like(i1)
+    norm[:, ...] = (inten / nrays) ** 0.5
+    if diffraction:
+        field = _diffracted_field(field, wavenumbers, d=-focus, n=n, mode=mode, betamax=betamax)
+        i2 = field2intensity(field)
+        ratio = (i1.sum((-2, -1)) / i2.sum((-2, -1))) ** 0.5
+        norm = norm * ratio[..., None, None]
+    field = norm[..., None, :, :] * field
+    return field, wavelengths, pixelsize
