@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the diffractive transfer-matrix path on one MI355X.
+
+Workload (the 512x512x128 scaled-up plain-method case): smooth random
+nematic texture (seed 0; tilt and twist as sums of 16 random Fourier modes,
+|k| <= 8 cycles per box, amplitudes N(0, 0.3) rad, planar base tilt and a
+90 degree linear twist), eps_o = 1.5^2, eps_e = 1.7^2, pixel 100 nm, layer
+thickness 1 px, 9 wavelengths 380-780 nm, x and y polarisations at normal
+incidence (18 field planes), diffraction = 1, npass = 1.
+
+A step is one full transfer_field pass (all layers, all planes) with the
+inputs resident in HBM.  value = voxel*wavelength*plane steps per second:
+H*W*L*n_wavelengths*n_polarisations / seconds per pass.
+
+Prints ONE JSON line.  Options: --method 4x4|2x2, --precision c64|c128.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def build_inputs(size, layers, nlam, seed=0):
+    import dtmm_amd as dm
+    from dtmm_amd.samples import random_director_data
+    t = time.time()
+    optical = [random_director_data((layers, size, size), seed=seed, no=1.5, ne=1.7)]
+    wl = np.linspace(380., 780., nlam)
+    fd = dm.illumination_data((size, size), wl, pixelsize=100, beta=0., phi=0., jones=None)
+    log("inputs built in %.1f s: field %s" % (time.time() - t, fd[0].shape))
+    return fd, optical
+
+
+def cpu_baseline(fd, optical, method, target_s=15.0):
+    """Oracle (complex128 NumPy restatement) on the first n layers of the
+    same workload: per-pixel math on one thread, FFTs on all cores."""
+    import oracle
+    field, wl, ps = fd
+    d, eps, ang = optical[0]
+    cores = len(os.sched_getaffinity(0))
+    oracle.set_fft_workers(cores)
+
+    def run(n):
+        t = time.time()
+        oracle.transfer_field((field, wl, ps), [(d[:n], eps[:n], ang[:n])], beta=0., phi=0.,
+                              method=method)
+        return time.time() - t
+
+    # the first layer also builds the k-space matrices (cached by the
+    # reference as well), so the rate is the marginal time of further layers
+    t1 = run(1)
+    n = 2 if t1 > target_s / 3 else int(max(2, min(len(d), 1 + target_s / max(t1, 1e-3))))
+    tn = run(n)
+    per_layer = max(tn - t1, 1e-9) / (n - 1)
+    H, W = field.shape[-2:]
+    planes = int(np.prod(field.shape[:-3]))
+    steps = H * W * planes
+    return {"value": steps / per_layer, "unit": "voxel*lambda steps/s", "cores": cores,
+            "kind": "port",
+            "sample": "oracle %s transfer of the first 1 and %d of %d layers, all %d planes of "
+                      "%dx%d (%.1f s and %.1f s; rate from the %d extra layers); per-pixel "
+                      "math 1 thread, scipy.fft on %d threads"
+                      % (method, n, len(d), planes, H, W, t1, tn, n - 1, cores)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--method", default="4x4", choices=["4x4", "2x2"])
+    ap.add_argument("--precision", default="c64", choices=["c64", "c128"])
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--layers", type=int, default=128)
+    ap.add_argument("--wavelengths", type=int, default=9)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-ref", action="store_true", help="skip the c128 comparison run")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has one GPU and the "
+              "path shards over independent field planes with no communication, so multi-GPU "
+              "runs are not part of this build (got --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import dtmm_amd as dm
+    from dtmm_amd import _lib
+    from dtmm_amd.transfer import Plan, Stack
+
+    if not torch.cuda.is_available():
+        log("no GPU")
+        sys.exit(1)
+    lib = _lib.load()
+    fd, optical = build_inputs(args.size, args.layers, args.wavelengths)
+    field, wl, ps = fd
+    H, W = field.shape[-2:]
+    nlam = len(wl)
+    n_rays = int(np.prod(field.shape[:-4]))
+    planes = n_rays * nlam
+    ks = dm.k0(wl, ps)
+
+    t = time.time()
+    stack = Stack(optical, (H, W), nstep=1, nin=1.5, nout=1.5)
+    L = stack.n_layers
+    log("stack uploaded in %.1f s (%d layers)" % (time.time() - t, L))
+
+    def make(precision):
+        plan = Plan((H, W), ks, [0.], [0.], n_rays, n_rays, args.method, precision, 0.8)
+        host = np.ascontiguousarray(field.reshape((n_rays, nlam, 4, H, W)))
+        fin = torch.from_numpy(host).to(device="cuda", dtype=plan.dtype).contiguous()
+        fout = torch.empty_like(fin)
+        ws = torch.empty(max(plan.workspace_bytes, 1), dtype=torch.uint8, device="cuda")
+        return plan, fin, fout, ws
+
+    plan, fin, fout, ws = make(args.precision)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        plan.execute(stack, fin, fout, ws, stream=stream)
+    torch.cuda.synchronize()
+
+    _lib.check(lib.dtmmx_timing_begin(8 * args.steps * (L + 2)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute(stack, fin, fout, ws, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (ctypes.c_double * 2)()
+    cnt = (ctypes.c_int * 2)()
+    _lib.check(lib.dtmmx_timing_end(ms, cnt, 2))
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    steps = H * W * L * planes
+    value = steps / (ms_per_step / 1e3)
+    nc = 4 if args.method == "4x4" else 2
+    cb = 8 if args.precision == "c64" else 16
+    out_np = fout.cpu().numpy()
+    if not np.all(np.isfinite(out_np)):
+        log("non-finite output")
+        sys.exit(1)
+
+    # Roofline of the dominant per-layer kernel: algorithmic bytes per launch
+    # = field read + write of every plane (2 * nc * cb per pixel per plane),
+    # plus the row kernel's material read (eps0..2 + theta, phi as float64,
+    # 40 B per pixel).
+    field_bytes = 2 * nc * cb * H * W * planes
+    kern = {
+        "row_layer": {"ms": ms[0], "n": cnt[0], "bytes": field_bytes + 40 * H * W},
+        "col_kspace": {"ms": ms[1], "n": cnt[1], "bytes": field_bytes},
+    }
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    avg_ms = kern[dom]["ms"] / max(kern[dom]["n"], 1)
+    achieved = kern[dom]["bytes"] / (avg_ms / 1e3) / 1e9
+    log("kernels: " + ", ".join("%s %d x %.3f ms" % (k, v["n"], v["ms"] / max(v["n"], 1))
+                                 for k, v in kern.items()))
+
+    extra = {}
+    if not args.no_ref and args.precision == "c64":
+        plan2, fin2, fout2, ws2 = make("c128")
+        plan2.execute(stack, fin2, fout2, ws2, stream=stream)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        plan2.execute(stack, fin2, fout2, ws2, stream=stream)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        ref = fout2.cpu().numpy()
+        err = float(np.abs(out_np - ref).max() / np.abs(ref).max())
+        i64 = dm.field2intensity(out_np.astype(np.complex128))
+        iref = dm.field2intensity(ref)
+        ierr = float(np.abs(i64 - iref).max() / np.abs(iref).max())
+        extra = {"c128_value": steps / (t3 - t2), "c64_over_c128": (t3 - t2) * 1e3 / ms_per_step,
+                 "rel_field_err_vs_c128": err, "rel_intensity_err_vs_c128": ierr}
+        del plan2, fin2, fout2, ws2
+        log("c128 pass %.1f ms, c64 vs c128 rel field err %.2e" % ((t3 - t2) * 1e3, err))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(fd, optical, args.method, args.cpu_seconds)
+        log("cpu baseline: %s" % cpu["sample"])
+
+    rec = {
+        "metric": "voxel*lambda steps/s (transfer_field, 4x4 unless noted), c64 with c128 "
+                  "reference, % HBM peak",
+        "value": value,
+        "unit": "voxel*lambda steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (fp32 complex, fp64 eigensolve)" if args.precision == "c64" else "c128",
+        "data": "synthetic (seeded random nematic director, plane-wave illumination)",
+        "config": {"workload": "%dx%dx%d random director, %d wavelengths x 2 pol, %s, "
+                               "diffraction=1, npass=1" % (H, W, L, nlam, args.method),
+                   "height": H, "width": W, "layers": L, "wavelengths": nlam, "planes": planes,
+                   "method": args.method, "precision": args.precision},
+        "hbm_frac_per_step": (steps * 2 * nc * cb / (ms_per_step / 1e3) / 1e9) / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "bytes_per_launch": kern[dom]["bytes"]},
+        "cpu_baseline": cpu,
+    }
+    rec.update(extra)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -31,6 +31,8 @@ EXPORTS = (
     "dtmmx_transfer",
     "dtmmx_fft2",
     "dtmmx_host_alphaf",
+    "dtmmx_timing_begin",
+    "dtmmx_timing_end",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -104,6 +106,10 @@ def load():
     lib.dtmmx_fft2.restype = ctypes.c_int
     lib.dtmmx_host_alphaf.argtypes = [ctypes.c_int] + [_dp] * 7
     lib.dtmmx_host_alphaf.restype = ctypes.c_int
+    lib.dtmmx_timing_begin.argtypes = [ctypes.c_int]
+    lib.dtmmx_timing_begin.restype = ctypes.c_int
+    lib.dtmmx_timing_end.argtypes = [_dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    lib.dtmmx_timing_end.restype = ctypes.c_int
     if lib.dtmmx_abi_version() != 1:
         raise RuntimeError("dtmmx ABI version mismatch")
     _lib = lib

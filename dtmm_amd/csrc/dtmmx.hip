@@ -581,6 +581,28 @@ struct dtmmx_plan {
 
 namespace dtmmx {
 
+// Optional per-launch timing of the per-layer kernels with HIP events on the
+// launch stream (class 0: row layer kernel, 1: column k-space kernel).
+struct KTimer {
+  bool on = false;
+  int cap = 0, used = 0;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> cls;
+};
+static KTimer g_timer;
+
+static void timer_start(int cls, hipStream_t s) {
+  if (!g_timer.on || g_timer.used >= g_timer.cap) return;
+  g_timer.cls[g_timer.used] = cls;
+  (void)hipEventRecord(g_timer.ev[2 * g_timer.used], s);
+}
+
+static void timer_stop(hipStream_t s) {
+  if (!g_timer.on || g_timer.used >= g_timer.cap) return;
+  (void)hipEventRecord(g_timer.ev[2 * g_timer.used + 1], s);
+  ++g_timer.used;
+}
+
 template <typename T>
 static int row_pass(const dtmmx_plan* P, cpx<T>* field, int nplanes, int nc, int op,
                     hipStream_t s, const RowArgs<T>* proto = nullptr, int ppg = 0) {
@@ -611,10 +633,15 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, int nplanes, int nc, int
   if (nc == 1 && op == OP_INV) return launch_row_n<T, 1, OP_INV>(w, a, grid, s);
   if (nc == 2 && op == OP_FWD) return launch_row_n<T, 2, OP_FWD>(w, a, grid, s);
   if (nc == 2 && op == OP_INV) return launch_row_n<T, 2, OP_INV>(w, a, grid, s);
-  if (nc == 2 && op == OP_ROP) return launch_row_n<T, 2, OP_ROP>(w, a, grid, s);
+  if (op == OP_ROP) {
+    timer_start(0, s);
+    int r = nc == 2 ? launch_row_n<T, 2, OP_ROP>(w, a, grid, s)
+                    : launch_row_n<T, 4, OP_ROP>(w, a, grid, s);
+    timer_stop(s);
+    return r;
+  }
   if (nc == 4 && op == OP_FWD) return launch_row_n<T, 4, OP_FWD>(w, a, grid, s);
   if (nc == 4 && op == OP_INV) return launch_row_n<T, 4, OP_INV>(w, a, grid, s);
-  if (nc == 4 && op == OP_ROP) return launch_row_n<T, 4, OP_ROP>(w, a, grid, s);
   return fail(DTMMX_E_ARG, "bad row pass");
 }
 
@@ -633,7 +660,12 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, int nlines, int nc, int 
   const int h = a.H;
   if (op == OP_FWD) return launch_col_n<T, OP_FWD>(h, a, nlines, s);
   if (op == OP_INV) return launch_col_n<T, OP_INV>(h, a, nlines, s);
-  if (op == OP_DIAG) return launch_col_n<T, OP_DIAG>(h, a, nlines, s);
+  if (op == OP_DIAG) {
+    timer_start(1, s);
+    int r = launch_col_n<T, OP_DIAG>(h, a, nlines, s);
+    timer_stop(s);
+    return r;
+  }
   return fail(DTMMX_E_ARG, "bad column pass");
 }
 
@@ -926,6 +958,47 @@ int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, v
   hipStream_t s = (hipStream_t)stream;
   if (P->d.precision == DTMMX_C64) return transfer_impl<float>(P, S, fin, fout, ws, status, s);
   return transfer_impl<double>(P, S, fin, fout, ws, status, s);
+}
+
+int dtmmx_timing_begin(int capacity) {
+  if (capacity <= 0) return fail(DTMMX_E_ARG, "capacity must be positive");
+  for (auto e : g_timer.ev) (void)hipEventDestroy(e);
+  g_timer.ev.assign(2 * size_t(capacity), nullptr);
+  g_timer.cls.assign(capacity, -1);
+  for (auto& e : g_timer.ev) HIP_TRY(hipEventCreate(&e));
+  g_timer.cap = capacity;
+  g_timer.used = 0;
+  g_timer.on = true;
+  return DTMMX_OK;
+}
+
+int dtmmx_timing_end(double* ms, int* count, int nclass) {
+  if (!g_timer.on) return fail(DTMMX_E_ARG, "timing not started");
+  g_timer.on = false;
+  for (int c = 0; c < nclass; ++c) {
+    ms[c] = 0.;
+    count[c] = 0;
+  }
+  int rc = DTMMX_OK;
+  for (int i = 0; i < g_timer.used; ++i) {
+    float t = 0.f;
+    hipError_t e = hipEventSynchronize(g_timer.ev[2 * i + 1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, g_timer.ev[2 * i], g_timer.ev[2 * i + 1]);
+    if (e != hipSuccess) {
+      rc = fail(DTMMX_E_HIP, std::string("timing: ") + hipGetErrorString(e));
+      break;
+    }
+    const int c = g_timer.cls[i];
+    if (c >= 0 && c < nclass) {
+      ms[c] += t;
+      count[c] += 1;
+    }
+  }
+  for (auto e : g_timer.ev) (void)hipEventDestroy(e);
+  g_timer.ev.clear();
+  g_timer.cls.clear();
+  g_timer.cap = g_timer.used = 0;
+  return rc;
 }
 
 int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int precision,

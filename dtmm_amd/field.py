@@ -213,5 +213,5 @@ def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensi
         i2 = field2intensity(field)
         ratio = (i1.sum((-2, -1)) / i2.sum((-2, -1))) ** 0.5
         norm = norm * ratio[..., None, None]
-    field = norm[..., None, :, :] * field
+    field = np.ascontiguousarray(norm[..., None, :, :] * field)
     return field, wavelengths, pixelsize

@@ -198,7 +198,9 @@ class Plan:
             stream = torch.cuda.current_stream(field_in.device)
         for t in (field_in, field_out):
             if t.dtype != self.dtype or not t.is_contiguous() or not t.is_cuda:
-                raise ValueError("fields must be contiguous %s device tensors" % self.dtype)
+                raise ValueError("fields must be contiguous %s device tensors (got %s, "
+                                 "contiguous=%s, device=%s)" % (self.dtype, t.dtype,
+                                                                t.is_contiguous(), t.device))
             if tuple(t.shape) != (self.n_rays, len(self.ks), 4) + self.shape:
                 raise ValueError("field shape %s does not match the plan" % (tuple(t.shape),))
         if self.workspace_bytes and (workspace is None or

@@ -114,6 +114,15 @@ int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, const void*
 int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int precision,
                void* stream);
 
+/* Optional kernel timing (no counterpart in the reference, which has no
+ * profiling hooks): between begin and end every per-layer kernel launch is
+ * bracketed by HIP events on its stream.  end() waits for the last event and
+ * returns, per kernel class, the summed milliseconds and the launch count
+ * (class 0: row layer kernel, class 1: column k-space kernel).  Not
+ * thread-safe; for benchmarking. */
+int dtmmx_timing_begin(int capacity);
+int dtmmx_timing_end(double* ms, int* count, int nclass);
+
 /* Host evaluation of the per-pixel eigen-solution used on the device
  * (dtmm/tmm.py:435 alphaf for isotropic and uniaxial voxels), for checks
  * without a GPU.  Inputs are n voxels; outputs alpha[n][4] and

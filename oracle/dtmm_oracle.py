@@ -34,7 +34,7 @@ __all__ = [
     "second_E_diffraction_matrix", "propagate_4x4_effective_1",
     "propagate_2x2_effective_1", "transfer_4x4", "transfer_2x2",
     "transfer_field", "illumination_data", "blackman", "planewave",
-    "jonesvec", "avec", "fvec", "auxiliary_matrix", "poynting",
+    "jonesvec", "avec", "fvec", "auxiliary_matrix", "poynting", "set_fft_workers",
 ]
 
 # dtmm/dtmm.ini:6 (betamax = 0.8) and dtmm/conf.py:148, :503-510 (nin/nout
@@ -159,13 +159,30 @@ def dotmdmf(a, d, b, f):
     return _v2f(np.matmul(a, v))
 
 
+_FFT_WORKERS = None
+
+
+def set_fft_workers(n):
+    """Use scipy.fft with `n` threads for the 2D FFTs (the reference's default
+    backend, dtmm/fft.py:178-200 with workers = core count, dtmm/conf.py:661);
+    None selects numpy.fft on one thread."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = n
+
+
 def fft2(f):
     """2D FFT over the last two axes (dtmm/fft.py:277)."""
+    if _FFT_WORKERS:
+        import scipy.fft
+        return scipy.fft.fft2(f, workers=_FFT_WORKERS)
     return np.fft.fft2(f)
 
 
 def ifft2(f):
     """2D inverse FFT over the last two axes (dtmm/fft.py:306)."""
+    if _FFT_WORKERS:
+        import scipy.fft
+        return scipy.fft.ifft2(f, workers=_FFT_WORKERS)
     return np.fft.ifft2(f)
 
 

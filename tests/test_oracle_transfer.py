@@ -1,0 +1,89 @@
+"""Pinning of the oracle's transfer chain by closed forms (CPU only).
+
+* A homogeneous planar-uniaxial slab at normal incidence acts on a uniform
+  plane wave as a Jones retarder with phases k0 d ne and k0 d no (the method
+  is exact for homogeneous layers, doc/method.rst:29).  The input medium
+  index equals the effective-medium index so no backward part is created.
+* A homogeneous isotropic slab acts on a windowed beam as exact
+  angular-spectrum propagation exp(i k0 L sqrt(eps - beta^2)) per FFT mode,
+  zero for beta >= betamax: this pins the sign and the mean-beam correction
+  of every k-space operator.
+* Linearity in the input field.
+"""
+import numpy as np
+import pytest
+
+import oracle as o
+
+
+def _uniform_wave(shape, wl, n, pol):
+    H, W = shape
+    c, s = pol
+    f = np.zeros((len(wl), 4, H, W), complex)
+    f[:, 0] = c
+    f[:, 1] = n * c
+    f[:, 2] = s
+    f[:, 3] = -n * s
+    return f
+
+
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+def test_uniaxial_slab_is_jones_retarder(method):
+    no, ne, L, phid = 1.5, 1.62, 7, 0.6
+    shape = (8, 8)
+    wl = np.array([480., 620.])
+    ps = 100.
+    neff = np.sqrt((2 * no ** 2 + ne ** 2) / 3)
+    eps = np.broadcast_to([no ** 2, no ** 2, ne ** 2], (L, 1, 1, 3))
+    ang = np.broadcast_to([0., np.pi / 2, phid], (L, 1, 1, 3))
+    pol = (np.cos(0.3), np.sin(0.3))
+    field = _uniform_wave(shape, wl, neff, pol)
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, ang)], beta=0., phi=0.,
+                           method=method, nin=neff, nout=neff)[0]
+    k = o.k0(wl, ps)
+    c, s = np.cos(phid), np.sin(phid)
+    R = np.array([[c, -s], [s, c]])
+    for i in range(len(wl)):
+        J = R @ np.diag([np.exp(1j * k[i] * L * ne), np.exp(1j * k[i] * L * no)]) @ R.T
+        e = J @ np.array(pol)
+        np.testing.assert_allclose(out[i, 0], e[0], atol=1e-12)
+        np.testing.assert_allclose(out[i, 2], e[1], atol=1e-12)
+
+
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+def test_isotropic_slab_is_angular_spectrum_propagation(method):
+    n, L = 1.55, 9
+    shape = (24, 32)
+    wl = np.array([500., 700.])
+    ps = 100.
+    win = o.blackman(shape)
+    beta, phi = 0.1, 0.7
+    field, _, _ = o.illumination_data(shape, wl, pixelsize=ps, beta=beta, phi=phi, jones=(1, 1j),
+                                      window=win, n=n)
+    eps = np.broadcast_to([n ** 2] * 3, (L, 1, 1, 3))
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                           beta=beta, phi=phi, method=method, nin=n, nout=n)[0]
+    k = o.k0(wl, ps)
+    b, _ = o.betaphi(shape, k)
+    prop = np.exp(1j * k[:, None, None] * L * np.sqrt(n ** 2 - b ** 2 + 0j))
+    prop[b >= o.BETAMAX] = 0.
+    expect = np.fft.ifft2(np.fft.fft2(field) * prop[:, None])
+    comps = [0, 1, 2, 3] if method == "4x4" else [0, 2]
+    for c in comps:
+        np.testing.assert_allclose(out[:, c], expect[:, c], atol=1e-12 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+def test_transfer_is_linear(method):
+    rng = np.random.default_rng(0)
+    L, shape = 3, (12, 16)
+    wl = np.array([550.])
+    eps = np.broadcast_to([2.25, 2.25, 2.6], (L,) + shape + (3,))
+    ang = np.stack([np.zeros((L,) + shape), rng.uniform(0, 3, (L,) + shape),
+                    rng.uniform(-3, 3, (L,) + shape)], -1)
+    f1 = rng.standard_normal((1, 4) + shape) + 1j * rng.standard_normal((1, 4) + shape)
+    f2 = rng.standard_normal((1, 4) + shape) + 1j * rng.standard_normal((1, 4) + shape)
+    a, b = 0.3 - 1.2j, 2.1 + 0.4j
+    opt = [(np.ones(L), eps, ang)]
+    t = lambda f: o.transfer_field((f, wl, 100.), opt, beta=0., phi=0., method=method)[0]
+    np.testing.assert_allclose(t(a * f1 + b * f2), a * t(f1) + b * t(f2), atol=1e-12)
