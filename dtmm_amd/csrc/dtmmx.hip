@@ -19,21 +19,14 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
-#include "berreman.hpp"
-#include "cpx.hpp"
-#include "dtmmx.h"
-#include "fft.hpp"
+#include "kernels.hpp"
 
 namespace dtmmx {
-
-constexpr int NT = 256;  // threads per work-group for every kernel
-constexpr int MAXL = DTMMX_MAX_WAVELENGTHS;
-constexpr int MAXB = DTMMX_MAX_BEAMS;
-
 thread_local std::string g_err;
 
 static int fail(int code, const std::string& msg) {
@@ -47,450 +40,6 @@ static int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess)                                                                   \
       return fail(DTMMX_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));            \
   } while (0)
-
-// Wavelengths and beams, passed by value to every kernel.
-struct Beams {
-  int nlam;           // wavelengths
-  int rays_per_beam;  // consecutive rays sharing a beam
-  double k0[MAXL];
-  double beta[MAXB];
-  double phi[MAXB];
-};
-
-__host__ __device__ inline double fftfreq(int i, int n) {
-  // numpy.fft.fftfreq(n)[i] = k * (1.0 / n) with k in [0..(n-1)/2, -n/2..-1]
-  int k = (i < (n + 1) / 2) ? i : i - n;
-  return double(k) * (1.0 / double(n));
-}
-
-__host__ __device__ inline int mode_index(int i, int n) {  // |k| of fftfreq
-  return (i < (n + 1) / 2) ? i : n - i;
-}
-
-// ---------------------------------------------------------------------------
-// Row kernel: one work-group transforms NC component lines of one row y of
-// a set of planes that share one beam.
-// ---------------------------------------------------------------------------
-
-enum { OP_FWD = 0, OP_INV = 1, OP_ROP = 2, OP_DIAG = 3 };
-
-template <typename T>
-struct RowArgs {
-  cpx<T>* field;
-  int H, W;
-  int nplanes;           // total planes (rays * wavelengths)
-  int planes_per_group;  // planes sharing one beam
-  int ppw;               // planes per work-group
-  int nchunk;            // work-groups per group per row
-  const cpx<T>* tw;      // twiddles of length W
-  // layer operator (OP_ROP)
-  const double* eps_re;
-  const double* eps_im;
-  const double* ang;
-  size_t comp_stride;  // elements between eps components (L*H*W)
-  double d;            // layer thickness
-  int method4;         // 1: 4x4 (ok4), 0: 2x2 (ok2)
-  int* status;
-  Beams bm;
-};
-
-template <typename T>
-struct RopT {
-  cpx<T> a[4];
-  cpx<T> P0[4];
-  cpx<T> P1[4];
-  T ok;
-};
-
-template <typename T, int N, int NC, int OP>
-__global__ __launch_bounds__(NT) void row_kernel(RowArgs<T> A) {
-  constexpr int PPT = (N + NT - 1) / NT;  // pixels per thread in the layer operator
-  __shared__ cpx<T> buf[NC * N];
-  const int tid = threadIdx.x;
-  const int y = blockIdx.x;
-  const int g = blockIdx.y / A.nchunk;
-  const int ch = blockIdx.y - g * A.nchunk;
-  const int q0 = g * A.planes_per_group + ch * A.ppw;
-  const int q1 = min(q0 + A.ppw, (g + 1) * A.planes_per_group);
-  const size_t plane_elems = size_t(A.H) * N;
-
-  RopT<T> rd[PPT];
-  if constexpr (OP == OP_ROP) {
-    const int beam = g;  // one group per beam
-    const double beta = A.bm.beta[beam];
-    const double phi0 = A.bm.phi[beam];
-    const double c0 = cos(phi0), s0 = sin(phi0);
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-      const int x = tid + k * NT;
-      if (x < N) {
-        const size_t o = size_t(y) * N + x;
-        cd e[3];
-        for (int c = 0; c < 3; ++c)
-          e[c] = mk<double>(A.eps_re[c * A.comp_stride + o],
-                            A.eps_im ? A.eps_im[c * A.comp_stride + o] : 0.);
-        const double theta = A.ang[1 * A.comp_stride + o];
-        const double phi = A.ang[2 * A.comp_stride + o];
-        Modes m;
-        int kind;
-        layer_modes(beta, c0, s0, phi0, e[0], e[1], e[2], theta, phi, m, kind);
-        RopData r;
-        rop_data(m, kind, r);
-        if (kind == KIND_BIAXIAL && A.status) atomicOr(A.status, 1);
-        for (int i = 0; i < 4; ++i) rd[k].a[i] = cast<T>(r.a[i]);
-        for (int i = 0; i < 2; ++i)
-          for (int j = 0; j < 2; ++j) {
-            rd[k].P0[2 * i + j] = cast<T>(r.P0[i][j]);
-            rd[k].P1[2 * i + j] = cast<T>(r.P1[i][j]);
-          }
-        rd[k].ok = (A.method4 ? r.ok4 : r.ok2) ? T(1) : T(0);
-        if (kind == KIND_BIAXIAL) rd[k].ok = T(__builtin_nan(""));
-      }
-    }
-  }
-
-  for (int q = q0; q < q1; ++q) {
-    cpx<T>* base = A.field + size_t(q) * NC * plane_elems + size_t(y) * N;
-    for (int idx = tid; idx < NC * N; idx += NT) {
-      const int c = idx / N;
-      const int x = idx - c * N;
-      buf[idx] = base[c * plane_elems + x];
-    }
-    __syncthreads();
-    if constexpr (OP == OP_FWD) {
-      lds_fft<T, N, NC, NT, -1>(buf, N, A.tw, tid);
-    } else if constexpr (OP == OP_INV) {
-      lds_fft<T, N, NC, NT, +1>(buf, N, A.tw, tid);
-    } else {
-      lds_fft<T, N, NC, NT, +1>(buf, N, A.tw, tid);
-      const int lam = q % A.bm.nlam;
-      const T kd = T(A.bm.k0[lam] * A.d);
-#pragma unroll
-      for (int k = 0; k < PPT; ++k) {
-        const int x = tid + k * NT;
-        if (x < N) {
-          const RopT<T>& r = rd[k];
-          cpx<T> p0 = cexpi(r.a[0] * kd);
-          cpx<T> p2 = cexpi(r.a[2] * kd);
-          cpx<T> dp = (p0 - p2) * r.ok;
-          p2 = p2 * r.ok;
-          if constexpr (NC == 4) {
-            cpx<T> p1 = cexpi(r.a[1] * kd);
-            cpx<T> p3 = cexpi(r.a[3] * kd);
-            cpx<T> dn = (p1 - p3) * r.ok;
-            p3 = p3 * r.ok;
-            cpx<T> u0 = buf[x], u1 = buf[N + x], u2 = buf[2 * N + x], u3 = buf[3 * N + x];
-            cpx<T> w0 = r.P0[0] * u0 + r.P0[1] * u2;
-            cpx<T> w2 = r.P0[2] * u0 + r.P0[3] * u2;
-            cpx<T> w1 = r.P1[0] * u1 + r.P1[1] * u3;
-            cpx<T> w3 = r.P1[2] * u1 + r.P1[3] * u3;
-            buf[x] = p2 * u0 + dp * w0;
-            buf[2 * N + x] = p2 * u2 + dp * w2;
-            buf[N + x] = p3 * u1 + dn * w1;
-            buf[3 * N + x] = p3 * u3 + dn * w3;
-          } else {
-            cpx<T> u0 = buf[x], u1 = buf[N + x];
-            cpx<T> w0 = r.P0[0] * u0 + r.P0[1] * u1;
-            cpx<T> w1 = r.P0[2] * u0 + r.P0[3] * u1;
-            buf[x] = p2 * u0 + dp * w0;
-            buf[N + x] = p2 * u1 + dp * w1;
-          }
-        }
-      }
-      __syncthreads();
-      lds_fft<T, N, NC, NT, -1>(buf, N, A.tw, tid);
-    }
-    for (int idx = tid; idx < NC * N; idx += NT) {
-      const int c = idx / N;
-      const int x = idx - c * N;
-      base[c * plane_elems + x] = buf[idx];
-    }
-    __syncthreads();
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Column kernel: one work-group transforms TX adjacent columns of one
-// component plane.  OP_DIAG applies the diagonal k-space operator between
-// the forward and inverse transforms.
-// ---------------------------------------------------------------------------
-
-template <typename T>
-struct ColArgs {
-  cpx<T>* field;
-  int H, W;
-  int NC;             // components per plane
-  const cpx<T>* tw;   // twiddles of length H
-  const int* amax;    // [nlam][H]: largest passing |kx| index per ky row, -1 if none
-  double da, db;      // thicknesses of the two half steps
-  cd ea, eb;          // isotropic effective eps of the two half steps
-  double scale;
-  int dense_sign;     // 1: backward components (odd c) get exp(-i phase)
-  Beams bm;
-};
-
-template <typename T>
-__device__ __forceinline__ cpx<T> diag_factor(const ColArgs<T>& A, int lam, int beam, int ky,
-                                              int kx, int sign) {
-  const int mx = mode_index(kx, A.W), my = mode_index(ky, A.H);
-  if (mx > A.amax[lam * A.H + ky]) return mk<T>(T(0), T(0));
-  (void)my;
-  const T k0 = T(A.bm.k0[lam]);
-  const T l = T(2 * M_PI) / k0;
-  const T fx = T(mx) / T(A.W), fy = T(my) / T(A.H);
-  const T b2 = l * l * (fx * fx + fy * fy);
-  const T bc = T(A.bm.beta[beam]);
-  cpx<T> ph = mk<T>(T(0), T(0));
-  if (A.da != 0.) {
-    cpx<T> e = cast<T>(A.ea);
-    cpx<T> ak = csqrt(e - mk<T>(b2));
-    cpx<T> ac = csqrt(e - mk<T>(bc * bc));
-    ph = ph + (ak - ac) * T(0.5 * A.da);
-  }
-  if (A.db != 0.) {
-    cpx<T> e = cast<T>(A.eb);
-    cpx<T> ak = csqrt(e - mk<T>(b2));
-    cpx<T> ac = csqrt(e - mk<T>(bc * bc));
-    ph = ph + (ak - ac) * T(0.5 * A.db);
-  }
-  ph = ph * (k0 * T(sign));
-  return cexpi(ph) * T(A.scale);
-}
-
-template <typename T, int N, int TX, int LSTR, int OP>
-__global__ __launch_bounds__(NT) void col_kernel(ColArgs<T> A) {
-  __shared__ cpx<T> buf[TX * LSTR];
-  const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TX;
-  const int pc = blockIdx.y;
-  cpx<T>* base = A.field + size_t(pc) * N * A.W;
-  for (int idx = tid; idx < TX * N; idx += NT) {
-    const int yy = idx / TX;
-    const int xx = idx - yy * TX;
-    const int col = x0 + xx;
-    buf[xx * LSTR + yy] = (col < A.W) ? base[size_t(yy) * A.W + col] : mk<T>(T(0), T(0));
-  }
-  __syncthreads();
-  if constexpr (OP == OP_FWD) {
-    lds_fft<T, N, TX, NT, -1>(buf, LSTR, A.tw, tid);
-  } else if constexpr (OP == OP_INV) {
-    lds_fft<T, N, TX, NT, +1>(buf, LSTR, A.tw, tid);
-  } else {
-    lds_fft<T, N, TX, NT, -1>(buf, LSTR, A.tw, tid);
-    const int c = pc % A.NC;
-    const int q = pc / A.NC;
-    const int lam = q % A.bm.nlam;
-    const int beam = (q / A.bm.nlam) / A.bm.rays_per_beam;
-    const int sign = (A.dense_sign && (c & 1)) ? -1 : 1;
-    for (int idx = tid; idx < TX * N; idx += NT) {
-      const int xx = idx / N;
-      const int ky = idx - xx * N;
-      const int col = x0 + xx;
-      if (col < A.W) buf[xx * LSTR + ky] = buf[xx * LSTR + ky] * diag_factor(A, lam, beam, ky, col, sign);
-    }
-    __syncthreads();
-    lds_fft<T, N, TX, NT, +1>(buf, LSTR, A.tw, tid);
-  }
-  for (int idx = tid; idx < TX * N; idx += NT) {
-    const int yy = idx / TX;
-    const int xx = idx - yy * TX;
-    const int col = x0 + xx;
-    if (col < A.W) base[size_t(yy) * A.W + col] = buf[xx * LSTR + yy];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Dense per-k operators (fp64), one thread per (k, plane).
-// ---------------------------------------------------------------------------
-
-enum { KOP_HEAD4 = 0, KOP_TAIL4 = 1, KOP_BOUND4 = 2, KOP_PROJ2 = 3 };
-
-template <typename T>
-struct KArgs {
-  const cpx<T>* src;
-  cpx<T>* dst;
-  int H, W;
-  int nplanes;
-  const int* amax;
-  int kop;
-  double d0, d1;  // thickness of the layer(s) involved
-  cd e0, e1;      // effective eps of the layer(s) involved
-  cd ein;         // input medium eps (PROJ2)
-  double scale;
-  Beams bm;
-};
-
-// Modes of an isotropic medium at in-plane wave vector (beta, phi), rotated.
-__device__ inline void iso_at(double beta, double phi, cd eps, Modes& m) {
-  iso_modes(beta, eps, m);
-  rotate_rows(m, cos(phi), sin(phi));
-}
-
-__device__ inline void emat(const Modes& m, cd E[4][4]) {  // dtmm/tmm.py:990-996
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) E[i][j] = mk<double>(0.);
-  E[0][0] = m.F[0][0];
-  E[0][2] = m.F[0][2];
-  E[2][0] = m.F[2][0];
-  E[2][2] = m.F[2][2];
-  E[1][1] = m.F[0][1];
-  E[1][3] = m.F[0][3];
-  E[3][1] = m.F[2][1];
-  E[3][3] = m.F[2][3];
-}
-
-// X . diag(exp(i kd a)) . Y
-__device__ inline void xpy(const cd X[4][4], const cd a[4], double kd, const cd Y[4][4], cd O[4][4]) {
-  cd p[4];
-  for (int i = 0; i < 4; ++i) p[i] = cexpi(a[i] * kd);
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) {
-      cd s = mk<double>(0.);
-      for (int k = 0; k < 4; ++k) s = s + X[i][k] * p[k] * Y[k][j];
-      O[i][j] = s;
-    }
-}
-
-// Mean-beam correction E_c P(-kd) E_c^-1 (dtmm/matrix.py:132-139).
-__device__ inline void correction(double beta, double phi, cd eps, double kd, cd C[4][4]) {
-  Modes m;
-  iso_at(beta, phi, eps, m);
-  cd E[4][4], Ei[4][4];
-  emat(m, E);
-  inv4(E, Ei);
-  xpy(E, m.a, -kd, Ei, C);
-}
-
-// First half step E P(kd) F^-1 at wave vector k (dtmm/matrix.py:109-118).
-__device__ inline void first_half(double beta, double phi, cd eps, double kd, cd D[4][4]) {
-  Modes m;
-  iso_at(beta, phi, eps, m);
-  cd E[4][4], Fi[4][4];
-  emat(m, E);
-  inv4(m.F, Fi);
-  xpy(E, m.a, kd, Fi, D);
-}
-
-// Second half step F P(kd) E^-1 at wave vector k (dtmm/matrix.py:120-130).
-__device__ inline void second_half(double beta, double phi, cd eps, double kd, cd D[4][4]) {
-  Modes m;
-  iso_at(beta, phi, eps, m);
-  cd E[4][4], Ei[4][4];
-  emat(m, E);
-  inv4(E, Ei);
-  xpy(m.F, m.a, kd, Ei, D);
-}
-
-template <typename T>
-__global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
-  const int k = blockIdx.x * NT + threadIdx.x;
-  const int q = blockIdx.y;
-  const size_t HW = size_t(A.H) * A.W;
-  if (k >= int(HW)) return;
-  const int ky = k / A.W, kx = k - ky * A.W;
-  const int lam = q % A.bm.nlam;
-  const int beam = (q / A.bm.nlam) / A.bm.rays_per_beam;
-  const double k0 = A.bm.k0[lam];
-  const double bb = A.bm.beta[beam], pb = A.bm.phi[beam];
-  const bool pass = mode_index(kx, A.W) <= A.amax[lam * A.H + ky];
-  const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
-  const double beta = (2 * M_PI / k0) * sqrt(fx * fx + fy * fy);
-  const double phi = atan2(fy, fx);
-  const int nin = (A.kop == KOP_TAIL4 || A.kop == KOP_BOUND4) ? 4 : (A.kop == KOP_PROJ2 ? 4 : 4);
-  const int nout = (A.kop == KOP_PROJ2) ? 2 : 4;
-  const cpx<T>* s = A.src + size_t(q) * nin * HW + k;
-  cpx<T>* o = A.dst + size_t(q) * nout * HW + k;
-  cd v[4];
-  for (int c = 0; c < nin; ++c) v[c] = cast<double>(s[c * HW]);
-  cd M[4][4];
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) M[i][j] = mk<double>(0.);
-  if (pass) {
-    if (A.kop == KOP_HEAD4) {
-      cd C[4][4], D[4][4];
-      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C);
-      first_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D);
-      matmul4(C, D, M);
-    } else if (A.kop == KOP_TAIL4) {
-      cd C[4][4], D[4][4];
-      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C);
-      second_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D);
-      matmul4(D, C, M);
-    } else if (A.kop == KOP_BOUND4) {
-      cd C0[4][4], D0[4][4], C1[4][4], D1[4][4], X[4][4], Y[4][4];
-      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C0);
-      second_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D0);
-      correction(bb, pb, A.e1, k0 * A.d1 * 0.5, C1);
-      first_half(beta, phi, A.e1, k0 * A.d1 * 0.5, D1);
-      matmul4(D0, C0, X);  // second corrected of layer 0
-      matmul4(C1, D1, Y);  // first corrected of layer 1
-      matmul4(Y, X, M);
-    } else {
-      // rows (Ex, Ey) of the forward projection in the input medium
-      // (dtmm/diffract.py:242-251) times the first 2x2 half step.
-      Modes m;
-      iso_at(beta, phi, A.ein, m);
-      cd Fi[4][4];
-      inv4(m.F, Fi);
-      cd P[4][4];
-      for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) P[i][j] = m.F[i][0] * Fi[0][j] + m.F[i][2] * Fi[2][j];
-      cd ph = mk<double>(0.);
-      if (A.d0 != 0.) {
-        cd ak = csqrt(A.e0 - mk<double>(beta * beta));
-        cd ac = csqrt(A.e0 - mk<double>(bb * bb));
-        ph = (ak - ac) * (0.5 * A.d0 * k0);
-      }
-      cd K = cexpi(ph);
-      for (int j = 0; j < 4; ++j) {
-        M[0][j] = K * P[0][j];
-        M[1][j] = K * P[2][j];
-      }
-    }
-  }
-  for (int i = 0; i < nout; ++i) {
-    cd r = mk<double>(0.);
-    for (int j = 0; j < nin; ++j) r = r + M[i][j] * v[j];
-    o[i * HW] = cast<T>(r * A.scale);
-  }
-}
-
-// (Ex, Ey) -> (Ex, Hy, Ey, Hx) with the mean-beam H/E relation of the output
-// medium (dtmm/propagate_2x2.py:81-85, dtmm/tmm.py:1051-1072).
-template <typename T>
-struct E2HArgs {
-  const cpx<T>* src;
-  cpx<T>* dst;
-  int H, W;
-  cd eout;
-  Beams bm;
-};
-
-template <typename T>
-__global__ __launch_bounds__(NT) void e2h_kernel(E2HArgs<T> A) {
-  const int k = blockIdx.x * NT + threadIdx.x;
-  const int q = blockIdx.y;
-  const size_t HW = size_t(A.H) * A.W;
-  if (k >= int(HW)) return;
-  const int beam = (q / A.bm.nlam) / A.bm.rays_per_beam;
-  Modes m;
-  iso_modes(A.bm.beta[beam], A.eout, m);
-  rotate_rows(m, cos(A.bm.phi[beam]), sin(A.bm.phi[beam]));
-  cd Am[2][2] = {{m.F[0][0], m.F[0][2]}, {m.F[2][0], m.F[2][2]}};
-  cd Bm[2][2] = {{m.F[1][0], m.F[1][2]}, {m.F[3][0], m.F[3][2]}};
-  cd Ai[2][2];
-  inv2(Am, Ai);
-  cd G[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) G[i][j] = Bm[i][0] * Ai[0][j] + Bm[i][1] * Ai[1][j];
-  const cpx<T>* s = A.src + size_t(q) * 2 * HW + k;
-  cpx<T>* o = A.dst + size_t(q) * 4 * HW + k;
-  cd ex = cast<double>(s[0]), ey = cast<double>(s[HW]);
-  o[0] = cast<T>(ex);
-  o[HW] = cast<T>(G[0][0] * ex + G[0][1] * ey);
-  o[2 * HW] = cast<T>(ey);
-  o[3 * HW] = cast<T>(G[1][0] * ex + G[1][1] * ey);
-}
 
 // ---------------------------------------------------------------------------
 // Host side: size dispatch and the transfer driver.
@@ -506,27 +55,38 @@ static std::vector<cpx<T>> twiddles(int n) {
   return t;
 }
 
+#define DTMMX_SIZES(X)                                                                        \
+  X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) \
+  X(320) X(384) X(512) X(640) X(768) X(1024)
+
 static bool size_supported(int n) {
   switch (n) {
-    case 8: case 12: case 16: case 20: case 24: case 32: case 40: case 48: case 64: case 80:
-    case 96: case 128: case 160: case 192: case 256: case 320: case 384: case 512: case 640:
-    case 768: case 1024:
-      return true;
+#define X(N) case N:
+    DTMMX_SIZES(X)
+#undef X
+    return true;
     default:
       return false;
   }
 }
 
-#define DTMMX_SIZES(X)                                                                        \
-  X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) \
-  X(320) X(384) X(512) X(640) X(768) X(1024)
+// Rays per row work item: two rays share the eigen data and phases when the
+// line registers fit (complex64 up to 512 points).
+template <typename T, int N>
+constexpr int row_nr_max() {
+  return (sizeof(T) == 4 && N <= 512) ? 2 : 1;
+}
 
 template <typename T, int NC, int OP>
-static int launch_row_n(int n, const RowArgs<T>& a, dim3 grid, hipStream_t s) {
+static int launch_row(int n, int nr, const RowArgs<T>& a, int nwork, hipStream_t s) {
   switch (n) {
-#define X(N)                                                                  \
-  case N:                                                                     \
-    hipLaunchKernelGGL((row_kernel<T, N, NC, OP>), grid, dim3(NT), 0, s, a); \
+#define X(N)                                                                                \
+  case N:                                                                                   \
+    if (nr == 2 && row_nr_max<T, N>() >= 2)                                                 \
+      hipLaunchKernelGGL((row_kernel<T, N, (row_nr_max<T, N>() >= 2 ? 2 : 1), NC, OP>),      \
+                         dim3(nwork), dim3(64 * 2 * NC), 0, s, a);                          \
+    else                                                                                    \
+      hipLaunchKernelGGL((row_kernel<T, N, 1, NC, OP>), dim3(nwork), dim3(64 * NC), 0, s, a); \
     break;
     DTMMX_SIZES(X)
 #undef X
@@ -537,24 +97,39 @@ static int launch_row_n(int n, const RowArgs<T>& a, dim3 grid, hipStream_t s) {
   return DTMMX_OK;
 }
 
+// Columns per column work-group: the widest of 16, 8, 4, 2, 1 (16 gives
+// 128-byte row segments for complex64) whose LDS stays within 64 KiB and
+// whose work-group stays within 1024 threads.
 template <typename T, int N>
-struct ColShape {
-  static constexpr int BYTES = 65536;
-  static constexpr int CB = int(sizeof(cpx<T>));
-  static constexpr int TX0 = BYTES / (N * CB);
-  static constexpr int TX = TX0 >= 16 ? 16 : (TX0 >= 8 ? 8 : (TX0 >= 4 ? 4 : (TX0 >= 2 ? 2 : 1)));
-  static constexpr int LSTR = (TX * (N + 1) * CB <= BYTES) ? N + 1 : N;
-};
+constexpr int col_tx() {
+  constexpr int cb = int(sizeof(cpx<T>));
+  constexpr int nl = col_lanes<N>();
+  int tx = 16;
+  while (tx > 1 && (N * (tx + 1) * cb > 65536 || tx * nl > 1024)) tx /= 2;
+  return tx;
+}
 
 template <typename T, int OP>
-static int launch_col_n(int n, const ColArgs<T>& a, int nlines, hipStream_t s) {
+static int launch_col(int n, const ColArgs<T>& a, int nlines_per_lam, hipStream_t s) {
+  static const bool half_tx = [] {
+    const char* e = std::getenv("DTMMX_COL_HALF_TX");
+    return e && std::atoi(e) != 0;
+  }();
   switch (n) {
-#define X(N)                                                                              \
-  case N: {                                                                               \
-    using S = ColShape<T, N>;                                                             \
-    dim3 grid((a.W + S::TX - 1) / S::TX, nlines);                                         \
-    hipLaunchKernelGGL((col_kernel<T, N, S::TX, S::LSTR, OP>), grid, dim3(NT), 0, s, a);  \
-    break;                                                                                \
+#define X(N)                                                                                \
+  case N: {                                                                                 \
+    constexpr int TX = col_tx<T, N>();                                                      \
+    constexpr int TX2 = TX > 1 ? TX / 2 : 1;                                                \
+    if (half_tx) {                                                                          \
+      const int nwork = ((a.W + TX2 - 1) / TX2) * a.b.nlam_b * (nlines_per_lam / a.ncl);    \
+      hipLaunchKernelGGL((col_kernel<T, N, TX2, OP>), dim3(nwork), dim3(TX2 * col_lanes<N>()), \
+                         0, s, a);                                                          \
+    } else {                                                                                \
+      const int nwork = ((a.W + TX - 1) / TX) * a.b.nlam_b * (nlines_per_lam / a.ncl);      \
+      hipLaunchKernelGGL((col_kernel<T, N, TX, OP>), dim3(nwork), dim3(TX * col_lanes<N>()), 0, \
+                         s, a);                                                             \
+    }                                                                                       \
+    break;                                                                                  \
   }
     DTMMX_SIZES(X)
 #undef X
@@ -604,65 +179,67 @@ static void timer_stop(hipStream_t s) {
 }
 
 template <typename T>
-static int row_pass(const dtmmx_plan* P, cpx<T>* field, int nplanes, int nc, int op,
-                    hipStream_t s, const RowArgs<T>* proto = nullptr, int ppg = 0) {
+static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op, hipStream_t s,
+                    const cpx<T>* eig = nullptr, double d = 0.) {
   RowArgs<T> a{};
-  if (proto) a = *proto;
   a.field = field;
   a.H = P->d.height;
   a.W = P->d.width;
-  a.nplanes = nplanes;
+  a.b = b;
   a.tw = (const cpx<T>*)P->tw_w;
+  a.eig = eig;
+  a.d = d;
   a.bm = P->bm;
-  if (op == OP_ROP) {
-    a.planes_per_group = ppg;
-    const int groups = nplanes / ppg;
-    int nchunk = (2048 + a.H * groups - 1) / (a.H * groups);
-    if (nchunk < 1) nchunk = 1;
-    if (nchunk > ppg) nchunk = ppg;
-    a.ppw = (ppg + nchunk - 1) / nchunk;
-    a.nchunk = (ppg + a.ppw - 1) / a.ppw;
-  } else {
-    a.planes_per_group = 1;
-    a.ppw = 1;
-    a.nchunk = 1;
-  }
-  dim3 grid(a.H, (nplanes / a.planes_per_group) * a.nchunk);
+  int nr = 1;
+  if (sizeof(T) == 4 && a.W <= 512 && b.nrays % 2 == 0 &&
+      (op != OP_ROP || P->bm.rays_per_beam % 2 == 0))
+    nr = 2;
+  a.rays_per_group = nr;
+  const int nwork = a.H * (b.nrays / nr);
   const int w = a.W;
-  if (nc == 1 && op == OP_FWD) return launch_row_n<T, 1, OP_FWD>(w, a, grid, s);
-  if (nc == 1 && op == OP_INV) return launch_row_n<T, 1, OP_INV>(w, a, grid, s);
-  if (nc == 2 && op == OP_FWD) return launch_row_n<T, 2, OP_FWD>(w, a, grid, s);
-  if (nc == 2 && op == OP_INV) return launch_row_n<T, 2, OP_INV>(w, a, grid, s);
   if (op == OP_ROP) {
     timer_start(0, s);
-    int r = nc == 2 ? launch_row_n<T, 2, OP_ROP>(w, a, grid, s)
-                    : launch_row_n<T, 4, OP_ROP>(w, a, grid, s);
+    int r = nc == 2 ? launch_row<T, 2, OP_ROP>(w, nr, a, nwork, s)
+                    : launch_row<T, 4, OP_ROP>(w, nr, a, nwork, s);
     timer_stop(s);
     return r;
   }
-  if (nc == 4 && op == OP_FWD) return launch_row_n<T, 4, OP_FWD>(w, a, grid, s);
-  if (nc == 4 && op == OP_INV) return launch_row_n<T, 4, OP_INV>(w, a, grid, s);
+  if (nc == 1 && op == OP_FWD) return launch_row<T, 1, OP_FWD>(w, nr, a, nwork, s);
+  if (nc == 1 && op == OP_INV) return launch_row<T, 1, OP_INV>(w, nr, a, nwork, s);
+  if (nc == 2 && op == OP_FWD) return launch_row<T, 2, OP_FWD>(w, nr, a, nwork, s);
+  if (nc == 2 && op == OP_INV) return launch_row<T, 2, OP_INV>(w, nr, a, nwork, s);
+  if (nc == 4 && op == OP_FWD) return launch_row<T, 4, OP_FWD>(w, nr, a, nwork, s);
+  if (nc == 4 && op == OP_INV) return launch_row<T, 4, OP_INV>(w, nr, a, nwork, s);
   return fail(DTMMX_E_ARG, "bad row pass");
 }
 
 template <typename T>
-static int col_pass(const dtmmx_plan* P, cpx<T>* field, int nlines, int nc, int op, hipStream_t s,
+static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op, hipStream_t s,
                     const ColArgs<T>* proto = nullptr) {
   ColArgs<T> a{};
   if (proto) a = *proto;
   a.field = field;
   a.H = P->d.height;
   a.W = P->d.width;
+  a.b = b;
   a.NC = nc;
+  // lines per work-group: the components of one ray, or of two rays sharing a beam
+  a.ncl = nc;
+  if (b.nrays % 2 == 0 && P->bm.rays_per_beam % 2 == 0 && nc <= 2) a.ncl = 2 * nc;
+  if (const char* e = std::getenv("DTMMX_COL_NCL")) {
+    const int v = std::atoi(e);
+    if (v > 0 && nc % v == 0) a.ncl = v;
+  }
   a.tw = (const cpx<T>*)P->tw_h;
   a.amax = P->amax;
   a.bm = P->bm;
+  const int nlines = b.nrays * nc;
   const int h = a.H;
-  if (op == OP_FWD) return launch_col_n<T, OP_FWD>(h, a, nlines, s);
-  if (op == OP_INV) return launch_col_n<T, OP_INV>(h, a, nlines, s);
+  if (op == OP_FWD) return launch_col<T, OP_FWD>(h, a, nlines, s);
+  if (op == OP_INV) return launch_col<T, OP_INV>(h, a, nlines, s);
   if (op == OP_DIAG) {
     timer_start(1, s);
-    int r = launch_col_n<T, OP_DIAG>(h, a, nlines, s);
+    int r = launch_col<T, OP_DIAG>(h, a, nlines, s);
     timer_stop(s);
     return r;
   }
@@ -670,149 +247,176 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, int nlines, int nc, int 
 }
 
 template <typename T>
-static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, int nplanes, KArgs<T> a,
+static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, KArgs<T> a,
                   hipStream_t s) {
   a.src = src;
   a.dst = dst;
   a.H = P->d.height;
   a.W = P->d.width;
-  a.nplanes = nplanes;
+  a.b = b;
   a.amax = P->amax;
   a.bm = P->bm;
   const int HW = a.H * a.W;
-  dim3 grid((HW + NT - 1) / NT, nplanes);
+  dim3 grid((HW + NT - 1) / NT, b.nrays * b.nlam_b);
   hipLaunchKernelGGL((kspace_kernel<T>), grid, dim3(NT), 0, s, a);
   HIP_TRY(hipGetLastError());
   return DTMMX_OK;
 }
 
-#define TRY(x)                 \
-  do {                         \
-    int r_ = (x);              \
+template <typename T>
+static int eigen_pass(const dtmmx_plan* P, const dtmmx_stack* S, int l, int method4, cpx<T>* tab,
+                      int* status, hipStream_t s) {
+  EigArgs<T> a{};
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  a.tab = tab;
+  a.HW = int(HW);
+  a.eps_re = S->eps_re + size_t(l) * HW;
+  a.eps_im = S->eps_im ? S->eps_im + size_t(l) * HW : nullptr;
+  a.ang = S->angles + size_t(l) * HW;
+  a.comp_stride = size_t(S->n_layers) * HW;
+  a.method4 = method4;
+  a.status = status;
+  a.bm = P->bm;
+  dim3 grid(unsigned((HW + NT - 1) / NT), P->d.n_rays / P->d.rays_per_beam);
+  hipLaunchKernelGGL((eigen_kernel<T>), grid, dim3(NT), 0, s, a);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
+#define TRY(x)                     \
+  do {                             \
+    int r_ = (x);                  \
     if (r_ != DTMMX_OK) return r_; \
   } while (0)
+
+// Wavelengths per batch: the whole batch stays resident in the 256 MiB
+// memory-side cache across the row and column passes of a layer when it is
+// small enough.  DTMMX_BATCH_MB overrides the budget (0: no batching).
+static int lam_batch(const dtmmx_plan* P, size_t bytes_per_lam) {
+  long mb = 0;
+  if (const char* e = std::getenv("DTMMX_BATCH_MB")) mb = std::atol(e);
+  const int nlam = P->d.n_wavelengths;
+  if (mb <= 0) return nlam;
+  int nb = int(size_t(mb) * 1048576 / (bytes_per_lam ? bytes_per_lam : 1));
+  if (nb < 1) nb = 1;
+  return nb < nlam ? nb : nlam;
+}
 
 template <typename T>
 static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
                          void* ws, int* status, hipStream_t s) {
   const int H = P->d.height, W = P->d.width;
   const size_t HW = size_t(H) * W;
-  const int nplanes = P->d.n_rays * P->d.n_wavelengths;
-  const int ppg = P->d.rays_per_beam * P->d.n_wavelengths;
+  const int nrays = P->d.n_rays, nlam = P->d.n_wavelengths;
+  const int nplanes = nrays * nlam;
+  const int nbeam = nrays / P->d.rays_per_beam;
   const int L = S->n_layers;
   const double scale = 1.0 / double(HW);
+  const bool m4 = P->d.method == DTMMX_METHOD_4X4;
   auto eff = [&](int l) {
     return mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
   };
   cpx<T>* out = (cpx<T>*)fout;
   HIP_TRY(hipMemcpyAsync(out, fin, size_t(nplanes) * 4 * HW * sizeof(cpx<T>),
                          hipMemcpyDeviceToDevice, s));
-  const size_t cs = size_t(L) * HW;
-  auto rop_proto = [&](int l, int m4) {
-    RowArgs<T> a{};
-    a.eps_re = S->eps_re + size_t(l) * HW;
-    a.eps_im = S->eps_im ? S->eps_im + size_t(l) * HW : nullptr;
-    a.ang = S->angles + size_t(l) * HW;
-    a.comp_stride = cs;
-    a.d = S->thickness[l];
-    a.method4 = m4;
-    a.status = status;
-    return a;
-  };
-  if (P->d.method == DTMMX_METHOD_4X4) {
-    if (L == 0) return DTMMX_OK;  // ifft2(fft2(field_in))
-    TRY(row_pass<T>(P, out, nplanes, 4, OP_FWD, s));
-    TRY(col_pass<T>(P, out, nplanes * 4, 4, OP_FWD, s));
-    KArgs<T> k{};
-    k.kop = KOP_HEAD4;
-    k.d0 = S->thickness[0];
-    k.e0 = eff(0);
-    k.scale = scale;
-    TRY(kspace<T>(P, out, out, nplanes, k, s));
-    TRY(col_pass<T>(P, out, nplanes * 4, 4, OP_INV, s));
-    for (int l = 0; l < L; ++l) {
-      RowArgs<T> ra = rop_proto(l, 1);
-      TRY(row_pass<T>(P, out, nplanes, 4, OP_ROP, s, &ra, ppg));
-      if (l + 1 < L) {
-        cd e0 = eff(l), e1 = eff(l + 1);
-        if (eq(e0, e1)) {
-          ColArgs<T> ca{};
-          ca.da = S->thickness[l];
-          ca.db = S->thickness[l + 1];
-          ca.ea = e0;
-          ca.eb = e1;
-          ca.scale = scale;
-          ca.dense_sign = 1;
-          TRY(col_pass<T>(P, out, nplanes * 4, 4, OP_DIAG, s, &ca));
-        } else {
-          TRY(col_pass<T>(P, out, nplanes * 4, 4, OP_FWD, s));
-          KArgs<T> b{};
-          b.kop = KOP_BOUND4;
-          b.d0 = S->thickness[l];
-          b.d1 = S->thickness[l + 1];
-          b.e0 = e0;
-          b.e1 = e1;
-          b.scale = scale;
-          TRY(kspace<T>(P, out, out, nplanes, b, s));
-          TRY(col_pass<T>(P, out, nplanes * 4, 4, OP_INV, s));
+  // workspace: [eigen table of one layer][2x2: 2-component field]
+  cpx<T>* eig = (cpx<T>*)ws;
+  cpx<T>* e2 = eig + size_t(nbeam) * EIG_COMPS * HW;
+  const int nc = m4 ? 4 : 2;
+  const int nlb = lam_batch(P, size_t(nrays) * 4 * HW * sizeof(cpx<T>));
+  for (int lam0 = 0; lam0 < nlam; lam0 += nlb) {
+    Batch b{nrays, nlam, lam0, (lam0 + nlb <= nlam) ? nlb : nlam - lam0};
+    if (m4) {
+      if (L == 0) break;  // ifft2(fft2(field_in))
+      TRY(row_pass<T>(P, out, b, 4, OP_FWD, s));
+      TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+      KArgs<T> k{};
+      k.kop = KOP_HEAD4;
+      k.d0 = S->thickness[0];
+      k.e0 = eff(0);
+      k.scale = scale;
+      TRY(kspace<T>(P, out, out, b, k, s));
+      TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+      for (int l = 0; l < L; ++l) {
+        TRY(eigen_pass<T>(P, S, l, 1, eig, status, s));
+        TRY(row_pass<T>(P, out, b, 4, OP_ROP, s, eig, S->thickness[l]));
+        if (l + 1 < L) {
+          cd e0 = eff(l), e1 = eff(l + 1);
+          if (eq(e0, e1)) {
+            ColArgs<T> ca{};
+            ca.da = S->thickness[l];
+            ca.db = S->thickness[l + 1];
+            ca.ea = e0;
+            ca.eb = e1;
+            ca.scale = scale;
+            ca.dense_sign = 1;
+            TRY(col_pass<T>(P, out, b, 4, OP_DIAG, s, &ca));
+          } else {
+            TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+            KArgs<T> bd{};
+            bd.kop = KOP_BOUND4;
+            bd.d0 = S->thickness[l];
+            bd.d1 = S->thickness[l + 1];
+            bd.e0 = e0;
+            bd.e1 = e1;
+            bd.scale = scale;
+            TRY(kspace<T>(P, out, out, b, bd, s));
+            TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+          }
         }
       }
+      TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+      KArgs<T> t{};
+      t.kop = KOP_TAIL4;
+      t.d0 = S->thickness[L - 1];
+      t.e0 = eff(L - 1);
+      t.scale = scale;
+      TRY(kspace<T>(P, out, out, b, t, s));
+      TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+      TRY(row_pass<T>(P, out, b, 4, OP_INV, s));
+      continue;
     }
-    TRY(col_pass<T>(P, out, nplanes * 4, 4, OP_FWD, s));
-    KArgs<T> t{};
-    t.kop = KOP_TAIL4;
-    t.d0 = S->thickness[L - 1];
-    t.e0 = eff(L - 1);
-    t.scale = scale;
-    TRY(kspace<T>(P, out, out, nplanes, t, s));
-    TRY(col_pass<T>(P, out, nplanes * 4, 4, OP_INV, s));
-    TRY(row_pass<T>(P, out, nplanes, 4, OP_INV, s));
-    return DTMMX_OK;
-  }
-  // 2x2: out doubles as the 4-component scratch of the input projection.
-  cpx<T>* e2 = (cpx<T>*)ws;
-  TRY(row_pass<T>(P, out, nplanes, 4, OP_FWD, s));
-  TRY(col_pass<T>(P, out, nplanes * 4, 4, OP_FWD, s));
-  KArgs<T> h{};
-  h.kop = KOP_PROJ2;
-  h.d0 = L > 0 ? S->thickness[0] : 0.;
-  h.e0 = L > 0 ? eff(0) : mk<double>(S->n_out * S->n_out);
-  h.ein = mk<double>(S->n_in * S->n_in);
-  h.scale = scale;
-  TRY(kspace<T>(P, out, e2, nplanes, h, s));
-  TRY(col_pass<T>(P, e2, nplanes * 2, 2, OP_INV, s));
-  for (int l = 0; l < L; ++l) {
-    RowArgs<T> ra = rop_proto(l, 0);
-    TRY(row_pass<T>(P, e2, nplanes, 2, OP_ROP, s, &ra, ppg));
-    ColArgs<T> ca{};
-    ca.da = S->thickness[l];
-    ca.ea = eff(l);
-    if (l + 1 < L) {
-      ca.db = S->thickness[l + 1];
-      ca.eb = eff(l + 1);
-    } else {
-      ca.db = 0.;  // output medium: zero thickness, only the betamax mask
-      ca.eb = mk<double>(S->n_out * S->n_out);
+    // 2x2: out doubles as the 4-component scratch of the input projection.
+    TRY(row_pass<T>(P, out, b, 4, OP_FWD, s));
+    TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+    KArgs<T> h{};
+    h.kop = KOP_PROJ2;
+    h.d0 = L > 0 ? S->thickness[0] : 0.;
+    h.e0 = L > 0 ? eff(0) : mk<double>(S->n_out * S->n_out);
+    h.ein = mk<double>(S->n_in * S->n_in);
+    h.scale = scale;
+    TRY(kspace<T>(P, out, e2, b, h, s));
+    TRY(col_pass<T>(P, e2, b, 2, OP_INV, s));
+    for (int l = 0; l < L; ++l) {
+      TRY(eigen_pass<T>(P, S, l, 0, eig, status, s));
+      TRY(row_pass<T>(P, e2, b, 2, OP_ROP, s, eig, S->thickness[l]));
+      ColArgs<T> ca{};
+      ca.da = S->thickness[l];
+      ca.ea = eff(l);
+      if (l + 1 < L) {
+        ca.db = S->thickness[l + 1];
+        ca.eb = eff(l + 1);
+      } else {
+        ca.db = 0.;  // output medium: zero thickness, only the betamax mask
+        ca.eb = mk<double>(S->n_out * S->n_out);
+      }
+      ca.scale = scale;
+      ca.dense_sign = 0;
+      TRY(col_pass<T>(P, e2, b, 2, OP_DIAG, s, &ca));
     }
-    ca.scale = scale;
-    ca.dense_sign = 0;
-    TRY(col_pass<T>(P, e2, nplanes * 2, 2, OP_DIAG, s, &ca));
+    TRY(row_pass<T>(P, e2, b, 2, OP_INV, s));
+    E2HArgs<T> ea{};
+    ea.src = e2;
+    ea.dst = out;
+    ea.H = H;
+    ea.W = W;
+    ea.b = b;
+    ea.eout = mk<double>(S->n_out * S->n_out);
+    ea.bm = P->bm;
+    dim3 grid((unsigned)((HW + NT - 1) / NT), nrays * b.nlam_b);
+    hipLaunchKernelGGL((e2h_kernel<T>), grid, dim3(NT), 0, s, ea);
+    HIP_TRY(hipGetLastError());
   }
-  if (L == 0) {
-    // no layer: the mask of the output medium is already in the projection;
-    // undo the scale there by applying the inverse transform only.
-  }
-  TRY(row_pass<T>(P, e2, nplanes, 2, OP_INV, s));
-  E2HArgs<T> ea{};
-  ea.src = e2;
-  ea.dst = out;
-  ea.H = H;
-  ea.W = W;
-  ea.eout = mk<double>(S->n_out * S->n_out);
-  ea.bm = P->bm;
-  dim3 grid((unsigned)((HW + NT - 1) / NT), nplanes);
-  hipLaunchKernelGGL((e2h_kernel<T>), grid, dim3(NT), 0, s, ea);
-  HIP_TRY(hipGetLastError());
   return DTMMX_OK;
 }
 
@@ -828,8 +432,9 @@ static int fft2_impl(void* data, int batch, int H, int W, int inverse, hipStream
   HIP_TRY(hipMemcpy(P.tw_h, th.data(), th.size() * sizeof(cpx<T>), hipMemcpyHostToDevice));
   P.bm.nlam = 1;
   P.bm.rays_per_beam = 1;
-  int r = row_pass<T>(&P, (cpx<T>*)data, batch, 1, inverse ? OP_INV : OP_FWD, s);
-  if (r == DTMMX_OK) r = col_pass<T>(&P, (cpx<T>*)data, batch, 1, inverse ? OP_INV : OP_FWD, s);
+  Batch b{batch, 1, 0, 1};
+  int r = row_pass<T>(&P, (cpx<T>*)data, b, 1, inverse ? OP_INV : OP_FWD, s);
+  if (r == DTMMX_OK) r = col_pass<T>(&P, (cpx<T>*)data, b, 1, inverse ? OP_INV : OP_FWD, s);
   hipError_t e = hipStreamSynchronize(s);
   (void)hipFree(P.tw_w);
   (void)hipFree(P.tw_h);
@@ -839,6 +444,7 @@ static int fft2_impl(void* data, int batch, int H, int W, int inverse, hipStream
 }
 
 }  // namespace dtmmx
+
 
 extern "C" {
 
@@ -941,9 +547,12 @@ int dtmmx_plan_destroy(dtmmx_plan* P) {
 size_t dtmmx_plan_workspace_size(const dtmmx_plan* P) {
   if (!P) return 0;
   size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  size_t HW = size_t(P->d.height) * P->d.width;
   size_t planes = size_t(P->d.n_rays) * P->d.n_wavelengths;
-  if (P->d.method == DTMMX_METHOD_2X2) return planes * 2 * P->d.height * P->d.width * cb;
-  return 0;
+  size_t nbeam = size_t(P->d.n_rays / P->d.rays_per_beam);
+  size_t bytes = nbeam * EIG_COMPS * HW * cb;  // eigen table of one layer
+  if (P->d.method == DTMMX_METHOD_2X2) bytes += planes * 2 * HW * cb;
+  return bytes;
 }
 
 int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,

@@ -1,13 +1,11 @@
-// LDS-staged mixed-radix Stockham FFT for one work-group.
+// Mixed-radix Stockham FFT building blocks (radix 2, 3, 4, 5, 8).
 //
-// A work-group transforms NL independent lines of length N that sit in LDS
-// (line l at buf + l * lstride).  Each stage reads R inputs per butterfly
-// from LDS into registers, barriers, applies twiddles and the radix-R DFT,
-// writes the outputs to their self-sorted positions and barriers again, so a
-// single buffer suffices (no ping-pong copy).  Lengths are 2^a 3^b 5^c; the
-// radix plan is fixed at compile time (8s first, then one 4 or 2, then 3s,
-// then 5s).  Transforms are unnormalised; DIR = -1 is the forward transform
-// exp(-2 pi i jk/N), DIR = +1 the inverse.
+// Lengths are 2^a 3^b 5^c; the radix plan is fixed at compile time (8s
+// first, then one 4 or 2, then 3s, then 5s).  Transforms are unnormalised;
+// DIR = -1 is the forward transform exp(-2 pi i jk/N), DIR = +1 the inverse.
+// Stage s with radix R and PP = product of the earlier radices maps
+// butterfly i (of N/R) from inputs x[i + r N/R] to outputs
+// y[(i/PP) PP R + i%PP + r PP] after twiddling input r by w_N^(r (i%PP) N/(PP R)).
 #pragma once
 #include "cpx.hpp"
 
@@ -116,71 +114,154 @@ __device__ __forceinline__ void butterfly(cpx<T>* v) {
   }
 }
 
-// One Stockham stage of radix R with PP = product of the earlier radices.
-template <typename T, int N, int R, int PP, int NL, int NT, int DIR>
-__device__ __forceinline__ void stockham_stage(cpx<T>* buf, int lstride,
-                                               const cpx<T>* __restrict__ tw, int tid) {
-  constexpr int NB = N / R;
-  constexpr int JOBS = NL * NB;
-  constexpr int JPT = (JOBS + NT - 1) / NT;
-  cpx<T> v[JPT][R];
-#pragma unroll
-  for (int jj = 0; jj < JPT; ++jj) {
-    const int job = tid + jj * NT;
-    if ((JOBS % NT) == 0 || job < JOBS) {
-      const int line = job / NB;
-      const int i = job - line * NB;
-      const cpx<T>* b = buf + line * lstride;
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[jj][r] = b[i + r * NB];
-    }
+// ---- register-resident transforms ----
+
+// ---------------------------------------------------------------------------
+// Register-resident Stockham transforms.
+//
+// A group of NL lanes transforms one line of length N.  Stage s uses radix
+// R_s of the plan (REV selects the reversed radix order), NB_s = N / R_s
+// butterflies, and lane l owns butterflies i = l + NL*m.  Before a stage the
+// lane holds the stage inputs v[m*R + r] = x[i + r*NB]; after the last stage
+// it holds the outputs in natural order, v[m*R + r] = y[i + r*NB_last].
+// Stages exchange data through a per-line LDS buffer (addressed through a
+// stride-and-pad map chosen by the caller); the first stage inputs and the
+// last stage outputs therefore never touch LDS.  Using the forward plan for
+// one transform and the reversed plan for the next one makes the last radix
+// of the first equal the first radix of the second, so a pointwise operator
+// between them applies to registers without any data movement.
+// ---------------------------------------------------------------------------
+
+template <int N, bool REV>
+__host__ __device__ constexpr int radix_at(int s) {
+  return REV ? make_plan(N).r[make_plan(N).ns - 1 - s] : make_plan(N).r[s];
+}
+
+template <int N, bool REV>
+__host__ __device__ constexpr int pp_at(int s) {
+  int v = 1;
+  for (int i = 0; i < s; ++i) v *= radix_at<N, REV>(i);
+  return v;
+}
+
+template <int N, bool REV, int NL>
+__host__ __device__ constexpr int max_vals() {
+  int m = 0;
+  for (int s = 0; s < make_plan(N).ns; ++s) {
+    int R = radix_at<N, REV>(s);
+    int J = (N / R + NL - 1) / NL;
+    if (J * R > m) m = J * R;
   }
-  __syncthreads();
+  return m;
+}
+
+// LDS address of line position x: x * STRIDE + x / 8 * PADW (PADW = STRIDE
+// adds one padding slot per 8 positions; 0 disables padding).
+template <int STRIDE, int PADW>
+__device__ __forceinline__ int lds_at(int x) {
+  return x * STRIDE + (PADW ? (x >> 3) * PADW : 0);
+}
+
+// Barrier between LDS write and read phases: the whole work-group, or only
+// the wave when a line is owned by a single wave (LDS operations of one wave
+// execute in order, so a compiler fence at wavefront scope is enough).
+template <bool WAVE>
+__device__ __forceinline__ void exchange_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <typename T, int N, bool REV, int DIR, int NL, int STRIDE, int PADW, int S, int MAXV,
+          bool WAVE = false>
+__device__ __forceinline__ void reg_stages(cpx<T> (&v)[MAXV], cpx<T>* lds,
+                                           const cpx<T>* __restrict__ tw, int l) {
+  constexpr int NS = make_plan(N).ns;
+  constexpr int R = radix_at<N, REV>(S);
+  constexpr int PP = pp_at<N, REV>(S);
+  constexpr int NB = N / R;
+  constexpr int J = (NB + NL - 1) / NL;
 #pragma unroll
-  for (int jj = 0; jj < JPT; ++jj) {
-    const int job = tid + jj * NT;
-    if ((JOBS % NT) == 0 || job < JOBS) {
-      const int line = job / NB;
-      const int i = job - line * NB;
-      const int k = i % PP;
+  for (int m = 0; m < J; ++m) {
+    const int i = l + NL * m;
+    if ((NB % NL) == 0 || i < NB) {
       if constexpr (PP > 1) {
         constexpr int STEP = N / (PP * R);
+        const int k = i % PP;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           cpx<T> w = tw[r * k * STEP];
           if (DIR > 0) w.y = -w.y;
-          v[jj][r] = v[jj][r] * w;
+          v[m * R + r] = v[m * R + r] * w;
         }
       }
-      butterfly<R, DIR>(v[jj]);
-      cpx<T>* b = buf + line * lstride;
-      const int base = (i / PP) * PP * R + k;
-#pragma unroll
-      for (int r = 0; r < R; ++r) b[base + r * PP] = v[jj][r];
+      butterfly<R, DIR>(&v[m * R]);
     }
   }
-  __syncthreads();
-}
-
-template <typename T, int N, int NL, int NT, int DIR, int S>
-__device__ __forceinline__ void stockham_from(cpx<T>* buf, int lstride, const cpx<T>* tw, int tid) {
-  constexpr RadixPlan P = make_plan(N);
-  if constexpr (S < P.ns) {
-    constexpr int R = P.r[S];
-    constexpr int PP = plan_prod(P, S);
-    stockham_stage<T, N, R, PP, NL, NT, DIR>(buf, lstride, tw, tid);
-    stockham_from<T, N, NL, NT, DIR, S + 1>(buf, lstride, tw, tid);
+  if constexpr (S + 1 < NS) {
+#pragma unroll
+    for (int m = 0; m < J; ++m) {
+      const int i = l + NL * m;
+      if ((NB % NL) == 0 || i < NB) {
+        const int base = (i / PP) * PP * R + (i % PP);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lds_at<STRIDE, PADW>(base + r * PP)] = v[m * R + r];
+      }
+    }
+    exchange_sync<WAVE>();
+    constexpr int R2 = radix_at<N, REV>(S + 1);
+    constexpr int NB2 = N / R2;
+    constexpr int J2 = (NB2 + NL - 1) / NL;
+#pragma unroll
+    for (int m = 0; m < J2; ++m) {
+      const int i = l + NL * m;
+      if ((NB2 % NL) == 0 || i < NB2) {
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[m * R2 + r] = lds[lds_at<STRIDE, PADW>(i + r * NB2)];
+      }
+    }
+    exchange_sync<WAVE>();
+    reg_stages<T, N, REV, DIR, NL, STRIDE, PADW, S + 1, MAXV, WAVE>(v, lds, tw, l);
   }
 }
 
-// Full in-LDS transform of NL lines.  The caller must have synchronised the
-// work-group after filling `buf`; the routine ends with a barrier.
-// `tw[t] = exp(-2 pi i t / N)`, t < N.
-template <typename T, int N, int NL, int NT, int DIR>
-__device__ __forceinline__ void lds_fft(cpx<T>* buf, int lstride, const cpx<T>* __restrict__ tw,
-                                        int tid) {
+template <typename T, int N, bool REV, int DIR, int NL, int STRIDE, int PADW, int MAXV,
+          bool WAVE = false>
+__device__ __forceinline__ void reg_fft(cpx<T> (&v)[MAXV], cpx<T>* lds,
+                                        const cpx<T>* __restrict__ tw, int l) {
   static_assert(make_plan(N).ns > 0, "FFT length must be 2^a 3^b 5^c");
-  stockham_from<T, N, NL, NT, DIR, 0>(buf, lstride, tw, tid);
+  reg_stages<T, N, REV, DIR, NL, STRIDE, PADW, 0, MAXV, WAVE>(v, lds, tw, l);
+}
+
+// Position of value slot j (= m*R + r) of the first stage (inputs) or of the
+// last stage (natural-order outputs): i + r*NB with i = l + NL*m.
+template <int N, bool REV, int NL, bool LAST>
+__device__ __forceinline__ int slot_pos(int j, int l) {
+  constexpr int S = LAST ? make_plan(N).ns - 1 : 0;
+  constexpr int R = radix_at<N, REV>(S);
+  constexpr int NB = N / R;
+  const int m = j / R, r = j - (j / R) * R;
+  return l + NL * m + r * NB;
+}
+
+template <int N, bool REV, int NL, bool LAST>
+__device__ __forceinline__ bool slot_valid(int j, int l) {
+  constexpr int S = LAST ? make_plan(N).ns - 1 : 0;
+  constexpr int R = radix_at<N, REV>(S);
+  constexpr int NB = N / R;
+  const int m = j / R;
+  return (NB % NL) == 0 || (l + NL * m) < NB;
+}
+
+template <int N, bool REV, int NL, bool LAST>
+__host__ __device__ constexpr int slot_count() {
+  constexpr int S = LAST ? make_plan(N).ns - 1 : 0;
+  constexpr int R = radix_at<N, REV>(S);
+  return ((N / R + NL - 1) / NL) * R;
 }
 
 }  // namespace dtmmx

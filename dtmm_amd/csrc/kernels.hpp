@@ -1,0 +1,607 @@
+// Device kernels of the MI355X (gfx950) diffractive transfer-matrix engine.
+// Included by dtmmx.hip (host driver and C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "berreman.hpp"
+#include "cpx.hpp"
+#include "dtmmx.h"
+#include "fft.hpp"
+
+namespace dtmmx {
+
+
+constexpr int NT = 256;  // threads per work-group for every kernel
+constexpr int MAXL = DTMMX_MAX_WAVELENGTHS;
+constexpr int MAXB = DTMMX_MAX_BEAMS;
+
+// Wavelengths and beams, passed by value to every kernel.
+struct Beams {
+  int nlam;           // wavelengths
+  int rays_per_beam;  // consecutive rays sharing a beam
+  double k0[MAXL];
+  double beta[MAXB];
+  double phi[MAXB];
+};
+
+__host__ __device__ inline double fftfreq(int i, int n) {
+  // numpy.fft.fftfreq(n)[i] = k * (1.0 / n) with k in [0..(n-1)/2, -n/2..-1]
+  int k = (i < (n + 1) / 2) ? i : i - n;
+  return double(k) * (1.0 / double(n));
+}
+
+__host__ __device__ inline int mode_index(int i, int n) {  // |k| of fftfreq
+  return (i < (n + 1) / 2) ? i : n - i;
+}
+
+// ---------------------------------------------------------------------------
+// Launch geometry shared by the per-layer kernels.
+// ---------------------------------------------------------------------------
+
+enum { OP_FWD = 0, OP_INV = 1, OP_ROP = 2, OP_DIAG = 3 };
+
+// Work-item index for block b of n such that consecutive work items land on
+// the same XCD (blocks are dispatched round-robin over the 8 XCDs): work
+// that shares data (the planes of one row) then shares one L2.
+__device__ __forceinline__ int xcd_swizzle(int b, int n) {
+  const int x = b & 7, k = b >> 3, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// Planes of the current batch: ray r (0..nrays-1) and wavelength lam0 + j
+// (j < nlam_b); plane index q = r * nlam + lam0 + j in the field layout
+// [ray][wavelength][component][H][W].
+struct Batch {
+  int nrays;
+  int nlam;
+  int lam0, nlam_b;
+};
+
+// ---------------------------------------------------------------------------
+// Per-pixel eigen data of a layer (fp64 solve, stored in the field precision)
+// as 13 complex planes: alpha0..3, P0 (2x2), P1 (2x2), ok.  See RopData.
+// ---------------------------------------------------------------------------
+
+constexpr int EIG_COMPS = 13;
+
+template <typename T>
+struct EigArgs {
+  cpx<T>* tab;  // [nbeam][13][H*W]
+  int HW;
+  const double* eps_re;
+  const double* eps_im;
+  const double* ang;
+  size_t comp_stride;
+  int method4;
+  int* status;
+  Beams bm;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void eigen_kernel(EigArgs<T> A) {
+  const int p = blockIdx.x * NT + threadIdx.x;
+  const int beam = blockIdx.y;
+  if (p >= A.HW) return;
+  cd e[3];
+  for (int c = 0; c < 3; ++c)
+    e[c] = mk<double>(A.eps_re[c * A.comp_stride + p], A.eps_im ? A.eps_im[c * A.comp_stride + p] : 0.);
+  const double theta = A.ang[1 * A.comp_stride + p];
+  const double phi = A.ang[2 * A.comp_stride + p];
+  const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
+  Modes m;
+  int kind;
+  layer_modes(beta, cos(phi0), sin(phi0), phi0, e[0], e[1], e[2], theta, phi, m, kind);
+  RopData r;
+  rop_data(m, kind, r);
+  if (kind == KIND_BIAXIAL && A.status) atomicOr(A.status, 1);
+  cpx<T>* o = A.tab + size_t(beam) * EIG_COMPS * A.HW + p;
+  for (int i = 0; i < 4; ++i) o[i * A.HW] = cast<T>(r.a[i]);
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      o[(4 + 2 * i + j) * A.HW] = cast<T>(r.P0[i][j]);
+      o[(8 + 2 * i + j) * A.HW] = cast<T>(r.P1[i][j]);
+    }
+  T ok = (A.method4 ? r.ok4 : r.ok2) ? T(1) : T(0);
+  if (kind == KIND_BIAXIAL) ok = T(__builtin_nan(""));
+  o[12 * A.HW] = mk<T>(ok, T(0));
+}
+
+// ---------------------------------------------------------------------------
+// Row kernel: a work-group of NR*NC waves owns one row y of NR rays sharing
+// one beam and loops over the wavelengths of the batch; wave li transforms
+// component line li with its values in registers (first-stage inputs read
+// from HBM, last-stage outputs written to HBM).  OP_ROP: inverse
+// x-transform, per-pixel layer operator (through LDS, eigen data of the
+// row held in registers across wavelengths), forward x-transform.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct RowArgs {
+  cpx<T>* field;
+  int H, W;
+  Batch b;
+  int rays_per_group;   // NR
+  const cpx<T>* tw;     // twiddles of length W
+  const cpx<T>* eig;    // eigen table of this layer (OP_ROP)
+  double d;             // layer thickness
+  Beams bm;
+};
+
+template <int N>
+__host__ __device__ constexpr int line_stride() {  // per-line LDS footprint with padding
+  return N + N / 8 + 1;
+}
+
+// Minimum waves per SIMD the register allocator must allow: 4 for complex64
+// (at most 128 VGPRs), 2 for complex128.
+template <typename T>
+__host__ __device__ constexpr int min_waves() {
+  return sizeof(T) == 4 ? 4 : 2;
+}
+
+template <typename T, int N, int NR, int NC, int OP>
+__global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowArgs<T> A) {
+  constexpr int NL = 64;
+  constexpr int LINES = NR * NC;
+  constexpr int NTW = 64 * LINES;
+  constexpr int LS = line_stride<N>();
+  constexpr bool REV1 = (OP != OP_FWD);
+  constexpr int DIR1 = (OP == OP_FWD) ? -1 : +1;
+  constexpr int MV1 = max_vals<N, REV1, NL>();
+  constexpr int MV2 = max_vals<N, false, NL>();
+  constexpr int MAXV = MV1 > MV2 ? MV1 : MV2;
+  constexpr int NIN = slot_count<N, REV1, NL, false>();
+  constexpr int NMID = slot_count<N, REV1, NL, true>();
+  constexpr bool REVL = (OP == OP_ROP) ? false : REV1;
+  constexpr int NOUT = slot_count<N, REVL, NL, true>();
+  constexpr int PPX = (N + NTW - 1) / NTW;  // pixels per thread in the layer operator
+  __shared__ cpx<T> lds[LINES * LS];
+  __shared__ cpx<T> tws[N];  // twiddles in LDS: their reads never wait on global loads
+  const int tid = threadIdx.x;
+  const int li = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < N; i += NTW) tws[i] = A.tw[i];
+  const int rr = li / NC, comp = li - rr * NC;
+  const int w = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int ngroups = A.b.nrays / NR;
+  const int rg = w % ngroups;
+  const int y = w / ngroups;
+  const size_t HW = size_t(A.H) * N;
+  cpx<T>* line_lds = lds + li * LS;
+
+  cpx<T> eg[(OP == OP_ROP) ? PPX : 1][EIG_COMPS];
+  if constexpr (OP == OP_ROP) {
+    const int beam = (rg * NR) / A.bm.rays_per_beam;
+    const cpx<T>* src = A.eig + size_t(beam) * EIG_COMPS * HW + size_t(y) * N;
+#pragma unroll
+    for (int k = 0; k < PPX; ++k) {
+      const int x = tid + k * NTW;
+#pragma unroll
+      for (int m = 0; m < EIG_COMPS; ++m) eg[k][m] = (x < N) ? src[m * HW + x] : mk<T>(T(0), T(0));
+    }
+  }
+  __syncthreads();
+
+  // line of wavelength lam0 + jl for this wave
+  auto line_ptr = [&](int jl) {
+    const size_t q = size_t(rg * NR + rr) * A.b.nlam + A.b.lam0 + jl;
+    return A.field + (q * NC + comp) * HW + size_t(y) * N;
+  };
+  cpx<T> nxt[NIN];  // prefetched inputs of the next wavelength
+  {
+    const cpx<T>* g0 = line_ptr(0);
+#pragma unroll
+    for (int j = 0; j < NIN; ++j)
+      if (slot_valid<N, REV1, NL, false>(j, lane)) nxt[j] = g0[slot_pos<N, REV1, NL, false>(j, lane)];
+  }
+  for (int jl = 0; jl < A.b.nlam_b; ++jl) {
+    const int lam = A.b.lam0 + jl;
+    cpx<T>* gl = line_ptr(jl);
+    cpx<T> v[MAXV];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) v[j] = nxt[j];
+    if (jl + 1 < A.b.nlam_b) {
+      const cpx<T>* gn = line_ptr(jl + 1);
+#pragma unroll
+      for (int j = 0; j < NIN; ++j)
+        if (slot_valid<N, REV1, NL, false>(j, lane)) nxt[j] = gn[slot_pos<N, REV1, NL, false>(j, lane)];
+    }
+    reg_fft<T, N, REV1, DIR1, NL, 1, 1, MAXV, true>(v, line_lds, tws, lane);
+
+    if constexpr (OP == OP_ROP) {
+#pragma unroll
+      for (int j = 0; j < NMID; ++j)
+        if (slot_valid<N, REV1, NL, true>(j, lane))
+          line_lds[lds_at<1, 1>(slot_pos<N, REV1, NL, true>(j, lane))] = v[j];
+      __syncthreads();
+      const T kd = T(A.bm.k0[lam] * A.d);
+#pragma unroll
+      for (int k = 0; k < PPX; ++k) {
+        const int x = tid + k * NTW;
+        if (x < N) {
+          const cpx<T>* e = eg[k];
+          const int ax = lds_at<1, 1>(x);
+          const T ok = e[12].x;
+          cpx<T> p0 = cexpi(e[0] * kd), p2 = cexpi(e[2] * kd);
+          cpx<T> dp = (p0 - p2) * ok;
+          p2 = p2 * ok;
+          if constexpr (NC == 4) {
+            cpx<T> p1 = cexpi(e[1] * kd), p3 = cexpi(e[3] * kd);
+            cpx<T> dn = (p1 - p3) * ok;
+            p3 = p3 * ok;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              cpx<T>* L = lds + (r * 4) * LS + ax;
+              cpx<T> u0 = L[0], u1 = L[LS], u2 = L[2 * LS], u3 = L[3 * LS];
+              cpx<T> w0 = e[4] * u0 + e[5] * u2, w2 = e[6] * u0 + e[7] * u2;
+              cpx<T> w1 = e[8] * u1 + e[9] * u3, w3 = e[10] * u1 + e[11] * u3;
+              L[0] = p2 * u0 + dp * w0;
+              L[2 * LS] = p2 * u2 + dp * w2;
+              L[LS] = p3 * u1 + dn * w1;
+              L[3 * LS] = p3 * u3 + dn * w3;
+            }
+          } else {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              cpx<T>* L = lds + (r * 2) * LS + ax;
+              cpx<T> u0 = L[0], u1 = L[LS];
+              cpx<T> w0 = e[4] * u0 + e[5] * u1, w1 = e[6] * u0 + e[7] * u1;
+              L[0] = p2 * u0 + dp * w0;
+              L[LS] = p2 * u1 + dp * w1;
+            }
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < NMID; ++j)
+        if (slot_valid<N, REV1, NL, true>(j, lane))
+          v[j] = line_lds[lds_at<1, 1>(slot_pos<N, REV1, NL, true>(j, lane))];
+      exchange_sync<true>();
+      reg_fft<T, N, false, -1, NL, 1, 1, MAXV, true>(v, line_lds, tws, lane);
+    }
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j)
+      if (slot_valid<N, REVL, NL, true>(j, lane)) gl[slot_pos<N, REVL, NL, true>(j, lane)] = v[j];
+    exchange_sync<true>();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column kernel: a work-group owns TX adjacent columns of one wavelength
+// (TX lanes read 8*TX contiguous bytes of a row; NLC lanes share a column,
+// one radix-8 butterfly each) and transforms the lines of ncl components in
+// turn.  OP_DIAG: forward y-transform, diagonal k-space operator (computed
+// once per work-group and reused for every line), inverse y-transform.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct ColArgs {
+  cpx<T>* field;
+  int H, W;
+  Batch b;
+  int NC;             // components per plane
+  int ncl;            // lines per work-group (divides nrays * NC)
+  const cpx<T>* tw;   // twiddles of length H
+  const int* amax;    // [nlam][H]: largest passing |kx| index per ky row, -1 if none
+  double da, db;      // thicknesses of the two half steps
+  cd ea, eb;          // isotropic effective eps of the two half steps
+  double scale;
+  int dense_sign;     // 1: backward components (odd c) get exp(-i phase)
+  Beams bm;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> diag_factor(const ColArgs<T>& A, int lam, int beam, int ky, int kx) {
+  const int mx = mode_index(kx, A.W), my = mode_index(ky, A.H);
+  if (mx > A.amax[lam * A.H + ky]) return mk<T>(T(0), T(0));
+  const T k0 = T(A.bm.k0[lam]);
+  const T l = T(2 * M_PI) / k0;
+  const T fx = T(mx) / T(A.W), fy = T(my) / T(A.H);
+  const T b2 = l * l * (fx * fx + fy * fy);
+  const T bc = T(A.bm.beta[beam]);
+  cpx<T> ph = mk<T>(T(0), T(0));
+  if (A.da != 0.) {
+    cpx<T> e = cast<T>(A.ea);
+    ph = ph + (csqrt(e - mk<T>(b2)) - csqrt(e - mk<T>(bc * bc))) * T(0.5 * A.da);
+  }
+  if (A.db != 0.) {
+    cpx<T> e = cast<T>(A.eb);
+    ph = ph + (csqrt(e - mk<T>(b2)) - csqrt(e - mk<T>(bc * bc))) * T(0.5 * A.db);
+  }
+  return cexpi(ph * k0) * T(A.scale);
+}
+
+// Lanes per column: one radix-8 butterfly per lane, 16..128 lanes.
+template <int N>
+__host__ __device__ constexpr int col_lanes() {
+  return N / 8 < 16 ? 16 : (N / 8 > 128 ? 128 : N / 8);
+}
+
+template <typename T, int N, int TX, int OP>
+__global__ __launch_bounds__(TX * col_lanes<N>(), min_waves<T>()) void col_kernel(ColArgs<T> A) {
+  constexpr int NL = col_lanes<N>();
+  constexpr bool REV1 = (OP == OP_INV);
+  constexpr int DIR1 = (OP == OP_INV) ? +1 : -1;
+  constexpr int MV1 = max_vals<N, REV1, NL>();
+  constexpr int MV2 = max_vals<N, true, NL>();
+  constexpr int MAXV = MV1 > MV2 ? MV1 : MV2;
+  constexpr int NIN = slot_count<N, REV1, NL, false>();
+  constexpr int NMID = slot_count<N, REV1, NL, true>();
+  __shared__ cpx<T> lds[N * TX];
+  __shared__ cpx<T> tws[N];  // twiddles in LDS: their reads never wait on global loads
+  const int tid = threadIdx.x;
+  const int c = tid % TX, g = tid / TX;
+  for (int i = tid; i < N; i += TX * NL) tws[i] = A.tw[i];
+  __syncthreads();
+  const int ntiles = (A.W + TX - 1) / TX;
+  const int w = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int tile = w % ntiles;
+  const int t = w / ntiles;
+  const int lam = A.b.lam0 + t % A.b.nlam_b;
+  const int lg = t / A.b.nlam_b;
+  const int x = tile * TX + c;
+  const bool xin = x < A.W;
+  const size_t HW = size_t(N) * A.W;
+
+  cpx<T> K[(OP == OP_DIAG) ? NMID : 1];
+  if constexpr (OP == OP_DIAG) {
+    const int ray0 = (lg * A.ncl) / A.NC;  // all lines of a work-group share one beam
+    const int beam = ray0 / A.bm.rays_per_beam;
+#pragma unroll
+    for (int j = 0; j < NMID; ++j) {
+      K[j] = mk<T>(T(0), T(0));
+      if (xin && slot_valid<N, REV1, NL, true>(j, g))
+        K[j] = diag_factor(A, lam, beam, slot_pos<N, REV1, NL, true>(j, g), x);
+    }
+  }
+  auto line_base = [&](int li) {
+    const int line = lg * A.ncl + li;
+    const int ray = line / A.NC, comp = line % A.NC;
+    const size_t q = size_t(ray) * A.b.nlam + lam;
+    return A.field + (q * A.NC + comp) * HW + x;
+  };
+  cpx<T> nxt[NIN];  // prefetched inputs of the next line
+  {
+    const cpx<T>* b0 = line_base(0);
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) {
+      nxt[j] = mk<T>(T(0), T(0));
+      if (xin && slot_valid<N, REV1, NL, false>(j, g))
+        nxt[j] = b0[size_t(slot_pos<N, REV1, NL, false>(j, g)) * A.W];
+    }
+  }
+  for (int li = 0; li < A.ncl; ++li) {
+    const int comp = (lg * A.ncl + li) % A.NC;
+    cpx<T>* base = line_base(li);
+    cpx<T> v[MAXV];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) v[j] = nxt[j];
+    if (li + 1 < A.ncl) {
+      const cpx<T>* bn = line_base(li + 1);
+#pragma unroll
+      for (int j = 0; j < NIN; ++j)
+        if (xin && slot_valid<N, REV1, NL, false>(j, g))
+          nxt[j] = bn[size_t(slot_pos<N, REV1, NL, false>(j, g)) * A.W];
+    }
+    reg_fft<T, N, REV1, DIR1, NL, TX, 0>(v, lds + c, tws, g);
+    if constexpr (OP == OP_DIAG) {
+      const bool back = A.dense_sign && (comp & 1);
+      const T s2 = T(A.scale * A.scale);
+#pragma unroll
+      for (int j = 0; j < NMID; ++j) {
+        cpx<T> k = K[j];
+        if (back) {
+          const T n2 = norm2(k);
+          k = n2 > T(0) ? conj(k) * (s2 / n2) : mk<T>(T(0), T(0));
+        }
+        v[j] = v[j] * k;
+      }
+      reg_fft<T, N, true, +1, NL, TX, 0>(v, lds + c, tws, g);
+    }
+    constexpr bool REVL = (OP == OP_DIAG) ? true : REV1;
+    constexpr int NOUT = slot_count<N, REVL, NL, true>();
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j)
+      if (xin && slot_valid<N, REVL, NL, true>(j, g))
+        base[size_t(slot_pos<N, REVL, NL, true>(j, g)) * A.W] = v[j];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense per-k operators (fp64), one thread per (k, plane).
+// ---------------------------------------------------------------------------
+
+enum { KOP_HEAD4 = 0, KOP_TAIL4 = 1, KOP_BOUND4 = 2, KOP_PROJ2 = 3 };
+
+template <typename T>
+struct KArgs {
+  const cpx<T>* src;
+  cpx<T>* dst;
+  int H, W;
+  Batch b;
+  const int* amax;
+  int kop;
+  double d0, d1;  // thickness of the layer(s) involved
+  cd e0, e1;      // effective eps of the layer(s) involved
+  cd ein;         // input medium eps (PROJ2)
+  double scale;
+  Beams bm;
+};
+
+// Modes of an isotropic medium at in-plane wave vector (beta, phi), rotated.
+__device__ inline void iso_at(double beta, double phi, cd eps, Modes& m) {
+  iso_modes(beta, eps, m);
+  rotate_rows(m, cos(phi), sin(phi));
+}
+
+__device__ inline void emat(const Modes& m, cd E[4][4]) {  // dtmm/tmm.py:990-996
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) E[i][j] = mk<double>(0.);
+  E[0][0] = m.F[0][0];
+  E[0][2] = m.F[0][2];
+  E[2][0] = m.F[2][0];
+  E[2][2] = m.F[2][2];
+  E[1][1] = m.F[0][1];
+  E[1][3] = m.F[0][3];
+  E[3][1] = m.F[2][1];
+  E[3][3] = m.F[2][3];
+}
+
+// X . diag(exp(i kd a)) . Y
+__device__ inline void xpy(const cd X[4][4], const cd a[4], double kd, const cd Y[4][4], cd O[4][4]) {
+  cd p[4];
+  for (int i = 0; i < 4; ++i) p[i] = cexpi(a[i] * kd);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      cd s = mk<double>(0.);
+      for (int k = 0; k < 4; ++k) s = s + X[i][k] * p[k] * Y[k][j];
+      O[i][j] = s;
+    }
+}
+
+// Mean-beam correction E_c P(-kd) E_c^-1 (dtmm/matrix.py:132-139).
+__device__ inline void correction(double beta, double phi, cd eps, double kd, cd C[4][4]) {
+  Modes m;
+  iso_at(beta, phi, eps, m);
+  cd E[4][4], Ei[4][4];
+  emat(m, E);
+  inv4(E, Ei);
+  xpy(E, m.a, -kd, Ei, C);
+}
+
+// First half step E P(kd) F^-1 at wave vector k (dtmm/matrix.py:109-118).
+__device__ inline void first_half(double beta, double phi, cd eps, double kd, cd D[4][4]) {
+  Modes m;
+  iso_at(beta, phi, eps, m);
+  cd E[4][4], Fi[4][4];
+  emat(m, E);
+  inv4(m.F, Fi);
+  xpy(E, m.a, kd, Fi, D);
+}
+
+// Second half step F P(kd) E^-1 at wave vector k (dtmm/matrix.py:120-130).
+__device__ inline void second_half(double beta, double phi, cd eps, double kd, cd D[4][4]) {
+  Modes m;
+  iso_at(beta, phi, eps, m);
+  cd E[4][4], Ei[4][4];
+  emat(m, E);
+  inv4(E, Ei);
+  xpy(m.F, m.a, kd, Ei, D);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
+  const int k = blockIdx.x * NT + threadIdx.x;
+  const int ray = blockIdx.y / A.b.nlam_b;
+  const int lam = A.b.lam0 + blockIdx.y % A.b.nlam_b;
+  const int q = ray * A.b.nlam + lam;
+  const size_t HW = size_t(A.H) * A.W;
+  if (k >= int(HW)) return;
+  const int ky = k / A.W, kx = k - ky * A.W;
+  const int beam = ray / A.bm.rays_per_beam;
+  const double k0 = A.bm.k0[lam];
+  const double bb = A.bm.beta[beam], pb = A.bm.phi[beam];
+  const bool pass = mode_index(kx, A.W) <= A.amax[lam * A.H + ky];
+  const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
+  const double beta = (2 * M_PI / k0) * sqrt(fx * fx + fy * fy);
+  const double phi = atan2(fy, fx);
+  const int nin = 4;
+  const int nout = (A.kop == KOP_PROJ2) ? 2 : 4;
+  const cpx<T>* s = A.src + size_t(q) * nin * HW + k;
+  cpx<T>* o = A.dst + size_t(q) * nout * HW + k;
+  cd v[4];
+  for (int c = 0; c < nin; ++c) v[c] = cast<double>(s[c * HW]);
+  cd M[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) M[i][j] = mk<double>(0.);
+  if (pass) {
+    if (A.kop == KOP_HEAD4) {
+      cd C[4][4], D[4][4];
+      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C);
+      first_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D);
+      matmul4(C, D, M);
+    } else if (A.kop == KOP_TAIL4) {
+      cd C[4][4], D[4][4];
+      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C);
+      second_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D);
+      matmul4(D, C, M);
+    } else if (A.kop == KOP_BOUND4) {
+      cd C0[4][4], D0[4][4], C1[4][4], D1[4][4], X[4][4], Y[4][4];
+      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C0);
+      second_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D0);
+      correction(bb, pb, A.e1, k0 * A.d1 * 0.5, C1);
+      first_half(beta, phi, A.e1, k0 * A.d1 * 0.5, D1);
+      matmul4(D0, C0, X);  // second corrected of layer 0
+      matmul4(C1, D1, Y);  // first corrected of layer 1
+      matmul4(Y, X, M);
+    } else {
+      // rows (Ex, Ey) of the forward projection in the input medium
+      // (dtmm/diffract.py:242-251) times the first 2x2 half step.
+      Modes m;
+      iso_at(beta, phi, A.ein, m);
+      cd Fi[4][4];
+      inv4(m.F, Fi);
+      cd P[4][4];
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) P[i][j] = m.F[i][0] * Fi[0][j] + m.F[i][2] * Fi[2][j];
+      cd ph = mk<double>(0.);
+      if (A.d0 != 0.) {
+        cd ak = csqrt(A.e0 - mk<double>(beta * beta));
+        cd ac = csqrt(A.e0 - mk<double>(bb * bb));
+        ph = (ak - ac) * (0.5 * A.d0 * k0);
+      }
+      cd K = cexpi(ph);
+      for (int j = 0; j < 4; ++j) {
+        M[0][j] = K * P[0][j];
+        M[1][j] = K * P[2][j];
+      }
+    }
+  }
+  for (int i = 0; i < nout; ++i) {
+    cd r = mk<double>(0.);
+    for (int j = 0; j < nin; ++j) r = r + M[i][j] * v[j];
+    o[i * HW] = cast<T>(r * A.scale);
+  }
+}
+
+// (Ex, Ey) -> (Ex, Hy, Ey, Hx) with the mean-beam H/E relation of the output
+// medium (dtmm/propagate_2x2.py:81-85, dtmm/tmm.py:1051-1072).
+template <typename T>
+struct E2HArgs {
+  const cpx<T>* src;
+  cpx<T>* dst;
+  int H, W;
+  Batch b;
+  cd eout;
+  Beams bm;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void e2h_kernel(E2HArgs<T> A) {
+  const int k = blockIdx.x * NT + threadIdx.x;
+  const int ray = blockIdx.y / A.b.nlam_b;
+  const int q = ray * A.b.nlam + A.b.lam0 + blockIdx.y % A.b.nlam_b;
+  const size_t HW = size_t(A.H) * A.W;
+  if (k >= int(HW)) return;
+  const int beam = ray / A.bm.rays_per_beam;
+  Modes m;
+  iso_modes(A.bm.beta[beam], A.eout, m);
+  rotate_rows(m, cos(A.bm.phi[beam]), sin(A.bm.phi[beam]));
+  cd Am[2][2] = {{m.F[0][0], m.F[0][2]}, {m.F[2][0], m.F[2][2]}};
+  cd Bm[2][2] = {{m.F[1][0], m.F[1][2]}, {m.F[3][0], m.F[3][2]}};
+  cd Ai[2][2];
+  inv2(Am, Ai);
+  cd G[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) G[i][j] = Bm[i][0] * Ai[0][j] + Bm[i][1] * Ai[1][j];
+  const cpx<T>* s = A.src + size_t(q) * 2 * HW + k;
+  cpx<T>* o = A.dst + size_t(q) * 4 * HW + k;
+  cd ex = cast<double>(s[0]), ey = cast<double>(s[HW]);
+  o[0] = cast<T>(ex);
+  o[HW] = cast<T>(G[0][0] * ex + G[0][1] * ey);
+  o[2 * HW] = cast<T>(ey);
+  o[3 * HW] = cast<T>(G[1][0] * ex + G[1][1] * ey);
+}
+
+}  // namespace dtmmx
