@@ -133,16 +133,16 @@ def main():
         plan.execute(stack, fin, fout, ws, stream=stream)
     torch.cuda.synchronize()
 
-    _lib.check(lib.dtmmx_timing_begin(8 * args.steps * (L + 2)))
+    _lib.check(lib.dtmmx_timing_begin(8 * args.steps * (L + 4)))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.execute(stack, fin, fout, ws, stream=stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ms = (ctypes.c_double * 2)()
-    cnt = (ctypes.c_int * 2)()
-    _lib.check(lib.dtmmx_timing_end(ms, cnt, 2))
+    ms = (ctypes.c_double * 4)()
+    cnt = (ctypes.c_int * 4)()
+    _lib.check(lib.dtmmx_timing_end(ms, cnt, 4))
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     steps = H * W * L * planes
     value = steps / (ms_per_step / 1e3)
@@ -166,7 +166,13 @@ def main():
     avg_ms = kern[dom]["ms"] / max(kern[dom]["n"], 1)
     achieved = kern[dom]["bytes"] / (avg_ms / 1e3) / 1e9
     log("kernels: " + ", ".join("%s %d x %.3f ms" % (k, v["n"], v["ms"] / max(v["n"], 1))
-                                 for k, v in kern.items()))
+                                 for k, v in kern.items())
+        + ", eigen %d x %.3f ms, kspace %d x %.3f ms" % (cnt[2], ms[2] / max(cnt[2], 1),
+                                                        cnt[3], ms[3] / max(cnt[3], 1)))
+    other_ms = ms_per_step - (ms[0] + ms[1] + ms[2] + ms[3]) / args.steps
+    log("per pass: row %.2f ms, col %.2f ms, eigen %.2f ms, kspace %.2f ms, rest %.2f ms"
+        % (ms[0] / args.steps, ms[1] / args.steps, ms[2] / args.steps, ms[3] / args.steps,
+           other_ms))
 
     extra = {}
     if not args.no_ref and args.precision == "c64":

@@ -33,6 +33,7 @@ EXPORTS = (
     "dtmmx_host_alphaf",
     "dtmmx_timing_begin",
     "dtmmx_timing_end",
+    "dtmmx_host_rop",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -106,6 +107,8 @@ def load():
     lib.dtmmx_fft2.restype = ctypes.c_int
     lib.dtmmx_host_alphaf.argtypes = [ctypes.c_int] + [_dp] * 7
     lib.dtmmx_host_alphaf.restype = ctypes.c_int
+    lib.dtmmx_host_rop.argtypes = [ctypes.c_int] + [_dp] * 7 + [ctypes.POINTER(ctypes.c_int)]
+    lib.dtmmx_host_rop.restype = ctypes.c_int
     lib.dtmmx_timing_begin.argtypes = [ctypes.c_int]
     lib.dtmmx_timing_begin.restype = ctypes.c_int
     lib.dtmmx_timing_end.argtypes = [_dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]

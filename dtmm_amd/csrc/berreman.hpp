@@ -264,3 +264,100 @@ __host__ __device__ inline void rop_data(const Modes& m, int kind, RopData& r) {
 }
 
 }  // namespace dtmmx
+
+namespace dtmmx {
+
+// Lean per-pixel solve for the real-space layer operator.  Only the E rows
+// (Ex, Ey) of the field matrix enter E diag(p) E^-1, and that product is
+// invariant to the scale of each mode column, so the column normalisation of
+// dtmm/tmm.py:278-288 and the H rows are skipped.  Same eigenvalues and the
+// same projectors as layer_modes + rop_data (checked against the complex128
+// oracle by tests/test_abi.py through dtmmx_host_rop).
+__host__ __device__ inline void rop_lean(double beta, double c0, double s0, double phi0, cd e0,
+                                         cd e1, cd e2, double theta, double phi, RopData& r) {
+  cd Fx[4], Fy[4];  // rows Ex and Ey of F before the beam-azimuth rotation
+  const int kind = material_kind(e0, e1, e2);
+  r.kind = kind;
+  if (kind == KIND_ISO) {
+    const cd a = csqrt(e0 - mk<double>(beta * beta));
+    r.a[0] = a;
+    r.a[1] = -a;
+    r.a[2] = a;
+    r.a[3] = -a;
+    const double h = is_zero(a) ? 0. : 0.5;
+    Fx[0] = Fx[1] = mk<double>(h);
+    Fx[2] = Fx[3] = mk<double>(0.);
+    Fy[0] = Fy[1] = mk<double>(0.);
+    Fy[2] = Fy[3] = mk<double>(h);
+  } else if (kind == KIND_UNIAXIAL) {
+    double st, ct, sp, cp;
+    sincos(theta, &st, &ct);
+    sincos(phi - phi0, &sp, &cp);
+    const cd eps11 = e0, delta = e2 - e0;
+    if (beta == 0.) {
+      const cd evs = csqrt(eps11);
+      const cd u = eps11 + delta * (ct * ct);
+      const cd w = eps11 * (eps11 + delta);
+      const cd sq = cdiv(csqrt(u * w), u);
+      r.a[0] = sq;
+      r.a[1] = -sq;
+      r.a[2] = evs;
+      r.a[3] = -evs;
+      Fx[0] = Fx[1] = mk<double>(cp);
+      Fy[0] = Fy[1] = mk<double>(sp);
+      Fx[2] = Fx[3] = evs * (-sp);
+      Fy[2] = Fy[3] = evs * cp;
+    } else {
+      const cd ev02 = eps11 - mk<double>(beta * beta);
+      const cd evs = csqrt(ev02);
+      const cd u = eps11 + delta * (ct * ct);
+      const double gama = beta * cp;
+      const cd v = delta * (gama * 2. * st * ct);
+      const cd w = delta * (gama * gama * st * st) - eps11 * (ev02 + delta);
+      const cd sq = cdiv(csqrt(v * v - u * w * 4.), u) * 0.5;
+      const cd vh = cdiv(v, u) * 0.5;
+      const cd evpp = -vh + sq, evpm = -vh - sq;
+      r.a[0] = evpp;
+      r.a[1] = evpm;
+      r.a[2] = evs;
+      r.a[3] = -evs;
+      const double sfst = st * sp, cfst = cp * st, ctbeta = ct * beta;
+      const cd ctbe = cdiv(mk<double>(ctbeta), eps11);
+      const cd ev02cfsteps11 = cdiv(ev02 * cfst, eps11);
+      Fx[0] = evpp * ctbe - ev02cfsteps11;
+      Fy[0] = mk<double>(-sfst);
+      Fx[1] = -(evpm * ctbe) + ev02cfsteps11;
+      Fy[1] = mk<double>(sfst);
+      Fx[2] = evs * sfst;
+      Fy[2] = mk<double>(ctbeta) - evs * cfst;
+      Fx[3] = evs * (-sfst);
+      Fy[3] = evs * cfst + mk<double>(ctbeta);
+    }
+  } else {
+    const double nan = __builtin_nan("");
+    for (int i = 0; i < 4; ++i) {
+      r.a[i] = mk<double>(nan, nan);
+      Fx[i] = Fy[i] = mk<double>(nan, nan);
+    }
+  }
+  // beam-azimuth rotation of the E rows (dtmm/linalg.py:674-713)
+  cd Ex[4], Ey[4];
+  for (int j = 0; j < 4; ++j) {
+    Ex[j] = Fx[j] * c0 - Fy[j] * s0;
+    Ey[j] = Fx[j] * s0 + Fy[j] * c0;
+  }
+  cd ep[2][2] = {{Ex[0], Ex[2]}, {Ey[0], Ey[2]}};
+  cd en[2][2] = {{Ex[1], Ex[3]}, {Ey[1], Ey[3]}};
+  cd ip[2][2], in[2][2];
+  const bool okp = inv2(ep, ip);
+  const bool okn = inv2(en, in);
+  r.ok2 = okp;
+  r.ok4 = okp && okn;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      r.P0[i][j] = ep[i][0] * ip[0][j];
+      r.P1[i][j] = en[i][0] * in[0][j];
+    }
+}
+
+}  // namespace dtmmx

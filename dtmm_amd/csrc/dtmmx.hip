@@ -157,7 +157,8 @@ struct dtmmx_plan {
 namespace dtmmx {
 
 // Optional per-launch timing of the per-layer kernels with HIP events on the
-// launch stream (class 0: row layer kernel, 1: column k-space kernel).
+// launch stream (class 0: row layer kernel, 1: column k-space kernel,
+// 3: dense k-space head/tail kernel).
 struct KTimer {
   bool on = false;
   int cap = 0, used = 0;
@@ -180,15 +181,14 @@ static void timer_stop(hipStream_t s) {
 
 template <typename T>
 static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op, hipStream_t s,
-                    const cpx<T>* eig = nullptr, double d = 0.) {
+                    const RowArgs<T>* proto = nullptr) {
   RowArgs<T> a{};
+  if (proto) a = *proto;
   a.field = field;
   a.H = P->d.height;
   a.W = P->d.width;
   a.b = b;
   a.tw = (const cpx<T>*)P->tw_w;
-  a.eig = eig;
-  a.d = d;
   a.bm = P->bm;
   int nr = 1;
   if (sizeof(T) == 4 && a.W <= 512 && b.nrays % 2 == 0 &&
@@ -258,29 +258,25 @@ static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, 
   a.bm = P->bm;
   const int HW = a.H * a.W;
   dim3 grid((HW + NT - 1) / NT, b.nrays * b.nlam_b);
+  timer_start(3, s);
   hipLaunchKernelGGL((kspace_kernel<T>), grid, dim3(NT), 0, s, a);
+  timer_stop(s);
   HIP_TRY(hipGetLastError());
   return DTMMX_OK;
 }
 
+// Row-kernel arguments of the layer operator of layer l.
 template <typename T>
-static int eigen_pass(const dtmmx_plan* P, const dtmmx_stack* S, int l, int method4, cpx<T>* tab,
-                      int* status, hipStream_t s) {
-  EigArgs<T> a{};
-  const size_t HW = size_t(P->d.height) * P->d.width;
-  a.tab = tab;
-  a.HW = int(HW);
+static RowArgs<T> layer_args(const dtmmx_stack* S, size_t HW, int l, int method4, int* status) {
+  RowArgs<T> a{};
   a.eps_re = S->eps_re + size_t(l) * HW;
   a.eps_im = S->eps_im ? S->eps_im + size_t(l) * HW : nullptr;
   a.ang = S->angles + size_t(l) * HW;
   a.comp_stride = size_t(S->n_layers) * HW;
+  a.d = S->thickness[l];
   a.method4 = method4;
   a.status = status;
-  a.bm = P->bm;
-  dim3 grid(unsigned((HW + NT - 1) / NT), P->d.n_rays / P->d.rays_per_beam);
-  hipLaunchKernelGGL((eigen_kernel<T>), grid, dim3(NT), 0, s, a);
-  HIP_TRY(hipGetLastError());
-  return DTMMX_OK;
+  return a;
 }
 
 #define TRY(x)                     \
@@ -309,7 +305,6 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
   const size_t HW = size_t(H) * W;
   const int nrays = P->d.n_rays, nlam = P->d.n_wavelengths;
   const int nplanes = nrays * nlam;
-  const int nbeam = nrays / P->d.rays_per_beam;
   const int L = S->n_layers;
   const double scale = 1.0 / double(HW);
   const bool m4 = P->d.method == DTMMX_METHOD_4X4;
@@ -319,9 +314,8 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
   cpx<T>* out = (cpx<T>*)fout;
   HIP_TRY(hipMemcpyAsync(out, fin, size_t(nplanes) * 4 * HW * sizeof(cpx<T>),
                          hipMemcpyDeviceToDevice, s));
-  // workspace: [eigen table of one layer][2x2: 2-component field]
-  cpx<T>* eig = (cpx<T>*)ws;
-  cpx<T>* e2 = eig + size_t(nbeam) * EIG_COMPS * HW;
+  // workspace: 2x2 method: the 2-component field
+  cpx<T>* e2 = (cpx<T>*)ws;
   const int nc = m4 ? 4 : 2;
   const int nlb = lam_batch(P, size_t(nrays) * 4 * HW * sizeof(cpx<T>));
   for (int lam0 = 0; lam0 < nlam; lam0 += nlb) {
@@ -338,8 +332,8 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
       TRY(kspace<T>(P, out, out, b, k, s));
       TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
       for (int l = 0; l < L; ++l) {
-        TRY(eigen_pass<T>(P, S, l, 1, eig, status, s));
-        TRY(row_pass<T>(P, out, b, 4, OP_ROP, s, eig, S->thickness[l]));
+        const RowArgs<T> la = layer_args<T>(S, HW, l, 1, status);
+        TRY(row_pass<T>(P, out, b, 4, OP_ROP, s, &la));
         if (l + 1 < L) {
           cd e0 = eff(l), e1 = eff(l + 1);
           if (eq(e0, e1)) {
@@ -388,8 +382,8 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
     TRY(kspace<T>(P, out, e2, b, h, s));
     TRY(col_pass<T>(P, e2, b, 2, OP_INV, s));
     for (int l = 0; l < L; ++l) {
-      TRY(eigen_pass<T>(P, S, l, 0, eig, status, s));
-      TRY(row_pass<T>(P, e2, b, 2, OP_ROP, s, eig, S->thickness[l]));
+      const RowArgs<T> la = layer_args<T>(S, HW, l, 0, status);
+      TRY(row_pass<T>(P, e2, b, 2, OP_ROP, s, &la));
       ColArgs<T> ca{};
       ca.da = S->thickness[l];
       ca.ea = eff(l);
@@ -549,10 +543,8 @@ size_t dtmmx_plan_workspace_size(const dtmmx_plan* P) {
   size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
   size_t HW = size_t(P->d.height) * P->d.width;
   size_t planes = size_t(P->d.n_rays) * P->d.n_wavelengths;
-  size_t nbeam = size_t(P->d.n_rays / P->d.rays_per_beam);
-  size_t bytes = nbeam * EIG_COMPS * HW * cb;  // eigen table of one layer
-  if (P->d.method == DTMMX_METHOD_2X2) bytes += planes * 2 * HW * cb;
-  return bytes;
+  if (P->d.method == DTMMX_METHOD_2X2) return planes * 2 * HW * cb;
+  return 0;
 }
 
 int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
@@ -642,6 +634,36 @@ int dtmmx_host_alphaf(int n, const double* beta, const double* phi, const double
         fmat[32 * i + 8 * j + 2 * k + 1] = m.F[j][k].y;
       }
     }
+  }
+  return rc;
+}
+
+int dtmmx_host_rop(int n, const double* beta, const double* phi, const double* eps_re,
+                   const double* eps_im, const double* angles, double* alpha, double* proj,
+                   int* ok) {
+  if (n < 0 || !beta || !phi || !eps_re || !angles || !alpha || !proj || !ok)
+    return fail(DTMMX_E_ARG, "null argument");
+  int rc = DTMMX_OK;
+  for (int i = 0; i < n; ++i) {
+    cd e[3];
+    for (int c = 0; c < 3; ++c) e[c] = mk<double>(eps_re[3 * i + c], eps_im ? eps_im[3 * i + c] : 0.);
+    RopData r;
+    rop_lean(beta[i], std::cos(phi[i]), std::sin(phi[i]), phi[i], e[0], e[1], e[2],
+             angles[3 * i + 1], angles[3 * i + 2], r);
+    if (r.kind == KIND_BIAXIAL) rc = fail(DTMMX_E_UNSUPPORTED, "biaxial voxel");
+    for (int j = 0; j < 4; ++j) {
+      alpha[8 * i + 2 * j] = r.a[j].x;
+      alpha[8 * i + 2 * j + 1] = r.a[j].y;
+    }
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) {
+        proj[16 * i + 4 * a + 2 * b] = r.P0[a][b].x;
+        proj[16 * i + 4 * a + 2 * b + 1] = r.P0[a][b].y;
+        proj[16 * i + 8 + 4 * a + 2 * b] = r.P1[a][b].x;
+        proj[16 * i + 8 + 4 * a + 2 * b + 1] = r.P1[a][b].y;
+      }
+    ok[2 * i] = r.ok4 ? 1 : 0;
+    ok[2 * i + 1] = r.ok2 ? 1 : 0;
   }
   return rc;
 }

@@ -57,54 +57,7 @@ struct Batch {
   int lam0, nlam_b;
 };
 
-// ---------------------------------------------------------------------------
-// Per-pixel eigen data of a layer (fp64 solve, stored in the field precision)
-// as 13 complex planes: alpha0..3, P0 (2x2), P1 (2x2), ok.  See RopData.
-// ---------------------------------------------------------------------------
-
-constexpr int EIG_COMPS = 13;
-
-template <typename T>
-struct EigArgs {
-  cpx<T>* tab;  // [nbeam][13][H*W]
-  int HW;
-  const double* eps_re;
-  const double* eps_im;
-  const double* ang;
-  size_t comp_stride;
-  int method4;
-  int* status;
-  Beams bm;
-};
-
-template <typename T>
-__global__ __launch_bounds__(NT) void eigen_kernel(EigArgs<T> A) {
-  const int p = blockIdx.x * NT + threadIdx.x;
-  const int beam = blockIdx.y;
-  if (p >= A.HW) return;
-  cd e[3];
-  for (int c = 0; c < 3; ++c)
-    e[c] = mk<double>(A.eps_re[c * A.comp_stride + p], A.eps_im ? A.eps_im[c * A.comp_stride + p] : 0.);
-  const double theta = A.ang[1 * A.comp_stride + p];
-  const double phi = A.ang[2 * A.comp_stride + p];
-  const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
-  Modes m;
-  int kind;
-  layer_modes(beta, cos(phi0), sin(phi0), phi0, e[0], e[1], e[2], theta, phi, m, kind);
-  RopData r;
-  rop_data(m, kind, r);
-  if (kind == KIND_BIAXIAL && A.status) atomicOr(A.status, 1);
-  cpx<T>* o = A.tab + size_t(beam) * EIG_COMPS * A.HW + p;
-  for (int i = 0; i < 4; ++i) o[i * A.HW] = cast<T>(r.a[i]);
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) {
-      o[(4 + 2 * i + j) * A.HW] = cast<T>(r.P0[i][j]);
-      o[(8 + 2 * i + j) * A.HW] = cast<T>(r.P1[i][j]);
-    }
-  T ok = (A.method4 ? r.ok4 : r.ok2) ? T(1) : T(0);
-  if (kind == KIND_BIAXIAL) ok = T(__builtin_nan(""));
-  o[12 * A.HW] = mk<T>(ok, T(0));
-}
+constexpr int EIG_COMPS = 13;  // per-pixel operator data: alpha0..3, P0, P1, ok
 
 // ---------------------------------------------------------------------------
 // Row kernel: a work-group of NR*NC waves owns one row y of NR rays sharing
@@ -122,8 +75,14 @@ struct RowArgs {
   Batch b;
   int rays_per_group;   // NR
   const cpx<T>* tw;     // twiddles of length W
-  const cpx<T>* eig;    // eigen table of this layer (OP_ROP)
+  // layer operator (OP_ROP): material of this layer, [3][L*H*W] fp64
+  const double* eps_re;
+  const double* eps_im;
+  const double* ang;
+  size_t comp_stride;   // L*H*W
   double d;             // layer thickness
+  int method4;          // 1: 4x4 (E invertible), 0: 2x2 (E+ invertible)
+  int* status;
   Beams bm;
 };
 
@@ -168,15 +127,39 @@ __global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowAr
   const size_t HW = size_t(A.H) * N;
   cpx<T>* line_lds = lds + li * LS;
 
+  // per-pixel operator data of this row, solved in fp64 once and reused for
+  // every wavelength and ray of the work-group
   cpx<T> eg[(OP == OP_ROP) ? PPX : 1][EIG_COMPS];
   if constexpr (OP == OP_ROP) {
     const int beam = (rg * NR) / A.bm.rays_per_beam;
-    const cpx<T>* src = A.eig + size_t(beam) * EIG_COMPS * HW + size_t(y) * N;
+    const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
+    double s0, c0;
+    sincos(phi0, &s0, &c0);
 #pragma unroll
     for (int k = 0; k < PPX; ++k) {
       const int x = tid + k * NTW;
-#pragma unroll
-      for (int m = 0; m < EIG_COMPS; ++m) eg[k][m] = (x < N) ? src[m * HW + x] : mk<T>(T(0), T(0));
+      if (x < N) {
+        const size_t o = size_t(y) * N + x;
+        const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+        const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
+        const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
+                                 A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
+        RopData r;
+        rop_lean(beta, c0, s0, phi0, e0, e1, e2, A.ang[A.comp_stride + o],
+                 A.ang[2 * A.comp_stride + o], r);
+        if (r.kind == KIND_BIAXIAL && A.status) atomicOr(A.status, 1);
+        for (int i = 0; i < 4; ++i) eg[k][i] = cast<T>(r.a[i]);
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) {
+            eg[k][4 + 2 * i + j] = cast<T>(r.P0[i][j]);
+            eg[k][8 + 2 * i + j] = cast<T>(r.P1[i][j]);
+          }
+        T ok = (A.method4 ? r.ok4 : r.ok2) ? T(1) : T(0);
+        if (r.kind == KIND_BIAXIAL) ok = T(__builtin_nan(""));
+        eg[k][12] = mk<T>(ok, T(0));
+      } else {
+        for (int m = 0; m < EIG_COMPS; ++m) eg[k][m] = mk<T>(T(0), T(0));
+      }
     }
   }
   __syncthreads();
@@ -512,57 +495,85 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
   cpx<T>* o = A.dst + size_t(q) * nout * HW + k;
   cd v[4];
   for (int c = 0; c < nin; ++c) v[c] = cast<double>(s[c * HW]);
-  cd M[4][4];
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) M[i][j] = mk<double>(0.);
-  if (pass) {
-    if (A.kop == KOP_HEAD4) {
-      cd C[4][4], D[4][4];
-      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C);
-      first_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D);
-      matmul4(C, D, M);
-    } else if (A.kop == KOP_TAIL4) {
-      cd C[4][4], D[4][4];
-      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C);
-      second_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D);
-      matmul4(D, C, M);
-    } else if (A.kop == KOP_BOUND4) {
-      cd C0[4][4], D0[4][4], C1[4][4], D1[4][4], X[4][4], Y[4][4];
-      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C0);
-      second_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D0);
-      correction(bb, pb, A.e1, k0 * A.d1 * 0.5, C1);
-      first_half(beta, phi, A.e1, k0 * A.d1 * 0.5, D1);
-      matmul4(D0, C0, X);  // second corrected of layer 0
-      matmul4(C1, D1, Y);  // first corrected of layer 1
-      matmul4(Y, X, M);
-    } else {
-      // rows (Ex, Ey) of the forward projection in the input medium
-      // (dtmm/diffract.py:242-251) times the first 2x2 half step.
-      Modes m;
-      iso_at(beta, phi, A.ein, m);
-      cd Fi[4][4];
-      inv4(m.F, Fi);
-      cd P[4][4];
-      for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) P[i][j] = m.F[i][0] * Fi[0][j] + m.F[i][2] * Fi[2][j];
+  // in-plane direction of the mode: (cos, sin) of atan2(fy, fx), (1, 0) at k = 0
+  const double fr = sqrt(fx * fx + fy * fy);
+  const double c = fr > 0. ? fx / fr : 1., sn = fr > 0. ? fy / fr : 0.;
+  cd r[4] = {mk<double>(0.), mk<double>(0.), mk<double>(0.), mk<double>(0.)};
+  if (pass && A.kop == KOP_HEAD4) {
+    // C . E P(kd) F^-1 in closed form for an isotropic medium: field ->
+    // frame of the mode (g = R^T f) -> mode amplitudes m = F^-1 g -> half-step
+    // phases -> (E+x, E-x, E+y, E-y) -> mean-beam correction.
+    const cd a = csqrt(A.e0 - mk<double>(beta * beta));
+    if (!is_zero(a)) {
+      const double kd = k0 * A.d0 * 0.5;
+      const cd igp = cdiv(a, A.e0), igs = cdiv(mk<double>(-1.), a);
+      const cd g0 = v[0] * c + v[2] * sn, g1 = v[1] * c - v[3] * sn;
+      const cd g2 = v[2] * c - v[0] * sn, g3 = v[1] * sn + v[3] * c;
+      const cd p = cexpi(a * kd), pi = cexpi(a * -kd);
+      const cd m0 = (g0 + g1 * igp) * p, m1 = (g0 - g1 * igp) * pi;
+      const cd m2 = (g2 + g3 * igs) * p, m3 = (g2 - g3 * igs) * pi;
+      const cd ac = csqrt(A.e0 - mk<double>(bb * bb));
+      const cd cf = cexpi(ac * -kd), cb = cexpi(ac * kd);
+      r[0] = (m0 * c - m2 * sn) * cf * 0.5;
+      r[1] = (m1 * c - m3 * sn) * cb * 0.5;
+      r[2] = (m0 * sn + m2 * c) * cf * 0.5;
+      r[3] = (m1 * sn + m3 * c) * cb * 0.5;
+    }
+  } else if (pass && A.kop == KOP_TAIL4) {
+    // F P(kd) E^-1 . C in closed form (inverse path of KOP_HEAD4).
+    const cd a = csqrt(A.e0 - mk<double>(beta * beta));
+    if (!is_zero(a)) {
+      const double kd = k0 * A.d0 * 0.5;
+      const cd ac = csqrt(A.e0 - mk<double>(bb * bb));
+      const cd cf = cexpi(ac * -kd), cb = cexpi(ac * kd);
+      const cd epx = v[0] * cf, enx = v[1] * cb, epy = v[2] * cf, eny = v[3] * cb;
+      const cd p = cexpi(a * kd), pi = cexpi(a * -kd);
+      const cd m0 = (epx * c + epy * sn) * p * 2., m2 = (epy * c - epx * sn) * p * 2.;
+      const cd m1 = (enx * c + eny * sn) * pi * 2., m3 = (eny * c - enx * sn) * pi * 2.;
+      const cd gp = cdiv(A.e0, a), gs = -a;
+      const cd g0 = (m0 + m1) * 0.5, g1 = gp * (m0 - m1) * 0.5;
+      const cd g2 = (m2 + m3) * 0.5, g3 = gs * (m2 - m3) * 0.5;
+      r[0] = g0 * c - g2 * sn;
+      r[1] = g1 * c + g3 * sn;
+      r[2] = g0 * sn + g2 * c;
+      r[3] = g3 * c - g1 * sn;
+    }
+  } else if (pass && A.kop == KOP_PROJ2) {
+    // rows (Ex, Ey) of the forward projection F diag(1,0,1,0) F^-1 in the
+    // input medium (dtmm/diffract.py:242-251) times the first 2x2 half step
+    // exp(i k0 d/2 (a(k) - a_c)) of the first layer.
+    const cd ain = csqrt(A.ein - mk<double>(beta * beta));
+    if (!is_zero(ain)) {
+      const cd igp = cdiv(ain, A.ein), igs = cdiv(mk<double>(-1.), ain);
+      const cd g0 = v[0] * c + v[2] * sn, g1 = v[1] * c - v[3] * sn;
+      const cd g2 = v[2] * c - v[0] * sn, g3 = v[1] * sn + v[3] * c;
+      const cd h0 = (g0 + g1 * igp) * 0.5, h2 = (g2 + g3 * igs) * 0.5;
       cd ph = mk<double>(0.);
       if (A.d0 != 0.) {
         cd ak = csqrt(A.e0 - mk<double>(beta * beta));
         cd ac = csqrt(A.e0 - mk<double>(bb * bb));
         ph = (ak - ac) * (0.5 * A.d0 * k0);
       }
-      cd K = cexpi(ph);
-      for (int j = 0; j < 4; ++j) {
-        M[0][j] = K * P[0][j];
-        M[1][j] = K * P[2][j];
-      }
+      const cd K = cexpi(ph);
+      r[0] = K * (h0 * c - h2 * sn);
+      r[1] = K * (h0 * sn + h2 * c);
+    }
+  } else if (pass && A.kop == KOP_BOUND4) {
+    cd C0[4][4], D0[4][4], C1[4][4], D1[4][4], X[4][4], Y[4][4], M[4][4];
+    correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C0);
+    second_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D0);
+    correction(bb, pb, A.e1, k0 * A.d1 * 0.5, C1);
+    first_half(beta, phi, A.e1, k0 * A.d1 * 0.5, D1);
+    matmul4(D0, C0, X);  // second corrected of layer 0
+    matmul4(C1, D1, Y);  // first corrected of layer 1
+    matmul4(Y, X, M);
+    for (int i = 0; i < 4; ++i) {
+      cd t = mk<double>(0.);
+      for (int j = 0; j < 4; ++j) t = t + M[i][j] * v[j];
+      r[i] = t;
     }
   }
-  for (int i = 0; i < nout; ++i) {
-    cd r = mk<double>(0.);
-    for (int j = 0; j < nin; ++j) r = r + M[i][j] * v[j];
-    o[i * HW] = cast<T>(r * A.scale);
-  }
+  for (int i = 0; i < nout; ++i) o[i * HW] = cast<T>(r[i] * A.scale);
 }
 
 // (Ex, Ey) -> (Ex, Hy, Ey, Hx) with the mean-beam H/E relation of the output

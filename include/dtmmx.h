@@ -118,8 +118,9 @@ int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int pr
  * profiling hooks): between begin and end every per-layer kernel launch is
  * bracketed by HIP events on its stream.  end() waits for the last event and
  * returns, per kernel class, the summed milliseconds and the launch count
- * (class 0: row layer kernel, class 1: column k-space kernel).  Not
- * thread-safe; for benchmarking. */
+ * (class 0: row layer kernel including its per-pixel fp64 solve, class 1:
+ * column k-space kernel, class 3: dense k-space head/tail kernel; class 2
+ * is unused).  Not thread-safe; for benchmarking. */
 int dtmmx_timing_begin(int capacity);
 int dtmmx_timing_end(double* ms, int* count, int nclass);
 
@@ -130,6 +131,17 @@ int dtmmx_timing_end(double* ms, int* count, int nclass);
  * if any voxel is biaxial. */
 int dtmmx_host_alphaf(int n, const double* beta, const double* phi, const double* eps_re,
                       const double* eps_im, const double* angles, double* alpha, double* fmat);
+
+/* Host evaluation of the per-pixel data of the real-space layer operator
+ * E diag(exp(i kd alpha)) E^-1 (dtmm/propagate_4x4.py:88-97,
+ * dtmm/propagate_2x2.py:33-39) written as p2 I + (p0 - p2) P0 on
+ * (E+x, E+y) and p3 I + (p1 - p3) P1 on (E-x, E-y): alpha[n][4] and
+ * proj[n][2][2][2] (P0, P1) as interleaved complex128, ok[n][2] = E
+ * invertible for the 4x4 / 2x2 method.  The same routine runs inside the
+ * row kernel; this entry point exists for checks without a GPU. */
+int dtmmx_host_rop(int n, const double* beta, const double* phi, const double* eps_re,
+                   const double* eps_im, const double* angles, double* alpha, double* proj,
+                   int* ok);
 
 #ifdef __cplusplus
 }

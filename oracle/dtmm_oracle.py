@@ -25,7 +25,7 @@ import numpy as np
 
 __all__ = [
     "BETAMAX", "N_COVER", "k0", "betaphi", "betaxy", "alphaf", "alphaffi",
-    "inv", "dotmm", "dotmf", "dotmdmf", "dotmv", "phase_mat", "E_mat", "E2H_mat",
+    "inv", "dotmm", "dotmf", "dotmdm", "dotmdmf", "dotmv", "phase_mat", "E_mat", "E2H_mat",
     "normalize_f", "field2intensity", "fft2", "ifft2", "refind2eps",
     "uniaxial_order", "validate_optical_data", "create_layers",
     "projection_matrix", "field_diffraction_matrix", "diffract",

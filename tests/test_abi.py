@@ -1,0 +1,124 @@
+"""The C ABI library: loads, exports every symbol include/dtmmx.h declares,
+rejects bad plans before touching the GPU, and its host-side per-pixel
+solvers agree with the complex128 oracle (CPU only, no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as o
+from dtmm_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "dtmmx.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\*?\s+\*?(dtmmx_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = _declared()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.EXPORTS)
+    assert lib.dtmmx_abi_version() == 1
+
+
+def _plan_rc(**kw):
+    lib = _lib.load()
+    ks = np.array([1.0])
+    b = np.array([0.0])
+    d = _lib.PlanDesc()
+    d.height, d.width = kw.get("shape", (32, 32))
+    d.n_wavelengths = kw.get("nlam", 1)
+    d.n_rays = kw.get("n_rays", 1)
+    d.rays_per_beam = kw.get("rpb", 1)
+    d.method = kw.get("method", 4)
+    d.precision = kw.get("precision", 0)
+    d.betamax = 0.8
+    d.k0 = _lib.dptr(ks)
+    d.beta = _lib.dptr(b)
+    d.phi = _lib.dptr(b)
+    h = ctypes.c_void_p()
+    return lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)), lib.dtmmx_last_error()
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(shape=(0, 32)), _lib.DTMMX_E_ARG),
+    (dict(shape=(7, 32)), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(shape=(32, 1000)), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(nlam=0), _lib.DTMMX_E_ARG),
+    (dict(nlam=65), _lib.DTMMX_E_ARG),
+    (dict(n_rays=3, rpb=2), _lib.DTMMX_E_ARG),
+    (dict(method=3), _lib.DTMMX_E_ARG),
+    (dict(precision=2), _lib.DTMMX_E_ARG),
+])
+def test_plan_create_rejects_bad_descriptions(kw, code):
+    rc, msg = _plan_rc(**kw)
+    assert rc == code
+    assert msg
+
+
+def _voxels(n, seed, biaxial=False):
+    rng = np.random.default_rng(seed)
+    beta = rng.uniform(0, 0.9, n)
+    beta[: n // 4] = 0.
+    phi = rng.uniform(-3, 3, n)
+    eo = rng.uniform(1.5, 2.5, n)
+    ee = rng.uniform(1.5, 3.0, n)
+    eps = np.stack([eo, eo, ee], -1)
+    eps[3 * n // 4:] = eo[3 * n // 4:, None]  # isotropic voxels
+    ang = np.stack([np.zeros(n), rng.uniform(0, np.pi, n), rng.uniform(-np.pi, np.pi, n)], -1)
+    return beta, phi, np.ascontiguousarray(eps), np.ascontiguousarray(ang)
+
+
+def test_host_alphaf_matches_oracle():
+    lib = _lib.load()
+    beta, phi, eps, ang = _voxels(1000, 1)
+    n = len(beta)
+    al = np.zeros((n, 4, 2))
+    fm = np.zeros((n, 4, 4, 2))
+    _lib.check(lib.dtmmx_host_alphaf(n, _lib.dptr(beta), _lib.dptr(phi), _lib.dptr(eps), None,
+                                     _lib.dptr(ang), _lib.dptr(al), _lib.dptr(fm)))
+    ao, fo = o.alphaf(beta, phi, eps, ang)
+    np.testing.assert_allclose(al[..., 0] + 1j * al[..., 1], ao, atol=1e-13)
+    np.testing.assert_allclose(fm[..., 0] + 1j * fm[..., 1], fo, atol=1e-13)
+
+
+@pytest.mark.parametrize("absorbing", [False, True])
+def test_host_layer_operator_matches_oracle(absorbing):
+    """E diag(p) E^-1 from the engine's lean solve equals the oracle's
+    (dtmm/propagate_4x4.py:88-97 with E = E_mat(F) and its 4x4 inverse)."""
+    lib = _lib.load()
+    beta, phi, eps, ang = _voxels(600, 2)
+    n = len(beta)
+    eps_im = np.full(eps.shape, 0.02) if absorbing else None
+    al = np.zeros((n, 4, 2))
+    pr = np.zeros((n, 2, 2, 2, 2))
+    ok = np.zeros((n, 2), np.int32)
+    _lib.check(lib.dtmmx_host_rop(n, _lib.dptr(beta), _lib.dptr(phi), _lib.dptr(eps),
+                                  _lib.dptr(eps_im), _lib.dptr(ang), _lib.dptr(al), _lib.dptr(pr),
+                                  ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    a = al[..., 0] + 1j * al[..., 1]
+    P = pr[..., 0] + 1j * pr[..., 1]
+    epsc = eps + (1j * eps_im if absorbing else 0.)
+    ao, fo = o.alphaf(beta, phi, epsc, ang)
+    np.testing.assert_allclose(a, ao, atol=1e-13)
+    assert np.all(ok == 1)
+    kd = 1.7
+    p = np.exp(1j * kd * a)
+    # engine operator in the (E+x, E-x, E+y, E-y) basis
+    M = np.zeros((n, 4, 4), complex)
+    I2 = np.eye(2)
+    Mp = p[:, 2, None, None] * I2 + (p[:, 0] - p[:, 2])[:, None, None] * P[:, 0]
+    Mn = p[:, 3, None, None] * I2 + (p[:, 1] - p[:, 3])[:, None, None] * P[:, 1]
+    M[:, 0::2, 0::2] = Mp
+    M[:, 1::2, 1::2] = Mn
+    e = o.E_mat(fo)
+    ref = o.dotmdm(e, np.exp(1j * kd * ao), o.inv(e))
+    np.testing.assert_allclose(M, ref, atol=1e-11)
