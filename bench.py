@@ -89,7 +89,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ref", action="store_true", help="skip the c128 comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="time the steps without per-kernel HIP events")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (reference for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * 1048576 // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied %d MiB 5 times" % args.copy_calibration)
+        return
 
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has one GPU and the "
@@ -133,7 +149,12 @@ def main():
         plan.execute(stack, fin, fout, ws, stream=stream)
     torch.cuda.synchronize()
 
-    _lib.check(lib.dtmmx_timing_begin(8 * args.steps * (L + 4)))
+    # every 16th launch of each per-layer kernel is timed with HIP events on
+    # its stream inside the timed region (timing every launch would add about
+    # 8% to the pass through the extra stream commands)
+    timing = not args.no_kernel_timing
+    if timing:
+        _lib.check(lib.dtmmx_timing_begin(8 * args.steps * (L + 4), 16))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -142,7 +163,13 @@ def main():
     t1 = time.perf_counter()
     ms = (ctypes.c_double * 4)()
     cnt = (ctypes.c_int * 4)()
-    _lib.check(lib.dtmmx_timing_end(ms, cnt, 4))
+    if timing:
+        _lib.check(lib.dtmmx_timing_end(ms, cnt, 4))
+    else:
+        # per-kernel times from one further instrumented pass
+        _lib.check(lib.dtmmx_timing_begin(8 * (L + 4), 1))
+        plan.execute(stack, fin, fout, ws, stream=stream)
+        _lib.check(lib.dtmmx_timing_end(ms, cnt, 4))
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     steps = H * W * L * planes
     value = steps / (ms_per_step / 1e3)
@@ -167,12 +194,10 @@ def main():
     achieved = kern[dom]["bytes"] / (avg_ms / 1e3) / 1e9
     log("kernels: " + ", ".join("%s %d x %.3f ms" % (k, v["n"], v["ms"] / max(v["n"], 1))
                                  for k, v in kern.items())
-        + ", eigen %d x %.3f ms, kspace %d x %.3f ms" % (cnt[2], ms[2] / max(cnt[2], 1),
-                                                        cnt[3], ms[3] / max(cnt[3], 1)))
-    other_ms = ms_per_step - (ms[0] + ms[1] + ms[2] + ms[3]) / args.steps
-    log("per pass: row %.2f ms, col %.2f ms, eigen %.2f ms, kspace %.2f ms, rest %.2f ms"
-        % (ms[0] / args.steps, ms[1] / args.steps, ms[2] / args.steps, ms[3] / args.steps,
-           other_ms))
+        + ", kspace %d x %.3f ms" % (cnt[3], ms[3] / max(cnt[3], 1)))
+    per_pass = [ms[i] / max(cnt[i], 1) * n for i, n in enumerate((L, max(L - 1, 0), 0, 2))]
+    log("per pass (from launch averages): row %.2f ms, col %.2f ms, kspace %.2f ms, rest %.2f ms"
+        % (per_pass[0], per_pass[1], per_pass[3], ms_per_step - sum(per_pass)))
 
     extra = {}
     if not args.no_ref and args.precision == "c64":

@@ -7,7 +7,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdtmmx.so")
+# DTMMX_LIB selects another build of the same library (e.g. a variant built
+# with different compile-time options).
+LIB_PATH = os.environ.get("DTMMX_LIB") or os.path.join(HERE, "libdtmmx.so")
 
 DTMMX_OK = 0
 DTMMX_E_ARG = -1
@@ -109,7 +111,7 @@ def load():
     lib.dtmmx_host_alphaf.restype = ctypes.c_int
     lib.dtmmx_host_rop.argtypes = [ctypes.c_int] + [_dp] * 7 + [ctypes.POINTER(ctypes.c_int)]
     lib.dtmmx_host_rop.restype = ctypes.c_int
-    lib.dtmmx_timing_begin.argtypes = [ctypes.c_int]
+    lib.dtmmx_timing_begin.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.dtmmx_timing_begin.restype = ctypes.c_int
     lib.dtmmx_timing_end.argtypes = [_dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     lib.dtmmx_timing_end.restype = ctypes.c_int

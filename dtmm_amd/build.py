@@ -27,17 +27,19 @@ FLAGS = [
 ]
 
 
-def build(verbose=True):
-    """Compile the library (always recompiles)."""
-    cmd = [HIPCC] + FLAGS + [SRC, "-o", OUT + ".tmp"]
+def build(verbose=True, defines=(), out=None):
+    """Compile the library (always recompiles).  `defines` adds -D flags and
+    `out` another output path (for build variants)."""
+    out = out or OUT
+    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + [SRC, "-o", out + ".tmp"]
     t = time.time()
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(OUT + ".tmp", OUT)
+    os.replace(out + ".tmp", out)
     if verbose:
-        print("built %s in %.1f s" % (OUT, time.time() - t), flush=True)
-    return OUT
+        print("built %s in %.1f s" % (out, time.time() - t), flush=True)
+    return out
 
 
 if __name__ == "__main__":

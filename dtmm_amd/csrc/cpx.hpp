@@ -80,6 +80,45 @@ __device__ __forceinline__ cpx<double> cexpi(cpx<double> ph) {
   return mk<double>(c * m, s * m);
 }
 
+// sin and cos of a float phase |x| < 5e4 rad: Cody-Waite reduction by pi/2
+// (pi/2 split into three floats whose first two have short mantissas, so
+// k*C1 and k*C2 are exact for |k| < 2^15), then the Taylor series of sin to
+// r^9 and of cos to r^10 on |r| <= pi/4 (truncation below 2e-9, under the
+// float rounding of the result), and the quadrant of k.
+__host__ __device__ __forceinline__ void sincos_cw(float x, float* sp, float* cp) {
+  const float C1 = 1.5703125f, C2 = 4.837512969970703125e-4f, C3 = 7.549789954891882e-8f;
+  const float k = rintf(x * 0.636619772367581343f);
+  float r = fmaf(-k, C1, x);
+  r = fmaf(-k, C2, r);
+  r = fmaf(-k, C3, r);
+  const float r2 = r * r;
+  float sn = fmaf(r2, 2.7557319224e-6f, -1.9841269841e-4f);  // 1/9!, -1/7!
+  sn = fmaf(r2, sn, 8.3333333333e-3f);                        // 1/5!
+  sn = fmaf(r2, sn, -1.6666666667e-1f);                       // -1/3!
+  sn = fmaf(r2 * r, sn, r);
+  float cs = fmaf(r2, -2.7557319224e-7f, 2.4801587302e-5f);   // -1/10!, 1/8!
+  cs = fmaf(r2, cs, -1.3888888889e-3f);                       // -1/6!
+  cs = fmaf(r2, cs, 4.1666666667e-2f);                        // 1/4!
+  cs = fmaf(r2, cs, -0.5f);
+  cs = fmaf(r2, cs, 1.0f);
+  const int q = int(k) & 3;
+  float s = (q & 1) ? cs : sn;
+  float c = (q & 1) ? sn : cs;
+  if (q & 2) s = -s;
+  if ((q + 1) & 2) c = -c;
+  *sp = s;
+  *cp = c;
+}
+
+// Phase factors of the per-layer kernels (exp(i ph) for a complex phase).
+__device__ __forceinline__ cpx<float> cexpi_layer(cpx<float> ph) {
+  float s, c;
+  sincos_cw(ph.x, &s, &c);
+  float m = (ph.y == 0.f) ? 1.f : expf(-ph.y);
+  return mk<float>(c * m, s * m);
+}
+__device__ __forceinline__ cpx<double> cexpi_layer(cpx<double> ph) { return cexpi(ph); }
+
 inline cpx<double> cexpi_host(cpx<double> ph) {
   double m = (ph.y == 0.) ? 1. : std::exp(-ph.y);
   return mk<double>(std::cos(ph.x) * m, std::sin(ph.x) * m);

@@ -161,20 +161,26 @@ namespace dtmmx {
 // 3: dense k-space head/tail kernel).
 struct KTimer {
   bool on = false;
-  int cap = 0, used = 0;
+  bool armed = false;  // the current launch is being timed
+  int cap = 0, used = 0, stride = 1;
+  int seen[4] = {0, 0, 0, 0};
   std::vector<hipEvent_t> ev;
   std::vector<int> cls;
 };
 static KTimer g_timer;
 
 static void timer_start(int cls, hipStream_t s) {
+  g_timer.armed = false;
   if (!g_timer.on || g_timer.used >= g_timer.cap) return;
+  if (g_timer.seen[cls & 3]++ % g_timer.stride != 0) return;
+  g_timer.armed = true;
   g_timer.cls[g_timer.used] = cls;
   (void)hipEventRecord(g_timer.ev[2 * g_timer.used], s);
 }
 
 static void timer_stop(hipStream_t s) {
-  if (!g_timer.on || g_timer.used >= g_timer.cap) return;
+  if (!g_timer.armed) return;
+  g_timer.armed = false;
   (void)hipEventRecord(g_timer.ev[2 * g_timer.used + 1], s);
   ++g_timer.used;
 }
@@ -561,8 +567,11 @@ int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, v
   return transfer_impl<double>(P, S, fin, fout, ws, status, s);
 }
 
-int dtmmx_timing_begin(int capacity) {
+int dtmmx_timing_begin(int capacity, int stride) {
   if (capacity <= 0) return fail(DTMMX_E_ARG, "capacity must be positive");
+  if (stride <= 0) return fail(DTMMX_E_ARG, "stride must be positive");
+  g_timer.stride = stride;
+  for (int& c : g_timer.seen) c = 0;
   for (auto e : g_timer.ev) (void)hipEventDestroy(e);
   g_timer.ev.assign(2 * size_t(capacity), nullptr);
   g_timer.cls.assign(capacity, -1);

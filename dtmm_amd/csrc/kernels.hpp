@@ -204,11 +204,11 @@ __global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowAr
           const cpx<T>* e = eg[k];
           const int ax = lds_at<1, 1>(x);
           const T ok = e[12].x;
-          cpx<T> p0 = cexpi(e[0] * kd), p2 = cexpi(e[2] * kd);
+          cpx<T> p0 = cexpi_layer(e[0] * kd), p2 = cexpi_layer(e[2] * kd);
           cpx<T> dp = (p0 - p2) * ok;
           p2 = p2 * ok;
           if constexpr (NC == 4) {
-            cpx<T> p1 = cexpi(e[1] * kd), p3 = cexpi(e[3] * kd);
+            cpx<T> p1 = cexpi_layer(e[1] * kd), p3 = cexpi_layer(e[3] * kd);
             cpx<T> dn = (p1 - p3) * ok;
             p3 = p3 * ok;
 #pragma unroll
@@ -291,7 +291,7 @@ __device__ __forceinline__ cpx<T> diag_factor(const ColArgs<T>& A, int lam, int 
     cpx<T> e = cast<T>(A.eb);
     ph = ph + (csqrt(e - mk<T>(b2)) - csqrt(e - mk<T>(bc * bc))) * T(0.5 * A.db);
   }
-  return cexpi(ph * k0) * T(A.scale);
+  return cexpi_layer(ph * k0) * T(A.scale);
 }
 
 // Lanes per column: one radix-8 butterfly per lane, 16..128 lanes.

@@ -115,13 +115,15 @@ int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int pr
                void* stream);
 
 /* Optional kernel timing (no counterpart in the reference, which has no
- * profiling hooks): between begin and end every per-layer kernel launch is
- * bracketed by HIP events on its stream.  end() waits for the last event and
- * returns, per kernel class, the summed milliseconds and the launch count
+ * profiling hooks): between begin and end every stride-th launch of each
+ * per-layer kernel class is bracketed by HIP events on its stream (a stride
+ * above 1 keeps the event overhead out of the timed work).  end() waits for
+ * the last event and returns, per kernel class, the summed milliseconds and
+ * the count of the timed launches
  * (class 0: row layer kernel including its per-pixel fp64 solve, class 1:
  * column k-space kernel, class 3: dense k-space head/tail kernel; class 2
  * is unused).  Not thread-safe; for benchmarking. */
-int dtmmx_timing_begin(int capacity);
+int dtmmx_timing_begin(int capacity, int stride);
 int dtmmx_timing_end(double* ms, int* count, int nclass);
 
 /* Host evaluation of the per-pixel eigen-solution used on the device
