@@ -28,6 +28,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, each
+# scaled by the counter's reading for a 512 MiB device copy (FETCH x 2.000,
+# WRITE x 1.000); profiles/transfer4x4_c64_512x512_hbm_traffic.md.  Keyed by
+# (kernel, H, W, planes, method, precision); other workloads report null.
+MEASURED_TRAFFIC = {
+    ("row_layer", 512, 512, 18, "4x4", "c64"): 312.8e6,
+    ("col_kspace", 512, 512, 18, "4x4", "c64"): 314.9e6,
+}
+
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
@@ -243,7 +252,9 @@ def main():
                    "method": args.method, "precision": args.precision},
         "hbm_frac_per_step": (steps * 2 * nc * cb / (ms_per_step / 1e3) / 1e9) / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((dom, H, W, planes, args.method,
+                                                      args.precision)),
                      "avg_launch_ms": avg_ms, "bytes_per_launch": kern[dom]["bytes"]},
         "cpu_baseline": cpu,
     }
