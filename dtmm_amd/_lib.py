@@ -20,6 +20,7 @@ DTMMX_C64 = 0
 DTMMX_C128 = 1
 DTMMX_METHOD_2X2 = 2
 DTMMX_METHOD_4X4 = 4
+ABI_VERSION = 2
 MAX_WAVELENGTHS = 64
 MAX_BEAMS = 64
 
@@ -36,6 +37,7 @@ EXPORTS = (
     "dtmmx_timing_begin",
     "dtmmx_timing_end",
     "dtmmx_host_rop",
+    "dtmmx_host_windows",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -54,6 +56,7 @@ class PlanDesc(ctypes.Structure):
         ("k0", _dp),
         ("beta", _dp),
         ("phi", _dp),
+        ("diffraction", ctypes.c_int),
     ]
 
 
@@ -68,6 +71,7 @@ class Stack(ctypes.Structure):
         ("angles", ctypes.c_void_p),
         ("n_in", ctypes.c_double),
         ("n_out", ctypes.c_double),
+        ("n_steps", ctypes.POINTER(ctypes.c_int)),
     ]
 
 
@@ -111,11 +115,15 @@ def load():
     lib.dtmmx_host_alphaf.restype = ctypes.c_int
     lib.dtmmx_host_rop.argtypes = [ctypes.c_int] + [_dp] * 7 + [ctypes.POINTER(ctypes.c_int)]
     lib.dtmmx_host_rop.restype = ctypes.c_int
+    _ip = ctypes.POINTER(ctypes.c_int)
+    lib.dtmmx_host_windows.argtypes = [ctypes.POINTER(PlanDesc), ctypes.c_int, _ip, _ip, _ip,
+                                       _dp, _dp]
+    lib.dtmmx_host_windows.restype = ctypes.c_int
     lib.dtmmx_timing_begin.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.dtmmx_timing_begin.restype = ctypes.c_int
     lib.dtmmx_timing_end.argtypes = [_dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     lib.dtmmx_timing_end.restype = ctypes.c_int
-    if lib.dtmmx_abi_version() != 1:
+    if lib.dtmmx_abi_version() != ABI_VERSION:
         raise RuntimeError("dtmmx ABI version mismatch")
     _lib = lib
     return lib
@@ -134,3 +142,10 @@ def dptr(a):
     if a is None:
         return None
     return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    """ctypes int* of a C-contiguous int32 numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))

@@ -111,24 +111,13 @@ constexpr int col_tx() {
 
 template <typename T, int OP>
 static int launch_col(int n, const ColArgs<T>& a, int nlines_per_lam, hipStream_t s) {
-  static const bool half_tx = [] {
-    const char* e = std::getenv("DTMMX_COL_HALF_TX");
-    return e && std::atoi(e) != 0;
-  }();
   switch (n) {
 #define X(N)                                                                                \
   case N: {                                                                                 \
     constexpr int TX = col_tx<T, N>();                                                      \
-    constexpr int TX2 = TX > 1 ? TX / 2 : 1;                                                \
-    if (half_tx) {                                                                          \
-      const int nwork = ((a.W + TX2 - 1) / TX2) * a.b.nlam_b * (nlines_per_lam / a.ncl);    \
-      hipLaunchKernelGGL((col_kernel<T, N, TX2, OP>), dim3(nwork), dim3(TX2 * col_lanes<N>()), \
-                         0, s, a);                                                          \
-    } else {                                                                                \
-      const int nwork = ((a.W + TX - 1) / TX) * a.b.nlam_b * (nlines_per_lam / a.ncl);      \
-      hipLaunchKernelGGL((col_kernel<T, N, TX, OP>), dim3(nwork), dim3(TX * col_lanes<N>()), 0, \
-                         s, a);                                                             \
-    }                                                                                       \
+    const int nwork = ((a.W + TX - 1) / TX) * a.b.nlam_b * (nlines_per_lam / a.ncl);        \
+    hipLaunchKernelGGL((col_kernel<T, N, TX, OP>), dim3(nwork), dim3(TX * col_lanes<N>()), 0, \
+                       s, a);                                                               \
     break;                                                                                  \
   }
     DTMMX_SIZES(X)
@@ -152,6 +141,15 @@ struct dtmmx_plan {
   void* tw_w = nullptr;  // twiddles of width (precision type)
   void* tw_h = nullptr;
   int* amax = nullptr;   // [nlam][H]
+  // diffraction > 1: non-zero tent windows (i, j) of wn x wn in the
+  // reference's order i * wn + j, their mean beams [window][beam], the
+  // separable tent factors on the device and the mask of the window grid
+  int wn = 0;
+  std::vector<int> win_i, win_j;
+  std::vector<double> win_beta, win_phi;
+  double* wcx = nullptr;  // [beam][nlam][wn][W]
+  double* wcy = nullptr;  // [beam][nlam][wn][H]
+  int* wmask = nullptr;   // [nlam][H]: min(amax, largest |kx| the windows pass)
 };
 
 namespace dtmmx {
@@ -187,7 +185,7 @@ static void timer_stop(hipStream_t s) {
 
 template <typename T>
 static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op, hipStream_t s,
-                    const RowArgs<T>* proto = nullptr) {
+                    const RowArgs<T>* proto = nullptr, const Beams* bm = nullptr) {
   RowArgs<T> a{};
   if (proto) a = *proto;
   a.field = field;
@@ -195,7 +193,7 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   a.W = P->d.width;
   a.b = b;
   a.tw = (const cpx<T>*)P->tw_w;
-  a.bm = P->bm;
+  a.bm = bm ? *bm : P->bm;
   int nr = 1;
   if (sizeof(T) == 4 && a.W <= 512 && b.nrays % 2 == 0 &&
       (op != OP_ROP || P->bm.rays_per_beam % 2 == 0))
@@ -221,7 +219,7 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
 
 template <typename T>
 static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op, hipStream_t s,
-                    const ColArgs<T>* proto = nullptr) {
+                    const ColArgs<T>* proto = nullptr, const Beams* bm = nullptr) {
   ColArgs<T> a{};
   if (proto) a = *proto;
   a.field = field;
@@ -238,7 +236,7 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   }
   a.tw = (const cpx<T>*)P->tw_h;
   a.amax = P->amax;
-  a.bm = P->bm;
+  a.bm = bm ? *bm : P->bm;
   const int nlines = b.nrays * nc;
   const int h = a.H;
   if (op == OP_FWD) return launch_col<T, OP_FWD>(h, a, nlines, s);
@@ -246,6 +244,17 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   if (op == OP_DIAG) {
     timer_start(1, s);
     int r = launch_col<T, OP_DIAG>(h, a, nlines, s);
+    timer_stop(s);
+    return r;
+  }
+  if (op == OP_WIN) {
+    if (!P->wcx || !a.out) return fail(DTMMX_E_ARG, "window pass without windows");
+    a.amax = P->wmask;
+    a.wcx = P->wcx;
+    a.wcy = P->wcy;
+    a.wn = P->wn;
+    timer_start(1, s);
+    int r = launch_col<T, OP_WIN>(h, a, nlines, s);
     timer_stop(s);
     return r;
   }
@@ -271,18 +280,38 @@ static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, 
   return DTMMX_OK;
 }
 
-// Row-kernel arguments of the layer operator of layer l.
+// Row-kernel arguments of the layer operator of layer l, sub-step thickness d.
 template <typename T>
-static RowArgs<T> layer_args(const dtmmx_stack* S, size_t HW, int l, int method4, int* status) {
+static RowArgs<T> layer_args(const dtmmx_stack* S, size_t HW, int l, double d, int method4,
+                             int* status) {
   RowArgs<T> a{};
   a.eps_re = S->eps_re + size_t(l) * HW;
   a.eps_im = S->eps_im ? S->eps_im + size_t(l) * HW : nullptr;
   a.ang = S->angles + size_t(l) * HW;
   a.comp_stride = size_t(S->n_layers) * HW;
-  a.d = S->thickness[l];
+  a.d = d;
   a.method4 = method4;
   a.status = status;
   return a;
+}
+
+// One propagation step: layer l of the stack with sub-step thickness d and
+// isotropic effective eps e (transfer.py:933-934 splits layer l into
+// n_steps[l] steps of thickness[l] / n_steps[l]).
+struct Step {
+  int l;
+  double d;
+  cd e;
+};
+
+static std::vector<Step> expand_steps(const dtmmx_stack* S) {
+  std::vector<Step> v;
+  for (int l = 0; l < S->n_layers; ++l) {
+    const int ns = S->n_steps ? S->n_steps[l] : 1;
+    const cd e = mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
+    for (int k = 0; k < ns; ++k) v.push_back(Step{l, S->thickness[l] / double(ns), e});
+  }
+  return v;
 }
 
 #define TRY(x)                     \
@@ -304,6 +333,115 @@ static int lam_batch(const dtmmx_plan* P, size_t bytes_per_lam) {
   return nb < nlam ? nb : nlam;
 }
 
+// 4x4 method with diffraction > 1 (dtmm/propagate_4x4.py:390-419): per layer
+// and window w, the field (kept in the (E+x, E-x, E+y, E-y) basis of the
+// effective medium, without the mean-beam corrections) is multiplied in
+// k-space by the window, the common half-step phases and the window's
+// correction exp(-+i k0 d a_c(w)) of the whole layer (scalars per direction
+// commute with the per-pixel operator), propagated through the layer's
+// sub-steps with the window's mean beam, and summed into `acc`.
+template <typename T>
+static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, cpx<T>* tmp,
+                            cpx<T>* acc, Batch b, int* status, hipStream_t s) {
+  const int H = P->d.height, W = P->d.width;
+  const size_t HW = size_t(H) * W;
+  const int L = S->n_layers;
+  const double scale = 1.0 / double(HW);
+  const int nbeam = P->d.n_rays / P->d.rays_per_beam;
+  const int nwin = int(P->win_i.size());
+  auto eff = [&](int l) {
+    return mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
+  };
+  cpx<T>* A = out;
+  TRY(row_pass<T>(P, A, b, 4, OP_FWD, s));
+  TRY(col_pass<T>(P, A, b, 4, OP_FWD, s));
+  KArgs<T> k{};
+  k.kop = KOP_HEAD4;  // F -> Epn basis change only
+  k.d0 = 0.;
+  k.e0 = eff(0);
+  k.scale = 1.0 / double(H);
+  k.nocorr = 1;
+  TRY(kspace<T>(P, A, A, b, k, s));
+  TRY(col_pass<T>(P, A, b, 4, OP_INV, s));
+  double prev_d = 0.;  // half step of the previous layer still to apply
+  cd prev_e = eff(0);
+  for (int l = 0; l < L; ++l) {
+    const int ns = S->n_steps ? S->n_steps[l] : 1;
+    const double ds = S->thickness[l] / double(ns);
+    const cd el = eff(l);
+    if (!eq(el, prev_e)) {
+      TRY(col_pass<T>(P, A, b, 4, OP_FWD, s));
+      KArgs<T> bd{};
+      bd.kop = KOP_BOUND4;
+      bd.d0 = prev_d;
+      bd.e0 = prev_e;
+      bd.d1 = 0.;
+      bd.e1 = el;
+      bd.scale = 1.0 / double(H);
+      bd.nocorr = 1;
+      TRY(kspace<T>(P, A, A, b, bd, s));
+      TRY(col_pass<T>(P, A, b, 4, OP_INV, s));
+      prev_d = 0.;
+    }
+    for (int w = 0; w < nwin; ++w) {
+      Beams bw = P->bm;
+      for (int q = 0; q < nbeam; ++q) {
+        bw.beta[q] = P->win_beta[size_t(w) * nbeam + q];
+        bw.phi[q] = P->win_phi[size_t(w) * nbeam + q];
+      }
+      ColArgs<T> ca{};
+      ca.da = prev_d;
+      ca.ea = prev_e;
+      ca.kca = 0.;
+      ca.db = ds;
+      ca.eb = el;
+      ca.kcb = 2.0 * ns;
+      ca.scale = scale;
+      ca.dense_sign = 1;
+      ca.out = tmp;
+      ca.wi = P->win_i[w];
+      ca.wj = P->win_j[w];
+      TRY(col_pass<T>(P, A, b, 4, OP_WIN, s, &ca, &bw));
+      for (int st = 0; st < ns; ++st) {
+        RowArgs<T> la = layer_args<T>(S, HW, l, ds, 1, status);
+        if (st + 1 == ns) {
+          la.acc = acc;
+          la.acc_mode = (w == 0) ? 1 : 2;
+        }
+        TRY(row_pass<T>(P, tmp, b, 4, OP_ROP, s, &la, &bw));
+        if (st + 1 < ns) {
+          ColArgs<T> cs{};
+          cs.da = ds;
+          cs.db = ds;
+          cs.ea = el;
+          cs.eb = el;
+          cs.scale = scale;
+          cs.dense_sign = 1;
+          TRY(col_pass<T>(P, tmp, b, 4, OP_DIAG, s, &cs, &bw));
+        }
+      }
+    }
+    std::swap(A, acc);
+    prev_d = ds;
+    prev_e = el;
+  }
+  TRY(col_pass<T>(P, A, b, 4, OP_FWD, s));
+  KArgs<T> t{};
+  t.kop = KOP_TAIL4;  // last half step, then Epn -> F
+  t.d0 = prev_d;
+  t.e0 = prev_e;
+  t.scale = scale;
+  t.nocorr = 1;
+  TRY(kspace<T>(P, A, A, b, t, s));
+  TRY(col_pass<T>(P, A, b, 4, OP_INV, s));
+  TRY(row_pass<T>(P, A, b, 4, OP_INV, s));
+  if (A != out) {
+    const size_t bytes = size_t(b.nrays) * b.nlam * 4 * HW * sizeof(cpx<T>);
+    HIP_TRY(hipMemcpyAsync(out, A, bytes, hipMemcpyDeviceToDevice, s));
+  }
+  return DTMMX_OK;
+}
+
 template <typename T>
 static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
                          void* ws, int* status, hipStream_t s) {
@@ -311,18 +449,29 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
   const size_t HW = size_t(H) * W;
   const int nrays = P->d.n_rays, nlam = P->d.n_wavelengths;
   const int nplanes = nrays * nlam;
-  const int L = S->n_layers;
   const double scale = 1.0 / double(HW);
   const bool m4 = P->d.method == DTMMX_METHOD_4X4;
-  auto eff = [&](int l) {
-    return mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
-  };
   cpx<T>* out = (cpx<T>*)fout;
   HIP_TRY(hipMemcpyAsync(out, fin, size_t(nplanes) * 4 * HW * sizeof(cpx<T>),
                          hipMemcpyDeviceToDevice, s));
+  if (m4 && P->wn > 1) {
+    if (S->n_layers == 0) return DTMMX_OK;  // ifft2(fft2(field_in))
+    if (P->win_i.empty()) {  // no window passes betamax: every part is zero
+      const size_t n = size_t(nplanes) * 4 * HW * 2;
+      hipLaunchKernelGGL((zero_kernel<T>), dim3(unsigned((n + NT - 1) / NT)), dim3(NT), 0, s,
+                         (T*)out, n);
+      HIP_TRY(hipGetLastError());
+      return DTMMX_OK;
+    }
+    cpx<T>* tmp = (cpx<T>*)ws;
+    cpx<T>* acc = tmp + size_t(nplanes) * 4 * HW;
+    Batch b{nrays, nlam, 0, nlam};
+    return transfer_windows<T>(P, S, out, tmp, acc, b, status, s);
+  }
+  const std::vector<Step> st = expand_steps(S);
+  const int L = int(st.size());
   // workspace: 2x2 method: the 2-component field
   cpx<T>* e2 = (cpx<T>*)ws;
-  const int nc = m4 ? 4 : 2;
   const int nlb = lam_batch(P, size_t(nrays) * 4 * HW * sizeof(cpx<T>));
   for (int lam0 = 0; lam0 < nlam; lam0 += nlb) {
     Batch b{nrays, nlam, lam0, (lam0 + nlb <= nlam) ? nlb : nlam - lam0};
@@ -332,22 +481,24 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
       TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
       KArgs<T> k{};
       k.kop = KOP_HEAD4;
-      k.d0 = S->thickness[0];
-      k.e0 = eff(0);
+      k.d0 = st[0].d;
+      k.e0 = st[0].e;
       k.scale = scale;
       TRY(kspace<T>(P, out, out, b, k, s));
       TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
       for (int l = 0; l < L; ++l) {
-        const RowArgs<T> la = layer_args<T>(S, HW, l, 1, status);
+        const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status);
         TRY(row_pass<T>(P, out, b, 4, OP_ROP, s, &la));
         if (l + 1 < L) {
-          cd e0 = eff(l), e1 = eff(l + 1);
+          cd e0 = st[l].e, e1 = st[l + 1].e;
           if (eq(e0, e1)) {
             ColArgs<T> ca{};
-            ca.da = S->thickness[l];
-            ca.db = S->thickness[l + 1];
+            ca.da = st[l].d;
+            ca.db = st[l + 1].d;
             ca.ea = e0;
             ca.eb = e1;
+            ca.kca = 1.;
+            ca.kcb = 1.;
             ca.scale = scale;
             ca.dense_sign = 1;
             TRY(col_pass<T>(P, out, b, 4, OP_DIAG, s, &ca));
@@ -355,8 +506,8 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
             TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
             KArgs<T> bd{};
             bd.kop = KOP_BOUND4;
-            bd.d0 = S->thickness[l];
-            bd.d1 = S->thickness[l + 1];
+            bd.d0 = st[l].d;
+            bd.d1 = st[l + 1].d;
             bd.e0 = e0;
             bd.e1 = e1;
             bd.scale = scale;
@@ -368,8 +519,8 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
       TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
       KArgs<T> t{};
       t.kop = KOP_TAIL4;
-      t.d0 = S->thickness[L - 1];
-      t.e0 = eff(L - 1);
+      t.d0 = st[L - 1].d;
+      t.e0 = st[L - 1].e;
       t.scale = scale;
       TRY(kspace<T>(P, out, out, b, t, s));
       TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
@@ -381,21 +532,23 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
     TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
     KArgs<T> h{};
     h.kop = KOP_PROJ2;
-    h.d0 = L > 0 ? S->thickness[0] : 0.;
-    h.e0 = L > 0 ? eff(0) : mk<double>(S->n_out * S->n_out);
+    h.d0 = L > 0 ? st[0].d : 0.;
+    h.e0 = L > 0 ? st[0].e : mk<double>(S->n_out * S->n_out);
     h.ein = mk<double>(S->n_in * S->n_in);
     h.scale = scale;
     TRY(kspace<T>(P, out, e2, b, h, s));
     TRY(col_pass<T>(P, e2, b, 2, OP_INV, s));
     for (int l = 0; l < L; ++l) {
-      const RowArgs<T> la = layer_args<T>(S, HW, l, 0, status);
+      const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 0, status);
       TRY(row_pass<T>(P, e2, b, 2, OP_ROP, s, &la));
       ColArgs<T> ca{};
-      ca.da = S->thickness[l];
-      ca.ea = eff(l);
+      ca.da = st[l].d;
+      ca.ea = st[l].e;
+      ca.kca = 1.;
+      ca.kcb = 1.;
       if (l + 1 < L) {
-        ca.db = S->thickness[l + 1];
-        ca.eb = eff(l + 1);
+        ca.db = st[l + 1].d;
+        ca.eb = st[l + 1].e;
       } else {
         ca.db = 0.;  // output medium: zero thickness, only the betamax mask
         ca.eb = mk<double>(S->n_out * S->n_out);
@@ -443,53 +596,36 @@ static int fft2_impl(void* data, int batch, int H, int W, int inverse, hipStream
   return DTMMX_OK;
 }
 
-}  // namespace dtmmx
-
-
-extern "C" {
-
-int dtmmx_abi_version(void) { return DTMMX_ABI_VERSION; }
-
-const char* dtmmx_last_error(void) { return g_err.c_str(); }
-
-int dtmmx_plan_create(dtmmx_plan** out, const dtmmx_plan_desc* d) {
-  if (!out || !d) return fail(DTMMX_E_ARG, "null argument");
-  *out = nullptr;
-  if (d->height <= 0 || d->width <= 0) return fail(DTMMX_E_ARG, "empty grid");
-  if (!size_supported(d->height) || !size_supported(d->width))
-    return fail(DTMMX_E_UNSUPPORTED, "grid " + std::to_string(d->height) + "x" +
-                                         std::to_string(d->width) + " not built");
-  if (d->n_wavelengths <= 0 || d->n_wavelengths > MAXL)
-    return fail(DTMMX_E_ARG, "n_wavelengths out of range");
-  if (d->n_rays <= 0 || d->rays_per_beam <= 0 || d->n_rays % d->rays_per_beam != 0)
-    return fail(DTMMX_E_ARG, "n_rays must be a positive multiple of rays_per_beam");
-  const int nbeam = d->n_rays / d->rays_per_beam;
-  if (nbeam > MAXB) return fail(DTMMX_E_ARG, "too many beams");
-  if (d->method != DTMMX_METHOD_2X2 && d->method != DTMMX_METHOD_4X4)
-    return fail(DTMMX_E_ARG, "method must be 2 or 4");
-  if (d->precision != DTMMX_C64 && d->precision != DTMMX_C128)
-    return fail(DTMMX_E_ARG, "precision must be DTMMX_C64 or DTMMX_C128");
-  if (!d->k0 || !d->beta || !d->phi) return fail(DTMMX_E_ARG, "null k0/beta/phi");
-  dtmmx_plan* P = new dtmmx_plan();
-  P->d = *d;
-  P->k0.assign(d->k0, d->k0 + d->n_wavelengths);
-  P->beta.assign(d->beta, d->beta + nbeam);
-  P->phi.assign(d->phi, d->phi + nbeam);
-  P->d.k0 = P->k0.data();
-  P->d.beta = P->beta.data();
-  P->d.phi = P->phi.data();
-  std::memset(&P->bm, 0, sizeof(P->bm));
-  P->bm.nlam = d->n_wavelengths;
-  P->bm.rays_per_beam = d->rays_per_beam;
-  for (int i = 0; i < d->n_wavelengths; ++i) P->bm.k0[i] = d->k0[i];
-  for (int i = 0; i < nbeam; ++i) {
-    P->bm.beta[i] = d->beta[i];
-    P->bm.phi[i] = d->phi[i];
+// numpy.mod of two doubles: the remainder with the sign of b.
+static double np_mod(double a, double b) {
+  double m = std::fmod(a, b);
+  if (m != 0.) {
+    if ((b < 0.) != (m < 0.)) m += b;
+  } else {
+    m = std::copysign(0., b);
   }
-  (void)hipGetDevice(&P->device);
-  // betamax mask per (wavelength, ky): beta >= betamax is cut
-  // (dtmm/diffract.py:31, :55, :86) with beta computed as in
-  // dtmm/wave.py:77-78.
+  return m;
+}
+
+// Tent windows of dtmm/mode.py:10-139 for diffraction = n > 1 on the host,
+// in float64 as the reference computes them: window (i, j) is
+// cx_i(beta_x) * cy_j(beta_y), zero where |beta| >= betamax, with centres
+// numpy.linspace(-betamax, betamax, n) shifted by the beam's
+// (beta_x, beta_y) reduced modulo the spacing to the nearest multiple; the
+// mean beam of a window is the window^2-weighted mean (beta_x, beta_y) over
+// wavelengths and k; all-zero windows (over beams and wavelengths) are
+// dropped.  The window factors are separable, so the device keeps cx and cy.
+struct WinTables {
+  std::vector<int> wi, wj;             // non-zero windows
+  std::vector<double> beta, phi;       // [window][beam]
+  std::vector<double> cx, cy;          // [beam][lam][n][W], [beam][lam][n][H]
+  std::vector<int> mask;               // [lam][H]
+};
+
+// betamax mask per (wavelength, ky): the largest passing |kx| index (-1 if
+// none) with beta >= betamax cut (dtmm/diffract.py:31, :55, :86) and beta
+// computed as in dtmm/wave.py:77-78.
+static std::vector<int> betamax_rows(const dtmmx_plan_desc* d) {
   const int H = d->height, W = d->width;
   std::vector<int> amax(size_t(d->n_wavelengths) * H, -1);
   for (int l = 0; l < d->n_wavelengths; ++l) {
@@ -509,6 +645,196 @@ int dtmmx_plan_create(dtmmx_plan** out, const dtmmx_plan_desc* d) {
       amax[size_t(l) * H + y] = best;
     }
   }
+  return amax;
+}
+
+static void build_windows(const dtmmx_plan_desc* d, const std::vector<int>& amax, WinTables& t) {
+#pragma clang fp contract(off)
+  const int n = d->diffraction, H = d->height, W = d->width, nl = d->n_wavelengths;
+  const int nb = d->n_rays / d->rays_per_beam;
+  const double bm = d->betamax;
+  const double sp = 2 * bm / (n - 1);
+  std::vector<double> ax(n);
+  {
+    const double start = -bm, step = (bm - start) / double(n - 1);
+    for (int i = 0; i < n; ++i) ax[i] = double(i) * step + start;
+    ax[n - 1] = bm;
+  }
+  const double st = ax[1] - ax[0];
+  auto tent = [&](double f, double c, int typ) {
+    const double dx = (f - c) / st;
+    const double v = 1. - std::fabs(dx);
+    if (v < 0.) return 0.;
+    if ((typ < 0 && dx < 0.) || (typ > 0 && dx > 0.)) return 1.;
+    return v;
+  };
+  auto typ = [&](int i) { return i == 0 ? -1 : (i == n - 1 ? 1 : 0); };
+  // largest |kx| index with |beta| < betamax on the window's beta grid
+  std::vector<int> wmax(size_t(nl) * H);
+  std::vector<int>& mask = t.mask;
+  mask.assign(size_t(nl) * H, -1);
+  for (int l = 0; l < nl; ++l) {
+    const double lf = 2 * M_PI / d->k0[l];
+    for (int y = 0; y < H; ++y) {
+      const double by = lf * fftfreq(y, H);
+      int best = -1;
+      for (int m = 0; m <= W / 2; ++m) {
+        const double bx = lf * fftfreq(m, W);
+        if (std::sqrt(bx * bx + by * by) >= bm) break;
+        best = m;
+      }
+      wmax[size_t(l) * H + y] = best;
+      mask[size_t(l) * H + y] = std::min(best, amax[size_t(l) * H + y]);
+    }
+  }
+  std::vector<double>& cx = t.cx;
+  std::vector<double>& cy = t.cy;
+  cx.assign(size_t(nb) * nl * n * W, 0.);
+  cy.assign(size_t(nb) * nl * n * H, 0.);
+  std::vector<double> SX(size_t(nb) * n * n, 0.), SY(SX.size(), 0.), NN(SX.size(), 0.);
+  std::vector<double> sx(n), nx(n);
+  for (int q = 0; q < nb; ++q) {
+    const double ox = d->beta[q] * std::cos(d->phi[q]), oy = d->beta[q] * std::sin(d->phi[q]);
+    double xo = np_mod(ox, sp), xm = np_mod(ox, -sp);
+    if (std::fabs(xo) > std::fabs(xm)) xo = xm;
+    double yo = np_mod(oy, sp), ym = np_mod(oy, -sp);
+    if (std::fabs(yo) > std::fabs(ym)) yo = ym;
+    for (int l = 0; l < nl; ++l) {
+      const double lf = 2 * M_PI / d->k0[l];
+      double* cxl = cx.data() + (size_t(q) * nl + l) * n * W;
+      double* cyl = cy.data() + (size_t(q) * nl + l) * n * H;
+      for (int i = 0; i < n; ++i) {
+        for (int x = 0; x < W; ++x) cxl[i * W + x] = tent(lf * fftfreq(x, W), ax[i] + xo, typ(i));
+        for (int y = 0; y < H; ++y) cyl[i * H + y] = tent(lf * fftfreq(y, H), ax[i] + yo, typ(i));
+      }
+      for (int y = 0; y < H; ++y) {
+        const int thr = wmax[size_t(l) * H + y];
+        const double by = lf * fftfreq(y, H);
+        for (int i = 0; i < n; ++i) {
+          double a = 0., c = 0.;
+          for (int x = 0; x < W; ++x) {
+            if (mode_index(x, W) > thr) continue;
+            const double w2 = cxl[i * W + x] * cxl[i * W + x];
+            a += w2 * (lf * fftfreq(x, W));
+            c += w2;
+          }
+          sx[i] = a;
+          nx[i] = c;
+        }
+        for (int j = 0; j < n; ++j) {
+          const double wy = cyl[j * H + y] * cyl[j * H + y];
+          if (wy == 0.) continue;
+          for (int i = 0; i < n; ++i) {
+            const size_t o = (size_t(q) * n + i) * n + j;
+            SX[o] += wy * sx[i];
+            SY[o] += wy * nx[i] * by;
+            NN[o] += wy * nx[i];
+          }
+        }
+      }
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      bool any = false;
+      for (int q = 0; q < nb; ++q) any = any || NN[(size_t(q) * n + i) * n + j] > 0.;
+      if (!any) continue;
+      t.wi.push_back(i);
+      t.wj.push_back(j);
+      for (int q = 0; q < nb; ++q) {
+        const size_t o = (size_t(q) * n + i) * n + j;
+        const double bx = NN[o] > 0. ? SX[o] / NN[o] : 0., by = NN[o] > 0. ? SY[o] / NN[o] : 0.;
+        t.beta.push_back(std::sqrt(bx * bx + by * by));
+        t.phi.push_back(std::atan2(by, bx));
+      }
+    }
+}
+
+static int upload_windows(dtmmx_plan* P, const std::vector<int>& amax) {
+  WinTables t;
+  build_windows(&P->d, amax, t);
+  P->wn = P->d.diffraction;
+  P->win_i = t.wi;
+  P->win_j = t.wj;
+  P->win_beta = t.beta;
+  P->win_phi = t.phi;
+  const std::vector<double>& cx = t.cx;
+  const std::vector<double>& cy = t.cy;
+  const std::vector<int>& mask = t.mask;
+  hipError_t e = hipMalloc((void**)&P->wcx, cx.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&P->wcy, cy.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&P->wmask, mask.size() * sizeof(int));
+  if (e == hipSuccess)
+    e = hipMemcpy(P->wcx, cx.data(), cx.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(P->wcy, cy.data(), cy.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(P->wmask, mask.data(), mask.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? DTMMX_E_NOMEM : DTMMX_E_HIP,
+                std::string("window tables: ") + hipGetErrorString(e));
+  return DTMMX_OK;
+}
+
+}  // namespace dtmmx
+
+
+extern "C" {
+
+int dtmmx_abi_version(void) { return DTMMX_ABI_VERSION; }
+
+const char* dtmmx_last_error(void) { return g_err.c_str(); }
+
+static int validate_desc(const dtmmx_plan_desc* d) {
+  if (d->height <= 0 || d->width <= 0) return fail(DTMMX_E_ARG, "empty grid");
+  if (!size_supported(d->height) || !size_supported(d->width))
+    return fail(DTMMX_E_UNSUPPORTED, "grid " + std::to_string(d->height) + "x" +
+                                         std::to_string(d->width) + " not built");
+  if (d->n_wavelengths <= 0 || d->n_wavelengths > MAXL)
+    return fail(DTMMX_E_ARG, "n_wavelengths out of range");
+  if (d->n_rays <= 0 || d->rays_per_beam <= 0 || d->n_rays % d->rays_per_beam != 0)
+    return fail(DTMMX_E_ARG, "n_rays must be a positive multiple of rays_per_beam");
+  const int nbeam = d->n_rays / d->rays_per_beam;
+  if (nbeam > MAXB) return fail(DTMMX_E_ARG, "too many beams");
+  if (d->method != DTMMX_METHOD_2X2 && d->method != DTMMX_METHOD_4X4)
+    return fail(DTMMX_E_ARG, "method must be 2 or 4");
+  if (d->precision != DTMMX_C64 && d->precision != DTMMX_C128)
+    return fail(DTMMX_E_ARG, "precision must be DTMMX_C64 or DTMMX_C128");
+  if (!d->k0 || !d->beta || !d->phi) return fail(DTMMX_E_ARG, "null k0/beta/phi");
+  if (d->diffraction < 1) return fail(DTMMX_E_UNSUPPORTED, "diffraction must be >= 1");
+  if (d->diffraction > 1 && d->method != DTMMX_METHOD_4X4)
+    return fail(DTMMX_E_UNSUPPORTED, "diffraction > 1 is built for the 4x4 method only");
+  if (d->diffraction > 64) return fail(DTMMX_E_ARG, "diffraction above 64");
+  return DTMMX_OK;
+}
+
+int dtmmx_plan_create(dtmmx_plan** out, const dtmmx_plan_desc* d) {
+  if (!out || !d) return fail(DTMMX_E_ARG, "null argument");
+  *out = nullptr;
+  {
+    const int r = validate_desc(d);
+    if (r != DTMMX_OK) return r;
+  }
+  const int nbeam = d->n_rays / d->rays_per_beam;
+  dtmmx_plan* P = new dtmmx_plan();
+  P->d = *d;
+  P->k0.assign(d->k0, d->k0 + d->n_wavelengths);
+  P->beta.assign(d->beta, d->beta + nbeam);
+  P->phi.assign(d->phi, d->phi + nbeam);
+  P->d.k0 = P->k0.data();
+  P->d.beta = P->beta.data();
+  P->d.phi = P->phi.data();
+  std::memset(&P->bm, 0, sizeof(P->bm));
+  P->bm.nlam = d->n_wavelengths;
+  P->bm.rays_per_beam = d->rays_per_beam;
+  for (int i = 0; i < d->n_wavelengths; ++i) P->bm.k0[i] = d->k0[i];
+  for (int i = 0; i < nbeam; ++i) {
+    P->bm.beta[i] = d->beta[i];
+    P->bm.phi[i] = d->phi[i];
+  }
+  (void)hipGetDevice(&P->device);
+  const int H = d->height, W = d->width;
+  const std::vector<int> amax = betamax_rows(d);
   size_t cb = d->precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
   hipError_t e = hipMalloc(&P->tw_w, W * cb);
   if (e == hipSuccess) e = hipMalloc(&P->tw_h, H * cb);
@@ -531,6 +857,13 @@ int dtmmx_plan_create(dtmmx_plan** out, const dtmmx_plan_desc* d) {
     return fail(e == hipErrorOutOfMemory ? DTMMX_E_NOMEM : DTMMX_E_HIP,
                 std::string("plan tables: ") + hipGetErrorString(e));
   }
+  if (d->diffraction > 1) {
+    const int r = upload_windows(P, amax);
+    if (r != DTMMX_OK) {
+      dtmmx_plan_destroy(P);
+      return r;
+    }
+  }
   *out = P;
   return DTMMX_OK;
 }
@@ -540,6 +873,9 @@ int dtmmx_plan_destroy(dtmmx_plan* P) {
   if (P->tw_w) (void)hipFree(P->tw_w);
   if (P->tw_h) (void)hipFree(P->tw_h);
   if (P->amax) (void)hipFree(P->amax);
+  if (P->wcx) (void)hipFree(P->wcx);
+  if (P->wcy) (void)hipFree(P->wcy);
+  if (P->wmask) (void)hipFree(P->wmask);
   delete P;
   return DTMMX_OK;
 }
@@ -550,6 +886,7 @@ size_t dtmmx_plan_workspace_size(const dtmmx_plan* P) {
   size_t HW = size_t(P->d.height) * P->d.width;
   size_t planes = size_t(P->d.n_rays) * P->d.n_wavelengths;
   if (P->d.method == DTMMX_METHOD_2X2) return planes * 2 * HW * cb;
+  if (P->wn > 1) return 2 * planes * 4 * HW * cb;  // window part and the sum
   return 0;
 }
 
@@ -559,6 +896,9 @@ int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, v
   if (S->n_layers < 0) return fail(DTMMX_E_ARG, "negative layer count");
   if (S->n_layers > 0 && (!S->thickness || !S->eps_eff_re || !S->eps_re || !S->angles))
     return fail(DTMMX_E_ARG, "null stack array");
+  if (S->n_steps)
+    for (int l = 0; l < S->n_layers; ++l)
+      if (S->n_steps[l] < 1) return fail(DTMMX_E_ARG, "n_steps must be >= 1");
   if (ws_bytes < dtmmx_plan_workspace_size(P) || (dtmmx_plan_workspace_size(P) && !ws))
     return fail(DTMMX_E_ARG, "workspace too small");
   if (fin == fout) return fail(DTMMX_E_ARG, "field_in and field_out must not alias");
@@ -620,6 +960,31 @@ int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int pr
   if (precision == DTMMX_C64) return fft2_impl<float>(data, batch, height, width, inverse, s);
   if (precision == DTMMX_C128) return fft2_impl<double>(data, batch, height, width, inverse, s);
   return fail(DTMMX_E_ARG, "bad precision");
+}
+
+int dtmmx_host_windows(const dtmmx_plan_desc* d, int max_windows, int* n_windows, int* wi,
+                       int* wj, double* beta, double* phi) {
+  if (!d || !n_windows) return fail(DTMMX_E_ARG, "null argument");
+  {
+    const int r = validate_desc(d);
+    if (r != DTMMX_OK) return r;
+  }
+  if (d->diffraction < 2) return fail(DTMMX_E_ARG, "diffraction must be > 1");
+  WinTables t;
+  build_windows(d, betamax_rows(d), t);
+  const int n = int(t.wi.size());
+  *n_windows = n;
+  if (n > max_windows) return fail(DTMMX_E_ARG, "max_windows too small");
+  const int nb = d->n_rays / d->rays_per_beam;
+  for (int w = 0; w < n; ++w) {
+    if (wi) wi[w] = t.wi[w];
+    if (wj) wj[w] = t.wj[w];
+    for (int q = 0; q < nb; ++q) {
+      if (beta) beta[size_t(w) * nb + q] = t.beta[size_t(w) * nb + q];
+      if (phi) phi[size_t(w) * nb + q] = t.phi[size_t(w) * nb + q];
+    }
+  }
+  return DTMMX_OK;
 }
 
 int dtmmx_host_alphaf(int n, const double* beta, const double* phi, const double* eps_re,

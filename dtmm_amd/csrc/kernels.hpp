@@ -38,7 +38,7 @@ __host__ __device__ inline int mode_index(int i, int n) {  // |k| of fftfreq
 // Launch geometry shared by the per-layer kernels.
 // ---------------------------------------------------------------------------
 
-enum { OP_FWD = 0, OP_INV = 1, OP_ROP = 2, OP_DIAG = 3 };
+enum { OP_FWD = 0, OP_INV = 1, OP_ROP = 2, OP_DIAG = 3, OP_WIN = 4 };
 
 // Work-item index for block b of n such that consecutive work items land on
 // the same XCD (blocks are dispatched round-robin over the 8 XCDs): work
@@ -83,6 +83,10 @@ struct RowArgs {
   double d;             // layer thickness
   int method4;          // 1: 4x4 (E invertible), 0: 2x2 (E+ invertible)
   int* status;
+  // OP_ROP output: acc_mode 0 writes the result back to `field`; 1 writes
+  // it to `acc`; 2 adds it to `acc` (sum over k-space windows)
+  cpx<T>* acc;
+  int acc_mode;
   Beams bm;
 };
 
@@ -169,6 +173,7 @@ __global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowAr
     const size_t q = size_t(rg * NR + rr) * A.b.nlam + A.b.lam0 + jl;
     return A.field + (q * NC + comp) * HW + size_t(y) * N;
   };
+  const int acc_mode = (OP == OP_ROP) ? A.acc_mode : 0;
   cpx<T> nxt[NIN];  // prefetched inputs of the next wavelength
   {
     const cpx<T>* g0 = line_ptr(0);
@@ -242,9 +247,25 @@ __global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowAr
       exchange_sync<true>();
       reg_fft<T, N, false, -1, NL, 1, 1, MAXV, true>(v, line_lds, tws, lane);
     }
+    if (acc_mode == 0) {
 #pragma unroll
-    for (int j = 0; j < NOUT; ++j)
-      if (slot_valid<N, REVL, NL, true>(j, lane)) gl[slot_pos<N, REVL, NL, true>(j, lane)] = v[j];
+      for (int j = 0; j < NOUT; ++j)
+        if (slot_valid<N, REVL, NL, true>(j, lane)) gl[slot_pos<N, REVL, NL, true>(j, lane)] = v[j];
+    } else {
+      cpx<T>* al = A.acc + (gl - A.field);
+      if (acc_mode == 2) {
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j)
+          if (slot_valid<N, REVL, NL, true>(j, lane)) {
+            const int p = slot_pos<N, REVL, NL, true>(j, lane);
+            al[p] = al[p] + v[j];
+          }
+      } else {
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j)
+          if (slot_valid<N, REVL, NL, true>(j, lane)) al[slot_pos<N, REVL, NL, true>(j, lane)] = v[j];
+      }
+    }
     exchange_sync<true>();
   }
 }
@@ -268,8 +289,16 @@ struct ColArgs {
   const int* amax;    // [nlam][H]: largest passing |kx| index per ky row, -1 if none
   double da, db;      // thicknesses of the two half steps
   cd ea, eb;          // isotropic effective eps of the two half steps
+  double kca, kcb;    // weights of the mean-beam correction of each half step
   double scale;
   int dense_sign;     // 1: backward components (odd c) get exp(-i phase)
+  // OP_WIN: destination (the source `field` is left unchanged) and the tent
+  // window (wi, wj) of wn x wn: separable factors cx [beam][lam][wn][W] and
+  // cy [beam][lam][wn][H]
+  cpx<T>* out;
+  const double* wcx;
+  const double* wcy;
+  int wn, wi, wj;
   Beams bm;
 };
 
@@ -285,11 +314,13 @@ __device__ __forceinline__ cpx<T> diag_factor(const ColArgs<T>& A, int lam, int 
   cpx<T> ph = mk<T>(T(0), T(0));
   if (A.da != 0.) {
     cpx<T> e = cast<T>(A.ea);
-    ph = ph + (csqrt(e - mk<T>(b2)) - csqrt(e - mk<T>(bc * bc))) * T(0.5 * A.da);
+    ph = ph + csqrt(e - mk<T>(b2)) * T(0.5 * A.da);
+    if (A.kca != 0.) ph = ph - csqrt(e - mk<T>(bc * bc)) * T(0.5 * A.da * A.kca);
   }
   if (A.db != 0.) {
     cpx<T> e = cast<T>(A.eb);
-    ph = ph + (csqrt(e - mk<T>(b2)) - csqrt(e - mk<T>(bc * bc))) * T(0.5 * A.db);
+    ph = ph + csqrt(e - mk<T>(b2)) * T(0.5 * A.db);
+    if (A.kcb != 0.) ph = ph - csqrt(e - mk<T>(bc * bc)) * T(0.5 * A.db * A.kcb);
   }
   return cexpi_layer(ph * k0) * T(A.scale);
 }
@@ -326,23 +357,37 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), min_waves<T>()) void col_kerne
   const bool xin = x < A.W;
   const size_t HW = size_t(N) * A.W;
 
-  cpx<T> K[(OP == OP_DIAG) ? NMID : 1];
-  if constexpr (OP == OP_DIAG) {
+  constexpr bool KOP = (OP == OP_DIAG || OP == OP_WIN);
+  cpx<T> K[KOP ? NMID : 1];
+  T Wv[(OP == OP_WIN) ? NMID : 1];  // window factor cx(kx) * cy(ky)
+  if constexpr (KOP) {
     const int ray0 = (lg * A.ncl) / A.NC;  // all lines of a work-group share one beam
     const int beam = ray0 / A.bm.rays_per_beam;
+    double cx = 0.;
+    const double* cyt = nullptr;
+    if constexpr (OP == OP_WIN) {
+      const size_t bl = size_t(beam) * A.b.nlam + lam;
+      if (xin) cx = A.wcx[(bl * A.wn + A.wi) * A.W + x];
+      cyt = A.wcy + (bl * A.wn + A.wj) * N;
+    }
 #pragma unroll
     for (int j = 0; j < NMID; ++j) {
       K[j] = mk<T>(T(0), T(0));
-      if (xin && slot_valid<N, REV1, NL, true>(j, g))
-        K[j] = diag_factor(A, lam, beam, slot_pos<N, REV1, NL, true>(j, g), x);
+      if constexpr (OP == OP_WIN) Wv[j] = T(0);
+      if (xin && slot_valid<N, REV1, NL, true>(j, g)) {
+        const int ky = slot_pos<N, REV1, NL, true>(j, g);
+        K[j] = diag_factor(A, lam, beam, ky, x);
+        if constexpr (OP == OP_WIN) Wv[j] = T(cx * cyt[ky]);
+      }
     }
   }
-  auto line_base = [&](int li) {
+  auto line_off = [&](int li) {
     const int line = lg * A.ncl + li;
     const int ray = line / A.NC, comp = line % A.NC;
     const size_t q = size_t(ray) * A.b.nlam + lam;
-    return A.field + (q * A.NC + comp) * HW + x;
+    return (q * A.NC + comp) * HW + x;
   };
+  auto line_base = [&](int li) { return A.field + line_off(li); };
   cpx<T> nxt[NIN];  // prefetched inputs of the next line
   {
     const cpx<T>* b0 = line_base(0);
@@ -355,7 +400,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), min_waves<T>()) void col_kerne
   }
   for (int li = 0; li < A.ncl; ++li) {
     const int comp = (lg * A.ncl + li) % A.NC;
-    cpx<T>* base = line_base(li);
+    cpx<T>* base = (OP == OP_WIN) ? A.out + line_off(li) : line_base(li);
     cpx<T> v[MAXV];
 #pragma unroll
     for (int j = 0; j < NIN; ++j) v[j] = nxt[j];
@@ -367,7 +412,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), min_waves<T>()) void col_kerne
           nxt[j] = bn[size_t(slot_pos<N, REV1, NL, false>(j, g)) * A.W];
     }
     reg_fft<T, N, REV1, DIR1, NL, TX, 0>(v, lds + c, tws, g);
-    if constexpr (OP == OP_DIAG) {
+    if constexpr (KOP) {
       const bool back = A.dense_sign && (comp & 1);
       const T s2 = T(A.scale * A.scale);
 #pragma unroll
@@ -377,11 +422,12 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), min_waves<T>()) void col_kerne
           const T n2 = norm2(k);
           k = n2 > T(0) ? conj(k) * (s2 / n2) : mk<T>(T(0), T(0));
         }
+        if constexpr (OP == OP_WIN) k = k * Wv[j];
         v[j] = v[j] * k;
       }
       reg_fft<T, N, true, +1, NL, TX, 0>(v, lds + c, tws, g);
     }
-    constexpr bool REVL = (OP == OP_DIAG) ? true : REV1;
+    constexpr bool REVL = KOP ? true : REV1;
     constexpr int NOUT = slot_count<N, REVL, NL, true>();
 #pragma unroll
     for (int j = 0; j < NOUT; ++j)
@@ -409,6 +455,7 @@ struct KArgs {
   cd e0, e1;      // effective eps of the layer(s) involved
   cd ein;         // input medium eps (PROJ2)
   double scale;
+  int nocorr;     // 1: no mean-beam correction (HEAD4, TAIL4, BOUND4)
   Beams bm;
 };
 
@@ -513,7 +560,8 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
       const cd m0 = (g0 + g1 * igp) * p, m1 = (g0 - g1 * igp) * pi;
       const cd m2 = (g2 + g3 * igs) * p, m3 = (g2 - g3 * igs) * pi;
       const cd ac = csqrt(A.e0 - mk<double>(bb * bb));
-      const cd cf = cexpi(ac * -kd), cb = cexpi(ac * kd);
+      const cd cf = A.nocorr ? mk<double>(1.) : cexpi(ac * -kd);
+      const cd cb = A.nocorr ? mk<double>(1.) : cexpi(ac * kd);
       r[0] = (m0 * c - m2 * sn) * cf * 0.5;
       r[1] = (m1 * c - m3 * sn) * cb * 0.5;
       r[2] = (m0 * sn + m2 * c) * cf * 0.5;
@@ -525,7 +573,8 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
     if (!is_zero(a)) {
       const double kd = k0 * A.d0 * 0.5;
       const cd ac = csqrt(A.e0 - mk<double>(bb * bb));
-      const cd cf = cexpi(ac * -kd), cb = cexpi(ac * kd);
+      const cd cf = A.nocorr ? mk<double>(1.) : cexpi(ac * -kd);
+      const cd cb = A.nocorr ? mk<double>(1.) : cexpi(ac * kd);
       const cd epx = v[0] * cf, enx = v[1] * cb, epy = v[2] * cf, eny = v[3] * cb;
       const cd p = cexpi(a * kd), pi = cexpi(a * -kd);
       const cd m0 = (epx * c + epy * sn) * p * 2., m2 = (epy * c - epx * sn) * p * 2.;
@@ -560,9 +609,14 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
     }
   } else if (pass && A.kop == KOP_BOUND4) {
     cd C0[4][4], D0[4][4], C1[4][4], D1[4][4], X[4][4], Y[4][4], M[4][4];
-    correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C0);
+    if (A.nocorr) {
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) C0[i][j] = C1[i][j] = mk<double>(i == j ? 1. : 0.);
+    } else {
+      correction(bb, pb, A.e0, k0 * A.d0 * 0.5, C0);
+      correction(bb, pb, A.e1, k0 * A.d1 * 0.5, C1);
+    }
     second_half(beta, phi, A.e0, k0 * A.d0 * 0.5, D0);
-    correction(bb, pb, A.e1, k0 * A.d1 * 0.5, C1);
     first_half(beta, phi, A.e1, k0 * A.d1 * 0.5, D1);
     matmul4(D0, C0, X);  // second corrected of layer 0
     matmul4(C1, D1, Y);  // first corrected of layer 1
@@ -574,6 +628,13 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
     }
   }
   for (int i = 0; i < nout; ++i) o[i * HW] = cast<T>(r[i] * A.scale);
+}
+
+// Zero n values of type T.
+template <typename T>
+__global__ __launch_bounds__(NT) void zero_kernel(T* p, size_t n) {
+  const size_t i = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (i < n) p[i] = T(0);
 }
 
 // (Ex, Ey) -> (Ex, Hy, Ey, Hx) with the mean-beam H/E relation of the output

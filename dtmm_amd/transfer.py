@@ -3,8 +3,8 @@
 Mirrors dtmm/transfer.py:322 ``transfer_field`` (same arguments, defaults,
 return value and errors) for the modes built on the GPU:
 
-* method "4x4": reflection mode 1, diffraction 1, npass 1
-  (dtmm/transfer.py:618-884, dtmm/propagate_4x4.py:376-389, :80-101);
+* method "4x4": reflection mode 1, diffraction >= 1, npass 1
+  (dtmm/transfer.py:618-884, dtmm/propagate_4x4.py:376-444, :80-101);
 * method "2x2": reflection mode 0, diffraction 1, npass 1
   (dtmm/transfer.py:1009-1183, dtmm/propagate_2x2.py:193-211, :19-87).
 
@@ -90,16 +90,17 @@ def field2betaphi(field_in, ks, multiray=False):
 
 class Stack:
     """Layer stack resident on the device: material as float64
-    structure-of-arrays [3][L][H][W] plus per-layer thickness and isotropic
-    effective eps.  Built from reference optical data with input/output
-    media and nstep sub-layers (dtmm/transfer.py:926-1005, eff_data=0)."""
+    structure-of-arrays [3][L][H][W] plus per-layer thickness, sub-step
+    count and isotropic effective eps.  Built from reference optical data
+    (dtmm/transfer.py:926-1005, eff_data=0); layer l is propagated in
+    nstep sub-steps of thickness d / nstep (transfer.py:933-934)."""
 
     def __init__(self, optical_data, shape, nstep=1, nin=N_COVER, nout=N_COVER,
                  split_layers=False, device="cuda"):
         torch = _torch()
         blocks = validate_optical_data(optical_data, shape=shape)
         H, W = shape
-        thick, eff, eps_l, ang_l = [], [], [], []
+        thick, steps, eff, eps_l = [], [], [], []
         for d, material, angles in blocks:
             n = len(d)
             substeps = np.broadcast_to(np.asarray(nstep), (n,))
@@ -114,12 +115,13 @@ class Stack:
                 ns = int(substeps[i])
                 if ns < 1:
                     raise ValueError("nstep must be >= 1")
-                for _ in range(ns):
-                    thick.append(float(d[i]) / ns)
-                    eff.append(complex(effs[i]))
-                    eps_l.append((len(eps_l), mat[i], ang[i]))
+                thick.append(float(d[i]))
+                steps.append(ns)
+                eff.append(complex(effs[i]))
+                eps_l.append((len(eps_l), mat[i], ang[i]))
         self.n_layers = len(thick)
         self.thickness = np.asarray(thick, np.float64)
+        self.n_steps = np.asarray(steps, np.int32)
         self.eps_eff_re = np.asarray([e.real for e in eff], np.float64)
         self.eps_eff_im = np.asarray([e.imag for e in eff], np.float64)
         self.nin = float(nin)
@@ -151,6 +153,7 @@ class Stack:
         s.angles = self.angles.data_ptr()
         s.n_in = self.nin
         s.n_out = self.nout
+        s.n_steps = _lib.iptr(self.n_steps)
         return s
 
 
@@ -158,7 +161,7 @@ class Plan:
     """Owner of a ``dtmmx_plan``: grid, wavelengths, beams, method, precision."""
 
     def __init__(self, shape, ks, beta, phi, n_rays, rays_per_beam, method, precision="c64",
-                 betamax=BETAMAX):
+                 betamax=BETAMAX, diffraction=1):
         lib = _lib.load()
         self.shape = tuple(shape)
         self.ks = np.ascontiguousarray(ks, np.float64)
@@ -168,6 +171,7 @@ class Plan:
         self.rays_per_beam = int(rays_per_beam)
         self.method = method
         self.precision = precision
+        self.diffraction = int(diffraction)
         d = _lib.PlanDesc()
         d.height, d.width = self.shape
         d.n_wavelengths = len(self.ks)
@@ -179,6 +183,7 @@ class Plan:
         d.k0 = _lib.dptr(self.ks)
         d.beta = _lib.dptr(self.beta)
         d.phi = _lib.dptr(self.phi)
+        d.diffraction = self.diffraction
         h = ctypes.c_void_p()
         _lib.check(lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)))
         self._h = h
@@ -253,10 +258,8 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk):
     if kind == "4x4":
         if reflection != 1:
             raise NotImplementedError("4x4 method: only reflection mode 1 is supported")
-        if diffraction != 1:
-            if diffraction == 0 or diffraction < 0 or diffraction == np.inf:
-                raise NotImplementedError("4x4 method: only diffraction=1 is supported")
-            raise NotImplementedError("4x4 method: diffraction > 1 (mode windows) not supported yet")
+        if not (diffraction >= 1) or diffraction == np.inf:
+            raise NotImplementedError("4x4 method: only finite diffraction >= 1 is supported")
     else:
         if reflection != 0:
             raise NotImplementedError("2x2 method: only reflection mode 0 is supported "
@@ -313,7 +316,8 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
         raise ValueError("rays do not divide into beams")
     rays_per_beam = n_rays // len(b)
 
-    plan = Plan(shape, ks, b, p, n_rays, rays_per_beam, kind, precision, betamax)
+    plan = Plan(shape, ks, b, p, n_rays, rays_per_beam, kind, precision, betamax,
+                diffraction=int(diffraction))
     stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout, split_layers=split_layers)
     dt = plan.dtype
     fin = torch.from_numpy(np.ascontiguousarray(field_np.reshape((n_rays, len(ks), 4) + shape)))

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define DTMMX_ABI_VERSION 1
+#define DTMMX_ABI_VERSION 2
 
 enum {
   DTMMX_OK = 0,
@@ -67,9 +67,15 @@ typedef struct {
   const double* k0;   /* [n_wavelengths] host: 2 pi pixelsize / lambda  */
   const double* beta; /* [n_rays / rays_per_beam] host                  */
   const double* phi;  /* [n_rays / rays_per_beam] host                  */
+  int diffraction;    /* 1: one mean beam per ray (propagate_4x4.py:382);
+                         n > 1 (4x4 only): the k-space field is split by
+                         the non-zero n x n tent windows of dtmm/mode.py,
+                         each propagated with its window's mean beam
+                         (propagate_4x4.py:390-419)                      */
 } dtmmx_plan_desc;
 
-/* A layer stack (already split into nstep sub-layers, transfer.py:934). */
+/* A layer stack.  Layer l is propagated as n_steps[l] sub-steps of
+ * thickness[l] / n_steps[l] (transfer.py:933-934: (n, (t/n, ...))). */
 typedef struct {
   int n_layers;
   const double* thickness;   /* [n_layers] host, pixel units            */
@@ -79,6 +85,7 @@ typedef struct {
   const double* eps_im;      /* device [3][n_layers][H][W] or NULL       */
   const double* angles;      /* device [3][n_layers][H][W]               */
   double n_in, n_out;        /* input/output media (transfer.py:936-939) */
+  const int* n_steps;        /* [n_layers] host, >= 1; NULL: all 1       */
 } dtmmx_stack;
 
 typedef struct dtmmx_plan dtmmx_plan;
@@ -96,8 +103,8 @@ int dtmmx_plan_destroy(dtmmx_plan* plan);
 size_t dtmmx_plan_workspace_size(const dtmmx_plan* plan);
 
 /* Full transfer through the stack, npass = 1.
- *   4x4 (reflection mode 1, diffraction 1): replaces transfer.py:694-875
- *       with propagate_4x4.py:376-389 and :80-101 per layer.
+ *   4x4 (reflection mode 1, diffraction >= 1): replaces transfer.py:694-875
+ *       with propagate_4x4.py:376-444 and :80-101 per layer.
  *   2x2 (reflection mode 0, diffraction 1): replaces transfer.py:1071-1176
  *       with propagate_2x2.py:193-211 and :19-87 per interface.
  * field_in and field_out are device buffers of n_rays*n_wavelengths*4*H*W
@@ -125,6 +132,16 @@ int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int pr
  * is unused).  Not thread-safe; for benchmarking. */
 int dtmmx_timing_begin(int capacity, int stride);
 int dtmmx_timing_end(double* ms, int* count, int nclass);
+
+/* Host evaluation of the k-space windows a plan with diffraction > 1 uses
+ * (dtmm/mode.py:119-139 fft_mask with the beam offsets of
+ * propagate_4x4.py:401-402), for checks without a GPU: the non-zero
+ * windows in the reference's order, as tent indices (wi, wj) of the
+ * diffraction x diffraction grid, and their mean beams beta, phi
+ * [window][beam].  *n_windows receives the count; fails with DTMMX_E_ARG if
+ * it exceeds max_windows.  Output arrays may be NULL. */
+int dtmmx_host_windows(const dtmmx_plan_desc* desc, int max_windows, int* n_windows, int* wi,
+                       int* wj, double* beta, double* phi);
 
 /* Host evaluation of the per-pixel eigen-solution used on the device
  * (dtmm/tmm.py:435 alphaf for isotropic and uniaxial voxels), for checks

@@ -9,8 +9,8 @@ Scope (what the GPU engine implements and this module checks):
 * wave-vector grids ``betaphi``/``betaxy``/``k0``          (dtmm/wave.py)
 * per-pixel Berreman eigen-solution ``alphaf``             (dtmm/tmm.py, rotation.py)
 * isotropic-medium k-space diffraction/correction matrices (dtmm/diffract.py, matrix.py)
-* ``transfer_4x4`` with reflection=1, diffraction=1, npass=1 (dtmm/transfer.py,
-  dtmm/propagate_4x4.py)
+* ``transfer_4x4`` with reflection=1, diffraction >= 1, npass=1 (dtmm/transfer.py,
+  dtmm/propagate_4x4.py); diffraction > 1 uses the tent windows of dtmm/mode.py
 * ``transfer_2x2`` with reflection=0, diffraction in {0, 1}, npass=1
   (dtmm/transfer.py, dtmm/propagate_2x2.py)
 * ``illumination_data`` and ``field2intensity``            (dtmm/field.py)
@@ -35,6 +35,7 @@ __all__ = [
     "propagate_2x2_effective_1", "transfer_4x4", "transfer_2x2",
     "transfer_field", "illumination_data", "blackman", "planewave",
     "jonesvec", "avec", "fvec", "auxiliary_matrix", "poynting", "set_fft_workers",
+    "fft_window", "fft_windows", "fft_betaxy_mean", "fft_mask",
 ]
 
 # dtmm/dtmm.ini:6 (betamax = 0.8) and dtmm/conf.py:148, :503-510 (nin/nout
@@ -104,6 +105,89 @@ def blackman(shape):
     out = 0.42 + 0.5 * np.cos(np.pi * r) + 0.08 * np.cos(2 * np.pi * r)
     out[r >= 1.] = 0.
     return np.asarray(out, FDTYPE)
+
+
+# ---------------------------------------------------------------------------
+# mode.py: tent windows in (beta_x, beta_y) for diffraction > 1
+# ---------------------------------------------------------------------------
+
+def fft_window(fbetax, fbetay, betax, betay, step, xtyp, ytyp, betamax):
+    """One tent window on the FFT grid (dtmm/mode.py:10-50).
+
+    The tent is 1 - |d| with d = (beta_x - centre) / step, clipped at 0; the
+    first (xtyp < 0) and last (xtyp > 0) tents are flat (1) on their outer
+    side within one step; zero where beta >= betamax.  `betax`, `betay` (the
+    tent centres) broadcast over leading axes like the reference's gufunc
+    loop dimensions."""
+    bx = np.asarray(betax, FDTYPE)[..., None, None]
+    by = np.asarray(betay, FDTYPE)[..., None, None]
+
+    def tent(f, c, typ):
+        dd = (f - c) / step
+        cc = 1. - np.abs(dd)
+        if typ < 0:
+            cc = np.where(dd < 0, 1., cc)
+        elif typ > 0:
+            cc = np.where(dd > 0, 1., cc)
+        return np.where(1. - np.abs(dd) < 0., 0., cc)
+
+    cx = tent(fbetax, bx, xtyp)
+    cy = tent(fbetay, by, ytyp)
+    beta = (fbetax ** 2 + fbetay ** 2) ** 0.5
+    coeff = np.where(beta >= betamax, 0., cx * cy)
+    return np.where(coeff > 0., coeff, 0.)
+
+
+def fft_windows(betax, betay, n, betax_off=0., betay_off=0., betamax=BETAMAX):
+    """n*n tent windows, index i*n + j for centre (ax[i], ay[j]) shifted by the
+    beam offset reduced to the nearest multiple of the spacing
+    (dtmm/mode.py:52-102)."""
+    d = 2 * betamax / (n - 1)
+
+    def reduce(off):
+        o = np.mod(off, d)
+        om = np.mod(off, -d)
+        return np.where(np.abs(o) > np.abs(om), om, o)
+
+    xoffset = reduce(np.asarray(betax_off, FDTYPE))
+    yoffset = reduce(np.asarray(betay_off, FDTYPE))
+    ax = np.linspace(-betamax, betamax, n)
+    ay = np.linspace(-betamax, betamax, n)
+    step = ax[1] - ax[0]
+    out = None
+    for i, bx in enumerate(ax):
+        xtyp = -1 if i == 0 else (1 if i == n - 1 else 0)
+        for j, by in enumerate(ay):
+            ytyp = -1 if j == 0 else (1 if j == n - 1 else 0)
+            w = fft_window(betax, betay, bx + xoffset, by + yoffset, step, xtyp, ytyp, betamax)
+            if out is None:
+                out = np.empty((n * n,) + w.shape, w.dtype)
+            out[i * n + j] = w
+    return out
+
+
+def fft_betaxy_mean(betax, betay, windows):
+    """Intensity (window squared) weighted mean beta_x, beta_y of each window,
+    summed over every axis of the beta grid, wavelengths included
+    (dtmm/mode.py:108-117); 0 for an all-zero window."""
+    axis = tuple(range(-betax.ndim, 0))
+    w2 = windows ** 2
+    bx = (betax * w2).sum(axis=axis)
+    by = (betay * w2).sum(axis=axis)
+    norm = w2.sum(axis=axis)
+    m = norm > 0
+    return np.divide(bx, norm, where=m, out=bx), np.divide(by, norm, where=m, out=by)
+
+
+def fft_mask(shape, ks, n, betax_off=0., betay_off=0., betamax=BETAMAX):
+    """Non-zero windows and their mean (beta, phi) (dtmm/mode.py:119-139)."""
+    betax, betay = betaxy(shape[-2:], np.asarray(ks, FDTYPE)[..., None])
+    windows = fft_windows(betax, betay, n, betax_off, betay_off, betamax)
+    bx, by = fft_betaxy_mean(betax, betay, windows)
+    beta = (bx ** 2 + by ** 2) ** 0.5
+    phi = np.arctan2(by, bx)
+    nonzero = np.asarray([not np.all(w == 0.) for w in windows])
+    return windows[nonzero], (beta[nonzero], phi[nonzero])
 
 
 # ---------------------------------------------------------------------------
@@ -803,18 +887,45 @@ def _transfer_ray_4x4_1(field, ks, layer, dmat1, dmat2, beta, phi, nsteps):
 def propagate_4x4_effective_1(field, ks, layer, effective_layer, beta=0, phi=0, nsteps=1,
                               diffraction=1, betamax=BETAMAX, cache=None):
     """Layer step of the 4x4 method, reflection mode 1
-    (dtmm/propagate_4x4.py:376-444; diffraction == 1 only)."""
-    if diffraction != 1:
-        raise NotImplementedError("oracle covers diffraction == 1 for the 4x4 method")
+    (dtmm/propagate_4x4.py:376-444).
+
+    diffraction == 1: one mean beam.  diffraction = n > 1: the k-space field
+    is split by the non-zero n*n tent windows (fft_mask), each part is
+    propagated through all `nsteps` sub-steps with its window's mean beam, and
+    the parts are summed (propagate_4x4.py:390-419)."""
     d_eff, epsv_eff, epsa_eff = effective_layer
     shape = field.shape[-2:]
     cache = cache or _Cache()
-    key = _key(shape, ks, beta, phi, d_eff, epsv_eff, epsa_eff, betamax)
-    dmat1 = cache.get(("d1",) + key, lambda: first_corrected_Epn_diffraction_matrix(
-        shape, ks, beta, phi, d_eff / 2, epsv_eff, epsa_eff, betamax))
-    dmat2 = cache.get(("d2",) + key, lambda: second_corrected_Epn_diffraction_matrix(
-        shape, ks, beta, phi, d_eff / 2, epsv_eff, epsa_eff, betamax))
-    return _transfer_ray_4x4_1(field, ks, layer, dmat1, dmat2, beta, phi, nsteps)
+    if diffraction == 1:
+        key = _key(shape, ks, beta, phi, d_eff, epsv_eff, epsa_eff, betamax)
+        dmat1 = cache.get(("d1",) + key, lambda: first_corrected_Epn_diffraction_matrix(
+            shape, ks, beta, phi, d_eff / 2, epsv_eff, epsa_eff, betamax))
+        dmat2 = cache.get(("d2",) + key, lambda: second_corrected_Epn_diffraction_matrix(
+            shape, ks, beta, phi, d_eff / 2, epsv_eff, epsa_eff, betamax))
+        return _transfer_ray_4x4_1(field, ks, layer, dmat1, dmat2, beta, phi, nsteps)
+    if not diffraction > 1:
+        raise NotImplementedError("oracle covers diffraction >= 1 for the 4x4 method")
+    beta = np.asarray(beta, FDTYPE)
+    phi = np.asarray(phi, FDTYPE)
+    if beta.ndim > 0:
+        bshape = beta.shape
+        beta = beta[..., 0]
+        phi = phi[..., 0]
+    else:
+        bshape = ()
+    windows, (betas, phis) = fft_mask(field.shape, ks, int(diffraction),
+                                      beta * np.cos(phi), beta * np.sin(phi), betamax)
+    nw = len(windows)
+    betas = betas.reshape((nw,) + bshape)
+    phis = phis.reshape((nw,) + bshape)
+    dmats1 = first_corrected_Epn_diffraction_matrix(shape, ks, betas, phis, d_eff / 2,
+                                                    epsv_eff, epsa_eff, betamax)
+    dmats2 = second_corrected_Epn_diffraction_matrix(shape, ks, betas, phis, d_eff / 2,
+                                                     epsv_eff, epsa_eff, betamax)
+    fout = np.zeros_like(field)
+    for window, b, p, dm1, dm2 in zip(windows, betas, phis, dmats1, dmats2):
+        fout += _transfer_ray_4x4_1(field * window, ks, layer, dm1, dm2, b, p, nsteps)
+    return fout
 
 
 def _transfer_ray_2x2_1(fft_field, ks, layer, dmat1, dmat2, beta, phi, nsteps, mode, bulk):

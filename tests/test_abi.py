@@ -26,7 +26,23 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(_lib.EXPORTS)
-    assert lib.dtmmx_abi_version() == 1
+    assert lib.dtmmx_abi_version() == _lib.ABI_VERSION
+
+
+def _desc(ks, beta, phi, **kw):
+    d = _lib.PlanDesc()
+    d.height, d.width = kw.get("shape", (32, 32))
+    d.n_wavelengths = len(ks)
+    d.n_rays = kw.get("n_rays", len(beta))
+    d.rays_per_beam = kw.get("rpb", 1)
+    d.method = kw.get("method", 4)
+    d.precision = kw.get("precision", 0)
+    d.betamax = kw.get("betamax", 0.8)
+    d.k0 = _lib.dptr(ks)
+    d.beta = _lib.dptr(beta)
+    d.phi = _lib.dptr(phi)
+    d.diffraction = kw.get("diffraction", 1)
+    return d
 
 
 def _plan_rc(**kw):
@@ -44,6 +60,7 @@ def _plan_rc(**kw):
     d.k0 = _lib.dptr(ks)
     d.beta = _lib.dptr(b)
     d.phi = _lib.dptr(b)
+    d.diffraction = kw.get("diffraction", 1)
     h = ctypes.c_void_p()
     return lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)), lib.dtmmx_last_error()
 
@@ -57,6 +74,8 @@ def _plan_rc(**kw):
     (dict(n_rays=3, rpb=2), _lib.DTMMX_E_ARG),
     (dict(method=3), _lib.DTMMX_E_ARG),
     (dict(precision=2), _lib.DTMMX_E_ARG),
+    (dict(diffraction=0), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(method=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
 ])
 def test_plan_create_rejects_bad_descriptions(kw, code):
     rc, msg = _plan_rc(**kw)
@@ -122,3 +141,46 @@ def test_host_layer_operator_matches_oracle(absorbing):
     e = o.E_mat(fo)
     ref = o.dotmdm(e, np.exp(1j * kd * ao), o.inv(e))
     np.testing.assert_allclose(M, ref, atol=1e-11)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5])
+@pytest.mark.parametrize("case", [
+    dict(shape=(32, 48), wl=(450., 650.), beta=(0.,), phi=(0.,)),
+    dict(shape=(24, 40), wl=(550.,), beta=(0.12,), phi=(0.4,)),
+    dict(shape=(40, 32), wl=(500., 600., 700.), beta=(0., 0.2, 0.3), phi=(0., 1.1, -2.4)),
+])
+def test_host_windows_match_oracle(n, case):
+    """The plan's window list and mean beams (dtmm/mode.py fft_mask with the
+    beam offsets of propagate_4x4.py:401-402) against the oracle."""
+    lib = _lib.load()
+    ks = o.k0(np.asarray(case["wl"]), 100.)
+    beta = np.asarray(case["beta"], float)
+    phi = np.asarray(case["phi"], float)
+    nb = len(beta)
+    d = _desc(ks, beta, phi, shape=case["shape"], diffraction=n)
+    cnt = ctypes.c_int()
+    wi = np.zeros(n * n, np.int32)
+    wj = np.zeros(n * n, np.int32)
+    wb = np.zeros(n * n * nb)
+    wp = np.zeros(n * n * nb)
+    rc = lib.dtmmx_host_windows(ctypes.byref(d), n * n, ctypes.byref(cnt), _lib.iptr(wi),
+                                _lib.iptr(wj), _lib.dptr(wb), _lib.dptr(wp))
+    assert rc == 0, lib.dtmmx_last_error()
+    m = cnt.value
+    # oracle: per beam offset, as the reference broadcasts beta (rays, 1, 1, 1)
+    off_b = beta[:, None, None]
+    off_p = phi[:, None, None]
+    ks_b = np.asarray(ks)
+    windows, (betas, phis) = o.fft_mask(case["shape"], ks_b, n, off_b * np.cos(off_p),
+                                              off_b * np.sin(off_p))
+    full = o.fft_windows(*o.betaxy(case["shape"], ks_b[..., None]), n,
+                              off_b * np.cos(off_p), off_b * np.sin(off_p))
+    keep = [i for i in range(n * n) if not np.all(full[i] == 0.)]
+    assert m == len(keep)
+    assert [int(a) * n + int(b) for a, b in zip(wi[:m], wj[:m])] == keep
+    betas = betas.reshape(m, nb)
+    phis = phis.reshape(m, nb)
+    got_b = wb[:m * nb].reshape(m, nb)
+    got_p = wp[:m * nb].reshape(m, nb)
+    np.testing.assert_allclose(got_b * np.cos(got_p), betas * np.cos(phis), atol=1e-13)
+    np.testing.assert_allclose(got_b * np.sin(got_p), betas * np.sin(phis), atol=1e-13)

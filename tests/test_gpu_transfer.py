@@ -99,3 +99,42 @@ def test_config1_droplet_full_size(precision):
     flush("case config1 %s" % precision)
     fd, optical = droplet_case(shape=(60, 96, 96), radius=30, wavelengths=(550.,), jones=(1, 0))
     _compare(fd, optical, "4x4", precision)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("n", [3, 5])
+def test_diffraction_windows(n, precision):
+    """diffraction = n > 1: the field is split by the n x n k-space tent
+    windows, each propagated with its own mean beam, and summed."""
+    flush("case windows n=%d %s" % (n, precision))
+    fd, optical = random_case(shape=(6, 32, 48), wavelengths=(450., 650.), jones=None)
+    _compare(fd, optical, "4x4", precision, diffraction=n)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [2, 4])
+def test_diffraction_windows_multi_beam(n):
+    flush("case windows multi_beam n=%d" % n)
+    beta = [0., 0.2, 0.15]
+    phi = [0., 1.0, -2.0]
+    fd, optical = random_case(shape=(4, 32, 32), wavelengths=(500., 600.), jones=None,
+                              beta=beta, phi=phi)
+    _compare(fd, optical, "4x4", "c128", beta=beta, phi=phi, diffraction=n)
+
+
+@pytest.mark.gpu
+def test_diffraction_windows_nstep_offaxis():
+    """Sub-steps run inside each window (propagate_4x4.py:408-413)."""
+    flush("case windows nstep2 offaxis")
+    fd, optical = random_case(shape=(4, 32, 32), wavelengths=(550.,), jones=(1, 1j),
+                              beta=0.1, phi=0.3)
+    _compare(fd, optical, "4x4", "c128", beta=0.1, phi=0.3, nstep=2, diffraction=3)
+
+
+@pytest.mark.gpu
+def test_diffraction_windows_two_blocks():
+    flush("case windows two_blocks")
+    fd, b1 = random_case(shape=(3, 32, 32), wavelengths=(550.,), jones=(1, 0))
+    _, b2 = droplet_case(shape=(4, 32, 32), radius=12, ne=1.75, nhost=1.55)
+    _compare(fd, [b1[0], b2[0]], "4x4", "c128", nin=1.0, nout=1.33, diffraction=3)

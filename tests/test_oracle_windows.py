@@ -1,0 +1,117 @@
+"""Pinning of the oracle's k-space windows for diffraction > 1 (CPU only).
+
+dtmm/mode.py splits the k-space field by n x n tent windows in
+(beta_x, beta_y); the reference's tests hold no vector for them, so they
+are pinned by what the construction fixes:
+
+* the tents form a partition of unity inside betamax (every k-mode is
+  propagated exactly once);
+* the beam's own (beta_x, beta_y) sits on a tent centre (the offset is the
+  beam reduced modulo the spacing), so exactly one window holds it;
+* on a symmetric (odd) grid with a normal beam the centre window's mean
+  beam is normal incidence;
+* with a homogeneous isotropic slab equal to the effective medium every
+  window propagates its part exactly (its correction cancels its phase),
+  so the sum is exact angular-spectrum propagation;
+* a uniform plane wave through a homogeneous uniaxial slab only meets the
+  centre window and comes out as the Jones retarder.
+"""
+import numpy as np
+import pytest
+
+import oracle as o
+
+
+def _grid(shape, wl, ps=100.):
+    ks = o.k0(wl, ps)
+    bx, by = o.betaxy(shape, np.asarray(ks)[..., None])
+    return ks, bx, by
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5])
+@pytest.mark.parametrize("off", [(0., 0.), (0.13, -0.07), (-0.31, 0.22)])
+def test_windows_partition_of_unity(n, off):
+    ks, bx, by = _grid((16, 20), [450., 550., 650.])
+    w = o.fft_windows(bx, by, n, off[0], off[1])
+    inside = (bx ** 2 + by ** 2) ** 0.5 < o.BETAMAX
+    assert w.shape == (n * n,) + bx.shape
+    assert np.all(w >= 0.)
+    np.testing.assert_allclose(w.sum(0), inside.astype(float), atol=1e-14)
+
+
+@pytest.mark.parametrize("n", [3, 5, 7])
+@pytest.mark.parametrize("beam", [(0.13, 0.4), (0.31, -2.1), (0.05, 3.0), (0.45, 1.2)])
+def test_beam_sits_on_one_tent_centre(n, beam):
+    b, p = beam
+    ox, oy = b * np.cos(p), b * np.sin(p)
+    pt_x, pt_y = np.array([[ox]]), np.array([[oy]])
+    w = o.fft_windows(pt_x, pt_y, n, ox, oy)[:, 0, 0]
+    assert np.sum(np.abs(w - 1.) < 1e-12) == 1
+    assert np.sum(w > 1e-12) == 1
+
+
+@pytest.mark.parametrize("n", [3, 5])
+def test_centre_window_of_symmetric_grid_is_normal(n):
+    ks, bx, by = _grid((15, 15), [500., 600.])
+    windows, (betas, phis) = o.fft_mask((15, 15), ks, n)
+    centre = (n // 2) * n + n // 2
+    full = o.fft_windows(bx, by, n)
+    keep = [i for i in range(n * n) if not np.all(full[i] == 0.)]
+    c = keep.index(centre)
+    assert betas[c] < 1e-14
+    # mirrored windows (i, j) and (n-1-i, n-1-j) have the same |beta| and
+    # opposite directions
+    for a, idx in enumerate(keep):
+        i, j = divmod(idx, n)
+        m = keep.index((n - 1 - i) * n + (n - 1 - j))
+        np.testing.assert_allclose(betas[a], betas[m], atol=1e-14)
+        if betas[a] > 1e-12:
+            d = np.angle(np.exp(1j * (phis[a] - phis[m])))
+            np.testing.assert_allclose(abs(d), np.pi, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5])
+def test_windowed_isotropic_slab_is_angular_spectrum_propagation(n):
+    eps_n, L = 1.55, 5
+    shape = (24, 32)
+    wl = np.array([500., 700.])
+    ps = 100.
+    beta, phi = 0.1, 0.7
+    field, _, _ = o.illumination_data(shape, wl, pixelsize=ps, beta=beta, phi=phi, jones=(1, 1j),
+                                      window=o.blackman(shape), n=eps_n)
+    eps = np.broadcast_to([eps_n ** 2] * 3, (L, 1, 1, 3))
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                           beta=beta, phi=phi, method="4x4", nin=eps_n, nout=eps_n,
+                           diffraction=n)[0]
+    k = o.k0(wl, ps)
+    b, _ = o.betaphi(shape, k)
+    prop = np.exp(1j * k[:, None, None] * L * np.sqrt(eps_n ** 2 - b ** 2 + 0j))
+    prop[b >= o.BETAMAX] = 0.
+    expect = np.fft.ifft2(np.fft.fft2(field) * prop[:, None])
+    np.testing.assert_allclose(out, expect, atol=1e-12 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("n", [3, 5])
+def test_windowed_uniaxial_slab_is_jones_retarder(n):
+    no, ne, L, phid = 1.5, 1.62, 4, 0.6
+    shape = (9, 9)
+    wl = np.array([480., 620.])
+    ps = 100.
+    neff = np.sqrt((2 * no ** 2 + ne ** 2) / 3)
+    eps = np.broadcast_to([no ** 2, no ** 2, ne ** 2], (L, 1, 1, 3))
+    ang = np.broadcast_to([0., np.pi / 2, phid], (L, 1, 1, 3))
+    pol = (np.cos(0.3), np.sin(0.3))
+    H, W = shape
+    field = np.zeros((len(wl), 4, H, W), complex)
+    field[:, 0], field[:, 1] = pol[0], neff * pol[0]
+    field[:, 2], field[:, 3] = pol[1], -neff * pol[1]
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, ang)], beta=0., phi=0.,
+                           method="4x4", nin=neff, nout=neff, diffraction=n)[0]
+    k = o.k0(wl, ps)
+    c, s = np.cos(phid), np.sin(phid)
+    R = np.array([[c, -s], [s, c]])
+    for i in range(len(wl)):
+        J = R @ np.diag([np.exp(1j * k[i] * L * ne), np.exp(1j * k[i] * L * no)]) @ R.T
+        e = J @ np.array(pol)
+        np.testing.assert_allclose(out[i, 0], e[0], atol=1e-12)
+        np.testing.assert_allclose(out[i, 2], e[1], atol=1e-12)
