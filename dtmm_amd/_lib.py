@@ -57,6 +57,9 @@ class PlanDesc(ctypes.Structure):
         ("beta", _dp),
         ("phi", _dp),
         ("diffraction", ctypes.c_int),
+        ("npass", ctypes.c_int),
+        ("norm", ctypes.c_int),
+        ("smooth", ctypes.c_double),
     ]
 
 

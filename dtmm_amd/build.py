@@ -3,8 +3,10 @@
 Usage: ``python -m dtmm_amd.build`` (or ``dtmm_amd.build.build()``).
 """
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -18,7 +20,6 @@ FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
-    "-shared",
     "-fPIC",
     # never emit packed fp32 arithmetic (v_pk_*_f32)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -27,15 +28,35 @@ FLAGS = [
 ]
 
 
+# Translation units compiled in parallel (host + ABI, complex64, complex128).
+PARTS = (1, 2, 3)
+
+
 def build(verbose=True, defines=(), out=None):
     """Compile the library (always recompiles).  `defines` adds -D flags and
     `out` another output path (for build variants)."""
     out = out or OUT
-    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + [SRC, "-o", out + ".tmp"]
     t = time.time()
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    tmp = tempfile.mkdtemp(prefix="dtmmx_build_")
+    try:
+        objs, procs = [], []
+        for part in PARTS:
+            obj = os.path.join(tmp, "part%d.o" % part)
+            cmd = ([HIPCC] + FLAGS + ["-c", "-DDTMMX_PART=%d" % part] +
+                   ["-D" + d for d in defines] + [SRC, "-o", obj])
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append(subprocess.Popen(cmd))
+            objs.append(obj)
+        codes = [p.wait() for p in procs]
+        if any(codes):
+            raise RuntimeError("hipcc failed (exit codes %s)" % (codes,))
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", out + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
     os.replace(out + ".tmp", out)
     if verbose:
         print("built %s in %.1f s" % (out, time.time() - t), flush=True)

@@ -26,8 +26,22 @@
 
 #include "kernels.hpp"
 
+// The library is compiled as one translation unit (DTMMX_PART 0) or, to
+// build in parallel, as three: 1 = host code and the C ABI, 2 = the
+// complex64 kernels and driver, 3 = the complex128 ones.
+#ifndef DTMMX_PART
+#define DTMMX_PART 0
+#endif
+#define DTMMX_HOST_PART (DTMMX_PART == 0 || DTMMX_PART == 1)
+#define DTMMX_C64_PART (DTMMX_PART == 0 || DTMMX_PART == 2)
+#define DTMMX_C128_PART (DTMMX_PART == 0 || DTMMX_PART == 3)
+
 namespace dtmmx {
+#if DTMMX_HOST_PART
 thread_local std::string g_err;
+#else
+extern thread_local std::string g_err;
+#endif
 
 static int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -165,7 +179,11 @@ struct KTimer {
   std::vector<hipEvent_t> ev;
   std::vector<int> cls;
 };
-static KTimer g_timer;
+#if DTMMX_HOST_PART
+KTimer g_timer;
+#else
+extern KTimer g_timer;
+#endif
 
 static void timer_start(int cls, hipStream_t s) {
   g_timer.armed = false;
@@ -304,14 +322,55 @@ struct Step {
   cd e;
 };
 
-static std::vector<Step> expand_steps(const dtmmx_stack* S) {
+// dir = -1: the backward pass of npass > 1, layers in reverse order with
+// negative thickness (transfer.py:744-760).
+static std::vector<Step> expand_steps(const dtmmx_stack* S, int dir) {
   std::vector<Step> v;
-  for (int l = 0; l < S->n_layers; ++l) {
+  const int L = S->n_layers;
+  for (int k = 0; k < L; ++k) {
+    const int l = dir > 0 ? k : L - 1 - k;
     const int ns = S->n_steps ? S->n_steps[l] : 1;
     const cd e = mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
-    for (int k = 0; k < ns; ++k) v.push_back(Step{l, S->thickness[l] / double(ns), e});
+    for (int j = 0; j < ns; ++j) v.push_back(Step{l, (S->thickness[l] * dir) / double(ns), e});
   }
   return v;
+}
+
+// Workspace layout of a transfer call (byte offsets; 256-byte aligned).
+struct WsLayout {
+  size_t pass_field, pass_field0;  // npass > 1: propagated field, transmitted input
+  size_t win_tmp, win_acc;         // diffraction > 1: window part, window sum
+  size_t e2;                       // 2x2: the 2-component field
+  size_t part, sums;               // npass > 1: reduction partials and sums
+  size_t total;
+  int nblk;
+};
+
+static WsLayout ws_layout(const dtmmx_plan* P) {
+  WsLayout w{};
+  const size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const size_t planes = size_t(P->d.n_rays) * P->d.n_wavelengths;
+  const size_t f4 = planes * 4 * HW * cb;
+  auto take = [&](size_t bytes) {
+    const size_t o = w.total;
+    w.total += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  w.nblk = int((HW + NT - 1) / NT);
+  const bool m4 = P->d.method == DTMMX_METHOD_4X4;
+  if (m4 && P->d.npass > 1) {
+    w.pass_field = take(f4);
+    w.pass_field0 = take(f4);
+    w.part = take(2 * planes * size_t(w.nblk) * sizeof(double));
+    w.sums = take(4 * planes * sizeof(double));
+  }
+  if (m4 && P->wn > 1) {
+    w.win_tmp = take(f4);
+    w.win_acc = take(f4);
+  }
+  if (!m4) w.e2 = take(planes * 2 * HW * cb);
+  return w;
 }
 
 #define TRY(x)                     \
@@ -342,7 +401,7 @@ static int lam_batch(const dtmmx_plan* P, size_t bytes_per_lam) {
 // sub-steps with the window's mean beam, and summed into `acc`.
 template <typename T>
 static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, cpx<T>* tmp,
-                            cpx<T>* acc, Batch b, int* status, hipStream_t s) {
+                            cpx<T>* acc, Batch b, int* status, hipStream_t s, int dir) {
   const int H = P->d.height, W = P->d.width;
   const size_t HW = size_t(H) * W;
   const int L = S->n_layers;
@@ -358,16 +417,17 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
   KArgs<T> k{};
   k.kop = KOP_HEAD4;  // F -> Epn basis change only
   k.d0 = 0.;
-  k.e0 = eff(0);
+  k.e0 = eff(dir > 0 ? 0 : L - 1);
   k.scale = 1.0 / double(H);
   k.nocorr = 1;
   TRY(kspace<T>(P, A, A, b, k, s));
   TRY(col_pass<T>(P, A, b, 4, OP_INV, s));
   double prev_d = 0.;  // half step of the previous layer still to apply
-  cd prev_e = eff(0);
-  for (int l = 0; l < L; ++l) {
+  cd prev_e = eff(dir > 0 ? 0 : L - 1);
+  for (int kl = 0; kl < L; ++kl) {
+    const int l = dir > 0 ? kl : L - 1 - kl;
     const int ns = S->n_steps ? S->n_steps[l] : 1;
-    const double ds = S->thickness[l] / double(ns);
+    const double ds = (S->thickness[l] * dir) / double(ns);
     const cd el = eff(l);
     if (!eq(el, prev_e)) {
       TRY(col_pass<T>(P, A, b, 4, OP_FWD, s));
@@ -442,18 +502,19 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
   return DTMMX_OK;
 }
 
+// One pass through the stack in place on `out` (real-space field in and
+// out); dir = -1 runs the layers backward (npass > 1).
 template <typename T>
-static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
-                         void* ws, int* status, hipStream_t s) {
+static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, void* ws,
+                     int* status, hipStream_t s, int dir) {
   const int H = P->d.height, W = P->d.width;
   const size_t HW = size_t(H) * W;
   const int nrays = P->d.n_rays, nlam = P->d.n_wavelengths;
   const int nplanes = nrays * nlam;
   const double scale = 1.0 / double(HW);
   const bool m4 = P->d.method == DTMMX_METHOD_4X4;
-  cpx<T>* out = (cpx<T>*)fout;
-  HIP_TRY(hipMemcpyAsync(out, fin, size_t(nplanes) * 4 * HW * sizeof(cpx<T>),
-                         hipMemcpyDeviceToDevice, s));
+  const WsLayout wl = ws_layout(P);
+  char* wsb = (char*)ws;
   if (m4 && P->wn > 1) {
     if (S->n_layers == 0) return DTMMX_OK;  // ifft2(fft2(field_in))
     if (P->win_i.empty()) {  // no window passes betamax: every part is zero
@@ -463,15 +524,15 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
       HIP_TRY(hipGetLastError());
       return DTMMX_OK;
     }
-    cpx<T>* tmp = (cpx<T>*)ws;
-    cpx<T>* acc = tmp + size_t(nplanes) * 4 * HW;
+    cpx<T>* tmp = (cpx<T>*)(wsb + wl.win_tmp);
+    cpx<T>* acc = (cpx<T>*)(wsb + wl.win_acc);
     Batch b{nrays, nlam, 0, nlam};
-    return transfer_windows<T>(P, S, out, tmp, acc, b, status, s);
+    return transfer_windows<T>(P, S, out, tmp, acc, b, status, s, dir);
   }
-  const std::vector<Step> st = expand_steps(S);
+  const std::vector<Step> st = expand_steps(S, dir);
   const int L = int(st.size());
   // workspace: 2x2 method: the 2-component field
-  cpx<T>* e2 = (cpx<T>*)ws;
+  cpx<T>* e2 = (cpx<T>*)(wsb + wl.e2);
   const int nlb = lam_batch(P, size_t(nrays) * 4 * HW * sizeof(cpx<T>));
   for (int lam0 = 0; lam0 < nlam; lam0 += nlb) {
     Batch b{nrays, nlam, lam0, (lam0 + nlb <= nlam) ? nlb : nlam - lam0};
@@ -574,6 +635,149 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* 
 }
 
 template <typename T>
+static int fft2_planes(const dtmmx_plan* P, cpx<T>* f, int inverse, hipStream_t s) {
+  Batch b{P->d.n_rays, P->d.n_wavelengths, 0, P->d.n_wavelengths};
+  if (!inverse) {
+    TRY(row_pass<T>(P, f, b, 4, OP_FWD, s));
+    return col_pass<T>(P, f, b, 4, OP_FWD, s);
+  }
+  TRY(col_pass<T>(P, f, b, 4, OP_INV, s));
+  return row_pass<T>(P, f, b, 4, OP_INV, s);
+}
+
+// k-space normalisation step on the spectrum of a whole field (in place).
+template <typename T>
+static int pass_op(const dtmmx_plan* P, cpx<T>* f, PArgs<T> a, hipStream_t s) {
+  a.src = f;
+  a.dst = f;
+  a.H = P->d.height;
+  a.W = P->d.width;
+  a.nlam = P->d.n_wavelengths;
+  a.amax = P->amax;
+  a.bm = P->bm;
+  const size_t HW = size_t(a.H) * a.W;
+  dim3 grid(unsigned((HW + NT - 1) / NT), unsigned(P->d.n_rays * P->d.n_wavelengths));
+  hipLaunchKernelGGL((pass_kernel<T>), grid, dim3(NT), 0, s, a);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
+static int sum_rows(const double* part, int rows, int nb, double scale, double* sums,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(sum_rows_kernel, dim3(rows), dim3(NT), 0, s, part, nb, scale, sums);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
+// npass > 1 (dtmm/transfer.py:697-872, norm "fft" or "total", no reference
+// field): forward and backward passes with the output normalised onto the
+// forward modes of the output medium and the input-side field rescaled to
+// the incoming intensity between passes.
+template <typename T>
+static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin, cpx<T>* fout,
+                          void* ws, int* status, hipStream_t s) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const int nplanes = P->d.n_rays * P->d.n_wavelengths;
+  const size_t n = size_t(nplanes) * 4 * HW;
+  const size_t bytes = n * sizeof(cpx<T>);
+  const double scale = 1.0 / double(HW);
+  const WsLayout wl = ws_layout(P);
+  char* wsb = (char*)ws;
+  cpx<T>* g = (cpx<T>*)(wsb + wl.pass_field);
+  cpx<T>* f0 = (cpx<T>*)(wsb + wl.pass_field0);
+  double* part = (double*)(wsb + wl.part);
+  double* sums = (double*)(wsb + wl.sums);  // [i0 | unused | i0f or i1 | i2]
+  double* i0 = sums;
+  double* i0f = sums + 2 * nplanes;
+  const int npass = P->d.npass;
+  const double smooth = P->d.smooth < 0. ? 0. : P->d.smooth;
+  const cd ein = mk<double>(S->n_in * S->n_in), eout = mk<double>(S->n_out * S->n_out);
+  const unsigned eb = unsigned((n + NT - 1) / NT);
+  // field0 = transmitted_field(field_in, nin); i0 = its total intensity
+  HIP_TRY(hipMemcpyAsync(f0, fin, bytes, hipMemcpyDeviceToDevice, s));
+  TRY(fft2_planes<T>(P, f0, 0, s));
+  {
+    PArgs<T> a{};
+    a.pop = POP_PROJ;
+    a.eps = ein;
+    a.scale = scale;
+    a.part = part;
+    TRY(pass_op<T>(P, f0, a, s));
+    TRY(sum_rows(part, nplanes, wl.nblk, scale, i0, s));
+  }
+  TRY(fft2_planes<T>(P, f0, 1, s));
+  HIP_TRY(hipMemcpyAsync(g, fin, bytes, hipMemcpyDeviceToDevice, s));
+  for (int i = 0; i < npass; ++i) {
+    TRY(propagate<T>(P, S, g, ws, status, s, (i % 2 == 0) ? 1 : -1));
+    const double sigma = smooth * double(npass - i) / double(npass);
+    if (i % 2 == 0) {
+      if (i != npass - 1) {
+        TRY(fft2_planes<T>(P, g, 0, s));
+        if (P->d.norm & DTMMX_NORM_FFT) {
+          PArgs<T> a{};
+          a.pop = POP_NORMFFT;
+          a.eps = eout;
+          a.notch = 1;
+          a.x0 = S->n_in;
+          a.sigma = sigma;
+          a.scale = scale;
+          TRY(pass_op<T>(P, g, a, s));
+        } else {
+          PArgs<T> a{};
+          a.pop = POP_TOTALSUM;
+          a.eps = eout;
+          a.part = part;
+          TRY(pass_op<T>(P, g, a, s));
+          TRY(sum_rows(part, 2 * nplanes, wl.nblk, scale, i0f, s));
+          a.pop = POP_TOTALAPPLY;
+          a.part = nullptr;
+          a.sums = i0f;
+          a.notch = 1;
+          a.x0 = S->n_in;
+          a.sigma = sigma;
+          a.scale = scale;
+          TRY(pass_op<T>(P, g, a, s));
+        }
+        TRY(fft2_planes<T>(P, g, 1, s));
+      }
+      hipLaunchKernelGGL((accumulate_kernel<T>), dim3(eb), dim3(NT), 0, s, g, fout,
+                         i == 0 ? 1 : 0, n);
+      HIP_TRY(hipGetLastError());
+    } else {
+      HIP_TRY(hipMemcpyAsync(fin, g, bytes, hipMemcpyDeviceToDevice, s));
+      if (i != npass - 1) {
+        TRY(fft2_planes<T>(P, g, 0, s));
+        PArgs<T> a{};
+        a.pop = POP_PROJ;
+        a.eps = ein;
+        a.notch = 1;
+        a.x0 = S->n_in;
+        a.sigma = sigma;
+        a.scale = scale;
+        a.part = part;
+        TRY(pass_op<T>(P, g, a, s));
+        TRY(sum_rows(part, nplanes, wl.nblk, scale, i0f, s));
+        TRY(fft2_planes<T>(P, g, 1, s));
+        hipLaunchKernelGGL((reflect_update_kernel<T>), dim3(eb), dim3(NT), 0, s, g, fout, fin,
+                           (const cpx<T>*)f0, (const double*)i0, (const double*)i0f, 4 * HW, n);
+        HIP_TRY(hipGetLastError());
+      }
+    }
+  }
+  return DTMMX_OK;
+}
+
+template <typename T>
+static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
+                         void* ws, int* status, hipStream_t s) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
+  if (P->d.npass > 1) return transfer_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s);
+  HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+  return propagate<T>(P, S, (cpx<T>*)fout, ws, status, s, 1);
+}
+
+template <typename T>
 static int fft2_impl(void* data, int batch, int H, int W, int inverse, hipStream_t s) {
   dtmmx_plan P{};
   P.d.height = H;
@@ -596,6 +800,34 @@ static int fft2_impl(void* data, int batch, int H, int W, int inverse, hipStream
   return DTMMX_OK;
 }
 
+// Precision-specific entry points (defined in the part that builds them).
+int transfer_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
+                 int* status, hipStream_t s);
+int transfer_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
+                  int* status, hipStream_t s);
+int fft2_c64(void* data, int batch, int H, int W, int inverse, hipStream_t s);
+int fft2_c128(void* data, int batch, int H, int W, int inverse, hipStream_t s);
+
+#if DTMMX_C64_PART
+int transfer_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
+                 int* status, hipStream_t s) {
+  return transfer_impl<float>(P, S, fin, fout, ws, status, s);
+}
+int fft2_c64(void* data, int batch, int H, int W, int inverse, hipStream_t s) {
+  return fft2_impl<float>(data, batch, H, W, inverse, s);
+}
+#endif
+#if DTMMX_C128_PART
+int transfer_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
+                  int* status, hipStream_t s) {
+  return transfer_impl<double>(P, S, fin, fout, ws, status, s);
+}
+int fft2_c128(void* data, int batch, int H, int W, int inverse, hipStream_t s) {
+  return fft2_impl<double>(data, batch, H, W, inverse, s);
+}
+#endif
+
+#if DTMMX_HOST_PART
 // numpy.mod of two doubles: the remainder with the sign of b.
 static double np_mod(double a, double b) {
   double m = std::fmod(a, b);
@@ -776,9 +1008,10 @@ static int upload_windows(dtmmx_plan* P, const std::vector<int>& amax) {
   return DTMMX_OK;
 }
 
+#endif  // DTMMX_HOST_PART
 }  // namespace dtmmx
 
-
+#if DTMMX_HOST_PART
 extern "C" {
 
 int dtmmx_abi_version(void) { return DTMMX_ABI_VERSION; }
@@ -805,6 +1038,11 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->diffraction > 1 && d->method != DTMMX_METHOD_4X4)
     return fail(DTMMX_E_UNSUPPORTED, "diffraction > 1 is built for the 4x4 method only");
   if (d->diffraction > 64) return fail(DTMMX_E_ARG, "diffraction above 64");
+  if (d->npass < 1) return fail(DTMMX_E_ARG, "npass must be >= 1");
+  if (d->npass > 1 && d->method != DTMMX_METHOD_4X4)
+    return fail(DTMMX_E_UNSUPPORTED, "npass > 1 is built for the 4x4 method only");
+  if (d->npass > 1 && d->norm != 0 && d->norm != DTMMX_NORM_FFT)
+    return fail(DTMMX_E_UNSUPPORTED, "npass > 1: only norm 0 (total) and 1 (fft) are built");
   return DTMMX_OK;
 }
 
@@ -882,15 +1120,10 @@ int dtmmx_plan_destroy(dtmmx_plan* P) {
 
 size_t dtmmx_plan_workspace_size(const dtmmx_plan* P) {
   if (!P) return 0;
-  size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
-  size_t HW = size_t(P->d.height) * P->d.width;
-  size_t planes = size_t(P->d.n_rays) * P->d.n_wavelengths;
-  if (P->d.method == DTMMX_METHOD_2X2) return planes * 2 * HW * cb;
-  if (P->wn > 1) return 2 * planes * 4 * HW * cb;  // window part and the sum
-  return 0;
+  return ws_layout(P).total;
 }
 
-int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
+int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
                    void* ws, size_t ws_bytes, int* status, void* stream) {
   if (!P || !S || !fin || !fout) return fail(DTMMX_E_ARG, "null argument");
   if (S->n_layers < 0) return fail(DTMMX_E_ARG, "negative layer count");
@@ -903,8 +1136,8 @@ int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, v
     return fail(DTMMX_E_ARG, "workspace too small");
   if (fin == fout) return fail(DTMMX_E_ARG, "field_in and field_out must not alias");
   hipStream_t s = (hipStream_t)stream;
-  if (P->d.precision == DTMMX_C64) return transfer_impl<float>(P, S, fin, fout, ws, status, s);
-  return transfer_impl<double>(P, S, fin, fout, ws, status, s);
+  if (P->d.precision == DTMMX_C64) return transfer_c64(P, S, fin, fout, ws, status, s);
+  return transfer_c128(P, S, fin, fout, ws, status, s);
 }
 
 int dtmmx_timing_begin(int capacity, int stride) {
@@ -957,8 +1190,8 @@ int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int pr
   if (!size_supported(height) || !size_supported(width))
     return fail(DTMMX_E_UNSUPPORTED, "fft size not built");
   hipStream_t s = (hipStream_t)stream;
-  if (precision == DTMMX_C64) return fft2_impl<float>(data, batch, height, width, inverse, s);
-  if (precision == DTMMX_C128) return fft2_impl<double>(data, batch, height, width, inverse, s);
+  if (precision == DTMMX_C64) return fft2_c64(data, batch, height, width, inverse, s);
+  if (precision == DTMMX_C128) return fft2_c128(data, batch, height, width, inverse, s);
   return fail(DTMMX_E_ARG, "bad precision");
 }
 
@@ -1043,3 +1276,4 @@ int dtmmx_host_rop(int n, const double* beta, const double* phi, const double* e
 }
 
 }  // extern "C"
+#endif  // DTMMX_HOST_PART

@@ -630,6 +630,183 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
   for (int i = 0; i < nout; ++i) o[i * HW] = cast<T>(r[i] * A.scale);
 }
 
+// ---------------------------------------------------------------------------
+// Pass normalisation kernels (npass > 1, dtmm/transfer.py:815-872): one
+// thread per (k, plane) on the unnormalised 2D spectrum of a whole field.
+// ---------------------------------------------------------------------------
+
+enum { POP_PROJ = 0, POP_NORMFFT = 1, POP_TOTALSUM = 2, POP_TOTALAPPLY = 3 };
+
+template <typename T>
+struct PArgs {
+  const cpx<T>* src;
+  cpx<T>* dst;
+  int H, W;
+  int nlam;
+  const int* amax;
+  int pop;
+  cd eps;             // isotropic medium of the forward projection
+  int notch;          // 1: multiply by the notch filter of dtmm/denoise.py
+  double x0, sigma;   // notch centre (beta) and width
+  double scale;       // applied to the written values
+  double* part;       // per-block partial sums [2][nplanes][gridDim.x]
+  const double* sums; // per-plane sums [2][nplanes] (POP_TOTALAPPLY)
+  Beams bm;
+};
+
+// Poynting z of one (Ex, Hy, Ey, Hx) sample (dtmm/field.py:771-799).
+__host__ __device__ inline double intensity4(const cd v[4]) {
+  return (v[0].x * v[1].x + v[0].y * v[1].y) - (v[2].x * v[3].x + v[2].y * v[3].y);
+}
+
+// Forward-mode projection F diag(1, 0, 1, 0) F^-1 of an isotropic medium at
+// in-plane wave vector beta (direction (c, sn)) in closed form
+// (dtmm/diffract.py:242-251): rotate into the mode frame, keep the forward
+// p and s amplitudes, rebuild.  Zero where a = sqrt(eps - beta^2) == 0.
+__host__ __device__ inline void iso_forward_projection(double beta, double c, double sn, cd eps,
+                                                       const cd v[4], cd r[4]) {
+  const cd a = csqrt(eps - mk<double>(beta * beta));
+  if (is_zero(a)) {
+    for (int i = 0; i < 4; ++i) r[i] = mk<double>(0.);
+    return;
+  }
+  const cd g0 = v[0] * c + v[2] * sn, g1 = v[1] * c - v[3] * sn;
+  const cd g2 = v[2] * c - v[0] * sn, g3 = v[1] * sn + v[3] * c;
+  const cd mp = (g0 + g1 * cdiv(a, eps)) * 0.5;  // forward p amplitude / 2
+  const cd ms = (g2 - cdiv(g3, a)) * 0.5;        // forward s amplitude / 2
+  const cd h0 = mp, h1 = cdiv(eps, a) * mp, h2 = ms, h3 = -(a * ms);
+  r[0] = h0 * c - h2 * sn;
+  r[1] = h1 * c + h3 * sn;
+  r[2] = h0 * sn + h2 * c;
+  r[3] = h3 * c - h1 * sn;
+}
+
+// Notch filter around beta = x0 (dtmm/denoise.py:21-22).
+__device__ inline double notch_filter(double b, double x0, double sigma) {
+  double d = fabs(b - x0);
+  d = d < 0. ? 0. : (d > x0 ? x0 : d);
+  return (1. - exp(-d / sigma)) / (1. - exp(-x0 / sigma));
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void pass_kernel(PArgs<T> A) {
+  __shared__ double red[2][NT];
+  const int tid = threadIdx.x;
+  const int k = blockIdx.x * NT + tid;
+  const int q = blockIdx.y;
+  const int nplanes = gridDim.y;
+  const int lam = q % A.nlam;
+  const size_t HW = size_t(A.H) * A.W;
+  double s0 = 0., s1 = 0.;
+  if (k < int(HW)) {
+    const int ky = k / A.W, kx = k - ky * A.W;
+    const double k0 = A.bm.k0[lam];
+    const bool pass = mode_index(kx, A.W) <= A.amax[lam * A.H + ky];
+    const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
+    const double fr = sqrt(fx * fx + fy * fy);
+    const double beta = (2 * M_PI / fabs(k0)) * fr;
+    const double c = fr > 0. ? fx / fr : 1., sn = fr > 0. ? fy / fr : 0.;
+    const cpx<T>* src = A.src + size_t(q) * 4 * HW + k;
+    cd v[4], p[4];
+    for (int i = 0; i < 4; ++i) v[i] = cast<double>(src[i * HW]);
+    if (pass) {
+      iso_forward_projection(beta, c, sn, A.eps, v, p);
+    } else {
+      for (int i = 0; i < 4; ++i) p[i] = mk<double>(0.);
+    }
+    double f = 1.;
+    if (A.notch) f = notch_filter(beta, A.x0, A.sigma);
+    if (A.pop == POP_TOTALSUM) {
+      s0 = intensity4(v);
+      s1 = intensity4(p);
+    } else {
+      double g = 1.;
+      if (A.pop == POP_NORMFFT) {
+        double i1 = intensity4(v), i2 = intensity4(p);
+        if (i2 == 0.) {
+          i2 = 1.;
+          i1 = 0.;
+        }
+        g = fabs(fmin(fmax(i1 / i2, -1.), 1.));
+      } else if (A.pop == POP_TOTALAPPLY) {
+        double i1 = A.sums[q], i2 = A.sums[nplanes + q];
+        if (i2 == 0.) {
+          i2 = 1.;
+          i1 = 0.;
+        }
+        g = fabs(fmin(fmax(i1 / i2, -1.), 1.));
+      }
+      for (int i = 0; i < 4; ++i) p[i] = p[i] * (g * f);
+      s0 = intensity4(p);
+      cpx<T>* dst = A.dst + size_t(q) * 4 * HW + k;
+      for (int i = 0; i < 4; ++i) dst[i * HW] = cast<T>(p[i] * A.scale);
+    }
+  }
+  if (A.part) {
+    red[0][tid] = s0;
+    red[1][tid] = s1;
+    __syncthreads();
+    for (int m = NT / 2; m > 0; m >>= 1) {
+      if (tid < m) {
+        red[0][tid] += red[0][tid + m];
+        red[1][tid] += red[1][tid + m];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const size_t nb = gridDim.x;
+      A.part[size_t(q) * nb + blockIdx.x] = red[0][0];
+      A.part[(size_t(nplanes) + q) * nb + blockIdx.x] = red[1][0];
+    }
+  }
+}
+
+// sums[j] = scale * sum_b part[j][b] for 2 * nplanes rows (one block each).
+static __global__ __launch_bounds__(NT) void sum_rows_kernel(const double* part, int nb, double scale,
+                                                       double* sums) {
+  __shared__ double red[NT];
+  const int r = blockIdx.x, tid = threadIdx.x;
+  double a = 0.;
+  for (int b = tid; b < nb; b += NT) a += part[size_t(r) * nb + b];
+  red[tid] = a;
+  __syncthreads();
+  for (int m = NT / 2; m > 0; m >>= 1) {
+    if (tid < m) red[tid] += red[tid + m];
+    __syncthreads();
+  }
+  if (tid == 0) sums[r] = red[0] * scale;
+}
+
+// Odd-pass update (dtmm/transfer.py:855-866) with fact = i0 / i0f per plane:
+// out *= fact; fin = fact * fin + (f0 - fact * g); g = f0 - fact * g.
+template <typename T>
+__global__ __launch_bounds__(NT) void reflect_update_kernel(cpx<T>* g, cpx<T>* fout, cpx<T>* fin,
+                                                             const cpx<T>* f0, const double* i0,
+                                                             const double* i0f, size_t per_plane,
+                                                             size_t n) {
+  const size_t i = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (i >= n) return;
+  const size_t q = i / per_plane;
+  const double fact = i0[q] / i0f[q];
+  const cd gi = cast<double>(g[i]) * fact;
+  fout[i] = cast<T>(cast<double>(fout[i]) * fact);
+  const cd d = cast<double>(f0[i]) - gi;
+  fin[i] = cast<T>(cast<double>(fin[i]) * fact + d);
+  g[i] = cast<T>(d);
+}
+
+// Even-pass accumulation (dtmm/transfer.py:838-840): out += g; g = out
+// (first: out = g).
+template <typename T>
+__global__ __launch_bounds__(NT) void accumulate_kernel(cpx<T>* g, cpx<T>* fout, int first,
+                                                         size_t n) {
+  const size_t i = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (i >= n) return;
+  const cpx<T> v = first ? g[i] : fout[i] + g[i];
+  fout[i] = v;
+  g[i] = v;
+}
+
 // Zero n values of type T.
 template <typename T>
 __global__ __launch_bounds__(NT) void zero_kernel(T* p, size_t n) {

@@ -3,8 +3,9 @@
 Mirrors dtmm/transfer.py:322 ``transfer_field`` (same arguments, defaults,
 return value and errors) for the modes built on the GPU:
 
-* method "4x4": reflection mode 1, diffraction >= 1, npass 1
-  (dtmm/transfer.py:618-884, dtmm/propagate_4x4.py:376-444, :80-101);
+* method "4x4": reflection mode 1, diffraction >= 1, npass >= 1 (norm
+  DTMM_NORM_FFT or 0) (dtmm/transfer.py:618-884, dtmm/propagate_4x4.py:376-444,
+  :80-101);
 * method "2x2": reflection mode 0, diffraction 1, npass 1
   (dtmm/transfer.py:1009-1183, dtmm/propagate_2x2.py:193-211, :19-87).
 
@@ -161,7 +162,7 @@ class Plan:
     """Owner of a ``dtmmx_plan``: grid, wavelengths, beams, method, precision."""
 
     def __init__(self, shape, ks, beta, phi, n_rays, rays_per_beam, method, precision="c64",
-                 betamax=BETAMAX, diffraction=1):
+                 betamax=BETAMAX, diffraction=1, npass=1, norm=DTMM_NORM_FFT, smooth=SMOOTH):
         lib = _lib.load()
         self.shape = tuple(shape)
         self.ks = np.ascontiguousarray(ks, np.float64)
@@ -172,6 +173,7 @@ class Plan:
         self.method = method
         self.precision = precision
         self.diffraction = int(diffraction)
+        self.npass = int(npass)
         d = _lib.PlanDesc()
         d.height, d.width = self.shape
         d.n_wavelengths = len(self.ks)
@@ -184,6 +186,9 @@ class Plan:
         d.beta = _lib.dptr(self.beta)
         d.phi = _lib.dptr(self.phi)
         d.diffraction = self.diffraction
+        d.npass = self.npass
+        d.norm = int(norm)
+        d.smooth = float(smooth)
         h = ctypes.c_void_p()
         _lib.check(lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)))
         self._h = h
@@ -197,7 +202,9 @@ class Plan:
 
     def execute(self, stack, field_in, field_out, workspace=None, status=None, stream=None):
         """Run the transfer on device tensors of shape (n_rays, n_wavelengths,
-        4, H, W); asynchronous on `stream` (default: torch's current)."""
+        4, H, W); asynchronous on `stream` (default: torch's current).  With
+        npass > 1 `field_in` receives the field at the input side (input plus
+        reflection), as the reference writes it into its input array."""
         torch = _torch()
         if stream is None:
             stream = torch.cuda.current_stream(field_in.device)
@@ -228,7 +235,7 @@ class Plan:
             pass
 
 
-def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk):
+def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, norm=DTMM_NORM_FFT):
     """Defaults and reflection auto-selection (dtmm/transfer.py:433-465) and
     the envelope of modes built on the GPU."""
     if npass == -1 or npass == np.inf:
@@ -249,8 +256,13 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk):
         raise ValueError("Invalid reflection. The 4x4 method is either reflection mode 1 or 2.")
     if kind == "2x2" and reflection not in (0, 1, 2):
         raise ValueError("Invalid reflection. The 2x2 method supports reflection mode 0,1 or 2.")
-    if npass != 1:
-        raise NotImplementedError("npass > 1 (reflection iterations) is not supported yet")
+    if npass < 1:
+        raise ValueError("npass must be >= 1")
+    if npass > 1 and kind == "2x2":
+        raise NotImplementedError("npass > 1 is supported for the 4x4 method only")
+    if npass > 1 and (norm & DTMM_NORM_REF):
+        raise NotImplementedError("npass > 1 with DTMM_NORM_REF (reference-field "
+                                  "normalisation) is not supported")
     if ret_bulk:
         raise NotImplementedError("ret_bulk is not supported yet")
     if eff_data not in (None, 0, "isotropic"):
@@ -291,7 +303,8 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
     npass = 1 if npass is None else npass
     diffraction = 1 if diffraction is None else diffraction
     betamax = BETAMAX if betamax is None else betamax
-    kind, reflection = _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk)
+    kind, reflection = _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk,
+                                      norm)
 
     field_in, wavelengths, pixelsize = field_data
     if isinstance(field_in, tuple):
@@ -317,7 +330,7 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
     rays_per_beam = n_rays // len(b)
 
     plan = Plan(shape, ks, b, p, n_rays, rays_per_beam, kind, precision, betamax,
-                diffraction=int(diffraction))
+                diffraction=int(diffraction), npass=int(npass), norm=norm, smooth=smooth)
     stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout, split_layers=split_layers)
     dt = plan.dtype
     fin = torch.from_numpy(np.ascontiguousarray(field_np.reshape((n_rays, len(ks), 4) + shape)))
@@ -329,6 +342,10 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
     if int(status[0].item()) & 1:
         raise NotImplementedError("biaxial voxels (eps0 != eps1) are not supported yet")
     res = fout.cpu().numpy().reshape(field_np.shape)
+    if npass > 1 and isinstance(field_in, np.ndarray) and field_in.flags.writeable:
+        # the reference leaves the input-side field (with reflections) in
+        # the caller's input array (dtmm/transfer.py:843, :862-866)
+        field_in[...] = fin.cpu().numpy().reshape(field_np.shape)
     if out is not None:
         out[...] = res
         res = out

@@ -50,6 +50,7 @@ enum {
 enum { DTMMX_C64 = 0, DTMMX_C128 = 1 };
 enum { DTMMX_METHOD_2X2 = 2, DTMMX_METHOD_4X4 = 4 };
 
+#define DTMMX_NORM_FFT 1
 #define DTMMX_MAX_WAVELENGTHS 64
 #define DTMMX_MAX_BEAMS 64
 
@@ -72,6 +73,14 @@ typedef struct {
                          the non-zero n x n tent windows of dtmm/mode.py,
                          each propagated with its window's mean beam
                          (propagate_4x4.py:390-419)                      */
+  int npass;          /* >= 1; > 1 (4x4 only): alternating forward and
+                         backward passes with the normalisations of
+                         transfer.py:815-872 (reflections)                */
+  int norm;           /* npass > 1: DTMMX_NORM_FFT (1, per-mode rescaling,
+                         transfer.py:78-91) or 0 (total intensity,
+                         transfer.py:206-221); DTMM_NORM_REF is not built */
+  double smooth;      /* npass > 1: notch width of dtmm/denoise.py (dtmm.ini:8,
+                         0.1); negative values count as 0                 */
 } dtmmx_plan_desc;
 
 /* A layer stack.  Layer l is propagated as n_steps[l] sub-steps of
@@ -102,16 +111,21 @@ int dtmmx_plan_destroy(dtmmx_plan* plan);
 /* Device scratch bytes one transfer call on this plan needs. */
 size_t dtmmx_plan_workspace_size(const dtmmx_plan* plan);
 
-/* Full transfer through the stack, npass = 1.
- *   4x4 (reflection mode 1, diffraction >= 1): replaces transfer.py:694-875
- *       with propagate_4x4.py:376-444 and :80-101 per layer.
- *   2x2 (reflection mode 0, diffraction 1): replaces transfer.py:1071-1176
- *       with propagate_2x2.py:193-211 and :19-87 per interface.
+/* Full transfer through the stack.
+ *   4x4 (reflection mode 1, diffraction >= 1, npass >= 1): replaces
+ *       transfer.py:694-875 with propagate_4x4.py:376-444 and :80-101 per
+ *       layer.
+ *   2x2 (reflection mode 0, diffraction 1, npass 1): replaces
+ *       transfer.py:1071-1176 with propagate_2x2.py:193-211 and :19-87 per
+ *       interface.
  * field_in and field_out are device buffers of n_rays*n_wavelengths*4*H*W
- * complex values; they may not alias.  field_in is not modified.
+ * complex values; they may not alias.  With npass == 1 field_in is not
+ * modified; with npass > 1 it receives the field at the input side
+ * (input plus reflection), as the reference writes it into the caller's
+ * array (transfer.py:843, :862-866).
  * status (device int[4], may be NULL) receives flags: status[0] bit 0 =
  * a biaxial voxel was met (not built: its output is NaN). */
-int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, const void* field_in,
+int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field_in,
                    void* field_out, void* workspace, size_t workspace_bytes, int* status,
                    void* stream);
 

@@ -9,8 +9,9 @@ Scope (what the GPU engine implements and this module checks):
 * wave-vector grids ``betaphi``/``betaxy``/``k0``          (dtmm/wave.py)
 * per-pixel Berreman eigen-solution ``alphaf``             (dtmm/tmm.py, rotation.py)
 * isotropic-medium k-space diffraction/correction matrices (dtmm/diffract.py, matrix.py)
-* ``transfer_4x4`` with reflection=1, diffraction >= 1, npass=1 (dtmm/transfer.py,
-  dtmm/propagate_4x4.py); diffraction > 1 uses the tent windows of dtmm/mode.py
+* ``transfer_4x4`` with reflection=1, diffraction >= 1, npass >= 1 (dtmm/transfer.py,
+  dtmm/propagate_4x4.py); diffraction > 1 uses the tent windows of dtmm/mode.py;
+  npass > 1 the normalisations of transfer.py and dtmm/denoise.py
 * ``transfer_2x2`` with reflection=0, diffraction in {0, 1}, npass=1
   (dtmm/transfer.py, dtmm/propagate_2x2.py)
 * ``illumination_data`` and ``field2intensity``            (dtmm/field.py)
@@ -36,6 +37,8 @@ __all__ = [
     "transfer_field", "illumination_data", "blackman", "planewave",
     "jonesvec", "avec", "fvec", "auxiliary_matrix", "poynting", "set_fft_workers",
     "fft_window", "fft_windows", "fft_betaxy_mean", "fft_mask",
+    "DTMM_NORM_FFT", "DTMM_NORM_REF", "exp_notch_filter", "denoise_fftfield", "denoise_field",
+    "normalize_field", "project_normalized_fft", "project_normalized_total", "total_intensity",
 ]
 
 # dtmm/dtmm.ini:6 (betamax = 0.8) and dtmm/conf.py:148, :503-510 (nin/nout
@@ -1001,15 +1004,95 @@ def _validate_betaphi(beta, phi, extendeddim=0):
     return beta, phi
 
 
+# norm flags (dtmm/transfer.py:30-31)
+DTMM_NORM_FFT = 1 << 0
+DTMM_NORM_REF = 1 << 1
+
+
+def exp_notch_filter(x, x0, sigma):
+    """(dtmm/denoise.py:21-22)."""
+    return np.asarray((1 - np.exp(-np.abs(x - x0).clip(0, x0) / sigma)) /
+                      (1 - np.exp(-x0 / sigma)), FDTYPE)
+
+
+def denoise_fftfield(ffield, ks, beta, smooth=1.):
+    """Attenuates the k-modes around |k| = beta (dtmm/denoise.py:30-39)."""
+    b, _ = betaphi(ffield.shape[-2:], ks)
+    f = exp_notch_filter(b, beta, smooth)
+    return ffield * f[..., None, :, :]
+
+
+def denoise_field(field, ks, beta, smooth=1.):
+    """(dtmm/denoise.py:24-28)."""
+    return ifft2(denoise_fftfield(fft2(field), ks, beta, smooth))
+
+
+def normalize_field(field, intensity_in, intensity_out):
+    """Per-pixel amplitude factor |clip(I_in / I_out, -1, 1)|, 0 where
+    I_out == 0 (dtmm/transfer.py:67-75)."""
+    i_out = np.array(intensity_out, copy=True)
+    i_in = np.array(intensity_in, copy=True)
+    m = i_out == 0.
+    i_out[m] = 1.
+    i_in[m] = 0.
+    fact = (i_in / i_out)[..., None, :, :]
+    return field * np.abs(np.clip(fact, -1, 1))
+
+
+def project_normalized_fft(field, dmat):
+    """Projection in k-space, each mode rescaled to the intensity it had
+    before (dtmm/transfer.py:78-91, ref=None, window=None)."""
+    f1 = fft2(field)
+    i1 = field2intensity(f1)
+    f2 = dotmf(dmat, f1)
+    i2 = field2intensity(f2)
+    return ifft2(normalize_field(f2, i1, i2))
+
+
+def total_intensity(field):
+    """Intensity summed over pixels (dtmm/transfer.py:200-204)."""
+    i = field2intensity(field)
+    return i.sum(axis=(-2, -1))
+
+
+def normalize_field_total(field, i1, i2):
+    """(dtmm/transfer.py:189-198)."""
+    i2 = np.array(i2, copy=True)
+    i1 = np.array(i1, copy=True)
+    m = i2 == 0.
+    i2[m] = 1.
+    i1[m] = 0.
+    fact = np.abs(np.clip(i1 / i2, -1, 1))
+    return field * fact[..., None, None, None]
+
+
+def project_normalized_total(field, dmat):
+    """(dtmm/transfer.py:206-221, ref=None, window=None)."""
+    i1 = total_intensity(field)
+    out = ifft2(dotmf(dmat, fft2(field)))
+    i2 = total_intensity(out)
+    return normalize_field_total(out, i1, i2)
+
+
 def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1,
-                 nstep=1, diffraction=1, reflection=1, betamax=BETAMAX, layer_callback=None):
-    """4x4 transfer, npass=1, reflection mode 1 (dtmm/transfer.py:618-884).
+                 nstep=1, diffraction=1, reflection=1, betamax=BETAMAX, norm=DTMM_NORM_FFT,
+                 smooth=SMOOTH, layer_callback=None):
+    """4x4 transfer, reflection mode 1 (dtmm/transfer.py:618-884).
 
     The input is not projected onto forward modes and the input/output media
-    are not propagated (transfer.py:694, :700).
+    are not propagated (transfer.py:694, :700).  npass > 1 alternates
+    forward and backward passes (reversed layers, negative thickness,
+    transfer.py:744-804) with the normalisations of transfer.py:815-872;
+    like the reference, the input array of field_data (when complex128)
+    then receives the field at the input side, reflections included.
     """
-    if npass != 1 or reflection != 1:
-        raise NotImplementedError("oracle covers npass=1, reflection=1 for the 4x4 method")
+    if reflection != 1:
+        raise NotImplementedError("oracle covers reflection=1 for the 4x4 method")
+    if norm & DTMM_NORM_REF:
+        raise NotImplementedError("oracle covers norm without DTMM_NORM_REF")
+    norm = "fft" if norm & DTMM_NORM_FFT else "total"
+    if smooth < 0:
+        smooth = 0.
     field_in, wavelengths, pixelsize = field_data
     field_in = np.asarray(field_in, CDTYPE)
     layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
@@ -1017,15 +1100,56 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     n = len(layers)
     beta, phi = _validate_betaphi(beta, phi, extendeddim=field_in.ndim - 2)
     cache = _Cache()
-    field = fft2(field_in)
-    for j in range(1, n - 1):
-        nsub, layer = layers[j]
-        _, eff = eff_layers[j]
-        field = propagate_4x4_effective_1(field, ks, layer, eff, beta, phi, nsub,
-                                          diffraction, betamax, cache)
-        if layer_callback is not None:
-            layer_callback(j)
-    return ifft2(field), wavelengths, pixelsize
+    field_out = np.zeros_like(field_in)
+    indices = list(range(1, n - 1))
+    field0 = transmitted_field(field_in, ks, n=nin, betamax=betamax)
+    i0 = total_intensity(field0)
+    field = fft2(field_in.copy())
+    if npass > 1:
+        pin_mat = projection_matrix(field.shape[-2:], ks, epsv=refind2eps([nin] * 3), mode=+1,
+                                    betamax=betamax)
+        pout_mat = projection_matrix(field.shape[-2:], ks, epsv=refind2eps([nout] * 3), mode=+1,
+                                     betamax=betamax)
+    for i in range(npass):
+        direction = (-1) ** i
+        for j in indices:
+            nsub, (thickness, ev, ea) = layers[j]
+            _, (d, e, a) = eff_layers[j]
+            field = propagate_4x4_effective_1(field, ks, (thickness * direction, ev, ea),
+                                              (d * direction, e, a), beta, phi, nsub,
+                                              diffraction, betamax, cache)
+            if layer_callback is not None:
+                layer_callback(j)
+        indices.reverse()
+        field = ifft2(field)
+        if npass == 1:
+            field_out[...] = field
+            break
+        sigma = smooth * (npass - i) / npass
+        if i % 2 == 0:
+            if i == 0:
+                field = field - field_out
+            if i != npass - 1:
+                if norm == "fft":
+                    field = project_normalized_fft(field, pout_mat)
+                else:
+                    field = project_normalized_total(field, pout_mat)
+                field = denoise_field(field, ks, nin, sigma)
+            field_out += field
+            field = field_out.copy()
+        else:
+            field_in[...] = field
+            if i != npass - 1:
+                ffield = dotmf(pin_mat, fft2(field))
+                field = ifft2(denoise_fftfield(ffield, ks, nin, sigma))
+                fact = (i0 / total_intensity(field))[..., None, None, None]
+                field = field * fact
+                field_out *= fact
+                field_in *= fact
+                field = field0 - field
+                field_in += field
+        field = fft2(field)
+    return field_out, wavelengths, pixelsize
 
 
 def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1, nstep=1,
@@ -1073,7 +1197,7 @@ def _transmitted_field_direct(field, beta, phi, n=1.):
 
 def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
                    npass=None, nstep=1, diffraction=None, reflection=None, method=None,
-                   betamax=None, layer_callback=None):
+                   betamax=None, norm=DTMM_NORM_FFT, smooth=SMOOTH, layer_callback=None):
     """Defaults and dispatch of dtmm.transfer_field (dtmm/transfer.py:322-615)
     for the covered modes.  The configured default method is the quoted string
     '"2x2"' (dtmm/dtmm.ini:23 read by configparser), which makes the automatic
@@ -1094,7 +1218,7 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
                 reflection = 2
     if method == "4x4":
         return transfer_4x4(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
-                            diffraction, reflection, betamax, layer_callback)
+                            diffraction, reflection, betamax, norm, smooth, layer_callback)
     return transfer_2x2(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
                         diffraction, reflection, betamax, layer_callback)
 

@@ -42,6 +42,9 @@ def _desc(ks, beta, phi, **kw):
     d.beta = _lib.dptr(beta)
     d.phi = _lib.dptr(phi)
     d.diffraction = kw.get("diffraction", 1)
+    d.npass = kw.get("npass", 1)
+    d.norm = kw.get("norm", 1)
+    d.smooth = 0.1
     return d
 
 
@@ -61,6 +64,9 @@ def _plan_rc(**kw):
     d.beta = _lib.dptr(b)
     d.phi = _lib.dptr(b)
     d.diffraction = kw.get("diffraction", 1)
+    d.npass = kw.get("npass", 1)
+    d.norm = kw.get("norm", 1)
+    d.smooth = 0.1
     h = ctypes.c_void_p()
     return lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)), lib.dtmmx_last_error()
 
@@ -76,6 +82,9 @@ def _plan_rc(**kw):
     (dict(precision=2), _lib.DTMMX_E_ARG),
     (dict(diffraction=0), _lib.DTMMX_E_UNSUPPORTED),
     (dict(method=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(npass=0), _lib.DTMMX_E_ARG),
+    (dict(method=2, npass=3), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(npass=3, norm=2), _lib.DTMMX_E_UNSUPPORTED),
 ])
 def test_plan_create_rejects_bad_descriptions(kw, code):
     rc, msg = _plan_rc(**kw)

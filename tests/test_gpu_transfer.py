@@ -138,3 +138,49 @@ def test_diffraction_windows_two_blocks():
     fd, b1 = random_case(shape=(3, 32, 32), wavelengths=(550.,), jones=(1, 0))
     _, b2 = droplet_case(shape=(4, 32, 32), radius=12, ne=1.75, nhost=1.55)
     _compare(fd, [b1[0], b2[0]], "4x4", "c128", nin=1.0, nout=1.33, diffraction=3)
+
+
+def _compare_npass(fd, optical, precision, **kw):
+    """npass > 1: output field and the input-side field left in the input
+    array (reflections) against the oracle."""
+    field, wl, ps = fd
+    f_ref = np.array(field, dtype=np.complex128)
+    ref = oracle.transfer_field((f_ref, wl, ps), optical, method="4x4", **kw)[0]
+    dt = np.complex64 if precision == "c64" else np.complex128
+    f_gpu = np.array(field, dtype=dt)
+    flush("  GPU npass transfer %s field %s %s" % (precision, field.shape, kw))
+    out = dm.transfer_field((f_gpu, wl, ps), optical, method="4x4", precision=precision, **kw)[0]
+    err = rel_err(out, ref)
+    rerr = rel_err(f_gpu, f_ref)
+    flush("  rel field err %.3e  input-side err %.3e" % (err, rerr))
+    assert err < TOL[precision], err
+    assert rerr < TOL[precision], rerr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_npass3_reflections(precision):
+    flush("case npass3 %s" % precision)
+    fd, optical = random_case(shape=(6, 32, 48), wavelengths=(450., 650.), jones=None)
+    _compare_npass(fd, optical, precision, beta=0., phi=0., npass=3, nin=1.0, nout=1.33)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("npass,norm", [(2, 1), (4, 0), (5, 1)])
+def test_npass_norms(npass, norm):
+    flush("case npass=%d norm=%d" % (npass, norm))
+    fd, optical = droplet_case(shape=(8, 32, 32), radius=12, ne=1.75, nhost=1.55,
+                               wavelengths=(550.,), jones=(1, 0))
+    _compare_npass(fd, optical, "c128", beta=0., phi=0., npass=npass, norm=norm, nin=1.2,
+                   nout=1.0)
+
+
+@pytest.mark.gpu
+def test_npass_with_windows_multi_beam():
+    flush("case npass3 windows multi_beam")
+    beta = [0., 0.15]
+    phi = [0., 0.8]
+    fd, optical = random_case(shape=(4, 32, 32), wavelengths=(500., 600.), jones=None,
+                              beta=beta, phi=phi)
+    _compare_npass(fd, optical, "c128", beta=beta, phi=phi, npass=3, diffraction=3, nin=1.0,
+                   nout=1.5, nstep=2)

@@ -87,3 +87,70 @@ def test_transfer_is_linear(method):
     opt = [(np.ones(L), eps, ang)]
     t = lambda f: o.transfer_field((f, wl, 100.), opt, beta=0., phi=0., method=method)[0]
     np.testing.assert_allclose(t(a * f1 + b * f2), a * t(f1) + b * t(f2), atol=1e-12)
+
+
+def _airy(nin, ns, nout, k, L):
+    """Normal-incidence Fabry-Perot transmission and reflection amplitudes of
+    the E field for a slab of index ns and thickness L."""
+    r01 = (nin - ns) / (nin + ns)
+    t01 = 2 * nin / (nin + ns)
+    r12 = (ns - nout) / (ns + nout)
+    t12 = 2 * ns / (ns + nout)
+    ph = np.exp(1j * k * ns * L)
+    den = 1 + r01 * r12 * ph ** 2
+    return t01 * t12 * ph / den, (r01 + r12 * ph ** 2) / den
+
+
+@pytest.mark.parametrize("npass", [3, 5])
+@pytest.mark.parametrize("norm", [o.DTMM_NORM_FFT, 0])
+@pytest.mark.parametrize("diffraction", [1, 3])
+def test_npass_slab_reaches_fabry_perot(npass, norm, diffraction):
+    """npass > 1 on a homogeneous isotropic slab between different media
+    reproduces the Airy transmission and reflection of the slab (the
+    passes add the multiply reflected waves, transfer.py:815-872); the
+    reflection is left in the input array, as the reference does."""
+    nin, ns, nout, L = 1.0, 1.5, 1.25, 7
+    shape = (8, 8)
+    wl = np.array([550., 633.])
+    ps = 100.
+    ks = o.k0(wl, ps)
+    f = np.zeros((len(wl), 4) + shape, complex)
+    f[:, 0], f[:, 1] = 1., nin
+    eps = np.broadcast_to([ns ** 2] * 3, (L, 1, 1, 3))
+    out = o.transfer_field((f, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                           beta=0., phi=0., method="4x4", nin=nin, nout=nout, npass=npass,
+                           norm=norm, diffraction=diffraction)[0]
+    for i, k in enumerate(ks):
+        t, r = _airy(nin, ns, nout, k, L)
+        ep = (out[i, 0] + out[i, 1] / nout) / 2
+        em = (out[i, 0] - out[i, 1] / nout) / 2
+        np.testing.assert_allclose(ep, t, atol=1e-12)
+        np.testing.assert_allclose(em, 0., atol=1e-12)
+        refl = (f[i, 0] - f[i, 1] / nin) / 2
+        inc = (f[i, 0] + f[i, 1] / nin) / 2
+        np.testing.assert_allclose(refl, r, atol=1e-12)
+        np.testing.assert_allclose(inc, 1., atol=1e-12)
+
+
+def test_npass_one_pass_has_no_reflection():
+    nin, ns, nout, L = 1.0, 1.5, 1.25, 5
+    shape = (8, 8)
+    wl = np.array([550.])
+    f = np.zeros((1, 4) + shape, complex)
+    f[:, 0], f[:, 1] = 1., nin
+    f_copy = f.copy()
+    eps = np.broadcast_to([ns ** 2] * 3, (L, 1, 1, 3))
+    o.transfer_field((f, wl, 100.), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))], beta=0.,
+                     phi=0., method="4x4", nin=nin, nout=nout, npass=1)
+    np.testing.assert_array_equal(f, f_copy)
+
+
+def test_notch_filter_closed_form():
+    """exp_notch_filter is 1 far from the notch, 0 at its centre
+    (dtmm/denoise.py:21-22)."""
+    x = np.array([0., 0.75, 1.5, 2.0, 3.0])
+    f = o.exp_notch_filter(x, 1.5, 0.1)
+    np.testing.assert_allclose(f[0], 1.)
+    np.testing.assert_allclose(f[2], 0.)
+    np.testing.assert_allclose(f[1], (1 - np.exp(-7.5)) / (1 - np.exp(-15.)))
+    np.testing.assert_allclose(f[4], 1.)
