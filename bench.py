@@ -12,6 +12,12 @@ A step is one full transfer_field pass (all layers, all planes) with the
 inputs resident in HBM.  value = voxel*wavelength*plane steps per second:
 H*W*L*n_wavelengths*n_polarisations / seconds per pass.
 
+--workload c5 measures the reflection workload instead: 1024x1024x200, a
+3x3 grid of radial droplets (radius 80 px, no 1.5, ne 1.6, host 1.5) between
+glass (nin = nout = 1.52), 5 wavelengths 450-650 nm, 3 Blackman-windowed
+beams at beta 0.1, phi 0/120/240 deg, both polarisations (30 planes), 4x4,
+npass = 3, diffraction = 5; value there counts npass passes.
+
 Prints ONE JSON line.  Options: --method 4x4|2x2, --precision c64|c128.
 """
 import argparse
@@ -40,6 +46,58 @@ MEASURED_TRAFFIC = {
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+C5_BEAMS = ([0.1, 0.1, 0.1], [0., 2 * np.pi / 3, 4 * np.pi / 3])
+
+
+def build_inputs_c5(size, layers, nlam):
+    import dtmm_amd as dm
+    from dtmm_amd.samples import droplet_array_data, droplet_grid_centres
+    t = time.time()
+    optical = [droplet_array_data((layers, size, size), 80. * size / 1024,
+                                  droplet_grid_centres(size, 3), no=1.5, ne=1.6, nhost=1.5)]
+    wl = np.linspace(450., 650., nlam)
+    beta, phi = C5_BEAMS
+    fd = dm.illumination_data((size, size), wl, pixelsize=100, beta=beta, phi=phi, jones=None,
+                              window=dm.blackman((size, size)), n=1.52)
+    log("inputs built in %.1f s: field %s" % (time.time() - t, fd[0].shape))
+    return fd, optical
+
+
+def cpu_baseline_c5(fd, optical, crop, target_s=15.0):
+    """Oracle on a crop x crop window of the same workload, npass = 1, the
+    marginal time of a second layer (windows, all planes)."""
+    import oracle
+    field, wl, ps = fd
+    d, eps, ang = optical[0]
+    H, W = field.shape[-2:]
+    y0, x0 = (H - crop) // 2, (W - crop) // 2
+    f = np.ascontiguousarray(field[..., y0:y0 + crop, x0:x0 + crop])
+    e = eps[:, y0:y0 + crop, x0:x0 + crop]
+    a = ang[:, y0:y0 + crop, x0:x0 + crop]
+    z0 = len(d) // 2
+    cores = len(os.sched_getaffinity(0))
+    oracle.set_fft_workers(cores)
+    beta, phi = C5_BEAMS
+
+    def run(n):
+        t = time.time()
+        oracle.transfer_field((f.copy(), wl, ps), [(d[z0:z0 + n], e[z0:z0 + n], a[z0:z0 + n])],
+                              beta=beta, phi=phi, method="4x4", npass=1, diffraction=5,
+                              nin=1.52, nout=1.52)
+        return time.time() - t
+
+    t1 = run(1)
+    t2 = run(2)
+    per_layer = max(t2 - t1, 1e-9)
+    planes = int(np.prod(f.shape[:-3]))
+    return {"value": crop * crop * planes / per_layer, "unit": "voxel*lambda steps/s",
+            "cores": cores, "kind": "port",
+            "sample": "oracle 4x4 diffraction=5 transfer of 1 and 2 mid-stack layers of a "
+                      "%dx%d crop, all %d planes, npass=1 (%.1f s and %.1f s; rate from the "
+                      "extra layer); per-pixel math 1 thread, scipy.fft on %d threads"
+                      % (crop, crop, planes, t1, t2, cores)}
 
 
 def build_inputs(size, layers, nlam, seed=0):
@@ -88,6 +146,10 @@ def cpu_baseline(fd, optical, method, target_s=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c5"],
+                    help="c2: 512^2 x 128 random texture (default); c5: 1024^2 x 200 droplet "
+                         "grid with npass=3, diffraction=5")
+    ap.add_argument("--cpu-crop", type=int, default=96, help="c5: oracle crop size")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--method", default="4x4", choices=["4x4", "2x2"])
@@ -131,21 +193,37 @@ def main():
         log("no GPU")
         sys.exit(1)
     lib = _lib.load()
-    fd, optical = build_inputs(args.size, args.layers, args.wavelengths)
+    c5 = args.workload == "c5"
+    if c5:
+        if args.method != "4x4":
+            log("the c5 workload is 4x4 only")
+            sys.exit(2)
+        size = args.size if args.size != 512 else 1024
+        layers = args.layers if args.layers != 128 else 200
+        nlam = args.wavelengths if args.wavelengths != 9 else 5
+        fd, optical = build_inputs_c5(size, layers, nlam)
+        beams, phis = C5_BEAMS
+        npass, diffraction, nin, nout = 3, 5, 1.52, 1.52
+    else:
+        fd, optical = build_inputs(args.size, args.layers, args.wavelengths)
+        beams, phis = [0.], [0.]
+        npass, diffraction, nin, nout = 1, 1, 1.5, 1.5
     field, wl, ps = fd
     H, W = field.shape[-2:]
     nlam = len(wl)
     n_rays = int(np.prod(field.shape[:-4]))
+    rays_per_beam = n_rays // len(beams)
     planes = n_rays * nlam
     ks = dm.k0(wl, ps)
 
     t = time.time()
-    stack = Stack(optical, (H, W), nstep=1, nin=1.5, nout=1.5)
+    stack = Stack(optical, (H, W), nstep=1, nin=nin, nout=nout)
     L = stack.n_layers
     log("stack uploaded in %.1f s (%d layers)" % (time.time() - t, L))
 
     def make(precision):
-        plan = Plan((H, W), ks, [0.], [0.], n_rays, n_rays, args.method, precision, 0.8)
+        plan = Plan((H, W), ks, beams, phis, n_rays, rays_per_beam, args.method, precision, 0.8,
+                    diffraction=diffraction, npass=npass)
         host = np.ascontiguousarray(field.reshape((n_rays, nlam, 4, H, W)))
         fin = torch.from_numpy(host).to(device="cuda", dtype=plan.dtype).contiguous()
         fout = torch.empty_like(fin)
@@ -153,21 +231,46 @@ def main():
         return plan, fin, fout, ws
 
     plan, fin, fout, ws = make(args.precision)
+    # npass > 1 overwrites the input with the input-side field: every step
+    # starts from a fresh copy of the illumination
+    fin0 = fin.clone() if npass > 1 else None
+    nwin = 1
+    if diffraction > 1:
+        d = _lib.PlanDesc()
+        nwin_c = ctypes.c_int()
+        bb = np.asarray(beams, np.float64)
+        pp = np.asarray(phis, np.float64)
+        d.height, d.width, d.n_wavelengths = H, W, nlam
+        d.n_rays, d.rays_per_beam, d.method, d.precision = n_rays, rays_per_beam, 4, 0
+        d.betamax, d.k0, d.beta, d.phi = 0.8, _lib.dptr(ks), _lib.dptr(bb), _lib.dptr(pp)
+        d.diffraction, d.npass, d.norm, d.smooth = diffraction, 1, 1, 0.1
+        rc = lib.dtmmx_host_windows(ctypes.byref(d), diffraction * diffraction,
+                                    ctypes.byref(nwin_c), None, None, None, None)
+        _lib.check(rc)
+        nwin = nwin_c.value
+        log("%d non-empty k-space windows" % nwin)
+
+    def step(p, fi, fo, w):
+        if fin0 is not None:
+            fi.copy_(fin0 if fi.dtype == fin0.dtype else fin0.to(fi.dtype))
+        p.execute(stack, fi, fo, w, stream=stream)
+
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
-        plan.execute(stack, fin, fout, ws, stream=stream)
+        step(plan, fin, fout, ws)
     torch.cuda.synchronize()
 
     # every 16th launch of each per-layer kernel is timed with HIP events on
     # its stream inside the timed region (timing every launch would add about
     # 8% to the pass through the extra stream commands)
     timing = not args.no_kernel_timing
+    launches = 2 * nwin * (L + 4) * npass + 16  # per-layer kernel launches of one step
     if timing:
-        _lib.check(lib.dtmmx_timing_begin(8 * args.steps * (L + 4), 16))
+        _lib.check(lib.dtmmx_timing_begin(2 * args.steps * launches // 16 + 64, 16))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        plan.execute(stack, fin, fout, ws, stream=stream)
+        step(plan, fin, fout, ws)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (ctypes.c_double * 4)()
@@ -176,11 +279,11 @@ def main():
         _lib.check(lib.dtmmx_timing_end(ms, cnt, 4))
     else:
         # per-kernel times from one further instrumented pass
-        _lib.check(lib.dtmmx_timing_begin(8 * (L + 4), 1))
-        plan.execute(stack, fin, fout, ws, stream=stream)
+        _lib.check(lib.dtmmx_timing_begin(2 * launches + 64, 1))
+        step(plan, fin, fout, ws)
         _lib.check(lib.dtmmx_timing_end(ms, cnt, 4))
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    steps = H * W * L * planes
+    steps = H * W * L * planes * npass
     value = steps / (ms_per_step / 1e3)
     nc = 4 if args.method == "4x4" else 2
     cb = 8 if args.precision == "c64" else 16
@@ -194,8 +297,11 @@ def main():
     # plus the row kernel's material read (eps0..2 + theta, phi as float64,
     # 40 B per pixel).
     field_bytes = 2 * nc * cb * H * W * planes
+    # with k-space windows the row kernel also adds its result to the window
+    # sum (read + write) except for the first window of a layer
+    row_field = field_bytes * (1. + 0.5 * (nwin - 1) / nwin) if nwin > 1 else field_bytes
     kern = {
-        "row_layer": {"ms": ms[0], "n": cnt[0], "bytes": field_bytes + 40 * H * W},
+        "row_layer": {"ms": ms[0], "n": cnt[0], "bytes": row_field + 40 * H * W},
         "col_kspace": {"ms": ms[1], "n": cnt[1], "bytes": field_bytes},
     }
     dom = max(kern, key=lambda k: kern[k]["ms"])
@@ -204,17 +310,18 @@ def main():
     log("kernels: " + ", ".join("%s %d x %.3f ms" % (k, v["n"], v["ms"] / max(v["n"], 1))
                                  for k, v in kern.items())
         + ", kspace %d x %.3f ms" % (cnt[3], ms[3] / max(cnt[3], 1)))
-    per_pass = [ms[i] / max(cnt[i], 1) * n for i, n in enumerate((L, max(L - 1, 0), 0, 2))]
+    per_pass = [ms[i] / max(cnt[i], 1) * n * npass
+                for i, n in enumerate((L * nwin, (L * nwin if nwin > 1 else max(L - 1, 0)), 0, 2))]
     log("per pass (from launch averages): row %.2f ms, col %.2f ms, kspace %.2f ms, rest %.2f ms"
         % (per_pass[0], per_pass[1], per_pass[3], ms_per_step - sum(per_pass)))
 
     extra = {}
     if not args.no_ref and args.precision == "c64":
         plan2, fin2, fout2, ws2 = make("c128")
-        plan2.execute(stack, fin2, fout2, ws2, stream=stream)
+        step(plan2, fin2, fout2, ws2)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        plan2.execute(stack, fin2, fout2, ws2, stream=stream)
+        step(plan2, fin2, fout2, ws2)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
         ref = fout2.cpu().numpy()
@@ -229,8 +336,17 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(fd, optical, args.method, args.cpu_seconds)
+        if c5:
+            cpu = cpu_baseline_c5(fd, optical, args.cpu_crop, args.cpu_seconds)
+        else:
+            cpu = cpu_baseline(fd, optical, args.method, args.cpu_seconds)
         log("cpu baseline: %s" % cpu["sample"])
+    if c5:
+        workload = ("%dx%dx%d 3x3 radial droplet grid, %d wavelengths x 3 beams x 2 pol, 4x4, "
+                    "diffraction=5 (%d windows), npass=3, nin=nout=1.52" % (H, W, L, nlam, nwin))
+    else:
+        workload = ("%dx%dx%d random director, %d wavelengths x 2 pol, %s, diffraction=1, "
+                    "npass=1" % (H, W, L, nlam, args.method))
 
     rec = {
         "metric": "voxel*lambda steps/s (transfer_field, 4x4 unless noted), c64 with c128 "
@@ -246,11 +362,12 @@ def main():
         "vs_baseline": None,
         "dtype": "c64 (fp32 complex, fp64 eigensolve)" if args.precision == "c64" else "c128",
         "data": "synthetic (seeded random nematic director, plane-wave illumination)",
-        "config": {"workload": "%dx%dx%d random director, %d wavelengths x 2 pol, %s, "
-                               "diffraction=1, npass=1" % (H, W, L, nlam, args.method),
+        "config": {"workload": workload,
                    "height": H, "width": W, "layers": L, "wavelengths": nlam, "planes": planes,
-                   "method": args.method, "precision": args.precision},
-        "hbm_frac_per_step": (steps * 2 * nc * cb / (ms_per_step / 1e3) / 1e9) / HBM_PEAK_GBS,
+                   "method": args.method, "precision": args.precision, "npass": npass,
+                   "diffraction": diffraction},
+        "hbm_frac_per_step": (steps * nwin * 2 * nc * cb / (ms_per_step / 1e3) / 1e9)
+        / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get((dom, H, W, planes, args.method,

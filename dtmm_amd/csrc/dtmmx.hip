@@ -98,9 +98,10 @@ static int launch_row(int n, int nr, const RowArgs<T>& a, int nwork, hipStream_t
   case N:                                                                                   \
     if (nr == 2 && row_nr_max<T, N>() >= 2)                                                 \
       hipLaunchKernelGGL((row_kernel<T, N, (row_nr_max<T, N>() >= 2 ? 2 : 1), NC, OP>),      \
-                         dim3(nwork), dim3(64 * 2 * NC), 0, s, a);                          \
+                         dim3(nwork), dim3(row_lanes<N>() * 2 * NC), 0, s, a);              \
     else                                                                                    \
-      hipLaunchKernelGGL((row_kernel<T, N, 1, NC, OP>), dim3(nwork), dim3(64 * NC), 0, s, a); \
+      hipLaunchKernelGGL((row_kernel<T, N, 1, NC, OP>), dim3(nwork),                        \
+                         dim3(row_lanes<N>() * NC), 0, s, a);                               \
     break;
     DTMMX_SIZES(X)
 #undef X

@@ -102,11 +102,36 @@ __host__ __device__ constexpr int min_waves() {
   return sizeof(T) == 4 ? 4 : 2;
 }
 
+// Lanes per row line: one wave up to 512 points; at 1024 points two waves
+// share a line (8 values per lane instead of 16, and half the pixels of the
+// layer operator per lane), exchanging through LDS with work-group barriers.
+template <int N>
+__host__ __device__ constexpr int row_lanes() {
+  return N >= 1024 ? 128 : 64;
+}
+
+// Row kernel of 1024 points: the line takes 8 values per lane but the eigen
+// data of 2 pixels per lane takes 52 VGPRs, so the budget is 256 VGPRs
+// (2 waves per SIMD; complex128: 1).
+template <typename T, int N>
+__host__ __device__ constexpr int row_min_waves() {
+  return N >= 1024 ? (sizeof(T) == 4 ? 2 : 1) : min_waves<T>();
+}
+
+// Column kernel of 1024 points: 168 VGPRs (3 waves per SIMD) hold the
+// diagonal factor, the line and the prefetched line without spilling.
+template <typename T, int N>
+__host__ __device__ constexpr int col_min_waves() {
+  return (N >= 1024 && sizeof(T) == 4) ? 3 : min_waves<T>();
+}
+
 template <typename T, int N, int NR, int NC, int OP>
-__global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowArgs<T> A) {
-  constexpr int NL = 64;
+__global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) void row_kernel(
+    RowArgs<T> A) {
+  constexpr int NL = row_lanes<N>();
+  constexpr bool WV = (NL == 64);  // one wave per line: wave-local exchanges
   constexpr int LINES = NR * NC;
-  constexpr int NTW = 64 * LINES;
+  constexpr int NTW = NL * LINES;
   constexpr int LS = line_stride<N>();
   constexpr bool REV1 = (OP != OP_FWD);
   constexpr int DIR1 = (OP == OP_FWD) ? -1 : +1;
@@ -121,7 +146,7 @@ __global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowAr
   __shared__ cpx<T> lds[LINES * LS];
   __shared__ cpx<T> tws[N];  // twiddles in LDS: their reads never wait on global loads
   const int tid = threadIdx.x;
-  const int li = tid >> 6, lane = tid & 63;
+  const int li = tid / NL, lane = tid % NL;
   for (int i = tid; i < N; i += NTW) tws[i] = A.tw[i];
   const int rr = li / NC, comp = li - rr * NC;
   const int w = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -174,8 +199,9 @@ __global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowAr
     return A.field + (q * NC + comp) * HW + size_t(y) * N;
   };
   const int acc_mode = (OP == OP_ROP) ? A.acc_mode : 0;
-  cpx<T> nxt[NIN];  // prefetched inputs of the next wavelength
-  {
+  constexpr bool PF = true;
+  cpx<T> nxt[PF ? NIN : 1];  // prefetched inputs of the next wavelength
+  if constexpr (PF) {
     const cpx<T>* g0 = line_ptr(0);
 #pragma unroll
     for (int j = 0; j < NIN; ++j)
@@ -185,15 +211,22 @@ __global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowAr
     const int lam = A.b.lam0 + jl;
     cpx<T>* gl = line_ptr(jl);
     cpx<T> v[MAXV];
+    if constexpr (PF) {
 #pragma unroll
-    for (int j = 0; j < NIN; ++j) v[j] = nxt[j];
-    if (jl + 1 < A.b.nlam_b) {
-      const cpx<T>* gn = line_ptr(jl + 1);
+      for (int j = 0; j < NIN; ++j) v[j] = nxt[j];
+      if (jl + 1 < A.b.nlam_b) {
+        const cpx<T>* gn = line_ptr(jl + 1);
+#pragma unroll
+        for (int j = 0; j < NIN; ++j)
+          if (slot_valid<N, REV1, NL, false>(j, lane))
+            nxt[j] = gn[slot_pos<N, REV1, NL, false>(j, lane)];
+      }
+    } else {
 #pragma unroll
       for (int j = 0; j < NIN; ++j)
-        if (slot_valid<N, REV1, NL, false>(j, lane)) nxt[j] = gn[slot_pos<N, REV1, NL, false>(j, lane)];
+        if (slot_valid<N, REV1, NL, false>(j, lane)) v[j] = gl[slot_pos<N, REV1, NL, false>(j, lane)];
     }
-    reg_fft<T, N, REV1, DIR1, NL, 1, 1, MAXV, true>(v, line_lds, tws, lane);
+    reg_fft<T, N, REV1, DIR1, NL, 1, 1, MAXV, WV>(v, line_lds, tws, lane);
 
     if constexpr (OP == OP_ROP) {
 #pragma unroll
@@ -244,8 +277,8 @@ __global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowAr
       for (int j = 0; j < NMID; ++j)
         if (slot_valid<N, REV1, NL, true>(j, lane))
           v[j] = line_lds[lds_at<1, 1>(slot_pos<N, REV1, NL, true>(j, lane))];
-      exchange_sync<true>();
-      reg_fft<T, N, false, -1, NL, 1, 1, MAXV, true>(v, line_lds, tws, lane);
+      exchange_sync<WV>();
+      reg_fft<T, N, false, -1, NL, 1, 1, MAXV, WV>(v, line_lds, tws, lane);
     }
     if (acc_mode == 0) {
 #pragma unroll
@@ -266,7 +299,7 @@ __global__ __launch_bounds__(64 * NR * NC, min_waves<T>()) void row_kernel(RowAr
           if (slot_valid<N, REVL, NL, true>(j, lane)) al[slot_pos<N, REVL, NL, true>(j, lane)] = v[j];
       }
     }
-    exchange_sync<true>();
+    exchange_sync<WV>();
   }
 }
 
@@ -332,7 +365,7 @@ __host__ __device__ constexpr int col_lanes() {
 }
 
 template <typename T, int N, int TX, int OP>
-__global__ __launch_bounds__(TX * col_lanes<N>(), min_waves<T>()) void col_kernel(ColArgs<T> A) {
+__global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N>())) void col_kernel(ColArgs<T> A) {
   constexpr int NL = col_lanes<N>();
   constexpr bool REV1 = (OP == OP_INV);
   constexpr int DIR1 = (OP == OP_INV) ? +1 : -1;

@@ -50,21 +50,36 @@ def random_director_data(shape, seed=0, no=1.5, ne=1.7, theta0=np.pi / 2, twist=
 
 def droplet_array_data(shape, radius, centres, no=1.5, ne=1.6, nhost=1.5):
     """Radial nematic droplets at `centres` ((x, y) offsets in pixels from the
-    box centre, droplets centred in z) in an isotropic host."""
+    box centre, droplets centred in z) in an isotropic host; voxel
+    coordinates are half-pixel centred as in dtmm/data.py:773-777."""
     nz, ny, nx = shape
     az, ay, ax = [np.arange(-l / 2. + .5, l / 2. + .5) for l in shape]
-    zz, yy, xx = np.meshgrid(az, ay, ax, indexing="ij")
     director = np.zeros(shape + (3,), np.float64)
     mask = np.zeros(shape, bool)
     for cx, cy in centres:
-        dx, dy = xx - cx, yy - cy
-        r = np.sqrt(dx ** 2 + dy ** 2 + zz ** 2)
+        # bounding box of the droplet (whole grid lines within radius)
+        sl = []
+        for a, c in ((az, 0.), (ay, cy), (ax, cx)):
+            idx = np.nonzero(np.abs(a - c) <= radius)[0]
+            sl.append(slice(idx[0], idx[-1] + 1) if len(idx) else slice(0, 0))
+        zz, yy, xx = np.meshgrid(az[sl[0]], ay[sl[1]] - cy, ax[sl[2]] - cx, indexing="ij")
+        r = np.sqrt(xx ** 2 + yy ** 2 + zz ** 2)
         m = (r <= radius) & (r != 0)
-        director[..., 0][m] = dx[m] / r[m]
-        director[..., 1][m] = dy[m] / r[m]
-        director[..., 2][m] = zz[m] / r[m]
-        mask |= r <= radius
+        d = director[sl[0], sl[1], sl[2]]
+        d[..., 0][m] = xx[m] / r[m]
+        d[..., 1][m] = yy[m] / r[m]
+        d[..., 2][m] = zz[m] / r[m]
+        mask[sl[0], sl[1], sl[2]] |= r <= radius
     return director2data(director, mask=mask, no=no, ne=ne, nhost=nhost)
 
 
-__all__ = ["random_director", "random_director_data", "droplet_array_data", "refind2eps"]
+def droplet_grid_centres(size, n=3):
+    """(x, y) offsets of an n x n grid of droplets spaced size / n apart,
+    centred on the box."""
+    step = size / n
+    offs = [(i - (n - 1) / 2.) * step for i in range(n)]
+    return [(x, y) for y in offs for x in offs]
+
+
+__all__ = ["random_director", "random_director_data", "droplet_array_data",
+           "droplet_grid_centres", "refind2eps"]

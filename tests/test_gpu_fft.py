@@ -11,7 +11,7 @@ from helpers import flush
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["c64", "c128"])
-@pytest.mark.parametrize("shape", [(96, 96), (24, 40), (512, 64), (128, 1024), (20, 12), (384, 192)])
+@pytest.mark.parametrize("shape", [(96, 96), (24, 40), (512, 64), (128, 1024), (1024, 32), (20, 12), (384, 192)])
 def test_fft2(shape, precision):
     import torch
     flush("case fft2 %s %s" % (shape, precision))

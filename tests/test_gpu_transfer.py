@@ -184,3 +184,25 @@ def test_npass_with_windows_multi_beam():
                               beta=beta, phi=phi)
     _compare_npass(fd, optical, "c128", beta=beta, phi=phi, npass=3, diffraction=3, nin=1.0,
                    nout=1.5, nstep=2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+@pytest.mark.parametrize("shape", [(3, 32, 1024), (3, 1024, 32)])
+def test_1024_point_lines(shape, method, precision):
+    """1024-point rows (two waves per line) and columns."""
+    flush("case 1024 lines %s %s %s" % (shape, method, precision))
+    fd, optical = random_case(shape=shape, wavelengths=(500., 600.), jones=None)
+    _compare(fd, optical, method, precision)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(2, 32, 1024), (2, 1024, 32)])
+def test_1024_point_lines_windows_npass(shape):
+    flush("case 1024 lines windows npass %s" % (shape,))
+    beta = [0.1, 0.1]
+    phi = [0., 2.1]
+    fd, optical = random_case(shape=shape, wavelengths=(550.,), jones=None, beta=beta, phi=phi)
+    _compare_npass(fd, optical, "c64", beta=beta, phi=phi, npass=3, diffraction=3, nin=1.52,
+                   nout=1.52)
