@@ -298,11 +298,14 @@ def main():
     # 40 B per pixel).
     field_bytes = 2 * nc * cb * H * W * planes
     # with k-space windows the row kernel also adds its result to the window
-    # sum (read + write) except for the first window of a layer
+    # sum (read + write) except for the first window of a layer, and the
+    # column kernel reads the field once and writes one window row per tent
+    # row (diffraction of them)
     row_field = field_bytes * (1. + 0.5 * (nwin - 1) / nwin) if nwin > 1 else field_bytes
+    col_field = field_bytes * (1 + diffraction) / 2. if nwin > 1 else field_bytes
     kern = {
         "row_layer": {"ms": ms[0], "n": cnt[0], "bytes": row_field + 40 * H * W},
-        "col_kspace": {"ms": ms[1], "n": cnt[1], "bytes": field_bytes},
+        "col_kspace": {"ms": ms[1], "n": cnt[1], "bytes": col_field},
     }
     dom = max(kern, key=lambda k: kern[k]["ms"])
     avg_ms = kern[dom]["ms"] / max(kern[dom]["n"], 1)
@@ -310,8 +313,10 @@ def main():
     log("kernels: " + ", ".join("%s %d x %.3f ms" % (k, v["n"], v["ms"] / max(v["n"], 1))
                                  for k, v in kern.items())
         + ", kspace %d x %.3f ms" % (cnt[3], ms[3] / max(cnt[3], 1)))
+    # launches per pass: row L (or L * windows), column L - 1 (or L window
+    # splits), dense k-space 2
     per_pass = [ms[i] / max(cnt[i], 1) * n * npass
-                for i, n in enumerate((L * nwin, (L * nwin if nwin > 1 else max(L - 1, 0)), 0, 2))]
+                for i, n in enumerate((L * nwin, (L if nwin > 1 else max(L - 1, 0)), 0, 2))]
     log("per pass (from launch averages): row %.2f ms, col %.2f ms, kspace %.2f ms, rest %.2f ms"
         % (per_pass[0], per_pass[1], per_pass[3], ms_per_step - sum(per_pass)))
 

@@ -215,7 +215,7 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   a.bm = bm ? *bm : P->bm;
   int nr = 1;
   if (sizeof(T) == 4 && a.W <= 512 && b.nrays % 2 == 0 &&
-      (op != OP_ROP || P->bm.rays_per_beam % 2 == 0))
+      ((op != OP_ROP && op != OP_ROPW) || P->bm.rays_per_beam % 2 == 0))
     nr = 2;
   a.rays_per_group = nr;
   const int nwork = a.H * (b.nrays / nr);
@@ -224,6 +224,13 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
     timer_start(0, s);
     int r = nc == 2 ? launch_row<T, 2, OP_ROP>(w, nr, a, nwork, s)
                     : launch_row<T, 4, OP_ROP>(w, nr, a, nwork, s);
+    timer_stop(s);
+    return r;
+  }
+  if (op == OP_ROPW) {
+    if (nc != 4 || !a.wcx) return fail(DTMMX_E_ARG, "window row pass needs 4 components");
+    timer_start(0, s);
+    int r = launch_row<T, 4, OP_ROPW>(w, nr, a, nwork, s);
     timer_stop(s);
     return r;
   }
@@ -266,14 +273,15 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
     timer_stop(s);
     return r;
   }
-  if (op == OP_WIN) {
+  if (op == OP_WIN || op == OP_SPLIT) {
     if (!P->wcx || !a.out) return fail(DTMMX_E_ARG, "window pass without windows");
     a.amax = P->wmask;
     a.wcx = P->wcx;
     a.wcy = P->wcy;
     a.wn = P->wn;
     timer_start(1, s);
-    int r = launch_col<T, OP_WIN>(h, a, nlines, s);
+    int r = op == OP_WIN ? launch_col<T, OP_WIN>(h, a, nlines, s)
+                         : launch_col<T, OP_SPLIT>(h, a, nlines, s);
     timer_stop(s);
     return r;
   }
@@ -340,7 +348,8 @@ static std::vector<Step> expand_steps(const dtmmx_stack* S, int dir) {
 // Workspace layout of a transfer call (byte offsets; 256-byte aligned).
 struct WsLayout {
   size_t pass_field, pass_field0;  // npass > 1: propagated field, transmitted input
-  size_t win_tmp, win_acc;         // diffraction > 1: window part, window sum
+  size_t win_tmp, win_acc;         // diffraction > 1: window rows (wn fields; the
+                                   // first is the window part of sub-stepped layers), sum
   size_t e2;                       // 2x2: the 2-component field
   size_t part, sums;               // npass > 1: reduction partials and sums
   size_t total;
@@ -367,7 +376,7 @@ static WsLayout ws_layout(const dtmmx_plan* P) {
     w.sums = take(4 * planes * sizeof(double));
   }
   if (m4 && P->wn > 1) {
-    w.win_tmp = take(f4);
+    w.win_tmp = take(f4 * P->wn);
     w.win_acc = take(f4);
   }
   if (!m4) w.e2 = take(planes * 2 * HW * cb);
@@ -443,6 +452,42 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
       TRY(kspace<T>(P, A, A, b, bd, s));
       TRY(col_pass<T>(P, A, b, 4, OP_INV, s));
       prev_d = 0.;
+    }
+    if (ns == 1) {
+      // single-step layer: the tent windows are separable, cx(kx) cy(ky),
+      // so one column pass writes the wn window rows cy_j * (common
+      // half-step phases) * field, and the row pass of window (i, j) applies
+      // cx_i and the window's mean-beam correction to row j on its input
+      ColArgs<T> ca{};
+      ca.da = prev_d;
+      ca.ea = prev_e;
+      ca.db = ds;
+      ca.eb = el;
+      ca.scale = scale;
+      ca.dense_sign = 1;
+      ca.out = tmp;
+      TRY(col_pass<T>(P, A, b, 4, OP_SPLIT, s, &ca));
+      const size_t fsz = size_t(b.nrays) * b.nlam * 4 * HW;
+      for (int w = 0; w < nwin; ++w) {
+        Beams bw = P->bm;
+        for (int q = 0; q < nbeam; ++q) {
+          bw.beta[q] = P->win_beta[size_t(w) * nbeam + q];
+          bw.phi[q] = P->win_phi[size_t(w) * nbeam + q];
+        }
+        RowArgs<T> la = layer_args<T>(S, HW, l, ds, 1, status);
+        la.acc = acc;
+        la.acc_mode = (w == 0) ? 1 : 2;
+        la.wcx = P->wcx;
+        la.wn = P->wn;
+        la.wi = P->win_i[w];
+        la.we = el;
+        la.wd = ds;
+        TRY(row_pass<T>(P, tmp + size_t(P->win_j[w]) * fsz, b, 4, OP_ROPW, s, &la, &bw));
+      }
+      std::swap(A, acc);
+      prev_d = ds;
+      prev_e = el;
+      continue;
     }
     for (int w = 0; w < nwin; ++w) {
       Beams bw = P->bm;

@@ -38,7 +38,7 @@ __host__ __device__ inline int mode_index(int i, int n) {  // |k| of fftfreq
 // Launch geometry shared by the per-layer kernels.
 // ---------------------------------------------------------------------------
 
-enum { OP_FWD = 0, OP_INV = 1, OP_ROP = 2, OP_DIAG = 3, OP_WIN = 4 };
+enum { OP_FWD = 0, OP_INV = 1, OP_ROP = 2, OP_DIAG = 3, OP_WIN = 4, OP_SPLIT = 5, OP_ROPW = 6 };
 
 // Work-item index for block b of n such that consecutive work items land on
 // the same XCD (blocks are dispatched round-robin over the 8 XCDs): work
@@ -87,6 +87,14 @@ struct RowArgs {
   // it to `acc`; 2 adds it to `acc` (sum over k-space windows)
   cpx<T>* acc;
   int acc_mode;
+  // OP_ROPW (OP_ROP with an input window): the input line (indexed by kx)
+  // is multiplied by the tent factor cx [beam][lam][wn][W] of window column
+  // wi and by the window's mean-beam correction
+  // exp(-+i k0 wd sqrt(we - beta^2)) on forward / backward components
+  const double* wcx;
+  int wn, wi;
+  cd we;
+  double wd;
   Beams bm;
 };
 
@@ -120,9 +128,10 @@ __host__ __device__ constexpr int row_min_waves() {
 
 // Column kernel of 1024 points: 168 VGPRs (3 waves per SIMD) hold the
 // diagonal factor, the line and the prefetched line without spilling.
-template <typename T, int N>
+template <typename T, int N, int OP>
 __host__ __device__ constexpr int col_min_waves() {
-  return (N >= 1024 && sizeof(T) == 4) ? 3 : min_waves<T>();
+  return OP == OP_SPLIT ? (sizeof(T) == 4 ? 2 : 1)
+                        : ((N >= 1024 && sizeof(T) == 4) ? 3 : min_waves<T>());
 }
 
 template <typename T, int N, int NR, int NC, int OP>
@@ -140,7 +149,8 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   constexpr int MAXV = MV1 > MV2 ? MV1 : MV2;
   constexpr int NIN = slot_count<N, REV1, NL, false>();
   constexpr int NMID = slot_count<N, REV1, NL, true>();
-  constexpr bool REVL = (OP == OP_ROP) ? false : REV1;
+  constexpr bool ROPX = (OP == OP_ROP || OP == OP_ROPW);  // layer operator kernels
+  constexpr bool REVL = ROPX ? false : REV1;
   constexpr int NOUT = slot_count<N, REVL, NL, true>();
   constexpr int PPX = (N + NTW - 1) / NTW;  // pixels per thread in the layer operator
   __shared__ cpx<T> lds[LINES * LS];
@@ -158,8 +168,8 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
 
   // per-pixel operator data of this row, solved in fp64 once and reused for
   // every wavelength and ray of the work-group
-  cpx<T> eg[(OP == OP_ROP) ? PPX : 1][EIG_COMPS];
-  if constexpr (OP == OP_ROP) {
+  cpx<T> eg[ROPX ? PPX : 1][EIG_COMPS];
+  if constexpr (ROPX) {
     const int beam = (rg * NR) / A.bm.rays_per_beam;
     const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
     double s0, c0;
@@ -198,7 +208,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
     const size_t q = size_t(rg * NR + rr) * A.b.nlam + A.b.lam0 + jl;
     return A.field + (q * NC + comp) * HW + size_t(y) * N;
   };
-  const int acc_mode = (OP == OP_ROP) ? A.acc_mode : 0;
+  const int acc_mode = ROPX ? A.acc_mode : 0;
   constexpr bool PF = true;
   cpx<T> nxt[PF ? NIN : 1];  // prefetched inputs of the next wavelength
   if constexpr (PF) {
@@ -226,9 +236,21 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
       for (int j = 0; j < NIN; ++j)
         if (slot_valid<N, REV1, NL, false>(j, lane)) v[j] = gl[slot_pos<N, REV1, NL, false>(j, lane)];
     }
+    if constexpr (OP == OP_ROPW) {
+      // window column factor and mean-beam correction of this line
+      const int beam = (rg * NR + rr) / A.bm.rays_per_beam;
+      const T bw = T(A.bm.beta[beam]);
+      const cpx<T> ac = csqrt(cast<T>(A.we) - mk<T>(bw * bw));
+      const cpx<T> cr = cexpi_layer(ac * T(((comp & 1) ? 1. : -1.) * A.bm.k0[lam] * A.wd));
+      const double* cxl = A.wcx + ((size_t(beam) * A.b.nlam + lam) * A.wn + A.wi) * N;
+#pragma unroll
+      for (int j = 0; j < NIN; ++j)
+        if (slot_valid<N, REV1, NL, false>(j, lane))
+          v[j] = v[j] * (cr * T(cxl[slot_pos<N, REV1, NL, false>(j, lane)]));
+    }
     reg_fft<T, N, REV1, DIR1, NL, 1, 1, MAXV, WV>(v, line_lds, tws, lane);
 
-    if constexpr (OP == OP_ROP) {
+    if constexpr (ROPX) {
 #pragma unroll
       for (int j = 0; j < NMID; ++j)
         if (slot_valid<N, REV1, NL, true>(j, lane))
@@ -365,7 +387,8 @@ __host__ __device__ constexpr int col_lanes() {
 }
 
 template <typename T, int N, int TX, int OP>
-__global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N>())) void col_kernel(ColArgs<T> A) {
+__global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) void col_kernel(
+    ColArgs<T> A) {
   constexpr int NL = col_lanes<N>();
   constexpr bool REV1 = (OP == OP_INV);
   constexpr int DIR1 = (OP == OP_INV) ? +1 : -1;
@@ -390,9 +413,10 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N>())) void 
   const bool xin = x < A.W;
   const size_t HW = size_t(N) * A.W;
 
-  constexpr bool KOP = (OP == OP_DIAG || OP == OP_WIN);
+  constexpr bool KOP = (OP == OP_DIAG || OP == OP_WIN || OP == OP_SPLIT);
   cpx<T> K[KOP ? NMID : 1];
   T Wv[(OP == OP_WIN) ? NMID : 1];  // window factor cx(kx) * cy(ky)
+  const double* cys = nullptr;        // OP_SPLIT: cy [wn][H] of this beam and wavelength
   if constexpr (KOP) {
     const int ray0 = (lg * A.ncl) / A.NC;  // all lines of a work-group share one beam
     const int beam = ray0 / A.bm.rays_per_beam;
@@ -403,6 +427,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N>())) void 
       if (xin) cx = A.wcx[(bl * A.wn + A.wi) * A.W + x];
       cyt = A.wcy + (bl * A.wn + A.wj) * N;
     }
+    if constexpr (OP == OP_SPLIT) cys = A.wcy + (size_t(beam) * A.b.nlam + lam) * A.wn * N;
 #pragma unroll
     for (int j = 0; j < NMID; ++j) {
       K[j] = mk<T>(T(0), T(0));
@@ -433,7 +458,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N>())) void 
   }
   for (int li = 0; li < A.ncl; ++li) {
     const int comp = (lg * A.ncl + li) % A.NC;
-    cpx<T>* base = (OP == OP_WIN) ? A.out + line_off(li) : line_base(li);
+    cpx<T>* base = (OP == OP_WIN || OP == OP_SPLIT) ? A.out + line_off(li) : line_base(li);
     cpx<T> v[MAXV];
 #pragma unroll
     for (int j = 0; j < NIN; ++j) v[j] = nxt[j];
@@ -457,6 +482,29 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N>())) void 
         }
         if constexpr (OP == OP_WIN) k = k * Wv[j];
         v[j] = v[j] * k;
+      }
+      if constexpr (OP == OP_SPLIT) {
+        // one inverse transform per window row j: cy_j(ky) times the line
+        constexpr int NOUT = slot_count<N, true, NL, true>();
+        const size_t fstride = size_t(A.b.nrays) * A.b.nlam * A.NC * HW;
+        for (int wj = 0; wj < A.wn; ++wj) {
+          cpx<T> u[MAXV];
+#pragma unroll
+          for (int j = 0; j < NMID; ++j) {
+            T cy = T(0);
+            if (slot_valid<N, REV1, NL, true>(j, g)) cy = T(cys[wj * N + slot_pos<N, REV1, NL, true>(j, g)]);
+            u[j] = v[j] * cy;
+          }
+          __syncthreads();
+          reg_fft<T, N, true, +1, NL, TX, 0>(u, lds + c, tws, g);
+          cpx<T>* ob = base + wj * fstride;
+#pragma unroll
+          for (int j = 0; j < NOUT; ++j)
+            if (xin && slot_valid<N, true, NL, true>(j, g))
+              ob[size_t(slot_pos<N, true, NL, true>(j, g)) * A.W] = u[j];
+        }
+        __syncthreads();
+        continue;
       }
       reg_fft<T, N, true, +1, NL, TX, 0>(v, lds + c, tws, g);
     }
