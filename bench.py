@@ -244,6 +244,7 @@ def main():
         d.n_rays, d.rays_per_beam, d.method, d.precision = n_rays, rays_per_beam, 4, 0
         d.betamax, d.k0, d.beta, d.phi = 0.8, _lib.dptr(ks), _lib.dptr(bb), _lib.dptr(pp)
         d.diffraction, d.npass, d.norm, d.smooth = diffraction, 1, 1, 0.1
+        d.reflection = 1
         rc = lib.dtmmx_host_windows(ctypes.byref(d), diffraction * diffraction,
                                     ctypes.byref(nwin_c), None, None, None, None)
         _lib.check(rc)

@@ -60,6 +60,7 @@ class PlanDesc(ctypes.Structure):
         ("npass", ctypes.c_int),
         ("norm", ctypes.c_int),
         ("smooth", ctypes.c_double),
+        ("reflection", ctypes.c_int),
     ]
 
 

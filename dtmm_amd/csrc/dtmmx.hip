@@ -643,11 +643,29 @@ static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, voi
     h.e0 = L > 0 ? st[0].e : mk<double>(S->n_out * S->n_out);
     h.ein = mk<double>(S->n_in * S->n_in);
     h.scale = scale;
+    h.fresnel = P->d.reflection;
     TRY(kspace<T>(P, out, e2, b, h, s));
     TRY(col_pass<T>(P, e2, b, 2, OP_INV, s));
+    const cd eout = mk<double>(S->n_out * S->n_out);
     for (int l = 0; l < L; ++l) {
       const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 0, status);
       TRY(row_pass<T>(P, e2, b, 2, OP_ROP, s, &la));
+      const cd enext = (l + 1 < L) ? st[l + 1].e : eout;
+      if (P->d.reflection && !eq(st[l].e, enext)) {
+        // reflection mode 1: Fresnel transmission at the interface between
+        // different effective media (propagate_2x2.py:28-30, :62-65)
+        TRY(col_pass<T>(P, e2, b, 2, OP_FWD, s));
+        KArgs<T> tr{};
+        tr.kop = KOP_TR2;
+        tr.d0 = st[l].d;
+        tr.e0 = st[l].e;
+        tr.d1 = (l + 1 < L) ? st[l + 1].d : 0.;
+        tr.e1 = enext;
+        tr.scale = scale;
+        TRY(kspace<T>(P, e2, e2, b, tr, s));
+        TRY(col_pass<T>(P, e2, b, 2, OP_INV, s));
+        continue;
+      }
       ColArgs<T> ca{};
       ca.da = st[l].d;
       ca.ea = st[l].e;
@@ -1089,6 +1107,10 @@ static int validate_desc(const dtmmx_plan_desc* d) {
     return fail(DTMMX_E_UNSUPPORTED, "npass > 1 is built for the 4x4 method only");
   if (d->npass > 1 && d->norm != 0 && d->norm != DTMMX_NORM_FFT)
     return fail(DTMMX_E_UNSUPPORTED, "npass > 1: only norm 0 (total) and 1 (fft) are built");
+  if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1)
+    return fail(DTMMX_E_UNSUPPORTED, "the 4x4 method is built for reflection mode 1");
+  if (d->method == DTMMX_METHOD_2X2 && d->reflection != 0 && d->reflection != 1)
+    return fail(DTMMX_E_UNSUPPORTED, "the 2x2 method is built for reflection modes 0 and 1");
   return DTMMX_OK;
 }
 

@@ -522,7 +522,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
 // Dense per-k operators (fp64), one thread per (k, plane).
 // ---------------------------------------------------------------------------
 
-enum { KOP_HEAD4 = 0, KOP_TAIL4 = 1, KOP_BOUND4 = 2, KOP_PROJ2 = 3 };
+enum { KOP_HEAD4 = 0, KOP_TAIL4 = 1, KOP_BOUND4 = 2, KOP_PROJ2 = 3, KOP_TR2 = 4 };
 
 template <typename T>
 struct KArgs {
@@ -537,8 +537,24 @@ struct KArgs {
   cd ein;         // input medium eps (PROJ2)
   double scale;
   int nocorr;     // 1: no mean-beam correction (HEAD4, TAIL4, BOUND4)
+  int fresnel;    // PROJ2: 1 applies the interface transmission ein -> e0
   Beams bm;
 };
+
+// Fresnel transmission of the tangential E field from isotropic medium e0
+// into e1 at in-plane wave vector beta: p (in the plane of incidence)
+// 2 g0 / (g0 + g1) with g = eps / a, s 2 a0 / (a0 + a1), a = sqrt(eps - beta^2);
+// zero where a mode does not exist (a == 0).
+__device__ inline void fresnel_t(double beta, cd e0, cd e1, cd& tp, cd& ts) {
+  const cd a0 = csqrt(e0 - mk<double>(beta * beta)), a1 = csqrt(e1 - mk<double>(beta * beta));
+  if (is_zero(a0) || is_zero(a1)) {
+    tp = ts = mk<double>(0.);
+    return;
+  }
+  const cd g0 = cdiv(e0, a0), g1 = cdiv(e1, a1);
+  tp = cdiv(g0 * 2., g0 + g1);
+  ts = cdiv(a0 * 2., a0 + a1);
+}
 
 // Modes of an isotropic medium at in-plane wave vector (beta, phi), rotated.
 __device__ inline void iso_at(double beta, double phi, cd eps, Modes& m) {
@@ -617,8 +633,8 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
   const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
   const double beta = (2 * M_PI / k0) * sqrt(fx * fx + fy * fy);
   const double phi = atan2(fy, fx);
-  const int nin = 4;
-  const int nout = (A.kop == KOP_PROJ2) ? 2 : 4;
+  const int nin = (A.kop == KOP_TR2) ? 2 : 4;
+  const int nout = (A.kop == KOP_PROJ2 || A.kop == KOP_TR2) ? 2 : 4;
   const cpx<T>* s = A.src + size_t(q) * nin * HW + k;
   cpx<T>* o = A.dst + size_t(q) * nout * HW + k;
   cd v[4];
@@ -677,7 +693,13 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
       const cd igp = cdiv(ain, A.ein), igs = cdiv(mk<double>(-1.), ain);
       const cd g0 = v[0] * c + v[2] * sn, g1 = v[1] * c - v[3] * sn;
       const cd g2 = v[2] * c - v[0] * sn, g3 = v[1] * sn + v[3] * c;
-      const cd h0 = (g0 + g1 * igp) * 0.5, h2 = (g2 + g3 * igs) * 0.5;
+      cd h0 = (g0 + g1 * igp) * 0.5, h2 = (g2 + g3 * igs) * 0.5;
+      if (A.fresnel) {
+        cd tp, ts;
+        fresnel_t(beta, A.ein, A.e0, tp, ts);
+        h0 = h0 * tp;
+        h2 = h2 * ts;
+      }
       cd ph = mk<double>(0.);
       if (A.d0 != 0.) {
         cd ak = csqrt(A.e0 - mk<double>(beta * beta));
@@ -688,6 +710,23 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
       r[0] = K * (h0 * c - h2 * sn);
       r[1] = K * (h0 * sn + h2 * c);
     }
+  } else if (pass && A.kop == KOP_TR2) {
+    // K_b . T(e0 -> e1) . K_a on (Ex, Ey): half step of the layer before
+    // the interface, Fresnel transmission of the tangential E field
+    // (dtmm/tmm.py:925-950 for isotropic media), half step after it.
+    cd ph = mk<double>(0.);
+    if (A.d0 != 0.)
+      ph = ph + (csqrt(A.e0 - mk<double>(beta * beta)) - csqrt(A.e0 - mk<double>(bb * bb))) *
+                    (0.5 * A.d0 * k0);
+    if (A.d1 != 0.)
+      ph = ph + (csqrt(A.e1 - mk<double>(beta * beta)) - csqrt(A.e1 - mk<double>(bb * bb))) *
+                    (0.5 * A.d1 * k0);
+    const cd K = cexpi(ph);
+    cd tp, ts;
+    fresnel_t(beta, A.e0, A.e1, tp, ts);
+    const cd ep = (v[0] * c + v[1] * sn) * tp, es = (v[1] * c - v[0] * sn) * ts;
+    r[0] = K * (ep * c - es * sn);
+    r[1] = K * (ep * sn + es * c);
   } else if (pass && A.kop == KOP_BOUND4) {
     cd C0[4][4], D0[4][4], C1[4][4], D1[4][4], X[4][4], Y[4][4], M[4][4];
     if (A.nocorr) {

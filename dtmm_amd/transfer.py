@@ -6,8 +6,9 @@ return value and errors) for the modes built on the GPU:
 * method "4x4": reflection mode 1, diffraction >= 1, npass >= 1 (norm
   DTMM_NORM_FFT or 0) (dtmm/transfer.py:618-884, dtmm/propagate_4x4.py:376-444,
   :80-101);
-* method "2x2": reflection mode 0, diffraction 1, npass 1
-  (dtmm/transfer.py:1009-1183, dtmm/propagate_2x2.py:193-211, :19-87).
+* method "2x2" (and the configured default method '"2x2"'): reflection
+  mode 0 or 1, diffraction 1, npass 1 (dtmm/transfer.py:1009-1183,
+  dtmm/propagate_2x2.py:193-211, :19-87).
 
 Everything else the reference accepts raises ``NotImplementedError``.  The
 propagation runs in ``libdtmmx.so`` (HIP, gfx950); torch provides device
@@ -162,7 +163,8 @@ class Plan:
     """Owner of a ``dtmmx_plan``: grid, wavelengths, beams, method, precision."""
 
     def __init__(self, shape, ks, beta, phi, n_rays, rays_per_beam, method, precision="c64",
-                 betamax=BETAMAX, diffraction=1, npass=1, norm=DTMM_NORM_FFT, smooth=SMOOTH):
+                 betamax=BETAMAX, diffraction=1, npass=1, norm=DTMM_NORM_FFT, smooth=SMOOTH,
+                 reflection=None):
         lib = _lib.load()
         self.shape = tuple(shape)
         self.ks = np.ascontiguousarray(ks, np.float64)
@@ -189,6 +191,7 @@ class Plan:
         d.npass = self.npass
         d.norm = int(norm)
         d.smooth = float(smooth)
+        d.reflection = int(reflection if reflection is not None else (1 if method == "4x4" else 0))
         h = ctypes.c_void_p()
         _lib.check(lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)))
         self._h = h
@@ -273,10 +276,8 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
         if not (diffraction >= 1) or diffraction == np.inf:
             raise NotImplementedError("4x4 method: only finite diffraction >= 1 is supported")
     else:
-        if reflection != 0:
-            raise NotImplementedError("2x2 method: only reflection mode 0 is supported "
-                                      "(note: the configured default method '\"2x2\"' selects "
-                                      "reflection mode 1; pass method='2x2')")
+        if reflection not in (0, 1):
+            raise NotImplementedError("2x2 method: reflection mode 2 is not supported")
         if diffraction != 1:
             raise NotImplementedError("2x2 method: only diffraction=1 is supported")
     return kind, reflection
@@ -330,7 +331,8 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
     rays_per_beam = n_rays // len(b)
 
     plan = Plan(shape, ks, b, p, n_rays, rays_per_beam, kind, precision, betamax,
-                diffraction=int(diffraction), npass=int(npass), norm=norm, smooth=smooth)
+                diffraction=int(diffraction), npass=int(npass), norm=norm, smooth=smooth,
+                reflection=reflection)
     stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout, split_layers=split_layers)
     dt = plan.dtype
     fin = torch.from_numpy(np.ascontiguousarray(field_np.reshape((n_rays, len(ks), 4) + shape)))

@@ -81,6 +81,9 @@ typedef struct {
                          transfer.py:206-221); DTMM_NORM_REF is not built */
   double smooth;      /* npass > 1: notch width of dtmm/denoise.py (dtmm.ini:8,
                          0.1); negative values count as 0                 */
+  int reflection;     /* 4x4: 1.  2x2: 0, or 1 = Fresnel transmission at
+                         interfaces between different effective media
+                         (propagate_2x2.py:28-30, diffract.py:215-225)    */
 } dtmmx_plan_desc;
 
 /* A layer stack.  Layer l is propagated as n_steps[l] sub-steps of
@@ -115,7 +118,7 @@ size_t dtmmx_plan_workspace_size(const dtmmx_plan* plan);
  *   4x4 (reflection mode 1, diffraction >= 1, npass >= 1): replaces
  *       transfer.py:694-875 with propagate_4x4.py:376-444 and :80-101 per
  *       layer.
- *   2x2 (reflection mode 0, diffraction 1, npass 1): replaces
+ *   2x2 (reflection mode 0 or 1, diffraction 1, npass 1): replaces
  *       transfer.py:1071-1176 with propagate_2x2.py:193-211 and :19-87 per
  *       interface.
  * field_in and field_out are device buffers of n_rays*n_wavelengths*4*H*W

@@ -12,8 +12,9 @@ Scope (what the GPU engine implements and this module checks):
 * ``transfer_4x4`` with reflection=1, diffraction >= 1, npass >= 1 (dtmm/transfer.py,
   dtmm/propagate_4x4.py); diffraction > 1 uses the tent windows of dtmm/mode.py;
   npass > 1 the normalisations of transfer.py and dtmm/denoise.py
-* ``transfer_2x2`` with reflection=0, diffraction in {0, 1}, npass=1
-  (dtmm/transfer.py, dtmm/propagate_2x2.py)
+* ``transfer_2x2`` with reflection in {0, 1}, diffraction in {0, 1}, npass=1
+  (dtmm/transfer.py, dtmm/propagate_2x2.py); reflection 1 applies the
+  Fresnel interface transmission of dtmm/tmm.py tr_mat
 * ``illumination_data`` and ``field2intensity``            (dtmm/field.py)
 
 Pinning of each function is done in ``tests/test_oracle_*.py`` (golden
@@ -37,7 +38,7 @@ __all__ = [
     "transfer_field", "illumination_data", "blackman", "planewave",
     "jonesvec", "avec", "fvec", "auxiliary_matrix", "poynting", "set_fft_workers",
     "fft_window", "fft_windows", "fft_betaxy_mean", "fft_mask",
-    "DTMM_NORM_FFT", "DTMM_NORM_REF", "exp_notch_filter", "denoise_fftfield", "denoise_field",
+    "DTMM_NORM_FFT", "DTMM_NORM_REF", "S_mat", "Etri_mat", "tr_mat", "E_tr_matrix", "exp_notch_filter", "denoise_fftfield", "denoise_field",
     "normalize_field", "project_normalized_fft", "project_normalized_total", "total_intensity",
 ]
 
@@ -931,16 +932,58 @@ def propagate_4x4_effective_1(field, ks, layer, effective_layer, beta=0, phi=0, 
     return fout
 
 
-def _transfer_ray_2x2_1(fft_field, ks, layer, dmat1, dmat2, beta, phi, nsteps, mode, bulk):
-    """Forward 2x2 layer step without reflections
-    (dtmm/propagate_2x2.py:19-87 with reflection=0, refl=None)."""
+def S_mat(fmatin, fmatout, fmatini=None, mode=+1):
+    """Blocks of F_in^-1 F_out coupling forward output modes to the input
+    modes of the same / opposite direction (dtmm/tmm.py:850-878)."""
+    if fmatini is None:
+        fmatini = inv(fmatin)
+    S = dotmm(fmatini, fmatout)
+    if mode == +1:
+        return S[..., ::2, ::2], S[..., 1::2, 0::2]
+    return S[..., 1::2, 1::2], S[..., 0::2, 1::2]
+
+
+def Etri_mat(fmatin, fmatout, fmatini=None, mode=+1):
+    """(dtmm/tmm.py:1025-1049)."""
+    A = E_mat(fmatin, mode=mode)
+    Ai = inv(A)
+    St, Sr = S_mat(fmatin, fmatout, fmatini, mode)
+    ei = dotmm(inv(St), Ai)
+    return ei, dotmm(Sr, ei)
+
+
+def tr_mat(fmatin, fmatout, fmatini=None, mode=+1):
+    """2x2 transmission and reflection matrices of the tangential E field at
+    an interface (dtmm/tmm.py:925-950)."""
+    er = E_mat(fmatin, mode=-mode)
+    et = E_mat(fmatout, mode=mode)
+    eti, eri = Etri_mat(fmatin, fmatout, fmatini, mode)
+    return dotmm(et, eti), dotmm(er, eri)
+
+
+def E_tr_matrix(shape, ks, epsv_in=(1., 1., 1.), epsa_in=(0., 0., 0.), epsv_out=(1., 1., 1.),
+                epsa_out=(0., 0., 0.), mode=+1, betamax=BETAMAX):
+    """Interface transmission/reflection on the FFT grid
+    (dtmm/diffract.py:215-225)."""
+    _, fin, fini = diffraction_alphaffi(shape, ks, epsv_in, epsa_in, betamax)
+    _, fout, _ = diffraction_alphaffi(shape, ks, epsv_out, epsa_out, betamax)
+    return tr_mat(fin, fout, fini, mode)
+
+
+def _transfer_ray_2x2_1(fft_field, ks, layer, dmat1, dmat2, beta, phi, nsteps, mode, bulk,
+                        tmat=None):
+    """Forward 2x2 layer step (dtmm/propagate_2x2.py:19-87 with refl=None):
+    reflection mode 1 passes the Fresnel transmission `tmat` of the
+    effective-medium interface, applied before the first sub-step."""
     d, epsv, epsa = layer
     alpha, fmat = alphaf(beta, phi, epsv, epsa)
     e = E_mat(fmat, mode=mode)
     ei = inv(e)
     kd = ks * d
     p = phase_mat(alpha, kd[..., None, None], mode=mode)
-    for _ in range(nsteps):
+    for j in range(nsteps):
+        if j == 0 and tmat is not None:
+            fft_field = dotmf(tmat, fft_field)
         if dmat1 is not None:
             fft_field = dotmf(dmat1, fft_field)
         field = ifft2(fft_field)
@@ -958,11 +1001,15 @@ def _transfer_ray_2x2_1(fft_field, ks, layer, dmat1, dmat2, beta, phi, nsteps, m
 
 def propagate_2x2_effective_1(field, ks, layer_out, effective_layer_out, beta=0, phi=0,
                               nsteps=1, diffraction=1, betamax=BETAMAX, mode=+1, bulk=None,
-                              cache=None):
-    """Layer step of the 2x2 method, reflection mode 0
+                              cache=None, reflection=0, effective_layer_in=None):
+    """Layer step of the 2x2 method, reflection mode 0 or 1
     (dtmm/propagate_2x2.py:193-211; diffraction in {0, 1})."""
     d_out, epsv_out, epsa_out = effective_layer_out
     shape = field.shape[-2:]
+    tmat = None
+    if reflection:
+        _, epsv_in, epsa_in = effective_layer_in
+        tmat = E_tr_matrix(shape, ks, epsv_in, epsa_in, epsv_out, epsa_out, mode, betamax)[0]
     if diffraction == 1:
         cache = cache or _Cache()
         key = _key(shape, ks, beta, phi, d_out, epsv_out, epsa_out, betamax, mode)
@@ -974,7 +1021,8 @@ def propagate_2x2_effective_1(field, ks, layer_out, effective_layer_out, beta=0,
         dmat1 = dmat2 = None
     else:
         raise NotImplementedError("oracle covers diffraction in {0, 1} for the 2x2 method")
-    return _transfer_ray_2x2_1(field, ks, layer_out, dmat1, dmat2, beta, phi, nsteps, mode, bulk)
+    return _transfer_ray_2x2_1(field, ks, layer_out, dmat1, dmat2, beta, phi, nsteps, mode, bulk,
+                               tmat)
 
 
 # ---------------------------------------------------------------------------
@@ -1154,10 +1202,10 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
 
 def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1, nstep=1,
                  diffraction=1, reflection=0, betamax=BETAMAX, layer_callback=None):
-    """2x2 (E-field) transfer, npass=1, reflection mode 0
+    """2x2 (E-field) transfer, npass=1, reflection mode 0 or 1
     (dtmm/transfer.py:1009-1183)."""
-    if npass != 1 or reflection != 0:
-        raise NotImplementedError("oracle covers npass=1, reflection=0 for the 2x2 method")
+    if npass != 1 or reflection not in (0, 1):
+        raise NotImplementedError("oracle covers npass=1, reflection 0 or 1 for the 2x2 method")
     field_in, wavelengths, pixelsize = field_data
     field_in = np.asarray(field_in, CDTYPE)
     layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
@@ -1177,9 +1225,11 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     for j in range(n):
         nsub, layer_out = layers[j + 1]
         _, eff_out = eff_layers[j + 1]
+        _, eff_in = eff_layers[j]
         bulk = field_out if j + 1 == n else None
         field = propagate_2x2_effective_1(field, ks, layer_out, eff_out, beta, phi, nsub,
-                                          diffraction, betamax, +1, bulk, cache)
+                                          diffraction, betamax, +1, bulk, cache, reflection,
+                                          eff_in)
         if layer_callback is not None:
             layer_callback(j)
     return field_out, wavelengths, pixelsize

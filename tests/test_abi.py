@@ -45,6 +45,7 @@ def _desc(ks, beta, phi, **kw):
     d.npass = kw.get("npass", 1)
     d.norm = kw.get("norm", 1)
     d.smooth = 0.1
+    d.reflection = kw.get("reflection", 1 if d.method == 4 else 0)
     return d
 
 
@@ -67,6 +68,7 @@ def _plan_rc(**kw):
     d.npass = kw.get("npass", 1)
     d.norm = kw.get("norm", 1)
     d.smooth = 0.1
+    d.reflection = kw.get("reflection", 1 if d.method == 4 else 0)
     h = ctypes.c_void_p()
     return lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)), lib.dtmmx_last_error()
 
@@ -85,6 +87,8 @@ def _plan_rc(**kw):
     (dict(npass=0), _lib.DTMMX_E_ARG),
     (dict(method=2, npass=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(npass=3, norm=2), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(reflection=2), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(method=2, reflection=2), _lib.DTMMX_E_UNSUPPORTED),
 ])
 def test_plan_create_rejects_bad_descriptions(kw, code):
     rc, msg = _plan_rc(**kw)

@@ -206,3 +206,34 @@ def test_1024_point_lines_windows_npass(shape):
     fd, optical = random_case(shape=shape, wavelengths=(550.,), jones=None, beta=beta, phi=phi)
     _compare_npass(fd, optical, "c64", beta=beta, phi=phi, npass=3, diffraction=3, nin=1.52,
                    nout=1.52)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_2x2_reflection1(precision):
+    """2x2 with reflection mode 1: Fresnel transmission at the input, block
+    and output interfaces (propagate_2x2.py:28-30)."""
+    flush("case 2x2 reflection1 %s" % precision)
+    fd, optical = random_case(shape=(6, 32, 48), wavelengths=(450., 650.), jones=None)
+    _compare(fd, optical, "2x2", precision, reflection=1, nin=1.0, nout=1.33)
+
+
+@pytest.mark.gpu
+def test_2x2_reflection1_two_blocks_multi_beam():
+    flush("case 2x2 reflection1 two blocks multi beam")
+    beta = [0., 0.2]
+    phi = [0., 1.0]
+    fd, b1 = random_case(shape=(3, 32, 32), wavelengths=(550.,), jones=None, beta=beta, phi=phi)
+    _, b2 = droplet_case(shape=(4, 32, 32), radius=12, ne=1.75, nhost=1.55)
+    _compare(fd, [b1[0], b2[0]], "2x2", "c128", beta=beta, phi=phi, reflection=1, nin=1.2,
+             nout=1.0, nstep=2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_default_method(precision):
+    """transfer_field without method: the configured '"2x2"' default, which
+    selects reflection mode 1 (transfer.py:459)."""
+    flush("case default method %s" % precision)
+    fd, optical = droplet_case(shape=(8, 32, 32), radius=12)
+    _compare(fd, optical, None, precision)

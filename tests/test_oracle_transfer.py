@@ -154,3 +154,69 @@ def test_notch_filter_closed_form():
     np.testing.assert_allclose(f[2], 0.)
     np.testing.assert_allclose(f[1], (1 - np.exp(-7.5)) / (1 - np.exp(-15.)))
     np.testing.assert_allclose(f[4], 1.)
+
+
+def test_interface_transmission_is_fresnel():
+    """E_tr_matrix (dtmm/diffract.py:215-225, tmm.py tr_mat) on the FFT grid
+    against the textbook Fresnel transmission of the tangential E field:
+    s: 2 a1 / (a1 + a2); p: 2 (e1/a1) / (e1/a1 + e2/a2), a = sqrt(eps - beta^2),
+    in the plane of incidence (rotation by the mode azimuth)."""
+    shape = (12, 16)
+    ks = o.k0(np.array([500., 640.]), 100.)
+    n1, n2 = 1.0, 1.6
+    t, r = o.E_tr_matrix(shape, ks, o.refind2eps([n1] * 3), (0., 0., 0.),
+                         o.refind2eps([n2] * 3), (0., 0., 0.))
+    beta, phi = o.betaphi(shape, ks)
+    a1 = np.sqrt(n1 ** 2 - beta ** 2 + 0j)
+    a2 = np.sqrt(n2 ** 2 - beta ** 2 + 0j)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ts = 2 * a1 / (a1 + a2)
+        g1, g2 = n1 ** 2 / a1, n2 ** 2 / a2
+        tp = 2 * g1 / (g1 + g2)
+    c, s = np.cos(phi), np.sin(phi)
+    exp = np.empty(beta.shape + (2, 2), complex)
+    exp[..., 0, 0] = c * c * tp + s * s * ts
+    exp[..., 0, 1] = c * s * (tp - ts)
+    exp[..., 1, 0] = c * s * (tp - ts)
+    exp[..., 1, 1] = s * s * tp + c * c * ts
+    m = beta < o.BETAMAX
+    np.testing.assert_allclose(t[m], exp[m], atol=1e-13)
+    np.testing.assert_array_equal(t[~m], 0.)
+
+
+def test_2x2_reflection1_slab_is_single_pass_fresnel():
+    """2x2 with reflection mode 1 on a homogeneous isotropic slab: the
+    interfaces transmit with the Fresnel coefficients (no multiple
+    reflections in the 2x2 method)."""
+    nin, ns, nout, L = 1.0, 1.5, 1.25, 7
+    shape = (8, 8)
+    wl = np.array([550., 633.])
+    ks = o.k0(wl, 100.)
+    f = np.zeros((len(wl), 4) + shape, complex)
+    f[:, 0], f[:, 1] = 1., nin
+    eps = np.broadcast_to([ns ** 2] * 3, (L, 1, 1, 3))
+    out = o.transfer_field((f, wl, 100.), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))], beta=0.,
+                           phi=0., method="2x2", reflection=1, nin=nin, nout=nout)[0]
+    for i, k in enumerate(ks):
+        t = 2 * nin / (nin + ns) * 2 * ns / (ns + nout) * np.exp(1j * k * ns * L)
+        np.testing.assert_allclose(out[i, 0], t, atol=1e-12)
+        np.testing.assert_allclose(out[i, 1], nout * t, atol=1e-12)
+
+
+def test_default_method_is_2x2_with_reflection():
+    """The configured default method is the quoted string '"2x2"'
+    (dtmm.ini:23 through configparser), so transfer_field without a method
+    takes the 2x2 path with reflection mode 1 (transfer.py:459)."""
+    nin, ns, nout, L = 1.0, 1.5, 1.25, 3
+    shape = (8, 8)
+    wl = np.array([550.])
+    f = np.zeros((1, 4) + shape, complex)
+    f[:, 0], f[:, 1] = 1., nin
+    eps = np.broadcast_to([ns ** 2] * 3, (L, 1, 1, 3))
+    opt = [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))]
+    a = o.transfer_field((f, wl, 100.), opt, beta=0., phi=0., nin=nin, nout=nout)[0]
+    b = o.transfer_field((f, wl, 100.), opt, beta=0., phi=0., method="2x2", reflection=1,
+                         nin=nin, nout=nout)[0]
+    c = o.transfer_field((f, wl, 100.), opt, beta=0., phi=0., method="2x2", nin=nin, nout=nout)[0]
+    np.testing.assert_array_equal(a, b)
+    assert np.abs(a - c).max() > 1e-3
