@@ -18,6 +18,14 @@ glass (nin = nout = 1.52), 5 wavelengths 450-650 nm, 3 Blackman-windowed
 beams at beta 0.1, phi 0/120/240 deg, both polarisations (30 planes), 4x4,
 npass = 3, diffraction = 5; value there counts npass passes.
 
+--workload c3 measures a batch of 64 uneven samples in one batched engine
+call: grids uniform over {128,192,256,384,512}^2, depth uniform in 32..160
+layers, radial droplet (radius 30/96 of the grid) or smooth random texture,
+ne^2 uniform in [1.6^2, 1.8^2] (no 1.5), 7 beams (beta 0, and 0.2 at
+phi = k 60 deg) x 2 polarisations x 9 wavelengths 380-780 nm (126 planes),
+4x4, c64.  Samples of one (kind, grid) share the director of a 160-layer
+volume cut to their depth.
+
 Prints ONE JSON line.  Options: --method 4x4|2x2, --precision c64|c128.
 """
 import argparse
@@ -49,6 +57,117 @@ def log(msg):
 
 
 C5_BEAMS = ([0.1, 0.1, 0.1], [0., 2 * np.pi / 3, 4 * np.pi / 3])
+C3_BEAMS = ([0.] + [0.2] * 6, [0.] + [k * np.pi / 3 for k in range(6)])
+C3_SIZES = (128, 192, 256, 384, 512)
+
+
+def build_inputs_c3(nsamples, nlam, seed=0):
+    """Samples of the uneven batch: list of (field_data, optical_data, info)."""
+    import dtmm_amd as dm
+    from dtmm_amd.samples import random_director
+    t = time.time()
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(nsamples):
+        size = int(rng.choice(C3_SIZES))
+        layers = int(rng.integers(32, 161))
+        kind = str(rng.choice(["droplet", "random"]))
+        ne = float(np.sqrt(rng.uniform(1.6 ** 2, 1.8 ** 2)))
+        specs.append((size, layers, kind, ne))
+    wl = np.linspace(380., 780., nlam)
+    beta, phi = C3_BEAMS
+    fields, dirs = {}, {}
+    for size, _, kind, _ in specs:
+        if size not in fields:
+            f = dm.illumination_data((size, size), wl, pixelsize=100, beta=beta, phi=phi,
+                                     jones=None, window=dm.blackman((size, size)))[0]
+            fields[size] = f.astype(np.complex64)
+        key = (kind, size)
+        if key not in dirs:
+            shape = (160, size, size)
+            if kind == "random":
+                dirs[key] = (random_director(shape, seed=size), None)
+            else:
+                dirs[key] = (dm.nematic_droplet_director(shape, 30. * size / 96, "r"), None)
+    samples = []
+    for size, layers, kind, ne in specs:
+        director = dirs[(kind, size)][0][:layers]
+        if kind == "random":
+            optical = [dm.director2data(director, no=1.5, ne=ne)]
+        else:
+            mask = np.linalg.norm(director, axis=-1) > 0
+            optical = [dm.director2data(director, mask=mask, no=1.5, ne=ne, nhost=1.5)]
+        samples.append(((fields[size], wl, 100.), optical, (size, layers, kind, ne)))
+    log("c3 inputs built in %.1f s: %d samples" % (time.time() - t, len(samples)))
+    return samples
+
+
+def run_c3(args):
+    """The uneven batch through dtmm_amd.transfer_batch's engine call."""
+    import torch
+    from dtmm_amd import _lib
+    from dtmm_amd.transfer import _prepare
+    lib = _lib.load()
+    samples = build_inputs_c3(args.samples, args.wavelengths)
+    beta, phi = C3_BEAMS
+    t = time.time()
+    jobs = []
+    steps = 0
+    for fd, optical, (size, layers, kind, ne) in samples:
+        jobs.append(_prepare(fd, optical, beta=beta, phi=phi, method=args.method,
+                             precision=args.precision, nin=1.5, nout=1.5))
+        steps += size * size * layers * int(np.prod(fd[0].shape[:-3]))
+    torch.cuda.synchronize()
+    log("c3 uploaded in %.1f s (%.3g voxel*lambda*plane steps per batch)"
+        % (time.time() - t, steps))
+    structs = [j.stack.c_struct() for j in jobs]
+    arr = (_lib.Job * len(jobs))()
+    for a, j, st in zip(arr, jobs, structs):
+        a.plan = j.plan._h.value
+        a.stack = ctypes.pointer(st)
+        a.field_in = j.fin.data_ptr()
+        a.field_out = j.fout.data_ptr()
+        a.workspace = j.ws.data_ptr() if j.ws is not None else None
+        a.workspace_bytes = j.plan.workspace_bytes
+    stream = torch.cuda.current_stream()
+    status = torch.zeros(4, dtype=torch.int32, device="cuda")
+
+    def step():
+        _lib.check(lib.dtmmx_transfer_batch(arr, len(jobs), status.data_ptr(),
+                                            stream.cuda_stream))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    for j in jobs[:4]:
+        if not torch.isfinite(torch.view_as_real(j.fout)).all():
+            log("non-finite output")
+            sys.exit(1)
+    sizes = sorted(set(smp[2][0] for smp in samples))
+    rec = {
+        "metric": "voxel*lambda steps/s (transfer_field, 4x4 unless noted), c64 with c128 "
+                  "reference, % HBM peak",
+        "value": steps / (ms_per_step / 1e3),
+        "unit": "voxel*lambda steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (fp32 complex, fp64 eigensolve)" if args.precision == "c64" else "c128",
+        "data": "synthetic (seeded droplet / random nematic samples, plane-wave illumination)",
+        "config": {"workload": "batch of %d uneven samples (grids %s, 32-160 layers), 7 beams x "
+                               "2 pol x %d wavelengths, %s, one dtmmx_transfer_batch call"
+                               % (len(jobs), sizes, args.wavelengths, args.method),
+                   "samples": len(jobs), "planes_per_sample": 14 * args.wavelengths,
+                   "method": args.method, "precision": args.precision},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(rec), flush=True)
 
 
 def build_inputs_c5(size, layers, nlam):
@@ -146,9 +265,10 @@ def cpu_baseline(fd, optical, method, target_s=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c5"],
-                    help="c2: 512^2 x 128 random texture (default); c5: 1024^2 x 200 droplet "
-                         "grid with npass=3, diffraction=5")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"],
+                    help="c2: 512^2 x 128 random texture (default); c3: batch of uneven "
+                         "samples; c5: 1024^2 x 200 droplet grid with npass=3, diffraction=5")
+    ap.add_argument("--samples", type=int, default=64, help="c3: batch size")
     ap.add_argument("--cpu-crop", type=int, default=96, help="c5: oracle crop size")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
@@ -192,6 +312,9 @@ def main():
     if not torch.cuda.is_available():
         log("no GPU")
         sys.exit(1)
+    if args.workload == "c3":
+        run_c3(args)
+        return
     lib = _lib.load()
     c5 = args.workload == "c5"
     if c5:

@@ -1,16 +1,17 @@
 """dtmm_amd -- MI355X-native engine for the diffractive transfer-matrix path.
 
-Drop-in for ``dtmm.transfer_field`` (4x4 and 2x2 methods, diffraction=1,
-npass=1) with the reference's data model.  Propagation runs in hand-written
-HIP kernels for gfx950 (``dtmm_amd/csrc``) reached through the C ABI of
-``include/dtmmx.h``; there is no CPU fallback.
+Drop-in for ``dtmm.transfer_field`` (4x4 method: reflection mode 1, any
+diffraction and npass; 2x2 method: reflection modes 0 and 1) with the
+reference's data model, plus ``transfer_batch`` for samples of uneven size.
+Propagation runs in hand-written HIP kernels for gfx950 (``dtmm_amd/csrc``)
+reached through the C ABI of ``include/dtmmx.h``; there is no CPU fallback.
 """
 from .data import (refind2eps, uniaxial_order, director2data, director2angles,  # noqa: F401
                    director2order, nematic_droplet_director, nematic_droplet_data,
                    validate_optical_block, validate_optical_data)
 from .field import (illumination_data, field2intensity, blackman, k0, betaphi,  # noqa: F401
                     planewave)
-from .transfer import (transfer_field, Plan, Stack, DTMM_NORM_FFT, DTMM_NORM_REF,  # noqa: F401
-                       field2betaphi, mean_betaphi)
+from .transfer import (transfer_field, transfer_batch, Plan, Stack,  # noqa: F401
+                       DTMM_NORM_FFT, DTMM_NORM_REF, field2betaphi, mean_betaphi)
 
 __version__ = "0.1.0"

@@ -38,6 +38,7 @@ EXPORTS = (
     "dtmmx_timing_end",
     "dtmmx_host_rop",
     "dtmmx_host_windows",
+    "dtmmx_transfer_batch",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -79,6 +80,17 @@ class Stack(ctypes.Structure):
     ]
 
 
+class Job(ctypes.Structure):
+    _fields_ = [
+        ("plan", ctypes.c_void_p),
+        ("stack", ctypes.POINTER(Stack)),
+        ("field_in", ctypes.c_void_p),
+        ("field_out", ctypes.c_void_p),
+        ("workspace", ctypes.c_void_p),
+        ("workspace_bytes", ctypes.c_size_t),
+    ]
+
+
 class DtmmxError(RuntimeError):
     """Error code and message returned by the engine library."""
 
@@ -112,6 +124,9 @@ def load():
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                    ctypes.c_void_p, ctypes.c_void_p]
     lib.dtmmx_transfer.restype = ctypes.c_int
+    lib.dtmmx_transfer_batch.argtypes = [ctypes.POINTER(Job), ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+    lib.dtmmx_transfer_batch.restype = ctypes.c_int
     lib.dtmmx_fft2.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
     lib.dtmmx_fft2.restype = ctypes.c_int

@@ -1191,8 +1191,8 @@ size_t dtmmx_plan_workspace_size(const dtmmx_plan* P) {
   return ws_layout(P).total;
 }
 
-int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
-                   void* ws, size_t ws_bytes, int* status, void* stream) {
+static int check_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin,
+                          const void* fout, const void* ws, size_t ws_bytes) {
   if (!P || !S || !fin || !fout) return fail(DTMMX_E_ARG, "null argument");
   if (S->n_layers < 0) return fail(DTMMX_E_ARG, "negative layer count");
   if (S->n_layers > 0 && (!S->thickness || !S->eps_eff_re || !S->eps_re || !S->angles))
@@ -1203,6 +1203,35 @@ int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* f
   if (ws_bytes < dtmmx_plan_workspace_size(P) || (dtmmx_plan_workspace_size(P) && !ws))
     return fail(DTMMX_E_ARG, "workspace too small");
   if (fin == fout) return fail(DTMMX_E_ARG, "field_in and field_out must not alias");
+  return DTMMX_OK;
+}
+
+int dtmmx_transfer_batch(const dtmmx_job* jobs, int n, int* status, void* stream) {
+  if (n < 0 || (n > 0 && !jobs)) return fail(DTMMX_E_ARG, "bad job array");
+  for (int i = 0; i < n; ++i) {
+    const dtmmx_job& j = jobs[i];
+    const int r = check_transfer(j.plan, j.stack, j.field_in, j.field_out, j.workspace,
+                                 j.workspace_bytes);
+    if (r != DTMMX_OK) return fail(r, "job " + std::to_string(i) + ": " + g_err);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  for (int i = 0; i < n; ++i) {
+    const dtmmx_job& j = jobs[i];
+    const int r = j.plan->d.precision == DTMMX_C64
+                      ? transfer_c64(j.plan, j.stack, j.field_in, j.field_out, j.workspace, status, s)
+                      : transfer_c128(j.plan, j.stack, j.field_in, j.field_out, j.workspace, status,
+                                      s);
+    if (r != DTMMX_OK) return fail(r, "job " + std::to_string(i) + ": " + g_err);
+  }
+  return DTMMX_OK;
+}
+
+int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
+                   void* ws, size_t ws_bytes, int* status, void* stream) {
+  {
+    const int r = check_transfer(P, S, fin, fout, ws, ws_bytes);
+    if (r != DTMMX_OK) return r;
+  }
   hipStream_t s = (hipStream_t)stream;
   if (P->d.precision == DTMMX_C64) return transfer_c64(P, S, fin, fout, ws, status, s);
   return transfer_c128(P, S, fin, fout, ws, status, s);

@@ -3,7 +3,10 @@
 Host-side NumPy input builders with the reference's names and argument
 meanings (dtmm/field.py, dtmm/wave.py, dtmm/window.py).
 """
+import os
+
 import numpy as np
+import scipy.fft
 
 from .data import refind2eps
 
@@ -144,10 +147,11 @@ def _projection_fft(field, ks, n, mode, betamax):
     else:
         d[1::2] = 1.
     P = np.matmul(F * d, Fi)
-    ff = np.fft.fft2(field)
+    workers = len(os.sched_getaffinity(0))
+    ff = scipy.fft.fft2(field, workers=workers)
     v = np.moveaxis(ff, -3, -1)[..., None]
     v = np.matmul(P, v)[..., 0]
-    return np.fft.ifft2(np.moveaxis(v, -1, -3))
+    return scipy.fft.ifft2(np.moveaxis(v, -1, -3), workers=workers)
 
 
 def field2intensity(field):

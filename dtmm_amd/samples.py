@@ -13,23 +13,25 @@ from .data import director2data, refind2eps
 
 
 def _smooth_field(rng, shape, nmodes=16, kmax=8, sigma=0.3):
+    """Sum of nmodes random Fourier modes a cos(2 pi k.r / N + psi), as one
+    complex matrix product of the z factors with the (y, x) factors."""
     nz, ny, nx = shape
-    out = np.zeros(shape, np.float64)
     z = np.arange(nz) / nz
     y = np.arange(ny) / ny
     x = np.arange(nx) / nx
-    for _ in range(nmodes):
+    ez = np.empty((nmodes, nz), np.complex128)
+    exy = np.empty((nmodes, ny * nx), np.complex128)
+    for m in range(nmodes):
         while True:
             k = rng.integers(-kmax, kmax + 1, size=3)
             if 0 < np.sqrt((k ** 2).sum()) <= kmax:
                 break
         a = rng.normal(0., sigma)
         psi = rng.uniform(0., 2 * np.pi)
-        ez = np.exp(2j * np.pi * k[0] * z)
-        ey = np.exp(2j * np.pi * k[1] * y)
-        ex = np.exp(2j * np.pi * k[2] * x)
-        out += a * (np.exp(1j * psi) * ez[:, None, None] * ey[None, :, None] * ex[None, None, :]).real
-    return out
+        ez[m] = a * np.exp(1j * psi) * np.exp(2j * np.pi * k[0] * z)
+        exy[m] = (np.exp(2j * np.pi * k[1] * y)[:, None] *
+                  np.exp(2j * np.pi * k[2] * x)[None, :]).ravel()
+    return (ez.T @ exy).real.reshape(shape)
 
 
 def random_director(shape, seed=0, theta0=np.pi / 2, twist=np.pi / 2):

@@ -283,20 +283,34 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
     return kind, reflection
 
 
-def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
-                   npass=None, nstep=1, diffraction=None, reflection=None, method=None,
-                   multiray=False, norm=DTMM_NORM_FFT, betamax=None, smooth=SMOOTH,
-                   split_rays=False, split_diffraction=False, split_wavelengths=False,
-                   split_layers=False, eff_data=None, ret_bulk=False, out=None,
-                   precision=None):
-    """Transfers input field data through optical data (dtmm/transfer.py:322).
+class _Prepared:
+    """A transfer made ready on the device: plan, stack, fields, workspace."""
 
-    Returns ``(field_out, wavelengths, pixelsize)``.  ``precision`` ("c64" or
-    "c128") selects the GPU arithmetic; by default it follows the input
-    field dtype (complex64 -> "c64", otherwise "c128").  The split_* options
-    only bound memory in the reference and do not change the result; they
-    are accepted and ignored here (the GPU batches all planes).
-    """
+    def __init__(self, plan, stack, fin, fout, ws, field_in, field_np, npass, out, wavelengths,
+                 pixelsize):
+        self.plan, self.stack, self.fin, self.fout, self.ws = plan, stack, fin, fout, ws
+        self.field_in, self.field_np, self.npass, self.out = field_in, field_np, npass, out
+        self.wavelengths, self.pixelsize = wavelengths, pixelsize
+
+    def finish(self):
+        res = self.fout.cpu().numpy().reshape(self.field_np.shape)
+        if (self.npass > 1 and isinstance(self.field_in, np.ndarray)
+                and self.field_in.flags.writeable):
+            # the reference leaves the input-side field (with reflections) in
+            # the caller's input array (dtmm/transfer.py:843, :862-866)
+            self.field_in[...] = self.fin.cpu().numpy().reshape(self.field_np.shape)
+        if self.out is not None:
+            self.out[...] = res
+            res = self.out
+        return res, self.wavelengths, self.pixelsize
+
+
+def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None, npass=None,
+             nstep=1, diffraction=None, reflection=None, method=None, multiray=False,
+             norm=DTMM_NORM_FFT, betamax=None, smooth=SMOOTH, split_layers=False, eff_data=None,
+             ret_bulk=False, out=None, precision=None):
+    """Defaults, validation, plan and device data of one transfer_field call
+    (dtmm/transfer.py:433-552)."""
     torch = _require_gpu()
     nin = N_COVER if nin is None else nin
     nout = N_COVER if nout is None else nout
@@ -334,21 +348,84 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
                 diffraction=int(diffraction), npass=int(npass), norm=norm, smooth=smooth,
                 reflection=reflection)
     stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout, split_layers=split_layers)
-    dt = plan.dtype
     fin = torch.from_numpy(np.ascontiguousarray(field_np.reshape((n_rays, len(ks), 4) + shape)))
-    fin = fin.to(device="cuda", dtype=dt)
+    fin = fin.to(device="cuda", dtype=plan.dtype)
     fout = torch.empty_like(fin)
-    status = torch.zeros(4, dtype=torch.int32, device="cuda")
-    plan.execute(stack, fin, fout, status=status)
-    torch.cuda.synchronize()
+    ws = None
+    if plan.workspace_bytes:
+        ws = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
+    return _Prepared(plan, stack, fin, fout, ws, field_in, field_np, npass, out, wavelengths,
+                     pixelsize)
+
+
+def _check_status(status):
     if int(status[0].item()) & 1:
         raise NotImplementedError("biaxial voxels (eps0 != eps1) are not supported yet")
-    res = fout.cpu().numpy().reshape(field_np.shape)
-    if npass > 1 and isinstance(field_in, np.ndarray) and field_in.flags.writeable:
-        # the reference leaves the input-side field (with reflections) in
-        # the caller's input array (dtmm/transfer.py:843, :862-866)
-        field_in[...] = fin.cpu().numpy().reshape(field_np.shape)
-    if out is not None:
-        out[...] = res
-        res = out
-    return res, wavelengths, pixelsize
+
+
+def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
+                   npass=None, nstep=1, diffraction=None, reflection=None, method=None,
+                   multiray=False, norm=DTMM_NORM_FFT, betamax=None, smooth=SMOOTH,
+                   split_rays=False, split_diffraction=False, split_wavelengths=False,
+                   split_layers=False, eff_data=None, ret_bulk=False, out=None,
+                   precision=None):
+    """Transfers input field data through optical data (dtmm/transfer.py:322).
+
+    Returns ``(field_out, wavelengths, pixelsize)``.  ``precision`` ("c64" or
+    "c128") selects the GPU arithmetic; by default it follows the input
+    field dtype (complex64 -> "c64", otherwise "c128").  The split_* options
+    only bound memory in the reference and do not change the result; they
+    are accepted and ignored here (the GPU batches all planes).
+    """
+    torch = _require_gpu()
+    job = _prepare(field_data, optical_data, beta, phi, nin, nout, npass, nstep, diffraction,
+                   reflection, method, multiray, norm, betamax, smooth, split_layers, eff_data,
+                   ret_bulk, out, precision)
+    status = torch.zeros(4, dtype=torch.int32, device="cuda")
+    job.plan.execute(job.stack, job.fin, job.fout, workspace=job.ws, status=status)
+    torch.cuda.synchronize()
+    _check_status(status)
+    return job.finish()
+
+
+def transfer_batch(samples, **kwargs):
+    """Transfers several independent samples of uneven grid, depth and beams
+    in one batched engine call (``dtmmx_transfer_batch``).
+
+    ``samples`` is a sequence of ``(field_data, optical_data)`` pairs or of
+    dicts ``{"field_data": ..., "optical_data": ..., **per-sample options}``;
+    ``kwargs`` are transfer_field options shared by all samples.  Returns
+    the list of ``(field_out, wavelengths, pixelsize)``, each equal to what
+    transfer_field returns for that sample (the reference users loop
+    dtmm.transfer_field over their samples, dtmm/transfer.py:322)."""
+    torch = _require_gpu()
+    lib = _lib.load()
+    jobs = []
+    for smp in samples:
+        if isinstance(smp, dict):
+            opts = dict(kwargs)
+            opts.update({k: v for k, v in smp.items() if k not in ("field_data", "optical_data")})
+            fd, od = smp["field_data"], smp["optical_data"]
+        else:
+            fd, od = smp
+            opts = kwargs
+        for k in ("split_rays", "split_diffraction", "split_wavelengths"):
+            opts = {kk: vv for kk, vv in opts.items() if kk != k}
+        jobs.append(_prepare(fd, od, **opts))
+    if not jobs:
+        return []
+    structs = [j.stack.c_struct() for j in jobs]
+    arr = (_lib.Job * len(jobs))()
+    for a, j, st in zip(arr, jobs, structs):
+        a.plan = j.plan._h.value
+        a.stack = ctypes.pointer(st)
+        a.field_in = j.fin.data_ptr()
+        a.field_out = j.fout.data_ptr()
+        a.workspace = j.ws.data_ptr() if j.ws is not None else None
+        a.workspace_bytes = j.plan.workspace_bytes
+    status = torch.zeros(4, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    _lib.check(lib.dtmmx_transfer_batch(arr, len(jobs), status.data_ptr(), stream.cuda_stream))
+    torch.cuda.synchronize()
+    _check_status(status)
+    return [j.finish() for j in jobs]

@@ -132,6 +132,24 @@ int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field
                    void* field_out, void* workspace, size_t workspace_bytes, int* status,
                    void* stream);
 
+/* One independent transfer of a batch: its own plan (grid, wavelengths,
+ * beams, mode), stack, fields and workspace. */
+typedef struct {
+  const dtmmx_plan* plan;
+  const dtmmx_stack* stack;
+  void* field_in;
+  void* field_out;
+  void* workspace;
+  size_t workspace_bytes;
+} dtmmx_job;
+
+/* Batched execute for samples of uneven size and depth: runs the n jobs in
+ * order on `stream`, as n dtmmx_transfer calls would (the reference has no
+ * batch entry point; its users loop transfer_field over samples,
+ * transfer.py:322).  Every job is validated before any work is queued;
+ * status (device int[4], may be NULL) collects the flags of all jobs. */
+int dtmmx_transfer_batch(const dtmmx_job* jobs, int n, int* status, void* stream);
+
 /* In-place unnormalised 2D FFT of `batch` planes of H x W complex values
  * (forward exp(-i..) when inverse == 0), the transform of dtmm/fft.py:277
  * fft2 / :306 ifft2 without the 1/(HW) factor. */

@@ -237,3 +237,33 @@ def test_default_method(precision):
     flush("case default method %s" % precision)
     fd, optical = droplet_case(shape=(8, 32, 32), radius=12)
     _compare(fd, optical, None, precision)
+
+
+@pytest.mark.gpu
+def test_transfer_batch_uneven_samples():
+    """Samples of different grid, depth, beams and method in one batched
+    engine call, each against the oracle."""
+    flush("case batch of uneven samples")
+    beta7 = [0.] + [0.2] * 6
+    phi7 = [0.] + [k * np.pi / 3 for k in range(6)]
+    s1 = random_case(shape=(5, 32, 48), wavelengths=(450., 650.), jones=None)
+    s2 = droplet_case(shape=(9, 24, 24), radius=9, wavelengths=(550.,), jones=(1, 0))
+    s3 = random_case(shape=(3, 16, 32), wavelengths=(500., 600.), jones=None, beta=beta7,
+                     phi=phi7, ne=1.8)
+    samples = [
+        {"field_data": s1[0], "optical_data": s1[1], "beta": 0., "phi": 0.},
+        {"field_data": s2[0], "optical_data": s2[1], "beta": 0., "phi": 0., "method": "2x2"},
+        {"field_data": s3[0], "optical_data": s3[1], "beta": beta7, "phi": phi7},
+    ]
+    for precision, dt in (("c128", np.complex128), ("c64", np.complex64)):
+        batch = [dict(s, field_data=(s["field_data"][0].astype(dt),) + tuple(s["field_data"][1:]))
+                 for s in samples]
+        res = dm.transfer_batch(batch, method="4x4", precision=precision)
+        assert len(res) == 3
+        for smp, (out, wl, ps) in zip(samples, res):
+            fd = smp["field_data"]
+            ref = oracle.transfer_field(fd, smp["optical_data"], beta=smp["beta"], phi=smp["phi"],
+                                        method=smp.get("method", "4x4"))[0]
+            err = rel_err(out, ref)
+            flush("  %s sample %s err %.3e" % (precision, out.shape, err))
+            assert err < TOL[precision], err
