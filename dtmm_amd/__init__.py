@@ -13,5 +13,6 @@ from .field import (illumination_data, field2intensity, blackman, k0, betaphi,  
                     planewave)
 from .transfer import (transfer_field, transfer_batch, Plan, Stack,  # noqa: F401
                        DTMM_NORM_FFT, DTMM_NORM_REF, field2betaphi, mean_betaphi)
+from .adjoint import AdjointTransfer, polarizer4x4, analyser_intensity_loss  # noqa: F401
 
 __version__ = "0.1.0"

@@ -39,6 +39,11 @@ EXPORTS = (
     "dtmmx_host_rop",
     "dtmmx_host_windows",
     "dtmmx_transfer_batch",
+    "dtmmx_tape_size",
+    "dtmmx_adjoint_workspace_size",
+    "dtmmx_transfer_record",
+    "dtmmx_transfer_adjoint",
+    "dtmmx_host_rop_grad",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -127,6 +132,20 @@ def load():
     lib.dtmmx_transfer_batch.argtypes = [ctypes.POINTER(Job), ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_void_p]
     lib.dtmmx_transfer_batch.restype = ctypes.c_int
+    lib.dtmmx_tape_size.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stack)]
+    lib.dtmmx_tape_size.restype = ctypes.c_size_t
+    lib.dtmmx_adjoint_workspace_size.argtypes = [ctypes.c_void_p]
+    lib.dtmmx_adjoint_workspace_size.restype = ctypes.c_size_t
+    lib.dtmmx_transfer_record.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stack), ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.c_void_p, ctypes.c_void_p]
+    lib.dtmmx_transfer_record.restype = ctypes.c_int
+    lib.dtmmx_transfer_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stack), ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
+    lib.dtmmx_transfer_adjoint.restype = ctypes.c_int
+    lib.dtmmx_host_rop_grad.argtypes = [ctypes.c_int] + [_dp] * 9
+    lib.dtmmx_host_rop_grad.restype = ctypes.c_int
     lib.dtmmx_fft2.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
     lib.dtmmx_fft2.restype = ctypes.c_int

@@ -144,6 +144,27 @@ static int launch_col(int n, const ColArgs<T>& a, int nlines_per_lam, hipStream_
   return DTMMX_OK;
 }
 
+template <typename T>
+static int launch_row_adj(int n, const AdjArgs<T>& a, int nwork, hipStream_t s) {
+  switch (n) {
+#define X(N)                                                                                \
+  case N:                                                                                   \
+    if constexpr (sizeof(T) == 8 && N > 512) {                                              \
+      return fail(DTMMX_E_UNSUPPORTED, "complex128 adjoint is built up to 512 points");      \
+    } else {                                                                                \
+      hipLaunchKernelGGL((row_adj_kernel<T, N>), dim3(nwork), dim3(row_lanes<N>() * 4), 0, s, \
+                         a);                                                                \
+    }                                                                                       \
+    break;
+    DTMMX_SIZES(X)
+#undef X
+    default:
+      return fail(DTMMX_E_UNSUPPORTED, "unsupported width " + std::to_string(n));
+  }
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
 }  // namespace dtmmx
 
 using namespace dtmmx;
@@ -864,6 +885,174 @@ static int fft2_impl(void* data, int batch, int H, int W, int inverse, hipStream
   return DTMMX_OK;
 }
 
+// Steps (sub-steps of all layers) of a stack: one tape field each.
+static size_t total_steps(const dtmmx_stack* S) {
+  size_t n = 0;
+  for (int l = 0; l < S->n_layers; ++l) n += S->n_steps ? size_t(S->n_steps[l]) : 1;
+  return n;
+}
+
+static bool adjoint_supported(const dtmmx_plan* P) {
+  return P->d.method == DTMMX_METHOD_4X4 && P->wn <= 1 && P->d.npass == 1;
+}
+
+// Forward 4x4 transfer (diffraction 1, npass 1) that records the row-pass
+// input of every step on the tape for the adjoint.
+template <typename T>
+static int record_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
+                       void* tape, int* status, hipStream_t s) {
+  const int H = P->d.height, W = P->d.width;
+  const size_t HW = size_t(H) * W;
+  const int nrays = P->d.n_rays, nlam = P->d.n_wavelengths;
+  const size_t fsz = size_t(nrays) * nlam * 4 * HW;
+  const double scale = 1.0 / double(HW);
+  cpx<T>* out = (cpx<T>*)fout;
+  cpx<T>* tp = (cpx<T>*)tape;
+  HIP_TRY(hipMemcpyAsync(out, fin, fsz * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+  const std::vector<Step> st = expand_steps(S, 1);
+  const int L = int(st.size());
+  if (L == 0) return DTMMX_OK;
+  Batch b{nrays, nlam, 0, nlam};
+  TRY(row_pass<T>(P, out, b, 4, OP_FWD, s));
+  TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+  KArgs<T> k{};
+  k.kop = KOP_HEAD4;
+  k.d0 = st[0].d;
+  k.e0 = st[0].e;
+  k.scale = scale;
+  TRY(kspace<T>(P, out, out, b, k, s));
+  TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+  for (int l = 0; l < L; ++l) {
+    HIP_TRY(hipMemcpyAsync(tp + size_t(l) * fsz, out, fsz * sizeof(cpx<T>),
+                           hipMemcpyDeviceToDevice, s));
+    const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status);
+    TRY(row_pass<T>(P, out, b, 4, OP_ROP, s, &la));
+    if (l + 1 < L) {
+      if (eq(st[l].e, st[l + 1].e)) {
+        ColArgs<T> ca{};
+        ca.da = st[l].d;
+        ca.db = st[l + 1].d;
+        ca.ea = st[l].e;
+        ca.eb = st[l + 1].e;
+        ca.kca = 1.;
+        ca.kcb = 1.;
+        ca.scale = scale;
+        ca.dense_sign = 1;
+        TRY(col_pass<T>(P, out, b, 4, OP_DIAG, s, &ca));
+      } else {
+        TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+        KArgs<T> bd{};
+        bd.kop = KOP_BOUND4;
+        bd.d0 = st[l].d;
+        bd.d1 = st[l + 1].d;
+        bd.e0 = st[l].e;
+        bd.e1 = st[l + 1].e;
+        bd.scale = scale;
+        TRY(kspace<T>(P, out, out, b, bd, s));
+        TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+      }
+    }
+  }
+  TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+  KArgs<T> t{};
+  t.kop = KOP_TAIL4;
+  t.d0 = st[L - 1].d;
+  t.e0 = st[L - 1].e;
+  t.scale = scale;
+  TRY(kspace<T>(P, out, out, b, t, s));
+  TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+  TRY(row_pass<T>(P, out, b, 4, OP_INV, s));
+  return DTMMX_OK;
+}
+
+// Adjoint of record_impl's propagation applied to `seed` = dL/dconj(F_out)
+// (real space), accumulating dL/dtheta and dL/dphi of every voxel into
+// grad [2][n_layers][H][W] (dL = 2 Re <seed, dF_out>).  Every operator is
+// replaced by its conjugate transpose, in reverse order: unnormalised FFTs
+// by the opposite transform, diagonal k-space factors by their conjugates,
+// dense k-space operators by M^H, the per-pixel operator by R^H.
+template <typename T>
+static int adjoint_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape,
+                        const void* seed, double* grad, void* ws, int* status, hipStream_t s) {
+  const int H = P->d.height, W = P->d.width;
+  const size_t HW = size_t(H) * W;
+  const int nrays = P->d.n_rays, nlam = P->d.n_wavelengths;
+  const size_t fsz = size_t(nrays) * nlam * 4 * HW;
+  const double scale = 1.0 / double(HW);
+  const std::vector<Step> st = expand_steps(S, 1);
+  const int L = int(st.size());
+  if (L == 0) return DTMMX_OK;
+  cpx<T>* adj = (cpx<T>*)ws;
+  double* gpart = (double*)((char*)ws + (fsz * sizeof(cpx<T>) + 255) / 256 * 256);
+  const cpx<T>* tp = (const cpx<T>*)tape;
+  HIP_TRY(hipMemcpyAsync(adj, seed, fsz * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+  Batch b{nrays, nlam, 0, nlam};
+  TRY(row_pass<T>(P, adj, b, 4, OP_FWD, s));
+  TRY(col_pass<T>(P, adj, b, 4, OP_FWD, s));
+  KArgs<T> t{};
+  t.kop = KOP_TAIL4;
+  t.d0 = st[L - 1].d;
+  t.e0 = st[L - 1].e;
+  t.scale = scale;
+  t.adjoint = 1;
+  TRY(kspace<T>(P, adj, adj, b, t, s));
+  TRY(col_pass<T>(P, adj, b, 4, OP_INV, s));
+  const int nl = S->n_layers;
+  for (int l = L - 1; l >= 0; --l) {
+    if (l + 1 < L) {
+      if (eq(st[l].e, st[l + 1].e)) {
+        ColArgs<T> ca{};
+        ca.da = st[l].d;
+        ca.db = st[l + 1].d;
+        ca.ea = st[l].e;
+        ca.eb = st[l + 1].e;
+        ca.kca = 1.;
+        ca.kcb = 1.;
+        ca.scale = scale;
+        ca.dense_sign = 1;
+        ca.conj_k = 1;
+        TRY(col_pass<T>(P, adj, b, 4, OP_DIAG, s, &ca));
+      } else {
+        TRY(col_pass<T>(P, adj, b, 4, OP_FWD, s));
+        KArgs<T> bd{};
+        bd.kop = KOP_BOUND4;
+        bd.d0 = st[l].d;
+        bd.d1 = st[l + 1].d;
+        bd.e0 = st[l].e;
+        bd.e1 = st[l + 1].e;
+        bd.scale = scale;
+        bd.adjoint = 1;
+        TRY(kspace<T>(P, adj, adj, b, bd, s));
+        TRY(col_pass<T>(P, adj, b, 4, OP_INV, s));
+      }
+    }
+    AdjArgs<T> a{};
+    a.adj = adj;
+    a.tape = tp + size_t(l) * fsz;
+    a.H = H;
+    a.W = W;
+    a.b = b;
+    a.tw = (const cpx<T>*)P->tw_w;
+    const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status);
+    a.eps_re = la.eps_re;
+    a.eps_im = la.eps_im;
+    a.ang = la.ang;
+    a.comp_stride = la.comp_stride;
+    a.d = st[l].d;
+    a.status = status;
+    a.gpart = gpart;
+    a.bm = P->bm;
+    timer_start(0, s);
+    TRY(launch_row_adj<T>(W, a, H * nrays, s));
+    timer_stop(s);
+    hipLaunchKernelGGL(grad_sum_kernel, dim3(unsigned((HW + NT - 1) / NT)), dim3(NT), 0, s,
+                       (const double*)gpart, nrays, HW, grad + size_t(st[l].l) * HW,
+                       grad + (size_t(nl) + st[l].l) * HW);
+    HIP_TRY(hipGetLastError());
+  }
+  return DTMMX_OK;
+}
+
 // Precision-specific entry points (defined in the part that builds them).
 int transfer_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
                  int* status, hipStream_t s);
@@ -871,6 +1060,14 @@ int transfer_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fo
                   int* status, hipStream_t s);
 int fft2_c64(void* data, int batch, int H, int W, int inverse, hipStream_t s);
 int fft2_c128(void* data, int batch, int H, int W, int inverse, hipStream_t s);
+int record_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, void* tape,
+               int* status, hipStream_t s);
+int record_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, void* tape,
+                int* status, hipStream_t s);
+int adjoint_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape, const void* seed,
+                double* grad, void* ws, int* status, hipStream_t s);
+int adjoint_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape, const void* seed,
+                 double* grad, void* ws, int* status, hipStream_t s);
 
 #if DTMMX_C64_PART
 int transfer_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
@@ -880,6 +1077,14 @@ int transfer_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fou
 int fft2_c64(void* data, int batch, int H, int W, int inverse, hipStream_t s) {
   return fft2_impl<float>(data, batch, H, W, inverse, s);
 }
+int record_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, void* tape,
+               int* status, hipStream_t s) {
+  return record_impl<float>(P, S, fin, fout, tape, status, s);
+}
+int adjoint_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape, const void* seed,
+                double* grad, void* ws, int* status, hipStream_t s) {
+  return adjoint_impl<float>(P, S, tape, seed, grad, ws, status, s);
+}
 #endif
 #if DTMMX_C128_PART
 int transfer_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
@@ -888,6 +1093,14 @@ int transfer_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fo
 }
 int fft2_c128(void* data, int batch, int H, int W, int inverse, hipStream_t s) {
   return fft2_impl<double>(data, batch, H, W, inverse, s);
+}
+int record_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, void* tape,
+                int* status, hipStream_t s) {
+  return record_impl<double>(P, S, fin, fout, tape, status, s);
+}
+int adjoint_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape, const void* seed,
+                 double* grad, void* ws, int* status, hipStream_t s) {
+  return adjoint_impl<double>(P, S, tape, seed, grad, ws, status, s);
 }
 #endif
 
@@ -1206,6 +1419,50 @@ static int check_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void*
   return DTMMX_OK;
 }
 
+size_t dtmmx_tape_size(const dtmmx_plan* P, const dtmmx_stack* S) {
+  if (!P || !S) return 0;
+  const size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  const size_t fsz = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * P->d.height * P->d.width;
+  return total_steps(S) * fsz * cb;
+}
+
+size_t dtmmx_adjoint_workspace_size(const dtmmx_plan* P) {
+  if (!P) return 0;
+  const size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const size_t f = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW * cb;
+  return (f + 255) / 256 * 256 + size_t(P->d.n_rays) * 2 * HW * sizeof(double);
+}
+
+int dtmmx_transfer_record(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
+                          void* tape, size_t tape_bytes, int* status, void* stream) {
+  {
+    const int r = check_transfer(P, S, fin, fout, nullptr, 0);
+    if (r != DTMMX_OK) return r;
+  }
+  if (!adjoint_supported(P))
+    return fail(DTMMX_E_UNSUPPORTED, "the adjoint is built for 4x4, diffraction 1, npass 1");
+  if (!tape || tape_bytes < dtmmx_tape_size(P, S)) return fail(DTMMX_E_ARG, "tape too small");
+  hipStream_t s = (hipStream_t)stream;
+  if (P->d.precision == DTMMX_C64) return record_c64(P, S, fin, fout, tape, status, s);
+  return record_c128(P, S, fin, fout, tape, status, s);
+}
+
+int dtmmx_transfer_adjoint(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape,
+                           const void* seed, double* grad, void* ws, size_t ws_bytes, int* status,
+                           void* stream) {
+  if (!P || !S || !tape || !seed || !grad) return fail(DTMMX_E_ARG, "null argument");
+  if (!adjoint_supported(P))
+    return fail(DTMMX_E_UNSUPPORTED, "the adjoint is built for 4x4, diffraction 1, npass 1");
+  if (!ws || ws_bytes < dtmmx_adjoint_workspace_size(P))
+    return fail(DTMMX_E_ARG, "workspace too small");
+  if (S->n_layers > 0 && (!S->thickness || !S->eps_eff_re || !S->eps_re || !S->angles))
+    return fail(DTMMX_E_ARG, "null stack array");
+  hipStream_t s = (hipStream_t)stream;
+  if (P->d.precision == DTMMX_C64) return adjoint_c64(P, S, tape, seed, grad, ws, status, s);
+  return adjoint_c128(P, S, tape, seed, grad, ws, status, s);
+}
+
 int dtmmx_transfer_batch(const dtmmx_job* jobs, int n, int* status, void* stream) {
   if (n < 0 || (n > 0 && !jobs)) return fail(DTMMX_E_ARG, "bad job array");
   for (int i = 0; i < n; ++i) {
@@ -1338,6 +1595,40 @@ int dtmmx_host_alphaf(int n, const double* beta, const double* phi, const double
         fmat[32 * i + 8 * j + 2 * k + 1] = m.F[j][k].y;
       }
     }
+  }
+  return rc;
+}
+
+int dtmmx_host_rop_grad(int n, const double* beta, const double* phi, const double* eps_re,
+                        const double* eps_im, const double* angles, double* alpha, double* proj,
+                        double* dalpha, double* dproj) {
+  if (n < 0 || !beta || !phi || !eps_re || !angles || !alpha || !proj || !dalpha || !dproj)
+    return fail(DTMMX_E_ARG, "null argument");
+  int rc = DTMMX_OK;
+  auto put = [](double* o, cd z) {
+    o[0] = z.x;
+    o[1] = z.y;
+  };
+  for (int i = 0; i < n; ++i) {
+    cd e[3];
+    for (int c = 0; c < 3; ++c) e[c] = mk<double>(eps_re[3 * i + c], eps_im ? eps_im[3 * i + c] : 0.);
+    RopGrad g;
+    rop_grad(beta[i], std::cos(phi[i]), std::sin(phi[i]), phi[i], e[0], e[1], e[2],
+             angles[3 * i + 1], angles[3 * i + 2], g);
+    if (g.kind == KIND_BIAXIAL) rc = fail(DTMMX_E_UNSUPPORTED, "biaxial voxel");
+    for (int j = 0; j < 4; ++j) {
+      put(alpha + 8 * i + 2 * j, g.a[j]);
+      for (int k = 0; k < 2; ++k) put(dalpha + 16 * i + 8 * k + 2 * j, g.da[k][j]);
+    }
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) {
+        put(proj + 16 * i + 4 * a + 2 * b, g.P0[a][b]);
+        put(proj + 16 * i + 8 + 4 * a + 2 * b, g.P1[a][b]);
+        for (int k = 0; k < 2; ++k) {
+          put(dproj + 32 * i + 16 * k + 4 * a + 2 * b, g.dP0[k][a][b]);
+          put(dproj + 32 * i + 16 * k + 8 + 4 * a + 2 * b, g.dP1[k][a][b]);
+        }
+      }
   }
   return rc;
 }

@@ -5,6 +5,7 @@
 
 #include "berreman.hpp"
 #include "cpx.hpp"
+#include "rop_grad.hpp"
 #include "dtmmx.h"
 #include "fft.hpp"
 
@@ -326,6 +327,175 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint row kernel (4x4): for one sub-step of a layer, the transpose of
+// the row pass F_x R B_x applied to the adjoint field (B_x, R^H, F_x, in
+// place) and the director-angle gradient
+//   dL/dtheta, dL/dphi += 2 Re( vhat^H (dR/dangle) u )
+// with u = B_x g the real-space field the forward row pass saw (g recorded
+// on the tape) and vhat = B_x of the incoming adjoint.  One work-group per
+// (row y, ray); the per-pixel fp64 solve with forward-mode derivatives
+// (rop_grad) runs for every wavelength.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct AdjArgs {
+  cpx<T>* adj;          // adjoint field (mixed domain), in place
+  const cpx<T>* tape;   // recorded row-pass input of this sub-step
+  int H, W;
+  Batch b;
+  const cpx<T>* tw;
+  const double* eps_re;
+  const double* eps_im;
+  const double* ang;
+  size_t comp_stride;
+  double d;
+  int* status;
+  double* gpart;        // [nrays][2][H][W]: this sub-step's gradient per ray
+  Beams bm;
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(row_lanes<N>() * 4, 1) void row_adj_kernel(AdjArgs<T> A) {
+  constexpr int NL = row_lanes<N>();
+  constexpr bool WV = (NL == 64);
+  constexpr int LINES = 4;
+  constexpr int NTW = NL * LINES;
+  constexpr int LS = line_stride<N>();
+  constexpr int MV1 = max_vals<N, true, NL>();
+  constexpr int MV2 = max_vals<N, false, NL>();
+  constexpr int MAXV = MV1 > MV2 ? MV1 : MV2;
+  constexpr int NIN = slot_count<N, true, NL, false>();
+  constexpr int NMID = slot_count<N, true, NL, true>();
+  constexpr int NOUT = slot_count<N, false, NL, true>();
+  constexpr int PPX = (N + NTW - 1) / NTW;
+  __shared__ cpx<T> la[LINES * LS];  // adjoint lines
+  __shared__ cpx<T> lu[LINES * LS];  // forward lines
+  __shared__ cpx<T> tws[N];
+  const int tid = threadIdx.x;
+  const int comp = tid / NL, lane = tid % NL;
+  for (int i = tid; i < N; i += NTW) tws[i] = A.tw[i];
+  const int w = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int ray = w % A.b.nrays;
+  const int y = w / A.b.nrays;
+  const size_t HW = size_t(A.H) * N;
+  const int beam = ray / A.bm.rays_per_beam;
+  const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
+  double s0, c0;
+  sincos(phi0, &s0, &c0);
+  double gt[PPX], gp[PPX];
+  for (int k = 0; k < PPX; ++k) gt[k] = gp[k] = 0.;
+  __syncthreads();
+  for (int jl = 0; jl < A.b.nlam_b; ++jl) {
+    const int lam = A.b.lam0 + jl;
+    const size_t off = ((size_t(ray) * A.b.nlam + lam) * 4 + comp) * HW + size_t(y) * N;
+    cpx<T>* ga = A.adj + off;
+    const cpx<T>* gu = A.tape + off;
+    cpx<T> v[MAXV], u[MAXV];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j)
+      if (slot_valid<N, true, NL, false>(j, lane)) {
+        v[j] = ga[slot_pos<N, true, NL, false>(j, lane)];
+        u[j] = gu[slot_pos<N, true, NL, false>(j, lane)];
+      }
+    cpx<T>* lva = la + comp * LS;
+    cpx<T>* lvu = lu + comp * LS;
+    reg_fft<T, N, true, +1, NL, 1, 1, MAXV, WV>(v, lva, tws, lane);
+    reg_fft<T, N, true, +1, NL, 1, 1, MAXV, WV>(u, lvu, tws, lane);
+#pragma unroll
+    for (int j = 0; j < NMID; ++j)
+      if (slot_valid<N, true, NL, true>(j, lane)) {
+        const int p = lds_at<1, 1>(slot_pos<N, true, NL, true>(j, lane));
+        lva[p] = v[j];
+        lvu[p] = u[j];
+      }
+    __syncthreads();
+    const double kd = A.bm.k0[lam] * A.d;
+#pragma unroll
+    for (int k = 0; k < PPX; ++k) {
+      const int x = tid + k * NTW;
+      if (x >= N) continue;
+      const size_t o = size_t(y) * N + x;
+      const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+      const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
+      const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
+                               A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
+      RopGrad g;
+      rop_grad(beta, c0, s0, phi0, e0, e1, e2, A.ang[A.comp_stride + o], A.ang[2 * A.comp_stride + o],
+               g);
+      if (g.kind == KIND_BIAXIAL && A.status) atomicOr(A.status, 1);
+      const double ok = g.ok4 ? 1. : 0.;
+      const int ax = lds_at<1, 1>(x);
+      // blocks (E+x, E+y) = comps (0, 2) with P0, (E-x, E-y) = comps (1, 3) with P1
+      for (int blk = 0; blk < 2; ++blk) {
+        const int ca = blk, cb = blk + 2;
+        const int ia = blk, ib = blk + 2;  // alpha indices: a0, a2 (+), a1, a3 (-)
+        const cd pa = cexpi(g.a[ia] * kd), pb = cexpi(g.a[ib] * kd);
+        const cd(*P)[2] = blk == 0 ? g.P0 : g.P1;
+        const cd uh[2] = {cast<double>(lvu[ca * LS + ax]), cast<double>(lvu[cb * LS + ax])};
+        const cd vh[2] = {cast<double>(lva[ca * LS + ax]), cast<double>(lva[cb * LS + ax])};
+        const cd dp = (pa - pb) * ok, p2 = pb * ok;
+        // gradient: vhat^H (dp2 I + (dpa - dpb) P + (pa - pb) dP) u
+        for (int dir = 0; dir < 2; ++dir) {
+          const cd(*dP)[2] = blk == 0 ? g.dP0[dir] : g.dP1[dir];
+          const cd dpa = mk<double>(0., kd) * g.da[dir][ia] * pa;
+          const cd dpb = mk<double>(0., kd) * g.da[dir][ib] * pb;
+          cd s = mk<double>(0.);
+          for (int i = 0; i < 2; ++i) {
+            cd t = dpb * uh[i];
+            for (int j = 0; j < 2; ++j) t = t + ((dpa - dpb) * P[i][j] + (pa - pb) * dP[i][j]) * uh[j];
+            s = s + conj(vh[i]) * t;
+          }
+          const double gv = 2. * s.x * ok;
+          if (dir == 0) gt[k] += gv;
+          else gp[k] += gv;
+        }
+        // adjoint: uhat = (p2 I + dp P)^H vhat
+        cd r[2];
+        for (int i = 0; i < 2; ++i) {
+          cd t = conj(p2) * vh[i];
+          for (int j = 0; j < 2; ++j) t = t + conj(dp * P[j][i]) * vh[j];
+          r[i] = t;
+        }
+        lva[ca * LS + ax] = cast<T>(r[0]);
+        lva[cb * LS + ax] = cast<T>(r[1]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NMID; ++j)
+      if (slot_valid<N, true, NL, true>(j, lane)) v[j] = lva[lds_at<1, 1>(slot_pos<N, true, NL, true>(j, lane))];
+    exchange_sync<WV>();
+    reg_fft<T, N, false, -1, NL, 1, 1, MAXV, WV>(v, lva, tws, lane);
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j)
+      if (slot_valid<N, false, NL, true>(j, lane)) ga[slot_pos<N, false, NL, true>(j, lane)] = v[j];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < PPX; ++k) {
+    const int x = tid + k * NTW;
+    if (x >= N) continue;
+    const size_t o = size_t(y) * N + x;
+    A.gpart[(size_t(ray) * 2) * HW + o] = gt[k];
+    A.gpart[(size_t(ray) * 2 + 1) * HW + o] = gp[k];
+  }
+}
+
+// grad[0|1][pixel] += sum over rays of gpart[ray][0|1][pixel].
+static __global__ __launch_bounds__(NT) void grad_sum_kernel(const double* gpart, int nrays, size_t HW,
+                                                       double* gtheta, double* gphi) {
+  const size_t i = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (i >= HW) return;
+  double a = 0., b = 0.;
+  for (int r = 0; r < nrays; ++r) {
+    a += gpart[(size_t(r) * 2) * HW + i];
+    b += gpart[(size_t(r) * 2 + 1) * HW + i];
+  }
+  gtheta[i] += a;
+  gphi[i] += b;
+}
+
+// ---------------------------------------------------------------------------
 // Column kernel: a work-group owns TX adjacent columns of one wavelength
 // (TX lanes read 8*TX contiguous bytes of a row; NLC lanes share a column,
 // one radix-8 butterfly each) and transforms the lines of ncl components in
@@ -347,6 +517,7 @@ struct ColArgs {
   double kca, kcb;    // weights of the mean-beam correction of each half step
   double scale;
   int dense_sign;     // 1: backward components (odd c) get exp(-i phase)
+  int conj_k;         // 1: the adjoint (complex conjugate) of the diagonal operator
   // OP_WIN: destination (the source `field` is left unchanged) and the tent
   // window (wi, wj) of wn x wn: separable factors cx [beam][lam][wn][W] and
   // cy [beam][lam][wn][H]
@@ -435,6 +606,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
       if (xin && slot_valid<N, REV1, NL, true>(j, g)) {
         const int ky = slot_pos<N, REV1, NL, true>(j, g);
         K[j] = diag_factor(A, lam, beam, ky, x);
+        if (A.conj_k) K[j] = conj(K[j]);
         if constexpr (OP == OP_WIN) Wv[j] = T(cx * cyt[ky]);
       }
     }
@@ -538,6 +710,7 @@ struct KArgs {
   double scale;
   int nocorr;     // 1: no mean-beam correction (HEAD4, TAIL4, BOUND4)
   int fresnel;    // PROJ2: 1 applies the interface transmission ein -> e0
+  int adjoint;    // 1: apply the conjugate transpose (4-component operators)
   Beams bm;
 };
 
@@ -617,32 +790,13 @@ __device__ inline void second_half(double beta, double phi, cd eps, double kd, c
   xpy(m.F, m.a, kd, Ei, D);
 }
 
+// One dense k-space operator r = M(k) v (fp64) at wave vector (beta, phi)
+// with in-plane direction (c, sn), for plane wavenumber k0 and beam (bb, pb).
 template <typename T>
-__global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
-  const int k = blockIdx.x * NT + threadIdx.x;
-  const int ray = blockIdx.y / A.b.nlam_b;
-  const int lam = A.b.lam0 + blockIdx.y % A.b.nlam_b;
-  const int q = ray * A.b.nlam + lam;
-  const size_t HW = size_t(A.H) * A.W;
-  if (k >= int(HW)) return;
-  const int ky = k / A.W, kx = k - ky * A.W;
-  const int beam = ray / A.bm.rays_per_beam;
-  const double k0 = A.bm.k0[lam];
-  const double bb = A.bm.beta[beam], pb = A.bm.phi[beam];
-  const bool pass = mode_index(kx, A.W) <= A.amax[lam * A.H + ky];
-  const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
-  const double beta = (2 * M_PI / k0) * sqrt(fx * fx + fy * fy);
-  const double phi = atan2(fy, fx);
-  const int nin = (A.kop == KOP_TR2) ? 2 : 4;
-  const int nout = (A.kop == KOP_PROJ2 || A.kop == KOP_TR2) ? 2 : 4;
-  const cpx<T>* s = A.src + size_t(q) * nin * HW + k;
-  cpx<T>* o = A.dst + size_t(q) * nout * HW + k;
-  cd v[4];
-  for (int c = 0; c < nin; ++c) v[c] = cast<double>(s[c * HW]);
-  // in-plane direction of the mode: (cos, sin) of atan2(fy, fx), (1, 0) at k = 0
-  const double fr = sqrt(fx * fx + fy * fy);
-  const double c = fr > 0. ? fx / fr : 1., sn = fr > 0. ? fy / fr : 0.;
-  cd r[4] = {mk<double>(0.), mk<double>(0.), mk<double>(0.), mk<double>(0.)};
+__device__ inline void kop_apply(const KArgs<T>& A, bool pass, double beta, double phi, double c,
+                                 double sn, double k0, double bb, double pb, const cd v[4],
+                                 cd r[4]) {
+  for (int i = 0; i < 4; ++i) r[i] = mk<double>(0.);
   if (pass && A.kop == KOP_HEAD4) {
     // C . E P(kd) F^-1 in closed form for an isotropic medium: field ->
     // frame of the mode (g = R^T f) -> mode amplitudes m = F^-1 g -> half-step
@@ -745,6 +899,47 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
       cd t = mk<double>(0.);
       for (int j = 0; j < 4; ++j) t = t + M[i][j] * v[j];
       r[i] = t;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
+  const int k = blockIdx.x * NT + threadIdx.x;
+  const int ray = blockIdx.y / A.b.nlam_b;
+  const int lam = A.b.lam0 + blockIdx.y % A.b.nlam_b;
+  const int q = ray * A.b.nlam + lam;
+  const size_t HW = size_t(A.H) * A.W;
+  if (k >= int(HW)) return;
+  const int ky = k / A.W, kx = k - ky * A.W;
+  const int beam = ray / A.bm.rays_per_beam;
+  const double k0 = A.bm.k0[lam];
+  const double bb = A.bm.beta[beam], pb = A.bm.phi[beam];
+  const bool pass = mode_index(kx, A.W) <= A.amax[lam * A.H + ky];
+  const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
+  const double beta = (2 * M_PI / k0) * sqrt(fx * fx + fy * fy);
+  const double phi = atan2(fy, fx);
+  const int nin = (A.kop == KOP_TR2) ? 2 : 4;
+  const int nout = (A.kop == KOP_PROJ2 || A.kop == KOP_TR2) ? 2 : 4;
+  const cpx<T>* s = A.src + size_t(q) * nin * HW + k;
+  cpx<T>* o = A.dst + size_t(q) * nout * HW + k;
+  cd v[4];
+  for (int c = 0; c < nin; ++c) v[c] = cast<double>(s[c * HW]);
+  // in-plane direction of the mode: (cos, sin) of atan2(fy, fx), (1, 0) at k = 0
+  const double fr = sqrt(fx * fx + fy * fy);
+  const double c = fr > 0. ? fx / fr : 1., sn = fr > 0. ? fy / fr : 0.;
+  cd r[4] = {mk<double>(0.), mk<double>(0.), mk<double>(0.), mk<double>(0.)};
+  if (!A.adjoint) {
+    kop_apply(A, pass, beta, phi, c, sn, k0, bb, pb, v, r);
+  } else if (pass) {
+    // adjoint: r = M^H v with the columns of M from the unit vectors
+    for (int j = 0; j < 4; ++j) {
+      cd e[4] = {mk<double>(0.), mk<double>(0.), mk<double>(0.), mk<double>(0.)}, m[4];
+      e[j] = mk<double>(1.);
+      kop_apply(A, pass, beta, phi, c, sn, k0, bb, pb, e, m);
+      cd t = mk<double>(0.);
+      for (int i = 0; i < 4; ++i) t = t + conj(m[i]) * v[i];
+      r[j] = t;
     }
   }
   for (int i = 0; i < nout; ++i) o[i * HW] = cast<T>(r[i] * A.scale);

@@ -150,6 +150,27 @@ typedef struct {
  * status (device int[4], may be NULL) collects the flags of all jobs. */
 int dtmmx_transfer_batch(const dtmmx_job* jobs, int n, int* status, void* stream);
 
+/* Adjoint (reverse-mode) transfer for gradients with respect to the
+ * director angles.  The reference has no gradients; this is the adjoint of
+ * the 4x4 transfer above (diffraction 1, npass 1), for a real loss
+ * L(field_out) of the output field.
+ *
+ * dtmmx_transfer_record: the forward transfer, keeping the row-pass input
+ *   of every propagation step on `tape` (dtmmx_tape_size bytes).
+ * dtmmx_transfer_adjoint: from seed = dL/dconj(field_out) (device, field
+ *   layout; dL = 2 Re <seed, d field_out>) and the tape, adds dL/dtheta and
+ *   dL/dphi of every voxel to grad (device float64 [2][n_layers][H][W]:
+ *   theta = angles[1], phi = angles[2] of the stack), summed over rays,
+ *   wavelengths and sub-steps.  workspace: dtmmx_adjoint_workspace_size. */
+size_t dtmmx_tape_size(const dtmmx_plan* plan, const dtmmx_stack* stack);
+size_t dtmmx_adjoint_workspace_size(const dtmmx_plan* plan);
+int dtmmx_transfer_record(const dtmmx_plan* plan, const dtmmx_stack* stack, const void* field_in,
+                          void* field_out, void* tape, size_t tape_bytes, int* status,
+                          void* stream);
+int dtmmx_transfer_adjoint(const dtmmx_plan* plan, const dtmmx_stack* stack, const void* tape,
+                           const void* seed, double* grad, void* workspace,
+                           size_t workspace_bytes, int* status, void* stream);
+
 /* In-place unnormalised 2D FFT of `batch` planes of H x W complex values
  * (forward exp(-i..) when inverse == 0), the transform of dtmm/fft.py:277
  * fft2 / :306 ifft2 without the 1/(HW) factor. */
@@ -193,6 +214,13 @@ int dtmmx_host_alphaf(int n, const double* beta, const double* phi, const double
  * proj[n][2][2][2] (P0, P1) as interleaved complex128, ok[n][2] = E
  * invertible for the 4x4 / 2x2 method.  The same routine runs inside the
  * row kernel; this entry point exists for checks without a GPU. */
+/* dtmmx_host_rop with the theta and phi derivatives (no counterpart in the
+ * reference): dalpha[n][2][4] and dproj[n][2][2][2][2] ([direction][P0|P1])
+ * as interleaved complex128; the routine the adjoint row kernel runs. */
+int dtmmx_host_rop_grad(int n, const double* beta, const double* phi, const double* eps_re,
+                        const double* eps_im, const double* angles, double* alpha, double* proj,
+                        double* dalpha, double* dproj);
+
 int dtmmx_host_rop(int n, const double* beta, const double* phi, const double* eps_re,
                    const double* eps_im, const double* angles, double* alpha, double* proj,
                    int* ok);

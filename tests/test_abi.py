@@ -197,3 +197,45 @@ def test_host_windows_match_oracle(n, case):
     got_p = wp[:m * nb].reshape(m, nb)
     np.testing.assert_allclose(got_b * np.cos(got_p), betas * np.cos(phis), atol=1e-13)
     np.testing.assert_allclose(got_b * np.sin(got_p), betas * np.sin(phis), atol=1e-13)
+
+
+def test_host_rop_grad_matches_finite_differences():
+    """The forward-mode derivatives of the per-pixel operator data (used by
+    the adjoint row kernel) against central differences of dtmmx_host_rop,
+    and their values against dtmmx_host_rop itself."""
+    lib = _lib.load()
+    beta, phi, eps_re, ang = _voxels(64, 7)
+    n = len(beta)
+    args = [np.ascontiguousarray(a) for a in (beta, phi, eps_re)]
+    eim = None
+    a0 = np.zeros((n, 4, 2))
+    p0 = np.zeros((n, 2, 2, 2, 2))
+    da = np.zeros((n, 2, 4, 2))
+    dp = np.zeros((n, 2, 2, 2, 2, 2))
+    rc = lib.dtmmx_host_rop_grad(n, *[_lib.dptr(a) for a in args], _lib.dptr(eim),
+                                 _lib.dptr(np.ascontiguousarray(ang)), _lib.dptr(a0), _lib.dptr(p0),
+                                 _lib.dptr(da), _lib.dptr(dp))
+    assert rc == 0, lib.dtmmx_last_error()
+
+    def rop(angles):
+        a = np.zeros((n, 4, 2))
+        p = np.zeros((n, 2, 2, 2, 2))
+        ok = np.zeros((n, 2), np.int32)
+        rc = lib.dtmmx_host_rop(n, *[_lib.dptr(x) for x in args], _lib.dptr(eim),
+                                _lib.dptr(np.ascontiguousarray(angles)), _lib.dptr(a), _lib.dptr(p),
+                                ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        assert rc == 0
+        return a, p
+
+    av, pv = rop(ang)
+    np.testing.assert_allclose(a0, av, atol=1e-13)
+    np.testing.assert_allclose(p0, pv, atol=1e-12)
+    h = 1e-6
+    for k, col in ((0, 1), (1, 2)):
+        ap = ang.copy()
+        am = ang.copy()
+        ap[:, col] += h
+        am[:, col] -= h
+        (a_p, p_p), (a_m, p_m) = rop(ap), rop(am)
+        np.testing.assert_allclose(da[:, k], (a_p - a_m) / (2 * h), atol=2e-7)
+        np.testing.assert_allclose(dp[:, k], (p_p - p_m) / (2 * h), atol=2e-7)

@@ -267,3 +267,60 @@ def test_transfer_batch_uneven_samples():
             err = rel_err(out, ref)
             flush("  %s sample %s err %.3e" % (precision, out.shape, err))
             assert err < TOL[precision], err
+
+
+def _oracle_analyser_loss(fd, d, eps, ang, roi, jones=(1, 1), n=1.5):
+    """The analysed-intensity loss on the oracle's output field."""
+    field, wl, ps = fd
+    out = oracle.transfer_field((field, wl, ps), [(d, eps, ang)], beta=0., phi=0., method="4x4")[0]
+    m = dm.polarizer4x4(jones, n)
+    y0, y1, x0, x1 = roi
+    sub = out[..., :, y0:y1, x0:x1]
+    g = np.stack([sum(m[i, j] * sub[..., j, :, :] for j in range(4)) for i in range(4)], axis=-3)
+    return float(((g[..., 0, :, :] * g[..., 1, :, :].conj()).real -
+                  (g[..., 2, :, :] * g[..., 3, :, :].conj()).real).sum())
+
+
+@pytest.mark.gpu
+def test_adjoint_gradient_matches_oracle_finite_differences():
+    """Gradient of the analysed intensity (45 degree analyser, central
+    region) with respect to theta and phi of single voxels: the adjoint
+    transfer (c128) against central differences of the oracle loss, and the
+    c64 adjoint against the c128 one."""
+    flush("case adjoint gradient")
+    fd, optical = random_case(shape=(4, 24, 32), wavelengths=(500., 600.), jones=(1, 0))
+    d, eps, ang = optical[0]
+    eps = np.ascontiguousarray(np.broadcast_to(eps, ang.shape))
+    ang = np.ascontiguousarray(ang)
+    roi = (6, 18, 8, 24)
+    tr = dm.AdjointTransfer(fd, [(d, eps, ang)], beta=0., phi=0., precision="c128")
+    out = tr.forward()
+    loss, seed = dm.analyser_intensity_loss(out, (1, 1), 1.5, roi)
+    ref_loss = _oracle_analyser_loss(fd, d, eps, ang, roi)
+    flush("  loss gpu %.12g oracle %.12g" % (loss, ref_loss))
+    assert abs(loss - ref_loss) < 1e-9 * abs(ref_loss)
+    grad = tr.backward(seed).cpu().numpy()
+    gmax = np.abs(grad).max()
+    assert gmax > 0
+    rng = np.random.default_rng(5)
+    h = 1e-5
+    for _ in range(8):
+        l, y, x = rng.integers(0, 4), rng.integers(0, 24), rng.integers(0, 32)
+        for a in (1, 2):
+            ap = ang.copy()
+            am = ang.copy()
+            ap[l, y, x, a] += h
+            am[l, y, x, a] -= h
+            fdv = (_oracle_analyser_loss(fd, d, eps, ap, roi) -
+                   _oracle_analyser_loss(fd, d, eps, am, roi)) / (2 * h)
+            g = grad[a - 1, l, y, x]
+            flush("  voxel (%d,%d,%d) angle %d: adjoint %.9e fd %.9e" % (l, y, x, a, g, fdv))
+            assert abs(g - fdv) < 1e-6 * gmax
+    tr64 = dm.AdjointTransfer((fd[0].astype(np.complex64),) + tuple(fd[1:]), [(d, eps, ang)],
+                              beta=0., phi=0., precision="c64")
+    out64 = tr64.forward()
+    _, seed64 = dm.analyser_intensity_loss(out64, (1, 1), 1.5, roi)
+    g64 = tr64.backward(seed64).cpu().numpy()
+    err = np.abs(g64 - grad).max() / gmax
+    flush("  c64 vs c128 gradient err %.3e" % err)
+    assert err < 1e-3

@@ -220,3 +220,19 @@ def test_default_method_is_2x2_with_reflection():
     c = o.transfer_field((f, wl, 100.), opt, beta=0., phi=0., method="2x2", nin=nin, nout=nout)[0]
     np.testing.assert_array_equal(a, b)
     assert np.abs(a - c).max() > 1e-3
+
+
+def test_polarizer4x4_malus():
+    """The 4x4 analyser of the adjoint loss: an x-polarised forward wave at
+    normal incidence keeps cos^2 of its intensity through a linear
+    polarizer at angle a (Malus), and a backward wave is not converted."""
+    from dtmm_amd.adjoint import polarizer4x4
+    n = 1.5
+    f = np.array([1., n, 0., 0.], complex)  # forward x-polarised (Ex, Hy, Ey, Hx)
+    for a in (0., 0.3, np.pi / 4, 1.2):
+        m = polarizer4x4((np.cos(a), np.sin(a)), n)
+        g = m @ f
+        inten = (g[0] * g[1].conj()).real - (g[2] * g[3].conj()).real
+        np.testing.assert_allclose(inten, n * np.cos(a) ** 2, atol=1e-14)
+    m = polarizer4x4((1, 1), n)
+    np.testing.assert_allclose(m @ m, m, atol=1e-14)
