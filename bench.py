@@ -26,6 +26,14 @@ phi = k 60 deg) x 2 polarisations x 9 wavelengths 380-780 nm (126 planes),
 4x4, c64.  Samples of one (kind, grid) share the director of a 160-layer
 volume cut to their depth.
 
+--workload c4 measures a gradient evaluation: 256x256x64 random texture
+(the default workload's rules), 3 wavelengths 450/550/650 nm, x
+polarisation at normal incidence, 4x4; loss = intensity after a 45 degree
+analyser summed over the central 64x64; a step is the recording forward
+transfer, the loss seed and the adjoint pass giving dL/dtheta and dL/dphi
+of every voxel; value = voxel*wavelength*plane steps of the forward per
+second of the whole gradient evaluation.
+
 Prints ONE JSON line.  Options: --method 4x4|2x2, --precision c64|c128.
 """
 import argparse
@@ -170,6 +178,68 @@ def run_c3(args):
     print(json.dumps(rec), flush=True)
 
 
+def run_c4(args):
+    """Forward (recording) + adjoint gradient of the analysed-intensity loss."""
+    import torch
+    import dtmm_amd as dm
+    from dtmm_amd.samples import random_director_data
+    size = args.size if args.size != 512 else 256
+    layers = args.layers if args.layers != 128 else 64
+    t = time.time()
+    optical = [random_director_data((layers, size, size), seed=0, no=1.5, ne=1.7)]
+    wl = np.array([450., 550., 650.])
+    fd = dm.illumination_data((size, size), wl, pixelsize=100, beta=0., phi=0., jones=(1, 0))
+    dt = np.complex64 if args.precision == "c64" else np.complex128
+    fd = (fd[0].astype(dt),) + tuple(fd[1:])
+    tr = dm.AdjointTransfer(fd, optical, beta=0., phi=0., precision=args.precision)
+    log("c4 inputs in %.1f s, tape %.2f GB" % (time.time() - t, tr.tape_bytes / 1e9))
+    roi = (size // 2 - 32, size // 2 + 32, size // 2 - 32, size // 2 + 32)
+    grad = torch.zeros((2, tr.n_layers, size, size), dtype=torch.float64, device="cuda")
+
+    def step():
+        out = tr.forward()
+        loss, seed = dm.analyser_intensity_loss(out, (1, 1), 1.5, roi)
+        grad.zero_()
+        tr.backward(seed, grad)
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    planes = 3
+    steps = size * size * layers * planes
+    g = grad.cpu().numpy()
+    if not np.all(np.isfinite(g)):
+        log("non-finite gradient")
+        sys.exit(1)
+    log("loss %.6g, max |dL/dtheta| %.3e, max |dL/dphi| %.3e"
+        % (loss, np.abs(g[0]).max(), np.abs(g[1]).max()))
+    rec = {
+        "metric": "voxel*lambda steps/s (transfer_field, 4x4 unless noted), c64 with c128 "
+                  "reference, % HBM peak",
+        "value": steps / (ms_per_step / 1e3),
+        "unit": "voxel*lambda steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (fp32 complex, fp64 eigensolve)" if args.precision == "c64" else "c128",
+        "data": "synthetic (seeded random nematic director, plane-wave illumination)",
+        "config": {"workload": "gradient of the 45 degree analysed intensity over the central "
+                               "64x64 w.r.t. theta, phi of %dx%dx%d voxels, 3 wavelengths, x "
+                               "pol, 4x4 (recording forward + adjoint)" % (size, size, layers),
+                   "height": size, "width": size, "layers": layers, "wavelengths": 3,
+                   "planes": planes, "method": "4x4", "precision": args.precision},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def build_inputs_c5(size, layers, nlam):
     import dtmm_amd as dm
     from dtmm_amd.samples import droplet_array_data, droplet_grid_centres
@@ -265,7 +335,7 @@ def cpu_baseline(fd, optical, method, target_s=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: 512^2 x 128 random texture (default); c3: batch of uneven "
                          "samples; c5: 1024^2 x 200 droplet grid with npass=3, diffraction=5")
     ap.add_argument("--samples", type=int, default=64, help="c3: batch size")
@@ -314,6 +384,9 @@ def main():
         sys.exit(1)
     if args.workload == "c3":
         run_c3(args)
+        return
+    if args.workload == "c4":
+        run_c4(args)
         return
     lib = _lib.load()
     c5 = args.workload == "c5"

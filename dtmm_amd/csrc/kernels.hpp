@@ -431,8 +431,8 @@ __global__ __launch_bounds__(row_lanes<N>() * 4, 1) void row_adj_kernel(AdjArgs<
         const int ia = blk, ib = blk + 2;  // alpha indices: a0, a2 (+), a1, a3 (-)
         const cd pa = cexpi(g.a[ia] * kd), pb = cexpi(g.a[ib] * kd);
         const cd(*P)[2] = blk == 0 ? g.P0 : g.P1;
-        const cd uh[2] = {cast<double>(lvu[ca * LS + ax]), cast<double>(lvu[cb * LS + ax])};
-        const cd vh[2] = {cast<double>(lva[ca * LS + ax]), cast<double>(lva[cb * LS + ax])};
+        const cd uh[2] = {cast<double>(lu[ca * LS + ax]), cast<double>(lu[cb * LS + ax])};
+        const cd vh[2] = {cast<double>(la[ca * LS + ax]), cast<double>(la[cb * LS + ax])};
         const cd dp = (pa - pb) * ok, p2 = pb * ok;
         // gradient: vhat^H (dp2 I + (dpa - dpb) P + (pa - pb) dP) u
         for (int dir = 0; dir < 2; ++dir) {
@@ -456,8 +456,8 @@ __global__ __launch_bounds__(row_lanes<N>() * 4, 1) void row_adj_kernel(AdjArgs<
           for (int j = 0; j < 2; ++j) t = t + conj(dp * P[j][i]) * vh[j];
           r[i] = t;
         }
-        lva[ca * LS + ax] = cast<T>(r[0]);
-        lva[cb * LS + ax] = cast<T>(r[1]);
+        la[ca * LS + ax] = cast<T>(r[0]);
+        la[cb * LS + ax] = cast<T>(r[1]);
       }
     }
     __syncthreads();
