@@ -116,11 +116,14 @@ static int launch_row(int n, int nr, const RowArgs<T>& a, int nwork, hipStream_t
 // 128-byte row segments for complex64) whose LDS stays within 64 KiB and
 // whose work-group stays within 1024 threads.
 template <typename T, int N>
+#ifndef DTMMX_COL_LDS_CAP
+#define DTMMX_COL_LDS_CAP 65536
+#endif
 constexpr int col_tx() {
   constexpr int cb = int(sizeof(cpx<T>));
   constexpr int nl = col_lanes<N>();
   int tx = 16;
-  while (tx > 1 && (N * (tx + 1) * cb > 65536 || tx * nl > 1024)) tx /= 2;
+  while (tx > 1 && (N * (tx + 1) * cb > DTMMX_COL_LDS_CAP || tx * nl > 1024)) tx /= 2;
   return tx;
 }
 
@@ -322,7 +325,32 @@ static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, 
   const int HW = a.H * a.W;
   dim3 grid((HW + NT - 1) / NT, b.nrays * b.nlam_b);
   timer_start(3, s);
-  hipLaunchKernelGGL((kspace_kernel<T>), grid, dim3(NT), 0, s, a);
+  switch (a.kop) {
+    case KOP_HEAD4:
+      hipLaunchKernelGGL((kspace_kernel<T, KOP_HEAD4, false>), grid, dim3(NT), 0, s, a);
+      break;
+    case KOP_TAIL4:
+      if (a.adjoint)
+        hipLaunchKernelGGL((kspace_kernel<T, KOP_TAIL4, true>), grid, dim3(NT), 0, s, a);
+      else
+        hipLaunchKernelGGL((kspace_kernel<T, KOP_TAIL4, false>), grid, dim3(NT), 0, s, a);
+      break;
+    case KOP_BOUND4:
+      if (a.adjoint)
+        hipLaunchKernelGGL((kspace_kernel<T, KOP_BOUND4, true>), grid, dim3(NT), 0, s, a);
+      else
+        hipLaunchKernelGGL((kspace_kernel<T, KOP_BOUND4, false>), grid, dim3(NT), 0, s, a);
+      break;
+    case KOP_PROJ2:
+      hipLaunchKernelGGL((kspace_kernel<T, KOP_PROJ2, false>), grid, dim3(NT), 0, s, a);
+      break;
+    case KOP_TR2:
+      hipLaunchKernelGGL((kspace_kernel<T, KOP_TR2, false>), grid, dim3(NT), 0, s, a);
+      break;
+    default:
+      timer_stop(s);
+      return fail(DTMMX_E_ARG, "bad k-space operator");
+  }
   timer_stop(s);
   HIP_TRY(hipGetLastError());
   return DTMMX_OK;

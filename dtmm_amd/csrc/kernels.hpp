@@ -792,12 +792,13 @@ __device__ inline void second_half(double beta, double phi, cd eps, double kd, c
 
 // One dense k-space operator r = M(k) v (fp64) at wave vector (beta, phi)
 // with in-plane direction (c, sn), for plane wavenumber k0 and beam (bb, pb).
-template <typename T>
-__device__ inline void kop_apply(const KArgs<T>& A, bool pass, double beta, double phi, double c,
-                                 double sn, double k0, double bb, double pb, const cd v[4],
-                                 cd r[4]) {
+template <typename T, int KOP>
+__device__ __forceinline__ void kop_apply(const KArgs<T>& A, bool pass, double beta, double phi,
+                                          double c, double sn, double k0, double bb, double pb,
+                                          const cd v[4], cd r[4]) {
   for (int i = 0; i < 4; ++i) r[i] = mk<double>(0.);
-  if (pass && A.kop == KOP_HEAD4) {
+  if constexpr (KOP == KOP_HEAD4) {
+    if (!pass) return;
     // C . E P(kd) F^-1 in closed form for an isotropic medium: field ->
     // frame of the mode (g = R^T f) -> mode amplitudes m = F^-1 g -> half-step
     // phases -> (E+x, E-x, E+y, E-y) -> mean-beam correction.
@@ -818,7 +819,8 @@ __device__ inline void kop_apply(const KArgs<T>& A, bool pass, double beta, doub
       r[2] = (m0 * sn + m2 * c) * cf * 0.5;
       r[3] = (m1 * sn + m3 * c) * cb * 0.5;
     }
-  } else if (pass && A.kop == KOP_TAIL4) {
+  } else if constexpr (KOP == KOP_TAIL4) {
+    if (!pass) return;
     // F P(kd) E^-1 . C in closed form (inverse path of KOP_HEAD4).
     const cd a = csqrt(A.e0 - mk<double>(beta * beta));
     if (!is_zero(a)) {
@@ -838,7 +840,8 @@ __device__ inline void kop_apply(const KArgs<T>& A, bool pass, double beta, doub
       r[2] = g0 * sn + g2 * c;
       r[3] = g3 * c - g1 * sn;
     }
-  } else if (pass && A.kop == KOP_PROJ2) {
+  } else if constexpr (KOP == KOP_PROJ2) {
+    if (!pass) return;
     // rows (Ex, Ey) of the forward projection F diag(1,0,1,0) F^-1 in the
     // input medium (dtmm/diffract.py:242-251) times the first 2x2 half step
     // exp(i k0 d/2 (a(k) - a_c)) of the first layer.
@@ -864,7 +867,8 @@ __device__ inline void kop_apply(const KArgs<T>& A, bool pass, double beta, doub
       r[0] = K * (h0 * c - h2 * sn);
       r[1] = K * (h0 * sn + h2 * c);
     }
-  } else if (pass && A.kop == KOP_TR2) {
+  } else if constexpr (KOP == KOP_TR2) {
+    if (!pass) return;
     // K_b . T(e0 -> e1) . K_a on (Ex, Ey): half step of the layer before
     // the interface, Fresnel transmission of the tangential E field
     // (dtmm/tmm.py:925-950 for isotropic media), half step after it.
@@ -881,7 +885,8 @@ __device__ inline void kop_apply(const KArgs<T>& A, bool pass, double beta, doub
     const cd ep = (v[0] * c + v[1] * sn) * tp, es = (v[1] * c - v[0] * sn) * ts;
     r[0] = K * (ep * c - es * sn);
     r[1] = K * (ep * sn + es * c);
-  } else if (pass && A.kop == KOP_BOUND4) {
+  } else if constexpr (KOP == KOP_BOUND4) {
+    if (!pass) return;
     cd C0[4][4], D0[4][4], C1[4][4], D1[4][4], X[4][4], Y[4][4], M[4][4];
     if (A.nocorr) {
       for (int i = 0; i < 4; ++i)
@@ -903,7 +908,7 @@ __device__ inline void kop_apply(const KArgs<T>& A, bool pass, double beta, doub
   }
 }
 
-template <typename T>
+template <typename T, int KOP, bool ADJ>
 __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
   const int k = blockIdx.x * NT + threadIdx.x;
   const int ray = blockIdx.y / A.b.nlam_b;
@@ -919,8 +924,8 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
   const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
   const double beta = (2 * M_PI / k0) * sqrt(fx * fx + fy * fy);
   const double phi = atan2(fy, fx);
-  const int nin = (A.kop == KOP_TR2) ? 2 : 4;
-  const int nout = (A.kop == KOP_PROJ2 || A.kop == KOP_TR2) ? 2 : 4;
+  constexpr int nin = (KOP == KOP_TR2) ? 2 : 4;
+  constexpr int nout = (KOP == KOP_PROJ2 || KOP == KOP_TR2) ? 2 : 4;
   const cpx<T>* s = A.src + size_t(q) * nin * HW + k;
   cpx<T>* o = A.dst + size_t(q) * nout * HW + k;
   cd v[4];
@@ -929,14 +934,14 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
   const double fr = sqrt(fx * fx + fy * fy);
   const double c = fr > 0. ? fx / fr : 1., sn = fr > 0. ? fy / fr : 0.;
   cd r[4] = {mk<double>(0.), mk<double>(0.), mk<double>(0.), mk<double>(0.)};
-  if (!A.adjoint) {
-    kop_apply(A, pass, beta, phi, c, sn, k0, bb, pb, v, r);
+  if constexpr (!ADJ) {
+    kop_apply<T, KOP>(A, pass, beta, phi, c, sn, k0, bb, pb, v, r);
   } else if (pass) {
     // adjoint: r = M^H v with the columns of M from the unit vectors
     for (int j = 0; j < 4; ++j) {
       cd e[4] = {mk<double>(0.), mk<double>(0.), mk<double>(0.), mk<double>(0.)}, m[4];
       e[j] = mk<double>(1.);
-      kop_apply(A, pass, beta, phi, c, sn, k0, bb, pb, e, m);
+      kop_apply<T, KOP>(A, pass, beta, phi, c, sn, k0, bb, pb, e, m);
       cd t = mk<double>(0.);
       for (int i = 0; i < 4; ++i) t = t + conj(m[i]) * v[i];
       r[j] = t;
