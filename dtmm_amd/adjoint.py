@@ -42,25 +42,36 @@ class AdjointTransfer:
         self.tape = self.torch.empty(max(tb, 1), dtype=self.torch.uint8, device="cuda")
         self.tape_bytes = tb
         self.status = self.torch.zeros(4, dtype=self.torch.int32, device="cuda")
+        self.ws_bytes = int(self.lib.dtmmx_adjoint_workspace_size(self.job.plan._h))
+        self.ws = self.torch.empty(max(self.ws_bytes, 1), dtype=self.torch.uint8, device="cuda")
         self._recorded = False
 
     @property
     def n_layers(self):
         return self.job.stack.n_layers
 
+    def _record(self, stream):
+        j = self.job
+        _lib.check(self.lib.dtmmx_transfer_record(
+            j.plan._h, ctypes.byref(self._stack), j.fin.data_ptr(), j.fout.data_ptr(),
+            self.tape.data_ptr(), self.tape_bytes, self.status.data_ptr(), stream.cuda_stream))
+
+    def _adjoint(self, seed, grad, stream):
+        j = self.job
+        _lib.check(self.lib.dtmmx_transfer_adjoint(
+            j.plan._h, ctypes.byref(self._stack), self.tape.data_ptr(), seed.data_ptr(),
+            grad.data_ptr(), self.ws.data_ptr(), self.ws_bytes, self.status.data_ptr(),
+            stream.cuda_stream))
+
     def forward(self):
         """Runs the transfer, keeps the tape; returns the output field tensor
         (n_rays, n_wavelengths, 4, H, W) on the device."""
         torch = self.torch
-        j = self.job
-        stream = torch.cuda.current_stream()
-        _lib.check(self.lib.dtmmx_transfer_record(
-            j.plan._h, ctypes.byref(self._stack), j.fin.data_ptr(), j.fout.data_ptr(),
-            self.tape.data_ptr(), self.tape_bytes, self.status.data_ptr(), stream.cuda_stream))
+        self._record(torch.cuda.current_stream())
         torch.cuda.synchronize()
         _check_status(self.status)
         self._recorded = True
-        return j.fout
+        return self.job.fout
 
     def backward(self, seed, grad=None):
         """dL/dtheta and dL/dphi, (2, n_layers, H, W) float64 on the device,
@@ -76,15 +87,44 @@ class AdjointTransfer:
         H, W = j.plan.shape
         if grad is None:
             grad = torch.zeros((2, self.n_layers, H, W), dtype=torch.float64, device="cuda")
-        wsb = int(self.lib.dtmmx_adjoint_workspace_size(j.plan._h))
-        ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
-        stream = torch.cuda.current_stream()
-        _lib.check(self.lib.dtmmx_transfer_adjoint(
-            j.plan._h, ctypes.byref(self._stack), self.tape.data_ptr(), seed.data_ptr(),
-            grad.data_ptr(), ws.data_ptr(), wsb, self.status.data_ptr(), stream.cuda_stream))
+        self._adjoint(seed, grad, torch.cuda.current_stream())
         torch.cuda.synchronize()
         _check_status(self.status)
         return grad
+
+    def gradient_graph(self, loss_fn):
+        """Captures forward transfer + loss + adjoint in one HIP graph (the
+        per-layer launches are replayed without host work).  loss_fn(field)
+        must return (loss, seed) as device tensors without host
+        synchronisation.  Returns replay() -> (loss, grad) device tensors,
+        overwritten by every replay."""
+        torch = self.torch
+        H, W = self.job.plan.shape
+        grad = torch.zeros((2, self.n_layers, H, W), dtype=torch.float64, device="cuda")
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):  # warm-up outside the capture
+            self._record(side)
+            _, seed = loss_fn(self.job.fout)
+            self._adjoint(seed, grad, side)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        _check_status(self.status)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            stream = torch.cuda.current_stream()
+            self._record(stream)
+            loss, seed = loss_fn(self.job.fout)
+            grad.zero_()
+            self._adjoint(seed, grad, stream)
+        self._recorded = True
+
+        def replay():
+            graph.replay()
+            return loss, grad
+
+        replay.graph = graph
+        return replay
 
 
 def _iso_fmat_normal(n):
@@ -117,12 +157,13 @@ def polarizer4x4(jones, n=1.):
 _J = 0.5 * np.array([[0, 1, 0, 0], [1, 0, 0, 0], [0, 0, 0, -1], [0, 0, -1, 0]], np.complex128)
 
 
-def analyser_intensity_loss(field, jones=(1, 1), n=1.5, roi=None):
+def analyser_intensity_loss(field, jones=(1, 1), n=1.5, roi=None, as_tensor=False):
     """Loss = intensity after an analyser (polarizer4x4 of `jones` in a
     medium of index n) summed over the region of interest (y0, y1, x0, x1;
     default: the central quarter in each direction) and over all planes.
     Returns (loss, seed) with seed = dL/dconj(field) (a device tensor of
-    the field's shape and dtype)."""
+    the field's shape and dtype); loss is a float, or a 0-d device tensor
+    with as_tensor=True (no host synchronisation, for graph capture)."""
     import torch
     H, W = field.shape[-2:]
     if roi is None:
@@ -140,7 +181,9 @@ def analyser_intensity_loss(field, jones=(1, 1), n=1.5, roi=None):
     g = apply(m)
     inten = (g[..., 0, :, :] * g[..., 1, :, :].conj()).real - \
         (g[..., 2, :, :] * g[..., 3, :, :].conj()).real
-    loss = float(inten.sum().item())
+    loss = inten.sum()
+    if not as_tensor:
+        loss = float(loss.item())
     seed = torch.zeros_like(fr)
     seed[..., :, y0:y1, x0:x1] = apply(q)
     return loss, seed.to(field.dtype)

@@ -155,11 +155,16 @@ __host__ __device__ constexpr int max_vals() {
   return m;
 }
 
-// LDS address of line position x: x * STRIDE + x / 8 * PADW (PADW = STRIDE
-// adds one padding slot per 8 positions; 0 disables padding).
+// LDS address of line position x: x * STRIDE + (x >> DTMMX_LDS_PAD_SHIFT) *
+// PADW; PADW padding slots after every 2^shift positions (8 by default)
+// break up the power-of-two strides of the stage exchanges across the 64
+// LDS banks (0 disables padding).
+#ifndef DTMMX_LDS_PAD_SHIFT
+#define DTMMX_LDS_PAD_SHIFT 3
+#endif
 template <int STRIDE, int PADW>
 __device__ __forceinline__ int lds_at(int x) {
-  return x * STRIDE + (PADW ? (x >> 3) * PADW : 0);
+  return x * STRIDE + (PADW ? (x >> DTMMX_LDS_PAD_SHIFT) * PADW : 0);
 }
 
 // Barrier between LDS write and read phases: the whole work-group, or only

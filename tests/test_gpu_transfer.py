@@ -327,3 +327,29 @@ def test_adjoint_gradient_matches_oracle_finite_differences():
     err = np.abs(g64 - grad).max() / gmax
     flush("  c64 vs c128 gradient err %.3e" % err)
     assert err < 1e-3
+
+
+@pytest.mark.gpu
+def test_gradient_graph_replays_match_eager():
+    """The gradient evaluation captured as one HIP graph: every replay, with
+    eager work and synchronisations in between, equals the eager result."""
+    flush("case gradient graph replays")
+    fd, optical = random_case(shape=(3, 32, 64), wavelengths=(500., 600.), jones=(1, 0))
+    roi = (8, 24, 16, 48)
+    tr = dm.AdjointTransfer(fd, optical, beta=0., phi=0., precision="c128")
+    out = tr.forward()
+    loss0, seed = dm.analyser_intensity_loss(out, (1, 1), 1.5, roi)
+    g0 = tr.backward(seed).cpu().numpy()
+    replay = tr.gradient_graph(
+        lambda f: dm.analyser_intensity_loss(f, (1, 1), 1.5, roi, as_tensor=True))
+    import torch
+    for i in range(3):
+        loss, grad = replay()
+        torch.cuda.synchronize()
+        g = grad.cpu().numpy()
+        flush("  replay %d loss %.12g max diff %.3e" % (i, float(loss.item()), np.abs(g - g0).max()))
+        assert abs(float(loss.item()) - loss0) <= 1e-12 * abs(loss0)
+        np.testing.assert_allclose(g, g0, rtol=0, atol=1e-12 * np.abs(g0).max())
+        # eager work between replays
+        out2 = dm.transfer_field(fd, optical, beta=0., phi=0., method="4x4")[0]
+        assert np.isfinite(out2).all()
