@@ -20,7 +20,7 @@ DTMMX_C64 = 0
 DTMMX_C128 = 1
 DTMMX_METHOD_2X2 = 2
 DTMMX_METHOD_4X4 = 4
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_WAVELENGTHS = 64
 MAX_BEAMS = 64
 
@@ -67,6 +67,7 @@ class PlanDesc(ctypes.Structure):
         ("norm", ctypes.c_int),
         ("smooth", ctypes.c_double),
         ("reflection", ctypes.c_int),
+        ("checkpoint", ctypes.c_int),
     ]
 
 

@@ -11,6 +11,13 @@ angles[..., 2]) of every voxel.
 
 Convention: for a real loss L of the output field F, ``seed`` is
 dL/dconj(F), so that dL = 2 Re <seed, dF>.
+
+The tape keeps the row-pass input of every `checkpoint`-th propagation step
+(default 8); the adjoint recomputes the steps between two checkpoints, which
+bounds the tape at ceil(steps / checkpoint) fields plus checkpoint - 1
+fields of workspace.  ``AdjointTransfer.apply(theta, phi)`` is the same
+transfer as a ``torch.autograd.Function`` of device tensors of the director
+angles.
 """
 import ctypes
 
@@ -30,13 +37,17 @@ class AdjointTransfer:
     """
 
     def __init__(self, field_data, optical_data, beta=0., phi=0., nin=None, nout=None, nstep=1,
-                 betamax=None, precision=None, split_layers=False):
+                 betamax=None, precision=None, split_layers=False, checkpoint=8):
         self.torch = _require_gpu()
         self.lib = _lib.load()
+        if int(checkpoint) < 1:
+            raise ValueError("checkpoint must be >= 1")
+        self.checkpoint = int(checkpoint)
         self.job = _prepare(field_data, optical_data, beta=beta, phi=phi, nin=nin, nout=nout,
                             npass=1, nstep=nstep, diffraction=1, reflection=1, method="4x4",
                             norm=DTMM_NORM_FFT, betamax=betamax, smooth=SMOOTH,
-                            split_layers=split_layers, precision=precision)
+                            split_layers=split_layers, precision=precision,
+                            checkpoint=self.checkpoint)
         self._stack = self.job.stack.c_struct()
         tb = int(self.lib.dtmmx_tape_size(self.job.plan._h, ctypes.byref(self._stack)))
         self.tape = self.torch.empty(max(tb, 1), dtype=self.torch.uint8, device="cuda")
@@ -45,12 +56,34 @@ class AdjointTransfer:
         self.ws_bytes = int(self.lib.dtmmx_adjoint_workspace_size(self.job.plan._h))
         self.ws = self.torch.empty(max(self.ws_bytes, 1), dtype=self.torch.uint8, device="cuda")
         self._recorded = False
+        self._version = 0  # forward runs; a tape serves the backward of its own run only
 
     @property
     def n_layers(self):
         return self.job.stack.n_layers
 
+    def set_angles(self, theta, phi):
+        """Copies device tensors (n_layers, H, W) of the polar and azimuthal
+        director angles into the stack (angles[1], angles[2]); the effective
+        media do not depend on them."""
+        ang = self.job.stack.angles
+        shape = tuple(ang.shape[1:])
+        for t in (theta, phi):
+            if tuple(t.shape) != shape:
+                raise ValueError("angles must have shape %s, got %s" % (shape, tuple(t.shape)))
+        ang[1].copy_(theta)
+        ang[2].copy_(phi)
+
+    def apply(self, theta, phi):
+        """The transfer as a differentiable function of the director angles:
+        returns the output field (n_rays, n_wavelengths, 4, H, W) on the
+        device; backward gives dL/dtheta and dL/dphi in the dtype of the
+        inputs.  Each call runs the transfer (and records its tape); the
+        backward of a call must run before the next call."""
+        return _DirectorTransfer.apply(theta, phi, self)
+
     def _record(self, stream):
+        self._version += 1
         j = self.job
         _lib.check(self.lib.dtmmx_transfer_record(
             j.plan._h, ctypes.byref(self._stack), j.fin.data_ptr(), j.fout.data_ptr(),
@@ -125,6 +158,54 @@ class AdjointTransfer:
 
         replay.graph = graph
         return replay
+
+
+def _director_transfer_class():
+    import torch
+
+    class DirectorTransfer(torch.autograd.Function):
+        """field_out = transfer(theta, phi) with the adjoint as backward.
+        torch's gradient of a complex output is 2 dL/dconj(F), so the seed
+        is half of it."""
+
+        @staticmethod
+        def forward(ctx, theta, phi, tr):
+            tr.set_angles(theta.detach(), phi.detach())
+            tr._record(torch.cuda.current_stream())
+            tr._recorded = True
+            ctx.tr = tr
+            ctx.version = tr._version
+            ctx.dtypes = (theta.dtype, phi.dtype)
+            return tr.job.fout.clone()
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            tr = ctx.tr
+            if tr._version != ctx.version:
+                raise RuntimeError("the transfer ran again before this backward; the tape "
+                                   "holds the later run")
+            j = tr.job
+            seed = (0.5 * grad_out).to(dtype=j.fout.dtype).contiguous()
+            H, W = j.plan.shape
+            grad = torch.zeros((2, tr.n_layers, H, W), dtype=torch.float64, device="cuda")
+            tr._adjoint(seed, grad, torch.cuda.current_stream())
+            _check_status(tr.status)
+            return grad[0].to(ctx.dtypes[0]), grad[1].to(ctx.dtypes[1]), None
+
+    return DirectorTransfer
+
+
+class _LazyFunction:
+    """Defers the torch import to the first use."""
+    _cls = None
+
+    def apply(self, *args):
+        if _LazyFunction._cls is None:
+            _LazyFunction._cls = _director_transfer_class()
+        return _LazyFunction._cls.apply(*args)
+
+
+_DirectorTransfer = _LazyFunction()
 
 
 def _iso_fmat_normal(n):

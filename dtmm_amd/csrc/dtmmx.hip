@@ -127,13 +127,21 @@ constexpr int col_tx() {
   return tx;
 }
 
+// Small grids: the lines of a work-group are halved (down to one) until the
+// launch has at least COL_MIN_WORK work-groups, two per CU.
+constexpr int COL_MIN_WORK = 512;
+
 template <typename T, int OP>
-static int launch_col(int n, const ColArgs<T>& a, int nlines_per_lam, hipStream_t s) {
+static int launch_col(int n, const ColArgs<T>& a0, int nlines_per_lam, bool tune, hipStream_t s) {
+  ColArgs<T> a = a0;
   switch (n) {
 #define X(N)                                                                                \
   case N: {                                                                                 \
     constexpr int TX = col_tx<T, N>();                                                      \
-    const int nwork = ((a.W + TX - 1) / TX) * a.b.nlam_b * (nlines_per_lam / a.ncl);        \
+    const int ntile = (a.W + TX - 1) / TX;                                                  \
+    while (tune && a.ncl % 2 == 0 && ntile * a.b.nlam_b * (nlines_per_lam / a.ncl) < COL_MIN_WORK) \
+      a.ncl /= 2;                                                                           \
+    const int nwork = ntile * a.b.nlam_b * (nlines_per_lam / a.ncl);                        \
     hipLaunchKernelGGL((col_kernel<T, N, TX, OP>), dim3(nwork), dim3(TX * col_lanes<N>()), 0, \
                        s, a);                                                               \
     break;                                                                                  \
@@ -280,20 +288,24 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   // lines per work-group: the components of one ray, or of two rays sharing a beam
   a.ncl = nc;
   if (b.nrays % 2 == 0 && P->bm.rays_per_beam % 2 == 0 && nc <= 2) a.ncl = 2 * nc;
+  bool tune = true;
   if (const char* e = std::getenv("DTMMX_COL_NCL")) {
     const int v = std::atoi(e);
-    if (v > 0 && nc % v == 0) a.ncl = v;
+    if (v > 0 && nc % v == 0) {
+      a.ncl = v;
+      tune = false;
+    }
   }
   a.tw = (const cpx<T>*)P->tw_h;
   a.amax = P->amax;
   a.bm = bm ? *bm : P->bm;
   const int nlines = b.nrays * nc;
   const int h = a.H;
-  if (op == OP_FWD) return launch_col<T, OP_FWD>(h, a, nlines, s);
-  if (op == OP_INV) return launch_col<T, OP_INV>(h, a, nlines, s);
+  if (op == OP_FWD) return launch_col<T, OP_FWD>(h, a, nlines, tune, s);
+  if (op == OP_INV) return launch_col<T, OP_INV>(h, a, nlines, tune, s);
   if (op == OP_DIAG) {
     timer_start(1, s);
-    int r = launch_col<T, OP_DIAG>(h, a, nlines, s);
+    int r = launch_col<T, OP_DIAG>(h, a, nlines, tune, s);
     timer_stop(s);
     return r;
   }
@@ -304,8 +316,8 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
     a.wcy = P->wcy;
     a.wn = P->wn;
     timer_start(1, s);
-    int r = op == OP_WIN ? launch_col<T, OP_WIN>(h, a, nlines, s)
-                         : launch_col<T, OP_SPLIT>(h, a, nlines, s);
+    int r = op == OP_WIN ? launch_col<T, OP_WIN>(h, a, nlines, tune, s)
+                         : launch_col<T, OP_SPLIT>(h, a, nlines, tune, s);
     timer_stop(s);
     return r;
   }
@@ -926,6 +938,38 @@ static bool adjoint_supported(const dtmmx_plan* P) {
 
 // Forward 4x4 transfer (diffraction 1, npass 1) that records the row-pass
 // input of every step on the tape for the adjoint.
+static int ckpt_interval(const dtmmx_plan* P) { return P->d.checkpoint > 1 ? P->d.checkpoint : 1; }
+
+// The propagation from the row-pass output of step l to the row-pass input
+// of step l + 1 (one beam per ray, 4x4): the diagonal column pass between
+// equal effective media, else the dense boundary operator in k-space.
+template <typename T>
+static int step_transition(const dtmmx_plan* P, cpx<T>* f, Batch b, const std::vector<Step>& st,
+                           int l, double scale, hipStream_t s) {
+  if (eq(st[l].e, st[l + 1].e)) {
+    ColArgs<T> ca{};
+    ca.da = st[l].d;
+    ca.db = st[l + 1].d;
+    ca.ea = st[l].e;
+    ca.eb = st[l + 1].e;
+    ca.kca = 1.;
+    ca.kcb = 1.;
+    ca.scale = scale;
+    ca.dense_sign = 1;
+    return col_pass<T>(P, f, b, 4, OP_DIAG, s, &ca);
+  }
+  TRY(col_pass<T>(P, f, b, 4, OP_FWD, s));
+  KArgs<T> bd{};
+  bd.kop = KOP_BOUND4;
+  bd.d0 = st[l].d;
+  bd.d1 = st[l + 1].d;
+  bd.e0 = st[l].e;
+  bd.e1 = st[l + 1].e;
+  bd.scale = scale;
+  TRY(kspace<T>(P, f, f, b, bd, s));
+  return col_pass<T>(P, f, b, 4, OP_INV, s);
+}
+
 template <typename T>
 static int record_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
                        void* tape, int* status, hipStream_t s) {
@@ -950,36 +994,12 @@ static int record_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* fi
   k.scale = scale;
   TRY(kspace<T>(P, out, out, b, k, s));
   TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+  const int K = ckpt_interval(P);
   for (int l = 0; l < L; ++l) {
-    HIP_TRY(hipMemcpyAsync(tp + size_t(l) * fsz, out, fsz * sizeof(cpx<T>),
-                           hipMemcpyDeviceToDevice, s));
-    const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status);
+    RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status);
+    if (l % K == 0) la.save = tp + size_t(l / K) * fsz;  // checkpoint: the row kernel keeps its input
     TRY(row_pass<T>(P, out, b, 4, OP_ROP, s, &la));
-    if (l + 1 < L) {
-      if (eq(st[l].e, st[l + 1].e)) {
-        ColArgs<T> ca{};
-        ca.da = st[l].d;
-        ca.db = st[l + 1].d;
-        ca.ea = st[l].e;
-        ca.eb = st[l + 1].e;
-        ca.kca = 1.;
-        ca.kcb = 1.;
-        ca.scale = scale;
-        ca.dense_sign = 1;
-        TRY(col_pass<T>(P, out, b, 4, OP_DIAG, s, &ca));
-      } else {
-        TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
-        KArgs<T> bd{};
-        bd.kop = KOP_BOUND4;
-        bd.d0 = st[l].d;
-        bd.d1 = st[l + 1].d;
-        bd.e0 = st[l].e;
-        bd.e1 = st[l + 1].e;
-        bd.scale = scale;
-        TRY(kspace<T>(P, out, out, b, bd, s));
-        TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
-      }
-    }
+    if (l + 1 < L) TRY(step_transition<T>(P, out, b, st, l, scale, s));
   }
   TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
   KArgs<T> t{};
@@ -1010,8 +1030,17 @@ static int adjoint_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* t
   const std::vector<Step> st = expand_steps(S, 1);
   const int L = int(st.size());
   if (L == 0) return DTMMX_OK;
+  const int nbeam = nrays / P->d.rays_per_beam;
   cpx<T>* adj = (cpx<T>*)ws;
   double* gpart = (double*)((char*)ws + (fsz * sizeof(cpx<T>) + 255) / 256 * 256);
+  cpx<T>* table =
+      (cpx<T>*)((char*)gpart + (size_t(nrays) * nlam * 2 * HW * sizeof(double) + 255) / 256 * 256);
+  // checkpointed tape: wk holds the recomputed row input of a block's last
+  // step, blk those of the steps between the checkpoint and the last
+  const int K = ckpt_interval(P);
+  cpx<T>* wk = (cpx<T>*)((char*)table +
+                         (size_t(GRAD_COMPS) * nbeam * HW * sizeof(cpx<T>) + 255) / 256 * 256);
+  cpx<T>* blk = wk + fsz;
   const cpx<T>* tp = (const cpx<T>*)tape;
   HIP_TRY(hipMemcpyAsync(adj, seed, fsz * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
   Batch b{nrays, nlam, 0, nlam};
@@ -1054,14 +1083,52 @@ static int adjoint_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* t
         TRY(col_pass<T>(P, adj, b, 4, OP_INV, s));
       }
     }
+    const cpx<T>* row_in = tp + size_t(l) * fsz;  // row-pass input of step l
+    if (K > 1) {
+      const int l0 = l / K * K;
+      cpx<T>* ck = const_cast<cpx<T>*>(tp) + size_t(l / K) * fsz;
+      if (l == L - 1 || (l + 1) % K == 0) {
+        // last step of a block: recompute the block's forward from its checkpoint
+        for (int m = l0; m < l; ++m) {
+          RowArgs<T> lm = layer_args<T>(S, HW, st[m].l, st[m].d, 1, status);
+          if (m == l0) {
+            lm.acc = wk;  // reads the checkpoint, writes wk
+            lm.acc_mode = 1;
+            TRY(row_pass<T>(P, ck, b, 4, OP_ROP, s, &lm));
+          } else {
+            lm.save = blk + size_t(m - l0 - 1) * fsz;
+            TRY(row_pass<T>(P, wk, b, 4, OP_ROP, s, &lm));
+          }
+          TRY(step_transition<T>(P, wk, b, st, m, scale, s));
+        }
+      }
+      row_in = l == l0 ? ck : ((l == L - 1 || (l + 1) % K == 0) ? wk : blk + size_t(l - l0 - 1) * fsz);
+    }
+    const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status);
+    if (l + 1 == L || st[l + 1].l != st[l].l) {  // sub-steps of a layer share the table
+      EigArgs<T> e{};
+      e.eps_re = la.eps_re;
+      e.eps_im = la.eps_im;
+      e.ang = la.ang;
+      e.comp_stride = la.comp_stride;
+      e.HW = HW;
+      e.nbeam = nbeam;
+      e.table = table;
+      e.status = status;
+      e.bm = P->bm;
+      hipLaunchKernelGGL((eig_grad_kernel<T>), dim3(unsigned((HW + NT - 1) / NT), nbeam), dim3(NT), 0,
+                         s, e);
+      HIP_TRY(hipGetLastError());
+    }
     AdjArgs<T> a{};
     a.adj = adj;
-    a.tape = tp + size_t(l) * fsz;
+    a.table = table;
+    a.nbeam = nbeam;
+    a.tape = row_in;
     a.H = H;
     a.W = W;
     a.b = b;
     a.tw = (const cpx<T>*)P->tw_w;
-    const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status);
     a.eps_re = la.eps_re;
     a.eps_im = la.eps_im;
     a.ang = la.ang;
@@ -1071,10 +1138,10 @@ static int adjoint_impl(const dtmmx_plan* P, const dtmmx_stack* S, const void* t
     a.gpart = gpart;
     a.bm = P->bm;
     timer_start(0, s);
-    TRY(launch_row_adj<T>(W, a, H * nrays, s));
+    TRY(launch_row_adj<T>(W, a, H * nrays * nlam, s));
     timer_stop(s);
     hipLaunchKernelGGL(grad_sum_kernel, dim3(unsigned((HW + NT - 1) / NT)), dim3(NT), 0, s,
-                       (const double*)gpart, nrays, HW, grad + size_t(st[l].l) * HW,
+                       (const double*)gpart, nrays * nlam, HW, grad + size_t(st[l].l) * HW,
                        grad + (size_t(nl) + st[l].l) * HW);
     HIP_TRY(hipGetLastError());
   }
@@ -1348,6 +1415,7 @@ static int validate_desc(const dtmmx_plan_desc* d) {
     return fail(DTMMX_E_UNSUPPORTED, "npass > 1 is built for the 4x4 method only");
   if (d->npass > 1 && d->norm != 0 && d->norm != DTMMX_NORM_FFT)
     return fail(DTMMX_E_UNSUPPORTED, "npass > 1: only norm 0 (total) and 1 (fft) are built");
+  if (d->checkpoint < 0) return fail(DTMMX_E_ARG, "checkpoint interval must be >= 0");
   if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1)
     return fail(DTMMX_E_UNSUPPORTED, "the 4x4 method is built for reflection mode 1");
   if (d->method == DTMMX_METHOD_2X2 && d->reflection != 0 && d->reflection != 1)
@@ -1451,7 +1519,8 @@ size_t dtmmx_tape_size(const dtmmx_plan* P, const dtmmx_stack* S) {
   if (!P || !S) return 0;
   const size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
   const size_t fsz = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * P->d.height * P->d.width;
-  return total_steps(S) * fsz * cb;
+  const size_t K = size_t(ckpt_interval(P));
+  return (total_steps(S) + K - 1) / K * fsz * cb;
 }
 
 size_t dtmmx_adjoint_workspace_size(const dtmmx_plan* P) {
@@ -1459,7 +1528,11 @@ size_t dtmmx_adjoint_workspace_size(const dtmmx_plan* P) {
   const size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
   const size_t HW = size_t(P->d.height) * P->d.width;
   const size_t f = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW * cb;
-  return (f + 255) / 256 * 256 + size_t(P->d.n_rays) * 2 * HW * sizeof(double);
+  const size_t nbeam = size_t(P->d.n_rays / P->d.rays_per_beam);
+  const size_t K = size_t(ckpt_interval(P));
+  return (f + 255) / 256 * 256 +
+         (size_t(P->d.n_rays) * P->d.n_wavelengths * 2 * HW * sizeof(double) + 255) / 256 * 256 +
+         (size_t(GRAD_COMPS) * nbeam * HW * cb + 255) / 256 * 256 + (K > 1 ? (K - 1) * f : 0);
 }
 
 int dtmmx_transfer_record(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,

@@ -96,6 +96,9 @@ struct RowArgs {
   int wn, wi;
   cd we;
   double wd;
+  // OP_ROP: when set, the input lines are also stored here (field layout),
+  // the adjoint's record of the row-pass input
+  cpx<T>* save;
   Beams bm;
 };
 
@@ -237,6 +240,14 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
       for (int j = 0; j < NIN; ++j)
         if (slot_valid<N, REV1, NL, false>(j, lane)) v[j] = gl[slot_pos<N, REV1, NL, false>(j, lane)];
     }
+    if constexpr (OP == OP_ROP) {
+      if (A.save) {
+        cpx<T>* sl = A.save + (gl - A.field);
+#pragma unroll
+        for (int j = 0; j < NIN; ++j)
+          if (slot_valid<N, REV1, NL, false>(j, lane)) sl[slot_pos<N, REV1, NL, false>(j, lane)] = v[j];
+      }
+    }
     if constexpr (OP == OP_ROPW) {
       // window column factor and mean-beam correction of this line
       const int beam = (rg * NR + rr) / A.bm.rays_per_beam;
@@ -333,9 +344,64 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
 //   dL/dtheta, dL/dphi += 2 Re( vhat^H (dR/dangle) u )
 // with u = B_x g the real-space field the forward row pass saw (g recorded
 // on the tape) and vhat = B_x of the incoming adjoint.  One work-group per
-// (row y, ray); the per-pixel fp64 solve with forward-mode derivatives
-// (rop_grad) runs for every wavelength.
+// (row y, ray); the per-pixel operator data and derivatives come from the
+// layer's eig_grad_kernel table.
 // ---------------------------------------------------------------------------
+
+// Per-pixel operator data and its theta / phi derivatives for one layer and
+// every beam, solved once per layer (fp64 forward-mode, rop_grad) and read by
+// the adjoint row kernel for every wavelength and ray: table
+// [GRAD_COMPS][nbeam][H*W] of a[4], P0[2][2], P1[2][2], da[2][4],
+// dP0[2][2][2], dP1[2][2][2] and the invertibility flag.
+constexpr int GRAD_COMPS = 37;
+
+template <typename T>
+struct EigArgs {
+  const double* eps_re;
+  const double* eps_im;
+  const double* ang;
+  size_t comp_stride;
+  size_t HW;
+  int nbeam;
+  cpx<T>* table;
+  int* status;
+  Beams bm;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void eig_grad_kernel(EigArgs<T> A) {
+  const size_t o = size_t(blockIdx.x) * NT + threadIdx.x;
+  const int beam = blockIdx.y;
+  if (o >= A.HW) return;
+  const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
+  double s0, c0;
+  sincos(phi0, &s0, &c0);
+  const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+  const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
+  const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
+                           A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
+  RopGrad g;
+  rop_grad(beta, c0, s0, phi0, e0, e1, e2, A.ang[A.comp_stride + o], A.ang[2 * A.comp_stride + o], g);
+  if (g.kind == KIND_BIAXIAL && A.status) atomicOr(A.status, 1);
+  cpx<T>* t = A.table + size_t(beam) * A.HW + o;
+  const size_t cs = size_t(A.nbeam) * A.HW;
+  int c = 0;
+  auto put = [&](cd z) { t[(c++) * cs] = cast<T>(z); };
+  for (int i = 0; i < 4; ++i) put(g.a[i]);
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) put(g.P0[i][j]);
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) put(g.P1[i][j]);
+  for (int k = 0; k < 2; ++k)
+    for (int i = 0; i < 4; ++i) put(g.da[k][i]);
+  for (int k = 0; k < 2; ++k)
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) put(g.dP0[k][i][j]);
+  for (int k = 0; k < 2; ++k)
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) put(g.dP1[k][i][j]);
+  put(mk<double>(g.ok4 ? 1. : 0.));
+}
 
 template <typename T>
 struct AdjArgs {
@@ -350,12 +416,14 @@ struct AdjArgs {
   size_t comp_stride;
   double d;
   int* status;
-  double* gpart;        // [nrays][2][H][W]: this sub-step's gradient per ray
+  double* gpart;        // [nrays][nlam_b][2][H][W]: this sub-step's gradient per ray, wavelength
+  const cpx<T>* table;  // eig_grad_kernel output of this layer
+  int nbeam;
   Beams bm;
 };
 
 template <typename T, int N>
-__global__ __launch_bounds__(row_lanes<N>() * 4, 1) void row_adj_kernel(AdjArgs<T> A) {
+__global__ __launch_bounds__(row_lanes<N>() * 4, 2) void row_adj_kernel(AdjArgs<T> A) {
   constexpr int NL = row_lanes<N>();
   constexpr bool WV = (NL == 64);
   constexpr int LINES = 4;
@@ -375,17 +443,15 @@ __global__ __launch_bounds__(row_lanes<N>() * 4, 1) void row_adj_kernel(AdjArgs<
   const int comp = tid / NL, lane = tid % NL;
   for (int i = tid; i < N; i += NTW) tws[i] = A.tw[i];
   const int w = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int ray = w % A.b.nrays;
-  const int y = w / A.b.nrays;
+  const int rl = w % (A.b.nrays * A.b.nlam_b);  // (ray, wavelength) of this work-group
+  const int ray = rl / A.b.nlam_b, jl = rl % A.b.nlam_b;
+  const int y = w / (A.b.nrays * A.b.nlam_b);
   const size_t HW = size_t(A.H) * N;
   const int beam = ray / A.bm.rays_per_beam;
-  const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
-  double s0, c0;
-  sincos(phi0, &s0, &c0);
   double gt[PPX], gp[PPX];
   for (int k = 0; k < PPX; ++k) gt[k] = gp[k] = 0.;
   __syncthreads();
-  for (int jl = 0; jl < A.b.nlam_b; ++jl) {
+  {
     const int lam = A.b.lam0 + jl;
     const size_t off = ((size_t(ray) * A.b.nlam + lam) * 4 + comp) * HW + size_t(y) * N;
     cpx<T>* ga = A.adj + off;
@@ -415,30 +481,32 @@ __global__ __launch_bounds__(row_lanes<N>() * 4, 1) void row_adj_kernel(AdjArgs<
       const int x = tid + k * NTW;
       if (x >= N) continue;
       const size_t o = size_t(y) * N + x;
-      const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
-      const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
-      const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
-                               A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
-      RopGrad g;
-      rop_grad(beta, c0, s0, phi0, e0, e1, e2, A.ang[A.comp_stride + o], A.ang[2 * A.comp_stride + o],
-               g);
-      if (g.kind == KIND_BIAXIAL && A.status) atomicOr(A.status, 1);
-      const double ok = g.ok4 ? 1. : 0.;
+      // table entries of this pixel: a 0-3, P0 4-7, P1 8-11, da 12-19,
+      // dP0 20-27, dP1 28-35, ok 36
+      const cpx<T>* tp = A.table + size_t(beam) * HW + o;
+      const size_t cs = size_t(A.nbeam) * HW;
+      auto tb = [&](int c) { return cast<double>(tp[c * cs]); };
+      const double ok = tb(GRAD_COMPS - 1).x;
       const int ax = lds_at<1, 1>(x);
       // blocks (E+x, E+y) = comps (0, 2) with P0, (E-x, E-y) = comps (1, 3) with P1
       for (int blk = 0; blk < 2; ++blk) {
         const int ca = blk, cb = blk + 2;
         const int ia = blk, ib = blk + 2;  // alpha indices: a0, a2 (+), a1, a3 (-)
-        const cd pa = cexpi(g.a[ia] * kd), pb = cexpi(g.a[ib] * kd);
-        const cd(*P)[2] = blk == 0 ? g.P0 : g.P1;
+        const cd aa = tb(ia), ab = tb(ib);
+        const cd pa = cexpi(aa * kd), pb = cexpi(ab * kd);
+        cd P[2][2];
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) P[i][j] = tb(4 + 4 * blk + 2 * i + j);
         const cd uh[2] = {cast<double>(lu[ca * LS + ax]), cast<double>(lu[cb * LS + ax])};
         const cd vh[2] = {cast<double>(la[ca * LS + ax]), cast<double>(la[cb * LS + ax])};
         const cd dp = (pa - pb) * ok, p2 = pb * ok;
         // gradient: vhat^H (dp2 I + (dpa - dpb) P + (pa - pb) dP) u
         for (int dir = 0; dir < 2; ++dir) {
-          const cd(*dP)[2] = blk == 0 ? g.dP0[dir] : g.dP1[dir];
-          const cd dpa = mk<double>(0., kd) * g.da[dir][ia] * pa;
-          const cd dpb = mk<double>(0., kd) * g.da[dir][ib] * pb;
+          cd dP[2][2];
+          for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) dP[i][j] = tb(20 + 8 * blk + 4 * dir + 2 * i + j);
+          const cd dpa = mk<double>(0., kd) * tb(12 + 4 * dir + ia) * pa;
+          const cd dpb = mk<double>(0., kd) * tb(12 + 4 * dir + ib) * pb;
           cd s = mk<double>(0.);
           for (int i = 0; i < 2; ++i) {
             cd t = dpb * uh[i];
@@ -476,12 +544,12 @@ __global__ __launch_bounds__(row_lanes<N>() * 4, 1) void row_adj_kernel(AdjArgs<
     const int x = tid + k * NTW;
     if (x >= N) continue;
     const size_t o = size_t(y) * N + x;
-    A.gpart[(size_t(ray) * 2) * HW + o] = gt[k];
-    A.gpart[(size_t(ray) * 2 + 1) * HW + o] = gp[k];
+    A.gpart[(size_t(rl) * 2) * HW + o] = gt[k];
+    A.gpart[(size_t(rl) * 2 + 1) * HW + o] = gp[k];
   }
 }
 
-// grad[0|1][pixel] += sum over rays of gpart[ray][0|1][pixel].
+// grad[0|1][pixel] += sum over (ray, wavelength) r of gpart[r][0|1][pixel].
 static __global__ __launch_bounds__(NT) void grad_sum_kernel(const double* gpart, int nrays, size_t HW,
                                                        double* gtheta, double* gphi) {
   const size_t i = size_t(blockIdx.x) * NT + threadIdx.x;

@@ -164,7 +164,7 @@ class Plan:
 
     def __init__(self, shape, ks, beta, phi, n_rays, rays_per_beam, method, precision="c64",
                  betamax=BETAMAX, diffraction=1, npass=1, norm=DTMM_NORM_FFT, smooth=SMOOTH,
-                 reflection=None):
+                 reflection=None, checkpoint=1):
         lib = _lib.load()
         self.shape = tuple(shape)
         self.ks = np.ascontiguousarray(ks, np.float64)
@@ -192,6 +192,7 @@ class Plan:
         d.norm = int(norm)
         d.smooth = float(smooth)
         d.reflection = int(reflection if reflection is not None else (1 if method == "4x4" else 0))
+        d.checkpoint = int(checkpoint)
         h = ctypes.c_void_p()
         _lib.check(lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)))
         self._h = h
@@ -308,9 +309,9 @@ class _Prepared:
 def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None, npass=None,
              nstep=1, diffraction=None, reflection=None, method=None, multiray=False,
              norm=DTMM_NORM_FFT, betamax=None, smooth=SMOOTH, split_layers=False, eff_data=None,
-             ret_bulk=False, out=None, precision=None):
+             ret_bulk=False, out=None, precision=None, checkpoint=1):
     """Defaults, validation, plan and device data of one transfer_field call
-    (dtmm/transfer.py:433-552)."""
+    (dtmm/transfer.py:433-552); `checkpoint` is the adjoint tape interval."""
     torch = _require_gpu()
     nin = N_COVER if nin is None else nin
     nout = N_COVER if nout is None else nout
@@ -346,7 +347,7 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
 
     plan = Plan(shape, ks, b, p, n_rays, rays_per_beam, kind, precision, betamax,
                 diffraction=int(diffraction), npass=int(npass), norm=norm, smooth=smooth,
-                reflection=reflection)
+                reflection=reflection, checkpoint=checkpoint)
     stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout, split_layers=split_layers)
     fin = torch.from_numpy(np.ascontiguousarray(field_np.reshape((n_rays, len(ks), 4) + shape)))
     fin = fin.to(device="cuda", dtype=plan.dtype)

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define DTMMX_ABI_VERSION 2
+#define DTMMX_ABI_VERSION 3
 
 enum {
   DTMMX_OK = 0,
@@ -84,6 +84,10 @@ typedef struct {
   int reflection;     /* 4x4: 1.  2x2: 0, or 1 = Fresnel transmission at
                          interfaces between different effective media
                          (propagate_2x2.py:28-30, diffract.py:215-225)    */
+  int checkpoint;     /* adjoint tape (no counterpart in the reference):
+                         the forward keeps the row-pass input of every
+                         checkpoint-th propagation step and the adjoint
+                         recomputes the steps between; 0 or 1: every step */
 } dtmmx_plan_desc;
 
 /* A layer stack.  Layer l is propagated as n_steps[l] sub-steps of
@@ -156,7 +160,9 @@ int dtmmx_transfer_batch(const dtmmx_job* jobs, int n, int* status, void* stream
  * L(field_out) of the output field.
  *
  * dtmmx_transfer_record: the forward transfer, keeping the row-pass input
- *   of every propagation step on `tape` (dtmmx_tape_size bytes).
+ *   of every desc.checkpoint-th propagation step on `tape` (dtmmx_tape_size
+ *   bytes: ceil(steps / checkpoint) fields); the adjoint recomputes the
+ *   steps between two checkpoints from the first of them.
  * dtmmx_transfer_adjoint: from seed = dL/dconj(field_out) (device, field
  *   layout; dL = 2 Re <seed, d field_out>) and the tape, adds dL/dtheta and
  *   dL/dphi of every voxel to grad (device float64 [2][n_layers][H][W]:

@@ -269,10 +269,11 @@ def test_transfer_batch_uneven_samples():
             assert err < TOL[precision], err
 
 
-def _oracle_analyser_loss(fd, d, eps, ang, roi, jones=(1, 1), n=1.5):
+def _oracle_analyser_loss(fd, d, eps, ang, roi, jones=(1, 1), n=1.5, beta=0., phi=0., nstep=1):
     """The analysed-intensity loss on the oracle's output field."""
     field, wl, ps = fd
-    out = oracle.transfer_field((field, wl, ps), [(d, eps, ang)], beta=0., phi=0., method="4x4")[0]
+    out = oracle.transfer_field((field, wl, ps), [(d, eps, ang)], beta=beta, phi=phi, nstep=nstep,
+                                method="4x4")[0]
     m = dm.polarizer4x4(jones, n)
     y0, y1, x0, x1 = roi
     sub = out[..., :, y0:y1, x0:x1]
@@ -327,6 +328,43 @@ def test_adjoint_gradient_matches_oracle_finite_differences():
     err = np.abs(g64 - grad).max() / gmax
     flush("  c64 vs c128 gradient err %.3e" % err)
     assert err < 1e-3
+
+
+@pytest.mark.gpu
+def test_adjoint_gradient_beams_and_substeps():
+    """Two oblique beams of two polarisations each (four rays) and two
+    sub-steps per layer: the c128 adjoint gradient against central
+    differences of the oracle loss, which sums over every ray."""
+    flush("case adjoint gradient, 2 beams x 2 polarisations, nstep 2")
+    beta, phi = np.array([0., 0.15]), np.array([0., 0.7])
+    fd, optical = random_case(shape=(2, 16, 64), wavelengths=(500., 600.), beta=beta, phi=phi)
+    d, eps, ang = optical[0]
+    eps = np.ascontiguousarray(np.broadcast_to(eps, ang.shape))
+    ang = np.ascontiguousarray(ang)
+    roi = (4, 12, 16, 48)
+    tr = dm.AdjointTransfer(fd, [(d, eps, ang)], beta=beta, phi=phi, nstep=2, precision="c128")
+    out = tr.forward()
+    loss, seed = dm.analyser_intensity_loss(out, (1, 1), 1.5, roi)
+    ref_loss = _oracle_analyser_loss(fd, d, eps, ang, roi, beta=beta, phi=phi, nstep=2)
+    flush("  loss gpu %.12g oracle %.12g" % (loss, ref_loss))
+    assert abs(loss - ref_loss) < 1e-9 * abs(ref_loss)
+    grad = tr.backward(seed).cpu().numpy()
+    gmax = np.abs(grad).max()
+    assert gmax > 0
+    rng = np.random.default_rng(11)
+    h = 1e-5
+    for _ in range(4):
+        l, y, x = rng.integers(0, 2), rng.integers(4, 12), rng.integers(16, 48)
+        for a in (1, 2):
+            ap = ang.copy()
+            am = ang.copy()
+            ap[l, y, x, a] += h
+            am[l, y, x, a] -= h
+            fdv = (_oracle_analyser_loss(fd, d, eps, ap, roi, beta=beta, phi=phi, nstep=2) -
+                   _oracle_analyser_loss(fd, d, eps, am, roi, beta=beta, phi=phi, nstep=2)) / (2 * h)
+            g = grad[a - 1, l, y, x]
+            flush("  voxel (%d,%d,%d) angle %d: adjoint %.9e fd %.9e" % (l, y, x, a, g, fdv))
+            assert abs(g - fdv) < 1e-6 * gmax
 
 
 @pytest.mark.gpu
