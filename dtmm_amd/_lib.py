@@ -83,6 +83,7 @@ class Stack(ctypes.Structure):
         ("n_in", ctypes.c_double),
         ("n_out", ctypes.c_double),
         ("n_steps", ctypes.POINTER(ctypes.c_int)),
+        ("biaxial", ctypes.POINTER(ctypes.c_int)),
     ]
 
 

@@ -135,6 +135,217 @@ __host__ __device__ inline void uniaxial_modes(double beta, cd eps11, cd eps33, 
   }
 }
 
+// Givens rotation G = [c s; -conj(s) c] (c real) with G [a; b] = [r; 0].
+__host__ __device__ inline void givens(cd a, cd b, double& c, cd& s) {
+  const double aa = cabs(a), ab = cabs(b);
+  if (ab == 0.) {
+    c = 1.;
+    s = mk<double>(0.);
+    return;
+  }
+  if (aa == 0.) {
+    c = 0.;
+    s = mk<double>(1.);
+    return;
+  }
+  const double nrm = hypot(aa, ab);
+  c = aa / nrm;
+  s = (a * (1. / aa)) * conj(b) * (1. / nrm);
+}
+
+// Eigenvalues and eigenvectors of a general complex 4x4 matrix A (destroyed):
+// Householder reduction to Hessenberg form, single-shift complex QR with
+// Wilkinson shifts to the Schur form T = Z^H A Z, eigenvectors of T by
+// back-substitution, mapped by Z and scaled to unit 2-norm with the largest
+// component real and positive (the normalisation of LAPACK zgeev, which
+// numpy.linalg.eig calls).  lam[k] is the k-th diagonal entry of T and V[:,k]
+// its eigenvector.  Returns false if the iteration does not converge.
+__host__ __device__ inline bool eig4(cd A[4][4], cd lam[4], cd V[4][4]) {
+  cd Z[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) Z[i][j] = mk<double>(i == j ? 1. : 0.);
+  for (int k = 0; k < 2; ++k) {
+    cd v[3];
+    double xn = 0.;
+    for (int i = k + 1; i < 4; ++i) xn += norm2(A[i][k]);
+    xn = sqrt(xn);
+    const double a0 = cabs(A[k + 1][k]);
+    if (xn == 0. || xn == a0) continue;  // column already reduced
+    const cd ph = a0 > 0. ? A[k + 1][k] * (1. / a0) : mk<double>(1.);
+    const int m = 3 - k;
+    for (int i = 0; i < m; ++i) v[i] = A[k + 1 + i][k];
+    v[0] = v[0] + ph * xn;
+    double vn = 0.;
+    for (int i = 0; i < m; ++i) vn += norm2(v[i]);
+    const double f = 2. / vn;
+    for (int j = 0; j < 4; ++j) {  // A <- H A
+      cd t = mk<double>(0.);
+      for (int i = 0; i < m; ++i) t = t + conj(v[i]) * A[k + 1 + i][j];
+      t = t * f;
+      for (int i = 0; i < m; ++i) A[k + 1 + i][j] = A[k + 1 + i][j] - v[i] * t;
+    }
+    for (int i = 0; i < 4; ++i) {  // A <- A H, Z <- Z H
+      cd t = mk<double>(0.), u = mk<double>(0.);
+      for (int j = 0; j < m; ++j) {
+        t = t + A[i][k + 1 + j] * v[j];
+        u = u + Z[i][k + 1 + j] * v[j];
+      }
+      t = t * f;
+      u = u * f;
+      for (int j = 0; j < m; ++j) {
+        A[i][k + 1 + j] = A[i][k + 1 + j] - t * conj(v[j]);
+        Z[i][k + 1 + j] = Z[i][k + 1 + j] - u * conj(v[j]);
+      }
+    }
+    for (int i = k + 2; i < 4; ++i) A[i][k] = mk<double>(0.);
+  }
+  double hn = 0.;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) hn += cabs(A[i][j]);
+  if (hn == 0.) hn = 1.;
+  const double eps = 2.220446049250313e-16;
+  int hi = 3, it = 0, total = 0;
+  while (hi > 0) {
+    int l = hi;
+    for (; l > 0; --l) {
+      double sc = cabs(A[l - 1][l - 1]) + cabs(A[l][l]);
+      if (sc == 0.) sc = hn;
+      if (cabs(A[l][l - 1]) <= eps * sc) {
+        A[l][l - 1] = mk<double>(0.);
+        break;
+      }
+    }
+    if (l == hi) {
+      --hi;
+      it = 0;
+      continue;
+    }
+    if (++total > 200) return false;
+    cd mu;
+    if (it == 10 || it == 20) {
+      mu = A[hi][hi] + mk<double>(0.75 * cabs(A[hi][hi - 1]));  // exceptional shift
+    } else {
+      const cd a = A[hi - 1][hi - 1], b = A[hi - 1][hi], c = A[hi][hi - 1], d = A[hi][hi];
+      const cd hd = (a - d) * 0.5;
+      const cd disc = csqrt(hd * hd + b * c);
+      const cd m1 = (a + d) * 0.5 + disc, m2 = (a + d) * 0.5 - disc;
+      mu = cabs(m1 - d) <= cabs(m2 - d) ? m1 : m2;
+    }
+    ++it;
+    double gc[3];
+    cd gs[3];
+    for (int k = l; k <= hi; ++k) A[k][k] = A[k][k] - mu;
+    for (int k = l; k < hi; ++k) {
+      givens(A[k][k], A[k + 1][k], gc[k], gs[k]);
+      for (int j = k; j < 4; ++j) {
+        const cd x = A[k][j], y = A[k + 1][j];
+        A[k][j] = x * gc[k] + gs[k] * y;
+        A[k + 1][j] = y * gc[k] - conj(gs[k]) * x;
+      }
+    }
+    for (int k = l; k < hi; ++k) {
+      const int r = k + 2 < hi ? k + 2 : hi;
+      for (int i = 0; i <= r; ++i) {
+        const cd x = A[i][k], y = A[i][k + 1];
+        A[i][k] = x * gc[k] + y * conj(gs[k]);
+        A[i][k + 1] = y * gc[k] - x * gs[k];
+      }
+      for (int i = 0; i < 4; ++i) {
+        const cd x = Z[i][k], y = Z[i][k + 1];
+        Z[i][k] = x * gc[k] + y * conj(gs[k]);
+        Z[i][k + 1] = y * gc[k] - x * gs[k];
+      }
+    }
+    for (int k = l; k <= hi; ++k) A[k][k] = A[k][k] + mu;
+  }
+  const double smin = eps * hn > 1e-300 ? eps * hn : 1e-300;
+  for (int k = 0; k < 4; ++k) {
+    lam[k] = A[k][k];
+    cd x[4];
+    for (int i = 0; i < 4; ++i) x[i] = mk<double>(i == k ? 1. : 0.);
+    for (int i = k - 1; i >= 0; --i) {
+      cd t = mk<double>(0.);
+      for (int j = i + 1; j <= k; ++j) t = t + A[i][j] * x[j];
+      cd den = A[i][i] - A[k][k];
+      if (cabs(den) < smin) den = mk<double>(smin);
+      x[i] = -cdiv(t, den);
+    }
+    cd v[4];
+    double vn = 0.;
+    int im = 0;
+    double vm = -1.;
+    for (int i = 0; i < 4; ++i) {
+      cd t = mk<double>(0.);
+      for (int j = 0; j <= k; ++j) t = t + Z[i][j] * x[j];
+      v[i] = t;
+      vn += norm2(t);
+      if (cabs(t) > vm) {
+        vm = cabs(t);
+        im = i;
+      }
+    }
+    const cd ph = conj(v[im]) * (1. / (vm * sqrt(vn)));
+    for (int i = 0; i < 4; ++i) V[i][k] = v[i] * ph;
+  }
+  return true;
+}
+
+// Biaxial medium (dtmm/tmm.py:395-403): the principal tensor diag(e0, e1, e2)
+// rotated by R = R_phi R_theta R_psi (dtmm/rotation.py:125-150, :285-292),
+// the Berreman matrix of the rotated tensor (dtmm/tmm.py:99-121), its
+// eigen-decomposition (eig4) and the Poynting sort of dtmm/tmm.py:299-369:
+// in eigenvalue order, modes with Re(Ex Hy*) - Re(Ey Hx*) >= 0 fill columns
+// 0, 2 and the others columns 1, 3; if a direction has more than two modes
+// (or the iteration fails) alpha is NaN and F zero.  Columns are not rotated
+// by the beam azimuth here.  Returns false on failure.
+__host__ __device__ inline bool biaxial_modes(double beta, cd e0, cd e1, cd e2, double psi,
+                                              double theta, double phi, Modes& m) {
+  double sp, cp, st, ct, sf, cf;
+  sincos(psi, &sp, &cp);
+  sincos(theta, &st, &ct);
+  sincos(phi, &sf, &cf);
+  const double R[3][3] = {{ct * cf * cp - sf * sp, -ct * cf * sp - sf * cp, cf * st},
+                          {ct * sf * cp + cf * sp, cf * cp - ct * sf * sp, st * sf},
+                          {-cp * st, st * sp, ct}};
+  const cd d[3] = {e0, e1, e2};
+  cd ep[6];
+  const int pr[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+  for (int q = 0; q < 6; ++q) {
+    cd t = mk<double>(0.);
+    for (int k = 0; k < 3; ++k) t = t + d[k] * (R[pr[q][0]][k] * R[pr[q][1]][k]);
+    ep[q] = t;
+  }
+  const cd e2m = cdiv(mk<double>(1.), ep[2]);
+  const cd e4 = ep[4] * e2m, e5 = ep[5] * e2m;
+  const cd z = mk<double>(0.);
+  const cd l00 = e4 * (-beta), l01 = mk<double>(1.) - e2m * (beta * beta), l02 = e5 * (-beta);
+  const cd l10 = ep[0] - ep[4] * e4, l12 = ep[3] - ep[5] * e4;
+  const cd l32 = mk<double>(beta * beta) + ep[5] * e5 - ep[1];
+  cd L[4][4] = {{l00, l01, l02, z}, {l10, l00, l12, z}, {z, z, z, mk<double>(-1.)}, {-l12, -l02, l32, z}};
+  cd lam[4], V[4][4];
+  bool ok = eig4(L, lam, V);
+  int i = 0, j = 1;
+  for (int k = 0; ok && k < 4; ++k) {
+    const double p = (V[0][k] * conj(V[1][k])).x - (V[2][k] * conj(V[3][k])).x;
+    int& slot = p >= 0. ? i : j;
+    if (slot >= 4) {
+      ok = false;
+      break;
+    }
+    m.a[slot] = lam[k];
+    for (int r = 0; r < 4; ++r) m.F[r][slot] = V[r][k];
+    slot += 2;
+  }
+  if (!ok) {
+    const double nan = __builtin_nan("");
+    for (int c = 0; c < 4; ++c) {
+      m.a[c] = mk<double>(nan, nan);
+      for (int r = 0; r < 4; ++r) m.F[r][c] = mk<double>(0.);
+    }
+  }
+  return ok;
+}
+
 // Rotate the (Ex, Hy, Ey, Hx) rows by the beam azimuth (c, s) = (cos, sin).
 __host__ __device__ inline void rotate_rows(Modes& m, double c, double s) {
   for (int j = 0; j < 4; ++j) {
@@ -229,19 +440,15 @@ struct RopData {
 };
 
 __host__ __device__ inline void layer_modes(double beta, double c0, double s0, double phi0, cd e0,
-                                            cd e1, cd e2, double theta, double phi, Modes& m,
-                                            int& kind) {
+                                            cd e1, cd e2, double psi, double theta, double phi,
+                                            Modes& m, int& kind) {
   kind = material_kind(e0, e1, e2);
   if (kind == KIND_ISO) {
     iso_modes(beta, e0, m);
   } else if (kind == KIND_UNIAXIAL) {
     uniaxial_modes(beta, e0, e2, theta, phi - phi0, m);
   } else {
-    const double nan = __builtin_nan("");
-    for (int i = 0; i < 4; ++i) {
-      m.a[i] = mk<double>(nan, nan);
-      for (int j = 0; j < 4; ++j) m.F[i][j] = mk<double>(nan, nan);
-    }
+    biaxial_modes(beta, e0, e1, e2, psi, theta, phi - phi0, m);
   }
   rotate_rows(m, c0, s0);
 }
@@ -358,6 +565,22 @@ __host__ __device__ inline void rop_lean(double beta, double c0, double s0, doub
       r.P0[i][j] = ep[i][0] * ip[0][j];
       r.P1[i][j] = en[i][0] * in[0][j];
     }
+}
+
+// Per-pixel operator data of any material: the lean closed forms for
+// isotropic and uniaxial voxels, the numeric eigen-decomposition of
+// biaxial_modes (rotated by the beam azimuth) for biaxial ones.
+__host__ __device__ inline void rop_full(double beta, double c0, double s0, double phi0, cd e0,
+                                         cd e1, cd e2, double psi, double theta, double phi,
+                                         RopData& r) {
+  if (material_kind(e0, e1, e2) != KIND_BIAXIAL) {
+    rop_lean(beta, c0, s0, phi0, e0, e1, e2, theta, phi, r);
+    return;
+  }
+  Modes m;
+  biaxial_modes(beta, e0, e1, e2, psi, theta, phi - phi0, m);
+  rotate_rows(m, c0, s0);
+  rop_data(m, KIND_BIAXIAL, r);
 }
 
 }  // namespace dtmmx

@@ -36,6 +36,8 @@ __host__ __device__ __forceinline__ cpx<T> conj(cpx<T> a) { return mk<T>(a.x, -a
 template <typename T>
 __host__ __device__ __forceinline__ T norm2(cpx<T> a) { return a.x * a.x + a.y * a.y; }
 template <typename T>
+__host__ __device__ __forceinline__ T cabs(cpx<T> a) { return hypot(a.x, a.y); }
+template <typename T>
 __host__ __device__ __forceinline__ bool is_zero(cpx<T> a) { return a.x == T(0) && a.y == T(0); }
 template <typename T>
 __host__ __device__ __forceinline__ bool eq(cpx<T> a, cpx<T> b) { return a.x == b.x && a.y == b.y; }

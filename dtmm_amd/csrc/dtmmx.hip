@@ -252,6 +252,26 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   a.rays_per_group = nr;
   const int nwork = a.H * (b.nrays / nr);
   const int w = a.W;
+  if ((op == OP_ROP || op == OP_ROPW) && a.etab_ws) {
+    // layer with biaxial voxels: per-pixel data for this pass's beams first
+    const size_t HW = size_t(a.H) * a.W;
+    EigArgs<T> e{};
+    e.eps_re = a.eps_re;
+    e.eps_im = a.eps_im;
+    e.ang = a.ang;
+    e.comp_stride = a.comp_stride;
+    e.HW = HW;
+    e.nbeam = b.nrays / a.bm.rays_per_beam;
+    e.table = a.etab_ws;
+    e.status = a.status;
+    e.method4 = a.method4;
+    e.bm = a.bm;
+    hipLaunchKernelGGL((rop_table_kernel<T>), dim3(unsigned((HW + NT - 1) / NT), e.nbeam), dim3(NT), 0,
+                       s, e);
+    HIP_TRY(hipGetLastError());
+    a.etab = a.etab_ws;
+    a.etab_nbeam = e.nbeam;
+  }
   if (op == OP_ROP) {
     timer_start(0, s);
     int r = nc == 2 ? launch_row<T, 2, OP_ROP>(w, nr, a, nwork, s)
@@ -371,8 +391,9 @@ static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, 
 // Row-kernel arguments of the layer operator of layer l, sub-step thickness d.
 template <typename T>
 static RowArgs<T> layer_args(const dtmmx_stack* S, size_t HW, int l, double d, int method4,
-                             int* status) {
+                             int* status, void* etab_ws = nullptr) {
   RowArgs<T> a{};
+  if (S->biaxial && S->biaxial[l]) a.etab_ws = (cpx<T>*)etab_ws;
   a.eps_re = S->eps_re + size_t(l) * HW;
   a.eps_im = S->eps_im ? S->eps_im + size_t(l) * HW : nullptr;
   a.ang = S->angles + size_t(l) * HW;
@@ -413,6 +434,7 @@ struct WsLayout {
                                    // first is the window part of sub-stepped layers), sum
   size_t e2;                       // 2x2: the 2-component field
   size_t part, sums;               // npass > 1: reduction partials and sums
+  size_t etab;                     // per-pixel data of a layer with biaxial voxels
   size_t total;
   int nblk;
 };
@@ -441,6 +463,7 @@ static WsLayout ws_layout(const dtmmx_plan* P) {
     w.win_acc = take(f4);
   }
   if (!m4) w.e2 = take(planes * 2 * HW * cb);
+  w.etab = take(size_t(EIG_COMPS) * (P->d.n_rays / P->d.rays_per_beam) * HW * cb);
   return w;
 }
 
@@ -472,7 +495,8 @@ static int lam_batch(const dtmmx_plan* P, size_t bytes_per_lam) {
 // sub-steps with the window's mean beam, and summed into `acc`.
 template <typename T>
 static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, cpx<T>* tmp,
-                            cpx<T>* acc, Batch b, int* status, hipStream_t s, int dir) {
+                            cpx<T>* acc, Batch b, int* status, hipStream_t s, int dir,
+                            void* etab) {
   const int H = P->d.height, W = P->d.width;
   const size_t HW = size_t(H) * W;
   const int L = S->n_layers;
@@ -535,7 +559,7 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
           bw.beta[q] = P->win_beta[size_t(w) * nbeam + q];
           bw.phi[q] = P->win_phi[size_t(w) * nbeam + q];
         }
-        RowArgs<T> la = layer_args<T>(S, HW, l, ds, 1, status);
+        RowArgs<T> la = layer_args<T>(S, HW, l, ds, 1, status, etab);
         la.acc = acc;
         la.acc_mode = (w == 0) ? 1 : 2;
         la.wcx = P->wcx;
@@ -570,7 +594,7 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
       ca.wj = P->win_j[w];
       TRY(col_pass<T>(P, A, b, 4, OP_WIN, s, &ca, &bw));
       for (int st = 0; st < ns; ++st) {
-        RowArgs<T> la = layer_args<T>(S, HW, l, ds, 1, status);
+        RowArgs<T> la = layer_args<T>(S, HW, l, ds, 1, status, etab);
         if (st + 1 == ns) {
           la.acc = acc;
           la.acc_mode = (w == 0) ? 1 : 2;
@@ -634,7 +658,7 @@ static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, voi
     cpx<T>* tmp = (cpx<T>*)(wsb + wl.win_tmp);
     cpx<T>* acc = (cpx<T>*)(wsb + wl.win_acc);
     Batch b{nrays, nlam, 0, nlam};
-    return transfer_windows<T>(P, S, out, tmp, acc, b, status, s, dir);
+    return transfer_windows<T>(P, S, out, tmp, acc, b, status, s, dir, wsb + wl.etab);
   }
   const std::vector<Step> st = expand_steps(S, dir);
   const int L = int(st.size());
@@ -655,7 +679,7 @@ static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, voi
       TRY(kspace<T>(P, out, out, b, k, s));
       TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
       for (int l = 0; l < L; ++l) {
-        const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status);
+        const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status, wsb + wl.etab);
         TRY(row_pass<T>(P, out, b, 4, OP_ROP, s, &la));
         if (l + 1 < L) {
           cd e0 = st[l].e, e1 = st[l + 1].e;
@@ -709,7 +733,7 @@ static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, voi
     TRY(col_pass<T>(P, e2, b, 2, OP_INV, s));
     const cd eout = mk<double>(S->n_out * S->n_out);
     for (int l = 0; l < L; ++l) {
-      const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 0, status);
+      const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 0, status, wsb + wl.etab);
       TRY(row_pass<T>(P, e2, b, 2, OP_ROP, s, &la));
       const cd enext = (l + 1 < L) ? st[l + 1].e : eout;
       if (P->d.reflection && !eq(st[l].e, enext)) {
@@ -1501,7 +1525,7 @@ size_t dtmmx_plan_workspace_size(const dtmmx_plan* P) {
 }
 
 static int check_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin,
-                          const void* fout, const void* ws, size_t ws_bytes) {
+                          const void* fout, const void* ws, size_t ws_bytes, bool need_ws = true) {
   if (!P || !S || !fin || !fout) return fail(DTMMX_E_ARG, "null argument");
   if (S->n_layers < 0) return fail(DTMMX_E_ARG, "negative layer count");
   if (S->n_layers > 0 && (!S->thickness || !S->eps_eff_re || !S->eps_re || !S->angles))
@@ -1509,7 +1533,8 @@ static int check_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void*
   if (S->n_steps)
     for (int l = 0; l < S->n_layers; ++l)
       if (S->n_steps[l] < 1) return fail(DTMMX_E_ARG, "n_steps must be >= 1");
-  if (ws_bytes < dtmmx_plan_workspace_size(P) || (dtmmx_plan_workspace_size(P) && !ws))
+  if (need_ws &&
+      (ws_bytes < dtmmx_plan_workspace_size(P) || (dtmmx_plan_workspace_size(P) && !ws)))
     return fail(DTMMX_E_ARG, "workspace too small");
   if (fin == fout) return fail(DTMMX_E_ARG, "field_in and field_out must not alias");
   return DTMMX_OK;
@@ -1538,7 +1563,7 @@ size_t dtmmx_adjoint_workspace_size(const dtmmx_plan* P) {
 int dtmmx_transfer_record(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
                           void* tape, size_t tape_bytes, int* status, void* stream) {
   {
-    const int r = check_transfer(P, S, fin, fout, nullptr, 0);
+    const int r = check_transfer(P, S, fin, fout, nullptr, 0, false);
     if (r != DTMMX_OK) return r;
   }
   if (!adjoint_supported(P))
@@ -1686,8 +1711,7 @@ int dtmmx_host_alphaf(int n, const double* beta, const double* phi, const double
     Modes m;
     int kind;
     layer_modes(beta[i], std::cos(phi[i]), std::sin(phi[i]), phi[i], e[0], e[1], e[2],
-                angles[3 * i + 1], angles[3 * i + 2], m, kind);
-    if (kind == KIND_BIAXIAL) rc = fail(DTMMX_E_UNSUPPORTED, "biaxial voxel");
+                angles[3 * i], angles[3 * i + 1], angles[3 * i + 2], m, kind);
     for (int j = 0; j < 4; ++j) {
       alpha[8 * i + 2 * j] = m.a[j].x;
       alpha[8 * i + 2 * j + 1] = m.a[j].y;
@@ -1744,9 +1768,8 @@ int dtmmx_host_rop(int n, const double* beta, const double* phi, const double* e
     cd e[3];
     for (int c = 0; c < 3; ++c) e[c] = mk<double>(eps_re[3 * i + c], eps_im ? eps_im[3 * i + c] : 0.);
     RopData r;
-    rop_lean(beta[i], std::cos(phi[i]), std::sin(phi[i]), phi[i], e[0], e[1], e[2],
+    rop_full(beta[i], std::cos(phi[i]), std::sin(phi[i]), phi[i], e[0], e[1], e[2], angles[3 * i],
              angles[3 * i + 1], angles[3 * i + 2], r);
-    if (r.kind == KIND_BIAXIAL) rc = fail(DTMMX_E_UNSUPPORTED, "biaxial voxel");
     for (int j = 0; j < 4; ++j) {
       alpha[8 * i + 2 * j] = r.a[j].x;
       alpha[8 * i + 2 * j + 1] = r.a[j].y;

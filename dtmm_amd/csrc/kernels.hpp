@@ -99,6 +99,13 @@ struct RowArgs {
   // OP_ROP: when set, the input lines are also stored here (field layout),
   // the adjoint's record of the row-pass input
   cpx<T>* save;
+  // OP_ROP / OP_ROPW: per-pixel operator data of this layer precomputed by
+  // rop_table_kernel ([EIG_COMPS][etab_nbeam][H*W], layers with biaxial
+  // voxels); NULL: solved in the kernel.  etab_ws (host side): table space
+  // to fill before the launch.
+  const cpx<T>* etab;
+  int etab_nbeam;
+  cpx<T>* etab_ws;
   Beams bm;
 };
 
@@ -174,6 +181,20 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   // every wavelength and ray of the work-group
   cpx<T> eg[ROPX ? PPX : 1][EIG_COMPS];
   if constexpr (ROPX) {
+   if (A.etab) {
+    const int beam = (rg * NR) / A.bm.rays_per_beam;
+    const size_t cs = size_t(A.etab_nbeam) * HW;
+#pragma unroll
+    for (int k = 0; k < PPX; ++k) {
+      const int x = tid + k * NTW;
+      if (x < N) {
+        const cpx<T>* t = A.etab + size_t(beam) * HW + size_t(y) * N + x;
+        for (int m = 0; m < EIG_COMPS; ++m) eg[k][m] = t[m * cs];
+      } else {
+        for (int m = 0; m < EIG_COMPS; ++m) eg[k][m] = mk<T>(T(0), T(0));
+      }
+    }
+   } else {
     const int beam = (rg * NR) / A.bm.rays_per_beam;
     const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
     double s0, c0;
@@ -204,6 +225,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
         for (int m = 0; m < EIG_COMPS; ++m) eg[k][m] = mk<T>(T(0), T(0));
       }
     }
+   }
   }
   __syncthreads();
 
@@ -365,8 +387,38 @@ struct EigArgs {
   int nbeam;
   cpx<T>* table;
   int* status;
+  int method4;  // rop_table_kernel: 1 = 4x4 (E invertible), 0 = 2x2 (E+ invertible)
   Beams bm;
 };
+
+// Per-pixel operator data of one layer for every beam, in the row kernel's
+// order (a[4], P0, P1, invertibility factor): table [EIG_COMPS][nbeam][H*W].
+// For layers with biaxial voxels, whose numeric eigensolve (rop_full) runs
+// here once per layer instead of inside the row kernel.
+template <typename T>
+__global__ __launch_bounds__(NT) void rop_table_kernel(EigArgs<T> A) {
+  const size_t o = size_t(blockIdx.x) * NT + threadIdx.x;
+  const int beam = blockIdx.y;
+  if (o >= A.HW) return;
+  const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
+  double s0, c0;
+  sincos(phi0, &s0, &c0);
+  const size_t cs = A.comp_stride;
+  const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+  const cd e1 = mk<double>(A.eps_re[cs + o], A.eps_im ? A.eps_im[cs + o] : 0.);
+  const cd e2 = mk<double>(A.eps_re[2 * cs + o], A.eps_im ? A.eps_im[2 * cs + o] : 0.);
+  RopData r;
+  rop_full(beta, c0, s0, phi0, e0, e1, e2, A.ang[o], A.ang[cs + o], A.ang[2 * cs + o], r);
+  cpx<T>* t = A.table + size_t(beam) * A.HW + o;
+  const size_t ts = size_t(A.nbeam) * A.HW;
+  for (int i = 0; i < 4; ++i) t[i * ts] = cast<T>(r.a[i]);
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      t[(4 + 2 * i + j) * ts] = cast<T>(r.P0[i][j]);
+      t[(8 + 2 * i + j) * ts] = cast<T>(r.P1[i][j]);
+    }
+  t[12 * ts] = mk<T>((A.method4 ? r.ok4 : r.ok2) ? T(1) : T(0), T(0));
+}
 
 template <typename T>
 __global__ __launch_bounds__(NT) void eig_grad_kernel(EigArgs<T> A) {

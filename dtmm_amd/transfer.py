@@ -102,7 +102,7 @@ class Stack:
         torch = _torch()
         blocks = validate_optical_data(optical_data, shape=shape)
         H, W = shape
-        thick, steps, eff, eps_l = [], [], [], []
+        thick, steps, eff, eps_l, biax = [], [], [], [], []
         for d, material, angles in blocks:
             n = len(d)
             substeps = np.broadcast_to(np.asarray(nstep), (n,))
@@ -121,9 +121,12 @@ class Stack:
                 steps.append(ns)
                 eff.append(complex(effs[i]))
                 eps_l.append((len(eps_l), mat[i], ang[i]))
+                # numeric eigensolve where eps0 != eps1 (dtmm/tmm.py:384-403)
+                biax.append(int(np.any(mat[i][..., 0] != mat[i][..., 1])))
         self.n_layers = len(thick)
         self.thickness = np.asarray(thick, np.float64)
         self.n_steps = np.asarray(steps, np.int32)
+        self.biaxial = np.asarray(biax, np.int32)
         self.eps_eff_re = np.asarray([e.real for e in eff], np.float64)
         self.eps_eff_im = np.asarray([e.imag for e in eff], np.float64)
         self.nin = float(nin)
@@ -156,6 +159,7 @@ class Stack:
         s.n_in = self.nin
         s.n_out = self.nout
         s.n_steps = _lib.iptr(self.n_steps)
+        s.biaxial = _lib.iptr(self.biaxial) if self.biaxial.any() else None
         return s
 
 
@@ -361,7 +365,8 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
 
 def _check_status(status):
     if int(status[0].item()) & 1:
-        raise NotImplementedError("biaxial voxels (eps0 != eps1) are not supported yet")
+        raise NotImplementedError("biaxial voxels (eps0 != eps1) are not supported by the "
+                                  "adjoint transfer")
 
 
 def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,

@@ -102,6 +102,11 @@ typedef struct {
   const double* angles;      /* device [3][n_layers][H][W]               */
   double n_in, n_out;        /* input/output media (transfer.py:936-939) */
   const int* n_steps;        /* [n_layers] host, >= 1; NULL: all 1       */
+  const int* biaxial;        /* [n_layers] host: nonzero where the layer
+                                has voxels with eps0 != eps1 (numeric
+                                eigensolve, dtmm/tmm.py:395-403); NULL:
+                                none.  A biaxial voxel in an unflagged
+                                layer sets status bit 0 and yields NaN. */
 } dtmmx_stack;
 
 typedef struct dtmmx_plan dtmmx_plan;
@@ -115,7 +120,10 @@ const char* dtmmx_last_error(void);
 int dtmmx_plan_create(dtmmx_plan** plan, const dtmmx_plan_desc* desc);
 int dtmmx_plan_destroy(dtmmx_plan* plan);
 
-/* Device scratch bytes one transfer call on this plan needs. */
+/* Device scratch bytes one transfer call on this plan needs: the npass,
+ * window and 2x2 fields where the plan uses them, and the per-pixel
+ * operator table of one layer (13 complex values per pixel and beam) that
+ * layers flagged biaxial are solved into. */
 size_t dtmmx_plan_workspace_size(const dtmmx_plan* plan);
 
 /* Full transfer through the stack.
@@ -206,10 +214,12 @@ int dtmmx_host_windows(const dtmmx_plan_desc* desc, int max_windows, int* n_wind
                        int* wj, double* beta, double* phi);
 
 /* Host evaluation of the per-pixel eigen-solution used on the device
- * (dtmm/tmm.py:435 alphaf for isotropic and uniaxial voxels), for checks
- * without a GPU.  Inputs are n voxels; outputs alpha[n][4] and
- * fmat[n][4][4] as interleaved complex128.  Returns DTMMX_E_UNSUPPORTED
- * if any voxel is biaxial. */
+ * (dtmm/tmm.py:435 alphaf), for checks without a GPU.  Inputs are n
+ * voxels (angles = psi, theta, phi); outputs alpha[n][4] and fmat[n][4][4]
+ * as interleaved complex128.  Biaxial voxels: numeric eigenvectors of unit
+ * 2-norm with the largest component real (numpy.linalg.eig's scaling);
+ * within a direction the two modes follow the eigenvalue order of the
+ * solver, which the layer operator does not depend on. */
 int dtmmx_host_alphaf(int n, const double* beta, const double* phi, const double* eps_re,
                       const double* eps_im, const double* angles, double* alpha, double* fmat);
 
@@ -218,18 +228,20 @@ int dtmmx_host_alphaf(int n, const double* beta, const double* phi, const double
  * dtmm/propagate_2x2.py:33-39) written as p2 I + (p0 - p2) P0 on
  * (E+x, E+y) and p3 I + (p1 - p3) P1 on (E-x, E-y): alpha[n][4] and
  * proj[n][2][2][2] (P0, P1) as interleaved complex128, ok[n][2] = E
- * invertible for the 4x4 / 2x2 method.  The same routine runs inside the
- * row kernel; this entry point exists for checks without a GPU. */
-/* dtmmx_host_rop with the theta and phi derivatives (no counterpart in the
- * reference): dalpha[n][2][4] and dproj[n][2][2][2][2] ([direction][P0|P1])
- * as interleaved complex128; the routine the adjoint row kernel runs. */
-int dtmmx_host_rop_grad(int n, const double* beta, const double* phi, const double* eps_re,
-                        const double* eps_im, const double* angles, double* alpha, double* proj,
-                        double* dalpha, double* dproj);
-
+ * invertible for the 4x4 / 2x2 method.  The same routines run in the row
+ * kernel (isotropic, uniaxial) and in the biaxial layer-table kernel; this
+ * entry point exists for checks without a GPU. */
 int dtmmx_host_rop(int n, const double* beta, const double* phi, const double* eps_re,
                    const double* eps_im, const double* angles, double* alpha, double* proj,
                    int* ok);
+
+/* dtmmx_host_rop with the theta and phi derivatives (no counterpart in the
+ * reference): dalpha[n][2][4] and dproj[n][2][2][2][2] ([direction][P0|P1])
+ * as interleaved complex128; the routine the adjoint kernels run.
+ * Isotropic and uniaxial voxels only (DTMMX_E_UNSUPPORTED for biaxial). */
+int dtmmx_host_rop_grad(int n, const double* beta, const double* phi, const double* eps_re,
+                        const double* eps_im, const double* angles, double* alpha, double* proj,
+                        double* dalpha, double* dproj);
 
 #ifdef __cplusplus
 }

@@ -391,3 +391,60 @@ def test_gradient_graph_replays_match_eager():
         # eager work between replays
         out2 = dm.transfer_field(fd, optical, beta=0., phi=0., method="4x4")[0]
         assert np.isfinite(out2).all()
+
+
+def _biaxial_case(shape=(6, 24, 40), seed=5, wavelengths=(450., 650.), jones=None, beta=0.,
+                  phi=0., uniaxial_every_other=True):
+    """Random texture with biaxial principal values (1.5^2, 1.6^2, 1.75^2)
+    and a random third Euler angle psi; every other layer uniaxial, so
+    layers with and without the numeric eigensolve alternate."""
+    fd, optical = random_case(shape=shape, seed=seed, wavelengths=wavelengths, jones=jones,
+                              beta=beta, phi=phi)
+    d, eps, ang = optical[0]
+    rng = np.random.default_rng(seed)
+    eps = np.array(np.broadcast_to(eps, ang.shape))
+    eps[..., 0], eps[..., 1], eps[..., 2] = 1.5 ** 2, 1.6 ** 2, 1.75 ** 2
+    if uniaxial_every_other:
+        eps[1::2, ..., 1] = eps[1::2, ..., 0]
+    ang = np.array(ang)
+    ang[..., 0] = rng.uniform(-np.pi, np.pi, shape)
+    return fd, [(d, eps, ang)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+def test_biaxial_layers(method, precision):
+    """Biaxial voxels: per-layer table of the numeric eigensolve (Hessenberg
+    + shifted QR, Poynting sort) against the oracle's numpy.linalg.eig."""
+    flush("case biaxial %s %s" % (method, precision))
+    fd, optical = _biaxial_case()
+    _compare(fd, optical, method, precision)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+def test_biaxial_multi_beam_nstep(method):
+    flush("case biaxial multi beam nstep %s" % method)
+    beta, phi = [0., 0.2], [0., 0.8]
+    fd, optical = _biaxial_case(shape=(4, 32, 48), beta=beta, phi=phi)
+    _compare(fd, optical, method, "c128", beta=beta, phi=phi, nstep=2)
+
+
+@pytest.mark.gpu
+def test_biaxial_windows_npass():
+    """Biaxial layers under diffraction 3 (table per window beam) and npass 2."""
+    flush("case biaxial windows npass")
+    fd, optical = _biaxial_case(shape=(4, 32, 32), uniaxial_every_other=False)
+    _compare(fd, optical, "4x4", "c128", diffraction=3)
+    _compare_npass(fd, optical, "c128", beta=0., phi=0., npass=2, nin=1.0, nout=1.5)
+
+
+@pytest.mark.gpu
+def test_biaxial_adjoint_is_refused():
+    """The adjoint transfer has no biaxial derivatives: it raises."""
+    flush("case biaxial adjoint refused")
+    fd, optical = _biaxial_case(shape=(2, 16, 32), jones=(1, 0))
+    tr = dm.AdjointTransfer(fd, optical, beta=0., phi=0., precision="c128")
+    with pytest.raises(NotImplementedError):
+        tr.forward()
