@@ -167,6 +167,54 @@ __device__ __forceinline__ int lds_at(int x) {
   return x * STRIDE + (PADW ? (x >> DTMMX_LDS_PAD_SHIFT) * PADW : 0);
 }
 
+// Twiddle tables in LDS, one per radix plan (forward order and, unless the
+// plan reads the same backwards, reversed order).  Stage s >= 1 of a plan
+// with radix R, PP = product of the earlier radices and STEP = N / (PP R)
+// reads w^(r k STEP) for r = 1..R-1 and k = i % PP; it is stored at
+// r PP + k - R0 (R0 = first radix: the stage blocks (R - 1) PP telescope to
+// PP - R0).  A lane reads k + a compile-time offset, so the address is one
+// register plus immediate offsets, and lanes with consecutive k read
+// consecutive slots (no bank conflicts).
+template <int N>
+__host__ __device__ constexpr bool plan_palindrome() {
+  for (int s = 0; s < make_plan(N).ns; ++s)
+    if (radix_at<N, false>(s) != radix_at<N, true>(s)) return false;
+  return true;
+}
+template <int N, bool REV>
+__host__ __device__ constexpr int tw_table_off() {
+  return (REV && !plan_palindrome<N>()) ? N - radix_at<N, false>(0) : 0;
+}
+template <int N>
+__host__ __device__ constexpr int tw_table_slots() {
+  const int f = N - radix_at<N, false>(0);
+  const int n = plan_palindrome<N>() ? f : f + N - radix_at<N, true>(0);
+  return n > 0 ? n : 1;
+}
+
+template <typename T, int N, bool REV, int S>
+__device__ __forceinline__ void fill_tw_stage(cpx<T>* tws, const cpx<T>* tw, int tid, int nth) {
+  if constexpr (S < make_plan(N).ns) {
+    constexpr int R = radix_at<N, REV>(S), PP = pp_at<N, REV>(S);
+    constexpr int STEP = N / (PP * R);
+    constexpr int BASE = tw_table_off<N, REV>() + PP - radix_at<N, REV>(0);
+    for (int e = tid; e < (R - 1) * PP; e += nth) {
+      const int r = 1 + e / PP, k = e % PP;
+      tws[BASE + e] = tw[r * k * STEP];
+    }
+    fill_tw_stage<T, N, REV, S + 1>(tws, tw, tid, nth);
+  }
+}
+
+// Fills the twiddle tables of length-N plans from the plain table
+// tw[j] = exp(-2 pi i j / N) (nth threads, thread tid; the caller
+// synchronises before the first transform).
+template <typename T, int N>
+__device__ __forceinline__ void fill_tw_tables(cpx<T>* tws, const cpx<T>* tw, int tid, int nth) {
+  fill_tw_stage<T, N, false, 1>(tws, tw, tid, nth);
+  if constexpr (!plan_palindrome<N>()) fill_tw_stage<T, N, true, 1>(tws, tw, tid, nth);
+}
+
 // Barrier between LDS write and read phases: the whole work-group, or only
 // the wave when a line is owned by a single wave (LDS operations of one wave
 // execute in order, so a compiler fence at wavefront scope is enough).
@@ -199,7 +247,7 @@ __device__ __forceinline__ void reg_stages(cpx<T> (&v)[MAXV], cpx<T>* lds,
         const int k = i % PP;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          cpx<T> w = tw[r * k * STEP];
+          cpx<T> w = tw[tw_table_off<N, REV>() + r * PP + k - radix_at<N, REV>(0)];
           if (DIR > 0) w.y = -w.y;
           v[m * R + r] = v[m * R + r] * w;
         }

@@ -165,10 +165,10 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   constexpr int NOUT = slot_count<N, REVL, NL, true>();
   constexpr int PPX = (N + NTW - 1) / NTW;  // pixels per thread in the layer operator
   __shared__ cpx<T> lds[LINES * LS];
-  __shared__ cpx<T> tws[N];  // twiddles in LDS: their reads never wait on global loads
+  __shared__ cpx<T> tws[tw_table_slots<N>()];  // twiddles in LDS: their reads never wait on global loads
   const int tid = threadIdx.x;
   const int li = tid / NL, lane = tid % NL;
-  for (int i = tid; i < N; i += NTW) tws[i] = A.tw[i];
+  fill_tw_tables<T, N>(tws, A.tw, tid, NTW);
   const int rr = li / NC, comp = li - rr * NC;
   const int w = xcd_swizzle(blockIdx.x, gridDim.x);
   const int ngroups = A.b.nrays / NR;
@@ -177,6 +177,21 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   const size_t HW = size_t(A.H) * N;
   cpx<T>* line_lds = lds + li * LS;
 
+  // line of wavelength lam0 + jl for this wave
+  auto line_ptr = [&](int jl) {
+    const size_t q = size_t(rg * NR + rr) * A.b.nlam + A.b.lam0 + jl;
+    return A.field + (q * NC + comp) * HW + size_t(y) * N;
+  };
+  constexpr bool PF = true;
+  // prefetched inputs of the next wavelength; the first one is requested
+  // before the eigen data so that its latency overlaps the fp64 solve
+  cpx<T> nxt[PF ? NIN : 1];
+  if constexpr (PF) {
+    const cpx<T>* g0 = line_ptr(0);
+#pragma unroll
+    for (int j = 0; j < NIN; ++j)
+      if (slot_valid<N, REV1, NL, false>(j, lane)) nxt[j] = g0[slot_pos<N, REV1, NL, false>(j, lane)];
+  }
   // per-pixel operator data of this row, solved in fp64 once and reused for
   // every wavelength and ray of the work-group
   cpx<T> eg[ROPX ? PPX : 1][EIG_COMPS];
@@ -227,22 +242,27 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
     }
    }
   }
+  // Every pixel of this thread's wave has real mode pairs a1 = -a0,
+  // a3 = -a2 (lossless media at normal incidence, isotropic voxels): one
+  // sine and cosine per pair, and no exponential.  A wave vote keeps the
+  // choice uniform, so no lane runs both paths.
+  bool fast_ph = false;
+  if constexpr (ROPX) {
+    bool f = true;
+#pragma unroll
+    for (int k = 0; k < PPX; ++k) {
+      if (tid + k * NTW < N) {
+        const cpx<T>* e = eg[k];
+        f = f && e[0].y == T(0) && e[2].y == T(0);
+        if constexpr (NC == 4)
+          f = f && e[1].x == -e[0].x && e[1].y == T(0) && e[3].x == -e[2].x && e[3].y == T(0);
+      }
+    }
+    fast_ph = __all(f);
+  }
   __syncthreads();
 
-  // line of wavelength lam0 + jl for this wave
-  auto line_ptr = [&](int jl) {
-    const size_t q = size_t(rg * NR + rr) * A.b.nlam + A.b.lam0 + jl;
-    return A.field + (q * NC + comp) * HW + size_t(y) * N;
-  };
   const int acc_mode = ROPX ? A.acc_mode : 0;
-  constexpr bool PF = true;
-  cpx<T> nxt[PF ? NIN : 1];  // prefetched inputs of the next wavelength
-  if constexpr (PF) {
-    const cpx<T>* g0 = line_ptr(0);
-#pragma unroll
-    for (int j = 0; j < NIN; ++j)
-      if (slot_valid<N, REV1, NL, false>(j, lane)) nxt[j] = g0[slot_pos<N, REV1, NL, false>(j, lane)];
-  }
   for (int jl = 0; jl < A.b.nlam_b; ++jl) {
     const int lam = A.b.lam0 + jl;
     cpx<T>* gl = line_ptr(jl);
@@ -298,32 +318,44 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
           const cpx<T>* e = eg[k];
           const int ax = lds_at<1, 1>(x);
           const T ok = e[12].x;
-          cpx<T> p0 = cexpi_layer(e[0] * kd), p2 = cexpi_layer(e[2] * kd);
-          cpx<T> dp = (p0 - p2) * ok;
+          cpx<T> p0, p1, p2, p3;
+          if (fast_ph) {
+            p0 = cexpi_layer(mk<T>(e[0].x * kd, T(0)));
+            p2 = cexpi_layer(mk<T>(e[2].x * kd, T(0)));
+            p1 = conj(p0);
+            p3 = conj(p2);
+          } else {
+            p0 = cexpi_layer(e[0] * kd);
+            p2 = cexpi_layer(e[2] * kd);
+            if constexpr (NC == 4) {
+              p1 = cexpi_layer(e[1] * kd);
+              p3 = cexpi_layer(e[3] * kd);
+            }
+          }
+          // 2x2 blocks p I + (p' - p) P of the forward (and backward) modes
+          const cpx<T> dp = (p0 - p2) * ok;
           p2 = p2 * ok;
+          const cpx<T> f00 = p2 + dp * e[4], f01 = dp * e[5], f10 = dp * e[6], f11 = p2 + dp * e[7];
           if constexpr (NC == 4) {
-            cpx<T> p1 = cexpi_layer(e[1] * kd), p3 = cexpi_layer(e[3] * kd);
-            cpx<T> dn = (p1 - p3) * ok;
+            const cpx<T> dn = (p1 - p3) * ok;
             p3 = p3 * ok;
+            const cpx<T> b00 = p3 + dn * e[8], b01 = dn * e[9], b10 = dn * e[10], b11 = p3 + dn * e[11];
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
               cpx<T>* L = lds + (r * 4) * LS + ax;
-              cpx<T> u0 = L[0], u1 = L[LS], u2 = L[2 * LS], u3 = L[3 * LS];
-              cpx<T> w0 = e[4] * u0 + e[5] * u2, w2 = e[6] * u0 + e[7] * u2;
-              cpx<T> w1 = e[8] * u1 + e[9] * u3, w3 = e[10] * u1 + e[11] * u3;
-              L[0] = p2 * u0 + dp * w0;
-              L[2 * LS] = p2 * u2 + dp * w2;
-              L[LS] = p3 * u1 + dn * w1;
-              L[3 * LS] = p3 * u3 + dn * w3;
+              const cpx<T> u0 = L[0], u1 = L[LS], u2 = L[2 * LS], u3 = L[3 * LS];
+              L[0] = f00 * u0 + f01 * u2;
+              L[2 * LS] = f10 * u0 + f11 * u2;
+              L[LS] = b00 * u1 + b01 * u3;
+              L[3 * LS] = b10 * u1 + b11 * u3;
             }
           } else {
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
               cpx<T>* L = lds + (r * 2) * LS + ax;
-              cpx<T> u0 = L[0], u1 = L[LS];
-              cpx<T> w0 = e[4] * u0 + e[5] * u1, w1 = e[6] * u0 + e[7] * u1;
-              L[0] = p2 * u0 + dp * w0;
-              L[LS] = p2 * u1 + dp * w1;
+              const cpx<T> u0 = L[0], u1 = L[LS];
+              L[0] = f00 * u0 + f01 * u1;
+              L[LS] = f10 * u0 + f11 * u1;
             }
           }
         }
@@ -490,10 +522,10 @@ __global__ __launch_bounds__(row_lanes<N>() * 4, 2) void row_adj_kernel(AdjArgs<
   constexpr int PPX = (N + NTW - 1) / NTW;
   __shared__ cpx<T> la[LINES * LS];  // adjoint lines
   __shared__ cpx<T> lu[LINES * LS];  // forward lines
-  __shared__ cpx<T> tws[N];
+  __shared__ cpx<T> tws[tw_table_slots<N>()];
   const int tid = threadIdx.x;
   const int comp = tid / NL, lane = tid % NL;
-  for (int i = tid; i < N; i += NTW) tws[i] = A.tw[i];
+  fill_tw_tables<T, N>(tws, A.tw, tid, NTW);
   const int w = xcd_swizzle(blockIdx.x, gridDim.x);
   const int rl = w % (A.b.nrays * A.b.nlam_b);  // (ray, wavelength) of this work-group
   const int ray = rl / A.b.nlam_b, jl = rl % A.b.nlam_b;
@@ -648,27 +680,92 @@ struct ColArgs {
   Beams bm;
 };
 
+// A value known to be equal in every lane, moved to a scalar register.
+__device__ __forceinline__ float uniform(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double uniform(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readfirstlane(int(b & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane(int(b >> 32));
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Constants of the diagonal k-space operator shared by every k of a
+// work-group (its beam and wavelength are fixed): with
+// b2(k) = cx mx^2 + cy my^2 (= beta(k)^2) the phase is
+//   k0 (da/2 sqrt(ea - b2) + db/2 sqrt(eb - b2)) - pc,
+//   pc = k0 (da/2 kca sqrt(ea - bc^2) + db/2 kcb sqrt(eb - bc^2)).
+// `fast`: ea == eb and real, so one real square root per k.
 template <typename T>
-__device__ __forceinline__ cpx<T> diag_factor(const ColArgs<T>& A, int lam, int beam, int ky, int kx) {
+struct DiagConst {
+  T k0, cx, cy, ha, hb, er;
+  cpx<T> ea, eb, pc;
+  bool fast;
+};
+
+template <typename T>
+__device__ __forceinline__ DiagConst<T> diag_const(const ColArgs<T>& A, int lam, int beam) {
+  DiagConst<T> c;
+  const double k0 = A.bm.k0[lam];
+  const double l = 2 * M_PI / k0;
+  c.k0 = T(k0);
+  c.cx = T((l / A.W) * (l / A.W));
+  c.cy = T((l / A.H) * (l / A.H));
+  c.ea = cast<T>(A.ea);
+  c.eb = cast<T>(A.eb);
+  c.ha = T(0.5 * A.da * k0);
+  c.hb = T(0.5 * A.db * k0);
+  const T bc = T(A.bm.beta[beam]);
+  c.pc = mk<T>(T(0), T(0));
+  if (A.da != 0. && A.kca != 0.) c.pc = c.pc + csqrt(c.ea - mk<T>(bc * bc)) * T(0.5 * A.da * A.kca * k0);
+  if (A.db != 0. && A.kcb != 0.) c.pc = c.pc + csqrt(c.eb - mk<T>(bc * bc)) * T(0.5 * A.db * A.kcb * k0);
+  c.fast = (A.ea.x == A.eb.x && A.ea.y == A.eb.y && A.ea.y == 0.) || (A.db == 0. && A.ea.y == 0.);
+  c.er = T(A.ea.x);
+  if (A.db == 0.) c.hb = T(0);
+  if (A.da == 0.) c.ha = T(0);
+  // every field is uniform over the work-group: keep them in scalar registers
+  c.k0 = uniform(c.k0);
+  c.cx = uniform(c.cx);
+  c.cy = uniform(c.cy);
+  c.ha = uniform(c.ha);
+  c.hb = uniform(c.hb);
+  c.er = uniform(c.er);
+  c.ea = mk<T>(uniform(c.ea.x), uniform(c.ea.y));
+  c.eb = mk<T>(uniform(c.eb.x), uniform(c.eb.y));
+  c.pc = mk<T>(uniform(c.pc.x), uniform(c.pc.y));
+  return c;
+}
+
+template <typename T, bool FAST>
+__device__ __forceinline__ cpx<T> diag_factor(const ColArgs<T>& A, const DiagConst<T>& c, int lam,
+                                              int ky, int kx) {
   const int mx = mode_index(kx, A.W), my = mode_index(ky, A.H);
   if (mx > A.amax[lam * A.H + ky]) return mk<T>(T(0), T(0));
-  const T k0 = T(A.bm.k0[lam]);
-  const T l = T(2 * M_PI) / k0;
-  const T fx = T(mx) / T(A.W), fy = T(my) / T(A.H);
-  const T b2 = l * l * (fx * fx + fy * fy);
-  const T bc = T(A.bm.beta[beam]);
-  cpx<T> ph = mk<T>(T(0), T(0));
-  if (A.da != 0.) {
-    cpx<T> e = cast<T>(A.ea);
-    ph = ph + csqrt(e - mk<T>(b2)) * T(0.5 * A.da);
-    if (A.kca != 0.) ph = ph - csqrt(e - mk<T>(bc * bc)) * T(0.5 * A.da * A.kca);
+  const T b2 = c.cx * T(mx * mx) + c.cy * T(my * my);
+  cpx<T> ph;
+  if constexpr (FAST) {
+    // sqrt(er - b2) of a real medium: real, or positive imaginary past er
+    const T x = c.er - b2;
+    const T q = sqrt(fabs(x));
+    ph = (x >= T(0) ? mk<T>(q, T(0)) : mk<T>(T(0), q)) * (c.ha + c.hb) - c.pc;
+  } else {
+    ph = mk<T>(T(0), T(0)) - c.pc;
+    if (A.da != 0.) ph = ph + csqrt(c.ea - mk<T>(b2)) * c.ha;
+    if (A.db != 0.) ph = ph + csqrt(c.eb - mk<T>(b2)) * c.hb;
   }
-  if (A.db != 0.) {
-    cpx<T> e = cast<T>(A.eb);
-    ph = ph + csqrt(e - mk<T>(b2)) * T(0.5 * A.db);
-    if (A.kcb != 0.) ph = ph - csqrt(e - mk<T>(bc * bc)) * T(0.5 * A.db * A.kcb);
-  }
-  return cexpi_layer(ph * k0) * T(A.scale);
+  return cexpi_layer(ph) * T(A.scale);
+}
+
+// s^2 / k for the backward components (k = s exp(i ph) -> s exp(-i ph)),
+// zero where k is.
+__device__ __forceinline__ cpx<float> back_factor(cpx<float> k, float s2) {
+  const float n2 = norm2(k);
+  return n2 > 0.f ? conj(k) * (s2 * __builtin_amdgcn_rcpf(n2)) : mk<float>(0.f, 0.f);
+}
+__device__ __forceinline__ cpx<double> back_factor(cpx<double> k, double s2) {
+  const double n2 = norm2(k);
+  return n2 > 0. ? conj(k) * (s2 / n2) : mk<double>(0., 0.);
 }
 
 // Lanes per column: one radix-8 butterfly per lane, 16..128 lanes.
@@ -689,10 +786,10 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
   constexpr int NIN = slot_count<N, REV1, NL, false>();
   constexpr int NMID = slot_count<N, REV1, NL, true>();
   __shared__ cpx<T> lds[N * TX];
-  __shared__ cpx<T> tws[N];  // twiddles in LDS: their reads never wait on global loads
+  __shared__ cpx<T> tws[tw_table_slots<N>()];  // twiddles in LDS: their reads never wait on global loads
   const int tid = threadIdx.x;
   const int c = tid % TX, g = tid / TX;
-  for (int i = tid; i < N; i += TX * NL) tws[i] = A.tw[i];
+  fill_tw_tables<T, N>(tws, A.tw, tid, TX * NL);
   __syncthreads();
   const int ntiles = (A.W + TX - 1) / TX;
   const int w = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -704,6 +801,25 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
   const bool xin = x < A.W;
   const size_t HW = size_t(N) * A.W;
 
+  auto line_off = [&](int li) {
+    const int line = lg * A.ncl + li;
+    const int ray = line / A.NC, comp = line % A.NC;
+    const size_t q = size_t(ray) * A.b.nlam + lam;
+    return (q * A.NC + comp) * HW + x;
+  };
+  auto line_base = [&](int li) { return A.field + line_off(li); };
+  // prefetched inputs of the next line; the first line is requested before
+  // the k-space setup so that its latency overlaps the setup arithmetic
+  cpx<T> nxt[NIN];
+  {
+    const cpx<T>* b0 = line_base(0);
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) {
+      nxt[j] = mk<T>(T(0), T(0));
+      if (xin && slot_valid<N, REV1, NL, false>(j, g))
+        nxt[j] = b0[size_t(slot_pos<N, REV1, NL, false>(j, g)) * A.W];
+    }
+  }
   constexpr bool KOP = (OP == OP_DIAG || OP == OP_WIN || OP == OP_SPLIT);
   cpx<T> K[KOP ? NMID : 1];
   T Wv[(OP == OP_WIN) ? NMID : 1];  // window factor cx(kx) * cy(ky)
@@ -719,33 +835,27 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
       cyt = A.wcy + (bl * A.wn + A.wj) * N;
     }
     if constexpr (OP == OP_SPLIT) cys = A.wcy + (size_t(beam) * A.b.nlam + lam) * A.wn * N;
+    const DiagConst<T> dc = diag_const(A, lam, beam);
+#pragma unroll
+    for (int j = 0; j < NMID; ++j) K[j] = mk<T>(T(0), T(0));
+    if (dc.fast) {
+#pragma unroll
+      for (int j = 0; j < NMID; ++j)
+        if (xin && slot_valid<N, REV1, NL, true>(j, g))
+          K[j] = diag_factor<T, true>(A, dc, lam, slot_pos<N, REV1, NL, true>(j, g), x);
+    } else {
+#pragma unroll
+      for (int j = 0; j < NMID; ++j)
+        if (xin && slot_valid<N, REV1, NL, true>(j, g))
+          K[j] = diag_factor<T, false>(A, dc, lam, slot_pos<N, REV1, NL, true>(j, g), x);
+    }
 #pragma unroll
     for (int j = 0; j < NMID; ++j) {
-      K[j] = mk<T>(T(0), T(0));
-      if constexpr (OP == OP_WIN) Wv[j] = T(0);
-      if (xin && slot_valid<N, REV1, NL, true>(j, g)) {
-        const int ky = slot_pos<N, REV1, NL, true>(j, g);
-        K[j] = diag_factor(A, lam, beam, ky, x);
-        if (A.conj_k) K[j] = conj(K[j]);
-        if constexpr (OP == OP_WIN) Wv[j] = T(cx * cyt[ky]);
+      if (A.conj_k) K[j] = conj(K[j]);
+      if constexpr (OP == OP_WIN) {
+        Wv[j] = T(0);
+        if (xin && slot_valid<N, REV1, NL, true>(j, g)) Wv[j] = T(cx * cyt[slot_pos<N, REV1, NL, true>(j, g)]);
       }
-    }
-  }
-  auto line_off = [&](int li) {
-    const int line = lg * A.ncl + li;
-    const int ray = line / A.NC, comp = line % A.NC;
-    const size_t q = size_t(ray) * A.b.nlam + lam;
-    return (q * A.NC + comp) * HW + x;
-  };
-  auto line_base = [&](int li) { return A.field + line_off(li); };
-  cpx<T> nxt[NIN];  // prefetched inputs of the next line
-  {
-    const cpx<T>* b0 = line_base(0);
-#pragma unroll
-    for (int j = 0; j < NIN; ++j) {
-      nxt[j] = mk<T>(T(0), T(0));
-      if (xin && slot_valid<N, REV1, NL, false>(j, g))
-        nxt[j] = b0[size_t(slot_pos<N, REV1, NL, false>(j, g)) * A.W];
     }
   }
   for (int li = 0; li < A.ncl; ++li) {
@@ -768,10 +878,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
 #pragma unroll
       for (int j = 0; j < NMID; ++j) {
         cpx<T> k = K[j];
-        if (back) {
-          const T n2 = norm2(k);
-          k = n2 > T(0) ? conj(k) * (s2 / n2) : mk<T>(T(0), T(0));
-        }
+        if (back) k = back_factor(k, s2);
         if constexpr (OP == OP_WIN) k = k * Wv[j];
         v[j] = v[j] * k;
       }
