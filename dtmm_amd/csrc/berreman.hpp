@@ -480,8 +480,110 @@ namespace dtmmx {
 // dtmm/tmm.py:278-288 and the H rows are skipped.  Same eigenvalues and the
 // same projectors as layer_modes + rop_data (checked against the complex128
 // oracle by tests/test_abi.py through dtmmx_host_rop).
+// rop_lean in real fp64 arithmetic for a lossless (real eps) isotropic or
+// uniaxial voxel whose square-root arguments are all non-negative
+// (propagating modes): the same closed forms without complex square roots
+// and divisions.  Returns false, with r untouched, where that does not apply.
+__host__ __device__ inline bool rop_lean_real(double beta, double c0, double s0, double phi0, cd e0c,
+                                              cd e1c, cd e2c, double theta, double phi, RopData& r) {
+  if (e0c.y != 0. || e1c.y != 0. || e2c.y != 0. || e0c.x != e1c.x) return false;
+  const double eps11 = e0c.x, e2 = e2c.x;
+  double a[4], Fx[4], Fy[4];
+  int kind;
+  if (e2 == eps11) {
+    kind = KIND_ISO;
+    const double x = eps11 - beta * beta;
+    if (!(x >= 0.)) return false;
+    const double q = sqrt(x);
+    a[0] = q;
+    a[1] = -q;
+    a[2] = q;
+    a[3] = -q;
+    const double h = q == 0. ? 0. : 0.5;
+    Fx[0] = Fx[1] = h;
+    Fx[2] = Fx[3] = 0.;
+    Fy[0] = Fy[1] = 0.;
+    Fy[2] = Fy[3] = h;
+  } else {
+    kind = KIND_UNIAXIAL;
+    double st, ct, sp, cp;
+    sincos(theta, &st, &ct);
+    sincos(phi - phi0, &sp, &cp);
+    const double delta = e2 - eps11;
+    const double u = eps11 + delta * (ct * ct);
+    if (beta == 0.) {
+      const double w = eps11 * (eps11 + delta);
+      if (!(eps11 >= 0.) || u == 0. || !(u * w >= 0.)) return false;
+      const double evs = sqrt(eps11);
+      const double sq = sqrt(u * w) / u;
+      a[0] = sq;
+      a[1] = -sq;
+      a[2] = evs;
+      a[3] = -evs;
+      Fx[0] = Fx[1] = cp;
+      Fy[0] = Fy[1] = sp;
+      Fx[2] = Fx[3] = -evs * sp;
+      Fy[2] = Fy[3] = evs * cp;
+    } else {
+      const double ev02 = eps11 - beta * beta;
+      const double gama = beta * cp;
+      const double v = delta * (gama * 2. * st * ct);
+      const double w = delta * (gama * gama * st * st) - eps11 * (ev02 + delta);
+      const double disc = v * v - u * w * 4.;
+      if (!(ev02 >= 0.) || u == 0. || eps11 == 0. || !(disc >= 0.)) return false;
+      const double evs = sqrt(ev02);
+      const double sq = sqrt(disc) / u * 0.5;
+      const double vh = v / u * 0.5;
+      const double evpp = -vh + sq, evpm = -vh - sq;
+      a[0] = evpp;
+      a[1] = evpm;
+      a[2] = evs;
+      a[3] = -evs;
+      const double sfst = st * sp, cfst = cp * st, ctbeta = ct * beta;
+      const double ctbe = ctbeta / eps11;
+      const double ev02cfsteps11 = ev02 * cfst / eps11;
+      Fx[0] = evpp * ctbe - ev02cfsteps11;
+      Fy[0] = -sfst;
+      Fx[1] = -(evpm * ctbe) + ev02cfsteps11;
+      Fy[1] = sfst;
+      Fx[2] = evs * sfst;
+      Fy[2] = ctbeta - evs * cfst;
+      Fx[3] = -(evs * sfst);
+      Fy[3] = evs * cfst + ctbeta;
+    }
+  }
+  // beam-azimuth rotation of the E rows and the rank-1 projectors of the
+  // forward (ep) and backward (en) 2x2 blocks
+  double Ex[4], Ey[4];
+  for (int j = 0; j < 4; ++j) {
+    Ex[j] = Fx[j] * c0 - Fy[j] * s0;
+    Ey[j] = Fx[j] * s0 + Fy[j] * c0;
+  }
+  const double dp = Ex[0] * Ey[2] - Ex[2] * Ey[0];
+  const double dn = Ex[1] * Ey[3] - Ex[3] * Ey[1];
+  const bool okp = dp != 0., okn = dn != 0.;
+  const double ip = okp ? 1. / dp : 0., in = okn ? 1. / dn : 0.;
+  // first rows of the inverses: (Ey2, -Ex2) / dp and (Ey3, -Ex3) / dn
+  const double ip0 = Ey[2] * ip, ip1 = -(Ex[2] * ip);
+  const double in0 = Ey[3] * in, in1 = -(Ex[3] * in);
+  r.kind = kind;
+  for (int i = 0; i < 4; ++i) r.a[i] = mk<double>(a[i]);
+  r.P0[0][0] = mk<double>(Ex[0] * ip0);
+  r.P0[0][1] = mk<double>(Ex[0] * ip1);
+  r.P0[1][0] = mk<double>(Ey[0] * ip0);
+  r.P0[1][1] = mk<double>(Ey[0] * ip1);
+  r.P1[0][0] = mk<double>(Ex[1] * in0);
+  r.P1[0][1] = mk<double>(Ex[1] * in1);
+  r.P1[1][0] = mk<double>(Ey[1] * in0);
+  r.P1[1][1] = mk<double>(Ey[1] * in1);
+  r.ok2 = okp;
+  r.ok4 = okp && okn;
+  return true;
+}
+
 __host__ __device__ inline void rop_lean(double beta, double c0, double s0, double phi0, cd e0,
                                          cd e1, cd e2, double theta, double phi, RopData& r) {
+  if (rop_lean_real(beta, c0, s0, phi0, e0, e1, e2, theta, phi, r)) return;
   cd Fx[4], Fy[4];  // rows Ex and Ey of F before the beam-azimuth rotation
   const int kind = material_kind(e0, e1, e2);
   r.kind = kind;

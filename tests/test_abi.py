@@ -207,6 +207,37 @@ def test_host_layer_operator_matches_oracle(absorbing, biaxial):
     np.testing.assert_allclose(M, ref, atol=1e-11)
 
 
+def test_host_rop_real_path_matches_complex_path():
+    """Lossless isotropic / uniaxial voxels with propagating modes take a
+    real-arithmetic solve; a 1e-300 imaginary eps sends the same voxels
+    through the complex closed forms, and both must agree.  Voxels with
+    evanescent modes (beta^2 > eps) must stay on the complex solve: their
+    ordinary-mode alpha is imaginary."""
+    lib = _lib.load()
+    beta, phi, eps, ang = _voxels(800, 3)
+    n = len(beta)
+
+    def solve(b, eps_im):
+        al = np.zeros((n, 4, 2))
+        pr = np.zeros((n, 2, 2, 2, 2))
+        ok = np.zeros((n, 2), np.int32)
+        _lib.check(lib.dtmmx_host_rop(n, _lib.dptr(b), _lib.dptr(phi), _lib.dptr(eps),
+                                      _lib.dptr(eps_im), _lib.dptr(ang), _lib.dptr(al),
+                                      _lib.dptr(pr), ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return al[..., 0] + 1j * al[..., 1], pr, ok
+
+    a_r, p_r, ok_r = solve(beta, None)
+    a_c, p_c, ok_c = solve(beta, np.full(eps.shape, 1e-300))
+    assert np.array_equal(ok_r, ok_c)
+    assert np.all(a_r.imag == 0.)
+    np.testing.assert_allclose(a_r, a_c, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(p_r, p_c, rtol=0, atol=1e-11 * np.abs(p_c).max())
+    # evanescent: beta above sqrt(eps_o)
+    b_ev = np.sqrt(eps[:, 0]) * np.random.default_rng(4).uniform(1.05, 1.3, n)
+    a_e, _, _ = solve(b_ev, None)
+    np.testing.assert_allclose(np.abs(a_e[:, 2].imag), np.sqrt(b_ev ** 2 - eps[:, 0]), rtol=1e-12)
+
+
 @pytest.mark.parametrize("n", [2, 3, 5])
 @pytest.mark.parametrize("case", [
     dict(shape=(32, 48), wl=(450., 650.), beta=(0.,), phi=(0.,)),
