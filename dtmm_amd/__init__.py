@@ -2,7 +2,8 @@
 
 Drop-in for ``dtmm.transfer_field`` (4x4 method: reflection mode 1, any
 diffraction and npass; 2x2 method: reflection modes 0 and 1) with the
-reference's data model, plus ``transfer_batch`` for samples of uneven size.
+reference's data model, plus ``transfer_batch`` for samples of uneven size and
+``TransferGraph``, the same batch captured once as a HIP graph and replayed.
 Propagation runs in hand-written HIP kernels for gfx950 (``dtmm_amd/csrc``)
 reached through the C ABI of ``include/dtmmx.h``; there is no CPU fallback.
 """
@@ -11,7 +12,7 @@ from .data import (refind2eps, uniaxial_order, director2data, director2angles,  
                    validate_optical_block, validate_optical_data)
 from .field import (illumination_data, field2intensity, blackman, k0, betaphi,  # noqa: F401
                     planewave)
-from .transfer import (transfer_field, transfer_batch, Plan, Stack,  # noqa: F401
+from .transfer import (transfer_field, transfer_batch, TransferGraph, Plan, Stack,  # noqa: F401
                        DTMM_NORM_FFT, DTMM_NORM_REF, field2betaphi, mean_betaphi)
 from .adjoint import AdjointTransfer, polarizer4x4, analyser_intensity_loss  # noqa: F401
 

@@ -122,7 +122,11 @@ template <typename T, int N>
 constexpr int col_tx() {
   constexpr int cb = int(sizeof(cpx<T>));
   constexpr int nl = col_lanes<N>();
+#ifdef DTMMX_COL_TX
+  int tx = DTMMX_COL_TX;
+#else
   int tx = 16;
+#endif
   while (tx > 1 && (N * (tx + 1) * cb > DTMMX_COL_LDS_CAP || tx * nl > 1024)) tx /= 2;
   return tx;
 }
@@ -197,6 +201,10 @@ struct dtmmx_plan {
   double* wcx = nullptr;  // [beam][nlam][wn][W]
   double* wcy = nullptr;  // [beam][nlam][wn][H]
   int* wmask = nullptr;   // [nlam][H]: min(amax, largest |kx| the windows pass)
+  // launch shapes (0: chosen from the grid): column lines per work-group,
+  // row rays per work-group (dtmmx_plan_set_tiles, the autotuner's knobs)
+  int col_lines = 0;
+  int row_rays = 0;
 };
 
 namespace dtmmx {
@@ -247,7 +255,7 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   a.bm = bm ? *bm : P->bm;
   int nr = 1;
   if (sizeof(T) == 4 && a.W <= 512 && b.nrays % 2 == 0 &&
-      ((op != OP_ROP && op != OP_ROPW) || P->bm.rays_per_beam % 2 == 0))
+      ((op != OP_ROP && op != OP_ROPW) || P->bm.rays_per_beam % 2 == 0) && P->row_rays != 1)
     nr = 2;
   a.rays_per_group = nr;
   const int nwork = a.H * (b.nrays / nr);
@@ -309,9 +317,11 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   a.ncl = nc;
   if (b.nrays % 2 == 0 && P->bm.rays_per_beam % 2 == 0 && nc <= 2) a.ncl = 2 * nc;
   bool tune = true;
-  if (const char* e = std::getenv("DTMMX_COL_NCL")) {
-    const int v = std::atoi(e);
-    if (v > 0 && nc % v == 0) {
+  {
+    int v = P->col_lines;
+    if (const char* e = std::getenv("DTMMX_COL_NCL")) v = std::atoi(e);
+    const bool pair = b.nrays % 2 == 0 && P->bm.rays_per_beam % 2 == 0;
+    if (v > 0 && (nc % v == 0 || (pair && v == 2 * nc))) {
       a.ncl = v;
       tune = false;
     }
@@ -1587,6 +1597,23 @@ int dtmmx_transfer_adjoint(const dtmmx_plan* P, const dtmmx_stack* S, const void
   hipStream_t s = (hipStream_t)stream;
   if (P->d.precision == DTMMX_C64) return adjoint_c64(P, S, tape, seed, grad, ws, status, s);
   return adjoint_c128(P, S, tape, seed, grad, ws, status, s);
+}
+
+int dtmmx_plan_set_tiles(dtmmx_plan* P, int col_lines, int row_rays) {
+  if (!P) return fail(DTMMX_E_ARG, "null plan");
+  if (!(col_lines == 0 || col_lines == 1 || col_lines == 2 || col_lines == 4 || col_lines == 8))
+    return fail(DTMMX_E_ARG, "col_lines must be 0, 1, 2, 4 or 8");
+  if (row_rays < 0 || row_rays > 2) return fail(DTMMX_E_ARG, "row_rays must be 0, 1 or 2");
+  P->col_lines = col_lines;
+  P->row_rays = row_rays;
+  return DTMMX_OK;
+}
+
+int dtmmx_plan_get_tiles(const dtmmx_plan* P, int* col_lines, int* row_rays) {
+  if (!P || !col_lines || !row_rays) return fail(DTMMX_E_ARG, "null argument");
+  *col_lines = P->col_lines;
+  *row_rays = P->row_rays;
+  return DTMMX_OK;
 }
 
 int dtmmx_transfer_batch(const dtmmx_job* jobs, int n, int* status, void* stream) {

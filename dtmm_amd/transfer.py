@@ -20,6 +20,7 @@ import warnings
 import numpy as np
 
 from . import _lib
+from . import tune as _tune
 from .data import refind2eps, uniaxial_order, validate_optical_data
 from .field import blackman, k0 as _k0
 
@@ -374,19 +375,23 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
                    multiray=False, norm=DTMM_NORM_FFT, betamax=None, smooth=SMOOTH,
                    split_rays=False, split_diffraction=False, split_wavelengths=False,
                    split_layers=False, eff_data=None, ret_bulk=False, out=None,
-                   precision=None):
+                   precision=None, tune=False):
     """Transfers input field data through optical data (dtmm/transfer.py:322).
 
     Returns ``(field_out, wavelengths, pixelsize)``.  ``precision`` ("c64" or
     "c128") selects the GPU arithmetic; by default it follows the input
     field dtype (complex64 -> "c64", otherwise "c128").  The split_* options
     only bound memory in the reference and do not change the result; they
-    are accepted and ignored here (the GPU batches all planes).
+    are accepted and ignored here (the GPU batches all planes).  ``tune``
+    times the kernels' launch shapes on this transfer first (dtmm_amd.tune;
+    the winner is cached for later calls of the same workload shape).
     """
     torch = _require_gpu()
     job = _prepare(field_data, optical_data, beta, phi, nin, nout, npass, nstep, diffraction,
                    reflection, method, multiray, norm, betamax, smooth, split_layers, eff_data,
                    ret_bulk, out, precision)
+    if tune:
+        _tune.tune_plan(job.plan, job.stack, job.fin, job.fout, job.ws)
     status = torch.zeros(4, dtype=torch.int32, device="cuda")
     job.plan.execute(job.stack, job.fin, job.fout, workspace=job.ws, status=status)
     torch.cuda.synchronize()
@@ -394,18 +399,8 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
     return job.finish()
 
 
-def transfer_batch(samples, **kwargs):
-    """Transfers several independent samples of uneven grid, depth and beams
-    in one batched engine call (``dtmmx_transfer_batch``).
-
-    ``samples`` is a sequence of ``(field_data, optical_data)`` pairs or of
-    dicts ``{"field_data": ..., "optical_data": ..., **per-sample options}``;
-    ``kwargs`` are transfer_field options shared by all samples.  Returns
-    the list of ``(field_out, wavelengths, pixelsize)``, each equal to what
-    transfer_field returns for that sample (the reference users loop
-    dtmm.transfer_field over their samples, dtmm/transfer.py:322)."""
-    torch = _require_gpu()
-    lib = _lib.load()
+def _prepare_samples(samples, kwargs):
+    """_prepare for each sample of transfer_batch / TransferGraph."""
     jobs = []
     for smp in samples:
         if isinstance(smp, dict):
@@ -418,8 +413,12 @@ def transfer_batch(samples, **kwargs):
         for k in ("split_rays", "split_diffraction", "split_wavelengths"):
             opts = {kk: vv for kk, vv in opts.items() if kk != k}
         jobs.append(_prepare(fd, od, **opts))
-    if not jobs:
-        return []
+    return jobs
+
+
+def _job_array(jobs):
+    """The dtmmx_job descriptors of prepared jobs (and the stack structs they
+    point to, which must outlive the array)."""
     structs = [j.stack.c_struct() for j in jobs]
     arr = (_lib.Job * len(jobs))()
     for a, j, st in zip(arr, jobs, structs):
@@ -429,9 +428,99 @@ def transfer_batch(samples, **kwargs):
         a.field_out = j.fout.data_ptr()
         a.workspace = j.ws.data_ptr() if j.ws is not None else None
         a.workspace_bytes = j.plan.workspace_bytes
+    return arr, structs
+
+
+def transfer_batch(samples, tune=False, **kwargs):
+    """Transfers several independent samples of uneven grid, depth and beams
+    in one batched engine call (``dtmmx_transfer_batch``).
+
+    ``samples`` is a sequence of ``(field_data, optical_data)`` pairs or of
+    dicts ``{"field_data": ..., "optical_data": ..., **per-sample options}``;
+    ``kwargs`` are transfer_field options shared by all samples.  Returns
+    the list of ``(field_out, wavelengths, pixelsize)``, each equal to what
+    transfer_field returns for that sample (the reference users loop
+    dtmm.transfer_field over their samples, dtmm/transfer.py:322).  ``tune``
+    tunes each sample's launch shapes first (dtmm_amd.tune)."""
+    torch = _require_gpu()
+    lib = _lib.load()
+    jobs = _prepare_samples(samples, kwargs)
+    if not jobs:
+        return []
+    if tune:
+        for j in jobs:
+            _tune.tune_plan(j.plan, j.stack, j.fin, j.fout, j.ws)
+    arr, structs = _job_array(jobs)
     status = torch.zeros(4, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
     _lib.check(lib.dtmmx_transfer_batch(arr, len(jobs), status.data_ptr(), stream.cuda_stream))
     torch.cuda.synchronize()
     _check_status(status)
     return [j.finish() for j in jobs]
+
+
+class TransferGraph:
+    """A batch of transfers captured once as a HIP graph and replayed.
+
+    The samples are prepared as by ``transfer_batch`` (plans, stacks, device
+    fields, workspaces) and ``dtmmx_transfer_batch`` is captured into one
+    ``torch.cuda.CUDAGraph``: a replay launches every sample's kernels with a
+    single graph launch and no host work.  Inputs of the same shapes are
+    replaced with ``set_inputs``; every replay starts from the stored inputs
+    (samples with npass > 1 overwrite their input buffer with the input-side
+    field, so the graph first restores it).  Replays are ordered on the
+    stream they are launched from; ``results`` waits for them.  ``tune``
+    tunes each sample's launch shapes before the capture (dtmm_amd.tune).
+    """
+
+    def __init__(self, samples, tune=False, **kwargs):
+        torch = _require_gpu()
+        self._lib = _lib.load()
+        self.jobs = _prepare_samples(samples, kwargs)
+        if not self.jobs:
+            raise ValueError("no samples")
+        if tune:
+            for j in self.jobs:
+                _tune.tune_plan(j.plan, j.stack, j.fin, j.fout, j.ws)
+        self._arr, self._structs = _job_array(self.jobs)
+        self._inputs = [j.fin.clone() if j.npass > 1 else None for j in self.jobs]
+        self.status = torch.zeros(4, dtype=torch.int32, device="cuda")
+        # one eager run: argument checks and lazy initialisation outside capture
+        self._run(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        _check_status(self.status)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._run(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+
+    def _run(self, stream):
+        for j, x in zip(self.jobs, self._inputs):
+            if x is not None:
+                j.fin.copy_(x)
+        _lib.check(self._lib.dtmmx_transfer_batch(self._arr, len(self.jobs), self.status.data_ptr(),
+                                                  stream.cuda_stream))
+
+    def set_inputs(self, fields):
+        """New input fields, one per sample, each of the sample's original
+        shape (NumPy arrays or tensors)."""
+        torch = _torch()
+        if len(fields) != len(self.jobs):
+            raise ValueError("expected %d fields" % len(self.jobs))
+        for i, (j, f) in enumerate(zip(self.jobs, fields)):
+            t = torch.as_tensor(np.asarray(f) if not torch.is_tensor(f) else f)
+            if tuple(t.shape) != tuple(j.field_np.shape):
+                raise ValueError("field %d has shape %s, expected %s"
+                                 % (i, tuple(t.shape), tuple(j.field_np.shape)))
+            dst = self._inputs[i] if self._inputs[i] is not None else j.fin
+            dst.copy_(t.reshape(dst.shape).to(device=dst.device, dtype=dst.dtype))
+
+    def replay(self):
+        self.graph.replay()
+
+    def results(self):
+        """The outputs of the last replay, as transfer_batch returns them."""
+        torch = _torch()
+        torch.cuda.synchronize()
+        _check_status(self.status)
+        return [j.finish() for j in self.jobs]

@@ -126,6 +126,16 @@ int dtmmx_plan_destroy(dtmmx_plan* plan);
  * layers flagged biaxial are solved into. */
 size_t dtmmx_plan_workspace_size(const dtmmx_plan* plan);
 
+/* Launch shapes of the plan's per-layer kernels: col_lines = field lines
+ * per column work-group (1, 2, 4, or 8 when two rays share a beam; lines
+ * that do not divide the work fall back to the automatic choice),
+ * row_rays = rays per row work-group (1, or 2 sharing the per-pixel solve);
+ * 0 = chosen from the grid.  Results do not depend on them.  The reference
+ * has no counterpart (its numba kernels have no launch shape); they are the
+ * knobs of dtmm_amd.tune, which times candidates on the caller's workload. */
+int dtmmx_plan_set_tiles(dtmmx_plan* plan, int col_lines, int row_rays);
+int dtmmx_plan_get_tiles(const dtmmx_plan* plan, int* col_lines, int* row_rays);
+
 /* Full transfer through the stack.
  *   4x4 (reflection mode 1, diffraction >= 1, npass >= 1): replaces
  *       transfer.py:694-875 with propagate_4x4.py:376-444 and :80-101 per

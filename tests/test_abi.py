@@ -96,6 +96,13 @@ def test_plan_create_rejects_bad_descriptions(kw, code):
     assert msg
 
 
+def test_plan_tiles_reject_bad_arguments():
+    lib = _lib.load()
+    assert lib.dtmmx_plan_set_tiles(None, 0, 0) == _lib.DTMMX_E_ARG
+    c = ctypes.c_int()
+    assert lib.dtmmx_plan_get_tiles(None, ctypes.byref(c), ctypes.byref(c)) == _lib.DTMMX_E_ARG
+
+
 def _voxels(n, seed, biaxial=False):
     rng = np.random.default_rng(seed)
     beta = rng.uniform(0, 0.9, n)

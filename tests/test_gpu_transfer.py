@@ -269,6 +269,50 @@ def test_transfer_batch_uneven_samples():
             assert err < TOL[precision], err
 
 
+@pytest.mark.gpu
+def test_transfer_graph_replays_match_eager_batch():
+    """The batch captured as one HIP graph: every replay, with eager engine
+    work and host synchronisation between replays and new inputs set before
+    one of them, gives exactly the eager batch's output (same kernels, same
+    inputs), including a sample with npass = 3 that overwrites its input."""
+    flush("case transfer graph replays")
+    s1 = random_case(shape=(5, 32, 48), wavelengths=(450., 650.), jones=None, seed=3)
+    s2 = droplet_case(shape=(9, 24, 24), radius=9, wavelengths=(550.,), jones=(1, 0))
+    s3 = random_case(shape=(4, 32, 32), wavelengths=(500.,), jones=(1, 0), seed=5)
+    samples = [
+        {"field_data": s1[0], "optical_data": s1[1], "beta": 0., "phi": 0.},
+        {"field_data": s2[0], "optical_data": s2[1], "beta": 0., "phi": 0., "method": "2x2"},
+        {"field_data": s3[0], "optical_data": s3[1], "beta": 0., "phi": 0., "npass": 3,
+         "nin": 1.0, "nout": 1.3},
+    ]
+
+    def batch(fields):
+        b = []
+        for s, f in zip(samples, fields):
+            b.append(dict(s, field_data=(f.copy(),) + tuple(s["field_data"][1:])))
+        return b
+
+    fields0 = [s["field_data"][0].astype(np.complex64) for s in samples]
+    rng = np.random.default_rng(11)
+    fields1 = [(f * (0.5 + rng.uniform(size=f.shape))).astype(np.complex64) for f in fields0]
+    eager0 = [r[0] for r in dm.transfer_batch(batch(fields0), method="4x4", precision="c64")]
+    eager1 = [r[0] for r in dm.transfer_batch(batch(fields1), method="4x4", precision="c64")]
+    g = dm.TransferGraph(batch(fields0), method="4x4", precision="c64")
+    for rep, (fields, eager) in enumerate([(None, eager0), (None, eager0), (fields1, eager1),
+                                           (None, eager1), (fields0, eager0)]):
+        if fields is not None:
+            g.set_inputs(fields)
+        g.replay()
+        out = [r[0] for r in g.results()]
+        # eager engine work between replays
+        dm.transfer_field((fields0[1].copy(),) + tuple(samples[1]["field_data"][1:]),
+                          samples[1]["optical_data"], beta=0., phi=0., method="2x2",
+                          precision="c64")
+        for i, (a, b) in enumerate(zip(out, eager)):
+            flush("  replay %d sample %d max diff %.3e" % (rep, i, float(np.abs(a - b).max())))
+            assert np.array_equal(a, b), (rep, i)
+
+
 def _oracle_analyser_loss(fd, d, eps, ang, roi, jones=(1, 1), n=1.5, beta=0., phi=0., nstep=1):
     """The analysed-intensity loss on the oracle's output field."""
     field, wl, ps = fd
@@ -448,3 +492,38 @@ def test_biaxial_adjoint_is_refused():
     tr = dm.AdjointTransfer(fd, optical, beta=0., phi=0., precision="c128")
     with pytest.raises(NotImplementedError):
         tr.forward()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+def test_launch_shapes_do_not_change_results_and_tuner_picks_one(method):
+    """Every (column lines, row rays) launch shape gives the same output bit
+    for bit; the autotuner picks one of them and the tuned transfer matches
+    the oracle."""
+    import torch
+    from dtmm_amd import tune
+    flush("case launch shapes %s" % method)
+    fd, optical = random_case(shape=(6, 64, 96), wavelengths=(450., 550., 650.), jones=None)
+    field, wl, ps = fd
+    job = dm.transfer._prepare((field.astype(np.complex64), wl, ps), optical, beta=0., phi=0.,
+                               method=method, precision="c64")
+    cands = tune.candidates(job.plan)
+    assert (4 if method == "4x4" else 2) in [c for c, _ in cands]
+    assert len(cands) >= 4
+    outs = {}
+    for c, r in cands:
+        tune.set_tiles(job.plan, c, r)
+        assert tune.get_tiles(job.plan) == (c, r)
+        job.plan.execute(job.stack, job.fin, job.fout, workspace=job.ws)
+        torch.cuda.synchronize()
+        outs[(c, r)] = job.fout.cpu().numpy()
+    first = next(iter(outs.values()))
+    for k, v in outs.items():
+        assert np.array_equal(v, first), k
+    res = tune.tune_plan(job.plan, job.stack, job.fin, job.fout, job.ws, cache=False)
+    flush("  tuned %s: %s" % (res["tiles"], {k: round(v, 3) for k, v in res["ms"].items()}))
+    assert res["tiles"] in cands and set(res["ms"]) == set(cands)
+    out = dm.transfer_field((field.astype(np.complex64), wl, ps), optical, beta=0., phi=0.,
+                            method=method, precision="c64", tune=True)[0]
+    ref = oracle.transfer_field(fd, optical, beta=0., phi=0., method=method)[0]
+    assert rel_err(out, ref) < TOL["c64"]
