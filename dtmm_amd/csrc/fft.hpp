@@ -206,6 +206,68 @@ __device__ __forceinline__ void fill_tw_stage(cpx<T>* tws, const cpx<T>* tw, int
   }
 }
 
+// Two-phase fill of the twiddle tables by NTH threads: load() reads this
+// thread's entries into registers, store() writes them to LDS.  Vector loads
+// complete in issue order, so a kernel issues these small (L2-resident)
+// loads first, then its large field loads, and stores the table after its
+// setup arithmetic without waiting for the field.
+template <int N, bool REV, int S, int NTH>
+__host__ __device__ constexpr int tw_regs_from() {
+  int n = 0;
+  for (int s = S; s < make_plan(N).ns; ++s)
+    n += ((radix_at<N, REV>(s) - 1) * pp_at<N, REV>(s) + NTH - 1) / NTH;
+  return n;
+}
+template <int N, int NTH>
+__host__ __device__ constexpr int tw_regs() {
+  return tw_regs_from<N, false, 1, NTH>() + (plan_palindrome<N>() ? 0 : tw_regs_from<N, true, 1, NTH>());
+}
+
+template <typename T, int N, bool REV, int S, int NTH, int Q, int MAXE>
+__device__ __forceinline__ void tw_load_stage(cpx<T> (&r)[MAXE], const cpx<T>* tw, int tid) {
+  if constexpr (S < make_plan(N).ns) {
+    constexpr int R = radix_at<N, REV>(S), PP = pp_at<N, REV>(S);
+    constexpr int STEP = N / (PP * R), CNT = (R - 1) * PP, IT = (CNT + NTH - 1) / NTH;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int e = tid + i * NTH;
+      if (e < CNT) r[Q + i] = tw[(1 + e / PP) * (e % PP) * STEP];
+    }
+    tw_load_stage<T, N, REV, S + 1, NTH, Q + IT>(r, tw, tid);
+  }
+}
+
+template <typename T, int N, bool REV, int S, int NTH, int Q, int MAXE>
+__device__ __forceinline__ void tw_store_stage(const cpx<T> (&r)[MAXE], cpx<T>* tws, int tid) {
+  if constexpr (S < make_plan(N).ns) {
+    constexpr int R = radix_at<N, REV>(S), PP = pp_at<N, REV>(S);
+    constexpr int CNT = (R - 1) * PP, IT = (CNT + NTH - 1) / NTH;
+    constexpr int BASE = tw_table_off<N, REV>() + PP - radix_at<N, REV>(0);
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int e = tid + i * NTH;
+      if (e < CNT) tws[BASE + e] = r[Q + i];
+    }
+    tw_store_stage<T, N, REV, S + 1, NTH, Q + IT>(r, tws, tid);
+  }
+}
+
+template <typename T, int N, int NTH>
+struct TwFill {
+  static constexpr int MAXE = tw_regs<N, NTH>() > 0 ? tw_regs<N, NTH>() : 1;
+  cpx<T> r[MAXE];
+  __device__ __forceinline__ void load(const cpx<T>* tw, int tid) {
+    tw_load_stage<T, N, false, 1, NTH, 0>(r, tw, tid);
+    if constexpr (!plan_palindrome<N>())
+      tw_load_stage<T, N, true, 1, NTH, tw_regs_from<N, false, 1, NTH>()>(r, tw, tid);
+  }
+  __device__ __forceinline__ void store(cpx<T>* tws, int tid) const {
+    tw_store_stage<T, N, false, 1, NTH, 0>(r, tws, tid);
+    if constexpr (!plan_palindrome<N>())
+      tw_store_stage<T, N, true, 1, NTH, tw_regs_from<N, false, 1, NTH>()>(r, tws, tid);
+  }
+};
+
 // Fills the twiddle tables of length-N plans from the plain table
 // tw[j] = exp(-2 pi i j / N) (nth threads, thread tid; the caller
 // synchronises before the first transform).

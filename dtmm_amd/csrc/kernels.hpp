@@ -168,7 +168,17 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   __shared__ cpx<T> tws[tw_table_slots<N>()];  // twiddles in LDS: their reads never wait on global loads
   const int tid = threadIdx.x;
   const int li = tid / NL, lane = tid % NL;
-  fill_tw_tables<T, N>(tws, A.tw, tid, NTW);
+  // Prologue order where registers allow (EARLY): twiddles and this
+  // thread's material first, then the first field line, then the fp64 solve
+  // (loads complete in issue order, so the solve does not wait for the
+  // line).  Otherwise the twiddle table is filled at once and the material
+  // read inside the solve.
+  constexpr bool EARLY = tw_regs<N, NTW>() <= 2 || row_min_waves<T, N>() <= 2;
+  TwFill<T, N, NTW> twf;
+  if constexpr (EARLY)
+    twf.load(A.tw, tid);
+  else
+    fill_tw_tables<T, N>(tws, A.tw, tid, NTW);
   const int rr = li / NC, comp = li - rr * NC;
   const int w = xcd_swizzle(blockIdx.x, gridDim.x);
   const int ngroups = A.b.nrays / NR;
@@ -177,6 +187,25 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   const size_t HW = size_t(A.H) * N;
   cpx<T>* line_lds = lds + li * LS;
 
+  constexpr int PPM = (ROPX && EARLY) ? PPX : 1;
+  double me[PPM][6], mth[PPM], mph[PPM];  // eps (re, im) x 3, theta, phi
+  auto load_material = [&](int k) {
+    const size_t o = size_t(y) * N + tid + k * NTW;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      me[k][2 * q] = A.eps_re[q * A.comp_stride + o];
+      me[k][2 * q + 1] = A.eps_im ? A.eps_im[q * A.comp_stride + o] : 0.;
+    }
+    mth[k] = A.ang[A.comp_stride + o];
+    mph[k] = A.ang[2 * A.comp_stride + o];
+  };
+  if constexpr (ROPX && EARLY) {
+    if (!A.etab) {
+#pragma unroll
+      for (int k = 0; k < PPX; ++k)
+        if (tid + k * NTW < N) load_material(k);
+    }
+  }
   // line of wavelength lam0 + jl for this wave
   auto line_ptr = [&](int jl) {
     const size_t q = size_t(rg * NR + rr) * A.b.nlam + A.b.lam0 + jl;
@@ -218,14 +247,19 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
     for (int k = 0; k < PPX; ++k) {
       const int x = tid + k * NTW;
       if (x < N) {
-        const size_t o = size_t(y) * N + x;
-        const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
-        const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
-        const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
-                                 A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
         RopData r;
-        rop_lean(beta, c0, s0, phi0, e0, e1, e2, A.ang[A.comp_stride + o],
-                 A.ang[2 * A.comp_stride + o], r);
+        if constexpr (EARLY) {
+          rop_lean(beta, c0, s0, phi0, mk<double>(me[k][0], me[k][1]), mk<double>(me[k][2], me[k][3]),
+                   mk<double>(me[k][4], me[k][5]), mth[k], mph[k], r);
+        } else {
+          const size_t o = size_t(y) * N + x;
+          const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+          const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
+          const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
+                                   A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
+          rop_lean(beta, c0, s0, phi0, e0, e1, e2, A.ang[A.comp_stride + o],
+                   A.ang[2 * A.comp_stride + o], r);
+        }
         if (r.kind == KIND_BIAXIAL && A.status) atomicOr(A.status, 1);
         for (int i = 0; i < 4; ++i) eg[k][i] = cast<T>(r.a[i]);
         for (int i = 0; i < 2; ++i)
@@ -242,6 +276,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
     }
    }
   }
+  if constexpr (EARLY) twf.store(tws, tid);  // read after the barrier below
   // Every pixel of this thread's wave has real mode pairs a1 = -a0,
   // a3 = -a2 (lossless media at normal incidence, isotropic voxels): one
   // sine and cosine per pair, and no exponential.  A wave vote keeps the
@@ -738,10 +773,11 @@ __device__ __forceinline__ DiagConst<T> diag_const(const ColArgs<T>& A, int lam,
 }
 
 template <typename T, bool FAST>
-__device__ __forceinline__ cpx<T> diag_factor(const ColArgs<T>& A, const DiagConst<T>& c, int lam,
+__device__ __forceinline__ cpx<T> diag_factor(const ColArgs<T>& A, const DiagConst<T>& c, int amax,
                                               int ky, int kx) {
+  // amax: largest |kx| index passing betamax in row ky (-1: none)
   const int mx = mode_index(kx, A.W), my = mode_index(ky, A.H);
-  if (mx > A.amax[lam * A.H + ky]) return mk<T>(T(0), T(0));
+  if (mx > amax) return mk<T>(T(0), T(0));
   const T b2 = c.cx * T(mx * mx) + c.cy * T(my * my);
   cpx<T> ph;
   if constexpr (FAST) {
@@ -789,8 +825,11 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
   __shared__ cpx<T> tws[tw_table_slots<N>()];  // twiddles in LDS: their reads never wait on global loads
   const int tid = threadIdx.x;
   const int c = tid % TX, g = tid / TX;
-  fill_tw_tables<T, N>(tws, A.tw, tid, TX * NL);
-  __syncthreads();
+  // Prologue order: the small L2-resident loads (twiddles, betamax mask)
+  // first, then the first field line, then the setup arithmetic; loads
+  // complete in issue order, so the setup waits only for the small ones.
+  TwFill<T, N, TX * NL> twf;
+  twf.load(A.tw, tid);
   const int ntiles = (A.W + TX - 1) / TX;
   const int w = xcd_swizzle(blockIdx.x, gridDim.x);
   const int tile = w % ntiles;
@@ -800,6 +839,13 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
   const int x = tile * TX + c;
   const bool xin = x < A.W;
   const size_t HW = size_t(N) * A.W;
+  constexpr bool KOP = (OP == OP_DIAG || OP == OP_WIN || OP == OP_SPLIT);
+  int am[KOP ? NMID : 1];  // largest passing |kx| of each slot's ky row
+  if constexpr (KOP) {
+#pragma unroll
+    for (int j = 0; j < NMID; ++j)
+      am[j] = slot_valid<N, REV1, NL, true>(j, g) ? A.amax[lam * A.H + slot_pos<N, REV1, NL, true>(j, g)] : -1;
+  }
 
   auto line_off = [&](int li) {
     const int line = lg * A.ncl + li;
@@ -820,7 +866,6 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
         nxt[j] = b0[size_t(slot_pos<N, REV1, NL, false>(j, g)) * A.W];
     }
   }
-  constexpr bool KOP = (OP == OP_DIAG || OP == OP_WIN || OP == OP_SPLIT);
   cpx<T> K[KOP ? NMID : 1];
   T Wv[(OP == OP_WIN) ? NMID : 1];  // window factor cx(kx) * cy(ky)
   const double* cys = nullptr;        // OP_SPLIT: cy [wn][H] of this beam and wavelength
@@ -842,12 +887,12 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
 #pragma unroll
       for (int j = 0; j < NMID; ++j)
         if (xin && slot_valid<N, REV1, NL, true>(j, g))
-          K[j] = diag_factor<T, true>(A, dc, lam, slot_pos<N, REV1, NL, true>(j, g), x);
+          K[j] = diag_factor<T, true>(A, dc, am[j], slot_pos<N, REV1, NL, true>(j, g), x);
     } else {
 #pragma unroll
       for (int j = 0; j < NMID; ++j)
         if (xin && slot_valid<N, REV1, NL, true>(j, g))
-          K[j] = diag_factor<T, false>(A, dc, lam, slot_pos<N, REV1, NL, true>(j, g), x);
+          K[j] = diag_factor<T, false>(A, dc, am[j], slot_pos<N, REV1, NL, true>(j, g), x);
     }
 #pragma unroll
     for (int j = 0; j < NMID; ++j) {
@@ -858,6 +903,9 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
       }
     }
   }
+  // visible to the other waves after the first exchange barrier of the
+  // first transform, before the first twiddle read (stage 1)
+  twf.store(tws, tid);
   for (int li = 0; li < A.ncl; ++li) {
     const int comp = (lg * A.ncl + li) % A.NC;
     cpx<T>* base = (OP == OP_WIN || OP == OP_SPLIT) ? A.out + line_off(li) : line_base(li);
