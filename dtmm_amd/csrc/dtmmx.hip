@@ -926,11 +926,64 @@ static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin
   return DTMMX_OK;
 }
 
+// diffraction 0 (4x4 reflection mode 2, 2x2 reflection mode 0, one pass):
+// every pixel is an independent stack, the field stays in real space
+// (dtmm/transfer.py:694-700 and :1071-1099 with work_in_fft False).  The
+// 4x4 method propagates the input as it is, skipping the input and output
+// media; the 2x2 method keeps the forward part of the input in the input
+// medium, propagates E and builds H in the output medium.
+template <typename T>
+static int transfer_nodiff(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
+                           cpx<T>* fout, void* ws, hipStream_t s) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const bool m4 = P->d.method == DTMMX_METHOD_4X4;
+  const int nbeam = P->d.n_rays / P->d.rays_per_beam;
+  PixArgs<T> a{};
+  a.HW = HW;
+  a.nlam = P->d.n_wavelengths;
+  a.bm = P->bm;
+  const dim3 grid(unsigned((HW + NT - 1) / NT), unsigned(nbeam));
+  cpx<T>* f = m4 ? fout : (cpx<T>*)((char*)ws + ws_layout(P).e2);
+  if (m4) {
+    const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
+    HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+  } else {
+    a.src = fin;
+    a.field = f;
+    a.emed = mk<double>(S->n_in * S->n_in);
+    hipLaunchKernelGGL((pixel_kernel<T, PIX_IN2>), grid, dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+  }
+  a.field = f;
+  a.comp_stride = size_t(S->n_layers) * HW;
+  for (int l = 0; l < S->n_layers; ++l) {
+    a.eps_re = S->eps_re + size_t(l) * HW;
+    a.eps_im = S->eps_im ? S->eps_im + size_t(l) * HW : nullptr;
+    a.ang = S->angles + size_t(l) * HW;
+    a.d = S->thickness[l];
+    if (m4)
+      hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER4>), grid, dim3(NT), 0, s, a);
+    else
+      hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER2>), grid, dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+  }
+  if (!m4) {
+    a.src = f;
+    a.dst = fout;
+    a.emed = mk<double>(S->n_out * S->n_out);
+    hipLaunchKernelGGL((pixel_kernel<T, PIX_OUT2>), grid, dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+  }
+  return DTMMX_OK;
+}
+
 template <typename T>
 static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
                          void* ws, int* status, hipStream_t s) {
   const size_t HW = size_t(P->d.height) * P->d.width;
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
+  if (P->d.diffraction == 0)
+    return transfer_nodiff<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s);
   if (P->d.npass > 1) return transfer_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s);
   HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
   return propagate<T>(P, S, (cpx<T>*)fout, ws, status, s, 1);
@@ -967,7 +1020,7 @@ static size_t total_steps(const dtmmx_stack* S) {
 }
 
 static bool adjoint_supported(const dtmmx_plan* P) {
-  return P->d.method == DTMMX_METHOD_4X4 && P->wn <= 1 && P->d.npass == 1;
+  return P->d.method == DTMMX_METHOD_4X4 && P->wn <= 1 && P->d.npass == 1 && P->d.diffraction == 1;
 }
 
 // Forward 4x4 transfer (diffraction 1, npass 1) that records the row-pass
@@ -1440,7 +1493,15 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->precision != DTMMX_C64 && d->precision != DTMMX_C128)
     return fail(DTMMX_E_ARG, "precision must be DTMMX_C64 or DTMMX_C128");
   if (!d->k0 || !d->beta || !d->phi) return fail(DTMMX_E_ARG, "null k0/beta/phi");
-  if (d->diffraction < 1) return fail(DTMMX_E_UNSUPPORTED, "diffraction must be >= 1");
+  // diffraction 0: the per-pixel modes (4x4 reflection 2, 2x2 reflection 0),
+  // one pass
+  const bool nodiff = d->diffraction == 0 && d->npass == 1 &&
+                      ((d->method == DTMMX_METHOD_4X4 && d->reflection == 2) ||
+                       (d->method == DTMMX_METHOD_2X2 && d->reflection == 0));
+  if (d->diffraction < 1 && !nodiff)
+    return fail(DTMMX_E_UNSUPPORTED,
+                "diffraction must be >= 1, or 0 with npass 1 for the 4x4 method in reflection "
+                "mode 2 and the 2x2 method in reflection mode 0");
   if (d->diffraction > 1 && d->method != DTMMX_METHOD_4X4)
     return fail(DTMMX_E_UNSUPPORTED, "diffraction > 1 is built for the 4x4 method only");
   if (d->diffraction > 64) return fail(DTMMX_E_ARG, "diffraction above 64");
@@ -1450,8 +1511,9 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->npass > 1 && d->norm != 0 && d->norm != DTMMX_NORM_FFT)
     return fail(DTMMX_E_UNSUPPORTED, "npass > 1: only norm 0 (total) and 1 (fft) are built");
   if (d->checkpoint < 0) return fail(DTMMX_E_ARG, "checkpoint interval must be >= 0");
-  if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1)
-    return fail(DTMMX_E_UNSUPPORTED, "the 4x4 method is built for reflection mode 1");
+  if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !nodiff)
+    return fail(DTMMX_E_UNSUPPORTED,
+                "the 4x4 method is built for reflection mode 1, and mode 2 without diffraction");
   if (d->method == DTMMX_METHOD_2X2 && d->reflection != 0 && d->reflection != 1)
     return fail(DTMMX_E_UNSUPPORTED, "the 2x2 method is built for reflection modes 0 and 1");
   return DTMMX_OK;

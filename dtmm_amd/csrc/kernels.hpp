@@ -1448,4 +1448,168 @@ __global__ __launch_bounds__(NT) void e2h_kernel(E2HArgs<T> A) {
   o[3 * HW] = cast<T>(G[1][0] * ex + G[1][1] * ey);
 }
 
+
+// ---------------------------------------------------------------------------
+// Modes without diffraction (diffraction = 0): the field stays in real space
+// and every pixel is an independent 1D stack.  One thread per (pixel, beam)
+// solves the layer in fp64 once and applies it to every ray of the beam and
+// every wavelength, reading and writing the field in place.
+//   PIX_LAYER4: 4x4 reflection mode 2, F diag(exp(i k0 d alpha)) F^-1 on
+//               (Ex, Hy, Ey, Hx) (dtmm/propagate_4x4.py:18-51, dmatpn None);
+//   PIX_LAYER2: 2x2 reflection mode 0, E diag(p) E^-1 of the forward modes
+//               on (Ex, Ey) (dtmm/propagate_2x2.py:91-190, dmat None);
+//   PIX_IN2:    forward part of the 4-component input in the isotropic input
+//               medium, its E components to the 2-component field
+//               (dtmm/transfer.py:94-101);
+//   PIX_OUT2:   H of the forward modes of the isotropic output medium from E
+//               (dtmm/tmm.py:1051-1072), the 4-component output.
+// A layer's sub-steps are one application with the full thickness:
+// (F P F^-1)^n = F P^n F^-1.
+// ---------------------------------------------------------------------------
+
+enum { PIX_LAYER4 = 0, PIX_LAYER2 = 1, PIX_IN2 = 2, PIX_OUT2 = 3 };
+
+template <typename T>
+struct PixArgs {
+  cpx<T>* field;        // LAYER: [ray][lam][NC][H*W] in place; IN2: output E field
+  const cpx<T>* src;    // IN2: 4-component input; OUT2: E field
+  cpx<T>* dst;          // OUT2: 4-component output
+  size_t HW;
+  int nlam;
+  // layer material: [3][L*H*W] with this layer's offset applied
+  const double* eps_re;
+  const double* eps_im;
+  const double* ang;
+  size_t comp_stride;
+  double d;             // full layer thickness
+  cd emed;              // IN2 / OUT2: isotropic medium eps
+  Beams bm;
+};
+
+template <typename T, int PK>
+__global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
+  const size_t px = size_t(blockIdx.x) * NT + threadIdx.x;
+  const int beam = blockIdx.y;
+  if (px >= A.HW) return;
+  const double beta = A.bm.beta[beam], phi0 = A.bm.phi[beam];
+  double s0, c0;
+  sincos(phi0, &s0, &c0);
+  const int r0 = beam * A.bm.rays_per_beam, r1 = r0 + A.bm.rays_per_beam;
+  constexpr int NC = (PK == PIX_LAYER4) ? 4 : 2;
+  auto plane = [&](cpx<T>* base, int ray, int lam, int nc) {
+    return base + (size_t(ray) * A.nlam + lam) * nc * A.HW + px;
+  };
+  if constexpr (PK == PIX_LAYER4 || PK == PIX_LAYER2) {
+    const size_t o = px;
+    const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+    const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
+    const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
+                             A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
+    const double psi = A.ang[o], theta = A.ang[A.comp_stride + o], phi = A.ang[2 * A.comp_stride + o];
+    if constexpr (PK == PIX_LAYER4) {
+      Modes m;
+      int kind;
+      layer_modes(beta, c0, s0, phi0, e0, e1, e2, psi, theta, phi, m, kind);
+      cd fi[4][4];
+      inv4(m.F, fi);  // singular: zeros, as dtmm/linalg.py:510-513
+      cpx<T> F[4][4], Fi[4][4], a[4];
+      for (int i = 0; i < 4; ++i) {
+        a[i] = cast<T>(m.a[i]);
+        for (int j = 0; j < 4; ++j) {
+          F[i][j] = cast<T>(m.F[i][j]);
+          Fi[i][j] = cast<T>(fi[i][j]);
+        }
+      }
+      for (int lam = 0; lam < A.nlam; ++lam) {
+        const T kd = T(A.bm.k0[lam] * A.d);
+        cpx<T> p[4];
+        for (int i = 0; i < 4; ++i) p[i] = cexpi_layer(a[i] * kd);
+        for (int ray = r0; ray < r1; ++ray) {
+          cpx<T>* f = plane(A.field, ray, lam, 4);
+          cpx<T> v[4], u[4];
+          for (int c = 0; c < 4; ++c) v[c] = f[c * A.HW];
+          for (int i = 0; i < 4; ++i) {
+            cpx<T> t = Fi[i][0] * v[0];
+            for (int c = 1; c < 4; ++c) t = t + Fi[i][c] * v[c];
+            u[i] = t * p[i];
+          }
+          for (int i = 0; i < 4; ++i) {
+            cpx<T> t = F[i][0] * u[0];
+            for (int c = 1; c < 4; ++c) t = t + F[i][c] * u[c];
+            f[i * A.HW] = t;
+          }
+        }
+      }
+    } else {
+      RopData r;
+      rop_full(beta, c0, s0, phi0, e0, e1, e2, psi, theta, phi, r);
+      const T ok = r.ok2 ? T(1) : T(0);
+      const cpx<T> a0 = cast<T>(r.a[0]), a2 = cast<T>(r.a[2]);
+      cpx<T> P[2][2];
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) P[i][j] = cast<T>(r.P0[i][j]);
+      for (int lam = 0; lam < A.nlam; ++lam) {
+        const T kd = T(A.bm.k0[lam] * A.d);
+        const cpx<T> p0 = cexpi_layer(a0 * kd);
+        cpx<T> p2 = cexpi_layer(a2 * kd);
+        const cpx<T> dp = (p0 - p2) * ok;
+        p2 = p2 * ok;
+        const cpx<T> m00 = p2 + dp * P[0][0], m01 = dp * P[0][1], m10 = dp * P[1][0],
+                     m11 = p2 + dp * P[1][1];
+        for (int ray = r0; ray < r1; ++ray) {
+          cpx<T>* f = plane(A.field, ray, lam, 2);
+          const cpx<T> ex = f[0], ey = f[A.HW];
+          f[0] = m00 * ex + m01 * ey;
+          f[A.HW] = m10 * ex + m11 * ey;
+        }
+      }
+    }
+  } else {
+    // isotropic medium of the beam: F and its inverse (fp64)
+    Modes m;
+    int kind;
+    layer_modes(beta, c0, s0, phi0, A.emed, A.emed, A.emed, 0., 0., 0., m, kind);
+    if constexpr (PK == PIX_IN2) {
+      cd fi[4][4];
+      inv4(m.F, fi);
+      // rows Ex, Ey of F diag(1, 0, 1, 0) F^-1
+      cpx<T> Pm[2][4];
+      for (int i = 0; i < 2; ++i)
+        for (int c = 0; c < 4; ++c)
+          Pm[i][c] = cast<T>(m.F[2 * i][0] * fi[0][c] + m.F[2 * i][2] * fi[2][c]);
+      for (int lam = 0; lam < A.nlam; ++lam)
+        for (int ray = r0; ray < r1; ++ray) {
+          const cpx<T>* s = plane(const_cast<cpx<T>*>(A.src), ray, lam, 4);
+          cpx<T> v[4];
+          for (int c = 0; c < 4; ++c) v[c] = s[c * A.HW];
+          cpx<T>* f = plane(A.field, ray, lam, 2);
+          for (int i = 0; i < 2; ++i) {
+            cpx<T> t = Pm[i][0] * v[0];
+            for (int c = 1; c < 4; ++c) t = t + Pm[i][c] * v[c];
+            f[i * A.HW] = t;
+          }
+        }
+    } else {
+      // E2H = B A^-1 with A, B the E and H rows of the forward columns
+      const cd Am[2][2] = {{m.F[0][0], m.F[0][2]}, {m.F[2][0], m.F[2][2]}};
+      cd Ai[2][2];
+      inv2(Am, Ai);
+      cpx<T> G[2][2];
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+          G[i][j] = cast<T>(m.F[2 * i + 1][0] * Ai[0][j] + m.F[2 * i + 1][2] * Ai[1][j]);
+      for (int lam = 0; lam < A.nlam; ++lam)
+        for (int ray = r0; ray < r1; ++ray) {
+          const cpx<T>* e = plane(const_cast<cpx<T>*>(A.src), ray, lam, 2);
+          const cpx<T> ex = e[0], ey = e[A.HW];
+          cpx<T>* o = plane(A.dst, ray, lam, 4);
+          o[0] = ex;
+          o[A.HW] = G[0][0] * ex + G[0][1] * ey;
+          o[2 * A.HW] = ey;
+          o[3 * A.HW] = G[1][0] * ex + G[1][1] * ey;
+        }
+    }
+  }
+}
+
 }  // namespace dtmmx

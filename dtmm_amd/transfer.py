@@ -276,16 +276,31 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
         raise NotImplementedError("ret_bulk is not supported yet")
     if eff_data not in (None, 0, "isotropic"):
         raise NotImplementedError("only isotropic effective data (eff_data=0) is supported")
+    if diffraction == 0:
+        # no diffraction: every pixel is an independent stack (4x4 reflection
+        # mode 2, the reference's default for diffraction 0; 2x2 reflection 0)
+        if npass != 1:
+            raise NotImplementedError("diffraction=0 is supported with npass=1")
+        if kind == "4x4" and reflection != 2:
+            raise NotImplementedError("4x4 method with diffraction=0: only reflection mode 2 "
+                                      "is supported")
+        if kind == "2x2" and reflection != 0:
+            raise NotImplementedError("2x2 method with diffraction=0: only reflection mode 0 "
+                                      "is supported")
+        return kind, reflection
     if kind == "4x4":
         if reflection != 1:
-            raise NotImplementedError("4x4 method: only reflection mode 1 is supported")
+            raise NotImplementedError("4x4 method: reflection mode 2 is supported with "
+                                      "diffraction=0 only; otherwise reflection mode 1")
         if not (diffraction >= 1) or diffraction == np.inf:
-            raise NotImplementedError("4x4 method: only finite diffraction >= 1 is supported")
+            raise NotImplementedError("4x4 method: only diffraction 0 or finite diffraction >= 1 "
+                                      "is supported")
     else:
         if reflection not in (0, 1):
             raise NotImplementedError("2x2 method: reflection mode 2 is not supported")
         if diffraction != 1:
-            raise NotImplementedError("2x2 method: only diffraction=1 is supported")
+            raise NotImplementedError("2x2 method: only diffraction 0 (reflection 0) or 1 is "
+                                      "supported")
     return kind, reflection
 
 

@@ -140,6 +140,11 @@ int dtmmx_plan_get_tiles(const dtmmx_plan* plan, int* col_lines, int* row_rays);
  *   4x4 (reflection mode 1, diffraction >= 1, npass >= 1): replaces
  *       transfer.py:694-875 with propagate_4x4.py:376-444 and :80-101 per
  *       layer.
+ *   4x4 (reflection mode 2, diffraction 0, npass 1) and 2x2 (reflection
+ *       mode 0, diffraction 0, npass 1): per-pixel stacks in real space,
+ *       replacing propagate_4x4.py:156-175 with :18-51 (dmatpn None) and
+ *       propagate_2x2.py:286-305 with :91-190 (dmat None), the 2x2 input
+ *       projection transfer.py:94-101 and output E2H_mat (tmm.py:1051-1072).
  *   2x2 (reflection mode 0 or 1, diffraction 1, npass 1): replaces
  *       transfer.py:1071-1176 with propagate_2x2.py:193-211 and :19-87 per
  *       interface.

@@ -12,9 +12,11 @@ Scope (what the GPU engine implements and this module checks):
 * ``transfer_4x4`` with reflection=1, diffraction >= 1, npass >= 1 (dtmm/transfer.py,
   dtmm/propagate_4x4.py); diffraction > 1 uses the tent windows of dtmm/mode.py;
   npass > 1 the normalisations of transfer.py and dtmm/denoise.py
-* ``transfer_2x2`` with reflection in {0, 1}, diffraction in {0, 1}, npass=1
-  (dtmm/transfer.py, dtmm/propagate_2x2.py); reflection 1 applies the
-  Fresnel interface transmission of dtmm/tmm.py tr_mat
+* ``transfer_4x4`` with reflection=2, diffraction=0, npass=1: per-pixel
+  f.P.f^-1 sub-steps in real space (dtmm/propagate_4x4.py _transfer_ray_4x4_2)
+* ``transfer_2x2`` with reflection in {0, 1} and diffraction 1, or reflection 0
+  and diffraction 0, npass=1 (dtmm/transfer.py, dtmm/propagate_2x2.py);
+  reflection 1 applies the Fresnel interface transmission of dtmm/tmm.py tr_mat
 * ``illumination_data`` and ``field2intensity``            (dtmm/field.py)
 
 Pinning of each function is done in ``tests/test_oracle_*.py`` (golden
@@ -888,6 +890,19 @@ def _transfer_ray_4x4_1(field, ks, layer, dmat1, dmat2, beta, phi, nsteps):
     return field
 
 
+def _transfer_ray_4x4_2(field, ks, layer, beta, phi, nsteps):
+    """Reflection mode 2 without diffraction (dtmm/propagate_4x4.py:18-51 with
+    dmatpn None, called from :156-175 for diffraction 0): every sub-step
+    applies f diag(exp(i k0 d alpha)) f^-1 to the real-space field of each
+    pixel."""
+    d, epsv, epsa = layer
+    alpha, f, fi = alphaffi(beta, phi, epsv, epsa)
+    p = phase_mat(alpha, (ks * d)[..., None, None])
+    for _ in range(nsteps):
+        field = dotmdmf(f, p, fi, field)
+    return field
+
+
 def propagate_4x4_effective_1(field, ks, layer, effective_layer, beta=0, phi=0, nsteps=1,
                               diffraction=1, betamax=BETAMAX, cache=None):
     """Layer step of the 4x4 method, reflection mode 1
@@ -1134,8 +1149,24 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     like the reference, the input array of field_data (when complex128)
     then receives the field at the input side, reflections included.
     """
+    if reflection == 2 and diffraction == 0 and npass == 1:
+        # dtmm/transfer.py:694-700, :782-795 (propagate_4x4_effective_2), :874:
+        # the input is neither projected nor transformed, input and output
+        # media are skipped
+        field_in, wavelengths, pixelsize = field_data
+        field = np.array(field_in, CDTYPE)
+        layers, _ = create_layers(optical_data, nin, nout, nstep)
+        ks = k0(wavelengths, pixelsize)
+        beta, phi = _validate_betaphi(beta, phi, extendeddim=field.ndim - 2)
+        for j in range(1, len(layers) - 1):
+            nsub, layer = layers[j]
+            field = _transfer_ray_4x4_2(field, ks, layer, beta, phi, nsub)
+            if layer_callback is not None:
+                layer_callback(j)
+        return field, wavelengths, pixelsize
     if reflection != 1:
-        raise NotImplementedError("oracle covers reflection=1 for the 4x4 method")
+        raise NotImplementedError("oracle covers reflection=1, and reflection=2 with "
+                                  "diffraction=0 and npass=1, for the 4x4 method")
     if norm & DTMM_NORM_REF:
         raise NotImplementedError("oracle covers norm without DTMM_NORM_REF")
     norm = "fft" if norm & DTMM_NORM_FFT else "total"
@@ -1202,8 +1233,8 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
 
 def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1, nstep=1,
                  diffraction=1, reflection=0, betamax=BETAMAX, layer_callback=None):
-    """2x2 (E-field) transfer, npass=1, reflection mode 0 or 1
-    (dtmm/transfer.py:1009-1183)."""
+    """2x2 (E-field) transfer, npass=1, reflection mode 0 or 1 with diffraction
+    1, reflection mode 0 with diffraction 0 (dtmm/transfer.py:1009-1183)."""
     if npass != 1 or reflection not in (0, 1):
         raise NotImplementedError("oracle covers npass=1, reflection 0 or 1 for the 2x2 method")
     field_in, wavelengths, pixelsize = field_data
@@ -1219,7 +1250,28 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
         field0 = _transmitted_field_direct(field_in, beta, phi, n=nin)
     field = field0[..., ::2, :, :].copy()
     if diffraction == 0:
-        raise NotImplementedError("oracle covers diffraction == 1 for the 2x2 method transfer")
+        if reflection != 0:
+            raise NotImplementedError("oracle covers diffraction == 0 with reflection 0 for the "
+                                      "2x2 method")
+        # dtmm/transfer.py:1093-1099 (work_in_fft False), propagate_2x2.py:91-190 with
+        # reflection 0 and dmat None: two half-step e.p.e^-1 per sub-step in real
+        # space; the output medium (zero thickness) only builds H from E
+        for j in range(n):
+            nsub, (d, epsv, epsa) = layers[j + 1]
+            alpha, fmat = alphaf(beta, phi, epsv, epsa)
+            if d != 0.:
+                e = E_mat(fmat, mode=+1)
+                ei = inv(e)
+                p = phase_mat(alpha, (ks * d / 2)[..., None, None], mode=+1)
+                for _ in range(nsub):
+                    field = dotmdmf(e, p, ei, field)
+                    field = dotmdmf(e, p, ei, field)
+            if j + 1 == n:
+                field_out[..., 1::2, :, :] += dotmf(E2H_mat(fmat, mode=+1), field)
+                field_out[..., ::2, :, :] += field
+            if layer_callback is not None:
+                layer_callback(j)
+        return field_out, wavelengths, pixelsize
     field = fft2(field)
     cache = _Cache()
     for j in range(n):
@@ -1242,7 +1294,7 @@ def _transmitted_field_direct(field, beta, phi, n=1.):
     diag = np.zeros(fmat.shape[:-1], CDTYPE)
     diag[..., 0::2] = 1.
     pmat = dotmdm(fmat, diag, inv(fmat))
-    return _v2f(np.matmul(pmat[..., None, None, :, :], _f2v(field)))
+    return _v2f(np.matmul(pmat, _f2v(field)))
 
 
 def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,

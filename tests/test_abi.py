@@ -328,3 +328,19 @@ def test_host_rop_grad_matches_finite_differences():
         (a_p, p_p), (a_m, p_m) = rop(ap), rop(am)
         np.testing.assert_allclose(da[:, k], (a_p - a_m) / (2 * h), atol=2e-7)
         np.testing.assert_allclose(dp[:, k], (p_p - p_m) / (2 * h), atol=2e-7)
+
+
+def test_mode_resolution_no_diffraction():
+    """diffraction=0 selects reflection mode 2 for the 4x4 method
+    (dtmm/transfer.py:460-461) and keeps mode 0 for an explicit 2x2 method;
+    other combinations with diffraction 0 are refused before any GPU call."""
+    from dtmm_amd.transfer import _resolve_modes
+    assert _resolve_modes("4x4", 1, 0, None, None, False) == ("4x4", 2)
+    assert _resolve_modes("2x2", 1, 0, None, None, False) == ("2x2", 0)
+    assert _resolve_modes("4x4", 1, False, None, None, False) == ("4x4", 2)
+    with pytest.raises(NotImplementedError):
+        _resolve_modes('"2x2"', 1, 0, None, None, False)   # configured default: reflection 1
+    with pytest.raises(NotImplementedError):
+        _resolve_modes("4x4", 1, 0, 1, None, False)
+    with pytest.raises(NotImplementedError):
+        _resolve_modes("4x4", 1, 1, 2, None, False)

@@ -527,3 +527,33 @@ def test_launch_shapes_do_not_change_results_and_tuner_picks_one(method):
                             method=method, precision="c64", tune=True)[0]
     ref = oracle.transfer_field(fd, optical, beta=0., phi=0., method=method)[0]
     assert rel_err(out, ref) < TOL["c64"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+def test_no_diffraction(method, precision):
+    """diffraction=0: 4x4 in reflection mode 2 (the reference's choice for
+    diffraction 0) and 2x2 in reflection mode 0, per-pixel stacks in real
+    space; two beams, sub-steps and different input / output media."""
+    flush("case no diffraction %s %s" % (method, precision))
+    beta = [0., 0.2]
+    phi = [0., 1.0]
+    fd, optical = random_case(shape=(6, 32, 48), wavelengths=(450., 650.), jones=None,
+                              beta=beta, phi=phi)
+    _compare(fd, optical, method, precision, beta=beta, phi=phi, diffraction=0, nstep=2,
+             nin=1.0, nout=1.33)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+def test_no_diffraction_biaxial_absorbing(method):
+    flush("case no diffraction biaxial absorbing %s" % method)
+    fd, optical = random_case(shape=(4, 24, 32), wavelengths=(550.,), jones=(1, 1j), beta=0.15,
+                              phi=0.4)
+    d, eps, ang = optical[0]
+    eps = np.array(eps, complex)
+    eps[..., 1] = eps[..., 1] * 1.05 + 0.01j
+    ang = np.array(ang)
+    ang[..., 0] = 0.3
+    _compare(fd, [(d, eps, ang)], method, "c128", beta=0.15, phi=0.4, diffraction=0)

@@ -236,3 +236,125 @@ def test_polarizer4x4_malus():
         np.testing.assert_allclose(inten, n * np.cos(a) ** 2, atol=1e-14)
     m = polarizer4x4((1, 1), n)
     np.testing.assert_allclose(m @ m, m, atol=1e-14)
+
+
+def test_reflection2_no_diffraction_isotropic_slab_is_characteristic_matrix():
+    """4x4, reflection 2, diffraction 0: every pixel of a homogeneous
+    isotropic slab at normal incidence is multiplied by the textbook
+    characteristic matrix of (Ex, Hy): [[cos nd, i sin nd / n],
+    [i n sin nd, cos nd]] (nd = k0 L n), and of (Ey, Hx) with the opposite
+    sign of the off-diagonal terms.  The input is arbitrary (not a mode of
+    the input medium), so reflections at both faces are part of the result."""
+    n, L = 1.7, 5
+    shape = (4, 6)
+    wl = np.array([450., 600.])
+    ps = 100.
+    rng = np.random.default_rng(3)
+    field = rng.normal(size=(2, 4) + shape) + 1j * rng.normal(size=(2, 4) + shape)
+    eps = np.broadcast_to([n ** 2] * 3, (L, 1, 1, 3))
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                           beta=0., phi=0., method="4x4", diffraction=0, nin=1.0, nout=1.3,
+                           nstep=2)[0]
+    k = o.k0(wl, ps)
+    for i in range(2):
+        d = k[i] * L * n
+        c, s = np.cos(d), np.sin(d)
+        Mx = np.array([[c, 1j * s / n], [1j * n * s, c]])
+        My = np.array([[c, -1j * s / n], [-1j * n * s, c]])
+        ex = np.einsum("ab,byx->ayx", Mx, field[i, [0, 1]])
+        ey = np.einsum("ab,byx->ayx", My, field[i, [2, 3]])
+        np.testing.assert_allclose(out[i, [0, 1]], ex, atol=1e-12)
+        np.testing.assert_allclose(out[i, [2, 3]], ey, atol=1e-12)
+
+
+def test_reflection2_no_diffraction_is_product_of_berreman_exponentials():
+    """4x4, reflection 2, diffraction 0 through a stack of homogeneous layers
+    (tilted uniaxial, biaxial, isotropic) at oblique incidence: each pixel
+    equals expm(i k0 d Delta) of the layers' Berreman matrices applied in
+    order (scipy expm, not the eigen-decomposition), input and output media
+    skipped."""
+    from scipy.linalg import expm
+    shape = (3, 5)
+    wl = np.array([520.])
+    ps = 100.
+    beta, phi = 0.3, 0.8
+    d = np.array([2., 1., 3.])
+    eps = np.array([[2.2, 2.2, 2.9], [2.1, 2.4, 2.8], [2.5, 2.5, 2.5]])
+    ang = np.array([[0., 0.7, -1.1], [0.4, 1.2, 0.3], [0., 0., 0.]])
+    rng = np.random.default_rng(5)
+    field = rng.normal(size=(1, 4) + shape) + 1j * rng.normal(size=(1, 4) + shape)
+    block = (d, np.broadcast_to(eps[:, None, None], (3, 1, 1, 3)),
+             np.broadcast_to(ang[:, None, None], (3, 1, 1, 3)))
+    out = o.transfer_field((field, wl, ps), [block], beta=beta, phi=phi, method="4x4",
+                           diffraction=0, nin=1.2, nout=1.4)[0]
+    k = o.k0(wl, ps)[0]
+
+    def rz(a):
+        return np.array([[np.cos(a), -np.sin(a), 0.], [np.sin(a), np.cos(a), 0.], [0., 0., 1.]])
+
+    def ry(a):
+        return np.array([[np.cos(a), 0., np.sin(a)], [0., 1., 0.], [-np.sin(a), 0., np.cos(a)]])
+
+    M = np.eye(4, dtype=complex)
+    for j in range(3):
+        # principal frame (psi, theta, phi Euler angles) in the frame whose x
+        # axis lies in the plane of incidence
+        R = rz(ang[j, 2] - phi) @ ry(ang[j, 1]) @ rz(ang[j, 0])
+        et = R @ np.diag(eps[j]) @ R.T
+        e6 = np.array([et[0, 0], et[1, 1], et[2, 2], et[0, 1], et[0, 2], et[1, 2]])
+        D = o.auxiliary_matrix(beta, e6)
+        # the beam-azimuth frame (x along the beam plane) back to the lab frame
+        c, s = np.cos(phi), np.sin(phi)
+        T = np.array([[c, 0, -s, 0], [0, c, 0, s], [s, 0, c, 0], [0, -s, 0, c]])
+        M = T @ expm(1j * k * d[j] * D) @ np.linalg.inv(T) @ M
+    expect = np.einsum("ab,byx->ayx", M, field[0])
+    np.testing.assert_allclose(out[0], expect, atol=1e-11)
+
+
+def test_2x2_no_diffraction_uniaxial_slab_is_jones_retarder():
+    """2x2, reflection 0, diffraction 0 (the 2x2 method's real-space path):
+    a homogeneous planar-uniaxial slab at normal incidence acts on every
+    pixel's forward E field as a Jones retarder; the output H is the
+    output medium's n x E."""
+    no, ne, L, phid, nout = 1.5, 1.62, 7, 0.6, 1.33
+    shape = (4, 6)
+    wl = np.array([480., 620.])
+    ps = 100.
+    rng = np.random.default_rng(2)
+    e_in = rng.normal(size=(2, 2) + shape) + 1j * rng.normal(size=(2, 2) + shape)
+    field = np.zeros((2, 4) + shape, complex)
+    field[:, 0], field[:, 2] = e_in[:, 0], e_in[:, 1]
+    field[:, 1], field[:, 3] = 1.5 * e_in[:, 0], -1.5 * e_in[:, 1]   # forward in nin = 1.5
+    eps = np.broadcast_to([no ** 2, no ** 2, ne ** 2], (L, 1, 1, 3))
+    ang = np.broadcast_to([0., np.pi / 2, phid], (L, 1, 1, 3))
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, ang)], beta=0., phi=0.,
+                           method="2x2", diffraction=0, nin=1.5, nout=nout)[0]
+    k = o.k0(wl, ps)
+    c, s = np.cos(phid), np.sin(phid)
+    R = np.array([[c, -s], [s, c]])
+    for i in range(2):
+        J = R @ np.diag([np.exp(1j * k[i] * L * ne), np.exp(1j * k[i] * L * no)]) @ R.T
+        e = np.einsum("ab,byx->ayx", J, e_in[i])
+        np.testing.assert_allclose(out[i, 0], e[0], atol=1e-12)
+        np.testing.assert_allclose(out[i, 2], e[1], atol=1e-12)
+        np.testing.assert_allclose(out[i, 1], nout * e[0], atol=1e-12)
+        np.testing.assert_allclose(out[i, 3], -nout * e[1], atol=1e-12)
+
+
+def test_2x2_no_diffraction_projects_input_onto_forward_modes():
+    """The 2x2 real-space path first keeps the forward part of the input in
+    the input medium (dtmm/transfer.py:94-101): at normal incidence in index
+    n that is Ex_f = (Ex + Hy/n)/2, Ey_f = (Ey - Hx/n)/2.  With an isotropic
+    slab of the same index the output E is that part times exp(i k0 L n)."""
+    n, L = 1.45, 4
+    shape = (3, 5)
+    wl = np.array([550.])
+    ps = 100.
+    rng = np.random.default_rng(8)
+    field = rng.normal(size=(1, 4) + shape) + 1j * rng.normal(size=(1, 4) + shape)
+    eps = np.broadcast_to([n ** 2] * 3, (L, 1, 1, 3))
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                           beta=0., phi=0., method="2x2", diffraction=0, nin=n, nout=n)[0]
+    ph = np.exp(1j * o.k0(wl, ps)[0] * L * n)
+    np.testing.assert_allclose(out[0, 0], ph * (field[0, 0] + field[0, 1] / n) / 2, atol=1e-12)
+    np.testing.assert_allclose(out[0, 2], ph * (field[0, 2] - field[0, 3] / n) / 2, atol=1e-12)
