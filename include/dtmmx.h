@@ -24,9 +24,9 @@
  *   - Every call returns 0 on success or a negative DTMMX_E* code; the
  *     text of the last error of the calling thread is available from
  *     dtmmx_last_error().  Calls are asynchronous on the given stream.
- *   - A plan is immutable after creation and may be used from several
- *     threads on different streams, provided each call gets its own
- *     workspace.
+ *   - A plan is immutable once its launch shapes are set
+ *     (dtmmx_plan_set_tiles) and may be used from several threads on
+ *     different streams, provided each call gets its own workspace.
  */
 #ifndef DTMMX_H
 #define DTMMX_H
@@ -68,7 +68,9 @@ typedef struct {
   const double* k0;   /* [n_wavelengths] host: 2 pi pixelsize / lambda  */
   const double* beta; /* [n_rays / rays_per_beam] host                  */
   const double* phi;  /* [n_rays / rays_per_beam] host                  */
-  int diffraction;    /* 1: one mean beam per ray (propagate_4x4.py:382);
+  int diffraction;    /* 0 (npass 1; 4x4 reflection 2, 2x2 reflection 0):
+                         no diffraction, per-pixel stacks in real space;
+                         1: one mean beam per ray (propagate_4x4.py:382);
                          n > 1 (4x4 only): the k-space field is split by
                          the non-zero n x n tent windows of dtmm/mode.py,
                          each propagated with its window's mean beam
@@ -81,7 +83,8 @@ typedef struct {
                          transfer.py:206-221); DTMM_NORM_REF is not built */
   double smooth;      /* npass > 1: notch width of dtmm/denoise.py (dtmm.ini:8,
                          0.1); negative values count as 0                 */
-  int reflection;     /* 4x4: 1.  2x2: 0, or 1 = Fresnel transmission at
+  int reflection;     /* 4x4: 1, or 2 with diffraction 0.  2x2: 0, or 1 =
+                         Fresnel transmission at
                          interfaces between different effective media
                          (propagate_2x2.py:28-30, diffract.py:215-225)    */
   int checkpoint;     /* adjoint tape (no counterpart in the reference):
