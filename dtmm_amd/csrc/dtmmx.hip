@@ -977,6 +977,54 @@ static int transfer_nodiff(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<
   return DTMMX_OK;
 }
 
+// 4x4 reflection mode 2 with diffraction 1, one pass (dtmm/transfer.py
+// :694-700 and :782-795 with work_in_fft False; propagate_4x4.py:156-175 and
+// :18-51): the field stays in real space; each sub-step is e P/2 F^-1 per
+// pixel, a 2D transform, the diagonal k-space operator
+// exp(+-i k0 d (a(k) - a_c)) of corrected_Epn_diffraction_matrix
+// (matrix.py:38-43, :62-68) on the forward / backward E parts, the inverse
+// transform, and F P/2 e^-1 per pixel.
+template <typename T>
+static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
+                          cpx<T>* fout, hipStream_t s) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const int nbeam = P->d.n_rays / P->d.rays_per_beam;
+  const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
+  HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+  PixArgs<T> a{};
+  a.field = fout;
+  a.HW = HW;
+  a.nlam = P->d.n_wavelengths;
+  a.bm = P->bm;
+  a.comp_stride = size_t(S->n_layers) * HW;
+  const dim3 grid(unsigned((HW + NT - 1) / NT), unsigned(nbeam));
+  const std::vector<Step> st = expand_steps(S, 1);
+  const Batch b{P->d.n_rays, P->d.n_wavelengths, 0, P->d.n_wavelengths};
+  for (const Step& q : st) {
+    a.eps_re = S->eps_re + size_t(q.l) * HW;
+    a.eps_im = S->eps_im ? S->eps_im + size_t(q.l) * HW : nullptr;
+    a.ang = S->angles + size_t(q.l) * HW;
+    a.d = 0.5 * q.d;
+    hipLaunchKernelGGL((pixel_kernel<T, PIX_A4>), grid, dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    TRY(row_pass<T>(P, fout, b, 4, OP_FWD, s));
+    ColArgs<T> ca{};
+    ca.da = q.d;
+    ca.db = q.d;
+    ca.ea = q.e;
+    ca.eb = q.e;
+    ca.kca = 1.;
+    ca.kcb = 1.;
+    ca.scale = 1.0 / double(HW);
+    ca.dense_sign = 1;
+    TRY(col_pass<T>(P, fout, b, 4, OP_DIAG, s, &ca));
+    TRY(row_pass<T>(P, fout, b, 4, OP_INV, s));
+    hipLaunchKernelGGL((pixel_kernel<T, PIX_B4>), grid, dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+  }
+  return DTMMX_OK;
+}
+
 template <typename T>
 static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
                          void* ws, int* status, hipStream_t s) {
@@ -984,6 +1032,8 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, v
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
   if (P->d.diffraction == 0)
     return transfer_nodiff<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s);
+  if (P->d.method == DTMMX_METHOD_4X4 && P->d.reflection == 2)
+    return transfer_refl2<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, s);
   if (P->d.npass > 1) return transfer_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s);
   HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
   return propagate<T>(P, S, (cpx<T>*)fout, ws, status, s, 1);
@@ -1020,7 +1070,8 @@ static size_t total_steps(const dtmmx_stack* S) {
 }
 
 static bool adjoint_supported(const dtmmx_plan* P) {
-  return P->d.method == DTMMX_METHOD_4X4 && P->wn <= 1 && P->d.npass == 1 && P->d.diffraction == 1;
+  return P->d.method == DTMMX_METHOD_4X4 && P->wn <= 1 && P->d.npass == 1 && P->d.diffraction == 1 &&
+         P->d.reflection == 1;
 }
 
 // Forward 4x4 transfer (diffraction 1, npass 1) that records the row-pass
@@ -1498,6 +1549,9 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   const bool nodiff = d->diffraction == 0 && d->npass == 1 &&
                       ((d->method == DTMMX_METHOD_4X4 && d->reflection == 2) ||
                        (d->method == DTMMX_METHOD_2X2 && d->reflection == 0));
+  // 4x4 reflection mode 2 with one mean beam, one pass
+  const bool refl2 = d->method == DTMMX_METHOD_4X4 && d->reflection == 2 && d->npass == 1 &&
+                     d->diffraction <= 1;
   if (d->diffraction < 1 && !nodiff)
     return fail(DTMMX_E_UNSUPPORTED,
                 "diffraction must be >= 1, or 0 with npass 1 for the 4x4 method in reflection "
@@ -1511,9 +1565,10 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->npass > 1 && d->norm != 0 && d->norm != DTMMX_NORM_FFT)
     return fail(DTMMX_E_UNSUPPORTED, "npass > 1: only norm 0 (total) and 1 (fft) are built");
   if (d->checkpoint < 0) return fail(DTMMX_E_ARG, "checkpoint interval must be >= 0");
-  if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !nodiff)
+  if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !refl2)
     return fail(DTMMX_E_UNSUPPORTED,
-                "the 4x4 method is built for reflection mode 1, and mode 2 without diffraction");
+                "the 4x4 method is built for reflection mode 1, and mode 2 with diffraction 0 or 1 "
+                "and npass 1");
   if (d->method == DTMMX_METHOD_2X2 && d->reflection != 0 && d->reflection != 1)
     return fail(DTMMX_E_UNSUPPORTED, "the 2x2 method is built for reflection modes 0 and 1");
   return DTMMX_OK;

@@ -1462,12 +1462,19 @@ __global__ __launch_bounds__(NT) void e2h_kernel(E2HArgs<T> A) {
 //               medium, its E components to the 2-component field
 //               (dtmm/transfer.py:94-101);
 //   PIX_OUT2:   H of the forward modes of the isotropic output medium from E
-//               (dtmm/tmm.py:1051-1072), the 4-component output.
+//               (dtmm/tmm.py:1051-1072), the 4-component output;
+//   PIX_A4:     4x4 reflection mode 2 with diffraction, first half of a
+//               sub-step: e diag(exp(i k0 d alpha)) F^-1 (d = half the
+//               sub-step) maps (Ex, Hy, Ey, Hx) to the E parts of the
+//               forward and backward modes (E+x, E-x, E+y, E-y)
+//               (dtmm/propagate_4x4.py:45);
+//   PIX_B4:     the second half, F diag(exp(i k0 d alpha)) e^-1 back to
+//               (Ex, Hy, Ey, Hx) (dtmm/propagate_4x4.py:48).
 // A layer's sub-steps are one application with the full thickness:
 // (F P F^-1)^n = F P^n F^-1.
 // ---------------------------------------------------------------------------
 
-enum { PIX_LAYER4 = 0, PIX_LAYER2 = 1, PIX_IN2 = 2, PIX_OUT2 = 3 };
+enum { PIX_LAYER4 = 0, PIX_LAYER2 = 1, PIX_IN2 = 2, PIX_OUT2 = 3, PIX_A4 = 4, PIX_B4 = 5 };
 
 template <typename T>
 struct PixArgs {
@@ -1495,11 +1502,75 @@ __global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
   double s0, c0;
   sincos(phi0, &s0, &c0);
   const int r0 = beam * A.bm.rays_per_beam, r1 = r0 + A.bm.rays_per_beam;
-  constexpr int NC = (PK == PIX_LAYER4) ? 4 : 2;
   auto plane = [&](cpx<T>* base, int ray, int lam, int nc) {
     return base + (size_t(ray) * A.nlam + lam) * nc * A.HW + px;
   };
-  if constexpr (PK == PIX_LAYER4 || PK == PIX_LAYER2) {
+  if constexpr (PK == PIX_A4 || PK == PIX_B4) {
+    const size_t o = px;
+    const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+    const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
+    const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
+                             A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
+    Modes m;
+    int kind;
+    layer_modes(beta, c0, s0, phi0, e0, e1, e2, A.ang[o], A.ang[A.comp_stride + o],
+                A.ang[2 * A.comp_stride + o], m, kind);
+    cpx<T> F[4][4], a[4];
+    for (int i = 0; i < 4; ++i) {
+      a[i] = cast<T>(m.a[i]);
+      for (int j = 0; j < 4; ++j) F[i][j] = cast<T>(m.F[i][j]);
+    }
+    // PIX_A4: F^-1.  PIX_B4: inverses of the forward and backward E blocks
+    // (the 4x4 inverse of e, all zero when singular as dtmm/linalg.py:510-513)
+    cpx<T> G[4][4];
+    if constexpr (PK == PIX_A4) {
+      cd fi[4][4];
+      inv4(m.F, fi);
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) G[i][j] = cast<T>(fi[i][j]);
+    } else {
+      const cd ef[2][2] = {{m.F[0][0], m.F[0][2]}, {m.F[2][0], m.F[2][2]}};
+      const cd eb[2][2] = {{m.F[0][1], m.F[0][3]}, {m.F[2][1], m.F[2][3]}};
+      cd fi[2][2], bi[2][2];
+      const bool ok = inv2(ef, fi) & inv2(eb, bi);
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+          G[i][j] = ok ? cast<T>(fi[i][j]) : mk<T>(T(0), T(0));
+          G[2 + i][j] = ok ? cast<T>(bi[i][j]) : mk<T>(T(0), T(0));
+        }
+    }
+    for (int lam = 0; lam < A.nlam; ++lam) {
+      const T kd = T(A.bm.k0[lam] * A.d);
+      cpx<T> p[4];
+      for (int i = 0; i < 4; ++i) p[i] = cexpi_layer(a[i] * kd);
+      for (int ray = r0; ray < r1; ++ray) {
+        cpx<T>* f = plane(A.field, ray, lam, 4);
+        cpx<T> v[4], c[4];
+        for (int q = 0; q < 4; ++q) v[q] = f[q * A.HW];
+        if constexpr (PK == PIX_A4) {
+          for (int i = 0; i < 4; ++i) {
+            cpx<T> t = G[i][0] * v[0];
+            for (int q = 1; q < 4; ++q) t = t + G[i][q] * v[q];
+            c[i] = t * p[i];
+          }
+          f[0] = F[0][0] * c[0] + F[0][2] * c[2];       // E+x
+          f[A.HW] = F[0][1] * c[1] + F[0][3] * c[3];    // E-x
+          f[2 * A.HW] = F[2][0] * c[0] + F[2][2] * c[2];  // E+y
+          f[3 * A.HW] = F[2][1] * c[1] + F[2][3] * c[3];  // E-y
+        } else {
+          c[0] = (G[0][0] * v[0] + G[0][1] * v[2]) * p[0];
+          c[2] = (G[1][0] * v[0] + G[1][1] * v[2]) * p[2];
+          c[1] = (G[2][0] * v[1] + G[2][1] * v[3]) * p[1];
+          c[3] = (G[3][0] * v[1] + G[3][1] * v[3]) * p[3];
+          for (int i = 0; i < 4; ++i) {
+            cpx<T> t = F[i][0] * c[0];
+            for (int q = 1; q < 4; ++q) t = t + F[i][q] * c[q];
+            f[i * A.HW] = t;
+          }
+        }
+      }
+    }
+  } else if constexpr (PK == PIX_LAYER4 || PK == PIX_LAYER2) {
     const size_t o = px;
     const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
     const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);

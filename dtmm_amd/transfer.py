@@ -289,9 +289,12 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
                                       "is supported")
         return kind, reflection
     if kind == "4x4":
+        if reflection == 2 and diffraction == 1 and npass == 1:
+            return kind, reflection
         if reflection != 1:
             raise NotImplementedError("4x4 method: reflection mode 2 is supported with "
-                                      "diffraction=0 only; otherwise reflection mode 1")
+                                      "diffraction 0 or 1 and npass=1; otherwise reflection "
+                                      "mode 1")
         if not (diffraction >= 1) or diffraction == np.inf:
             raise NotImplementedError("4x4 method: only diffraction 0 or finite diffraction >= 1 "
                                       "is supported")

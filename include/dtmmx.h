@@ -83,7 +83,8 @@ typedef struct {
                          transfer.py:206-221); DTMM_NORM_REF is not built */
   double smooth;      /* npass > 1: notch width of dtmm/denoise.py (dtmm.ini:8,
                          0.1); negative values count as 0                 */
-  int reflection;     /* 4x4: 1, or 2 with diffraction 0.  2x2: 0, or 1 =
+  int reflection;     /* 4x4: 1, or 2 with diffraction 0 or 1 and npass 1.
+                         2x2: 0, or 1 =
                          Fresnel transmission at
                          interfaces between different effective media
                          (propagate_2x2.py:28-30, diffract.py:215-225)    */
@@ -143,6 +144,10 @@ int dtmmx_plan_get_tiles(const dtmmx_plan* plan, int* col_lines, int* row_rays);
  *   4x4 (reflection mode 1, diffraction >= 1, npass >= 1): replaces
  *       transfer.py:694-875 with propagate_4x4.py:376-444 and :80-101 per
  *       layer.
+ *   4x4 (reflection mode 2, diffraction 1, npass 1): real-space sub-steps
+ *       e P/2 F^-1, k-space E diffraction (corrected_Epn_diffraction_matrix,
+ *       matrix.py:38-43, :62-68), F P/2 e^-1, replacing
+ *       propagate_4x4.py:156-175 with :18-51.
  *   4x4 (reflection mode 2, diffraction 0, npass 1) and 2x2 (reflection
  *       mode 0, diffraction 0, npass 1): per-pixel stacks in real space,
  *       replacing propagate_4x4.py:156-175 with :18-51 (dmatpn None) and

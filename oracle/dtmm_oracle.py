@@ -12,8 +12,9 @@ Scope (what the GPU engine implements and this module checks):
 * ``transfer_4x4`` with reflection=1, diffraction >= 1, npass >= 1 (dtmm/transfer.py,
   dtmm/propagate_4x4.py); diffraction > 1 uses the tent windows of dtmm/mode.py;
   npass > 1 the normalisations of transfer.py and dtmm/denoise.py
-* ``transfer_4x4`` with reflection=2, diffraction=0, npass=1: per-pixel
-  f.P.f^-1 sub-steps in real space (dtmm/propagate_4x4.py _transfer_ray_4x4_2)
+* ``transfer_4x4`` with reflection=2, diffraction 0 or 1, npass=1: real-space
+  sub-steps of dtmm/propagate_4x4.py _transfer_ray_4x4_2 (per-pixel f.P.f^-1
+  without diffraction; e.P/2.f^-1, k-space E diffraction, f.P/2.e^-1 with it)
 * ``transfer_2x2`` with reflection in {0, 1} and diffraction 1, or reflection 0
   and diffraction 0, npass=1 (dtmm/transfer.py, dtmm/propagate_2x2.py);
   reflection 1 applies the Fresnel interface transmission of dtmm/tmm.py tr_mat
@@ -890,17 +891,58 @@ def _transfer_ray_4x4_1(field, ks, layer, dmat1, dmat2, beta, phi, nsteps):
     return field
 
 
-def _transfer_ray_4x4_2(field, ks, layer, beta, phi, nsteps):
-    """Reflection mode 2 without diffraction (dtmm/propagate_4x4.py:18-51 with
-    dmatpn None, called from :156-175 for diffraction 0): every sub-step
-    applies f diag(exp(i k0 d alpha)) f^-1 to the real-space field of each
-    pixel."""
+def corrected_Epn_diffraction_matrix(shape, ks, beta, phi, d, epsv, epsa, betamax=BETAMAX):
+    """C(d/2) . D(d) . C(d/2) of the forward (+1) and backward (-1) E modes
+    (dtmm/matrix.py:38-43, :62-68)."""
+    out = []
+    for mode in (+1, -1):
+        dmat = _E_diffraction_matrix(shape, ks, d, epsv, epsa, mode, betamax)
+        cmat = _E_correction_matrix(beta, phi, ks, d / 2., epsv, epsa, mode)
+        out.append(dotmm(cmat, dotmm(dmat, cmat)))
+    return out
+
+
+def _transfer_ray_4x4_2(field, ks, layer, beta, phi, nsteps, dmatpn=None):
+    """Reflection mode 2 in real space (dtmm/propagate_4x4.py:18-51).
+    Without diffraction (dmatpn None) every sub-step applies
+    f diag(exp(i k0 d alpha)) f^-1 to each pixel.  With diffraction it
+    applies e diag(exp(i k0 d alpha / 2)) f^-1 (to the E parts of the
+    forward and backward modes), diffracts the forward and backward parts
+    with dmatpn, and returns with f diag(exp(i k0 d alpha / 2)) e^-1."""
     d, epsv, epsa = layer
     alpha, f, fi = alphaffi(beta, phi, epsv, epsa)
-    p = phase_mat(alpha, (ks * d)[..., None, None])
+    if dmatpn is None:
+        p = phase_mat(alpha, (ks * d)[..., None, None])
+        for _ in range(nsteps):
+            field = dotmdmf(f, p, fi, field)
+        return field
+    dmatp, dmatn = dmatpn
+    p = phase_mat(alpha, (ks * d / 2)[..., None, None])
+    e = E_mat(f)
+    ei = inv(e)
     for _ in range(nsteps):
-        field = dotmdmf(f, p, fi, field)
+        field = dotmdmf(e, p, fi, field)
+        field[..., 0::2, :, :] = diffract(field[..., 0::2, :, :], dmatp)
+        field[..., 1::2, :, :] = diffract(field[..., 1::2, :, :], dmatn)
+        field = dotmdmf(f, p, ei, field)
     return field
+
+
+def propagate_4x4_effective_2(field, ks, layer, effective_layer, beta=0, phi=0, nsteps=1,
+                              diffraction=1, betamax=BETAMAX, cache=None):
+    """Layer step of the 4x4 method, reflection mode 2, diffraction 0 or 1
+    (dtmm/propagate_4x4.py:156-175)."""
+    if diffraction == 0:
+        return _transfer_ray_4x4_2(field, ks, layer, beta, phi, nsteps)
+    if diffraction != 1:
+        raise NotImplementedError("oracle covers reflection mode 2 with diffraction 0 or 1")
+    d_eff, epsv_eff, epsa_eff = effective_layer
+    shape = field.shape[-2:]
+    cache = cache or _Cache()
+    key = _key(shape, ks, beta, phi, d_eff, epsv_eff, epsa_eff, betamax)
+    dmatpn = cache.get(("pn",) + key, lambda: corrected_Epn_diffraction_matrix(
+        shape, ks, beta, phi, d_eff, epsv_eff, epsa_eff, betamax))
+    return _transfer_ray_4x4_2(field, ks, layer, beta, phi, nsteps, dmatpn)
 
 
 def propagate_4x4_effective_1(field, ks, layer, effective_layer, beta=0, phi=0, nsteps=1,
@@ -1149,24 +1191,27 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     like the reference, the input array of field_data (when complex128)
     then receives the field at the input side, reflections included.
     """
-    if reflection == 2 and diffraction == 0 and npass == 1:
-        # dtmm/transfer.py:694-700, :782-795 (propagate_4x4_effective_2), :874:
-        # the input is neither projected nor transformed, input and output
-        # media are skipped
+    if reflection == 2 and diffraction in (0, 1) and npass == 1:
+        # dtmm/transfer.py:694-700, :719-725 (work_in_fft False), :782-795
+        # (propagate_4x4_effective_2), :874: the input is neither projected
+        # nor transformed, input and output media are skipped
         field_in, wavelengths, pixelsize = field_data
         field = np.array(field_in, CDTYPE)
-        layers, _ = create_layers(optical_data, nin, nout, nstep)
+        layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
         ks = k0(wavelengths, pixelsize)
         beta, phi = _validate_betaphi(beta, phi, extendeddim=field.ndim - 2)
+        cache = _Cache()
         for j in range(1, len(layers) - 1):
             nsub, layer = layers[j]
-            field = _transfer_ray_4x4_2(field, ks, layer, beta, phi, nsub)
+            _, eff = eff_layers[j]
+            field = propagate_4x4_effective_2(field, ks, layer, eff, beta, phi, nsub, diffraction,
+                                              betamax, cache)
             if layer_callback is not None:
                 layer_callback(j)
         return field, wavelengths, pixelsize
     if reflection != 1:
         raise NotImplementedError("oracle covers reflection=1, and reflection=2 with "
-                                  "diffraction=0 and npass=1, for the 4x4 method")
+                                  "diffraction 0 or 1 and npass=1, for the 4x4 method")
     if norm & DTMM_NORM_REF:
         raise NotImplementedError("oracle covers norm without DTMM_NORM_REF")
     norm = "fft" if norm & DTMM_NORM_FFT else "total"

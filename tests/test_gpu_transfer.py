@@ -557,3 +557,30 @@ def test_no_diffraction_biaxial_absorbing(method):
     ang = np.array(ang)
     ang[..., 0] = 0.3
     _compare(fd, [(d, eps, ang)], method, "c128", beta=0.15, phi=0.4, diffraction=0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_reflection2_diffraction(precision):
+    """4x4 reflection mode 2 with diffraction 1: per-pixel half steps around
+    the k-space E diffraction, field in real space; two beams, sub-steps,
+    different input / output media."""
+    flush("case reflection 2 diffraction 1 %s" % precision)
+    beta = [0., 0.15]
+    phi = [0., -0.7]
+    fd, optical = random_case(shape=(5, 32, 48), wavelengths=(450., 650.), jones=None,
+                              beta=beta, phi=phi)
+    _compare(fd, optical, "4x4", precision, beta=beta, phi=phi, reflection=2, diffraction=1,
+             nstep=2, nin=1.0, nout=1.33)
+
+
+@pytest.mark.gpu
+def test_reflection2_diffraction_blocks_biaxial():
+    flush("case reflection 2 diffraction 1 blocks biaxial")
+    fd, b1 = random_case(shape=(3, 32, 32), wavelengths=(550.,), jones=(1, 0))
+    _, b2 = droplet_case(shape=(3, 32, 32), radius=12, ne=1.75, nhost=1.55)
+    d, eps, ang = b1[0]
+    eps = np.array(eps)
+    eps[..., 1] *= 1.03
+    _compare(fd, [(d, eps, ang), b2[0]], "4x4", "c128", reflection=2, diffraction=1, nin=1.2,
+             nout=1.0)

@@ -358,3 +358,62 @@ def test_2x2_no_diffraction_projects_input_onto_forward_modes():
     ph = np.exp(1j * o.k0(wl, ps)[0] * L * n)
     np.testing.assert_allclose(out[0, 0], ph * (field[0, 0] + field[0, 1] / n) / 2, atol=1e-12)
     np.testing.assert_allclose(out[0, 2], ph * (field[0, 2] - field[0, 3] / n) / 2, atol=1e-12)
+
+
+def test_reflection2_diffraction_plane_wave_on_grid_equals_no_diffraction():
+    """4x4, reflection 2, diffraction 1: a plane wave that is an FFT mode at
+    the beam's beta sees the k-space step D(d) C(d/2)^2 = 1, so through any
+    homogeneous stack the result equals the per-pixel (diffraction 0)
+    result, itself pinned by the Berreman exponentials."""
+    shape = (8, 16)
+    wl = np.array([500.])
+    ps = 100.
+    k = o.k0(wl, ps)[0]
+    beta = 2 * np.pi / k / 16   # kx index 1 of a 16-wide grid
+    x = np.arange(16)
+    rng = np.random.default_rng(6)
+    amp = rng.normal(size=4) + 1j * rng.normal(size=4)
+    field = (amp[:, None, None] * np.exp(1j * 2 * np.pi * x / 16)[None, None, :]
+             * np.ones(shape))[None]
+    d = np.array([2., 1.])
+    eps = np.array([[2.2, 2.2, 2.9], [2.3, 2.3, 2.3]])
+    ang = np.array([[0., 0.7, -1.1], [0., 0., 0.]])
+    block = (d, np.broadcast_to(eps[:, None, None], (2,) + shape + (3,)),
+             np.broadcast_to(ang[:, None, None], (2,) + shape + (3,)))
+    kw = dict(beta=beta, phi=0., method="4x4", reflection=2, nin=1.3, nout=1.1, nstep=2)
+    out1 = o.transfer_field((field, wl, ps), [block], diffraction=1, **kw)[0]
+    out0 = o.transfer_field((field, wl, ps), [block], diffraction=0, **kw)[0]
+    np.testing.assert_allclose(out1, out0, atol=1e-12 * np.abs(out0).max())
+
+
+def test_reflection2_diffraction_isotropic_slab_is_angular_spectrum_propagation():
+    """4x4, reflection 2, diffraction 1 through a homogeneous isotropic slab
+    of the effective medium at normal incidence: a windowed input whose H is
+    the per-pixel forward relation (Hy = n Ex, Hx = -n Ey) leaves with E
+    propagated exactly per FFT mode, exp(i k0 L sqrt(n^2 - beta^2)) and zero
+    for beta >= betamax, and H = n E."""
+    n, L = 1.55, 6
+    shape = (24, 32)
+    wl = np.array([500., 700.])
+    ps = 100.
+    win = o.blackman(shape)
+    rng = np.random.default_rng(9)
+    e = (rng.normal(size=(2, 2) + shape) + 1j * rng.normal(size=(2, 2) + shape)) * win
+    field = np.zeros((2, 4) + shape, complex)
+    field[:, 0], field[:, 2] = e[:, 0], e[:, 1]
+    field[:, 1], field[:, 3] = n * e[:, 0], -n * e[:, 1]
+    eps = np.broadcast_to([n ** 2] * 3, (L, 1, 1, 3))
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                           beta=0., phi=0., method="4x4", reflection=2, diffraction=1,
+                           nin=n, nout=n)[0]
+    k = o.k0(wl, ps)
+    b, _ = o.betaphi(shape, k)
+    prop = np.exp(1j * k[:, None, None] * L * np.sqrt(n ** 2 - b ** 2 + 0j))
+    prop[b >= o.BETAMAX] = 0.
+    ex = np.fft.ifft2(np.fft.fft2(e[:, 0]) * prop)
+    ey = np.fft.ifft2(np.fft.fft2(e[:, 1]) * prop)
+    tol = 1e-12 * np.abs(field).max()
+    np.testing.assert_allclose(out[:, 0], ex, atol=tol)
+    np.testing.assert_allclose(out[:, 2], ey, atol=tol)
+    np.testing.assert_allclose(out[:, 1], n * ex, atol=tol)
+    np.testing.assert_allclose(out[:, 3], -n * ey, atol=tol)
