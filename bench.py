@@ -347,6 +347,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ref", action="store_true", help="skip the c128 comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the automatic launch shapes instead of timing the candidates "
+                         "(dtmm_amd.tune) before the warm-up")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without per-kernel HIP events")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -424,6 +427,17 @@ def main():
         return plan, fin, fout, ws
 
     plan, fin, fout, ws = make(args.precision)
+    tiles = None
+    if not args.no_tune and not c5:
+        # launch shapes timed on this workload (the large windowed workload
+        # keeps the automatic shapes: one pass takes seconds)
+        from dtmm_amd import tune
+        t = time.time()
+        res = tune.tune_plan(plan, stack, fin, fout, ws, repeats=2, cache=False)
+        tiles = {"col_lines": res["tiles"][0], "row_rays": res["tiles"][1]}
+        log("tuned launch shapes in %.1f s: %s (ms per pass %s)"
+            % (time.time() - t, res["tiles"],
+               ", ".join("%s %.2f" % (k, v) for k, v in sorted(res["ms"].items()))))
     # npass > 1 overwrites the input with the input-side field: every step
     # starts from a fresh copy of the illumination
     fin0 = fin.clone() if npass > 1 else None
@@ -564,7 +578,7 @@ def main():
         "config": {"workload": workload,
                    "height": H, "width": W, "layers": L, "wavelengths": nlam, "planes": planes,
                    "method": args.method, "precision": args.precision, "npass": npass,
-                   "diffraction": diffraction},
+                   "diffraction": diffraction, "launch_shapes": tiles or "automatic"},
         "hbm_frac_per_step": (steps * nwin * 2 * nc * cb / (ms_per_step / 1e3) / 1e9)
         / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

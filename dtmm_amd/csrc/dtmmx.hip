@@ -131,8 +131,13 @@ constexpr int col_tx() {
   return tx;
 }
 
-// Small grids: the lines of a work-group are halved (down to one) until the
-// launch has at least COL_MIN_WORK work-groups, two per CU.
+// Lines per work-group: halved (down to two) while the launch has fewer
+// than COL_WORK_TARGET work-groups, about 8 per CU, so that short
+// work-groups overlap their setup with other work-groups' streaming (the
+// 512^2 x 18-plane pass: 2 lines 16.7 ms, 4 lines 17.1 ms, 1 line 17.9 ms);
+// then, for small grids, down to one until there are COL_MIN_WORK, two per
+// CU.
+constexpr int COL_WORK_TARGET = 2048;
 constexpr int COL_MIN_WORK = 512;
 
 template <typename T, int OP>
@@ -143,6 +148,9 @@ static int launch_col(int n, const ColArgs<T>& a0, int nlines_per_lam, bool tune
   case N: {                                                                                 \
     constexpr int TX = col_tx<T, N>();                                                      \
     const int ntile = (a.W + TX - 1) / TX;                                                  \
+    while (tune && a.ncl % 2 == 0 && a.ncl > 2 &&                                           \
+           ntile * a.b.nlam_b * (nlines_per_lam / a.ncl) < COL_WORK_TARGET)                   \
+      a.ncl /= 2;                                                                           \
     while (tune && a.ncl % 2 == 0 && ntile * a.b.nlam_b * (nlines_per_lam / a.ncl) < COL_MIN_WORK) \
       a.ncl /= 2;                                                                           \
     const int nwork = ntile * a.b.nlam_b * (nlines_per_lam / a.ncl);                        \
