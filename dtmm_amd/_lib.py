@@ -34,6 +34,7 @@ EXPORTS = (
     "dtmmx_plan_set_tiles",
     "dtmmx_plan_get_tiles",
     "dtmmx_transfer",
+    "dtmmx_transfer_bulk",
     "dtmmx_fft2",
     "dtmmx_host_alphaf",
     "dtmmx_timing_begin",
@@ -135,6 +136,9 @@ def load():
     lib.dtmmx_transfer.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stack), ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                    ctypes.c_void_p, ctypes.c_void_p]
+    lib.dtmmx_transfer_bulk.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stack), ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
     lib.dtmmx_transfer.restype = ctypes.c_int
     lib.dtmmx_transfer_batch.argtypes = [ctypes.POINTER(Job), ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_void_p]

@@ -653,9 +653,21 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
 
 // One pass through the stack in place on `out` (real-space field in and
 // out); dir = -1 runs the layers backward (npass > 1).
+// Copies the planes of a wavelength batch (all rays, 4 components).
+template <typename T>
+static int copy_planes(cpx<T>* dst, const cpx<T>* src, const Batch& b, size_t HW, hipStream_t s) {
+  const size_t per_lam = 4 * HW;
+  for (int r = 0; r < b.nrays; ++r) {
+    const size_t o = (size_t(r) * b.nlam + b.lam0) * per_lam;
+    HIP_TRY(hipMemcpyAsync(dst + o, src + o, size_t(b.nlam_b) * per_lam * sizeof(cpx<T>),
+                           hipMemcpyDeviceToDevice, s));
+  }
+  return DTMMX_OK;
+}
+
 template <typename T>
 static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, void* ws,
-                     int* status, hipStream_t s, int dir) {
+                     int* status, hipStream_t s, int dir, cpx<T>* bulk = nullptr) {
   const int H = P->d.height, W = P->d.width;
   const size_t HW = size_t(H) * W;
   const int nrays = P->d.n_rays, nlam = P->d.n_wavelengths;
@@ -699,6 +711,22 @@ static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, voi
       for (int l = 0; l < L; ++l) {
         const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status, wsb + wl.etab);
         TRY(row_pass<T>(P, out, b, 4, OP_ROP, s, &la));
+        if (bulk && (l + 1 == L || st[l + 1].l != st[l].l)) {
+          // field after layer st[l].l in the (x, y) field basis: the tail of
+          // this step applied to a copy (dtmm/transfer.py:777-803 with
+          // bulk_out, :876-877)
+          cpx<T>* slot = bulk + size_t(st[l].l) * nplanes * 4 * HW;
+          TRY(copy_planes<T>(slot, out, b, HW, s));
+          TRY(col_pass<T>(P, slot, b, 4, OP_FWD, s));
+          KArgs<T> t{};
+          t.kop = KOP_TAIL4;
+          t.d0 = st[l].d;
+          t.e0 = st[l].e;
+          t.scale = scale;
+          TRY(kspace<T>(P, slot, slot, b, t, s));
+          TRY(col_pass<T>(P, slot, b, 4, OP_INV, s));
+          TRY(row_pass<T>(P, slot, b, 4, OP_INV, s));
+        }
         if (l + 1 < L) {
           cd e0 = st[l].e, e1 = st[l + 1].e;
           if (eq(e0, e1)) {
@@ -942,7 +970,7 @@ static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin
 // medium, propagates E and builds H in the output medium.
 template <typename T>
 static int transfer_nodiff(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
-                           cpx<T>* fout, void* ws, hipStream_t s) {
+                           cpx<T>* fout, void* ws, hipStream_t s, cpx<T>* bulk = nullptr) {
   const size_t HW = size_t(P->d.height) * P->d.width;
   const bool m4 = P->d.method == DTMMX_METHOD_4X4;
   const int nbeam = P->d.n_rays / P->d.rays_per_beam;
@@ -974,6 +1002,11 @@ static int transfer_nodiff(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<
     else
       hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER2>), grid, dim3(NT), 0, s, a);
     HIP_TRY(hipGetLastError());
+    if (bulk && m4) {
+      const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
+      HIP_TRY(hipMemcpyAsync(bulk + size_t(l) * n, fout, n * sizeof(cpx<T>),
+                             hipMemcpyDeviceToDevice, s));
+    }
   }
   if (!m4) {
     a.src = f;
@@ -994,7 +1027,7 @@ static int transfer_nodiff(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<
 // transform, and F P/2 e^-1 per pixel.
 template <typename T>
 static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
-                          cpx<T>* fout, hipStream_t s) {
+                          cpx<T>* fout, hipStream_t s, cpx<T>* bulk = nullptr) {
   const size_t HW = size_t(P->d.height) * P->d.width;
   const int nbeam = P->d.n_rays / P->d.rays_per_beam;
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
@@ -1008,7 +1041,8 @@ static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T
   const dim3 grid(unsigned((HW + NT - 1) / NT), unsigned(nbeam));
   const std::vector<Step> st = expand_steps(S, 1);
   const Batch b{P->d.n_rays, P->d.n_wavelengths, 0, P->d.n_wavelengths};
-  for (const Step& q : st) {
+  for (size_t k = 0; k < st.size(); ++k) {
+    const Step& q = st[k];
     a.eps_re = S->eps_re + size_t(q.l) * HW;
     a.eps_im = S->eps_im ? S->eps_im + size_t(q.l) * HW : nullptr;
     a.ang = S->angles + size_t(q.l) * HW;
@@ -1029,22 +1063,25 @@ static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T
     TRY(row_pass<T>(P, fout, b, 4, OP_INV, s));
     hipLaunchKernelGGL((pixel_kernel<T, PIX_B4>), grid, dim3(NT), 0, s, a);
     HIP_TRY(hipGetLastError());
+    if (bulk && (k + 1 == st.size() || st[k + 1].l != q.l))
+      HIP_TRY(hipMemcpyAsync(bulk + size_t(q.l) * n, fout, n * sizeof(cpx<T>),
+                             hipMemcpyDeviceToDevice, s));
   }
   return DTMMX_OK;
 }
 
 template <typename T>
 static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
-                         void* ws, int* status, hipStream_t s) {
+                         void* ws, int* status, hipStream_t s, void* bulk = nullptr) {
   const size_t HW = size_t(P->d.height) * P->d.width;
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
   if (P->d.diffraction == 0)
-    return transfer_nodiff<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s);
+    return transfer_nodiff<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s, (cpx<T>*)bulk);
   if (P->d.method == DTMMX_METHOD_4X4 && P->d.reflection == 2)
-    return transfer_refl2<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, s);
+    return transfer_refl2<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, s, (cpx<T>*)bulk);
   if (P->d.npass > 1) return transfer_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s);
   HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
-  return propagate<T>(P, S, (cpx<T>*)fout, ws, status, s, 1);
+  return propagate<T>(P, S, (cpx<T>*)fout, ws, status, s, 1, (cpx<T>*)bulk);
 }
 
 template <typename T>
@@ -1299,6 +1336,10 @@ int transfer_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fou
                  int* status, hipStream_t s);
 int transfer_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
                   int* status, hipStream_t s);
+int bulk_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* bulk, void* ws,
+             int* status, hipStream_t s);
+int bulk_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* bulk, void* ws,
+              int* status, hipStream_t s);
 int fft2_c64(void* data, int batch, int H, int W, int inverse, hipStream_t s);
 int fft2_c128(void* data, int batch, int H, int W, int inverse, hipStream_t s);
 int record_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, void* tape,
@@ -1314,6 +1355,10 @@ int adjoint_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape, co
 int transfer_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
                  int* status, hipStream_t s) {
   return transfer_impl<float>(P, S, fin, fout, ws, status, s);
+}
+int bulk_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* bulk, void* ws,
+             int* status, hipStream_t s) {
+  return transfer_impl<float>(P, S, fin, fout, ws, status, s, bulk);
 }
 int fft2_c64(void* data, int batch, int H, int W, int inverse, hipStream_t s) {
   return fft2_impl<float>(data, batch, H, W, inverse, s);
@@ -1331,6 +1376,10 @@ int adjoint_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape, con
 int transfer_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
                   int* status, hipStream_t s) {
   return transfer_impl<double>(P, S, fin, fout, ws, status, s);
+}
+int bulk_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* bulk, void* ws,
+              int* status, hipStream_t s) {
+  return transfer_impl<double>(P, S, fin, fout, ws, status, s, bulk);
 }
 int fft2_c128(void* data, int batch, int H, int W, int inverse, hipStream_t s) {
   return fft2_impl<double>(data, batch, H, W, inverse, s);
@@ -1759,6 +1808,21 @@ int dtmmx_transfer_batch(const dtmmx_job* jobs, int n, int* status, void* stream
     if (r != DTMMX_OK) return fail(r, "job " + std::to_string(i) + ": " + g_err);
   }
   return DTMMX_OK;
+}
+
+int dtmmx_transfer_bulk(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
+                        void* bulk, void* ws, size_t ws_bytes, int* status, void* stream) {
+  {
+    const int r = check_transfer(P, S, fin, fout, ws, ws_bytes);
+    if (r != DTMMX_OK) return r;
+  }
+  if (!bulk) return fail(DTMMX_E_ARG, "null bulk buffer");
+  if (P->d.method != DTMMX_METHOD_4X4 || P->d.npass != 1 || P->d.diffraction > 1)
+    return fail(DTMMX_E_UNSUPPORTED,
+                "bulk output is built for the 4x4 method with npass 1 and diffraction 0 or 1");
+  hipStream_t s = (hipStream_t)stream;
+  if (P->d.precision == DTMMX_C64) return bulk_c64(P, S, fin, fout, bulk, ws, status, s);
+  return bulk_c128(P, S, fin, fout, bulk, ws, status, s);
 }
 
 int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,

@@ -235,6 +235,30 @@ class Plan:
             stream.cuda_stream))
         return field_out
 
+    def execute_bulk(self, stack, field_in, field_out, bulk, workspace=None, status=None,
+                     stream=None):
+        """execute() that also writes the field after every stack layer into
+        `bulk`, a contiguous device tensor (n_layers, n_rays, n_wavelengths,
+        4, H, W) of the plan's dtype (4x4 method, npass 1, diffraction 0 or
+        1; dtmm/transfer.py ret_bulk)."""
+        torch = _torch()
+        if stream is None:
+            stream = torch.cuda.current_stream(field_in.device)
+        want = (stack.n_layers, self.n_rays, len(self.ks), 4) + self.shape
+        if (bulk.dtype != self.dtype or not bulk.is_contiguous() or not bulk.is_cuda or
+                tuple(bulk.shape) != want):
+            raise ValueError("bulk must be a contiguous %s device tensor of shape %s"
+                             % (self.dtype, want))
+        if self.workspace_bytes and (workspace is None or
+                                     workspace.numel() * workspace.element_size() < self.workspace_bytes):
+            workspace = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=field_in.device)
+        s = stack.c_struct()
+        _lib.check(self._lib.dtmmx_transfer_bulk(
+            self._h, ctypes.byref(s), field_in.data_ptr(), field_out.data_ptr(), bulk.data_ptr(),
+            workspace.data_ptr() if workspace is not None else None, self.workspace_bytes,
+            status.data_ptr() if status is not None else None, stream.cuda_stream))
+        return bulk
+
     def __del__(self):
         try:
             if getattr(self, "_h", None) is not None and self._h.value:
@@ -272,8 +296,9 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
     if npass > 1 and (norm & DTMM_NORM_REF):
         raise NotImplementedError("npass > 1 with DTMM_NORM_REF (reference-field "
                                   "normalisation) is not supported")
-    if ret_bulk:
-        raise NotImplementedError("ret_bulk is not supported yet")
+    if ret_bulk and (method != "4x4" or npass != 1 or not (0 <= diffraction <= 1)):
+        raise NotImplementedError("ret_bulk is supported for the 4x4 method with npass=1 and "
+                                  "diffraction 0 or 1")
     if eff_data not in (None, 0, "isotropic"):
         raise NotImplementedError("only isotropic effective data (eff_data=0) is supported")
     if diffraction == 0:
@@ -411,6 +436,24 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
     if tune:
         _tune.tune_plan(job.plan, job.stack, job.fin, job.fout, job.ws)
     status = torch.zeros(4, dtype=torch.int32, device="cuda")
+    if ret_bulk:
+        # (n_layers + 2, ...): input, the field after every layer, output
+        # (dtmm/transfer.py:670-690, :876-877)
+        nl = job.stack.n_layers
+        bulk = torch.empty((nl,) + tuple(job.fin.shape), dtype=job.plan.dtype, device="cuda")
+        job.plan.execute_bulk(job.stack, job.fin, job.fout, bulk, workspace=job.ws, status=status)
+        torch.cuda.synchronize()
+        _check_status(status)
+        shp = job.field_np.shape
+        res = np.empty((nl + 2,) + shp, dtype=np.complex64 if job.plan.precision == "c64"
+                       else np.complex128)
+        res[0] = job.field_np
+        res[1:nl + 1] = bulk.cpu().numpy().reshape((nl,) + shp)
+        res[nl + 1] = job.fout.cpu().numpy().reshape(shp)
+        if out is not None:
+            out[...] = res
+            res = out
+        return res, job.wavelengths, job.pixelsize
     job.plan.execute(job.stack, job.fin, job.fout, workspace=job.ws, status=status)
     torch.cuda.synchronize()
     _check_status(status)

@@ -167,6 +167,17 @@ int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field
                    void* field_out, void* workspace, size_t workspace_bytes, int* status,
                    void* stream);
 
+/* dtmmx_transfer that also writes the field after every stack layer:
+ * bulk is device memory [n_layers][n_rays][n_wavelengths][4][H][W] (field
+ * layout, real space, components Ex, Hy, Ey, Hx).  Replaces the ret_bulk
+ * output of transfer_4x4 (transfer.py:670-690, :777-778, :876-877), whose
+ * bulk_out[1 .. n_layers] these slots are; bulk_out[0] and bulk_out[-1] are
+ * field_in and field_out.  Built for the 4x4 method with npass 1 and
+ * diffraction 0 or 1 (reflection 1 or 2). */
+int dtmmx_transfer_bulk(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field_in,
+                        void* field_out, void* bulk, void* workspace, size_t workspace_bytes,
+                        int* status, void* stream);
+
 /* One independent transfer of a batch: its own plan (grid, wavelengths,
  * beams, mode), stack, fields and workspace. */
 typedef struct {

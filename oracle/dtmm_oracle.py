@@ -1181,7 +1181,7 @@ def project_normalized_total(field, dmat):
 
 def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1,
                  nstep=1, diffraction=1, reflection=1, betamax=BETAMAX, norm=DTMM_NORM_FFT,
-                 smooth=SMOOTH, layer_callback=None):
+                 smooth=SMOOTH, layer_callback=None, ret_bulk=False):
     """4x4 transfer, reflection mode 1 (dtmm/transfer.py:618-884).
 
     The input is not projected onto forward modes and the input/output media
@@ -1190,7 +1190,12 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     transfer.py:744-804) with the normalisations of transfer.py:815-872;
     like the reference, the input array of field_data (when complex128)
     then receives the field at the input side, reflections included.
+    ret_bulk (npass 1) returns the fields of every layer index instead: the
+    input, the real-space field after each layer and the output
+    (transfer.py:670-690, :777-778, :876-877).
     """
+    if ret_bulk and npass != 1:
+        raise NotImplementedError("oracle covers ret_bulk with npass=1")
     if reflection == 2 and diffraction in (0, 1) and npass == 1:
         # dtmm/transfer.py:694-700, :719-725 (work_in_fft False), :782-795
         # (propagate_4x4_effective_2), :874: the input is neither projected
@@ -1201,13 +1206,20 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
         ks = k0(wavelengths, pixelsize)
         beta, phi = _validate_betaphi(beta, phi, extendeddim=field.ndim - 2)
         cache = _Cache()
+        bulk = np.zeros((len(layers),) + field.shape, CDTYPE) if ret_bulk else None
         for j in range(1, len(layers) - 1):
             nsub, layer = layers[j]
             _, eff = eff_layers[j]
             field = propagate_4x4_effective_2(field, ks, layer, eff, beta, phi, nsub, diffraction,
                                               betamax, cache)
+            if bulk is not None:
+                bulk[j] = field
             if layer_callback is not None:
                 layer_callback(j)
+        if bulk is not None:
+            bulk[0] = field_in
+            bulk[-1] = field
+            return bulk, wavelengths, pixelsize
         return field, wavelengths, pixelsize
     if reflection != 1:
         raise NotImplementedError("oracle covers reflection=1, and reflection=2 with "
@@ -1234,6 +1246,7 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
                                     betamax=betamax)
         pout_mat = projection_matrix(field.shape[-2:], ks, epsv=refind2eps([nout] * 3), mode=+1,
                                      betamax=betamax)
+    bulk = np.zeros((n,) + field_in.shape, CDTYPE) if ret_bulk else None
     for i in range(npass):
         direction = (-1) ** i
         for j in indices:
@@ -1242,12 +1255,18 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
             field = propagate_4x4_effective_1(field, ks, (thickness * direction, ev, ea),
                                               (d * direction, e, a), beta, phi, nsub,
                                               diffraction, betamax, cache)
+            if bulk is not None:
+                bulk[j] = ifft2(field)
             if layer_callback is not None:
                 layer_callback(j)
         indices.reverse()
         field = ifft2(field)
         if npass == 1:
             field_out[...] = field
+            if bulk is not None:
+                bulk[0] = field_in
+                bulk[-1] = field_out
+                return bulk, wavelengths, pixelsize
             break
         sigma = smooth * (npass - i) / npass
         if i % 2 == 0:
@@ -1344,7 +1363,8 @@ def _transmitted_field_direct(field, beta, phi, n=1.):
 
 def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
                    npass=None, nstep=1, diffraction=None, reflection=None, method=None,
-                   betamax=None, norm=DTMM_NORM_FFT, smooth=SMOOTH, layer_callback=None):
+                   betamax=None, norm=DTMM_NORM_FFT, smooth=SMOOTH, layer_callback=None,
+                   ret_bulk=False):
     """Defaults and dispatch of dtmm.transfer_field (dtmm/transfer.py:322-615)
     for the covered modes.  The configured default method is the quoted string
     '"2x2"' (dtmm/dtmm.ini:23 read by configparser), which makes the automatic
@@ -1365,7 +1385,10 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
                 reflection = 2
     if method == "4x4":
         return transfer_4x4(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
-                            diffraction, reflection, betamax, norm, smooth, layer_callback)
+                            diffraction, reflection, betamax, norm, smooth, layer_callback,
+                            ret_bulk)
+    if ret_bulk:
+        raise NotImplementedError("oracle covers ret_bulk for the 4x4 method")
     return transfer_2x2(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
                         diffraction, reflection, betamax, layer_callback)
 

@@ -584,3 +584,28 @@ def test_reflection2_diffraction_blocks_biaxial():
     eps[..., 1] *= 1.03
     _compare(fd, [(d, eps, ang), b2[0]], "4x4", "c128", reflection=2, diffraction=1, nin=1.2,
              nout=1.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("reflection,diffraction", [(1, 1), (2, 1), (2, 0)])
+def test_ret_bulk(reflection, diffraction, precision):
+    """ret_bulk: the field after every layer (plus input and output) against
+    the oracle's bulk, with sub-steps and two beams."""
+    flush("case ret_bulk reflection %d diffraction %d %s" % (reflection, diffraction, precision))
+    beta = [0., 0.15]
+    phi = [0., 0.9]
+    fd, optical = random_case(shape=(4, 32, 48), wavelengths=(450., 650.), jones=None,
+                              beta=beta, phi=phi)
+    field, wl, ps = fd
+    kw = dict(beta=beta, phi=phi, method="4x4", reflection=reflection, diffraction=diffraction,
+              nstep=2, nin=1.2, nout=1.0)
+    ref = oracle.transfer_field((field, wl, ps), optical, ret_bulk=True, **kw)[0]
+    dt = np.complex64 if precision == "c64" else np.complex128
+    out = dm.transfer_field((field.astype(dt), wl, ps), optical, precision=precision,
+                            ret_bulk=True, **kw)[0]
+    assert out.shape == ref.shape == (6,) + field.shape
+    for j in range(out.shape[0]):
+        err = rel_err(out[j], ref[j])
+        flush("  slot %d rel err %.3e" % (j, err))
+        assert err < TOL[precision], (j, err)

@@ -417,3 +417,46 @@ def test_reflection2_diffraction_isotropic_slab_is_angular_spectrum_propagation(
     np.testing.assert_allclose(out[:, 2], ey, atol=tol)
     np.testing.assert_allclose(out[:, 1], n * ex, atol=tol)
     np.testing.assert_allclose(out[:, 3], -n * ey, atol=tol)
+
+
+@pytest.mark.parametrize("reflection", [1, 2])
+def test_bulk_isotropic_slab_is_angular_spectrum_at_each_depth(reflection):
+    """ret_bulk through a homogeneous isotropic slab of the effective medium:
+    slot j is the input propagated by j layers (angular spectrum per FFT
+    mode, zero past betamax), slot 0 the input and the last slot the output.
+    Reflection mode 1 takes illumination_data's beam; reflection mode 2 a
+    beam whose H is the per-pixel forward relation (normal incidence)."""
+    n, L = 1.55, 4
+    shape = (16, 24)
+    wl = np.array([550.])
+    ps = 100.
+    win = o.blackman(shape)
+    if reflection == 1:
+        field, _, _ = o.illumination_data(shape, wl, pixelsize=ps, beta=0., phi=0.,
+                                          jones=(1, 1j), window=win, n=n)
+    else:
+        rng = np.random.default_rng(4)
+        e = (rng.normal(size=(1, 2) + shape) + 1j * rng.normal(size=(1, 2) + shape)) * win
+        field = np.zeros((1, 4) + shape, complex)
+        field[:, 0], field[:, 2] = e[:, 0], e[:, 1]
+        field[:, 1], field[:, 3] = n * e[:, 0], -n * e[:, 1]
+    eps = np.broadcast_to([n ** 2] * 3, (L, 1, 1, 3))
+    bulk = o.transfer_field((field, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                            beta=0., phi=0., method="4x4", reflection=reflection, nin=n, nout=n,
+                            ret_bulk=True)[0]
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                           beta=0., phi=0., method="4x4", reflection=reflection, nin=n,
+                           nout=n)[0]
+    assert bulk.shape == (L + 2,) + field.shape
+    np.testing.assert_allclose(bulk[0], field)
+    np.testing.assert_allclose(bulk[-1], out, atol=1e-14)
+    k = o.k0(wl, ps)
+    b, _ = o.betaphi(shape, k)
+    for j in range(1, L + 1):
+        prop = np.exp(1j * k[:, None, None] * j * np.sqrt(n ** 2 - b ** 2 + 0j))
+        prop[b >= o.BETAMAX] = 0.
+        expect = np.fft.ifft2(np.fft.fft2(field) * prop[:, None])
+        comps = [0, 1, 2, 3] if reflection == 1 else [0, 2]
+        for c in comps:
+            np.testing.assert_allclose(bulk[j][:, c], expect[:, c],
+                                       atol=1e-12 * np.abs(expect).max())
