@@ -170,9 +170,11 @@ def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensi
                       diffraction=True, eigenmodes=None, betamax=BETAMAX):
     """Forward (or backward) propagating input field data
     (dtmm/field.py:586-713).  Returns (field, wavelengths, pixelsize) with
-    field of shape (..., [2 pol], n_wavelengths, 4, H, W)."""
+    field of shape (..., [2 pol], n_wavelengths, 4, H, W).  The plane waves
+    are snapped to FFT eigen-waves exactly when ``window`` is None, as in the
+    reference (field.py:686-687); ``eigenmodes`` is accepted for signature
+    parity and, as there, does not change the field."""
     n = N_COVER if n is None else n
-    eigenmodes = (window is None) if eigenmodes is None else bool(eigenmodes)
     wavelengths = np.asarray(wavelengths)
     wavenumbers = 2 * np.pi / wavelengths * pixelsize
     if wavenumbers.ndim != 1:
@@ -189,10 +191,10 @@ def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensi
     b = b if b.ndim >= 1 else b[..., None]
     p = p if p.ndim >= 1 else p[..., None]
     waves = planewave(shape, wavenumbers, b, p)
-    if window is not None:
-        waves = waves * window
-    if eigenmodes:
+    if window is None:
         waves = wave2eigenwave(waves)
+    else:
+        waves = waves * window
     b = _beta[:, None, None, None] if _beta.ndim == 1 else _beta[..., None, None]
     p = _phi[:, None, None, None] if _phi.ndim == 1 else _phi[..., None, None]
     inten = _intensity[..., None, None, None, None]

@@ -18,7 +18,8 @@ Scope (what the GPU engine implements and this module checks):
 * ``transfer_2x2`` with reflection in {0, 1} and diffraction 1, or reflection 0
   and diffraction 0, npass=1 (dtmm/transfer.py, dtmm/propagate_2x2.py);
   reflection 1 applies the Fresnel interface transmission of dtmm/tmm.py tr_mat
-* ``illumination_data`` and ``field2intensity``            (dtmm/field.py)
+* ``illumination_data`` and ``field2intensity``            (dtmm/field.py),
+  with the eigen-wave snapping of ``window=None``          (dtmm/wave.py)
 
 Pinning of each function is done in ``tests/test_oracle_*.py`` (golden
 vectors of dtmm/test/test_wave.py, closed forms, invariants, brute force).
@@ -39,7 +40,7 @@ __all__ = [
     "second_E_diffraction_matrix", "propagate_4x4_effective_1",
     "propagate_2x2_effective_1", "transfer_4x4", "transfer_2x2",
     "transfer_field", "illumination_data", "blackman", "planewave",
-    "jonesvec", "avec", "fvec", "auxiliary_matrix", "poynting", "set_fft_workers",
+    "jonesvec", "eigenmask", "eigenindices", "eigenwave", "wave2eigenwave", "avec", "fvec", "auxiliary_matrix", "poynting", "set_fft_workers",
     "fft_window", "fft_windows", "fft_betaxy_mean", "fft_mask",
     "DTMM_NORM_FFT", "DTMM_NORM_REF", "S_mat", "Etri_mat", "tr_mat", "E_tr_matrix", "exp_notch_filter", "denoise_fftfield", "denoise_field",
     "normalize_field", "project_normalized_fft", "project_normalized_total", "total_intensity",
@@ -97,6 +98,45 @@ def planewave(shape, k0, beta, phi):
     ky = np.asarray(k0 * beta * np.sin(phi), dtype=FDTYPE)
     out = np.exp(1j * (kx * xx + ky * yy))
     return out / out[..., 0, 0][..., None, None]
+
+
+def eigenmask(shape, k0, betamax=BETAMAX):
+    """FFT modes with beta <= betamax (dtmm/wave.py:256-277)."""
+    b, _ = betaphi(shape, k0)
+    return b <= betamax
+
+
+def eigenindices(shape, k0, betamax=BETAMAX):
+    """(row, column) indices of the modes of eigenmask, one array per wave
+    number (dtmm/wave.py:201-225)."""
+    mask = eigenmask(shape, k0, betamax)
+    if mask.ndim == 2:
+        return np.argwhere(mask)
+    return tuple(np.argwhere(m) for m in mask)
+
+
+def eigenwave(shape, i, j, amplitude=None):
+    """Plane wave of FFT mode (i, j); by default of unit amplitude in real
+    space (dtmm/wave.py:140-171)."""
+    f = np.zeros(shape, CDTYPE)
+    if amplitude is None:
+        amplitude = np.multiply.reduce(shape[-2:])
+    f[..., i, j] = amplitude
+    return np.fft.ifft2(f)
+
+
+def wave2eigenwave(wave):
+    """Each 2D wave replaced by the eigen-wave of its strongest FFT mode,
+    with the same total power and a real positive coefficient
+    (dtmm/wave.py:432-476)."""
+    wave = np.asarray(wave)
+    shape = wave.shape
+    flat = wave.reshape((-1,) + shape[-2:])
+    out = np.empty(flat.shape, CDTYPE)
+    for k, a in enumerate(np.abs(np.fft.fft2(flat)) ** 2):
+        ii, jj = np.unravel_index(np.argmax(a), a.shape)
+        out[k] = eigenwave(a.shape, ii, jj, amplitude=a.sum() ** 0.5)
+    return out.reshape(shape)
 
 
 # ---------------------------------------------------------------------------
@@ -1414,11 +1454,9 @@ def _diffracted_field(field, ks, d=0., n=1., mode="t", betamax=BETAMAX):
 def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensity=1.,
                       n=None, focus=0., window=None, jones=None, diffraction=True,
                       betamax=BETAMAX):
-    """Forward-propagating input field data (dtmm/field.py:586-713).
-    Only the windowed (non-eigenmode) construction is restated; parity
-    unpinned for window=None (eigen-snapping, field.py:686-687)."""
-    if window is None:
-        raise NotImplementedError("oracle restates the windowed illumination only")
+    """Forward-propagating input field data (dtmm/field.py:586-713); without
+    a window the plane waves are snapped to FFT eigen-waves
+    (field.py:686-687)."""
     n = N_COVER if n is None else n
     wavelengths = np.asarray(wavelengths)
     wavenumbers = 2 * np.pi / wavelengths * pixelsize
@@ -1433,7 +1471,11 @@ def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensi
     p = _phi[:, None] if _phi.ndim == 1 else _phi
     bb = b if b.ndim >= 1 else b[..., None]
     pp = p if p.ndim >= 1 else p[..., None]
-    waves = planewave(shape, wavenumbers, bb, pp) * window
+    waves = planewave(shape, wavenumbers, bb, pp)
+    if window is None:
+        waves = wave2eigenwave(waves)
+    else:
+        waves = waves * window
     b = _beta[:, None, None, None] if _beta.ndim == 1 else _beta[..., None, None]
     p = _phi[:, None, None, None] if _phi.ndim == 1 else _phi[..., None, None]
     inten = _intensity[..., None, None, None, None]

@@ -1,0 +1,130 @@
+"""Pinning of the oracle's eigen-wave snapping and of the product's
+``illumination_data(window=None)`` against it (CPU only).
+
+Without a window, dtmm/field.py:686-687 replaces each illumination plane
+wave by the FFT eigen-wave of its strongest mode (dtmm/wave.py:432-476).
+The oracle's restatement is pinned by
+
+* the reference's golden eigen-indices and eigen-mask for shape (4, 3),
+  wave numbers (1, 2), betamax 1.8 (dtmm/test/test_wave.py:57-69, 156-164);
+* the reference's known answers: the (0, 0) eigen-wave is all ones with
+  and without an explicit amplitude (test_wave.py:117-125), a tilted
+  plane wave snaps to modes (0, 1) and (0, 2) (test_wave.py:197-203), and
+  a plane wave built on a grid mode's (beta, phi) is that eigen-wave
+  (test_wave.py:208-230);
+* brute force: the snapped wave has exactly one non-zero FFT coefficient,
+  real and positive, at the strongest mode of the input, and the same total
+  power; snapping is idempotent.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as o
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _golden():
+    return json.load(open(os.path.join(HERE, "golden", "wave_vectors.json")))
+
+
+def test_eigenindices_golden_vectors():
+    g = _golden()
+    shape, ks, bm = tuple(g["shape"]), g["ks"], g["eigenindices_betamax"]
+    out = o.eigenindices(shape, ks, betamax=bm)
+    assert len(out) == len(ks)
+    for got, want in zip(out, g["eigenindices"]):
+        np.testing.assert_array_equal(got, np.asarray(want))
+    # one wave number: a single array, the second one of the pair
+    np.testing.assert_array_equal(o.eigenindices(shape, ks[1], betamax=bm),
+                                  np.asarray(g["eigenindices"][1]))
+
+
+def test_eigenmask_matches_golden_beta():
+    g = _golden()
+    shape, ks, bm = tuple(g["shape"]), g["ks"], g["eigenindices_betamax"]
+    mask = o.eigenmask(shape, ks, betamax=bm)
+    np.testing.assert_array_equal(mask, np.asarray(g["beta"]) <= bm)
+    for m, idx in zip(mask, g["eigenindices"]):
+        assert set(map(tuple, np.argwhere(m))) == set(map(tuple, idx))
+
+
+def test_eigenwave_known_answers():
+    np.testing.assert_allclose(o.eigenwave((4, 3), 0, 0, amplitude=12), np.ones((4, 3)))
+    w = o.eigenwave((4, 3), 0, 0)
+    assert w.dtype == np.complex128
+    np.testing.assert_allclose(w, np.ones((4, 3)))
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (15, 15), (12, 13), (7, 6)])
+def test_planewave_on_grid_mode_is_eigenwave(shape):
+    for k in (1., 0.1, 10.):
+        b, p = o.betaphi(shape, k)
+        for i in range(shape[0]):
+            for j in range(shape[1]):
+                np.testing.assert_allclose(o.planewave(shape, k, b[i, j], p[i, j]),
+                                           o.eigenwave(shape, i, j), rtol=1e-5, atol=1e-8)
+
+
+def test_wave2eigenwave_known_answer():
+    w = o.planewave((12, 14), np.asarray([1., 2.]), 0.5, 0.)
+    e = o.wave2eigenwave(w)
+    assert e.dtype == np.complex128
+    np.testing.assert_allclose(e[0], o.eigenwave((12, 14), 0, 1), rtol=1e-5, atol=1e-8)
+    np.testing.assert_allclose(e[1], o.eigenwave((12, 14), 0, 2), rtol=1e-5, atol=1e-8)
+
+
+def test_wave2eigenwave_brute_force():
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal((3, 2, 9, 10)) + 1j * rng.standard_normal((3, 2, 9, 10))
+    e = o.wave2eigenwave(w)
+    assert e.shape == w.shape
+    F = np.fft.fft2(w).reshape(-1, 9, 10)
+    E = np.fft.fft2(e).reshape(-1, 9, 10)
+    for f, g in zip(F, E):
+        p = np.abs(f) ** 2
+        i, j = np.unravel_index(np.argmax(p), p.shape)
+        want = np.zeros_like(g)
+        want[i, j] = np.sqrt(p.sum())
+        np.testing.assert_allclose(g, want, atol=1e-9 * np.sqrt(p.sum()))
+    # same power in real space (Parseval), and snapping is idempotent
+    np.testing.assert_allclose((np.abs(e) ** 2).sum((-2, -1)), (np.abs(w) ** 2).sum((-2, -1)))
+    np.testing.assert_allclose(o.wave2eigenwave(e), e, atol=1e-12)
+
+
+def test_illumination_without_window_on_grid_beam_equals_flat_window():
+    """A beam whose (beta, phi) is an FFT mode is already an eigen-wave, so
+    snapping changes nothing: the result equals the windowed construction
+    with a window of ones."""
+    shape, wl, ps = (12, 16), [500.], 100.
+    k = o.k0(wl, ps)[0]
+    b, p = o.betaphi(shape, k)
+    i, j = 1, 2
+    kw = dict(pixelsize=ps, beta=float(b[i, j]), phi=float(p[i, j]), jones=(1, 1j))
+    f0, _, _ = o.illumination_data(shape, wl, window=None, **kw)
+    f1, _, _ = o.illumination_data(shape, wl, window=np.ones(shape), **kw)
+    np.testing.assert_allclose(f0, f1, rtol=1e-9, atol=1e-12)
+
+
+def test_illumination_without_window_is_a_single_mode_per_wave():
+    shape, wl, ps = (12, 16), [450., 600.], 100.
+    f, _, _ = o.illumination_data(shape, wl, pixelsize=ps, beta=[0.13, 0.31], phi=[0.4, 2.0],
+                                  window=None)
+    nz = np.abs(np.fft.fft2(f)) > 1e-9 * np.abs(f).max() * np.prod(shape)
+    assert (nz.sum((-2, -1)) <= 1).all()
+
+
+@pytest.mark.parametrize("jones", [None, (1, 1j)])
+def test_product_illumination_without_window_matches_oracle(jones):
+    import dtmm_amd as dm
+    shape, wl, ps = (12, 16), [450., 600.], 100.
+    kw = dict(pixelsize=ps, beta=[0.13, 0.31], phi=[0.4, 2.0], jones=jones, window=None)
+    got, _, _ = dm.illumination_data(shape, wl, **kw)
+    want, _, _ = o.illumination_data(shape, wl, **kw)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12)
+    # eigenmodes does not change the field, as in the reference
+    got2, _, _ = dm.illumination_data(shape, wl, eigenmodes=False, **kw)
+    np.testing.assert_array_equal(got2, got)
