@@ -3,7 +3,8 @@
 Drop-in for ``dtmm.transfer_field`` (4x4 method: reflection mode 1, any
 diffraction and npass; 2x2 method: reflection modes 0 and 1) with the
 reference's data model, plus ``transfer_batch`` for samples of uneven size and
-``TransferGraph``, the same batch captured once as a HIP graph and replayed.
+``TransferGraph``, the same batch captured once as a HIP graph and replayed,
+and the polarizing-microscope step after it (``pom``: specter and colour).
 Propagation runs in hand-written HIP kernels for gfx950 (``dtmm_amd/csrc``)
 reached through the C ABI of ``include/dtmmx.h``; there is no CPU fallback.
 """
@@ -15,5 +16,7 @@ from .field import (illumination_data, field2intensity, blackman, k0, betaphi,  
 from .transfer import (transfer_field, transfer_batch, TransferGraph, Plan, Stack,  # noqa: F401
                        DTMM_NORM_FFT, DTMM_NORM_REF, field2betaphi, mean_betaphi)
 from .adjoint import AdjointTransfer, polarizer4x4, analyser_intensity_loss  # noqa: F401
+from .pom import calculate_pom_specter, field2specter, specter2color  # noqa: F401
+from . import pom  # noqa: F401
 
 __version__ = "0.1.0"

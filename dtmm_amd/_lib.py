@@ -47,6 +47,10 @@ EXPORTS = (
     "dtmmx_transfer_record",
     "dtmmx_transfer_adjoint",
     "dtmmx_host_rop_grad",
+    "dtmmx_pom_workspace_size",
+    "dtmmx_pom_specter",
+    "dtmmx_specter2color_workspace_size",
+    "dtmmx_specter2color",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -98,6 +102,27 @@ class Job(ctypes.Structure):
         ("field_out", ctypes.c_void_p),
         ("workspace", ctypes.c_void_p),
         ("workspace_bytes", ctypes.c_size_t),
+    ]
+
+
+class PomDesc(ctypes.Structure):
+    _fields_ = [
+        ("height", ctypes.c_int),
+        ("width", ctypes.c_int),
+        ("n_wavelengths", ctypes.c_int),
+        ("k0", _dp),
+        ("n_fields", ctypes.c_int),
+        ("n_pol", ctypes.c_int),
+        ("jones", ctypes.c_double * 4),
+        ("diffraction", ctypes.c_int),
+        ("mode", ctypes.c_int),
+        ("d", ctypes.c_double),
+        ("eps_re", ctypes.c_double),
+        ("eps_im", ctypes.c_double),
+        ("betamax", ctypes.c_double),
+        ("pmat", _dp),
+        ("n_pmat", ctypes.c_int),
+        ("precision", ctypes.c_int),
     ]
 
 
@@ -172,6 +197,18 @@ def load():
     lib.dtmmx_timing_begin.restype = ctypes.c_int
     lib.dtmmx_timing_end.argtypes = [_dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     lib.dtmmx_timing_end.restype = ctypes.c_int
+    lib.dtmmx_pom_workspace_size.argtypes = [ctypes.POINTER(PomDesc)]
+    lib.dtmmx_pom_workspace_size.restype = ctypes.c_size_t
+    lib.dtmmx_pom_specter.argtypes = [ctypes.POINTER(PomDesc), ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    lib.dtmmx_pom_specter.restype = ctypes.c_int
+    lib.dtmmx_specter2color_workspace_size.argtypes = [ctypes.c_longlong, ctypes.c_int]
+    lib.dtmmx_specter2color_workspace_size.restype = ctypes.c_size_t
+    lib.dtmmx_specter2color.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
+                                        ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    lib.dtmmx_specter2color.restype = ctypes.c_int
     if lib.dtmmx_abi_version() != ABI_VERSION:
         raise RuntimeError("dtmmx ABI version mismatch")
     _lib = lib

@@ -397,6 +397,9 @@ static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, 
     case KOP_TR2:
       hipLaunchKernelGGL((kspace_kernel<T, KOP_TR2, false>), grid, dim3(NT), 0, s, a);
       break;
+    case KOP_FIELD4:
+      hipLaunchKernelGGL((kspace_kernel<T, KOP_FIELD4, false>), grid, dim3(NT), 0, s, a);
+      break;
     default:
       timer_stop(s);
       return fail(DTMMX_E_ARG, "bad k-space operator");
@@ -1351,7 +1354,133 @@ int adjoint_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape, con
 int adjoint_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape, const void* seed,
                  double* grad, void* ws, int* status, hipStream_t s);
 
+// Device tables of one polarizing-microscope specter call, uploaded by the
+// host part into the caller's workspace.
+struct PomJob {
+  const dtmmx_pom_desc* d;
+  const void* field;
+  void* spec;
+  void* wfield;        // combined / copied fields (diffraction only)
+  void* tw_w;
+  void* tw_h;
+  int* amax;           // [nlam][H] betamax mask rows
+  const void* pmat;    // [npm][4][4] complex double, or null
+};
+
+struct ColorJob {
+  const void* spec;
+  void* out;
+  long long n;
+  int nlam;
+  const double* cmf;   // device [nlam][3]
+  double* part;        // device [nblocks]
+  int norm_mode;
+  double norm;
+  int gamma_mode;
+  double gamma;
+  int gray;
+};
+
+int pom_c64(const PomJob& j, hipStream_t s);
+int pom_c128(const PomJob& j, hipStream_t s);
+int color_c64(const ColorJob& j, hipStream_t s);
+int color_c128(const ColorJob& j, hipStream_t s);
+
+// Specter of dtmmx_pom_specter: (Jones combination) -> FFT -> F P F^-1 of
+// the medium (with 1/HW) -> IFFT -> analyser and Poynting per pixel.
+template <typename T>
+static int pom_impl(const PomJob& j, hipStream_t s) {
+  const dtmmx_pom_desc* d = j.d;
+  const int H = d->height, W = d->width, nlam = d->n_wavelengths, nf = d->n_fields;
+  const long long HW = (long long)H * W;
+  PomArgs<T> a{};
+  a.HW = HW;
+  a.nlam = nlam;
+  a.nf = nf;
+  a.npol = d->n_pol;
+  a.jc = mk<double>(d->jones[0], d->jones[1]);
+  a.js = mk<double>(d->jones[2], d->jones[3]);
+  a.pmat = (const cd*)j.pmat;
+  a.npm = d->n_pmat;
+  a.spec = (T*)j.spec;
+  const unsigned gx = unsigned((HW + NT - 1) / NT);
+  if (!d->diffraction) {
+    a.src = (const cpx<T>*)j.field;
+    hipLaunchKernelGGL((pom_specter_kernel<T>), dim3(gx), dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    return DTMMX_OK;
+  }
+  cpx<T>* F = (cpx<T>*)j.wfield;
+  const size_t nplanes = size_t(nf) * nlam * 4;
+  if (d->n_pol == 2) {
+    a.src = (const cpx<T>*)j.field;
+    a.dst = F;
+    hipLaunchKernelGGL((pom_combine_kernel<T>), dim3(gx, unsigned(nplanes)), dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+  } else {
+    HIP_TRY(hipMemcpyAsync(F, j.field, nplanes * size_t(HW) * sizeof(cpx<T>),
+                           hipMemcpyDeviceToDevice, s));
+  }
+  dtmmx_plan P{};
+  P.d.height = H;
+  P.d.width = W;
+  P.d.n_wavelengths = nlam;
+  P.d.betamax = d->betamax;
+  P.tw_w = j.tw_w;
+  P.tw_h = j.tw_h;
+  P.amax = j.amax;
+  P.bm.nlam = nlam;
+  P.bm.rays_per_beam = nf;  // one (unused) beam for all fields
+  for (int l = 0; l < nlam; ++l) P.bm.k0[l] = d->k0[l];
+  Batch b{nf, nlam, 0, nlam};
+  TRY(row_pass<T>(&P, F, b, 4, OP_FWD, s));
+  TRY(col_pass<T>(&P, F, b, 4, OP_FWD, s));
+  KArgs<T> k{};
+  k.kop = KOP_FIELD4;
+  k.d0 = d->d;
+  k.e0 = mk<double>(d->eps_re, d->eps_im);
+  k.scale = 1.0 / double(HW);
+  k.fmode = d->mode;
+  TRY(kspace<T>(&P, F, F, b, k, s));
+  TRY(col_pass<T>(&P, F, b, 4, OP_INV, s));
+  TRY(row_pass<T>(&P, F, b, 4, OP_INV, s));
+  a.src = F;
+  a.npol = 1;
+  hipLaunchKernelGGL((pom_specter_kernel<T>), dim3(gx), dim3(NT), 0, s, a);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
+template <typename T>
+static int color_impl(const ColorJob& j, hipStream_t s) {
+  ColorArgs<T> a{};
+  a.spec = (const T*)j.spec;
+  a.out = (T*)j.out;
+  a.n = j.n;
+  a.nlam = j.nlam;
+  a.cmf = j.cmf;
+  a.part = j.part;
+  a.nblocks = int((j.n + NT - 1) / NT);
+  a.norm = j.norm_mode == 1 ? j.norm : 0.;
+  a.norm_auto = j.norm_mode == 2;
+  a.gray = j.gray;
+  a.gamma_mode = j.gamma_mode;
+  a.gamma = j.gamma;
+  const dim3 grid(unsigned(a.nblocks));
+  if (a.norm_auto) {
+    hipLaunchKernelGGL((color_kernel<T, 0>), grid, dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((color_kernel<T, 1>), dim3(1), dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL((color_kernel<T, 2>), grid, dim3(NT), 0, s, a);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
 #if DTMMX_C64_PART
+int pom_c64(const PomJob& j, hipStream_t s) { return pom_impl<float>(j, s); }
+int color_c64(const ColorJob& j, hipStream_t s) { return color_impl<float>(j, s); }
 int transfer_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
                  int* status, hipStream_t s) {
   return transfer_impl<float>(P, S, fin, fout, ws, status, s);
@@ -1373,6 +1502,8 @@ int adjoint_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* tape, con
 }
 #endif
 #if DTMMX_C128_PART
+int pom_c128(const PomJob& j, hipStream_t s) { return pom_impl<double>(j, s); }
+int color_c128(const ColorJob& j, hipStream_t s) { return color_impl<double>(j, s); }
 int transfer_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, void* ws,
                   int* status, hipStream_t s) {
   return transfer_impl<double>(P, S, fin, fout, ws, status, s);
@@ -1889,6 +2020,159 @@ int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int pr
   if (precision == DTMMX_C64) return fft2_c64(data, batch, height, width, inverse, s);
   if (precision == DTMMX_C128) return fft2_c128(data, batch, height, width, inverse, s);
   return fail(DTMMX_E_ARG, "bad precision");
+}
+
+static int validate_pom(const dtmmx_pom_desc* d) {
+  if (!d) return fail(DTMMX_E_ARG, "null descriptor");
+  if (d->height <= 0 || d->width <= 0) return fail(DTMMX_E_ARG, "empty grid");
+  if (d->diffraction && (!size_supported(d->height) || !size_supported(d->width)))
+    return fail(DTMMX_E_UNSUPPORTED, "grid " + std::to_string(d->height) + "x" +
+                                         std::to_string(d->width) + " not built");
+  if (d->n_wavelengths <= 0 || d->n_wavelengths > MAXL)
+    return fail(DTMMX_E_ARG, "n_wavelengths out of range");
+  if (!d->k0) return fail(DTMMX_E_ARG, "null k0");
+  for (int l = 0; l < d->n_wavelengths; ++l)
+    if (!(d->k0[l] > 0.) || !std::isfinite(d->k0[l])) return fail(DTMMX_E_ARG, "k0 must be > 0");
+  if (d->n_fields <= 0) return fail(DTMMX_E_ARG, "n_fields must be >= 1");
+  // grid y of the combination and k-space kernels: one block row per plane
+  if ((long long)d->n_fields * d->n_wavelengths * 4 > 65535)
+    return fail(DTMMX_E_ARG, "too many fields x wavelengths");
+  if (d->n_pol != 1 && d->n_pol != 2) return fail(DTMMX_E_ARG, "n_pol must be 1 or 2");
+  if (d->mode < -1 || d->mode > 1) return fail(DTMMX_E_ARG, "mode must be -1, 0 or 1");
+  if (d->pmat && d->n_pmat != 1 && d->n_pmat != d->n_wavelengths)
+    return fail(DTMMX_E_ARG, "n_pmat must be 1 or n_wavelengths");
+  if (d->precision != DTMMX_C64 && d->precision != DTMMX_C128)
+    return fail(DTMMX_E_ARG, "bad precision");
+  if (!std::isfinite(d->d) || !std::isfinite(d->betamax) || !std::isfinite(d->eps_re) ||
+      !std::isfinite(d->eps_im))
+    return fail(DTMMX_E_ARG, "non-finite parameter");
+  return DTMMX_OK;
+}
+
+struct PomLayout {
+  size_t pmat = 0, field = 0, tw_w = 0, tw_h = 0, amax = 0, total = 0;
+};
+
+static PomLayout pom_layout(const dtmmx_pom_desc* d) {
+  PomLayout L;
+  const size_t cs = d->precision == DTMMX_C64 ? 8 : 16;
+  size_t o = 0;
+  auto take = [&](size_t n) {
+    const size_t r = o;
+    o += (n + 255) / 256 * 256;
+    return r;
+  };
+  L.pmat = take(d->pmat ? size_t(d->n_pmat) * 16 * 16 : 0);
+  if (d->diffraction) {
+    const size_t HW = size_t(d->height) * d->width;
+    L.field = take(size_t(d->n_fields) * d->n_wavelengths * 4 * HW * cs);
+    L.tw_w = take(size_t(d->width) * cs);
+    L.tw_h = take(size_t(d->height) * cs);
+    L.amax = take(size_t(d->n_wavelengths) * d->height * sizeof(int));
+  }
+  L.total = o;
+  return L;
+}
+
+size_t dtmmx_pom_workspace_size(const dtmmx_pom_desc* d) {
+  if (validate_pom(d) != DTMMX_OK) return 0;
+  return pom_layout(d).total;
+}
+
+int dtmmx_pom_specter(const dtmmx_pom_desc* d, const void* field, void* specter, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+  TRY(validate_pom(d));
+  if (!field || !specter) return fail(DTMMX_E_ARG, "null field or specter");
+  const PomLayout L = pom_layout(d);
+  if (L.total && (!workspace || workspace_bytes < L.total))
+    return fail(DTMMX_E_ARG, "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  char* w = (char*)workspace;
+  PomJob j{};
+  j.d = d;
+  j.field = field;
+  j.spec = specter;
+  bool upload = false;
+  if (d->pmat) {
+    HIP_TRY(hipMemcpyAsync(w + L.pmat, d->pmat, size_t(d->n_pmat) * 16 * 16,
+                           hipMemcpyHostToDevice, s));
+    j.pmat = w + L.pmat;
+    upload = true;
+  }
+  std::vector<cpx<float>> tf_w, tf_h;
+  std::vector<cpx<double>> td_w, td_h;
+  std::vector<int> amax;
+  if (d->diffraction) {
+    j.wfield = w + L.field;
+    j.tw_w = w + L.tw_w;
+    j.tw_h = w + L.tw_h;
+    j.amax = (int*)(w + L.amax);
+    if (d->precision == DTMMX_C64) {
+      tf_w = twiddles<float>(d->width);
+      tf_h = twiddles<float>(d->height);
+      HIP_TRY(hipMemcpyAsync(j.tw_w, tf_w.data(), tf_w.size() * sizeof(cpx<float>),
+                             hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(j.tw_h, tf_h.data(), tf_h.size() * sizeof(cpx<float>),
+                             hipMemcpyHostToDevice, s));
+    } else {
+      td_w = twiddles<double>(d->width);
+      td_h = twiddles<double>(d->height);
+      HIP_TRY(hipMemcpyAsync(j.tw_w, td_w.data(), td_w.size() * sizeof(cpx<double>),
+                             hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(j.tw_h, td_h.data(), td_h.size() * sizeof(cpx<double>),
+                             hipMemcpyHostToDevice, s));
+    }
+    dtmmx_plan_desc pd{};
+    pd.height = d->height;
+    pd.width = d->width;
+    pd.n_wavelengths = d->n_wavelengths;
+    pd.k0 = d->k0;
+    pd.betamax = d->betamax;
+    amax = betamax_rows(&pd);
+    HIP_TRY(hipMemcpyAsync(j.amax, amax.data(), amax.size() * sizeof(int), hipMemcpyHostToDevice,
+                           s));
+    upload = true;
+  }
+  // the host tables above must outlive their copies
+  if (upload) HIP_TRY(hipStreamSynchronize(s));
+  return d->precision == DTMMX_C64 ? pom_c64(j, s) : pom_c128(j, s);
+}
+
+size_t dtmmx_specter2color_workspace_size(long long n_pixels, int n_wavelengths) {
+  if (n_pixels <= 0 || n_wavelengths <= 0) return 0;
+  const size_t nb = size_t((n_pixels + NT - 1) / NT);
+  return (size_t(n_wavelengths) * 3 * sizeof(double) + 255) / 256 * 256 + nb * sizeof(double);
+}
+
+int dtmmx_specter2color(const void* specter, void* rgb, long long n_pixels, int n_wavelengths,
+                        const double* cmf, int norm_mode, double norm, int gamma_mode,
+                        double gamma, int gray, int precision, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+  if (!specter || !rgb || !cmf) return fail(DTMMX_E_ARG, "null argument");
+  if (n_pixels <= 0 || n_wavelengths <= 0) return fail(DTMMX_E_ARG, "empty specter");
+  if (norm_mode < 0 || norm_mode > 2) return fail(DTMMX_E_ARG, "norm_mode must be 0, 1 or 2");
+  if (gamma_mode < 0 || gamma_mode > 2) return fail(DTMMX_E_ARG, "gamma_mode must be 0, 1 or 2");
+  if (precision != DTMMX_C64 && precision != DTMMX_C128) return fail(DTMMX_E_ARG, "bad precision");
+  const size_t need = dtmmx_specter2color_workspace_size(n_pixels, n_wavelengths);
+  if (!workspace || workspace_bytes < need) return fail(DTMMX_E_ARG, "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  char* w = (char*)workspace;
+  HIP_TRY(hipMemcpyAsync(w, cmf, size_t(n_wavelengths) * 3 * sizeof(double),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  ColorJob j{};
+  j.spec = specter;
+  j.out = rgb;
+  j.n = n_pixels;
+  j.nlam = n_wavelengths;
+  j.cmf = (const double*)w;
+  j.part = (double*)(w + (size_t(n_wavelengths) * 3 * sizeof(double) + 255) / 256 * 256);
+  j.norm_mode = norm_mode;
+  j.norm = norm;
+  j.gamma_mode = gamma_mode;
+  j.gamma = gamma;
+  j.gray = gray;
+  return precision == DTMMX_C64 ? color_c64(j, s) : color_c128(j, s);
 }
 
 int dtmmx_host_windows(const dtmmx_plan_desc* d, int max_windows, int* n_windows, int* wi,

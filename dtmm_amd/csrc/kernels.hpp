@@ -969,7 +969,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
 // Dense per-k operators (fp64), one thread per (k, plane).
 // ---------------------------------------------------------------------------
 
-enum { KOP_HEAD4 = 0, KOP_TAIL4 = 1, KOP_BOUND4 = 2, KOP_PROJ2 = 3, KOP_TR2 = 4 };
+enum { KOP_HEAD4 = 0, KOP_TAIL4 = 1, KOP_BOUND4 = 2, KOP_PROJ2 = 3, KOP_TR2 = 4, KOP_FIELD4 = 5 };
 
 template <typename T>
 struct KArgs {
@@ -986,6 +986,7 @@ struct KArgs {
   int nocorr;     // 1: no mean-beam correction (HEAD4, TAIL4, BOUND4)
   int fresnel;    // PROJ2: 1 applies the interface transmission ein -> e0
   int adjoint;    // 1: apply the conjugate transpose (4-component operators)
+  int fmode;      // FIELD4: 0 both directions, +1 forward modes only, -1 backward only
   Beams bm;
 };
 
@@ -1160,6 +1161,31 @@ __device__ __forceinline__ void kop_apply(const KArgs<T>& A, bool pass, double b
     const cd ep = (v[0] * c + v[1] * sn) * tp, es = (v[1] * c - v[0] * sn) * ts;
     r[0] = K * (ep * c - es * sn);
     r[1] = K * (ep * sn + es * c);
+  } else if constexpr (KOP == KOP_FIELD4) {
+    if (!pass) return;
+    // F P(kd) F^-1 of an isotropic medium in closed form
+    // (dtmm/diffract.py:107-119 field_diffraction_matrix, phase_matrix
+    // :96-105): mode frame g = R^T v, amplitudes m = F^-1 g, phases of the
+    // kept directions, back to the field.
+    const cd a = csqrt(A.e0 - mk<double>(beta * beta));
+    if (!is_zero(a)) {
+      const double kd = k0 * A.d0;
+      const cd igp = cdiv(a, A.e0), igs = cdiv(mk<double>(-1.), a);
+      const cd g0 = v[0] * c + v[2] * sn, g1 = v[1] * c - v[3] * sn;
+      const cd g2 = v[2] * c - v[0] * sn, g3 = v[1] * sn + v[3] * c;
+      const cd zero = mk<double>(0.);
+      const cd p = A.fmode < 0 ? zero : cexpi(a * kd);
+      const cd pi = A.fmode > 0 ? zero : cexpi(a * -kd);
+      const cd m0 = (g0 + g1 * igp) * p, m1 = (g0 - g1 * igp) * pi;
+      const cd m2 = (g2 + g3 * igs) * p, m3 = (g2 - g3 * igs) * pi;
+      const cd gp = cdiv(A.e0, a), gs = -a;
+      const cd h0 = (m0 + m1) * 0.5, h1 = gp * (m0 - m1) * 0.5;
+      const cd h2 = (m2 + m3) * 0.5, h3 = gs * (m2 - m3) * 0.5;
+      r[0] = h0 * c - h2 * sn;
+      r[1] = h1 * c + h3 * sn;
+      r[2] = h0 * sn + h2 * c;
+      r[3] = h3 * c - h1 * sn;
+    }
   } else if constexpr (KOP == KOP_BOUND4) {
     if (!pass) return;
     cd C0[4][4], D0[4][4], C1[4][4], D1[4][4], X[4][4], Y[4][4], M[4][4];
@@ -1226,6 +1252,181 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------------
+// Polarizing-microscope specter (dtmm/field_viewer.py:70-143, :1262-1289;
+// dtmm/field.py:800-845) and its colour conversion (dtmm/color.py:52-200).
+// ---------------------------------------------------------------------------
+
+// Poynting z of one (Ex, Hy, Ey, Hx) sample (dtmm/field.py:771-799).
+__host__ __device__ inline double intensity4(const cd v[4]) {
+  return (v[0].x * v[1].x + v[0].y * v[1].y) - (v[2].x * v[3].x + v[2].y * v[3].y);
+}
+
+template <typename T>
+struct PomArgs {
+  const cpx<T>* src;  // fields [nf][npol][nlam][4][HW]
+  cpx<T>* dst;        // combined fields [nf][nlam][4][HW]
+  T* spec;            // specter [HW][nlam]
+  long long HW;
+  int nlam, nf, npol;
+  cd jc, js;          // weights of the x and y polarised fields (npol == 2)
+  const cd* pmat;     // [npm][4][4] analyser matrices, or null
+  int npm;
+};
+
+// One thread per (pixel, plane): dst = jc * field_x + js * field_y.
+template <typename T>
+__global__ __launch_bounds__(NT) void pom_combine_kernel(PomArgs<T> A) {
+  const long long p = (long long)blockIdx.x * NT + threadIdx.x;
+  if (p >= A.HW) return;
+  const int q = blockIdx.y;  // (field, wavelength, component)
+  const int f = q / (A.nlam * 4), r = q - f * A.nlam * 4;
+  const cpx<T>* s = A.src + (size_t(f) * 2 * A.nlam * 4 + r) * A.HW + p;
+  const cd x = cast<double>(s[0]), y = cast<double>(s[size_t(A.nlam) * 4 * A.HW]);
+  A.dst[size_t(q) * A.HW + p] = cast<T>(A.jc * x + A.js * y);
+}
+
+// One thread per pixel: for every wavelength, the sum over fields of the
+// Poynting z of (analyser matrix) . (field), written with the wavelength
+// last.
+template <typename T>
+__global__ __launch_bounds__(NT) void pom_specter_kernel(PomArgs<T> A) {
+  const long long p = (long long)blockIdx.x * NT + threadIdx.x;
+  if (p >= A.HW) return;
+  for (int lam = 0; lam < A.nlam; ++lam) {
+    double acc = 0.;
+    for (int f = 0; f < A.nf; ++f) {
+      cd v[4];
+      if (A.npol == 2) {
+        const cpx<T>* s = A.src + (size_t(f) * 2 * A.nlam + lam) * 4 * A.HW + p;
+        const size_t py = size_t(A.nlam) * 4 * A.HW;
+        for (int c = 0; c < 4; ++c)
+          v[c] = A.jc * cast<double>(s[c * A.HW]) + A.js * cast<double>(s[c * A.HW + py]);
+      } else {
+        const cpx<T>* s = A.src + (size_t(f) * A.nlam + lam) * 4 * A.HW + p;
+        for (int c = 0; c < 4; ++c) v[c] = cast<double>(s[c * A.HW]);
+      }
+      if (A.pmat) {
+        const cd* m = A.pmat + (A.npm > 1 ? lam : 0) * 16;
+        cd w[4];
+        for (int i = 0; i < 4; ++i) {
+          cd t = mk<double>(0.);
+          for (int j = 0; j < 4; ++j) t = t + m[i * 4 + j] * v[j];
+          w[i] = t;
+        }
+        acc += intensity4(w);
+      } else {
+        acc += intensity4(v);
+      }
+    }
+    A.spec[size_t(p) * A.nlam + lam] = T(acc);
+  }
+}
+
+template <typename T>
+struct ColorArgs {
+  const T* spec;      // [n][nlam]
+  T* out;             // [n][3]
+  long long n;
+  int nlam;
+  const double* cmf;  // [nlam][3]
+  double* part;       // per-block maxima of the XYZ values (stage 0)
+  int nblocks;
+  double norm;        // divide XYZ by this (stage 2 reads part[0] when auto)
+  int norm_auto;
+  int gray;
+  int gamma_mode;     // 0 none, 1 sRGB curve, 2 power 1/gamma
+  double gamma;
+};
+
+// sRGB of D65 white: the IEC 61966-2-1 XYZ -> linear RGB matrix and transfer
+// curve constants (dtmm/color.py:41-50).
+__device__ inline double srgb_curve(double v) {
+  if (v < 0.0031308) return v < 0. ? 0. : 12.92 * v;
+  return v > 1. ? 1. : (1. + 0.055) * pow(v, 1. / 2.4) - 0.055;
+}
+
+__device__ inline double power_curve(double v, double g) {
+  if (v > 1.) return 1.;
+  if (v < 0.) return 0.;
+  return pow(v, 1. / g);
+}
+
+template <typename T>
+__device__ inline void spec_to_xyz(const T* spec, const double* cmf, int nlam, double xyz[3]) {
+  xyz[0] = xyz[1] = xyz[2] = 0.;
+  for (int i = 0; i < nlam; ++i) {
+    const double s = double(spec[i]);
+    xyz[0] = xyz[0] + cmf[3 * i] * s;
+    xyz[1] = xyz[1] + cmf[3 * i + 1] * s;
+    xyz[2] = xyz[2] + cmf[3 * i + 2] * s;
+  }
+}
+
+template <typename T>
+__device__ inline void xyz_to_out(const ColorArgs<T>& A, double xyz[3], double norm, long long p) {
+  if (norm != 0.)
+    for (int k = 0; k < 3; ++k) xyz[k] = xyz[k] / norm;
+  double o[3];
+  if (A.gray) {
+    o[0] = o[1] = o[2] = xyz[1];
+  } else {
+    o[0] = 3.2404542 * xyz[0] + -1.5371385 * xyz[1] + -0.4985314 * xyz[2];
+    o[1] = -0.9692660 * xyz[0] + 1.8760108 * xyz[1] + 0.0415560 * xyz[2];
+    o[2] = 0.0556434 * xyz[0] + -0.2040259 * xyz[1] + 1.0572252 * xyz[2];
+  }
+  for (int k = 0; k < 3; ++k) {
+    double v = o[k];
+    if (A.gamma_mode == 1) v = srgb_curve(v);
+    else if (A.gamma_mode == 2) v = power_curve(v, A.gamma);
+    A.out[size_t(p) * 3 + k] = T(v);
+  }
+}
+
+// STAGE 0: XYZ into out, per-block maximum into part (automatic norm).
+// STAGE 1: one work-group reduces part to part[0].
+// STAGE 2: XYZ (from out when automatic, else from the specter) -> colour.
+template <typename T, int STAGE>
+__global__ __launch_bounds__(NT) void color_kernel(ColorArgs<T> A) {
+  __shared__ double red[NT];
+  if constexpr (STAGE == 1) {
+    double m = -INFINITY;
+    for (int i = threadIdx.x; i < A.nblocks; i += NT) m = fmax(m, A.part[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) A.part[0] = red[0];
+    return;
+  } else {
+    const long long p = (long long)blockIdx.x * NT + threadIdx.x;
+    double xyz[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (p < A.n) {
+      if (STAGE == 2 && A.norm_auto) {
+        for (int k = 0; k < 3; ++k) xyz[k] = double(A.out[size_t(p) * 3 + k]);
+      } else {
+        spec_to_xyz<T>(A.spec + size_t(p) * A.nlam, A.cmf, A.nlam, xyz);
+      }
+    }
+    if constexpr (STAGE == 0) {
+      if (p < A.n)
+        for (int k = 0; k < 3; ++k) A.out[size_t(p) * 3 + k] = T(xyz[k]);
+      red[threadIdx.x] = fmax(xyz[0], fmax(xyz[1], xyz[2]));
+      __syncthreads();
+      for (int w = NT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) A.part[blockIdx.x] = red[0];
+    } else {
+      if (p >= A.n) return;
+      xyz_to_out<T>(A, xyz, A.norm_auto ? A.part[0] : A.norm, p);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Pass normalisation kernels (npass > 1, dtmm/transfer.py:815-872): one
 // thread per (k, plane) on the unnormalised 2D spectrum of a whole field.
 // ---------------------------------------------------------------------------
@@ -1248,11 +1449,6 @@ struct PArgs {
   const double* sums; // per-plane sums [2][nplanes] (POP_TOTALAPPLY)
   Beams bm;
 };
-
-// Poynting z of one (Ex, Hy, Ey, Hx) sample (dtmm/field.py:771-799).
-__host__ __device__ inline double intensity4(const cd v[4]) {
-  return (v[0].x * v[1].x + v[0].y * v[1].y) - (v[2].x * v[3].x + v[2].y * v[3].y);
-}
 
 // Forward-mode projection F diag(1, 0, 1, 0) F^-1 of an isotropic medium at
 // in-plane wave vector beta (direction (c, sn)) in closed form

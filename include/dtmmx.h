@@ -225,6 +225,57 @@ int dtmmx_transfer_adjoint(const dtmmx_plan* plan, const dtmmx_stack* stack, con
 int dtmmx_fft2(void* data, int batch, int height, int width, int inverse, int precision,
                void* stream);
 
+/* Polarizing-microscope specter of transferred fields, the step after the
+ * transfer: dtmm/field_viewer.py:70-143 calculate_pom_field (input_fft =
+ * output_fft = False, window None) summed over fields as in
+ * FieldViewer._calculate_specter (:1262-1289), with the specter of
+ * dtmm/field.py:800-845 field2specter / field2spectersum.
+ * Per field: the two polarised fields are combined with the Jones weights
+ * (n_pol 2, field_viewer.py:122-127); the combination is diffracted by
+ * field_diffraction_matrix (dtmm/diffract.py:107-119: F P(k0 d) F^-1 of the
+ * isotropic medium eps, zero where beta >= betamax; mode 0 both directions,
+ * +1 forward, -1 backward modes only) when diffraction != 0; the analyser
+ * matrix pmat (per wavelength or shared) is applied per pixel
+ * (field_viewer.py:140-141); the Poynting z component of the result is
+ * summed over the n_fields fields into specter[H][W][n_wavelengths] (real:
+ * float for DTMMX_C64, double for DTMMX_C128; overwritten). */
+typedef struct {
+  int height, width;
+  int n_wavelengths;  /* <= DTMMX_MAX_WAVELENGTHS                          */
+  const double* k0;   /* [n_wavelengths] host: 2 pi pixelsize / lambda     */
+  int n_fields;       /* fields summed into the specter                    */
+  int n_pol;          /* 1: field [n_fields][n_wavelengths][4][H][W];
+                         2: field [n_fields][2][n_wavelengths][4][H][W]     */
+  double jones[4];    /* n_pol 2: re, im of the x weight, re, im of the y  */
+  int diffraction;    /* 1: apply the diffraction matrix; 0: none          */
+  int mode;           /* 0, +1 or -1 (see above)                           */
+  double d;           /* focus: propagation distance, pixel units          */
+  double eps_re, eps_im; /* isotropic medium (n^2)                         */
+  double betamax;
+  const double* pmat; /* host [n_pmat][4][4] complex (re, im), or NULL     */
+  int n_pmat;         /* 1 (shared) or n_wavelengths                       */
+  int precision;      /* DTMMX_C64 or DTMMX_C128                            */
+} dtmmx_pom_desc;
+
+size_t dtmmx_pom_workspace_size(const dtmmx_pom_desc* desc);
+/* Enqueues on `stream` after a short synchronous upload of the small tables
+ * (analyser matrices, twiddles, betamax mask) into the workspace. */
+int dtmmx_pom_specter(const dtmmx_pom_desc* desc, const void* field, void* specter,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* Specter to colour (dtmm/color.py:133-200 specter2color): XYZ = specter .
+ * cmf (cmf: host [n_wavelengths][3]); norm_mode 0: no normalisation, 1:
+ * XYZ / norm, 2: XYZ / max of all XYZ values (norm = True); gray: Y in all
+ * three channels, else the sRGB matrix of D65 white; gamma_mode 0: none, 1:
+ * sRGB curve (gamma = True), 2: v^(1/gamma) clipped to [0, 1].
+ * specter [n_pixels][n_wavelengths] -> rgb [n_pixels][3], real of the
+ * given precision (float for DTMMX_C64, double for DTMMX_C128). */
+size_t dtmmx_specter2color_workspace_size(long long n_pixels, int n_wavelengths);
+int dtmmx_specter2color(const void* specter, void* rgb, long long n_pixels, int n_wavelengths,
+                        const double* cmf, int norm_mode, double norm, int gamma_mode,
+                        double gamma, int gray, int precision, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
 /* Optional kernel timing (no counterpart in the reference, which has no
  * profiling hooks): between begin and end every stride-th launch of each
  * per-layer kernel class is bracketed by HIP events on its stream (a stride

@@ -18,6 +18,9 @@ Scope (what the GPU engine implements and this module checks):
 * ``transfer_2x2`` with reflection in {0, 1} and diffraction 1, or reflection 0
   and diffraction 0, npass=1 (dtmm/transfer.py, dtmm/propagate_2x2.py);
   reflection 1 applies the Fresnel interface transmission of dtmm/tmm.py tr_mat
+* the polarizing-microscope specter after the transfer: ``calculate_pom_field``,
+  ``field2specter``, ``pom_specter`` (dtmm/field_viewer.py, field.py), the Jones
+  4x4 matrices (dtmm/jones.py, jones4.py) and ``specter2color`` (dtmm/color.py)
 * ``illumination_data`` and ``field2intensity``            (dtmm/field.py),
   with the eigen-wave snapping of ``window=None``          (dtmm/wave.py)
 
@@ -44,6 +47,9 @@ __all__ = [
     "fft_window", "fft_windows", "fft_betaxy_mean", "fft_mask",
     "DTMM_NORM_FFT", "DTMM_NORM_REF", "S_mat", "Etri_mat", "tr_mat", "E_tr_matrix", "exp_notch_filter", "denoise_fftfield", "denoise_field",
     "normalize_field", "project_normalized_fft", "project_normalized_total", "total_intensity",
+    "polarizer", "as4x4", "jonesmat4x4", "ray_jonesmat4x4", "field2specter",
+    "calculate_pom_field", "pom_specter", "XYZ2RGBD65", "apply_gamma", "apply_srgb_gamma",
+    "specter2color",
 ]
 
 # dtmm/dtmm.ini:6 (betamax = 0.8) and dtmm/conf.py:148, :503-510 (nin/nout
@@ -1500,3 +1506,124 @@ def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensi
         norm = norm * ratio[..., None, None]
     field = norm[..., None, :, :] * field
     return field, wavelengths, pixelsize
+
+
+# ---------------------------------------------------------------------------
+# jones.py, jones4.py, field_viewer.py, color.py (the step after the transfer)
+# ---------------------------------------------------------------------------
+
+def polarizer(jones):
+    """j j^H (dtmm/jones.py:236-278)."""
+    j = np.asarray(jones, CDTYPE)
+    return j[..., :, None] * np.conj(j[..., None, :])
+
+
+def as4x4(jmat):
+    """J on the forward, conj(J) on the backward modes (dtmm/jones.py:371-397)."""
+    jmat = np.asarray(jmat, CDTYPE)
+    out = np.zeros(jmat.shape[:-2] + (4, 4), CDTYPE)
+    out[..., 0::2, 0::2] = jmat
+    out[..., 1::2, 1::2] = np.conj(jmat)
+    return out
+
+
+def jonesmat4x4(jmat, fmat):
+    """F as4x4(J) F^-1 with F normalised (dtmm/jones4.py:75-101)."""
+    f = normalize_f(fmat)
+    return dotmm(f, dotmm(as4x4(jmat), inv(f)))
+
+
+def ray_jonesmat4x4(jmat, beta=0., phi=0., epsv=(1., 1., 1.), epsa=(0., 0., 0.)):
+    """(dtmm/jones4.py:205-246)."""
+    _, f = alphaf(beta, phi, epsv, epsa)
+    c, s = np.cos(phi), np.sin(phi)
+    r = np.asarray([[c, -s], [s, c]], CDTYPE)
+    return jonesmat4x4(np.swapaxes(r, -1, -2) @ np.asarray(jmat, CDTYPE) @ r, f)
+
+
+def field2specter(field):
+    """Poynting z per wavelength with the wavelength axis last:
+    (..., nlam, 4, H, W) -> (..., H, W, nlam) (dtmm/field.py:800-841)."""
+    return np.moveaxis(field2intensity(field), -3, -1)
+
+
+def calculate_pom_field(field, jvec=None, pmat=None, dmat=None):
+    """Jones combination of the x/y fields, diffraction by dmat, analyser
+    pmat in real space (dtmm/field_viewer.py:70-143, input_fft = output_fft =
+    False, window None)."""
+    field = np.asarray(field, CDTYPE)
+    if jvec is not None:
+        if field.shape[-5] != 2:
+            raise ValueError("Invalid field shape.")
+        c, s = jvec
+        field = field[..., 0, :, :, :, :] * c + field[..., 1, :, :, :, :] * s
+    if dmat is not None:
+        field = diffract(field, dmat)
+    if pmat is not None:
+        field = dotmf(pmat, field)
+    return field
+
+
+def pom_specter(field, ks, jvec=None, pmat=None, focus=0., n=1., mode=None,
+                betamax=BETAMAX, diffraction=True):
+    """FieldViewer.calculate_specter (dtmm/field_viewer.py:1240-1330): the
+    diffraction matrix of the immersion medium (:1017-1029), then per leading
+    field calculate_pom_field and field2specter, summed."""
+    field = np.asarray(field, CDTYPE)
+    ks = np.asarray(ks, FDTYPE)
+    dmat = None
+    if diffraction:
+        dmat = field_diffraction_matrix(field.shape[-2:], ks, d=focus,
+                                        epsv=refind2eps([n] * 3), epsa=(0., 0., 0.),
+                                        mode=mode, betamax=betamax)
+    if pmat is not None:
+        pmat = np.asarray(pmat, CDTYPE)
+        if pmat.ndim == 3:  # one per wavelength, broadcast over pixels (:969-975)
+            pmat = pmat[:, None, None, :, :]
+    nd = 5 if jvec is None else 6
+    tmp = field.reshape((-1,) + field.shape[-(nd - 1):])
+    spec = 0.
+    for data in tmp:
+        spec = spec + field2specter(calculate_pom_field(data, jvec, pmat, dmat))
+    return spec
+
+
+# sRGB of D65 white (dtmm/color.py:39-50): the IEC 61966-2-1 matrix and curve.
+XYZ2RGBD65 = np.array([[3.2404542, -1.5371385, -0.4985314],
+                       [-0.9692660, 1.8760108, 0.0415560],
+                       [0.0556434, -0.2040259, 1.0572252]])
+
+
+def apply_gamma(value, gamma):
+    """(dtmm/color.py:52-69)."""
+    v = np.asarray(value, FDTYPE)
+    return np.where(v > 1., 1., np.where(v < 0, 0., np.abs(v) ** (1. / gamma)))
+
+
+def apply_srgb_gamma(value):
+    """(dtmm/color.py:71-85)."""
+    v = np.asarray(value, FDTYPE)
+    lo = np.where(v < 0., 0., 12.92 * v)
+    hi = np.where(v > 1., 1., (1 + 0.055) * np.abs(v) ** (1 / 2.4) - 0.055)
+    return np.where(v < 0.0031308, lo, hi)
+
+
+def specter2color(spec, cmf, norm=False, gamma=True, gray=False):
+    """(dtmm/color.py:111-200)."""
+    cmf = np.asarray(cmf, FDTYPE)
+    if cmf.shape[-1] != 3:
+        raise ValueError("Grayscale cmf! Cannot convert to color.")
+    out = np.asarray(spec, FDTYPE) @ cmf
+    if norm is True:
+        out = out / out.max()
+    elif norm != 0:
+        out = out / norm
+    if gray:
+        out = np.repeat(out[..., 1:2], 3, axis=-1)
+    else:
+        out = out @ XYZ2RGBD65.T
+    if gamma is True:
+        out = apply_srgb_gamma(out)
+    elif gamma is not False:
+        out = apply_gamma(out, gamma)
+    return out

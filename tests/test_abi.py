@@ -358,3 +358,58 @@ def test_mode_resolution_no_diffraction():
         _resolve_modes("4x4", 3, 1, None, None, True)
     with pytest.raises(NotImplementedError):
         _resolve_modes("4x4", 1, 3, None, None, True)
+
+
+def _pom_desc(**kw):
+    d = _lib.PomDesc()
+    ks = np.asarray([1.0, 1.2])
+    d.height, d.width, d.n_wavelengths = 32, 48, 2
+    d.k0 = _lib.dptr(ks)
+    d.n_fields, d.n_pol, d.diffraction, d.mode = 2, 1, 1, 0
+    d.eps_re, d.eps_im, d.betamax, d.d = 1.0, 0.0, 0.8, 0.0
+    d.precision = _lib.DTMMX_C64
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d, ks
+
+
+def test_pom_workspace_size_and_rejects_bad_descriptions():
+    """Host-side sizing and validation of dtmmx_pom_specter (no GPU call)."""
+    lib = _lib.load()
+    d, ks = _pom_desc()
+    n = lib.dtmmx_pom_workspace_size(ctypes.byref(d))
+    r = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    assert n == r(2 * 2 * 4 * 32 * 48 * 8) + r(48 * 8) + r(32 * 8) + r(2 * 32 * 4)
+    d0, ks0 = _pom_desc(diffraction=0)
+    assert lib.dtmmx_pom_workspace_size(ctypes.byref(d0)) == 0
+    bad = [dict(n_pol=3), dict(mode=2), dict(n_fields=0), dict(n_wavelengths=0),
+           dict(n_wavelengths=_lib.MAX_WAVELENGTHS + 1), dict(precision=7)]
+    for kw in bad:
+        db, ksb = _pom_desc(**kw)
+        assert lib.dtmmx_pom_workspace_size(ctypes.byref(db)) == 0, kw
+        rc = lib.dtmmx_pom_specter(ctypes.byref(db), None, None, None, 0, None)
+        assert rc == _lib.DTMMX_E_ARG, kw
+    du, ksu = _pom_desc(height=30)
+    assert lib.dtmmx_pom_specter(ctypes.byref(du), None, None, None, 0, None) == \
+        _lib.DTMMX_E_UNSUPPORTED
+    kneg = np.asarray([1.0, -1.0])
+    dn, _ = _pom_desc()
+    dn.k0 = _lib.dptr(kneg)
+    assert lib.dtmmx_pom_specter(ctypes.byref(dn), None, None, None, 0, None) == _lib.DTMMX_E_ARG
+    # valid description, missing buffers
+    assert lib.dtmmx_pom_specter(ctypes.byref(d), None, None, None, 0, None) == _lib.DTMMX_E_ARG
+
+
+def test_specter2color_rejects_bad_arguments():
+    lib = _lib.load()
+    cmf = np.ones((3, 3))
+    assert lib.dtmmx_specter2color_workspace_size(1000, 3) == 256 + 4 * 8
+    p = ctypes.c_void_p(8)  # never dereferenced: every call below fails validation
+    for args in [(0, 3, 0, 0), (10, 0, 0, 0), (10, 3, 3, 0), (10, 3, 0, 3)]:
+        npix, nlam, nm, gm = args
+        rc = lib.dtmmx_specter2color(p, p, npix, nlam, _lib.dptr(cmf), nm, 0., gm, 0., 0,
+                                     _lib.DTMMX_C64, p, 1 << 20, None)
+        assert rc == _lib.DTMMX_E_ARG, args
+    rc = lib.dtmmx_specter2color(p, p, 10, 3, _lib.dptr(cmf), 0, 0., 0, 0., 0, _lib.DTMMX_C64,
+                                 p, 8, None)
+    assert rc == _lib.DTMMX_E_ARG
