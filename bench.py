@@ -237,6 +237,79 @@ def run_c4(args):
     print(json.dumps(rec), flush=True)
 
 
+def run_pom(args):
+    """The polarizing-microscope step after the transfer on its output shape
+    (2 polarisations x 9 wavelengths x 4 x 512 x 512): x polarizer, crossed
+    analyser, refocus by 3 pixels in a medium of index 1.5."""
+    import torch
+    import dtmm_amd as dm
+    from dtmm_amd import pom
+    H = W = args.size
+    nlam = args.wavelengths
+    wl = np.linspace(400., 700., nlam)
+    ks = dm.k0(wl, 100.)
+    dt = torch.complex64 if args.precision == "c64" else torch.complex128
+    g = torch.Generator(device="cuda").manual_seed(3)
+    field = torch.randn((1, 2, nlam, 4, H, W), dtype=dt, device="cuda", generator=g)
+    pmat = pom.jonesmat4x4(pom.polarizer([0., 1.]), 1.5)
+    kw = dict(jvec=(1., 0.), pmat=pmat, focus=3., n=1.5, betamax=0.8)
+    for _ in range(args.warmup):
+        spec = pom.calculate_pom_specter(field, ks, **kw)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        spec = pom.calculate_pom_specter(field, ks, **kw)
+    e1.record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    ev_ms = e0.elapsed_time(e1) / args.steps
+    if not torch.isfinite(spec).all():
+        log("non-finite specter")
+        sys.exit(1)
+    units = H * W * nlam
+    # algorithmic bytes: both polarised fields read once, the specter written once
+    csize = 8 if args.precision == "c64" else 16
+    alg = 2 * nlam * 4 * H * W * csize + H * W * nlam * csize // 2
+    log("pom step %.3f ms per call (events %.3f ms)" % (ms_per_step, ev_ms))
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        cores = len(os.sched_getaffinity(0))
+        oracle.set_fft_workers(cores)
+        nl = min(2, nlam)
+        fn = field[:, :, :nl].cpu().numpy().astype(np.complex128)
+        t = time.time()
+        oracle.pom_specter(fn, ks[:nl], **kw)
+        tc = time.time() - t
+        cpu = {"value": H * W * nl / tc, "unit": "pixel*lambda/s", "cores": cores,
+               "kind": "port",
+               "sample": "oracle pom_specter (complex128) on %d of %d wavelengths of the same "
+                         "field, %.2f s; scipy.fft on %d threads" % (nl, nlam, tc, cores)}
+    rec = {
+        "metric": "pixel*lambda/s of the polarizing-microscope specter (calculate_pom_specter)",
+        "value": units / (ms_per_step / 1e3),
+        "unit": "pixel*lambda/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (fp32 complex, fp64 operators)" if args.precision == "c64" else "c128",
+        "data": "synthetic (seeded random field of the transfer's output shape)",
+        "config": {"workload": "POM specter of a %dx%d field, %d wavelengths x 2 pol (x "
+                               "polarizer, crossed analyser, focus 3 px in n 1.5)"
+                               % (H, W, nlam),
+                   "height": H, "width": W, "wavelengths": nlam, "precision": args.precision},
+        "roofline": {"bound": "hbm", "kernel": "whole step (7 launches)",
+                     "achieved": alg / (ms_per_step / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": alg / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": alg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def build_inputs_c5(size, layers, nlam):
     import dtmm_amd as dm
     from dtmm_amd.samples import droplet_array_data, droplet_grid_centres
@@ -332,9 +405,10 @@ def cpu_baseline(fd, optical, method, target_s=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "pom"],
                     help="c2: 512^2 x 128 random texture (default); c3: batch of uneven "
-                         "samples; c5: 1024^2 x 200 droplet grid with npass=3, diffraction=5")
+                         "samples; c5: 1024^2 x 200 droplet grid with npass=3, diffraction=5; "
+                         "pom: the polarizing-microscope specter of a 512^2 x 9 x 2 field")
     ap.add_argument("--samples", type=int, default=64, help="c3: batch size")
     ap.add_argument("--cpu-crop", type=int, default=96, help="c5: oracle crop size")
     ap.add_argument("--steps", type=int, default=5)
@@ -387,6 +461,9 @@ def main():
         return
     if args.workload == "c4":
         run_c4(args)
+        return
+    if args.workload == "pom":
+        run_pom(args)
         return
     lib = _lib.load()
     c5 = args.workload == "c5"

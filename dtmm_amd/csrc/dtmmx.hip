@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1364,7 +1365,8 @@ struct PomJob {
   void* tw_w;
   void* tw_h;
   int* amax;           // [nlam][H] betamax mask rows
-  const void* pmat;    // [npm][4][4] complex double, or null
+  const void* pmat;    // device [npm][4][4] complex double (npm > 1), or null
+  const double* pm0;   // host [4][4] complex: the shared analyser (npm == 1), or null
 };
 
 struct ColorJob {
@@ -1372,7 +1374,8 @@ struct ColorJob {
   void* out;
   long long n;
   int nlam;
-  const double* cmf;   // device [nlam][3]
+  const double* cmf;   // device [nlam][3] (nlam > COLOR_MAXL), else null
+  const double* cmf_h; // host [nlam][3] (nlam <= COLOR_MAXL: kernel arguments)
   double* part;        // device [nblocks]
   int norm_mode;
   double norm;
@@ -1401,7 +1404,9 @@ static int pom_impl(const PomJob& j, hipStream_t s) {
   a.jc = mk<double>(d->jones[0], d->jones[1]);
   a.js = mk<double>(d->jones[2], d->jones[3]);
   a.pmat = (const cd*)j.pmat;
-  a.npm = d->n_pmat;
+  a.npm = d->pmat ? d->n_pmat : 0;
+  if (j.pm0)
+    for (int i = 0; i < 16; ++i) a.pm0[i] = mk<double>(j.pm0[2 * i], j.pm0[2 * i + 1]);
   a.spec = (T*)j.spec;
   const unsigned gx = unsigned((HW + NT - 1) / NT);
   if (!d->diffraction) {
@@ -1459,6 +1464,8 @@ static int color_impl(const ColorJob& j, hipStream_t s) {
   a.n = j.n;
   a.nlam = j.nlam;
   a.cmf = j.cmf;
+  if (!j.cmf)
+    for (int i = 0; i < 3 * j.nlam; ++i) a.cmf_v[i] = j.cmf_h[i];
   a.part = j.part;
   a.nblocks = int((j.n + NT - 1) / NT);
   a.norm = j.norm_mode == 1 ? j.norm : 0.;
@@ -2074,6 +2081,81 @@ static PomLayout pom_layout(const dtmmx_pom_desc* d) {
   return L;
 }
 
+// Device tables of the polarizing-microscope diffraction (twiddles of both
+// sides, betamax rows) kept per device, grid, wavelengths and betamax for
+// the life of the process, so that a repeated call enqueues its kernels
+// without uploading anything.  At most POM_TABLES sets; beyond that a call
+// uploads its tables into the workspace and waits for the upload.
+struct PomTables {
+  int dev, prec, H, W;
+  std::vector<double> k0;
+  double betamax;
+  void* tw_w;
+  void* tw_h;
+  int* amax;
+};
+static std::mutex g_pom_mu;
+static std::vector<PomTables> g_pom_tables;
+constexpr size_t POM_TABLES = 16;
+
+// 0: tables found or made, 1: the cache is full, < 0: error.
+static int pom_tables(const dtmmx_pom_desc* d, void** tw_w, void** tw_h, int** amax) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::vector<double> k0(d->k0, d->k0 + d->n_wavelengths);
+  std::lock_guard<std::mutex> lock(g_pom_mu);
+  for (const PomTables& t : g_pom_tables)
+    if (t.dev == dev && t.prec == d->precision && t.H == d->height && t.W == d->width &&
+        t.k0 == k0 && t.betamax == d->betamax) {
+      *tw_w = t.tw_w;
+      *tw_h = t.tw_h;
+      *amax = t.amax;
+      return 0;
+    }
+  if (g_pom_tables.size() >= POM_TABLES) return 1;
+  PomTables t{dev, d->precision, d->height, d->width, k0, d->betamax, nullptr, nullptr, nullptr};
+  const size_t cs = d->precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  dtmmx_plan_desc pd{};
+  pd.height = d->height;
+  pd.width = d->width;
+  pd.n_wavelengths = d->n_wavelengths;
+  pd.k0 = d->k0;
+  pd.betamax = d->betamax;
+  const std::vector<int> rows = betamax_rows(&pd);
+  auto release = [&]() {
+    (void)hipFree(t.tw_w);
+    (void)hipFree(t.tw_h);
+    (void)hipFree(t.amax);
+  };
+  if (hipMalloc(&t.tw_w, size_t(d->width) * cs) != hipSuccess ||
+      hipMalloc(&t.tw_h, size_t(d->height) * cs) != hipSuccess ||
+      hipMalloc((void**)&t.amax, rows.size() * sizeof(int)) != hipSuccess) {
+    release();
+    return fail(DTMMX_E_NOMEM, "pom tables: device allocation failed");
+  }
+  hipError_t e;
+  if (d->precision == DTMMX_C64) {
+    auto a = twiddles<float>(d->width), b = twiddles<float>(d->height);
+    e = hipMemcpy(t.tw_w, a.data(), a.size() * cs, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.tw_h, b.data(), b.size() * cs, hipMemcpyHostToDevice);
+  } else {
+    auto a = twiddles<double>(d->width), b = twiddles<double>(d->height);
+    e = hipMemcpy(t.tw_w, a.data(), a.size() * cs, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.tw_h, b.data(), b.size() * cs, hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(t.amax, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    release();
+    return fail(DTMMX_E_HIP, std::string("pom tables: ") + hipGetErrorString(e));
+  }
+  *tw_w = t.tw_w;
+  *tw_h = t.tw_h;
+  *amax = t.amax;
+  g_pom_tables.push_back(std::move(t));
+  return 0;
+}
+
 size_t dtmmx_pom_workspace_size(const dtmmx_pom_desc* d) {
   if (validate_pom(d) != DTMMX_OK) return 0;
   return pom_layout(d).total;
@@ -2093,7 +2175,9 @@ int dtmmx_pom_specter(const dtmmx_pom_desc* d, const void* field, void* specter,
   j.field = field;
   j.spec = specter;
   bool upload = false;
-  if (d->pmat) {
+  if (d->pmat && d->n_pmat == 1) {
+    j.pm0 = d->pmat;  // passed by value in the kernel arguments
+  } else if (d->pmat) {
     HIP_TRY(hipMemcpyAsync(w + L.pmat, d->pmat, size_t(d->n_pmat) * 16 * 16,
                            hipMemcpyHostToDevice, s));
     j.pmat = w + L.pmat;
@@ -2102,8 +2186,13 @@ int dtmmx_pom_specter(const dtmmx_pom_desc* d, const void* field, void* specter,
   std::vector<cpx<float>> tf_w, tf_h;
   std::vector<cpx<double>> td_w, td_h;
   std::vector<int> amax;
+  int cached = 1;
   if (d->diffraction) {
     j.wfield = w + L.field;
+    cached = pom_tables(d, &j.tw_w, &j.tw_h, &j.amax);
+    if (cached < 0) return cached;
+  }
+  if (d->diffraction && cached == 1) {
     j.tw_w = w + L.tw_w;
     j.tw_h = w + L.tw_h;
     j.amax = (int*)(w + L.amax);
@@ -2157,15 +2246,19 @@ int dtmmx_specter2color(const void* specter, void* rgb, long long n_pixels, int 
   if (!workspace || workspace_bytes < need) return fail(DTMMX_E_ARG, "workspace too small");
   hipStream_t s = (hipStream_t)stream;
   char* w = (char*)workspace;
-  HIP_TRY(hipMemcpyAsync(w, cmf, size_t(n_wavelengths) * 3 * sizeof(double),
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
   ColorJob j{};
+  if (n_wavelengths > COLOR_MAXL) {
+    HIP_TRY(hipMemcpyAsync(w, cmf, size_t(n_wavelengths) * 3 * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    j.cmf = (const double*)w;
+  } else {
+    j.cmf_h = cmf;
+  }
   j.spec = specter;
   j.out = rgb;
   j.n = n_pixels;
   j.nlam = n_wavelengths;
-  j.cmf = (const double*)w;
   j.part = (double*)(w + (size_t(n_wavelengths) * 3 * sizeof(double) + 255) / 256 * 256);
   j.norm_mode = norm_mode;
   j.norm = norm;

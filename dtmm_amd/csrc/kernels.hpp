@@ -1269,9 +1269,20 @@ struct PomArgs {
   long long HW;
   int nlam, nf, npol;
   cd jc, js;          // weights of the x and y polarised fields (npol == 2)
-  const cd* pmat;     // [npm][4][4] analyser matrices, or null
-  int npm;
+  const cd* pmat;     // [npm][4][4] analyser matrices (npm > 1), or null
+  int npm;            // 0: no analyser, 1: pm0, > 1: pmat per wavelength
+  cd pm0[16];         // the shared analyser matrix (npm == 1)
 };
+
+__device__ __forceinline__ void mat4v(const cd* m, const cd v[4], cd w[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    cd t = mk<double>(0.);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t = t + m[i * 4 + j] * v[j];
+    w[i] = t;
+  }
+}
 
 // One thread per (pixel, plane): dst = jc * field_x + js * field_y.
 template <typename T>
@@ -1292,35 +1303,57 @@ template <typename T>
 __global__ __launch_bounds__(NT) void pom_specter_kernel(PomArgs<T> A) {
   const long long p = (long long)blockIdx.x * NT + threadIdx.x;
   if (p >= A.HW) return;
-  for (int lam = 0; lam < A.nlam; ++lam) {
-    double acc = 0.;
+  const cpx<T>* __restrict__ src = A.src;
+  T* __restrict__ spec = A.spec;
+  const size_t HW = size_t(A.HW);
+  // four wavelengths per round: all their loads are issued before any math
+  constexpr int LB = 4;
+  for (int lam0 = 0; lam0 < A.nlam; lam0 += LB) {
+    const int nb = min(LB, A.nlam - lam0);
+    double acc[LB] = {0., 0., 0., 0.};
     for (int f = 0; f < A.nf; ++f) {
-      cd v[4];
-      if (A.npol == 2) {
-        const cpx<T>* s = A.src + (size_t(f) * 2 * A.nlam + lam) * 4 * A.HW + p;
-        const size_t py = size_t(A.nlam) * 4 * A.HW;
-        for (int c = 0; c < 4; ++c)
-          v[c] = A.jc * cast<double>(s[c * A.HW]) + A.js * cast<double>(s[c * A.HW + py]);
-      } else {
-        const cpx<T>* s = A.src + (size_t(f) * A.nlam + lam) * 4 * A.HW + p;
-        for (int c = 0; c < 4; ++c) v[c] = cast<double>(s[c * A.HW]);
-      }
-      if (A.pmat) {
-        const cd* m = A.pmat + (A.npm > 1 ? lam : 0) * 16;
-        cd w[4];
-        for (int i = 0; i < 4; ++i) {
-          cd t = mk<double>(0.);
-          for (int j = 0; j < 4; ++j) t = t + m[i * 4 + j] * v[j];
-          w[i] = t;
+      cd v[LB][4];
+#pragma unroll
+      for (int k = 0; k < LB; ++k) {
+        if (k >= nb) break;
+        const int lam = lam0 + k;
+        if (A.npol == 2) {
+          const cpx<T>* s = src + (size_t(f) * 2 * A.nlam + lam) * 4 * HW + p;
+          const size_t py = size_t(A.nlam) * 4 * HW;
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            v[k][c] = A.jc * cast<double>(s[c * HW]) + A.js * cast<double>(s[c * HW + py]);
+        } else {
+          const cpx<T>* s = src + (size_t(f) * A.nlam + lam) * 4 * HW + p;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) v[k][c] = cast<double>(s[c * HW]);
         }
-        acc += intensity4(w);
-      } else {
-        acc += intensity4(v);
+      }
+#pragma unroll
+      for (int k = 0; k < LB; ++k) {
+        if (k >= nb) break;
+        // separate branches keep each matrix in its own address space
+        // (kernel arguments / global), so its uniform loads stay scalar
+        if (A.npm == 0) {
+          acc[k] += intensity4(v[k]);
+        } else if (A.npm == 1) {
+          cd w[4];
+          mat4v(A.pm0, v[k], w);
+          acc[k] += intensity4(w);
+        } else {
+          cd w[4];
+          mat4v(A.pmat + (lam0 + k) * 16, v[k], w);
+          acc[k] += intensity4(w);
+        }
       }
     }
-    A.spec[size_t(p) * A.nlam + lam] = T(acc);
+#pragma unroll
+    for (int k = 0; k < LB; ++k)
+      if (k < nb) spec[size_t(p) * A.nlam + lam0 + k] = T(acc[k]);
   }
 }
+
+constexpr int COLOR_MAXL = 128;  // wavelengths whose cmf travels in the kernel arguments
 
 template <typename T>
 struct ColorArgs {
@@ -1328,7 +1361,8 @@ struct ColorArgs {
   T* out;             // [n][3]
   long long n;
   int nlam;
-  const double* cmf;  // [nlam][3]
+  const double* cmf;  // device [nlam][3] (nlam > COLOR_MAXL), else null
+  double cmf_v[COLOR_MAXL * 3];
   double* part;       // per-block maxima of the XYZ values (stage 0)
   int nblocks;
   double norm;        // divide XYZ by this (stage 2 reads part[0] when auto)
@@ -1406,7 +1440,7 @@ __global__ __launch_bounds__(NT) void color_kernel(ColorArgs<T> A) {
       if (STAGE == 2 && A.norm_auto) {
         for (int k = 0; k < 3; ++k) xyz[k] = double(A.out[size_t(p) * 3 + k]);
       } else {
-        spec_to_xyz<T>(A.spec + size_t(p) * A.nlam, A.cmf, A.nlam, xyz);
+        spec_to_xyz<T>(A.spec + size_t(p) * A.nlam, A.cmf ? A.cmf : A.cmf_v, A.nlam, xyz);
       }
     }
     if constexpr (STAGE == 0) {

@@ -258,8 +258,12 @@ typedef struct {
 } dtmmx_pom_desc;
 
 size_t dtmmx_pom_workspace_size(const dtmmx_pom_desc* desc);
-/* Enqueues on `stream` after a short synchronous upload of the small tables
- * (analyser matrices, twiddles, betamax mask) into the workspace. */
+/* Enqueues its kernels on `stream` and returns.  The first call for a
+ * device, grid, set of k0, betamax and precision allocates and uploads its
+ * FFT twiddles and betamax rows and keeps them for the life of the process
+ * (at most 16 such sets; further sets are uploaded into the workspace and
+ * waited for).  A shared analyser travels in the kernel arguments; one per
+ * wavelength (n_pmat > 1) is uploaded into the workspace and waited for. */
 int dtmmx_pom_specter(const dtmmx_pom_desc* desc, const void* field, void* specter,
                       void* workspace, size_t workspace_bytes, void* stream);
 
@@ -269,7 +273,9 @@ int dtmmx_pom_specter(const dtmmx_pom_desc* desc, const void* field, void* spect
  * three channels, else the sRGB matrix of D65 white; gamma_mode 0: none, 1:
  * sRGB curve (gamma = True), 2: v^(1/gamma) clipped to [0, 1].
  * specter [n_pixels][n_wavelengths] -> rgb [n_pixels][3], real of the
- * given precision (float for DTMMX_C64, double for DTMMX_C128). */
+ * given precision (float for DTMMX_C64, double for DTMMX_C128).  The cmf
+ * travels in the kernel arguments for up to 128 wavelengths (no upload);
+ * a longer one is uploaded into the workspace and waited for. */
 size_t dtmmx_specter2color_workspace_size(long long n_pixels, int n_wavelengths);
 int dtmmx_specter2color(const void* specter, void* rgb, long long n_pixels, int n_wavelengths,
                         const double* cmf, int norm_mode, double norm, int gamma_mode,

@@ -139,3 +139,31 @@ def test_specter2color_matches_oracle(dtype, norm, gamma, gray):
     err = np.abs(got - ref).max()
     flush("  abs err %.3e" % err)
     assert err < (1e-12 if dtype == "float64" else 1e-5)
+
+
+@pytest.mark.gpu
+def test_specter2color_long_cmf():
+    """More wavelengths than the kernel arguments carry: the cmf is uploaded."""
+    from dtmm_amd import pom
+    flush("case specter2color long cmf")
+    rng = np.random.default_rng(4)
+    spec = rng.random((9, 11, 150)) * 0.01
+    cmf = rng.random((150, 3)) * 0.1
+    ref = oracle.specter2color(spec, cmf, norm=True)
+    got = pom.specter2color(spec, cmf, norm=True)
+    assert np.abs(got - ref).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_pom_specter_repeated_calls_reuse_tables():
+    """Calls alternating between two grids and two betamax values (kept
+    device tables) give the oracle's result every time."""
+    from dtmm_amd import pom
+    flush("case pom repeated calls")
+    ks = oracle.k0([500., 600.], 100.)
+    for rep in range(2):
+        for shape, bm in (((32, 48), 0.8), ((64, 40), 0.6), ((32, 48), 0.6)):
+            f = _field((1,), 2, shape, 40 + rep)
+            ref = oracle.pom_specter(f, ks, focus=2., n=1.3, betamax=bm)
+            got = pom.calculate_pom_specter(f, ks, focus=2., n=1.3, betamax=bm)
+            assert rel_err(got, ref) < 1e-9, (rep, shape, bm)
