@@ -528,6 +528,11 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
   auto eff = [&](int l) {
     return mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
   };
+  // Layers with biaxial voxels get the per-beam operator table of the
+  // window's mean beams (built by the row pass that receives etab_ws).
+  auto win_layer_args = [&](int l, double ds) {
+    return layer_args<T>(S, HW, l, ds, 1, status, etab);
+  };
   cpx<T>* A = out;
   TRY(row_pass<T>(P, A, b, 4, OP_FWD, s));
   TRY(col_pass<T>(P, A, b, 4, OP_FWD, s));
@@ -581,7 +586,7 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
           bw.beta[q] = P->win_beta[size_t(w) * nbeam + q];
           bw.phi[q] = P->win_phi[size_t(w) * nbeam + q];
         }
-        RowArgs<T> la = layer_args<T>(S, HW, l, ds, 1, status, etab);
+        RowArgs<T> la = win_layer_args(l, ds);
         la.acc = acc;
         la.acc_mode = (w == 0) ? 1 : 2;
         la.wcx = P->wcx;
@@ -616,7 +621,13 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
       ca.wj = P->win_j[w];
       TRY(col_pass<T>(P, A, b, 4, OP_WIN, s, &ca, &bw));
       for (int st = 0; st < ns; ++st) {
-        RowArgs<T> la = layer_args<T>(S, HW, l, ds, 1, status, etab);
+        RowArgs<T> la = win_layer_args(l, ds);
+        if (st > 0 && la.etab_ws) {
+          // the table of this window and layer is built by the first sub-step
+          la.etab = la.etab_ws;
+          la.etab_nbeam = b.nrays / P->bm.rays_per_beam;
+          la.etab_ws = nullptr;
+        }
         if (st + 1 == ns) {
           la.acc = acc;
           la.acc_mode = (w == 0) ? 1 : 2;

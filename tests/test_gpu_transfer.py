@@ -133,6 +133,17 @@ def test_diffraction_windows_nstep_offaxis():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape,precision", [((3, 32, 32), "c128"), ((2, 16, 1024), "c64")])
+def test_diffraction_windows_nstep_two_pols(shape, precision):
+    """Windows with sub-steps and two rays per beam, where the row kernel
+    takes one ray per work-group (complex128, and 1024-point rows)."""
+    flush("case windows nstep2 two pols %s %s" % (shape, precision))
+    fd, optical = random_case(shape=shape, wavelengths=(500., 650.), jones=None,
+                              beta=0.1, phi=0.3)
+    _compare(fd, optical, "4x4", precision, beta=0.1, phi=0.3, nstep=2, diffraction=3)
+
+
+@pytest.mark.gpu
 def test_diffraction_windows_two_blocks():
     flush("case windows two_blocks")
     fd, b1 = random_case(shape=(3, 32, 32), wavelengths=(550.,), jones=(1, 0))
@@ -473,6 +484,15 @@ def test_biaxial_multi_beam_nstep(method):
     beta, phi = [0., 0.2], [0., 0.8]
     fd, optical = _biaxial_case(shape=(4, 32, 48), beta=beta, phi=phi)
     _compare(fd, optical, method, "c128", beta=beta, phi=phi, nstep=2)
+
+
+@pytest.mark.gpu
+def test_biaxial_windows_nstep():
+    """Biaxial layers with sub-steps inside each window: the window's
+    operator table is built by the first sub-step and reused by the next."""
+    flush("case biaxial windows nstep2")
+    fd, optical = _biaxial_case(shape=(3, 32, 32), uniaxial_every_other=False)
+    _compare(fd, optical, "4x4", "c128", diffraction=3, nstep=2)
 
 
 @pytest.mark.gpu
