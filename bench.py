@@ -310,6 +310,96 @@ def run_pom(args):
     print(json.dumps(rec), flush=True)
 
 
+def run_tmm1d(args):
+    """The non-iterative 1D transfer matrix method on the benchmark's material:
+    one stack per pixel of the 512 x 512 x 128 random director, one
+    wavelength, an x-polarised wave at normal incidence between media 1.0
+    and 1.5."""
+    import torch
+    import dtmm_amd as dm
+    from dtmm_amd import _lib
+    size, layers = args.size, args.layers
+    optical = build_inputs(size, layers, 1)[1]
+    d, eps, ang = optical[0]
+    n = size * size
+    k = float(dm.k0(550., 100.))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")  # noqa: E731
+    kd = dev(np.repeat((np.asarray(d, np.float64) * k)[:, None], n, 1))
+    er = dev(np.moveaxis(np.asarray(eps, np.float64).reshape(layers, n, 3), -1, 0))
+    an = dev(np.moveaxis(np.asarray(ang, np.float64).reshape(layers, n, 3), -1, 0))
+    beta, phi = 0., 0.
+    f0 = np.zeros(4, np.complex128)
+    f0[0], f0[1] = 1., 1.  # x polarised unit wave in air at normal incidence
+    fin0 = torch.from_numpy(np.broadcast_to(f0, (n, 4)).copy()).to("cuda")
+    fin = torch.empty_like(fin0)
+    fout = torch.empty_like(fin0)
+    desc = _lib.TmmDesc()
+    desc.n_elements, desc.n_layers = n, layers
+    desc.kd, desc.eps_re, desc.eps_im, desc.angles = kd.data_ptr(), er.data_ptr(), None, \
+        an.data_ptr()
+    desc.beta, desc.phi, desc.n_in, desc.n_out = beta, phi, 1.0, 1.5
+    lib = _lib.load()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        fin.copy_(fin0)
+        _lib.check(lib.dtmmx_tmm_transfer4x4(ctypes.byref(desc), ctypes.c_void_p(fin.data_ptr()),
+                                             ctypes.c_void_p(fout.data_ptr()), 0, None,
+                                             ctypes.c_void_p(stream)))
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    if not torch.isfinite(torch.view_as_real(fout)).all():
+        log("non-finite output")
+        sys.exit(1)
+    units = n * layers
+    alg = units * 7 * 8 + n * 4 * 16 * 3  # kd, eps, angles per layer; fields in/out
+    log("tmm1d %.3f ms per call" % ms)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        ns = 2048
+        fo = oracle.transfer4x4(np.broadcast_to(f0, (ns, 4)), np.asarray(d, np.float64) * k,
+                                np.asarray(eps, np.float64).reshape(layers, n, 3)[:, :ns],
+                                np.asarray(ang, np.float64).reshape(layers, n, 3)[:, :ns],
+                                beta, phi, nin=1.0, nout=1.5)
+        t = time.time()
+        oracle.transfer4x4(np.broadcast_to(f0, (ns, 4)), np.asarray(d, np.float64) * k,
+                           np.asarray(eps, np.float64).reshape(layers, n, 3)[:, :ns],
+                           np.asarray(ang, np.float64).reshape(layers, n, 3)[:, :ns],
+                           beta, phi, nin=1.0, nout=1.5)
+        tc = time.time() - t
+        err = float(np.abs(fout[:ns].cpu().numpy() - fo[0]).max() / np.abs(fo[0]).max())
+        log("GPU vs oracle on %d stacks: %.2e" % (ns, err))
+        cpu = {"value": ns * layers / tc, "unit": "stack*layer/s", "cores": 1, "kind": "port",
+               "sample": "oracle transfer4x4 (numpy, complex128) on %d of the %d stacks, "
+                         "%.2f s" % (ns, n, tc)}
+    rec = {
+        "metric": "stack*layer/s of the non-iterative 1D transfer matrix method (transfer4x4)",
+        "value": units / (ms / 1e3),
+        "unit": "stack*layer/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 (fp64)",
+        "data": "synthetic (seeded random nematic director, one stack per pixel)",
+        "config": {"workload": "%d stacks of %d layers (one per pixel of the %dx%d random "
+                               "director), 550 nm, normal incidence, 4x4 with reflections"
+                               % (n, layers, size, size),
+                   "stacks": n, "layers": layers},
+        "roofline": {"bound": "hbm", "kernel": "tmm_kernel (fp64 compute, spilling)",
+                     "achieved": alg / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": alg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def build_inputs_c5(size, layers, nlam):
     import dtmm_amd as dm
     from dtmm_amd.samples import droplet_array_data, droplet_grid_centres
@@ -405,10 +495,12 @@ def cpu_baseline(fd, optical, method, target_s=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "pom"],
+    ap.add_argument("--workload", default="c2",
+                    choices=["c2", "c3", "c4", "c5", "pom", "tmm1d"],
                     help="c2: 512^2 x 128 random texture (default); c3: batch of uneven "
                          "samples; c5: 1024^2 x 200 droplet grid with npass=3, diffraction=5; "
-                         "pom: the polarizing-microscope specter of a 512^2 x 9 x 2 field")
+                         "pom: the polarizing-microscope specter of a 512^2 x 9 x 2 field; "
+                         "tmm1d: the 1D transfer matrix method, one stack per pixel")
     ap.add_argument("--samples", type=int, default=64, help="c3: batch size")
     ap.add_argument("--cpu-crop", type=int, default=96, help="c5: oracle crop size")
     ap.add_argument("--steps", type=int, default=5)
@@ -464,6 +556,9 @@ def main():
         return
     if args.workload == "pom":
         run_pom(args)
+        return
+    if args.workload == "tmm1d":
+        run_tmm1d(args)
         return
     lib = _lib.load()
     c5 = args.workload == "c5"

@@ -18,5 +18,6 @@ from .transfer import (transfer_field, transfer_batch, TransferGraph, Plan, Stac
 from .adjoint import AdjointTransfer, polarizer4x4, analyser_intensity_loss  # noqa: F401
 from .pom import calculate_pom_specter, field2specter, specter2color  # noqa: F401
 from . import pom  # noqa: F401
+from . import tmm  # noqa: F401
 
 __version__ = "0.1.0"

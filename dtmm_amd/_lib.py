@@ -51,6 +51,7 @@ EXPORTS = (
     "dtmmx_pom_specter",
     "dtmmx_specter2color_workspace_size",
     "dtmmx_specter2color",
+    "dtmmx_tmm_transfer4x4",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -123,6 +124,21 @@ class PomDesc(ctypes.Structure):
         ("pmat", _dp),
         ("n_pmat", ctypes.c_int),
         ("precision", ctypes.c_int),
+    ]
+
+
+class TmmDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_elements", ctypes.c_longlong),
+        ("n_layers", ctypes.c_int),
+        ("kd", ctypes.c_void_p),
+        ("eps_re", ctypes.c_void_p),
+        ("eps_im", ctypes.c_void_p),
+        ("angles", ctypes.c_void_p),
+        ("beta", ctypes.c_double),
+        ("phi", ctypes.c_double),
+        ("n_in", ctypes.c_double),
+        ("n_out", ctypes.c_double),
     ]
 
 
@@ -209,6 +225,10 @@ def load():
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
     lib.dtmmx_specter2color.restype = ctypes.c_int
+    lib.dtmmx_tmm_transfer4x4.argtypes = [ctypes.POINTER(TmmDesc), ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p]
+    lib.dtmmx_tmm_transfer4x4.restype = ctypes.c_int
     if lib.dtmmx_abi_version() != ABI_VERSION:
         raise RuntimeError("dtmmx ABI version mismatch")
     _lib = lib

@@ -2279,6 +2279,51 @@ int dtmmx_specter2color(const void* specter, void* rgb, long long n_pixels, int 
   return precision == DTMMX_C64 ? color_c64(j, s) : color_c128(j, s);
 }
 
+int dtmmx_tmm_transfer4x4(const dtmmx_tmm_desc* d, void* fvec_in, void* fvec_out, int has_out,
+                          void* cmat, void* stream) {
+  if (!d) return fail(DTMMX_E_ARG, "null descriptor");
+  if (d->n_elements <= 0 || d->n_layers <= 0) return fail(DTMMX_E_ARG, "empty stack");
+  if (!d->kd || !d->eps_re || !d->angles) return fail(DTMMX_E_ARG, "null layer data");
+  if (!fvec_in && !cmat) return fail(DTMMX_E_ARG, "nothing to compute");
+  if (fvec_in && !fvec_out) return fail(DTMMX_E_ARG, "fvec_out is required with fvec_in");
+  if (!std::isfinite(d->beta) || !std::isfinite(d->phi) || !std::isfinite(d->n_in) ||
+      !std::isfinite(d->n_out))
+    return fail(DTMMX_E_ARG, "non-finite beam or media");
+  if ((d->n_elements + NT - 1) / NT > 0x7fffffffLL) return fail(DTMMX_E_ARG, "too many stacks");
+  TmmArgs a{};
+  a.n = d->n_elements;
+  a.L = d->n_layers;
+  a.kd = d->kd;
+  a.eps_re = d->eps_re;
+  a.eps_im = d->eps_im;
+  a.ang = d->angles;
+  a.beta = d->beta;
+  a.phi = d->phi;
+  // f_iso of the media (tmm.py:1074-1091) and their inverses, on the host
+  auto media = [&](double n, cd* f, cd* fi) {
+    Modes m;
+    iso_modes(d->beta, mk<double>(n * n), m);
+    rotate_rows(m, std::cos(d->phi), std::sin(d->phi));
+    cd inv[4][4];
+    inv4(m.F, inv);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        f[i * 4 + j] = m.F[i][j];
+        fi[i * 4 + j] = inv[i][j];
+      }
+  };
+  media(d->n_in, a.fin, a.fini);
+  media(d->n_out, a.fout, a.fouti);
+  a.fvec_in = (cd*)fvec_in;
+  a.fvec_out = (cd*)fvec_out;
+  a.has_out = has_out ? 1 : 0;
+  a.cmat = (cd*)cmat;
+  hipLaunchKernelGGL(tmm_kernel, dim3(unsigned((d->n_elements + NT - 1) / NT)), dim3(NT), 0,
+                     (hipStream_t)stream, a);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
 int dtmmx_host_windows(const dtmmx_plan_desc* d, int max_windows, int* n_windows, int* wi,
                        int* wj, double* beta, double* phi) {
   if (!d || !n_windows) return fail(DTMMX_E_ARG, "null argument");

@@ -1461,6 +1461,191 @@ __global__ __launch_bounds__(NT) void color_kernel(ColorArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------------
+// Non-iterative 1D transfer matrix method (dtmm/tmm.py:1112-1673, method
+// "4x4"): one thread per independent stack, fp64 throughout.
+// ---------------------------------------------------------------------------
+
+struct TmmArgs {
+  long long n;          // stacks
+  int L;                // layers
+  const double* kd;     // [L][n]
+  const double* eps_re; // [3][L][n]
+  const double* eps_im; // [3][L][n] or null
+  const double* ang;    // [3][L][n]
+  double beta, phi;
+  cd fin[16], fini[16], fout[16], fouti[16];  // isotropic input/output media
+  cd* fvec_in;          // [n][4]: incident field, replaced by incident + reflected
+  cd* fvec_out;         // [n][4]: transmitted field (incoming backward waves read
+                        // first when has_out)
+  int has_out;
+  cd* cmat;             // [n][4][4] characteristic matrix, or null
+};
+
+// Columns scaled to unit |Poynting z| (dtmm/tmm.py:699-724); zero columns stay zero.
+__host__ __device__ inline void normalize_cols(cd F[4][4]) {
+  for (int j = 0; j < 4; ++j) {
+    const double p = (F[0][j].x * F[1][j].x + F[0][j].y * F[1][j].y) -
+                     (F[2][j].x * F[3][j].x + F[2][j].y * F[3][j].y);
+    const double n = sqrt(fabs(p));
+    const double sc = n == 0. ? 0. : 1. / n;
+    for (int i = 0; i < 4; ++i) F[i][j] = F[i][j] * sc;
+  }
+}
+
+__device__ inline void mat16(const cd* a, cd o[4][4]) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) o[i][j] = a[i * 4 + j];
+}
+
+__device__ inline void matv4(const cd m[4][4], const cd v[4], cd o[4]) {
+  for (int i = 0; i < 4; ++i) {
+    cd t = mk<double>(0.);
+    for (int j = 0; j < 4; ++j) t = t + m[i][j] * v[j];
+    o[i] = t;
+  }
+}
+
+// Biaxial voxels (numeric eigensolve) out of line: the closed-form layers
+// keep the kernel's registers.
+__device__ __noinline__ void tmm_biaxial_modes(double beta, double c0, double s0, double phi0,
+                                               cd e0, cd e1, cd e2, double psi, double theta,
+                                               double phi, Modes& m) {
+  biaxial_modes(beta, e0, e1, e2, psi, theta, phi - phi0, m);
+  rotate_rows(m, c0, s0);
+}
+
+static __global__ __launch_bounds__(NT) void tmm_kernel(TmmArgs A) {
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= A.n) return;
+  double s0, c0;
+  sincos(A.phi, &s0, &c0);
+  const size_t cs = size_t(A.L) * size_t(A.n);
+  cd C[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) C[i][j] = mk<double>(i == j ? 1. : 0.);
+  for (int l = 0; l < A.L; ++l) {
+    // layer_mat: F P(-kd) F^-1 with F normalised (tmm.py:1112-1213); the
+    // stack is M_1 . M_2 ... M_n (tmm.py:1215-1296)
+    const size_t o = size_t(l) * size_t(A.n) + size_t(e);
+    const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+    const cd e1 = mk<double>(A.eps_re[cs + o], A.eps_im ? A.eps_im[cs + o] : 0.);
+    const cd e2 = mk<double>(A.eps_re[2 * cs + o], A.eps_im ? A.eps_im[2 * cs + o] : 0.);
+    Modes m;
+    const int kind = material_kind(e0, e1, e2);
+    if (kind == KIND_BIAXIAL) {
+      tmm_biaxial_modes(A.beta, c0, s0, A.phi, e0, e1, e2, A.ang[o], A.ang[cs + o],
+                        A.ang[2 * cs + o], m);
+    } else {
+      if (kind == KIND_ISO)
+        iso_modes(A.beta, e0, m);
+      else
+        uniaxial_modes(A.beta, e0, e2, A.ang[cs + o], A.ang[2 * cs + o] - A.phi, m);
+      rotate_rows(m, c0, s0);
+    }
+    normalize_cols(m.F);
+    cd Fi[4][4];
+    inv4(m.F, Fi);
+    cd p[4];
+    const double kd = -A.kd[o];
+    for (int q = 0; q < 4; ++q) p[q] = cexpi(m.a[q] * kd);
+    // C <- C . F . diag(p) . F^-1, one row at a time in place
+    for (int i = 0; i < 4; ++i) {
+      cd r[4];
+      for (int jj = 0; jj < 4; ++jj) {
+        cd t = mk<double>(0.);
+        for (int k = 0; k < 4; ++k) t = t + C[i][k] * m.F[k][jj];
+        r[jj] = t * p[jj];
+      }
+      for (int jj = 0; jj < 4; ++jj) {
+        cd t = mk<double>(0.);
+        for (int k = 0; k < 4; ++k) t = t + r[k] * Fi[k][jj];
+        C[i][jj] = t;
+      }
+    }
+  }
+  if (A.cmat)
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) A.cmat[size_t(e) * 16 + i * 4 + j] = C[i][j];
+  if (!A.fvec_in) return;
+  // transmit4x4 (tmm.py:1491-1563) with system_mat (:1298-1336) and
+  // reflection_mat (:1338-1357); the media matrices are read from the
+  // kernel arguments
+  cd S[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      cd t = mk<double>(0.);
+      for (int k = 0; k < 4; ++k) t = t + A.fini[i * 4 + k] * C[k][j];
+      S[i][j] = t;
+    }
+  for (int i = 0; i < 4; ++i) {
+    cd r[4];
+    for (int j = 0; j < 4; ++j) {
+      cd t = mk<double>(0.);
+      for (int k = 0; k < 4; ++k) t = t + S[i][k] * A.fout[k * 4 + j];
+      r[j] = t;
+    }
+    for (int j = 0; j < 4; ++j) S[i][j] = r[j];
+  }
+  // reflection_mat: m1 = [-S[:, 0], e1, -S[:, 2], e3], m2 = [-e0, S[:, 1], -e2, S[:, 3]]
+  cd m1[4][4];
+  for (int i = 0; i < 4; ++i) {
+    m1[i][0] = -S[i][0];
+    m1[i][1] = mk<double>(i == 1 ? 1. : 0.);
+    m1[i][2] = -S[i][2];
+    m1[i][3] = mk<double>(i == 3 ? 1. : 0.);
+  }
+  cd m1i[4][4];
+  inv4(m1, m1i);
+  cd fv[4], av[4], a[4], bv[4], out[4];
+  for (int i = 0; i < 4; ++i) fv[i] = A.fvec_in[size_t(e) * 4 + i];
+  for (int i = 0; i < 4; ++i) {
+    cd t = mk<double>(0.);
+    for (int k = 0; k < 4; ++k) t = t + A.fini[i * 4 + k] * fv[k];
+    av[i] = t;
+  }
+  av[1] = av[3] = mk<double>(0.);
+  for (int i = 0; i < 4; ++i) {
+    a[i] = av[i];
+    bv[i] = mk<double>(0.);
+  }
+  if (A.has_out) {
+    cd fo[4];
+    for (int i = 0; i < 4; ++i) fo[i] = A.fvec_out[size_t(e) * 4 + i];
+    for (int q = 1; q < 4; q += 2) {
+      cd t = mk<double>(0.);
+      for (int k = 0; k < 4; ++k) t = t + A.fouti[q * 4 + k] * fo[k];
+      bv[q] = a[q] = t;
+    }
+  }
+  // out = m1^-1 . (m2 . a), with m2 . a = (-a0, S a odd, ...) written out
+  cd m2a[4];
+  for (int i = 0; i < 4; ++i) {
+    cd t = S[i][1] * a[1] + S[i][3] * a[3];
+    if (i == 0) t = t - a[0];
+    if (i == 2) t = t - a[2];
+    m2a[i] = t;
+  }
+  for (int i = 0; i < 4; ++i) {
+    cd t = mk<double>(0.);
+    for (int k = 0; k < 4; ++k) t = t + m1i[i][k] * m2a[k];
+    out[i] = t;
+  }
+  av[1] = out[1];
+  av[3] = out[3];
+  bv[0] = out[0];
+  bv[2] = out[2];
+  for (int i = 0; i < 4; ++i) {
+    cd t = mk<double>(0.), u = mk<double>(0.);
+    for (int k = 0; k < 4; ++k) {
+      t = t + A.fin[i * 4 + k] * av[k];
+      u = u + A.fout[i * 4 + k] * bv[k];
+    }
+    A.fvec_in[size_t(e) * 4 + i] = t;
+    A.fvec_out[size_t(e) * 4 + i] = u;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Pass normalisation kernels (npass > 1, dtmm/transfer.py:815-872): one
 // thread per (k, plane) on the unnormalised 2D spectrum of a whole field.
 // ---------------------------------------------------------------------------

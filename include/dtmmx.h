@@ -282,6 +282,33 @@ int dtmmx_specter2color(const void* specter, void* rgb, long long n_pixels, int 
                         double gamma, int gray, int precision, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/* Non-iterative 1D transfer matrix method, method "4x4" of
+ * dtmm/tmm.py:1565-1673 transfer4x4 for n_elements independent stacks of
+ * n_layers homogeneous layers each (every stack has its own kd, eps and
+ * angles per layer), one beam (beta, phi) between isotropic input and output
+ * media n_in, n_out: the characteristic matrix C = M_1 ... M_n with
+ * M = F P(-kd) F^-1 (layer_mat, stack_mat, :1112-1296) and the boundary
+ * solve of transmit4x4 (:1491-1563, system_mat :1298, reflection_mat
+ * :1338).  fvec_in [n][4] complex128 (device): the incident field, replaced
+ * by incident + reflected; fvec_out [n][4] complex128 (device): the
+ * transmitted field, whose backward waves are read first as incoming from
+ * the output side when has_out != 0; cmat [n][4][4] complex128 (device) or
+ * NULL receives C; fvec_in may be NULL when only cmat is wanted.  fp64
+ * throughout.  Enqueued on `stream`. */
+typedef struct {
+  long long n_elements;
+  int n_layers;
+  const double* kd;      /* device [n_layers][n_elements]: k0 * thickness   */
+  const double* eps_re;  /* device [3][n_layers][n_elements]               */
+  const double* eps_im;  /* device [3][n_layers][n_elements] or NULL       */
+  const double* angles;  /* device [3][n_layers][n_elements]: psi, theta, phi */
+  double beta, phi;
+  double n_in, n_out;
+} dtmmx_tmm_desc;
+
+int dtmmx_tmm_transfer4x4(const dtmmx_tmm_desc* desc, void* fvec_in, void* fvec_out, int has_out,
+                          void* cmat, void* stream);
+
 /* Optional kernel timing (no counterpart in the reference, which has no
  * profiling hooks): between begin and end every stride-th launch of each
  * per-layer kernel class is bracketed by HIP events on its stream (a stride

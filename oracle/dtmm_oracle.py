@@ -21,6 +21,9 @@ Scope (what the GPU engine implements and this module checks):
 * the polarizing-microscope specter after the transfer: ``calculate_pom_field``,
   ``field2specter``, ``pom_specter`` (dtmm/field_viewer.py, field.py), the Jones
   4x4 matrices (dtmm/jones.py, jones4.py) and ``specter2color`` (dtmm/color.py)
+* the non-iterative 1D transfer matrix method (method "4x4"): ``layer_mat``,
+  ``stack_mat``, ``system_mat``, ``reflection_mat``, ``transmit4x4``,
+  ``transfer4x4``                                          (dtmm/tmm.py)
 * ``illumination_data`` and ``field2intensity``            (dtmm/field.py),
   with the eigen-wave snapping of ``window=None``          (dtmm/wave.py)
 
@@ -49,7 +52,8 @@ __all__ = [
     "normalize_field", "project_normalized_fft", "project_normalized_total", "total_intensity",
     "polarizer", "as4x4", "jonesmat4x4", "ray_jonesmat4x4", "field2specter",
     "calculate_pom_field", "pom_specter", "XYZ2RGBD65", "apply_gamma", "apply_srgb_gamma",
-    "specter2color",
+    "specter2color", "f_iso", "layer_mat", "stack_mat", "system_mat", "reflection_mat",
+    "transmit4x4", "transfer4x4",
 ]
 
 # dtmm/dtmm.ini:6 (betamax = 0.8) and dtmm/conf.py:148, :503-510 (nin/nout
@@ -1627,3 +1631,87 @@ def specter2color(spec, cmf, norm=False, gamma=True, gray=False):
     elif gamma is not False:
         out = apply_gamma(out, gamma)
     return out
+
+
+# ---------------------------------------------------------------------------
+# tmm.py: the non-iterative 1D transfer matrix method (method "4x4")
+# ---------------------------------------------------------------------------
+
+def f_iso(beta=0., phi=0., n=1.):
+    """Field matrix of an isotropic medium, not normalised (dtmm/tmm.py:1074-1091)."""
+    return alphaf(beta, phi, refind2eps([n] * 3), np.zeros(3))[1]
+
+
+def layer_mat(kd, epsv, epsa, beta=0., phi=0.):
+    """Characteristic matrix F . P(-kd) . F^-1 of one layer with F normalised
+    (dtmm/tmm.py:1112-1213, method "4x4")."""
+    alpha, f = alphaf(beta, phi, epsv, epsa)
+    f = normalize_f(f)
+    fi = inv(f)
+    return dotmdm(f, phase_mat(alpha, -np.asarray(kd, FDTYPE)), fi)
+
+
+def stack_mat(kd, epsv, epsa, beta=0., phi=0.):
+    """M_1 . M_2 ... M_n (dtmm/tmm.py:1215-1296, method "4x4")."""
+    out = None
+    for i in range(len(kd)):
+        m = layer_mat(kd[i], epsv[i], epsa[i], beta, phi)
+        out = m.copy() if out is None else dotmm(out, m)
+    return out
+
+
+def system_mat(cmat, fmatin=None, fmatout=None):
+    """Fin^-1 . C . Fout (dtmm/tmm.py:1298-1336, 4x4 characteristic matrix)."""
+    if fmatin is None:
+        fmatin = f_iso()
+    if fmatout is None:
+        fmatout = fmatin
+    return dotmm(dotmm(inv(fmatin), cmat), fmatout)
+
+
+def reflection_mat(smat):
+    """(dtmm/tmm.py:1338-1357)."""
+    smat = np.asarray(smat, CDTYPE)
+    m1 = np.zeros_like(smat)
+    m2 = np.zeros_like(smat)
+    for i in range(smat.shape[-1] // 2):
+        m1[..., 2 * i + 1, 2 * i + 1] = 1.
+        m2[..., 2 * i, 2 * i] = -1.
+    m1[..., :, 0::2] = -smat[..., :, 0::2]
+    m2[..., :, 1::2] = smat[..., :, 1::2]
+    return dotmm(inv(m1), m2)
+
+
+def transmit4x4(fvec_in, cmat, fmatin=None, fmatout=None, fvec_out=None):
+    """Output field of the stack; returns (fvec_out, fvec_in with the
+    reflected waves added) (dtmm/tmm.py:1491-1563)."""
+    fvec_in = np.asarray(fvec_in, CDTYPE)
+    if fmatin is None:
+        fmatin = f_iso()
+    if fmatout is None:
+        fmatout = fmatin
+    fmatini, fmatouti = inv(fmatin), inv(fmatout)
+    shape = np.broadcast(fvec_in[..., None], cmat[..., 0:4, 0:4], fmatin, fmatout).shape[:-1]
+    smat = system_mat(cmat, fmatin, fmatout)
+    avec = dotmv(fmatini, fvec_in)
+    a = np.zeros(shape, CDTYPE)
+    a[..., 0::2] = avec[..., 0::2]
+    avec = a.copy()
+    if fvec_out is not None:
+        bvec = dotmv(fmatouti, fvec_out)
+        a[..., 1::2] = bvec[..., 1::2]
+    else:
+        bvec = np.zeros_like(avec)
+    out = dotmv(reflection_mat(smat), a)
+    avec[..., 1::2] = out[..., 1::2]
+    bvec[..., ::2] = out[..., ::2]
+    return dotmv(fmatout, bvec), dotmv(fmatin, avec)
+
+
+def transfer4x4(fvec_in, kd, epsv, epsa, beta=0., phi=0., nin=1., nout=1., fvec_out=None):
+    """(dtmm/tmm.py:1565-1673, method "4x4"): returns (fvec_out, fvec_in
+    with reflections)."""
+    fmatin = f_iso(beta, phi, nin)
+    fmatout = f_iso(beta, phi, nout)
+    cmat = stack_mat(kd, epsv, epsa, beta, phi)
+    return transmit4x4(fvec_in, cmat, fmatin, fmatout, fvec_out)
