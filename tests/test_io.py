@@ -1,0 +1,82 @@
+"""The .dtms / .dtmf file formats (dtmm/data.py:1530-1627,
+dtmm/field.py:903-967), host only: round trips, the byte layout (magic,
+version, numpy.save records), version-1 stacks, and refusal of pickled
+records."""
+import io
+
+import numpy as np
+import pytest
+
+import dtmm_amd as dm
+from dtmm_amd import io as dio
+
+
+def _blocks():
+    rng = np.random.default_rng(1)
+    b1 = (np.ones(3), rng.uniform(2, 3, (3, 4, 5, 3)), rng.uniform(-1, 1, (3, 4, 5, 3)))
+    b2 = (np.asarray([0.5, 2.]), rng.uniform(2, 3, (2, 4, 5, 3)) + 0.01j,
+          rng.uniform(-1, 1, (2, 4, 5, 3)))
+    return [b1, b2]
+
+
+def test_stack_round_trip_and_layout(tmp_path):
+    blocks = _blocks()
+    p = tmp_path / "sample"
+    dio.save_stack(str(p), blocks)
+    raw = (tmp_path / "sample.dtms").read_bytes()
+    assert raw[:5] == b"dtms\x02"
+    # the body is plain numpy.save records: d, epsv, epsa per block
+    f = io.BytesIO(raw[5:])
+    for d, e, a in blocks:
+        np.testing.assert_array_equal(np.load(f), d)
+        np.testing.assert_array_equal(np.load(f), e)
+        np.testing.assert_array_equal(np.load(f), a)
+    assert f.read() == b""
+    got = dio.load_stack(str(tmp_path / "sample.dtms"))
+    assert len(got) == 2
+    for (d, e, a), (d2, e2, a2) in zip(blocks, got):
+        np.testing.assert_array_equal(d, d2)
+        np.testing.assert_array_equal(e, e2)
+        np.testing.assert_array_equal(a, a2)
+
+
+def test_version1_stack_single_block_without_epsa():
+    f = io.BytesIO()
+    f.write(b"dtms\x01")
+    np.save(f, np.ones(2))
+    np.save(f, np.full((2, 3, 3, 3), 2.25))
+    f.seek(0)
+    d, e, a = dio.load_stack(f)
+    assert a is None and e.shape == (2, 3, 3, 3)
+
+
+def test_pickled_record_is_refused():
+    f = io.BytesIO()
+    f.write(b"dtms\x02")
+    np.save(f, np.ones(1))
+    np.save(f, np.asarray(("cauchy", [1.5, 0.01]), dtype=object), allow_pickle=True)
+    np.save(f, np.zeros((1, 3)))
+    f.seek(0)
+    with pytest.raises(NotImplementedError):
+        dio.load_stack(f)
+
+
+def test_bad_magic_and_newer_version():
+    with pytest.raises(OSError):
+        dio.load_stack(io.BytesIO(b"xxxx\x02"))
+    with pytest.raises(OSError):
+        dio.load_stack(io.BytesIO(b"dtms\x03"))
+    with pytest.raises(OSError):
+        dio.load_field(io.BytesIO(b"dtmf\x01"))
+
+
+def test_field_round_trip(tmp_path):
+    fd = dm.illumination_data((8, 12), [500., 600.], pixelsize=50., jones=None,
+                              window=dm.blackman((8, 12)))
+    dio.save_field(str(tmp_path / "f"), fd)
+    raw = (tmp_path / "f.dtmf").read_bytes()
+    assert raw[:5] == b"dtmf\x00"
+    field, wl, ps = dio.load_field(str(tmp_path / "f.dtmf"))
+    np.testing.assert_array_equal(field, fd[0])
+    np.testing.assert_array_equal(wl, fd[1])
+    assert ps == 50.
