@@ -52,6 +52,8 @@ EXPORTS = (
     "dtmmx_specter2color_workspace_size",
     "dtmmx_specter2color",
     "dtmmx_tmm_transfer4x4",
+    "dtmmx_director2data_blocks",
+    "dtmmx_director2data",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -229,6 +231,13 @@ def load():
                                           ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                           ctypes.c_void_p]
     lib.dtmmx_tmm_transfer4x4.restype = ctypes.c_int
+    lib.dtmmx_director2data_blocks.argtypes = [ctypes.c_longlong]
+    lib.dtmmx_director2data_blocks.restype = ctypes.c_longlong
+    lib.dtmmx_director2data.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.dtmmx_director2data.restype = ctypes.c_int
     if lib.dtmmx_abi_version() != ABI_VERSION:
         raise RuntimeError("dtmmx ABI version mismatch")
     _lib = lib

@@ -1461,6 +1461,58 @@ __global__ __launch_bounds__(NT) void color_kernel(ColorArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------------
+// Optical data of a director field (dtmm/data.py:277-320 director2data with
+// director2order :962-972, uniaxial_order :1215-1274, director2angles
+// :975-989): one thread per voxel, grid (blocks per layer, layers).
+// ---------------------------------------------------------------------------
+
+struct DirArgs {
+  const double* director;     // [n][3]
+  const unsigned char* mask;  // [n] or null
+  long long layer_size, n;    // voxels per layer, all voxels
+  double eo, ee, eh;          // no^2, ne^2, nhost^2
+  double scale;               // scale_factor (the order is divided by it)
+  double* eps_re;             // [3][n]
+  double* angles;             // [3][n]
+  double* sums;               // [layers][gridDim.x]: sums of (e0 + e1 + e2) / 3
+};
+
+static __global__ __launch_bounds__(NT) void director2data_kernel(DirArgs A) {
+#pragma clang fp contract(off)
+  __shared__ double red[NT];
+  const long long x = (long long)blockIdx.x * NT + threadIdx.x;
+  const long long v = (long long)blockIdx.y * A.layer_size + x;
+  double iso = 0.;
+  if (x < A.layer_size) {
+    const double dx = A.director[3 * v], dy = A.director[3 * v + 1], dz = A.director[3 * v + 2];
+    double e0, e2;
+    if (A.mask && !A.mask[v]) {
+      e0 = e2 = A.eh;
+    } else {
+      const double o = sqrt(sqrt(dx * dx + dy * dy + dz * dz)) / A.scale;
+      const double m = (A.eo + A.eo + A.ee) / 3.;
+      const double delta = A.ee - (A.eo + A.eo) / 2.;
+      e0 = o == 0. ? m : m - 1. / 3. * o * delta;
+      e2 = o == 0. ? m : m + 2. / 3. * o * delta;
+    }
+    A.eps_re[v] = e0;
+    A.eps_re[A.n + v] = e0;
+    A.eps_re[2 * A.n + v] = e2;
+    A.angles[v] = 0.;
+    A.angles[A.n + v] = atan2(sqrt(dx * dx + dy * dy), dz);
+    A.angles[2 * A.n + v] = atan2(dy, dx);
+    iso = (e0 + e0 + e2) / 3.;
+  }
+  red[threadIdx.x] = iso;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) A.sums[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
 // Non-iterative 1D transfer matrix method (dtmm/tmm.py:1112-1673, method
 // "4x4"): one thread per independent stack, fp64 throughout.
 // ---------------------------------------------------------------------------

@@ -103,6 +103,7 @@ class Stack:
         torch = _torch()
         blocks = validate_optical_data(optical_data, shape=shape)
         H, W = shape
+        self.shape = (H, W)
         thick, steps, eff, eps_l, biax = [], [], [], [], []
         for d, material, angles in blocks:
             n = len(d)
@@ -147,6 +148,74 @@ class Stack:
         self.eps_im = torch.from_numpy(eps_im).to(device) if eps_im is not None else None
         self.angles = torch.from_numpy(angs).to(device)
         self.has_imag_eff = bool(np.any(self.eps_eff_im != 0))
+
+    @classmethod
+    def from_director(cls, director, mask=None, no=1.5, ne=1.6, nhost=None, scale_factor=1.,
+                      thickness=None, nstep=1, nin=N_COVER, nout=N_COVER, split_layers=False):
+        """The stack of ``director2data(director, mask, no, ne, nhost,
+        scale_factor, thickness)`` (dtmm/data.py:277-320) built on the GPU
+        (``dtmmx_director2data``): the director (L, H, W, 3) -- a device
+        tensor stays on the device -- becomes the device material arrays and
+        the block's effective medium (the mean isotropic eps, per layer with
+        ``split_layers``) without a host copy of the material.  Pass the
+        result as ``optical_data`` to ``transfer_field``."""
+        torch = _torch()
+        if isinstance(director, torch.Tensor):
+            if not director.is_cuda:
+                raise ValueError("tensor inputs must be device tensors")
+            dr = director.to(torch.float64).contiguous()
+        else:
+            dr = torch.from_numpy(np.ascontiguousarray(np.asarray(director, np.float64)))
+            dr = dr.to("cuda")
+        if dr.dim() != 4 or dr.shape[-1] != 3:
+            raise ValueError("director must have shape (layers, height, width, 3)")
+        L, H, W = (int(x) for x in dr.shape[:3])
+        mk = None
+        if mask is not None:
+            if isinstance(mask, torch.Tensor):
+                mk = mask.to(device="cuda", dtype=torch.uint8).contiguous()
+            else:
+                mk = torch.from_numpy(np.ascontiguousarray(np.asarray(mask, np.uint8))).to("cuda")
+            if tuple(mk.shape) != (L, H, W):
+                raise ValueError("mask must have shape (layers, height, width)")
+            if nhost is None:
+                raise ValueError("nhost is required with a mask")
+        lib = _lib.load()
+        nb = int(lib.dtmmx_director2data_blocks(H * W))
+        eps_re = torch.empty((3, L, H, W), dtype=torch.float64, device="cuda")
+        angles = torch.empty((3, L, H, W), dtype=torch.float64, device="cuda")
+        sums = torch.empty((L, nb), dtype=torch.float64, device="cuda")
+        stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib.dtmmx_director2data(
+            ctypes.c_void_p(dr.data_ptr()), ctypes.c_void_p(mk.data_ptr()) if mk is not None
+            else None, H * W, L, float(no), float(ne),
+            float(nhost) if nhost is not None else float("nan"), float(scale_factor),
+            ctypes.c_void_p(eps_re.data_ptr()), ctypes.c_void_p(angles.data_ptr()),
+            ctypes.c_void_p(sums.data_ptr()), ctypes.c_void_p(stream)))
+        layer_sum = sums.sum(dim=1).cpu().numpy()
+        if split_layers:
+            eff = layer_sum / (H * W)
+        else:
+            eff = np.full(L, layer_sum.sum() / (L * H * W))
+        self = cls.__new__(cls)
+        self.shape = (H, W)
+        self.n_layers = L
+        self.thickness = (np.ones(L) if thickness is None
+                          else np.asarray(thickness, np.float64).reshape(L))
+        steps = np.broadcast_to(np.asarray(nstep), (L,))
+        if np.any(steps < 1):
+            raise ValueError("nstep must be >= 1")
+        self.n_steps = np.asarray(steps, np.int32)
+        self.biaxial = np.zeros(L, np.int32)  # director data is uniaxial (eps0 == eps1)
+        self.eps_eff_re = np.asarray(eff, np.float64)
+        self.eps_eff_im = np.zeros(L, np.float64)
+        self.nin = float(nin)
+        self.nout = float(nout)
+        self.eps_re = eps_re
+        self.eps_im = None
+        self.angles = angles
+        self.has_imag_eff = False
+        return self
 
     def c_struct(self):
         s = _lib.Stack()
@@ -396,7 +465,17 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
     plan = Plan(shape, ks, b, p, n_rays, rays_per_beam, kind, precision, betamax,
                 diffraction=int(diffraction), npass=int(npass), norm=norm, smooth=smooth,
                 reflection=reflection, checkpoint=checkpoint)
-    stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout, split_layers=split_layers)
+    if isinstance(optical_data, Stack):
+        # a stack already on the device (Stack.from_director): its own layers
+        # and sub-steps, this call's input and output media
+        stack = optical_data
+        if tuple(stack.shape) != tuple(shape):
+            raise ValueError("the stack's grid %s does not match the field's %s"
+                             % (tuple(stack.shape), tuple(shape)))
+        stack.nin, stack.nout = float(nin), float(nout)
+    else:
+        stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout,
+                      split_layers=split_layers)
     fin = torch.from_numpy(np.ascontiguousarray(field_np.reshape((n_rays, len(ks), 4) + shape)))
     fin = fin.to(device="cuda", dtype=plan.dtype)
     fout = torch.empty_like(fin)

@@ -282,6 +282,21 @@ int dtmmx_specter2color(const void* specter, void* rgb, long long n_pixels, int 
                         double gamma, int gray, int precision, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/* Optical data of a director field on the device: dtmm/data.py:277-320
+ * director2data (order sqrt(|n|) / scale_factor, uniaxial_order of
+ * (no, no, ne), host index where mask is 0) and director2angles (psi 0,
+ * theta = atan2(sqrt(x^2 + y^2), z), phi = atan2(y, x)).  director: device
+ * [n_layers * layer_size][3] double (the reference's (..., 3) layout); mask:
+ * device [n_layers * layer_size] bytes or NULL; eps_re and angles: device
+ * [3][n_layers * layer_size] (the dtmmx_stack layout).  layer_sums: device
+ * [n_layers][dtmmx_director2data_blocks(layer_size)] partial sums of the
+ * isotropic part (e0 + e1 + e2) / 3, whose mean is the effective medium of
+ * dtmm/transfer.py:34-41.  Enqueued on `stream`. */
+long long dtmmx_director2data_blocks(long long layer_size);
+int dtmmx_director2data(const double* director, const unsigned char* mask, long long layer_size,
+                        int n_layers, double no, double ne, double nhost, double scale_factor,
+                        double* eps_re, double* angles, double* layer_sums, void* stream);
+
 /* Non-iterative 1D transfer matrix method, method "4x4" of
  * dtmm/tmm.py:1565-1673 transfer4x4 for n_elements independent stacks of
  * n_layers homogeneous layers each (every stack has its own kd, eps and

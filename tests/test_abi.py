@@ -16,7 +16,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     text = open(os.path.join(ROOT, "include", "dtmmx.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\*?\s+\*?(dtmmx_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:long\s+)?[a-z_]+\*?\s+\*?(dtmmx_\w+)\s*\(",
+                                 text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -413,3 +414,21 @@ def test_specter2color_rejects_bad_arguments():
     rc = lib.dtmmx_specter2color(p, p, 10, 3, _lib.dptr(cmf), 0, 0., 0, 0., 0, _lib.DTMMX_C64,
                                  p, 8, None)
     assert rc == _lib.DTMMX_E_ARG
+
+
+def test_director2data_rejects_bad_arguments():
+    """Host-side validation of dtmmx_director2data (no GPU call)."""
+    lib = _lib.load()
+    p = ctypes.c_void_p(8)  # never dereferenced: every call below fails validation
+    assert lib.dtmmx_director2data_blocks(1000) == 4
+    assert lib.dtmmx_director2data_blocks(0) == 0
+    bad = [
+        (None, None, 10, 1, 1.5, 1.6, 1.5, 1.0),        # null director
+        (p, None, 0, 1, 1.5, 1.6, 1.5, 1.0),            # empty layer
+        (p, None, 10, 0, 1.5, 1.6, 1.5, 1.0),           # no layers
+        (p, None, 10, 1, 1.5, 1.6, 1.5, 0.0),           # scale factor 0
+        (p, p, 10, 1, 1.5, 1.6, float("nan"), 1.0),     # mask without nhost
+    ]
+    for d, m, ls, nl, no, ne, nh, sf in bad:
+        rc = lib.dtmmx_director2data(d, m, ls, nl, no, ne, nh, sf, p, p, p, None)
+        assert rc == _lib.DTMMX_E_ARG, (ls, nl, sf)
