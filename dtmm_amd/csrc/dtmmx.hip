@@ -917,8 +917,15 @@ static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin
   }
   TRY(fft2_planes<T>(P, f0, 1, s));
   HIP_TRY(hipMemcpyAsync(g, fin, bytes, hipMemcpyDeviceToDevice, s));
+  // reflection mode 2 propagates in real space (work_in_fft False,
+  // dtmm/transfer.py:719-725); the normalisations between passes are the same
+  const bool r2 = P->d.method == DTMMX_METHOD_4X4 && P->d.reflection == 2;
   for (int i = 0; i < npass; ++i) {
-    TRY(propagate<T>(P, S, g, ws, status, s, (i % 2 == 0) ? 1 : -1));
+    const int dir = (i % 2 == 0) ? 1 : -1;
+    if (r2)
+      TRY(refl2_pass<T>(P, S, g, s, dir));
+    else
+      TRY(propagate<T>(P, S, g, ws, status, s, dir));
     const double sigma = smooth * double(npass - i) / double(npass);
     if (i % 2 == 0) {
       if (i != npass - 1) {
@@ -1040,13 +1047,16 @@ static int transfer_nodiff(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<
 // exp(+-i k0 d (a(k) - a_c)) of corrected_Epn_diffraction_matrix
 // (matrix.py:38-43, :62-68) on the forward / backward E parts, the inverse
 // transform, and F P/2 e^-1 per pixel.
+// One pass of 4x4 reflection mode 2 in place on the real-space field f: the
+// layers in the order of dir with thickness times dir (dtmm/transfer.py
+// :752-760, :782-785).  Diffraction 0: F P F^-1 per pixel and layer;
+// diffraction 1: the sub-steps described above.
 template <typename T>
-static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
-                          cpx<T>* fout, hipStream_t s, cpx<T>* bulk = nullptr) {
+static int refl2_pass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fout, hipStream_t s,
+                      int dir, cpx<T>* bulk = nullptr) {
   const size_t HW = size_t(P->d.height) * P->d.width;
   const int nbeam = P->d.n_rays / P->d.rays_per_beam;
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
-  HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
   PixArgs<T> a{};
   a.field = fout;
   a.HW = HW;
@@ -1054,7 +1064,20 @@ static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T
   a.bm = P->bm;
   a.comp_stride = size_t(S->n_layers) * HW;
   const dim3 grid(unsigned((HW + NT - 1) / NT), unsigned(nbeam));
-  const std::vector<Step> st = expand_steps(S, 1);
+  if (P->d.diffraction == 0) {
+    const int L = S->n_layers;
+    for (int k = 0; k < L; ++k) {
+      const int l = dir > 0 ? k : L - 1 - k;
+      a.eps_re = S->eps_re + size_t(l) * HW;
+      a.eps_im = S->eps_im ? S->eps_im + size_t(l) * HW : nullptr;
+      a.ang = S->angles + size_t(l) * HW;
+      a.d = S->thickness[l] * dir;
+      hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER4>), grid, dim3(NT), 0, s, a);
+      HIP_TRY(hipGetLastError());
+    }
+    return DTMMX_OK;
+  }
+  const std::vector<Step> st = expand_steps(S, dir);
   const Batch b{P->d.n_rays, P->d.n_wavelengths, 0, P->d.n_wavelengths};
   for (size_t k = 0; k < st.size(); ++k) {
     const Step& q = st[k];
@@ -1086,10 +1109,20 @@ static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T
 }
 
 template <typename T>
+static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
+                          cpx<T>* fout, hipStream_t s, cpx<T>* bulk = nullptr) {
+  const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * size_t(P->d.height) *
+                   P->d.width;
+  HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+  return refl2_pass<T>(P, S, fout, s, 1, bulk);
+}
+
+template <typename T>
 static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
                          void* ws, int* status, hipStream_t s, void* bulk = nullptr) {
   const size_t HW = size_t(P->d.height) * P->d.width;
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
+  if (P->d.npass > 1) return transfer_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s);
   if (P->d.diffraction == 0)
     return transfer_nodiff<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s, (cpx<T>*)bulk);
   if (P->d.method == DTMMX_METHOD_4X4 && P->d.reflection == 2)
@@ -1752,16 +1785,15 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   if (!d->k0 || !d->beta || !d->phi) return fail(DTMMX_E_ARG, "null k0/beta/phi");
   // diffraction 0: the per-pixel modes (4x4 reflection 2, 2x2 reflection 0),
   // one pass
-  const bool nodiff = d->diffraction == 0 && d->npass == 1 &&
+  const bool nodiff = d->diffraction == 0 &&
                       ((d->method == DTMMX_METHOD_4X4 && d->reflection == 2) ||
-                       (d->method == DTMMX_METHOD_2X2 && d->reflection == 0));
-  // 4x4 reflection mode 2 with one mean beam, one pass
-  const bool refl2 = d->method == DTMMX_METHOD_4X4 && d->reflection == 2 && d->npass == 1 &&
-                     d->diffraction <= 1;
+                       (d->method == DTMMX_METHOD_2X2 && d->reflection == 0 && d->npass == 1));
+  // 4x4 reflection mode 2 with one mean beam (any npass)
+  const bool refl2 = d->method == DTMMX_METHOD_4X4 && d->reflection == 2 && d->diffraction <= 1;
   if (d->diffraction < 1 && !nodiff)
     return fail(DTMMX_E_UNSUPPORTED,
-                "diffraction must be >= 1, or 0 with npass 1 for the 4x4 method in reflection "
-                "mode 2 and the 2x2 method in reflection mode 0");
+                "diffraction must be >= 1, or 0 for the 4x4 method in reflection mode 2 and the "
+                "2x2 method in reflection mode 0 with npass 1");
   if (d->diffraction > 1 && d->method != DTMMX_METHOD_4X4)
     return fail(DTMMX_E_UNSUPPORTED, "diffraction > 1 is built for the 4x4 method only");
   if (d->diffraction > 64) return fail(DTMMX_E_ARG, "diffraction above 64");
@@ -1773,8 +1805,7 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->checkpoint < 0) return fail(DTMMX_E_ARG, "checkpoint interval must be >= 0");
   if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !refl2)
     return fail(DTMMX_E_UNSUPPORTED,
-                "the 4x4 method is built for reflection mode 1, and mode 2 with diffraction 0 or 1 "
-                "and npass 1");
+                "the 4x4 method is built for reflection mode 1, and mode 2 with diffraction 0 or 1");
   if (d->method == DTMMX_METHOD_2X2 && d->reflection != 0 && d->reflection != 1)
     return fail(DTMMX_E_UNSUPPORTED, "the 2x2 method is built for reflection modes 0 and 1");
   return DTMMX_OK;

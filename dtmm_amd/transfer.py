@@ -373,8 +373,9 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
     if diffraction == 0:
         # no diffraction: every pixel is an independent stack (4x4 reflection
         # mode 2, the reference's default for diffraction 0; 2x2 reflection 0)
-        if npass != 1:
-            raise NotImplementedError("diffraction=0 is supported with npass=1")
+        if npass != 1 and not (kind == "4x4" and reflection == 2):
+            raise NotImplementedError("diffraction=0 with npass > 1 is supported for the 4x4 "
+                                      "method in reflection mode 2")
         if kind == "4x4" and reflection != 2:
             raise NotImplementedError("4x4 method with diffraction=0: only reflection mode 2 "
                                       "is supported")
@@ -383,12 +384,11 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
                                       "is supported")
         return kind, reflection
     if kind == "4x4":
-        if reflection == 2 and diffraction == 1 and npass == 1:
+        if reflection == 2 and diffraction == 1:
             return kind, reflection
         if reflection != 1:
             raise NotImplementedError("4x4 method: reflection mode 2 is supported with "
-                                      "diffraction 0 or 1 and npass=1; otherwise reflection "
-                                      "mode 1")
+                                      "diffraction 0 or 1; otherwise reflection mode 1")
         if not (diffraction >= 1) or diffraction == np.inf:
             raise NotImplementedError("4x4 method: only diffraction 0 or finite diffraction >= 1 "
                                       "is supported")

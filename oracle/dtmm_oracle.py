@@ -1271,9 +1271,11 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
             bulk[-1] = field
             return bulk, wavelengths, pixelsize
         return field, wavelengths, pixelsize
-    if reflection != 1:
+    if reflection != 1 and not (reflection == 2 and diffraction in (0, 1)):
         raise NotImplementedError("oracle covers reflection=1, and reflection=2 with "
-                                  "diffraction 0 or 1 and npass=1, for the 4x4 method")
+                                  "diffraction 0 or 1, for the 4x4 method")
+    # reflection mode 2 works in real space (dtmm/transfer.py:719-725)
+    work_in_fft = reflection != 2
     if norm & DTMM_NORM_REF:
         raise NotImplementedError("oracle covers norm without DTMM_NORM_REF")
     norm = "fft" if norm & DTMM_NORM_FFT else "total"
@@ -1290,7 +1292,7 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     indices = list(range(1, n - 1))
     field0 = transmitted_field(field_in, ks, n=nin, betamax=betamax)
     i0 = total_intensity(field0)
-    field = fft2(field_in.copy())
+    field = fft2(field_in.copy()) if work_in_fft else field_in.copy()
     if npass > 1:
         pin_mat = projection_matrix(field.shape[-2:], ks, epsv=refind2eps([nin] * 3), mode=+1,
                                     betamax=betamax)
@@ -1302,15 +1304,21 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
         for j in indices:
             nsub, (thickness, ev, ea) = layers[j]
             _, (d, e, a) = eff_layers[j]
-            field = propagate_4x4_effective_1(field, ks, (thickness * direction, ev, ea),
-                                              (d * direction, e, a), beta, phi, nsub,
-                                              diffraction, betamax, cache)
+            if reflection == 2:
+                field = propagate_4x4_effective_2(field, ks, (thickness * direction, ev, ea),
+                                                  (d * direction, e, a), beta, phi, nsub,
+                                                  diffraction, betamax, cache)
+            else:
+                field = propagate_4x4_effective_1(field, ks, (thickness * direction, ev, ea),
+                                                  (d * direction, e, a), beta, phi, nsub,
+                                                  diffraction, betamax, cache)
             if bulk is not None:
-                bulk[j] = ifft2(field)
+                bulk[j] = ifft2(field) if work_in_fft else field
             if layer_callback is not None:
                 layer_callback(j)
         indices.reverse()
-        field = ifft2(field)
+        if work_in_fft:
+            field = ifft2(field)
         if npass == 1:
             field_out[...] = field
             if bulk is not None:
@@ -1341,7 +1349,8 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
                 field_in *= fact
                 field = field0 - field
                 field_in += field
-        field = fft2(field)
+        if work_in_fft:
+            field = fft2(field)
     return field_out, wavelengths, pixelsize
 
 

@@ -89,7 +89,7 @@ def _plan_rc(**kw):
     (dict(method=2, npass=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(npass=3, norm=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(reflection=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
-    (dict(reflection=2, npass=2), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(method=2, reflection=0, npass=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(reflection=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(method=2, reflection=2), _lib.DTMMX_E_UNSUPPORTED),
 ])
@@ -336,8 +336,9 @@ def test_host_rop_grad_matches_finite_differences():
 def test_mode_resolution_no_diffraction():
     """diffraction=0 selects reflection mode 2 for the 4x4 method
     (dtmm/transfer.py:460-461) and keeps mode 0 for an explicit 2x2 method;
-    reflection mode 2 is also built with diffraction 1 (npass 1); other
-    combinations are refused before any GPU call."""
+    reflection mode 2 is also built with diffraction 1, and with npass > 1
+    (diffraction 0 or 1); other combinations are refused before any GPU
+    call."""
     from dtmm_amd.transfer import _resolve_modes
     assert _resolve_modes("4x4", 1, 0, None, None, False) == ("4x4", 2)
     assert _resolve_modes("2x2", 1, 0, None, None, False) == ("2x2", 0)
@@ -349,8 +350,12 @@ def test_mode_resolution_no_diffraction():
     assert _resolve_modes("4x4", 1, 1, 2, None, False) == ("4x4", 2)
     with pytest.raises(NotImplementedError):
         _resolve_modes("4x4", 1, 3, 2, None, False)
+    assert _resolve_modes("4x4", 2, 1, 2, None, False) == ("4x4", 2)
+    assert _resolve_modes("4x4", 3, 0, 2, None, False) == ("4x4", 2)
     with pytest.raises(NotImplementedError):
-        _resolve_modes("4x4", 2, 1, 2, None, False)
+        _resolve_modes("4x4", 3, 0, None, None, False)  # npass > 1 selects reflection 1
+    with pytest.raises(NotImplementedError):
+        _resolve_modes("2x2", 2, 0, 0, None, False)
     # ret_bulk: the 4x4 method with npass 1 and diffraction 0 or 1 only
     assert _resolve_modes("4x4", 1, 1, None, None, True) == ("4x4", 1)
     with pytest.raises(NotImplementedError):

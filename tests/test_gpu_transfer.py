@@ -177,6 +177,19 @@ def test_npass3_reflections(precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("diffraction,npass,norm", [(1, 3, 1), (0, 3, 1), (1, 2, 0), (0, 4, 0)])
+def test_reflection2_npass(diffraction, npass, norm, precision):
+    """4x4 reflection mode 2 with npass > 1: real-space passes with the
+    normalisations between them (dtmm/transfer.py:719-725, :815-872)."""
+    flush("case refl2 npass d=%d npass=%d norm=%d %s" % (diffraction, npass, norm, precision))
+    fd, optical = random_case(shape=(5, 32, 48), wavelengths=(480., 640.), jones=None,
+                              beta=0.1, phi=0.4)
+    _compare_npass(fd, optical, precision, beta=0.1, phi=0.4, npass=npass, norm=norm,
+                   diffraction=diffraction, reflection=2, nin=1.0, nout=1.4)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("npass,norm", [(2, 1), (4, 0), (5, 1)])
 def test_npass_norms(npass, norm):
     flush("case npass=%d norm=%d" % (npass, norm))

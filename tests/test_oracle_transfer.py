@@ -132,6 +132,30 @@ def test_npass_slab_reaches_fabry_perot(npass, norm, diffraction):
         np.testing.assert_allclose(inc, 1., atol=1e-12)
 
 
+@pytest.mark.parametrize("npass", [3, 5])
+@pytest.mark.parametrize("norm", [o.DTMM_NORM_FFT, 0])
+@pytest.mark.parametrize("diffraction", [0, 1])
+def test_npass_reflection2_slab_reaches_fabry_perot(npass, norm, diffraction):
+    """Reflection mode 2 (real-space passes, transfer.py:719-725) with
+    npass > 1 on the same slab: the Airy transmission and reflection."""
+    nin, ns, nout, L = 1.0, 1.5, 1.25, 7
+    shape = (8, 8)
+    wl = np.array([550., 633.])
+    ps = 100.
+    ks = o.k0(wl, ps)
+    f = np.zeros((len(wl), 4) + shape, complex)
+    f[:, 0], f[:, 1] = 1., nin
+    eps = np.broadcast_to([ns ** 2] * 3, (L, 1, 1, 3))
+    out = o.transfer_field((f, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                           beta=0., phi=0., method="4x4", nin=nin, nout=nout, npass=npass,
+                           norm=norm, diffraction=diffraction, reflection=2)[0]
+    for i, k in enumerate(ks):
+        t, r = _airy(nin, ns, nout, k, L)
+        np.testing.assert_allclose((out[i, 0] + out[i, 1] / nout) / 2, t, atol=1e-12)
+        np.testing.assert_allclose((out[i, 0] - out[i, 1] / nout) / 2, 0., atol=1e-12)
+        np.testing.assert_allclose((f[i, 0] - f[i, 1] / nin) / 2, r, atol=1e-12)
+
+
 def test_npass_one_pass_has_no_reflection():
     nin, ns, nout, L = 1.0, 1.5, 1.25, 5
     shape = (8, 8)
