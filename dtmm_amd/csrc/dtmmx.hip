@@ -883,6 +883,11 @@ static int sum_rows(const double* part, int rows, int nb, double scale, double* 
 // field): forward and backward passes with the output normalised onto the
 // forward modes of the output medium and the input-side field rescaled to
 // the incoming intensity between passes.
+// One pass of 4x4 reflection mode 2 in place (defined below).
+template <typename T>
+static int refl2_pass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fout, hipStream_t s,
+                      int dir, cpx<T>* bulk = nullptr, void* ws = nullptr);
+
 template <typename T>
 static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin, cpx<T>* fout,
                           void* ws, int* status, hipStream_t s) {
@@ -923,7 +928,7 @@ static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin
   for (int i = 0; i < npass; ++i) {
     const int dir = (i % 2 == 0) ? 1 : -1;
     if (r2)
-      TRY(refl2_pass<T>(P, S, g, s, dir));
+      TRY(refl2_pass<T>(P, S, g, s, dir, nullptr, ws));
     else
       TRY(propagate<T>(P, S, g, ws, status, s, dir));
     const double sigma = smooth * double(npass - i) / double(npass);
@@ -1052,8 +1057,73 @@ static int transfer_nodiff(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<
 // :752-760, :782-785).  Diffraction 0: F P F^-1 per pixel and layer;
 // diffraction 1: the sub-steps described above.
 template <typename T>
+static int refl2_steps(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fout, hipStream_t s,
+                       const std::vector<Step>& st, const Beams& bm, cpx<T>* bulk);
+
+// diffraction > 1: per layer, the spectrum of the field is split by the tent
+// windows; each part goes back to real space, through all of the layer's
+// sub-steps with its window's mean beams, and the parts are summed
+// (dtmm/propagate_4x4.py:176-204).  ws: the window-row region of the
+// workspace (two fields) and the window sum.
+template <typename T>
+static int refl2_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* f, hipStream_t s,
+                         int dir, void* ws) {
+  const int H = P->d.height, W = P->d.width;
+  const size_t HW = size_t(H) * W;
+  const int nbeam = P->d.n_rays / P->d.rays_per_beam;
+  const int nlam = P->d.n_wavelengths;
+  const size_t n = size_t(P->d.n_rays) * nlam * 4 * HW;
+  if (size_t(P->d.n_rays) * nlam * 4 > 65535) return fail(DTMMX_E_ARG, "too many planes");
+  const WsLayout wl = ws_layout(P);
+  cpx<T>* fk = (cpx<T>*)((char*)ws + wl.win_tmp);
+  cpx<T>* part = fk + n;
+  cpx<T>* acc = (cpx<T>*)((char*)ws + wl.win_acc);
+  const std::vector<Step> all = expand_steps(S, dir);
+  const unsigned eb = unsigned((n + NT - 1) / NT);
+  const dim3 pg(unsigned((HW + NT - 1) / NT), unsigned(size_t(P->d.n_rays) * nlam * 4));
+  for (size_t k0 = 0; k0 < all.size();) {
+    size_t k1 = k0;
+    while (k1 < all.size() && all[k1].l == all[k0].l) ++k1;
+    const std::vector<Step> st(all.begin() + k0, all.begin() + k1);
+    HIP_TRY(hipMemcpyAsync(fk, f, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+    TRY(fft2_planes<T>(P, fk, 0, s));
+    const int nwin = int(P->win_i.size());
+    for (int w = 0; w < nwin; ++w) {
+      Beams bw = P->bm;
+      for (int q = 0; q < nbeam; ++q) {
+        bw.beta[q] = P->win_beta[size_t(w) * nbeam + q];
+        bw.phi[q] = P->win_phi[size_t(w) * nbeam + q];
+      }
+      hipLaunchKernelGGL((win_part_kernel<T>), pg, dim3(NT), 0, s, (const cpx<T>*)fk, part, H, W,
+                         nlam, P->d.rays_per_beam, P->wn, P->win_i[w], P->win_j[w],
+                         (const double*)P->wcx, (const double*)P->wcy, (const int*)P->wmask,
+                         1.0 / double(HW));
+      HIP_TRY(hipGetLastError());
+      TRY(fft2_planes<T>(P, part, 1, s));
+      TRY(refl2_steps<T>(P, S, part, s, st, bw, nullptr));
+      hipLaunchKernelGGL((accumulate_kernel<T>), dim3(eb), dim3(NT), 0, s, part, acc,
+                         w == 0 ? 1 : 0, n);
+      HIP_TRY(hipGetLastError());
+    }
+    if (nwin == 0) {  // no window passes betamax: every part is zero
+      hipLaunchKernelGGL((zero_kernel<T>), dim3(unsigned((2 * n + NT - 1) / NT)), dim3(NT), 0, s,
+                         (T*)f, 2 * n);
+      HIP_TRY(hipGetLastError());
+    } else {
+      HIP_TRY(hipMemcpyAsync(f, acc, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+    }
+    k0 = k1;
+  }
+  return DTMMX_OK;
+}
+
+template <typename T>
 static int refl2_pass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fout, hipStream_t s,
-                      int dir, cpx<T>* bulk = nullptr) {
+                      int dir, cpx<T>* bulk, void* ws) {
+  if (P->d.diffraction > 1) {
+    if (!ws) return fail(DTMMX_E_ARG, "reflection mode 2 with diffraction > 1 needs a workspace");
+    return refl2_windows<T>(P, S, fout, s, dir, ws);
+  }
   const size_t HW = size_t(P->d.height) * P->d.width;
   const int nbeam = P->d.n_rays / P->d.rays_per_beam;
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
@@ -1077,7 +1147,24 @@ static int refl2_pass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fout, h
     }
     return DTMMX_OK;
   }
-  const std::vector<Step> st = expand_steps(S, dir);
+  return refl2_steps<T>(P, S, fout, s, expand_steps(S, dir), P->bm, bulk);
+}
+
+// The diffraction-1 sub-steps of reflection mode 2 in place on the
+// real-space field, for the beams bm (the plan's, or a window's mean beams).
+template <typename T>
+static int refl2_steps(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fout, hipStream_t s,
+                       const std::vector<Step>& st, const Beams& bm, cpx<T>* bulk) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const int nbeam = P->d.n_rays / P->d.rays_per_beam;
+  const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
+  PixArgs<T> a{};
+  a.field = fout;
+  a.HW = HW;
+  a.nlam = P->d.n_wavelengths;
+  a.bm = bm;
+  a.comp_stride = size_t(S->n_layers) * HW;
+  const dim3 grid(unsigned((HW + NT - 1) / NT), unsigned(nbeam));
   const Batch b{P->d.n_rays, P->d.n_wavelengths, 0, P->d.n_wavelengths};
   for (size_t k = 0; k < st.size(); ++k) {
     const Step& q = st[k];
@@ -1097,7 +1184,7 @@ static int refl2_pass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fout, h
     ca.kcb = 1.;
     ca.scale = 1.0 / double(HW);
     ca.dense_sign = 1;
-    TRY(col_pass<T>(P, fout, b, 4, OP_DIAG, s, &ca));
+    TRY(col_pass<T>(P, fout, b, 4, OP_DIAG, s, &ca, &bm));
     TRY(row_pass<T>(P, fout, b, 4, OP_INV, s));
     hipLaunchKernelGGL((pixel_kernel<T, PIX_B4>), grid, dim3(NT), 0, s, a);
     HIP_TRY(hipGetLastError());
@@ -1110,11 +1197,11 @@ static int refl2_pass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fout, h
 
 template <typename T>
 static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
-                          cpx<T>* fout, hipStream_t s, cpx<T>* bulk = nullptr) {
+                          cpx<T>* fout, void* ws, hipStream_t s, cpx<T>* bulk = nullptr) {
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * size_t(P->d.height) *
                    P->d.width;
   HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
-  return refl2_pass<T>(P, S, fout, s, 1, bulk);
+  return refl2_pass<T>(P, S, fout, s, 1, bulk, ws);
 }
 
 template <typename T>
@@ -1126,7 +1213,7 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, v
   if (P->d.diffraction == 0)
     return transfer_nodiff<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s, (cpx<T>*)bulk);
   if (P->d.method == DTMMX_METHOD_4X4 && P->d.reflection == 2)
-    return transfer_refl2<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, s, (cpx<T>*)bulk);
+    return transfer_refl2<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s, (cpx<T>*)bulk);
   if (P->d.npass > 1) return transfer_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s);
   HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
   return propagate<T>(P, S, (cpx<T>*)fout, ws, status, s, 1, (cpx<T>*)bulk);
@@ -1788,8 +1875,8 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   const bool nodiff = d->diffraction == 0 &&
                       ((d->method == DTMMX_METHOD_4X4 && d->reflection == 2) ||
                        (d->method == DTMMX_METHOD_2X2 && d->reflection == 0 && d->npass == 1));
-  // 4x4 reflection mode 2 with one mean beam (any npass)
-  const bool refl2 = d->method == DTMMX_METHOD_4X4 && d->reflection == 2 && d->diffraction <= 1;
+  // 4x4 reflection mode 2 (any npass; diffraction > 1: tent windows)
+  const bool refl2 = d->method == DTMMX_METHOD_4X4 && d->reflection == 2;
   if (d->diffraction < 1 && !nodiff)
     return fail(DTMMX_E_UNSUPPORTED,
                 "diffraction must be >= 1, or 0 for the 4x4 method in reflection mode 2 and the "
@@ -1805,7 +1892,7 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->checkpoint < 0) return fail(DTMMX_E_ARG, "checkpoint interval must be >= 0");
   if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !refl2)
     return fail(DTMMX_E_UNSUPPORTED,
-                "the 4x4 method is built for reflection mode 1, and mode 2 with diffraction 0 or 1");
+                "the 4x4 method is built for reflection modes 1 and 2");
   if (d->method == DTMMX_METHOD_2X2 && d->reflection != 0 && d->reflection != 1)
     return fail(DTMMX_E_UNSUPPORTED, "the 2x2 method is built for reflection modes 0 and 1");
   return DTMMX_OK;

@@ -1859,6 +1859,33 @@ __global__ __launch_bounds__(NT) void reflect_update_kernel(cpx<T>* g, cpx<T>* f
 
 // Even-pass accumulation (dtmm/transfer.py:838-840): out += g; g = out
 // (first: out = g).
+// One tent window's part of a 2D spectrum (reflection mode 2 with
+// diffraction > 1, dtmm/propagate_4x4.py:176-204): dst = src * cx_i(kx) *
+// cy_j(ky) * scale where the mode passes the window mask, else 0; one thread
+// per (k, plane), planes [ray][wavelength][component].
+template <typename T>
+__global__ __launch_bounds__(NT) void win_part_kernel(const cpx<T>* src, cpx<T>* dst, int H,
+                                                      int W, int nlam, int rays_per_beam,
+                                                      int wn, int wi, int wj, const double* wcx,
+                                                      const double* wcy, const int* wmask,
+                                                      double scale) {
+  const size_t HW = size_t(H) * W;
+  const size_t k = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (k >= HW) return;
+  const int p = blockIdx.y;
+  const int ray = p / (nlam * 4), lam = (p / 4) % nlam;
+  const int beam = ray / rays_per_beam;
+  const int ky = int(k / W), kx = int(k - size_t(ky) * W);
+  const size_t bl = size_t(beam) * nlam + lam;
+  const size_t o = size_t(p) * HW + k;
+  cpx<T> v = mk<T>(T(0), T(0));
+  if (mode_index(kx, W) <= wmask[lam * H + ky]) {
+    const double f = wcx[(bl * wn + wi) * W + kx] * wcy[(bl * wn + wj) * H + ky] * scale;
+    v = src[o] * T(f);
+  }
+  dst[o] = v;
+}
+
 template <typename T>
 __global__ __launch_bounds__(NT) void accumulate_kernel(cpx<T>* g, cpx<T>* fout, int first,
                                                          size_t n) {

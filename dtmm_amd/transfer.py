@@ -384,11 +384,10 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
                                       "is supported")
         return kind, reflection
     if kind == "4x4":
-        if reflection == 2 and diffraction == 1:
+        if reflection == 2 and diffraction >= 1 and diffraction != np.inf:
             return kind, reflection
         if reflection != 1:
-            raise NotImplementedError("4x4 method: reflection mode 2 is supported with "
-                                      "diffraction 0 or 1; otherwise reflection mode 1")
+            raise NotImplementedError("4x4 method: reflection modes 1 and 2 are supported")
         if not (diffraction >= 1) or diffraction == np.inf:
             raise NotImplementedError("4x4 method: only diffraction 0 or finite diffraction >= 1 "
                                       "is supported")

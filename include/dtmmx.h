@@ -83,8 +83,9 @@ typedef struct {
                          transfer.py:206-221); DTMM_NORM_REF is not built */
   double smooth;      /* npass > 1: notch width of dtmm/denoise.py (dtmm.ini:8,
                          0.1); negative values count as 0                 */
-  int reflection;     /* 4x4: 1, or 2 with diffraction 0 or 1 (any npass:
-                         real-space passes, transfer.py:719-725).
+  int reflection;     /* 4x4: 1, or 2 (real space, transfer.py:719-725; any
+                         npass; diffraction > 1: the tent windows' parts,
+                         propagate_4x4.py:176-204).
                          2x2: 0, or 1 =
                          Fresnel transmission at
                          interfaces between different effective media

@@ -984,10 +984,36 @@ def propagate_4x4_effective_2(field, ks, layer, effective_layer, beta=0, phi=0, 
     (dtmm/propagate_4x4.py:156-175)."""
     if diffraction == 0:
         return _transfer_ray_4x4_2(field, ks, layer, beta, phi, nsteps)
-    if diffraction != 1:
-        raise NotImplementedError("oracle covers reflection mode 2 with diffraction 0 or 1")
     d_eff, epsv_eff, epsa_eff = effective_layer
     shape = field.shape[-2:]
+    if diffraction > 1:
+        # the real-space field split by the k-space tent windows, each part
+        # through all sub-steps with its window's mean beam, summed
+        # (propagate_4x4.py:176-204)
+        if diffraction == np.inf:
+            raise NotImplementedError("oracle covers finite diffraction")
+        beta = np.asarray(beta, FDTYPE)
+        phi = np.asarray(phi, FDTYPE)
+        if beta.ndim > 0:
+            bshape = beta.shape
+            beta = beta[..., 0]
+            phi = phi[..., 0]
+        else:
+            bshape = ()
+        windows, (betas, phis) = fft_mask(field.shape, ks, int(diffraction),
+                                          beta * np.cos(phi), beta * np.sin(phi), betamax)
+        nw = len(windows)
+        betas = betas.reshape((nw,) + bshape)
+        phis = phis.reshape((nw,) + bshape)
+        dmatps, dmatns = corrected_Epn_diffraction_matrix(shape, ks, betas, phis, d_eff,
+                                                          epsv_eff, epsa_eff, betamax)
+        fk = fft2(field)
+        fout = np.zeros_like(field)
+        for window, b, p, dmp, dmn in zip(windows, betas, phis, dmatps, dmatns):
+            fout += _transfer_ray_4x4_2(ifft2(fk * window), ks, layer, b, p, nsteps, (dmp, dmn))
+        return fout
+    if diffraction != 1:
+        raise NotImplementedError("oracle covers reflection mode 2 with diffraction >= 0")
     cache = cache or _Cache()
     key = _key(shape, ks, beta, phi, d_eff, epsv_eff, epsa_eff, betamax)
     dmatpn = cache.get(("pn",) + key, lambda: corrected_Epn_diffraction_matrix(
@@ -1246,7 +1272,7 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     """
     if ret_bulk and npass != 1:
         raise NotImplementedError("oracle covers ret_bulk with npass=1")
-    if reflection == 2 and diffraction in (0, 1) and npass == 1:
+    if reflection == 2 and diffraction >= 0 and npass == 1:
         # dtmm/transfer.py:694-700, :719-725 (work_in_fft False), :782-795
         # (propagate_4x4_effective_2), :874: the input is neither projected
         # nor transformed, input and output media are skipped
@@ -1271,9 +1297,8 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
             bulk[-1] = field
             return bulk, wavelengths, pixelsize
         return field, wavelengths, pixelsize
-    if reflection != 1 and not (reflection == 2 and diffraction in (0, 1)):
-        raise NotImplementedError("oracle covers reflection=1, and reflection=2 with "
-                                  "diffraction 0 or 1, for the 4x4 method")
+    if reflection not in (1, 2):
+        raise NotImplementedError("oracle covers reflection modes 1 and 2 for the 4x4 method")
     # reflection mode 2 works in real space (dtmm/transfer.py:719-725)
     work_in_fft = reflection != 2
     if norm & DTMM_NORM_REF:

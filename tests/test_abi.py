@@ -88,7 +88,6 @@ def _plan_rc(**kw):
     (dict(npass=0), _lib.DTMMX_E_ARG),
     (dict(method=2, npass=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(npass=3, norm=2), _lib.DTMMX_E_UNSUPPORTED),
-    (dict(reflection=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(method=2, reflection=0, npass=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(reflection=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(method=2, reflection=2), _lib.DTMMX_E_UNSUPPORTED),
@@ -336,9 +335,8 @@ def test_host_rop_grad_matches_finite_differences():
 def test_mode_resolution_no_diffraction():
     """diffraction=0 selects reflection mode 2 for the 4x4 method
     (dtmm/transfer.py:460-461) and keeps mode 0 for an explicit 2x2 method;
-    reflection mode 2 is also built with diffraction 1, and with npass > 1
-    (diffraction 0 or 1); other combinations are refused before any GPU
-    call."""
+    reflection mode 2 is also built with finite diffraction >= 1 and with
+    npass > 1; other combinations are refused before any GPU call."""
     from dtmm_amd.transfer import _resolve_modes
     assert _resolve_modes("4x4", 1, 0, None, None, False) == ("4x4", 2)
     assert _resolve_modes("2x2", 1, 0, None, None, False) == ("2x2", 0)
@@ -348,8 +346,9 @@ def test_mode_resolution_no_diffraction():
     with pytest.raises(NotImplementedError):
         _resolve_modes("4x4", 1, 0, 1, None, False)
     assert _resolve_modes("4x4", 1, 1, 2, None, False) == ("4x4", 2)
+    assert _resolve_modes("4x4", 1, 3, 2, None, False) == ("4x4", 2)
     with pytest.raises(NotImplementedError):
-        _resolve_modes("4x4", 1, 3, 2, None, False)
+        _resolve_modes("4x4", 1, np.inf, 2, None, False)
     assert _resolve_modes("4x4", 2, 1, 2, None, False) == ("4x4", 2)
     assert _resolve_modes("4x4", 3, 0, 2, None, False) == ("4x4", 2)
     with pytest.raises(NotImplementedError):

@@ -190,6 +190,25 @@ def test_reflection2_npass(diffraction, npass, norm, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("diffraction,npass", [(3, 1), (2, 1), (3, 2)])
+def test_reflection2_windows(diffraction, npass, precision):
+    """4x4 reflection mode 2 with diffraction > 1: per layer, the real-space
+    parts of the tent windows through the sub-steps with their mean beams,
+    summed (dtmm/propagate_4x4.py:176-204); two oblique beams, sub-steps."""
+    flush("case refl2 windows d=%d npass=%d %s" % (diffraction, npass, precision))
+    beta, phi = [0.1, 0.2], [0.4, -1.0]
+    fd, optical = random_case(shape=(3, 24, 32), wavelengths=(500., 620.), jones=None,
+                              beta=beta, phi=phi)
+    if npass == 1:
+        _compare(fd, optical, "4x4", precision, beta=beta, phi=phi, reflection=2,
+                 diffraction=diffraction, nstep=2, nin=1.1, nout=1.3)
+    else:
+        _compare_npass(fd, optical, precision, beta=beta, phi=phi, npass=npass, reflection=2,
+                       diffraction=diffraction, nin=1.1, nout=1.3)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("npass,norm", [(2, 1), (4, 0), (5, 1)])
 def test_npass_norms(npass, norm):
     flush("case npass=%d norm=%d" % (npass, norm))

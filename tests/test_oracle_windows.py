@@ -115,3 +115,44 @@ def test_windowed_uniaxial_slab_is_jones_retarder(n):
         e = J @ np.array(pol)
         np.testing.assert_allclose(out[i, 0], e[0], atol=1e-12)
         np.testing.assert_allclose(out[i, 2], e[1], atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_reflection2_windows_are_the_sum_of_window_parts(n):
+    """Reflection mode 2, diffraction n > 1 (propagate_4x4.py:176-204): for one
+    layer the result is the sum over the tent windows of the diffraction-1
+    reflection-2 step (pinned separately) applied to each window's part
+    ifft2(fft2(field) * window) with that window's mean beam; a layer of zero
+    thickness returns the band-limited E of the input (the windows sum to one
+    inside betamax; only the E parts are diffracted, H is rebuilt per pixel)."""
+    shape = (12, 16)
+    wl = np.array([520., 640.])
+    ps = 100.
+    beta, phi = 0.1, 0.7
+    rng = np.random.default_rng(4)
+    field = rng.normal(size=(2, 4) + shape) + 1j * rng.normal(size=(2, 4) + shape)
+    eps = np.stack([rng.uniform(2.2, 2.4, shape), rng.uniform(2.2, 2.4, shape),
+                    rng.uniform(2.6, 2.9, shape)], -1)[None]
+    eps[..., 1] = eps[..., 0]
+    ang = np.stack([np.zeros(shape), rng.uniform(0, 1.5, shape), rng.uniform(-3, 3, shape)],
+                   -1)[None]
+    block = (np.asarray([1.3]), eps, ang)
+    out = o.transfer_field((field, wl, ps), [block], beta=beta, phi=phi, method="4x4",
+                           reflection=2, diffraction=n, nstep=2, nin=1.2, nout=1.4)[0]
+    ks = o.k0(wl, ps)
+    windows, (betas, phis) = o.fft_mask(field.shape, ks, n, beta * np.cos(phi),
+                                        beta * np.sin(phi), o.BETAMAX)
+    fk = np.fft.fft2(field)
+    expect = np.zeros_like(field)
+    for w, b, p in zip(windows, betas, phis):
+        part = np.fft.ifft2(fk * w)
+        expect += o.transfer_field((part, wl, ps), [block], beta=float(b), phi=float(p),
+                                   method="4x4", reflection=2, diffraction=1, nstep=2,
+                                   nin=1.2, nout=1.4)[0]
+    np.testing.assert_allclose(out, expect, atol=1e-12 * np.abs(expect).max())
+    zero = (np.asarray([0.]), eps, ang)
+    out0 = o.transfer_field((field, wl, ps), [zero], beta=beta, phi=phi, method="4x4",
+                            reflection=2, diffraction=n)[0]
+    bb, _ = o.betaphi(shape, ks)
+    lim = np.fft.ifft2(fk * (bb < o.BETAMAX)[:, None])
+    np.testing.assert_allclose(out0[:, 0::2], lim[:, 0::2], atol=1e-12 * np.abs(lim).max())
