@@ -670,8 +670,9 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
 // out); dir = -1 runs the layers backward (npass > 1).
 // Copies the planes of a wavelength batch (all rays, 4 components).
 template <typename T>
-static int copy_planes(cpx<T>* dst, const cpx<T>* src, const Batch& b, size_t HW, hipStream_t s) {
-  const size_t per_lam = 4 * HW;
+static int copy_planes(cpx<T>* dst, const cpx<T>* src, const Batch& b, size_t HW, hipStream_t s,
+                       int nc = 4) {
+  const size_t per_lam = size_t(nc) * HW;
   for (int r = 0; r < b.nrays; ++r) {
     const size_t o = (size_t(r) * b.nlam + b.lam0) * per_lam;
     HIP_TRY(hipMemcpyAsync(dst + o, src + o, size_t(b.nlam_b) * per_lam * sizeof(cpx<T>),
@@ -709,7 +710,10 @@ static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, voi
   const int L = int(st.size());
   // workspace: 2x2 method: the 2-component field
   cpx<T>* e2 = (cpx<T>*)(wsb + wl.e2);
-  const int nlb = lam_batch(P, size_t(nrays) * 4 * HW * sizeof(cpx<T>));
+  // with bulk output the 2x2 method runs all wavelengths as one batch: its
+  // bulk step uses `out` as scratch once the input projection of every plane
+  // has read it
+  const int nlb = (bulk && !m4) ? nlam : lam_batch(P, size_t(nrays) * 4 * HW * sizeof(cpx<T>));
   for (int lam0 = 0; lam0 < nlam; lam0 += nlb) {
     Batch b{nrays, nlam, lam0, (lam0 + nlb <= nlam) ? nlb : nlam - lam0};
     if (m4) {
@@ -796,6 +800,40 @@ static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, voi
     for (int l = 0; l < L; ++l) {
       const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 0, status, wsb + wl.etab);
       TRY(row_pass<T>(P, e2, b, 2, OP_ROP, s, &la));
+      if (bulk && (l + 1 == L || st[l + 1].l != st[l].l)) {
+        // field after layer st[l].l (dtmm/propagate_2x2.py:80-85): E after the
+        // layer's second half step, in real space, and H of the forward modes
+        // of each voxel; `out` is free during the layer loop
+        cpx<T>* tmp = out;
+        TRY(copy_planes<T>(tmp, e2, b, HW, s, 2));
+        ColArgs<T> cb{};
+        cb.da = st[l].d;
+        cb.ea = st[l].e;
+        cb.db = 0.;
+        cb.eb = st[l].e;
+        cb.kca = 1.;
+        cb.kcb = 1.;
+        cb.scale = scale;
+        cb.dense_sign = 0;
+        TRY(col_pass<T>(P, tmp, b, 2, OP_DIAG, s, &cb));
+        TRY(row_pass<T>(P, tmp, b, 2, OP_INV, s));
+        PixArgs<T> pa{};
+        pa.src = tmp;
+        pa.dst = bulk + size_t(st[l].l) * nplanes * 4 * HW;
+        pa.HW = HW;
+        pa.nlam = nlam;
+        pa.lam0 = b.lam0;
+        pa.nlam_b = b.nlam_b;
+        pa.eps_re = S->eps_re + size_t(st[l].l) * HW;
+        pa.eps_im = S->eps_im ? S->eps_im + size_t(st[l].l) * HW : nullptr;
+        pa.ang = S->angles + size_t(st[l].l) * HW;
+        pa.comp_stride = size_t(S->n_layers) * HW;
+        pa.bm = P->bm;
+        hipLaunchKernelGGL((pixel_kernel<T, PIX_E2H>),
+                           dim3(unsigned((HW + NT - 1) / NT), unsigned(nrays / P->d.rays_per_beam)),
+                           dim3(NT), 0, s, pa);
+        HIP_TRY(hipGetLastError());
+      }
       const cd enext = (l + 1 < L) ? st[l + 1].e : eout;
       if (P->d.reflection && !eq(st[l].e, enext)) {
         // reflection mode 1: Fresnel transmission at the interface between
@@ -1033,6 +1071,14 @@ static int transfer_nodiff(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<
       const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
       HIP_TRY(hipMemcpyAsync(bulk + size_t(l) * n, fout, n * sizeof(cpx<T>),
                              hipMemcpyDeviceToDevice, s));
+    } else if (bulk) {
+      // 2x2: E after the layer and H of each voxel's forward modes
+      // (dtmm/propagate_2x2.py:185-188)
+      PixArgs<T> pa = a;
+      pa.src = f;
+      pa.dst = bulk + size_t(l) * P->d.n_rays * P->d.n_wavelengths * 4 * HW;
+      hipLaunchKernelGGL((pixel_kernel<T, PIX_E2H>), grid, dim3(NT), 0, s, pa);
+      HIP_TRY(hipGetLastError());
     }
   }
   if (!m4) {
@@ -2084,9 +2130,8 @@ int dtmmx_transfer_bulk(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, vo
     if (r != DTMMX_OK) return r;
   }
   if (!bulk) return fail(DTMMX_E_ARG, "null bulk buffer");
-  if (P->d.method != DTMMX_METHOD_4X4 || P->d.npass != 1 || P->d.diffraction > 1)
-    return fail(DTMMX_E_UNSUPPORTED,
-                "bulk output is built for the 4x4 method with npass 1 and diffraction 0 or 1");
+  if (P->d.npass != 1 || P->d.diffraction > 1)
+    return fail(DTMMX_E_UNSUPPORTED, "bulk output is built with npass 1 and diffraction 0 or 1");
   hipStream_t s = (hipStream_t)stream;
   if (P->d.precision == DTMMX_C64) return bulk_c64(P, S, fin, fout, bulk, ws, status, s);
   return bulk_c128(P, S, fin, fout, bulk, ws, status, s);

@@ -1968,7 +1968,10 @@ __global__ __launch_bounds__(NT) void e2h_kernel(E2HArgs<T> A) {
 // (F P F^-1)^n = F P^n F^-1.
 // ---------------------------------------------------------------------------
 
-enum { PIX_LAYER4 = 0, PIX_LAYER2 = 1, PIX_IN2 = 2, PIX_OUT2 = 3, PIX_A4 = 4, PIX_B4 = 5 };
+//   PIX_E2H:    as PIX_OUT2 with the forward modes of the layer's own voxel
+//               (the 2x2 field of every layer, dtmm/propagate_2x2.py:80-85).
+enum { PIX_LAYER4 = 0, PIX_LAYER2 = 1, PIX_IN2 = 2, PIX_OUT2 = 3, PIX_A4 = 4, PIX_B4 = 5,
+       PIX_E2H = 6 };
 
 template <typename T>
 struct PixArgs {
@@ -1984,6 +1987,7 @@ struct PixArgs {
   size_t comp_stride;
   double d;             // full layer thickness
   cd emed;              // IN2 / OUT2: isotropic medium eps
+  int lam0, nlam_b;     // OUT2 / E2H: wavelengths lam0 .. lam0 + nlam_b (nlam_b 0: all)
   Beams bm;
 };
 
@@ -2130,10 +2134,22 @@ __global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
       }
     }
   } else {
-    // isotropic medium of the beam: F and its inverse (fp64)
+    // isotropic medium of the beam (PIX_E2H: the layer's voxel): F and its
+    // inverse (fp64)
     Modes m;
     int kind;
-    layer_modes(beta, c0, s0, phi0, A.emed, A.emed, A.emed, 0., 0., 0., m, kind);
+    if constexpr (PK == PIX_E2H) {
+      const size_t o = px;
+      const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+      const cd e1 = mk<double>(A.eps_re[A.comp_stride + o],
+                               A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
+      const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
+                               A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
+      layer_modes(beta, c0, s0, phi0, e0, e1, e2, A.ang[o], A.ang[A.comp_stride + o],
+                  A.ang[2 * A.comp_stride + o], m, kind);
+    } else {
+      layer_modes(beta, c0, s0, phi0, A.emed, A.emed, A.emed, 0., 0., 0., m, kind);
+    }
     if constexpr (PK == PIX_IN2) {
       cd fi[4][4];
       inv4(m.F, fi);
@@ -2163,7 +2179,8 @@ __global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
       for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
           G[i][j] = cast<T>(m.F[2 * i + 1][0] * Ai[0][j] + m.F[2 * i + 1][2] * Ai[1][j]);
-      for (int lam = 0; lam < A.nlam; ++lam)
+      const int la0 = A.nlam_b ? A.lam0 : 0, la1 = A.nlam_b ? A.lam0 + A.nlam_b : A.nlam;
+      for (int lam = la0; lam < la1; ++lam)
         for (int ray = r0; ray < r1; ++ray) {
           const cpx<T>* e = plane(const_cast<cpx<T>*>(A.src), ray, lam, 2);
           const cpx<T> ex = e[0], ey = e[A.HW];

@@ -365,9 +365,8 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
     if npass > 1 and (norm & DTMM_NORM_REF):
         raise NotImplementedError("npass > 1 with DTMM_NORM_REF (reference-field "
                                   "normalisation) is not supported")
-    if ret_bulk and (method != "4x4" or npass != 1 or not (0 <= diffraction <= 1)):
-        raise NotImplementedError("ret_bulk is supported for the 4x4 method with npass=1 and "
-                                  "diffraction 0 or 1")
+    if ret_bulk and (npass != 1 or not (0 <= diffraction <= 1)):
+        raise NotImplementedError("ret_bulk is supported with npass=1 and diffraction 0 or 1")
     if eff_data not in (None, 0, "isotropic"):
         raise NotImplementedError("only isotropic effective data (eff_data=0) is supported")
     if diffraction == 0:

@@ -174,8 +174,10 @@ int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field
  * layout, real space, components Ex, Hy, Ey, Hx).  Replaces the ret_bulk
  * output of transfer_4x4 (transfer.py:670-690, :777-778, :876-877), whose
  * bulk_out[1 .. n_layers] these slots are; bulk_out[0] and bulk_out[-1] are
- * field_in and field_out.  Built for the 4x4 method with npass 1 and
- * diffraction 0 or 1 (reflection 1 or 2). */
+ * field_in and field_out.  The 2x2 method's slots (transfer_2x2,
+ * transfer.py:1045-1050, propagate_2x2.py:80-85) hold E after each layer and
+ * the H of each voxel's forward modes.  Built with npass 1 and diffraction 0
+ * or 1 (4x4: reflection 1 or 2; 2x2: reflection 0 or 1). */
 int dtmmx_transfer_bulk(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field_in,
                         void* field_out, void* bulk, void* workspace, size_t workspace_bytes,
                         int* status, void* stream);

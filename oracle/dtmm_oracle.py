@@ -1380,9 +1380,13 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
 
 
 def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1, nstep=1,
-                 diffraction=1, reflection=0, betamax=BETAMAX, layer_callback=None):
+                 diffraction=1, reflection=0, betamax=BETAMAX, layer_callback=None,
+                 ret_bulk=False):
     """2x2 (E-field) transfer, npass=1, reflection mode 0 or 1 with diffraction
-    1, reflection mode 0 with diffraction 0 (dtmm/transfer.py:1009-1183)."""
+    1, reflection mode 0 with diffraction 0 (dtmm/transfer.py:1009-1183).
+    ret_bulk returns (n_layers + 2, ...): the input, the field after every
+    layer (E and the H of each voxel's forward modes) and the output
+    (transfer.py:1045-1050, :1142-1150)."""
     if npass != 1 or reflection not in (0, 1):
         raise NotImplementedError("oracle covers npass=1, reflection 0 or 1 for the 2x2 method")
     field_in, wavelengths, pixelsize = field_data
@@ -1391,7 +1395,13 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     ks = k0(wavelengths, pixelsize)
     n = len(layers) - 1
     beta, phi = _validate_betaphi(beta, phi, extendeddim=field_in.ndim - 2)
-    field_out = np.zeros_like(field_in)
+    if ret_bulk:
+        bulk_out = np.zeros((n + 1,) + field_in.shape, CDTYPE)
+        bulk_out[0] = field_in
+        field_out = bulk_out[-1]
+    else:
+        bulk_out = None
+        field_out = np.zeros_like(field_in)
     if diffraction > 0:
         field0 = transmitted_field(field_in, ks, n=nin, betamax=betamax)
     else:
@@ -1414,25 +1424,26 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
                 for _ in range(nsub):
                     field = dotmdmf(e, p, ei, field)
                     field = dotmdmf(e, p, ei, field)
-            if j + 1 == n:
-                field_out[..., 1::2, :, :] += dotmf(E2H_mat(fmat, mode=+1), field)
-                field_out[..., ::2, :, :] += field
+            bulk = field_out if j + 1 == n else (bulk_out[j + 1] if ret_bulk else None)
+            if bulk is not None:
+                bulk[..., 1::2, :, :] += dotmf(E2H_mat(fmat, mode=+1), field)
+                bulk[..., ::2, :, :] += field
             if layer_callback is not None:
                 layer_callback(j)
-        return field_out, wavelengths, pixelsize
+        return (bulk_out if ret_bulk else field_out), wavelengths, pixelsize
     field = fft2(field)
     cache = _Cache()
     for j in range(n):
         nsub, layer_out = layers[j + 1]
         _, eff_out = eff_layers[j + 1]
         _, eff_in = eff_layers[j]
-        bulk = field_out if j + 1 == n else None
+        bulk = field_out if j + 1 == n else (bulk_out[j + 1] if ret_bulk else None)
         field = propagate_2x2_effective_1(field, ks, layer_out, eff_out, beta, phi, nsub,
                                           diffraction, betamax, +1, bulk, cache, reflection,
                                           eff_in)
         if layer_callback is not None:
             layer_callback(j)
-    return field_out, wavelengths, pixelsize
+    return (bulk_out if ret_bulk else field_out), wavelengths, pixelsize
 
 
 def _transmitted_field_direct(field, beta, phi, n=1.):
@@ -1471,10 +1482,8 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
         return transfer_4x4(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
                             diffraction, reflection, betamax, norm, smooth, layer_callback,
                             ret_bulk)
-    if ret_bulk:
-        raise NotImplementedError("oracle covers ret_bulk for the 4x4 method")
     return transfer_2x2(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
-                        diffraction, reflection, betamax, layer_callback)
+                        diffraction, reflection, betamax, layer_callback, ret_bulk)
 
 
 # ---------------------------------------------------------------------------

@@ -355,10 +355,9 @@ def test_mode_resolution_no_diffraction():
         _resolve_modes("4x4", 3, 0, None, None, False)  # npass > 1 selects reflection 1
     with pytest.raises(NotImplementedError):
         _resolve_modes("2x2", 2, 0, 0, None, False)
-    # ret_bulk: the 4x4 method with npass 1 and diffraction 0 or 1 only
+    # ret_bulk: npass 1 and diffraction 0 or 1 only
     assert _resolve_modes("4x4", 1, 1, None, None, True) == ("4x4", 1)
-    with pytest.raises(NotImplementedError):
-        _resolve_modes("2x2", 1, 1, None, None, True)
+    assert _resolve_modes("2x2", 1, 1, None, None, True) == ("2x2", 0)
     with pytest.raises(NotImplementedError):
         _resolve_modes("4x4", 3, 1, None, None, True)
     with pytest.raises(NotImplementedError):

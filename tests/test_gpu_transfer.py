@@ -640,17 +640,20 @@ def test_reflection2_diffraction_blocks_biaxial():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["c128", "c64"])
-@pytest.mark.parametrize("reflection,diffraction", [(1, 1), (2, 1), (2, 0)])
-def test_ret_bulk(reflection, diffraction, precision):
+@pytest.mark.parametrize("method,reflection,diffraction", [("4x4", 1, 1), ("4x4", 2, 1),
+                                                           ("4x4", 2, 0), ("2x2", 0, 1),
+                                                           ("2x2", 1, 1), ("2x2", 0, 0)])
+def test_ret_bulk(method, reflection, diffraction, precision):
     """ret_bulk: the field after every layer (plus input and output) against
     the oracle's bulk, with sub-steps and two beams."""
-    flush("case ret_bulk reflection %d diffraction %d %s" % (reflection, diffraction, precision))
+    flush("case ret_bulk %s reflection %d diffraction %d %s"
+          % (method, reflection, diffraction, precision))
     beta = [0., 0.15]
     phi = [0., 0.9]
     fd, optical = random_case(shape=(4, 32, 48), wavelengths=(450., 650.), jones=None,
                               beta=beta, phi=phi)
     field, wl, ps = fd
-    kw = dict(beta=beta, phi=phi, method="4x4", reflection=reflection, diffraction=diffraction,
+    kw = dict(beta=beta, phi=phi, method=method, reflection=reflection, diffraction=diffraction,
               nstep=2, nin=1.2, nout=1.0)
     ref = oracle.transfer_field((field, wl, ps), optical, ret_bulk=True, **kw)[0]
     dt = np.complex64 if precision == "c64" else np.complex128

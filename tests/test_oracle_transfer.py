@@ -484,3 +484,48 @@ def test_bulk_isotropic_slab_is_angular_spectrum_at_each_depth(reflection):
         for c in comps:
             np.testing.assert_allclose(bulk[j][:, c], expect[:, c],
                                        atol=1e-12 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("diffraction", [1, 0])
+def test_2x2_bulk_isotropic_slab(diffraction):
+    """2x2 ret_bulk (transfer.py:1045-1050, propagate_2x2.py:80-85) through a
+    homogeneous isotropic slab of the effective medium at normal incidence:
+    slot j holds E propagated by j layers (angular spectrum per FFT mode,
+    zero past betamax, with diffraction 1; a plane wave's phase with
+    diffraction 0) and H = n (Ex, -Ey) of the forward modes; slot 0 is the
+    input and the last slot the output."""
+    n, L = 1.55, 4
+    shape = (16, 24)
+    wl = np.array([550.])
+    ps = 100.
+    if diffraction == 1:
+        field, _, _ = o.illumination_data(shape, wl, pixelsize=ps, beta=0., phi=0.,
+                                          jones=(1, 1j), window=o.blackman(shape), n=n)
+    else:
+        field = np.zeros((1, 4) + shape, complex)
+        field[:, 0], field[:, 1], field[:, 2], field[:, 3] = 1., n, 0.5j, -0.5j * n
+    eps = np.broadcast_to([n ** 2] * 3, (L, 1, 1, 3))
+    block = [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))]
+    kw = dict(beta=0., phi=0., method="2x2", reflection=0, diffraction=diffraction, nin=n,
+              nout=n)
+    bulk = o.transfer_field((field, wl, ps), block, ret_bulk=True, **kw)[0]
+    out = o.transfer_field((field, wl, ps), block, **kw)[0]
+    assert bulk.shape == (L + 2,) + field.shape
+    np.testing.assert_allclose(bulk[0], field)
+    np.testing.assert_allclose(bulk[-1], out, atol=1e-14)
+    k = o.k0(wl, ps)
+    b, _ = o.betaphi(shape, k)
+    for j in range(1, L + 1):
+        if diffraction == 1:
+            prop = np.exp(1j * k[:, None, None] * j * np.sqrt(n ** 2 - b ** 2 + 0j))
+            prop[b >= o.BETAMAX] = 0.
+            ex = np.fft.ifft2(np.fft.fft2(field[:, 0]) * prop)
+            ey = np.fft.ifft2(np.fft.fft2(field[:, 2]) * prop)
+        else:
+            ph = np.exp(1j * k[:, None, None] * j * n)
+            ex, ey = field[:, 0] * ph, field[:, 2] * ph
+        tol = 1e-12 * np.abs(field).max()
+        np.testing.assert_allclose(bulk[j][:, 0], ex, atol=tol)
+        np.testing.assert_allclose(bulk[j][:, 2], ey, atol=tol)
+        np.testing.assert_allclose(bulk[j][:, 1], n * ex, atol=tol * n)
+        np.testing.assert_allclose(bulk[j][:, 3], -n * ey, atol=tol * n)
