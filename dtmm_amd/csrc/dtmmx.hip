@@ -1250,12 +1250,131 @@ static int transfer_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T
   return refl2_pass<T>(P, S, fout, s, 1, bulk, ws);
 }
 
+// 2x2 reflection mode 2, one pass, diffraction 0 or 1 (dtmm/transfer.py
+// :1071-1099 with work_in_fft False, propagate_2x2.py:91-190 with refl
+// None): E in real space; each layer is entered through the per-pixel
+// interface transmission from the previous one (PIX_T2, first half step),
+// then the corrected E diffraction of the layer's effective medium, then a
+// plain half step (PIX_LAYER2 at half thickness); the output medium is
+// entered the same way and builds H.
+template <typename T>
+static int transfer_2x2_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
+                              cpx<T>* fout, void* ws, hipStream_t s, cpx<T>* bulk) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const int nbeam = P->d.n_rays / P->d.rays_per_beam;
+  const int nlam = P->d.n_wavelengths;
+  const size_t nplanes = size_t(P->d.n_rays) * nlam;
+  const double scale = 1.0 / double(HW);
+  const bool diff = P->d.diffraction != 0;
+  const Batch b{P->d.n_rays, nlam, 0, nlam};
+  cpx<T>* e2 = (cpx<T>*)((char*)ws + ws_layout(P).e2);
+  const dim3 grid(unsigned((HW + NT - 1) / NT), unsigned(nbeam));
+  const cd ein = mk<double>(S->n_in * S->n_in), eout = mk<double>(S->n_out * S->n_out);
+  PixArgs<T> a{};
+  a.HW = HW;
+  a.nlam = nlam;
+  a.bm = P->bm;
+  a.comp_stride = size_t(S->n_layers) * HW;
+  // forward part of the input in the input medium (transfer.py:1071-1075)
+  if (diff) {
+    HIP_TRY(hipMemcpyAsync(fout, fin, nplanes * 4 * HW * sizeof(cpx<T>), hipMemcpyDeviceToDevice,
+                           s));
+    TRY(row_pass<T>(P, fout, b, 4, OP_FWD, s));
+    TRY(col_pass<T>(P, fout, b, 4, OP_FWD, s));
+    KArgs<T> h{};
+    h.kop = KOP_PROJ2;
+    h.d0 = 0.;
+    h.e0 = ein;
+    h.ein = ein;
+    h.scale = scale;
+    h.fresnel = 0;
+    TRY(kspace<T>(P, fout, e2, b, h, s));
+    TRY(col_pass<T>(P, e2, b, 2, OP_INV, s));
+    TRY(row_pass<T>(P, e2, b, 2, OP_INV, s));
+  } else {
+    a.src = fin;
+    a.field = e2;
+    a.emed = ein;
+    hipLaunchKernelGGL((pixel_kernel<T, PIX_IN2>), grid, dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+  }
+  a.field = e2;
+  // E diffraction of one layer's effective medium (corrected, full thickness d)
+  auto diffract2 = [&](double d, cd e) -> int {
+    TRY(row_pass<T>(P, e2, b, 2, OP_FWD, s));
+    ColArgs<T> ca{};
+    ca.da = d;
+    ca.db = d;
+    ca.ea = e;
+    ca.eb = e;
+    ca.kca = 1.;
+    ca.kcb = 1.;
+    ca.scale = scale;
+    ca.dense_sign = 0;
+    TRY(col_pass<T>(P, e2, b, 2, OP_DIAG, s, &ca));
+    return row_pass<T>(P, e2, b, 2, OP_INV, s);
+  };
+  const int L = S->n_layers;
+  for (int l = 0; l <= L; ++l) {
+    const bool out_medium = l == L;
+    a.prev_iso = l == 0;
+    a.pemed = ein;
+    if (l > 0) {
+      a.peps_re = S->eps_re + size_t(l - 1) * HW;
+      a.peps_im = S->eps_im ? S->eps_im + size_t(l - 1) * HW : nullptr;
+      a.pang = S->angles + size_t(l - 1) * HW;
+    }
+    a.cur_iso = out_medium;
+    a.emed = eout;
+    if (!out_medium) {
+      a.eps_re = S->eps_re + size_t(l) * HW;
+      a.eps_im = S->eps_im ? S->eps_im + size_t(l) * HW : nullptr;
+      a.ang = S->angles + size_t(l) * HW;
+    }
+    const int ns = out_medium ? 1 : (S->n_steps ? S->n_steps[l] : 1);
+    const double ds = out_medium ? 0. : S->thickness[l] / double(ns);
+    const cd ee = out_medium ? eout
+                             : mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
+    for (int j = 0; j < ns; ++j) {
+      a.d = 0.5 * ds;
+      if (j == 0) {
+        hipLaunchKernelGGL((pixel_kernel<T, PIX_T2>), grid, dim3(NT), 0, s, a);
+        HIP_TRY(hipGetLastError());
+      } else if (ds != 0.) {
+        hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER2>), grid, dim3(NT), 0, s, a);
+        HIP_TRY(hipGetLastError());
+      }
+      if (diff) TRY(diffract2(ds, ee));
+      if (ds != 0.) {
+        hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER2>), grid, dim3(NT), 0, s, a);
+        HIP_TRY(hipGetLastError());
+      }
+    }
+    if (bulk && !out_medium) {
+      PixArgs<T> pa = a;
+      pa.src = e2;
+      pa.dst = bulk + size_t(l) * nplanes * 4 * HW;
+      hipLaunchKernelGGL((pixel_kernel<T, PIX_E2H>), grid, dim3(NT), 0, s, pa);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  PixArgs<T> po = a;
+  po.src = e2;
+  po.dst = fout;
+  po.emed = eout;
+  hipLaunchKernelGGL((pixel_kernel<T, PIX_OUT2>), grid, dim3(NT), 0, s, po);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
 template <typename T>
 static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
                          void* ws, int* status, hipStream_t s, void* bulk = nullptr) {
   const size_t HW = size_t(P->d.height) * P->d.width;
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
   if (P->d.npass > 1) return transfer_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s);
+  if (P->d.method == DTMMX_METHOD_2X2 && P->d.reflection == 2)
+    return transfer_2x2_refl2<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s, (cpx<T>*)bulk);
   if (P->d.diffraction == 0)
     return transfer_nodiff<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s, (cpx<T>*)bulk);
   if (P->d.method == DTMMX_METHOD_4X4 && P->d.reflection == 2)
@@ -1920,7 +2039,8 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   // one pass
   const bool nodiff = d->diffraction == 0 &&
                       ((d->method == DTMMX_METHOD_4X4 && d->reflection == 2) ||
-                       (d->method == DTMMX_METHOD_2X2 && d->reflection == 0 && d->npass == 1));
+                       (d->method == DTMMX_METHOD_2X2 && (d->reflection == 0 || d->reflection == 2) &&
+                        d->npass == 1));
   // 4x4 reflection mode 2 (any npass; diffraction > 1: tent windows)
   const bool refl2 = d->method == DTMMX_METHOD_4X4 && d->reflection == 2;
   if (d->diffraction < 1 && !nodiff)
@@ -1939,8 +2059,10 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !refl2)
     return fail(DTMMX_E_UNSUPPORTED,
                 "the 4x4 method is built for reflection modes 1 and 2");
-  if (d->method == DTMMX_METHOD_2X2 && d->reflection != 0 && d->reflection != 1)
-    return fail(DTMMX_E_UNSUPPORTED, "the 2x2 method is built for reflection modes 0 and 1");
+  if (d->method == DTMMX_METHOD_2X2 && d->reflection != 0 && d->reflection != 1 &&
+      !(d->reflection == 2 && d->diffraction <= 1))
+    return fail(DTMMX_E_UNSUPPORTED, "the 2x2 method is built for reflection modes 0 and 1, "
+                                     "and 2 with diffraction 0 or 1");
   return DTMMX_OK;
 }
 

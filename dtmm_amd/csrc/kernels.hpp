@@ -1970,8 +1970,14 @@ __global__ __launch_bounds__(NT) void e2h_kernel(E2HArgs<T> A) {
 
 //   PIX_E2H:    as PIX_OUT2 with the forward modes of the layer's own voxel
 //               (the 2x2 field of every layer, dtmm/propagate_2x2.py:80-85).
+//   PIX_T2:     2x2 reflection mode 2, first half step of a layer entered
+//               through the interface from the previous layer's voxel (or
+//               the isotropic input medium): e diag(p) Eti with
+//               Eti = St^-1 A^-1 (dtmm/tmm.py:999-1022,
+//               dtmm/propagate_2x2.py:172-177); the current layer may be the
+//               isotropic output medium (emed, cur_iso).
 enum { PIX_LAYER4 = 0, PIX_LAYER2 = 1, PIX_IN2 = 2, PIX_OUT2 = 3, PIX_A4 = 4, PIX_B4 = 5,
-       PIX_E2H = 6 };
+       PIX_E2H = 6, PIX_T2 = 7 };
 
 template <typename T>
 struct PixArgs {
@@ -1988,8 +1994,25 @@ struct PixArgs {
   double d;             // full layer thickness
   cd emed;              // IN2 / OUT2: isotropic medium eps
   int lam0, nlam_b;     // OUT2 / E2H: wavelengths lam0 .. lam0 + nlam_b (nlam_b 0: all)
+  // T2: the previous layer's material (same layout) or, with prev_iso, the
+  // isotropic medium pemed; cur_iso: the current layer is emed
+  const double* peps_re;
+  const double* peps_im;
+  const double* pang;
+  int prev_iso, cur_iso;
+  cd pemed;
   Beams bm;
 };
+
+__device__ inline void pixel_modes(const double* er, const double* ei, const double* ang,
+                                   size_t cs, size_t o, double beta, double c0, double s0,
+                                   double phi0, Modes& m) {
+  const cd e0 = mk<double>(er[o], ei ? ei[o] : 0.);
+  const cd e1 = mk<double>(er[cs + o], ei ? ei[cs + o] : 0.);
+  const cd e2 = mk<double>(er[2 * cs + o], ei ? ei[2 * cs + o] : 0.);
+  int kind;
+  layer_modes(beta, c0, s0, phi0, e0, e1, e2, ang[o], ang[cs + o], ang[2 * cs + o], m, kind);
+}
 
 template <typename T, int PK>
 __global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
@@ -2131,6 +2154,49 @@ __global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
           f[0] = m00 * ex + m01 * ey;
           f[A.HW] = m10 * ex + m11 * ey;
         }
+      }
+    }
+  } else if constexpr (PK == PIX_T2) {
+    Modes mi, mo;
+    int kind;
+    if (A.prev_iso)
+      layer_modes(beta, c0, s0, phi0, A.pemed, A.pemed, A.pemed, 0., 0., 0., mi, kind);
+    else
+      pixel_modes(A.peps_re, A.peps_im, A.pang, A.comp_stride, px, beta, c0, s0, phi0, mi);
+    if (A.cur_iso)
+      layer_modes(beta, c0, s0, phi0, A.emed, A.emed, A.emed, 0., 0., 0., mo, kind);
+    else
+      pixel_modes(A.eps_re, A.eps_im, A.ang, A.comp_stride, px, beta, c0, s0, phi0, mo);
+    // Eti = St^-1 A^-1: A the E block of the incoming forward modes, St the
+    // forward-forward block of F_in^-1 F_out
+    cd fi[4][4];
+    inv4(mi.F, fi);
+    cd St[2][2], Am[2][2];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        cd t = mk<double>(0.);
+        for (int k = 0; k < 4; ++k) t = t + fi[2 * i][k] * mo.F[k][2 * j];
+        St[i][j] = t;
+        Am[i][j] = mi.F[2 * i][2 * j];
+      }
+    cd Sti[2][2], Ai[2][2], Eti[2][2];
+    inv2(St, Sti);
+    inv2(Am, Ai);
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) Eti[i][j] = Sti[i][0] * Ai[0][j] + Sti[i][1] * Ai[1][j];
+    const cd e[2][2] = {{mo.F[0][0], mo.F[0][2]}, {mo.F[2][0], mo.F[2][2]}};
+    for (int lam = 0; lam < A.nlam; ++lam) {
+      const double kd = A.bm.k0[lam] * A.d;
+      const cd p0 = cexpi(mo.a[0] * kd), p2 = cexpi(mo.a[2] * kd);
+      cpx<T> M[2][2];
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+          M[i][j] = cast<T>(e[i][0] * p0 * Eti[0][j] + e[i][1] * p2 * Eti[1][j]);
+      for (int ray = r0; ray < r1; ++ray) {
+        cpx<T>* f = plane(A.field, ray, lam, 2);
+        const cpx<T> ex = f[0], ey = f[A.HW];
+        f[0] = M[0][0] * ex + M[0][1] * ey;
+        f[A.HW] = M[1][0] * ex + M[1][1] * ey;
       }
     }
   } else {

@@ -378,9 +378,9 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
         if kind == "4x4" and reflection != 2:
             raise NotImplementedError("4x4 method with diffraction=0: only reflection mode 2 "
                                       "is supported")
-        if kind == "2x2" and reflection != 0:
-            raise NotImplementedError("2x2 method with diffraction=0: only reflection mode 0 "
-                                      "is supported")
+        if kind == "2x2" and reflection not in (0, 2):
+            raise NotImplementedError("2x2 method with diffraction=0: reflection modes 0 and 2 "
+                                      "are supported")
         return kind, reflection
     if kind == "4x4":
         if reflection == 2 and diffraction >= 1 and diffraction != np.inf:
@@ -391,8 +391,11 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
             raise NotImplementedError("4x4 method: only diffraction 0 or finite diffraction >= 1 "
                                       "is supported")
     else:
+        if reflection == 2 and diffraction == 1 and npass == 1:
+            return kind, reflection
         if reflection not in (0, 1):
-            raise NotImplementedError("2x2 method: reflection mode 2 is not supported")
+            raise NotImplementedError("2x2 method: reflection mode 2 is supported with "
+                                      "diffraction 0 or 1 and npass=1")
         if diffraction != 1:
             raise NotImplementedError("2x2 method: only diffraction 0 (reflection 0) or 1 is "
                                       "supported")

@@ -86,7 +86,9 @@ typedef struct {
   int reflection;     /* 4x4: 1, or 2 (real space, transfer.py:719-725; any
                          npass; diffraction > 1: the tent windows' parts,
                          propagate_4x4.py:176-204).
-                         2x2: 0, or 1 =
+                         2x2: 0; 2 (diffraction 0 or 1, npass 1: real
+                         space with the per-pixel interface transmission
+                         into every layer, propagate_2x2.py:91-190); or 1 =
                          Fresnel transmission at
                          interfaces between different effective media
                          (propagate_2x2.py:28-30, diffract.py:215-225)    */

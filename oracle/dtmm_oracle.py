@@ -53,7 +53,7 @@ __all__ = [
     "polarizer", "as4x4", "jonesmat4x4", "ray_jonesmat4x4", "field2specter",
     "calculate_pom_field", "pom_specter", "XYZ2RGBD65", "apply_gamma", "apply_srgb_gamma",
     "specter2color", "f_iso", "layer_mat", "stack_mat", "system_mat", "reflection_mat",
-    "transmit4x4", "transfer4x4",
+    "transmit4x4", "transfer4x4", "Eti_mat", "corrected_E_diffraction_matrix",
 ]
 
 # dtmm/dtmm.ini:6 (betamax = 0.8) and dtmm/conf.py:148, :503-510 (nin/nout
@@ -889,6 +889,14 @@ def first_E_diffraction_matrix(shape, ks, beta, phi, d, epsv, epsa, mode=+1, bet
     return dotmm(cmat, dmat)
 
 
+def corrected_E_diffraction_matrix(shape, ks, beta, phi, d, epsv, epsa, mode=+1,
+                                   betamax=BETAMAX):
+    """C(d/2) . D(d) . C(d/2) (dtmm/matrix.py:38-43)."""
+    dmat = _E_diffraction_matrix(shape, ks, d, epsv, epsa, mode, betamax)
+    cmat = _E_correction_matrix(beta, phi, ks, d / 2., epsv, epsa, mode)
+    return dotmm(cmat, dotmm(dmat, cmat))
+
+
 def second_E_diffraction_matrix(shape, ks, beta, phi, d, epsv, epsa, mode=+1, betamax=BETAMAX):
     """D . C (dtmm/matrix.py:46-51)."""
     dmat = _E_diffraction_matrix(shape, ks, d, epsv, epsa, mode, betamax)
@@ -1074,6 +1082,42 @@ def S_mat(fmatin, fmatout, fmatini=None, mode=+1):
     if mode == +1:
         return S[..., ::2, ::2], S[..., 1::2, 0::2]
     return S[..., 1::2, 1::2], S[..., 0::2, 1::2]
+
+
+def Eti_mat(fmatin, fmatout, fmatini=None, mode=+1):
+    """E transmission into the modes of fmatout, no reflections
+    (dtmm/tmm.py:999-1022)."""
+    A = E_mat(fmatin, mode=mode)
+    St, _ = S_mat(fmatin, fmatout, fmatini, mode)
+    return dotmm(inv(St), inv(A))
+
+
+def _transfer_ray_2x2_2(field, ks, in_layer, out_layer, dmat, beta, phi, nsteps, mode=+1,
+                        bulk=None):
+    """2x2 reflection mode 2 in real space, one pass (dtmm/propagate_2x2.py
+    :91-190 with refl None): the first half step carries the interface
+    transmission Eti from the previous layer's voxel modes, then the E
+    diffraction dmat, then a plain half step e P e^-1."""
+    _, epsv_in, epsa_in = in_layer
+    _, fmat_in = alphaf(beta, phi, epsv_in, epsa_in)
+    d, epsv, epsa = out_layer
+    alpha, fmat = alphaf(beta, phi, epsv, epsa)
+    e = E_mat(fmat, mode=mode)
+    p = phase_mat(alpha, (np.asarray(ks) * d / 2)[..., None, None], mode=mode)
+    ei0 = inv(e)
+    for j in range(nsteps):
+        if j == 0:
+            field = dotmdmf(e, p, Eti_mat(fmat_in, fmat, mode=mode), field)
+        elif d != 0.:
+            field = dotmdmf(e, p, ei0, field)
+        if dmat is not None:
+            field = diffract(field, dmat)
+        if d != 0.:
+            field = dotmdmf(e, p, ei0, field)
+    if bulk is not None:
+        bulk[..., 1::2, :, :] += dotmf(E2H_mat(fmat, mode=mode), field)
+        bulk[..., ::2, :, :] += field
+    return field
 
 
 def Etri_mat(fmatin, fmatout, fmatini=None, mode=+1):
@@ -1387,8 +1431,8 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     ret_bulk returns (n_layers + 2, ...): the input, the field after every
     layer (E and the H of each voxel's forward modes) and the output
     (transfer.py:1045-1050, :1142-1150)."""
-    if npass != 1 or reflection not in (0, 1):
-        raise NotImplementedError("oracle covers npass=1, reflection 0 or 1 for the 2x2 method")
+    if npass != 1 or reflection not in (0, 1, 2):
+        raise NotImplementedError("oracle covers npass=1 for the 2x2 method")
     field_in, wavelengths, pixelsize = field_data
     field_in = np.asarray(field_in, CDTYPE)
     layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
@@ -1407,6 +1451,27 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     else:
         field0 = _transmitted_field_direct(field_in, beta, phi, n=nin)
     field = field0[..., ::2, :, :].copy()
+    if reflection == 2:
+        # real space (work_in_fft False, transfer.py:1086-1093), each layer
+        # entered through the interface from the previous one
+        if diffraction > 1:
+            raise NotImplementedError("oracle covers 2x2 reflection 2 with diffraction 0 or 1")
+        cache = _Cache()
+        for j in range(n):
+            nsub, layer_out = layers[j + 1]
+            _, layer_in = layers[j]
+            _, (d_eff, e_eff, a_eff) = eff_layers[j + 1]
+            dmat = None
+            if diffraction:
+                key = _key(field.shape[-2:], ks, beta, phi, d_eff, e_eff, a_eff, betamax)
+                dmat = cache.get(("e2",) + key, lambda: corrected_E_diffraction_matrix(
+                    field.shape[-2:], ks, beta, phi, d_eff, e_eff, a_eff, +1, betamax))
+            bulk = field_out if j + 1 == n else (bulk_out[j + 1] if ret_bulk else None)
+            field = _transfer_ray_2x2_2(field, ks, layer_in, layer_out, dmat, beta, phi, nsub,
+                                        +1, bulk)
+            if layer_callback is not None:
+                layer_callback(j)
+        return (bulk_out if ret_bulk else field_out), wavelengths, pixelsize
     if diffraction == 0:
         if reflection != 0:
             raise NotImplementedError("oracle covers diffraction == 0 with reflection 0 for the "

@@ -90,7 +90,7 @@ def _plan_rc(**kw):
     (dict(npass=3, norm=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(method=2, reflection=0, npass=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(reflection=3), _lib.DTMMX_E_UNSUPPORTED),
-    (dict(method=2, reflection=2), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(method=2, reflection=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
 ])
 def test_plan_create_rejects_bad_descriptions(kw, code):
     rc, msg = _plan_rc(**kw)
@@ -355,6 +355,10 @@ def test_mode_resolution_no_diffraction():
         _resolve_modes("4x4", 3, 0, None, None, False)  # npass > 1 selects reflection 1
     with pytest.raises(NotImplementedError):
         _resolve_modes("2x2", 2, 0, 0, None, False)
+    assert _resolve_modes("2x2", 1, 0, 2, None, False) == ("2x2", 2)
+    assert _resolve_modes("2x2", 1, 1, 2, None, False) == ("2x2", 2)
+    with pytest.raises(NotImplementedError):
+        _resolve_modes("2x2", 1, 3, 2, None, False)
     # ret_bulk: npass 1 and diffraction 0 or 1 only
     assert _resolve_modes("4x4", 1, 1, None, None, True) == ("4x4", 1)
     assert _resolve_modes("2x2", 1, 1, None, None, True) == ("2x2", 0)

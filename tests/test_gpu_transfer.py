@@ -209,6 +209,32 @@ def test_reflection2_windows(diffraction, npass, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("diffraction", [1, 0])
+def test_2x2_reflection2(diffraction, precision):
+    """2x2 reflection mode 2 (real space, per-pixel interface transmission
+    into every layer, dtmm/propagate_2x2.py:91-190): two oblique beams,
+    sub-steps, two blocks with different effective media, and the bulk."""
+    flush("case 2x2 refl2 d=%d %s" % (diffraction, precision))
+    beta, phi = [0.05, 0.15], [0.3, -1.2]
+    fd, b1 = random_case(shape=(3, 32, 48), wavelengths=(470., 630.), jones=None,
+                         beta=beta, phi=phi)
+    _, b2 = droplet_case(shape=(2, 32, 48), radius=10, ne=1.72, nhost=1.55)
+    kw = dict(beta=beta, phi=phi, reflection=2, diffraction=diffraction, nstep=2, nin=1.1,
+              nout=1.35)
+    _compare(fd, [b1[0], b2[0]], "2x2", precision, **kw)
+    field, wl, ps = fd
+    ref = oracle.transfer_field((field, wl, ps), [b1[0], b2[0]], method="2x2", ret_bulk=True,
+                                **kw)[0]
+    dt = np.complex64 if precision == "c64" else np.complex128
+    out = dm.transfer_field((field.astype(dt), wl, ps), [b1[0], b2[0]], method="2x2",
+                            precision=precision, ret_bulk=True, **kw)[0]
+    assert out.shape == ref.shape
+    for j in range(out.shape[0]):
+        assert rel_err(out[j], ref[j]) < TOL[precision], j
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("npass,norm", [(2, 1), (4, 0), (5, 1)])
 def test_npass_norms(npass, norm):
     flush("case npass=%d norm=%d" % (npass, norm))

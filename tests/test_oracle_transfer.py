@@ -208,9 +208,11 @@ def test_interface_transmission_is_fresnel():
     np.testing.assert_array_equal(t[~m], 0.)
 
 
-def test_2x2_reflection1_slab_is_single_pass_fresnel():
-    """2x2 with reflection mode 1 on a homogeneous isotropic slab: the
-    interfaces transmit with the Fresnel coefficients (no multiple
+@pytest.mark.parametrize("reflection,diffraction", [(1, 1), (2, 1), (2, 0)])
+def test_2x2_reflection_slab_is_single_pass_fresnel(reflection, diffraction):
+    """2x2 with reflection mode 1 (k-space Fresnel factors) or 2 (per-pixel
+    interface matrices Eti in real space) on a homogeneous isotropic slab:
+    the interfaces transmit with the Fresnel coefficients (no multiple
     reflections in the 2x2 method)."""
     nin, ns, nout, L = 1.0, 1.5, 1.25, 7
     shape = (8, 8)
@@ -220,7 +222,8 @@ def test_2x2_reflection1_slab_is_single_pass_fresnel():
     f[:, 0], f[:, 1] = 1., nin
     eps = np.broadcast_to([ns ** 2] * 3, (L, 1, 1, 3))
     out = o.transfer_field((f, wl, 100.), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))], beta=0.,
-                           phi=0., method="2x2", reflection=1, nin=nin, nout=nout)[0]
+                           phi=0., method="2x2", reflection=reflection, diffraction=diffraction,
+                           nin=nin, nout=nout)[0]
     for i, k in enumerate(ks):
         t = 2 * nin / (nin + ns) * 2 * ns / (ns + nout) * np.exp(1j * k * ns * L)
         np.testing.assert_allclose(out[i, 0], t, atol=1e-12)
@@ -529,3 +532,35 @@ def test_2x2_bulk_isotropic_slab(diffraction):
         np.testing.assert_allclose(bulk[j][:, 2], ey, atol=tol)
         np.testing.assert_allclose(bulk[j][:, 1], n * ex, atol=tol * n)
         np.testing.assert_allclose(bulk[j][:, 3], -n * ey, atol=tol * n)
+
+
+@pytest.mark.parametrize("pol", [(1, 0), (0, 1)])
+def test_2x2_reflection2_oblique_interfaces_are_fresnel(pol):
+    """2x2 reflection mode 2 without diffraction at oblique incidence through
+    two homogeneous isotropic layers: the transmitted tangential E is the
+    product of the Fresnel transmissions (p: 2 g0 / (g0 + g1), g = eps / a;
+    s: 2 a0 / (a0 + a1), a = sqrt(eps - beta^2)) and the layers' phases."""
+    n0, n1, n2, n3 = 1.0, 1.5, 1.7, 1.2
+    beta, phi = 0.5, 0.
+    shape = (4, 4)
+    wl = np.array([600.])
+    k = o.k0(wl, 100.)[0]
+    f0 = o.fvec(o.f_iso(beta, phi, n0), jones=pol)
+    field = np.broadcast_to(f0[None, :, None, None], (1, 4) + shape).copy()
+    eps = np.asarray([[[[n1 ** 2] * 3]], [[[n2 ** 2] * 3]]])
+    d = np.asarray([1.3, 0.7])
+    out = o.transfer_field((field, wl, 100.), [(d, np.broadcast_to(eps, (2, 1, 1, 3)),
+                                                 np.zeros((2, 1, 1, 3)))],
+                           beta=beta, phi=phi, method="2x2", reflection=2, diffraction=0,
+                           nin=n0, nout=n3)[0]
+    ns = [n0, n1, n2, n3]
+    a = [np.sqrt(x * x - beta * beta) for x in ns]
+    g = [x * x / y for x, y in zip(ns, a)]
+    if pol == (1, 0):
+        t = np.prod([2 * g[i] / (g[i] + g[i + 1]) for i in range(3)])
+        comp, e_in = 0, f0[0]
+    else:
+        t = np.prod([2 * a[i] / (a[i] + a[i + 1]) for i in range(3)])
+        comp, e_in = 2, f0[2]
+    ph = np.exp(1j * k * (a[1] * d[0] + a[2] * d[1]))
+    np.testing.assert_allclose(out[0, comp], t * ph * e_in, atol=1e-12)
