@@ -12,7 +12,8 @@ from .data import (refind2eps, uniaxial_order, director2data, director2angles,  
                    director2order, nematic_droplet_director, nematic_droplet_data,
                    validate_optical_block, validate_optical_data)
 from .field import (illumination_data, field2intensity, blackman, k0, betaphi,  # noqa: F401
-                    planewave)
+                    planewave, aperture, illumination_aperture, aperture2rays,
+                    illumination_rays)
 from .transfer import (transfer_field, transfer_batch, TransferGraph, Plan, Stack,  # noqa: F401
                        DTMM_NORM_FFT, DTMM_NORM_REF, field2betaphi, mean_betaphi)
 from .adjoint import AdjointTransfer, polarizer4x4, analyser_intensity_loss  # noqa: F401

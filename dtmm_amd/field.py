@@ -221,3 +221,65 @@ def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensi
         norm = norm * ratio[..., None, None]
     field = np.ascontiguousarray(norm[..., None, :, :] * field)
     return field, wavelengths, pixelsize
+
+
+def _radius(shape, scale=1.):
+    """Radius of the half-pixel-centred grid, 1 at the edge (dtmm/window.py:17-23)."""
+    ny, nx = [l / 2 for l in shape]
+    ay, ax = [np.arange(-l / 2. + .5, l / 2. + .5) for l in shape]
+    xx, yy = np.meshgrid(ax, ay, indexing="xy")
+    return np.sqrt((xx / (nx * scale)) ** 2 + (yy / (ny * scale)) ** 2)
+
+
+def _tukey(r, alpha, rmax=1.):
+    """Flat top with a raised-cosine edge of relative width alpha
+    (dtmm/window.py:143-156)."""
+    out = np.ones(np.shape(r), FDTYPE)
+    w = alpha * rmax
+    if w > 0.:
+        edge = r > rmax - w
+        out[edge] = 0.5 * (1 - np.cos(np.pi * (r[edge] - rmax) / w))
+    out[r >= rmax] = 0.
+    return out
+
+
+def aperture(shape, diameter=1., smooth=0.05):
+    """(Annular) aperture window (dtmm/window.py:107-141)."""
+    inner, outer = diameter if np.ndim(diameter) else (0, diameter)
+    a = _tukey(_radius(shape, outer), smooth)
+    if inner == 0:
+        return a
+    return a * (1 - _tukey(_radius(shape, inner), smooth))
+
+
+def illumination_aperture(diameter=5., smooth=0.1):
+    """Diaphragm image of an (annular) field aperture (dtmm/field.py:356-369)."""
+    if np.ndim(diameter):
+        inner, outer = (int(round(x)) for x in diameter)
+        if inner >= outer:
+            raise ValueError("Invalid dimater, rounded inner diameter should be lower that "
+                             "outer diameter.")
+    else:
+        inner, outer = 0, int(round(diameter))
+    return aperture((outer, outer), (inner / outer, 1.), min(max(smooth, 0), 1))
+
+
+def aperture2rays(diaphragm, betastep=0.1, norm=True):
+    """beta, phi and intensity of the lit pixels of a diaphragm image
+    (dtmm/field.py:342-354)."""
+    diaphragm = np.asarray(diaphragm)
+    ay, ax = [np.arange(-l / 2. + .5, l / 2. + .5) for l in diaphragm.shape]
+    xx, yy = np.meshgrid(ax, ay, indexing="xy")
+    lit = diaphragm > 0.
+    intensity = diaphragm[lit]
+    if norm:
+        intensity = intensity * (len(intensity) / intensity.sum())
+    return (np.asarray(np.hypot(xx, yy)[lit] * betastep, FDTYPE),
+            np.asarray(np.arctan2(yy, xx)[lit], FDTYPE), np.asarray(intensity, FDTYPE))
+
+
+def illumination_rays(NA, diameter=5., smooth=0.1):
+    """beta, phi, intensity of rays filling a numerical aperture NA
+    (dtmm/field.py:371-404); pass them to illumination_data."""
+    low, high = diameter if np.ndim(diameter) else (0, diameter)
+    return aperture2rays(illumination_aperture((low, high), smooth), betastep=2. * NA / (high - 1))

@@ -54,6 +54,7 @@ __all__ = [
     "calculate_pom_field", "pom_specter", "XYZ2RGBD65", "apply_gamma", "apply_srgb_gamma",
     "specter2color", "f_iso", "layer_mat", "stack_mat", "system_mat", "reflection_mat",
     "transmit4x4", "transfer4x4", "Eti_mat", "corrected_E_diffraction_matrix",
+    "tukey", "aperture", "illumination_aperture", "aperture2rays", "illumination_rays",
 ]
 
 # dtmm/dtmm.ini:6 (betamax = 0.8) and dtmm/conf.py:148, :503-510 (nin/nout
@@ -1823,3 +1824,78 @@ def transfer4x4(fvec_in, kd, epsv, epsa, beta=0., phi=0., nin=1., nout=1., fvec_
     fmatout = f_iso(beta, phi, nout)
     cmat = stack_mat(kd, epsv, epsa, beta, phi)
     return transmit4x4(fvec_in, cmat, fmatin, fmatout, fvec_out)
+
+
+# ---------------------------------------------------------------------------
+# window.py aperture and field.py illumination rays
+# ---------------------------------------------------------------------------
+
+def _radius(shape, scale=1.):
+    """Radius of the half-pixel-centred grid, 1 at the edge (dtmm/window.py:17-23)."""
+    ny, nx = [l / 2 for l in shape]
+    ay, ax = [np.arange(-l / 2. + .5, l / 2. + .5) for l in shape]
+    xx, yy = np.meshgrid(ax, ay, indexing="xy")
+    return ((xx / (nx * scale)) ** 2 + (yy / (ny * scale)) ** 2) ** 0.5
+
+
+def tukey(r, alpha=0.1, rmax=1.):
+    """(dtmm/window.py:143-156)."""
+    r = np.asarray(r, FDTYPE)
+    out = np.ones(r.shape, FDTYPE)
+    alpha = alpha * rmax
+    if alpha > 0.:
+        m = r > rmax - alpha
+        out[m] = (0.5 * (1 - np.cos(np.pi * (r - rmax) / alpha)))[m]
+    out[r >= rmax] = 0.
+    return out
+
+
+def aperture(shape, diameter=1., smooth=0.05):
+    """(Annular) aperture window (dtmm/window.py:107-141)."""
+    try:
+        d1, d2 = diameter
+    except TypeError:
+        d1, d2 = 0, diameter
+    a = tukey(_radius(shape, d2), smooth)
+    if d1 == 0:
+        return a
+    return a * (1 - tukey(_radius(shape, d1), smooth))
+
+
+def illumination_aperture(diameter=5., smooth=0.1):
+    """(dtmm/field.py:356-369)."""
+    try:
+        inner, outer = diameter
+        inner, outer = int(round(inner)), int(round(outer))
+        if inner >= outer:
+            raise ValueError("Invalid dimater, rounded inner diameter should be lower that "
+                             "outer diameter.")
+    except TypeError:
+        inner, outer = 0, int(round(diameter))
+    return aperture((outer, outer), (inner / outer, 1.), min(max(smooth, 0), 1))
+
+
+def aperture2rays(diaphragm, betastep=0.1, norm=True):
+    """beta, phi, intensity of the lit pixels of a diaphragm image
+    (dtmm/field.py:342-354)."""
+    shape = diaphragm.shape
+    ay, ax = [np.arange(-l / 2. + .5, l / 2. + .5) for l in shape]
+    xx, yy = np.meshgrid(ax, ay, indexing="xy")
+    phi = np.arctan2(yy, xx)
+    beta = np.sqrt(xx ** 2 + yy ** 2) * betastep
+    mask = diaphragm > 0.
+    intensity = diaphragm[mask]
+    if norm:
+        intensity = intensity / intensity.sum() * len(intensity)
+    return (np.asarray(beta[mask], FDTYPE), np.asarray(phi[mask], FDTYPE),
+            np.asarray(intensity, FDTYPE))
+
+
+def illumination_rays(NA, diameter=5., smooth=0.1):
+    """(dtmm/field.py:371-404)."""
+    try:
+        low, high = diameter
+    except TypeError:
+        low, high = 0, diameter
+    a = illumination_aperture((low, high), smooth)
+    return aperture2rays(a, betastep=2. * NA / (high - 1), norm=True)
