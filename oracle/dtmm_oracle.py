@@ -12,12 +12,18 @@ Scope (what the GPU engine implements and this module checks):
 * ``transfer_4x4`` with reflection=1, diffraction >= 1, npass >= 1 (dtmm/transfer.py,
   dtmm/propagate_4x4.py); diffraction > 1 uses the tent windows of dtmm/mode.py;
   npass > 1 the normalisations of transfer.py and dtmm/denoise.py
-* ``transfer_4x4`` with reflection=2, diffraction 0 or 1, npass=1: real-space
-  sub-steps of dtmm/propagate_4x4.py _transfer_ray_4x4_2 (per-pixel f.P.f^-1
-  without diffraction; e.P/2.f^-1, k-space E diffraction, f.P/2.e^-1 with it)
-* ``transfer_2x2`` with reflection in {0, 1} and diffraction 1, or reflection 0
-  and diffraction 0, npass=1 (dtmm/transfer.py, dtmm/propagate_2x2.py);
-  reflection 1 applies the Fresnel interface transmission of dtmm/tmm.py tr_mat
+* ``transfer_4x4`` with reflection=2, any finite diffraction and npass:
+  real-space sub-steps of dtmm/propagate_4x4.py _transfer_ray_4x4_2 (per-pixel
+  f.P.f^-1 without diffraction; e.P/2.f^-1, k-space E diffraction,
+  f.P/2.e^-1 with it; the tent windows' real-space parts for diffraction > 1)
+  and the pass normalisations in real space
+* ``transfer_2x2`` with npass=1: reflection 0 or 1 with diffraction 1,
+  reflection 0 with diffraction 0, and reflection 2 (real space, the
+  interface transmission Eti_mat into every layer) with diffraction 0 or 1
+  (dtmm/transfer.py, dtmm/propagate_2x2.py); reflection 1 applies the Fresnel
+  interface transmission of dtmm/tmm.py tr_mat; ret_bulk for both methods
+* the illumination rays ``illumination_rays`` / ``aperture2rays`` /
+  ``aperture``                                             (dtmm/field.py, window.py)
 * the polarizing-microscope specter after the transfer: ``calculate_pom_field``,
   ``field2specter``, ``pom_specter`` (dtmm/field_viewer.py, field.py), the Jones
   4x4 matrices (dtmm/jones.py, jones4.py) and ``specter2color`` (dtmm/color.py)
