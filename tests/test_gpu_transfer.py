@@ -236,11 +236,12 @@ def test_2x2_reflection2(diffraction, precision):
 
 @pytest.mark.gpu
 def test_illumination_rays_multi_beam():
-    """Beams from illumination_rays (dtmm/field.py:371-404): five rays of a
-    3-pixel aperture at NA 0.1, both polarisations, through a droplet."""
+    """Beams from illumination_rays (dtmm/field.py:371-404): the nine rays of
+    a 3-pixel aperture (the corner centres lie at r = sqrt(2) / 1.5 < 1) at
+    NA 0.1, both polarisations, through a droplet."""
     flush("case illumination rays")
     beta, phi, inten = dm.illumination_rays(0.1, 3, smooth=0.)
-    assert len(beta) == 5
+    assert len(beta) == 9
     _, optical = droplet_case(shape=(6, 24, 24), radius=8)
     fd = dm.illumination_data((24, 24), [520.], pixelsize=100, beta=beta, phi=phi,
                               intensity=inten, jones=None, window=dm.blackman((24, 24)))
