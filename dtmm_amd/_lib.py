@@ -54,6 +54,7 @@ EXPORTS = (
     "dtmmx_tmm_transfer4x4",
     "dtmmx_director2data_blocks",
     "dtmmx_director2data",
+    "dtmmx_q2data",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -238,6 +239,12 @@ def load():
                                         ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.dtmmx_director2data.restype = ctypes.c_int
+    lib.dtmmx_q2data.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
+                                 ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p]
+    lib.dtmmx_q2data.restype = ctypes.c_int
     if lib.dtmmx_abi_version() != ABI_VERSION:
         raise RuntimeError("dtmmx ABI version mismatch")
     _lib = lib

@@ -2598,6 +2598,36 @@ int dtmmx_director2data(const double* director, const unsigned char* mask, long 
   return DTMMX_OK;
 }
 
+int dtmmx_q2data(const double* qtensor, const unsigned char* mask, long long layer_size,
+                 int n_layers, double no, double ne, double nhost, double scale_factor,
+                 int biaxial, double* eps_re, double* angles, double* layer_sums, void* stream) {
+  if (!qtensor || !eps_re || !angles || !layer_sums) return fail(DTMMX_E_ARG, "null argument");
+  if (layer_size <= 0 || n_layers <= 0) return fail(DTMMX_E_ARG, "empty Q tensor field");
+  if (n_layers > 65535) return fail(DTMMX_E_ARG, "too many layers");
+  const long long nb = dtmmx_director2data_blocks(layer_size);
+  if (nb > 0x7fffffffLL) return fail(DTMMX_E_ARG, "layer too large");
+  if (!std::isfinite(no) || !std::isfinite(ne) || !std::isfinite(scale_factor) ||
+      scale_factor == 0.)
+    return fail(DTMMX_E_ARG, "bad refractive indices or scale factor");
+  if (mask && !std::isfinite(nhost)) return fail(DTMMX_E_ARG, "nhost is required with a mask");
+  QArgs a{};
+  a.q = qtensor;
+  a.mask = mask;
+  a.layer_size = layer_size;
+  a.n = layer_size * n_layers;
+  a.epsa = (ne * ne - no * no) / scale_factor;
+  a.epsm = (no * no * 2. + ne * ne) / 3.;
+  a.eh = nhost * nhost;
+  a.biaxial = biaxial ? 1 : 0;
+  a.eps_re = eps_re;
+  a.angles = angles;
+  a.sums = layer_sums;
+  hipLaunchKernelGGL(q2data_kernel, dim3(unsigned(nb), unsigned(n_layers)), dim3(NT), 0,
+                     (hipStream_t)stream, a);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
 int dtmmx_tmm_transfer4x4(const dtmmx_tmm_desc* d, void* fvec_in, void* fvec_out, int has_out,
                           void* cmat, void* stream) {
   if (!d) return fail(DTMMX_E_ARG, "null descriptor");

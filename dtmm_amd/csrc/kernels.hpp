@@ -1512,6 +1512,145 @@ static __global__ __launch_bounds__(NT) void director2data_kernel(DirArgs A) {
   if (threadIdx.x == 0) A.sums[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
 }
 
+// Optical data of a Q-tensor field (dtmm/data.py:322-373 Q2data): per voxel
+// eps = (ne^2 - no^2) / scale_factor * Q + (2 no^2 + ne^2) / 3 (Q2eps,
+// data.py:432-487), its eigenvalues and the Euler angles of its eigenframe
+// (eps2epsva, data.py:489-513).  The eigenvalues are sorted as tensor_eig
+// sorts them (dtmm/linalg.py:172-222): ascending, then the most distinct one
+// is moved to index 2 by a cyclic permutation, which keeps the frame
+// right-handed.  The eigensolve is cyclic Jacobi (a different method from the
+// reference's closed form: the eigenvalues agree to rounding and each
+// eigenvector up to its sign, so the rebuilt tensor R diag(eps) R^T is the
+// same).
+struct QArgs {
+  const double* q;            // [n][6]: Q00, Q11, Q22, Q01, Q02, Q12
+  const unsigned char* mask;  // [n] or null
+  long long layer_size, n;
+  double epsa, epsm, eh;      // (ne^2 - no^2) / scale_factor, (2 no^2 + ne^2) / 3, nhost^2
+  int biaxial;                // 0: uniaxial_order(1, eig) (e0 = e1 = their mean)
+  double* eps_re;             // [3][n]
+  double* angles;             // [3][n]
+  double* sums;               // [layers][gridDim.x]: sums of (e0 + e1 + e2) / 3
+};
+
+__device__ inline void jacobi_rotate(double a[3][3], double v[3][3], int p, int q) {
+  if (a[p][q] == 0.) return;
+  const double th = (a[q][q] - a[p][p]) / (2. * a[p][q]);
+  const double t = fabs(th) > 1e150 ? 0.5 / th
+                                    : copysign(1., th) / (fabs(th) + sqrt(th * th + 1.));
+  const double c = 1. / sqrt(t * t + 1.), s = t * c;
+  const double apq = a[p][q];
+  a[p][p] -= t * apq;
+  a[q][q] += t * apq;
+  a[p][q] = a[q][p] = 0.;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    if (r != p && r != q) {
+      const double arp = a[r][p], arq = a[r][q];
+      a[r][p] = a[p][r] = c * arp - s * arq;
+      a[r][q] = a[q][r] = s * arp + c * arq;
+    }
+    const double vrp = v[r][p], vrq = v[r][q];
+    v[r][p] = c * vrp - s * vrq;
+    v[r][q] = s * vrp + c * vrq;
+  }
+}
+
+__device__ inline void swap_eig(double w[3], double v[3][3], int i, int j) {
+  const double t = w[i];
+  w[i] = w[j];
+  w[j] = t;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double u = v[r][i];
+    v[r][i] = v[r][j];
+    v[r][j] = u;
+  }
+}
+
+static __global__ __launch_bounds__(NT) void q2data_kernel(QArgs A) {
+  __shared__ double red[NT];
+  const long long x = (long long)blockIdx.x * NT + threadIdx.x;
+  const long long v = (long long)blockIdx.y * A.layer_size + x;
+  double iso = 0.;
+  if (x < A.layer_size) {
+    const double* q = A.q + 6 * v;
+    double a[3][3], R[3][3] = {{1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.}};
+    a[0][0] = A.epsa * q[0] + A.epsm;
+    a[1][1] = A.epsa * q[1] + A.epsm;
+    a[2][2] = A.epsa * q[2] + A.epsm;
+    a[0][1] = a[1][0] = A.epsa * q[3];
+    a[0][2] = a[2][0] = A.epsa * q[4];
+    a[1][2] = a[2][1] = A.epsa * q[5];
+    for (int sweep = 0; sweep < 16; ++sweep) {
+      const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+      const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+      if (!(off > 1e-36 * dia)) break;
+      jacobi_rotate(a, R, 0, 1);
+      jacobi_rotate(a, R, 0, 2);
+      jacobi_rotate(a, R, 1, 2);
+    }
+    double w[3] = {a[0][0], a[1][1], a[2][2]};
+    if (w[1] < w[0]) swap_eig(w, R, 0, 1);
+    if (w[2] < w[1]) swap_eig(w, R, 1, 2);
+    if (w[1] < w[0]) swap_eig(w, R, 0, 1);
+    // right-handed frame: column 2 = column 0 x column 1
+    R[0][2] = R[1][0] * R[2][1] - R[2][0] * R[1][1];
+    R[1][2] = R[2][0] * R[0][1] - R[0][0] * R[2][1];
+    R[2][2] = R[0][0] * R[1][1] - R[1][0] * R[0][1];
+    // the most distinct eigenvalue to index 2 (dtmm/linalg.py:179-193)
+    const double m2 = fabs(w[1] - w[0]), m1 = fabs(w[2] - w[0]), m0 = fabs(w[2] - w[1]);
+    int i0 = 0, i1 = 1, i2 = 2;
+    if (m2 > m1 || m2 > m0) {
+      if (m1 > m0) {
+        i0 = 1; i1 = 2; i2 = 0;
+      } else {
+        i0 = 2; i1 = 0; i2 = 1;
+      }
+    }
+    double e0 = w[i0], e1 = w[i1], e2 = w[i2];
+    double S[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      S[r][0] = R[r][i0];
+      S[r][1] = R[r][i1];
+      S[r][2] = R[r][i2];
+    }
+    // Euler angles (dtmm/rotation.py:154-171)
+    const double r22 = fmax(-1., fmin(1., S[2][2]));
+    const double theta = acos(r22);
+    double psi, phi;
+    if ((S[1][2] == 0. && S[0][2] == 0.) || theta == 0.) {
+      psi = 0.;
+      phi = acos(fmax(-1., fmin(1., S[1][1])));
+    } else {
+      phi = atan2(S[1][2], S[0][2]);
+      psi = atan2(S[2][1], -S[2][0]);
+    }
+    if (!A.biaxial) {  // uniaxial_order(1, eig) (dtmm/data.py:1215-1233)
+      const double m = (e0 + e1 + e2) / 3.;
+      const double delta = e2 - (e0 + e1) / 2.;
+      e0 = e1 = m - 1. / 3. * delta;
+      e2 = m + 2. / 3. * delta;
+    }
+    if (A.mask && !A.mask[v]) e0 = e1 = e2 = A.eh;
+    A.eps_re[v] = e0;
+    A.eps_re[A.n + v] = e1;
+    A.eps_re[2 * A.n + v] = e2;
+    A.angles[v] = psi;
+    A.angles[A.n + v] = theta;
+    A.angles[2 * A.n + v] = phi;
+    iso = (e0 + e1 + e2) / 3.;
+  }
+  red[threadIdx.x] = iso;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) A.sums[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
 // ---------------------------------------------------------------------------
 // Non-iterative 1D transfer matrix method (dtmm/tmm.py:1112-1673, method
 // "4x4"): one thread per independent stack, fp64 throughout.

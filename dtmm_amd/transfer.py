@@ -159,16 +159,50 @@ class Stack:
         the block's effective medium (the mean isotropic eps, per layer with
         ``split_layers``) without a host copy of the material.  Pass the
         result as ``optical_data`` to ``transfer_field``."""
+        lib = _lib.load()
+
+        def launch(src, mk, H, W, L, eps_re, angles, sums, stream):
+            return lib.dtmmx_director2data(
+                src, mk, H * W, L, float(no), float(ne),
+                float(nhost) if nhost is not None else float("nan"), float(scale_factor),
+                eps_re, angles, sums, stream)
+        return cls._from_device_field(director, 3, "director", mask, nhost, launch, False,
+                                      thickness, nstep, nin, nout, split_layers)
+
+    @classmethod
+    def from_Q(cls, qtensor, mask=None, no=1.5, ne=1.6, nhost=None, scale_factor=1.,
+               biaxial=False, thickness=None, nstep=1, nin=N_COVER, nout=N_COVER,
+               split_layers=False):
+        """The stack of ``Q2data(qtensor, mask, no, ne, nhost, scale_factor,
+        biaxial, thickness)`` (dtmm/data.py:322-373) built on the GPU
+        (``dtmmx_q2data``): the Q tensor (L, H, W, 6) with elements Q00, Q11,
+        Q22, Q01, Q02, Q12 -- a device tensor stays on the device -- becomes
+        the device material arrays (eigenvalues of eps, the most distinct
+        last) and the Euler angles of the eigenframe, plus the effective
+        medium as in ``from_director``."""
+        lib = _lib.load()
+
+        def launch(src, mk, H, W, L, eps_re, angles, sums, stream):
+            return lib.dtmmx_q2data(
+                src, mk, H * W, L, float(no), float(ne),
+                float(nhost) if nhost is not None else float("nan"), float(scale_factor),
+                1 if biaxial else 0, eps_re, angles, sums, stream)
+        return cls._from_device_field(qtensor, 6, "qtensor", mask, nhost, launch, bool(biaxial),
+                                      thickness, nstep, nin, nout, split_layers)
+
+    @classmethod
+    def _from_device_field(cls, field, ncomp, name, mask, nhost, launch, biaxial, thickness,
+                           nstep, nin, nout, split_layers):
         torch = _torch()
-        if isinstance(director, torch.Tensor):
-            if not director.is_cuda:
+        if isinstance(field, torch.Tensor):
+            if not field.is_cuda:
                 raise ValueError("tensor inputs must be device tensors")
-            dr = director.to(torch.float64).contiguous()
+            dr = field.to(torch.float64).contiguous()
         else:
-            dr = torch.from_numpy(np.ascontiguousarray(np.asarray(director, np.float64)))
+            dr = torch.from_numpy(np.ascontiguousarray(np.asarray(field, np.float64)))
             dr = dr.to("cuda")
-        if dr.dim() != 4 or dr.shape[-1] != 3:
-            raise ValueError("director must have shape (layers, height, width, 3)")
+        if dr.dim() != 4 or dr.shape[-1] != ncomp:
+            raise ValueError("%s must have shape (layers, height, width, %d)" % (name, ncomp))
         L, H, W = (int(x) for x in dr.shape[:3])
         mk = None
         if mask is not None:
@@ -186,12 +220,11 @@ class Stack:
         angles = torch.empty((3, L, H, W), dtype=torch.float64, device="cuda")
         sums = torch.empty((L, nb), dtype=torch.float64, device="cuda")
         stream = torch.cuda.current_stream().cuda_stream
-        _lib.check(lib.dtmmx_director2data(
+        _lib.check(launch(
             ctypes.c_void_p(dr.data_ptr()), ctypes.c_void_p(mk.data_ptr()) if mk is not None
-            else None, H * W, L, float(no), float(ne),
-            float(nhost) if nhost is not None else float("nan"), float(scale_factor),
-            ctypes.c_void_p(eps_re.data_ptr()), ctypes.c_void_p(angles.data_ptr()),
-            ctypes.c_void_p(sums.data_ptr()), ctypes.c_void_p(stream)))
+            else None, H, W, L, ctypes.c_void_p(eps_re.data_ptr()),
+            ctypes.c_void_p(angles.data_ptr()), ctypes.c_void_p(sums.data_ptr()),
+            ctypes.c_void_p(stream)))
         layer_sum = sums.sum(dim=1).cpu().numpy()
         if split_layers:
             eff = layer_sum / (H * W)
@@ -206,7 +239,8 @@ class Stack:
         if np.any(steps < 1):
             raise ValueError("nstep must be >= 1")
         self.n_steps = np.asarray(steps, np.int32)
-        self.biaxial = np.zeros(L, np.int32)  # director data is uniaxial (eps0 == eps1)
+        # director data and uniaxial Q data have eps0 == eps1 in every voxel
+        self.biaxial = np.full(L, 1 if biaxial else 0, np.int32)
         self.eps_eff_re = np.asarray(eff, np.float64)
         self.eps_eff_im = np.zeros(L, np.float64)
         self.nin = float(nin)

@@ -303,6 +303,19 @@ int dtmmx_director2data(const double* director, const unsigned char* mask, long 
                         int n_layers, double no, double ne, double nhost, double scale_factor,
                         double* eps_re, double* angles, double* layer_sums, void* stream);
 
+/* Optical data of a Q-tensor field on the device: dtmm/data.py:322-373
+ * Q2data with Q2eps (data.py:432-487), eps2epsva (data.py:489-513) and the
+ * eigenvalue order of tensor_eig (dtmm/linalg.py:172-222, the most distinct
+ * eigenvalue at index 2); biaxial = 0 applies uniaxial_order(1, ...).
+ * qtensor: device [n_layers * layer_size][6] double (Q00, Q11, Q22, Q01,
+ * Q02, Q12, the reference's (..., 6) layout); the other arguments as in
+ * dtmmx_director2data (layer_sums has dtmmx_director2data_blocks(layer_size)
+ * entries per layer).  The angles describe the same eigenframe as the
+ * reference's up to the sign of each eigenvector.  Enqueued on `stream`. */
+int dtmmx_q2data(const double* qtensor, const unsigned char* mask, long long layer_size,
+                 int n_layers, double no, double ne, double nhost, double scale_factor,
+                 int biaxial, double* eps_re, double* angles, double* layer_sums, void* stream);
+
 /* Non-iterative 1D transfer matrix method, method "4x4" of
  * dtmm/tmm.py:1565-1673 transfer4x4 for n_elements independent stacks of
  * n_layers homogeneous layers each (every stack has its own kd, eps and

@@ -46,6 +46,7 @@ __all__ = [
     "inv", "dotmm", "dotmf", "dotmdm", "dotmdmf", "dotmv", "phase_mat", "E_mat", "E2H_mat",
     "normalize_f", "field2intensity", "fft2", "ifft2", "refind2eps",
     "uniaxial_order", "validate_optical_data", "create_layers",
+    "director2Q", "Q2eps", "tensor_eig", "rotation_angles", "Q2data", "epsva2eps",
     "projection_matrix", "field_diffraction_matrix", "diffract",
     "transmitted_field", "first_corrected_Epn_diffraction_matrix",
     "second_corrected_Epn_diffraction_matrix", "first_E_diffraction_matrix",
@@ -680,6 +681,85 @@ def uniaxial_order(order, eps):
     e2 = np.where(o >= 0, e1, eps[..., 1])
     e3 = np.where(o >= 0, e3, eps[..., 2])
     return np.stack([e1, e2, e3], -1)
+
+
+def director2Q(director, order=1.):
+    """Q tensor (..., 6) of a director field (dtmm/data.py:376-407); the
+    order S is the square root of the director length (director2order,
+    data.py:963-973)."""
+    n = np.asarray(director, FDTYPE)
+    S = np.sqrt(np.sqrt(np.sum(n * n, axis=-1)))
+    d = S / 3 * order
+    return np.stack([order * n[..., 0] ** 2 - d, order * n[..., 1] ** 2 - d,
+                     order * n[..., 2] ** 2 - d, order * n[..., 0] * n[..., 1],
+                     order * n[..., 0] * n[..., 2], order * n[..., 1] * n[..., 2]], -1)
+
+
+def Q2eps(q, no=1.5, ne=1.6, scale_factor=1.):
+    """eps (..., 6) of a Q tensor (..., 6) (dtmm/data.py:432-487)."""
+    q = np.asarray(q, FDTYPE)
+    epsa = (ne ** 2 - no ** 2) / scale_factor
+    epsm = (no ** 2 * 2 + ne ** 2) / 3.
+    out = np.empty_like(q)
+    out[..., 0:3] = epsa * q[..., 0:3] + epsm
+    out[..., 3:] = epsa * q[..., 3:]
+    return out
+
+
+def tensor_eig(t6):
+    """Eigenvalues and eigenvectors (columns) of a symmetric tensor (..., 6),
+    ordered as dtmm/linalg.py:172-222 orders them: ascending with a
+    right-handed frame, then the most distinct eigenvalue moved to index 2 by
+    a cyclic permutation.  numpy.linalg.eigh replaces the reference's closed
+    form (linalg.py:225-315); eigenvectors agree up to sign."""
+    t6 = np.asarray(t6, FDTYPE)
+    m = np.empty(t6.shape[:-1] + (3, 3))
+    for k, (i, j) in enumerate(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
+        m[..., i, j] = m[..., j, i] = t6[..., k]
+    w, v = np.linalg.eigh(m)
+    v[..., :, 2] = np.cross(v[..., :, 0], v[..., :, 1])
+    m2 = np.abs(w[..., 1] - w[..., 0])
+    m1 = np.abs(w[..., 2] - w[..., 0])
+    m0 = np.abs(w[..., 2] - w[..., 1])
+    move = (m2 > m1) | (m2 > m0)
+    perm = np.where((move & (m1 > m0))[..., None], [1, 2, 0],
+                    np.where(move[..., None], [2, 0, 1], [0, 1, 2]))
+    w = np.take_along_axis(w, perm, -1)
+    v = np.take_along_axis(v, perm[..., None, :], -1)
+    return w, v
+
+
+def rotation_angles(R):
+    """Euler angles (psi, theta, phi) of R = R_phi R_theta R_psi
+    (dtmm/rotation.py:154-171)."""
+    R = np.asarray(R, FDTYPE)
+    theta = np.arccos(np.clip(R[..., 2, 2], -1, 1))
+    flat = ((R[..., 1, 2] == 0.) & (R[..., 0, 2] == 0.)) | (theta == 0.)
+    phi = np.where(flat, np.arccos(np.clip(R[..., 1, 1], -1, 1)),
+                   np.arctan2(R[..., 1, 2], R[..., 0, 2]))
+    psi = np.where(flat, 0., np.arctan2(R[..., 2, 1], -R[..., 2, 0]))
+    return np.stack([psi, theta, phi], -1)
+
+
+def Q2data(q, mask=None, no=1.5, ne=1.6, nhost=None, scale_factor=1., biaxial=False):
+    """(thickness, epsv, epsa) of a Q-tensor field (L, H, W, 6)
+    (dtmm/data.py:322-373 with eps2epsva, data.py:489-513)."""
+    q = np.asarray(q, FDTYPE)
+    w, v = tensor_eig(Q2eps(q, no, ne, scale_factor))
+    ang = rotation_angles(v)
+    if not biaxial:
+        w = uniaxial_order(1., w)
+    if mask is not None:
+        w[np.logical_not(mask), :] = nhost ** 2
+    return np.ones(q.shape[0]), w, ang
+
+
+def epsva2eps(epsv, epsa):
+    """(..., 6) tensor R diag(epsv) R^T of eigenvalues and Euler angles
+    (dtmm/data.py:515-534)."""
+    epsa = np.asarray(epsa, FDTYPE)
+    R = _rotation_matrix(epsa[..., 0], epsa[..., 1], epsa[..., 2])
+    return _rotate_diagonal_tensor(R, np.asarray(epsv))
 
 
 def validate_optical_block(block):
