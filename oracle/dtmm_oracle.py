@@ -1150,10 +1150,13 @@ def propagate_4x4_effective_1(field, ks, layer, effective_layer, beta=0, phi=0, 
     nw = len(windows)
     betas = betas.reshape((nw,) + bshape)
     phis = phis.reshape((nw,) + bshape)
-    dmats1 = first_corrected_Epn_diffraction_matrix(shape, ks, betas, phis, d_eff / 2,
-                                                    epsv_eff, epsa_eff, betamax)
-    dmats2 = second_corrected_Epn_diffraction_matrix(shape, ks, betas, phis, d_eff / 2,
-                                                     epsv_eff, epsa_eff, betamax)
+    # the window matrices are pure functions of the effective layer and the
+    # window beams; the reference caches them (dtmm/conf.py:224-330)
+    key = _key(shape, ks, betas, phis, d_eff, epsv_eff, epsa_eff, betamax)
+    dmats1 = cache.get(("w1",) + key, lambda: first_corrected_Epn_diffraction_matrix(
+        shape, ks, betas, phis, d_eff / 2, epsv_eff, epsa_eff, betamax))
+    dmats2 = cache.get(("w2",) + key, lambda: second_corrected_Epn_diffraction_matrix(
+        shape, ks, betas, phis, d_eff / 2, epsv_eff, epsa_eff, betamax))
     fout = np.zeros_like(field)
     for window, b, p, dm1, dm2 in zip(windows, betas, phis, dmats1, dmats2):
         fout += _transfer_ray_4x4_1(field * window, ks, layer, dm1, dm2, b, p, nsteps)

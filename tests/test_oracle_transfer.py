@@ -564,3 +564,97 @@ def test_2x2_reflection2_oblique_interfaces_are_fresnel(pol):
         comp, e_in = 2, f0[2]
     ph = np.exp(1j * k * (a[1] * d[0] + a[2] * d[1]))
     np.testing.assert_allclose(out[0, comp], t * ph * e_in, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Inhomogeneous, layer-varying stacks.  The pins above all use identical
+# homogeneous layers; these two fix what they leave open: the layer order,
+# the (H, W) indexing of the material against the field, and the domain
+# (real space vs k space) the per-pixel operator acts in.
+# ---------------------------------------------------------------------------
+
+def _random_stack(rng, L, shape, biaxial=False):
+    H, W = shape
+    e0 = rng.uniform(2.2, 2.4, (L, H, W))
+    e2 = rng.uniform(2.5, 2.9, (L, H, W))
+    e1 = rng.uniform(2.2, 2.4, (L, H, W)) if biaxial else e0
+    eps = np.stack([e0, e1, e2], -1)
+    ang = np.stack([rng.uniform(0, 1, (L, H, W)) if biaxial else np.zeros((L, H, W)),
+                    rng.uniform(0, np.pi, (L, H, W)), rng.uniform(-np.pi, np.pi, (L, H, W))], -1)
+    return eps, ang
+
+
+def _roll_hw(a, dy, dx, axes=(-2, -1)):
+    return np.roll(np.roll(a, dy, axis=axes[0]), dx, axis=axes[1])
+
+
+@pytest.mark.parametrize("method,diffraction,npass", [
+    ("4x4", 1, 1), ("4x4", 3, 1), ("4x4", 1, 3), ("4x4", 3, 3),
+    ("2x2", 1, 1), ("2x2", 0, 1), ("4x4", 0, 1)])
+def test_transfer_is_translation_equivariant(method, diffraction, npass):
+    """Every operator of the path is either per pixel (the layer operator,
+    acting on the material at the same pixel) or per k (diffraction, tent
+    windows, pass normalisations, all functions of the spatial frequency
+    only) on a periodic grid, so rolling the material and the input by
+    (dy, dx) rolls the output by the same.  An oracle that indexed the
+    material transposed (H <-> W) against the field, or applied the per-pixel
+    operator in k space, fails this on a non-square grid with dy != dx."""
+    rng = np.random.default_rng(11)
+    L, shape = 4, (12, 20)
+    wl = np.array([480., 640.])
+    eps, ang = _random_stack(rng, L, shape)
+    f = (rng.standard_normal((2, len(wl), 4) + shape) +
+         1j * rng.standard_normal((2, len(wl), 4) + shape))
+    dy, dx = 5, 3
+    kw = dict(beta=0.1, phi=0.4, method=method, diffraction=diffraction, npass=npass,
+              nin=1.5, nout=1.6)
+    out = o.transfer_field((f.copy(), wl, 100.), [(np.ones(L), eps, ang)], **kw)[0]
+    out_r = o.transfer_field((_roll_hw(f, dy, dx), wl, 100.),
+                             [(np.ones(L), _roll_hw(eps, dy, dx, (1, 2)),
+                               _roll_hw(ang, dy, dx, (1, 2)))], **kw)[0]
+    np.testing.assert_allclose(out_r, _roll_hw(out, dy, dx), atol=1e-12 * np.abs(out).max())
+    # the material is not laterally uniform, so an unrolled material gives
+    # another result (the check is not vacuous)
+    out_m = o.transfer_field((_roll_hw(f, dy, dx), wl, 100.), [(np.ones(L), eps, ang)], **kw)[0]
+    assert np.abs(out_m - _roll_hw(out, dy, dx)).max() > 1e-3 * np.abs(out).max()
+
+
+@pytest.mark.parametrize("method", ["4x4", "2x2"])
+@pytest.mark.parametrize("nstep", [1, 2])
+def test_twisted_stack_is_ordered_jones_product(method, nstep):
+    """A laterally uniform twisted stack of in-plane uniaxial layers (theta
+    = pi/2, phi_j from 0 to pi/2 across the layers, the 90 degree twisted
+    nematic of examples/1D/tmm_twisted_nematic_reference.py) at normal
+    incidence, entered from the effective medium: each layer is a Jones
+    retarder R(phi_j) diag(exp(i k ne d), exp(i k no d)) R(phi_j)^T, and the
+    output E is their ordered product (layer 1 first) applied to the input.
+    The retarders do not commute, so a reversed layer loop fails this."""
+    no, ne, L = 1.5, 1.7, 9
+    shape = (6, 8)
+    wl = np.array([450., 550., 650.])
+    ps = 100.
+    d = np.linspace(0.7, 1.3, L)
+    phis = np.linspace(0, np.pi / 2, L)
+    neff = np.sqrt((2 * no ** 2 + ne ** 2) / 3)
+    eps = np.broadcast_to([no ** 2, no ** 2, ne ** 2], (L, 1, 1, 3))
+    ang = np.zeros((L, 1, 1, 3))
+    ang[:, 0, 0, 1] = np.pi / 2
+    ang[:, 0, 0, 2] = phis
+    pol = (np.cos(0.2), np.sin(0.2))
+    field = _uniform_wave(shape, wl, neff, pol)
+    out = o.transfer_field((field, wl, ps), [(d, eps, ang)], beta=0., phi=0., method=method,
+                           nin=neff, nout=neff, nstep=nstep)[0]
+    k = o.k0(wl, ps)
+    for i in range(len(wl)):
+        J = np.eye(2, dtype=complex)
+        for j in range(L):
+            c, s = np.cos(phis[j]), np.sin(phis[j])
+            R = np.array([[c, -s], [s, c]])
+            J = R @ np.diag([np.exp(1j * k[i] * d[j] * ne), np.exp(1j * k[i] * d[j] * no)]) \
+                @ R.T @ J
+        e = J @ np.array(pol)
+        np.testing.assert_allclose(out[i, 0], e[0], atol=1e-12)
+        np.testing.assert_allclose(out[i, 2], e[1], atol=1e-12)
+        # the output medium's forward wave: H = n E
+        np.testing.assert_allclose(out[i, 1], neff * e[0], atol=1e-12)
+        np.testing.assert_allclose(out[i, 3], -neff * e[1], atol=1e-12)
