@@ -102,7 +102,7 @@ class AdjointTransfer:
         torch = self.torch
         self._record(torch.cuda.current_stream())
         torch.cuda.synchronize()
-        _check_status(self.status)
+        _check_status(self.status, adjoint=True)
         self._recorded = True
         return self.job.fout
 
@@ -122,7 +122,7 @@ class AdjointTransfer:
             grad = torch.zeros((2, self.n_layers, H, W), dtype=torch.float64, device="cuda")
         self._adjoint(seed, grad, torch.cuda.current_stream())
         torch.cuda.synchronize()
-        _check_status(self.status)
+        _check_status(self.status, adjoint=True)
         return grad
 
     def gradient_graph(self, loss_fn):
@@ -142,7 +142,7 @@ class AdjointTransfer:
             self._adjoint(seed, grad, side)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
-        _check_status(self.status)
+        _check_status(self.status, adjoint=True)
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             stream = torch.cuda.current_stream()
@@ -189,7 +189,7 @@ def _director_transfer_class():
             H, W = j.plan.shape
             grad = torch.zeros((2, tr.n_layers, H, W), dtype=torch.float64, device="cuda")
             tr._adjoint(seed, grad, torch.cuda.current_stream())
-            _check_status(tr.status)
+            _check_status(tr.status, adjoint=True)
             return grad[0].to(ctx.dtypes[0]), grad[1].to(ctx.dtypes[1]), None
 
     return DirectorTransfer

@@ -1582,18 +1582,35 @@ static __global__ __launch_bounds__(NT) void q2data_kernel(QArgs A) {
     a[0][1] = a[1][0] = A.epsa * q[3];
     a[0][2] = a[2][0] = A.epsa * q[4];
     a[1][2] = a[2][1] = A.epsa * q[5];
-    for (int sweep = 0; sweep < 16; ++sweep) {
-      const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
-      const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
-      if (!(off > 1e-36 * dia)) break;
-      jacobi_rotate(a, R, 0, 1);
-      jacobi_rotate(a, R, 0, 2);
-      jacobi_rotate(a, R, 1, 2);
+    // A tensor whose off-diagonal part, relative to its largest element, has
+    // a squared norm of at most 1e-12 counts as diagonal: its eigenvalues
+    // keep the input order with the identity frame (dtmm/linalg.py:251-318).
+    const double scale = fmax(fmax(fmax(fabs(a[0][0]), fabs(a[1][1])), fmax(fabs(a[2][2]),
+                              fabs(a[0][1]))), fmax(fabs(a[0][2]), fabs(a[1][2])));
+    const double sm = scale > 0. ? 1. / scale : 0.;
+    const double s01 = a[0][1] * sm, s02 = a[0][2] * sm, s12 = a[1][2] * sm;
+    const bool diagonal = !(s01 * s01 + s02 * s02 + s12 * s12 > 1e-12);
+    double w[3];
+    if (diagonal) {
+      w[0] = a[0][0] * sm * scale;
+      w[1] = a[1][1] * sm * scale;
+      w[2] = a[2][2] * sm * scale;
+    } else {
+      for (int sweep = 0; sweep < 16; ++sweep) {
+        const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+        if (!(off > 1e-36 * dia)) break;
+        jacobi_rotate(a, R, 0, 1);
+        jacobi_rotate(a, R, 0, 2);
+        jacobi_rotate(a, R, 1, 2);
+      }
+      w[0] = a[0][0];
+      w[1] = a[1][1];
+      w[2] = a[2][2];
+      if (w[1] < w[0]) swap_eig(w, R, 0, 1);
+      if (w[2] < w[1]) swap_eig(w, R, 1, 2);
+      if (w[1] < w[0]) swap_eig(w, R, 0, 1);
     }
-    double w[3] = {a[0][0], a[1][1], a[2][2]};
-    if (w[1] < w[0]) swap_eig(w, R, 0, 1);
-    if (w[2] < w[1]) swap_eig(w, R, 1, 2);
-    if (w[1] < w[0]) swap_eig(w, R, 0, 1);
     // right-handed frame: column 2 = column 0 x column 1
     R[0][2] = R[1][0] * R[2][1] - R[2][0] * R[1][1];
     R[1][2] = R[2][0] * R[0][1] - R[0][0] * R[2][1];

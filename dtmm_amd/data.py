@@ -1,13 +1,18 @@
 """Optical-data model: blocks of (thickness, eps eigenvalues, Euler angles).
 
-Host-side NumPy input builders with the reference's names and argument
-meanings (dtmm/data.py).  They build inputs; the propagation itself runs
-only on the GPU (``dtmm_amd.transfer``).
+Host-side NumPy input builders with the reference's names, arguments and
+results (dtmm/data.py): director and droplet samples, the order-parameter
+scaling of eps, block validation and the dispersive (wavelength-dependent)
+materials.  They build inputs; the propagation itself runs only on the GPU
+(``dtmm_amd.transfer``).
 """
 import numpy as np
 
 FDTYPE = np.float64
 CDTYPE = np.complex128
+
+# unit director of the fixed droplet profiles (dtmm/data.py:803-856)
+_PROFILE_AXES = {"x": (1., 0., 0.), "y": (0., 1., 0.), "z": (0., 0., 1.)}
 
 
 def refind2eps(refind):
@@ -50,51 +55,53 @@ def director2angles(director):
 
 def director2data(director, mask=None, no=1.5, ne=1.6, nhost=None, scale_factor=1.,
                   thickness=None):
-    """Optical block from a director field (dtmm/data.py:277-320)."""
-    director = np.asarray(director, FDTYPE)
-    material = np.empty(director.shape, FDTYPE)
-    material[...] = refind2eps([no, no, ne])[None, ...]
-    material = uniaxial_order(director2order(director) / scale_factor, material)
+    """Optical block (thickness, eps, angles) of a director field
+    (dtmm/data.py:277-320).
+
+    The length of the director sets the nematic order S = sqrt(|n|) /
+    scale_factor, which scales the anisotropy of the (no^2, no^2, ne^2)
+    tensor about its mean (``uniaxial_order``); voxels where `mask` is False
+    hold the isotropic host nhost^2.  Layer thicknesses default to 1.
+    """
+    n = np.asarray(director, FDTYPE)
+    pure = np.broadcast_to(refind2eps(np.asarray([no, no, ne], FDTYPE)), n.shape)
+    eps = uniaxial_order(director2order(n) / scale_factor, pure)
     if mask is not None:
-        material[np.logical_not(mask), :] = refind2eps([nhost, nhost, nhost])[None, ...]
-    if thickness is None:
-        thickness = np.ones(shape=(material.shape[0],))
-    return thickness, material, director2angles(director)
+        host = refind2eps(np.full(3, nhost, FDTYPE))
+        eps = np.where(np.asarray(mask, bool)[..., None], eps, host)
+    d = np.ones(n.shape[0]) if thickness is None else thickness
+    return d, eps, director2angles(n)
 
 
-def _r3(shape):
-    """Half-pixel-centred coordinates (dtmm/data.py:773-777)."""
-    az, ay, ax = [np.arange(-l / 2. + .5, l / 2. + .5) for l in shape]
-    zz, yy, xx = np.meshgrid(az, ay, ax, indexing="ij")
-    return xx, yy, zz
+def _voxel_centres(shape):
+    """Voxel-centre coordinates (z, y, x) of a box, each relative to the box
+    centre and shaped to broadcast against the box; an even edge has no
+    voxel at 0 (dtmm/data.py:773-777)."""
+    return np.ix_(*[np.arange(n, dtype=FDTYPE) - (n - 1) / 2. for n in shape])
 
 
 def nematic_droplet_director(shape, radius, profile="r", retmask=False):
-    """Director of a nematic droplet (dtmm/data.py:803-856)."""
-    nz, ny, nx = shape
-    out = np.zeros(shape=(nz, ny, nx, 3), dtype=FDTYPE)
-    xx, yy, zz = _r3(shape)
-    r = (xx ** 2 + yy ** 2 + zz ** 2) ** 0.5
-    mask = (r <= radius)
-    m = np.logical_and(mask, r != 0.)
-    rm = r[m]
-    if profile == "r":
-        out[..., 0][m] = xx[m] / rm
-        out[..., 1][m] = yy[m] / rm
-        out[..., 2][m] = zz[m] / rm
-    elif isinstance(profile, str):
-        index = {"x": 0, "y": 1, "z": 2}
-        if profile not in index:
+    """Director of a nematic droplet of `radius` centred in the box
+    (dtmm/data.py:803-856).  profile "r" points along the radius vector; "x",
+    "y", "z" or a 3-vector is a uniform director.  Voxels outside the droplet
+    (and its centre, for "r") hold a zero director.  retmask: also the
+    droplet mask r <= radius."""
+    z, y, x = _voxel_centres(shape)
+    r = np.sqrt(x ** 2 + y ** 2 + z ** 2)
+    inside = r <= radius
+    filled = inside & (r != 0.)
+    if isinstance(profile, str):
+        if profile == "r":
+            with np.errstate(invalid="ignore", divide="ignore"):
+                vec = np.stack(np.broadcast_arrays(x / r, y / r, z / r), -1)
+        elif profile in _PROFILE_AXES:
+            vec = np.asarray(_PROFILE_AXES[profile], FDTYPE)
+        else:
             raise ValueError("Unsupported profile type!")
-        out[..., index[profile]][m] = 1.
     else:
-        x, y, z = profile
-        out[..., 0][m] = x
-        out[..., 1][m] = y
-        out[..., 2][m] = z
-    if retmask:
-        return mask, out
-    return out
+        vec = np.asarray(profile, FDTYPE).reshape(3)
+    director = np.where(filled[..., None], vec, FDTYPE(0.))
+    return (inside, director) if retmask else director
 
 
 def nematic_droplet_data(shape, radius, profile="r", no=1.5, ne=1.6, nhost=1.5):
@@ -103,46 +110,158 @@ def nematic_droplet_data(shape, radius, profile="r", no=1.5, ne=1.6, nhost=1.5):
     return director2data(director, mask=mask, no=no, ne=ne, nhost=nhost)
 
 
-def validate_optical_block(data, shape=None):
-    """Cast a block and add singleton axes so it broadcasts to (L, H, W, 3)
-    (dtmm/data.py:534-618).  Callable (dispersive) materials are not
-    supported by the engine and raise."""
-    thickness, material, angles = data
-    if callable(material):
-        raise NotImplementedError("dispersive (callable) materials are not supported")
-    thickness = np.asarray(thickness, dtype=FDTYPE)
-    material = np.asarray(material)
-    if np.issubdtype(material.dtype, np.complexfloating):
-        material = np.asarray(material, dtype=CDTYPE)
-    else:
-        material = np.asarray(material, dtype=FDTYPE)
-    angles = np.asarray(angles, dtype=FDTYPE)
-    if thickness.ndim == 0:
-        thickness = thickness[None]
-        material = material[None, ...]
-        angles = angles[None, ...]
-    elif thickness.ndim != 1:
-        raise ValueError("Thickess dimension should be 1.")
-    n = len(thickness)
-    if material.ndim == 2:
-        material = material[:, None, None, :]
-    elif material.ndim == 3:
-        material = material[:, None, :, :]
-    elif material.ndim != 4:
-        raise ValueError("Invalid material dimensions.")
-    bshape = (n, 1, 1, 3) if shape is None else (n, shape[0], shape[1], 3)
-    np.broadcast_shapes(material.shape, bshape)
-    np.broadcast_shapes(angles.shape, bshape)
-    return thickness, material, angles
+# ---------------------------------------------------------------------------
+# dispersive materials (dtmm/data.py:1094-1204)
+# ---------------------------------------------------------------------------
+
+class EpsilonDispersive:
+    """A wavelength-dependent eps: per-voxel coefficients of shape
+    (..., 3, n), one row of n coefficients per eigenvalue.  Calling the
+    object with a wavelength in nanometres returns the eps eigenvalues
+    (..., 3) there.  Built from `values`, or as `shape` + (3, n) coefficients
+    of a non-dispersive eps = 1 (first coefficient 1, the others 0)."""
+
+    def __init__(self, values=None, shape=None, n=None, broadcast=False, copy=False):
+        n = 1 if n is None else int(n)
+        if n < 0:
+            raise ValueError("n must be >= 0")
+        full = (tuple(shape) if shape is not None else ()) + (3, n)
+        if values is None:
+            if shape is None:
+                raise ValueError("Coefficients were not set, so you need to provide n and "
+                                 "shape parameters.")
+            c = np.zeros(full, FDTYPE)
+            c[..., 0] = 1.
+        else:
+            c = np.asarray(values, FDTYPE)
+            # at least one row per eigenvalue
+            c = np.broadcast_to(c, np.broadcast_shapes(c.shape, (3, c.shape[-1] if c.ndim else n)))
+            if broadcast:
+                c = np.broadcast_to(c, np.broadcast_shapes(c.shape, full))
+            if copy:
+                c = c.copy()
+        self.coefficients = c
+
+    def __getitem__(self, index):
+        return type(self)(self.coefficients[index])
+
+    def __call__(self, wavelength):
+        raise NotImplementedError("a dispersive material defines __call__(wavelength)")
 
 
-def validate_optical_data(data, shape=None):
-    """List of validated blocks (dtmm/data.py:620-643)."""
+class EpsilonCauchy(EpsilonDispersive):
+    """Cauchy dispersion n = c0 + c1 / w^2 + c2 / w^4 + ..., eps = n^2, with
+    w the wavelength in microns (dtmm/data.py:1110-1129, :1183-1193)."""
+
+    def __call__(self, wavelength):
+        c = self.coefficients
+        inv_w2 = 1. / (np.asarray(wavelength, FDTYPE) / 1000.) ** 2
+        n = c[..., 0]
+        for i in range(1, c.shape[-1]):
+            n = n + c[..., i] * inv_w2 ** i
+        return n ** 2
+
+
+class EpsilonSellmeier(EpsilonDispersive):
+    """Sellmeier dispersion eps = c0 + w^2 c1 / (w^2 - c2) + w^2 c3 / (w^2 - c4)
+    + ..., w in microns, an odd number of coefficients
+    (dtmm/data.py:1072-1093, :1195-1204)."""
+
+    def __call__(self, wavelength):
+        c = self.coefficients
+        w2 = (np.asarray(wavelength, FDTYPE) / 1000.) ** 2
+        eps = c[..., 0]
+        for i in range((c.shape[-1] - 1) // 2):
+            eps = eps + w2 * c[..., 1 + 2 * i] / (w2 - c[..., 2 + 2 * i])
+        return eps
+
+
+def is_optical_block_dispersive(block):
+    """Whether the block's eps is a callable of the wavelength
+    (dtmm/data.py:1131-1134)."""
+    return callable(block[1])
+
+
+def is_optical_data_dispersive(optical_data):
+    """Whether any block of the optical data is dispersive
+    (dtmm/data.py:1136-1142)."""
+    return any(is_optical_block_dispersive(b) for b in optical_data)
+
+
+# ---------------------------------------------------------------------------
+# validation (dtmm/data.py:534-682)
+# ---------------------------------------------------------------------------
+
+def _with_lateral_axes(a, rank, what):
+    """Insert the missing (height, width) singleton axes after the layer axis
+    so that `a` has `rank` axes: per-layer values (n, 3) become (n, 1, 1, 3)
+    and row data (n, W, 3) become (n, 1, W, 3)."""
+    missing = rank - a.ndim
+    if not 0 <= missing <= 2:
+        raise ValueError("Invalid %s dimensions." % what)
+    return a.reshape(a.shape[:1] + (1,) * missing + a.shape[1:])
+
+
+def validate_optical_block(data, shape=None, wavelength=None):
+    """Checks one block (thickness, eps, angles) and returns it cast to the
+    engine's types with the layer axis first (dtmm/data.py:534-618).
+
+    A scalar thickness is a one-layer block.  eps with fewer than four axes
+    gains singleton lateral axes; eps and angles must broadcast to
+    (layers, height, width, 3) of `shape`.  A dispersive eps is evaluated at
+    `wavelength` (nm) when one is given; without one, an
+    ``EpsilonDispersive`` keeps its coefficients (with a trailing coefficient
+    axis) and any other callable is an error."""
     if isinstance(data, list):
         if shape is None:
             raise ValueError("For heterogeneous data, you must provide the `shape` argument.")
-        return [validate_optical_block(d, shape) for d in data]
-    return validate_optical_data([data], shape if shape is not None else (1, 1))
+        return [validate_optical_block(b, shape, wavelength) for b in data]
+    thickness, material, angles = data
+    kind = None
+    if callable(material):
+        if wavelength is not None:
+            material = material(wavelength)
+        elif isinstance(material, EpsilonDispersive):
+            kind = type(material)
+            material = material.coefficients
+        else:
+            raise ValueError("Epsilon is a callable. You must provide wavelength! Use "
+                             "split_wavelength = True.")
+    thickness = np.asarray(thickness, FDTYPE)
+    material = np.asarray(material)
+    material = material.astype(CDTYPE if np.iscomplexobj(material) else FDTYPE, copy=False)
+    angles = np.asarray(angles, FDTYPE)
+    if thickness.ndim > 1:
+        raise ValueError("thickness must be a scalar or a 1D array of layer thicknesses.")
+    if thickness.ndim == 0:
+        thickness, material, angles = thickness[None], material[None], angles[None]
+    n = len(thickness)
+    lateral = (1, 1) if shape is None else tuple(shape)
+    rank = 4 if kind is None else 5
+    material = _with_lateral_axes(material, rank, "material" if kind is None else
+                                  "material coefficients")
+    np.broadcast_shapes(material.shape, (n,) + lateral + (3,) + (() if kind is None else (1,)))
+    np.broadcast_shapes(angles.shape, (n,) + lateral + (3,))
+    if kind is not None:
+        material = kind(material)
+    return thickness, material, angles
+
+
+def validate_optical_data(data, shape=None, wavelength=None):
+    """List of validated blocks (dtmm/data.py:620-643); a single block is
+    taken as a one-block list."""
+    if not isinstance(data, list):
+        return validate_optical_data([data], shape if shape is not None else (1, 1), wavelength)
+    if shape is None:
+        raise ValueError("For heterogeneous data, you must provide the `shape` argument.")
+    return [validate_optical_block(b, shape, wavelength) for b in data]
+
+
+def evaluate_optical_block(block, wavelength=550):
+    """The block with a dispersive eps evaluated at `wavelength` (nm)
+    (dtmm/data.py:682-687)."""
+    d, epsv, epsa = block
+    return d, (epsv(wavelength) if callable(epsv) else epsv), epsa
 
 
 def isotropic_effective_eps(block):

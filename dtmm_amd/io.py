@@ -108,8 +108,10 @@ def load_stack(file):
 def save_field(file, field_data):
     """Save field data in ``.dtmf`` format (dtmm/field.py:906-935)."""
     field, wavelengths, pixelsize = field_data
-    field = np.asarray(field)
-    wavelengths = np.asarray(wavelengths)
+    # validate_field_data casts to complex128 / float64 before saving
+    # (dtmm/field.py:891-894), so the stored arrays always have those dtypes
+    field = np.asarray(field, np.complex128)
+    wavelengths = np.asarray(wavelengths, np.float64)
     if field.ndim < 4 or field.shape[-3] != 4:
         raise ValueError("field must have shape (..., n_wavelengths, 4, H, W)")
     if wavelengths.ndim != 1 or wavelengths.shape[0] != field.shape[-4]:

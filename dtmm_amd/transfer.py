@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _lib
 from . import tune as _tune
-from .data import refind2eps, uniaxial_order, validate_optical_data
+from .data import is_optical_data_dispersive, uniaxial_order, validate_optical_data
 from .field import blackman, k0 as _k0
 
 DTMM_NORM_FFT = 1 << 0
@@ -176,11 +176,19 @@ class Stack:
         """The stack of ``Q2data(qtensor, mask, no, ne, nhost, scale_factor,
         biaxial, thickness)`` (dtmm/data.py:322-373) built on the GPU
         (``dtmmx_q2data``): the Q tensor (L, H, W, 6) with elements Q00, Q11,
-        Q22, Q01, Q02, Q12 -- a device tensor stays on the device -- becomes
-        the device material arrays (eigenvalues of eps, the most distinct
-        last) and the Euler angles of the eigenframe, plus the effective
-        medium as in ``from_director``."""
+        Q22, Q01, Q02, Q12, or the full (L, H, W, 3, 3) matrix as Q2eps also
+        takes it (data.py:463-472) -- a device tensor stays on the device --
+        becomes the device material arrays (eigenvalues of eps, the most
+        distinct last) and the Euler angles of the eigenframe, plus the
+        effective medium as in ``from_director``."""
         lib = _lib.load()
+        torch = _torch()
+        if tuple(qtensor.shape[-2:]) == (3, 3) and len(qtensor.shape) == 5:
+            rows, cols = [0, 1, 2, 0, 0, 1], [0, 1, 2, 1, 2, 2]
+            if isinstance(qtensor, torch.Tensor):
+                qtensor = qtensor[..., rows, cols]
+            else:
+                qtensor = np.asarray(qtensor)[..., rows, cols]
 
         def launch(src, mk, H, W, L, eps_re, angles, sums, stream):
             return lib.dtmmx_q2data(
@@ -284,6 +292,10 @@ class Plan:
         self.precision = precision
         self.diffraction = int(diffraction)
         self.npass = int(npass)
+        self.norm = int(norm)
+        self.betamax = float(betamax)
+        self.reflection = int(reflection if reflection is not None else
+                              (1 if method == "4x4" else 0))
         d = _lib.PlanDesc()
         d.height, d.width = self.shape
         d.n_wavelengths = len(self.ks)
@@ -299,7 +311,7 @@ class Plan:
         d.npass = self.npass
         d.norm = int(norm)
         d.smooth = float(smooth)
-        d.reflection = int(reflection if reflection is not None else (1 if method == "4x4" else 0))
+        d.reflection = self.reflection
         d.checkpoint = int(checkpoint)
         h = ctypes.c_void_p()
         _lib.check(lib.dtmmx_plan_create(ctypes.byref(h), ctypes.byref(d)))
@@ -476,7 +488,7 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
 
     field_in, wavelengths, pixelsize = field_data
     if isinstance(field_in, tuple):
-        raise NotImplementedError("tuple (per-wavelength) field data is not supported")
+        raise ValueError("a tuple of per-wavelength fields goes through transfer_field")
     field_np = np.asarray(field_in)
     shape = field_np.shape[-2:]
     wavelengths = np.asarray(wavelengths)
@@ -521,10 +533,13 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
                      pixelsize)
 
 
-def _check_status(status):
+def _check_status(status, adjoint=False):
     if int(status[0].item()) & 1:
-        raise NotImplementedError("biaxial voxels (eps0 != eps1) are not supported by the "
-                                  "adjoint transfer")
+        if adjoint:
+            raise NotImplementedError("biaxial voxels (eps0 != eps1) are not supported by the "
+                                      "adjoint transfer")
+        raise NotImplementedError("a biaxial voxel (eps0 != eps1) was met in a layer not "
+                                  "flagged biaxial (Stack.biaxial); its output is NaN")
 
 
 def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
@@ -537,13 +552,81 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
 
     Returns ``(field_out, wavelengths, pixelsize)``.  ``precision`` ("c64" or
     "c128") selects the GPU arithmetic; by default it follows the input
-    field dtype (complex64 -> "c64", otherwise "c128").  The split_* options
-    only bound memory in the reference and do not change the result; they
-    are accepted and ignored here (the GPU batches all planes).  ``tune``
-    times the kernels' launch shapes on this transfer first (dtmm_amd.tune;
-    the winner is cached for later calls of the same workload shape).
+    field dtype (complex64 -> "c64", otherwise "c128").  ``split_rays`` and
+    ``split_diffraction`` only bound memory in the reference and do not
+    change the result; they are accepted and ignored (the GPU batches all
+    planes).  ``tune`` times the kernels' launch shapes on this transfer
+    first (dtmm_amd.tune; the winner is cached for later calls of the same
+    workload shape).
+
+    Per-wavelength transfers, as in the reference (transfer.py:486-528):
+    with ``split_wavelengths`` (and no ``ret_bulk``) each wavelength of the
+    (..., wavelengths, 4, H, W) input is transferred on its own and the
+    result is put back together; a tuple of per-wavelength fields, each
+    (..., 4, H, W), returns a tuple of ``(field, wavelength, pixelsize)``.
+    The optical data is evaluated per wavelength, so a dispersive eps
+    (``EpsilonCauchy``, ``EpsilonSellmeier`` or any callable of the
+    wavelength in nm) is accepted on this route only (transfer.py:535-536).
     """
-    torch = _require_gpu()
+    kw = dict(beta=beta, phi=phi, nin=nin, nout=nout, npass=npass, nstep=nstep,
+              diffraction=diffraction, reflection=reflection, method=method, multiray=multiray,
+              norm=norm, betamax=betamax, smooth=smooth, split_layers=split_layers,
+              eff_data=eff_data, ret_bulk=ret_bulk, precision=precision, tune=tune)
+    field_in, wavelengths, pixelsize = field_data
+    if (not isinstance(field_in, tuple) and not (split_wavelengths and not ret_bulk) and
+            not isinstance(optical_data, Stack)):
+        shape = np.shape(field_in)[-2:]
+        if is_optical_data_dispersive(validate_optical_data(optical_data, shape=shape)):
+            raise ValueError("You are using dispersive data, so you must use "
+                             "`split_wavelengths=True`")
+    _require_gpu()
+    if split_wavelengths and not isinstance(field_in, tuple) and not ret_bulk:
+        arr = field_in if isinstance(field_in, np.ndarray) else np.asarray(field_in)
+        res = np.empty_like(arr) if out is None else out
+        parts = tuple(arr[..., i, :, :, :] for i in range(len(wavelengths)))
+        outs = [res[..., i, :, :, :] for i in range(len(wavelengths))]
+        _transfer_wavelengths(parts, wavelengths, pixelsize, optical_data, outs, kw)
+        return res, wavelengths, pixelsize
+    if isinstance(field_in, tuple):
+        return _transfer_wavelengths(field_in, wavelengths, pixelsize, optical_data, out, kw)
+    return _transfer_one(field_data, optical_data, out=out, **kw)
+
+
+def _transfer_wavelengths(fields, wavelengths, pixelsize, optical_data, outs, kw):
+    """One transfer per wavelength (dtmm/transfer.py:510-528).  Each field
+    (..., 4, H, W) gets a wavelength axis of length 1 as a view, so the
+    input-side field of npass > 1 lands in the caller's array.  Optical
+    data that does not depend on the wavelength is uploaded once."""
+    if len(fields) != len(wavelengths):
+        raise ValueError("the tuple of fields needs one wavelength per field")
+    outs = [None] * len(fields) if outs is None else list(outs)
+    stack = optical_data
+    if not isinstance(optical_data, Stack):
+        shape = np.shape(fields[0])[-2:]
+        if not is_optical_data_dispersive(validate_optical_data(optical_data, shape=shape)):
+            stack = Stack(optical_data, shape, nstep=kw["nstep"],
+                          nin=N_COVER if kw["nin"] is None else kw["nin"],
+                          nout=N_COVER if kw["nout"] is None else kw["nout"],
+                          split_layers=kw["split_layers"])
+    results = []
+    for f, w, o in zip(fields, wavelengths, outs):
+        f = f if isinstance(f, np.ndarray) else np.asarray(f)
+        opt = stack if isinstance(stack, Stack) else \
+            validate_optical_data(optical_data, shape=f.shape[-2:], wavelength=w)
+        o1 = None if o is None else np.expand_dims(o, -4)
+        r = _transfer_one((np.expand_dims(f, -4), np.asarray([w]), pixelsize), opt, out=o1,
+                          **kw)[0]
+        results.append((o if o is not None else np.squeeze(r, -4), w, pixelsize))
+    return tuple(results)
+
+
+def _transfer_one(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
+                  npass=None, nstep=1, diffraction=None, reflection=None, method=None,
+                  multiray=False, norm=DTMM_NORM_FFT, betamax=None, smooth=SMOOTH,
+                  split_layers=False, eff_data=None, ret_bulk=False, out=None, precision=None,
+                  tune=False):
+    """One transfer of transfer_field on the GPU (dtmm/transfer.py:554-615)."""
+    torch = _torch()
     job = _prepare(field_data, optical_data, beta, phi, nin, nout, npass, nstep, diffraction,
                    reflection, method, multiray, norm, betamax, smooth, split_layers, eff_data,
                    ret_bulk, out, precision)

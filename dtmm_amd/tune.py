@@ -21,7 +21,8 @@ _CACHE = {}
 def signature(plan):
     """What the launch-shape choice depends on."""
     return (plan.shape, len(plan.ks), plan.n_rays, plan.rays_per_beam, plan.method,
-            plan.precision, plan.diffraction, plan.npass)
+            plan.precision, plan.diffraction, plan.npass, plan.reflection, plan.norm,
+            plan.betamax)
 
 
 def candidates(plan):

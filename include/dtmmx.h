@@ -179,7 +179,7 @@ int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field
  * field_in and field_out.  The 2x2 method's slots (transfer_2x2,
  * transfer.py:1045-1050, propagate_2x2.py:80-85) hold E after each layer and
  * the H of each voxel's forward modes.  Built with npass 1 and diffraction 0
- * or 1 (4x4: reflection 1 or 2; 2x2: reflection 0 or 1). */
+ * or 1 (4x4: reflection 1 or 2; 2x2: reflection 0, 1 or 2). */
 int dtmmx_transfer_bulk(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field_in,
                         void* field_out, void* bulk, void* workspace, size_t workspace_bytes,
                         int* status, void* stream);

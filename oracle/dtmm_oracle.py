@@ -62,6 +62,8 @@ __all__ = [
     "specter2color", "f_iso", "layer_mat", "stack_mat", "system_mat", "reflection_mat",
     "transmit4x4", "transfer4x4", "Eti_mat", "corrected_E_diffraction_matrix",
     "tukey", "aperture", "illumination_aperture", "aperture2rays", "illumination_rays",
+    "cauchy2eps", "sellmeier2eps", "is_optical_data_dispersive", "mean_betaphi",
+    "field2betaphi",
 ]
 
 # dtmm/dtmm.ini:6 (betamax = 0.8) and dtmm/conf.py:148, :503-510 (nin/nout
@@ -170,6 +172,34 @@ def blackman(shape):
     out = 0.42 + 0.5 * np.cos(np.pi * r) + 0.08 * np.cos(2 * np.pi * r)
     out[r >= 1.] = 0.
     return np.asarray(out, FDTYPE)
+
+
+def mean_betaphi(field, k0):
+    """Power-weighted mean beam of a field (..., 4, H, W): the spectrum of
+    the Blackman-windowed field weights the (beta_x, beta_y) of every FFT
+    mode, summed over the components; beta = |mean|, phi = its angle
+    (dtmm/field.py:849-872, _fft_betaphi).  k0 broadcasts against the
+    leading axes (one per wavelength)."""
+    field = np.asarray(field)
+    f = fft2(field * blackman(field.shape[-2:]))
+    bx, by = betaxy(field.shape[-2:], k0)
+    s = (f * np.conj(f)).real
+    sx = (s * bx[..., None, :, :]).sum(axis=(-3, -2, -1))
+    sy = (s * by[..., None, :, :]).sum(axis=(-3, -2, -1))
+    return np.sqrt(sx ** 2 + sy ** 2) / s.sum(axis=(-3, -2, -1)), np.arctan2(sy, sx)
+
+
+def field2betaphi(field_in, ks, multiray=False):
+    """The beam transfer_field uses when beta and phi are not given
+    (dtmm/field.py:492-500, transfer.py:662-664): the mean of mean_betaphi
+    over every wavelength and plane, or per ray (first axis) with multiray
+    and more than one leading axis."""
+    beta, phi = mean_betaphi(field_in, ks)
+    field_in = np.asarray(field_in)
+    if field_in.ndim > 4 and multiray:
+        axes = tuple(range(1, field_in.ndim - 3))
+        return beta.mean(axis=axes), phi.mean(axis=axes)
+    return beta.mean(), phi.mean()
 
 
 # ---------------------------------------------------------------------------
@@ -710,14 +740,23 @@ def tensor_eig(t6):
     """Eigenvalues and eigenvectors (columns) of a symmetric tensor (..., 6),
     ordered as dtmm/linalg.py:172-222 orders them: ascending with a
     right-handed frame, then the most distinct eigenvalue moved to index 2 by
-    a cyclic permutation.  numpy.linalg.eigh replaces the reference's closed
-    form (linalg.py:225-315); eigenvectors agree up to sign."""
+    a cyclic permutation.  A tensor whose off-diagonal elements, scaled by
+    its largest element, have a squared norm of at most 1e-12 is taken as
+    diagonal: its eigenvalues keep the input order with the identity frame
+    before the cyclic permutation (linalg.py:251-318).  numpy.linalg.eigh
+    replaces the reference's closed form (linalg.py:225-315); eigenvectors
+    agree up to sign."""
     t6 = np.asarray(t6, FDTYPE)
     m = np.empty(t6.shape[:-1] + (3, 3))
     for k, (i, j) in enumerate(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
         m[..., i, j] = m[..., j, i] = t6[..., k]
     w, v = np.linalg.eigh(m)
     v[..., :, 2] = np.cross(v[..., :, 0], v[..., :, 1])
+    scale = np.abs(t6).max(axis=-1)
+    sm = np.divide(1., scale, out=np.zeros_like(scale), where=scale > 0)
+    diagonal = ~(((t6[..., 3:] * sm[..., None]) ** 2).sum(-1) > 1e-12)
+    w = np.where(diagonal[..., None], t6[..., :3] * sm[..., None] * scale[..., None], w)
+    v = np.where(diagonal[..., None, None], np.eye(3), v)
     m2 = np.abs(w[..., 1] - w[..., 0])
     m1 = np.abs(w[..., 2] - w[..., 0])
     m0 = np.abs(w[..., 2] - w[..., 1])
@@ -762,10 +801,46 @@ def epsva2eps(epsv, epsa):
     return _rotate_diagonal_tensor(R, np.asarray(epsv))
 
 
-def validate_optical_block(block):
+def cauchy2eps(coefficients, wavelength):
+    """eps = (c0 + c1 / w^2 + c2 / w^4 + ...)^2 with w in microns, per row of
+    coefficients (..., n) (dtmm/data.py:1110-1129, EpsilonCauchy.__call__
+    :1183-1193 divides the nanometre wavelength by 1000)."""
+    c = np.asarray(coefficients, FDTYPE)
+    w = wavelength / 1000.
+    n = c[..., 0].copy()
+    for i in range(1, c.shape[-1]):
+        n += c[..., i] / w ** (2 * i)
+    return n ** 2
+
+
+def sellmeier2eps(coefficients, wavelength):
+    """eps = c0 + sum_i w^2 c_(2i+1) / (w^2 - c_(2i+2)) with w in microns
+    (dtmm/data.py:1072-1093, EpsilonSellmeier.__call__ :1195-1204)."""
+    c = np.asarray(coefficients, FDTYPE)
+    w = wavelength / 1000.
+    eps = c[..., 0].copy()
+    for i in range((c.shape[-1] - 1) // 2):
+        eps += c[..., 1 + 2 * i] * w ** 2 / (w ** 2 - c[..., 2 + 2 * i])
+    return eps
+
+
+def is_optical_data_dispersive(data):
+    """Whether any block's eps is a callable of the wavelength
+    (dtmm/data.py:1131-1142)."""
+    blocks = data if isinstance(data, list) else [data]
+    return any(callable(b[1]) for b in blocks)
+
+
+def validate_optical_block(block, wavelength=None):
     """Cast and add singleton axes so that a block broadcasts to
-    (L, H, W, 3) (dtmm/data.py:534-618; dispersive blocks not covered)."""
+    (L, H, W, 3) (dtmm/data.py:534-618).  A callable eps is evaluated at
+    `wavelength` (nm), as validate_optical_data(..., wavelength=w) does for
+    the per-wavelength transfers of split_wavelengths."""
     d, material, angles = block
+    if callable(material):
+        if wavelength is None:
+            raise ValueError("Epsilon is a callable. You must provide wavelength!")
+        material = material(wavelength)
     d = np.asarray(d, FDTYPE)
     material = np.asarray(material)
     material = np.asarray(material, CDTYPE if np.iscomplexobj(material) else FDTYPE)
@@ -779,11 +854,11 @@ def validate_optical_block(block):
     return d, material, angles
 
 
-def validate_optical_data(data):
+def validate_optical_data(data, wavelength=None):
     """A list of validated blocks (dtmm/data.py:620-643)."""
     if isinstance(data, list):
-        return [validate_optical_block(b) for b in data]
-    return [validate_optical_block(data)]
+        return [validate_optical_block(b, wavelength) for b in data]
+    return [validate_optical_block(data, wavelength)]
 
 
 def _isotropic_effective_data(block):
@@ -800,6 +875,9 @@ def _isotropic_effective_data(block):
 def create_layers(optical_data, nin, nout, nstep):
     """Layer and effective-layer lists with the input/output media as
     zero-thickness layers (dtmm/transfer.py:926-1005; eff_data=0 only)."""
+    if is_optical_data_dispersive(optical_data):
+        raise ValueError("You are using dispersive data, so you must use "
+                         "`split_wavelengths=True`")
     blocks = validate_optical_data(optical_data)
     layers, eff_layers = [], []
     nb = len(blocks)
@@ -1614,11 +1692,53 @@ def _transmitted_field_direct(field, beta, phi, n=1.):
 def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
                    npass=None, nstep=1, diffraction=None, reflection=None, method=None,
                    betamax=None, norm=DTMM_NORM_FFT, smooth=SMOOTH, layer_callback=None,
-                   ret_bulk=False):
+                   ret_bulk=False, split_wavelengths=False, multiray=False):
     """Defaults and dispatch of dtmm.transfer_field (dtmm/transfer.py:322-615)
     for the covered modes.  The configured default method is the quoted string
     '"2x2"' (dtmm/dtmm.ini:23 read by configparser), which makes the automatic
-    reflection mode 1 (transfer.py:459)."""
+    reflection mode 1 (transfer.py:459).
+
+    Per-wavelength transfers (transfer.py:486-528): with split_wavelengths
+    (and no ret_bulk) the (..., wavelengths, 4, H, W) input is cut into one
+    field per wavelength and the result put back together; a tuple of
+    per-wavelength fields (each (..., 4, H, W)) returns a tuple of
+    (field, wavelength, pixelsize) results.  Each wavelength is an
+    independent transfer with the optical data evaluated there (a callable
+    eps is called with the wavelength), so dispersive data is allowed only
+    on this route (transfer.py:535-536)."""
+    field_in, wavelengths, pixelsize = field_data
+    kw = dict(beta=beta, phi=phi, nin=nin, nout=nout, npass=npass, nstep=nstep,
+              diffraction=diffraction, reflection=reflection, method=method, betamax=betamax,
+              norm=norm, smooth=smooth, layer_callback=layer_callback, ret_bulk=ret_bulk,
+              multiray=multiray)
+    if split_wavelengths and not isinstance(field_in, tuple) and not ret_bulk:
+        field_in = np.asarray(field_in)
+        parts = tuple(field_in[..., i, :, :, :] for i in range(len(wavelengths)))
+        res = transfer_field((parts, wavelengths, pixelsize), optical_data, **kw)
+        return np.stack([r[0] for r in res], -4), wavelengths, pixelsize
+    if isinstance(field_in, tuple):
+        out = []
+        for f, w in zip(field_in, wavelengths):
+            # one wavelength: a wavelength axis of length 1 computes what the
+            # reference's scalar wave number does
+            f1 = np.expand_dims(f, -4)
+            opt = validate_optical_data(optical_data, wavelength=w)
+            r = _transfer_field(f1, np.asarray([w], FDTYPE), pixelsize, opt, **kw)[0]
+            out.append((np.squeeze(r, -4), w, pixelsize))
+        return tuple(out)
+    return _transfer_field(field_in, wavelengths, pixelsize, optical_data, **kw)
+
+
+def _transfer_field(field_in, wavelengths, pixelsize, optical_data, beta=None, phi=None,
+                    nin=None, nout=None, npass=None, nstep=1, diffraction=None, reflection=None,
+                    method=None, betamax=None, norm=DTMM_NORM_FFT, smooth=SMOOTH,
+                    layer_callback=None, ret_bulk=False, multiray=False):
+    """The single transfer of transfer_field (dtmm/transfer.py:554-615).
+    Without beta and phi the beam is inferred from the input field
+    (transfer.py:662-664, :1040-1042)."""
+    field_data = (field_in, wavelengths, pixelsize)
+    if beta is None and phi is None:
+        beta, phi = field2betaphi(field_in, k0(wavelengths, pixelsize), multiray)
     nin = N_COVER if nin is None else nin
     nout = N_COVER if nout is None else nout
     method = '"2x2"' if method is None else method

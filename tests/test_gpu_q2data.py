@@ -24,6 +24,9 @@ def _q(shape, seed):
     q[0, 0, 0] = 0.                                         # isotropic voxel
     q[0, 0, 1] = (-1. / 3, -1. / 3, 2. / 3, 0., 0., 0.)     # director along z
     q[0, 0, 2] = (2. / 3, -1. / 3, -1. / 3, 0., 0., 0.)     # diagonal, distinct axis x
+    # diagonal tensors keep their input order (no ascending sort first)
+    q[0, 0, 3] = (0.3, 0.2, -0.5, 0., 0., 0.)
+    q[0, 0, 4] = (0.1, 0.05, 0.25, 1e-9, 0., 0.)
     return q
 
 
@@ -49,6 +52,10 @@ def test_q2data_matches_oracle(biaxial, masked):
         np.testing.assert_allclose(o.epsva2eps(eps_g, ang_g), o.epsva2eps(epsv, epsa), atol=1e-13)
     if mask is None:
         np.testing.assert_allclose(ang_g[0, 0, 2], [np.pi / 2, np.pi / 2, 0.], atol=1e-15)
+        np.testing.assert_array_equal(ang_g[0, 0, 3:5], 0.)
+        if biaxial:
+            e = o.Q2eps(q[0, 0, 3:5], 1.5, 1.7, 0.9)[..., :3]
+            np.testing.assert_allclose(eps_g[0, 0, 3:5], e, rtol=1e-15)
     np.testing.assert_allclose(st.eps_eff_re, epsv.sum(-1).mean() / 3., rtol=1e-13)
 
 

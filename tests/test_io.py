@@ -80,3 +80,16 @@ def test_field_round_trip(tmp_path):
     np.testing.assert_array_equal(field, fd[0])
     np.testing.assert_array_equal(wl, fd[1])
     assert ps == 50.
+
+
+def test_field_is_saved_in_double_precision(tmp_path):
+    """save_field casts like validate_field_data (dtmm/field.py:891-894): a
+    complex64 field and integer wavelengths are stored as complex128 and
+    float64, so the file is the reference's byte for byte."""
+    f = (np.arange(2 * 4 * 3 * 5).reshape(2, 4, 3, 5) * (1 + 0.5j)).astype(np.complex64)
+    path = str(tmp_path / "f.dtmf")
+    dio.save_field(path, (f, [500, 600], 100))
+    g, wl, ps = dio.load_field(path)
+    assert g.dtype == np.complex128 and wl.dtype == np.float64
+    np.testing.assert_array_equal(g, f.astype(np.complex128))
+    np.testing.assert_array_equal(wl, [500., 600.])

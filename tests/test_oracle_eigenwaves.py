@@ -128,3 +128,63 @@ def test_product_illumination_without_window_matches_oracle(jones):
     # eigenmodes does not change the field, as in the reference
     got2, _, _ = dm.illumination_data(shape, wl, eigenmodes=False, **kw)
     np.testing.assert_array_equal(got2, got)
+
+
+# ---------------------------------------------------------------------------
+# beam inference when transfer_field gets no beta and phi
+# (dtmm/field.py:492-500, :849-872)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("i,j", [(0, 0), (0, 3), (2, -1), (-3, 2)])
+def test_mean_betaphi_of_a_grid_plane_wave_is_its_beam(i, j):
+    """A plane wave on FFT mode (i, j) windowed by the radial Blackman
+    window has a spectrum symmetric about that mode (the window is real and
+    even), so the power-weighted mean of (beta_x, beta_y) is the mode's own
+    (beta_x, beta_y) on a grid large enough for the window's spread not to
+    wrap."""
+    shape = (48, 64)
+    wl = np.array([500., 600.])
+    ks = o.k0(wl, 100.)
+    yy, xx = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    wave = np.exp(2j * np.pi * (i * yy / shape[0] + j * xx / shape[1]))
+    f = np.zeros((2, 4) + shape, complex)
+    f[:, 0] = wave
+    f[:, 1] = 1.5 * wave
+    beta, phi = o.mean_betaphi(f, ks)
+    bx = 2 * np.pi / ks * j / shape[1]
+    by = 2 * np.pi / ks * i / shape[0]
+    np.testing.assert_allclose(beta, np.hypot(bx, by), atol=1e-12)
+    if (i, j) != (0, 0):
+        np.testing.assert_allclose(phi, np.arctan2(by, bx), atol=1e-12)
+
+
+def test_field2betaphi_multiray_keeps_the_ray_axis():
+    shape = (32, 32)
+    ks = o.k0(np.array([550.]), 100.)
+    yy, xx = np.meshgrid(np.arange(32), np.arange(32), indexing="ij")
+    rays = []
+    for j in (0, 2, 4):
+        w = np.exp(2j * np.pi * j * xx / 32)
+        f = np.zeros((2, 1, 4) + shape, complex)
+        f[:, :, 0] = w
+        f[0, :, 1] = w
+        f[1, :, 2] = w
+        rays.append(f)
+    field = np.stack(rays)          # (ray, pol, wavelength, 4, H, W)
+    b, p = o.field2betaphi(field, ks, multiray=True)
+    assert b.shape == (3,)
+    np.testing.assert_allclose(b, [2 * np.pi / ks[0] * j / 32 for j in (0, 2, 4)], atol=1e-12)
+    b1, _ = o.field2betaphi(field, ks, multiray=False)
+    np.testing.assert_allclose(b1, b.mean(), atol=1e-12)
+
+
+def test_product_field2betaphi_matches_oracle():
+    import dtmm_amd as dm
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal((3, 2, 4, 20, 24)) + 1j * rng.standard_normal((3, 2, 4, 20, 24))
+    ks = o.k0(np.array([450., 650.]), 100.)
+    for multiray in (False, True):
+        a = dm.field2betaphi(f, ks, multiray)
+        b = o.field2betaphi(f, ks, multiray)
+        np.testing.assert_allclose(a[0], b[0], rtol=1e-12)
+        np.testing.assert_allclose(a[1], b[1], rtol=1e-12, atol=1e-15)

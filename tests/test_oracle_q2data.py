@@ -6,8 +6,15 @@ The reference's tests hold no vector for it, so it is pinned by:
 
 * the rebuild: R(angles) diag(eigenvalues) R^T equals the eps tensor of Q
   (a wrong angle formula, sign or transposed frame fails it);
-* the order rule on diagonal tensors, by hand: (3, 1, 1) -> (1, 1, 3) with
-  angles (pi/2, pi/2, 0); (1, 3, 3.5) -> (3, 3.5, 1);
+* the order rule on diagonal tensors, by hand: a diagonal tensor keeps its
+  input order and the identity frame, then the most distinct eigenvalue
+  moves to index 2 by a cyclic permutation only when it is not there
+  already: (3, 1, 1) -> (1, 1, 3) with angles (pi/2, pi/2, 0);
+  (1, 3, 3.5) -> (3, 3.5, 1); (3.5, 3, 1) and (2.3, 2.2, 2.6) stay, with
+  angles 0 (an ascending sort first would give (3, 3.5, 1) and
+  (2.2, 2.3, 2.6));
+* the diagonal threshold: off-diagonal elements 1e-7 of the largest one
+  (squared norm 2e-14) count as diagonal, 1e-5 (2e-10) do not;
 * a uniaxial Q of a unit director n: eigenvalues (no^2, no^2, ne^2) and the
   last eigenvector +-n.
 """
@@ -33,15 +40,26 @@ def test_rebuild_is_the_eps_tensor():
     assert np.all(np.abs(np.diff(epsv[..., :2], axis=-1)) <= np.abs(epsv[..., 2:] - epsv[..., :2]).min(-1, keepdims=True) + 1e-12)
 
 
-@pytest.mark.parametrize("t6,w_exp", [((3., 1., 1., 0., 0., 0.), (1., 1., 3.)),
-                                      ((1., 3., 3.5, 0., 0., 0.), (3., 3.5, 1.))])
-def test_order_rule_on_diagonal_tensors(t6, w_exp):
+@pytest.mark.parametrize("t6,w_exp,ang_exp", [
+    ((3., 1., 1., 0., 0., 0.), (1., 1., 3.), (np.pi / 2, np.pi / 2, 0.)),
+    ((1., 3., 3.5, 0., 0., 0.), (3., 3.5, 1.), None),
+    ((3.5, 3., 1., 0., 0., 0.), (3.5, 3., 1.), (0., 0., 0.)),
+    ((2.3, 2.2, 2.6, 0., 0., 0.), (2.3, 2.2, 2.6), (0., 0., 0.))])
+def test_order_rule_on_diagonal_tensors(t6, w_exp, ang_exp):
     w, v = o.tensor_eig(np.asarray(t6))
     np.testing.assert_allclose(w, w_exp, atol=1e-14)
     np.testing.assert_allclose(np.linalg.det(v), 1., atol=1e-14)
-    if t6[0] == 3.:
-        np.testing.assert_allclose(np.abs(o.rotation_angles(v)), [np.pi / 2, np.pi / 2, 0.],
-                                   atol=1e-14)
+    if ang_exp is not None:
+        np.testing.assert_allclose(np.abs(o.rotation_angles(v)), ang_exp, atol=1e-14)
+
+
+def test_diagonal_threshold():
+    near = o.tensor_eig(np.asarray([3.5, 3., 1., 3.5e-7, 0., 0.]))
+    assert tuple(near[0]) == (3.5, 3., 1.)
+    np.testing.assert_array_equal(near[1], np.eye(3))
+    off = o.tensor_eig(np.asarray([3.5, 3., 1., 3.5e-5, 0., 0.]))
+    np.testing.assert_allclose(np.sort(off[0]), [1., 3., 3.5], atol=1e-8)
+    assert not np.array_equal(off[1], np.eye(3))
 
 
 def test_uniaxial_q_of_a_director():
