@@ -55,6 +55,7 @@ EXPORTS = (
     "dtmmx_director2data_blocks",
     "dtmmx_director2data",
     "dtmmx_q2data",
+    "dtmmx_transfer_noalloc",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -184,6 +185,10 @@ def load():
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
     lib.dtmmx_transfer.restype = ctypes.c_int
+    lib.dtmmx_transfer_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stack),
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p]
+    lib.dtmmx_transfer_noalloc.restype = ctypes.c_int
     lib.dtmmx_transfer_batch.argtypes = [ctypes.POINTER(Job), ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_void_p]
     lib.dtmmx_transfer_batch.restype = ctypes.c_int

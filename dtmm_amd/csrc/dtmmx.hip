@@ -2013,6 +2013,41 @@ static int upload_windows(dtmmx_plan* P, const std::vector<int>& amax) {
 }  // namespace dtmmx
 
 #if DTMMX_HOST_PART
+// Workspace of dtmmx_transfer_noalloc, the transfer for callers that cannot
+// allocate.  Each GPU the library runs on holds its own copy of the array
+// for the life of the process; in host code `scratch` only names the
+// symbol, and its device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[DTMMX_NOALLOC_SCRATCH_BYTES];
+
+namespace dtmmx {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the device of `stream` (the current device
+// for the null stream), looked up once per device with that device current.
+static int scratch_address(hipStream_t stream, void** out) {
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  int dev = cur;
+  if (stream) HIP_TRY(hipStreamGetDevice(stream, &dev));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+    return fail(DTMMX_E_ARG, "device index " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    if (dev != cur) HIP_TRY(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (dev != cur) HIP_TRY(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(DTMMX_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DTMMX_OK;
+}
+}  // namespace dtmmx
+
 extern "C" {
 
 int dtmmx_abi_version(void) { return DTMMX_ABI_VERSION; }
@@ -2266,6 +2301,25 @@ int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* f
     if (r != DTMMX_OK) return r;
   }
   hipStream_t s = (hipStream_t)stream;
+  if (P->d.precision == DTMMX_C64) return transfer_c64(P, S, fin, fout, ws, status, s);
+  return transfer_c128(P, S, fin, fout, ws, status, s);
+}
+
+int dtmmx_transfer_noalloc(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
+                           int* status, void* stream) {
+  const size_t need = dtmmx_plan_workspace_size(P);
+  if (P && need > DTMMX_NOALLOC_SCRATCH_BYTES)
+    return fail(DTMMX_E_ARG, "the plan's workspace (" + std::to_string(need) +
+                                 " bytes) exceeds the no-allocation scratch (" +
+                                 std::to_string(size_t(DTMMX_NOALLOC_SCRATCH_BYTES)) +
+                                 " bytes): use dtmmx_transfer with a workspace");
+  hipStream_t s = (hipStream_t)stream;
+  void* ws = nullptr;
+  {
+    const int r = check_transfer(P, S, fin, fout, nullptr, 0, false);
+    if (r != DTMMX_OK) return r;
+  }
+  if (need) TRY(scratch_address(s, &ws));
   if (P->d.precision == DTMMX_C64) return transfer_c64(P, S, fin, fout, ws, status, s);
   return transfer_c128(P, S, fin, fout, ws, status, s);
 }

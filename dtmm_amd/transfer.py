@@ -275,6 +275,11 @@ class Stack:
         return s
 
 
+# size of the library's per-GPU scratch array used by Plan.execute_noalloc
+# (DTMMX_NOALLOC_SCRATCH_BYTES of include/dtmmx.h)
+NOALLOC_SCRATCH_BYTES = 128 << 20
+
+
 class Plan:
     """Owner of a ``dtmmx_plan``: grid, wavelengths, beams, method, precision."""
 
@@ -348,6 +353,26 @@ class Plan:
             workspace.data_ptr() if workspace is not None else None,
             self.workspace_bytes, status.data_ptr() if status is not None else None,
             stream.cuda_stream))
+        return field_out
+
+    def execute_noalloc(self, stack, field_in, field_out, status=None, stream=None):
+        """execute() through ``dtmmx_transfer_noalloc``: the workspace is the
+        library's static per-GPU scratch array (at most
+        ``NOALLOC_SCRATCH_BYTES``), so the call allocates no device memory.
+        Calls that overlap on one GPU share that array and must not run
+        concurrently."""
+        torch = _torch()
+        if stream is None:
+            stream = torch.cuda.current_stream(field_in.device)
+        for t in (field_in, field_out):
+            if t.dtype != self.dtype or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError("fields must be contiguous %s device tensors" % self.dtype)
+            if tuple(t.shape) != (self.n_rays, len(self.ks), 4) + self.shape:
+                raise ValueError("field shape %s does not match the plan" % (tuple(t.shape),))
+        s = stack.c_struct()
+        _lib.check(self._lib.dtmmx_transfer_noalloc(
+            self._h, ctypes.byref(s), field_in.data_ptr(), field_out.data_ptr(),
+            status.data_ptr() if status is not None else None, stream.cuda_stream))
         return field_out
 
     def execute_bulk(self, stack, field_in, field_out, bulk, workspace=None, status=None,

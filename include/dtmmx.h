@@ -171,6 +171,21 @@ int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field
                    void* field_out, void* workspace, size_t workspace_bytes, int* status,
                    void* stream);
 
+/* dtmmx_transfer for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It has no counterpart in the reference.  The same kernels run with their
+ * workspace in a static device array of DTMMX_NOALLOC_SCRATCH_BYTES that the
+ * library holds on every GPU it runs on until the process ends; the call
+ * allocates nothing, and the caller passes only the plan, the stack, the
+ * fields and the stream.  It fails with DTMMX_E_ARG, before any work, when
+ * dtmmx_plan_workspace_size(plan) exceeds the array.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (concurrent
+ * streams or threads) race and must be serialised by the caller.  No kernel
+ * relies on the array's contents at the start of a call. */
+#define DTMMX_NOALLOC_SCRATCH_BYTES (128u << 20)
+int dtmmx_transfer_noalloc(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field_in,
+                           void* field_out, int* status, void* stream);
+
 /* dtmmx_transfer that also writes the field after every stack layer:
  * bulk is device memory [n_layers][n_rays][n_wavelengths][4][H][W] (field
  * layout, real space, components Ex, Hy, Ey, Hx).  Replaces the ret_bulk
