@@ -56,6 +56,7 @@ EXPORTS = (
     "dtmmx_director2data",
     "dtmmx_q2data",
     "dtmmx_transfer_noalloc",
+    "dtmmx_host_sincos",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -213,6 +214,9 @@ def load():
     lib.dtmmx_host_alphaf.restype = ctypes.c_int
     lib.dtmmx_host_rop.argtypes = [ctypes.c_int] + [_dp] * 7 + [ctypes.POINTER(ctypes.c_int)]
     lib.dtmmx_host_rop.restype = ctypes.c_int
+    _fp = ctypes.POINTER(ctypes.c_float)
+    lib.dtmmx_host_sincos.argtypes = [ctypes.c_int, _fp, _fp, _fp]
+    lib.dtmmx_host_sincos.restype = ctypes.c_int
     _ip = ctypes.POINTER(ctypes.c_int)
     lib.dtmmx_host_windows.argtypes = [ctypes.POINTER(PlanDesc), ctypes.c_int, _ip, _ip, _ip,
                                        _dp, _dp]

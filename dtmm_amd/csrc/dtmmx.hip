@@ -2810,6 +2810,12 @@ int dtmmx_host_rop_grad(int n, const double* beta, const double* phi, const doub
   return rc;
 }
 
+int dtmmx_host_sincos(int n, const float* x, float* s, float* c) {
+  if (n < 0 || (n > 0 && (!x || !s || !c))) return fail(DTMMX_E_ARG, "bad sincos arguments");
+  for (int i = 0; i < n; ++i) sincos_cw(x[i], s + i, c + i);
+  return DTMMX_OK;
+}
+
 int dtmmx_host_rop(int n, const double* beta, const double* phi, const double* eps_re,
                    const double* eps_im, const double* angles, double* alpha, double* proj,
                    int* ok) {

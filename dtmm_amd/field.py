@@ -129,10 +129,13 @@ def _fvec(fmat, jones, amplitude, mode=+1):
     return np.matmul(_normalize_f(fmat), a[..., None])[..., 0]
 
 
-def _projection_fft(field, ks, n, mode, betamax):
-    """Apply F diag(1 on forward (mode=+1) modes) F^-1 per FFT mode in a
-    medium of index n, zero for beta >= betamax (dtmm/diffract.py:107-119,
-    :254-299)."""
+def _projection_fft(field, ks, n, mode, betamax, d=0.):
+    """Apply F diag(exp(i k0 d alpha) on the modes of direction `mode`, 0 on
+    the others) F^-1 per FFT mode in a medium of index n, zero for
+    beta >= betamax: the field diffraction matrix of dtmm/diffract.py:107-119
+    (d = 0: the projection onto one direction), applied as in :254-299.
+    alpha = +sqrt(n^2 - beta^2) for the forward modes (columns 0, 2) and its
+    negative for the backward ones (columns 1, 3)."""
     shape = field.shape[-2:]
     beta, phi = betaphi(shape, ks)
     F = _iso_fmat(beta, phi, n)
@@ -141,12 +144,14 @@ def _projection_fft(field, ks, n, mode, betamax):
     Fi = np.zeros_like(F)
     Fi[ok] = np.linalg.inv(F[ok])
     F[mask] = 0.
-    d = np.zeros(4)
+    a = np.sqrt(n ** 2 - beta ** 2 + 0j)
+    kd = np.asarray(ks, FDTYPE)[:, None, None] * d
+    ph = np.zeros(beta.shape + (4,), np.complex128)
     if mode == +1:
-        d[0::2] = 1.
+        ph[..., 0] = ph[..., 2] = np.exp(1j * kd * a)
     else:
-        d[1::2] = 1.
-    P = np.matmul(F * d, Fi)
+        ph[..., 1] = ph[..., 3] = np.exp(-1j * kd * a)
+    P = np.matmul(F * ph[..., None, :], Fi)
     workers = len(os.sched_getaffinity(0))
     ff = scipy.fft.fft2(field, workers=workers)
     v = np.moveaxis(ff, -3, -1)[..., None]
@@ -213,9 +218,9 @@ def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensi
     norm = np.ones_like(i1)
     norm[:, ...] = (inten / nrays) ** 0.5
     if diffraction:
-        if focus != 0:
-            raise NotImplementedError("focus != 0 is not supported by this builder")
-        field = _projection_fft(field, wavenumbers, n, mode, betamax)
+        # the beams focus `focus` pixels into the sample: diffracted back by
+        # -focus in the cover medium (dtmm/field.py:703-711)
+        field = _projection_fft(field, wavenumbers, n, mode, betamax, d=-focus)
         i2 = field2intensity(field)
         ratio = (i1.sum((-2, -1)) / i2.sum((-2, -1))) ** 0.5
         norm = norm * ratio[..., None, None]

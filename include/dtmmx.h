@@ -402,6 +402,12 @@ int dtmmx_host_rop(int n, const double* beta, const double* phi, const double* e
                    const double* eps_im, const double* angles, double* alpha, double* proj,
                    int* ok);
 
+/* Host evaluation of the float sine and cosine the complex64 layer kernels
+ * use for every phase factor exp(i kd alpha) (dtmm/tmm.py:582-585 phase_mat
+ * computes them in the reference's precision): s[i], c[i] of x[i] for n
+ * values, for checks without a GPU. */
+int dtmmx_host_sincos(int n, const float* x, float* s, float* c);
+
 /* dtmmx_host_rop with the theta and phi derivatives (no counterpart in the
  * reference): dalpha[n][2][4] and dproj[n][2][2][2][2] ([direction][P0|P1])
  * as interleaved complex128; the routine the adjoint kernels run.

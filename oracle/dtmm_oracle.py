@@ -1781,16 +1781,17 @@ def _diffracted_field(field, ks, d=0., n=1., mode="t", betamax=BETAMAX):
 
 def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensity=1.,
                       n=None, focus=0., window=None, jones=None, diffraction=True,
-                      betamax=BETAMAX):
-    """Forward-propagating input field data (dtmm/field.py:586-713); without
-    a window the plane waves are snapped to FFT eigen-waves
-    (field.py:686-687)."""
+                      betamax=BETAMAX, backdir=False):
+    """Forward (backward with backdir) propagating input field data
+    (dtmm/field.py:586-713); without a window the plane waves are snapped to
+    FFT eigen-waves (field.py:686-687); with diffraction the field is
+    diffracted by -focus in the cover medium (:703-711)."""
     n = N_COVER if n is None else n
     wavelengths = np.asarray(wavelengths)
     wavenumbers = 2 * np.pi / wavelengths * pixelsize
     if jones is None:
         intensity = intensity / 2.
-    mode = +1
+    mode = -1 if backdir else +1
     _beta = np.asarray(beta, FDTYPE)
     _phi = np.asarray(phi, FDTYPE)
     _intensity = np.asarray(intensity, FDTYPE)

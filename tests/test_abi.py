@@ -439,3 +439,22 @@ def test_director2data_rejects_bad_arguments():
     for d, m, ls, nl, no, ne, nh, sf in bad:
         rc = lib.dtmmx_director2data(d, m, ls, nl, no, ne, nh, sf, p, p, p, None)
         assert rc == _lib.DTMMX_E_ARG, (ls, nl, sf)
+
+
+def test_float_sincos_of_the_layer_kernels():
+    """sincos_cw (the complex64 kernels' phase factors) against the exact
+    sine and cosine of the same float argument, over phases up to 1e6 rad
+    (thick layers): error below 1e-7."""
+    lib = _lib.load()
+    rng = np.random.default_rng(9)
+    x = np.concatenate([rng.uniform(-10, 10, 2000), rng.uniform(-1e6, 1e6, 4000),
+                        np.array([0., 5e4, -5e4, 6.5e4, 1e6, np.pi / 4, 3 * np.pi / 4])])
+    x = x.astype(np.float32)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    fp = ctypes.POINTER(ctypes.c_float)
+    _lib.check(lib.dtmmx_host_sincos(len(x), x.ctypes.data_as(fp), s.ctypes.data_as(fp),
+                                     c.ctypes.data_as(fp)))
+    xd = x.astype(np.float64)
+    assert np.abs(s - np.sin(xd)).max() < 1e-7
+    assert np.abs(c - np.cos(xd)).max() < 1e-7
