@@ -17,6 +17,7 @@
 // components only.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -210,6 +211,12 @@ struct dtmmx_plan {
   double* wcx = nullptr;  // [beam][nlam][wn][W]
   double* wcy = nullptr;  // [beam][nlam][wn][H]
   int* wmask = nullptr;   // [nlam][H]: min(amax, largest |kx| the windows pass)
+  // diffraction < 0 (exact per-mode propagation): per wavelength the k
+  // indices with beta < betamax in ascending beta, their counts, and the
+  // number of mode chunks one launch splits them into
+  int* full_modes = nullptr;   // [nlam][H*W]
+  int* full_nmodes = nullptr;  // [nlam]
+  int full_chunks = 0;
   // launch shapes (0: chosen from the grid): column lines per work-group,
   // row rays per work-group (dtmmx_plan_set_tiles, the autotuner's knobs)
   int col_lines = 0;
@@ -457,6 +464,7 @@ struct WsLayout {
   size_t e2;                       // 2x2: the 2-component field
   size_t part, sums;               // npass > 1: reduction partials and sums
   size_t etab;                     // per-pixel data of a layer with biaxial voxels
+  size_t full_spec, full_part;     // diffraction < 0: spectrum, per-chunk partial sums
   size_t total;
   int nblk;
 };
@@ -485,6 +493,10 @@ static WsLayout ws_layout(const dtmmx_plan* P) {
     w.win_acc = take(f4);
   }
   if (!m4) w.e2 = take(planes * 2 * HW * cb);
+  if (P->d.diffraction < 0) {
+    w.full_spec = take(f4);
+    w.full_part = take(f4 * size_t(P->full_chunks > 0 ? P->full_chunks : 1));
+  }
   w.etab = take(size_t(EIG_COMPS) * (P->d.n_rays / P->d.rays_per_beam) * HW * cb);
   return w;
 }
@@ -1367,11 +1379,71 @@ static int transfer_2x2_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const c
   return DTMMX_OK;
 }
 
+// 4x4 method with diffraction < 0, one pass (dtmm/transfer.py:719-725 in
+// real space, :800-803; propagate_4x4.py:515-564): every sub-step of every
+// stack layer (input and output media skipped) transforms the field,
+// propagates each FFT mode exactly at every pixel and sums the modes.  The
+// reference divides the sub-step thickness t / n by n again (:522), so a
+// sub-step propagates t / n^2.
+template <typename T>
+static int transfer_full(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
+                         cpx<T>* fout, void* ws, hipStream_t s) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const int nrays = P->d.n_rays, nlam = P->d.n_wavelengths;
+  const size_t n = size_t(nrays) * nlam * 4 * HW;
+  HIP_TRY(hipMemcpyAsync(fout, fin, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+  if (!ws) return fail(DTMMX_E_ARG, "diffraction < 0 needs a workspace");
+  const WsLayout wl = ws_layout(P);
+  cpx<T>* spec = (cpx<T>*)((char*)ws + wl.full_spec);
+  cpx<T>* part = (cpx<T>*)((char*)ws + wl.full_part);
+  const Batch b{nrays, nlam, 0, nlam};
+  const int ngroup = (nrays + FULL_RAYS - 1) / FULL_RAYS;
+  for (int l = 0; l < S->n_layers; ++l) {
+    const int ns = S->n_steps ? S->n_steps[l] : 1;
+    FullArgs<T> a{};
+    a.spec = spec;
+    a.part = part;
+    a.H = P->d.height;
+    a.W = P->d.width;
+    a.HW = HW;
+    a.nrays = nrays;
+    a.nlam = nlam;
+    const size_t off = size_t(l) * HW;
+    a.eps_re = S->eps_re + off;
+    a.eps_im = S->eps_im ? S->eps_im + off : nullptr;
+    a.ang = S->angles + off;
+    a.comp_stride = size_t(S->n_layers) * HW;
+    a.d = S->thickness[l] / double(ns) / double(ns);
+    a.modes = P->full_modes;
+    a.nmodes = P->full_nmodes;
+    a.nchunk = P->full_chunks;
+    a.bm = P->bm;
+    for (int st = 0; st < ns; ++st) {
+      HIP_TRY(hipMemcpyAsync(spec, fout, n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+      TRY(row_pass<T>(P, spec, b, 4, OP_FWD, s));
+      TRY(col_pass<T>(P, spec, b, 4, OP_FWD, s));
+      hipLaunchKernelGGL((full_kernel<T>),
+                         dim3(unsigned((HW + NT - 1) / NT), unsigned(nlam * ngroup),
+                              unsigned(P->full_chunks)),
+                         dim3(NT), 0, s, a);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL((chunk_sum_kernel<T>), dim3(unsigned((n + NT - 1) / NT)), dim3(NT), 0, s,
+                         (const cpx<T>*)part, fout, n, P->full_chunks);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  return DTMMX_OK;
+}
+
 template <typename T>
 static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
                          void* ws, int* status, hipStream_t s, void* bulk = nullptr) {
   const size_t HW = size_t(P->d.height) * P->d.width;
   const size_t n = size_t(P->d.n_rays) * P->d.n_wavelengths * 4 * HW;
+  if (P->d.diffraction < 0) {
+    if (bulk) return fail(DTMMX_E_UNSUPPORTED, "bulk output is not built for diffraction < 0");
+    return transfer_full<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s);
+  }
   if (P->d.npass > 1) return transfer_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s);
   if (P->d.method == DTMMX_METHOD_2X2 && P->d.reflection == 2)
     return transfer_2x2_refl2<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s, (cpx<T>*)bulk);
@@ -2054,6 +2126,49 @@ int dtmmx_abi_version(void) { return DTMMX_ABI_VERSION; }
 
 const char* dtmmx_last_error(void) { return g_err.c_str(); }
 
+// Mode lists of exact per-mode propagation (dtmm/propagate_4x4.py:530-546):
+// per wavelength the FFT modes with beta < betamax (strict), beta computed
+// as dtmm/wave.py:54-81 does, in ascending beta with ties in row-major
+// order (Python's stable sort); and the chunk count that gives a launch
+// about 2^18 threads.
+static int upload_full_modes(dtmmx_plan* P) {
+  const int H = P->d.height, W = P->d.width, nl = P->d.n_wavelengths;
+  const size_t HW = size_t(H) * W;
+  std::vector<int> modes(size_t(nl) * HW, 0), counts(nl, 0);
+  std::vector<std::pair<double, int>> b;
+  for (int l = 0; l < nl; ++l) {
+    b.clear();
+    const double k0 = std::fabs(P->d.k0[l]);
+    for (int ky = 0; ky < H; ++ky)
+      for (int kx = 0; kx < W; ++kx) {
+        const double fx = fftfreq(kx, W), fy = fftfreq(ky, H);
+        const double beta = (2 * M_PI / k0) * std::sqrt(fx * fx + fy * fy);
+        if (beta < P->d.betamax) b.push_back({beta, ky * W + kx});
+      }
+    std::stable_sort(b.begin(), b.end(),
+                     [](const std::pair<double, int>& u, const std::pair<double, int>& v) {
+                       return u.first < v.first;
+                     });
+    counts[l] = int(b.size());
+    for (size_t i = 0; i < b.size(); ++i) modes[size_t(l) * HW + i] = b[i].second;
+  }
+  const int ngroup = (P->d.n_rays + FULL_RAYS - 1) / FULL_RAYS;
+  const long long threads = (long long)HW * nl * ngroup;
+  long long ch = (262144 + threads - 1) / threads;
+  P->full_chunks = int(ch < 1 ? 1 : (ch > 32 ? 32 : ch));
+  hipError_t e = hipMalloc((void**)&P->full_modes, modes.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void**)&P->full_nmodes, counts.size() * sizeof(int));
+  if (e == hipSuccess)
+    e = hipMemcpy(P->full_modes, modes.data(), modes.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(P->full_nmodes, counts.data(), counts.size() * sizeof(int),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? DTMMX_E_NOMEM : DTMMX_E_HIP,
+                std::string("mode lists: ") + hipGetErrorString(e));
+  return DTMMX_OK;
+}
+
 static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->height <= 0 || d->width <= 0) return fail(DTMMX_E_ARG, "empty grid");
   if (!size_supported(d->height) || !size_supported(d->width))
@@ -2078,7 +2193,13 @@ static int validate_desc(const dtmmx_plan_desc* d) {
                         d->npass == 1));
   // 4x4 reflection mode 2 (any npass; diffraction > 1: tent windows)
   const bool refl2 = d->method == DTMMX_METHOD_4X4 && d->reflection == 2;
-  if (d->diffraction < 1 && !nodiff)
+  // diffraction < 0: exact per-mode propagation (4x4, one pass)
+  const bool full = d->diffraction < 0;
+  if (full && (d->method != DTMMX_METHOD_4X4 || d->npass != 1))
+    return fail(DTMMX_E_UNSUPPORTED,
+                "diffraction < 0 (exact per-mode propagation) is built for the 4x4 method "
+                "with npass 1");
+  if (d->diffraction < 1 && !nodiff && !full)
     return fail(DTMMX_E_UNSUPPORTED,
                 "diffraction must be >= 1, or 0 for the 4x4 method in reflection mode 2 and the "
                 "2x2 method in reflection mode 0 with npass 1");
@@ -2091,7 +2212,7 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->npass > 1 && d->norm != 0 && d->norm != DTMMX_NORM_FFT)
     return fail(DTMMX_E_UNSUPPORTED, "npass > 1: only norm 0 (total) and 1 (fft) are built");
   if (d->checkpoint < 0) return fail(DTMMX_E_ARG, "checkpoint interval must be >= 0");
-  if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !refl2)
+  if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !refl2 && !full)
     return fail(DTMMX_E_UNSUPPORTED,
                 "the 4x4 method is built for reflection modes 1 and 2");
   if (d->method == DTMMX_METHOD_2X2 && d->reflection != 0 && d->reflection != 1 &&
@@ -2157,6 +2278,13 @@ int dtmmx_plan_create(dtmmx_plan** out, const dtmmx_plan_desc* d) {
       return r;
     }
   }
+  if (d->diffraction < 0) {
+    const int r = upload_full_modes(P);
+    if (r != DTMMX_OK) {
+      dtmmx_plan_destroy(P);
+      return r;
+    }
+  }
   *out = P;
   return DTMMX_OK;
 }
@@ -2169,6 +2297,8 @@ int dtmmx_plan_destroy(dtmmx_plan* P) {
   if (P->wcx) (void)hipFree(P->wcx);
   if (P->wcy) (void)hipFree(P->wcy);
   if (P->wmask) (void)hipFree(P->wmask);
+  if (P->full_modes) (void)hipFree(P->full_modes);
+  if (P->full_nmodes) (void)hipFree(P->full_nmodes);
   delete P;
   return DTMMX_OK;
 }

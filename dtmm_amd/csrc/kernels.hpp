@@ -2416,4 +2416,112 @@ __global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Exact per-mode propagation (diffraction < 0 or inf;
+// dtmm/propagate_4x4.py:515-564): every FFT mode of the field is a plane
+// wave with its own (beta, phi), propagated at each pixel by that pixel's
+// F P(kd) F^-1 at the mode's (beta, phi), and the results are summed.
+// One thread per pixel, a chunk of the wavelength's mode list per grid z.
+// ---------------------------------------------------------------------------
+
+constexpr int FULL_RAYS = 2;  // rays per thread (they share the per-mode solve)
+
+template <typename T>
+struct FullArgs {
+  const cpx<T>* spec;   // unnormalised FFT of the field, [ray][lam][4][H*W]
+  cpx<T>* part;         // partial sums [chunk][ray][lam][4][H*W]
+  int H, W;
+  size_t HW;
+  int nrays, nlam;
+  const double* eps_re;  // this layer's material, [3][L*H*W] with the offset applied
+  const double* eps_im;
+  const double* ang;
+  size_t comp_stride;
+  double d;              // kd = k0 * d for one step
+  const int* modes;      // [lam][HW]: k indices (ky * W + kx) with beta < betamax, ascending beta
+  const int* nmodes;     // [lam]
+  int nchunk;
+  Beams bm;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void full_kernel(FullArgs<T> A) {
+  const size_t p = size_t(blockIdx.x) * NT + threadIdx.x;
+  const int ngroup = (A.nrays + FULL_RAYS - 1) / FULL_RAYS;
+  const int lam = blockIdx.y / ngroup;
+  const int r0 = (blockIdx.y % ngroup) * FULL_RAYS;
+  const int chunk = blockIdx.z;
+  if (p >= A.HW) return;
+  const int y = int(p / A.W), x = int(p - size_t(y) * A.W);
+  const size_t cs = A.comp_stride;
+  const cd e0 = mk<double>(A.eps_re[p], A.eps_im ? A.eps_im[p] : 0.);
+  const cd e1 = mk<double>(A.eps_re[cs + p], A.eps_im ? A.eps_im[cs + p] : 0.);
+  const cd e2 = mk<double>(A.eps_re[2 * cs + p], A.eps_im ? A.eps_im[2 * cs + p] : 0.);
+  const double psi = A.ang[p], theta = A.ang[cs + p], phi = A.ang[2 * cs + p];
+  const double k0 = A.bm.k0[lam];
+  const double kd = k0 * A.d;
+  const int nm = A.nmodes[lam];
+  const int m0 = int((long long)nm * chunk / A.nchunk);
+  const int m1 = int((long long)nm * (chunk + 1) / A.nchunk);
+  const double inv_hw = 1. / double(A.HW);
+  cd acc[FULL_RAYS][4];
+  for (int r = 0; r < FULL_RAYS; ++r)
+    for (int c = 0; c < 4; ++c) acc[r][c] = mk<double>(0.);
+  for (int m = m0; m < m1; ++m) {
+    const int k = A.modes[size_t(lam) * A.HW + m];
+    const int ky = k / A.W, kx = k - ky * A.W;
+    const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
+    const double beta = (2 * M_PI / fabs(k0)) * sqrt(fx * fx + fy * fy);
+    const double phim = atan2(fy, fx);
+    Modes md;
+    int kind;
+    layer_modes(beta, cos(phim), sin(phim), phim, e0, e1, e2, psi, theta, phi, md, kind);
+    cd Fi[4][4];
+    if (!inv4(md.F, Fi))
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) Fi[i][j] = mk<double>(0.);
+    cd pm[4];
+    for (int i = 0; i < 4; ++i) pm[i] = cexpi(md.a[i] * kd);
+    // the mode's plane wave at this pixel, exp(2 pi i (ky y / H + kx x / W)) / HW
+    const long long ny = (long long)ky * y % A.H, nx = (long long)kx * x % A.W;
+    const double arg = 2 * M_PI * (double(ny) / A.H + double(nx) / A.W);
+    double sw, cw;
+    sincos(arg, &sw, &cw);
+    const cd wave = mk<double>(cw * inv_hw, sw * inv_hw);
+    for (int r = 0; r < FULL_RAYS; ++r) {
+      if (r0 + r >= A.nrays) break;
+      const cpx<T>* sp = A.spec + (size_t(r0 + r) * A.nlam + lam) * 4 * A.HW + k;
+      cd v[4], u[4];
+      for (int c = 0; c < 4; ++c) v[c] = cast<double>(sp[c * A.HW]) * wave;
+      for (int i = 0; i < 4; ++i) {  // u = P F^-1 v
+        cd t = mk<double>(0.);
+        for (int j = 0; j < 4; ++j) t = t + Fi[i][j] * v[j];
+        u[i] = t * pm[i];
+      }
+      for (int i = 0; i < 4; ++i) {
+        cd t = mk<double>(0.);
+        for (int j = 0; j < 4; ++j) t = t + md.F[i][j] * u[j];
+        acc[r][i] = acc[r][i] + t;
+      }
+    }
+  }
+  for (int r = 0; r < FULL_RAYS; ++r) {
+    if (r0 + r >= A.nrays) break;
+    cpx<T>* o = A.part + ((size_t(chunk) * A.nrays + r0 + r) * A.nlam + lam) * 4 * A.HW + p;
+    for (int c = 0; c < 4; ++c) o[c * A.HW] = cast<T>(acc[r][c]);
+  }
+}
+
+// dst[i] = sum over chunks of part[chunk][i], chunks in order (deterministic).
+template <typename T>
+__global__ __launch_bounds__(NT) void chunk_sum_kernel(const cpx<T>* part, cpx<T>* dst, size_t n,
+                                                       int nchunk) {
+  const size_t i = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (i >= n) return;
+  cpx<T> s = part[i];
+  for (int c = 1; c < nchunk; ++c) s = s + part[size_t(c) * n + i];
+  dst[i] = s;
+}
+
 }  // namespace dtmmx

@@ -440,6 +440,12 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
         raise NotImplementedError("ret_bulk is supported with npass=1 and diffraction 0 or 1")
     if eff_data not in (None, 0, "isotropic"):
         raise NotImplementedError("only isotropic effective data (eff_data=0) is supported")
+    if diffraction < 0 or diffraction == np.inf:
+        # exact per-mode propagation (dtmm/propagate_4x4.py:515-564)
+        if kind != "4x4" or npass != 1 or ret_bulk:
+            raise NotImplementedError("diffraction < 0 or inf (exact per-mode propagation) is "
+                                      "supported for the 4x4 method with npass=1 and no bulk")
+        return kind, reflection
     if diffraction == 0:
         # no diffraction: every pixel is an independent stack (4x4 reflection
         # mode 2, the reference's default for diffraction 0; 2x2 reflection 0)
@@ -534,8 +540,10 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
         raise ValueError("rays do not divide into beams")
     rays_per_beam = n_rays // len(b)
 
+    # exact per-mode propagation travels as diffraction -1 through the ABI
+    ndiff = -1 if (diffraction < 0 or diffraction == np.inf) else int(diffraction)
     plan = Plan(shape, ks, b, p, n_rays, rays_per_beam, kind, precision, betamax,
-                diffraction=int(diffraction), npass=int(npass), norm=norm, smooth=smooth,
+                diffraction=ndiff, npass=int(npass), norm=norm, smooth=smooth,
                 reflection=reflection, checkpoint=checkpoint)
     if isinstance(optical_data, Stack):
         # a stack already on the device (Stack.from_director): its own layers

@@ -68,7 +68,11 @@ typedef struct {
   const double* k0;   /* [n_wavelengths] host: 2 pi pixelsize / lambda  */
   const double* beta; /* [n_rays / rays_per_beam] host                  */
   const double* phi;  /* [n_rays / rays_per_beam] host                  */
-  int diffraction;    /* 0 (npass 1; 4x4 reflection 2, 2x2 reflection 0):
+  int diffraction;    /* < 0 (4x4, npass 1): exact per-mode propagation
+                         (propagate_4x4.py:515-564; the reference's -1 or
+                         inf), every FFT mode with beta < betamax
+                         propagated at each pixel with its own beam;
+                         0 (npass 1; 4x4 reflection 2, 2x2 reflection 0):
                          no diffraction, per-pixel stacks in real space;
                          1: one mean beam per ray (propagate_4x4.py:382);
                          n > 1 (4x4 only): the k-space field is split by

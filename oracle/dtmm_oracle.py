@@ -63,6 +63,7 @@ __all__ = [
     "transmit4x4", "transfer4x4", "Eti_mat", "corrected_E_diffraction_matrix",
     "tukey", "aperture", "illumination_aperture", "aperture2rays", "illumination_rays",
     "cauchy2eps", "sellmeier2eps", "is_optical_data_dispersive", "mean_betaphi",
+    "propagate_4x4_full",
     "field2betaphi",
 ]
 
@@ -1241,6 +1242,41 @@ def propagate_4x4_effective_1(field, ks, layer, effective_layer, beta=0, phi=0, 
     return fout
 
 
+def propagate_4x4_full(field, ks, layer, nsteps=1, betamax=BETAMAX):
+    """Exact per-mode propagation through one layer (dtmm/propagate_4x4.py:
+    515-564): for every step and wavelength the real-space field (..., 4, H,
+    W) is split into its FFT modes with beta < betamax (strict, :537), each
+    mode's plane wave is multiplied at every pixel by F P(kd) F^-1 of that
+    pixel's material at the mode's own (beta, phi), and the results are
+    summed in ascending order of beta (stable, :546).  kd = k0 d / nsteps
+    with d the thickness in `layer` (:522); transfer_4x4 passes the
+    sub-step thickness t / nsteps there, so each of the nsteps steps
+    propagates t / nsteps^2 (restated as the reference computes it)."""
+    d, epsv, epsa = layer
+    ks = np.asarray(ks, FDTYPE)
+    shape = field.shape[-2:]
+    kd = ks * d / nsteps
+    ii, jj = np.meshgrid(range(shape[0]), range(shape[1]), indexing="ij")
+    out = np.array(field, CDTYPE)
+    for _ in range(nsteps):
+        src = out
+        out = np.empty_like(src)
+        for i in range(len(ks)):
+            ff = np.fft.fft2(src[..., i, :, :, :])
+            acc = np.zeros_like(src[..., i, :, :, :])
+            b, p = betaphi(shape, ks[i])
+            mask = b < betamax
+            amp = ff[..., mask]
+            betas, phis, iind, jind = b[mask], p[mask], ii[mask], jj[mask]
+            for j in sorted(range(len(betas)), key=lambda q: betas[q]):
+                alpha, f, fi = alphaffi(betas[j], phis[j], epsv, epsa)
+                pm = np.exp(1j * kd[i] * alpha)
+                w = eigenwave(amp.shape[:-1] + shape, iind[j], jind[j], amplitude=amp[..., j])
+                acc += dotmdmf(f, pm, fi, w)
+            out[..., i, :, :, :] = acc
+    return out
+
+
 def S_mat(fmatin, fmatout, fmatini=None, mode=+1):
     """Blocks of F_in^-1 F_out coupling forward output modes to the input
     modes of the same / opposite direction (dtmm/tmm.py:850-878)."""
@@ -1484,6 +1520,19 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     """
     if ret_bulk and npass != 1:
         raise NotImplementedError("oracle covers ret_bulk with npass=1")
+    if diffraction < 0 or diffraction == np.inf:
+        # exact per-mode propagation of every layer in real space
+        # (dtmm/transfer.py:719-725 work_in_fft False, :782-803), npass 1
+        if npass != 1 or ret_bulk:
+            raise NotImplementedError("oracle covers full diffraction with npass 1")
+        field_in, wavelengths, pixelsize = field_data
+        field = np.array(field_in, CDTYPE)
+        layers, _ = create_layers(optical_data, nin, nout, nstep)
+        ks = k0(wavelengths, pixelsize)
+        for j in range(1, len(layers) - 1):
+            nsub, layer = layers[j]
+            field = propagate_4x4_full(field, ks, layer, nsub, betamax)
+        return field, wavelengths, pixelsize
     if reflection == 2 and diffraction >= 0 and npass == 1:
         # dtmm/transfer.py:694-700, :719-725 (work_in_fft False), :782-795
         # (propagate_4x4_effective_2), :874: the input is neither projected

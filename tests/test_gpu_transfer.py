@@ -704,3 +704,19 @@ def test_ret_bulk(method, reflection, diffraction, precision):
         err = rel_err(out[j], ref[j])
         flush("  slot %d rel err %.3e" % (j, err))
         assert err < TOL[precision], (j, err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("diffraction,nstep,biax", [(-1, 1, False), (np.inf, 2, True)])
+def test_full_diffraction(diffraction, nstep, biax, precision):
+    """diffraction < 0 or inf: exact per-mode propagation of every FFT mode
+    with its own beam at every pixel (dtmm/propagate_4x4.py:515-564), both
+    polarisations, two wavelengths, sub-steps, and biaxial voxels."""
+    flush("case full diffraction %s nstep=%d biaxial=%s %s" % (diffraction, nstep, biax,
+                                                                 precision))
+    if biax:
+        fd, optical = _biaxial_case(shape=(3, 16, 24), wavelengths=(480., 620.), jones=None)
+    else:
+        fd, optical = random_case(shape=(3, 16, 24), wavelengths=(480., 620.), jones=None)
+    _compare(fd, optical, "4x4", precision, diffraction=diffraction, nstep=nstep)
