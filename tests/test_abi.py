@@ -347,8 +347,13 @@ def test_mode_resolution_no_diffraction():
         _resolve_modes("4x4", 1, 0, 1, None, False)
     assert _resolve_modes("4x4", 1, 1, 2, None, False) == ("4x4", 2)
     assert _resolve_modes("4x4", 1, 3, 2, None, False) == ("4x4", 2)
+    # diffraction inf or < 0: exact per-mode propagation, 4x4 with one pass
+    assert _resolve_modes("4x4", 1, np.inf, 2, None, False) == ("4x4", 2)
+    assert _resolve_modes("4x4", 1, -1, None, None, False) == ("4x4", 1)
     with pytest.raises(NotImplementedError):
-        _resolve_modes("4x4", 1, np.inf, 2, None, False)
+        _resolve_modes("4x4", 3, np.inf, None, None, False)
+    with pytest.raises(NotImplementedError):
+        _resolve_modes("2x2", 1, -1, None, None, False)
     assert _resolve_modes("4x4", 2, 1, 2, None, False) == ("4x4", 2)
     assert _resolve_modes("4x4", 3, 0, 2, None, False) == ("4x4", 2)
     with pytest.raises(NotImplementedError):
