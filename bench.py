@@ -414,39 +414,81 @@ def build_inputs_c5(size, layers, nlam):
     return fd, optical
 
 
-def cpu_baseline_c5(fd, optical, crop, target_s=15.0):
-    """Oracle on a crop x crop window of the same workload, npass = 1, the
-    marginal time of a second layer (windows, all planes)."""
+def _layer_times(run_once, nlayers):
+    """Per-layer wall times of one oracle transfer through `nlayers` layers,
+    from its layer callbacks: the first layer (which also builds the cached
+    k-space matrices) is the warm-up, the later nlayers - 1 are the samples."""
+    stamps = [time.perf_counter()]
+    run_once(lambda j: stamps.append(time.perf_counter()))
+    return np.diff(stamps)[1:nlayers]
+
+
+def _plane_groups(n_rays, nlam, ngroups):
+    """(ray, first, last) wavelength ranges splitting the planes into about
+    ngroups contiguous groups."""
+    per_ray = max(1, min(nlam, ngroups // max(n_rays, 1)))
+    groups = []
+    for r in range(n_rays):
+        edges = np.linspace(0, nlam, per_ray + 1).astype(int)
+        groups += [(r, a, b) for a, b in zip(edges[:-1], edges[1:]) if b > a]
+    return groups
+
+
+def cpu_rates(field, wl, ps, opt, nlayers, kw):
+    """Oracle rates (voxel*lambda steps/s) on the host.  Primary mode, as the
+    reference runs by default: per-pixel math on one thread, scipy.fft on
+    every core; the median of the per-layer times after a warm-up layer.
+    All-cores mode: the planes split over one thread per core (NumPy's
+    per-pixel loops release the GIL), FFTs single-threaded in each; the sum
+    of the groups' rates from the median of their per-layer times."""
     import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    cores = len(os.sched_getaffinity(0))
+    H, W = field.shape[-2:]
+    lead = field.shape[:-4]
+    n_rays = int(np.prod(lead)) if lead else 1
+    nlam = len(wl)
+    f = field.reshape((n_rays, nlam, 4, H, W))
+
+    def transfer(ff, ww, cb):
+        oracle.transfer_field((np.array(ff), ww, ps), opt, layer_callback=cb, **kw)
+
+    oracle.set_fft_workers(cores)
+    dt = _layer_times(lambda cb: transfer(f, wl, cb), nlayers)
+    primary = H * W * n_rays * nlam / float(np.median(dt))
+    groups = _plane_groups(n_rays, nlam, cores)
+    oracle.set_fft_workers(1)
+    try:
+        with ThreadPoolExecutor(len(groups)) as ex:
+            res = list(ex.map(lambda g: (g, _layer_times(
+                lambda cb: transfer(f[g[0], g[1]:g[2]], wl[g[1]:g[2]], cb), nlayers)), groups))
+    finally:
+        oracle.set_fft_workers(cores)
+    allc = sum(H * W * (b - a) / float(np.median(t)) for (r, a, b), t in res)
+    return primary, allc, dt, cores, len(groups)
+
+
+def cpu_baseline_c5(fd, optical, crop, nlayers=4):
+    """Oracle on a crop x crop window of the same workload (windows, all
+    planes), npass = 1, from mid-stack layers."""
     field, wl, ps = fd
     d, eps, ang = optical[0]
     H, W = field.shape[-2:]
     y0, x0 = (H - crop) // 2, (W - crop) // 2
     f = np.ascontiguousarray(field[..., y0:y0 + crop, x0:x0 + crop])
-    e = eps[:, y0:y0 + crop, x0:x0 + crop]
-    a = ang[:, y0:y0 + crop, x0:x0 + crop]
     z0 = len(d) // 2
-    cores = len(os.sched_getaffinity(0))
-    oracle.set_fft_workers(cores)
+    opt = [(d[z0:z0 + nlayers], eps[z0:z0 + nlayers, y0:y0 + crop, x0:x0 + crop],
+            ang[z0:z0 + nlayers, y0:y0 + crop, x0:x0 + crop])]
     beta, phi = C5_BEAMS
-
-    def run(n):
-        t = time.time()
-        oracle.transfer_field((f.copy(), wl, ps), [(d[z0:z0 + n], e[z0:z0 + n], a[z0:z0 + n])],
-                              beta=beta, phi=phi, method="4x4", npass=1, diffraction=5,
-                              nin=1.52, nout=1.52)
-        return time.time() - t
-
-    t1 = run(1)
-    t2 = run(2)
-    per_layer = max(t2 - t1, 1e-9)
+    kw = dict(beta=beta, phi=phi, method="4x4", npass=1, diffraction=5, nin=1.52, nout=1.52)
+    primary, allc, dt, cores, ngroups = cpu_rates(f, wl, ps, opt, nlayers, kw)
     planes = int(np.prod(f.shape[:-3]))
-    return {"value": crop * crop * planes / per_layer, "unit": "voxel*lambda steps/s",
-            "cores": cores, "kind": "port",
-            "sample": "oracle 4x4 diffraction=5 transfer of 1 and 2 mid-stack layers of a "
-                      "%dx%d crop, all %d planes, npass=1 (%.1f s and %.1f s; rate from the "
-                      "extra layer); per-pixel math 1 thread, scipy.fft on %d threads"
-                      % (crop, crop, planes, t1, t2, cores)}
+    return {"value": primary, "unit": "voxel*lambda steps/s", "cores": cores, "kind": "port",
+            "sample": "oracle 4x4 diffraction=5 transfer through %d mid-stack layers of a %dx%d "
+                      "crop, all %d planes, npass=1; median of the layer times after the first "
+                      "(%s s); per-pixel math 1 thread, scipy.fft on %d threads"
+                      % (nlayers, crop, crop, planes, ", ".join("%.2f" % t for t in dt), cores),
+            "all_cores_value": allc, "all_cores_threads": ngroups}
 
 
 def build_inputs(size, layers, nlam, seed=0):
@@ -460,36 +502,23 @@ def build_inputs(size, layers, nlam, seed=0):
     return fd, optical
 
 
-def cpu_baseline(fd, optical, method, target_s=15.0):
-    """Oracle (complex128 NumPy restatement) on the first n layers of the
-    same workload: per-pixel math on one thread, FFTs on all cores."""
-    import oracle
+def cpu_baseline(fd, optical, method, nlayers=4):
+    """Oracle (complex128 NumPy restatement) through the first `nlayers`
+    layers of the same workload, all planes (cpu_rates)."""
     field, wl, ps = fd
     d, eps, ang = optical[0]
-    cores = len(os.sched_getaffinity(0))
-    oracle.set_fft_workers(cores)
-
-    def run(n):
-        t = time.time()
-        oracle.transfer_field((field, wl, ps), [(d[:n], eps[:n], ang[:n])], beta=0., phi=0.,
-                              method=method)
-        return time.time() - t
-
-    # the first layer also builds the k-space matrices (cached by the
-    # reference as well), so the rate is the marginal time of further layers
-    t1 = run(1)
-    n = 2 if t1 > target_s / 3 else int(max(2, min(len(d), 1 + target_s / max(t1, 1e-3))))
-    tn = run(n)
-    per_layer = max(tn - t1, 1e-9) / (n - 1)
     H, W = field.shape[-2:]
+    opt = [(d[:nlayers], eps[:nlayers], ang[:nlayers])]
+    primary, allc, dt, cores, ngroups = cpu_rates(field, wl, ps, opt, nlayers,
+                                                  dict(beta=0., phi=0., method=method))
     planes = int(np.prod(field.shape[:-3]))
-    steps = H * W * planes
-    return {"value": steps / per_layer, "unit": "voxel*lambda steps/s", "cores": cores,
-            "kind": "port",
-            "sample": "oracle %s transfer of the first 1 and %d of %d layers, all %d planes of "
-                      "%dx%d (%.1f s and %.1f s; rate from the %d extra layers); per-pixel "
-                      "math 1 thread, scipy.fft on %d threads"
-                      % (method, n, len(d), planes, H, W, t1, tn, n - 1, cores)}
+    return {"value": primary, "unit": "voxel*lambda steps/s", "cores": cores, "kind": "port",
+            "sample": "oracle %s transfer through the first %d of %d layers, all %d planes of "
+                      "%dx%d; median of the layer times after the first (%s s); per-pixel math "
+                      "1 thread, scipy.fft on %d threads"
+                      % (method, nlayers, len(d), planes, H, W,
+                         ", ".join("%.2f" % t for t in dt), cores),
+            "all_cores_value": allc, "all_cores_threads": ngroups}
 
 
 def main():
@@ -511,8 +540,11 @@ def main():
     ap.add_argument("--layers", type=int, default=128)
     ap.add_argument("--wavelengths", type=int, default=9)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-2x2", action="store_true",
+                    help="skip the 2x2 half of the default workload")
     ap.add_argument("--no-ref", action="store_true", help="skip the c128 comparison run")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-layers", type=int, default=4,
+                    help="oracle layers of the CPU baseline (the first is the warm-up)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the automatic launch shapes instead of timing the candidates "
                          "(dtmm_amd.tune) before the warm-up")
@@ -541,9 +573,6 @@ def main():
         sys.exit(2)
 
     import torch
-    import dtmm_amd as dm
-    from dtmm_amd import _lib
-    from dtmm_amd.transfer import Plan, Stack
 
     if not torch.cuda.is_available():
         log("no GPU")
@@ -560,7 +589,6 @@ def main():
     if args.workload == "tmm1d":
         run_tmm1d(args)
         return
-    lib = _lib.load()
     c5 = args.workload == "c5"
     if c5:
         if args.method != "4x4":
@@ -570,10 +598,32 @@ def main():
         layers = args.layers if args.layers != 128 else 200
         nlam = args.wavelengths if args.wavelengths != 9 else 5
         fd, optical = build_inputs_c5(size, layers, nlam)
+    else:
+        fd, optical = build_inputs(args.size, args.layers, args.wavelengths)
+    rec = measure(args, args.method, fd, optical, c5)
+    if not c5 and args.method == "4x4" and not args.no_2x2:
+        # the 2x2 half of the same workload (the plain method's other path)
+        r2 = measure(args, "2x2", fd, optical, False)
+        rec.update({"value_2x2": r2["value"], "ms_per_step_2x2": r2["ms_per_step"],
+                    "roofline_2x2": r2["roofline"], "cpu_baseline_2x2": r2["cpu_baseline"]})
+        for k in ("c128_value", "c64_over_c128", "rel_field_err_vs_c128",
+                  "rel_intensity_err_vs_c128"):
+            if k in r2:
+                rec[k + "_2x2"] = r2[k]
+    print(json.dumps(rec), flush=True)
+
+
+def measure(args, method, fd, optical, c5):
+    """One bench line's measurements of `method` on the workload."""
+    import torch
+    import dtmm_amd as dm
+    from dtmm_amd import _lib
+    from dtmm_amd.transfer import Plan, Stack
+    lib = _lib.load()
+    if c5:
         beams, phis = C5_BEAMS
         npass, diffraction, nin, nout = 3, 5, 1.52, 1.52
     else:
-        fd, optical = build_inputs(args.size, args.layers, args.wavelengths)
         beams, phis = [0.], [0.]
         npass, diffraction, nin, nout = 1, 1, 1.5, 1.5
     field, wl, ps = fd
@@ -587,10 +637,10 @@ def main():
     t = time.time()
     stack = Stack(optical, (H, W), nstep=1, nin=nin, nout=nout)
     L = stack.n_layers
-    log("stack uploaded in %.1f s (%d layers)" % (time.time() - t, L))
+    log("%s: stack uploaded in %.1f s (%d layers)" % (method, time.time() - t, L))
 
     def make(precision):
-        plan = Plan((H, W), ks, beams, phis, n_rays, rays_per_beam, args.method, precision, 0.8,
+        plan = Plan((H, W), ks, beams, phis, n_rays, rays_per_beam, method, precision, 0.8,
                     diffraction=diffraction, npass=npass)
         host = np.ascontiguousarray(field.reshape((n_rays, nlam, 4, H, W)))
         fin = torch.from_numpy(host).to(device="cuda", dtype=plan.dtype).contiguous()
@@ -607,8 +657,8 @@ def main():
         t = time.time()
         res = tune.tune_plan(plan, stack, fin, fout, ws, repeats=2, cache=False)
         tiles = {"col_lines": res["tiles"][0], "row_rays": res["tiles"][1]}
-        log("tuned launch shapes in %.1f s: %s (ms per pass %s)"
-            % (time.time() - t, res["tiles"],
+        log("%s: tuned launch shapes in %.1f s: %s (ms per pass %s)"
+            % (method, time.time() - t, res["tiles"],
                ", ".join("%s %.2f" % (k, v) for k, v in sorted(res["ms"].items()))))
     # npass > 1 overwrites the input with the input-side field: every step
     # starts from a fresh copy of the illumination
@@ -665,7 +715,7 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     steps = H * W * L * planes * npass
     value = steps / (ms_per_step / 1e3)
-    nc = 4 if args.method == "4x4" else 2
+    nc = 4 if method == "4x4" else 2
     cb = 8 if args.precision == "c64" else 16
     out_np = fout.cpu().numpy()
     if not np.all(np.isfinite(out_np)):
@@ -690,8 +740,11 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["ms"])
     avg_ms = kern[dom]["ms"] / max(kern[dom]["n"], 1)
     achieved = kern[dom]["bytes"] / (avg_ms / 1e3) / 1e9
-    log("kernels: " + ", ".join("%s %d x %.3f ms" % (k, v["n"], v["ms"] / max(v["n"], 1))
-                                 for k, v in kern.items())
+    log("%s kernels: " % method
+        + ", ".join("%s %d x %.3f ms (%.2f of peak)"
+                    % (k, v["n"], v["ms"] / max(v["n"], 1),
+                       v["bytes"] / max(v["ms"] / max(v["n"], 1), 1e-12) / 1e6 / HBM_PEAK_GBS)
+                    for k, v in kern.items())
         + ", kspace %d x %.3f ms" % (cnt[3], ms[3] / max(cnt[3], 1)))
     # launches per pass: row L (or L * windows), column L - 1 (or L window
     # splits), dense k-space 2
@@ -717,21 +770,23 @@ def main():
         extra = {"c128_value": steps / (t3 - t2), "c64_over_c128": (t3 - t2) * 1e3 / ms_per_step,
                  "rel_field_err_vs_c128": err, "rel_intensity_err_vs_c128": ierr}
         del plan2, fin2, fout2, ws2
-        log("c128 pass %.1f ms, c64 vs c128 rel field err %.2e" % ((t3 - t2) * 1e3, err))
+        log("%s: c128 pass %.1f ms, c64 vs c128 rel field err %.2e"
+            % (method, (t3 - t2) * 1e3, err))
 
     cpu = None
     if not args.no_cpu:
         if c5:
-            cpu = cpu_baseline_c5(fd, optical, args.cpu_crop, args.cpu_seconds)
+            cpu = cpu_baseline_c5(fd, optical, args.cpu_crop, args.cpu_layers)
         else:
-            cpu = cpu_baseline(fd, optical, args.method, args.cpu_seconds)
-        log("cpu baseline: %s" % cpu["sample"])
+            cpu = cpu_baseline(fd, optical, method, args.cpu_layers)
+        log("%s cpu baseline: %.3g (all cores %.3g): %s"
+            % (method, cpu["value"], cpu["all_cores_value"], cpu["sample"]))
     if c5:
         workload = ("%dx%dx%d 3x3 radial droplet grid, %d wavelengths x 3 beams x 2 pol, 4x4, "
                     "diffraction=5 (%d windows), npass=3, nin=nout=1.52" % (H, W, L, nlam, nwin))
     else:
         workload = ("%dx%dx%d random director, %d wavelengths x 2 pol, %s, diffraction=1, "
-                    "npass=1" % (H, W, L, nlam, args.method))
+                    "npass=1" % (H, W, L, nlam, method))
 
     rec = {
         "metric": "voxel*lambda steps/s (transfer_field, 4x4 unless noted), c64 with c128 "
@@ -749,19 +804,19 @@ def main():
         "data": "synthetic (seeded random nematic director, plane-wave illumination)",
         "config": {"workload": workload,
                    "height": H, "width": W, "layers": L, "wavelengths": nlam, "planes": planes,
-                   "method": args.method, "precision": args.precision, "npass": npass,
+                   "method": method, "precision": args.precision, "npass": npass,
                    "diffraction": diffraction, "launch_shapes": tiles or "automatic"},
         "hbm_frac_per_step": (steps * nwin * 2 * nc * cb / (ms_per_step / 1e3) / 1e9)
         / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((dom, H, W, planes, args.method,
+                     "traffic": MEASURED_TRAFFIC.get((dom, H, W, planes, method,
                                                       args.precision)),
                      "avg_launch_ms": avg_ms, "bytes_per_launch": kern[dom]["bytes"]},
         "cpu_baseline": cpu,
     }
     rec.update(extra)
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 if __name__ == "__main__":
