@@ -16,6 +16,7 @@ from .field import (illumination_data, field2intensity, blackman, k0, betaphi,  
                     illumination_rays)
 from .transfer import (transfer_field, transfer_batch, TransferGraph, Plan, Stack,  # noqa: F401
                        DTMM_NORM_FFT, DTMM_NORM_REF, field2betaphi, mean_betaphi)
+from .propagate import propagate_4x4_effective_1, propagate_2x2_effective_1  # noqa: F401
 from .adjoint import AdjointTransfer, polarizer4x4, analyser_intensity_loss  # noqa: F401
 from .pom import calculate_pom_specter, field2specter, specter2color  # noqa: F401
 from . import pom  # noqa: F401

@@ -57,6 +57,7 @@ EXPORTS = (
     "dtmmx_q2data",
     "dtmmx_transfer_noalloc",
     "dtmmx_host_sincos",
+    "dtmmx_propagate_layer",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -190,6 +191,11 @@ def load():
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_void_p]
     lib.dtmmx_transfer_noalloc.restype = ctypes.c_int
+    lib.dtmmx_propagate_layer.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stack),
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.c_void_p, ctypes.c_void_p]
+    lib.dtmmx_propagate_layer.restype = ctypes.c_int
     lib.dtmmx_transfer_batch.argtypes = [ctypes.POINTER(Job), ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_void_p]
     lib.dtmmx_transfer_batch.restype = ctypes.c_int

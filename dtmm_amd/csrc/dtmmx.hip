@@ -1379,6 +1379,81 @@ static int transfer_2x2_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const c
   return DTMMX_OK;
 }
 
+// One layer step on a k-space field (the layer-level functions
+// dtmm/propagate_4x4.py:376-389 propagate_4x4_effective_1 with :80-101 and
+// dtmm/propagate_2x2.py:193-211 propagate_2x2_effective_1 with :19-76, mode
+// +1, diffraction 1): every stack layer with its sub-steps, the field in
+// and out in the reference's k-space convention (numpy fft2 of the
+// real-space field).
+//   4x4: D1 (closed form, first corrected Epn half step), then per sub-step
+//        the inverse transform, per-pixel operator and forward transform,
+//        the full diagonal step between sub-steps, and D2 at the end.
+//   2x2: the Fresnel transmission of the tangential E field from the
+//        isotropic medium e_in (reflection mode 1; e_in equal to the
+//        layer's effective medium gives 1) with the first E half step,
+//        then the same sub-steps, and the second E half step.
+template <typename T>
+static int propagate_layer_k(const dtmmx_plan* P, const dtmmx_stack* S, const cpx<T>* fin,
+                             cpx<T>* fout, cd e_in, void* ws, int* status, hipStream_t s) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const bool m4 = P->d.method == DTMMX_METHOD_4X4;
+  const int nc = m4 ? 4 : 2;
+  const Batch b{P->d.n_rays, P->d.n_wavelengths, 0, P->d.n_wavelengths};
+  const std::vector<Step> st = expand_steps(S, 1);
+  const int L = int(st.size());
+  const double scale = 1.0 / double(HW);
+  const WsLayout wl = ws_layout(P);
+  void* etab = ws ? (char*)ws + wl.etab : nullptr;
+  KArgs<T> h{};
+  if (m4) {
+    h.kop = KOP_HEAD4;
+    h.d0 = st[0].d;
+    h.e0 = st[0].e;
+  } else {
+    h.kop = KOP_TR2;
+    h.d0 = 0.;
+    h.e0 = P->d.reflection ? e_in : st[0].e;
+    h.d1 = st[0].d;
+    h.e1 = st[0].e;
+  }
+  h.scale = scale;
+  TRY(kspace<T>(P, fin, fout, b, h, s));
+  TRY(col_pass<T>(P, fout, b, nc, OP_INV, s));
+  for (int l = 0; l < L; ++l) {
+    const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, m4 ? 1 : 0, status, etab);
+    TRY(row_pass<T>(P, fout, b, nc, OP_ROP, s, &la));
+    if (l + 1 < L) {
+      if (!eq(st[l].e, st[l + 1].e))
+        return fail(DTMMX_E_UNSUPPORTED, "the layer step takes layers of one effective medium");
+      ColArgs<T> ca{};
+      ca.da = st[l].d;
+      ca.db = st[l + 1].d;
+      ca.ea = st[l].e;
+      ca.eb = st[l + 1].e;
+      ca.kca = 1.;
+      ca.kcb = 1.;
+      ca.scale = scale;
+      ca.dense_sign = m4 ? 1 : 0;
+      TRY(col_pass<T>(P, fout, b, nc, OP_DIAG, s, &ca));
+    }
+  }
+  TRY(col_pass<T>(P, fout, b, nc, OP_FWD, s));
+  KArgs<T> t{};
+  if (m4) {
+    t.kop = KOP_TAIL4;
+    t.d0 = st[L - 1].d;
+    t.e0 = st[L - 1].e;
+  } else {
+    t.kop = KOP_TR2;
+    t.d0 = st[L - 1].d;
+    t.e0 = st[L - 1].e;
+    t.d1 = 0.;
+    t.e1 = st[L - 1].e;
+  }
+  t.scale = 1.;
+  return kspace<T>(P, fout, fout, b, t, s);
+}
+
 // 4x4 method with diffraction < 0, one pass (dtmm/transfer.py:719-725 in
 // real space, :800-803; propagate_4x4.py:515-564): every sub-step of every
 // stack layer (input and output media skipped) transforms the field,
@@ -1714,6 +1789,10 @@ int bulk_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, 
               int* status, hipStream_t s);
 int fft2_c64(void* data, int batch, int H, int W, int inverse, hipStream_t s);
 int fft2_c128(void* data, int batch, int H, int W, int inverse, hipStream_t s);
+int layer_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, cd e_in,
+              void* ws, int* status, hipStream_t s);
+int layer_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, cd e_in,
+               void* ws, int* status, hipStream_t s);
 int record_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, void* tape,
                int* status, hipStream_t s);
 int record_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, void* tape,
@@ -1867,6 +1946,11 @@ int bulk_c64(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, v
 int fft2_c64(void* data, int batch, int H, int W, int inverse, hipStream_t s) {
   return fft2_impl<float>(data, batch, H, W, inverse, s);
 }
+int layer_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, cd e_in,
+              void* ws, int* status, hipStream_t s) {
+  return propagate_layer_k<float>(P, S, (const cpx<float>*)fin, (cpx<float>*)fout, e_in, ws,
+                                  status, s);
+}
 int record_c64(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, void* tape,
                int* status, hipStream_t s) {
   return record_impl<float>(P, S, fin, fout, tape, status, s);
@@ -1889,6 +1973,11 @@ int bulk_c128(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout, 
 }
 int fft2_c128(void* data, int batch, int H, int W, int inverse, hipStream_t s) {
   return fft2_impl<double>(data, batch, H, W, inverse, s);
+}
+int layer_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, cd e_in,
+               void* ws, int* status, hipStream_t s) {
+  return propagate_layer_k<double>(P, S, (const cpx<double>*)fin, (cpx<double>*)fout, e_in, ws,
+                                   status, s);
 }
 int record_c128(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout, void* tape,
                 int* status, hipStream_t s) {
@@ -2433,6 +2522,26 @@ int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* f
   hipStream_t s = (hipStream_t)stream;
   if (P->d.precision == DTMMX_C64) return transfer_c64(P, S, fin, fout, ws, status, s);
   return transfer_c128(P, S, fin, fout, ws, status, s);
+}
+
+int dtmmx_propagate_layer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
+                          double e_in_re, double e_in_im, void* ws, size_t ws_bytes, int* status,
+                          void* stream) {
+  {
+    const int r = check_transfer(P, S, fin, fout, ws, ws_bytes);
+    if (r != DTMMX_OK) return r;
+  }
+  if (S->n_layers < 1) return fail(DTMMX_E_ARG, "the layer step needs a layer");
+  if (P->d.diffraction != 1 || P->d.npass != 1 || P->wn > 1)
+    return fail(DTMMX_E_UNSUPPORTED, "the layer step is built for diffraction 1 and npass 1");
+  if ((P->d.method == DTMMX_METHOD_4X4 && P->d.reflection != 1) ||
+      (P->d.method == DTMMX_METHOD_2X2 && P->d.reflection > 1))
+    return fail(DTMMX_E_UNSUPPORTED, "the layer step is built for 4x4 reflection 1 and 2x2 "
+                                     "reflection 0 or 1");
+  hipStream_t s = (hipStream_t)stream;
+  const cd e_in = mk<double>(e_in_re, e_in_im);
+  if (P->d.precision == DTMMX_C64) return layer_c64(P, S, fin, fout, e_in, ws, status, s);
+  return layer_c128(P, S, fin, fout, e_in, ws, status, s);
 }
 
 int dtmmx_transfer_noalloc(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,

@@ -175,6 +175,22 @@ int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field
                    void* field_out, void* workspace, size_t workspace_bytes, int* status,
                    void* stream);
 
+/* One layer step on a k-space field, the layer-level functions of the
+ * reference: dtmm/propagate_4x4.py:376 propagate_4x4_effective_1 (4x4,
+ * reflection 1; field [n_rays][n_wavelengths][4][H][W]) and
+ * dtmm/propagate_2x2.py:193 propagate_2x2_effective_1 (2x2, mode +1, no
+ * reflection storage; the E field [n_rays][n_wavelengths][2][H][W]), with
+ * diffraction 1 and npass 1.  The stack holds the layer (its sub-steps are
+ * n_steps, its isotropic effective medium eps_eff); field_in and field_out
+ * are the numpy fft2 of the real-space field (same convention in and out)
+ * and may not alias.  2x2 reflection 1: the first sub-step starts with the
+ * Fresnel transmission of the tangential E field from the isotropic medium
+ * e_in (the effective medium of the layer before, propagate_2x2.py:28-30).
+ * workspace: dtmmx_plan_workspace_size bytes. */
+int dtmmx_propagate_layer(const dtmmx_plan* plan, const dtmmx_stack* stack, const void* field_in,
+                          void* field_out, double e_in_re, double e_in_im, void* workspace,
+                          size_t workspace_bytes, int* status, void* stream);
+
 /* dtmmx_transfer for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * It has no counterpart in the reference.  The same kernels run with their
