@@ -319,6 +319,11 @@ def dotmdm(a, d, b):
     return np.matmul(a * np.asarray(d)[..., None, :], b)
 
 
+def dotmd(a, d):
+    """a . diag(d) (dtmm/linalg.py:1207)."""
+    return a * np.asarray(d)[..., None, :]
+
+
 def _f2v(f):
     return np.moveaxis(f, -3, -1)[..., None]
 
@@ -1297,31 +1302,49 @@ def Eti_mat(fmatin, fmatout, fmatini=None, mode=+1):
 
 
 def _transfer_ray_2x2_2(field, ks, in_layer, out_layer, dmat, beta, phi, nsteps, mode=+1,
-                        bulk=None):
-    """2x2 reflection mode 2 in real space, one pass (dtmm/propagate_2x2.py
-    :91-190 with refl None): the first half step carries the interface
-    transmission Eti from the previous layer's voxel modes, then the E
-    diffraction dmat, then a plain half step e P e^-1."""
-    _, epsv_in, epsa_in = in_layer
-    _, fmat_in = alphaf(beta, phi, epsv_in, epsa_in)
+                        bulk=None, reflection=2, refl=None):
+    """2x2 layer step in real space (dtmm/propagate_2x2.py:91-190).  With
+    `reflection` the first half step carries the interface from the in_layer
+    voxel modes: without `refl` the transmission Eti (:172-177); with `refl`
+    (npass > 1, :120-160) the tangential E transmission t of tr_mat, the
+    stored wave refl <- field - t.field (that is, minus the reflected
+    tangential E, because t = 1 + r) and field <- t.field + (the old refl),
+    the backward passes (mode -1) adding that field and its H to `bulk`
+    before the layer.  Then the E diffraction dmat and a plain half step
+    e P e^-1; forward passes add E and H to `bulk` after the layer.  Returns
+    (field, refl)."""
     d, epsv, epsa = out_layer
     alpha, fmat = alphaf(beta, phi, epsv, epsa)
     e = E_mat(fmat, mode=mode)
     p = phase_mat(alpha, (np.asarray(ks) * d / 2)[..., None, None], mode=mode)
     ei0 = inv(e)
     for j in range(nsteps):
-        if j == 0:
-            field = dotmdmf(e, p, Eti_mat(fmat_in, fmat, mode=mode), field)
+        if j == 0 and reflection != 0:
+            _, epsv_in, epsa_in = in_layer
+            _, fmat_in = alphaf(beta, phi, epsv_in, epsa_in)
+            if refl is not None:
+                tmat, _ = tr_mat(fmat_in, fmat, mode=mode)
+                trans = refl.copy()
+                tf = dotmf(tmat, field)
+                refl[...] = field - tf
+                field = tf + trans
+                if mode == -1 and bulk is not None:
+                    bulk[..., ::2, :, :] += field
+                    bulk[..., 1::2, :, :] += dotmf(E2H_mat(fmat, mode=mode), field)
+                field = dotmf(ei0, field)
+                field = dotmf(dotmd(e, p), field) if d != 0. else dotmf(e, field)
+            else:
+                field = dotmdmf(e, p, Eti_mat(fmat_in, fmat, mode=mode), field)
         elif d != 0.:
             field = dotmdmf(e, p, ei0, field)
         if dmat is not None:
             field = diffract(field, dmat)
         if d != 0.:
             field = dotmdmf(e, p, ei0, field)
-    if bulk is not None:
+    if mode == +1 and bulk is not None:
         bulk[..., 1::2, :, :] += dotmf(E2H_mat(fmat, mode=mode), field)
         bulk[..., ::2, :, :] += field
-    return field
+    return field, refl
 
 
 def Etri_mat(fmatin, fmatout, fmatini=None, mode=+1):
@@ -1352,10 +1375,14 @@ def E_tr_matrix(shape, ks, epsv_in=(1., 1., 1.), epsa_in=(0., 0., 0.), epsv_out=
 
 
 def _transfer_ray_2x2_1(fft_field, ks, layer, dmat1, dmat2, beta, phi, nsteps, mode, bulk,
-                        tmat=None):
-    """Forward 2x2 layer step (dtmm/propagate_2x2.py:19-87 with refl=None):
+                        tmat=None, rmat=None, refl=None):
+    """2x2 layer step on the k-space E field (dtmm/propagate_2x2.py:19-87):
     reflection mode 1 passes the Fresnel transmission `tmat` of the
-    effective-medium interface, applied before the first sub-step."""
+    effective-medium interface before the first sub-step.  With `refl`
+    (npass > 1, :45-58) the reflected wave rmat.field replaces refl in place
+    and the old refl is added to the transmitted field; backward passes
+    (mode -1) add that field (real space) and its H to `bulk` before the
+    layer, forward passes after it."""
     d, epsv, epsa = layer
     alpha, fmat = alphaf(beta, phi, epsv, epsa)
     e = E_mat(fmat, mode=mode)
@@ -1364,7 +1391,16 @@ def _transfer_ray_2x2_1(fft_field, ks, layer, dmat1, dmat2, beta, phi, nsteps, m
     p = phase_mat(alpha, kd[..., None, None], mode=mode)
     for j in range(nsteps):
         if j == 0 and tmat is not None:
-            fft_field = dotmf(tmat, fft_field)
+            if refl is not None:
+                trans = refl.copy()
+                refl[...] = dotmf(rmat, fft_field)
+                fft_field = dotmf(tmat, fft_field) + trans
+                if mode == -1 and bulk is not None:
+                    field = ifft2(fft_field)
+                    bulk[..., ::2, :, :] += field
+                    bulk[..., 1::2, :, :] += dotmf(E2H_mat(fmat, mode=mode), field)
+            else:
+                fft_field = dotmf(tmat, fft_field)
         if dmat1 is not None:
             fft_field = dotmf(dmat1, fft_field)
         field = ifft2(fft_field)
@@ -1382,15 +1418,16 @@ def _transfer_ray_2x2_1(fft_field, ks, layer, dmat1, dmat2, beta, phi, nsteps, m
 
 def propagate_2x2_effective_1(field, ks, layer_out, effective_layer_out, beta=0, phi=0,
                               nsteps=1, diffraction=1, betamax=BETAMAX, mode=+1, bulk=None,
-                              cache=None, reflection=0, effective_layer_in=None):
+                              cache=None, reflection=0, effective_layer_in=None, refl=None):
     """Layer step of the 2x2 method, reflection mode 0 or 1
-    (dtmm/propagate_2x2.py:193-211; diffraction in {0, 1})."""
+    (dtmm/propagate_2x2.py:193-211; diffraction in {0, 1}).  `refl`
+    (npass > 1) is updated in place."""
     d_out, epsv_out, epsa_out = effective_layer_out
     shape = field.shape[-2:]
-    tmat = None
+    tmat = rmat = None
     if reflection:
         _, epsv_in, epsa_in = effective_layer_in
-        tmat = E_tr_matrix(shape, ks, epsv_in, epsa_in, epsv_out, epsa_out, mode, betamax)[0]
+        tmat, rmat = E_tr_matrix(shape, ks, epsv_in, epsa_in, epsv_out, epsa_out, mode, betamax)
     if diffraction == 1:
         cache = cache or _Cache()
         key = _key(shape, ks, beta, phi, d_out, epsv_out, epsa_out, betamax, mode)
@@ -1399,11 +1436,15 @@ def propagate_2x2_effective_1(field, ks, layer_out, effective_layer_out, beta=0,
         dmat2 = cache.get(("e2",) + key, lambda: second_E_diffraction_matrix(
             shape, ks, beta, phi, d_out / 2, epsv_out, epsa_out, mode, betamax))
     elif diffraction == 0:
+        if refl is not None:
+            # the reference applies dmat1 unconditionally after storing the
+            # reflection (propagate_2x2.py:58), which fails without diffraction
+            raise NotImplementedError("reflection mode 1 with npass > 1 needs diffraction 1")
         dmat1 = dmat2 = None
     else:
         raise NotImplementedError("oracle covers diffraction in {0, 1} for the 2x2 method")
     return _transfer_ray_2x2_1(field, ks, layer_out, dmat1, dmat2, beta, phi, nsteps, mode, bulk,
-                               tmat)
+                               tmat, rmat, refl)
 
 
 # ---------------------------------------------------------------------------
@@ -1643,15 +1684,24 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
 def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1, nstep=1,
                  diffraction=1, reflection=0, betamax=BETAMAX, layer_callback=None,
                  ret_bulk=False):
-    """2x2 (E-field) transfer, npass=1, reflection mode 0 or 1 with diffraction
-    1, reflection mode 0 with diffraction 0 (dtmm/transfer.py:1009-1183).
-    ret_bulk returns (n_layers + 2, ...): the input, the field after every
-    layer (E and the H of each voxel's forward modes) and the output
-    (transfer.py:1045-1050, :1142-1150)."""
-    if npass != 1 or reflection not in (0, 1, 2):
-        raise NotImplementedError("oracle covers npass=1 for the 2x2 method")
-    field_in, wavelengths, pixelsize = field_data
-    field_in = np.asarray(field_in, CDTYPE)
+    """2x2 (E-field) transfer (dtmm/transfer.py:1009-1183): reflection mode 0
+    (diffraction 0 or 1), 1 (diffraction 1; diffraction 0 with npass 1) and
+    2 (diffraction 0 or 1), any npass with reflection 1 or 2.
+
+    npass > 1 (transfer.py:1076-1083, :1105-1178): the caller's input array
+    is overwritten with its forward part and collects the backward field of
+    every backward pass; the reflected waves of every interface are kept
+    between passes; passes alternate direction, the odd ones run the layers
+    in reverse with negative thickness and mode -1.  ret_bulk returns
+    (n_layers + 2, ...): the input side, the field after every layer (E and
+    the H of each voxel's modes, summed over passes) and the output
+    (transfer.py:1045-1050, :1130-1140)."""
+    if reflection not in (0, 1, 2):
+        raise ValueError("Invalid reflection. The 2x2 method supports reflection mode 0,1 or 2.")
+    if npass > 1 and reflection == 0:
+        raise ValueError("Reflection mode `0` not compatible with npass > 1")
+    user_in, wavelengths, pixelsize = field_data
+    field_in = np.asarray(user_in, CDTYPE)
     layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
     ks = k0(wavelengths, pixelsize)
     n = len(layers) - 1
@@ -1659,6 +1709,7 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     if ret_bulk:
         bulk_out = np.zeros((n + 1,) + field_in.shape, CDTYPE)
         bulk_out[0] = field_in
+        field_in = bulk_out[0]
         field_out = bulk_out[-1]
     else:
         bulk_out = None
@@ -1668,63 +1719,64 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     else:
         field0 = _transmitted_field_direct(field_in, beta, phi, n=nin)
     field = field0[..., ::2, :, :].copy()
-    if reflection == 2:
-        # real space (work_in_fft False, transfer.py:1086-1093), each layer
-        # entered through the interface from the previous one
-        if diffraction > 1:
-            raise NotImplementedError("oracle covers 2x2 reflection 2 with diffraction 0 or 1")
-        cache = _Cache()
-        for j in range(n):
-            nsub, layer_out = layers[j + 1]
-            _, layer_in = layers[j]
-            _, (d_eff, e_eff, a_eff) = eff_layers[j + 1]
-            dmat = None
-            if diffraction:
-                key = _key(field.shape[-2:], ks, beta, phi, d_eff, e_eff, a_eff, betamax)
-                dmat = cache.get(("e2",) + key, lambda: corrected_E_diffraction_matrix(
-                    field.shape[-2:], ks, beta, phi, d_eff, e_eff, a_eff, +1, betamax))
-            bulk = field_out if j + 1 == n else (bulk_out[j + 1] if ret_bulk else None)
-            field = _transfer_ray_2x2_2(field, ks, layer_in, layer_out, dmat, beta, phi, nsub,
-                                        +1, bulk)
-            if layer_callback is not None:
-                layer_callback(j)
-        return (bulk_out if ret_bulk else field_out), wavelengths, pixelsize
-    if diffraction == 0:
-        if reflection != 0:
-            raise NotImplementedError("oracle covers diffraction == 0 with reflection 0 for the "
-                                      "2x2 method")
-        # dtmm/transfer.py:1093-1099 (work_in_fft False), propagate_2x2.py:91-190 with
-        # reflection 0 and dmat None: two half-step e.p.e^-1 per sub-step in real
-        # space; the output medium (zero thickness) only builds H from E
-        for j in range(n):
-            nsub, (d, epsv, epsa) = layers[j + 1]
-            alpha, fmat = alphaf(beta, phi, epsv, epsa)
-            if d != 0.:
-                e = E_mat(fmat, mode=+1)
-                ei = inv(e)
-                p = phase_mat(alpha, (ks * d / 2)[..., None, None], mode=+1)
-                for _ in range(nsub):
-                    field = dotmdmf(e, p, ei, field)
-                    field = dotmdmf(e, p, ei, field)
-            bulk = field_out if j + 1 == n else (bulk_out[j + 1] if ret_bulk else None)
-            if bulk is not None:
-                bulk[..., 1::2, :, :] += dotmf(E2H_mat(fmat, mode=+1), field)
-                bulk[..., ::2, :, :] += field
-            if layer_callback is not None:
-                layer_callback(j)
-        return (bulk_out if ret_bulk else field_out), wavelengths, pixelsize
-    field = fft2(field)
+    if npass > 1:
+        field_in[...] = field0
+        refl = np.zeros((n + 1,) + field.shape, CDTYPE)
+    else:
+        refl = [None] * n
+    if reflection == 2 and diffraction > 1:
+        raise NotImplementedError("oracle covers 2x2 reflection 2 with diffraction 0 or 1")
+    if diffraction not in (0, 1):
+        raise NotImplementedError("oracle covers diffraction 0 or 1 for the 2x2 method")
+    # k-space for reflection 0/1 with diffraction, and reflection 1 always
+    # (transfer.py:1085-1093); real space otherwise
+    work_in_fft = reflection == 1 or (reflection == 0 and diffraction > 0)
+    if work_in_fft:
+        field = fft2(field)
+    shape = field.shape[-2:]
     cache = _Cache()
-    for j in range(n):
-        nsub, layer_out = layers[j + 1]
-        _, eff_out = eff_layers[j + 1]
-        _, eff_in = eff_layers[j]
-        bulk = field_out if j + 1 == n else (bulk_out[j + 1] if ret_bulk else None)
-        field = propagate_2x2_effective_1(field, ks, layer_out, eff_out, beta, phi, nsub,
-                                          diffraction, betamax, +1, bulk, cache, reflection,
-                                          eff_in)
-        if layer_callback is not None:
-            layer_callback(j)
+    indices = list(range(n))
+    for i in range(npass):
+        if i > 0:
+            field[...] = 0.
+        direction = (-1) ** i
+        for j in indices:
+            jin = j + (1 - direction) // 2
+            jout = j + (1 + direction) // 2
+            _, input_layer = layers[jin]
+            nsub, (t_out, ev_out, ea_out) = layers[jout]
+            output_layer = (t_out * direction, ev_out, ea_out)
+            _, (d_in, e_in, a_in) = eff_layers[jin]
+            _, (d_out, e_out, a_out) = eff_layers[jout]
+            eff_in = (d_in * direction, e_in, a_in)
+            eff_out = (d_out * direction, e_out, a_out)
+            if jout == 0:
+                bulk = field_in
+            elif jout == n:
+                bulk = field_out
+            else:
+                bulk = None if bulk_out is None else bulk_out[jout]
+            if work_in_fft:
+                field = propagate_2x2_effective_1(field, ks, output_layer, eff_out, beta, phi,
+                                                  nsub, diffraction, betamax, direction, bulk,
+                                                  cache, reflection, eff_in, refl[j])
+            else:
+                dmat = None
+                if diffraction:
+                    key = _key(shape, ks, beta, phi, eff_out[0], e_out, a_out, betamax, direction)
+                    dmat = cache.get(("e2",) + key, lambda: corrected_E_diffraction_matrix(
+                        shape, ks, beta, phi, eff_out[0], e_out, a_out, direction, betamax))
+                field, _ = _transfer_ray_2x2_2(field, ks, input_layer, output_layer, dmat, beta,
+                                               phi, nsub, direction, bulk, reflection, refl[j])
+            if layer_callback is not None:
+                layer_callback(j)
+        indices.reverse()
+    if npass > 1 and not ret_bulk and field_in is not user_in:
+        # the reference writes into the caller's array (transfer.py:1080)
+        try:
+            user_in[...] = field_in
+        except (TypeError, ValueError):
+            pass
     return (bulk_out if ret_bulk else field_out), wavelengths, pixelsize
 
 

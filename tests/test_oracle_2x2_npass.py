@@ -1,0 +1,252 @@
+"""Pinning of the oracle's multi-pass 2x2 transfer (npass > 1, reflection
+modes 1 and 2) by physics that does not share its steps (CPU only).
+
+* A homogeneous isotropic slab on a plane wave that lies on the FFT grid:
+  every pass adds one term of the Fabry-Perot series of the tangential E
+  field, with the Fresnel coefficients t = 2 Y0 / (Y0 + Y1), r = t - 1 of the
+  admittances Y = eps / a (p) and a (s), a = sqrt(eps - beta^2).  After
+  npass passes the output holds the first (npass + 1) // 2 terms and the
+  input side the first npass // 2 reflected terms.  Reflection mode 2 keeps
+  field - t.field, minus the reflected wave, so its reflected field has the
+  opposite sign (dtmm/propagate_2x2.py:127-131); mode 1 keeps r.field
+  (:46-47).
+* Many passes through a stack of isotropic layers sum every multiple
+  reflection: the result is the exact thin-film solution, computed here by
+  the characteristic-matrix method (Born and Wolf, Principles of Optics,
+  section 1.6), which has no passes at all.
+* The backward step through a layer of thickness -d (mode -1) equals the
+  forward step through the layer mirrored in z (director theta -> pi -
+  theta): the z mirror keeps the tangential E field.
+"""
+import numpy as np
+import pytest
+
+import oracle as o
+from oracle import dtmm_oracle as om
+
+WL = np.array([500.])
+PS = 100.
+SHAPE = (16, 16)
+
+
+def _admittance(eps, beta, pol):
+    a = np.sqrt(complex(eps) - beta ** 2)
+    return (eps / a if pol == "p" else a), a
+
+
+def _plane_wave(eps, beta, pol, backward=False):
+    """Forward (or backward) eigenwave of an isotropic medium, phi = 0, on
+    the grid: (Ex, Hy, Ey, Hx) with Hy = +-Y Ex (p), Hx = -+Y Ey (s)."""
+    Y, _ = _admittance(eps, beta, pol)
+    x = np.arange(SHAPE[1])
+    k = o.k0(WL, PS)[0]
+    ramp = np.exp(1j * k * beta * x)[None, :] * np.ones(SHAPE)
+    sg = -1. if backward else 1.
+    f = np.zeros((1, 4) + SHAPE, complex)
+    if pol == "p":
+        f[0, 0] = ramp
+        f[0, 1] = sg * Y * ramp
+    else:
+        f[0, 2] = ramp
+        f[0, 3] = -sg * Y * ramp
+    return f
+
+
+# beta = lambda / (W * pixelsize): one grid step in kx
+BETA = 500. / (16 * 100.)
+
+
+@pytest.mark.parametrize("pol", ["p", "s"])
+@pytest.mark.parametrize("reflection,diffraction", [(2, 0), (2, 1), (1, 1)])
+@pytest.mark.parametrize("npass", [1, 2, 3, 4, 5])
+def test_slab_fabry_perot_series(pol, reflection, diffraction, npass):
+    n0, n1, n2, d = 1.2, 1.9, 1.5, 3.7
+    e0, e1, e2 = n0 ** 2, n1 ** 2, n2 ** 2
+    Y0, _ = _admittance(e0, BETA, pol)
+    Y1, a1 = _admittance(e1, BETA, pol)
+    Y2, _ = _admittance(e2, BETA, pol)
+    t01, t10, t12 = 2 * Y0 / (Y0 + Y1), 2 * Y1 / (Y0 + Y1), 2 * Y1 / (Y1 + Y2)
+    r01, r10, r12 = t01 - 1, t10 - 1, t12 - 1
+    ph = np.exp(1j * o.k0(WL, PS)[0] * d * a1)
+    q = r10 * r12 * ph ** 2
+    tout = t01 * t12 * ph * sum(q ** m for m in range((npass + 1) // 2))
+    sign = -1. if reflection == 2 else 1.
+    nb = npass // 2
+    rin = 0.
+    if nb:
+        rin = sign * (r01 + t01 * t10 * r12 * ph ** 2 * sum(q ** m for m in range(nb)))
+    fin = _plane_wave(e0, BETA, pol)
+    user = fin.copy()
+    out = o.transfer_2x2((user, WL, PS), [(np.array([d]), np.full((1,) + SHAPE + (3,), e1),
+                                            np.zeros((1,) + SHAPE + (3,)))],
+                         beta=BETA, phi=0., nin=n0, nout=n2, npass=npass,
+                         diffraction=diffraction, reflection=reflection)[0]
+    want = tout * _plane_wave(e2, BETA, pol)
+    np.testing.assert_allclose(out, want, atol=1e-12)
+    # the caller's input array: forward part plus the reflected wave
+    want_in = fin + rin * _plane_wave(e0, BETA, pol, backward=True)
+    if npass == 1:
+        np.testing.assert_array_equal(user, fin)
+    else:
+        np.testing.assert_allclose(user, want_in, atol=1e-12)
+
+
+def _char_matrix_rt(ns, ds, beta, pol, k):
+    """Tangential-E transmission and reflection of a thin-film stack by the
+    characteristic matrix [[cos D, i sin D / Y], [i Y sin D, cos D]] per
+    layer, D = k d a (Born and Wolf, section 1.6.2)."""
+    Ys = [_admittance(n ** 2, beta, pol) for n in ns]
+    M = np.eye(2, dtype=complex)
+    for (Y, a), d in zip(Ys[1:-1], ds):
+        D = k * d * a
+        M = np.array([[np.cos(D), 1j * np.sin(D) / Y], [1j * Y * np.sin(D), np.cos(D)]]) @ M
+    Y0, YN = Ys[0][0], Ys[-1][0]
+    # t = M00 (1 + r) + M01 Y0 (1 - r);  YN t = M10 (1 + r) + M11 Y0 (1 - r)
+    A = np.array([[1., -(M[0, 0] - M[0, 1] * Y0)], [YN, -(M[1, 0] - M[1, 1] * Y0)]])
+    b = np.array([M[0, 0] + M[0, 1] * Y0, M[1, 0] + M[1, 1] * Y0])
+    t, r = np.linalg.solve(A, b)
+    return t, r
+
+
+@pytest.mark.parametrize("pol", ["p", "s"])
+@pytest.mark.parametrize("reflection,diffraction", [(2, 0), (2, 1), (1, 1)])
+def test_many_passes_sum_to_thin_film_solution(pol, reflection, diffraction):
+    ns = [1.0, 1.5, 2.1, 1.3, 1.1]
+    ds = [2.3, 1.7, 3.1]
+    # one block per layer: reflection mode 1 reflects at the interfaces of the
+    # blocks' effective media (each block's volume mean, dtmm/transfer.py
+    # :34-41), which are then the layers themselves
+    data = [(np.array([d]), np.full((1,) + SHAPE + (3,), n ** 2), np.zeros((1,) + SHAPE + (3,)))
+            for n, d in zip(ns[1:-1], ds)]
+    fin = _plane_wave(ns[0] ** 2, BETA, pol)
+    user = fin.copy()
+    out = o.transfer_2x2((user, WL, PS), data, beta=BETA, phi=0., nin=ns[0], nout=ns[-1],
+                         npass=41, diffraction=diffraction, reflection=reflection)[0]
+    t, r = _char_matrix_rt(ns, ds, BETA, pol, o.k0(WL, PS)[0])
+    np.testing.assert_allclose(out, t * _plane_wave(ns[-1] ** 2, BETA, pol), atol=1e-10)
+    sign = -1. if reflection == 2 else 1.
+    np.testing.assert_allclose(user - fin, sign * r * _plane_wave(ns[0] ** 2, BETA, pol, True),
+                               atol=1e-10)
+    # the check is not vacuous: one pass misses the multiple reflections
+    one = o.transfer_2x2((fin.copy(), WL, PS), data, beta=BETA, phi=0., nin=ns[0], nout=ns[-1],
+                         npass=1, diffraction=diffraction, reflection=reflection)[0]
+    assert np.abs(one - out).max() > 1e-3
+
+
+@pytest.mark.parametrize("reflection,diffraction", [(2, 0), (2, 1), (1, 1)])
+@pytest.mark.parametrize("npass", [2, 3, 4])
+def test_bulk_of_slab_sums_forward_and_backward_waves(reflection, diffraction, npass):
+    pol = "p"
+    n0, n1, n2, d = 1.2, 1.9, 1.5, 3.7
+    e0, e1, e2 = n0 ** 2, n1 ** 2, n2 ** 2
+    Y0, _ = _admittance(e0, BETA, pol)
+    Y1, a1 = _admittance(e1, BETA, pol)
+    Y2, _ = _admittance(e2, BETA, pol)
+    t01, t10, t12 = 2 * Y0 / (Y0 + Y1), 2 * Y1 / (Y0 + Y1), 2 * Y1 / (Y1 + Y2)
+    r10, r12 = t10 - 1, t12 - 1
+    ph = np.exp(1j * o.k0(WL, PS)[0] * d * a1)
+    q = r10 * r12 * ph ** 2
+    sign = -1. if reflection == 2 else 1.
+    fwd = t01 * ph * sum(q ** m for m in range((npass + 1) // 2))
+    bwd = sign * r12 * t01 * ph * sum(q ** m for m in range(npass // 2))
+    fin = _plane_wave(e0, BETA, pol)
+    data = [(np.array([d]), np.full((1,) + SHAPE + (3,), e1), np.zeros((1,) + SHAPE + (3,)))]
+    bulk = o.transfer_2x2((fin.copy(), WL, PS), data, beta=BETA, phi=0., nin=n0, nout=n2,
+                          npass=npass, diffraction=diffraction, reflection=reflection,
+                          ret_bulk=True)[0]
+    assert bulk.shape == (3, 1, 4) + SHAPE
+    want = fwd * _plane_wave(e1, BETA, pol) + bwd * _plane_wave(e1, BETA, pol, backward=True)
+    np.testing.assert_allclose(bulk[1], want, atol=1e-12)
+    # the ends are the input side and the output of the plain transfer
+    user = fin.copy()
+    out = o.transfer_2x2((user, WL, PS), data, beta=BETA, phi=0., nin=n0, nout=n2, npass=npass,
+                         diffraction=diffraction, reflection=reflection)[0]
+    np.testing.assert_allclose(bulk[2], out, atol=1e-13)
+    np.testing.assert_allclose(bulk[0], user, atol=1e-13)
+
+
+def _tilted_layer(theta, shape=(6, 10), seed=3):
+    rng = np.random.default_rng(seed)
+    eps = np.empty(shape + (3,))
+    eps[..., 0] = eps[..., 1] = 1.5 ** 2
+    eps[..., 2] = 1.5 ** 2 + rng.uniform(0.1, 0.4, shape)
+    ang = np.zeros(shape + (3,))
+    ang[..., 1] = theta
+    ang[..., 2] = rng.uniform(-np.pi, np.pi, shape)
+    return eps, ang
+
+
+@pytest.mark.parametrize("with_dmat", [False, True])
+def test_backward_step_is_forward_step_of_z_mirrored_layer(with_dmat):
+    rng = np.random.default_rng(5)
+    shape = (6, 10)
+    theta = rng.uniform(0.2, 1.2, shape)
+    eps, ang = _tilted_layer(theta, shape)
+    eps_m, ang_m = _tilted_layer(np.pi - theta, shape)
+    ein, ain = _tilted_layer(rng.uniform(0.2, 1.2, shape), shape, seed=9)
+    ein_m = ein.copy()
+    ain_m = ain.copy()
+    ain_m[..., 1] = np.pi - ain[..., 1]
+    ks = o.k0(np.array([450., 610.]), 100.)
+    field = rng.standard_normal((2, 2) + shape) + 1j * rng.standard_normal((2, 2) + shape)
+    beta, phi, d = 0.3, 0.8, 1.3
+    e_eff = np.full(3, 1.6 ** 2)
+    fm_in = o.alphaf(beta, phi, ein_m, ain_m)[1]
+    fm = o.alphaf(beta, phi, eps_m, ang_m)[1]
+    for refl in (None, True):
+        dmat_b = dmat_f = None
+        if with_dmat:
+            dmat_b = o.corrected_E_diffraction_matrix(shape, ks, beta, phi, -d, e_eff,
+                                                      np.zeros(3), -1)
+            dmat_f = o.corrected_E_diffraction_matrix(shape, ks, beta, phi, d, e_eff,
+                                                      np.zeros(3), +1)
+        r0 = None if refl is None else (rng.standard_normal((2, 2) + shape) + 0j)
+        rb = None if r0 is None else r0.copy()
+        rf = None if r0 is None else r0.copy()
+        bulk_b = np.zeros((2, 4) + shape, complex)
+        fb, _ = om._transfer_ray_2x2_2(field.copy(), ks, (0., ein, ain), (-d, eps, ang), dmat_b,
+                                      beta, phi, 2, -1, bulk_b, 2, rb)
+        ff, _ = om._transfer_ray_2x2_2(field.copy(), ks, (0., ein_m, ain_m), (d, eps_m, ang_m),
+                                      dmat_f, beta, phi, 2, +1, None, 2, rf)
+        np.testing.assert_allclose(fb, ff, atol=1e-12)
+        if refl:
+            np.testing.assert_allclose(rb, rf, atol=1e-12)
+            # the backward bulk holds E after the interface and H of the
+            # backward modes; the mirror keeps E and flips the sign of H
+            t_f, _ = o.tr_mat(fm_in, fm, mode=+1)
+            g = o.dotmf(t_f, field) + r0
+            np.testing.assert_allclose(bulk_b[:, ::2], g, atol=1e-12)
+            np.testing.assert_allclose(bulk_b[:, 1::2], -o.dotmf(o.E2H_mat(fm, +1), g),
+                                       atol=1e-12)
+    # the layer is really anisotropic and tilted: the unmirrored forward
+    # step is another operator
+    fx, _ = om._transfer_ray_2x2_2(field.copy(), ks, (0., ein, ain), (d, eps, ang), None,
+                                  beta, phi, 2, +1, None, 0, None)
+    fm, _ = om._transfer_ray_2x2_2(field.copy(), ks, (0., ein_m, ain_m), (d, eps_m, ang_m), None,
+                                  beta, phi, 2, +1, None, 0, None)
+    assert np.abs(fx - fm).max() > 1e-3
+
+
+@pytest.mark.parametrize("method_reflection", [("2x2", 1), ("2x2", 2)])
+def test_npass_translation_equivariant(method_reflection):
+    method, reflection = method_reflection
+    rng = np.random.default_rng(17)
+    L, shape = 3, (12, 20)
+    wl = np.array([520.])
+    eps = np.empty((L,) + shape + (3,))
+    eps[..., 0] = eps[..., 1] = rng.uniform(2.2, 2.4, (L,) + shape)
+    eps[..., 2] = eps[..., 0] + rng.uniform(0.1, 0.3, (L,) + shape)
+    ang = np.stack([np.zeros((L,) + shape), rng.uniform(0, np.pi, (L,) + shape),
+                    rng.uniform(-np.pi, np.pi, (L,) + shape)], -1)
+    f = rng.standard_normal((1, 4) + shape) + 1j * rng.standard_normal((1, 4) + shape)
+    dy, dx = 5, 3
+    kw = dict(beta=0.1, phi=0.4, method=method, diffraction=1, npass=4, nin=1.2, nout=1.7,
+              reflection=reflection)
+    fa = f.copy()
+    out = o.transfer_field((fa, wl, 100.), [(np.ones(L), eps, ang)], **kw)[0]
+    fb = np.roll(np.roll(f, dy, -2), dx, -1)
+    out_r = o.transfer_field((fb, wl, 100.), [(np.ones(L), np.roll(np.roll(eps, dy, 1), dx, 2),
+                                               np.roll(np.roll(ang, dy, 1), dx, 2))], **kw)[0]
+    roll = lambda a: np.roll(np.roll(a, dy, -2), dx, -1)  # noqa: E731
+    np.testing.assert_allclose(out_r, roll(out), atol=1e-12 * np.abs(out).max())
+    np.testing.assert_allclose(fb, roll(fa), atol=1e-12 * np.abs(fa).max())
