@@ -31,6 +31,7 @@ EXPORTS = (
     "dtmmx_plan_create",
     "dtmmx_plan_destroy",
     "dtmmx_plan_workspace_size",
+    "dtmmx_transfer_workspace_size",
     "dtmmx_plan_set_tiles",
     "dtmmx_plan_get_tiles",
     "dtmmx_transfer",
@@ -177,6 +178,8 @@ def load():
     lib.dtmmx_plan_destroy.restype = ctypes.c_int
     lib.dtmmx_plan_workspace_size.argtypes = [ctypes.c_void_p]
     lib.dtmmx_plan_workspace_size.restype = ctypes.c_size_t
+    lib.dtmmx_transfer_workspace_size.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.dtmmx_transfer_workspace_size.restype = ctypes.c_size_t
     lib.dtmmx_plan_set_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.dtmmx_plan_get_tiles.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int)]

@@ -430,12 +430,13 @@ __host__ __device__ inline void matmul4(const cd a[4][4], const cd b[4][4], cd o
 //   E+ diag(p0, p2) E+^-1 = p2 I + (p0 - p2) P0,  P0 = E+[:,0] (x) E+^-1[0,:]
 //   E- diag(p1, p3) E-^-1 = p3 I + (p1 - p3) P1,  P1 = E-[:,0] (x) E-^-1[0,:]
 // ok4: E = blockdiag(E+, E-) is invertible (4x4 method, dtmm/tmm.py:974-997
-// with linalg.inv); ok2: E+ is invertible (2x2 method, mode = +1).
+// with linalg.inv); ok2: E+ is invertible (2x2 method, mode = +1); okb: E-
+// is invertible (2x2 method, mode = -1).
 struct RopData {
   cd a[4];
   cd P0[2][2];
   cd P1[2][2];
-  bool ok4, ok2;
+  bool ok4, ok2, okb;
   int kind;
 };
 
@@ -462,6 +463,7 @@ __host__ __device__ inline void rop_data(const Modes& m, int kind, RopData& r) {
   bool okp = inv2(ep, ip);
   bool okn = inv2(en, in);
   r.ok2 = okp;
+  r.okb = okn;
   r.ok4 = okp && okn;
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) {
@@ -577,6 +579,7 @@ __host__ __device__ inline bool rop_lean_real(double beta, double c0, double s0,
   r.P1[1][0] = mk<double>(Ey[1] * in0);
   r.P1[1][1] = mk<double>(Ey[1] * in1);
   r.ok2 = okp;
+  r.okb = okn;
   r.ok4 = okp && okn;
   return true;
 }
@@ -661,6 +664,7 @@ __host__ __device__ inline void rop_lean(double beta, double c0, double s0, doub
   const bool okp = inv2(ep, ip);
   const bool okn = inv2(en, in);
   r.ok2 = okp;
+  r.okb = okn;
   r.ok4 = okp && okn;
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) {

@@ -289,6 +289,7 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
     e.table = a.etab_ws;
     e.status = a.status;
     e.method4 = a.method4;
+    e.bwd2 = a.bwd2;
     e.bm = a.bm;
     hipLaunchKernelGGL((rop_table_kernel<T>), dim3(unsigned((HW + NT - 1) / NT), e.nbeam), dim3(NT), 0,
                        s, e);
@@ -465,7 +466,9 @@ struct WsLayout {
   size_t part, sums;               // npass > 1: reduction partials and sums
   size_t etab;                     // per-pixel data of a layer with biaxial voxels
   size_t full_spec, full_part;     // diffraction < 0: spectrum, per-chunk partial sums
-  size_t total;
+  size_t tmp2;                     // 2x2 npass > 1: a 2-component field
+  size_t total;                    // the stored reflections of 2x2 npass > 1 follow
+                                   // (stack_ws_bytes)
   int nblk;
 };
 
@@ -493,12 +496,24 @@ static WsLayout ws_layout(const dtmmx_plan* P) {
     w.win_acc = take(f4);
   }
   if (!m4) w.e2 = take(planes * 2 * HW * cb);
+  if (!m4 && P->d.npass > 1) w.tmp2 = take(planes * 2 * HW * cb);
   if (P->d.diffraction < 0) {
     w.full_spec = take(f4);
     w.full_part = take(f4 * size_t(P->full_chunks > 0 ? P->full_chunks : 1));
   }
   w.etab = take(size_t(EIG_COMPS) * (P->d.n_rays / P->d.rays_per_beam) * HW * cb);
   return w;
+}
+
+// Workspace that depends on the stack: the reflected wave of every interface
+// (input medium, layers, output medium: n_layers + 1) kept between the
+// passes of the 2x2 method with npass > 1 (dtmm/transfer.py:1081-1083).
+static size_t stack_ws_bytes(const dtmmx_plan* P, const dtmmx_stack* S) {
+  if (P->d.method != DTMMX_METHOD_2X2 || P->d.npass <= 1 || !S) return 0;
+  const size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const size_t planes = size_t(P->d.n_rays) * P->d.n_wavelengths;
+  return size_t(S->n_layers + 1) * ((planes * 2 * HW * cb + 255) / 256 * 256);
 }
 
 #define TRY(x)                     \
@@ -1379,6 +1394,269 @@ static int transfer_2x2_refl2(const dtmmx_plan* P, const dtmmx_stack* S, const c
   return DTMMX_OK;
 }
 
+// The 2x2 method with npass > 1 (dtmm/transfer.py:1076-1178), reflection
+// mode 2 (real space, diffraction 0 or 1, dtmm/propagate_2x2.py:91-190) or
+// reflection mode 1 (k space, diffraction 1, :19-87).  The layers are
+// indexed as the reference's list: 0 the input medium, 1..L the stack, L + 1
+// the output medium; interface j lies between j and j + 1 and keeps the wave
+// reflected there (refl[j], workspace after the plan's part) for the next
+// pass.  field_in is overwritten with its forward part and receives the
+// backward field of every backward pass; field_out sums the forward passes;
+// bulk (L slots) sums the field after each layer of the forward passes and
+// the field entering it in the backward passes.
+template <typename T>
+static int transfer_2x2_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin,
+                              cpx<T>* fout, void* ws, int* status, hipStream_t s, cpx<T>* bulk) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const int nbeam = P->d.n_rays / P->d.rays_per_beam;
+  const int nlam = P->d.n_wavelengths;
+  const size_t nplanes = size_t(P->d.n_rays) * nlam;
+  const size_t n4 = nplanes * 4 * HW, n2 = nplanes * 2 * HW;
+  const double scale = 1.0 / double(HW);
+  const bool diff = P->d.diffraction != 0;
+  const bool r1 = P->d.reflection == 1;
+  const Batch b{P->d.n_rays, nlam, 0, nlam};
+  const WsLayout wl = ws_layout(P);
+  cpx<T>* e2 = (cpx<T>*)((char*)ws + wl.e2);
+  cpx<T>* tmp = (cpx<T>*)((char*)ws + wl.tmp2);
+  const size_t rstride = (n2 * sizeof(cpx<T>) + 255) / 256 * 256;
+  char* reflb = (char*)ws + wl.total;
+  auto refl = [&](int j) { return (cpx<T>*)(reflb + size_t(j) * rstride); };
+  const int L = S->n_layers;
+  const cd ein = mk<double>(S->n_in * S->n_in), eout = mk<double>(S->n_out * S->n_out);
+  const dim3 grid(unsigned((HW + NT - 1) / NT), unsigned(nbeam));
+  auto zero = [&](void* p, size_t nreal) -> int {
+    hipLaunchKernelGGL((zero_kernel<T>), dim3(unsigned((nreal + NT - 1) / NT)), dim3(NT), 0, s,
+                       (T*)p, nreal);
+    HIP_TRY(hipGetLastError());
+    return DTMMX_OK;
+  };
+  // layer r of the reference's list
+  struct Lay {
+    bool iso;
+    int l, ns;
+    double t;  // sub-step thickness
+    cd e, eeff;
+  };
+  auto lay = [&](int r) {
+    Lay y{};
+    if (r == 0 || r == L + 1) {
+      y.iso = true;
+      y.l = -1;
+      y.ns = 1;
+      y.t = 0.;
+      y.e = y.eeff = r == 0 ? ein : eout;
+    } else {
+      y.iso = false;
+      y.l = r - 1;
+      y.ns = S->n_steps ? S->n_steps[y.l] : 1;
+      y.t = S->thickness[y.l] / double(y.ns);
+      y.eeff = mk<double>(S->eps_eff_re[y.l], S->eps_eff_im ? S->eps_eff_im[y.l] : 0.);
+    }
+    return y;
+  };
+  auto set_cur = [&](PixArgs<T>& a, const Lay& y) {
+    a.cur_iso = y.iso;
+    a.emed = y.e;
+    if (!y.iso) {
+      a.eps_re = S->eps_re + size_t(y.l) * HW;
+      a.eps_im = S->eps_im ? S->eps_im + size_t(y.l) * HW : nullptr;
+      a.ang = S->angles + size_t(y.l) * HW;
+    }
+  };
+  // H of the modes of direction dir from the 2-component real-space field
+  // src, added to the 4-component target
+  auto add_bulk = [&](const cpx<T>* src, cpx<T>* dst, const Lay& y, int dir) -> int {
+    PixArgs<T> pa{};
+    pa.HW = HW;
+    pa.nlam = nlam;
+    pa.bm = P->bm;
+    pa.comp_stride = size_t(L) * HW;
+    pa.src = src;
+    pa.dst = dst;
+    pa.mode = dir;
+    pa.accum = 1;
+    set_cur(pa, y);
+    if (y.iso)
+      hipLaunchKernelGGL((pixel_kernel<T, PIX_OUT2>), grid, dim3(NT), 0, s, pa);
+    else
+      hipLaunchKernelGGL((pixel_kernel<T, PIX_E2H>), grid, dim3(NT), 0, s, pa);
+    HIP_TRY(hipGetLastError());
+    return DTMMX_OK;
+  };
+  TRY(zero(fout, 2 * n4));
+  if (L > 0 && bulk) TRY(zero(bulk, 2 * n4 * size_t(L)));
+  for (int j = 0; j <= L; ++j) TRY(zero(refl(j), 2 * n2));
+  // forward part of the input in the input medium, left in field_in
+  // (transfer.py:1071-1080)
+  if (diff) {
+    TRY(fft2_planes<T>(P, fin, 0, s));
+    PArgs<T> a{};
+    a.pop = POP_PROJ;
+    a.eps = ein;
+    a.scale = scale;
+    TRY(pass_op<T>(P, fin, a, s));
+    TRY(fft2_planes<T>(P, fin, 1, s));
+    // E components (0 and 2 of each plane) to the 2-component field
+    HIP_TRY(hipMemcpy2DAsync(e2, HW * sizeof(cpx<T>), fin, 2 * HW * sizeof(cpx<T>),
+                             HW * sizeof(cpx<T>), nplanes * 2, hipMemcpyDeviceToDevice, s));
+  } else {
+    PixArgs<T> a{};
+    a.HW = HW;
+    a.nlam = nlam;
+    a.bm = P->bm;
+    a.src = fin;
+    a.dst = fin;
+    a.field = e2;
+    a.emed = ein;
+    hipLaunchKernelGGL((pixel_kernel<T, PIX_IN2>), grid, dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+  }
+  // target of the bulk terms of layer r: field_in, field_out or a bulk slot
+  auto target = [&](int r) -> cpx<T>* {
+    if (r == 0) return fin;
+    if (r == L + 1) return fout;
+    return bulk ? bulk + size_t(r - 1) * n4 : nullptr;
+  };
+  for (int i = 0; i < P->d.npass; ++i) {
+    const int dir = (i % 2 == 0) ? 1 : -1;
+    if (i > 0)
+      TRY(zero(e2, 2 * n2));
+    else if (r1)
+      TRY(row_pass<T>(P, e2, b, 2, OP_FWD, s));  // real space -> the row-pass output form
+    for (int k = 0; k <= L; ++k) {
+      const int j = dir > 0 ? k : L - k;
+      const int jin = dir > 0 ? j : j + 1, jout = dir > 0 ? j + 1 : j;
+      const Lay yi = lay(jin), yo = lay(jout);
+      cpx<T>* tg = target(jout);
+      if (r1) {
+        // k space: interface (D2 of the layer left, Fresnel, stored
+        // reflection, D1 of the layer entered), then the sub-steps
+        TRY(col_pass<T>(P, e2, b, 2, OP_FWD, s));
+        TrrArgs<T> ta{};
+        ta.src = e2;
+        ta.dst = e2;
+        ta.refl = refl(j);
+        ta.g = (dir < 0 && tg) ? tmp : nullptr;
+        ta.H = P->d.height;
+        ta.W = P->d.width;
+        ta.b = b;
+        ta.amax = P->amax;
+        ta.da = yi.t;
+        ta.db = yo.t;
+        ta.ea = yi.eeff;
+        ta.eb = yo.eeff;
+        ta.fresnel = !eq(yi.eeff, yo.eeff);
+        ta.scale = scale;
+        ta.bm = P->bm;
+        hipLaunchKernelGGL((trr2_kernel<T>), dim3(unsigned((HW + NT - 1) / NT), unsigned(nplanes)),
+                           dim3(NT), 0, s, ta);
+        HIP_TRY(hipGetLastError());
+        if (ta.g) {
+          TRY(col_pass<T>(P, tmp, b, 2, OP_INV, s));
+          TRY(row_pass<T>(P, tmp, b, 2, OP_INV, s));
+          TRY(add_bulk(tmp, tg, yo, -1));
+        }
+        if (dir < 0 && jout == 0) break;  // the rest of this pass reaches no output
+        TRY(col_pass<T>(P, e2, b, 2, OP_INV, s));
+        if (yo.iso) {  // output medium: E and H of its forward modes
+          TRY(row_pass<T>(P, e2, b, 2, OP_INV, s));
+          TRY(add_bulk(e2, tg, yo, +1));
+          continue;
+        }
+        for (int q = 0; q < yo.ns; ++q) {
+          RowArgs<T> la = layer_args<T>(S, HW, yo.l, yo.t * dir, 0, status, (char*)ws + wl.etab);
+          la.bwd2 = dir < 0;
+          TRY(row_pass<T>(P, e2, b, 2, OP_ROP, s, &la));
+          if (q + 1 < yo.ns) {
+            ColArgs<T> ca{};
+            ca.da = yo.t;
+            ca.db = yo.t;
+            ca.ea = yo.eeff;
+            ca.eb = yo.eeff;
+            ca.kca = 1.;
+            ca.kcb = 1.;
+            ca.scale = scale;
+            ca.dense_sign = 0;
+            TRY(col_pass<T>(P, e2, b, 2, OP_DIAG, s, &ca));
+          }
+        }
+        if (dir > 0 && tg) {
+          // field after the layer (its second half step) in real space
+          TRY(copy_planes<T>(tmp, e2, b, HW, s, 2));
+          ColArgs<T> cb{};
+          cb.da = yo.t;
+          cb.ea = yo.eeff;
+          cb.db = 0.;
+          cb.eb = yo.eeff;
+          cb.kca = 1.;
+          cb.kcb = 1.;
+          cb.scale = scale;
+          cb.dense_sign = 0;
+          TRY(col_pass<T>(P, tmp, b, 2, OP_DIAG, s, &cb));
+          TRY(row_pass<T>(P, tmp, b, 2, OP_INV, s));
+          TRY(add_bulk(tmp, tg, yo, +1));
+        }
+        continue;
+      }
+      // reflection mode 2, real space
+      PixArgs<T> a{};
+      a.HW = HW;
+      a.nlam = nlam;
+      a.bm = P->bm;
+      a.comp_stride = size_t(L) * HW;
+      a.field = e2;
+      a.mode = dir;
+      a.prev_iso = yi.iso;
+      a.pemed = yi.e;
+      if (!yi.iso) {
+        a.peps_re = S->eps_re + size_t(yi.l) * HW;
+        a.peps_im = S->eps_im ? S->eps_im + size_t(yi.l) * HW : nullptr;
+        a.pang = S->angles + size_t(yi.l) * HW;
+      }
+      set_cur(a, yo);
+      const double ds = yo.t * dir;
+      for (int q = 0; q < yo.ns; ++q) {
+        a.d = 0.5 * ds;
+        if (q == 0) {
+          PixArgs<T> t = a;
+          t.refl = refl(j);
+          t.dst = dir < 0 ? tg : nullptr;
+          hipLaunchKernelGGL((pixel_kernel<T, PIX_TRR>), grid, dim3(NT), 0, s, t);
+          HIP_TRY(hipGetLastError());
+          if (dir < 0 && jout == 0) break;
+        } else if (ds != 0.) {
+          hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER2>), grid, dim3(NT), 0, s, a);
+          HIP_TRY(hipGetLastError());
+        }
+        if (diff) {
+          // corrected E diffraction of the layer's effective medium; the
+          // backward one at -d equals the forward one at d (isotropic medium)
+          TRY(row_pass<T>(P, e2, b, 2, OP_FWD, s));
+          ColArgs<T> ca{};
+          ca.da = yo.t;
+          ca.db = yo.t;
+          ca.ea = yo.eeff;
+          ca.eb = yo.eeff;
+          ca.kca = 1.;
+          ca.kcb = 1.;
+          ca.scale = scale;
+          ca.dense_sign = 0;
+          TRY(col_pass<T>(P, e2, b, 2, OP_DIAG, s, &ca));
+          TRY(row_pass<T>(P, e2, b, 2, OP_INV, s));
+        }
+        if (ds != 0.) {
+          hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER2>), grid, dim3(NT), 0, s, a);
+          HIP_TRY(hipGetLastError());
+        }
+      }
+      if (dir < 0 && jout == 0) break;
+      if (dir > 0 && tg) TRY(add_bulk(e2, tg, yo, +1));
+    }
+  }
+  return DTMMX_OK;
+}
+
 // One layer step on a k-space field (the layer-level functions
 // dtmm/propagate_4x4.py:376-389 propagate_4x4_effective_1 with :80-101 and
 // dtmm/propagate_2x2.py:193-211 propagate_2x2_effective_1 with :19-76, mode
@@ -1519,6 +1797,8 @@ static int transfer_impl(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, v
     if (bulk) return fail(DTMMX_E_UNSUPPORTED, "bulk output is not built for diffraction < 0");
     return transfer_full<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s);
   }
+  if (P->d.npass > 1 && P->d.method == DTMMX_METHOD_2X2)
+    return transfer_2x2_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s, (cpx<T>*)bulk);
   if (P->d.npass > 1) return transfer_npass<T>(P, S, (cpx<T>*)fin, (cpx<T>*)fout, ws, status, s);
   if (P->d.method == DTMMX_METHOD_2X2 && P->d.reflection == 2)
     return transfer_2x2_refl2<T>(P, S, (const cpx<T>*)fin, (cpx<T>*)fout, ws, s, (cpx<T>*)bulk);
@@ -2278,8 +2558,8 @@ static int validate_desc(const dtmmx_plan_desc* d) {
   // one pass
   const bool nodiff = d->diffraction == 0 &&
                       ((d->method == DTMMX_METHOD_4X4 && d->reflection == 2) ||
-                       (d->method == DTMMX_METHOD_2X2 && (d->reflection == 0 || d->reflection == 2) &&
-                        d->npass == 1));
+                       (d->method == DTMMX_METHOD_2X2 &&
+                        ((d->reflection == 0 && d->npass == 1) || d->reflection == 2)));
   // 4x4 reflection mode 2 (any npass; diffraction > 1: tent windows)
   const bool refl2 = d->method == DTMMX_METHOD_4X4 && d->reflection == 2;
   // diffraction < 0: exact per-mode propagation (4x4, one pass)
@@ -2296,9 +2576,15 @@ static int validate_desc(const dtmmx_plan_desc* d) {
     return fail(DTMMX_E_UNSUPPORTED, "diffraction > 1 is built for the 4x4 method only");
   if (d->diffraction > 64) return fail(DTMMX_E_ARG, "diffraction above 64");
   if (d->npass < 1) return fail(DTMMX_E_ARG, "npass must be >= 1");
-  if (d->npass > 1 && d->method != DTMMX_METHOD_4X4)
-    return fail(DTMMX_E_UNSUPPORTED, "npass > 1 is built for the 4x4 method only");
-  if (d->npass > 1 && d->norm != 0 && d->norm != DTMMX_NORM_FFT)
+  // 2x2 with npass > 1: reflection mode 1 (diffraction 1) or 2 (diffraction 0
+  // or 1); its passes have no normalisation (dtmm/transfer.py:1105-1178)
+  if (d->npass > 1 && d->method == DTMMX_METHOD_2X2 &&
+      !((d->reflection == 1 && d->diffraction == 1) ||
+        (d->reflection == 2 && d->diffraction <= 1)))
+    return fail(DTMMX_E_UNSUPPORTED, "npass > 1 with the 2x2 method is built for reflection "
+                                     "mode 1 with diffraction 1 and mode 2 with diffraction 0 "
+                                     "or 1");
+  if (d->npass > 1 && d->method == DTMMX_METHOD_4X4 && d->norm != 0 && d->norm != DTMMX_NORM_FFT)
     return fail(DTMMX_E_UNSUPPORTED, "npass > 1: only norm 0 (total) and 1 (fft) are built");
   if (d->checkpoint < 0) return fail(DTMMX_E_ARG, "checkpoint interval must be >= 0");
   if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !refl2 && !full)
@@ -2397,6 +2683,11 @@ size_t dtmmx_plan_workspace_size(const dtmmx_plan* P) {
   return ws_layout(P).total;
 }
 
+size_t dtmmx_transfer_workspace_size(const dtmmx_plan* P, const dtmmx_stack* S) {
+  if (!P) return 0;
+  return ws_layout(P).total + stack_ws_bytes(P, S);
+}
+
 static int check_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin,
                           const void* fout, const void* ws, size_t ws_bytes, bool need_ws = true) {
   if (!P || !S || !fin || !fout) return fail(DTMMX_E_ARG, "null argument");
@@ -2406,9 +2697,9 @@ static int check_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void*
   if (S->n_steps)
     for (int l = 0; l < S->n_layers; ++l)
       if (S->n_steps[l] < 1) return fail(DTMMX_E_ARG, "n_steps must be >= 1");
-  if (need_ws &&
-      (ws_bytes < dtmmx_plan_workspace_size(P) || (dtmmx_plan_workspace_size(P) && !ws)))
-    return fail(DTMMX_E_ARG, "workspace too small");
+  const size_t need = dtmmx_transfer_workspace_size(P, S);
+  if (need_ws && (ws_bytes < need || (need && !ws)))
+    return fail(DTMMX_E_ARG, "workspace too small: " + std::to_string(need) + " bytes needed");
   if (fin == fout) return fail(DTMMX_E_ARG, "field_in and field_out must not alias");
   return DTMMX_OK;
 }
@@ -2506,8 +2797,9 @@ int dtmmx_transfer_bulk(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, vo
     if (r != DTMMX_OK) return r;
   }
   if (!bulk) return fail(DTMMX_E_ARG, "null bulk buffer");
-  if (P->d.npass != 1 || P->d.diffraction > 1)
-    return fail(DTMMX_E_UNSUPPORTED, "bulk output is built with npass 1 and diffraction 0 or 1");
+  if ((P->d.npass != 1 && P->d.method != DTMMX_METHOD_2X2) || P->d.diffraction > 1)
+    return fail(DTMMX_E_UNSUPPORTED, "bulk output is built with diffraction 0 or 1 and npass 1 "
+                                     "(4x4) or any npass (2x2)");
   hipStream_t s = (hipStream_t)stream;
   if (P->d.precision == DTMMX_C64) return bulk_c64(P, S, fin, fout, bulk, ws, status, s);
   return bulk_c128(P, S, fin, fout, bulk, ws, status, s);
@@ -2546,9 +2838,9 @@ int dtmmx_propagate_layer(const dtmmx_plan* P, const dtmmx_stack* S, const void*
 
 int dtmmx_transfer_noalloc(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
                            int* status, void* stream) {
-  const size_t need = dtmmx_plan_workspace_size(P);
+  const size_t need = dtmmx_transfer_workspace_size(P, S);
   if (P && need > DTMMX_NOALLOC_SCRATCH_BYTES)
-    return fail(DTMMX_E_ARG, "the plan's workspace (" + std::to_string(need) +
+    return fail(DTMMX_E_ARG, "the transfer's workspace (" + std::to_string(need) +
                                  " bytes) exceeds the no-allocation scratch (" +
                                  std::to_string(size_t(DTMMX_NOALLOC_SCRATCH_BYTES)) +
                                  " bytes): use dtmmx_transfer with a workspace");

@@ -83,6 +83,7 @@ struct RowArgs {
   size_t comp_stride;   // L*H*W
   double d;             // layer thickness
   int method4;          // 1: 4x4 (E invertible), 0: 2x2 (E+ invertible)
+  int bwd2;             // 2x2 (NC == 2): 1 propagates the backward modes (mode -1)
   int* status;
   // OP_ROP output: acc_mode 0 writes the result back to `field`; 1 writes
   // it to `acc`; 2 adds it to `acc` (sum over k-space windows)
@@ -261,6 +262,24 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
                    A.ang[2 * A.comp_stride + o], r);
         }
         if (r.kind == KIND_BIAXIAL && A.status) atomicOr(A.status, 1);
+        if constexpr (NC == 2) {
+          // the 2x2 operator reads a[0], a[2] and P0: the backward modes'
+          // a[1], a[3] and P1 take their place for mode -1
+          if (A.bwd2) {
+            const cd t0 = r.a[0], t2 = r.a[2];
+            r.a[0] = r.a[1];
+            r.a[2] = r.a[3];
+            r.a[1] = t0;
+            r.a[3] = t2;
+            for (int i = 0; i < 2; ++i)
+              for (int j = 0; j < 2; ++j) {
+                const cd t = r.P0[i][j];
+                r.P0[i][j] = r.P1[i][j];
+                r.P1[i][j] = t;
+              }
+            r.ok2 = r.okb;
+          }
+        }
         for (int i = 0; i < 4; ++i) eg[k][i] = cast<T>(r.a[i]);
         for (int i = 0; i < 2; ++i)
           for (int j = 0; j < 2; ++j) {
@@ -455,6 +474,7 @@ struct EigArgs {
   cpx<T>* table;
   int* status;
   int method4;  // rop_table_kernel: 1 = 4x4 (E invertible), 0 = 2x2 (E+ invertible)
+  int bwd2;     // rop_table_kernel, 2x2: backward modes (mode -1) in the forward slots
   Beams bm;
 };
 
@@ -478,13 +498,18 @@ __global__ __launch_bounds__(NT) void rop_table_kernel(EigArgs<T> A) {
   rop_full(beta, c0, s0, phi0, e0, e1, e2, A.ang[o], A.ang[cs + o], A.ang[2 * cs + o], r);
   cpx<T>* t = A.table + size_t(beam) * A.HW + o;
   const size_t ts = size_t(A.nbeam) * A.HW;
-  for (int i = 0; i < 4; ++i) t[i * ts] = cast<T>(r.a[i]);
+  const bool bw = !A.method4 && A.bwd2;
+  const int f0 = bw ? 1 : 0, f2 = bw ? 3 : 2, b1 = bw ? 0 : 1, b3 = bw ? 2 : 3;
+  t[0] = cast<T>(r.a[f0]);
+  t[ts] = cast<T>(r.a[b1]);
+  t[2 * ts] = cast<T>(r.a[f2]);
+  t[3 * ts] = cast<T>(r.a[b3]);
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) {
-      t[(4 + 2 * i + j) * ts] = cast<T>(r.P0[i][j]);
-      t[(8 + 2 * i + j) * ts] = cast<T>(r.P1[i][j]);
+      t[(4 + 2 * i + j) * ts] = cast<T>(bw ? r.P1[i][j] : r.P0[i][j]);
+      t[(8 + 2 * i + j) * ts] = cast<T>(bw ? r.P0[i][j] : r.P1[i][j]);
     }
-  t[12 * ts] = mk<T>((A.method4 ? r.ok4 : r.ok2) ? T(1) : T(0), T(0));
+  t[12 * ts] = mk<T>((A.method4 ? r.ok4 : (bw ? r.okb : r.ok2)) ? T(1) : T(0), T(0));
 }
 
 template <typename T>
@@ -1249,6 +1274,101 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
     }
   }
   for (int i = 0; i < nout; ++i) o[i * HW] = cast<T>(r[i] * A.scale);
+}
+
+// ---------------------------------------------------------------------------
+// Interface of the 2x2 method in reflection mode 1 with stored reflections
+// (npass > 1, dtmm/propagate_2x2.py:45-58 with E_tr_matrix of
+// dtmm/diffract.py:215-225), one thread per (k, plane) on the full spectrum
+// of the E field:
+//   u = K_a v scale   (second half step of the layer before the interface)
+//   g = T u + refl    (Fresnel transmission between the isotropic effective
+//                      media, plus the wave reflected there by the previous
+//                      pass)
+//   refl <- (T - 1) u (the reflected tangential E: t = 1 + r)
+//   dst = K_b g       (first half step of the layer after it)
+// K = exp(i k0 d/2 (a(k) - a(beta_beam))), zero above betamax.  The
+// backward direction (mode -1) has the same scalars with the thickness |d|
+// and the same tangential Fresnel coefficients (a mirror in z keeps the
+// tangential E field).  Without fresnel (equal media) T = 1 and refl is not
+// touched.  With g set, g also receives T u + refl (the field entering the
+// layer, for the bulk of a backward pass).
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct TrrArgs {
+  const cpx<T>* src;
+  cpx<T>* dst;
+  cpx<T>* refl;
+  cpx<T>* g;
+  int H, W;
+  Batch b;
+  const int* amax;
+  double da, db;  // sub-step thickness (>= 0) of the layers before / after
+  cd ea, eb;      // their effective eps
+  int fresnel;    // 1: transmission from ea into eb
+  double scale;
+  Beams bm;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void trr2_kernel(TrrArgs<T> A) {
+  const int k = blockIdx.x * NT + threadIdx.x;
+  const int ray = blockIdx.y / A.b.nlam_b;
+  const int lam = A.b.lam0 + blockIdx.y % A.b.nlam_b;
+  const int q = ray * A.b.nlam + lam;
+  const size_t HW = size_t(A.H) * A.W;
+  if (k >= int(HW)) return;
+  const int ky = k / A.W, kx = k - ky * A.W;
+  const int beam = ray / A.bm.rays_per_beam;
+  const double k0 = A.bm.k0[lam];
+  const double bb = A.bm.beta[beam];
+  const bool pass = mode_index(kx, A.W) <= A.amax[lam * A.H + ky];
+  const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
+  const double beta = (2 * M_PI / k0) * sqrt(fx * fx + fy * fy);
+  const double fr = sqrt(fx * fx + fy * fy);
+  const double c = fr > 0. ? fx / fr : 1., sn = fr > 0. ? fy / fr : 0.;
+  const size_t o = size_t(q) * 2 * HW + k;
+  cd u0 = mk<double>(0.), u1 = mk<double>(0.), r0 = mk<double>(0.), r1 = mk<double>(0.);
+  cd g0 = mk<double>(0.), g1 = mk<double>(0.), d0 = mk<double>(0.), d1 = mk<double>(0.);
+  if (pass) {
+    cd pa = mk<double>(0.), pb = mk<double>(0.);
+    if (A.da != 0.)
+      pa = (csqrt(A.ea - mk<double>(beta * beta)) - csqrt(A.ea - mk<double>(bb * bb))) *
+           (0.5 * A.da * k0);
+    if (A.db != 0.)
+      pb = (csqrt(A.eb - mk<double>(beta * beta)) - csqrt(A.eb - mk<double>(bb * bb))) *
+           (0.5 * A.db * k0);
+    const cd Ka = cexpi(pa) * A.scale, Kb = cexpi(pb);
+    u0 = cast<double>(A.src[o]) * Ka;
+    u1 = cast<double>(A.src[o + HW]) * Ka;
+    cd t0 = u0, t1 = u1;
+    if (A.fresnel) {
+      cd tp, ts;
+      fresnel_t(beta, A.ea, A.eb, tp, ts);
+      const cd ep = (u0 * c + u1 * sn) * tp, es = (u1 * c - u0 * sn) * ts;
+      t0 = ep * c - es * sn;
+      t1 = ep * sn + es * c;
+      r0 = cast<double>(A.refl[o]);
+      r1 = cast<double>(A.refl[o + HW]);
+    }
+    g0 = t0 + r0;
+    g1 = t1 + r1;
+    d0 = g0 * Kb;
+    d1 = g1 * Kb;
+    u0 = t0 - u0;  // the new stored reflection
+    u1 = t1 - u1;
+  }
+  if (A.fresnel) {
+    A.refl[o] = cast<T>(u0);
+    A.refl[o + HW] = cast<T>(u1);
+  }
+  if (A.g) {
+    A.g[o] = cast<T>(g0);
+    A.g[o + HW] = cast<T>(g1);
+  }
+  A.dst[o] = cast<T>(d0);
+  A.dst[o + HW] = cast<T>(d1);
 }
 
 // ---------------------------------------------------------------------------
@@ -2132,8 +2252,19 @@ __global__ __launch_bounds__(NT) void e2h_kernel(E2HArgs<T> A) {
 //               Eti = St^-1 A^-1 (dtmm/tmm.py:999-1022,
 //               dtmm/propagate_2x2.py:172-177); the current layer may be the
 //               isotropic output medium (emed, cur_iso).
+//   PIX_TRR:    2x2 reflection mode 2 with stored reflections (npass > 1,
+//               dtmm/propagate_2x2.py:120-151): at the interface from the
+//               previous layer's voxel (or isotropic medium) into this one,
+//               t = e_out St^-1 A^-1 of tr_mat (dtmm/tmm.py:925-950) for the
+//               modes of `mode`; refl <- field - t field, field <- t field +
+//               (old refl); with dst (mode -1) that field and its H are added
+//               to dst; then the half step e diag(p) e^-1 of this layer.
+// PIX_LAYER2 and PIX_TRR use the backward modes for mode -1; PIX_E2H and
+// PIX_OUT2 then build H of the backward modes, and with accum add the
+// 4-component result to dst instead of writing it.  PIX_IN2 with dst also
+// writes the whole 4-component forward part there (dst may be src).
 enum { PIX_LAYER4 = 0, PIX_LAYER2 = 1, PIX_IN2 = 2, PIX_OUT2 = 3, PIX_A4 = 4, PIX_B4 = 5,
-       PIX_E2H = 6, PIX_T2 = 7 };
+       PIX_E2H = 6, PIX_T2 = 7, PIX_TRR = 8 };
 
 template <typename T>
 struct PixArgs {
@@ -2157,6 +2288,9 @@ struct PixArgs {
   const double* pang;
   int prev_iso, cur_iso;
   cd pemed;
+  int mode;             // +1 (or 0): forward modes; -1: backward modes (npass > 1)
+  int accum;            // OUT2 / E2H: add to dst
+  cpx<T>* refl;         // TRR: stored reflection of this interface, field layout
   Beams bm;
 };
 
@@ -2291,11 +2425,12 @@ __global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
     } else {
       RopData r;
       rop_full(beta, c0, s0, phi0, e0, e1, e2, psi, theta, phi, r);
-      const T ok = r.ok2 ? T(1) : T(0);
-      const cpx<T> a0 = cast<T>(r.a[0]), a2 = cast<T>(r.a[2]);
+      const bool bw = A.mode < 0;
+      const T ok = (bw ? r.okb : r.ok2) ? T(1) : T(0);
+      const cpx<T> a0 = cast<T>(r.a[bw ? 1 : 0]), a2 = cast<T>(r.a[bw ? 3 : 2]);
       cpx<T> P[2][2];
       for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) P[i][j] = cast<T>(r.P0[i][j]);
+        for (int j = 0; j < 2; ++j) P[i][j] = cast<T>(bw ? r.P1[i][j] : r.P0[i][j]);
       for (int lam = 0; lam < A.nlam; ++lam) {
         const T kd = T(A.bm.k0[lam] * A.d);
         const cpx<T> p0 = cexpi_layer(a0 * kd);
@@ -2355,6 +2490,77 @@ __global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
         f[A.HW] = M[1][0] * ex + M[1][1] * ey;
       }
     }
+  } else if constexpr (PK == PIX_TRR) {
+    Modes mi, mo;
+    int kind;
+    if (A.prev_iso)
+      layer_modes(beta, c0, s0, phi0, A.pemed, A.pemed, A.pemed, 0., 0., 0., mi, kind);
+    else
+      pixel_modes(A.peps_re, A.peps_im, A.pang, A.comp_stride, px, beta, c0, s0, phi0, mi);
+    if (A.cur_iso)
+      layer_modes(beta, c0, s0, phi0, A.emed, A.emed, A.emed, 0., 0., 0., mo, kind);
+    else
+      pixel_modes(A.eps_re, A.eps_im, A.ang, A.comp_stride, px, beta, c0, s0, phi0, mo);
+    // columns of the modes of this direction (dtmm/tmm.py:850-878 S_mat,
+    // :974-997 E_mat)
+    const int q0 = A.mode < 0 ? 1 : 0, q1 = A.mode < 0 ? 3 : 2;
+    const int qs[2] = {q0, q1};
+    cd fi[4][4];
+    inv4(mi.F, fi);
+    cd St[2][2], Am[2][2], e[2][2];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        cd t = mk<double>(0.);
+        for (int k = 0; k < 4; ++k) t = t + fi[qs[i]][k] * mo.F[k][qs[j]];
+        St[i][j] = t;
+        Am[i][j] = mi.F[2 * i][qs[j]];
+        e[i][j] = mo.F[2 * i][qs[j]];
+      }
+    cd Sti[2][2], Ai[2][2], ei[2][2];
+    inv2(St, Sti);
+    inv2(Am, Ai);
+    inv2(e, ei);
+    cd Eti[2][2], Tm[2][2], G[2][2];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) Eti[i][j] = Sti[i][0] * Ai[0][j] + Sti[i][1] * Ai[1][j];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        Tm[i][j] = e[i][0] * Eti[0][j] + e[i][1] * Eti[1][j];
+        // E2H = B e^-1, B the H rows of the same columns (dtmm/tmm.py:1051-1072)
+        G[i][j] = mo.F[2 * i + 1][q0] * ei[0][j] + mo.F[2 * i + 1][q1] * ei[1][j];
+      }
+    cpx<T> Tt[2][2], Gt[2][2];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        Tt[i][j] = cast<T>(Tm[i][j]);
+        Gt[i][j] = cast<T>(G[i][j]);
+      }
+    for (int lam = 0; lam < A.nlam; ++lam) {
+      const double kd = A.bm.k0[lam] * A.d;
+      const cd p0 = A.d != 0. ? cexpi(mo.a[q0] * kd) : mk<double>(1.);
+      const cd p1 = A.d != 0. ? cexpi(mo.a[q1] * kd) : mk<double>(1.);
+      cpx<T> L[2][2];
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) L[i][j] = cast<T>(e[i][0] * p0 * ei[0][j] + e[i][1] * p1 * ei[1][j]);
+      for (int ray = r0; ray < r1; ++ray) {
+        cpx<T>* f = plane(A.field, ray, lam, 2);
+        cpx<T>* rf = plane(A.refl, ray, lam, 2);
+        const cpx<T> ex = f[0], ey = f[A.HW];
+        const cpx<T> tx = Tt[0][0] * ex + Tt[0][1] * ey, ty = Tt[1][0] * ex + Tt[1][1] * ey;
+        const cpx<T> gx = tx + rf[0], gy = ty + rf[A.HW];
+        rf[0] = ex - tx;
+        rf[A.HW] = ey - ty;
+        if (A.dst) {
+          cpx<T>* o = plane(A.dst, ray, lam, 4);
+          o[0] = o[0] + gx;
+          o[A.HW] = o[A.HW] + (Gt[0][0] * gx + Gt[0][1] * gy);
+          o[2 * A.HW] = o[2 * A.HW] + gy;
+          o[3 * A.HW] = o[3 * A.HW] + (Gt[1][0] * gx + Gt[1][1] * gy);
+        }
+        f[0] = L[0][0] * gx + L[0][1] * gy;
+        f[A.HW] = L[1][0] * gx + L[1][1] * gy;
+      }
+    }
   } else {
     // isotropic medium of the beam (PIX_E2H: the layer's voxel): F and its
     // inverse (fp64)
@@ -2375,42 +2581,58 @@ __global__ __launch_bounds__(NT) void pixel_kernel(PixArgs<T> A) {
     if constexpr (PK == PIX_IN2) {
       cd fi[4][4];
       inv4(m.F, fi);
-      // rows Ex, Ey of F diag(1, 0, 1, 0) F^-1
-      cpx<T> Pm[2][4];
-      for (int i = 0; i < 2; ++i)
+      // F diag(1, 0, 1, 0) F^-1 (rows Ex, Ey without dst)
+      cpx<T> Pm[4][4];
+      for (int i = 0; i < 4; ++i)
         for (int c = 0; c < 4; ++c)
-          Pm[i][c] = cast<T>(m.F[2 * i][0] * fi[0][c] + m.F[2 * i][2] * fi[2][c]);
+          Pm[i][c] = cast<T>(m.F[i][0] * fi[0][c] + m.F[i][2] * fi[2][c]);
       for (int lam = 0; lam < A.nlam; ++lam)
         for (int ray = r0; ray < r1; ++ray) {
           const cpx<T>* s = plane(const_cast<cpx<T>*>(A.src), ray, lam, 4);
-          cpx<T> v[4];
+          cpx<T> v[4], w[4];
           for (int c = 0; c < 4; ++c) v[c] = s[c * A.HW];
-          cpx<T>* f = plane(A.field, ray, lam, 2);
-          for (int i = 0; i < 2; ++i) {
+          for (int i = 0; i < 4; ++i) {
             cpx<T> t = Pm[i][0] * v[0];
             for (int c = 1; c < 4; ++c) t = t + Pm[i][c] * v[c];
-            f[i * A.HW] = t;
+            w[i] = t;
+          }
+          cpx<T>* f = plane(A.field, ray, lam, 2);
+          f[0] = w[0];
+          f[A.HW] = w[2];
+          if (A.dst) {
+            cpx<T>* o = plane(A.dst, ray, lam, 4);
+            for (int i = 0; i < 4; ++i) o[i * A.HW] = w[i];
           }
         }
     } else {
-      // E2H = B A^-1 with A, B the E and H rows of the forward columns
-      const cd Am[2][2] = {{m.F[0][0], m.F[0][2]}, {m.F[2][0], m.F[2][2]}};
+      // E2H = B A^-1 with A, B the E and H rows of the forward (mode -1:
+      // backward) columns
+      const int q0 = A.mode < 0 ? 1 : 0, q1 = A.mode < 0 ? 3 : 2;
+      const cd Am[2][2] = {{m.F[0][q0], m.F[0][q1]}, {m.F[2][q0], m.F[2][q1]}};
       cd Ai[2][2];
       inv2(Am, Ai);
       cpx<T> G[2][2];
       for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
-          G[i][j] = cast<T>(m.F[2 * i + 1][0] * Ai[0][j] + m.F[2 * i + 1][2] * Ai[1][j]);
+          G[i][j] = cast<T>(m.F[2 * i + 1][q0] * Ai[0][j] + m.F[2 * i + 1][q1] * Ai[1][j]);
       const int la0 = A.nlam_b ? A.lam0 : 0, la1 = A.nlam_b ? A.lam0 + A.nlam_b : A.nlam;
       for (int lam = la0; lam < la1; ++lam)
         for (int ray = r0; ray < r1; ++ray) {
           const cpx<T>* e = plane(const_cast<cpx<T>*>(A.src), ray, lam, 2);
           const cpx<T> ex = e[0], ey = e[A.HW];
           cpx<T>* o = plane(A.dst, ray, lam, 4);
-          o[0] = ex;
-          o[A.HW] = G[0][0] * ex + G[0][1] * ey;
-          o[2 * A.HW] = ey;
-          o[3 * A.HW] = G[1][0] * ex + G[1][1] * ey;
+          const cpx<T> hy = G[0][0] * ex + G[0][1] * ey, hx = G[1][0] * ex + G[1][1] * ey;
+          if (A.accum) {
+            o[0] = o[0] + ex;
+            o[A.HW] = o[A.HW] + hy;
+            o[2 * A.HW] = o[2 * A.HW] + ey;
+            o[3 * A.HW] = o[3 * A.HW] + hx;
+          } else {
+            o[0] = ex;
+            o[A.HW] = hy;
+            o[2 * A.HW] = ey;
+            o[3 * A.HW] = hx;
+          }
         }
     }
   }

@@ -3,12 +3,19 @@
 Mirrors dtmm/transfer.py:322 ``transfer_field`` (same arguments, defaults,
 return value and errors) for the modes built on the GPU:
 
-* method "4x4": reflection mode 1, diffraction >= 1, npass >= 1 (norm
-  DTMM_NORM_FFT or 0) (dtmm/transfer.py:618-884, dtmm/propagate_4x4.py:376-444,
-  :80-101);
-* method "2x2" (and the configured default method '"2x2"'): reflection
-  mode 0 or 1, diffraction 1, npass 1 (dtmm/transfer.py:1009-1183,
-  dtmm/propagate_2x2.py:193-211, :19-87).
+* method "4x4" (dtmm/transfer.py:618-884): reflection mode 1 with
+  diffraction >= 1 and any npass (norm DTMM_NORM_FFT or 0,
+  dtmm/propagate_4x4.py:376-444, :80-101); reflection mode 2 with
+  diffraction 0, 1 or the tent windows of diffraction > 1 and any npass
+  (:156-204); diffraction < 0 or inf, the exact per-mode propagation
+  (npass 1, :515-564);
+* method "2x2" (and the configured default method '"2x2"',
+  dtmm/transfer.py:1009-1183): reflection mode 0 (diffraction 0 or 1) and
+  1 (diffraction 1) with npass 1 (dtmm/propagate_2x2.py:193-211, :19-87);
+  reflection mode 2 with diffraction 0 or 1 (:91-190); npass > 1 with
+  reflection mode 1 (diffraction 1) or 2 (diffraction 0 or 1);
+* ret_bulk with diffraction 0 or 1 (4x4: npass 1), per-wavelength
+  transfers (split_wavelengths, tuples of fields, dispersive data).
 
 Everything else the reference accepts raises ``NotImplementedError``.  The
 propagation runs in ``libdtmmx.so`` (HIP, gfx950); torch provides device
@@ -329,6 +336,13 @@ class Plan:
         torch = _torch()
         return torch.complex64 if self.precision == "c64" else torch.complex128
 
+    def workspace_for(self, stack):
+        """Workspace bytes of a transfer of `stack`: the plan's part plus the
+        reflected wave of every interface that the 2x2 method keeps between
+        passes (npass > 1)."""
+        s = stack.c_struct()
+        return int(self._lib.dtmmx_transfer_workspace_size(self._h, ctypes.byref(s)))
+
     def execute(self, stack, field_in, field_out, workspace=None, status=None, stream=None):
         """Run the transfer on device tensors of shape (n_rays, n_wavelengths,
         4, H, W); asynchronous on `stream` (default: torch's current).  With
@@ -344,14 +358,14 @@ class Plan:
                                                                 t.is_contiguous(), t.device))
             if tuple(t.shape) != (self.n_rays, len(self.ks), 4) + self.shape:
                 raise ValueError("field shape %s does not match the plan" % (tuple(t.shape),))
-        if self.workspace_bytes and (workspace is None or
-                                     workspace.numel() * workspace.element_size() < self.workspace_bytes):
-            workspace = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=field_in.device)
+        need = self.workspace_for(stack)
+        if need and (workspace is None or workspace.numel() * workspace.element_size() < need):
+            workspace = torch.empty(need, dtype=torch.uint8, device=field_in.device)
         s = stack.c_struct()
         _lib.check(self._lib.dtmmx_transfer(
             self._h, ctypes.byref(s), field_in.data_ptr(), field_out.data_ptr(),
             workspace.data_ptr() if workspace is not None else None,
-            self.workspace_bytes, status.data_ptr() if status is not None else None,
+            need, status.data_ptr() if status is not None else None,
             stream.cuda_stream))
         return field_out
 
@@ -379,8 +393,8 @@ class Plan:
                      stream=None):
         """execute() that also writes the field after every stack layer into
         `bulk`, a contiguous device tensor (n_layers, n_rays, n_wavelengths,
-        4, H, W) of the plan's dtype (4x4 method, npass 1, diffraction 0 or
-        1; dtmm/transfer.py ret_bulk)."""
+        4, H, W) of the plan's dtype (diffraction 0 or 1; npass 1, or any
+        npass with the 2x2 method; dtmm/transfer.py ret_bulk)."""
         torch = _torch()
         if stream is None:
             stream = torch.cuda.current_stream(field_in.device)
@@ -389,13 +403,13 @@ class Plan:
                 tuple(bulk.shape) != want):
             raise ValueError("bulk must be a contiguous %s device tensor of shape %s"
                              % (self.dtype, want))
-        if self.workspace_bytes and (workspace is None or
-                                     workspace.numel() * workspace.element_size() < self.workspace_bytes):
-            workspace = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=field_in.device)
+        need = self.workspace_for(stack)
+        if need and (workspace is None or workspace.numel() * workspace.element_size() < need):
+            workspace = torch.empty(need, dtype=torch.uint8, device=field_in.device)
         s = stack.c_struct()
         _lib.check(self._lib.dtmmx_transfer_bulk(
             self._h, ctypes.byref(s), field_in.data_ptr(), field_out.data_ptr(), bulk.data_ptr(),
-            workspace.data_ptr() if workspace is not None else None, self.workspace_bytes,
+            workspace.data_ptr() if workspace is not None else None, need,
             status.data_ptr() if status is not None else None, stream.cuda_stream))
         return bulk
 
@@ -432,12 +446,19 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
     if npass < 1:
         raise ValueError("npass must be >= 1")
     if npass > 1 and kind == "2x2":
-        raise NotImplementedError("npass > 1 is supported for the 4x4 method only")
-    if npass > 1 and (norm & DTMM_NORM_REF):
+        if reflection == 0:
+            raise ValueError("Reflection mode `0` not compatible with npass > 1")
+        if not ((reflection == 1 and diffraction == 1) or
+                (reflection == 2 and diffraction in (0, 1))):
+            raise NotImplementedError("npass > 1 with the 2x2 method is supported for "
+                                      "reflection mode 1 with diffraction 1 and reflection "
+                                      "mode 2 with diffraction 0 or 1")
+    if npass > 1 and kind == "4x4" and (norm & DTMM_NORM_REF):
         raise NotImplementedError("npass > 1 with DTMM_NORM_REF (reference-field "
                                   "normalisation) is not supported")
-    if ret_bulk and (npass != 1 or not (0 <= diffraction <= 1)):
-        raise NotImplementedError("ret_bulk is supported with npass=1 and diffraction 0 or 1")
+    if ret_bulk and (not (0 <= diffraction <= 1) or (npass != 1 and kind == "4x4")):
+        raise NotImplementedError("ret_bulk is supported with diffraction 0 or 1, and npass=1 "
+                                  "for the 4x4 method")
     if eff_data not in (None, 0, "isotropic"):
         raise NotImplementedError("only isotropic effective data (eff_data=0) is supported")
     if diffraction < 0 or diffraction == np.inf:
@@ -449,9 +470,9 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
     if diffraction == 0:
         # no diffraction: every pixel is an independent stack (4x4 reflection
         # mode 2, the reference's default for diffraction 0; 2x2 reflection 0)
-        if npass != 1 and not (kind == "4x4" and reflection == 2):
-            raise NotImplementedError("diffraction=0 with npass > 1 is supported for the 4x4 "
-                                      "method in reflection mode 2")
+        if npass != 1 and reflection != 2:
+            raise NotImplementedError("diffraction=0 with npass > 1 is supported in reflection "
+                                      "mode 2")
         if kind == "4x4" and reflection != 2:
             raise NotImplementedError("4x4 method with diffraction=0: only reflection mode 2 "
                                       "is supported")
@@ -468,11 +489,11 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
             raise NotImplementedError("4x4 method: only diffraction 0 or finite diffraction >= 1 "
                                       "is supported")
     else:
-        if reflection == 2 and diffraction == 1 and npass == 1:
+        if reflection == 2 and diffraction == 1:
             return kind, reflection
         if reflection not in (0, 1):
             raise NotImplementedError("2x2 method: reflection mode 2 is supported with "
-                                      "diffraction 0 or 1 and npass=1")
+                                      "diffraction 0 or 1")
         if diffraction != 1:
             raise NotImplementedError("2x2 method: only diffraction 0 (reflection 0) or 1 is "
                                       "supported")
@@ -560,8 +581,9 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
     fin = fin.to(device="cuda", dtype=plan.dtype)
     fout = torch.empty_like(fin)
     ws = None
-    if plan.workspace_bytes:
-        ws = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device="cuda")
+    need = plan.workspace_for(stack)
+    if need:
+        ws = torch.empty(need, dtype=torch.uint8, device="cuda")
     return _Prepared(plan, stack, fin, fout, ws, field_in, field_np, npass, out, wavelengths,
                      pixelsize)
 
@@ -677,7 +699,9 @@ def _transfer_one(field_data, optical_data, beta=None, phi=None, nin=None, nout=
         shp = job.field_np.shape
         res = np.empty((nl + 2,) + shp, dtype=np.complex64 if job.plan.precision == "c64"
                        else np.complex128)
-        res[0] = job.field_np
+        # the input side: with npass > 1 its forward part plus the backward
+        # field of every backward pass (dtmm/transfer.py:1050, :1080)
+        res[0] = job.fin.cpu().numpy().reshape(shp) if job.npass > 1 else job.field_np
         res[1:nl + 1] = bulk.cpu().numpy().reshape((nl,) + shp)
         res[nl + 1] = job.fout.cpu().numpy().reshape(shp)
         if out is not None:
@@ -718,7 +742,7 @@ def _job_array(jobs):
         a.field_in = j.fin.data_ptr()
         a.field_out = j.fout.data_ptr()
         a.workspace = j.ws.data_ptr() if j.ws is not None else None
-        a.workspace_bytes = j.plan.workspace_bytes
+        a.workspace_bytes = j.ws.numel() if j.ws is not None else 0
     return arr, structs
 
 

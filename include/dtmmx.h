@@ -79,10 +79,14 @@ typedef struct {
                          the non-zero n x n tent windows of dtmm/mode.py,
                          each propagated with its window's mean beam
                          (propagate_4x4.py:390-419)                      */
-  int npass;          /* >= 1; > 1 (4x4 only): alternating forward and
-                         backward passes with the normalisations of
-                         transfer.py:815-872 (reflections)                */
-  int norm;           /* npass > 1: DTMMX_NORM_FFT (1, per-mode rescaling,
+  int npass;          /* >= 1; > 1: alternating forward and backward
+                         passes (reflections); 4x4 with the
+                         normalisations of transfer.py:815-872, 2x2
+                         (reflection 1 with diffraction 1, reflection 2
+                         with diffraction 0 or 1) with the reflected wave
+                         of every interface kept between passes
+                         (transfer.py:1076-1178)                          */
+  int norm;           /* 4x4, npass > 1: DTMMX_NORM_FFT (1, per-mode rescaling,
                          transfer.py:78-91) or 0 (total intensity,
                          transfer.py:206-221); DTMM_NORM_REF is not built */
   double smooth;      /* npass > 1: notch width of dtmm/denoise.py (dtmm.ini:8,
@@ -90,9 +94,9 @@ typedef struct {
   int reflection;     /* 4x4: 1, or 2 (real space, transfer.py:719-725; any
                          npass; diffraction > 1: the tent windows' parts,
                          propagate_4x4.py:176-204).
-                         2x2: 0; 2 (diffraction 0 or 1, npass 1: real
-                         space with the per-pixel interface transmission
-                         into every layer, propagate_2x2.py:91-190); or 1 =
+                         2x2: 0; 2 (diffraction 0 or 1: real space with
+                         the per-pixel interface transmission into every
+                         layer, propagate_2x2.py:91-190); or 1 =
                          Fresnel transmission at
                          interfaces between different effective media
                          (propagate_2x2.py:28-30, diffract.py:215-225)    */
@@ -138,6 +142,15 @@ int dtmmx_plan_destroy(dtmmx_plan* plan);
  * layers flagged biaxial are solved into. */
 size_t dtmmx_plan_workspace_size(const dtmmx_plan* plan);
 
+/* Device scratch bytes of a transfer of this stack: the plan's part plus
+ * what depends on the stack, the reflected 2-component wave of each of the
+ * n_layers + 1 interfaces that the 2x2 method with npass > 1 keeps between
+ * passes (the refl array of transfer.py:1081-1083).  Equal to
+ * dtmmx_plan_workspace_size for every other plan.  dtmmx_transfer,
+ * dtmmx_transfer_bulk and dtmmx_transfer_batch check the workspace against
+ * it. */
+size_t dtmmx_transfer_workspace_size(const dtmmx_plan* plan, const dtmmx_stack* stack);
+
 /* Launch shapes of the plan's per-layer kernels: col_lines = field lines
  * per column work-group (1, 2, 4, or 8 when two rays share a beam; lines
  * that do not divide the work fall back to the automatic choice),
@@ -164,6 +177,14 @@ int dtmmx_plan_get_tiles(const dtmmx_plan* plan, int* col_lines, int* row_rays);
  *   2x2 (reflection mode 0 or 1, diffraction 1, npass 1): replaces
  *       transfer.py:1071-1176 with propagate_2x2.py:193-211 and :19-87 per
  *       interface.
+ *   2x2 (npass > 1; reflection mode 1 with diffraction 1, mode 2 with
+ *       diffraction 0 or 1): the pass loop of transfer.py:1105-1178 with
+ *       propagate_2x2.py:19-87 (mode 1, k space) or :91-190 (mode 2, real
+ *       space) and the reflected wave of every interface kept between
+ *       passes; backward passes propagate the backward modes of each voxel.
+ *       Reflection mode 2 keeps field - t.field at each interface, minus
+ *       the reflected wave, as the reference does, so its reflected field
+ *       has the opposite sign of mode 1's.
  * field_in and field_out are device buffers of n_rays*n_wavelengths*4*H*W
  * complex values; they may not alias.  With npass == 1 field_in is not
  * modified; with npass > 1 it receives the field at the input side
@@ -198,8 +219,8 @@ int dtmmx_propagate_layer(const dtmmx_plan* plan, const dtmmx_stack* stack, cons
  * library holds on every GPU it runs on until the process ends; the call
  * allocates nothing, and the caller passes only the plan, the stack, the
  * fields and the stream.  It fails with DTMMX_E_ARG, before any work, when
- * dtmmx_plan_workspace_size(plan) exceeds the array.  Every call on one GPU
- * shares that GPU's array: calls that overlap on one GPU (concurrent
+ * dtmmx_transfer_workspace_size(plan, stack) exceeds the array.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (concurrent
  * streams or threads) race and must be serialised by the caller.  No kernel
  * relies on the array's contents at the start of a call. */
 #define DTMMX_NOALLOC_SCRATCH_BYTES (128u << 20)
@@ -213,8 +234,12 @@ int dtmmx_transfer_noalloc(const dtmmx_plan* plan, const dtmmx_stack* stack, voi
  * bulk_out[1 .. n_layers] these slots are; bulk_out[0] and bulk_out[-1] are
  * field_in and field_out.  The 2x2 method's slots (transfer_2x2,
  * transfer.py:1045-1050, propagate_2x2.py:80-85) hold E after each layer and
- * the H of each voxel's forward modes.  Built with npass 1 and diffraction 0
- * or 1 (4x4: reflection 1 or 2; 2x2: reflection 0, 1 or 2). */
+ * the H of each voxel's forward modes; with npass > 1 each slot sums that
+ * over the forward passes and, over the backward passes, the field entering
+ * the layer from the next one with the H of its backward modes
+ * (propagate_2x2.py:54-57, :136-142).  Built with diffraction 0 or 1 and
+ * npass 1 (4x4: reflection 1 or 2; 2x2: reflection 0, 1 or 2) or any npass
+ * (2x2 as dtmmx_transfer). */
 int dtmmx_transfer_bulk(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field_in,
                         void* field_out, void* bulk, void* workspace, size_t workspace_bytes,
                         int* status, void* stream);

@@ -89,6 +89,8 @@ def _plan_rc(**kw):
     (dict(method=2, npass=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(npass=3, norm=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(method=2, reflection=0, npass=2), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(method=2, reflection=1, diffraction=0, npass=2), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(method=2, reflection=2, diffraction=3, npass=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(reflection=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(method=2, reflection=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
 ])
@@ -358,13 +360,22 @@ def test_mode_resolution_no_diffraction():
     assert _resolve_modes("4x4", 3, 0, 2, None, False) == ("4x4", 2)
     with pytest.raises(NotImplementedError):
         _resolve_modes("4x4", 3, 0, None, None, False)  # npass > 1 selects reflection 1
-    with pytest.raises(NotImplementedError):
-        _resolve_modes("2x2", 2, 0, 0, None, False)
+    with pytest.raises(ValueError):
+        _resolve_modes("2x2", 2, 0, 0, None, False)  # as dtmm/transfer.py:1077-1078
     assert _resolve_modes("2x2", 1, 0, 2, None, False) == ("2x2", 2)
+    # 2x2 with npass > 1: reflection 1 (the default, dtmm/transfer.py:459-463)
+    # with diffraction 1, reflection 2 with diffraction 0 or 1
+    assert _resolve_modes("2x2", 3, 1, None, None, False) == ("2x2", 1)
+    assert _resolve_modes("2x2", 3, 1, 2, None, False) == ("2x2", 2)
+    assert _resolve_modes("2x2", 4, 0, 2, None, True) == ("2x2", 2)
+    with pytest.raises(NotImplementedError):
+        _resolve_modes("2x2", 3, 0, 1, None, False)
+    with pytest.raises(NotImplementedError):
+        _resolve_modes("2x2", 3, 3, None, None, False)  # selects reflection 2, windows
     assert _resolve_modes("2x2", 1, 1, 2, None, False) == ("2x2", 2)
     with pytest.raises(NotImplementedError):
         _resolve_modes("2x2", 1, 3, 2, None, False)
-    # ret_bulk: npass 1 and diffraction 0 or 1 only
+    # ret_bulk: diffraction 0 or 1; the 4x4 method with npass 1
     assert _resolve_modes("4x4", 1, 1, None, None, True) == ("4x4", 1)
     assert _resolve_modes("2x2", 1, 1, None, None, True) == ("2x2", 0)
     with pytest.raises(NotImplementedError):

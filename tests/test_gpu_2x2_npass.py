@@ -1,0 +1,142 @@
+"""GPU parity of the 2x2 method with npass > 1 (dtmm/transfer.py:1105-1178):
+reflection mode 1 (k space, diffraction 1) and mode 2 (real space,
+diffraction 0 and 1), against the complex128 oracle on the same inputs.
+
+Each case compares the output field, the caller's input array (its forward
+part plus the backward field of every backward pass, written in place as
+the reference does) and, with ret_bulk, every bulk slot.  Two blocks with
+different effective media make reflection mode 1 reflect inside the stack;
+oblique beams and a biaxial layer exercise the backward modes of each voxel.
+
+Tolerances (max|F - F_ref| / max|F_ref|): c128 1e-9, c64 1e-4.
+"""
+import numpy as np
+import pytest
+
+import dtmm_amd as dm
+import oracle
+from dtmm_amd.samples import random_director_data
+from helpers import flush, rel_err
+
+TOL = {"c128": 1e-9, "c64": 1e-4}
+CASES = [(1, 1), (2, 1), (2, 0)]
+
+
+def _biaxial_block(shape, seed):
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape
+    eps = np.empty(shape + (3,))
+    eps[..., 0] = rng.uniform(2.2, 2.3, shape)
+    eps[..., 1] = eps[..., 0] + rng.uniform(0.05, 0.1, shape)
+    eps[..., 2] = eps[..., 1] + rng.uniform(0.1, 0.2, shape)
+    ang = np.stack([rng.uniform(0, np.pi, shape), rng.uniform(0.2, 1.3, shape),
+                    rng.uniform(-np.pi, np.pi, shape)], -1)
+    return np.full(nz, 1.0), eps, ang
+
+
+def _case(shape=(3, 24, 40), wavelengths=(450., 650.), beta=0.1, phi=0.5, biaxial=False):
+    nz, ny, nx = shape
+    blocks = [random_director_data(shape, seed=3, no=1.5, ne=1.7),
+              random_director_data((2, ny, nx), seed=7, no=1.9, ne=2.05)]
+    if biaxial:
+        blocks.append(_biaxial_block((2, ny, nx), 11))
+    fd = dm.illumination_data((ny, nx), list(wavelengths), pixelsize=100, beta=beta, phi=phi,
+                              jones=None, window=dm.blackman((ny, nx)))
+    return fd, blocks
+
+
+def _compare(fd, optical, precision, **kw):
+    field, wl, ps = fd
+    ref_in = np.array(field, np.complex128)
+    ref = oracle.transfer_field((ref_in, wl, ps), optical, method="2x2", **kw)[0]
+    dt = np.complex64 if precision == "c64" else np.complex128
+    gin = np.array(field, dt)
+    flush("  GPU 2x2 npass %s field %s %s" % (kw, field.shape, precision))
+    out = dm.transfer_field((gin, wl, ps), optical, method="2x2", precision=precision, **kw)[0]
+    assert out.shape == ref.shape
+    err = rel_err(out, ref)
+    flush("  rel output err %.3e" % err)
+    assert np.isfinite(err) and err < TOL[precision], err
+    if not kw.get("ret_bulk"):
+        # the input side: forward part plus the reflected field
+        ein = rel_err(gin, ref_in)
+        refl = rel_err(gin - field, ref_in - field)
+        flush("  rel input-side err %.3e (reflected part %.3e)" % (ein, refl))
+        assert ein < TOL[precision], ein
+        assert refl < 10 * TOL[precision], refl
+        # the check is not vacuous: something was reflected
+        assert np.abs(ref_in - field).max() > 1e-3 * np.abs(field).max()
+    return err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("npass", [2, 3])
+@pytest.mark.parametrize("reflection,diffraction", CASES)
+def test_two_blocks_oblique(reflection, diffraction, npass, precision):
+    flush("case 2x2 npass two blocks refl %d diff %d npass %d %s"
+          % (reflection, diffraction, npass, precision))
+    fd, optical = _case()
+    _compare(fd, optical, precision, beta=0.1, phi=0.5, nin=1.0, nout=1.25, npass=npass,
+             diffraction=diffraction, reflection=reflection)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("reflection,diffraction", CASES)
+def test_ret_bulk_npass4(reflection, diffraction):
+    flush("case 2x2 npass bulk refl %d diff %d" % (reflection, diffraction))
+    fd, optical = _case(shape=(2, 16, 32), wavelengths=(520.,))
+    _compare(fd, optical, "c128", beta=0.1, phi=0.5, nin=1.0, nout=1.25, npass=4,
+             diffraction=diffraction, reflection=reflection, ret_bulk=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("reflection,diffraction", CASES)
+def test_biaxial_layer_and_substeps(reflection, diffraction):
+    flush("case 2x2 npass biaxial nstep 2 refl %d diff %d" % (reflection, diffraction))
+    fd, optical = _case(shape=(2, 16, 32), wavelengths=(520.,), biaxial=True)
+    _compare(fd, optical, "c128", beta=0.12, phi=-0.7, nin=1.1, nout=1.3, npass=5, nstep=2,
+             diffraction=diffraction, reflection=reflection)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("reflection", [1, 2])
+def test_multi_beam(reflection):
+    flush("case 2x2 npass multi beam refl %d" % reflection)
+    beta = np.array([0., 0.15])
+    phi = np.array([0., 1.0])
+    ny, nx = 24, 32
+    blocks = [random_director_data((3, ny, nx), seed=5, no=1.5, ne=1.7)]
+    fd = dm.illumination_data((ny, nx), [480., 600.], pixelsize=100, beta=beta, phi=phi,
+                              jones=None, window=dm.blackman((ny, nx)))
+    _compare(fd, blocks, "c128", beta=beta, phi=phi, nin=1.0, nout=1.4, npass=3,
+             diffraction=1, reflection=reflection)
+
+
+@pytest.mark.gpu
+def test_default_reflection_mode_is_1():
+    """npass > 1 without a reflection mode picks mode 1 for the 2x2 method
+    (dtmm/transfer.py:459-463)."""
+    flush("case 2x2 npass default reflection")
+    fd, optical = _case(shape=(2, 16, 32), wavelengths=(520.,))
+    field, wl, ps = fd
+    a = dm.transfer_field((field.copy(), wl, ps), optical, method="2x2", npass=3, beta=0.1,
+                          phi=0.5, nin=1.0, nout=1.25, precision="c128")[0]
+    b = dm.transfer_field((field.copy(), wl, ps), optical, method="2x2", npass=3, beta=0.1,
+                          phi=0.5, nin=1.0, nout=1.25, precision="c128", reflection=1)[0]
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_workspace_size_counts_interfaces():
+    """The reflected wave of each of the n_layers + 1 interfaces is part of
+    the workspace of a multi-pass 2x2 transfer, and only of that."""
+    shape = (16, 32)
+    ks = dm.k0(np.array([500., 600.]), 100.)
+    optical = [random_director_data((3,) + shape, seed=2)]
+    stack = dm.Stack(optical, shape)
+    p1 = dm.Plan(shape, ks, np.zeros(1), np.zeros(1), 2, 2, "2x2", "c64", npass=1, reflection=2)
+    p3 = dm.Plan(shape, ks, np.zeros(1), np.zeros(1), 2, 2, "2x2", "c64", npass=3, reflection=2)
+    assert p1.workspace_for(stack) == p1.workspace_bytes
+    slot = (2 * len(ks) * 2 * shape[0] * shape[1] * 8 + 255) // 256 * 256
+    assert p3.workspace_for(stack) == p3.workspace_bytes + 4 * slot
