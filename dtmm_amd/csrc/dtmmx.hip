@@ -2540,9 +2540,15 @@ static int upload_full_modes(dtmmx_plan* P) {
 
 static int validate_desc(const dtmmx_plan_desc* d) {
   if (d->height <= 0 || d->width <= 0) return fail(DTMMX_E_ARG, "empty grid");
-  if (!size_supported(d->height) || !size_supported(d->width))
+  // without diffraction every pixel is an independent stack and no transform
+  // runs (the 4x4 passes' normalisations aside): any grid, down to the single
+  // pixel of the reference's 1D examples
+  const bool no_fft = d->diffraction == 0 && !(d->method == DTMMX_METHOD_4X4 && d->npass > 1);
+  if (!no_fft && (!size_supported(d->height) || !size_supported(d->width)))
     return fail(DTMMX_E_UNSUPPORTED, "grid " + std::to_string(d->height) + "x" +
                                          std::to_string(d->width) + " not built");
+  if (size_t(d->height) * size_t(d->width) > (size_t(1) << 28))
+    return fail(DTMMX_E_ARG, "grid above 2^28 pixels");
   if (d->n_wavelengths <= 0 || d->n_wavelengths > MAXL)
     return fail(DTMMX_E_ARG, "n_wavelengths out of range");
   if (d->n_rays <= 0 || d->rays_per_beam <= 0 || d->n_rays % d->rays_per_beam != 0)

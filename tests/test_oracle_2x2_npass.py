@@ -250,3 +250,24 @@ def test_npass_translation_equivariant(method_reflection):
     roll = lambda a: np.roll(np.roll(a, dy, -2), dx, -1)  # noqa: E731
     np.testing.assert_allclose(out_r, roll(out), atol=1e-12 * np.abs(out).max())
     np.testing.assert_allclose(fb, roll(fa), atol=1e-12 * np.abs(fa).max())
+
+
+def test_4x4_passes_without_betamax_are_ill_conditioned():
+    """The reference's 4x4 multi-pass transfer with betamax inf
+    (examples/3D/reflections_2x2vs4x4.py:32) normalises every mode by the
+    ratio of two intensities (dtmm/transfer.py:78-91) that are both zero,
+    up to rounding, for the modes evanescent in the cover media: a relative
+    input change of 1e-14 moves the output by more than 1e-3 of its size,
+    so results at that setting are defined only up to rounding.  The 2x2
+    passes have no normalisation and stay stable."""
+    import dtmm_amd as dm
+    optical = [dm.nematic_droplet_data((4, 16, 16), 5, profile="r", no=3.5, ne=3.6, nhost=3.5)]
+    f, wl, ps = dm.illumination_data((16, 16), [380., 780.], beta=0., pixelsize=200)
+    rng = np.random.default_rng(0)
+    fp = f * (1 + 1e-14 * rng.standard_normal(f.shape))
+    for method, unstable in (("4x4", True), ("2x2", False)):
+        kw = dict(method=method, beta=0., phi=0., betamax=np.inf, npass=3, reflection=2)
+        a = o.transfer_field((f.copy(), wl, ps), optical, **kw)[0]
+        b = o.transfer_field((fp.copy(), wl, ps), optical, **kw)[0]
+        change = np.abs(b - a).max() / np.abs(a).max()
+        assert (change > 1e-3) if unstable else (change < 1e-10), (method, change)
