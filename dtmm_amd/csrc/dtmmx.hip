@@ -409,6 +409,9 @@ static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, 
     case KOP_FIELD4:
       hipLaunchKernelGGL((kspace_kernel<T, KOP_FIELD4, false>), grid, dim3(NT), 0, s, a);
       break;
+    case KOP_E2H4:
+      hipLaunchKernelGGL((kspace_kernel<T, KOP_E2H4, false>), grid, dim3(NT), 0, s, a);
+      break;
     default:
       timer_stop(s);
       return fail(DTMMX_E_ARG, "bad k-space operator");
@@ -467,6 +470,7 @@ struct WsLayout {
   size_t etab;                     // per-pixel data of a layer with biaxial voxels
   size_t full_spec, full_part;     // diffraction < 0: spectrum, per-chunk partial sums
   size_t tmp2;                     // 2x2 npass > 1: a 2-component field
+  size_t refb;                     // 4x4 npass > 1 with DTMMX_NORM_REF: reference field
   size_t total;                    // the stored reflections of 2x2 npass > 1 follow
                                    // (stack_ws_bytes)
   int nblk;
@@ -490,6 +494,10 @@ static WsLayout ws_layout(const dtmmx_plan* P) {
     w.pass_field0 = take(f4);
     w.part = take(2 * planes * size_t(w.nblk) * sizeof(double));
     w.sums = take(4 * planes * sizeof(double));
+    if (P->d.norm & DTMMX_NORM_REF) {
+      w.refb = take(f4);
+      w.e2 = take(planes * 2 * HW * cb);
+    }
   }
   if (m4 && P->wn > 1) {
     w.win_tmp = take(f4 * P->wn);
@@ -944,8 +952,78 @@ static int sum_rows(const double* part, int rows, int nb, double scale, double* 
   return DTMMX_OK;
 }
 
-// npass > 1 (dtmm/transfer.py:697-872, norm "fft" or "total", no reference
-// field): forward and backward passes with the output normalised onto the
+// The reference field of DTMMX_NORM_REF through one forward pass
+// (dtmm/transfer.py:769-775, :805-806): its E components propagated with
+// the 2x2 method without reflections in real space (propagate_2x2.py
+// :284-305 with :91-190: per sub-step a half step e P e^-1 of each voxel's
+// forward modes, the corrected E diffraction of the layer's effective
+// medium, a second half step), then H of the forward modes of the output
+// medium from E on the FFT grid (jones2H, zero above betamax).
+template <typename T>
+static int ref_forward(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* refb, cpx<T>* e2,
+                       hipStream_t s) {
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const int nlam = P->d.n_wavelengths;
+  const size_t nplanes = size_t(P->d.n_rays) * nlam;
+  const int nbeam = P->d.n_rays / P->d.rays_per_beam;
+  const Batch b{P->d.n_rays, nlam, 0, nlam};
+  const double scale = 1.0 / double(HW);
+  const size_t cb = sizeof(cpx<T>);
+  HIP_TRY(hipMemcpy2DAsync(e2, HW * cb, refb, 2 * HW * cb, HW * cb, nplanes * 2,
+                           hipMemcpyDeviceToDevice, s));
+  PixArgs<T> a{};
+  a.field = e2;
+  a.HW = HW;
+  a.nlam = nlam;
+  a.bm = P->bm;
+  a.comp_stride = size_t(S->n_layers) * HW;
+  a.mode = 1;
+  const dim3 grid(unsigned((HW + NT - 1) / NT), unsigned(nbeam));
+  for (int l = 0; l < S->n_layers; ++l) {
+    const int ns = S->n_steps ? S->n_steps[l] : 1;
+    const double ds = S->thickness[l] / double(ns);
+    const cd ee = mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
+    a.eps_re = S->eps_re + size_t(l) * HW;
+    a.eps_im = S->eps_im ? S->eps_im + size_t(l) * HW : nullptr;
+    a.ang = S->angles + size_t(l) * HW;
+    a.d = 0.5 * ds;
+    for (int q = 0; q < ns; ++q) {
+      if (ds != 0.) {
+        hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER2>), grid, dim3(NT), 0, s, a);
+        HIP_TRY(hipGetLastError());
+      }
+      if (P->d.diffraction) {
+        TRY(row_pass<T>(P, e2, b, 2, OP_FWD, s));
+        ColArgs<T> ca{};
+        ca.da = ds;
+        ca.db = ds;
+        ca.ea = ee;
+        ca.eb = ee;
+        ca.kca = 1.;
+        ca.kcb = 1.;
+        ca.scale = scale;
+        ca.dense_sign = 0;
+        TRY(col_pass<T>(P, e2, b, 2, OP_DIAG, s, &ca));
+        TRY(row_pass<T>(P, e2, b, 2, OP_INV, s));
+      }
+      if (ds != 0.) {
+        hipLaunchKernelGGL((pixel_kernel<T, PIX_LAYER2>), grid, dim3(NT), 0, s, a);
+        HIP_TRY(hipGetLastError());
+      }
+    }
+  }
+  HIP_TRY(hipMemcpy2DAsync(refb, 2 * HW * cb, e2, HW * cb, HW * cb, nplanes * 2,
+                           hipMemcpyDeviceToDevice, s));
+  TRY(fft2_planes<T>(P, refb, 0, s));
+  KArgs<T> k{};
+  k.kop = KOP_E2H4;
+  k.e0 = mk<double>(S->n_out * S->n_out);
+  k.scale = scale;
+  TRY(kspace<T>(P, refb, refb, b, k, s));
+  return fft2_planes<T>(P, refb, 1, s);
+}
+
+// npass > 1 (dtmm/transfer.py:697-872): forward and backward passes with the output normalised onto the
 // forward modes of the output medium and the input-side field rescaled to
 // the incoming intensity between passes.
 // One pass of 4x4 reflection mode 2 in place (defined below).
@@ -973,6 +1051,12 @@ static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin
   const double smooth = P->d.smooth < 0. ? 0. : P->d.smooth;
   const cd ein = mk<double>(S->n_in * S->n_in), eout = mk<double>(S->n_out * S->n_out);
   const unsigned eb = unsigned((n + NT - 1) / NT);
+  // DTMMX_NORM_REF: the reference starts as the input field
+  // (transfer.py:710-713)
+  const bool cref = (P->d.norm & DTMMX_NORM_REF) != 0;
+  cpx<T>* refb = cref ? (cpx<T>*)(wsb + wl.refb) : nullptr;
+  cpx<T>* e2r = cref ? (cpx<T>*)(wsb + wl.e2) : nullptr;
+  if (cref) HIP_TRY(hipMemcpyAsync(refb, fin, bytes, hipMemcpyDeviceToDevice, s));
   // field0 = transmitted_field(field_in, nin); i0 = its total intensity
   HIP_TRY(hipMemcpyAsync(f0, fin, bytes, hipMemcpyDeviceToDevice, s));
   TRY(fft2_planes<T>(P, f0, 0, s));
@@ -990,8 +1074,13 @@ static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin
   // reflection mode 2 propagates in real space (work_in_fft False,
   // dtmm/transfer.py:719-725); the normalisations between passes are the same
   const bool r2 = P->d.method == DTMMX_METHOD_4X4 && P->d.reflection == 2;
+  // the reference's amplitude factor before each normalisation
+  // (transfer.py:822-823)
+  const double ref_scale = std::sqrt(S->n_in / S->n_out);
   for (int i = 0; i < npass; ++i) {
     const int dir = (i % 2 == 0) ? 1 : -1;
+    // the reference is read only by the normalisation after this pass
+    if (cref && dir > 0 && i != npass - 1) TRY(ref_forward<T>(P, S, refb, e2r, s));
     if (r2)
       TRY(refl2_pass<T>(P, S, g, s, dir, nullptr, ws));
     else
@@ -1008,7 +1097,39 @@ static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin
           a.x0 = S->n_in;
           a.sigma = sigma;
           a.scale = scale;
+          if (cref) {
+            TRY(fft2_planes<T>(P, refb, 0, s));
+            a.ref = refb;
+            a.ref_scale = ref_scale;
+          }
           TRY(pass_op<T>(P, g, a, s));
+        } else if (cref) {
+          // "local": forward part in the output medium, per-pixel
+          // rescaling against the reference, then the notch filter
+          // (transfer.py:103-138, denoise.py:24-28)
+          PArgs<T> a{};
+          a.pop = POP_PROJ;
+          a.eps = eout;
+          a.scale = scale;
+          TRY(pass_op<T>(P, g, a, s));
+          TRY(fft2_planes<T>(P, g, 1, s));
+          LocalArgs<T> la{};
+          la.f = g;
+          la.ref = refb;
+          la.ref_scale = ref_scale;
+          la.HW = HW;
+          la.eps = eout;
+          hipLaunchKernelGGL((local_norm_kernel<T>), dim3(unsigned((HW + NT - 1) / NT), nplanes),
+                             dim3(NT), 0, s, la);
+          HIP_TRY(hipGetLastError());
+          TRY(fft2_planes<T>(P, g, 0, s));
+          PArgs<T> d{};
+          d.pop = POP_NOTCH;
+          d.notch = 1;
+          d.x0 = S->n_in;
+          d.sigma = sigma;
+          d.scale = scale;
+          TRY(pass_op<T>(P, g, d, s));
         } else {
           PArgs<T> a{};
           a.pop = POP_TOTALSUM;
@@ -1048,6 +1169,9 @@ static int transfer_npass(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* fin
         hipLaunchKernelGGL((reflect_update_kernel<T>), dim3(eb), dim3(NT), 0, s, g, fout, fin,
                            (const cpx<T>*)f0, (const double*)i0, (const double*)i0f, 4 * HW, n);
         HIP_TRY(hipGetLastError());
+        // the next forward pass starts its reference from this field
+        // (transfer.py:867-868)
+        if (cref) HIP_TRY(hipMemcpyAsync(refb, g, bytes, hipMemcpyDeviceToDevice, s));
       }
     }
   }
@@ -2590,8 +2714,12 @@ static int validate_desc(const dtmmx_plan_desc* d) {
     return fail(DTMMX_E_UNSUPPORTED, "npass > 1 with the 2x2 method is built for reflection "
                                      "mode 1 with diffraction 1 and mode 2 with diffraction 0 "
                                      "or 1");
-  if (d->npass > 1 && d->method == DTMMX_METHOD_4X4 && d->norm != 0 && d->norm != DTMMX_NORM_FFT)
-    return fail(DTMMX_E_UNSUPPORTED, "npass > 1: only norm 0 (total) and 1 (fft) are built");
+  if (d->npass > 1 && d->method == DTMMX_METHOD_4X4 &&
+      (d->norm & ~(DTMMX_NORM_FFT | DTMMX_NORM_REF)) != 0)
+    return fail(DTMMX_E_ARG, "norm must combine DTMMX_NORM_FFT and DTMMX_NORM_REF");
+  if (d->npass > 1 && d->method == DTMMX_METHOD_4X4 && (d->norm & DTMMX_NORM_REF) &&
+      d->diffraction > 1)
+    return fail(DTMMX_E_UNSUPPORTED, "DTMMX_NORM_REF is built with diffraction 0 or 1");
   if (d->checkpoint < 0) return fail(DTMMX_E_ARG, "checkpoint interval must be >= 0");
   if (d->method == DTMMX_METHOD_4X4 && d->reflection != 1 && !refl2 && !full)
     return fail(DTMMX_E_UNSUPPORTED,

@@ -994,7 +994,8 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
 // Dense per-k operators (fp64), one thread per (k, plane).
 // ---------------------------------------------------------------------------
 
-enum { KOP_HEAD4 = 0, KOP_TAIL4 = 1, KOP_BOUND4 = 2, KOP_PROJ2 = 3, KOP_TR2 = 4, KOP_FIELD4 = 5 };
+enum { KOP_HEAD4 = 0, KOP_TAIL4 = 1, KOP_BOUND4 = 2, KOP_PROJ2 = 3, KOP_TR2 = 4, KOP_FIELD4 = 5,
+       KOP_E2H4 = 6 };
 
 template <typename T>
 struct KArgs {
@@ -1211,6 +1212,20 @@ __device__ __forceinline__ void kop_apply(const KArgs<T>& A, bool pass, double b
       r[2] = h0 * sn + h2 * c;
       r[3] = h3 * c - h1 * sn;
     }
+  } else if constexpr (KOP == KOP_E2H4) {
+    // E components kept as they are (at every k), H of the forward modes of
+    // the isotropic medium e0 from them, zero above betamax
+    // (dtmm/transfer.py:255-268 jones2H): p (E along k) with eps / a, s with
+    // -a in the frame of the mode
+    r[0] = v[0];
+    r[2] = v[2];
+    if (!pass) return;
+    const cd a = csqrt(A.e0 - mk<double>(beta * beta));
+    if (is_zero(a)) return;
+    const cd g0 = v[0] * c + v[2] * sn, g2 = v[2] * c - v[0] * sn;
+    const cd g1 = cdiv(A.e0, a) * g0, g3 = -(a * g2);
+    r[1] = g1 * c + g3 * sn;
+    r[3] = g3 * c - g1 * sn;
   } else if constexpr (KOP == KOP_BOUND4) {
     if (!pass) return;
     cd C0[4][4], D0[4][4], C1[4][4], D1[4][4], X[4][4], Y[4][4], M[4][4];
@@ -1978,7 +1993,7 @@ static __global__ __launch_bounds__(NT) void tmm_kernel(TmmArgs A) {
 // thread per (k, plane) on the unnormalised 2D spectrum of a whole field.
 // ---------------------------------------------------------------------------
 
-enum { POP_PROJ = 0, POP_NORMFFT = 1, POP_TOTALSUM = 2, POP_TOTALAPPLY = 3 };
+enum { POP_PROJ = 0, POP_NORMFFT = 1, POP_TOTALSUM = 2, POP_TOTALAPPLY = 3, POP_NOTCH = 4 };
 
 template <typename T>
 struct PArgs {
@@ -1994,6 +2009,10 @@ struct PArgs {
   double scale;       // applied to the written values
   double* part;       // per-block partial sums [2][nplanes][gridDim.x]
   const double* sums; // per-plane sums [2][nplanes] (POP_TOTALAPPLY)
+  const cpx<T>* ref;  // POP_NORMFFT: spectrum of the reference field whose
+                      // intensities the modes are rescaled to (NULL: the
+                      // field's own, dtmm/transfer.py:78-91)
+  double ref_scale;   // amplitude factor of the reference
   Beams bm;
 };
 
@@ -2047,7 +2066,9 @@ __global__ __launch_bounds__(NT) void pass_kernel(PArgs<T> A) {
     const cpx<T>* src = A.src + size_t(q) * 4 * HW + k;
     cd v[4], p[4];
     for (int i = 0; i < 4; ++i) v[i] = cast<double>(src[i * HW]);
-    if (pass) {
+    if (A.pop == POP_NOTCH) {  // the notch filter alone, at every k
+      for (int i = 0; i < 4; ++i) p[i] = v[i];
+    } else if (pass) {
       iso_forward_projection(beta, c, sn, A.eps, v, p);
     } else {
       for (int i = 0; i < 4; ++i) p[i] = mk<double>(0.);
@@ -2061,6 +2082,12 @@ __global__ __launch_bounds__(NT) void pass_kernel(PArgs<T> A) {
       double g = 1.;
       if (A.pop == POP_NORMFFT) {
         double i1 = intensity4(v), i2 = intensity4(p);
+        if (A.ref) {
+          cd w[4];
+          const cpx<T>* rs = A.ref + size_t(q) * 4 * HW + k;
+          for (int i = 0; i < 4; ++i) w[i] = cast<double>(rs[i * HW]);
+          i1 = intensity4(w) * (A.ref_scale * A.ref_scale);
+        }
         if (i2 == 0.) {
           i2 = 1.;
           i1 = 0.;
@@ -2113,6 +2140,77 @@ static __global__ __launch_bounds__(NT) void sum_rows_kernel(const double* part,
     __syncthreads();
   }
   if (tid == 0) sums[r] = red[0] * scale;
+}
+
+// Local normalisation with a reference field (dtmm/transfer.py:103-138),
+// one thread per (pixel, plane) on real-space fields: f is the forward part
+// of the field in the output medium; its normal-incidence forward amplitudes
+// (unit-flux modes of that medium, dtmm/tmm.py:701-725, :1851-1873) give the
+// pixel's Jones vector j, P1 = F as4x4(j j^H) F^-1 (dtmm/jones4.py:75-103,
+// jones.py:371-398), and
+//   f <- f |clip(I(P1 ref) / I(f), -1, 1)| + (1 - P1) ref
+// with the factor 0 where I(f) = 0 (transfer.py:66-75).  A pixel whose
+// forward amplitudes are both zero gets NaN, as the reference's
+// normalisation of a zero Jones vector does.
+template <typename T>
+struct LocalArgs {
+  cpx<T>* f;
+  const cpx<T>* ref;
+  double ref_scale;  // amplitude factor of the reference
+  size_t HW;
+  cd eps;  // output medium
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void local_norm_kernel(LocalArgs<T> A) {
+  const size_t px = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (px >= A.HW) return;
+  const size_t o = size_t(blockIdx.y) * 4 * A.HW + px;
+  Modes m;
+  iso_modes(0., A.eps, m);
+  // unit-flux columns
+  for (int j = 0; j < 4; ++j) {
+    const double pz = (m.F[0][j].x * m.F[1][j].x + m.F[0][j].y * m.F[1][j].y) -
+                      (m.F[2][j].x * m.F[3][j].x + m.F[2][j].y * m.F[3][j].y);
+    const double n = sqrt(fabs(pz));
+    const double sc = n == 0. ? 0. : 1. / n;
+    for (int i = 0; i < 4; ++i) m.F[i][j] = m.F[i][j] * sc;
+  }
+  cd Fi[4][4];
+  inv4(m.F, Fi);
+  cd fv[4], rv[4];
+  for (int i = 0; i < 4; ++i) {
+    fv[i] = cast<double>(A.f[o + i * A.HW]);
+    rv[i] = cast<double>(A.ref[o + i * A.HW]) * A.ref_scale;
+  }
+  cd a[4];
+  for (int i = 0; i < 4; ++i) {
+    cd t = mk<double>(0.);
+    for (int k = 0; k < 4; ++k) t = t + Fi[i][k] * fv[k];
+    a[i] = t;
+  }
+  const double nrm = sqrt(a[0].x * a[0].x + a[0].y * a[0].y + a[2].x * a[2].x + a[2].y * a[2].y);
+  const cd j0 = a[0] * (1. / nrm), j1 = a[2] * (1. / nrm);
+  // J = j j^H on the forward modes (0, 2), conj(J) on the backward (1, 3)
+  const cd J[2][2] = {{j0 * conj(j0), j0 * conj(j1)}, {j1 * conj(j0), j1 * conj(j1)}};
+  cd b[4];  // as4x4(J) Fi rv
+  for (int i = 0; i < 4; ++i) {
+    cd t = mk<double>(0.);
+    for (int k = 0; k < 4; ++k) t = t + Fi[i][k] * rv[k];
+    b[i] = t;
+  }
+  const cd c0 = J[0][0] * b[0] + J[0][1] * b[2], c2 = J[1][0] * b[0] + J[1][1] * b[2];
+  const cd c1 = conj(J[0][0]) * b[1] + conj(J[0][1]) * b[3];
+  const cd c3 = conj(J[1][0]) * b[1] + conj(J[1][1]) * b[3];
+  cd p1r[4];
+  for (int i = 0; i < 4; ++i) p1r[i] = m.F[i][0] * c0 + m.F[i][1] * c1 + m.F[i][2] * c2 + m.F[i][3] * c3;
+  double i1 = intensity4(p1r), i2 = intensity4(fv);
+  if (i2 == 0.) {
+    i2 = 1.;
+    i1 = 0.;
+  }
+  const double g = fabs(fmin(fmax(i1 / i2, -1.), 1.));
+  for (int i = 0; i < 4; ++i) A.f[o + i * A.HW] = cast<T>(fv[i] * g + (rv[i] - p1r[i]));
 }
 
 // Odd-pass update (dtmm/transfer.py:855-866) with fact = i0 / i0f per plane:

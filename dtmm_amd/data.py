@@ -104,6 +104,37 @@ def nematic_droplet_director(shape, radius, profile="r", retmask=False):
     return (inside, director) if retmask else director
 
 
+def sphere_mask(shape, radius, offset=(0, 0, 0)):
+    """True inside a sphere of `radius` about the box centre shifted by
+    offset (x, y, z) (dtmm/data.py:780-805).  The reference adds the z offset
+    where it subtracts the x and y ones (``zz--offset[2]``); kept for
+    identical masks."""
+    z, y, x = _voxel_centres(shape)
+    dx, dy, dz = offset
+    return np.sqrt((x - dx) ** 2 + (y - dy) ** 2 + (z + dz) ** 2) <= radius
+
+
+def cholesteric_director(shape, pitch, hand="left"):
+    """In-plane director rotating by 2 pi per `pitch` layers, layer 0 along
+    x; "left" turns clockwise seen from +z (dtmm/data.py:864-895)."""
+    sign = {"left": -1., "right": 1.}.get(hand)
+    if sign is None:
+        raise ValueError("Unknown handedness '{}'".format(hand))
+    nz, ny, nx = shape
+    phi = sign * 2 * np.pi / pitch * np.arange(nz)
+    out = np.zeros((nz, ny, nx, 3), FDTYPE)
+    out[..., 0] = np.cos(phi)[:, None, None]
+    out[..., 1] = np.sin(phi)[:, None, None]
+    return out
+
+
+def cholesteric_droplet_data(shape, radius, pitch, hand="left", no=1.5, ne=1.6, nhost=1.5):
+    """Optical block of a cholesteric droplet in a host (dtmm/data.py
+    :929-960)."""
+    return director2data(cholesteric_director(shape, pitch, hand=hand),
+                         mask=sphere_mask(shape, radius), no=no, ne=ne, nhost=nhost)
+
+
 def nematic_droplet_data(shape, radius, profile="r", no=1.5, ne=1.6, nhost=1.5):
     """Optical block of a nematic droplet in a host (dtmm/data.py:897-927)."""
     mask, director = nematic_droplet_director(shape, radius, profile=profile, retmask=True)

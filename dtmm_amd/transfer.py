@@ -4,11 +4,12 @@ Mirrors dtmm/transfer.py:322 ``transfer_field`` (same arguments, defaults,
 return value and errors) for the modes built on the GPU:
 
 * method "4x4" (dtmm/transfer.py:618-884): reflection mode 1 with
-  diffraction >= 1 and any npass (norm DTMM_NORM_FFT or 0,
-  dtmm/propagate_4x4.py:376-444, :80-101); reflection mode 2 with
-  diffraction 0, 1 or the tent windows of diffraction > 1 and any npass
-  (:156-204); diffraction < 0 or inf, the exact per-mode propagation
-  (npass 1, :515-564);
+  diffraction >= 1 and any npass (dtmm/propagate_4x4.py:376-444, :80-101);
+  reflection mode 2 with diffraction 0, 1 or the tent windows of
+  diffraction > 1 and any npass (:156-204); npass > 1 with every norm
+  (DTMM_NORM_FFT, 0, and DTMM_NORM_REF alone or with DTMM_NORM_FFT at
+  diffraction 0 or 1); diffraction < 0 or inf, the exact per-mode
+  propagation (npass 1, :515-564);
 * method "2x2" (and the configured default method '"2x2"',
   dtmm/transfer.py:1009-1183): reflection mode 0 (diffraction 0 or 1) and
   1 (diffraction 1) with npass 1 (dtmm/propagate_2x2.py:193-211, :19-87);
@@ -453,9 +454,9 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
             raise NotImplementedError("npass > 1 with the 2x2 method is supported for "
                                       "reflection mode 1 with diffraction 1 and reflection "
                                       "mode 2 with diffraction 0 or 1")
-    if npass > 1 and kind == "4x4" and (norm & DTMM_NORM_REF):
+    if npass > 1 and kind == "4x4" and (norm & DTMM_NORM_REF) and not (0 <= diffraction <= 1):
         raise NotImplementedError("npass > 1 with DTMM_NORM_REF (reference-field "
-                                  "normalisation) is not supported")
+                                  "normalisation) is supported with diffraction 0 or 1")
     if ret_bulk and (not (0 <= diffraction <= 1) or (npass != 1 and kind == "4x4")):
         raise NotImplementedError("ret_bulk is supported with diffraction 0 or 1, and npass=1 "
                                   "for the 4x4 method")
@@ -537,6 +538,10 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
     betamax = BETAMAX if betamax is None else betamax
     kind, reflection = _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk,
                                       norm)
+    if kind == "4x4" and npass > 1 and smooth <= 0:
+        # the notch filter of the pass normalisation divides by its width
+        # (dtmm/transfer.py:632-635, dtmm/denoise.py:21-22)
+        raise ZeroDivisionError("float division by zero")
 
     field_in, wavelengths, pixelsize = field_data
     if isinstance(field_in, tuple):

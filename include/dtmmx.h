@@ -51,6 +51,7 @@ enum { DTMMX_C64 = 0, DTMMX_C128 = 1 };
 enum { DTMMX_METHOD_2X2 = 2, DTMMX_METHOD_4X4 = 4 };
 
 #define DTMMX_NORM_FFT 1
+#define DTMMX_NORM_REF 2
 #define DTMMX_MAX_WAVELENGTHS 64
 #define DTMMX_MAX_BEAMS 64
 
@@ -87,8 +88,14 @@ typedef struct {
                          of every interface kept between passes
                          (transfer.py:1076-1178)                          */
   int norm;           /* 4x4, npass > 1: DTMMX_NORM_FFT (1, per-mode rescaling,
-                         transfer.py:78-91) or 0 (total intensity,
-                         transfer.py:206-221); DTMM_NORM_REF is not built */
+                         transfer.py:78-91), 0 (total intensity,
+                         transfer.py:206-221), DTMMX_NORM_REF (2, per-pixel
+                         "local" rescaling, transfer.py:103-138) or
+                         DTMMX_NORM_FFT | DTMMX_NORM_REF; with
+                         DTMMX_NORM_REF the intensities come from a
+                         reference field propagated with the 2x2 method
+                         in the forward passes (transfer.py:769-775,
+                         :805-835; diffraction 0 or 1)                  */
   double smooth;      /* npass > 1: notch width of dtmm/denoise.py (dtmm.ini:8,
                          0.1); negative values count as 0                 */
   int reflection;     /* 4x4: 1, or 2 (real space, transfer.py:719-725; any

@@ -55,7 +55,8 @@ __all__ = [
     "transfer_field", "illumination_data", "blackman", "planewave",
     "jonesvec", "eigenmask", "eigenindices", "eigenwave", "wave2eigenwave", "avec", "fvec", "auxiliary_matrix", "poynting", "set_fft_workers",
     "fft_window", "fft_windows", "fft_betaxy_mean", "fft_mask",
-    "DTMM_NORM_FFT", "DTMM_NORM_REF", "S_mat", "Etri_mat", "tr_mat", "E_tr_matrix", "exp_notch_filter", "denoise_fftfield", "denoise_field",
+    "DTMM_NORM_FFT", "DTMM_NORM_REF", "normalize_f", "fvec2avec", "f_iso",
+    "polarizer", "as4x4", "ray_jonesmat4x4", "jones2H", "project_normalized_local", "S_mat", "Etri_mat", "tr_mat", "E_tr_matrix", "exp_notch_filter", "denoise_fftfield", "denoise_field",
     "normalize_field", "project_normalized_fft", "project_normalized_total", "total_intensity",
     "polarizer", "as4x4", "jonesmat4x4", "ray_jonesmat4x4", "field2specter",
     "calculate_pom_field", "pom_specter", "XYZ2RGBD65", "apply_gamma", "apply_srgb_gamma",
@@ -1509,11 +1510,12 @@ def normalize_field(field, intensity_in, intensity_out):
     return field * np.abs(np.clip(fact, -1, 1))
 
 
-def project_normalized_fft(field, dmat):
+def project_normalized_fft(field, dmat, ref=None):
     """Projection in k-space, each mode rescaled to the intensity it had
-    before (dtmm/transfer.py:78-91, ref=None, window=None)."""
+    before, or that the reference field has (dtmm/transfer.py:78-91,
+    window=None)."""
     f1 = fft2(field)
-    i1 = field2intensity(f1)
+    i1 = field2intensity(f1 if ref is None else fft2(ref))
     f2 = dotmf(dmat, f1)
     i2 = field2intensity(f2)
     return ifft2(normalize_field(f2, i1, i2))
@@ -1544,6 +1546,83 @@ def project_normalized_total(field, dmat):
     return normalize_field_total(out, i1, i2)
 
 
+def normalize_f(fmat):
+    """Columns of a field matrix scaled to unit |Poynting z|; zero columns
+    stay zero (dtmm/tmm.py:701-725)."""
+    fmat = np.asarray(fmat, CDTYPE)
+    p = (fmat[..., 0, :] * np.conj(fmat[..., 1, :])).real - \
+        (fmat[..., 2, :] * np.conj(fmat[..., 3, :])).real
+    n = np.abs(p) ** 0.5
+    nz = n != 0.
+    scale = np.zeros_like(n)
+    scale[nz] = 1. / n[nz]
+    return fmat * scale[..., None, :]
+
+
+def fvec2avec(fvec, fmat):
+    """Mode amplitudes of field vectors in the normalised modes of fmat
+    (dtmm/tmm.py:1851-1873)."""
+    return dotmv(inv(normalize_f(fmat)), fvec)
+
+
+def f_iso(beta=0., phi=0., n=1.):
+    """Field matrix of an isotropic medium (dtmm/tmm.py:1074-1091)."""
+    return alphaf(beta, phi, refind2eps([n] * 3), np.zeros(3))[1]
+
+
+def polarizer(jones):
+    """Jones polarizer j j^H (dtmm/jones.py:236-279)."""
+    j = np.asarray(jones)
+    return j[..., :, None] * np.conj(j[..., None, :])
+
+
+def as4x4(jmat):
+    """Jones matrix on the forward modes, its conjugate on the backward ones
+    (dtmm/jones.py:371-398)."""
+    jmat = np.asarray(jmat)
+    out = np.zeros(jmat.shape[:-2] + (4, 4), jmat.dtype)
+    out[..., 0::2, 0::2] = jmat
+    out[..., 1::2, 1::2] = np.conj(jmat)
+    return out
+
+
+def ray_jonesmat4x4(jmat, epsv=(1., 1., 1.)):
+    """4x4 field-basis matrix of a Jones matrix at normal incidence
+    (dtmm/jones4.py:205-246 with beta = phi = 0, so the frame rotation is the
+    identity; jonesmat4x4 :75-103)."""
+    f = normalize_f(alphaf(0., 0., epsv, np.zeros(3))[1])
+    return dotmm(f, dotmm(as4x4(jmat), inv(f)))
+
+
+def jones2H(jones, ks, n=1., betamax=BETAMAX, mode=+1):
+    """(Ex, Ey) -> (Hy, Hx) of the modes of an isotropic medium on the FFT
+    grid, zero above betamax (dtmm/transfer.py:255-268)."""
+    alpha, f, fi = diffraction_alphaffi(jones.shape[-2:], ks, refind2eps([n] * 3),
+                                        np.zeros(3), betamax)
+    return diffract(jones, E2H_mat(f, mode=mode))
+
+
+def project_normalized_local(field, dmat, nout, ref=None):
+    """Forward projection in the output medium, each pixel rescaled to the
+    intensity the reference field has in that pixel's polarisation, plus the
+    reference's part of the other polarisation (dtmm/transfer.py:103-138,
+    window None).  The Jones vector of each pixel comes from the normal-
+    incidence forward amplitudes of the projected field."""
+    f = ifft2(dotmf(dmat, fft2(field)))
+    a = fvec2avec(np.moveaxis(f, -3, -1), f_iso(n=nout))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        jmat = polarizer(jonesvec(a[..., 0::2]))  # a zero vector gives NaN as there
+    pmat1 = ray_jonesmat4x4(jmat, epsv=refind2eps([nout] * 3))
+    pmat2 = -pmat1
+    for i in range(4):
+        pmat2[..., i, i] += 1.
+    src = field if ref is None else ref
+    intensity1 = field2intensity(dotmf(pmat1, src))
+    ref = dotmf(pmat2, src)
+    intensity2 = field2intensity(f)
+    return normalize_field(f, intensity1, intensity2) + ref
+
+
 def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1,
                  nstep=1, diffraction=1, reflection=1, betamax=BETAMAX, norm=DTMM_NORM_FFT,
                  smooth=SMOOTH, layer_callback=None, ret_bulk=False):
@@ -1552,7 +1631,9 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     The input is not projected onto forward modes and the input/output media
     are not propagated (transfer.py:694, :700).  npass > 1 alternates
     forward and backward passes (reversed layers, negative thickness,
-    transfer.py:744-804) with the normalisations of transfer.py:815-872;
+    transfer.py:744-804) with the normalisations of transfer.py:815-872
+    (norm DTMM_NORM_FFT, 0 "total", DTMM_NORM_REF "local" or FFT|REF, the
+    last two with a reference field propagated by the 2x2 method);
     like the reference, the input array of field_data (when complex128)
     then receives the field at the input side, reflections included.
     ret_bulk (npass 1) returns the fields of every layer index instead: the
@@ -1603,9 +1684,16 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
         raise NotImplementedError("oracle covers reflection modes 1 and 2 for the 4x4 method")
     # reflection mode 2 works in real space (dtmm/transfer.py:719-725)
     work_in_fft = reflection != 2
-    if norm & DTMM_NORM_REF:
-        raise NotImplementedError("oracle covers norm without DTMM_NORM_REF")
-    norm = "fft" if norm & DTMM_NORM_FFT else "total"
+    # DTMM_NORM_REF (transfer.py:631-639): a reference field, propagated with
+    # the 2x2 method (reflection 0) in the forward passes, sets the
+    # intensities the normalisations restore; without DTMM_NORM_FFT the
+    # normalisation is the per-pixel "local" one.  (The block of
+    # transfer.py:705-708 compares the renamed norm string with 2 and never
+    # runs.)
+    calc_reference = bool(norm & DTMM_NORM_REF)
+    if calc_reference and diffraction > 1:
+        raise NotImplementedError("oracle covers DTMM_NORM_REF with diffraction 0 or 1")
+    norm = "fft" if norm & DTMM_NORM_FFT else ("local" if calc_reference else "total")
     if smooth < 0:
         smooth = 0.
     field_in, wavelengths, pixelsize = field_data
@@ -1619,6 +1707,7 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
     indices = list(range(1, n - 1))
     field0 = transmitted_field(field_in, ks, n=nin, betamax=betamax)
     i0 = total_intensity(field0)
+    ref = field_in.copy() if calc_reference else None
     field = fft2(field_in.copy()) if work_in_fft else field_in.copy()
     if npass > 1:
         pin_mat = projection_matrix(field.shape[-2:], ks, epsv=refind2eps([nin] * 3), mode=+1,
@@ -1631,6 +1720,19 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
         for j in indices:
             nsub, (thickness, ev, ea) = layers[j]
             _, (d, e, a) = eff_layers[j]
+            if calc_reference and i % 2 == 0:
+                # the reference's E field through this layer
+                # (transfer.py:769-775, propagate_2x2.py:284-305)
+                dmat = None
+                if diffraction:
+                    key = _key(field.shape[-2:], ks, beta, phi, d * direction, e, a, betamax,
+                               direction)
+                    dmat = cache.get(("e2r",) + key, lambda: corrected_E_diffraction_matrix(
+                        field.shape[-2:], ks, beta, phi, d * direction, e, a, direction,
+                        betamax))
+                ref[..., ::2, :, :], _ = _transfer_ray_2x2_2(
+                    ref[..., ::2, :, :], ks, layers[j - 1][1], (thickness * direction, ev, ea),
+                    dmat, beta, phi, nsub, direction, None, 0, None)
             if reflection == 2:
                 field = propagate_4x4_effective_2(field, ks, (thickness * direction, ev, ea),
                                                   (d * direction, e, a), beta, phi, nsub,
@@ -1643,6 +1745,9 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
                 bulk[j] = ifft2(field) if work_in_fft else field
             if layer_callback is not None:
                 layer_callback(j)
+        if calc_reference and i % 2 == 0:
+            # H of the forward modes of the output medium (transfer.py:805-806)
+            ref[..., 1::2, :, :] = jones2H(ref[..., ::2, :, :], ks, n=nout, betamax=betamax)
         indices.reverse()
         if work_in_fft:
             field = ifft2(field)
@@ -1658,8 +1763,12 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
             if i == 0:
                 field = field - field_out
             if i != npass - 1:
+                if calc_reference:
+                    ref = ref * (nin / nout) ** 0.5
                 if norm == "fft":
-                    field = project_normalized_fft(field, pout_mat)
+                    field = project_normalized_fft(field, pout_mat, ref=ref)
+                elif norm == "local":
+                    field = project_normalized_local(field, pout_mat, nout, ref=ref)
                 else:
                     field = project_normalized_total(field, pout_mat)
                 field = denoise_field(field, ks, nin, sigma)
@@ -1676,6 +1785,8 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
                 field_in *= fact
                 field = field0 - field
                 field_in += field
+                if calc_reference:
+                    ref = field.copy()
         if work_in_fft:
             field = fft2(field)
     return field_out, wavelengths, pixelsize

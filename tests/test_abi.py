@@ -87,7 +87,8 @@ def _plan_rc(**kw):
     (dict(method=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(npass=0), _lib.DTMMX_E_ARG),
     (dict(method=2, npass=3), _lib.DTMMX_E_UNSUPPORTED),
-    (dict(npass=3, norm=2), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(npass=3, norm=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(npass=3, norm=4), _lib.DTMMX_E_ARG),
     (dict(method=2, reflection=0, npass=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(method=2, reflection=1, diffraction=0, npass=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(method=2, reflection=2, diffraction=3, npass=2), _lib.DTMMX_E_UNSUPPORTED),
@@ -375,6 +376,11 @@ def test_mode_resolution_no_diffraction():
     assert _resolve_modes("2x2", 1, 1, 2, None, False) == ("2x2", 2)
     with pytest.raises(NotImplementedError):
         _resolve_modes("2x2", 1, 3, 2, None, False)
+    # DTMM_NORM_REF with npass > 1: diffraction 0 or 1
+    assert _resolve_modes("4x4", 3, 1, None, None, False, 2) == ("4x4", 1)
+    assert _resolve_modes("4x4", 3, 1, None, None, False, 3) == ("4x4", 1)
+    with pytest.raises(NotImplementedError):
+        _resolve_modes("4x4", 3, 3, None, None, False, 2)
     # ret_bulk: diffraction 0 or 1; the 4x4 method with npass 1
     assert _resolve_modes("4x4", 1, 1, None, None, True) == ("4x4", 1)
     assert _resolve_modes("2x2", 1, 1, None, None, True) == ("2x2", 0)

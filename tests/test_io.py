@@ -93,3 +93,24 @@ def test_field_is_saved_in_double_precision(tmp_path):
     assert g.dtype == np.complex128 and wl.dtype == np.float64
     np.testing.assert_array_equal(g, f.astype(np.complex128))
     np.testing.assert_array_equal(wl, [500., 600.])
+
+
+def test_cholesteric_droplet_builders():
+    """sphere_mask counts the voxel centres within the radius (the z offset
+    enters with the reference's opposite sign), and the cholesteric
+    director turns by 2 pi per pitch, left-handed clockwise."""
+    import dtmm_amd as dm
+    shape = (6, 9, 10)
+    c = [np.arange(n) - (n - 1) / 2. for n in shape]
+    zz, yy, xx = np.meshgrid(*c, indexing="ij")
+    for off in ((0, 0, 0), (1., -0.5, 1.5)):
+        want = (xx - off[0]) ** 2 + (yy - off[1]) ** 2 + (zz + off[2]) ** 2 <= 3.2 ** 2
+        np.testing.assert_array_equal(dm.sphere_mask(shape, 3.2, off), want)
+    d = dm.cholesteric_director(shape, 4.)
+    np.testing.assert_allclose(np.arctan2(d[1, 0, 0, 1], d[1, 0, 0, 0]), -np.pi / 2, atol=1e-15)
+    np.testing.assert_allclose(d[4], d[0], atol=1e-12)
+    np.testing.assert_allclose(dm.cholesteric_director(shape, 4., "right")[1, 0, 0, 1], 1.,
+                               atol=1e-15)
+    t, eps, ang = dm.cholesteric_droplet_data(shape, 3.2, 4., no=1.5, ne=1.7, nhost=1.4)
+    outside = ~dm.sphere_mask(shape, 3.2)
+    np.testing.assert_allclose(eps[outside], 1.4 ** 2)
