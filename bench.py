@@ -52,11 +52,13 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, each
 # scaled by the counter's reading for a 512 MiB device copy (FETCH x 2.000,
-# WRITE x 1.000); profiles/transfer4x4_c64_512x512_hbm_traffic.md.  Keyed by
+# WRITE x 1.000); profiles/transfer_c64_512x512_hbm_traffic.md.  Keyed by
 # (kernel, H, W, planes, method, precision); other workloads report null.
 MEASURED_TRAFFIC = {
     ("row_layer", 512, 512, 18, "4x4", "c64"): 312.8e6,
-    ("col_kspace", 512, 512, 18, "4x4", "c64"): 303.3e6,
+    ("col_kspace", 512, 512, 18, "4x4", "c64"): 303.8e6,
+    ("row_layer", 512, 512, 18, "2x2", "c64"): 161.9e6,
+    ("col_kspace", 512, 512, 18, "2x2", "c64"): 151.5e6,
 }
 
 

@@ -160,8 +160,46 @@ class DtmmxError(RuntimeError):
 _lib = None
 
 
+def source_hash():
+    """SHA-256 of the library's sources (dtmm_amd/csrc/*.hip, *.hpp and
+    include/*.h, in name order); None when they are not present."""
+    import glob
+    import hashlib
+    root = os.path.dirname(HERE)
+    files = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")) +
+                   glob.glob(os.path.join(HERE, "csrc", "*.hpp")) +
+                   glob.glob(os.path.join(root, "include", "*.h")))
+    if not files:
+        return None
+    h = hashlib.sha256()
+    for f in files:
+        h.update(os.path.basename(f).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def _check_fresh(path):
+    """The build writes the hash of the sources it compiled next to the
+    library; a library whose sources have changed since is refused."""
+    want = source_hash()
+    if want is None:
+        return
+    rec = path + ".srchash"
+    if not os.path.exists(rec):
+        raise RuntimeError("%s has no build record (%s): rebuild it with `python -m "
+                           "dtmm_amd.build`" % (path, os.path.basename(rec)))
+    with open(rec) as fh:
+        parts = fh.read().split()
+    got = parts[0] if parts else ""
+    if got != want:
+        raise RuntimeError("%s was built from other sources than dtmm_amd/csrc and include/ "
+                           "hold now: rebuild it with `python -m dtmm_amd.build`" % path)
+
+
 def load():
-    """Load (once) and return the engine library; raises if it is absent."""
+    """Load (once) and return the engine library; raises if it is absent or
+    was built from other sources than the ones present."""
     global _lib
     if _lib is not None:
         return _lib
@@ -169,6 +207,7 @@ def load():
         raise RuntimeError(
             "dtmm_amd engine library %s is missing: build it with "
             "`python -m dtmm_amd.build` (there is no CPU fallback)" % LIB_PATH)
+    _check_fresh(LIB_PATH)
     lib = ctypes.CDLL(LIB_PATH)
     lib.dtmmx_abi_version.restype = ctypes.c_int
     lib.dtmmx_last_error.restype = ctypes.c_char_p

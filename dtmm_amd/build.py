@@ -32,11 +32,22 @@ FLAGS = [
 PARTS = (1, 2, 3)
 
 
+def _source_hash():
+    # _lib's hash of the sources, loaded by path so that building needs no
+    # package import
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("_dtmmx_lib", os.path.join(HERE, "_lib.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.source_hash()
+
+
 def build(verbose=True, defines=(), out=None):
     """Compile the library (always recompiles).  `defines` adds -D flags and
     `out` another output path (for build variants)."""
     out = out or OUT
     t = time.time()
+    srchash = _source_hash()
     tmp = tempfile.mkdtemp(prefix="dtmmx_build_")
     try:
         objs, procs = [], []
@@ -58,6 +69,9 @@ def build(verbose=True, defines=(), out=None):
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     os.replace(out + ".tmp", out)
+    # record of the sources compiled (checked when the library is loaded)
+    with open(out + ".srchash", "w") as fh:
+        fh.write("%s %s\n" % (srchash, " ".join(defines)))
     if verbose:
         print("built %s in %.1f s" % (out, time.time() - t), flush=True)
     return out

@@ -15,6 +15,7 @@
 // Epn -> F after the last one, and handle boundaries between different
 // effective media.  The 2x2 method is the same with the two forward
 // components only.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,6 +50,51 @@ static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+// Optional per-launch timing of the per-layer kernels with HIP events on the
+// launch stream (class 0: row layer kernel, 1: column k-space kernel,
+// 3: dense k-space head/tail kernel).
+struct KTimer {
+  bool on = false;
+  bool armed = false;  // the current launch is being timed
+  int cap = 0, used = 0, stride = 1;
+  int seen[4] = {0, 0, 0, 0};
+  std::vector<hipEvent_t> ev;
+  std::vector<int> cls;
+};
+#if DTMMX_HOST_PART
+KTimer g_timer;
+#else
+extern KTimer g_timer;
+#endif
+
+// A timed launch passes its event pair to hipExtLaunchKernel, which stamps
+// them with the kernel's own start and end (the dispatch's timestamps, as a
+// kernel trace reads them), so the measured time holds no queue or event
+// overhead.  timer_start arms the pair for the next DTMMX_LAUNCH on the
+// stream; timer_stop keeps it.
+static void timer_start(int cls, hipStream_t) {
+  g_timer.armed = false;
+  if (!g_timer.on || g_timer.used >= g_timer.cap) return;
+  if (g_timer.seen[cls & 3]++ % g_timer.stride != 0) return;
+  g_timer.armed = true;
+  g_timer.cls[g_timer.used] = cls;
+}
+
+static void timer_stop(hipStream_t) {
+  if (!g_timer.armed) return;
+  g_timer.armed = false;
+  ++g_timer.used;
+}
+
+#define DTMMX_LAUNCH(K, G, B, S, A)                                                     \
+  do {                                                                                  \
+    if (g_timer.armed)                                                                  \
+      hipExtLaunchKernelGGL(K, G, B, 0, S, g_timer.ev[2 * g_timer.used],                \
+                            g_timer.ev[2 * g_timer.used + 1], 0, A);                    \
+    else                                                                                \
+      hipLaunchKernelGGL(K, G, B, 0, S, A);                                             \
+  } while (0)
 
 #define HIP_TRY(expr)                                                                       \
   do {                                                                                      \
@@ -99,11 +145,11 @@ static int launch_row(int n, int nr, const RowArgs<T>& a, int nwork, hipStream_t
 #define X(N)                                                                                \
   case N:                                                                                   \
     if (nr == 2 && row_nr_max<T, N>() >= 2)                                                 \
-      hipLaunchKernelGGL((row_kernel<T, N, (row_nr_max<T, N>() >= 2 ? 2 : 1), NC, OP>),      \
-                         dim3(nwork), dim3(row_lanes<N>() * 2 * NC), 0, s, a);              \
+      DTMMX_LAUNCH((row_kernel<T, N, (row_nr_max<T, N>() >= 2 ? 2 : 1), NC, OP>),           \
+                   dim3(nwork), dim3(row_lanes<N>() * 2 * NC), s, a);                       \
     else                                                                                    \
-      hipLaunchKernelGGL((row_kernel<T, N, 1, NC, OP>), dim3(nwork),                        \
-                         dim3(row_lanes<N>() * NC), 0, s, a);                               \
+      DTMMX_LAUNCH((row_kernel<T, N, 1, NC, OP>), dim3(nwork), dim3(row_lanes<N>() * NC), s, \
+                   a);                                                                      \
     break;
     DTMMX_SIZES(X)
 #undef X
@@ -156,8 +202,7 @@ static int launch_col(int n, const ColArgs<T>& a0, int nlines_per_lam, bool tune
     while (tune && a.ncl % 2 == 0 && ntile * a.b.nlam_b * (nlines_per_lam / a.ncl) < COL_MIN_WORK) \
       a.ncl /= 2;                                                                           \
     const int nwork = ntile * a.b.nlam_b * (nlines_per_lam / a.ncl);                        \
-    hipLaunchKernelGGL((col_kernel<T, N, TX, OP>), dim3(nwork), dim3(TX * col_lanes<N>()), 0, \
-                       s, a);                                                               \
+    DTMMX_LAUNCH((col_kernel<T, N, TX, OP>), dim3(nwork), dim3(TX * col_lanes<N>()), s, a);  \
     break;                                                                                  \
   }
     DTMMX_SIZES(X)
@@ -177,8 +222,7 @@ static int launch_row_adj(int n, const AdjArgs<T>& a, int nwork, hipStream_t s) 
     if constexpr (sizeof(T) == 8 && N > 512) {                                              \
       return fail(DTMMX_E_UNSUPPORTED, "complex128 adjoint is built up to 512 points");      \
     } else {                                                                                \
-      hipLaunchKernelGGL((row_adj_kernel<T, N>), dim3(nwork), dim3(row_lanes<N>() * 4), 0, s, \
-                         a);                                                                \
+      DTMMX_LAUNCH((row_adj_kernel<T, N>), dim3(nwork), dim3(row_lanes<N>() * 4), s, a);    \
     }                                                                                       \
     break;
     DTMMX_SIZES(X)
@@ -225,38 +269,7 @@ struct dtmmx_plan {
 
 namespace dtmmx {
 
-// Optional per-launch timing of the per-layer kernels with HIP events on the
-// launch stream (class 0: row layer kernel, 1: column k-space kernel,
-// 3: dense k-space head/tail kernel).
-struct KTimer {
-  bool on = false;
-  bool armed = false;  // the current launch is being timed
-  int cap = 0, used = 0, stride = 1;
-  int seen[4] = {0, 0, 0, 0};
-  std::vector<hipEvent_t> ev;
-  std::vector<int> cls;
-};
-#if DTMMX_HOST_PART
-KTimer g_timer;
-#else
-extern KTimer g_timer;
-#endif
 
-static void timer_start(int cls, hipStream_t s) {
-  g_timer.armed = false;
-  if (!g_timer.on || g_timer.used >= g_timer.cap) return;
-  if (g_timer.seen[cls & 3]++ % g_timer.stride != 0) return;
-  g_timer.armed = true;
-  g_timer.cls[g_timer.used] = cls;
-  (void)hipEventRecord(g_timer.ev[2 * g_timer.used], s);
-}
-
-static void timer_stop(hipStream_t s) {
-  if (!g_timer.armed) return;
-  g_timer.armed = false;
-  (void)hipEventRecord(g_timer.ev[2 * g_timer.used + 1], s);
-  ++g_timer.used;
-}
 
 template <typename T>
 static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op, hipStream_t s,
@@ -386,31 +399,31 @@ static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, 
   timer_start(3, s);
   switch (a.kop) {
     case KOP_HEAD4:
-      hipLaunchKernelGGL((kspace_kernel<T, KOP_HEAD4, false>), grid, dim3(NT), 0, s, a);
+      DTMMX_LAUNCH((kspace_kernel<T, KOP_HEAD4, false>), grid, dim3(NT), s, a);
       break;
     case KOP_TAIL4:
       if (a.adjoint)
-        hipLaunchKernelGGL((kspace_kernel<T, KOP_TAIL4, true>), grid, dim3(NT), 0, s, a);
+        DTMMX_LAUNCH((kspace_kernel<T, KOP_TAIL4, true>), grid, dim3(NT), s, a);
       else
-        hipLaunchKernelGGL((kspace_kernel<T, KOP_TAIL4, false>), grid, dim3(NT), 0, s, a);
+        DTMMX_LAUNCH((kspace_kernel<T, KOP_TAIL4, false>), grid, dim3(NT), s, a);
       break;
     case KOP_BOUND4:
       if (a.adjoint)
-        hipLaunchKernelGGL((kspace_kernel<T, KOP_BOUND4, true>), grid, dim3(NT), 0, s, a);
+        DTMMX_LAUNCH((kspace_kernel<T, KOP_BOUND4, true>), grid, dim3(NT), s, a);
       else
-        hipLaunchKernelGGL((kspace_kernel<T, KOP_BOUND4, false>), grid, dim3(NT), 0, s, a);
+        DTMMX_LAUNCH((kspace_kernel<T, KOP_BOUND4, false>), grid, dim3(NT), s, a);
       break;
     case KOP_PROJ2:
-      hipLaunchKernelGGL((kspace_kernel<T, KOP_PROJ2, false>), grid, dim3(NT), 0, s, a);
+      DTMMX_LAUNCH((kspace_kernel<T, KOP_PROJ2, false>), grid, dim3(NT), s, a);
       break;
     case KOP_TR2:
-      hipLaunchKernelGGL((kspace_kernel<T, KOP_TR2, false>), grid, dim3(NT), 0, s, a);
+      DTMMX_LAUNCH((kspace_kernel<T, KOP_TR2, false>), grid, dim3(NT), s, a);
       break;
     case KOP_FIELD4:
-      hipLaunchKernelGGL((kspace_kernel<T, KOP_FIELD4, false>), grid, dim3(NT), 0, s, a);
+      DTMMX_LAUNCH((kspace_kernel<T, KOP_FIELD4, false>), grid, dim3(NT), s, a);
       break;
     case KOP_E2H4:
-      hipLaunchKernelGGL((kspace_kernel<T, KOP_E2H4, false>), grid, dim3(NT), 0, s, a);
+      DTMMX_LAUNCH((kspace_kernel<T, KOP_E2H4, false>), grid, dim3(NT), s, a);
       break;
     default:
       timer_stop(s);
@@ -2838,6 +2851,23 @@ static int check_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void*
   return DTMMX_OK;
 }
 
+// status bit 2: the output holds NaN or infinite values
+static int check_finite(const dtmmx_plan* P, const void* f, size_t nfields, int* status,
+                        hipStream_t s) {
+  if (!status) return DTMMX_OK;
+  const size_t n = nfields * 2 * size_t(P->d.n_rays) * P->d.n_wavelengths * 4 *
+                   size_t(P->d.height) * P->d.width;
+  const unsigned nb = unsigned(std::min<size_t>((n + NT - 1) / NT, 4096));
+  if (P->d.precision == DTMMX_C64)
+    hipLaunchKernelGGL((finite_kernel<float>), dim3(nb), dim3(NT), 0, s, (const float*)f, n,
+                       status);
+  else
+    hipLaunchKernelGGL((finite_kernel<double>), dim3(nb), dim3(NT), 0, s, (const double*)f, n,
+                       status);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
 size_t dtmmx_tape_size(const dtmmx_plan* P, const dtmmx_stack* S) {
   if (!P || !S) return 0;
   const size_t cb = P->d.precision == DTMMX_C64 ? sizeof(cpx<float>) : sizeof(cpx<double>);
@@ -2920,6 +2950,7 @@ int dtmmx_transfer_batch(const dtmmx_job* jobs, int n, int* status, void* stream
                       : transfer_c128(j.plan, j.stack, j.field_in, j.field_out, j.workspace, status,
                                       s);
     if (r != DTMMX_OK) return fail(r, "job " + std::to_string(i) + ": " + g_err);
+    TRY(check_finite(j.plan, j.field_out, 1, status, s));
   }
   return DTMMX_OK;
 }
@@ -2935,8 +2966,10 @@ int dtmmx_transfer_bulk(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, vo
     return fail(DTMMX_E_UNSUPPORTED, "bulk output is built with diffraction 0 or 1 and npass 1 "
                                      "(4x4) or any npass (2x2)");
   hipStream_t s = (hipStream_t)stream;
-  if (P->d.precision == DTMMX_C64) return bulk_c64(P, S, fin, fout, bulk, ws, status, s);
-  return bulk_c128(P, S, fin, fout, bulk, ws, status, s);
+  const int r = P->d.precision == DTMMX_C64 ? bulk_c64(P, S, fin, fout, bulk, ws, status, s)
+                                           : bulk_c128(P, S, fin, fout, bulk, ws, status, s);
+  if (r != DTMMX_OK) return r;
+  return check_finite(P, fout, 1, status, s);
 }
 
 int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* fout,
@@ -2946,8 +2979,10 @@ int dtmmx_transfer(const dtmmx_plan* P, const dtmmx_stack* S, void* fin, void* f
     if (r != DTMMX_OK) return r;
   }
   hipStream_t s = (hipStream_t)stream;
-  if (P->d.precision == DTMMX_C64) return transfer_c64(P, S, fin, fout, ws, status, s);
-  return transfer_c128(P, S, fin, fout, ws, status, s);
+  const int r = P->d.precision == DTMMX_C64 ? transfer_c64(P, S, fin, fout, ws, status, s)
+                                           : transfer_c128(P, S, fin, fout, ws, status, s);
+  if (r != DTMMX_OK) return r;
+  return check_finite(P, fout, 1, status, s);
 }
 
 int dtmmx_propagate_layer(const dtmmx_plan* P, const dtmmx_stack* S, const void* fin, void* fout,
@@ -2985,8 +3020,10 @@ int dtmmx_transfer_noalloc(const dtmmx_plan* P, const dtmmx_stack* S, void* fin,
     if (r != DTMMX_OK) return r;
   }
   if (need) TRY(scratch_address(s, &ws));
-  if (P->d.precision == DTMMX_C64) return transfer_c64(P, S, fin, fout, ws, status, s);
-  return transfer_c128(P, S, fin, fout, ws, status, s);
+  const int r = P->d.precision == DTMMX_C64 ? transfer_c64(P, S, fin, fout, ws, status, s)
+                                           : transfer_c128(P, S, fin, fout, ws, status, s);
+  if (r != DTMMX_OK) return r;
+  return check_finite(P, fout, 1, status, s);
 }
 
 int dtmmx_timing_begin(int capacity, int stride) {

@@ -2270,6 +2270,16 @@ __global__ __launch_bounds__(NT) void accumulate_kernel(cpx<T>* g, cpx<T>* fout,
   g[i] = v;
 }
 
+// Sets status bit 2 when any of the n values is NaN or infinite (the
+// output check of a transfer; one atomic per wave that finds one).
+template <typename T>
+__global__ __launch_bounds__(NT) void finite_kernel(const T* p, size_t n, int* status) {
+  bool bad = false;
+  for (size_t i = size_t(blockIdx.x) * NT + threadIdx.x; i < n; i += size_t(gridDim.x) * NT)
+    bad = bad || !isfinite(p[i]);
+  if (__any(bad) && (threadIdx.x % warpSize) == 0) atomicOr(status, 4);
+}
+
 // Zero n values of type T.
 template <typename T>
 __global__ __launch_bounds__(NT) void zero_kernel(T* p, size_t n) {

@@ -594,7 +594,11 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
 
 
 def _check_status(status, adjoint=False):
-    if int(status[0].item()) & 1:
+    flags = int(status[0].item())
+    if flags & 4 and not flags & 1:
+        warnings.warn("the output field holds NaN or infinite values", RuntimeWarning,
+                      stacklevel=3)
+    if flags & 1:
         if adjoint:
             raise NotImplementedError("biaxial voxels (eps0 != eps1) are not supported by the "
                                       "adjoint transfer")

@@ -197,8 +197,11 @@ int dtmmx_plan_get_tiles(const dtmmx_plan* plan, int* col_lines, int* row_rays);
  * modified; with npass > 1 it receives the field at the input side
  * (input plus reflection), as the reference writes it into the caller's
  * array (transfer.py:843, :862-866).
- * status (device int[4], may be NULL) receives flags: status[0] bit 0 =
- * a biaxial voxel was met (not built: its output is NaN). */
+ * status (device int[4], may be NULL) receives flags (set, never cleared):
+ * status[0] bit 0 = a biaxial voxel was met in a layer not flagged biaxial
+ * (its output is NaN); bit 2 = the output holds NaN or infinite values
+ * (checked after the transfer, one read of field_out; the reference has no
+ * such check). */
 int dtmmx_transfer(const dtmmx_plan* plan, const dtmmx_stack* stack, void* field_in,
                    void* field_out, void* workspace, size_t workspace_bytes, int* status,
                    void* stream);

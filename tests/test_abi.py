@@ -480,3 +480,20 @@ def test_float_sincos_of_the_layer_kernels():
     xd = x.astype(np.float64)
     assert np.abs(s - np.sin(xd)).max() < 1e-7
     assert np.abs(c - np.cos(xd)).max() < 1e-7
+
+
+def test_stale_library_is_refused(tmp_path):
+    """The build records the hash of the sources it compiled next to the
+    library; a record that does not match the sources present, or no record,
+    refuses the library before it is loaded."""
+    lib = tmp_path / "libx.so"
+    lib.write_bytes(b"not loaded")
+    with pytest.raises(RuntimeError, match="no build record"):
+        _lib._check_fresh(str(lib))
+    (tmp_path / "libx.so.srchash").write_text("0" * 64 + "\n")
+    with pytest.raises(RuntimeError, match="other sources"):
+        _lib._check_fresh(str(lib))
+    (tmp_path / "libx.so.srchash").write_text(_lib.source_hash() + " \n")
+    _lib._check_fresh(str(lib))
+    # the library in the tree passes (it was built from these sources)
+    _lib._check_fresh(_lib.LIB_PATH)
