@@ -733,7 +733,16 @@ def measure(args, method, fd, optical, c5):
     # sum (read + write) except for the first window of a layer, and the
     # column kernel reads the field once and writes one window row per tent
     # row (diffraction of them)
-    row_field = field_bytes * (1. + 0.5 * (nwin - 1) / nwin) if nwin > 1 else field_bytes
+    # The fused window row kernel (one launch per layer) reads the n window
+    # rows once and writes the layer's output once; the per-window launches
+    # (DTMMX_FUSED_WINDOWS=0) read a window row and read + write the sum
+    fused = nwin > 1 and os.environ.get("DTMMX_FUSED_WINDOWS", "1") != "0"
+    if fused:
+        row_field = field_bytes * (diffraction + 1) / 2.
+    elif nwin > 1:
+        row_field = field_bytes * (1. + 0.5 * (nwin - 1) / nwin)
+    else:
+        row_field = field_bytes
     col_field = field_bytes * (1 + diffraction) / 2. if nwin > 1 else field_bytes
     kern = {
         "row_layer": {"ms": ms[0], "n": cnt[0], "bytes": row_field + 40 * H * W},
@@ -751,7 +760,8 @@ def measure(args, method, fd, optical, c5):
     # launches per pass: row L (or L * windows), column L - 1 (or L window
     # splits), dense k-space 2
     per_pass = [ms[i] / max(cnt[i], 1) * n * npass
-                for i, n in enumerate((L * nwin, (L if nwin > 1 else max(L - 1, 0)), 0, 2))]
+                for i, n in enumerate((L * (1 if fused else nwin), (L if nwin > 1 else max(L - 1, 0)),
+                                       0, 2))]
     log("per pass (from launch averages): row %.2f ms, col %.2f ms, kspace %.2f ms, rest %.2f ms"
         % (per_pass[0], per_pass[1], per_pass[3], ms_per_step - sum(per_pass)))
 

@@ -96,6 +96,15 @@ static void timer_stop(hipStream_t) {
       hipLaunchKernelGGL(K, G, B, 0, S, A);                                             \
   } while (0)
 
+#define DTMMX_LAUNCH_SHM(K, G, B, SHM, S, A)                                            \
+  do {                                                                                  \
+    if (g_timer.armed)                                                                  \
+      hipExtLaunchKernelGGL(K, G, B, SHM, S, g_timer.ev[2 * g_timer.used],              \
+                            g_timer.ev[2 * g_timer.used + 1], 0, A);                    \
+    else                                                                                \
+      hipLaunchKernelGGL(K, G, B, SHM, S, A);                                           \
+  } while (0)
+
 #define HIP_TRY(expr)                                                                       \
   do {                                                                                      \
     hipError_t e_ = (expr);                                                                 \
@@ -151,6 +160,63 @@ static int launch_row(int n, int nr, const RowArgs<T>& a, int nwork, hipStream_t
       DTMMX_LAUNCH((row_kernel<T, N, 1, NC, OP>), dim3(nwork), dim3(row_lanes<N>() * NC), s, \
                    a);                                                                      \
     break;
+    DTMMX_SIZES(X)
+#undef X
+    default:
+      return fail(DTMMX_E_UNSUPPORTED, "unsupported width " + std::to_string(n));
+  }
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
+// Fused window row pass: NR rays of one beam per work-group (two for
+// complex64 lines up to 512 points) and a chunk of wavelengths whose
+// per-window sums fill the LDS beside the kernel's own buffers.  Returns
+// DTMMX_E_UNSUPPORTED (nothing launched) when not even one wavelength fits.
+static int max_lds_per_block() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0, x = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&x, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      x = 65536;
+    v = x;
+  }
+  return v;
+}
+
+template <typename T>
+static int launch_winrow(int n, int nr, WinRowArgs<T> a, int nlam, hipStream_t s) {
+  switch (n) {
+#define X(N)                                                                                   \
+  case N: {                                                                                    \
+    constexpr int NRM = row_nr_max<T, N>() >= 2 ? 2 : 1;                                       \
+    auto go = [&](auto nrc) -> int {                                                           \
+      constexpr int NRX = decltype(nrc)::value;                                                \
+      constexpr int NTW = row_lanes<N>() * NRX * 4;                                            \
+      constexpr int PPX = (N + NTW - 1) / NTW;                                                 \
+      const size_t stat = (size_t(NRX * 4) * line_stride<N>() + tw_table_slots<N>()) * sizeof(cpx<T>); \
+      const size_t per = size_t(PPX) * NRX * 4 * NTW * sizeof(cpx<T>);                         \
+      const size_t cap = size_t(max_lds_per_block());                                          \
+      int c = cap > stat ? int((cap - stat) / per) : 0;                                        \
+      if (const char* e = std::getenv("DTMMX_WIN_C")) c = std::min(c, std::atoi(e));          \
+      c = std::min(c, nlam);                                                                   \
+      if (c < 1) return fail(DTMMX_E_UNSUPPORTED, "window sums do not fit in LDS");           \
+      a.nchunk = c;                                                                            \
+      const int nwork = a.H * (a.b.nrays / NRX) * ((nlam + c - 1) / c);                        \
+      const size_t shm = size_t(c) * per;                                                      \
+      (void)hipFuncSetAttribute((const void*)winrow_kernel<T, N, NRX, 4>,                      \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(shm));         \
+      timer_start(0, s);                                                                       \
+      DTMMX_LAUNCH_SHM((winrow_kernel<T, N, NRX, 4>), dim3(nwork), dim3(NTW), shm, s, a);      \
+      timer_stop(s);                                                                           \
+      return DTMMX_OK;                                                                         \
+    };                                                                                         \
+    int r = (nr == 2 && NRM == 2) ? go(std::integral_constant<int, NRM>{})                     \
+                                  : go(std::integral_constant<int, 1>{});                      \
+    if (r != DTMMX_OK) return r;                                                               \
+    break;                                                                                     \
+  }
     DTMMX_SIZES(X)
 #undef X
     default:
@@ -255,6 +321,10 @@ struct dtmmx_plan {
   double* wcx = nullptr;  // [beam][nlam][wn][W]
   double* wcy = nullptr;  // [beam][nlam][wn][H]
   int* wmask = nullptr;   // [nlam][H]: min(amax, largest |kx| the windows pass)
+  // the windows in tent-row order (j, then i) for the fused window row
+  // kernel: win_ord [2][nwin] (i, j), win_bp [2][nwin][nbeam] (beta, phi)
+  int* win_ord = nullptr;
+  double* win_bp = nullptr;
   // diffraction < 0 (exact per-mode propagation): per wavelength the k
   // indices with beta < betamax in ascending beta, their counts, and the
   // number of mode chunks one launch splits them into
@@ -563,6 +633,13 @@ static int lam_batch(const dtmmx_plan* P, size_t bytes_per_lam) {
 // correction exp(-+i k0 d a_c(w)) of the whole layer (scalars per direction
 // commute with the per-pixel operator), propagated through the layer's
 // sub-steps with the window's mean beam, and summed into `acc`.
+// DTMMX_FUSED_WINDOWS=0 runs one row launch per window instead of the fused
+// window row kernel (same arithmetic, summed in a different order).
+static bool fused_windows_off() {
+  const char* e = std::getenv("DTMMX_FUSED_WINDOWS");
+  return e && e[0] == '0';
+}
+
 template <typename T>
 static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, cpx<T>* tmp,
                             cpx<T>* acc, Batch b, int* status, hipStream_t s, int dir,
@@ -628,6 +705,49 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
       ca.out = tmp;
       TRY(col_pass<T>(P, A, b, 4, OP_SPLIT, s, &ca));
       const size_t fsz = size_t(b.nrays) * b.nlam * 4 * HW;
+      const bool biax = S->biaxial && S->biaxial[l];
+      if (!biax && !fused_windows_off()) {
+        // every window in one launch, summed on chip
+        WinRowArgs<T> wa{};
+        wa.tmp = tmp;
+        wa.out = acc;
+        wa.fsz = fsz;
+        wa.H = H;
+        wa.W = W;
+        wa.b = b;
+        wa.tw = (const cpx<T>*)P->tw_w;
+        const RowArgs<T> la = layer_args<T>(S, HW, l, ds, 1, status);
+        wa.eps_re = la.eps_re;
+        wa.eps_im = la.eps_im;
+        wa.ang = la.ang;
+        wa.comp_stride = la.comp_stride;
+        wa.d = ds;
+        wa.method4 = 1;
+        wa.status = status;
+        wa.wcx = P->wcx;
+        wa.wn = P->wn;
+        wa.nwin = nwin;
+        wa.win_i = P->win_ord;
+        wa.win_j = P->win_ord + nwin;
+        wa.win_beta = P->win_bp;
+        wa.win_phi = P->win_bp + size_t(nwin) * nbeam;
+        wa.nbeam = nbeam;
+        wa.we = el;
+        wa.wd = ds;
+        wa.bm = P->bm;
+        int nr = 1;
+        if (sizeof(T) == 4 && W <= 512 && b.nrays % 2 == 0 && P->bm.rays_per_beam % 2 == 0 &&
+            P->row_rays != 1)
+          nr = 2;
+        const int r = launch_winrow<T>(W, nr, wa, b.nlam_b, s);
+        if (r == DTMMX_OK) {
+          std::swap(A, acc);
+          prev_d = ds;
+          prev_e = el;
+          continue;
+        }
+        if (r != DTMMX_E_UNSUPPORTED) return r;
+      }
       for (int w = 0; w < nwin; ++w) {
         Beams bw = P->bm;
         for (int q = 0; q < nbeam; ++q) {
@@ -2581,6 +2701,33 @@ static int upload_windows(dtmmx_plan* P, const std::vector<int>& amax) {
     e = hipMemcpy(P->wcy, cy.data(), cy.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(P->wmask, mask.data(), mask.size() * sizeof(int), hipMemcpyHostToDevice);
+  // tent-row order: the n windows of one row read the same window-row lines
+  const int nwin = int(t.wi.size());
+  const int nb = P->d.n_rays / P->d.rays_per_beam;
+  std::vector<int> perm(nwin);
+  for (int w = 0; w < nwin; ++w) perm[w] = w;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+    return t.wj[a] != t.wj[b] ? t.wj[a] < t.wj[b] : t.wi[a] < t.wi[b];
+  });
+  std::vector<int> ord(2 * size_t(nwin));
+  std::vector<double> bp(2 * size_t(nwin) * nb);
+  for (int k = 0; k < nwin; ++k) {
+    const int w = perm[k];
+    ord[k] = t.wi[w];
+    ord[nwin + k] = t.wj[w];
+    for (int q = 0; q < nb; ++q) {
+      bp[size_t(k) * nb + q] = t.beta[size_t(w) * nb + q];
+      bp[(size_t(nwin) + k) * nb + q] = t.phi[size_t(w) * nb + q];
+    }
+  }
+  if (nwin > 0) {
+    if (e == hipSuccess) e = hipMalloc((void**)&P->win_ord, ord.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&P->win_bp, bp.size() * sizeof(double));
+    if (e == hipSuccess)
+      e = hipMemcpy(P->win_ord, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(P->win_bp, bp.data(), bp.size() * sizeof(double), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess)
     return fail(e == hipErrorOutOfMemory ? DTMMX_E_NOMEM : DTMMX_E_HIP,
                 std::string("window tables: ") + hipGetErrorString(e));
@@ -2819,6 +2966,8 @@ int dtmmx_plan_destroy(dtmmx_plan* P) {
   if (P->wcx) (void)hipFree(P->wcx);
   if (P->wcy) (void)hipFree(P->wcy);
   if (P->wmask) (void)hipFree(P->wmask);
+  if (P->win_ord) (void)hipFree(P->win_ord);
+  if (P->win_bp) (void)hipFree(P->win_bp);
   if (P->full_modes) (void)hipFree(P->full_modes);
   if (P->full_nmodes) (void)hipFree(P->full_nmodes);
   delete P;

@@ -69,6 +69,16 @@ constexpr int EIG_COMPS = 13;  // per-pixel operator data: alpha0..3, P0, P1, ok
 // row held in registers across wavelengths), forward x-transform.
 // ---------------------------------------------------------------------------
 
+// LDS address maps of the row kernel's layer-operator exchange (padding
+// slots every 2^DTMMX_LDS_PAD_SHIFT positions: 1, or none: 0) and of the
+// column kernel's transform exchanges (0: x * TX + c; 1: padded).
+#ifndef DTMMX_ROP_PADW
+#define DTMMX_ROP_PADW 1
+#endif
+#ifndef DTMMX_COL_PADW
+#define DTMMX_COL_PADW 0
+#endif
+
 template <typename T>
 struct RowArgs {
   cpx<T>* field;
@@ -362,7 +372,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
 #pragma unroll
       for (int j = 0; j < NMID; ++j)
         if (slot_valid<N, REV1, NL, true>(j, lane))
-          line_lds[lds_at<1, 1>(slot_pos<N, REV1, NL, true>(j, lane))] = v[j];
+          line_lds[lds_at<1, DTMMX_ROP_PADW>(slot_pos<N, REV1, NL, true>(j, lane))] = v[j];
       __syncthreads();
       const T kd = T(A.bm.k0[lam] * A.d);
 #pragma unroll
@@ -370,7 +380,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
         const int x = tid + k * NTW;
         if (x < N) {
           const cpx<T>* e = eg[k];
-          const int ax = lds_at<1, 1>(x);
+          const int ax = lds_at<1, DTMMX_ROP_PADW>(x);
           const T ok = e[12].x;
           cpx<T> p0, p1, p2, p3;
           if (fast_ph) {
@@ -418,7 +428,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
 #pragma unroll
       for (int j = 0; j < NMID; ++j)
         if (slot_valid<N, REV1, NL, true>(j, lane))
-          v[j] = line_lds[lds_at<1, 1>(slot_pos<N, REV1, NL, true>(j, lane))];
+          v[j] = line_lds[lds_at<1, DTMMX_ROP_PADW>(slot_pos<N, REV1, NL, true>(j, lane))];
       exchange_sync<WV>();
       reg_fft<T, N, false, -1, NL, 1, 1, MAXV, WV>(v, line_lds, tws, lane);
     }
@@ -442,6 +452,269 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
       }
     }
     exchange_sync<WV>();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused window row kernel (diffraction n > 1, single-step layers): every
+// tent window of the layer in one launch.  The column split pass has written
+// the n window rows t_j = IFFT_y(cy_j Q FFT_y(field)); window (i, j) needs
+//   FFT_x( R_w IFFT_x( cx_i c_w t_j ) )
+// with R_w the per-pixel layer operator of the window's mean beam and c_w
+// its mean-beam correction, and the layer's output is the sum over windows
+// (dtmm/propagate_4x4.py:390-419).  The forward transform is linear, so the
+// sum is formed in real space and transformed once per line:
+//   out = FFT_x( sum_w R_w IFFT_x( cx_i c_w t_j ) ).
+// A work-group owns one row y, NR rays of one beam and a chunk of
+// wavelengths; per window it solves the operator of its pixels once (fp64,
+// reused for the chunk's wavelengths and rays) and adds R_w u to an
+// accumulator kept in LDS, one slot per thread, pixel, line and wavelength.
+// Each window row t_j is read from HBM once per layer (windows come in
+// tent-row order, so the n windows of a row read the same lines back to
+// back) and the output is written once.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct WinRowArgs {
+  const cpx<T>* tmp;  // window rows [wn][ray][lam][comp][H][W]
+  cpx<T>* out;        // layer output [ray][lam][comp][H][W]
+  size_t fsz;         // elements of one window-row field
+  int H, W;
+  Batch b;
+  int nchunk;         // wavelengths per work-group (C)
+  const cpx<T>* tw;
+  const double* eps_re;
+  const double* eps_im;
+  const double* ang;
+  size_t comp_stride;
+  double d;           // layer thickness (single step)
+  int method4;
+  int* status;
+  const double* wcx;  // [beam][lam][wn][W]
+  int wn;
+  int nwin;
+  const int* win_i;   // [nwin], tent-row order
+  const int* win_j;
+  const double* win_beta;  // [nwin][nbeam]
+  const double* win_phi;
+  int nbeam;
+  cd we;              // window correction: effective eps and thickness
+  double wd;
+  Beams bm;
+};
+
+// The window sums fill the LDS, so one work-group runs per CU: the register
+// budget is that of two waves per SIMD (complex128: one).
+template <typename T, int N, int NR, int NC>
+__global__ __launch_bounds__(row_lanes<N>() * NR * NC, (sizeof(T) == 4 ? 2 : 1)) void winrow_kernel(
+    WinRowArgs<T> A) {
+  constexpr int NL = row_lanes<N>();
+  constexpr bool WV = (NL == 64);
+  constexpr int LINES = NR * NC;
+  constexpr int NTW = NL * LINES;
+  constexpr int LS = line_stride<N>();
+  constexpr int MV1 = max_vals<N, true, NL>();
+  constexpr int MV2 = max_vals<N, false, NL>();
+  constexpr int MAXV = MV1 > MV2 ? MV1 : MV2;
+  constexpr int NIN = slot_count<N, true, NL, false>();
+  constexpr int NMID = slot_count<N, true, NL, true>();
+  constexpr int NOUT = slot_count<N, false, NL, true>();
+  constexpr int PPX = (N + NTW - 1) / NTW;
+  __shared__ cpx<T> lds[LINES * LS];
+  __shared__ cpx<T> tws[tw_table_slots<N>()];
+  extern __shared__ __align__(16) unsigned char winrow_dyn[];
+  cpx<T>* accs = reinterpret_cast<cpx<T>*>(winrow_dyn);  // [C][PPX][LINES][NTW]
+  const int tid = threadIdx.x;
+  const int li = tid / NL, lane = tid % NL;
+  const int rr = li / NC, comp = li - rr * NC;
+  const int w = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int ngroups = A.b.nrays / NR;
+  const int nchunks = (A.b.nlam_b + A.nchunk - 1) / A.nchunk;
+  const int rg = w % ngroups;
+  const int t = w / ngroups;
+  const int ch = t % nchunks;
+  const int y = t / nchunks;
+  const int lam_lo = A.b.lam0 + ch * A.nchunk;
+  const int nl = min(A.nchunk, A.b.nlam_b - ch * A.nchunk);
+  const size_t HW = size_t(A.H) * N;
+  const int beam = (rg * NR) / A.bm.rays_per_beam;
+  cpx<T>* line_lds = lds + li * LS;
+  fill_tw_tables<T, N>(tws, A.tw, tid, NTW);
+
+  // line of (window row j, wavelength lam) for this wave
+  auto line_off = [&](int lam) {
+    const size_t q = size_t(rg * NR + rr) * A.b.nlam + lam;
+    return (q * NC + comp) * HW + size_t(y) * N;
+  };
+  // first window's input requested before the material loads and the solve
+  cpx<T> nxt[NIN];
+  {
+    const cpx<T>* g0 = A.tmp + size_t(A.win_j[0]) * A.fsz + line_off(lam_lo);
+#pragma unroll
+    for (int j = 0; j < NIN; ++j)
+      if (slot_valid<N, true, NL, false>(j, lane)) nxt[j] = g0[slot_pos<N, true, NL, false>(j, lane)];
+  }
+  const size_t acc_stride = size_t(PPX) * LINES * NTW;  // elements per wavelength
+  for (int jl = 0; jl < nl; ++jl)
+#pragma unroll
+    for (int k = 0; k < PPX; ++k)
+#pragma unroll
+      for (int l = 0; l < LINES; ++l) accs[jl * acc_stride + (k * LINES + l) * NTW + tid] = mk<T>(T(0), T(0));
+  __syncthreads();
+
+  cpx<T> eg[PPX][EIG_COMPS];
+  bool fast_ph = false;
+  const int nit = A.nwin * nl;
+  for (int it = 0; it < nit; ++it) {
+    const int wi = it / nl, jl = it - wi * nl;
+    const int lam = lam_lo + jl;
+    const double bw = A.win_beta[size_t(wi) * A.nbeam + beam];
+    if (jl == 0) {
+      // per-pixel operator of this window's mean beam, solved while this
+      // window's first line is in flight (the material is re-read from the
+      // cache: registers are kept for the lines)
+      const double phi0 = A.win_phi[size_t(wi) * A.nbeam + beam];
+      double s0, c0;
+      sincos(phi0, &s0, &c0);
+      bool f = true;
+#pragma unroll
+      for (int k = 0; k < PPX; ++k) {
+        const int x = tid + k * NTW;
+        const size_t o = size_t(y) * N + (x < N ? x : 0);
+        const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+        const cd e1 = mk<double>(A.eps_re[A.comp_stride + o], A.eps_im ? A.eps_im[A.comp_stride + o] : 0.);
+        const cd e2 = mk<double>(A.eps_re[2 * A.comp_stride + o],
+                                 A.eps_im ? A.eps_im[2 * A.comp_stride + o] : 0.);
+        RopData r;
+        rop_lean(bw, c0, s0, phi0, e0, e1, e2, A.ang[A.comp_stride + o], A.ang[2 * A.comp_stride + o], r);
+        if (r.kind == KIND_BIAXIAL && x < N && A.status) atomicOr(A.status, 1);
+        for (int i = 0; i < 4; ++i) eg[k][i] = cast<T>(r.a[i]);
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) {
+            eg[k][4 + 2 * i + j] = cast<T>(r.P0[i][j]);
+            eg[k][8 + 2 * i + j] = cast<T>(r.P1[i][j]);
+          }
+        T ok = (A.method4 ? r.ok4 : r.ok2) ? T(1) : T(0);
+        if (r.kind == KIND_BIAXIAL) ok = T(__builtin_nan(""));
+        eg[k][12] = mk<T>(ok, T(0));
+        if (x < N) {
+          const cpx<T>* e = eg[k];
+          f = f && e[0].y == T(0) && e[2].y == T(0);
+          if constexpr (NC == 4)
+            f = f && e[1].x == -e[0].x && e[1].y == T(0) && e[3].x == -e[2].x && e[3].y == T(0);
+        }
+      }
+      fast_ph = __all(f);
+    }
+    cpx<T> v[MAXV];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) v[j] = nxt[j];
+    if (it + 1 < nit) {
+      const int wn1 = (it + 1) / nl, jn = (it + 1) - wn1 * nl;
+      const cpx<T>* gn = A.tmp + size_t(A.win_j[wn1]) * A.fsz + line_off(lam_lo + jn);
+#pragma unroll
+      for (int j = 0; j < NIN; ++j)
+        if (slot_valid<N, true, NL, false>(j, lane)) nxt[j] = gn[slot_pos<N, true, NL, false>(j, lane)];
+    }
+    {
+      // tent column factor and mean-beam correction of this window
+      const int i = A.win_i[wi];
+      const T bwt = T(bw);
+      const cpx<T> ac = csqrt(cast<T>(A.we) - mk<T>(bwt * bwt));
+      const cpx<T> cr = cexpi_layer(ac * T(((comp & 1) ? 1. : -1.) * A.bm.k0[lam] * A.wd));
+      const double* cxl = A.wcx + ((size_t(beam) * A.b.nlam + lam) * A.wn + i) * N;
+#pragma unroll
+      for (int j = 0; j < NIN; ++j)
+        if (slot_valid<N, true, NL, false>(j, lane))
+          v[j] = v[j] * (cr * T(cxl[slot_pos<N, true, NL, false>(j, lane)]));
+    }
+    reg_fft<T, N, true, +1, NL, 1, 1, MAXV, WV>(v, line_lds, tws, lane);
+#pragma unroll
+    for (int j = 0; j < NMID; ++j)
+      if (slot_valid<N, true, NL, true>(j, lane))
+        line_lds[lds_at<1, DTMMX_ROP_PADW>(slot_pos<N, true, NL, true>(j, lane))] = v[j];
+    __syncthreads();
+    const T kd = T(A.bm.k0[lam] * A.d);
+    cpx<T>* acl = accs + jl * acc_stride;
+#pragma unroll
+    for (int k = 0; k < PPX; ++k) {
+      const int x = tid + k * NTW;
+      if (x < N) {
+        const cpx<T>* e = eg[k];
+        const int ax = lds_at<1, DTMMX_ROP_PADW>(x);
+        const T ok = e[12].x;
+        cpx<T> p0, p1, p2, p3;
+        if (fast_ph) {
+          p0 = cexpi_layer(mk<T>(e[0].x * kd, T(0)));
+          p2 = cexpi_layer(mk<T>(e[2].x * kd, T(0)));
+          p1 = conj(p0);
+          p3 = conj(p2);
+        } else {
+          p0 = cexpi_layer(e[0] * kd);
+          p2 = cexpi_layer(e[2] * kd);
+          if constexpr (NC == 4) {
+            p1 = cexpi_layer(e[1] * kd);
+            p3 = cexpi_layer(e[3] * kd);
+          }
+        }
+        const cpx<T> dp = (p0 - p2) * ok;
+        p2 = p2 * ok;
+        const cpx<T> f00 = p2 + dp * e[4], f01 = dp * e[5], f10 = dp * e[6], f11 = p2 + dp * e[7];
+        cpx<T>* a = acl + size_t(k) * LINES * NTW + tid;
+        if constexpr (NC == 4) {
+          const cpx<T> dn = (p1 - p3) * ok;
+          p3 = p3 * ok;
+          const cpx<T> b00 = p3 + dn * e[8], b01 = dn * e[9], b10 = dn * e[10], b11 = p3 + dn * e[11];
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const cpx<T>* L = lds + (r * 4) * LS + ax;
+            const cpx<T> u0 = L[0], u1 = L[LS], u2 = L[2 * LS], u3 = L[3 * LS];
+            cpx<T>* ar = a + (r * 4) * NTW;
+            ar[0] = ar[0] + (f00 * u0 + f01 * u2);
+            ar[2 * NTW] = ar[2 * NTW] + (f10 * u0 + f11 * u2);
+            ar[NTW] = ar[NTW] + (b00 * u1 + b01 * u3);
+            ar[3 * NTW] = ar[3 * NTW] + (b10 * u1 + b11 * u3);
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const cpx<T>* L = lds + (r * 2) * LS + ax;
+            const cpx<T> u0 = L[0], u1 = L[LS];
+            cpx<T>* ar = a + (r * 2) * NTW;
+            ar[0] = ar[0] + (f00 * u0 + f01 * u1);
+            ar[NTW] = ar[NTW] + (f10 * u0 + f11 * u1);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // one forward transform per line and wavelength of the summed windows
+  for (int jl = 0; jl < nl; ++jl) {
+    const cpx<T>* acl = accs + jl * acc_stride;
+#pragma unroll
+    for (int k = 0; k < PPX; ++k) {
+      const int x = tid + k * NTW;
+      if (x < N) {
+        const int ax = lds_at<1, DTMMX_ROP_PADW>(x);
+#pragma unroll
+        for (int l = 0; l < LINES; ++l) lds[l * LS + ax] = acl[(size_t(k) * LINES + l) * NTW + tid];
+      }
+    }
+    __syncthreads();
+    cpx<T> v[MAXV];
+#pragma unroll
+    for (int j = 0; j < NMID; ++j)
+      if (slot_valid<N, true, NL, true>(j, lane))
+        v[j] = line_lds[lds_at<1, DTMMX_ROP_PADW>(slot_pos<N, true, NL, true>(j, lane))];
+    exchange_sync<WV>();
+    reg_fft<T, N, false, -1, NL, 1, 1, MAXV, WV>(v, line_lds, tws, lane);
+    cpx<T>* gl = A.out + line_off(lam_lo + jl);
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j)
+      if (slot_valid<N, false, NL, true>(j, lane)) gl[slot_pos<N, false, NL, true>(j, lane)] = v[j];
+    __syncthreads();
   }
 }
 
@@ -846,7 +1119,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
   constexpr int MAXV = MV1 > MV2 ? MV1 : MV2;
   constexpr int NIN = slot_count<N, REV1, NL, false>();
   constexpr int NMID = slot_count<N, REV1, NL, true>();
-  __shared__ cpx<T> lds[N * TX];
+  __shared__ cpx<T> lds[(N + (DTMMX_COL_PADW ? (N >> DTMMX_LDS_PAD_SHIFT) : 0)) * TX];
   __shared__ cpx<T> tws[tw_table_slots<N>()];  // twiddles in LDS: their reads never wait on global loads
   const int tid = threadIdx.x;
   const int c = tid % TX, g = tid / TX;
@@ -944,7 +1217,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
         if (xin && slot_valid<N, REV1, NL, false>(j, g))
           nxt[j] = bn[size_t(slot_pos<N, REV1, NL, false>(j, g)) * A.W];
     }
-    reg_fft<T, N, REV1, DIR1, NL, TX, 0>(v, lds + c, tws, g);
+    reg_fft<T, N, REV1, DIR1, NL, TX, DTMMX_COL_PADW * TX>(v, lds + c, tws, g);
     if constexpr (KOP) {
       const bool back = A.dense_sign && (comp & 1);
       const T s2 = T(A.scale * A.scale);
@@ -968,7 +1241,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
             u[j] = v[j] * cy;
           }
           __syncthreads();
-          reg_fft<T, N, true, +1, NL, TX, 0>(u, lds + c, tws, g);
+          reg_fft<T, N, true, +1, NL, TX, DTMMX_COL_PADW * TX>(u, lds + c, tws, g);
           cpx<T>* ob = base + wj * fstride;
 #pragma unroll
           for (int j = 0; j < NOUT; ++j)
@@ -978,7 +1251,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
         __syncthreads();
         continue;
       }
-      reg_fft<T, N, true, +1, NL, TX, 0>(v, lds + c, tws, g);
+      reg_fft<T, N, true, +1, NL, TX, DTMMX_COL_PADW * TX>(v, lds + c, tws, g);
     }
     constexpr bool REVL = KOP ? true : REV1;
     constexpr int NOUT = slot_count<N, REVL, NL, true>();
