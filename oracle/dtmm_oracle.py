@@ -1442,10 +1442,54 @@ def propagate_2x2_effective_1(field, ks, layer_out, effective_layer_out, beta=0,
             # reflection (propagate_2x2.py:58), which fails without diffraction
             raise NotImplementedError("reflection mode 1 with npass > 1 needs diffraction 1")
         dmat1 = dmat2 = None
+    elif diffraction > 1 and diffraction != np.inf:
+        return _propagate_2x2_windows(field, ks, layer_out, effective_layer_out, beta, phi,
+                                      nsteps, int(diffraction), betamax, mode, bulk, cache,
+                                      tmat, rmat, refl)
     else:
-        raise NotImplementedError("oracle covers diffraction in {0, 1} for the 2x2 method")
+        raise NotImplementedError("oracle covers finite diffraction >= 0 for the 2x2 method")
     return _transfer_ray_2x2_1(field, ks, layer_out, dmat1, dmat2, beta, phi, nsteps, mode, bulk,
                                tmat, rmat, refl)
+
+
+def _propagate_2x2_windows(field, ks, layer_out, effective_layer_out, beta, phi, nsteps, n,
+                           betamax, mode, bulk, cache, tmat, rmat, refl):
+    """diffraction = n > 1 (dtmm/propagate_2x2.py:212-257): the k-space E
+    field (and the stored reflection) is split by the n*n tent windows of
+    fft_mask, each part goes through the layer step with its window's mean
+    beam and that beam's E diffraction matrices, and the parts (and their
+    reflections) are summed.  `bulk` receives every part's contribution."""
+    d_out, epsv_out, epsa_out = effective_layer_out
+    shape = field.shape[-2:]
+    beta = np.asarray(beta, FDTYPE)
+    phi = np.asarray(phi, FDTYPE)
+    if beta.ndim > 0:
+        bshape = beta.shape
+        beta = beta[..., 0]
+        phi = phi[..., 0]
+    else:
+        bshape = ()
+    windows, (betas, phis) = fft_mask(field.shape, ks, n, beta * np.cos(phi),
+                                      beta * np.sin(phi), betamax)
+    nw = len(windows)
+    betas = betas.reshape((nw,) + bshape)
+    phis = phis.reshape((nw,) + bshape)
+    key = _key(shape, ks, betas, phis, d_out, epsv_out, epsa_out, betamax, mode)
+    dmats1 = cache.get(("ew1",) + key, lambda: first_E_diffraction_matrix(
+        shape, ks, betas, phis, d_out / 2, epsv_out, epsa_out, mode, betamax))
+    dmats2 = cache.get(("ew2",) + key, lambda: second_E_diffraction_matrix(
+        shape, ks, betas, phis, d_out / 2, epsv_out, epsa_out, mode, betamax))
+    fout = np.zeros_like(field)
+    refl_sum = None if refl is None else np.zeros_like(refl)
+    for window, b, p, dm1, dm2 in zip(windows, betas, phis, dmats1, dmats2):
+        rpart = None if refl is None else refl * window
+        fout += _transfer_ray_2x2_1(field * window, ks, layer_out, dm1, dm2, b, p, nsteps, mode,
+                                    bulk, tmat, rmat, rpart)
+        if refl is not None and tmat is not None:
+            refl_sum += rpart
+    if refl is not None:
+        refl[...] = refl_sum
+    return fout
 
 
 # ---------------------------------------------------------------------------
@@ -1837,8 +1881,8 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
         refl = [None] * n
     if reflection == 2 and diffraction > 1:
         raise NotImplementedError("oracle covers 2x2 reflection 2 with diffraction 0 or 1")
-    if diffraction not in (0, 1):
-        raise NotImplementedError("oracle covers diffraction 0 or 1 for the 2x2 method")
+    if not (diffraction >= 0 and diffraction != np.inf):
+        raise NotImplementedError("oracle covers finite diffraction >= 0 for the 2x2 method")
     # k-space for reflection 0/1 with diffraction, and reflection 1 always
     # (transfer.py:1085-1093); real space otherwise
     work_in_fft = reflection == 1 or (reflection == 0 and diffraction > 0)

@@ -590,7 +590,7 @@ def _roll_hw(a, dy, dx, axes=(-2, -1)):
 
 @pytest.mark.parametrize("method,diffraction,npass", [
     ("4x4", 1, 1), ("4x4", 3, 1), ("4x4", 1, 3), ("4x4", 3, 3),
-    ("2x2", 1, 1), ("2x2", 0, 1), ("4x4", 0, 1)])
+    ("2x2", 1, 1), ("2x2", 0, 1), ("4x4", 0, 1), ("2x2", 3, 1)])
 def test_transfer_is_translation_equivariant(method, diffraction, npass):
     """Every operator of the path is either per pixel (the layer operator,
     acting on the material at the same pixel) or per k (diffraction, tent

@@ -156,3 +156,96 @@ def test_reflection2_windows_are_the_sum_of_window_parts(n):
     bb, _ = o.betaphi(shape, ks)
     lim = np.fft.ifft2(fk * (bb < o.BETAMAX)[:, None])
     np.testing.assert_allclose(out0[:, 0::2], lim[:, 0::2], atol=1e-12 * np.abs(lim).max())
+
+
+# ---------------------------------------------------------------------------
+# 2x2 method with windows (dtmm/propagate_2x2.py:212-257)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [2, 3, 5])
+@pytest.mark.parametrize("reflection", [0, 1])
+def test_2x2_windowed_isotropic_slab_is_angular_spectrum_propagation(n, reflection):
+    """Through a homogeneous isotropic slab equal to the effective medium every
+    window's correction cancels its own phase, so the summed E is the input's
+    forward part propagated exactly per FFT mode (betamax mask)."""
+    eps_n, L = 1.55, 4
+    shape = (24, 32)
+    wl = np.array([500., 700.])
+    ps = 100.
+    beta, phi = 0.1, 0.7
+    field, _, _ = o.illumination_data(shape, wl, pixelsize=ps, beta=beta, phi=phi, jones=(1, 1j),
+                                      window=o.blackman(shape), n=eps_n)
+    eps = np.broadcast_to([eps_n ** 2] * 3, (L, 1, 1, 3))
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, np.zeros((L, 1, 1, 3)))],
+                           beta=beta, phi=phi, method="2x2", reflection=reflection,
+                           nin=eps_n, nout=eps_n, diffraction=n)[0]
+    k = o.k0(wl, ps)
+    b, _ = o.betaphi(shape, k)
+    prop = np.exp(1j * k[:, None, None] * L * np.sqrt(eps_n ** 2 - b ** 2 + 0j))
+    prop[b >= o.BETAMAX] = 0.
+    fwd = o.transmitted_field(field, k, n=eps_n)
+    expect = np.fft.ifft2(np.fft.fft2(fwd) * prop[:, None])
+    np.testing.assert_allclose(out[:, 0::2], expect[:, 0::2], atol=1e-12 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("n", [3, 5])
+def test_2x2_windowed_uniaxial_slab_is_jones_retarder(n):
+    no, ne, L, phid = 1.5, 1.62, 4, 0.6
+    shape = (9, 9)
+    wl = np.array([480., 620.])
+    ps = 100.
+    neff = np.sqrt((2 * no ** 2 + ne ** 2) / 3)
+    eps = np.broadcast_to([no ** 2, no ** 2, ne ** 2], (L, 1, 1, 3))
+    ang = np.broadcast_to([0., np.pi / 2, phid], (L, 1, 1, 3))
+    pol = (np.cos(0.3), np.sin(0.3))
+    H, W = shape
+    field = np.zeros((len(wl), 4, H, W), complex)
+    field[:, 0], field[:, 1] = pol[0], neff * pol[0]
+    field[:, 2], field[:, 3] = pol[1], -neff * pol[1]
+    out = o.transfer_field((field, wl, ps), [(np.ones(L), eps, ang)], beta=0., phi=0.,
+                           method="2x2", nin=neff, nout=neff, diffraction=n)[0]
+    k = o.k0(wl, ps)
+    c, s = np.cos(phid), np.sin(phid)
+    R = np.array([[c, -s], [s, c]])
+    for i in range(len(wl)):
+        J = R @ np.diag([np.exp(1j * k[i] * L * ne), np.exp(1j * k[i] * L * no)]) @ R.T
+        e = J @ np.array(pol)
+        np.testing.assert_allclose(out[i, 0], e[0], atol=1e-12)
+        np.testing.assert_allclose(out[i, 2], e[1], atol=1e-12)
+        # output medium: H of the normal forward wave
+        np.testing.assert_allclose(out[i, 1], neff * e[0], atol=1e-12)
+        np.testing.assert_allclose(out[i, 3], -neff * e[1], atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [2, 3])
+@pytest.mark.parametrize("reflection", [0, 1])
+def test_2x2_windows_are_the_sum_of_window_parts(n, reflection):
+    """One layer of the 2x2 method with diffraction n > 1: the output E is the
+    sum over the tent windows of the diffraction-1 transfer (pinned
+    separately) of each window's part ifft2(fft2(field) * window) with that
+    window's mean beam.  (H is rebuilt per window in the output medium, so
+    only E is compared.)"""
+    shape = (12, 16)
+    wl = np.array([520., 640.])
+    ps = 100.
+    beta, phi = 0.1, 0.7
+    rng = np.random.default_rng(5)
+    field = rng.normal(size=(2, 4) + shape) + 1j * rng.normal(size=(2, 4) + shape)
+    eps = np.stack([rng.uniform(2.2, 2.4, shape), rng.uniform(2.2, 2.4, shape),
+                    rng.uniform(2.6, 2.9, shape)], -1)[None]
+    eps[..., 1] = eps[..., 0]
+    ang = np.stack([np.zeros(shape), rng.uniform(0, 1.5, shape), rng.uniform(-3, 3, shape)],
+                   -1)[None]
+    block = (np.asarray([1.3]), eps, ang)
+    kw = dict(method="2x2", reflection=reflection, nstep=2, nin=1.2, nout=1.4)
+    out = o.transfer_field((field, wl, ps), [block], beta=beta, phi=phi, diffraction=n, **kw)[0]
+    ks = o.k0(wl, ps)
+    windows, (betas, phis) = o.fft_mask(field.shape, ks, n, beta * np.cos(phi),
+                                        beta * np.sin(phi), o.BETAMAX)
+    fk = np.fft.fft2(field)
+    expect = np.zeros_like(field)
+    for w, b, p in zip(windows, betas, phis):
+        part = np.fft.ifft2(fk * w)
+        expect += o.transfer_field((part, wl, ps), [block], beta=float(b), phi=float(p),
+                                   diffraction=1, **kw)[0]
+    np.testing.assert_allclose(out[:, 0::2], expect[:, 0::2], atol=1e-12 * np.abs(expect).max())
