@@ -185,7 +185,7 @@ static int max_lds_per_block() {
   return v;
 }
 
-template <typename T>
+template <typename T, int NC>
 static int launch_winrow(int n, int nr, WinRowArgs<T> a, int nlam, hipStream_t s) {
   switch (n) {
 #define X(N)                                                                                   \
@@ -193,10 +193,10 @@ static int launch_winrow(int n, int nr, WinRowArgs<T> a, int nlam, hipStream_t s
     constexpr int NRM = row_nr_max<T, N>() >= 2 ? 2 : 1;                                       \
     auto go = [&](auto nrc) -> int {                                                           \
       constexpr int NRX = decltype(nrc)::value;                                                \
-      constexpr int NTW = row_lanes<N>() * NRX * 4;                                            \
+      constexpr int NTW = row_lanes<N>() * NRX * NC;                                           \
       constexpr int PPX = (N + NTW - 1) / NTW;                                                 \
-      const size_t stat = (size_t(NRX * 4) * line_stride<N>() + tw_table_slots<N>()) * sizeof(cpx<T>); \
-      const size_t per = size_t(PPX) * NRX * 4 * NTW * sizeof(cpx<T>);                         \
+      const size_t stat = (size_t(NRX * NC) * line_stride<N>() + tw_table_slots<N>()) * sizeof(cpx<T>); \
+      const size_t per = size_t(PPX) * NRX * NC * NTW * sizeof(cpx<T>);                        \
       const size_t cap = size_t(max_lds_per_block());                                          \
       int c = cap > stat ? int((cap - stat) / per) : 0;                                        \
       if (const char* e = std::getenv("DTMMX_WIN_C")) c = std::min(c, std::atoi(e));          \
@@ -205,10 +205,10 @@ static int launch_winrow(int n, int nr, WinRowArgs<T> a, int nlam, hipStream_t s
       a.nchunk = c;                                                                            \
       const int nwork = a.H * (a.b.nrays / NRX) * ((nlam + c - 1) / c);                        \
       const size_t shm = size_t(c) * per;                                                      \
-      (void)hipFuncSetAttribute((const void*)winrow_kernel<T, N, NRX, 4>,                      \
+      (void)hipFuncSetAttribute((const void*)winrow_kernel<T, N, NRX, NC>,                     \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(shm));         \
       timer_start(0, s);                                                                       \
-      DTMMX_LAUNCH_SHM((winrow_kernel<T, N, NRX, 4>), dim3(nwork), dim3(NTW), shm, s, a);      \
+      DTMMX_LAUNCH_SHM((winrow_kernel<T, N, NRX, NC>), dim3(nwork), dim3(NTW), shm, s, a);     \
       timer_stop(s);                                                                           \
       return DTMMX_OK;                                                                         \
     };                                                                                         \
@@ -495,6 +495,12 @@ static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, 
     case KOP_E2H4:
       DTMMX_LAUNCH((kspace_kernel<T, KOP_E2H4, false>), grid, dim3(NT), s, a);
       break;
+    case KOP_IF2W:
+      DTMMX_LAUNCH((kspace_kernel<T, KOP_IF2W, false>), grid, dim3(NT), s, a);
+      break;
+    case KOP_OUT2W:
+      DTMMX_LAUNCH((kspace_kernel<T, KOP_OUT2W, false>), grid, dim3(NT), s, a);
+      break;
     default:
       timer_stop(s);
       return fail(DTMMX_E_ARG, "bad k-space operator");
@@ -587,6 +593,10 @@ static WsLayout ws_layout(const dtmmx_plan* P) {
     w.win_acc = take(f4);
   }
   if (!m4) w.e2 = take(planes * 2 * HW * cb);
+  if (!m4 && P->wn > 1) {
+    w.win_tmp = take(planes * 2 * HW * cb * P->wn);
+    w.win_acc = take(planes * 2 * HW * cb);
+  }
   if (!m4 && P->d.npass > 1) w.tmp2 = take(planes * 2 * HW * cb);
   if (P->d.diffraction < 0) {
     w.full_spec = take(f4);
@@ -739,7 +749,7 @@ static int transfer_windows(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* o
         if (sizeof(T) == 4 && W <= 512 && b.nrays % 2 == 0 && P->bm.rays_per_beam % 2 == 0 &&
             P->row_rays != 1)
           nr = 2;
-        const int r = launch_winrow<T>(W, nr, wa, b.nlam_b, s);
+        const int r = launch_winrow<T, 4>(W, nr, wa, b.nlam_b, s);
         if (r == DTMMX_OK) {
           std::swap(A, acc);
           prev_d = ds;
@@ -849,6 +859,186 @@ static int copy_planes(cpx<T>* dst, const cpx<T>* src, const Batch& b, size_t HW
   return DTMMX_OK;
 }
 
+// 2x2 method with tent windows (diffraction n > 1, reflection mode 0 or 1,
+// npass 1; dtmm/propagate_2x2.py:212-257).  As the 4x4 windows: the E field
+// is projected onto the forward modes of the input medium (and through the
+// Fresnel interface into the first layer in reflection mode 1); per layer the
+// column pass writes the window rows with the common half-step phases and
+// the window row pass propagates every window's part with its own mean beam
+// and sums them; interfaces between different effective media get the
+// Fresnel transmission (mode 1); the output medium builds H per window.
+// `out` holds the 4-component input on entry and the output on return; `e2`
+// is the 2-component field.
+template <typename T>
+static int transfer_windows_2x2(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out,
+                                cpx<T>* e2, cpx<T>* tmp, cpx<T>* acc, Batch b, int* status,
+                                hipStream_t s, void* etab) {
+  const int H = P->d.height, W = P->d.width;
+  const size_t HW = size_t(H) * W;
+  const int L = S->n_layers;
+  const double scale = 1.0 / double(HW);
+  const int nbeam = P->d.n_rays / P->d.rays_per_beam;
+  const int nwin = int(P->win_i.size());
+  const cd ein = mk<double>(S->n_in * S->n_in), eout = mk<double>(S->n_out * S->n_out);
+  auto eff = [&](int l) {
+    return mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
+  };
+  TRY(row_pass<T>(P, out, b, 4, OP_FWD, s));
+  TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+  KArgs<T> h{};
+  h.kop = KOP_PROJ2;  // forward projection (and Fresnel into the first layer) only
+  h.d0 = 0.;
+  h.e0 = L > 0 ? eff(0) : eout;
+  h.ein = ein;
+  h.scale = 1.0 / double(H);
+  h.fresnel = P->d.reflection;
+  TRY(kspace<T>(P, out, e2, b, h, s));
+  TRY(col_pass<T>(P, e2, b, 2, OP_INV, s));
+  cpx<T>* A = e2;
+  double prev_d = 0.;  // half step of the previous layer still to apply
+  cd prev_e = L > 0 ? eff(0) : eout;
+  const size_t fsz = size_t(b.nrays) * b.nlam * 2 * HW;
+  for (int l = 0; l < L; ++l) {
+    const int ns = S->n_steps ? S->n_steps[l] : 1;
+    const double ds = S->thickness[l] / double(ns);
+    const cd el = eff(l);
+    if (!eq(el, prev_e)) {
+      TRY(col_pass<T>(P, A, b, 2, OP_FWD, s));
+      KArgs<T> t{};
+      t.kop = KOP_IF2W;
+      t.d0 = prev_d;
+      t.e0 = prev_e;
+      t.e1 = el;
+      t.fresnel = P->d.reflection;
+      t.scale = 1.0 / double(H);
+      TRY(kspace<T>(P, A, A, b, t, s));
+      TRY(col_pass<T>(P, A, b, 2, OP_INV, s));
+      prev_d = 0.;
+      prev_e = el;
+    }
+    const bool biax = S->biaxial && S->biaxial[l];
+    bool done = false;
+    if (ns == 1) {
+      ColArgs<T> ca{};
+      ca.da = prev_d;
+      ca.ea = prev_e;
+      ca.db = ds;
+      ca.eb = el;
+      ca.scale = scale;
+      ca.dense_sign = 0;
+      ca.out = tmp;
+      if (!biax && !fused_windows_off()) {
+        TRY(col_pass<T>(P, A, b, 2, OP_SPLIT, s, &ca));
+        WinRowArgs<T> wa{};
+        wa.tmp = tmp;
+        wa.out = acc;
+        wa.fsz = fsz;
+        wa.H = H;
+        wa.W = W;
+        wa.b = b;
+        wa.tw = (const cpx<T>*)P->tw_w;
+        const RowArgs<T> la = layer_args<T>(S, HW, l, ds, 0, status);
+        wa.eps_re = la.eps_re;
+        wa.eps_im = la.eps_im;
+        wa.ang = la.ang;
+        wa.comp_stride = la.comp_stride;
+        wa.d = ds;
+        wa.method4 = 0;
+        wa.status = status;
+        wa.wcx = P->wcx;
+        wa.wn = P->wn;
+        wa.nwin = nwin;
+        wa.win_i = P->win_ord;
+        wa.win_j = P->win_ord + nwin;
+        wa.win_beta = P->win_bp;
+        wa.win_phi = P->win_bp + size_t(nwin) * nbeam;
+        wa.nbeam = nbeam;
+        wa.we = el;
+        wa.wd = ds;
+        wa.bm = P->bm;
+        int nr = 1;
+        if (sizeof(T) == 4 && W <= 512 && b.nrays % 2 == 0 && P->bm.rays_per_beam % 2 == 0 &&
+            P->row_rays != 1)
+          nr = 2;
+        const int r = launch_winrow<T, 2>(W, nr, wa, b.nlam_b, s);
+        if (r != DTMMX_OK && r != DTMMX_E_UNSUPPORTED) return r;
+        done = r == DTMMX_OK;
+      }
+    }
+    if (!done) {
+      // one window at a time: its part through the layer's sub-steps with its
+      // mean beam, added to the window sum
+      for (int w = 0; w < nwin; ++w) {
+        Beams bw = P->bm;
+        for (int q = 0; q < nbeam; ++q) {
+          bw.beta[q] = P->win_beta[size_t(w) * nbeam + q];
+          bw.phi[q] = P->win_phi[size_t(w) * nbeam + q];
+        }
+        ColArgs<T> ca{};
+        ca.da = prev_d;
+        ca.ea = prev_e;
+        ca.kca = 0.;
+        ca.db = ds;
+        ca.eb = el;
+        ca.kcb = 2.0 * ns;
+        ca.scale = scale;
+        ca.dense_sign = 0;
+        ca.out = tmp;
+        ca.wi = P->win_i[w];
+        ca.wj = P->win_j[w];
+        TRY(col_pass<T>(P, A, b, 2, OP_WIN, s, &ca, &bw));
+        for (int st = 0; st < ns; ++st) {
+          RowArgs<T> la = layer_args<T>(S, HW, l, ds, 0, status, etab);
+          if (st > 0 && la.etab_ws) {
+            la.etab = la.etab_ws;
+            la.etab_nbeam = b.nrays / P->bm.rays_per_beam;
+            la.etab_ws = nullptr;
+          }
+          if (st + 1 == ns) {
+            la.acc = acc;
+            la.acc_mode = (w == 0) ? 1 : 2;
+          }
+          TRY(row_pass<T>(P, tmp, b, 2, OP_ROP, s, &la, &bw));
+          if (st + 1 < ns) {
+            ColArgs<T> cs{};
+            cs.da = ds;
+            cs.db = ds;
+            cs.ea = el;
+            cs.eb = el;
+            cs.scale = scale;
+            cs.dense_sign = 0;
+            TRY(col_pass<T>(P, tmp, b, 2, OP_DIAG, s, &cs, &bw));
+          }
+        }
+      }
+    }
+    std::swap(A, acc);
+    prev_d = ds;
+    prev_e = el;
+  }
+  // output medium: last half step, Fresnel (mode 1), H per window
+  TRY(col_pass<T>(P, A, b, 2, OP_FWD, s));
+  KArgs<T> o{};
+  o.kop = KOP_OUT2W;
+  o.d0 = prev_d;
+  o.e0 = prev_e;
+  o.e1 = eout;
+  o.fresnel = P->d.reflection;
+  o.scale = scale;
+  o.wcx = P->wcx;
+  o.wcy = P->wcy;
+  o.win_i = P->win_ord;
+  o.win_j = P->win_ord + nwin;
+  o.win_beta = P->win_bp;
+  o.win_phi = P->win_bp + size_t(nwin) * nbeam;
+  o.wn = P->wn;
+  o.nwin = nwin;
+  o.nbeam = nbeam;
+  TRY(kspace<T>(P, A, out, b, o, s));
+  TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+  return row_pass<T>(P, out, b, 4, OP_INV, s);
+}
+
 template <typename T>
 static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, void* ws,
                      int* status, hipStream_t s, int dir, cpx<T>* bulk = nullptr) {
@@ -860,6 +1050,18 @@ static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, voi
   const bool m4 = P->d.method == DTMMX_METHOD_4X4;
   const WsLayout wl = ws_layout(P);
   char* wsb = (char*)ws;
+  if (!m4 && P->wn > 1) {
+    if (P->win_i.empty()) {  // no window passes betamax: every part is zero
+      const size_t n = size_t(nplanes) * 4 * HW * 2;
+      hipLaunchKernelGGL((zero_kernel<T>), dim3(unsigned((n + NT - 1) / NT)), dim3(NT), 0, s,
+                         (T*)out, n);
+      HIP_TRY(hipGetLastError());
+      return DTMMX_OK;
+    }
+    Batch b{nrays, nlam, 0, nlam};
+    return transfer_windows_2x2<T>(P, S, out, (cpx<T>*)(wsb + wl.e2), (cpx<T>*)(wsb + wl.win_tmp),
+                                   (cpx<T>*)(wsb + wl.win_acc), b, status, s, wsb + wl.etab);
+  }
   if (m4 && P->wn > 1) {
     if (S->n_layers == 0) return DTMMX_OK;  // ifft2(fft2(field_in))
     if (P->win_i.empty()) {  // no window passes betamax: every part is zero
@@ -2862,8 +3064,10 @@ static int validate_desc(const dtmmx_plan_desc* d) {
     return fail(DTMMX_E_UNSUPPORTED,
                 "diffraction must be >= 1, or 0 for the 4x4 method in reflection mode 2 and the "
                 "2x2 method in reflection mode 0 with npass 1");
-  if (d->diffraction > 1 && d->method != DTMMX_METHOD_4X4)
-    return fail(DTMMX_E_UNSUPPORTED, "diffraction > 1 is built for the 4x4 method only");
+  if (d->diffraction > 1 && d->method != DTMMX_METHOD_4X4 &&
+      !(d->npass == 1 && (d->reflection == 0 || d->reflection == 1)))
+    return fail(DTMMX_E_UNSUPPORTED, "diffraction > 1 is built for the 4x4 method and the 2x2 "
+                                     "method in reflection modes 0 and 1 with npass 1");
   if (d->diffraction > 64) return fail(DTMMX_E_ARG, "diffraction above 64");
   if (d->npass < 1) return fail(DTMMX_E_ARG, "npass must be >= 1");
   // 2x2 with npass > 1: reflection mode 1 (diffraction 1) or 2 (diffraction 0

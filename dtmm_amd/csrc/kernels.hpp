@@ -621,7 +621,9 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (sizeof(T) == 4 ? 2 : 1))
       const int i = A.win_i[wi];
       const T bwt = T(bw);
       const cpx<T> ac = csqrt(cast<T>(A.we) - mk<T>(bwt * bwt));
-      const cpx<T> cr = cexpi_layer(ac * T(((comp & 1) ? 1. : -1.) * A.bm.k0[lam] * A.wd));
+      // forward components (4x4: even, 2x2: both) exp(-i k0 d a_c), backward exp(+i ...)
+      const double sg = (NC == 4 && (comp & 1)) ? 1. : -1.;
+      const cpx<T> cr = cexpi_layer(ac * T(sg * A.bm.k0[lam] * A.wd));
       const double* cxl = A.wcx + ((size_t(beam) * A.b.nlam + lam) * A.wn + i) * N;
 #pragma unroll
       for (int j = 0; j < NIN; ++j)
@@ -1268,7 +1270,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
 // ---------------------------------------------------------------------------
 
 enum { KOP_HEAD4 = 0, KOP_TAIL4 = 1, KOP_BOUND4 = 2, KOP_PROJ2 = 3, KOP_TR2 = 4, KOP_FIELD4 = 5,
-       KOP_E2H4 = 6 };
+       KOP_E2H4 = 6, KOP_IF2W = 7, KOP_OUT2W = 8 };
 
 template <typename T>
 struct KArgs {
@@ -1286,6 +1288,15 @@ struct KArgs {
   int fresnel;    // PROJ2: 1 applies the interface transmission ein -> e0
   int adjoint;    // 1: apply the conjugate transpose (4-component operators)
   int fmode;      // FIELD4: 0 both directions, +1 forward modes only, -1 backward only
+  // OUT2W: the tent windows (separable factors [beam][lam][wn][W or H], the
+  // window list and each window's mean beam [nwin][nbeam])
+  const double* wcx;
+  const double* wcy;
+  const int* win_i;
+  const int* win_j;
+  const double* win_beta;
+  const double* win_phi;
+  int wn, nwin, nbeam;
   Beams bm;
 };
 
@@ -1460,6 +1471,19 @@ __device__ __forceinline__ void kop_apply(const KArgs<T>& A, bool pass, double b
     const cd ep = (v[0] * c + v[1] * sn) * tp, es = (v[1] * c - v[0] * sn) * ts;
     r[0] = K * (ep * c - es * sn);
     r[1] = K * (ep * sn + es * c);
+  } else if constexpr (KOP == KOP_IF2W) {
+    if (!pass) return;
+    // 2x2 with tent windows, interface between layers of different
+    // effective media: the common half step exp(i k0 d0/2 a0(k)) of the
+    // layer before it (each window applies its own mean-beam correction with
+    // its part), then the Fresnel transmission e0 -> e1 in reflection mode 1
+    cd K = mk<double>(1.);
+    if (A.d0 != 0.) K = cexpi(csqrt(A.e0 - mk<double>(beta * beta)) * (0.5 * A.d0 * k0));
+    cd tp = mk<double>(1.), ts = mk<double>(1.);
+    if (A.fresnel) fresnel_t(beta, A.e0, A.e1, tp, ts);
+    const cd ep = (v[0] * c + v[1] * sn) * tp, es = (v[1] * c - v[0] * sn) * ts;
+    r[0] = K * (ep * c - es * sn);
+    r[1] = K * (ep * sn + es * c);
   } else if constexpr (KOP == KOP_FIELD4) {
     if (!pass) return;
     // F P(kd) F^-1 of an isotropic medium in closed form
@@ -1538,8 +1562,8 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
   const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
   const double beta = (2 * M_PI / k0) * sqrt(fx * fx + fy * fy);
   const double phi = atan2(fy, fx);
-  constexpr int nin = (KOP == KOP_TR2) ? 2 : 4;
-  constexpr int nout = (KOP == KOP_PROJ2 || KOP == KOP_TR2) ? 2 : 4;
+  constexpr int nin = (KOP == KOP_TR2 || KOP == KOP_IF2W || KOP == KOP_OUT2W) ? 2 : 4;
+  constexpr int nout = (KOP == KOP_PROJ2 || KOP == KOP_TR2 || KOP == KOP_IF2W) ? 2 : 4;
   const cpx<T>* s = A.src + size_t(q) * nin * HW + k;
   cpx<T>* o = A.dst + size_t(q) * nout * HW + k;
   cd v[4];
@@ -1548,7 +1572,43 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
   const double fr = sqrt(fx * fx + fy * fy);
   const double c = fr > 0. ? fx / fr : 1., sn = fr > 0. ? fy / fr : 0.;
   cd r[4] = {mk<double>(0.), mk<double>(0.), mk<double>(0.), mk<double>(0.)};
-  if constexpr (!ADJ) {
+  if constexpr (KOP == KOP_OUT2W) {
+    // 2x2 with tent windows, output medium e1 (dtmm/propagate_2x2.py:
+    // 212-257 with the zero-thickness output layer): the last layer's common
+    // half step exp(i k0 d0/2 a0(k)), the Fresnel transmission e0 -> e1 in
+    // reflection mode 1, then E = sum_w w(k) E' and H = sum_w w(k) G_w E'
+    // with G_w the E -> H matrix of the forward modes of e1 at window w's
+    // mean beam (each window part builds its H with its own beam)
+    if (pass) {
+      kop_apply<T, KOP_IF2W>(A, pass, beta, phi, c, sn, k0, bb, pb, v, r);
+      const size_t bl = size_t(beam) * A.b.nlam + lam;
+      cd sw = mk<double>(0.), G00 = sw, G01 = sw, G10 = sw, G11 = sw;
+      for (int w = 0; w < A.nwin; ++w) {
+        const double f = A.wcx[(bl * A.wn + A.win_i[w]) * A.W + kx] *
+                         A.wcy[(bl * A.wn + A.win_j[w]) * A.H + ky];
+        if (f == 0.) continue;
+        const double bw = A.win_beta[size_t(w) * A.nbeam + beam];
+        const double pw = A.win_phi[size_t(w) * A.nbeam + beam];
+        Modes m;
+        iso_modes(bw, A.e1, m);
+        rotate_rows(m, cos(pw), sin(pw));
+        cd Am[2][2] = {{m.F[0][0], m.F[0][2]}, {m.F[2][0], m.F[2][2]}};
+        cd Bm[2][2] = {{m.F[1][0], m.F[1][2]}, {m.F[3][0], m.F[3][2]}};
+        cd Ai[2][2];
+        inv2(Am, Ai);
+        sw = sw + mk<double>(f);
+        G00 = G00 + (Bm[0][0] * Ai[0][0] + Bm[0][1] * Ai[1][0]) * f;
+        G01 = G01 + (Bm[0][0] * Ai[0][1] + Bm[0][1] * Ai[1][1]) * f;
+        G10 = G10 + (Bm[1][0] * Ai[0][0] + Bm[1][1] * Ai[1][0]) * f;
+        G11 = G11 + (Bm[1][0] * Ai[0][1] + Bm[1][1] * Ai[1][1]) * f;
+      }
+      const cd ex = r[0], ey = r[1];
+      r[0] = sw * ex;
+      r[1] = G00 * ex + G01 * ey;
+      r[2] = sw * ey;
+      r[3] = G10 * ex + G11 * ey;
+    }
+  } else if constexpr (!ADJ) {
     kop_apply<T, KOP>(A, pass, beta, phi, c, sn, k0, bb, pb, v, r);
   } else if (pass) {
     // adjoint: r = M^H v with the columns of M from the unit vectors

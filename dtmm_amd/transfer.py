@@ -13,6 +13,8 @@ return value and errors) for the modes built on the GPU:
 * method "2x2" (and the configured default method '"2x2"',
   dtmm/transfer.py:1009-1183): reflection mode 0 (diffraction 0 or 1) and
   1 (diffraction 1) with npass 1 (dtmm/propagate_2x2.py:193-211, :19-87);
+  reflection modes 0 and 1 with the tent windows of diffraction > 1 and
+  npass 1 (:212-257);
   reflection mode 2 with diffraction 0 or 1 (:91-190); npass > 1 with
   reflection mode 1 (diffraction 1) or 2 (diffraction 0 or 1);
 * ret_bulk with diffraction 0 or 1 (4x4: npass 1), per-wavelength
@@ -495,9 +497,12 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
         if reflection not in (0, 1):
             raise NotImplementedError("2x2 method: reflection mode 2 is supported with "
                                       "diffraction 0 or 1")
-        if diffraction != 1:
-            raise NotImplementedError("2x2 method: only diffraction 0 (reflection 0) or 1 is "
-                                      "supported")
+        if diffraction > 1 and (npass != 1 or ret_bulk):
+            raise NotImplementedError("2x2 method with diffraction > 1: npass 1 without ret_bulk "
+                                      "is supported")
+        if not (diffraction >= 1) or diffraction == np.inf:
+            raise NotImplementedError("2x2 method: diffraction 0 (reflection 0) or finite "
+                                      "diffraction >= 1 is supported")
     return kind, reflection
 
 

@@ -376,6 +376,11 @@ def test_mode_resolution_no_diffraction():
     assert _resolve_modes("2x2", 1, 1, 2, None, False) == ("2x2", 2)
     with pytest.raises(NotImplementedError):
         _resolve_modes("2x2", 1, 3, 2, None, False)
+    # 2x2 with tent windows (diffraction > 1): reflection 0 or 1, npass 1, no bulk
+    assert _resolve_modes("2x2", 1, 3, None, None, False) == ("2x2", 0)
+    assert _resolve_modes('"2x2"', 1, 5, None, None, False) == ('2x2', 1)
+    with pytest.raises(NotImplementedError):
+        _resolve_modes("2x2", 1, 3, None, None, True)
     # DTMM_NORM_REF with npass > 1: diffraction 0 or 1
     assert _resolve_modes("4x4", 3, 1, None, None, False, 2) == ("4x4", 1)
     assert _resolve_modes("4x4", 3, 1, None, None, False, 3) == ("4x4", 1)
