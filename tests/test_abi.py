@@ -84,7 +84,7 @@ def _plan_rc(**kw):
     (dict(method=3), _lib.DTMMX_E_ARG),
     (dict(precision=2), _lib.DTMMX_E_ARG),
     (dict(diffraction=0), _lib.DTMMX_E_UNSUPPORTED),
-    (dict(method=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
+    (dict(method=2, diffraction=3, reflection=1, npass=2), _lib.DTMMX_E_UNSUPPORTED),
     (dict(npass=0), _lib.DTMMX_E_ARG),
     (dict(method=2, npass=3), _lib.DTMMX_E_UNSUPPORTED),
     (dict(npass=3, norm=2, diffraction=3), _lib.DTMMX_E_UNSUPPORTED),
