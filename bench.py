@@ -451,19 +451,28 @@ def cpu_rates(field, wl, ps, opt, nlayers, kw):
     n_rays = int(np.prod(lead)) if lead else 1
     nlam = len(wl)
     f = field.reshape((n_rays, nlam, 4, H, W))
+    # per-beam beta, phi (beams x polarisations in the field's leading axes)
+    # become per-ray values of the flattened ray axis
+    beta = np.ravel(kw.get("beta", 0.))
+    phi = np.ravel(kw.get("phi", 0.))
+    rb, rp = np.repeat(beta, n_rays // beta.size), np.repeat(phi, n_rays // phi.size)
+    rest = {k: v for k, v in kw.items() if k not in ("beta", "phi")}
 
-    def transfer(ff, ww, cb):
-        oracle.transfer_field((np.array(ff), ww, ps), opt, layer_callback=cb, **kw)
+    def transfer(ff, ww, cb, b, p):
+        oracle.transfer_field((np.array(ff), ww, ps), opt, layer_callback=cb, beta=b, phi=p,
+                              **rest)
 
     oracle.set_fft_workers(cores)
-    dt = _layer_times(lambda cb: transfer(f, wl, cb), nlayers)
+    b_all, p_all = (rb, rp) if beta.size > 1 else (float(beta[0]), float(phi[0]))
+    dt = _layer_times(lambda cb: transfer(f, wl, cb, b_all, p_all), nlayers)
     primary = H * W * n_rays * nlam / float(np.median(dt))
     groups = _plane_groups(n_rays, nlam, cores)
     oracle.set_fft_workers(1)
     try:
         with ThreadPoolExecutor(len(groups)) as ex:
             res = list(ex.map(lambda g: (g, _layer_times(
-                lambda cb: transfer(f[g[0], g[1]:g[2]], wl[g[1]:g[2]], cb), nlayers)), groups))
+                lambda cb: transfer(f[g[0], g[1]:g[2]], wl[g[1]:g[2]], cb, float(rb[g[0]]),
+                                    float(rp[g[0]])), nlayers)), groups))
     finally:
         oracle.set_fft_workers(cores)
     allc = sum(H * W * (b - a) / float(np.median(t)) for (r, a, b), t in res)

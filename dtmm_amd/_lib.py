@@ -20,7 +20,7 @@ DTMMX_C64 = 0
 DTMMX_C128 = 1
 DTMMX_METHOD_2X2 = 2
 DTMMX_METHOD_4X4 = 4
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_WAVELENGTHS = 64
 MAX_BEAMS = 64
 
@@ -53,6 +53,7 @@ EXPORTS = (
     "dtmmx_specter2color_workspace_size",
     "dtmmx_specter2color",
     "dtmmx_tmm_transfer4x4",
+    "dtmmx_tmm_transfer",
     "dtmmx_director2data_blocks",
     "dtmmx_director2data",
     "dtmmx_q2data",
@@ -146,7 +147,16 @@ class TmmDesc(ctypes.Structure):
         ("phi", ctypes.c_double),
         ("n_in", ctypes.c_double),
         ("n_out", ctypes.c_double),
+        ("method", ctypes.c_int),
+        ("cfact", ctypes.c_double),
+        ("stack_beta", ctypes.c_double),
+        ("stack_phi", ctypes.c_double),
+        ("reflect_in", ctypes.c_int),
+        ("reflect_out", ctypes.c_int),
     ]
+
+
+TMM_METHODS = {"4x4": 0, "4x4_1": 1, "4x4_2": 2, "4x4_r": 3, "2x2": 4, "2x2_1": 5}
 
 
 class DtmmxError(RuntimeError):
@@ -289,6 +299,10 @@ def load():
                                           ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                           ctypes.c_void_p]
     lib.dtmmx_tmm_transfer4x4.restype = ctypes.c_int
+    lib.dtmmx_tmm_transfer.argtypes = [ctypes.POINTER(TmmDesc), ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_void_p]
+    lib.dtmmx_tmm_transfer.restype = ctypes.c_int
     lib.dtmmx_director2data_blocks.argtypes = [ctypes.c_longlong]
     lib.dtmmx_director2data_blocks.restype = ctypes.c_longlong
     lib.dtmmx_director2data.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,

@@ -3737,16 +3737,20 @@ int dtmmx_q2data(const double* qtensor, const unsigned char* mask, long long lay
   return DTMMX_OK;
 }
 
-int dtmmx_tmm_transfer4x4(const dtmmx_tmm_desc* d, void* fvec_in, void* fvec_out, int has_out,
-                          void* cmat, void* stream) {
+static int tmm_run(const dtmmx_tmm_desc* d, int method, double sbeta, double sphi,
+                   void* fvec_in, void* fvec_out, int has_out, void* cmat, void* stream) {
   if (!d) return fail(DTMMX_E_ARG, "null descriptor");
   if (d->n_elements <= 0 || d->n_layers <= 0) return fail(DTMMX_E_ARG, "empty stack");
   if (!d->kd || !d->eps_re || !d->angles) return fail(DTMMX_E_ARG, "null layer data");
   if (!fvec_in && !cmat) return fail(DTMMX_E_ARG, "nothing to compute");
   if (fvec_in && !fvec_out) return fail(DTMMX_E_ARG, "fvec_out is required with fvec_in");
+  if (method < DTMMX_TMM_4X4 || method > DTMMX_TMM_2X2_1) return fail(DTMMX_E_ARG, "unknown method");
   if (!std::isfinite(d->beta) || !std::isfinite(d->phi) || !std::isfinite(d->n_in) ||
-      !std::isfinite(d->n_out))
+      !std::isfinite(d->n_out) || !std::isfinite(sbeta) || !std::isfinite(sphi))
     return fail(DTMMX_E_ARG, "non-finite beam or media");
+  if (method == DTMMX_TMM_4X4_2 || method == DTMMX_TMM_4X4_R) {
+    if (!std::isfinite(d->cfact)) return fail(DTMMX_E_ARG, "non-finite cfact");
+  }
   if ((d->n_elements + NT - 1) / NT > 0x7fffffffLL) return fail(DTMMX_E_ARG, "too many stacks");
   TmmArgs a{};
   a.n = d->n_elements;
@@ -3755,9 +3759,15 @@ int dtmmx_tmm_transfer4x4(const dtmmx_tmm_desc* d, void* fvec_in, void* fvec_out
   a.eps_re = d->eps_re;
   a.eps_im = d->eps_im;
   a.ang = d->angles;
-  a.beta = d->beta;
-  a.phi = d->phi;
-  // f_iso of the media (tmm.py:1074-1091) and their inverses, on the host
+  a.beta = sbeta;
+  a.phi = sphi;
+  a.cfact = d->cfact;
+  a.mbeta = d->beta;
+  a.mphi = d->phi;
+  a.reflect_in = d->reflect_in ? 1 : 0;
+  a.reflect_out = d->reflect_out ? 1 : 0;
+  // f_iso of the media at the beam (beta, phi) (tmm.py:1074-1091) and their
+  // inverses, on the host
   auto media = [&](double n, cd* f, cd* fi) {
     Modes m;
     iso_modes(d->beta, mk<double>(n * n), m);
@@ -3776,10 +3786,31 @@ int dtmmx_tmm_transfer4x4(const dtmmx_tmm_desc* d, void* fvec_in, void* fvec_out
   a.fvec_out = (cd*)fvec_out;
   a.has_out = has_out ? 1 : 0;
   a.cmat = (cd*)cmat;
-  hipLaunchKernelGGL(tmm_kernel, dim3(unsigned((d->n_elements + NT - 1) / NT)), dim3(NT), 0,
-                     (hipStream_t)stream, a);
+  const dim3 grid(unsigned((d->n_elements + NT - 1) / NT));
+  hipStream_t s = (hipStream_t)stream;
+  switch (method) {
+    case DTMMX_TMM_4X4: hipLaunchKernelGGL((tmm_kernel<0>), grid, dim3(NT), 0, s, a); break;
+    case DTMMX_TMM_4X4_1: hipLaunchKernelGGL((tmm_kernel<1>), grid, dim3(NT), 0, s, a); break;
+    case DTMMX_TMM_4X4_2: hipLaunchKernelGGL((tmm_kernel<2>), grid, dim3(NT), 0, s, a); break;
+    case DTMMX_TMM_4X4_R: hipLaunchKernelGGL((tmm_kernel<3>), grid, dim3(NT), 0, s, a); break;
+    case DTMMX_TMM_2X2: hipLaunchKernelGGL((tmm2_kernel<false>), grid, dim3(NT), 0, s, a); break;
+    default: hipLaunchKernelGGL((tmm2_kernel<true>), grid, dim3(NT), 0, s, a); break;
+  }
   HIP_TRY(hipGetLastError());
   return DTMMX_OK;
+}
+
+int dtmmx_tmm_transfer4x4(const dtmmx_tmm_desc* d, void* fvec_in, void* fvec_out, int has_out,
+                          void* cmat, void* stream) {
+  if (!d) return fail(DTMMX_E_ARG, "null descriptor");
+  return tmm_run(d, DTMMX_TMM_4X4, d->beta, d->phi, fvec_in, fvec_out, has_out, cmat, stream);
+}
+
+int dtmmx_tmm_transfer(const dtmmx_tmm_desc* d, void* fvec_in, void* fvec_out, int has_out,
+                       void* cmat, void* stream) {
+  if (!d) return fail(DTMMX_E_ARG, "null descriptor");
+  return tmm_run(d, d->method, d->stack_beta, d->stack_phi, fvec_in, fvec_out, has_out, cmat,
+                 stream);
 }
 
 int dtmmx_host_windows(const dtmmx_plan_desc* d, int max_windows, int* n_windows, int* wi,

@@ -2148,13 +2148,20 @@ struct TmmArgs {
   const double* eps_re; // [3][L][n]
   const double* eps_im; // [3][L][n] or null
   const double* ang;    // [3][L][n]
-  double beta, phi;
+  double beta, phi;     // beam of the layers
+  double cfact;         // 4x4_2, 4x4_r: backward-mode damping
   cd fin[16], fini[16], fout[16], fouti[16];  // isotropic input/output media
   cd* fvec_in;          // [n][4]: incident field, replaced by incident + reflected
+                        // (4x4 methods; read only by the 2x2 methods)
   cd* fvec_out;         // [n][4]: transmitted field (incoming backward waves read
-                        // first when has_out)
+                        // first when has_out, 4x4 methods)
   int has_out;
-  cd* cmat;             // [n][4][4] characteristic matrix, or null
+  cd* cmat;             // characteristic matrix [n][4][4] (4x4_r: [n][8][8],
+                        // 2x2 methods: [n][2][2]), or null
+  // 2x2 methods: Fresnel transmission at the first / last interface, with the
+  // first / last layer's modes at the media beam (mbeta, mphi)
+  int reflect_in, reflect_out;
+  double mbeta, mphi;
 };
 
 // Columns scaled to unit |Poynting z| (dtmm/tmm.py:699-724); zero columns stay zero.
@@ -2190,58 +2197,101 @@ __device__ __noinline__ void tmm_biaxial_modes(double beta, double c0, double s0
   rotate_rows(m, c0, s0);
 }
 
+// Modes (alphaf, dtmm/tmm.py:435-464) of layer l of stack e at the beam
+// (beta, phi0), rotated to the lab frame, not normalised.
+__device__ inline void tmm_layer_modes(const TmmArgs& A, int l, long long e, double beta,
+                                       double phi0, double c0, double s0, Modes& m) {
+  const size_t cs = size_t(A.L) * size_t(A.n);
+  const size_t o = size_t(l) * size_t(A.n) + size_t(e);
+  const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+  const cd e1 = mk<double>(A.eps_re[cs + o], A.eps_im ? A.eps_im[cs + o] : 0.);
+  const cd e2 = mk<double>(A.eps_re[2 * cs + o], A.eps_im ? A.eps_im[2 * cs + o] : 0.);
+  const int kind = material_kind(e0, e1, e2);
+  if (kind == KIND_BIAXIAL) {
+    tmm_biaxial_modes(beta, c0, s0, phi0, e0, e1, e2, A.ang[o], A.ang[cs + o], A.ang[2 * cs + o], m);
+  } else {
+    if (kind == KIND_ISO)
+      iso_modes(beta, e0, m);
+    else
+      uniaxial_modes(beta, e0, e2, A.ang[cs + o], A.ang[2 * cs + o] - phi0, m);
+    rotate_rows(m, c0, s0);
+  }
+}
+
+// C <- C . F . diag(p) . F^-1, one row at a time in place
+__device__ inline void mul_layer(cd C[4][4], const cd F[4][4], const cd p[4], const cd Fi[4][4]) {
+  for (int i = 0; i < 4; ++i) {
+    cd r[4];
+    for (int jj = 0; jj < 4; ++jj) {
+      cd t = mk<double>(0.);
+      for (int k = 0; k < 4; ++k) t = t + C[i][k] * F[k][jj];
+      r[jj] = t * p[jj];
+    }
+    for (int jj = 0; jj < 4; ++jj) {
+      cd t = mk<double>(0.);
+      for (int k = 0; k < 4; ++k) t = t + r[k] * Fi[k][jj];
+      C[i][jj] = t;
+    }
+  }
+}
+
+// The 4x4 methods of dtmm/tmm.py:1112-1213 (layer_mat), chained as in
+// stack_mat (:1215-1296) and solved as in transmit4x4 (:1491-1563):
+//   M = 0 "4x4":   F P(-kd) F^-1, F normalised;
+//   M = 1 "4x4_1": backward phases zeroed;
+//   M = 2 "4x4_2": backward modes damped, alpha += 2i cfact Re(alpha);
+//   M = 3 "4x4_r": the 8x8 matrix [[F P2 F^-1, 0], [F P F^-1, 0]] (P2 the
+//                  "4x4_2" phases, P the plain ones) of layer_mat; its
+//                  chained product is [[A, 0], [B, 0]] with A the "4x4_2"
+//                  stack and B = F1 P F1^-1 . A_2 ... A_n, and system_mat
+//                  keeps rows 0, 5, 2, 7 of its first four columns after
+//                  transforming only the diagonal blocks: S = (Fin^-1 A
+//                  Fout rows 0, 2; B rows 1, 3).
+template <int M>
 static __global__ __launch_bounds__(NT) void tmm_kernel(TmmArgs A) {
   const long long e = (long long)blockIdx.x * NT + threadIdx.x;
   if (e >= A.n) return;
   double s0, c0;
   sincos(A.phi, &s0, &c0);
-  const size_t cs = size_t(A.L) * size_t(A.n);
-  cd C[4][4];
+  cd C[4][4], B[4][4];
   for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) C[i][j] = mk<double>(i == j ? 1. : 0.);
+    for (int j = 0; j < 4; ++j) C[i][j] = B[i][j] = mk<double>(i == j ? 1. : 0.);
   for (int l = 0; l < A.L; ++l) {
-    // layer_mat: F P(-kd) F^-1 with F normalised (tmm.py:1112-1213); the
-    // stack is M_1 . M_2 ... M_n (tmm.py:1215-1296)
-    const size_t o = size_t(l) * size_t(A.n) + size_t(e);
-    const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
-    const cd e1 = mk<double>(A.eps_re[cs + o], A.eps_im ? A.eps_im[cs + o] : 0.);
-    const cd e2 = mk<double>(A.eps_re[2 * cs + o], A.eps_im ? A.eps_im[2 * cs + o] : 0.);
     Modes m;
-    const int kind = material_kind(e0, e1, e2);
-    if (kind == KIND_BIAXIAL) {
-      tmm_biaxial_modes(A.beta, c0, s0, A.phi, e0, e1, e2, A.ang[o], A.ang[cs + o],
-                        A.ang[2 * cs + o], m);
-    } else {
-      if (kind == KIND_ISO)
-        iso_modes(A.beta, e0, m);
-      else
-        uniaxial_modes(A.beta, e0, e2, A.ang[cs + o], A.ang[2 * cs + o] - A.phi, m);
-      rotate_rows(m, c0, s0);
-    }
+    tmm_layer_modes(A, l, e, A.beta, A.phi, c0, s0, m);
     normalize_cols(m.F);
     cd Fi[4][4];
     inv4(m.F, Fi);
     cd p[4];
-    const double kd = -A.kd[o];
-    for (int q = 0; q < 4; ++q) p[q] = cexpi(m.a[q] * kd);
-    // C <- C . F . diag(p) . F^-1, one row at a time in place
-    for (int i = 0; i < 4; ++i) {
-      cd r[4];
-      for (int jj = 0; jj < 4; ++jj) {
-        cd t = mk<double>(0.);
-        for (int k = 0; k < 4; ++k) t = t + C[i][k] * m.F[k][jj];
-        r[jj] = t * p[jj];
-      }
-      for (int jj = 0; jj < 4; ++jj) {
-        cd t = mk<double>(0.);
-        for (int k = 0; k < 4; ++k) t = t + r[k] * Fi[k][jj];
-        C[i][jj] = t;
+    const double kd = -A.kd[size_t(l) * size_t(A.n) + size_t(e)];
+    for (int q = 0; q < 4; ++q) {
+      cd a = m.a[q];
+      if ((M == 2 || M == 3) && (q & 1)) a = a + mk<double>(0., 2. * A.cfact * a.x);
+      p[q] = cexpi(a * kd);
+      if (M == 1 && (q & 1)) p[q] = mk<double>(0.);
+    }
+    if constexpr (M == 3) {
+      if (l == 0) {
+        cd pp[4];
+        for (int q = 0; q < 4; ++q) pp[q] = cexpi(m.a[q] * kd);
+        mul_layer(B, m.F, pp, Fi);
+      } else {
+        mul_layer(B, m.F, p, Fi);
       }
     }
+    mul_layer(C, m.F, p, Fi);
   }
-  if (A.cmat)
-    for (int i = 0; i < 4; ++i)
-      for (int j = 0; j < 4; ++j) A.cmat[size_t(e) * 16 + i * 4 + j] = C[i][j];
+  if (A.cmat) {
+    if constexpr (M == 3) {
+      cd* o = A.cmat + size_t(e) * 64;
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j)
+          o[i * 8 + j] = (j < 4) ? (i < 4 ? C[i][j] : B[i - 4][j]) : mk<double>(0.);
+    } else {
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) A.cmat[size_t(e) * 16 + i * 4 + j] = C[i][j];
+    }
+  }
   if (!A.fvec_in) return;
   // transmit4x4 (tmm.py:1491-1563) with system_mat (:1298-1336) and
   // reflection_mat (:1338-1357); the media matrices are read from the
@@ -2261,6 +2311,12 @@ static __global__ __launch_bounds__(NT) void tmm_kernel(TmmArgs A) {
       r[j] = t;
     }
     for (int j = 0; j < 4; ++j) S[i][j] = r[j];
+  }
+  if constexpr (M == 3) {
+    for (int j = 0; j < 4; ++j) {
+      S[1][j] = B[1][j];
+      S[3][j] = B[3][j];
+    }
   }
   // reflection_mat: m1 = [-S[:, 0], e1, -S[:, 2], e3], m2 = [-e0, S[:, 1], -e2, S[:, 3]]
   cd m1[4][4];
@@ -2318,6 +2374,133 @@ static __global__ __launch_bounds__(NT) void tmm_kernel(TmmArgs A) {
     }
     A.fvec_in[size_t(e) * 4 + i] = t;
     A.fvec_out[size_t(e) * 4 + i] = u;
+  }
+}
+
+__device__ inline void mat2mul(const cd a[2][2], const cd b[2][2], cd o[2][2]) {
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) o[i][j] = a[i][0] * b[0][j] + a[i][1] * b[1][j];
+}
+
+// E rows / forward-mode columns of F (E_mat mode +1, dtmm/tmm.py:974-997)
+__device__ inline void e_fwd(const cd F[4][4], cd E[2][2]) {
+  E[0][0] = F[0][0];
+  E[0][1] = F[0][2];
+  E[1][0] = F[2][0];
+  E[1][1] = F[2][2];
+}
+
+// Eti_mat (dtmm/tmm.py:999-1022): inv(St) . inv(A), A = E_mat(Fin, +1),
+// St = (Fin^-1 Fout)[::2, ::2]
+__device__ inline void eti_mat(const cd Fin[4][4], const cd Fout[4][4], cd O[2][2]) {
+  cd Fii[4][4];
+  inv4(Fin, Fii);
+  cd St[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      cd t = mk<double>(0.);
+      for (int k = 0; k < 4; ++k) t = t + Fii[2 * i][k] * Fout[k][2 * j];
+      St[i][j] = t;
+    }
+  cd Ea[2][2], Sti[2][2], Eai[2][2];
+  e_fwd(Fin, Ea);
+  inv2(St, Sti);
+  inv2(Ea, Eai);
+  mat2mul(Sti, Eai, O);
+}
+
+// t_mat (dtmm/tmm.py:952-972): E_mat(Fout, +1) . Eti_mat(Fin, Fout)
+__device__ inline void t_mat2(const cd Fin[4][4], const cd Fout[4][4], cd O[2][2]) {
+  cd Et[2][2], Eti[2][2];
+  e_fwd(Fout, Et);
+  eti_mat(Fin, Fout, Eti);
+  mat2mul(Et, Eti, O);
+}
+
+// The 2x2 methods (dtmm/tmm.py:1112-1213, :1215-1296, :1675-1748):
+// M = E P(kd) E^-1 of the forward modes (F not normalised), chained as
+// M_n ... M_1; ETI ("2x2_1") enters every layer after the first through
+// Eti_mat from the previous layer's modes.  transfer: E = the forward part
+// of fvec_in in the input medium (fvec2E), optional Fresnel t_mat into the
+// first layer and out of the last one (their modes at the media beam),
+// then E2fvec in the output medium.
+template <bool ETI>
+static __global__ __launch_bounds__(NT) void tmm2_kernel(TmmArgs A) {
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= A.n) return;
+  double s0, c0;
+  sincos(A.phi, &s0, &c0);
+  cd C[2][2] = {{mk<double>(1.), mk<double>(0.)}, {mk<double>(0.), mk<double>(1.)}};
+  cd Fp[4][4];
+  for (int l = 0; l < A.L; ++l) {
+    Modes m;
+    tmm_layer_modes(A, l, e, A.beta, A.phi, c0, s0, m);
+    cd E[2][2], Ei[2][2];
+    e_fwd(m.F, E);
+    if (ETI && l > 0)
+      eti_mat(Fp, m.F, Ei);
+    else
+      inv2(E, Ei);
+    const double kd = A.kd[size_t(l) * size_t(A.n) + size_t(e)];
+    const cd p0 = cexpi(m.a[0] * kd), p2 = cexpi(m.a[2] * kd);
+    cd Ml[2][2], T[2][2];
+    for (int i = 0; i < 2; ++i) {
+      Ml[i][0] = E[i][0] * p0 * Ei[0][0] + E[i][1] * p2 * Ei[1][0];
+      Ml[i][1] = E[i][0] * p0 * Ei[0][1] + E[i][1] * p2 * Ei[1][1];
+    }
+    mat2mul(Ml, C, T);
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) C[i][j] = T[i][j];
+    if (ETI)
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) Fp[i][j] = m.F[i][j];
+  }
+  if (A.cmat)
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) A.cmat[size_t(e) * 4 + i * 2 + j] = C[i][j];
+  if (!A.fvec_in) return;
+  cd fin[4][4], fini[4][4], fout[4][4];
+  mat16(A.fin, fin);
+  mat16(A.fini, fini);
+  mat16(A.fout, fout);
+  // fvec2E: rows Ex, Ey of Fin diag(1, 0, 1, 0) Fin^-1 applied to fvec_in
+  cd fv[4], am[4];
+  for (int i = 0; i < 4; ++i) fv[i] = A.fvec_in[size_t(e) * 4 + i];
+  matv4(fini, fv, am);
+  cd ev[2];
+  for (int i = 0; i < 2; ++i) ev[i] = fin[2 * i][0] * am[0] + fin[2 * i][2] * am[2];
+  double ms, mc;
+  sincos(A.mphi, &ms, &mc);
+  auto apply = [&](const cd M2[2][2]) {
+    const cd x = M2[0][0] * ev[0] + M2[0][1] * ev[1], y = M2[1][0] * ev[0] + M2[1][1] * ev[1];
+    ev[0] = x;
+    ev[1] = y;
+  };
+  if (A.reflect_in) {
+    Modes m;
+    tmm_layer_modes(A, 0, e, A.mbeta, A.mphi, mc, ms, m);
+    cd T[2][2];
+    t_mat2(fin, m.F, T);
+    apply(T);
+  }
+  apply(C);
+  if (A.reflect_out) {
+    Modes m;
+    tmm_layer_modes(A, A.L - 1, e, A.mbeta, A.mphi, mc, ms, m);
+    cd T[2][2];
+    t_mat2(m.F, fout, T);
+    apply(T);
+  }
+  // E2fvec in the output medium: H = B A^-1 E with the forward modes of Fout
+  cd Ea[2][2], Eai[2][2];
+  e_fwd(fout, Ea);
+  inv2(Ea, Eai);
+  cd* o = A.fvec_out + size_t(e) * 4;
+  for (int i = 0; i < 2; ++i) {
+    const cd b0 = fout[2 * i + 1][0] * Eai[0][0] + fout[2 * i + 1][2] * Eai[1][0];
+    const cd b1 = fout[2 * i + 1][0] * Eai[0][1] + fout[2 * i + 1][2] * Eai[1][1];
+    o[2 * i] = ev[i];
+    o[2 * i + 1] = b0 * ev[0] + b1 * ev[1];
   }
 }
 

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define DTMMX_ABI_VERSION 3
+#define DTMMX_ABI_VERSION 4
 
 enum {
   DTMMX_OK = 0,
@@ -408,10 +408,39 @@ typedef struct {
   const double* angles;  /* device [3][n_layers][n_elements]: psi, theta, phi */
   double beta, phi;
   double n_in, n_out;
+  /* read by dtmmx_tmm_transfer only */
+  int method;                 /* DTMMX_TMM_4X4 ... DTMMX_TMM_2X2_1 */
+  double cfact;               /* 4x4_2, 4x4_r: backward-mode damping (stack_mat's 0.01) */
+  double stack_beta, stack_phi;  /* beam of the layers (the reference's transfer
+                                    builds them at its default 0, 0) */
+  int reflect_in, reflect_out;   /* 2x2 methods: Fresnel at the first / last interface */
 } dtmmx_tmm_desc;
 
 int dtmmx_tmm_transfer4x4(const dtmmx_tmm_desc* desc, void* fvec_in, void* fvec_out, int has_out,
                           void* cmat, void* stream);
+
+/* The non-iterative 1D methods of dtmm/tmm.py (replaces layer_mat
+ * :1112-1213, stack_mat :1215-1296, transfer4x4 :1565-1673, transfer2x2
+ * :1675-1695 and transfer :1697-1748 for every element of the inputs'
+ * broadcast shape).  desc->method selects:
+ *   DTMMX_TMM_4X4, _4X4_1 (backward phases zeroed), _4X4_2 (backward modes
+ *   damped by cfact), _4X4_R (8x8 characteristic matrix): transmit4x4 as
+ *   dtmmx_tmm_transfer4x4, with the layers at (stack_beta, stack_phi) and the
+ *   isotropic media at (beta, phi); cmat [n][4][4] ([n][8][8] for _4X4_R);
+ *   _2X2, _2X2_1 (layers entered through Eti_mat): cmat [n][2][2] is
+ *   M_n ... M_1; with fvec_in, fvec_out [n][4] receives E2fvec(C fvec2E(fvec_in))
+ *   in the output medium, with the Fresnel t_mat of the first / last
+ *   interface when reflect_in / reflect_out (those layers' modes at the
+ *   media beam); fvec_in is read only and has_out is ignored.
+ * fp64 throughout, enqueued on `stream`. */
+#define DTMMX_TMM_4X4 0
+#define DTMMX_TMM_4X4_1 1
+#define DTMMX_TMM_4X4_2 2
+#define DTMMX_TMM_4X4_R 3
+#define DTMMX_TMM_2X2 4
+#define DTMMX_TMM_2X2_1 5
+int dtmmx_tmm_transfer(const dtmmx_tmm_desc* desc, void* fvec_in, void* fvec_out, int has_out,
+                       void* cmat, void* stream);
 
 /* Optional kernel timing (no counterpart in the reference, which has no
  * profiling hooks): between begin and end every stride-th launch of each

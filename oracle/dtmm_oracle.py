@@ -61,7 +61,8 @@ __all__ = [
     "polarizer", "as4x4", "jonesmat4x4", "ray_jonesmat4x4", "field2specter",
     "calculate_pom_field", "pom_specter", "XYZ2RGBD65", "apply_gamma", "apply_srgb_gamma",
     "specter2color", "f_iso", "layer_mat", "stack_mat", "system_mat", "reflection_mat",
-    "transmit4x4", "transfer4x4", "Eti_mat", "corrected_E_diffraction_matrix",
+    "transmit4x4", "transfer4x4", "transfer2x2", "transfer1d", "projection_mat", "t_mat",
+    "fvec2E", "E2fvec", "TMM_METHODS", "Eti_mat", "corrected_E_diffraction_matrix",
     "tukey", "aperture", "illumination_aperture", "aperture2rays", "illumination_rays",
     "cauchy2eps", "sellmeier2eps", "is_optical_data_dispersive", "mean_betaphi",
     "propagate_4x4_full",
@@ -2217,31 +2218,106 @@ def f_iso(beta=0., phi=0., n=1.):
     return alphaf(beta, phi, refind2eps([n] * 3), np.zeros(3))[1]
 
 
-def layer_mat(kd, epsv, epsa, beta=0., phi=0.):
-    """Characteristic matrix F . P(-kd) . F^-1 of one layer with F normalised
-    (dtmm/tmm.py:1112-1213, method "4x4")."""
-    alpha, f = alphaf(beta, phi, epsv, epsa)
-    f = normalize_f(f)
-    fi = inv(f)
-    return dotmdm(f, phase_mat(alpha, -np.asarray(kd, FDTYPE)), fi)
+TMM_METHODS = ("4x4", "4x4_1", "4x4_2", "4x4_r", "2x2", "2x2_1")
 
 
-def stack_mat(kd, epsv, epsa, beta=0., phi=0.):
-    """M_1 . M_2 ... M_n (dtmm/tmm.py:1215-1296, method "4x4")."""
+def layer_mat(kd, epsv, epsa, beta=0., phi=0., cfact=0.1, method="4x4", fmatin=None,
+              retfmat=False):
+    """Characteristic matrix of one layer (dtmm/tmm.py:1112-1213).
+
+    "4x4": F P(-kd) F^-1 with F normalised; "4x4_1": the backward phases
+    zeroed; "4x4_2": the backward modes damped, alpha += 2i cfact Re(alpha);
+    "4x4_r": the 8x8 matrix f1 pmat f1^-1 with f1 = diag(F, F) and pmat
+    holding the forward-undamped, backward-damped phases (rows 0-3) and the
+    plain phases copied to rows 4-7; "2x2": E P(kd) E^-1 of the forward
+    modes (F not normalised); "2x2_1": the same with E^-1 replaced by the
+    transmission from the previous layer's modes `fmatin` (Eti_mat)."""
+    if method in ("2x2", "2x2_1"):
+        alpha, fmat = alphaf(beta, phi, epsv, epsa)
+        f = E_mat(fmat, mode=+1)
+        if fmatin is not None and method == "2x2_1":
+            fi = Eti_mat(fmatin, fmat, mode=+1)
+        else:
+            fi = inv(f)
+        pmat = phase_mat(alpha[..., ::2], kd)
+    elif method in ("4x4", "4x4_1", "4x4_r", "4x4_2"):
+        alpha, fmat = alphaf(beta, phi, epsv, epsa)
+        fmat = normalize_f(fmat)
+        fi = inv(fmat)
+        f = fmat
+        kd = np.asarray(kd, FDTYPE)
+        if method == "4x4_r":
+            alpha1 = alpha.copy()
+            alpha1[..., 1::2] = alpha1[..., 1::2] + alpha1[..., 1::2].real * 2j * cfact
+            alpha = alpha.copy()
+            alpha[..., ::2] = alpha[..., ::2] - alpha[..., ::2].real * 2j * cfact
+            pmat1 = phase_mat(alpha1, -kd)
+            pmat2 = phase_mat(alpha, -kd)
+            pmat = np.zeros(pmat1.shape[:-1] + (8, 8), CDTYPE)
+            for i in range(4):
+                pmat[..., i, i] = pmat1[..., i]
+            pmat[..., 4, 0] = pmat1[..., 0]
+            pmat[..., 6, 2] = pmat1[..., 2]
+            pmat[..., 5, 1] = pmat2[..., 1]
+            pmat[..., 7, 3] = pmat2[..., 3]
+            f1 = np.zeros_like(pmat)
+            f1[..., 0:4, 0:4] = f
+            f1[..., 4:8, 4:8] = f
+            out = dotmm(f1, dotmm(pmat, inv(f1)))
+            return (fmat, out) if retfmat else out
+        if method == "4x4_2":
+            alpha = alpha.copy()
+            alpha[..., 1::2] = alpha[..., 1::2] + alpha[..., 1::2].real * 2j * cfact
+        pmat = phase_mat(alpha, -kd)
+        if method == "4x4_1":
+            pmat[..., 1::2] = 0.
+    else:
+        raise ValueError("Unknown method!")
+    out = dotmdm(f, pmat, fi)
+    return (fmat, out) if retfmat else out
+
+
+def stack_mat(kd, epsv, epsa, beta=0., phi=0., cfact=0.01, method="4x4"):
+    """M_1 . M_2 ... M_n for the 4x4 methods, M_n ... M_2 . M_1 for the 2x2
+    ones; "2x2_1" enters each layer from the previous one's modes
+    (dtmm/tmm.py:1215-1296)."""
     out = None
+    fmat = None
     for i in range(len(kd)):
-        m = layer_mat(kd[i], epsv[i], epsa[i], beta, phi)
-        out = m.copy() if out is None else dotmm(out, m)
+        if method == "2x2_1":
+            fmat, m = layer_mat(kd[i], epsv[i], epsa[i], beta, phi, cfact, method, fmatin=fmat,
+                                retfmat=True)
+        else:
+            m = layer_mat(kd[i], epsv[i], epsa[i], beta, phi, cfact, method)
+        if out is None:
+            out = m.copy()
+        elif method.startswith("2x2"):
+            out = dotmm(m, out)
+        else:
+            out = dotmm(out, m)
     return out
 
 
 def system_mat(cmat, fmatin=None, fmatout=None):
-    """Fin^-1 . C . Fout (dtmm/tmm.py:1298-1336, 4x4 characteristic matrix)."""
+    """Fin^-1 . C . Fout (dtmm/tmm.py:1298-1336).  An 8x8 matrix ("4x4_r")
+    has its two diagonal blocks transformed and is reduced to 4x4 by the
+    reference's selection matrices: rows 0, 5, 2, 7 of its first four
+    columns."""
     if fmatin is None:
         fmatin = f_iso()
     if fmatout is None:
         fmatout = fmatin
-    return dotmm(dotmm(inv(fmatin), cmat), fmatout)
+    fmatini = inv(fmatin)
+    if cmat.shape[-1] == 8:
+        c = np.array(cmat, CDTYPE)
+        c[..., 0:4, 0:4] = dotmm(dotmm(fmatini, c[..., 0:4, 0:4]), fmatout)
+        c[..., 4:8, 4:8] = dotmm(dotmm(fmatini, c[..., 4:8, 4:8]), fmatout)
+        m1 = np.zeros((8, 4))
+        m1[np.arange(4), np.arange(4)] = 1.
+        m0 = np.zeros((4, 8))
+        m0[0, 0] = m0[1, 5] = m0[2, 2] = m0[3, 7] = 1.
+        return np.matmul(m0, np.matmul(c, m1))
+    return dotmm(dotmm(fmatini, cmat), fmatout)
 
 
 def reflection_mat(smat):
@@ -2283,13 +2359,86 @@ def transmit4x4(fvec_in, cmat, fmatin=None, fmatout=None, fvec_out=None):
     return dotmv(fmatout, bvec), dotmv(fmatin, avec)
 
 
-def transfer4x4(fvec_in, kd, epsv, epsa, beta=0., phi=0., nin=1., nout=1., fvec_out=None):
-    """(dtmm/tmm.py:1565-1673, method "4x4"): returns (fvec_out, fvec_in
-    with reflections)."""
+def transfer4x4(fvec_in, kd, epsv, epsa, beta=0., phi=0., nin=1., nout=1., fvec_out=None,
+                method="4x4"):
+    """(dtmm/tmm.py:1565-1673): returns (fvec_out, fvec_in with
+    reflections).  As in the reference, the stack matrix is built by
+    stack_mat(kd, epsv, epsa, method=method) with its default beam (beta 0,
+    phi 0, :1655) and cfact 0.01, while the media use beta and phi."""
+    if method not in ("4x4", "4x4_1", "4x4_r", "4x4_2"):
+        raise ValueError("Unknown method '{}'!".format(method))
     fmatin = f_iso(beta, phi, nin)
     fmatout = f_iso(beta, phi, nout)
-    cmat = stack_mat(kd, epsv, epsa, beta, phi)
+    cmat = stack_mat(kd, epsv, epsa, method=method)
     return transmit4x4(fvec_in, cmat, fmatin, fmatout, fvec_out)
+
+
+def projection_mat(fmat, mode=+1):
+    """F diag(1 on the modes of `mode`) F^-1 (dtmm/tmm.py:742-768)."""
+    diag = np.zeros(fmat.shape[:-1], CDTYPE)
+    if mode == +1:
+        diag[..., 0::2] = 1.
+    else:
+        diag[..., 1::2] = 1.
+    return dotmdm(fmat, diag, inv(fmat))
+
+
+def t_mat(fmatin, fmatout, mode=+1):
+    """E transmission from the modes of fmatin into those of fmatout
+    (dtmm/tmm.py:952-972)."""
+    return dotmm(E_mat(fmatout, mode=mode), Eti_mat(fmatin, fmatout, mode=mode))
+
+
+def fvec2E(fvec, fmat=None, mode=+1):
+    """(dtmm/tmm.py:1359-1388)."""
+    if fmat is None:
+        fmat = f_iso()
+    return dotmv(projection_mat(fmat, mode), np.asarray(fvec, CDTYPE))[..., ::2]
+
+
+def E2fvec(evec, fmat=None, mode=+1):
+    """(dtmm/tmm.py:1390-1412)."""
+    evec = np.asarray(evec, CDTYPE)
+    if fmat is None:
+        fmat = f_iso()
+    out = np.empty(evec.shape[:-1] + (4,), CDTYPE)
+    out[..., ::2] = evec
+    out[..., 1::2] = dotmv(E2H_mat(fmat, mode), evec)
+    return out
+
+
+def transfer2x2(evec, kd, epsv, epsa, beta=0., phi=0., nin=None, nout=None, method="2x2"):
+    """(dtmm/tmm.py:1675-1695): optional Fresnel transmission from the
+    isotropic input medium into the first layer and from the last layer into
+    the output medium (beta, phi); the stack matrix with its default beam."""
+    if method not in ("2x2", "2x2_1"):
+        raise ValueError("Unknown method!")
+    evec = np.asarray(evec, CDTYPE)
+    if nin is not None:
+        fin = f_iso(beta, phi, nin)
+        _, fout = alphaf(beta, phi, epsv[0], epsa[0])
+        evec = dotmv(t_mat(fin, fout), evec)
+    evec = dotmv(stack_mat(kd, epsv, epsa, method=method), evec)
+    if nout is not None:
+        fout = f_iso(beta, phi, nout)
+        _, fin = alphaf(beta, phi, epsv[-1], epsa[-1])
+        evec = dotmv(t_mat(fin, fout), evec)
+    return evec
+
+
+def transfer1d(fvec_in, kd, epsv, epsa, beta=0., phi=0., nin=1., nout=1., method="2x2",
+               reflect_in=False, reflect_out=False, fvec_out=None):
+    """``transfer`` (dtmm/tmm.py:1697-1748).  The 2x2 methods return only the
+    transmitted field (fvec_in untouched); the 4x4 methods return (fvec_out,
+    fvec_in with reflections) as transfer4x4 does."""
+    if method.startswith("2x2"):
+        fin = f_iso(beta, phi, nin)
+        fout = f_iso(beta, phi, nout)
+        evec = fvec2E(fvec_in, fin, +1)
+        evec = transfer2x2(evec, kd, epsv, epsa, beta, phi, nin if reflect_in else None,
+                           nout if reflect_out else None, method)
+        return E2fvec(evec, fout, +1)
+    return transfer4x4(fvec_in, kd, epsv, epsa, beta, phi, nin, nout, fvec_out, method)
 
 
 # ---------------------------------------------------------------------------
