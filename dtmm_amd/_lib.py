@@ -54,6 +54,8 @@ EXPORTS = (
     "dtmmx_specter2color",
     "dtmmx_tmm_transfer4x4",
     "dtmmx_tmm_transfer",
+    "dtmmx_illumination_workspace_size",
+    "dtmmx_illumination",
     "dtmmx_director2data_blocks",
     "dtmmx_director2data",
     "dtmmx_q2data",
@@ -157,6 +159,20 @@ class TmmDesc(ctypes.Structure):
 
 
 TMM_METHODS = {"4x4": 0, "4x4_1": 1, "4x4_2": 2, "4x4_r": 3, "2x2": 4, "2x2_1": 5}
+
+
+class IllumDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_double),
+        ("focus", ctypes.c_double),
+        ("mode", ctypes.c_int),
+        ("diffraction", ctypes.c_int),
+        ("polarized", ctypes.c_int),
+        ("jones_re", ctypes.c_double * 2),
+        ("jones_im", ctypes.c_double * 2),
+        ("intensity", ctypes.c_void_p),
+        ("window", ctypes.c_void_p),
+    ]
 
 
 class DtmmxError(RuntimeError):
@@ -303,6 +319,11 @@ def load():
                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_void_p]
     lib.dtmmx_tmm_transfer.restype = ctypes.c_int
+    lib.dtmmx_illumination_workspace_size.argtypes = [ctypes.c_void_p]
+    lib.dtmmx_illumination_workspace_size.restype = ctypes.c_size_t
+    lib.dtmmx_illumination.argtypes = [ctypes.c_void_p, ctypes.POINTER(IllumDesc), ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    lib.dtmmx_illumination.restype = ctypes.c_int
     lib.dtmmx_director2data_blocks.argtypes = [ctypes.c_longlong]
     lib.dtmmx_director2data_blocks.restype = ctypes.c_longlong
     lib.dtmmx_director2data.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,

@@ -2571,6 +2571,8 @@ struct ColorJob {
 };
 
 int pom_c64(const PomJob& j, hipStream_t s);
+int illum_c64(const dtmmx_plan* P, const dtmmx_illum_desc* d, void* f, char* ws, hipStream_t s);
+int illum_c128(const dtmmx_plan* P, const dtmmx_illum_desc* d, void* f, char* ws, hipStream_t s);
 int pom_c128(const PomJob& j, hipStream_t s);
 int color_c64(const ColorJob& j, hipStream_t s);
 int color_c128(const ColorJob& j, hipStream_t s);
@@ -2671,6 +2673,107 @@ static int color_impl(const ColorJob& j, hipStream_t s) {
   return DTMMX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Illumination field (dtmm/field.py:586-713)
+// ---------------------------------------------------------------------------
+
+struct BeamValues {
+  double v[MAXB];
+};
+
+// Power normalisation factors of the illumination (field.py:699-711):
+// sqrt(intensity of the beam / beams), times sqrt(power before / power
+// after) of the diffraction-limited projection.
+static __global__ __launch_bounds__(NT) void illum_factor_kernel2(const double* p1, const double* p2,
+                                                            BeamValues inten, int nplanes,
+                                                            int nlam, int rays_per_beam,
+                                                            int nbeams, double* out) {
+  const int q = blockIdx.x * NT + threadIdx.x;
+  if (q >= nplanes) return;
+  const int beam = (q / nlam) / rays_per_beam;
+  double f = sqrt(inten.v[beam] / nbeams);
+  if (p2) f *= sqrt(p1[q] / p2[q]);
+  out[q] = f;
+}
+
+template <typename T>
+static int illum_impl(const dtmmx_plan* P, const dtmmx_illum_desc* d, cpx<T>* field, char* ws,
+                      hipStream_t s) {
+  const int H = P->d.height, W = P->d.width;
+  const size_t HW = size_t(H) * W;
+  const int nlam = P->d.n_wavelengths, nrays = P->d.n_rays;
+  const int planes = nrays * nlam;
+  const int nblk = int((HW + NT - 1) / NT);
+  const int nbeams = nrays / P->d.rays_per_beam;
+  double* part1 = (double*)ws;
+  double* part2 = part1 + size_t(planes) * nblk;
+  double* sum1 = part2 + size_t(planes) * nblk;
+  double* sum2 = sum1 + planes;
+  double* fac = sum2 + planes;
+  IllumArgs<T> a{};
+  a.field = field;
+  a.H = H;
+  a.W = W;
+  a.nlam = nlam;
+  a.nrays = nrays;
+  a.rays_per_beam = P->d.rays_per_beam;
+  a.n = d->n;
+  a.mode = d->mode;
+  a.polarized = d->polarized;
+  a.jones[0] = mk<double>(d->jones_re[0], d->jones_im[0]);
+  a.jones[1] = mk<double>(d->jones_re[1], d->jones_im[1]);
+  a.window = d->window;
+  a.part = d->diffraction ? part1 : nullptr;
+  a.bm = P->bm;
+  const dim3 grid((unsigned)nblk, (unsigned)planes);
+  hipLaunchKernelGGL((illum_kernel<T, ILL_BUILD>), grid, dim3(NT), 0, s, a);
+  HIP_TRY(hipGetLastError());
+  BeamValues iv{};
+  for (int b = 0; b < nbeams; ++b) iv.v[b] = d->intensity[b];
+  if (d->diffraction) {
+    TRY(fft2_planes<T>(P, field, 0, s));
+    KArgs<T> k{};
+    k.kop = KOP_FIELD4;
+    k.fmode = d->mode > 0 ? 1 : -1;
+    k.d0 = -d->focus;
+    k.e0 = mk<double>(d->n * d->n);
+    k.scale = 1.0 / double(HW);
+    const Batch b{nrays, nlam, 0, nlam};
+    TRY(kspace<T>(P, field, field, b, k, s));
+    TRY(fft2_planes<T>(P, field, 1, s));
+    IllumArgs<T> a2 = a;
+    a2.part = part2;
+    hipLaunchKernelGGL((illum_kernel<T, ILL_POWER>), grid, dim3(NT), 0, s, a2);
+    HIP_TRY(hipGetLastError());
+    TRY(sum_rows(part1, planes, nblk, 1., sum1, s));
+    TRY(sum_rows(part2, planes, nblk, 1., sum2, s));
+    hipLaunchKernelGGL(illum_factor_kernel2, dim3(unsigned((planes + NT - 1) / NT)), dim3(NT), 0, s,
+                       (const double*)sum1, (const double*)sum2, iv, planes, nlam,
+                       P->d.rays_per_beam, nbeams, fac);
+  } else {
+    hipLaunchKernelGGL(illum_factor_kernel2, dim3(unsigned((planes + NT - 1) / NT)), dim3(NT), 0, s,
+                       (const double*)nullptr, (const double*)nullptr, iv, planes, nlam,
+                       P->d.rays_per_beam, nbeams, fac);
+  }
+  HIP_TRY(hipGetLastError());
+  IllumArgs<T> a3 = a;
+  a3.part = nullptr;
+  a3.scale = fac;
+  hipLaunchKernelGGL((illum_kernel<T, ILL_SCALE>), grid, dim3(NT), 0, s, a3);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
+#if DTMMX_C64_PART
+int illum_c64(const dtmmx_plan* P, const dtmmx_illum_desc* d, void* f, char* ws, hipStream_t s) {
+  return illum_impl<float>(P, d, (cpx<float>*)f, ws, s);
+}
+#endif
+#if DTMMX_C128_PART
+int illum_c128(const dtmmx_plan* P, const dtmmx_illum_desc* d, void* f, char* ws, hipStream_t s) {
+  return illum_impl<double>(P, d, (cpx<double>*)f, ws, s);
+}
+#endif
 #if DTMMX_C64_PART
 int pom_c64(const PomJob& j, hipStream_t s) { return pom_impl<float>(j, s); }
 int color_c64(const ColorJob& j, hipStream_t s) { return color_impl<float>(j, s); }
@@ -3811,6 +3914,46 @@ int dtmmx_tmm_transfer(const dtmmx_tmm_desc* d, void* fvec_in, void* fvec_out, i
   if (!d) return fail(DTMMX_E_ARG, "null descriptor");
   return tmm_run(d, d->method, d->stack_beta, d->stack_phi, fvec_in, fvec_out, has_out, cmat,
                  stream);
+}
+
+size_t dtmmx_illumination_workspace_size(const dtmmx_plan* P) {
+  if (!P) return 0;
+  const size_t HW = size_t(P->d.height) * P->d.width;
+  const size_t nblk = (HW + NT - 1) / NT;
+  const size_t planes = size_t(P->d.n_rays) * P->d.n_wavelengths;
+  return (2 * planes * nblk + 3 * planes) * sizeof(double) + 256;
+}
+
+int dtmmx_illumination(const dtmmx_plan* P, const dtmmx_illum_desc* d, void* field,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+  if (!P || !d || !field || !workspace || !d->intensity) return fail(DTMMX_E_ARG, "null argument");
+  if (workspace_bytes < dtmmx_illumination_workspace_size(P))
+    return fail(DTMMX_E_ARG, "workspace too small");
+  if (!(d->n > 0.) || !std::isfinite(d->n) || !std::isfinite(d->focus))
+    return fail(DTMMX_E_ARG, "bad medium index or focus");
+  if (d->mode != 1 && d->mode != -1) return fail(DTMMX_E_ARG, "mode must be +1 or -1");
+  if (!d->polarized && P->d.rays_per_beam != 2)
+    return fail(DTMMX_E_ARG, "unpolarised light needs two rays (x, y) per beam");
+  if (d->polarized && P->d.rays_per_beam != 1)
+    return fail(DTMMX_E_ARG, "a Jones vector needs one ray per beam");
+  if (d->polarized) {
+    const double nj = d->jones_re[0] * d->jones_re[0] + d->jones_im[0] * d->jones_im[0] +
+                      d->jones_re[1] * d->jones_re[1] + d->jones_im[1] * d->jones_im[1];
+    if (!(nj > 0.) || !std::isfinite(nj)) return fail(DTMMX_E_ARG, "bad Jones vector");
+  }
+  if (d->diffraction && P->d.diffraction != 1)
+    return fail(DTMMX_E_ARG, "diffraction needs a plan with diffraction 1");
+  if (d->diffraction && !size_supported(P->d.height))
+    return fail(DTMMX_E_UNSUPPORTED, "grid size has no transform");
+  const int nbeams = P->d.n_rays / P->d.rays_per_beam;
+  for (int b = 0; b < nbeams; ++b)
+    if (!(d->intensity[b] >= 0.) || !std::isfinite(d->intensity[b]))
+      return fail(DTMMX_E_ARG, "bad intensity");
+  // 256-byte aligned start inside the workspace
+  char* ws = (char*)(((uintptr_t)workspace + 255) & ~uintptr_t(255));
+  hipStream_t s = (hipStream_t)stream;
+  if (P->d.precision == DTMMX_C64) return illum_c64(P, d, field, ws, s);
+  return illum_c128(P, d, field, ws, s);
 }
 
 int dtmmx_host_windows(const dtmmx_plan_desc* d, int max_windows, int* n_windows, int* wi,

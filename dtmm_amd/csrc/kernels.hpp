@@ -2505,6 +2505,111 @@ static __global__ __launch_bounds__(NT) void tmm2_kernel(TmmArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// Illumination field on the device (dtmm/field.py:586-713): plane waves of
+// each beam (wave.py:376-409; times a window, or snapped to the strongest FFT
+// eigen-wave, wave.py:432-476, when there is none) carried by the normalised
+// field vector of the isotropic medium with the ray's Jones vector
+// (field.py:566-582, tmm.py:1750-1843).  One thread per (pixel, plane); per
+// block the partial sums of the Poynting z of the planes (for the power
+// normalisation of field.py:703-711).
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct IllumArgs {
+  cpx<T>* field;        // [ray][lam][4][H][W]
+  int H, W;
+  int nlam, nrays, rays_per_beam;
+  double n;             // refractive index of the medium
+  int mode;             // +1 forward, -1 backward
+  int polarized;        // 1: jones for every ray; 0: x (pol 0) and y (pol 1)
+  cd jones[2];
+  const double* window; // [H][W] or null (eigen-wave snapping)
+  double* part;         // [planes][gridDim.x] partial Poynting sums, or null
+  const double* scale;  // FSCALE: [planes] factors, applied in place
+  Beams bm;
+};
+
+enum { ILL_BUILD = 0, ILL_POWER = 1, ILL_SCALE = 2 };
+
+// The strongest FFT mode of a sampled plane wave exp(i k x), x = 0..n-1: the
+// integer frequency nearest to k n / (2 pi), ties to the lower index (the
+// first maximum of |fft|^2, as numpy's argmax reads it).
+__device__ inline int eigen_index(double k, int n) {
+  const double u = k * n / (2 * M_PI);
+  long long m = (long long)ceil(u - 0.5);
+  m %= n;
+  if (m < 0) m += n;
+  return int(m);
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(NT) void illum_kernel(IllumArgs<T> A) {
+  __shared__ double red[NT];
+  const int tid = threadIdx.x;
+  const size_t HW = size_t(A.H) * A.W;
+  const size_t k = size_t(blockIdx.x) * NT + tid;
+  const int q = blockIdx.y;  // plane = ray * nlam + lam
+  const int ray = q / A.nlam, lam = q - ray * A.nlam;
+  cpx<T>* f = A.field + size_t(q) * 4 * HW;
+  double pz = 0.;
+  if (k < HW) {
+    if constexpr (OP == ILL_BUILD) {
+      const int y = int(k / A.W), x = int(k - size_t(y) * A.W);
+      const int beam = ray / A.rays_per_beam, pol = ray - beam * A.rays_per_beam;
+      const double beta = A.bm.beta[beam], phi = A.bm.phi[beam], k0 = A.bm.k0[lam];
+      double sp, cp;
+      sincos(phi, &sp, &cp);
+      const double kx = k0 * beta * cp, ky = k0 * beta * sp;
+      cd wave;
+      if (A.window) {
+        double s, c;
+        sincos(kx * x + ky * y, &s, &c);
+        wave = mk<double>(c, s) * A.window[k];
+      } else {
+        const int jj = eigen_index(kx, A.W), ii = eigen_index(ky, A.H);
+        const double a = 2 * M_PI * (double((long long)jj * x % A.W) / A.W +
+                                     double((long long)ii * y % A.H) / A.H);
+        double s, c;
+        sincos(a, &s, &c);
+        wave = mk<double>(c, s);
+      }
+      // Jones vector of the ray, normalised, in the frame of the beam
+      cd j0 = A.polarized ? A.jones[0] : mk<double>(pol == 0 ? 1. : 0.);
+      cd j1 = A.polarized ? A.jones[1] : mk<double>(pol == 0 ? 0. : 1.);
+      const double nj = sqrt(norm2(j0) + norm2(j1));
+      j0 = j0 * (1. / nj);
+      j1 = j1 * (1. / nj);
+      const cd jc = j0 * cp + j1 * sp, js = j1 * cp - j0 * sp;  // rotation by -phi
+      Modes m;
+      iso_modes(beta, mk<double>(A.n * A.n), m);
+      rotate_rows(m, cp, sp);
+      normalize_cols(m.F);
+      const int c0 = A.mode > 0 ? 0 : 1, c1 = c0 + 2;
+      for (int i = 0; i < 4; ++i) {
+        const cd v = (m.F[i][c0] * jc + m.F[i][c1] * js) * wave;
+        f[i * HW + k] = cast<T>(v);
+      }
+    } else if constexpr (OP == ILL_SCALE) {
+      const T sc = T(A.scale[q]);
+      for (int i = 0; i < 4; ++i) f[i * HW + k] = f[i * HW + k] * sc;
+    }
+    if (A.part) {
+      const cd ex = cast<double>(f[k]), hy = cast<double>(f[HW + k]);
+      const cd ey = cast<double>(f[2 * HW + k]), hx = cast<double>(f[3 * HW + k]);
+      pz = (ex.x * hy.x + ex.y * hy.y) - (ey.x * hx.x + ey.y * hx.y);
+    }
+  }
+  if (!A.part) return;
+  red[tid] = pz;
+  __syncthreads();
+  for (int m = NT / 2; m > 0; m >>= 1) {
+    if (tid < m) red[tid] += red[tid + m];
+    __syncthreads();
+  }
+  if (tid == 0) A.part[size_t(q) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
 // Pass normalisation kernels (npass > 1, dtmm/transfer.py:815-872): one
 // thread per (k, plane) on the unnormalised 2D spectrum of a whole field.
 // ---------------------------------------------------------------------------

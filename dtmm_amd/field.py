@@ -172,13 +172,23 @@ def field2intensity(field):
 
 def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensity=1.,
                       n=None, focus=0., window=None, backdir=False, jones=None,
-                      diffraction=True, eigenmodes=None, betamax=BETAMAX):
+                      diffraction=True, eigenmodes=None, betamax=BETAMAX, device=None,
+                      precision="c128"):
     """Forward (or backward) propagating input field data
     (dtmm/field.py:586-713).  Returns (field, wavelengths, pixelsize) with
     field of shape (..., [2 pol], n_wavelengths, 4, H, W).  The plane waves
     are snapped to FFT eigen-waves exactly when ``window`` is None, as in the
     reference (field.py:686-687); ``eigenmodes`` is accepted for signature
-    parity and, as there, does not change the field."""
+    parity and, as there, does not change the field.
+
+    ``device="cuda"`` builds the field on the GPU (``dtmmx_illumination``) and
+    returns it as a device tensor of ``precision`` ("c64" or "c128"), which
+    ``transfer_field`` takes without a host copy; ``window`` may then be a
+    device tensor.  Beams are scalars or 1D sequences there."""
+    if device is not None:
+        return _illumination_device(shape, wavelengths, pixelsize, beta, phi, intensity, n,
+                                    focus, window, backdir, jones, diffraction, betamax, device,
+                                    precision)
     n = N_COVER if n is None else n
     wavelengths = np.asarray(wavelengths)
     wavenumbers = 2 * np.pi / wavelengths * pixelsize
@@ -226,6 +236,72 @@ def illumination_data(shape, wavelengths, pixelsize=1., beta=0., phi=0., intensi
         norm = norm * ratio[..., None, None]
     field = np.ascontiguousarray(norm[..., None, :, :] * field)
     return field, wavelengths, pixelsize
+
+
+def _illumination_device(shape, wavelengths, pixelsize, beta, phi, intensity, n, focus, window,
+                         backdir, jones, diffraction, betamax, device, precision):
+    """The GPU build of illumination_data (dtmmx_illumination)."""
+    import ctypes
+    import torch
+    from . import _lib
+    from .transfer import Plan
+    if not torch.cuda.is_available():
+        raise RuntimeError("dtmm_amd needs a ROCm GPU (no CPU fallback)")
+    if str(device) not in ("cuda", "cuda:0") and not str(device).startswith("cuda"):
+        raise ValueError("device must be a CUDA (HIP) device")
+    n = N_COVER if n is None else n
+    wavelengths = np.asarray(wavelengths)
+    ks = k0(wavelengths, pixelsize)
+    if ks.ndim != 1:
+        raise ValueError("Wavelengths should be 1D array")
+    _beta = np.asarray(beta, FDTYPE)
+    _phi = np.asarray(phi, FDTYPE)
+    if _beta.ndim > 1 or _phi.ndim > 1:
+        raise NotImplementedError("the device build takes scalar or 1D beams")
+    b, p = np.broadcast_arrays(np.atleast_1d(_beta), np.atleast_1d(_phi))
+    nbeams = len(b)
+    if np.ndim(intensity) != 0:
+        # the reference broadcasts intensity[..., None, None, None, None]
+        # into the (beams, wavelengths, H, W) norm (field.py:696, :703),
+        # which only a scalar survives
+        raise ValueError("intensity must be a scalar")
+    inten = np.full((nbeams,), float(intensity))
+    if jones is None:
+        inten = inten / 2.
+    rpb = 1 if jones is not None else 2
+    H, W = shape
+    # the transforms of the diffraction-limited projection come from a plan
+    # with diffraction 1 (the 4x4 method's reflection mode 2 without it)
+    plan = Plan(shape, ks, b, p, nbeams * rpb, rpb, "4x4", precision, betamax,
+                diffraction=1 if diffraction else 0, reflection=1 if diffraction else 2)
+    field = torch.empty((nbeams * rpb, len(ks), 4, H, W), dtype=plan.dtype, device="cuda")
+    win = None
+    if window is not None:
+        win = torch.as_tensor(window, dtype=torch.float64, device="cuda").contiguous()
+        if tuple(win.shape) != (H, W):
+            raise ValueError("window must have the shape of the field")
+    lib = _lib.load()
+    ws = torch.empty(int(lib.dtmmx_illumination_workspace_size(plan._h)), dtype=torch.uint8,
+                     device="cuda")
+    d = _lib.IllumDesc()
+    d.n, d.focus = float(n), float(focus)
+    d.mode = -1 if backdir else 1
+    d.diffraction = 1 if diffraction else 0
+    d.polarized = 1 if jones is not None else 0
+    if jones is not None:
+        j = np.asarray(jones, np.complex128)
+        d.jones_re[0], d.jones_re[1] = j.real
+        d.jones_im[0], d.jones_im[1] = j.imag
+    d.intensity = inten.ctypes.data
+    d.window = win.data_ptr() if win is not None else None
+    stream = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.dtmmx_illumination(plan._h, ctypes.byref(d), ctypes.c_void_p(field.data_ptr()),
+                                      ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                                      ctypes.c_void_p(stream)))
+    torch.cuda.synchronize()
+    shp = ((nbeams,) if _beta.ndim == 1 or _phi.ndim == 1 else ()) + \
+        ((2,) if jones is None else ()) + (len(ks), 4, H, W)
+    return field.reshape(shp), wavelengths, pixelsize
 
 
 def _radius(shape, scale=1.):

@@ -506,6 +506,21 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
     return kind, reflection
 
 
+class _DeviceField:
+    """Shape and dtype of a device field (a torch tensor on the GPU) in the
+    places where the host path reads them from a NumPy array."""
+
+    def __init__(self, t):
+        import torch
+        self.t = t
+        self.shape = tuple(t.shape)
+        self.ndim = t.dim()
+        self.dtype = np.complex64 if t.dtype == torch.complex64 else np.complex128
+
+    def host(self):
+        return self.t.cpu().numpy()
+
+
 class _Prepared:
     """A transfer made ready on the device: plan, stack, fields, workspace."""
 
@@ -514,8 +529,18 @@ class _Prepared:
         self.plan, self.stack, self.fin, self.fout, self.ws = plan, stack, fin, fout, ws
         self.field_in, self.field_np, self.npass, self.out = field_in, field_np, npass, out
         self.wavelengths, self.pixelsize = wavelengths, pixelsize
+        self.on_device = isinstance(field_np, _DeviceField)
 
     def finish(self):
+        if self.on_device:
+            # a device field in, a device field out (no host copy)
+            res = self.fout.reshape(self.field_np.shape)
+            if self.npass > 1:
+                self.field_in.copy_(self.fin.reshape(self.field_np.shape))
+            if self.out is not None:
+                self.out.copy_(res)
+                res = self.out
+            return res, self.wavelengths, self.pixelsize
         res = self.fout.cpu().numpy().reshape(self.field_np.shape)
         if (self.npass > 1 and isinstance(self.field_in, np.ndarray)
                 and self.field_in.flags.writeable):
@@ -551,7 +576,11 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
     field_in, wavelengths, pixelsize = field_data
     if isinstance(field_in, tuple):
         raise ValueError("a tuple of per-wavelength fields goes through transfer_field")
-    field_np = np.asarray(field_in)
+    if torch.is_tensor(field_in) and field_in.is_cuda:
+        # a field that already lives on the device (illumination_data(device=...))
+        field_np = _DeviceField(field_in)
+    else:
+        field_np = np.asarray(field_in.cpu() if torch.is_tensor(field_in) else field_in)
     shape = field_np.shape[-2:]
     wavelengths = np.asarray(wavelengths)
     ks = _k0(wavelengths, pixelsize)
@@ -560,7 +589,8 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
     if precision is None:
         precision = "c64" if field_np.dtype == np.complex64 else "c128"
     if beta is None and phi is None:
-        beta, phi = field2betaphi(field_np, ks, multiray)
+        beta, phi = field2betaphi(field_np.host() if isinstance(field_np, _DeviceField) else
+                                  field_np, ks, multiray)
     b, p, per_ray = _validate_betaphi(beta, phi)
     lead = field_np.shape[:-4]
     n_rays = int(np.prod(lead)) if lead else 1
@@ -587,8 +617,12 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
     else:
         stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout,
                       split_layers=split_layers)
-    fin = torch.from_numpy(np.ascontiguousarray(field_np.reshape((n_rays, len(ks), 4) + shape)))
-    fin = fin.to(device="cuda", dtype=plan.dtype)
+    if isinstance(field_np, _DeviceField):
+        fin = field_in.reshape((n_rays, len(ks), 4) + shape).to(dtype=plan.dtype).clone()
+    else:
+        fin = torch.from_numpy(np.ascontiguousarray(field_np.reshape((n_rays, len(ks), 4) +
+                                                                     shape)))
+        fin = fin.to(device="cuda", dtype=plan.dtype)
     fout = torch.empty_like(fin)
     ws = None
     need = plan.workspace_for(stack)
@@ -715,7 +749,8 @@ def _transfer_one(field_data, optical_data, beta=None, phi=None, nin=None, nout=
                        else np.complex128)
         # the input side: with npass > 1 its forward part plus the backward
         # field of every backward pass (dtmm/transfer.py:1050, :1080)
-        res[0] = job.fin.cpu().numpy().reshape(shp) if job.npass > 1 else job.field_np
+        res[0] = job.fin.cpu().numpy().reshape(shp) if job.npass > 1 else \
+            (job.field_np.host() if job.on_device else job.field_np)
         res[1:nl + 1] = bulk.cpu().numpy().reshape((nl,) + shp)
         res[nl + 1] = job.fout.cpu().numpy().reshape(shp)
         if out is not None:

@@ -442,6 +442,39 @@ int dtmmx_tmm_transfer4x4(const dtmmx_tmm_desc* desc, void* fvec_in, void* fvec_
 int dtmmx_tmm_transfer(const dtmmx_tmm_desc* desc, void* fvec_in, void* fvec_out, int has_out,
                        void* cmat, void* stream);
 
+/* Illumination field on the device (replaces dtmm/field.py:586-713
+ * illumination_data, with illumination_waves :459-475, waves2field2
+ * :566-582, planewave wave.py:376-409 and wave2eigenwave wave.py:432-476).
+ * For every ray (beam x polarisation) and wavelength of `plan`: the beam's
+ * plane wave, normalised to 1 at pixel (0, 0), times `window`, or, when
+ * window is NULL, snapped to the strongest FFT eigen-wave of the grid (unit
+ * amplitude); carried by the normalised forward (mode +1) or backward (-1)
+ * field vector of the isotropic medium of index n with the ray's Jones
+ * vector (polarized: `jones` for every ray; otherwise x for the first and y
+ * for the second ray of each beam, rays_per_beam 2).  With diffraction the
+ * field is then multiplied per FFT mode by F P(-focus) F^-1 of the medium on
+ * the modes of its direction (zero for beta >= the plan's betamax) and
+ * rescaled to the power it had before.  Every plane is finally scaled by
+ * sqrt(intensity[beam] / n_beams).  The plan's method and reflection are not
+ * used; diffraction needs a plan with diffraction 1 (its transforms).
+ * field [n_rays][n_wavelengths][4][H][W] in the plan's precision (device),
+ * window [H][W] float64 (device), intensity on the host.  Enqueued on
+ * `stream`; the workspace is dtmmx_illumination_workspace_size bytes. */
+typedef struct {
+  double n;
+  double focus;
+  int mode;
+  int diffraction;
+  int polarized;
+  double jones_re[2], jones_im[2];
+  const double* intensity;  /* host [n_beams] */
+  const double* window;     /* device [H][W] or NULL */
+} dtmmx_illum_desc;
+
+size_t dtmmx_illumination_workspace_size(const dtmmx_plan* plan);
+int dtmmx_illumination(const dtmmx_plan* plan, const dtmmx_illum_desc* desc, void* field,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
 /* Optional kernel timing (no counterpart in the reference, which has no
  * profiling hooks): between begin and end every stride-th launch of each
  * per-layer kernel class is bracketed by HIP events on its stream (a stride
