@@ -62,7 +62,10 @@ __all__ = [
     "calculate_pom_field", "pom_specter", "XYZ2RGBD65", "apply_gamma", "apply_srgb_gamma",
     "specter2color", "f_iso", "layer_mat", "stack_mat", "system_mat", "reflection_mat",
     "transmit4x4", "transfer4x4", "transfer2x2", "transfer1d", "projection_mat", "t_mat",
-    "fvec2E", "E2fvec", "TMM_METHODS", "Eti_mat", "corrected_E_diffraction_matrix",
+    "fvec2E", "E2fvec", "TMM_METHODS", "betax1", "eigenmask1", "eigenbetax1", "eigenindices1",
+    "eigenwave1", "field2modes1", "modes2field1", "bdotmm", "bdotmd", "bdotdm", "layer_mat2d",
+    "stack_mat2d", "f_iso2d", "system_mat2d", "reflection_mat2d", "reflect2d", "transfer2d",
+    "Eti_mat", "corrected_E_diffraction_matrix",
     "tukey", "aperture", "illumination_aperture", "aperture2rays", "illumination_rays",
     "cauchy2eps", "sellmeier2eps", "is_optical_data_dispersive", "mean_betaphi",
     "propagate_4x4_full",
@@ -2439,6 +2442,226 @@ def transfer1d(fvec_in, kd, epsv, epsa, beta=0., phi=0., nin=1., nout=1., method
                            nout if reflect_out else None, method)
         return E2fvec(evec, fout, +1)
     return transfer4x4(fvec_in, kd, epsv, epsa, beta, phi, nin, nout, fvec_out, method)
+
+
+# ---------------------------------------------------------------------------
+# tmm2d.py: the non-iterative 4x4 method for 2D data (x, z), in the space of
+# the 1D FFT modes of the x axis
+# ---------------------------------------------------------------------------
+
+def betax1(n, k0):
+    """(dtmm/wave.py:507-527)."""
+    k0 = np.abs(np.asarray(k0, FDTYPE)[..., None])
+    return (2 * np.pi / k0) * np.asarray(np.fft.fftfreq(n), FDTYPE)
+
+
+def eigenmask1(n, k0, betamax=BETAMAX):
+    """Modes with |beta_x| <= betamax (dtmm/wave.py:530-552)."""
+    return np.abs(betax1(n, k0)) <= betamax
+
+
+def _per_k(mask, k0, fn):
+    k0 = np.asarray(k0)
+    if k0.ndim == 0:
+        return fn(mask, k0)
+    return tuple(fn(mask[i], k0[i]) for i in range(len(k0)))
+
+
+def eigenbetax1(n, k0, betamax=BETAMAX):
+    """(dtmm/wave.py:609-632)."""
+    b = betax1(n, k0)
+    m = np.abs(b) <= betamax
+    if m.ndim == 2:
+        return tuple(b[i][m[i]] for i in range(m.shape[0]))
+    return b[m]
+
+
+def eigenindices1(n, k0, betamax=BETAMAX):
+    """(dtmm/wave.py:647-671)."""
+    ii = np.arange(n)
+    m = eigenmask1(n, k0, betamax)
+    if m.ndim == 2:
+        return tuple(ii[m[i]] for i in range(m.shape[0]))
+    return ii[m]
+
+
+def eigenwave1(n, i, amplitude=None):
+    """ifft of a unit spike (amplitude n by default) at mode i
+    (dtmm/wave.py:482-504)."""
+    f = np.zeros((n,), CDTYPE)
+    f[..., i] = n if amplitude is None else amplitude
+    return np.fft.ifft(f)
+
+
+def field2modes1(field, k0, betamax=BETAMAX):
+    """(mask, modes (..., n, 4)) of a (..., [lambda], 4, W) field
+    (dtmm/field.py:216-250); one modes array per wavelength for several."""
+    if isinstance(field, tuple):
+        out = tuple(field2modes1(field[i], k0[i], betamax) for i in range(len(field)))
+        return tuple(o[0] for o in out), tuple(o[1] for o in out)
+    f = np.fft.fft(np.asarray(field, CDTYPE))
+    k0 = np.asarray(k0)
+    mask = eigenmask1(f.shape[-1], k0, betamax)
+    if k0.ndim == 0:
+        return mask, np.moveaxis(f[..., mask], -2, -1)
+    return mask, tuple(np.moveaxis(f[..., i, :, :][..., mask[i]], -2, -1)
+                       for i in range(len(k0)))
+
+
+def modes2field1(mask, modes):
+    """Inverse of field2modes1 (dtmm/field.py:307-340)."""
+    if isinstance(mask, tuple):
+        return tuple(modes2field1(mask[i], modes[i]) for i in range(len(mask)))
+    if mask.ndim == 1:
+        out = np.zeros(modes.shape[:-2] + (4,) + mask.shape[-1:], CDTYPE)
+        out[..., mask] = np.moveaxis(modes, -1, -2)
+        return np.fft.ifft(out)
+    out = np.zeros(modes[0].shape[:-2] + (len(mask), 4) + mask.shape[-1:], CDTYPE)
+    for i, (mode, m) in enumerate(zip(modes, mask)):
+        o = out[..., i, :, :]
+        o[..., m] = np.moveaxis(mode, -1, -2)
+    return np.fft.ifft(out)
+
+
+def bdotmm(m1, m2):
+    """Product of two n x n block matrices of 4x4 blocks (dtmm/linalg.py:1101-1116)."""
+    return np.einsum("ikab,kjbc->ijac", m1, m2)
+
+
+def bdotmd(m1, m2):
+    """Full times block diagonal (n, 4, 4) (dtmm/linalg.py:1134-1145)."""
+    return np.einsum("ijab,jbc->ijac", m1, m2)
+
+
+def bdotdm(m1, m2):
+    """Block diagonal (n, 4, 4) times full (dtmm/linalg.py:1156-1168)."""
+    return np.einsum("iab,ijbc->ijac", m1, m2)
+
+
+def _layer_mat2d(k0, d, epsv, epsa, mask, betaxs, betay, indices, method):
+    """Column j: the FFT (restricted to the mask) of F P(-k0 d) F^-1 at mode
+    j's beam times mode j's eigen-wave (dtmm/tmm2d.py:61-92)."""
+    n = len(betaxs)
+    kd = k0 * d
+    W = epsv.shape[-2]
+    out = np.empty((n, n, 4, 4), CDTYPE)
+    beta = np.sqrt(betaxs ** 2 + betay ** 2)
+    phi = np.arctan2(betay, betaxs)
+    for j, (b, p) in enumerate(zip(beta, phi)):
+        alpha, f, fi = alphaffi(b, p, epsv, epsa)
+        pmat = phase_mat(alpha, -kd)
+        if method == "4x4_1":
+            pmat[..., 1::2] = 0.
+        elif method != "4x4":
+            raise ValueError("Unsupported method!")
+        wave = eigenwave1(W, indices[j], amplitude=1.)
+        m = dotmdm(f, pmat, fi)
+        mf = np.fft.fft(m * wave[..., None, None], axis=-3)
+        out[:, j, :, :] = mf[mask, ...]
+    return out
+
+
+def layer_mat2d(k0, d, epsv, epsa, betay=0., method="4x4", mask=None):
+    """(dtmm/tmm2d.py:20-59): one (n, n, 4, 4) block matrix per wavelength."""
+    if method not in ("4x4", "4x4_1"):
+        raise ValueError("Unsupported method: '{}'".format(method))
+    k0 = np.asarray(k0)
+    W = epsv.shape[-2]
+    if mask is None:
+        mask = eigenmask1(W, k0)
+    betas = _per_k(mask, k0, lambda m, k: betax1(W, k)[m])
+    indices = _per_k(mask, k0, lambda m, k: np.arange(W)[m])
+    if k0.ndim == 0:
+        return _layer_mat2d(k0, d, epsv, epsa, mask, betas, betay, indices, method)
+    return tuple(_layer_mat2d(k0[i], d, epsv, epsa, mask[i], betas[i], betay, indices[i],
+                              method) for i in range(len(k0)))
+
+
+def stack_mat2d(k, d, epsv, epsa, betay=0., method="4x4", mask=None):
+    """M_1 . M_2 ... M_n of the block matrices (dtmm/tmm2d.py:94-119)."""
+    out = None
+    for i in range(len(d)):
+        mat = layer_mat2d(k, d[i], epsv[i], epsa[i], betay=betay, mask=mask, method=method)
+        if out is None:
+            out = tuple(m.copy() for m in mat) if isinstance(mat, tuple) else mat.copy()
+        elif isinstance(mat, tuple):
+            out = tuple(bdotmm(o, m) for o, m in zip(out, mat))
+        else:
+            out = bdotmm(out, mat)
+    return out
+
+
+def f_iso2d(shape, k0, n=1., betay=0., betamax=BETAMAX):
+    """Field matrices of the isotropic medium for every mode (dtmm/tmm2d.py:121-134)."""
+    k0 = np.asarray(k0)
+    betax = eigenbetax1(shape, k0, betamax)
+
+    def one(bx):
+        return f_iso(np.sqrt(bx ** 2 + betay ** 2), np.arctan2(betay, bx), n)
+    if k0.ndim == 0:
+        return one(betax)
+    return tuple(one(b) for b in betax)
+
+
+def system_mat2d(fmatin, cmat, fmatout):
+    """Fin^-1 . C . Fout block-wise (dtmm/tmm2d.py:144-159)."""
+    if isinstance(fmatin, tuple):
+        return tuple(system_mat2d(fi, c, fo) for fi, c, fo in zip(fmatin, cmat, fmatout))
+    return bdotmd(bdotdm(inv(fmatin), cmat), fmatout)
+
+
+def reflection_mat2d(smat):
+    """(dtmm/tmm2d.py:161-182): the 4n x 4n reflection matrix."""
+    if isinstance(smat, tuple):
+        return tuple(reflection_mat2d(s) for s in smat)
+    n = smat.shape[-4]
+    sm = np.moveaxis(smat, -2, -3).reshape(smat.shape[:-4] + (4 * n, 4 * n))
+    return reflection_mat(sm)
+
+
+def reflect2d(fvecin, fmatin, rmat, fmatout, fvecout=None):
+    """Transmitted mode vectors; fvecin receives incident + reflected
+    (dtmm/tmm2d.py:184-232).  Returns (fvec_out, fvec_in)."""
+    if isinstance(fvecin, tuple):
+        res = [reflect2d(fvecin[i], fmatin[i], rmat[i], fmatout[i],
+                         None if fvecout is None else fvecout[i]) for i in range(len(fvecin))]
+        return tuple(r[0] for r in res), tuple(r[1] for r in res)
+    avec = dotmv(inv(fmatin), fvecin)
+    a = np.zeros(avec.shape, CDTYPE)
+    a[..., 0::2] = avec[..., 0::2]
+    if fvecout is not None:
+        bvec = dotmv(inv(fmatout), fvecout)
+        a[..., 1::2] = bvec[..., 1::2]
+    else:
+        bvec = np.zeros_like(avec)
+    av = a.reshape(a.shape[:-2] + (a.shape[-2] * a.shape[-1],))
+    out = dotmv(rmat, av).reshape(a.shape)
+    avec[..., 1::2] = out[..., 1::2]
+    bvec[..., ::2] = out[..., ::2]
+    return dotmv(fmatout, bvec), dotmv(fmatin, avec)
+
+
+def transfer2d(field_data_in, optical_data, betay=0., nin=1., nout=1., method="4x4",
+               betamax=BETAMAX, field_out=None):
+    """(dtmm/tmm2d.py:234-263): returns (field_out, wavelengths, pixelsize)
+    and writes incident + reflected into the input field array."""
+    f, w, p = field_data_in
+    shape = f.shape[-1]
+    d, epsv, epsa = optical_data
+    ks = k0(w, p)
+    fmode_out = None
+    if field_out is not None:
+        _, fmode_out = field2modes1(field_out, ks, betamax=betamax)
+    mask, fmode_in = field2modes1(f, ks, betamax=betamax)
+    fmatin = f_iso2d(shape, ks, nin, betay, betamax)
+    fmatout = f_iso2d(shape, ks, nout, betay, betamax)
+    cmat = stack_mat2d(ks, d, epsv, epsa, betay=betay, mask=mask, method=method)
+    smat = system_mat2d(fmatin, cmat, fmatout)
+    rmat = reflection_mat2d(smat)
+    fmode_out, fmode_in = reflect2d(fmode_in, fmatin, rmat, fmatout, fmode_out)
+    out = modes2field1(mask, fmode_out)
+    f[...] = modes2field1(mask, fmode_in)
+    return out, w, p
 
 
 # ---------------------------------------------------------------------------
