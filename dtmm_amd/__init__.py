@@ -22,6 +22,7 @@ from .adjoint import AdjointTransfer, polarizer4x4, analyser_intensity_loss  # n
 from .pom import calculate_pom_specter, field2specter, specter2color  # noqa: F401
 from . import pom  # noqa: F401
 from . import tmm  # noqa: F401
+from . import tmm2d  # noqa: F401
 from .io import save_stack, load_stack, save_field, load_field  # noqa: F401
 
 __version__ = "0.1.0"

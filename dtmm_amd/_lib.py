@@ -56,6 +56,13 @@ EXPORTS = (
     "dtmmx_tmm_transfer",
     "dtmmx_illumination_workspace_size",
     "dtmmx_illumination",
+    "dtmmx_tmm2d_workspace_size",
+    "dtmmx_tmm2d_stack",
+    "dtmmx_tmm2d_system",
+    "dtmmx_tmm2d_reflection_workspace_size",
+    "dtmmx_tmm2d_reflection",
+    "dtmmx_tmm2d_reflect",
+    "dtmmx_tmm2d_modes",
     "dtmmx_director2data_blocks",
     "dtmmx_director2data",
     "dtmmx_q2data",
@@ -159,6 +166,22 @@ class TmmDesc(ctypes.Structure):
 
 
 TMM_METHODS = {"4x4": 0, "4x4_1": 1, "4x4_2": 2, "4x4_r": 3, "2x2": 4, "2x2_1": 5}
+
+
+class Tmm2dDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int),
+        ("n_layers", ctypes.c_int),
+        ("n_modes", ctypes.c_int),
+        ("mode_index", ctypes.c_void_p),
+        ("k0", ctypes.c_double),
+        ("betay", ctypes.c_double),
+        ("thickness", ctypes.c_void_p),
+        ("eps_re", ctypes.c_void_p),
+        ("eps_im", ctypes.c_void_p),
+        ("angles", ctypes.c_void_p),
+        ("method", ctypes.c_int),
+    ]
 
 
 class IllumDesc(ctypes.Structure):
@@ -324,6 +347,23 @@ def load():
     lib.dtmmx_illumination.argtypes = [ctypes.c_void_p, ctypes.POINTER(IllumDesc), ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
     lib.dtmmx_illumination.restype = ctypes.c_int
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    lib.dtmmx_tmm2d_workspace_size.argtypes = [ctypes.POINTER(Tmm2dDesc)]
+    lib.dtmmx_tmm2d_workspace_size.restype = sz
+    lib.dtmmx_tmm2d_stack.argtypes = [ctypes.POINTER(Tmm2dDesc), vp, vp, sz, vp]
+    lib.dtmmx_tmm2d_stack.restype = ctypes.c_int
+    lib.dtmmx_tmm2d_system.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp, sz, vp]
+    lib.dtmmx_tmm2d_system.restype = ctypes.c_int
+    lib.dtmmx_tmm2d_reflection_workspace_size.argtypes = [ctypes.c_int]
+    lib.dtmmx_tmm2d_reflection_workspace_size.restype = sz
+    lib.dtmmx_tmm2d_reflection.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
+    lib.dtmmx_tmm2d_reflection.restype = ctypes.c_int
+    lib.dtmmx_tmm2d_reflect.argtypes = [ctypes.c_int, ctypes.c_longlong, vp, vp, vp, vp, vp,
+                                        ctypes.c_int, vp, sz, vp]
+    lib.dtmmx_tmm2d_reflect.restype = ctypes.c_int
+    lib.dtmmx_tmm2d_modes.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_longlong, vp, vp,
+                                      ctypes.c_int, vp, sz, vp]
+    lib.dtmmx_tmm2d_modes.restype = ctypes.c_int
     lib.dtmmx_director2data_blocks.argtypes = [ctypes.c_longlong]
     lib.dtmmx_director2data_blocks.restype = ctypes.c_longlong
     lib.dtmmx_director2data.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,

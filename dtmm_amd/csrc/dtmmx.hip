@@ -3956,6 +3956,183 @@ int dtmmx_illumination(const dtmmx_plan* P, const dtmmx_illum_desc* d, void* fie
   return illum_c128(P, d, field, ws, s);
 }
 
+// ---------------------------------------------------------------------------
+// 2D non-iterative method (dtmm/tmm2d.py), one wavelength per call
+// ---------------------------------------------------------------------------
+
+static size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+static char* align_ws(void* ws) { return (char*)(((uintptr_t)ws + 255) & ~uintptr_t(255)); }
+
+size_t dtmmx_tmm2d_workspace_size(const dtmmx_tmm2d_desc* d) {
+  if (!d || d->n_modes <= 0 || d->width <= 0) return 0;
+  const size_t N = 4 * size_t(d->n_modes);
+  const size_t cb = sizeof(cd);
+  return 256 + al256(size_t(d->n_modes) * d->width * 16 * cb) + al256(size_t(d->width) * cb) +
+         3 * al256(N * N * cb);
+}
+
+int dtmmx_tmm2d_stack(const dtmmx_tmm2d_desc* d, void* cmat, void* ws, size_t ws_bytes,
+                      void* stream) {
+  if (!d || !cmat || !ws) return fail(DTMMX_E_ARG, "null argument");
+  if (d->width <= 0 || d->n_layers <= 0 || d->n_modes <= 0 || !d->mode_index || !d->thickness ||
+      !d->eps_re || !d->angles)
+    return fail(DTMMX_E_ARG, "empty or incomplete 2D stack");
+  if (d->method != DTMMX_TMM_4X4 && d->method != DTMMX_TMM_4X4_1)
+    return fail(DTMMX_E_ARG, "Unsupported method");
+  if (!(d->k0 > 0.) || !std::isfinite(d->betay)) return fail(DTMMX_E_ARG, "bad wavenumber or betay");
+  if (ws_bytes < dtmmx_tmm2d_workspace_size(d)) return fail(DTMMX_E_ARG, "workspace too small");
+  const int W = d->width, n = d->n_modes, N = 4 * n;
+  const size_t cb = sizeof(cd);
+  char* w = align_ws(ws);
+  cd* M = (cd*)w;
+  w += al256(size_t(n) * W * 16 * cb);
+  cd* tw = (cd*)w;
+  w += al256(size_t(W) * cb);
+  cd* Cl = (cd*)w;
+  w += al256(size_t(N) * N * cb);
+  cd* P = (cd*)w;
+  w += al256(size_t(N) * N * cb);
+  cd* Q = (cd*)w;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(t2d_tw_kernel, dim3(unsigned((W + NT - 1) / NT)), dim3(NT), 0, s, tw, W);
+  HIP_TRY(hipGetLastError());
+  const size_t NN = size_t(N) * N;
+  const dim3 gN(unsigned((NN + NT - 1) / NT));
+  const dim3 gm(unsigned((N + 15) / 16), unsigned((N + 15) / 16));
+  const size_t stride = size_t(d->n_layers) * W;
+  for (int l = 0; l < d->n_layers; ++l) {
+    T2dArgs a{};
+    a.W = W;
+    a.n = n;
+    a.idx = d->mode_index;
+    a.k0 = d->k0;
+    a.betay = d->betay;
+    a.kd = d->k0 * d->thickness[l];
+    a.eps_re = d->eps_re + size_t(l) * W;
+    a.eps_im = d->eps_im ? d->eps_im + size_t(l) * W : nullptr;
+    a.ang = d->angles + size_t(l) * W;
+    a.stride = stride;
+    a.method = d->method == DTMMX_TMM_4X4_1 ? 1 : 0;
+    a.M = M;
+    a.tw = tw;
+    // the first layer's matrix goes straight to the product
+    a.C = (l == 0) ? (d->n_layers == 1 ? (cd*)cmat : P) : Cl;
+    hipLaunchKernelGGL(t2d_mode_kernel, dim3(unsigned((W + NT - 1) / NT), unsigned(n)), dim3(NT),
+                       0, s, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(t2d_dft_kernel, gN, dim3(NT), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    if (l > 0) {
+      cd* dst = (l + 1 == d->n_layers) ? (cd*)cmat : Q;
+      hipLaunchKernelGGL(zgemm_kernel, gm, dim3(256), 0, s, (const cd*)P, (const cd*)Cl, dst, N);
+      HIP_TRY(hipGetLastError());
+      std::swap(P, Q);
+    }
+  }
+  return DTMMX_OK;
+}
+
+int dtmmx_tmm2d_system(int n, const void* fmatin, const void* cmat, const void* fmatout,
+                       void* smat, void* ws, size_t ws_bytes, void* stream) {
+  if (n <= 0 || !fmatin || !cmat || !fmatout || !smat || !ws) return fail(DTMMX_E_ARG, "null argument");
+  if (ws_bytes < 256 + al256(size_t(n) * 16 * sizeof(cd))) return fail(DTMMX_E_ARG, "workspace too small");
+  cd* fini = (cd*)align_ws(ws);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(t2d_inv_kernel, dim3(unsigned((n + NT - 1) / NT)), dim3(NT), 0, s,
+                     (const cd*)fmatin, fini, n);
+  HIP_TRY(hipGetLastError());
+  const size_t NN = size_t(4 * n) * (4 * n);
+  hipLaunchKernelGGL(t2d_system_kernel, dim3(unsigned((NN + NT - 1) / NT)), dim3(NT), 0, s,
+                     (const cd*)fini, (const cd*)cmat, (const cd*)fmatout, (cd*)smat, n);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
+size_t dtmmx_tmm2d_reflection_workspace_size(int n) {
+  if (n <= 0) return 0;
+  const size_t N = 4 * size_t(n);
+  return 256 + al256(N * 2 * N * sizeof(cd)) + al256(N * sizeof(cd));
+}
+
+int dtmmx_tmm2d_reflection(int n, const void* smat, void* rmat, void* ws, size_t ws_bytes,
+                           void* stream) {
+  if (n <= 0 || !smat || !rmat || !ws) return fail(DTMMX_E_ARG, "null argument");
+  if (ws_bytes < dtmmx_tmm2d_reflection_workspace_size(n)) return fail(DTMMX_E_ARG, "workspace too small");
+  const int N = 4 * n;
+  char* w = align_ws(ws);
+  cd* G = (cd*)w;
+  w += al256(size_t(N) * 2 * N * sizeof(cd));
+  cd* fac = (cd*)w;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t NG = size_t(N) * 2 * N;
+  hipLaunchKernelGGL(gj_init_kernel, dim3(unsigned((NG + NT - 1) / NT)), dim3(NT), 0, s,
+                     (const cd*)smat, G, N);
+  HIP_TRY(hipGetLastError());
+  for (int k = 0; k < N; ++k) {
+    hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(NT), 0, s, G, N, k, fac);
+    hipLaunchKernelGGL(gj_elim_kernel, dim3(unsigned((NG + NT - 1) / NT)), dim3(NT), 0, s, G, N, k,
+                       (const cd*)fac);
+  }
+  HIP_TRY(hipGetLastError());
+  const size_t NN = size_t(N) * N;
+  hipLaunchKernelGGL(gj_extract_kernel, dim3(unsigned((NN + NT - 1) / NT)), dim3(NT), 0, s,
+                     (const cd*)G, (cd*)rmat, N);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
+int dtmmx_tmm2d_reflect(int n, long long nvec, void* fvec_in, const void* fmatin,
+                        const void* rmat, const void* fmatout, void* fvec_out, int has_out,
+                        void* ws, size_t ws_bytes, void* stream) {
+  if (n <= 0 || nvec <= 0 || !fvec_in || !fmatin || !rmat || !fmatout || !fvec_out || !ws)
+    return fail(DTMMX_E_ARG, "null argument");
+  const size_t N = 4 * size_t(n);
+  const size_t need = 256 + 2 * al256(size_t(n) * 16 * sizeof(cd)) + al256(2 * nvec * N * sizeof(cd));
+  if (ws_bytes < need) return fail(DTMMX_E_ARG, "workspace too small");
+  char* w = align_ws(ws);
+  cd* fini = (cd*)w;
+  w += al256(size_t(n) * 16 * sizeof(cd));
+  cd* fouti = (cd*)w;
+  w += al256(size_t(n) * 16 * sizeof(cd));
+  cd* tmp = (cd*)w;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 gi(unsigned((n + NT - 1) / NT));
+  hipLaunchKernelGGL(t2d_inv_kernel, gi, dim3(NT), 0, s, (const cd*)fmatin, fini, n);
+  hipLaunchKernelGGL(t2d_inv_kernel, gi, dim3(NT), 0, s, (const cd*)fmatout, fouti, n);
+  HIP_TRY(hipGetLastError());
+  const dim3 gv(unsigned((nvec * N + NT - 1) / NT));
+  hipLaunchKernelGGL(t2d_reflect_kernel, gv, dim3(NT), 0, s, (cd*)fvec_in, (const cd*)fmatin,
+                     (const cd*)fini, (const cd*)rmat, (const cd*)fmatout, (const cd*)fouti,
+                     (cd*)fvec_out, has_out ? 1 : 0, n, nvec, tmp);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(t2d_reflect_out_kernel, gv, dim3(NT), 0, s, (cd*)fvec_in, (const cd*)fmatin,
+                     (const cd*)fmatout, (cd*)fvec_out, n, nvec, (const cd*)tmp);
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
+int dtmmx_tmm2d_modes(int W, int n, const int* idx, long long nvec, const void* field,
+                      void* modes, int inverse, void* ws, size_t ws_bytes, void* stream) {
+  if (W <= 0 || n <= 0 || nvec <= 0 || !idx || !field || !modes || !ws)
+    return fail(DTMMX_E_ARG, "null argument");
+  if (ws_bytes < 256 + al256(size_t(W) * sizeof(cd))) return fail(DTMMX_E_ARG, "workspace too small");
+  cd* tw = (cd*)align_ws(ws);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(t2d_tw_kernel, dim3(unsigned((W + NT - 1) / NT)), dim3(NT), 0, s, tw, W);
+  HIP_TRY(hipGetLastError());
+  if (!inverse) {
+    const size_t ne = size_t(nvec) * n * 4;
+    hipLaunchKernelGGL(t2d_field2modes_kernel, dim3(unsigned((ne + NT - 1) / NT)), dim3(NT), 0, s,
+                       (const cd*)field, (cd*)modes, idx, (const cd*)tw, W, n, nvec);
+  } else {
+    const size_t ne = size_t(nvec) * 4 * W;
+    hipLaunchKernelGGL(t2d_modes2field_kernel, dim3(unsigned((ne + NT - 1) / NT)), dim3(NT), 0, s,
+                       (const cd*)modes, (cd*)field, idx, (const cd*)tw, W, n, nvec);
+  }
+  HIP_TRY(hipGetLastError());
+  return DTMMX_OK;
+}
+
 int dtmmx_host_windows(const dtmmx_plan_desc* d, int max_windows, int* n_windows, int* wi,
                        int* wj, double* beta, double* phi) {
   if (!d || !n_windows) return fail(DTMMX_E_ARG, "null argument");

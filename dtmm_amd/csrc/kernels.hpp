@@ -2610,6 +2610,313 @@ __global__ __launch_bounds__(NT) void illum_kernel(IllumArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------------
+// The 2D non-iterative method (dtmm/tmm2d.py) for one wavelength, in the
+// space of the n masked 1D FFT modes of the x axis.  Block matrices are kept
+// flat: element (4 i + a, 4 j + b) of a 4n x 4n complex128 matrix is
+// component (a, b) of block (i, j).  fp64 throughout.
+// ---------------------------------------------------------------------------
+
+struct T2dArgs {
+  int W, n;
+  const int* idx;        // [n] FFT bin of every mode
+  double k0, betay;
+  double kd;             // k0 * thickness of this layer
+  const double* eps_re;  // this layer: [3][stride] with pixel x at offset x
+  const double* eps_im;
+  const double* ang;
+  size_t stride;
+  int method;            // 0: 4x4, 1: 4x4_1 (backward phases zeroed)
+  cd* M;                 // [n][W][16]: F P(-kd) F^-1 of pixel x at mode j's beam
+  const cd* tw;          // [W]: exp(2 pi i t / W)
+  cd* C;                 // [4n][4n]
+};
+
+// M[j][x] = F P(-k0 d) F^-1 of pixel x for the beam of mode j
+// (beta_x = 2 pi / k0 fftfreq(idx_j), beta_y; dtmm/tmm2d.py:67-79).
+static __global__ __launch_bounds__(NT) void t2d_mode_kernel(T2dArgs A) {
+  const int x = blockIdx.x * NT + threadIdx.x;
+  const int j = blockIdx.y;
+  if (x >= A.W) return;
+  const double bx = (2 * M_PI / A.k0) * fftfreq(A.idx[j], A.W);
+  const double beta = sqrt(bx * bx + A.betay * A.betay), phi = atan2(A.betay, bx);
+  double s0, c0;
+  sincos(phi, &s0, &c0);
+  const cd e0 = mk<double>(A.eps_re[x], A.eps_im ? A.eps_im[x] : 0.);
+  const cd e1 = mk<double>(A.eps_re[A.stride + x], A.eps_im ? A.eps_im[A.stride + x] : 0.);
+  const cd e2 = mk<double>(A.eps_re[2 * A.stride + x], A.eps_im ? A.eps_im[2 * A.stride + x] : 0.);
+  Modes m;
+  int kind;
+  layer_modes(beta, c0, s0, phi, e0, e1, e2, A.ang[x], A.ang[A.stride + x],
+              A.ang[2 * A.stride + x], m, kind);
+  cd Fi[4][4];
+  inv4(m.F, Fi);
+  cd p[4];
+  for (int q = 0; q < 4; ++q) {
+    p[q] = cexpi(m.a[q] * (-A.kd));
+    if (A.method == 1 && (q & 1)) p[q] = mk<double>(0.);
+  }
+  cd* o = A.M + (size_t(j) * A.W + x) * 16;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      cd t = mk<double>(0.);
+      for (int q = 0; q < 4; ++q) t = t + m.F[a][q] * p[q] * Fi[q][b];
+      o[a * 4 + b] = t;
+    }
+}
+
+// C(4i + a, 4j + b) = 1/W sum_x M[j][x][a][b] exp(2 pi i (idx_j - idx_i) x / W):
+// the masked FFT of M_j times mode j's eigen-wave (dtmm/tmm2d.py:81-87).
+static __global__ __launch_bounds__(NT) void t2d_dft_kernel(T2dArgs A) {
+  const int N = 4 * A.n;
+  const size_t e = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (e >= size_t(N) * N) return;
+  const int r = int(e / N), c = int(e - size_t(r) * N);
+  const int i = r >> 2, a = r & 3, j = c >> 2, b = c & 3;
+  int dk = A.idx[j] - A.idx[i];
+  dk %= A.W;
+  if (dk < 0) dk += A.W;
+  const cd* mj = A.M + size_t(j) * A.W * 16 + a * 4 + b;
+  cd t = mk<double>(0.);
+  int ph = 0;
+  for (int x = 0; x < A.W; ++x) {
+    t = t + mj[size_t(x) * 16] * A.tw[ph];
+    ph += dk;
+    if (ph >= A.W) ph -= A.W;
+  }
+  A.C[e] = t * (1. / A.W);
+}
+
+// out = a . b for N x N complex128 matrices (row-major), 16 x 16 tiles.
+static __global__ __launch_bounds__(256) void zgemm_kernel(const cd* a, const cd* b, cd* out,
+                                                         int N) {
+  __shared__ cd ta[16][17], tb[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
+  cd acc = mk<double>(0.);
+  for (int k0 = 0; k0 < N; k0 += 16) {
+    ta[ty][tx] = (row < N && k0 + tx < N) ? a[size_t(row) * N + k0 + tx] : mk<double>(0.);
+    tb[ty][tx] = (k0 + ty < N && col < N) ? b[size_t(k0 + ty) * N + col] : mk<double>(0.);
+    __syncthreads();
+    for (int k = 0; k < 16; ++k) acc = acc + ta[ty][k] * tb[k][tx];
+    __syncthreads();
+  }
+  if (row < N && col < N) out[size_t(row) * N + col] = acc;
+}
+
+// S = diag(Fin_i^-1) . C . diag(Fout_j) block-wise (dtmm/tmm2d.py:144-148),
+// with fini and fout [n][4][4].
+static __global__ __launch_bounds__(NT) void t2d_system_kernel(const cd* fini, const cd* C,
+                                                             const cd* fout, cd* S, int n) {
+  const int N = 4 * n;
+  const size_t e = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (e >= size_t(N) * N) return;
+  const int r = int(e / N), c = int(e - size_t(r) * N);
+  const int i = r >> 2, a = r & 3, j = c >> 2, b = c & 3;
+  cd t = mk<double>(0.);
+  for (int p = 0; p < 4; ++p) {
+    cd u = mk<double>(0.);
+    for (int q = 0; q < 4; ++q) u = u + C[size_t(4 * i + p) * N + 4 * j + q] * fout[(j * 4 + q) * 4 + b];
+    t = t + fini[(i * 4 + a) * 4 + p] * u;
+  }
+  S[e] = t;
+}
+
+// reflection_mat (dtmm/tmm.py:1338-1357) of an N x N system matrix: the
+// augmented [m1 | m2] with m1 = -S on even columns and e_c on odd ones,
+// m2 = -e_c on even columns and S on odd ones; Gauss-Jordan elimination
+// then leaves m1^-1 m2 in the right half.
+static __global__ __launch_bounds__(NT) void gj_init_kernel(const cd* S, cd* G, int N) {
+  const size_t e = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (e >= size_t(N) * 2 * N) return;
+  const int r = int(e / (2 * N)), c = int(e - size_t(r) * 2 * N);
+  cd v;
+  if (c < N) {
+    v = (c & 1) ? mk<double>(r == c ? 1. : 0.) : -S[size_t(r) * N + c];
+  } else {
+    const int cc = c - N;
+    v = (cc & 1) ? S[size_t(r) * N + cc] : mk<double>(r == cc ? -1. : 0.);
+  }
+  G[e] = v;
+}
+
+// Pivot of column k: the row r >= k with the largest |re| + |im| (the
+// measure of LAPACK's izamax), the first one on ties; rows k and p are
+// swapped and row k is scaled by its pivot.  One work-group.
+static __global__ __launch_bounds__(NT) void gj_pivot_kernel(cd* G, int N, int k, cd* fac) {
+  __shared__ double bv[NT];
+  __shared__ int bi[NT];
+  const int tid = threadIdx.x;
+  const int W2 = 2 * N;
+  double best = -1.;
+  int bidx = k;
+  for (int r = k + tid; r < N; r += NT) {
+    const cd v = G[size_t(r) * W2 + k];
+    const double m = fabs(v.x) + fabs(v.y);
+    if (m > best) {
+      best = m;
+      bidx = r;
+    }
+  }
+  bv[tid] = best;
+  bi[tid] = bidx;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double o = bv[tid + s];
+      const int oi = bi[tid + s];
+      if (o > bv[tid] || (o == bv[tid] && oi < bi[tid])) {
+        bv[tid] = o;
+        bi[tid] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const int p = bi[0];
+  if (p != k)
+    for (int c = tid; c < W2; c += NT) {
+      const cd t = G[size_t(k) * W2 + c];
+      G[size_t(k) * W2 + c] = G[size_t(p) * W2 + c];
+      G[size_t(p) * W2 + c] = t;
+    }
+  __syncthreads();
+  const cd piv = G[size_t(k) * W2 + k];
+  const double n2 = norm2(piv);
+  const cd ip = n2 > 0. ? conj(piv) * (1. / n2) : mk<double>(0.);
+  __syncthreads();
+  for (int c = tid; c < W2; c += NT) G[size_t(k) * W2 + c] = G[size_t(k) * W2 + c] * ip;
+  __syncthreads();
+  // the elimination factors of column k (read before any row changes)
+  for (int r = tid; r < N; r += NT) fac[r] = (r == k) ? mk<double>(0.) : G[size_t(r) * W2 + k];
+}
+
+static __global__ __launch_bounds__(NT) void gj_elim_kernel(cd* G, int N, int k, const cd* fac) {
+  const size_t e = size_t(blockIdx.x) * NT + threadIdx.x;
+  const int W2 = 2 * N;
+  if (e >= size_t(N) * W2) return;
+  const int r = int(e / W2), c = int(e - size_t(r) * W2);
+  if (r == k) return;
+  const cd f = fac[r];
+  if (f.x == 0. && f.y == 0.) return;
+  G[e] = G[e] - f * G[size_t(k) * W2 + c];
+}
+
+static __global__ __launch_bounds__(NT) void gj_extract_kernel(const cd* G, cd* R, int N) {
+  const size_t e = size_t(blockIdx.x) * NT + threadIdx.x;
+  if (e >= size_t(N) * N) return;
+  const int r = int(e / N), c = int(e - size_t(r) * N);
+  R[e] = G[size_t(r) * 2 * N + N + c];
+}
+
+// reflect2d (dtmm/tmm2d.py:184-213) for nvec sets of n mode vectors
+// [nvec][n][4]: a = (Fin^-1 fin on even, Fout^-1 fout on odd components),
+// out = R a; fin <- Fin (a_even, out_odd), fout <- Fout (out_even, 0 odd).
+static __global__ __launch_bounds__(NT) void t2d_reflect_kernel(cd* fvin, const cd* fin,
+                                                              const cd* fini, const cd* R,
+                                                              const cd* fout, const cd* fouti,
+                                                              cd* fvout, int has_out, int n,
+                                                              long long nvec, cd* tmp) {
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  const int N = 4 * n;
+  if (e >= nvec * N) return;
+  const long long v = e / N;
+  const int r = int(e - v * N), i = r >> 2, a = r & 3;
+  // a vector of this set, assembled by every thread of the set (read only)
+  auto avec = [&](int rr) {
+    const int ii = rr >> 2, aa = rr & 3;
+    const cd* fv = (aa & 1) ? (has_out ? fvout : nullptr) : fvin;
+    if (!fv) return mk<double>(0.);
+    const cd* M = (aa & 1) ? fouti : fini;
+    cd t = mk<double>(0.);
+    for (int q = 0; q < 4; ++q) t = t + M[(ii * 4 + aa) * 4 + q] * fv[(v * n + ii) * 4 + q];
+    return t;
+  };
+  cd o = mk<double>(0.);
+  for (int c = 0; c < N; ++c) o = o + R[size_t(r) * N + c] * avec(c);
+  // keep: (a_even of the input, out_odd) for the input side, (out_even, 0) out
+  tmp[e] = (a & 1) ? o : avec(r);
+  tmp[nvec * N + e] = (a & 1) ? avec(r) : o;
+  (void)i;
+}
+
+static __global__ __launch_bounds__(NT) void t2d_reflect_out_kernel(cd* fvin, const cd* fin,
+                                                                  const cd* fout, cd* fvout,
+                                                                  int n, long long nvec,
+                                                                  const cd* tmp) {
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  const int N = 4 * n;
+  if (e >= nvec * N) return;
+  const long long v = e / N;
+  const int r = int(e - v * N), i = r >> 2, a = r & 3;
+  const cd* ai = tmp + v * N + 4 * i;
+  const cd* bo = tmp + nvec * N + v * N + 4 * i;
+  cd s = mk<double>(0.), t = mk<double>(0.);
+  for (int q = 0; q < 4; ++q) {
+    s = s + fin[(i * 4 + a) * 4 + q] * ai[q];
+    t = t + fout[(i * 4 + a) * 4 + q] * bo[q];
+  }
+  fvin[e] = s;
+  fvout[e] = t;
+}
+
+// exp(2 pi i t / W), t = 0 .. W - 1
+static __global__ __launch_bounds__(NT) void t2d_tw_kernel(cd* tw, int W) {
+  const int t = blockIdx.x * NT + threadIdx.x;
+  if (t >= W) return;
+  double s, c;
+  sincospi(2.0 * t / W, &s, &c);
+  tw[t] = mk<double>(c, s);
+}
+
+// inverses of n 4x4 field matrices
+static __global__ __launch_bounds__(NT) void t2d_inv_kernel(const cd* f, cd* fi, int n) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  cd m[4][4], o[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) m[a][b] = f[(i * 4 + a) * 4 + b];
+  inv4(m, o);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) fi[(i * 4 + a) * 4 + b] = o[a][b];
+}
+
+// Masked 1D DFT of the field (dtmm/field.py:216-250) and its inverse
+// (:307-340): modes[v][i][c] = sum_x f[v][c][x] exp(-2 pi i idx_i x / W),
+// f[v][c][x] = 1/W sum_i modes[v][i][c] exp(2 pi i idx_i x / W).
+static __global__ __launch_bounds__(NT) void t2d_field2modes_kernel(const cd* f, cd* modes,
+                                                                  const int* idx, const cd* tw,
+                                                                  int W, int n, long long nvec) {
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= nvec * n * 4) return;
+  const long long v = e / (4 * n);
+  const int r = int(e - v * 4 * n), i = r >> 2, c = r & 3;
+  const cd* fl = f + (v * 4 + c) * W;
+  cd t = mk<double>(0.);
+  int ph = 0;
+  const int k = idx[i] % W;
+  for (int x = 0; x < W; ++x) {
+    t = t + fl[x] * conj(tw[ph]);
+    ph += k;
+    if (ph >= W) ph -= W;
+  }
+  modes[e] = t;
+}
+
+static __global__ __launch_bounds__(NT) void t2d_modes2field_kernel(const cd* modes, cd* f,
+                                                                  const int* idx, const cd* tw,
+                                                                  int W, int n, long long nvec) {
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= nvec * 4 * W) return;
+  const long long v = e / (4 * W);
+  const int r = int(e - v * 4 * W), c = r / W, x = r - c * W;
+  cd t = mk<double>(0.);
+  for (int i = 0; i < n; ++i) {
+    const int ph = int((long long)(idx[i] % W) * x % W);
+    t = t + modes[(v * n + i) * 4 + c] * tw[ph];
+  }
+  f[e] = t * (1. / W);
+}
+
+// ---------------------------------------------------------------------------
 // Pass normalisation kernels (npass > 1, dtmm/transfer.py:815-872): one
 // thread per (k, plane) on the unnormalised 2D spectrum of a whole field.
 // ---------------------------------------------------------------------------

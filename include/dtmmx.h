@@ -475,6 +475,54 @@ size_t dtmmx_illumination_workspace_size(const dtmmx_plan* plan);
 int dtmmx_illumination(const dtmmx_plan* plan, const dtmmx_illum_desc* desc, void* field,
                        void* workspace, size_t workspace_bytes, void* stream);
 
+/* The non-iterative 4x4 method for 2D data (replaces dtmm/tmm2d.py:20-263)
+ * for one wavelength, in the space of the n masked 1D FFT modes of the x
+ * axis (mode_index: their FFT bins, ascending, on the device).  Block
+ * matrices are flat 4n x 4n complex128 row-major: element (4 i + a, 4 j + b)
+ * is component (a, b) of block (i, j) of the reference's (n, n, 4, 4).
+ * fp64 throughout; every call is enqueued on `stream`.
+ *
+ * dtmmx_tmm2d_stack: stack_mat2d (:94-119) with layer_mat2d (:20-92): per
+ * layer, block column j is the masked FFT of F P(-k0 d) F^-1 (at mode j's
+ * beam: beta_x of its bin, betay) times mode j's eigen-wave, and the stack is
+ * M_1 . M_2 ... M_n; method DTMMX_TMM_4X4 or DTMMX_TMM_4X4_1.
+ * dtmmx_tmm2d_system: system_mat2d (:144-159), fmatin/fmatout [n][4][4].
+ * dtmmx_tmm2d_reflection: reflection_mat2d (:161-182) of a system matrix.
+ * dtmmx_tmm2d_reflect: reflect2d (:184-232) for nvec sets of mode vectors
+ * [nvec][n][4]; fvec_in receives incident + reflected, fvec_out the
+ * transmitted field (its backward waves are read first when has_out).
+ * dtmmx_tmm2d_modes: field2modes1 (dtmm/field.py:216-250) of a field
+ * [nvec][4][W] into modes [nvec][n][4], or with inverse modes2field1
+ * (:307-340). */
+typedef struct {
+  int width;
+  int n_layers;
+  int n_modes;
+  const int* mode_index;     /* device [n_modes] */
+  double k0;
+  double betay;
+  const double* thickness;   /* host [n_layers] */
+  const double* eps_re;      /* device [3][n_layers][width] */
+  const double* eps_im;      /* device [3][n_layers][width] or NULL */
+  const double* angles;      /* device [3][n_layers][width] */
+  int method;
+} dtmmx_tmm2d_desc;
+
+size_t dtmmx_tmm2d_workspace_size(const dtmmx_tmm2d_desc* desc);
+int dtmmx_tmm2d_stack(const dtmmx_tmm2d_desc* desc, void* cmat, void* workspace,
+                      size_t workspace_bytes, void* stream);
+int dtmmx_tmm2d_system(int n_modes, const void* fmatin, const void* cmat, const void* fmatout,
+                       void* smat, void* workspace, size_t workspace_bytes, void* stream);
+size_t dtmmx_tmm2d_reflection_workspace_size(int n_modes);
+int dtmmx_tmm2d_reflection(int n_modes, const void* smat, void* rmat, void* workspace,
+                           size_t workspace_bytes, void* stream);
+int dtmmx_tmm2d_reflect(int n_modes, long long nvec, void* fvec_in, const void* fmatin,
+                        const void* rmat, const void* fmatout, void* fvec_out, int has_out,
+                        void* workspace, size_t workspace_bytes, void* stream);
+int dtmmx_tmm2d_modes(int width, int n_modes, const int* mode_index, long long nvec,
+                      const void* field, void* modes, int inverse, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
 /* Optional kernel timing (no counterpart in the reference, which has no
  * profiling hooks): between begin and end every stride-th launch of each
  * per-layer kernel class is bracketed by HIP events on its stream (a stride

@@ -1,0 +1,175 @@
+"""GPU parity of the 2D non-iterative method (dtmm/tmm2d.py:20-263) against
+the oracle, fp64: the mode-coupling block matrices, the boundary solve and the
+field transfer for laterally varying stacks (gratings of rotating directors,
+absorbing and biaxial layers), both methods, one and several wavelengths,
+oblique incidence along y and a given backward field.  Tolerance: 1e-9 of
+the largest reference entry (fp64 with a different DFT summation order and
+Gauss-Jordan pivot sequence)."""
+import numpy as np
+import pytest
+
+import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def _close(got, ref, tol=TOL):
+    got, ref = np.asarray(got), np.asarray(ref)
+    assert got.shape == ref.shape
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = np.abs(got - ref).max() / scale
+    assert err < tol, err
+
+
+def _grating(W, L, seed, biaxial=False, absorbing=False):
+    """A stack of L layers whose director rotates along x (a twisted
+    nematic grating), thicknesses in pixel units."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(W) / W
+    eps = np.empty((L, W, 3), complex)
+    ang = np.empty((L, W, 3))
+    for l in range(L):
+        eps[l] = [2.25, 2.25 if not biaxial else 2.4, 2.89]
+        if absorbing:
+            eps[l] += [0.02j, 0.01j, 0.05j]
+        ang[l, :, 0] = 0.
+        ang[l, :, 1] = np.pi / 2 * (0.5 + 0.4 * np.sin(2 * np.pi * x + l * 0.3))
+        ang[l, :, 2] = 2 * np.pi * x + 0.2 * l + rng.uniform(-0.1, 0.1, W)
+    d = rng.uniform(0.3, 1.2, L)
+    return d, eps, ang
+
+
+def _field(W, nlam, seed, ks):
+    rng = np.random.default_rng(seed)
+    shape = (2, nlam, 4, W) if nlam > 1 else (2, 4, W)
+    f = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    ff = np.fft.fft(f)
+    mask = o.eigenmask1(W, ks)
+    if nlam > 1:
+        for i in range(nlam):
+            ff[:, i][..., ~mask[i]] = 0.
+    else:
+        ff[..., ~mask] = 0.
+    return np.fft.ifft(ff)
+
+
+@pytest.fixture(scope="module")
+def m():
+    import dtmm_amd
+    return dtmm_amd.tmm2d
+
+
+@pytest.mark.parametrize("method", ["4x4", "4x4_1"])
+@pytest.mark.parametrize("betay", [0., 0.12])
+def test_stack_mat2d(m, method, betay):
+    W, L = 32, 5
+    d, eps, ang = _grating(W, L, 1, absorbing=True)
+    ks = o.k0(np.array([480., 610.]), 60.)
+    got = m.stack_mat2d(ks, d, eps, ang, betay=betay, method=method)
+    ref = o.stack_mat2d(ks, d, eps, ang, betay=betay, method=method)
+    assert isinstance(got, tuple) and len(got) == 2
+    for g, r in zip(got, ref):
+        _close(g, r)
+
+
+def test_layer_mat2d_single_biaxial(m):
+    W = 24
+    d, eps, ang = _grating(W, 1, 2, biaxial=True)
+    k = float(o.k0(530., 70.))
+    _close(m.layer_mat2d(k, d[0], eps[0], ang[0], betay=0.05),
+           o.layer_mat2d(k, d[0], eps[0], ang[0], betay=0.05))
+
+
+def test_system_and_reflection(m):
+    W, L = 28, 3
+    d, eps, ang = _grating(W, L, 3)
+    k = float(o.k0(550., 80.))
+    cm = o.stack_mat2d(k, d, eps, ang, betay=0.1)
+    fin = o.f_iso2d(W, k, 1.3, 0.1)
+    fout = o.f_iso2d(W, k, 1.6, 0.1)
+    _close(m.f_iso2d(W, k, 1.3, 0.1), fin, 1e-13)
+    sm = o.system_mat2d(fin, cm, fout)
+    _close(m.system_mat2d(fin, cm, fout), sm)
+    _close(m.reflection_mat2d(sm), o.reflection_mat2d(sm))
+
+
+@pytest.mark.parametrize("with_out", [False, True])
+def test_reflect2d(m, with_out):
+    W = 30
+    d, eps, ang = _grating(W, 2, 4)
+    k = float(o.k0(500., 90.))
+    fin = o.f_iso2d(W, k, 1.2)
+    fout = o.f_iso2d(W, k, 1.5)
+    rm = o.reflection_mat2d(o.system_mat2d(fin, o.stack_mat2d(k, d, eps, ang), fout))
+    n = fin.shape[0]
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((3, n, 4)) + 1j * rng.standard_normal((3, n, 4))
+    b = rng.standard_normal((3, n, 4)) + 1j * rng.standard_normal((3, n, 4)) if with_out else None
+    ref_out, ref_in = o.reflect2d(a.copy(), fin, rm, fout, None if b is None else b.copy())
+    ga = a.copy()
+    got_out = m.reflect2d(ga, fin, rm, fout, None if b is None else b.copy())
+    _close(got_out, ref_out)
+    _close(ga, ref_in)
+
+
+def test_modes_round_trip(m):
+    W = 40
+    ks = o.k0(np.array([450., 520., 640.]), 75.)
+    f = _field(W, 3, 6, ks)
+    mask_r, modes_r = o.field2modes1(f, ks)
+    mask, modes = m.field2modes1(f, ks)
+    np.testing.assert_array_equal(mask, mask_r)
+    for g, r in zip(modes, modes_r):
+        _close(g, r, 1e-12)
+    _close(m.modes2field1(mask, modes), o.modes2field1(mask_r, modes_r), 1e-12)
+
+
+@pytest.mark.parametrize("method,betay,nlam,with_out", [
+    ("4x4", 0., 1, False),
+    ("4x4", 0.1, 2, False),
+    ("4x4_1", 0., 2, False),
+    ("4x4", 0.05, 1, True),
+])
+def test_transfer2d(m, method, betay, nlam, with_out):
+    W, L = 36, 6
+    d, eps, ang = _grating(W, L, 7, absorbing=(method == "4x4_1"))
+    wl = np.array([470., 600.])[:nlam] if nlam > 1 else 540.
+    p = 65.
+    ks = o.k0(wl, p)
+    f = _field(W, nlam, 8, ks)
+    fo = _field(W, nlam, 9, ks) if with_out else None
+    fr = f.copy()
+    ref, _, _ = o.transfer2d((fr, wl, p), (d, eps, ang), betay=betay, nin=1.1, nout=1.4,
+                             method=method, field_out=None if fo is None else fo.copy())
+    fg = f.copy()
+    got, w2, p2 = m.transfer2d((fg, wl, p), (d, eps, ang), betay=betay, nin=1.1, nout=1.4,
+                               method=method, field_out=None if fo is None else fo.copy())
+    assert p2 == p
+    _close(got, ref)
+    _close(fg, fr)
+
+
+def test_transfer2d_thick_grating(m):
+    """A 40-layer cholesteric-like grating, 96 pixels wide: the product
+    chain and the Gauss-Jordan solve at a realistic mode count."""
+    W, L = 96, 40
+    d, eps, ang = _grating(W, L, 10)
+    d = np.full(L, 0.25)
+    wl, p = 550., 100.
+    ks = o.k0(wl, p)
+    f = _field(W, 1, 11, ks)
+    fr = f.copy()
+    ref, _, _ = o.transfer2d((fr, wl, p), (d, eps, ang), nin=1.5, nout=1.5)
+    fg = f.copy()
+    got, _, _ = m.transfer2d((fg, wl, p), (d, eps, ang), nin=1.5, nout=1.5)
+    _close(got, ref, 1e-8)
+    _close(fg, fr, 1e-8)
+
+
+def test_rejects_unknown_method(m):
+    W = 16
+    d, eps, ang = _grating(W, 1, 12)
+    with pytest.raises(ValueError):
+        m.stack_mat2d(float(o.k0(500., 50.)), d, eps, ang, method="2x2")
