@@ -11,7 +11,8 @@ reached through the C ABI of ``include/dtmmx.h``; there is no CPU fallback.
 from .data import (refind2eps, uniaxial_order, director2data, director2angles,  # noqa: F401
                    director2order, nematic_droplet_director, nematic_droplet_data,
                    sphere_mask, cholesteric_director, cholesteric_droplet_data,
-                   validate_optical_block, validate_optical_data)
+                   validate_optical_block, validate_optical_data, effective_block,
+                   effective_data, eig_symmetry, eps2epsva, epsva2eps)
 from .field import (illumination_data, field2intensity, blackman, k0, betaphi,  # noqa: F401
                     planewave, aperture, illumination_aperture, aperture2rays,
                     illumination_rays)

@@ -20,7 +20,7 @@ DTMMX_C64 = 0
 DTMMX_C128 = 1
 DTMMX_METHOD_2X2 = 2
 DTMMX_METHOD_4X4 = 4
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAX_WAVELENGTHS = 64
 MAX_BEAMS = 64
 
@@ -109,6 +109,10 @@ class Stack(ctypes.Structure):
         ("n_out", ctypes.c_double),
         ("n_steps", ctypes.POINTER(ctypes.c_int)),
         ("biaxial", ctypes.POINTER(ctypes.c_int)),
+        ("eff_eps_re", _dp),
+        ("eff_eps_im", _dp),
+        ("eff_angles", _dp),
+        ("eff_thickness", _dp),
     ]
 
 

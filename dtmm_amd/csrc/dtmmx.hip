@@ -454,6 +454,36 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   return fail(DTMMX_E_ARG, "bad column pass");
 }
 
+// Boundary operator of anisotropic effective media on the full spectrum
+// (in place): GOP_HEAD first(m1), GOP_TAIL second(m0), GOP_BOUND both.
+template <typename T>
+static int gkspace(const dtmmx_plan* P, cpx<T>* f, Batch b, int gop, const GMed* m0,
+                   const GMed* m1, hipStream_t s) {
+  GArgs<T> a{};
+  a.src = f;
+  a.dst = f;
+  a.H = P->d.height;
+  a.W = P->d.width;
+  a.b = b;
+  a.amax = P->amax;
+  a.bm = P->bm;
+  a.scale = 1.0 / (double(a.H) * a.W);
+  if (m0) a.m0 = *m0;
+  if (m1) a.m1 = *m1;
+  const int HW = a.H * a.W;
+  const int nbeam = b.nrays / P->bm.rays_per_beam;
+  dim3 grid((HW + NT - 1) / NT, nbeam * b.nlam_b);
+  timer_start(3, s);
+  if (gop == GOP_HEAD)
+    DTMMX_LAUNCH((gkspace_kernel<T, GOP_HEAD>), grid, dim3(NT), s, a);
+  else if (gop == GOP_TAIL)
+    DTMMX_LAUNCH((gkspace_kernel<T, GOP_TAIL>), grid, dim3(NT), s, a);
+  else
+    DTMMX_LAUNCH((gkspace_kernel<T, GOP_BOUND>), grid, dim3(NT), s, a);
+  timer_stop(s);
+  return DTMMX_OK;
+}
+
 template <typename T>
 static int kspace(const dtmmx_plan* P, const cpx<T>* src, cpx<T>* dst, Batch b, KArgs<T> a,
                   hipStream_t s) {
@@ -533,6 +563,7 @@ struct Step {
   int l;
   double d;
   cd e;
+  GMed g;  // anisotropic effective medium (stacks with eff_eps_re)
 };
 
 // dir = -1: the backward pass of npass > 1, layers in reverse order with
@@ -544,7 +575,15 @@ static std::vector<Step> expand_steps(const dtmmx_stack* S, int dir) {
     const int l = dir > 0 ? k : L - 1 - k;
     const int ns = S->n_steps ? S->n_steps[l] : 1;
     const cd e = mk<double>(S->eps_eff_re[l], S->eps_eff_im ? S->eps_eff_im[l] : 0.);
-    for (int j = 0; j < ns; ++j) v.push_back(Step{l, (S->thickness[l] * dir) / double(ns), e});
+    GMed g{};
+    if (S->eff_eps_re) {
+      for (int c = 0; c < 3; ++c) {
+        g.e[c] = mk<double>(S->eff_eps_re[3 * l + c], S->eff_eps_im ? S->eff_eps_im[3 * l + c] : 0.);
+        g.ang[c] = S->eff_angles[3 * l + c];
+      }
+      g.d = ((S->eff_thickness ? S->eff_thickness[l] : S->thickness[l]) * dir) / double(ns);
+    }
+    for (int j = 0; j < ns; ++j) v.push_back(Step{l, (S->thickness[l] * dir) / double(ns), e, g});
   }
   return v;
 }
@@ -1086,6 +1125,35 @@ static int propagate(const dtmmx_plan* P, const dtmmx_stack* S, cpx<T>* out, voi
   const int nlb = (bulk && !m4) ? nlam : lam_batch(P, size_t(nrays) * 4 * HW * sizeof(cpx<T>));
   for (int lam0 = 0; lam0 < nlam; lam0 += nlb) {
     Batch b{nrays, nlam, lam0, (lam0 + nlb <= nlam) ? nlb : nlam - lam0};
+    if (m4 && S->eff_eps_re) {
+      // anisotropic effective media: dense boundary operators on the full
+      // spectrum between the row passes (dtmm/propagate_4x4.py:376-388)
+      if (L == 0) break;
+      TRY(row_pass<T>(P, out, b, 4, OP_FWD, s));
+      TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+      TRY(gkspace<T>(P, out, b, GOP_HEAD, nullptr, &st[0].g, s));
+      TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+      for (int l = 0; l < L; ++l) {
+        const RowArgs<T> la = layer_args<T>(S, HW, st[l].l, st[l].d, 1, status, wsb + wl.etab);
+        TRY(row_pass<T>(P, out, b, 4, OP_ROP, s, &la));
+        if (bulk && (l + 1 == L || st[l + 1].l != st[l].l)) {
+          cpx<T>* slot = bulk + size_t(st[l].l) * nplanes * 4 * HW;
+          TRY(copy_planes<T>(slot, out, b, HW, s));
+          TRY(col_pass<T>(P, slot, b, 4, OP_FWD, s));
+          TRY(gkspace<T>(P, slot, b, GOP_TAIL, &st[l].g, nullptr, s));
+          TRY(col_pass<T>(P, slot, b, 4, OP_INV, s));
+          TRY(row_pass<T>(P, slot, b, 4, OP_INV, s));
+        }
+        TRY(col_pass<T>(P, out, b, 4, OP_FWD, s));
+        if (l + 1 < L)
+          TRY(gkspace<T>(P, out, b, GOP_BOUND, &st[l].g, &st[l + 1].g, s));
+        else
+          TRY(gkspace<T>(P, out, b, GOP_TAIL, &st[l].g, nullptr, s));
+        TRY(col_pass<T>(P, out, b, 4, OP_INV, s));
+      }
+      TRY(row_pass<T>(P, out, b, 4, OP_INV, s));
+      continue;
+    }
     if (m4) {
       if (L == 0) break;  // ifft2(fft2(field_in))
       TRY(row_pass<T>(P, out, b, 4, OP_FWD, s));
@@ -3304,6 +3372,16 @@ static int check_transfer(const dtmmx_plan* P, const dtmmx_stack* S, const void*
   if (need_ws && (ws_bytes < need || (need && !ws)))
     return fail(DTMMX_E_ARG, "workspace too small: " + std::to_string(need) + " bytes needed");
   if (fin == fout) return fail(DTMMX_E_ARG, "field_in and field_out must not alias");
+  if (S->eff_eps_re && S->n_layers > 0) {
+    if (!S->eff_angles) return fail(DTMMX_E_ARG, "anisotropic effective media need eff_angles");
+    const bool k4 = P->d.method == DTMMX_METHOD_4X4 && P->d.reflection == 1 &&
+                    P->d.diffraction == 1 && P->wn <= 1 &&
+                    !(P->d.npass > 1 && (P->d.norm & DTMMX_NORM_REF));
+    if (!(P->d.diffraction == 0 || k4))
+      return fail(DTMMX_E_UNSUPPORTED,
+                  "anisotropic effective media are built for the 4x4 method in reflection mode 1 "
+                  "with diffraction 1 (without DTMMX_NORM_REF), and for diffraction 0");
+  }
   return DTMMX_OK;
 }
 
@@ -3352,6 +3430,8 @@ int dtmmx_transfer_record(const dtmmx_plan* P, const dtmmx_stack* S, const void*
   }
   if (!adjoint_supported(P))
     return fail(DTMMX_E_UNSUPPORTED, "the adjoint is built for 4x4, diffraction 1, npass 1");
+  if (S->eff_eps_re)
+    return fail(DTMMX_E_UNSUPPORTED, "the adjoint is built for isotropic effective media");
   if (!tape || tape_bytes < dtmmx_tape_size(P, S)) return fail(DTMMX_E_ARG, "tape too small");
   hipStream_t s = (hipStream_t)stream;
   if (P->d.precision == DTMMX_C64) return record_c64(P, S, fin, fout, tape, status, s);
@@ -3364,6 +3444,8 @@ int dtmmx_transfer_adjoint(const dtmmx_plan* P, const dtmmx_stack* S, const void
   if (!P || !S || !tape || !seed || !grad) return fail(DTMMX_E_ARG, "null argument");
   if (!adjoint_supported(P))
     return fail(DTMMX_E_UNSUPPORTED, "the adjoint is built for 4x4, diffraction 1, npass 1");
+  if (S->eff_eps_re)
+    return fail(DTMMX_E_UNSUPPORTED, "the adjoint is built for isotropic effective media");
   if (!ws || ws_bytes < dtmmx_adjoint_workspace_size(P))
     return fail(DTMMX_E_ARG, "workspace too small");
   if (S->n_layers > 0 && (!S->thickness || !S->eps_eff_re || !S->eps_re || !S->angles))
@@ -3449,6 +3531,8 @@ int dtmmx_propagate_layer(const dtmmx_plan* P, const dtmmx_stack* S, const void*
     if (r != DTMMX_OK) return r;
   }
   if (S->n_layers < 1) return fail(DTMMX_E_ARG, "the layer step needs a layer");
+  if (S->eff_eps_re)
+    return fail(DTMMX_E_UNSUPPORTED, "the layer step is built for isotropic effective media");
   if (P->d.diffraction != 1 || P->d.npass != 1 || P->wn > 1)
     return fail(DTMMX_E_UNSUPPORTED, "the layer step is built for diffraction 1 and npass 1");
   if ((P->d.method == DTMMX_METHOD_4X4 && P->d.reflection != 1) ||

@@ -1625,6 +1625,128 @@ __global__ __launch_bounds__(NT) void kspace_kernel(KArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------------
+// Anisotropic effective media (eff_data 1, 2, per-layer symmetries or a
+// custom effective block; dtmm/transfer.py:926-976, dtmm/data.py:1364-1419)
+// for the 4x4 method in reflection mode 1.  The diffraction step of a layer
+// (dtmm/propagate_4x4.py:376-388, dtmm/matrix.py:109-174) is
+//   first  = C E P(kd/2) F^-1,   second = F P(kd/2) E^-1 C,
+//   C = E_c P_c(-kd/2) E_c^-1 at the mean beam,
+// with F, E, alpha the modes of the layer's effective medium at each k (zero
+// above betamax) and kd = k0 d_eff.  For an anisotropic medium these are
+// dense per k, so every layer boundary applies second(l) then first(l + 1)
+// on the full spectrum, as the boundary between different isotropic media
+// does.  One thread per (k, wavelength, beam) builds the 4x4 operator in
+// fp64 and applies it to every ray of the beam.
+// ---------------------------------------------------------------------------
+
+struct GMed {
+  cd e[3];        // eigenvalues of eps
+  double ang[3];  // Euler angles psi, theta, phi
+  double d;       // thickness of the effective step (negative: backward pass)
+};
+
+enum { GOP_HEAD = 0, GOP_TAIL = 1, GOP_BOUND = 2 };
+
+template <typename T>
+struct GArgs {
+  const cpx<T>* src;
+  cpx<T>* dst;
+  int H, W;
+  Batch b;
+  const int* amax;
+  GMed m0, m1;  // effective media before and after the boundary
+  double scale;
+  Beams bm;
+};
+
+__device__ inline void g_modes(double beta, double phi, const GMed& g, Modes& m) {
+  int kind;
+  layer_modes(beta, cos(phi), sin(phi), phi, g.e[0], g.e[1], g.e[2], g.ang[0], g.ang[1],
+              g.ang[2], m, kind);
+}
+
+// E_c P_c(-kd) E_c^-1 at the mean beam (dtmm/matrix.py:132-139)
+__device__ inline void g_correction(double bb, double pb, const GMed& g, double kd, cd C[4][4]) {
+  Modes m;
+  g_modes(bb, pb, g, m);
+  cd E[4][4], Ei[4][4];
+  emat(m, E);
+  inv4(E, Ei);
+  xpy(E, m.a, -kd, Ei, C);
+}
+
+// C E P(kd) F^-1 (dtmm/matrix.py:141-157 with :109-118), kd = k0 d / 2
+__device__ inline void g_first(double beta, double phi, double bb, double pb, const GMed& g,
+                               double k0, cd M[4][4]) {
+  const double kd = 0.5 * k0 * g.d;
+  Modes m;
+  g_modes(beta, phi, g, m);
+  cd E[4][4], Fi[4][4], D[4][4], C[4][4];
+  emat(m, E);
+  inv4(m.F, Fi);
+  xpy(E, m.a, kd, Fi, D);
+  g_correction(bb, pb, g, kd, C);
+  matmul4(C, D, M);
+}
+
+// F P(kd) E^-1 C (dtmm/matrix.py:159-174 with :120-130), kd = k0 d / 2
+__device__ inline void g_second(double beta, double phi, double bb, double pb, const GMed& g,
+                                double k0, cd M[4][4]) {
+  const double kd = 0.5 * k0 * g.d;
+  Modes m;
+  g_modes(beta, phi, g, m);
+  cd E[4][4], Ei[4][4], D[4][4], C[4][4];
+  emat(m, E);
+  inv4(E, Ei);
+  xpy(m.F, m.a, kd, Ei, D);
+  g_correction(bb, pb, g, kd, C);
+  matmul4(D, C, M);
+}
+
+template <typename T, int GOP>
+__global__ __launch_bounds__(NT) void gkspace_kernel(GArgs<T> A) {
+  const int k = blockIdx.x * NT + threadIdx.x;
+  const int nbeam = A.b.nrays / A.bm.rays_per_beam;
+  const int beam = blockIdx.y / A.b.nlam_b;
+  const int lam = A.b.lam0 + blockIdx.y % A.b.nlam_b;
+  const size_t HW = size_t(A.H) * A.W;
+  if (k >= int(HW) || beam >= nbeam) return;
+  const int ky = k / A.W, kx = k - ky * A.W;
+  const double k0 = A.bm.k0[lam];
+  const double bb = A.bm.beta[beam], pb = A.bm.phi[beam];
+  const bool pass = mode_index(kx, A.W) <= A.amax[lam * A.H + ky];
+  const double fx = fftfreq(kx, A.W), fy = fftfreq(ky, A.H);
+  const double beta = (2 * M_PI / k0) * sqrt(fx * fx + fy * fy);
+  const double phi = atan2(fy, fx);
+  cd M[4][4];
+  if (pass) {
+    if constexpr (GOP == GOP_HEAD) {
+      g_first(beta, phi, bb, pb, A.m1, k0, M);
+    } else if constexpr (GOP == GOP_TAIL) {
+      g_second(beta, phi, bb, pb, A.m0, k0, M);
+    } else {
+      cd X[4][4], Y[4][4];
+      g_second(beta, phi, bb, pb, A.m0, k0, X);
+      g_first(beta, phi, bb, pb, A.m1, k0, Y);
+      matmul4(Y, X, M);
+    }
+  }
+  for (int r = 0; r < A.bm.rays_per_beam; ++r) {
+    const size_t q = size_t(beam * A.bm.rays_per_beam + r) * A.b.nlam + lam;
+    const cpx<T>* s = A.src + q * 4 * HW + k;
+    cpx<T>* o = A.dst + q * 4 * HW + k;
+    cd v[4];
+    for (int c = 0; c < 4; ++c) v[c] = cast<double>(s[c * HW]);
+    for (int i = 0; i < 4; ++i) {
+      cd t = mk<double>(0.);
+      if (pass)
+        for (int j = 0; j < 4; ++j) t = t + M[i][j] * v[j];
+      o[i * HW] = cast<T>(t * A.scale);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Interface of the 2x2 method in reflection mode 1 with stored reflections
 // (npass > 1, dtmm/propagate_2x2.py:45-58 with E_tr_matrix of
 // dtmm/diffract.py:215-225), one thread per (k, plane) on the full spectrum

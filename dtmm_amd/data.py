@@ -300,3 +300,176 @@ def isotropic_effective_eps(block):
     medium of the diffraction step for eff_data=0 (dtmm/transfer.py:34-41)."""
     d, material, angles = block
     return uniaxial_order(0., material).mean(axis=(0, 1, 2))
+
+
+# ---------------------------------------------------------------------------
+# Effective media of the diffraction step (eff_data of transfer_field)
+# ---------------------------------------------------------------------------
+
+_SYMMETRY_NAMES = {"isotropic": 0, "uniaxial": 1, "biaxial": 2}
+
+
+def _symmetry_id(value):
+    if isinstance(value, str):
+        if value not in _SYMMETRY_NAMES:
+            raise ValueError("Invalid symmetry argument.")
+        return _SYMMETRY_NAMES[value]
+    if value in (0, 1, 2, -1):
+        return int(value)
+    raise ValueError("Invalid symmetry argument.")
+
+
+def parse_symmetry(arg):
+    """An int symmetry (0 isotropic, 1 uniaxial, 2 biaxial, -1 unchanged) or
+    a tuple of per-layer symmetries (dtmm/data.py:1341-1362)."""
+    if isinstance(arg, str):
+        return _symmetry_id(arg)
+    try:
+        items = list(arg)
+    except TypeError:
+        return _symmetry_id(arg)
+    return tuple(parse_symmetry(a) for a in items)
+
+
+def eig_symmetry(order, eig):
+    """Eigenvalues made isotropic (order 0), uniaxial (1: the first two
+    averaged, the third kept) or left as they are (2 or -1)
+    (dtmm/data.py:1276-1306)."""
+    o = np.array(order, int)
+    return uniaxial_order(np.where(o > 1, -1, o), eig)
+
+
+def _sym_matrix(t6):
+    t = np.asarray(t6)
+    xx, yy, zz, xy, xz, yz = (t[..., i] for i in range(6))
+    return np.stack([np.stack([xx, xy, xz], -1), np.stack([xy, yy, yz], -1),
+                     np.stack([xz, yz, zz], -1)], -2)
+
+
+def tensor_eig(t6):
+    """Eigenvalues w (..., 3) and eigenframe v (..., 3, 3), eigenvectors as
+    columns, of real symmetric tensors (xx, yy, zz, xy, xz, yz), ordered as
+    the reference orders them (dtmm/linalg.py:364-424): ascending with a
+    right-handed frame, or -- when the off-diagonal part, relative to the
+    largest element, has a squared norm of at most 1e-12 -- the diagonal in
+    its own order with the identity frame; then the eigenvalue farthest from
+    the other two is moved to index 2 by a cyclic shift.  The uniaxial
+    effective medium keeps that eigenvalue as its extraordinary one."""
+    t = np.asarray(t6, FDTYPE)
+    w, v = np.linalg.eigh(_sym_matrix(t))
+    v[..., :, 2] = np.cross(v[..., :, 0], v[..., :, 1])
+    big = np.abs(t).max(axis=-1)
+    inv = np.divide(1., big, out=np.zeros_like(big), where=big > 0)
+    off = ((t[..., 3:] * inv[..., None]) ** 2).sum(-1)
+    flat = off <= 1e-12
+    w = np.where(flat[..., None], t[..., :3], w)
+    v = np.where(flat[..., None, None], np.eye(3), v)
+    # distance of each pair: g01 = |w1 - w0|, g02 = |w2 - w0|, g12 = |w2 - w1|
+    g01 = np.abs(w[..., 1] - w[..., 0])
+    g02 = np.abs(w[..., 2] - w[..., 0])
+    g12 = np.abs(w[..., 2] - w[..., 1])
+    shift = (g01 > g02) | (g01 > g12)
+    # pair (1, 2) closest: w0 stands out and goes last; else pair (0, 2): w1
+    last0 = shift & (g02 > g12)
+    last1 = shift & ~(g02 > g12)
+    order = np.broadcast_to(np.arange(3), w.shape).copy()
+    order[last0] = (1, 2, 0)
+    order[last1] = (2, 0, 1)
+    w = np.take_along_axis(w, order, -1)
+    v = np.take_along_axis(v, order[..., None, :], -1)
+    return w, v
+
+
+def rotation_angles(r):
+    """Euler angles (psi, theta, phi) of a rotation r = Rz(phi) Ry(theta)
+    Rz(psi) with the reference's rule for theta = 0, where only phi is kept,
+    as arccos of r[1, 1] (dtmm/rotation.py:233-260)."""
+    r = np.asarray(r, FDTYPE)
+    theta = np.arccos(np.clip(r[..., 2, 2], -1., 1.))
+    flat = ((r[..., 0, 2] == 0.) & (r[..., 1, 2] == 0.)) | (theta == 0.)
+    phi = np.where(flat, np.arccos(np.clip(r[..., 1, 1], -1., 1.)),
+                   np.arctan2(r[..., 1, 2], r[..., 0, 2]))
+    psi = np.where(flat, 0., np.arctan2(r[..., 2, 1], -r[..., 2, 0]))
+    return np.stack([psi, theta, phi], -1)
+
+
+def rotation_matrix(angles):
+    """Rz(phi) Ry(theta) Rz(psi) of angles (..., 3) = (psi, theta, phi)
+    (dtmm/rotation.py:199-227)."""
+    a = np.asarray(angles, FDTYPE)
+    cps, sps = np.cos(a[..., 0]), np.sin(a[..., 0])
+    cth, sth = np.cos(a[..., 1]), np.sin(a[..., 1])
+    cph, sph = np.cos(a[..., 2]), np.sin(a[..., 2])
+    return np.stack([
+        np.stack([cph * cth * cps - sph * sps, -cph * cth * sps - sph * cps, cph * sth], -1),
+        np.stack([sph * cth * cps + cph * sps, cph * cps - sph * cth * sps, sph * sth], -1),
+        np.stack([-sth * cps, sth * sps, cth], -1)], -2)
+
+
+def epsva2eps(epsv, epsa):
+    """Laboratory-frame tensors (xx, yy, zz, xy, xz, yz) of eigenvalues and
+    Euler angles (dtmm/data.py:515-532)."""
+    r = rotation_matrix(epsa)
+    e = np.asarray(epsv)
+    comp = lambda i, j: (r[..., i, :] * r[..., j, :] * e).sum(-1)
+    return np.stack([comp(0, 0), comp(1, 1), comp(2, 2), comp(0, 1), comp(0, 2), comp(1, 2)], -1)
+
+
+def eps2epsva(eps):
+    """Eigenvalues and Euler angles of tensors (..., 6) (dtmm/data.py:489-513).
+    Only real tensors are taken: for complex ones the reference warns that
+    its eigenframe is unreliable."""
+    t = np.asarray(eps)
+    if np.iscomplexobj(t):
+        if np.any(t.imag != 0):
+            raise NotImplementedError("anisotropic effective media of absorbing (complex) "
+                                      "tensors are not supported")
+        t = t.real
+    w, v = tensor_eig(t)
+    return w, rotation_angles(v)
+
+
+def effective_from_tensors(tensors, symmetry):
+    """Effective layers (epsv (n, 3), epsa (n, 3)) of one block from the mean
+    laboratory-frame tensor of each of its n layers (equal voxel counts), as
+    effective_block forms them (dtmm/data.py:1364-1419): one symmetry averages
+    the whole block, a sequence of per-layer symmetries each layer."""
+    t = np.asarray(tensors)
+    n = len(t)
+    order = np.asarray(parse_symmetry(symmetry), int)
+    if order.ndim == 0 and order == 0:
+        ev = np.broadcast_to(((t[:, 0] + t[:, 1] + t[:, 2]) / 3.).mean(), (3,))
+        return np.tile(ev, (n, 1)), np.zeros((n, 3))
+    if order.ndim == 0:
+        w, a = eps2epsva(t.mean(0))
+        w = eig_symmetry(order, w)
+        return np.tile(w, (n, 1)), np.tile(a, (n, 1))
+    if len(order) != n:
+        raise ValueError("Shape of the symmetry argument is incompatible with block size.")
+    w, a = eps2epsva(t)
+    return eig_symmetry(order, w), a
+
+
+def effective_block(optical_block, symmetry=0):
+    """Effective block (d, epsv, epsa) of an optical block with the material
+    averaged over the block, or over each layer for per-layer symmetries
+    (dtmm/data.py:1364-1419)."""
+    d, epsv, epsa = validate_optical_block(optical_block)
+    L = len(d)
+    shape = (L,) + tuple(np.broadcast_shapes(np.shape(epsv)[1:], np.shape(epsa)[1:]))
+    ev = np.broadcast_to(epsv, shape)
+    t = epsva2eps(ev, np.broadcast_to(epsa, shape)).reshape(L, -1, 6).mean(1)
+    order = np.asarray(parse_symmetry(symmetry), int)
+    if order.ndim == 0 and order == 0:
+        e = uniaxial_order(0., ev).reshape(-1, 3).mean(0)
+        return d, np.tile(e, (L, 1)), np.zeros((L, 3))
+    w, a = effective_from_tensors(t, symmetry)
+    return d, w, a
+
+
+def effective_data(optical_data, symmetry=0):
+    """effective_block of every block (dtmm/data.py:1421-1443)."""
+    blocks = optical_data if isinstance(optical_data, list) else [optical_data]
+    out = [effective_block(b, symmetry) for b in blocks]
+    return out if isinstance(optical_data, list) else out[0]
+

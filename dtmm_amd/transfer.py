@@ -24,6 +24,7 @@ Everything else the reference accepts raises ``NotImplementedError``.  The
 propagation runs in ``libdtmmx.so`` (HIP, gfx950); torch provides device
 memory and the stream.  There is no CPU fallback.
 """
+import copy
 import ctypes
 import warnings
 
@@ -31,7 +32,8 @@ import numpy as np
 
 from . import _lib
 from . import tune as _tune
-from .data import is_optical_data_dispersive, uniaxial_order, validate_optical_data
+from .data import (effective_from_tensors, is_optical_data_dispersive, parse_symmetry,
+                   uniaxial_order, validate_optical_data)
 from .field import blackman, k0 as _k0
 
 DTMM_NORM_FFT = 1 << 0
@@ -106,7 +108,18 @@ class Stack:
     structure-of-arrays [3][L][H][W] plus per-layer thickness, sub-step
     count and isotropic effective eps.  Built from reference optical data
     (dtmm/transfer.py:926-1005, eff_data=0); layer l is propagated in
-    nstep sub-steps of thickness d / nstep (transfer.py:933-934)."""
+    nstep sub-steps of thickness d / nstep (transfer.py:933-934).
+    Anisotropic effective media (other eff_data) are attached per transfer
+    by ``with_effective``."""
+
+    # anisotropic effective media: [L][3] eigenvalues (re, im), Euler angles,
+    # thickness of the diffraction step; None: the isotropic eps_eff
+    eff_eps_re = None
+    eff_eps_im = None
+    eff_angles = None
+    eff_thickness = None
+    # (first layer, layer count) of each optical block
+    blocks = None
 
     def __init__(self, optical_data, shape, nstep=1, nin=N_COVER, nout=N_COVER,
                  split_layers=False, device="cuda"):
@@ -115,8 +128,13 @@ class Stack:
         H, W = shape
         self.shape = (H, W)
         thick, steps, eff, eps_l, biax = [], [], [], [], []
+        self.blocks = []
         for d, material, angles in blocks:
             n = len(d)
+            if split_layers:
+                self.blocks += [(len(thick) + i, 1) for i in range(n)]
+            else:
+                self.blocks.append((len(thick), n))
             substeps = np.broadcast_to(np.asarray(nstep), (n,))
             mat = np.broadcast_to(material, (n, H, W, 3))
             ang = np.broadcast_to(angles, (n, H, W, 3))
@@ -267,7 +285,93 @@ class Stack:
         self.eps_im = None
         self.angles = angles
         self.has_imag_eff = False
+        self.blocks = [(i, 1) for i in range(L)] if split_layers else [(0, L)]
         return self
+
+    def layer_tensors(self):
+        """Mean laboratory-frame eps tensor (xx, yy, zz, xy, xz, yz) of every
+        layer, (L, 6) complex, from the device material: R diag(eps) R^T per
+        voxel with R = Rz(phi) Ry(theta) Rz(psi) (dtmm/data.py:515-532),
+        averaged over the layer."""
+        torch = _torch()
+        out = np.empty((self.n_layers, 6), np.complex128)
+        for l in range(self.n_layers):
+            a = self.angles[:, l]
+            cps, sps = torch.cos(a[0]), torch.sin(a[0])
+            cth, sth = torch.cos(a[1]), torch.sin(a[1])
+            cph, sph = torch.cos(a[2]), torch.sin(a[2])
+            r = ((cph * cth * cps - sph * sps, -cph * cth * sps - sph * cps, cph * sth),
+                 (sph * cth * cps + cph * sps, cph * cps - sph * cth * sps, sph * sth),
+                 (-sth * cps, sth * sps, cth))
+            e = [self.eps_re[c, l].to(torch.complex128) for c in range(3)]
+            if self.eps_im is not None:
+                e = [e[c] + 1j * self.eps_im[c, l] for c in range(3)]
+            for k, (i, j) in enumerate(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
+                t = r[i][0] * r[j][0] * e[0] + r[i][1] * r[j][1] * e[1] + r[i][2] * r[j][2] * e[2]
+                out[l, k] = complex(t.mean().item())
+        return out
+
+    def with_effective(self, eff_data, per_block=False):
+        """This stack with the effective media of ``eff_data``
+        (dtmm/transfer.py:926-1005): None or a symmetry 0 keeps the isotropic
+        block means; a symmetry 1 or 2 ("uniaxial", "biaxial") or a sequence
+        of per-layer symmetries averages the laboratory-frame tensors
+        (dtmm/data.py:1364-1419); a custom effective block (d_eff, epsv,
+        epsa) is taken as given.  ``per_block``: eff_data holds one entry per
+        optical block.  Effective media that come out isotropic with the
+        layer thickness stay on the isotropic path."""
+        blocks = self.blocks or [(0, self.n_layers)]
+        if per_block:
+            if len(eff_data) != len(blocks):
+                raise ValueError("Number of elements in eff_data must match length of "
+                                 "optical_data.")
+            effs = list(eff_data)
+        else:
+            effs = [eff_data] * len(blocks)
+
+        def default(e):
+            return e is None or (isinstance(e, (int, str)) and parse_symmetry(e) == 0)
+        if all(default(e) for e in effs):
+            return self
+        L = self.n_layers
+        ev = np.empty((L, 3), np.complex128)
+        ea = np.zeros((L, 3))
+        de = np.array(self.thickness, np.float64)
+        tensors = None
+        for (start, n), e in zip(blocks, effs):
+            sl = slice(start, start + n)
+            if default(e):
+                ev[sl] = (self.eps_eff_re[sl] + 1j * self.eps_eff_im[sl])[:, None]
+            elif _is_effective_block(e):
+                d_e, v_e, a_e = e
+                d_e = np.atleast_1d(np.asarray(d_e, np.float64))
+                if len(d_e) != n:
+                    raise ValueError("the effective block needs one layer per layer of its block")
+                ev[sl] = np.broadcast_to(np.asarray(v_e).reshape(-1, 3), (n, 3))
+                ea[sl] = np.broadcast_to(np.asarray(a_e, np.float64).reshape(-1, 3), (n, 3))
+                de[sl] = d_e
+            else:
+                if tensors is None:
+                    tensors = self.layer_tensors()
+                w, a = effective_from_tensors(tensors[sl], e)
+                ev[sl] = w
+                ea[sl] = a
+        out = copy.copy(self)
+        iso = (ev[:, 0] == ev[:, 1]) & (ev[:, 1] == ev[:, 2])
+        if iso.all() and np.array_equal(de, np.asarray(self.thickness, np.float64)):
+            out.eps_eff_re = np.ascontiguousarray(ev[:, 0].real)
+            out.eps_eff_im = np.ascontiguousarray(ev[:, 0].imag)
+            out.has_imag_eff = bool(np.any(out.eps_eff_im != 0))
+            return out
+        out.eff_eps_re = np.ascontiguousarray(ev.real)
+        out.eff_eps_im = np.ascontiguousarray(ev.imag) if np.any(ev.imag != 0) else None
+        out.eff_angles = np.ascontiguousarray(ea)
+        out.eff_thickness = np.ascontiguousarray(de)
+        return out
+
+    @property
+    def anisotropic_effective(self):
+        return self.eff_eps_re is not None
 
     def c_struct(self):
         s = _lib.Stack()
@@ -282,6 +386,11 @@ class Stack:
         s.n_out = self.nout
         s.n_steps = _lib.iptr(self.n_steps)
         s.biaxial = _lib.iptr(self.biaxial) if self.biaxial.any() else None
+        if self.eff_eps_re is not None:
+            s.eff_eps_re = _lib.dptr(self.eff_eps_re)
+            s.eff_eps_im = _lib.dptr(self.eff_eps_im) if self.eff_eps_im is not None else None
+            s.eff_angles = _lib.dptr(self.eff_angles)
+            s.eff_thickness = _lib.dptr(self.eff_thickness)
         return s
 
 
@@ -425,6 +534,13 @@ class Plan:
             pass
 
 
+def _is_effective_block(e):
+    """A custom effective block (d, epsv, epsa) rather than a symmetry or a
+    sequence of per-layer symmetries."""
+    return (isinstance(e, (tuple, list)) and len(e) == 3 and np.ndim(e[1]) >= 1
+            and np.ndim(e[2]) >= 1)
+
+
 def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, norm=DTMM_NORM_FFT):
     """Defaults and reflection auto-selection (dtmm/transfer.py:433-465) and
     the envelope of modes built on the GPU."""
@@ -462,8 +578,6 @@ def _resolve_modes(method, npass, diffraction, reflection, eff_data, ret_bulk, n
     if ret_bulk and (not (0 <= diffraction <= 1) or (npass != 1 and kind == "4x4")):
         raise NotImplementedError("ret_bulk is supported with diffraction 0 or 1, and npass=1 "
                                   "for the 4x4 method")
-    if eff_data not in (None, 0, "isotropic"):
-        raise NotImplementedError("only isotropic effective data (eff_data=0) is supported")
     if diffraction < 0 or diffraction == np.inf:
         # exact per-mode propagation (dtmm/propagate_4x4.py:515-564)
         if kind != "4x4" or npass != 1 or ret_bulk:
@@ -617,6 +731,18 @@ def _prepare(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
     else:
         stack = Stack(optical_data, shape, nstep=nstep, nin=nin, nout=nout,
                       split_layers=split_layers)
+    if diffraction > 0 and diffraction != np.inf:
+        # no effective medium enters without diffraction or in the exact
+        # per-mode propagation
+        per_block = (isinstance(optical_data, list) and eff_data is not None
+                     and not isinstance(eff_data, (int, str)))
+        stack = stack.with_effective(eff_data, per_block)
+        if stack.anisotropic_effective and not (
+                kind == "4x4" and reflection == 1 and diffraction == 1 and
+                not (npass > 1 and norm & DTMM_NORM_REF)):
+            raise NotImplementedError("anisotropic effective data is supported with the 4x4 "
+                                      "method in reflection mode 1 with diffraction 1 "
+                                      "(without DTMM_NORM_REF) and with diffraction 0")
     if isinstance(field_np, _DeviceField):
         fin = field_in.reshape((n_rays, len(ks), 4) + shape).to(dtype=plan.dtype).clone()
     else:

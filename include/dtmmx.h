@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define DTMMX_ABI_VERSION 4
+#define DTMMX_ABI_VERSION 5
 
 enum {
   DTMMX_OK = 0,
@@ -130,6 +130,20 @@ typedef struct {
                                 eigensolve, dtmm/tmm.py:395-403); NULL:
                                 none.  A biaxial voxel in an unflagged
                                 layer sets status bit 0 and yields NaN. */
+  /* Anisotropic effective media (eff_data 1, 2, per-layer symmetries or a
+   * custom effective block; dtmm/transfer.py:926-976, dtmm/data.py:1364-1419).
+   * NULL eff_eps_re: the isotropic eps_eff above.  Otherwise layer l's
+   * effective medium has eigenvalues eff_eps_re/im[l][0..2] (eff_eps_im may
+   * be NULL) and Euler angles eff_angles[l][0..2] (psi, theta, phi), and its
+   * diffraction step the thickness eff_thickness[l] (NULL: thickness[l]), all
+   * host arrays.  Built for the 4x4 method in reflection mode 1 with
+   * diffraction 1 (any npass without DTMMX_NORM_REF, bulk output) and for
+   * diffraction 0, where no effective medium enters; other plans fail with
+   * DTMMX_E_UNSUPPORTED, as do the adjoint and dtmmx_propagate_layer. */
+  const double* eff_eps_re;
+  const double* eff_eps_im;
+  const double* eff_angles;
+  const double* eff_thickness;
 } dtmmx_stack;
 
 typedef struct dtmmx_plan dtmmx_plan;

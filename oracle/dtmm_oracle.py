@@ -45,7 +45,8 @@ __all__ = [
     "BETAMAX", "N_COVER", "k0", "betaphi", "betaxy", "alphaf", "alphaffi",
     "inv", "dotmm", "dotmf", "dotmdm", "dotmdmf", "dotmv", "phase_mat", "E_mat", "E2H_mat",
     "normalize_f", "field2intensity", "fft2", "ifft2", "refind2eps",
-    "uniaxial_order", "validate_optical_data", "create_layers",
+    "uniaxial_order", "validate_optical_data", "create_layers", "effective_block",
+    "eig_symmetry", "eps2epsva",
     "director2Q", "Q2eps", "tensor_eig", "rotation_angles", "Q2data", "epsva2eps",
     "projection_matrix", "field_diffraction_matrix", "diffract",
     "transmitted_field", "first_corrected_Epn_diffraction_matrix",
@@ -883,21 +884,134 @@ def _isotropic_effective_data(block):
     return d, epseff, aeff
 
 
-def create_layers(optical_data, nin, nout, nstep):
+_SYMMETRY = {"isotropic": 0, "uniaxial": 1, "biaxial": 2}
+
+
+def _symmetry_int(arg):
+    """(dtmm/data.py:1343-1352)."""
+    if isinstance(arg, str):
+        try:
+            return _SYMMETRY[arg]
+        except KeyError:
+            raise ValueError("Invalid symmetry argument.")
+    if arg in (0, 1, 2, -1):
+        return arg
+    raise ValueError("Invalid symmetry argument.")
+
+
+def _parse_symmetry(arg):
+    """A symmetry or a sequence of per-layer symmetries (dtmm/data.py:1354-1362)."""
+    if isinstance(arg, str):
+        return _symmetry_int(arg)
+    try:
+        return tuple(_parse_symmetry(a) for a in arg)
+    except TypeError:
+        return _symmetry_int(arg)
+
+
+def eig_symmetry(order, eig):
+    """Isotropic (0), uniaxial (1) or unchanged (2, and -1) eigenvalues
+    (dtmm/data.py:1276-1306): orders above 1 become -1 for uniaxial_order."""
+    order = np.array(order, int)
+    order[order > 1] = -1
+    return uniaxial_order(order, eig)
+
+
+def eps2epsva(eps):
+    """Eigenvalues and Euler angles of tensors (..., 6) (dtmm/data.py:489-513)
+    for real tensors; the reference's complex branch warns that its
+    eigenvectors are then unreliable, and the oracle does not cover it."""
+    eps = np.asarray(eps)
+    if np.iscomplexobj(eps):
+        if np.any(eps.imag != 0):
+            raise NotImplementedError("oracle covers real effective tensors")
+        eps = eps.real
+    w, v = tensor_eig(eps)
+    return w, rotation_angles(v)
+
+
+def effective_block(block, symmetry=0):
+    """Effective layers of a block (dtmm/data.py:1364-1419): symmetry 0 is
+    the mean of the isotropic parts over the block; otherwise the
+    laboratory-frame tensors are averaged (over the whole block, or over each
+    layer for a sequence of per-layer symmetries), diagonalised, and made
+    uniaxial (1) or kept biaxial (2)."""
+    d, epsv, epsa = block
+    axis = list(range(epsv.ndim - 1))
+    if isinstance(symmetry, (str, int)) and symmetry in ("isotropic", 0):
+        ev = eig_symmetry(0, epsv).mean(tuple(axis))
+        ea = np.zeros_like(ev, FDTYPE)
+    else:
+        order = np.asarray(_parse_symmetry(symmetry), int)
+        if order.ndim != 0:
+            if len(order) != len(d):
+                raise ValueError("Shape of the symmetry argument is incompatible with block size.")
+            axis.pop(0)
+        shape = (len(d),) + tuple(np.broadcast_shapes(epsv.shape[1:], epsa.shape[1:]))
+        eps = epsva2eps(np.broadcast_to(epsv, shape), np.broadcast_to(epsa, shape))
+        eps = eps.mean(tuple(axis))
+        ev, ea = eps2epsva(eps)
+        ev = eig_symmetry(order, ev)
+    if ev.ndim == 1:
+        ev = np.asarray((ev,) * len(d)).copy()
+        ea = np.asarray((ea,) * len(d)).copy()
+    return d, ev, ea
+
+
+def _is_block(data):
+    """A custom effective block (d, epsv, epsa), as opposed to a symmetry
+    identifier or a sequence of per-layer symmetries."""
+    if not isinstance(data, (tuple, list)) or len(data) != 3:
+        return False
+    return np.ndim(data[1]) >= 1 and np.ndim(data[2]) >= 1
+
+
+def _effective_layers(block, eff_data):
+    """(d, epsv, epsa) of a block's effective layers, each epsv/epsa (L, 3)
+    (dtmm/transfer.py:941-948): a custom block as given, else
+    effective_block, the default (None) the isotropic mean."""
+    d = block[0]
+    if eff_data is None:
+        return _isotropic_effective_data(block)
+    if _is_block(eff_data):
+        de, ee, ae = eff_data
+        de = np.atleast_1d(np.asarray(de, FDTYPE))
+        ee = np.asarray(ee)
+        ee = np.asarray(ee, CDTYPE if np.iscomplexobj(ee) else FDTYPE)
+        ee = np.broadcast_to(ee.reshape(-1, 3) if ee.ndim > 1 else ee, (len(de), 3)).copy()
+        ae = np.broadcast_to(np.asarray(ae, FDTYPE).reshape(-1, 3) if np.ndim(ae) > 1
+                             else np.asarray(ae, FDTYPE), (len(de), 3)).copy()
+        if len(de) != len(d):
+            raise ValueError("the effective block needs one layer per layer of the block")
+        return de, ee, ae
+    de, ee, ae = effective_block(block, eff_data)
+    return de, np.asarray(ee).reshape(len(de), 3), np.asarray(ae, FDTYPE).reshape(len(de), 3)
+
+
+def create_layers(optical_data, nin, nout, nstep, eff_data=None):
     """Layer and effective-layer lists with the input/output media as
-    zero-thickness layers (dtmm/transfer.py:926-1005; eff_data=0 only)."""
+    zero-thickness layers (dtmm/transfer.py:926-1005): eff_data None (the
+    isotropic mean), a symmetry 0/1/2 ("isotropic", "uniaxial", "biaxial"),
+    a sequence of per-layer symmetries or a custom effective block, one for
+    all blocks or a list with one entry per block."""
     if is_optical_data_dispersive(optical_data):
         raise ValueError("You are using dispersive data, so you must use "
                          "`split_wavelengths=True`")
     blocks = validate_optical_data(optical_data)
-    layers, eff_layers = [], []
     nb = len(blocks)
+    if isinstance(optical_data, list) and not (eff_data is None or isinstance(eff_data, (str, int))):
+        if len(eff_data) != nb:
+            raise ValueError("Number of elements in eff_data must match length of optical_data.")
+        effs = list(eff_data)
+    else:
+        effs = [eff_data] * nb
+    layers, eff_layers = [], []
     for ib, (d, epsv, epsa) in enumerate(blocks):
         substeps = np.broadcast_to(np.asarray(nstep), (len(d),))
         ev_b = np.broadcast_to(epsv, (len(d),) + epsv.shape[1:])
         ea_b = np.broadcast_to(epsa, (len(d),) + epsa.shape[1:])
         ls = [(int(n), (t / n, ev, ea)) for n, t, ev, ea in zip(substeps, d, ev_b, ea_b)]
-        de, ee, ae = _isotropic_effective_data((d, epsv, epsa))
+        de, ee, ae = _effective_layers((d, epsv, epsa), effs[ib])
         es = [(int(n), (t / n, ev, ea)) for n, t, ev, ea in zip(substeps, de, ee, ae)]
         if ib == 0:
             ls.insert(0, (1, (0., np.broadcast_to(refind2eps([nin] * 3), ev_b[0].shape),
@@ -1673,7 +1787,7 @@ def project_normalized_local(field, dmat, nout, ref=None):
 
 def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1,
                  nstep=1, diffraction=1, reflection=1, betamax=BETAMAX, norm=DTMM_NORM_FFT,
-                 smooth=SMOOTH, layer_callback=None, ret_bulk=False):
+                 smooth=SMOOTH, layer_callback=None, ret_bulk=False, eff_data=None):
     """4x4 transfer, reflection mode 1 (dtmm/transfer.py:618-884).
 
     The input is not projected onto forward modes and the input/output media
@@ -1697,7 +1811,7 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
             raise NotImplementedError("oracle covers full diffraction with npass 1")
         field_in, wavelengths, pixelsize = field_data
         field = np.array(field_in, CDTYPE)
-        layers, _ = create_layers(optical_data, nin, nout, nstep)
+        layers, _ = create_layers(optical_data, nin, nout, nstep, eff_data)
         ks = k0(wavelengths, pixelsize)
         for j in range(1, len(layers) - 1):
             nsub, layer = layers[j]
@@ -1709,7 +1823,7 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
         # nor transformed, input and output media are skipped
         field_in, wavelengths, pixelsize = field_data
         field = np.array(field_in, CDTYPE)
-        layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
+        layers, eff_layers = create_layers(optical_data, nin, nout, nstep, eff_data)
         ks = k0(wavelengths, pixelsize)
         beta, phi = _validate_betaphi(beta, phi, extendeddim=field.ndim - 2)
         cache = _Cache()
@@ -1746,7 +1860,7 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
         smooth = 0.
     field_in, wavelengths, pixelsize = field_data
     field_in = np.asarray(field_in, CDTYPE)
-    layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
+    layers, eff_layers = create_layers(optical_data, nin, nout, nstep, eff_data)
     ks = k0(wavelengths, pixelsize)
     n = len(layers)
     beta, phi = _validate_betaphi(beta, phi, extendeddim=field_in.ndim - 2)
@@ -1842,7 +1956,7 @@ def transfer_4x4(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
 
 def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npass=1, nstep=1,
                  diffraction=1, reflection=0, betamax=BETAMAX, layer_callback=None,
-                 ret_bulk=False):
+                 ret_bulk=False, eff_data=None):
     """2x2 (E-field) transfer (dtmm/transfer.py:1009-1183): reflection mode 0
     (diffraction 0 or 1), 1 (diffraction 1; diffraction 0 with npass 1) and
     2 (diffraction 0 or 1), any npass with reflection 1 or 2.
@@ -1861,7 +1975,7 @@ def transfer_2x2(field_data, optical_data, beta=0., phi=0., nin=1., nout=1., npa
         raise ValueError("Reflection mode `0` not compatible with npass > 1")
     user_in, wavelengths, pixelsize = field_data
     field_in = np.asarray(user_in, CDTYPE)
-    layers, eff_layers = create_layers(optical_data, nin, nout, nstep)
+    layers, eff_layers = create_layers(optical_data, nin, nout, nstep, eff_data)
     ks = k0(wavelengths, pixelsize)
     n = len(layers) - 1
     beta, phi = _validate_betaphi(beta, phi, extendeddim=field_in.ndim - 2)
@@ -1952,7 +2066,7 @@ def _transmitted_field_direct(field, beta, phi, n=1.):
 def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout=None,
                    npass=None, nstep=1, diffraction=None, reflection=None, method=None,
                    betamax=None, norm=DTMM_NORM_FFT, smooth=SMOOTH, layer_callback=None,
-                   ret_bulk=False, split_wavelengths=False, multiray=False):
+                   ret_bulk=False, split_wavelengths=False, multiray=False, eff_data=None):
     """Defaults and dispatch of dtmm.transfer_field (dtmm/transfer.py:322-615)
     for the covered modes.  The configured default method is the quoted string
     '"2x2"' (dtmm/dtmm.ini:23 read by configparser), which makes the automatic
@@ -1970,7 +2084,7 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
     kw = dict(beta=beta, phi=phi, nin=nin, nout=nout, npass=npass, nstep=nstep,
               diffraction=diffraction, reflection=reflection, method=method, betamax=betamax,
               norm=norm, smooth=smooth, layer_callback=layer_callback, ret_bulk=ret_bulk,
-              multiray=multiray)
+              multiray=multiray, eff_data=eff_data)
     if split_wavelengths and not isinstance(field_in, tuple) and not ret_bulk:
         field_in = np.asarray(field_in)
         parts = tuple(field_in[..., i, :, :, :] for i in range(len(wavelengths)))
@@ -1992,7 +2106,7 @@ def transfer_field(field_data, optical_data, beta=None, phi=None, nin=None, nout
 def _transfer_field(field_in, wavelengths, pixelsize, optical_data, beta=None, phi=None,
                     nin=None, nout=None, npass=None, nstep=1, diffraction=None, reflection=None,
                     method=None, betamax=None, norm=DTMM_NORM_FFT, smooth=SMOOTH,
-                    layer_callback=None, ret_bulk=False, multiray=False):
+                    layer_callback=None, ret_bulk=False, multiray=False, eff_data=None):
     """The single transfer of transfer_field (dtmm/transfer.py:554-615).
     Without beta and phi the beam is inferred from the input field
     (transfer.py:662-664, :1040-1042)."""
@@ -2016,9 +2130,9 @@ def _transfer_field(field_in, wavelengths, pixelsize, optical_data, beta=None, p
     if method == "4x4":
         return transfer_4x4(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
                             diffraction, reflection, betamax, norm, smooth, layer_callback,
-                            ret_bulk)
+                            ret_bulk, eff_data)
     return transfer_2x2(field_data, optical_data, beta, phi, nin, nout, npass, nstep,
-                        diffraction, reflection, betamax, layer_callback, ret_bulk)
+                        diffraction, reflection, betamax, layer_callback, ret_bulk, eff_data)
 
 
 # ---------------------------------------------------------------------------
