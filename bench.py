@@ -402,6 +402,89 @@ def run_tmm1d(args):
     print(json.dumps(rec), flush=True)
 
 
+def run_tmm2d(args):
+    """The 2D non-iterative method on the cholesteric grating of the
+    reference's 2D example (examples/2D/cholesteric_grating.py): a tilted
+    cholesteric helix, 180 pixels (one x pitch) by 72 layers (two z
+    pitches), no 1.5 / ne 1.7, 19 wavelengths 380-780 nm, pixel size 10 nm,
+    unpolarised normal incidence (2 rays) between media of index 1.5."""
+    import torch
+    import dtmm_amd as dm
+    nlam, ps = 19, float(args.pixelsize)
+    pitch_z, pitch_x = 36, 180
+    size_x, size_z = pitch_x, 2 * pitch_z
+    tilt = np.arctan(pitch_z / pitch_x)
+    twist = (np.arange(0, 2 * np.pi * size_z / pitch_z, 2 * np.pi / pitch_z)[:, None] +
+             np.arange(0, 2 * np.pi * size_x / pitch_x, 2 * np.pi / pitch_x)[None, :])
+    n = np.stack([np.cos(twist), np.sin(twist), np.zeros_like(twist)], -1)
+    ry = np.array([[np.cos(tilt), 0, np.sin(tilt)], [0, 1, 0], [-np.sin(tilt), 0, np.cos(tilt)]])
+    n = n @ ry.T
+    L, W = twist.shape
+    epsa = dm.director2angles(n)
+    epsv = np.empty((L, W, 3), np.complex128)
+    epsv[..., :2] = 1.5 ** 2
+    epsv[..., 2] = 1.7 ** 2
+    d = np.ones(L)
+    wl = np.linspace(380, 780, nlam)
+    f3 = dm.illumination_data((W, 1), wl, jones=None, beta=0., phi=0., pixelsize=ps, n=1.5,
+                              betamax=0.8)[0]
+    f0 = np.ascontiguousarray(f3[..., 0])
+    optical = (d, epsv, epsa)
+    kw = dict(nin=1.5, nout=1.5)
+
+    def step():
+        f = f0.copy()
+        return dm.tmm2d.transfer2d((f, wl, ps), optical, **kw)[0]
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    if not np.isfinite(out).all():
+        log("non-finite output")
+        sys.exit(1)
+    ks = dm.k0(wl, ps)
+    nmodes = [int(dm.tmm2d.eigenmask1(W, k).sum()) for k in ks]
+    units = W * L * nlam
+    # per wavelength: the n x W x 4x4 layer blocks, their DFT coupling and the
+    # (4n)^3 complex stack product of every layer (8 real flops per MAC)
+    flops = sum(L * (8. * (4 * m) ** 3 + 8. * (4 * m) ** 2 * W) for m in nmodes)
+    log("tmm2d %.3f ms per call, modes %d..%d" % (ms, min(nmodes), max(nmodes)))
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        f = f0.copy()
+        t = time.time()
+        ref = oracle.transfer2d((f, wl, ps), optical, **kw)[0]
+        tc = time.time() - t
+        err = float(np.abs(out - ref).max() / np.abs(ref).max())
+        log("GPU vs oracle: %.2e (oracle %.2f s)" % (err, tc))
+        cpu = {"value": units / tc, "unit": "pixel*layer*lambda/s", "cores": 1, "kind": "port",
+               "sample": "oracle transfer2d (numpy, complex128) on the whole workload, %.2f s"
+                         % tc}
+    rec = {
+        "metric": "pixel*layer*lambda/s of the 2D non-iterative transfer (tmm2d.transfer2d)",
+        "value": units / (ms / 1e3),
+        "unit": "pixel*layer*lambda/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 (fp64)",
+        "data": "synthetic (the tilted cholesteric grating of the reference's 2D example)",
+        "config": {"workload": "cholesteric grating %d px x %d layers, %d wavelengths 380-780 nm, "
+                               "pixel %.0f nm, 2 rays, %d-%d modes, host arrays in and out"
+                               % (W, L, nlam, ps, min(nmodes), max(nmodes)),
+                   "width": W, "layers": L, "wavelengths": nlam},
+        "roofline": {"bound": "fp64", "kernel": "whole call (launch-bound at these mode counts)",
+                     "achieved": flops / (ms / 1e3) / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": flops / (ms / 1e3) / 1e12 / 78.6, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def build_inputs_c5(size, layers, nlam):
     import dtmm_amd as dm
     from dtmm_amd.samples import droplet_array_data, droplet_grid_centres
@@ -536,11 +619,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="c2",
-                    choices=["c2", "c3", "c4", "c5", "pom", "tmm1d"],
+                    choices=["c2", "c3", "c4", "c5", "pom", "tmm1d", "tmm2d"],
                     help="c2: 512^2 x 128 random texture (default); c3: batch of uneven "
                          "samples; c5: 1024^2 x 200 droplet grid with npass=3, diffraction=5; "
                          "pom: the polarizing-microscope specter of a 512^2 x 9 x 2 field; "
-                         "tmm1d: the 1D transfer matrix method, one stack per pixel")
+                         "tmm1d: the 1D transfer matrix method, one stack per pixel; "
+                         "tmm2d: the 2D method on the reference's cholesteric grating")
+    ap.add_argument("--pixelsize", type=float, default=10.,
+                    help="tmm2d: pixel size in nm (more modes for larger pixels)")
     ap.add_argument("--samples", type=int, default=64, help="c3: batch size")
     ap.add_argument("--cpu-crop", type=int, default=96, help="c5: oracle crop size")
     ap.add_argument("--steps", type=int, default=5)
@@ -596,6 +682,9 @@ def main():
         return
     if args.workload == "pom":
         run_pom(args)
+        return
+    if args.workload == "tmm2d":
+        run_tmm2d(args)
         return
     if args.workload == "tmm1d":
         run_tmm1d(args)

@@ -4047,12 +4047,24 @@ int dtmmx_illumination(const dtmmx_plan* P, const dtmmx_illum_desc* d, void* fie
 static size_t al256(size_t b) { return (b + 255) / 256 * 256; }
 static char* align_ws(void* ws) { return (char*)(((uintptr_t)ws + 255) & ~uintptr_t(255)); }
 
+// Layers per chunk: all layers of a chunk are built in one launch; their
+// per-pixel blocks take at most T2D_CHUNK_BYTES.
+static constexpr size_t T2D_CHUNK_BYTES = size_t(256) << 20;
+
+static int t2d_chunk(const dtmmx_tmm2d_desc* d) {
+  const size_t per = size_t(d->n_modes) * d->width * 16 * sizeof(cd);
+  size_t c = std::max<size_t>(1, T2D_CHUNK_BYTES / std::max<size_t>(per, 1));
+  c = std::min<size_t>(c, size_t(T2D_MAXL));
+  return int(std::min<size_t>(c, size_t(d->n_layers)));
+}
+
 size_t dtmmx_tmm2d_workspace_size(const dtmmx_tmm2d_desc* d) {
-  if (!d || d->n_modes <= 0 || d->width <= 0) return 0;
+  if (!d || d->n_modes <= 0 || d->width <= 0 || d->n_layers <= 0) return 0;
   const size_t N = 4 * size_t(d->n_modes);
   const size_t cb = sizeof(cd);
-  return 256 + al256(size_t(d->n_modes) * d->width * 16 * cb) + al256(size_t(d->width) * cb) +
-         3 * al256(N * N * cb);
+  const size_t lc = size_t(t2d_chunk(d));
+  return 256 + al256(lc * d->n_modes * d->width * 16 * cb) + al256(size_t(d->width) * cb) +
+         al256(lc * N * N * cb) + al256((lc + 1) / 2 * N * N * cb) + 2 * al256(N * N * cb);
 }
 
 int dtmmx_tmm2d_stack(const dtmmx_tmm2d_desc* d, void* cmat, void* ws, size_t ws_bytes,
@@ -4065,50 +4077,75 @@ int dtmmx_tmm2d_stack(const dtmmx_tmm2d_desc* d, void* cmat, void* ws, size_t ws
     return fail(DTMMX_E_ARG, "Unsupported method");
   if (!(d->k0 > 0.) || !std::isfinite(d->betay)) return fail(DTMMX_E_ARG, "bad wavenumber or betay");
   if (ws_bytes < dtmmx_tmm2d_workspace_size(d)) return fail(DTMMX_E_ARG, "workspace too small");
-  const int W = d->width, n = d->n_modes, N = 4 * n;
-  const size_t cb = sizeof(cd);
+  const int W = d->width, n = d->n_modes, N = 4 * n, L = d->n_layers;
+  const int lc = t2d_chunk(d);
+  const size_t cb = sizeof(cd), NN = size_t(N) * N;
   char* w = align_ws(ws);
   cd* M = (cd*)w;
-  w += al256(size_t(n) * W * 16 * cb);
+  w += al256(size_t(lc) * n * W * 16 * cb);
   cd* tw = (cd*)w;
   w += al256(size_t(W) * cb);
-  cd* Cl = (cd*)w;
-  w += al256(size_t(N) * N * cb);
-  cd* P = (cd*)w;
-  w += al256(size_t(N) * N * cb);
+  cd* A = (cd*)w;  // the chunk's layer matrices, then tree levels
+  w += al256(size_t(lc) * NN * cb);
+  cd* B = (cd*)w;
+  w += al256(size_t(lc + 1) / 2 * NN * cb);
+  cd* P = (cd*)w;  // product of the chunks so far
+  w += al256(NN * cb);
   cd* Q = (cd*)w;
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(t2d_tw_kernel, dim3(unsigned((W + NT - 1) / NT)), dim3(NT), 0, s, tw, W);
   HIP_TRY(hipGetLastError());
-  const size_t NN = size_t(N) * N;
-  const dim3 gN(unsigned((NN + NT - 1) / NT));
+  const size_t stride = size_t(L) * W;
   const dim3 gm(unsigned((N + 15) / 16), unsigned((N + 15) / 16));
-  const size_t stride = size_t(d->n_layers) * W;
-  for (int l = 0; l < d->n_layers; ++l) {
+  bool have = false;
+  for (int l0 = 0; l0 < L; l0 += lc) {
+    const int nl = std::min(lc, L - l0);
+    const bool last = l0 + nl == L;
     T2dArgs a{};
     a.W = W;
     a.n = n;
     a.idx = d->mode_index;
     a.k0 = d->k0;
     a.betay = d->betay;
-    a.kd = d->k0 * d->thickness[l];
-    a.eps_re = d->eps_re + size_t(l) * W;
-    a.eps_im = d->eps_im ? d->eps_im + size_t(l) * W : nullptr;
-    a.ang = d->angles + size_t(l) * W;
+    for (int l = 0; l < nl; ++l) a.kd[l] = d->k0 * d->thickness[l0 + l];
+    a.nl = nl;
+    a.eps_re = d->eps_re + size_t(l0) * W;
+    a.eps_im = d->eps_im ? d->eps_im + size_t(l0) * W : nullptr;
+    a.ang = d->angles + size_t(l0) * W;
     a.stride = stride;
     a.method = d->method == DTMMX_TMM_4X4_1 ? 1 : 0;
     a.M = M;
     a.tw = tw;
-    // the first layer's matrix goes straight to the product
-    a.C = (l == 0) ? (d->n_layers == 1 ? (cd*)cmat : P) : Cl;
-    hipLaunchKernelGGL(t2d_mode_kernel, dim3(unsigned((W + NT - 1) / NT), unsigned(n)), dim3(NT),
-                       0, s, a);
+    a.C = A;
+    hipLaunchKernelGGL(t2d_mode_kernel, dim3(unsigned((W + NT - 1) / NT), unsigned(n), unsigned(nl)),
+                       dim3(NT), 0, s, a);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(t2d_dft_kernel, gN, dim3(NT), 0, s, a);
+    hipLaunchKernelGGL(t2d_dft_kernel, dim3(unsigned(size_t(nl) * n * n)), dim3(64), 0, s, a);
     HIP_TRY(hipGetLastError());
-    if (l > 0) {
-      cd* dst = (l + 1 == d->n_layers) ? (cd*)cmat : Q;
-      hipLaunchKernelGGL(zgemm_kernel, gm, dim3(256), 0, s, (const cd*)P, (const cd*)Cl, dst, N);
+    // ordered product M_l0 . M_l0+1 ... of the chunk as a tree of pairwise
+    // products of neighbours (matrix products are associative)
+    cd* src = A;
+    cd* dst = B;
+    int cnt = nl;
+    while (cnt > 1) {
+      const int half = cnt / 2;
+      hipLaunchKernelGGL(zgemm_kernel, dim3(gm.x, gm.y, unsigned(half)), dim3(256), 0, s,
+                         (const cd*)src, 2 * NN, (const cd*)(src + NN), 2 * NN, dst, NN, N);
+      HIP_TRY(hipGetLastError());
+      if (cnt & 1)
+        HIP_TRY(hipMemcpyAsync(dst + size_t(half) * NN, src + size_t(cnt - 1) * NN, NN * cb,
+                               hipMemcpyDeviceToDevice, s));
+      cnt = half + (cnt & 1);
+      std::swap(src, dst);
+    }
+    // src holds the chunk's product
+    if (!have) {
+      HIP_TRY(hipMemcpyAsync(last ? (cd*)cmat : P, src, NN * cb, hipMemcpyDeviceToDevice, s));
+      have = true;
+    } else {
+      cd* out = last ? (cd*)cmat : Q;
+      hipLaunchKernelGGL(zgemm_kernel, gm, dim3(256), 0, s, (const cd*)P, size_t(0),
+                         (const cd*)src, size_t(0), out, size_t(0), N);
       HIP_TRY(hipGetLastError());
       std::swap(P, Q);
     }
@@ -4148,6 +4185,11 @@ int dtmmx_tmm2d_reflection(int n, const void* smat, void* rmat, void* ws, size_t
   w += al256(size_t(N) * 2 * N * sizeof(cd));
   cd* fac = (cd*)w;
   hipStream_t s = (hipStream_t)stream;
+  if (N <= GJ_SMALL) {
+    hipLaunchKernelGGL(gj_small_kernel, dim3(1), dim3(256), 0, s, (const cd*)smat, (cd*)rmat, N);
+    HIP_TRY(hipGetLastError());
+    return DTMMX_OK;
+  }
   const size_t NG = size_t(N) * 2 * N;
   hipLaunchKernelGGL(gj_init_kernel, dim3(unsigned((NG + NT - 1) / NT)), dim3(NT), 0, s,
                      (const cd*)smat, G, N);

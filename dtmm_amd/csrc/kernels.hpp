@@ -2738,80 +2738,97 @@ __global__ __launch_bounds__(NT) void illum_kernel(IllumArgs<T> A) {
 // component (a, b) of block (i, j).  fp64 throughout.
 // ---------------------------------------------------------------------------
 
+constexpr int T2D_MAXL = 256;  // layers of one chunk (their k0 d travel in the arguments)
+
 struct T2dArgs {
   int W, n;
   const int* idx;        // [n] FFT bin of every mode
   double k0, betay;
-  double kd;             // k0 * thickness of this layer
-  const double* eps_re;  // this layer: [3][stride] with pixel x at offset x
+  double kd[T2D_MAXL];   // k0 * thickness of each layer of the chunk
+  int nl;                // layers in the chunk
+  const double* eps_re;  // first layer of the chunk: [3][stride], layer l at l * W
   const double* eps_im;
   const double* ang;
   size_t stride;
   int method;            // 0: 4x4, 1: 4x4_1 (backward phases zeroed)
-  cd* M;                 // [n][W][16]: F P(-kd) F^-1 of pixel x at mode j's beam
+  cd* M;                 // [nl][n][W][16]: F P(-kd) F^-1 of pixel x at mode j's beam
   const cd* tw;          // [W]: exp(2 pi i t / W)
-  cd* C;                 // [4n][4n]
+  cd* C;                 // [nl][4n][4n]
 };
 
-// M[j][x] = F P(-k0 d) F^-1 of pixel x for the beam of mode j
-// (beta_x = 2 pi / k0 fftfreq(idx_j), beta_y; dtmm/tmm2d.py:67-79).
+// M[l][j][x] = F P(-k0 d_l) F^-1 of pixel x of layer l for the beam of mode
+// j (beta_x = 2 pi / k0 fftfreq(idx_j), beta_y; dtmm/tmm2d.py:67-79).
 static __global__ __launch_bounds__(NT) void t2d_mode_kernel(T2dArgs A) {
   const int x = blockIdx.x * NT + threadIdx.x;
-  const int j = blockIdx.y;
+  const int j = blockIdx.y, l = blockIdx.z;
   if (x >= A.W) return;
   const double bx = (2 * M_PI / A.k0) * fftfreq(A.idx[j], A.W);
   const double beta = sqrt(bx * bx + A.betay * A.betay), phi = atan2(A.betay, bx);
   double s0, c0;
   sincos(phi, &s0, &c0);
-  const cd e0 = mk<double>(A.eps_re[x], A.eps_im ? A.eps_im[x] : 0.);
-  const cd e1 = mk<double>(A.eps_re[A.stride + x], A.eps_im ? A.eps_im[A.stride + x] : 0.);
-  const cd e2 = mk<double>(A.eps_re[2 * A.stride + x], A.eps_im ? A.eps_im[2 * A.stride + x] : 0.);
+  const size_t o = size_t(l) * A.W + x;
+  const cd e0 = mk<double>(A.eps_re[o], A.eps_im ? A.eps_im[o] : 0.);
+  const cd e1 = mk<double>(A.eps_re[A.stride + o], A.eps_im ? A.eps_im[A.stride + o] : 0.);
+  const cd e2 = mk<double>(A.eps_re[2 * A.stride + o], A.eps_im ? A.eps_im[2 * A.stride + o] : 0.);
   Modes m;
   int kind;
-  layer_modes(beta, c0, s0, phi, e0, e1, e2, A.ang[x], A.ang[A.stride + x],
-              A.ang[2 * A.stride + x], m, kind);
+  layer_modes(beta, c0, s0, phi, e0, e1, e2, A.ang[o], A.ang[A.stride + o],
+              A.ang[2 * A.stride + o], m, kind);
   cd Fi[4][4];
   inv4(m.F, Fi);
   cd p[4];
   for (int q = 0; q < 4; ++q) {
-    p[q] = cexpi(m.a[q] * (-A.kd));
+    p[q] = cexpi(m.a[q] * (-A.kd[l]));
     if (A.method == 1 && (q & 1)) p[q] = mk<double>(0.);
   }
-  cd* o = A.M + (size_t(j) * A.W + x) * 16;
+  cd* out = A.M + ((size_t(l) * A.n + j) * A.W + x) * 16;
   for (int a = 0; a < 4; ++a)
     for (int b = 0; b < 4; ++b) {
       cd t = mk<double>(0.);
       for (int q = 0; q < 4; ++q) t = t + m.F[a][q] * p[q] * Fi[q][b];
-      o[a * 4 + b] = t;
+      out[a * 4 + b] = t;
     }
 }
 
-// C(4i + a, 4j + b) = 1/W sum_x M[j][x][a][b] exp(2 pi i (idx_j - idx_i) x / W):
+// C_l(4i + a, 4j + b) = 1/W sum_x M[l][j][x][a][b] exp(2 pi i (idx_j - idx_i) x / W):
 // the masked FFT of M_j times mode j's eigen-wave (dtmm/tmm2d.py:81-87).
-static __global__ __launch_bounds__(NT) void t2d_dft_kernel(T2dArgs A) {
-  const int N = 4 * A.n;
-  const size_t e = size_t(blockIdx.x) * NT + threadIdx.x;
-  if (e >= size_t(N) * N) return;
-  const int r = int(e / N), c = int(e - size_t(r) * N);
-  const int i = r >> 2, a = r & 3, j = c >> 2, b = c & 3;
-  int dk = A.idx[j] - A.idx[i];
-  dk %= A.W;
+// One wave per block (l, i, j): lanes split x and read 256 contiguous bytes
+// each (the 16 entries of one pixel), then the 16 sums are reduced in LDS.
+static __global__ __launch_bounds__(64) void t2d_dft_kernel(T2dArgs A) {
+  const int n = A.n, N = 4 * n;
+  const size_t blk = blockIdx.x;
+  const int l = int(blk / (size_t(n) * n));
+  const int i = int((blk / n) % n), j = int(blk % n);
+  const int lane = threadIdx.x;
+  __shared__ cd red[64][17];
+  int dk = (A.idx[j] - A.idx[i]) % A.W;
   if (dk < 0) dk += A.W;
-  const cd* mj = A.M + size_t(j) * A.W * 16 + a * 4 + b;
-  cd t = mk<double>(0.);
-  int ph = 0;
-  for (int x = 0; x < A.W; ++x) {
-    t = t + mj[size_t(x) * 16] * A.tw[ph];
-    ph += dk;
-    if (ph >= A.W) ph -= A.W;
+  const cd* mj = A.M + (size_t(l) * n + j) * A.W * 16;
+  cd acc[16];
+  for (int e = 0; e < 16; ++e) acc[e] = mk<double>(0.);
+  for (int x = lane; x < A.W; x += 64) {
+    const cd w = A.tw[int((long long)dk * x % A.W)];
+    const cd* m = mj + size_t(x) * 16;
+    for (int e = 0; e < 16; ++e) acc[e] = acc[e] + m[e] * w;
   }
-  A.C[e] = t * (1. / A.W);
+  for (int e = 0; e < 16; ++e) red[lane][e] = acc[e];
+  __syncthreads();
+  if (lane < 16) {
+    cd t = mk<double>(0.);
+    for (int r = 0; r < 64; ++r) t = t + red[r][lane];
+    const int a = lane >> 2, b = lane & 3;
+    A.C[size_t(l) * N * N + size_t(4 * i + a) * N + 4 * j + b] = t * (1. / A.W);
+  }
 }
 
-// out = a . b for N x N complex128 matrices (row-major), 16 x 16 tiles.
-static __global__ __launch_bounds__(256) void zgemm_kernel(const cd* a, const cd* b, cd* out,
-                                                         int N) {
+// out_z = a_z . b_z for N x N complex128 matrices (row-major), 16 x 16
+// tiles; batch z at offsets z * sa, z * sb, z * so.
+static __global__ __launch_bounds__(256) void zgemm_kernel(const cd* a, size_t sa, const cd* b,
+                                                         size_t sb, cd* out, size_t so, int N) {
   __shared__ cd ta[16][17], tb[16][17];
+  a += blockIdx.z * sa;
+  b += blockIdx.z * sb;
+  out += blockIdx.z * so;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
   cd acc = mk<double>(0.);
@@ -2927,6 +2944,73 @@ static __global__ __launch_bounds__(NT) void gj_extract_kernel(const cd* G, cd* 
   if (e >= size_t(N) * N) return;
   const int r = int(e / N), c = int(e - size_t(r) * N);
   R[e] = G[size_t(r) * 2 * N + N + c];
+}
+
+// reflection_mat for small systems (N <= GJ_SMALL) in one work-group: the
+// same augmented matrix, pivot rule and elimination steps as gj_init /
+// gj_pivot / gj_elim / gj_extract, held in LDS.
+constexpr int GJ_SMALL = 40;
+
+static __global__ __launch_bounds__(256) void gj_small_kernel(const cd* S, cd* R, int N) {
+  __shared__ cd G[GJ_SMALL][2 * GJ_SMALL];
+  __shared__ cd fac[GJ_SMALL];
+  __shared__ int piv_row;
+  const int tid = threadIdx.x, W2 = 2 * N;
+  for (int e = tid; e < N * W2; e += 256) {
+    const int r = e / W2, c = e - r * W2;
+    cd v;
+    if (c < N) {
+      v = (c & 1) ? mk<double>(r == c ? 1. : 0.) : -S[size_t(r) * N + c];
+    } else {
+      const int cc = c - N;
+      v = (cc & 1) ? S[size_t(r) * N + cc] : mk<double>(r == cc ? -1. : 0.);
+    }
+    G[r][c] = v;
+  }
+  __syncthreads();
+  for (int k = 0; k < N; ++k) {
+    if (tid == 0) {
+      double best = -1.;
+      int p = k;
+      for (int r = k; r < N; ++r) {
+        const double m = fabs(G[r][k].x) + fabs(G[r][k].y);
+        if (m > best) {
+          best = m;
+          p = r;
+        }
+      }
+      piv_row = p;
+    }
+    __syncthreads();
+    const int p = piv_row;
+    if (p != k)
+      for (int c = tid; c < W2; c += 256) {
+        const cd t = G[k][c];
+        G[k][c] = G[p][c];
+        G[p][c] = t;
+      }
+    __syncthreads();
+    const cd pv = G[k][k];
+    const double n2 = norm2(pv);
+    const cd ip = n2 > 0. ? conj(pv) * (1. / n2) : mk<double>(0.);
+    __syncthreads();
+    for (int c = tid; c < W2; c += 256) G[k][c] = G[k][c] * ip;
+    __syncthreads();
+    for (int r = tid; r < N; r += 256) fac[r] = (r == k) ? mk<double>(0.) : G[r][k];
+    __syncthreads();
+    for (int e = tid; e < N * W2; e += 256) {
+      const int r = e / W2, c = e - r * W2;
+      if (r == k) continue;
+      const cd f = fac[r];
+      if (f.x == 0. && f.y == 0.) continue;
+      G[r][c] = G[r][c] - f * G[k][c];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < N * N; e += 256) {
+    const int r = e / N, c = e - r * N;
+    R[e] = G[r][N + c];
+  }
 }
 
 // reflect2d (dtmm/tmm2d.py:184-213) for nvec sets of n mode vectors

@@ -126,7 +126,6 @@ def _stack_one(layers, k0, idx, betay, method):
     ws = _ws(lib.dtmmx_tmm2d_workspace_size(ctypes.byref(d)))
     _lib.check(lib.dtmmx_tmm2d_stack(ctypes.byref(d), ctypes.c_void_p(cm.data_ptr()),
                                      ctypes.c_void_p(ws.data_ptr()), ws.numel(), _stream()))
-    torch.cuda.synchronize()
     return cm
 
 
@@ -341,39 +340,33 @@ def transfer2d(field_data_in, optical_data, betay=0., nin=1., nout=1., method="4
     d, epsv, epsa = optical_data
     ks = np.asarray(2 * np.pi / np.asarray(w) * p, np.float64)
     layers = _Layers(d, epsv, epsa)
-    fa = np.asarray(f, np.complex128)
     multi = ks.ndim > 0
     kl = list(ks) if multi else [float(ks)]
-    outs = []
-    backs = []
+    fa = _dev(np.asarray(f, np.complex128))
+    fo_all = _dev(np.asarray(field_out, np.complex128)) if field_out is not None else None
+    outs, backs = [], []
     for i, kk in enumerate(kl):
         fi = fa[..., i, :, :] if multi else fa
+        lead = tuple(fi.shape[:-2])
+        nvec = int(np.prod(lead)) if lead else 1
         idx = eigenindices1(W, kk, betamax)
         n = len(idx)
-        lead = fi.shape[:-2]
-        nvec = int(np.prod(lead)) if lead else 1
         idx_d = _dev(idx.astype(np.int32))
-        mi = _modes_dev(_dev(np.ascontiguousarray(fi).reshape(nvec, 4, W)), idx_d, W, n, nvec,
-                        False)
-        if field_out is not None:
-            fo = np.asarray(field_out, np.complex128)
-            fo = fo[..., i, :, :] if multi else fo
-            mo = _modes_dev(_dev(np.ascontiguousarray(fo).reshape(nvec, 4, W)), idx_d, W, n,
-                            nvec, False)
+        mi = _modes_dev(fi.reshape(nvec, 4, W).contiguous(), idx_d, W, n, nvec, False)
+        if fo_all is not None:
+            fo = fo_all[..., i, :, :] if multi else fo_all
+            mo = _modes_dev(fo.reshape(nvec, 4, W).contiguous(), idx_d, W, n, nvec, False)
         else:
             mo = torch.zeros((nvec, n, 4), dtype=torch.complex128, device="cuda")
         fin = _dev(f_iso2d(W, kk, nin, betay, betamax), np.complex128)
         fout = _dev(f_iso2d(W, kk, nout, betay, betamax), np.complex128)
         cm = _stack_one(layers, kk, idx, betay, method)
         rm = _reflection_dev(_system_dev(fin, cm, fout, n), n)
-        _reflect_dev(mi, fin, rm, fout, mo, field_out is not None, n, nvec)
-        outs.append(_modes_dev(None, idx_d, W, n, nvec, True, mo))
-        backs.append(_modes_dev(None, idx_d, W, n, nvec, True, mi))
-        torch.cuda.synchronize()
-        outs[-1] = outs[-1].cpu().numpy().reshape(lead + (4, W))
-        backs[-1] = backs[-1].cpu().numpy().reshape(lead + (4, W))
-    out = np.stack(outs, -3) if multi else outs[0]
-    back = np.stack(backs, -3) if multi else backs[0]
+        _reflect_dev(mi, fin, rm, fout, mo, fo_all is not None, n, nvec)
+        outs.append(_modes_dev(None, idx_d, W, n, nvec, True, mo).reshape(lead + (4, W)))
+        backs.append(_modes_dev(None, idx_d, W, n, nvec, True, mi).reshape(lead + (4, W)))
+    out = (torch.stack(outs, -3) if multi else outs[0]).cpu().numpy()
+    back = (torch.stack(backs, -3) if multi else backs[0]).cpu().numpy()
     if isinstance(f, np.ndarray) and f.flags.writeable:
         f[...] = back
     if field_out is not None and isinstance(field_out, np.ndarray):

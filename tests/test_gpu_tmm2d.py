@@ -168,6 +168,16 @@ def test_transfer2d_thick_grating(m):
     _close(fg, fr, 1e-8)
 
 
+def test_stack_mat2d_many_layers(m):
+    """More layers than one launch takes (256): two chunks, odd tree levels."""
+    W, L = 16, 301
+    d, eps, ang = _grating(W, L, 13)
+    d = np.full(L, 0.05)
+    k = float(o.k0(520., 150.))
+    _close(m.stack_mat2d(k, d, eps, ang, betay=0.07), o.stack_mat2d(k, d, eps, ang, betay=0.07),
+           1e-8)
+
+
 def test_rejects_unknown_method(m):
     W = 16
     d, eps, ang = _grating(W, 1, 12)
