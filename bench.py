@@ -303,7 +303,10 @@ def run_pom(args):
                                "polarizer, crossed analyser, focus 3 px in n 1.5)"
                                % (H, W, nlam),
                    "height": H, "width": W, "wavelengths": nlam, "precision": args.precision},
-        "roofline": {"bound": "hbm", "kernel": "whole step (7 launches)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "whole step (3 launches: combination + forward x transforms, "
+                               "y transforms with the field diffraction operator, inverse x "
+                               "transforms + specter)",
                      "achieved": alg / (ms_per_step / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": alg / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": alg},

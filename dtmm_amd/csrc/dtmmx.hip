@@ -400,6 +400,15 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   if (nc == 2 && op == OP_INV) return launch_row<T, 2, OP_INV>(w, nr, a, nwork, s);
   if (nc == 4 && op == OP_FWD) return launch_row<T, 4, OP_FWD>(w, nr, a, nwork, s);
   if (nc == 4 && op == OP_INV) return launch_row<T, 4, OP_INV>(w, nr, a, nwork, s);
+  if (nc == 4 && op == OP_COMB) {
+    if (!a.csrc) return fail(DTMMX_E_ARG, "combination row pass without a source");
+    return launch_row<T, 4, OP_COMB>(w, nr, a, nwork, s);
+  }
+  if (nc == 4 && op == OP_SPEC) {
+    // every ray of the batch in one work-group: the specter sums over them
+    if (nr != b.nrays) return fail(DTMMX_E_ARG, "specter row pass needs all rays per work-group");
+    return launch_row<T, 4, OP_SPEC>(w, nr, a, nwork, s);
+  }
   return fail(DTMMX_E_ARG, "bad row pass");
 }
 
@@ -433,6 +442,15 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   const int h = a.H;
   if (op == OP_FWD) return launch_col<T, OP_FWD>(h, a, nlines, tune, s);
   if (op == OP_INV) return launch_col<T, OP_INV>(h, a, nlines, tune, s);
+  if (op == OP_FIELD) {
+    // a work-group takes the four components of a plane
+    if (nc != 4) return fail(DTMMX_E_ARG, "the field operator pass takes 4 components");
+    a.ncl = 4;
+    timer_start(1, s);
+    int r = launch_col<T, OP_FIELD>(h, a, nlines, false, s);
+    timer_stop(s);
+    return r;
+  }
   if (op == OP_DIAG) {
     timer_start(1, s);
     int r = launch_col<T, OP_DIAG>(h, a, nlines, tune, s);
@@ -2647,6 +2665,33 @@ int color_c128(const ColorJob& j, hipStream_t s);
 
 // Specter of dtmmx_pom_specter: (Jones combination) -> FFT -> F P F^-1 of
 // the medium (with 1/HW) -> IFFT -> analyser and Poynting per pixel.
+// The field diffraction F P(d) F^-1 of an isotropic medium eps on the x
+// spectrum of 4-component planes (after the forward row pass, before the
+// inverse one): one fused column pass for a real medium with eps > betamax^2
+// (every passing mode propagates), else forward column pass, the dense
+// k-space operator and the inverse column pass.
+template <typename T>
+static int field_diffraction(const dtmmx_plan* P, cpx<T>* f, Batch b, cd eps, double d,
+                             int fmode, double scale, hipStream_t s) {
+  if (eps.y == 0. && eps.x > P->d.betamax * P->d.betamax * (1. + 1e-9)) {
+    ColArgs<T> ca{};
+    ca.da = d;
+    ca.ea = eps;
+    ca.scale = scale;
+    ca.fmode = fmode;
+    return col_pass<T>(P, f, b, 4, OP_FIELD, s, &ca);
+  }
+  TRY(col_pass<T>(P, f, b, 4, OP_FWD, s));
+  KArgs<T> k{};
+  k.kop = KOP_FIELD4;
+  k.d0 = d;
+  k.e0 = eps;
+  k.scale = scale;
+  k.fmode = fmode;
+  TRY(kspace<T>(P, f, f, b, k, s));
+  return col_pass<T>(P, f, b, 4, OP_INV, s);
+}
+
 template <typename T>
 static int pom_impl(const PomJob& j, hipStream_t s) {
   const dtmmx_pom_desc* d = j.d;
@@ -2673,15 +2718,6 @@ static int pom_impl(const PomJob& j, hipStream_t s) {
   }
   cpx<T>* F = (cpx<T>*)j.wfield;
   const size_t nplanes = size_t(nf) * nlam * 4;
-  if (d->n_pol == 2) {
-    a.src = (const cpx<T>*)j.field;
-    a.dst = F;
-    hipLaunchKernelGGL((pom_combine_kernel<T>), dim3(gx, unsigned(nplanes)), dim3(NT), 0, s, a);
-    HIP_TRY(hipGetLastError());
-  } else {
-    HIP_TRY(hipMemcpyAsync(F, j.field, nplanes * size_t(HW) * sizeof(cpx<T>),
-                           hipMemcpyDeviceToDevice, s));
-  }
   dtmmx_plan P{};
   P.d.height = H;
   P.d.width = W;
@@ -2694,16 +2730,33 @@ static int pom_impl(const PomJob& j, hipStream_t s) {
   P.bm.rays_per_beam = nf;  // one (unused) beam for all fields
   for (int l = 0; l < nlam; ++l) P.bm.k0[l] = d->k0[l];
   Batch b{nf, nlam, 0, nlam};
-  TRY(row_pass<T>(&P, F, b, 4, OP_FWD, s));
-  TRY(col_pass<T>(&P, F, b, 4, OP_FWD, s));
-  KArgs<T> k{};
-  k.kop = KOP_FIELD4;
-  k.d0 = d->d;
-  k.e0 = mk<double>(d->eps_re, d->eps_im);
-  k.scale = 1.0 / double(HW);
-  k.fmode = d->mode;
-  TRY(kspace<T>(&P, F, F, b, k, s));
-  TRY(col_pass<T>(&P, F, b, 4, OP_INV, s));
+  if (d->n_pol == 2) {
+    // the Jones combination of the two polarised fields, read on the way
+    // into the forward x transforms
+    RowArgs<T> rc{};
+    rc.csrc = (const cpx<T>*)j.field;
+    rc.cjc = a.jc;
+    rc.cjs = a.js;
+    TRY(row_pass<T>(&P, F, b, 4, OP_COMB, s, &rc));
+  } else {
+    HIP_TRY(hipMemcpyAsync(F, j.field, nplanes * size_t(HW) * sizeof(cpx<T>),
+                           hipMemcpyDeviceToDevice, s));
+    TRY(row_pass<T>(&P, F, b, 4, OP_FWD, s));
+  }
+  TRY(field_diffraction<T>(&P, F, b, mk<double>(d->eps_re, d->eps_im), d->d, d->mode,
+                           1.0 / double(HW), s));
+  // the rays a row work-group takes (row_pass: two complex64 rays up to 512
+  // points when the count is even)
+  const int nr_row = (sizeof(T) == 4 && W <= 512 && nf % 2 == 0) ? 2 : 1;
+  if (nf == nr_row) {
+    // inverse x transforms with the specter of each pixel in the same pass
+    RowArgs<T> ra{};
+    ra.spec = (T*)j.spec;
+    ra.spec_npm = a.npm;
+    ra.spec_pmat = a.pmat;
+    for (int i = 0; i < 16; ++i) ra.spec_pm0[i] = a.pm0[i];
+    return row_pass<T>(&P, F, b, 4, OP_SPEC, s, &ra);
+  }
   TRY(row_pass<T>(&P, F, b, 4, OP_INV, s));
   a.src = F;
   a.npol = 1;
@@ -2799,16 +2852,13 @@ static int illum_impl(const dtmmx_plan* P, const dtmmx_illum_desc* d, cpx<T>* fi
   BeamValues iv{};
   for (int b = 0; b < nbeams; ++b) iv.v[b] = d->intensity[b];
   if (d->diffraction) {
-    TRY(fft2_planes<T>(P, field, 0, s));
-    KArgs<T> k{};
-    k.kop = KOP_FIELD4;
-    k.fmode = d->mode > 0 ? 1 : -1;
-    k.d0 = -d->focus;
-    k.e0 = mk<double>(d->n * d->n);
-    k.scale = 1.0 / double(HW);
+    // x transforms; y transforms with the field diffraction operator of the
+    // cover medium between them; inverse x transforms
     const Batch b{nrays, nlam, 0, nlam};
-    TRY(kspace<T>(P, field, field, b, k, s));
-    TRY(fft2_planes<T>(P, field, 1, s));
+    TRY(row_pass<T>(P, field, b, 4, OP_FWD, s));
+    TRY(field_diffraction<T>(P, field, b, mk<double>(d->n * d->n), -d->focus,
+                             d->mode > 0 ? 1 : -1, 1.0 / double(HW), s));
+    TRY(row_pass<T>(P, field, b, 4, OP_INV, s));
     IllumArgs<T> a2 = a;
     a2.part = part2;
     hipLaunchKernelGGL((illum_kernel<T, ILL_POWER>), grid, dim3(NT), 0, s, a2);

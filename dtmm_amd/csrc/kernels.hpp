@@ -39,7 +39,8 @@ __host__ __device__ inline int mode_index(int i, int n) {  // |k| of fftfreq
 // Launch geometry shared by the per-layer kernels.
 // ---------------------------------------------------------------------------
 
-enum { OP_FWD = 0, OP_INV = 1, OP_ROP = 2, OP_DIAG = 3, OP_WIN = 4, OP_SPLIT = 5, OP_ROPW = 6 };
+enum { OP_FWD = 0, OP_INV = 1, OP_ROP = 2, OP_DIAG = 3, OP_WIN = 4, OP_SPLIT = 5, OP_ROPW = 6,
+       OP_FIELD = 7, OP_SPEC = 8, OP_COMB = 9 };
 
 // Work-item index for block b of n such that consecutive work items land on
 // the same XCD (blocks are dispatched round-robin over the 8 XCDs): work
@@ -79,6 +80,21 @@ constexpr int EIG_COMPS = 13;  // per-pixel operator data: alpha0..3, P0, P1, ok
 #define DTMMX_COL_PADW 0
 #endif
 
+// Poynting z of one (Ex, Hy, Ey, Hx) sample (dtmm/field.py:771-799).
+__host__ __device__ inline double intensity4(const cd v[4]) {
+  return (v[0].x * v[1].x + v[0].y * v[1].y) - (v[2].x * v[3].x + v[2].y * v[3].y);
+}
+
+__device__ __forceinline__ void mat4v(const cd* m, const cd v[4], cd w[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    cd t = mk<double>(0.);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t = t + m[i * 4 + j] * v[j];
+    w[i] = t;
+  }
+}
+
 template <typename T>
 struct RowArgs {
   cpx<T>* field;
@@ -117,6 +133,18 @@ struct RowArgs {
   const cpx<T>* etab;
   int etab_nbeam;
   cpx<T>* etab_ws;
+  // OP_SPEC (the polarizing-microscope specter after the inverse x
+  // transforms): specter [H*W][nlam] (the sum over the work-group's rays of
+  // the Poynting z of the analysed field), the analyser: none (npm 0), the
+  // shared matrix pm0 (npm 1) or per wavelength pmat [nlam][4][4] (npm > 1)
+  T* spec;
+  int spec_npm;
+  const cd* spec_pmat;
+  cd spec_pm0[16];
+  // OP_COMB (forward x transforms of the Jones combination of two polarised
+  // fields): source [ray][2][nlam][NC][H][W] and the weights of x and y
+  const cpx<T>* csrc;
+  cd cjc, cjs;
   Beams bm;
 };
 
@@ -152,7 +180,7 @@ __host__ __device__ constexpr int row_min_waves() {
 // diagonal factor, the line and the prefetched line without spilling.
 template <typename T, int N, int OP>
 __host__ __device__ constexpr int col_min_waves() {
-  return OP == OP_SPLIT ? (sizeof(T) == 4 ? 2 : 1)
+  return (OP == OP_SPLIT || OP == OP_FIELD) ? (sizeof(T) == 4 ? 2 : 1)
                         : ((N >= 1024 && sizeof(T) == 4) ? 3 : min_waves<T>());
 }
 
@@ -164,8 +192,9 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   constexpr int LINES = NR * NC;
   constexpr int NTW = NL * LINES;
   constexpr int LS = line_stride<N>();
-  constexpr bool REV1 = (OP != OP_FWD);
-  constexpr int DIR1 = (OP == OP_FWD) ? -1 : +1;
+  constexpr bool FWDX = (OP == OP_FWD || OP == OP_COMB);  // forward x transforms
+  constexpr bool REV1 = !FWDX;
+  constexpr int DIR1 = FWDX ? -1 : +1;
   constexpr int MV1 = max_vals<N, REV1, NL>();
   constexpr int MV2 = max_vals<N, false, NL>();
   constexpr int MAXV = MV1 > MV2 ? MV1 : MV2;
@@ -226,11 +255,22 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   // prefetched inputs of the next wavelength; the first one is requested
   // before the eigen data so that its latency overlaps the fp64 solve
   cpx<T> nxt[PF ? NIN : 1];
+  // OP_COMB: the lines of the x and y polarised fields [ray][2][nlam][NC][H][W]
+  // are read and combined (jc x + js y, fp64) on their way in
+  cpx<T> nxt2[OP == OP_COMB ? NIN : 1];
+  auto pol_ptr = [&](int jl) {
+    const size_t q = size_t(rg * NR + rr) * 2 * A.b.nlam + A.b.lam0 + jl;
+    return A.csrc + (q * NC + comp) * HW + size_t(y) * N;
+  };
+  const size_t ypol = size_t(A.b.nlam) * NC * HW;
   if constexpr (PF) {
-    const cpx<T>* g0 = line_ptr(0);
+    const cpx<T>* g0 = (OP == OP_COMB) ? pol_ptr(0) : line_ptr(0);
 #pragma unroll
     for (int j = 0; j < NIN; ++j)
-      if (slot_valid<N, REV1, NL, false>(j, lane)) nxt[j] = g0[slot_pos<N, REV1, NL, false>(j, lane)];
+      if (slot_valid<N, REV1, NL, false>(j, lane)) {
+        nxt[j] = g0[slot_pos<N, REV1, NL, false>(j, lane)];
+        if constexpr (OP == OP_COMB) nxt2[j] = g0[ypol + slot_pos<N, REV1, NL, false>(j, lane)];
+      }
   }
   // per-pixel operator data of this row, solved in fp64 once and reused for
   // every wavelength and ray of the work-group
@@ -333,13 +373,20 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
     cpx<T> v[MAXV];
     if constexpr (PF) {
 #pragma unroll
-      for (int j = 0; j < NIN; ++j) v[j] = nxt[j];
+      for (int j = 0; j < NIN; ++j) {
+        if constexpr (OP == OP_COMB)
+          v[j] = cast<T>(A.cjc * cast<double>(nxt[j]) + A.cjs * cast<double>(nxt2[j]));
+        else
+          v[j] = nxt[j];
+      }
       if (jl + 1 < A.b.nlam_b) {
-        const cpx<T>* gn = line_ptr(jl + 1);
+        const cpx<T>* gn = (OP == OP_COMB) ? pol_ptr(jl + 1) : line_ptr(jl + 1);
 #pragma unroll
         for (int j = 0; j < NIN; ++j)
-          if (slot_valid<N, REV1, NL, false>(j, lane))
+          if (slot_valid<N, REV1, NL, false>(j, lane)) {
             nxt[j] = gn[slot_pos<N, REV1, NL, false>(j, lane)];
+            if constexpr (OP == OP_COMB) nxt2[j] = gn[ypol + slot_pos<N, REV1, NL, false>(j, lane)];
+          }
       }
     } else {
 #pragma unroll
@@ -367,6 +414,43 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
           v[j] = v[j] * (cr * T(cxl[slot_pos<N, REV1, NL, false>(j, lane)]));
     }
     reg_fft<T, N, REV1, DIR1, NL, 1, 1, MAXV, WV>(v, line_lds, tws, lane);
+
+    if constexpr (OP == OP_SPEC) {
+      // the real-space lines of the rays into LDS, then per pixel the sum
+      // over rays of the Poynting z of the analysed field; no field store
+#pragma unroll
+      for (int j = 0; j < NMID; ++j)
+        if (slot_valid<N, REV1, NL, true>(j, lane))
+          line_lds[lds_at<1, DTMMX_ROP_PADW>(slot_pos<N, REV1, NL, true>(j, lane))] = v[j];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < PPX; ++k) {
+        const int x = tid + k * NTW;
+        if (x < N) {
+          const int ax = lds_at<1, DTMMX_ROP_PADW>(x);
+          double acc = 0.;
+          for (int r = 0; r < NR; ++r) {
+            const cpx<T>* L = lds + (r * 4) * LS + ax;
+            const cd u[4] = {cast<double>(L[0]), cast<double>(L[LS]), cast<double>(L[2 * LS]),
+                             cast<double>(L[3 * LS])};
+            if (A.spec_npm == 0) {
+              acc += intensity4(u);
+            } else if (A.spec_npm == 1) {
+              cd w[4];
+              mat4v(A.spec_pm0, u, w);
+              acc += intensity4(w);
+            } else {
+              cd w[4];
+              mat4v(A.spec_pmat + size_t(lam) * 16, u, w);
+              acc += intensity4(w);
+            }
+          }
+          A.spec[(size_t(y) * N + x) * A.b.nlam + lam] = T(acc);
+        }
+      }
+      __syncthreads();
+      continue;
+    }
 
     if constexpr (ROPX) {
 #pragma unroll
@@ -1012,8 +1096,94 @@ struct ColArgs {
   const double* wcx;
   const double* wcy;
   int wn, wi, wj;
+  // OP_FIELD: the field diffraction operator of the isotropic medium ea over
+  // the thickness da (0 both directions, +1 forward modes, -1 backward)
+  int fmode;
   Beams bm;
 };
+
+// F P(kd) F^-1 of an isotropic medium e0 in closed form at in-plane wave
+// vector beta with direction (c, sn) (dtmm/diffract.py:107-119
+// field_diffraction_matrix, phase_matrix :96-105): the mode frame g = R^T v,
+// amplitudes m = F^-1 g, phases of the kept directions (fmode 0 both, +1
+// forward only, -1 backward only), back to the field; zero where the mode
+// does not exist.
+__device__ inline void field4_apply(cd e0, double kd, int fmode, double beta, double c, double sn,
+                                    const cd v[4], cd r[4]) {
+  for (int i = 0; i < 4; ++i) r[i] = mk<double>(0.);
+  const cd a = csqrt(e0 - mk<double>(beta * beta));
+  if (is_zero(a)) return;
+  const cd igp = cdiv(a, e0), igs = cdiv(mk<double>(-1.), a);
+  const cd g0 = v[0] * c + v[2] * sn, g1 = v[1] * c - v[3] * sn;
+  const cd g2 = v[2] * c - v[0] * sn, g3 = v[1] * sn + v[3] * c;
+  const cd zero = mk<double>(0.);
+  const cd p = fmode < 0 ? zero : cexpi(a * kd);
+  const cd pi = fmode > 0 ? zero : cexpi(a * -kd);
+  const cd m0 = (g0 + g1 * igp) * p, m1 = (g0 - g1 * igp) * pi;
+  const cd m2 = (g2 + g3 * igs) * p, m3 = (g2 - g3 * igs) * pi;
+  const cd gp = cdiv(e0, a), gs = -a;
+  const cd h0 = (m0 + m1) * 0.5, h1 = gp * (m0 - m1) * 0.5;
+  const cd h2 = (m2 + m3) * 0.5, h3 = gs * (m2 - m3) * 0.5;
+  r[0] = h0 * c - h2 * sn;
+  r[1] = h1 * c + h3 * sn;
+  r[2] = h0 * sn + h2 * c;
+  r[3] = h3 * c - h1 * sn;
+}
+
+// field4_apply for a real medium eps (eps > beta^2 wherever the betamax mask
+// passes, so a = sqrt(eps - beta^2) and the mode-frame factors are real and
+// exp(-i a kd) = conj(exp(i a kd))), in the field's precision and scaled:
+// the column pass OP_FIELD.
+template <typename T>
+struct FieldConst {
+  T e, ie, kd, l2, scale;  // eps, 1 / eps, k0 d, (2 pi / k0)^2, scale
+  bool fwd, bwd;
+};
+
+template <typename T>
+__device__ __forceinline__ FieldConst<T> field_const(double eps, double k0, double d, int fmode,
+                                                     double scale) {
+  FieldConst<T> c;
+  c.e = T(eps);
+  c.ie = T(1. / eps);
+  c.kd = T(k0 * d);
+  const double l = 2 * M_PI / k0;
+  c.l2 = T(l * l);
+  c.scale = T(scale);
+  c.fwd = fmode >= 0;
+  c.bwd = fmode <= 0;
+  return c;
+}
+
+template <typename T>
+__device__ __forceinline__ void field4_real(const FieldConst<T>& K, bool ok, T fx, T fy,
+                                            cpx<T> v0, cpx<T> v1, cpx<T> v2, cpx<T> v3,
+                                            cpx<T> r[4]) {
+  const T f2 = fx * fx + fy * fy;
+  const T a2 = K.e - K.l2 * f2;
+  if (!ok || !(a2 > T(0))) {
+    r[0] = r[1] = r[2] = r[3] = mk<T>(T(0), T(0));
+    return;
+  }
+  const T fr = sqrt(f2);
+  const T c = fr > T(0) ? fx / fr : T(1), sn = fr > T(0) ? fy / fr : T(0);
+  const T a = sqrt(a2), ia = T(1) / a;
+  const T igp = a * K.ie, gp = K.e * ia;
+  const cpx<T> g0 = v0 * c + v2 * sn, g1 = v1 * c - v3 * sn;
+  const cpx<T> g2 = v2 * c - v0 * sn, g3 = v1 * sn + v3 * c;
+  const cpx<T> ph = cexpi_layer(mk<T>(a * K.kd, T(0)));
+  const cpx<T> zero = mk<T>(T(0), T(0));
+  const cpx<T> p = K.fwd ? ph : zero, pi = K.bwd ? conj(ph) : zero;
+  const cpx<T> m0 = (g0 + g1 * igp) * p, m1 = (g0 - g1 * igp) * pi;
+  const cpx<T> m2 = (g2 - g3 * ia) * p, m3 = (g2 + g3 * ia) * pi;  // igs = -1 / a
+  const T hs = T(0.5) * K.scale;
+  const cpx<T> h0 = (m0 + m1) * hs, h1 = (m0 - m1) * (gp * hs);
+  const cpx<T> h2 = (m2 + m3) * hs, h3 = (m2 - m3) * (-a * hs);  // gs = -a
+  r[0] = h0 * c - h2 * sn;
+  r[1] = h1 * c + h3 * sn;
+  r[2] = h0 * sn + h2 * c;
+  r[3] = h3 * c - h1 * sn;
+}
 
 // A value known to be equal in every lane, moved to a scalar register.
 __device__ __forceinline__ float uniform(float v) {
@@ -1140,8 +1310,8 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
   const bool xin = x < A.W;
   const size_t HW = size_t(N) * A.W;
   constexpr bool KOP = (OP == OP_DIAG || OP == OP_WIN || OP == OP_SPLIT);
-  int am[KOP ? NMID : 1];  // largest passing |kx| of each slot's ky row
-  if constexpr (KOP) {
+  int am[(KOP || OP == OP_FIELD) ? NMID : 1];  // largest passing |kx| of each slot's ky row
+  if constexpr (KOP || OP == OP_FIELD) {
 #pragma unroll
     for (int j = 0; j < NMID; ++j)
       am[j] = slot_valid<N, REV1, NL, true>(j, g) ? A.amax[lam * A.H + slot_pos<N, REV1, NL, true>(j, g)] : -1;
@@ -1206,6 +1376,47 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
   // visible to the other waves after the first exchange barrier of the
   // first transform, before the first twiddle read (stage 1)
   twf.store(tws, tid);
+  if constexpr (OP == OP_FIELD) {
+    // the four components of a plane (lines li .. li + 3): forward y
+    // transforms, the field diffraction operator per k, inverse transforms
+    const FieldConst<T> fc = field_const<T>(A.ea.x, A.bm.k0[lam], A.da, A.fmode, A.scale);
+    const T fxt = T(fftfreq(x, A.W));
+    const int mx = mode_index(x, A.W);
+    for (int l4 = 0; l4 < A.ncl; l4 += 4) {
+      cpx<T> vv[4][MAXV];
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        const cpx<T>* bl = line_base(l4 + cc);
+#pragma unroll
+        for (int j = 0; j < MAXV; ++j) vv[cc][j] = mk<T>(T(0), T(0));
+#pragma unroll
+        for (int j = 0; j < NIN; ++j)
+          if (xin && slot_valid<N, false, NL, false>(j, g))
+            vv[cc][j] = bl[size_t(slot_pos<N, false, NL, false>(j, g)) * A.W];
+        reg_fft<T, N, false, -1, NL, TX, DTMMX_COL_PADW * TX>(vv[cc], lds + c, tws, g);
+      }
+#pragma unroll
+      for (int j = 0; j < NMID; ++j) {
+        const bool ok = xin && slot_valid<N, false, NL, true>(j, g) && mx <= am[j];
+        const T fy = T(fftfreq(slot_pos<N, false, NL, true>(j, g), A.H));
+        cpx<T> r[4];
+        field4_real<T>(fc, ok, fxt, fy, vv[0][j], vv[1][j], vv[2][j], vv[3][j], r);
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) vv[cc][j] = r[cc];
+      }
+      constexpr int NOUT = slot_count<N, true, NL, true>();
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        reg_fft<T, N, true, +1, NL, TX, DTMMX_COL_PADW * TX>(vv[cc], lds + c, tws, g);
+        cpx<T>* ob = line_base(l4 + cc);
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j)
+          if (xin && slot_valid<N, true, NL, true>(j, g))
+            ob[size_t(slot_pos<N, true, NL, true>(j, g)) * A.W] = vv[cc][j];
+      }
+    }
+    return;
+  }
   for (int li = 0; li < A.ncl; ++li) {
     const int comp = (lg * A.ncl + li) % A.NC;
     cpx<T>* base = (OP == OP_WIN || OP == OP_SPLIT) ? A.out + line_off(li) : line_base(li);
@@ -1486,29 +1697,7 @@ __device__ __forceinline__ void kop_apply(const KArgs<T>& A, bool pass, double b
     r[1] = K * (ep * sn + es * c);
   } else if constexpr (KOP == KOP_FIELD4) {
     if (!pass) return;
-    // F P(kd) F^-1 of an isotropic medium in closed form
-    // (dtmm/diffract.py:107-119 field_diffraction_matrix, phase_matrix
-    // :96-105): mode frame g = R^T v, amplitudes m = F^-1 g, phases of the
-    // kept directions, back to the field.
-    const cd a = csqrt(A.e0 - mk<double>(beta * beta));
-    if (!is_zero(a)) {
-      const double kd = k0 * A.d0;
-      const cd igp = cdiv(a, A.e0), igs = cdiv(mk<double>(-1.), a);
-      const cd g0 = v[0] * c + v[2] * sn, g1 = v[1] * c - v[3] * sn;
-      const cd g2 = v[2] * c - v[0] * sn, g3 = v[1] * sn + v[3] * c;
-      const cd zero = mk<double>(0.);
-      const cd p = A.fmode < 0 ? zero : cexpi(a * kd);
-      const cd pi = A.fmode > 0 ? zero : cexpi(a * -kd);
-      const cd m0 = (g0 + g1 * igp) * p, m1 = (g0 - g1 * igp) * pi;
-      const cd m2 = (g2 + g3 * igs) * p, m3 = (g2 - g3 * igs) * pi;
-      const cd gp = cdiv(A.e0, a), gs = -a;
-      const cd h0 = (m0 + m1) * 0.5, h1 = gp * (m0 - m1) * 0.5;
-      const cd h2 = (m2 + m3) * 0.5, h3 = gs * (m2 - m3) * 0.5;
-      r[0] = h0 * c - h2 * sn;
-      r[1] = h1 * c + h3 * sn;
-      r[2] = h0 * sn + h2 * c;
-      r[3] = h3 * c - h1 * sn;
-    }
+    field4_apply(A.e0, k0 * A.d0, A.fmode, beta, c, sn, v, r);
   } else if constexpr (KOP == KOP_E2H4) {
     // E components kept as they are (at every k), H of the forward modes of
     // the isotropic medium e0 from them, zero above betamax
@@ -1846,10 +2035,6 @@ __global__ __launch_bounds__(NT) void trr2_kernel(TrrArgs<T> A) {
 // dtmm/field.py:800-845) and its colour conversion (dtmm/color.py:52-200).
 // ---------------------------------------------------------------------------
 
-// Poynting z of one (Ex, Hy, Ey, Hx) sample (dtmm/field.py:771-799).
-__host__ __device__ inline double intensity4(const cd v[4]) {
-  return (v[0].x * v[1].x + v[0].y * v[1].y) - (v[2].x * v[3].x + v[2].y * v[3].y);
-}
 
 template <typename T>
 struct PomArgs {
@@ -1864,27 +2049,6 @@ struct PomArgs {
   cd pm0[16];         // the shared analyser matrix (npm == 1)
 };
 
-__device__ __forceinline__ void mat4v(const cd* m, const cd v[4], cd w[4]) {
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    cd t = mk<double>(0.);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) t = t + m[i * 4 + j] * v[j];
-    w[i] = t;
-  }
-}
-
-// One thread per (pixel, plane): dst = jc * field_x + js * field_y.
-template <typename T>
-__global__ __launch_bounds__(NT) void pom_combine_kernel(PomArgs<T> A) {
-  const long long p = (long long)blockIdx.x * NT + threadIdx.x;
-  if (p >= A.HW) return;
-  const int q = blockIdx.y;  // (field, wavelength, component)
-  const int f = q / (A.nlam * 4), r = q - f * A.nlam * 4;
-  const cpx<T>* s = A.src + (size_t(f) * 2 * A.nlam * 4 + r) * A.HW + p;
-  const cd x = cast<double>(s[0]), y = cast<double>(s[size_t(A.nlam) * 4 * A.HW]);
-  A.dst[size_t(q) * A.HW + p] = cast<T>(A.jc * x + A.js * y);
-}
 
 // One thread per pixel: for every wavelength, the sum over fields of the
 // Poynting z of (analyser matrix) . (field), written with the wavelength

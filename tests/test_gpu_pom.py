@@ -29,6 +29,9 @@ CASES = [
     ((1, 2), (1., 0.), "per_lambda", True, +1, 2.0, 1.5),
     ((2,), None, None, True, -1, 4.0, 1.2),
     ((2, 2), (0.3 - 0.1j, 0.9), "shared", False, None, 0., 1.0),
+    # a medium with eps below betamax^2: evanescent modes pass the mask, so
+    # the unfused k-space operator runs instead of the fused column pass
+    ((2,), None, None, True, None, 2.0, 0.75),
 ]
 
 
