@@ -928,6 +928,17 @@ def measure(args, method, fd, optical, c5):
                      "avg_launch_ms": avg_ms, "bytes_per_launch": kern[dom]["bytes"]},
         "cpu_baseline": cpu,
     }
+    # the fused FFT-diffraction kernel (forward y transforms, the k-space
+    # diffraction operator, inverse y transforms) beside the dominant one
+    ck = kern["col_kspace"]
+    if ck["n"]:
+        c_ms = ck["ms"] / ck["n"]
+        c_ach = ck["bytes"] / (c_ms / 1e3) / 1e9
+        rec["roofline_fft_diffraction"] = {
+            "bound": "hbm", "kernel": "col_kspace", "achieved": c_ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": c_ach / HBM_PEAK_GBS,
+            "traffic": MEASURED_TRAFFIC.get(("col_kspace", H, W, planes, method, args.precision)),
+            "avg_launch_ms": c_ms, "bytes_per_launch": ck["bytes"]}
     rec.update(extra)
     return rec
 
