@@ -759,10 +759,12 @@ def measure(args, method, fd, optical, c5):
         from dtmm_amd import tune
         t = time.time()
         res = tune.tune_plan(plan, stack, fin, fout, ws, repeats=2, cache=False)
-        tiles = {"col_lines": res["tiles"][0], "row_rays": res["tiles"][1]}
-        log("%s: tuned launch shapes in %.1f s: %s (ms per pass %s)"
-            % (method, time.time() - t, res["tiles"],
-               ", ".join("%s %.2f" % (k, v) for k, v in sorted(res["ms"].items()))))
+        tiles = {"col_lines": res["tiles"][0], "row_rays": res["tiles"][1],
+                 "row_split": res["row_split"]}
+        log("%s: tuned launch shapes in %.1f s: %s split %d (ms per pass %s; splits %s)"
+            % (method, time.time() - t, res["tiles"], res["row_split"],
+               ", ".join("%s %.2f" % (k, v) for k, v in sorted(res["ms"].items())),
+               ", ".join("%s %.2f" % (k, v) for k, v in sorted(res["split_ms"].items()))))
     # npass > 1 overwrites the input with the input-side field: every step
     # starts from a fresh copy of the illumination
     fin0 = fin.clone() if npass > 1 else None

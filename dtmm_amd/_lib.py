@@ -34,6 +34,8 @@ EXPORTS = (
     "dtmmx_transfer_workspace_size",
     "dtmmx_plan_set_tiles",
     "dtmmx_plan_get_tiles",
+    "dtmmx_plan_set_row_split",
+    "dtmmx_plan_get_row_split",
     "dtmmx_transfer",
     "dtmmx_transfer_bulk",
     "dtmmx_fft2",
@@ -275,6 +277,8 @@ def load():
     lib.dtmmx_plan_set_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.dtmmx_plan_get_tiles.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int)]
+    lib.dtmmx_plan_set_row_split.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.dtmmx_plan_get_row_split.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     lib.dtmmx_transfer.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stack), ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                    ctypes.c_void_p, ctypes.c_void_p]

@@ -169,6 +169,19 @@ static int launch_row(int n, int nr, const RowArgs<T>& a, int nwork, hipStream_t
   return DTMMX_OK;
 }
 
+// Compute units of the current device (cached per process; 256 if unknown).
+static int device_cus() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0, x = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&x, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || x <= 0)
+      x = 256;
+    v = x;
+  }
+  return v;
+}
+
 // Fused window row pass: NR rays of one beam per work-group (two for
 // complex64 lines up to 512 points) and a chunk of wavelengths whose
 // per-window sums fill the LDS beside the kernel's own buffers.  Returns
@@ -335,6 +348,9 @@ struct dtmmx_plan {
   // row rays per work-group (dtmmx_plan_set_tiles, the autotuner's knobs)
   int col_lines = 0;
   int row_rays = 0;
+  // row work-groups per (row, ray group), each taking a chunk of the
+  // wavelengths (dtmmx_plan_set_row_split; 0: chosen from the grid)
+  int row_split = 0;
 };
 
 namespace dtmmx {
@@ -357,8 +373,20 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
       ((op != OP_ROP && op != OP_ROPW) || P->bm.rays_per_beam % 2 == 0) && P->row_rays != 1)
     nr = 2;
   a.rays_per_group = nr;
-  const int nwork = a.H * (b.nrays / nr);
   const int w = a.W;
+  // wavelength chunks per (row, ray group): the plan's setting, else enough
+  // work-groups to give every SIMD of the device the row kernel's occupancy
+  // (4 waves of complex64, 2 of complex128), at most 8 and at most the
+  // batch's wavelengths
+  int split = P->row_split;
+  if (split <= 0) {
+    const long long waves = (long long)a.H * (b.nrays / nr) * nr * nc * (w >= 1024 ? 2 : 1);
+    const long long want = (long long)device_cus() * 4 * (sizeof(T) == 4 ? 4 : 2);
+    split = int((want + waves - 1) / (waves > 0 ? waves : 1));
+  }
+  split = std::max(1, std::min({split, 8, std::max(1, b.nlam_b)}));
+  a.lam_split = split;
+  const int nwork = a.H * (b.nrays / nr) * split;
   if ((op == OP_ROP || op == OP_ROPW) && a.etab_ws) {
     // layer with biaxial voxels: per-pixel data for this pass's beams first
     const size_t HW = size_t(a.H) * a.W;
@@ -3519,6 +3547,19 @@ int dtmmx_plan_get_tiles(const dtmmx_plan* P, int* col_lines, int* row_rays) {
   if (!P || !col_lines || !row_rays) return fail(DTMMX_E_ARG, "null argument");
   *col_lines = P->col_lines;
   *row_rays = P->row_rays;
+  return DTMMX_OK;
+}
+
+int dtmmx_plan_set_row_split(dtmmx_plan* P, int row_split) {
+  if (!P) return fail(DTMMX_E_ARG, "null plan");
+  if (row_split < 0 || row_split > 8) return fail(DTMMX_E_ARG, "row_split must be 0 to 8");
+  P->row_split = row_split;
+  return DTMMX_OK;
+}
+
+int dtmmx_plan_get_row_split(const dtmmx_plan* P, int* row_split) {
+  if (!P || !row_split) return fail(DTMMX_E_ARG, "null argument");
+  *row_split = P->row_split;
   return DTMMX_OK;
 }
 

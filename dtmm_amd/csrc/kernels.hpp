@@ -45,6 +45,21 @@ enum { OP_FWD = 0, OP_INV = 1, OP_ROP = 2, OP_DIAG = 3, OP_WIN = 4, OP_SPLIT = 5
 // Work-item index for block b of n such that consecutive work items land on
 // the same XCD (blocks are dispatched round-robin over the 8 XCDs): work
 // that shares data (the planes of one row) then shares one L2.
+// Marks the prefetched values v[0..n) as needed here: the compiler waits for
+// their loads at this point (called on every line, also where nothing was
+// prefetched, so that no path reaches the stores with loads pending).  The per-line kernels call it before storing a
+// line, because the counter of outstanding vector memory operations also
+// counts stores and drains in issue order: a wait for the prefetch placed
+// after the stores (where the compiler would put it, at the loop's end)
+// would also wait for the stores to be acknowledged, exposing their latency
+// on every line.  The memory clobber keeps the stores after the wait.
+template <int NV, typename V>
+__device__ __forceinline__ void complete_loads(const V (&v)[NV], int n) {
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    if (j < n) asm volatile("" ::"v"(v[j].x), "v"(v[j].y) : "memory");
+}
+
 __device__ __forceinline__ int xcd_swizzle(int b, int n) {
   const int x = b & 7, k = b >> 3, q = n >> 3, r = n & 7;
   return x * q + (x < r ? x : r) + k;
@@ -146,6 +161,12 @@ struct RowArgs {
   const cpx<T>* csrc;
   cd cjc, cjs;
   Beams bm;
+  // work-groups per (row, ray group): each takes a contiguous chunk of the
+  // batch's wavelengths (values below 2: one work-group takes them all).
+  // More chunks mean more work-groups in flight for grids that would
+  // otherwise leave SIMDs idle, at the price of one more per-pixel solve per
+  // chunk in the layer operator kernels.
+  int lam_split;
 };
 
 template <int N>
@@ -220,10 +241,19 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   else
     fill_tw_tables<T, N>(tws, A.tw, tid, NTW);
   const int rr = li / NC, comp = li - rr * NC;
-  const int w = xcd_swizzle(blockIdx.x, gridDim.x);
+  // consecutive work items (the wavelength chunks of one row first) land on
+  // one XCD, so the chunks of a row share its material in that XCD's L2
+  const int lsplit = A.lam_split > 1 ? A.lam_split : 1;
+  const int wsw = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int chunk = wsw % lsplit;
+  const int w = wsw / lsplit;
   const int ngroups = A.b.nrays / NR;
   const int rg = w % ngroups;
   const int y = w / ngroups;
+  // this work-group's wavelengths jb <= jl < je of the batch
+  const int lper = (A.b.nlam_b + lsplit - 1) / lsplit;
+  const int jb = min(chunk * lper, A.b.nlam_b);
+  const int je = min(jb + lper, A.b.nlam_b);
   const size_t HW = size_t(A.H) * N;
   cpx<T>* line_lds = lds + li * LS;
 
@@ -264,10 +294,10 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   };
   const size_t ypol = size_t(A.b.nlam) * NC * HW;
   if constexpr (PF) {
-    const cpx<T>* g0 = (OP == OP_COMB) ? pol_ptr(0) : line_ptr(0);
+    const cpx<T>* g0 = (OP == OP_COMB) ? pol_ptr(jb) : line_ptr(jb);
 #pragma unroll
     for (int j = 0; j < NIN; ++j)
-      if (slot_valid<N, REV1, NL, false>(j, lane)) {
+      if (jb < je && slot_valid<N, REV1, NL, false>(j, lane)) {
         nxt[j] = g0[slot_pos<N, REV1, NL, false>(j, lane)];
         if constexpr (OP == OP_COMB) nxt2[j] = g0[ypol + slot_pos<N, REV1, NL, false>(j, lane)];
       }
@@ -367,7 +397,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   __syncthreads();
 
   const int acc_mode = ROPX ? A.acc_mode : 0;
-  for (int jl = 0; jl < A.b.nlam_b; ++jl) {
+  for (int jl = jb; jl < je; ++jl) {
     const int lam = A.b.lam0 + jl;
     cpx<T>* gl = line_ptr(jl);
     cpx<T> v[MAXV];
@@ -379,7 +409,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
         else
           v[j] = nxt[j];
       }
-      if (jl + 1 < A.b.nlam_b) {
+      if (jl + 1 < je) {
         const cpx<T>* gn = (OP == OP_COMB) ? pol_ptr(jl + 1) : line_ptr(jl + 1);
 #pragma unroll
         for (int j = 0; j < NIN; ++j)
@@ -516,6 +546,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
       exchange_sync<WV>();
       reg_fft<T, N, false, -1, NL, 1, 1, MAXV, WV>(v, line_lds, tws, lane);
     }
+    if constexpr (PF) complete_loads(nxt, NIN);
     if (acc_mode == 0) {
 #pragma unroll
       for (int j = 0; j < NOUT; ++j)
@@ -1442,6 +1473,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
         v[j] = v[j] * k;
       }
       if constexpr (OP == OP_SPLIT) {
+        complete_loads(nxt, NIN);
         // one inverse transform per window row j: cy_j(ky) times the line
         constexpr int NOUT = slot_count<N, true, NL, true>();
         const size_t fstride = size_t(A.b.nrays) * A.b.nlam * A.NC * HW;
@@ -1468,6 +1500,7 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
     }
     constexpr bool REVL = KOP ? true : REV1;
     constexpr int NOUT = slot_count<N, REVL, NL, true>();
+    complete_loads(nxt, NIN);
 #pragma unroll
     for (int j = 0; j < NOUT; ++j)
       if (xin && slot_valid<N, REVL, NL, true>(j, g))

@@ -5,9 +5,12 @@ shares the k-space operator it computes once per work-group among them; the
 row kernel takes ``row_rays`` rays per work-group and shares the per-pixel
 fp64 solve among them.  More sharing means less arithmetic per line but
 fewer, longer work-groups, and which shape is fastest depends on the grid,
-the number of planes and the occupancy.  ``tune_plan`` runs the caller's own
-transfer once per candidate shape, times it with HIP events and keeps the
-fastest in the plan (``dtmmx_plan_set_tiles``).  The launch shape never
+the number of planes and the occupancy.  The row kernels can also split
+each row's wavelengths over ``row_split`` work-groups (more work-groups in
+flight, one more per-pixel solve per chunk).  ``tune_plan`` runs the
+caller's own transfer once per candidate shape, then once per split with
+the fastest shape, times each with HIP events and keeps the fastest in the
+plan (``dtmmx_plan_set_tiles``, ``dtmmx_plan_set_row_split``).  The launch shape never
 changes the result.  Winners are cached per workload signature for the
 life of the process.
 """
@@ -44,23 +47,41 @@ def set_tiles(plan, col_lines, row_rays):
     _lib.check(plan._lib.dtmmx_plan_set_tiles(plan._h, int(col_lines), int(row_rays)))
 
 
+def get_row_split(plan):
+    v = ctypes.c_int()
+    _lib.check(plan._lib.dtmmx_plan_get_row_split(plan._h, ctypes.byref(v)))
+    return v.value
+
+
+def set_row_split(plan, row_split):
+    _lib.check(plan._lib.dtmmx_plan_set_row_split(plan._h, int(row_split)))
+
+
+def split_candidates(plan):
+    """Explicit wavelength splits of the row kernels tried after the shape
+    (0, the automatic choice, is always a candidate)."""
+    return [s for s in (1, 2, 4) if s <= len(plan.ks)]
+
+
 def tune_plan(plan, stack, field_in, field_out, workspace=None, repeats=2, cache=True):
     """Times every candidate shape on this transfer and sets the fastest.
 
     ``field_in`` is restored before each run (npass > 1 writes the
     input-side field into it) and left as it was.  Returns
-    ``{"tiles": (col_lines, row_rays), "ms": {(c, r): ms}, "cached": bool}``.
+    ``{"tiles": (col_lines, row_rays), "row_split": n, "ms": {(c, r): ms},
+    "split_ms": {n: ms}, "cached": bool}`` (``split_ms`` only when tuned).
     """
     import torch
     key = signature(plan)
     if cache and key in _CACHE:
-        set_tiles(plan, *_CACHE[key])
-        return {"tiles": _CACHE[key], "ms": {}, "cached": True}
+        tiles, split = _CACHE[key]
+        set_tiles(plan, *tiles)
+        set_row_split(plan, split)
+        return {"tiles": tiles, "row_split": split, "ms": {}, "cached": True}
     saved = field_in.clone()
     stream = torch.cuda.current_stream()
-    times = {}
-    for c, r in candidates(plan):
-        set_tiles(plan, c, r)
+
+    def run():
         field_in.copy_(saved)
         plan.execute(stack, field_in, field_out, workspace=workspace)  # warm-up
         best = None
@@ -74,10 +95,25 @@ def tune_plan(plan, stack, field_in, field_out, workspace=None, repeats=2, cache
             t1.synchronize()
             ms = t0.elapsed_time(t1)
             best = ms if best is None else min(best, ms)
-        times[(c, r)] = best
-    field_in.copy_(saved)
+        return best
+
+    # the shapes with the automatic wavelength split, then the explicit
+    # splits with the fastest shape
+    times = {}
+    set_row_split(plan, 0)
+    for c, r in candidates(plan):
+        set_tiles(plan, c, r)
+        times[(c, r)] = run()
     tiles = min(times, key=times.get)
     set_tiles(plan, *tiles)
+    split_ms = {0: times[tiles]}
+    for sp in split_candidates(plan):
+        set_row_split(plan, sp)
+        split_ms[sp] = run()
+    split = min(split_ms, key=split_ms.get)
+    set_row_split(plan, split)
+    field_in.copy_(saved)
     if cache:
-        _CACHE[key] = tiles
-    return {"tiles": tiles, "ms": times, "cached": False}
+        _CACHE[key] = (tiles, split)
+    return {"tiles": tiles, "row_split": split, "ms": times, "split_ms": split_ms,
+            "cached": False}

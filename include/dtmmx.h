@@ -182,6 +182,14 @@ size_t dtmmx_transfer_workspace_size(const dtmmx_plan* plan, const dtmmx_stack* 
 int dtmmx_plan_set_tiles(dtmmx_plan* plan, int col_lines, int row_rays);
 int dtmmx_plan_get_tiles(const dtmmx_plan* plan, int* col_lines, int* row_rays);
 
+/* Row work-groups per (row, ray group) of the plan's row kernels, each
+ * taking a contiguous chunk of the wavelengths (1 to 8; 0 = chosen from the
+ * grid and the device's compute units, so that small grids and the
+ * two-component 2x2 lines still fill every SIMD).  Results do not depend on
+ * it.  No counterpart in the reference; a knob of dtmm_amd.tune. */
+int dtmmx_plan_set_row_split(dtmmx_plan* plan, int row_split);
+int dtmmx_plan_get_row_split(const dtmmx_plan* plan, int* row_split);
+
 /* Full transfer through the stack.
  *   4x4 (reflection mode 1, diffraction >= 1, npass >= 1): replaces
  *       transfer.py:694-875 with propagate_4x4.py:376-444 and :80-101 per

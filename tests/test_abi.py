@@ -106,6 +106,8 @@ def test_plan_tiles_reject_bad_arguments():
     assert lib.dtmmx_plan_set_tiles(None, 0, 0) == _lib.DTMMX_E_ARG
     c = ctypes.c_int()
     assert lib.dtmmx_plan_get_tiles(None, ctypes.byref(c), ctypes.byref(c)) == _lib.DTMMX_E_ARG
+    assert lib.dtmmx_plan_set_row_split(None, 0) == _lib.DTMMX_E_ARG
+    assert lib.dtmmx_plan_get_row_split(None, ctypes.byref(c)) == _lib.DTMMX_E_ARG
 
 
 def _voxels(n, seed, biaxial=False):

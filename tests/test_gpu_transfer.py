@@ -589,9 +589,9 @@ def test_biaxial_adjoint_is_refused():
 @pytest.mark.gpu
 @pytest.mark.parametrize("method", ["4x4", "2x2"])
 def test_launch_shapes_do_not_change_results_and_tuner_picks_one(method):
-    """Every (column lines, row rays) launch shape gives the same output bit
-    for bit; the autotuner picks one of them and the tuned transfer matches
-    the oracle."""
+    """Every (column lines, row rays) launch shape and every wavelength split
+    of the row kernels gives the same output bit for bit; the autotuner picks
+    one of them and the tuned transfer matches the oracle."""
     import torch
     from dtmm_amd import tune
     flush("case launch shapes %s" % method)
@@ -609,12 +609,24 @@ def test_launch_shapes_do_not_change_results_and_tuner_picks_one(method):
         job.plan.execute(job.stack, job.fin, job.fout, workspace=job.ws)
         torch.cuda.synchronize()
         outs[(c, r)] = job.fout.cpu().numpy()
+    # wavelength splits of the row kernels: 2 gives uneven chunks (2 + 1 of
+    # 3 wavelengths), 8 is capped at the 3 wavelengths
+    for sp in (1, 2, 3, 8):
+        tune.set_row_split(job.plan, sp)
+        assert tune.get_row_split(job.plan) == sp
+        job.plan.execute(job.stack, job.fin, job.fout, workspace=job.ws)
+        torch.cuda.synchronize()
+        outs[("split", sp)] = job.fout.cpu().numpy()
+    tune.set_row_split(job.plan, 0)
     first = next(iter(outs.values()))
     for k, v in outs.items():
         assert np.array_equal(v, first), k
     res = tune.tune_plan(job.plan, job.stack, job.fin, job.fout, job.ws, cache=False)
-    flush("  tuned %s: %s" % (res["tiles"], {k: round(v, 3) for k, v in res["ms"].items()}))
+    flush("  tuned %s split %s: %s %s" % (res["tiles"], res["row_split"],
+                                          {k: round(v, 3) for k, v in res["ms"].items()},
+                                          {k: round(v, 3) for k, v in res["split_ms"].items()}))
     assert res["tiles"] in cands and set(res["ms"]) == set(cands)
+    assert res["row_split"] in [0] + tune.split_candidates(job.plan)
     out = dm.transfer_field((field.astype(np.complex64), wl, ps), optical, beta=0., phi=0.,
                             method=method, precision="c64", tune=True)[0]
     ref = oracle.transfer_field(fd, optical, beta=0., phi=0., method=method)[0]
