@@ -60,6 +60,14 @@ __device__ __forceinline__ void complete_loads(const V (&v)[NV], int n) {
     if (j < n) asm volatile("" ::"v"(v[j].x), "v"(v[j].y) : "memory");
 }
 
+// Wave vote in a scalar register: true when every active lane's p holds.
+// The result of __all is uniform, but the compiler does not know it; read
+// from the first lane it is, so a branch on it stays a scalar branch
+// instead of being flattened into both paths with per-lane selects.
+__device__ __forceinline__ bool wave_all(bool p) {
+  return __builtin_amdgcn_readfirstlane(__all(p)) != 0;
+}
+
 __device__ __forceinline__ int xcd_swizzle(int b, int n) {
   const int x = b & 7, k = b >> 3, q = n >> 3, r = n & 7;
   return x * q + (x < r ? x : r) + k;
@@ -203,6 +211,68 @@ template <typename T, int N, int OP>
 __host__ __device__ constexpr int col_min_waves() {
   return (OP == OP_SPLIT || OP == OP_FIELD) ? (sizeof(T) == 4 ? 2 : 1)
                         : ((N >= 1024 && sizeof(T) == 4) ? 3 : min_waves<T>());
+}
+
+// The 2x2 blocks p I + (p' - p) P of one pixel's forward (f) and backward
+// (b, 4x4 only) modes for the phase kd: f = {f00, f01, f10, f11}.
+// FAST: every pixel of the wave has real mode pairs a1 = -a0, a3 = -a2, real
+// projectors and ok = 1 (lossless uniaxial or isotropic media at normal
+// incidence): one sine and cosine per pair and real-by-complex products.
+// The general path holds an empty volatile asm, which cannot be executed
+// speculatively: without it the compiler merges the two paths into per-lane
+// selects and every wave evaluates both.
+template <typename T, int NC>
+__device__ __forceinline__ void layer_blocks(const cpx<T>* e, T kd, bool fast, cpx<T> f[4],
+                                             cpx<T> b[4]) {
+  if (fast) {
+    const cpx<T> p0 = cexpi_layer(mk<T>(e[0].x * kd, T(0)));
+    const cpx<T> p2 = cexpi_layer(mk<T>(e[2].x * kd, T(0)));
+    const cpx<T> dp = p0 - p2;
+    f[0] = p2 + dp * e[4].x;
+    f[1] = dp * e[5].x;
+    f[2] = dp * e[6].x;
+    f[3] = p2 + dp * e[7].x;
+    if constexpr (NC == 4) {
+      // p1 = conj(p0), p3 = conj(p2)
+      const cpx<T> p3 = conj(p2), dn = conj(dp);
+      b[0] = p3 + dn * e[8].x;
+      b[1] = dn * e[9].x;
+      b[2] = dn * e[10].x;
+      b[3] = p3 + dn * e[11].x;
+    }
+  } else {
+    asm volatile("");
+    const T ok = e[12].x;
+    cpx<T> p0 = cexpi_layer(e[0] * kd);
+    cpx<T> p2 = cexpi_layer(e[2] * kd);
+    const cpx<T> dp = (p0 - p2) * ok;
+    p2 = p2 * ok;
+    f[0] = p2 + dp * e[4];
+    f[1] = dp * e[5];
+    f[2] = dp * e[6];
+    f[3] = p2 + dp * e[7];
+    if constexpr (NC == 4) {
+      const cpx<T> p1 = cexpi_layer(e[1] * kd);
+      cpx<T> p3 = cexpi_layer(e[3] * kd);
+      const cpx<T> dn = (p1 - p3) * ok;
+      p3 = p3 * ok;
+      b[0] = p3 + dn * e[8];
+      b[1] = dn * e[9];
+      b[2] = dn * e[10];
+      b[3] = p3 + dn * e[11];
+    }
+  }
+}
+
+// Whether one pixel's operator data allows layer_blocks' fast path.
+template <typename T, int NC>
+__device__ __forceinline__ bool fast_blocks_ok(const cpx<T>* e) {
+  bool f = e[0].y == T(0) && e[2].y == T(0) && e[12].x == T(1) && e[4].y == T(0) &&
+           e[5].y == T(0) && e[6].y == T(0) && e[7].y == T(0);
+  if constexpr (NC == 4)
+    f = f && e[1].x == -e[0].x && e[1].y == T(0) && e[3].x == -e[2].x && e[3].y == T(0) &&
+        e[8].y == T(0) && e[9].y == T(0) && e[10].y == T(0) && e[11].y == T(0);
+  return f;
 }
 
 template <typename T, int N, int NR, int NC, int OP>
@@ -376,23 +446,16 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
    }
   }
   if constexpr (EARLY) twf.store(tws, tid);  // read after the barrier below
-  // Every pixel of this thread's wave has real mode pairs a1 = -a0,
-  // a3 = -a2 (lossless media at normal incidence, isotropic voxels): one
-  // sine and cosine per pair, and no exponential.  A wave vote keeps the
-  // choice uniform, so no lane runs both paths.
+  // Every pixel of this thread's wave allows layer_blocks' fast path
+  // (lossless media at normal incidence).  A wave vote in a scalar register
+  // keeps the choice uniform, so no lane runs both paths.
   bool fast_ph = false;
   if constexpr (ROPX) {
     bool f = true;
 #pragma unroll
-    for (int k = 0; k < PPX; ++k) {
-      if (tid + k * NTW < N) {
-        const cpx<T>* e = eg[k];
-        f = f && e[0].y == T(0) && e[2].y == T(0);
-        if constexpr (NC == 4)
-          f = f && e[1].x == -e[0].x && e[1].y == T(0) && e[3].x == -e[2].x && e[3].y == T(0);
-      }
-    }
-    fast_ph = __all(f);
+    for (int k = 0; k < PPX; ++k)
+      if (tid + k * NTW < N) f = f && fast_blocks_ok<T, NC>(eg[k]);
+    fast_ph = wave_all(f);
   }
   __syncthreads();
 
@@ -493,31 +556,12 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
       for (int k = 0; k < PPX; ++k) {
         const int x = tid + k * NTW;
         if (x < N) {
-          const cpx<T>* e = eg[k];
           const int ax = lds_at<1, DTMMX_ROP_PADW>(x);
-          const T ok = e[12].x;
-          cpx<T> p0, p1, p2, p3;
-          if (fast_ph) {
-            p0 = cexpi_layer(mk<T>(e[0].x * kd, T(0)));
-            p2 = cexpi_layer(mk<T>(e[2].x * kd, T(0)));
-            p1 = conj(p0);
-            p3 = conj(p2);
-          } else {
-            p0 = cexpi_layer(e[0] * kd);
-            p2 = cexpi_layer(e[2] * kd);
-            if constexpr (NC == 4) {
-              p1 = cexpi_layer(e[1] * kd);
-              p3 = cexpi_layer(e[3] * kd);
-            }
-          }
-          // 2x2 blocks p I + (p' - p) P of the forward (and backward) modes
-          const cpx<T> dp = (p0 - p2) * ok;
-          p2 = p2 * ok;
-          const cpx<T> f00 = p2 + dp * e[4], f01 = dp * e[5], f10 = dp * e[6], f11 = p2 + dp * e[7];
+          cpx<T> fb[4], bb[4];
+          layer_blocks<T, NC>(eg[k], kd, fast_ph, fb, bb);
+          const cpx<T> f00 = fb[0], f01 = fb[1], f10 = fb[2], f11 = fb[3];
           if constexpr (NC == 4) {
-            const cpx<T> dn = (p1 - p3) * ok;
-            p3 = p3 * ok;
-            const cpx<T> b00 = p3 + dn * e[8], b01 = dn * e[9], b10 = dn * e[10], b11 = p3 + dn * e[11];
+            const cpx<T> b00 = bb[0], b01 = bb[1], b10 = bb[2], b11 = bb[3];
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
               cpx<T>* L = lds + (r * 4) * LS + ax;
@@ -712,14 +756,9 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (sizeof(T) == 4 ? 2 : 1))
         T ok = (A.method4 ? r.ok4 : r.ok2) ? T(1) : T(0);
         if (r.kind == KIND_BIAXIAL) ok = T(__builtin_nan(""));
         eg[k][12] = mk<T>(ok, T(0));
-        if (x < N) {
-          const cpx<T>* e = eg[k];
-          f = f && e[0].y == T(0) && e[2].y == T(0);
-          if constexpr (NC == 4)
-            f = f && e[1].x == -e[0].x && e[1].y == T(0) && e[3].x == -e[2].x && e[3].y == T(0);
-        }
+        if (x < N) f = f && fast_blocks_ok<T, NC>(eg[k]);
       }
-      fast_ph = __all(f);
+      fast_ph = wave_all(f);
     }
     cpx<T> v[MAXV];
 #pragma unroll
@@ -757,31 +796,13 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (sizeof(T) == 4 ? 2 : 1))
     for (int k = 0; k < PPX; ++k) {
       const int x = tid + k * NTW;
       if (x < N) {
-        const cpx<T>* e = eg[k];
         const int ax = lds_at<1, DTMMX_ROP_PADW>(x);
-        const T ok = e[12].x;
-        cpx<T> p0, p1, p2, p3;
-        if (fast_ph) {
-          p0 = cexpi_layer(mk<T>(e[0].x * kd, T(0)));
-          p2 = cexpi_layer(mk<T>(e[2].x * kd, T(0)));
-          p1 = conj(p0);
-          p3 = conj(p2);
-        } else {
-          p0 = cexpi_layer(e[0] * kd);
-          p2 = cexpi_layer(e[2] * kd);
-          if constexpr (NC == 4) {
-            p1 = cexpi_layer(e[1] * kd);
-            p3 = cexpi_layer(e[3] * kd);
-          }
-        }
-        const cpx<T> dp = (p0 - p2) * ok;
-        p2 = p2 * ok;
-        const cpx<T> f00 = p2 + dp * e[4], f01 = dp * e[5], f10 = dp * e[6], f11 = p2 + dp * e[7];
+        cpx<T> fb[4], bb[4];
+        layer_blocks<T, NC>(eg[k], kd, fast_ph, fb, bb);
+        const cpx<T> f00 = fb[0], f01 = fb[1], f10 = fb[2], f11 = fb[3];
         cpx<T>* a = acl + size_t(k) * LINES * NTW + tid;
         if constexpr (NC == 4) {
-          const cpx<T> dn = (p1 - p3) * ok;
-          p3 = p3 * ok;
-          const cpx<T> b00 = p3 + dn * e[8], b01 = dn * e[9], b10 = dn * e[10], b11 = p3 + dn * e[11];
+          const cpx<T> b00 = bb[0], b01 = bb[1], b10 = bb[2], b11 = bb[3];
 #pragma unroll
           for (int r = 0; r < NR; ++r) {
             const cpx<T>* L = lds + (r * 4) * LS + ax;
