@@ -125,6 +125,18 @@ __device__ __forceinline__ cpx<float> cexpi_layer(cpx<float> ph) {
 }
 __device__ __forceinline__ cpx<double> cexpi_layer(cpx<double> ph) { return cexpi(ph); }
 
+// exp(i x) of a real phase in the per-layer kernels.
+__device__ __forceinline__ cpx<float> cexpi_layer_real(float x) {
+  float s, c;
+  sincos_cw(x, &s, &c);
+  return mk<float>(c, s);
+}
+__device__ __forceinline__ cpx<double> cexpi_layer_real(double x) {
+  double s, c;
+  sincos(x, &s, &c);
+  return mk<double>(c, s);
+}
+
 inline cpx<double> cexpi_host(cpx<double> ph) {
   double m = (ph.y == 0.) ? 1. : std::exp(-ph.y);
   return mk<double>(std::cos(ph.x) * m, std::sin(ph.x) * m);

@@ -450,6 +450,12 @@ static int col_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   a.W = P->d.width;
   a.b = b;
   a.NC = nc;
+  {
+    // the passing modes all propagate in ea: real, eps above betamax^2 by a
+    // margin over the float rounding of beta^2 in the kernel
+    const double bm2 = P->d.betamax * P->d.betamax;
+    a.prop = (a.ea.y == 0. && a.ea.x > bm2 * (1. + 1e-5) + 1e-12) ? 1 : 0;
+  }
   // lines per work-group: the components of one ray, or of two rays sharing a beam
   a.ncl = nc;
   if (b.nrays % 2 == 0 && P->bm.rays_per_beam % 2 == 0 && nc <= 2) a.ncl = 2 * nc;

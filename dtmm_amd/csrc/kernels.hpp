@@ -1152,6 +1152,10 @@ struct ColArgs {
   // the thickness da (0 both directions, +1 forward modes, -1 backward)
   int fmode;
   Beams bm;
+  // 1: every mode passing the betamax mask propagates in the half-step
+  // medium ea (real, eps > betamax^2 with a margin over the float rounding
+  // of beta^2), so the diagonal factor of a real medium is a pure phase
+  int prop;
 };
 
 // F P(kd) F^-1 of an isotropic medium e0 in closed form at in-plane wave
@@ -1294,13 +1298,19 @@ __device__ __forceinline__ DiagConst<T> diag_const(const ColArgs<T>& A, int lam,
   return c;
 }
 
-template <typename T, bool FAST>
+template <typename T, bool FAST, bool PROP = false>
 __device__ __forceinline__ cpx<T> diag_factor(const ColArgs<T>& A, const DiagConst<T>& c, int amax,
                                               int ky, int kx) {
   // amax: largest |kx| index passing betamax in row ky (-1: none)
   const int mx = mode_index(kx, A.W), my = mode_index(ky, A.H);
   if (mx > amax) return mk<T>(T(0), T(0));
   const T b2 = c.cx * T(mx * mx) + c.cy * T(my * my);
+  if constexpr (PROP) {
+    // a real medium (FAST) whose passing modes all propagate and a real
+    // mean-beam correction: a phase only, no attenuation
+    const T q = sqrt(fmax(c.er - b2, T(0)));
+    return cexpi_layer_real(q * (c.ha + c.hb) - c.pc.x) * T(A.scale);
+  }
   cpx<T> ph;
   if constexpr (FAST) {
     // sqrt(er - b2) of a real medium: real, or positive imaginary past er
@@ -1405,7 +1415,13 @@ __global__ __launch_bounds__(TX * col_lanes<N>(), (col_min_waves<T, N, OP>())) v
     const DiagConst<T> dc = diag_const(A, lam, beam);
 #pragma unroll
     for (int j = 0; j < NMID; ++j) K[j] = mk<T>(T(0), T(0));
-    if (dc.fast) {
+    if (dc.fast && A.prop && dc.pc.y == T(0)) {
+#pragma unroll
+      for (int j = 0; j < NMID; ++j)
+        if (xin && slot_valid<N, REV1, NL, true>(j, g))
+          K[j] = diag_factor<T, true, true>(A, dc, am[j], slot_pos<N, REV1, NL, true>(j, g), x);
+    } else if (dc.fast) {
+      asm volatile("");  // a branch of its own, not merged with the one above
 #pragma unroll
       for (int j = 0; j < NMID; ++j)
         if (xin && slot_valid<N, REV1, NL, true>(j, g))
