@@ -56,7 +56,7 @@ HBM_PEAK_GBS = 8000.0
 # (kernel, H, W, planes, method, precision); other workloads report null.
 MEASURED_TRAFFIC = {
     ("row_layer", 512, 512, 18, "4x4", "c64"): 312.8e6,
-    ("col_kspace", 512, 512, 18, "4x4", "c64"): 303.8e6,
+    ("col_kspace", 512, 512, 18, "4x4", "c64"): 303.0e6,
     ("row_layer", 512, 512, 18, "2x2", "c64"): 161.9e6,
     ("col_kspace", 512, 512, 18, "2x2", "c64"): 151.5e6,
 }
