@@ -169,16 +169,17 @@ static int launch_row(int n, int nr, const RowArgs<T>& a, int nwork, hipStream_t
   return DTMMX_OK;
 }
 
-// Compute units of the current device (cached per process; 256 if unknown).
+// Compute units of the device current at the first call (256 if unknown),
+// kept for the process; the initialisation of a function-local static runs
+// once even when several threads plan at the same time.
 static int device_cus() {
-  static int v = -1;
-  if (v < 0) {
+  static const int v = [] {
     int dev = 0, x = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&x, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || x <= 0)
       x = 256;
-    v = x;
-  }
+    return x;
+  }();
   return v;
 }
 
@@ -187,14 +188,13 @@ static int device_cus() {
 // per-window sums fill the LDS beside the kernel's own buffers.  Returns
 // DTMMX_E_UNSUPPORTED (nothing launched) when not even one wavelength fits.
 static int max_lds_per_block() {
-  static int v = -1;
-  if (v < 0) {
+  static const int v = [] {  // initialised once, also under concurrent plans
     int dev = 0, x = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&x, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       x = 65536;
-    v = x;
-  }
+    return x;
+  }();
   return v;
 }
 
