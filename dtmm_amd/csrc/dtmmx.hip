@@ -149,10 +149,16 @@ constexpr int row_nr_max() {
 }
 
 template <typename T, int NC, int OP>
-static int launch_row(int n, int nr, const RowArgs<T>& a, int nwork, hipStream_t s) {
+static int launch_row(int n, int nr, const RowArgs<T>& a, int nwork, hipStream_t s, int nw = 1) {
   switch (n) {
 #define X(N)                                                                                \
   case N:                                                                                   \
+    if constexpr (OP == OP_ROP && NC == 2 && row_nr_max<T, N>() >= 2 && row_lanes<N>() == 64) { \
+      if (nw == 2 && nr == 2) {                                                             \
+        DTMMX_LAUNCH((row_kernel<T, N, 2, 2, OP_ROP, 2>), dim3(nwork), dim3(64 * 8), s, a);  \
+        break;                                                                              \
+      }                                                                                     \
+    }                                                                                       \
     if (nr == 2 && row_nr_max<T, N>() >= 2)                                                 \
       DTMMX_LAUNCH((row_kernel<T, N, (row_nr_max<T, N>() >= 2 ? 2 : 1), NC, OP>),           \
                    dim3(nwork), dim3(row_lanes<N>() * 2 * NC), s, a);                       \
@@ -374,13 +380,17 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
     nr = 2;
   a.rays_per_group = nr;
   const int w = a.W;
+  // two-component layer passes of two rays (complex64, one wave per line):
+  // two wavelengths per work-group in separate waves, 8 waves sharing one
+  // per-pixel solve
+  const int nw = (op == OP_ROP && nc == 2 && nr == 2 && sizeof(T) == 4 && w <= 512) ? 2 : 1;
   // wavelength chunks per (row, ray group): the plan's setting, else enough
   // work-groups to give every SIMD of the device the row kernel's occupancy
   // (4 waves of complex64, 2 of complex128), at most 8 and at most the
   // batch's wavelengths
   int split = P->row_split;
   if (split <= 0) {
-    const long long waves = (long long)a.H * (b.nrays / nr) * nr * nc * (w >= 1024 ? 2 : 1);
+    const long long waves = (long long)a.H * (b.nrays / nr) * nr * nc * nw * (w >= 1024 ? 2 : 1);
     const long long want = (long long)device_cus() * 4 * (sizeof(T) == 4 ? 4 : 2);
     split = int((want + waves - 1) / (waves > 0 ? waves : 1));
   }
@@ -410,7 +420,7 @@ static int row_pass(const dtmmx_plan* P, cpx<T>* field, Batch b, int nc, int op,
   }
   if (op == OP_ROP) {
     timer_start(0, s);
-    int r = nc == 2 ? launch_row<T, 2, OP_ROP>(w, nr, a, nwork, s)
+    int r = nc == 2 ? launch_row<T, 2, OP_ROP>(w, nr, a, nwork, s, nw)
                     : launch_row<T, 4, OP_ROP>(w, nr, a, nwork, s);
     timer_stop(s);
     return r;

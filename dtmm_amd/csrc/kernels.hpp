@@ -275,12 +275,17 @@ __device__ __forceinline__ bool fast_blocks_ok(const cpx<T>* e) {
   return f;
 }
 
-template <typename T, int N, int NR, int NC, int OP>
-__global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) void row_kernel(
+// NW > 1 (layer operator kernels with one wave per line): the work-group
+// transforms NW wavelengths at once in separate waves, so that one fp64
+// solve per pixel serves twice the waves (the two-component lines of the
+// 2x2 method otherwise give work-groups of only 4 waves).
+template <typename T, int N, int NR, int NC, int OP, int NW = 1>
+__global__ __launch_bounds__(row_lanes<N>() * NR * NC * NW, (row_min_waves<T, N>())) void row_kernel(
     RowArgs<T> A) {
   constexpr int NL = row_lanes<N>();
   constexpr bool WV = (NL == 64);  // one wave per line: wave-local exchanges
-  constexpr int LINES = NR * NC;
+  static_assert(NW == 1 || (WV && OP == OP_ROP), "wavelength-parallel waves: layer operator, one wave per line");
+  constexpr int LINES = NR * NC * NW;
   constexpr int NTW = NL * LINES;
   constexpr int LS = line_stride<N>();
   constexpr bool FWDX = (OP == OP_FWD || OP == OP_COMB);  // forward x transforms
@@ -310,7 +315,8 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
     twf.load(A.tw, tid);
   else
     fill_tw_tables<T, N>(tws, A.tw, tid, NTW);
-  const int rr = li / NC, comp = li - rr * NC;
+  const int wl = li / (NR * NC);  // wavelength slot of this wave (NW > 1)
+  const int rr = (li - wl * (NR * NC)) / NC, comp = li - wl * (NR * NC) - rr * NC;
   // consecutive work items (the wavelength chunks of one row first) land on
   // one XCD, so the chunks of a row share its material in that XCD's L2
   const int lsplit = A.lam_split > 1 ? A.lam_split : 1;
@@ -364,10 +370,10 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   };
   const size_t ypol = size_t(A.b.nlam) * NC * HW;
   if constexpr (PF) {
-    const cpx<T>* g0 = (OP == OP_COMB) ? pol_ptr(jb) : line_ptr(jb);
+    const cpx<T>* g0 = (OP == OP_COMB) ? pol_ptr(jb + wl) : line_ptr(jb + wl);
 #pragma unroll
     for (int j = 0; j < NIN; ++j)
-      if (jb < je && slot_valid<N, REV1, NL, false>(j, lane)) {
+      if (jb + wl < je && slot_valid<N, REV1, NL, false>(j, lane)) {
         nxt[j] = g0[slot_pos<N, REV1, NL, false>(j, lane)];
         if constexpr (OP == OP_COMB) nxt2[j] = g0[ypol + slot_pos<N, REV1, NL, false>(j, lane)];
       }
@@ -460,8 +466,10 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
   __syncthreads();
 
   const int acc_mode = ROPX ? A.acc_mode : 0;
-  for (int jl = jb; jl < je; ++jl) {
-    const int lam = A.b.lam0 + jl;
+  for (int j0 = jb; j0 < je; j0 += NW) {
+    const int jl = j0 + wl;
+    const bool act = jl < je;  // false only for the last slots when NW > 1
+    const int lam = A.b.lam0 + (act ? jl : j0);
     cpx<T>* gl = line_ptr(jl);
     cpx<T> v[MAXV];
     if constexpr (PF) {
@@ -472,8 +480,8 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
         else
           v[j] = nxt[j];
       }
-      if (jl + 1 < je) {
-        const cpx<T>* gn = (OP == OP_COMB) ? pol_ptr(jl + 1) : line_ptr(jl + 1);
+      if (jl + NW < je) {
+        const cpx<T>* gn = (OP == OP_COMB) ? pol_ptr(jl + NW) : line_ptr(jl + NW);
 #pragma unroll
         for (int j = 0; j < NIN; ++j)
           if (slot_valid<N, REV1, NL, false>(j, lane)) {
@@ -487,7 +495,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
         if (slot_valid<N, REV1, NL, false>(j, lane)) v[j] = gl[slot_pos<N, REV1, NL, false>(j, lane)];
     }
     if constexpr (OP == OP_ROP) {
-      if (A.save) {
+      if (A.save && act) {
         cpx<T>* sl = A.save + (gl - A.field);
 #pragma unroll
         for (int j = 0; j < NIN; ++j)
@@ -551,7 +559,11 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
         if (slot_valid<N, REV1, NL, true>(j, lane))
           line_lds[lds_at<1, DTMMX_ROP_PADW>(slot_pos<N, REV1, NL, true>(j, lane))] = v[j];
       __syncthreads();
-      const T kd = T(A.bm.k0[lam] * A.d);
+#pragma unroll
+      for (int ws = 0; ws < NW; ++ws) {
+       if (j0 + ws < je) {
+        const T kd = T(A.bm.k0[A.b.lam0 + j0 + ws] * A.d);
+        cpx<T>* lw = lds + (ws * NR * NC) * LS;  // the lines of wavelength slot ws
 #pragma unroll
       for (int k = 0; k < PPX; ++k) {
         const int x = tid + k * NTW;
@@ -564,7 +576,7 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
             const cpx<T> b00 = bb[0], b01 = bb[1], b10 = bb[2], b11 = bb[3];
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-              cpx<T>* L = lds + (r * 4) * LS + ax;
+              cpx<T>* L = lw + (r * 4) * LS + ax;
               const cpx<T> u0 = L[0], u1 = L[LS], u2 = L[2 * LS], u3 = L[3 * LS];
               L[0] = f00 * u0 + f01 * u2;
               L[2 * LS] = f10 * u0 + f11 * u2;
@@ -574,13 +586,15 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
           } else {
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-              cpx<T>* L = lds + (r * 2) * LS + ax;
+              cpx<T>* L = lw + (r * 2) * LS + ax;
               const cpx<T> u0 = L[0], u1 = L[LS];
               L[0] = f00 * u0 + f01 * u1;
               L[LS] = f10 * u0 + f11 * u1;
             }
           }
         }
+      }
+       }
       }
       __syncthreads();
 #pragma unroll
@@ -591,7 +605,8 @@ __global__ __launch_bounds__(row_lanes<N>() * NR * NC, (row_min_waves<T, N>())) 
       reg_fft<T, N, false, -1, NL, 1, 1, MAXV, WV>(v, line_lds, tws, lane);
     }
     if constexpr (PF) complete_loads(nxt, NIN);
-    if (acc_mode == 0) {
+    if (!act) {
+    } else if (acc_mode == 0) {
 #pragma unroll
       for (int j = 0; j < NOUT; ++j)
         if (slot_valid<N, REVL, NL, true>(j, lane)) gl[slot_pos<N, REVL, NL, true>(j, lane)] = v[j];
